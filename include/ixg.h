@@ -1,0 +1,164 @@
+/*
+ * ixg.h -- C ABI of the MI355X (gfx950) per-window search DP for infernox.
+ *
+ * Plain C types only: pointers, sizes, status codes. Every entry point
+ * returns IXG_OK (0) or a negative IXG_E* status; nothing throws. Host
+ * pointers are read before the call returns (inputs) or filled before it
+ * returns (outputs). Coordinates are 1-based and inclusive, chunk-local,
+ * exactly as the reference functions they replace.
+ *
+ * Each declaration names the reference function (file:line in infernox
+ * v0.2.0) whose signature and semantics it replaces.
+ */
+#ifndef IXG_H
+#define IXG_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define IXG_OK 0
+#define IXG_EINVAL (-1)   /* bad argument (shape, size, null pointer) */
+#define IXG_ENOMEM (-2)   /* device allocation or memory budget exceeded */
+#define IXG_EDEVICE (-3)  /* HIP runtime error (message via ixg_last_error) */
+#define IXG_ERANGE (-4)   /* an output list exceeded its capacity */
+
+#define IXG_KP 18         /* digitized RNA alphabet size (easel/alphabet.rs:46-157) */
+
+typedef struct ixg_ctx ixg_ctx;     /* one device + one stream + scratch */
+typedef struct ixg_p7 ixg_p7;       /* device-resident p7 filter profile */
+
+/* ---- context ------------------------------------------------------------ */
+/* Replaces nothing in the reference (which has no device); owns the HIP
+ * stream and scratch used by every call made with it. mem_budget = 0 means
+ * no limit. One ctx per calling thread. */
+int ixg_init(int device, size_t mem_budget, ixg_ctx **out);
+int ixg_shutdown(ixg_ctx *ctx);
+const char *ixg_last_error(void);
+/* Device bytes currently held by ctx scratch plus uploaded profiles. */
+size_t ixg_mem_in_use(const ixg_ctx *ctx);
+
+/* ---- p7 filter HMM (P7Profile, p7_hmm.rs:62-90) -------------------------- */
+typedef struct {
+  int32_t M;
+  const float *mat;     /* (M+1)*4, match emissions A,C,G,U; row 0 unused */
+  const float *trans;   /* (M+1)*7, [MM,MI,MD,IM,II,DM,DD] */
+  float compo[4];       /* mean residue composition (COMPO) */
+  double lmmu, lmlambda;  /* local MSV Gumbel */
+  double lvmu, lvlambda;  /* local Viterbi Gumbel */
+  double lftau, lflambda; /* local Forward exponential tail */
+} ixg_p7_hmm;
+
+/* Builds the MSV (cm_pipeline.rs:204 build_msv_filter), Viterbi
+ * (p7_vitfilter.rs:121 build_vit_filter), Forward (cm_pipeline.rs:969
+ * build_forward_filter) and bias (cm_pipeline.rs:1683 build_bias_filter)
+ * filter profiles on the host and uploads them; they stay resident in HBM
+ * until ixg_p7_free. max_length is the reference's maxW (om->max_length).
+ * With ctx == NULL only the host tables are built (for inspection through
+ * ixg_p7_*_tables); such a handle is freed with ixg_p7_free(NULL, p). */
+int ixg_p7_create(ixg_ctx *ctx, const ixg_p7_hmm *hmm, int32_t max_length, ixg_p7 **out);
+int ixg_p7_free(ixg_ctx *ctx, ixg_p7 *p7);
+
+/* Copies of the host-built tables, for parity checks against a CPU build.
+ * rbv: (M+1)*18 u8. bytes: {scale_b(float bits)x4, base_b, bias_b, tbm_b,
+ * tec_b, tjb_b}. prefix/suffix: (M+1) f32. Any pointer may be NULL. */
+int ixg_p7_msv_tables(const ixg_p7 *p7, uint8_t *rbv, uint8_t *bytes5, float *prefix, float *suffix);
+/* Viterbi: rwv (M+1)*18 i16, trans8 = 8 arrays of (M+1) i16 in the order
+ * tmm,tim,tdm,vbm,tmi,tii,tmd,tdd; ddbound (1 i16). */
+int ixg_p7_vit_tables(const ixg_p7 *p7, int16_t *rwv, int16_t *trans8, int16_t *ddbound);
+/* Forward: tmain nq*28, tddv nq*4, rfvv 18*nq*4 (f32); nq returned. */
+int ixg_p7_fwd_tables(const ixg_p7 *p7, int32_t *nq, float *tmain, float *tddv, float *rfvv);
+
+/* ---- records ------------------------------------------------------------- */
+typedef struct { int64_t n, length, k; float score; int32_t task; } ixg_rawwin; /* RawWin, cm_pipeline.rs:455 */
+typedef struct { int64_t start, end; float score; int32_t task; } ixg_window;   /* Window, cm_pipeline.rs:465 */
+typedef struct {                                                              /* F1F3Acct, cm_pipeline.rs:1814 */
+  uint64_t n_past_msv, pos_past_msv, n_past_msvbias, pos_past_msvbias;
+  uint64_t n_past_vit, pos_past_vit, n_past_vitbias, pos_past_vitbias;
+  uint64_t n_past_fwd, pos_past_fwd, n_past_fwdbias, pos_past_fwdbias;
+} ixg_acct;
+
+/* ---- F1: SSV longtarget diagonals ------------------------------------------
+ * Replaces cm_pipeline.rs:521 ssv_diagonals(&MsvFilter, &[u8], usize, f64)
+ * -> Vec<RawWin>. dsq is 1-indexed (dsq[1..=L]; dsq[0], dsq[L+1] are not
+ * read). Writes up to cap diagonals in the reference's order; *n gets the
+ * full count (IXG_ERANGE if it exceeds cap). */
+int ixg_ssv_diagonals(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *dsq, int64_t L, double f1,
+                      ixg_rawwin *out, int64_t cap, int64_t *n);
+
+/* ---- per-window scores (F1b, F2, F3, F3b) ------------------------------------
+ * A batch of nwin windows, window w = seq[off[w]+1 .. off[w]+len[w]] (the
+ * same 1-indexed convention: seq + off[w] plays the role of the window's
+ * sentinel-padded dsq), so off[w] + len[w] < seq_len. Residue codes must be
+ * below 18. ovf[w] = 1 where the reference returns None. */
+/* p7_vitfilter.rs:292 vit_score(&VitFilter, &[u8], usize) -> Option<f32> */
+int ixg_vit_scores(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *seq, int64_t seq_len,
+                   const int64_t *off, const int64_t *len, int64_t nwin, float *sc, uint8_t *ovf);
+/* cm_pipeline.rs:1420 forward_filter_score(&ForwardFilter, &[u8], usize) -> f32 (nats) */
+int ixg_fwd_scores(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *seq, int64_t seq_len,
+                   const int64_t *off, const int64_t *len, int64_t nwin, float *sc);
+/* cm_pipeline.rs:1722 bias_filter_score(&BiasFilter, &[u8], usize) -> f32 (nats) */
+int ixg_bias_scores(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *seq, int64_t seq_len,
+                    const int64_t *off, const int64_t *len, int64_t nwin, float *sc);
+/* cm_pipeline.rs:426 msv_score_reconfig(f, dsq, l, wlen=l) -> Option<f32> (F1b) */
+int ixg_msv_scores(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *seq, int64_t seq_len,
+                   const int64_t *off, const int64_t *len, int64_t nwin, float *sc, uint8_t *ovf);
+
+/* ---- the filter cascade -------------------------------------------------------- */
+#define IXG_DO_MSV      (1 << 0)
+#define IXG_DO_FWD      (1 << 1)
+#define IXG_DO_FWDBIAS  (1 << 2)
+#define IXG_DO_VIT      (1 << 3)
+#define IXG_DO_VITBIAS  (1 << 4)
+#define IXG_DO_MSVBIAS  (1 << 5)
+typedef struct {
+  double f1, f1b, f2, f2b, f3, f3b; /* P-value thresholds */
+  int64_t cmw;                       /* the CM's W (window split length) */
+  int64_t maxw;                      /* maxW (tiling when MSV is off) */
+  int32_t opts;                      /* IXG_DO_* */
+} ixg_f3opts;
+
+/* Replaces cm_pipeline.rs:1863 f3_filter_sequence(...) -> (Vec<Window>,
+ * F1F3Acct) for one chunk (dsq 1-indexed, L residues). */
+int ixg_f3_filter_sequence(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *dsq, int64_t L,
+                           const ixg_f3opts *o, ixg_window *out, int64_t cap, int64_t *n,
+                           ixg_acct *acct);
+
+/* ---- batched form over a device-resident target ---------------------------------
+ * A target is a digitized sequence kept in HBM together with its reverse
+ * complement (cm_search.rs:1822-1830: 3-d for d<4). Tasks are the
+ * reference's (chunk x strand) windows (cm_search.rs:1774 window_tasks).  */
+typedef struct ixg_target ixg_target;
+typedef struct {
+  int64_t start;   /* 1-based global start of the chunk (wgs) */
+  int64_t end;     /* 1-based global end (wge) */
+  int32_t strand;  /* 0 top, 1 bottom (reverse complement of [start,end]) */
+  int32_t ctx;     /* residues of maxW context shared with the previous chunk */
+} ixg_task;
+int ixg_target_create(ixg_ctx *ctx, const uint8_t *dsq, int64_t L, ixg_target **out);
+int ixg_target_free(ixg_ctx *ctx, ixg_target *t);
+/* Builds the reference's task list for one sequence of length L
+ * (cm_search.rs:1774-1808, CM_MAX_RESIDUE_COUNT = 100000). Returns the count;
+ * writes at most cap tasks. */
+int64_t ixg_window_tasks(int64_t L, int64_t maxw, int do_top, int do_bot, ixg_task *out, int64_t cap);
+
+/* Runs f3_filter_sequence for every task in one pass over the device:
+ * one F1 sweep over all tasks, then one F2/F3/F3b launch over every surviving
+ * window. out[] receives each task's merged windows (chunk-local coordinates,
+ * tagged with their task index) grouped by task in task order; acct[t] per
+ * task (may be NULL). *n = total windows (IXG_ERANGE if above cap). */
+int ixg_f3_filter_tasks(ixg_ctx *ctx, const ixg_p7 *p7, const ixg_target *tgt,
+                        const ixg_task *tasks, int64_t ntasks, const ixg_f3opts *o,
+                        ixg_window *out, int64_t cap, int64_t *n, ixg_acct *acct);
+
+/* Per-stage device timings (milliseconds, HIP events) of the most recent
+ * ixg_f3_filter_tasks call on ctx: [0] F1 sweep, [1] F1 resolve, [2] F2/F3/F3b
+ * window kernels, [3] host glue, [4] total. Also the launch counts. */
+int ixg_last_timings(const ixg_ctx *ctx, double *ms5, int64_t *cells5);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

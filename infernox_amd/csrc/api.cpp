@@ -1,0 +1,908 @@
+// C ABI implementation: device context, resident profiles and targets, and
+// the host side of the filter cascade (window extension/merge/split, P-value
+// gates, survivor merge, accounting), which mirrors cm_pipeline.rs line by
+// line while every DP runs on the GPU.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+#include <chrono>
+#include "ixg_internal.h"
+
+using namespace ixg;
+
+static thread_local std::string g_err;
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      g_err = std::string(#expr) + ": " + hipGetErrorString(e_);                      \
+      return e_ == hipErrorOutOfMemory ? IXG_ENOMEM : IXG_EDEVICE;                    \
+    }                                                                                 \
+  } while (0)
+#define RCHK(expr)            \
+  do {                        \
+    int r_ = (expr);          \
+    if (r_ != IXG_OK) return r_; \
+  } while (0)
+
+namespace {
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+};
+
+}  // namespace
+
+struct ixg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t budget = 0;
+  size_t in_use = 0;
+  std::vector<DevBuf> bufs;  // grow-only scratch slots
+  hipEvent_t ev[8];
+  double ms[5] = {0, 0, 0, 0, 0};
+  int64_t cells[5] = {0, 0, 0, 0, 0};
+};
+
+struct ixg_p7 {
+  MsvHost msv;
+  VitHost vit;
+  FwdHost fwd;
+  BiasHost bias;
+  uint32_t *d_cost4 = nullptr;
+  uint8_t *d_rbv = nullptr;
+  float *d_fwd = nullptr;   // tmain | tddv | rfvv
+  int16_t *d_vit = nullptr; // rwv | tr(8*(m+1))
+  size_t bytes = 0;
+  double vit_lvmu = 0, vit_lvlambda = 0;
+  SsvDev ssv_dev() const {
+    SsvDev d;
+    d.cost4 = d_cost4;
+    d.rbv = d_rbv;
+    d.m = msv.m;
+    d.bias = msv.bias_b;
+    d.base_b = msv.base_b;
+    d.tjb_b = msv.tjb_b;
+    d.tbm_b = msv.tbm_b;
+    int tj = msv.tjb_b + msv.tbm_b;
+    tj = tj > 255 ? 255 : tj;
+    d.xb = (uint8_t)(msv.base_b > tj ? msv.base_b - tj : 0);
+    d.scale_b = msv.scale_b;
+    return d;
+  }
+  FwdDev fwd_dev() const {
+    FwdDev d;
+    d.tmain = d_fwd;
+    d.tddv = d_fwd + fwd.tmain.size();
+    d.rfvv = d_fwd + fwd.tmain.size() + fwd.tddv.size();
+    d.m = fwd.m;
+    d.nq = fwd.nq;
+    return d;
+  }
+  VitDev vit_dev() const {
+    VitDev d;
+    d.rwv = d_vit;
+    d.tr = d_vit + vit.rwv.size();
+    d.m = vit.m;
+    d.base_w = vit.base_w;
+    d.xw_e_move = vit.xw_e_move;
+    d.xw_e_loop = vit.xw_e_loop;
+    d.scale_w = vit.scale_w;
+    return d;
+  }
+  BiasDev bias_dev() const {
+    BiasDev d;
+    memcpy(d.eo, bias.eo, sizeof(d.eo));
+    memcpy(d.t1, bias.t1, sizeof(d.t1));
+    memcpy(d.pi, bias.pi, sizeof(d.pi));
+    return d;
+  }
+};
+
+struct ixg_target {
+  uint8_t *d_both = nullptr;  // [fwd (L+2) | revcomp (L+2)], sentinel-padded
+  int64_t L = 0;
+  size_t bytes = 0;
+};
+
+// ---------------------------------------------------------------------------
+// device memory helpers
+// ---------------------------------------------------------------------------
+static int dmalloc(ixg_ctx *c, void **p, size_t n) {
+  if (n == 0) n = 1;
+  if (c->budget && c->in_use + n > c->budget) {
+    g_err = "device memory budget exceeded";
+    return IXG_ENOMEM;
+  }
+  HIPCHK(hipMalloc(p, n));
+  c->in_use += n;
+  return IXG_OK;
+}
+static void dfree(ixg_ctx *c, void *p, size_t n) {
+  if (!p) return;
+  (void)hipFree(p);
+  c->in_use -= std::min(c->in_use, n ? n : (size_t)1);
+}
+// Scratch slot `slot` grown to at least n bytes (contents not preserved).
+static int scratch(ixg_ctx *c, int slot, size_t n, void **out) {
+  if ((int)c->bufs.size() <= slot) c->bufs.resize(slot + 1);
+  DevBuf &b = c->bufs[slot];
+  if (b.cap < n) {
+    dfree(c, b.p, b.cap);
+    b.p = nullptr;
+    b.cap = 0;
+    size_t want = std::max(n, (size_t)4096);
+    want = std::max(want, n + n / 4);
+    RCHK(dmalloc(c, &b.p, want));
+    b.cap = want;
+  }
+  *out = b.p;
+  return IXG_OK;
+}
+enum Slot { S_TASKS, S_ITEMS, S_MASK, S_RAW, S_COUNTER, S_WINS, S_SC0, S_SC1, S_SC2, S_OVF0, S_OVF1, S_SEQ, S_NSLOTS };
+
+template <typename T>
+static int upload(ixg_ctx *c, int slot, const std::vector<T> &v, T **out) {
+  void *p;
+  RCHK(scratch(c, slot, v.size() * sizeof(T), &p));
+  if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  *out = (T *)p;
+  return IXG_OK;
+}
+
+extern "C" {
+
+const char *ixg_last_error(void) { return g_err.c_str(); }
+
+int ixg_init(int device, size_t mem_budget, ixg_ctx **out) {
+  if (!out) return IXG_EINVAL;
+  int n = 0;
+  HIPCHK(hipGetDeviceCount(&n));
+  if (device < 0 || device >= n) {
+    g_err = "no such device";
+    return IXG_EINVAL;
+  }
+  HIPCHK(hipSetDevice(device));
+  auto *c = new ixg_ctx();
+  c->device = device;
+  c->budget = mem_budget;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    g_err = hipGetErrorString(e);
+    delete c;
+    return IXG_EDEVICE;
+  }
+  for (auto &ev : c->ev) HIPCHK(hipEventCreate(&ev));
+  *out = c;
+  return IXG_OK;
+}
+
+int ixg_shutdown(ixg_ctx *c) {
+  if (!c) return IXG_EINVAL;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto &b : c->bufs) dfree(c, b.p, b.cap);
+  for (auto &ev : c->ev) (void)hipEventDestroy(ev);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return IXG_OK;
+}
+
+size_t ixg_mem_in_use(const ixg_ctx *c) { return c ? c->in_use : 0; }
+
+int ixg_p7_create(ixg_ctx *c, const ixg_p7_hmm *h, int32_t max_length, ixg_p7 **out) {
+  if (!h || !out || h->M < 1 || h->M > 1024 || !h->mat || !h->trans || max_length < 1) {
+    g_err = "ixg_p7_create: bad arguments (1 <= M <= 1024 required)";
+    return IXG_EINVAL;
+  }
+  auto p = std::make_unique<ixg_p7>();
+  build_msv(*h, max_length, p->msv);
+  build_vit(*h, p->vit);
+  build_fwd(*h, p->fwd);
+  build_bias(h->compo, h->M, p->bias);
+  p->vit_lvmu = h->lvmu;
+  p->vit_lvlambda = h->lvlambda;
+  if (!c) {  // host tables only (inspection); no device copy
+    *out = p.release();
+    return IXG_OK;
+  }
+  HIPCHK(hipSetDevice(c->device));
+  const int m = h->M;
+  std::vector<uint32_t> cost4(m + 1, 0xFFFFFFFFu);
+  for (int k = 0; k <= m; k++) {
+    uint32_t v = 0;
+    for (int x = 0; x < 4; x++) v |= (uint32_t)p->msv.rbv[k * KP + x] << (8 * x);
+    cost4[k] = v;
+  }
+  std::vector<float> fwd;
+  fwd.insert(fwd.end(), p->fwd.tmain.begin(), p->fwd.tmain.end());
+  fwd.insert(fwd.end(), p->fwd.tddv.begin(), p->fwd.tddv.end());
+  fwd.insert(fwd.end(), p->fwd.rfvv.begin(), p->fwd.rfvv.end());
+  std::vector<int16_t> vit(p->vit.rwv);
+  for (auto *a : {&p->vit.tmm, &p->vit.tim, &p->vit.tdm, &p->vit.vbm, &p->vit.tmi, &p->vit.tii, &p->vit.tmd,
+                  &p->vit.tdd})
+    vit.insert(vit.end(), a->begin(), a->end());
+  size_t b0 = cost4.size() * 4, b1 = p->msv.rbv.size(), b2 = fwd.size() * 4, b3 = vit.size() * 2;
+  RCHK(dmalloc(c, (void **)&p->d_cost4, b0));
+  RCHK(dmalloc(c, (void **)&p->d_rbv, b1));
+  RCHK(dmalloc(c, (void **)&p->d_fwd, b2));
+  RCHK(dmalloc(c, (void **)&p->d_vit, b3));
+  p->bytes = b0 + b1 + b2 + b3;
+  HIPCHK(hipMemcpy(p->d_cost4, cost4.data(), b0, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p->d_rbv, p->msv.rbv.data(), b1, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p->d_fwd, fwd.data(), b2, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(p->d_vit, vit.data(), b3, hipMemcpyHostToDevice));
+  *out = p.release();
+  return IXG_OK;
+}
+
+int ixg_p7_free(ixg_ctx *c, ixg_p7 *p) {
+  if (!p) return IXG_EINVAL;
+  if (!c) {
+    if (p->d_cost4) return IXG_EINVAL;  // device copy needs its context
+    delete p;
+    return IXG_OK;
+  }
+  dfree(c, p->d_cost4, (p->msv.m + 1) * 4);
+  dfree(c, p->d_rbv, p->msv.rbv.size());
+  dfree(c, p->d_fwd, (p->fwd.tmain.size() + p->fwd.tddv.size() + p->fwd.rfvv.size()) * 4);
+  dfree(c, p->d_vit, (p->vit.rwv.size() + 8 * (p->vit.m + 1)) * 2);
+  delete p;
+  return IXG_OK;
+}
+
+int ixg_p7_msv_tables(const ixg_p7 *p, uint8_t *rbv, uint8_t *bytes, float *prefix, float *suffix) {
+  if (!p) return IXG_EINVAL;
+  if (rbv) memcpy(rbv, p->msv.rbv.data(), p->msv.rbv.size());
+  if (bytes) {
+    memcpy(bytes, &p->msv.scale_b, 4);
+    bytes[4] = p->msv.base_b;
+    bytes[5] = p->msv.bias_b;
+    bytes[6] = p->msv.tbm_b;
+    bytes[7] = p->msv.tec_b;
+    bytes[8] = p->msv.tjb_b;
+  }
+  if (prefix) memcpy(prefix, p->msv.prefix.data(), p->msv.prefix.size() * 4);
+  if (suffix) memcpy(suffix, p->msv.suffix.data(), p->msv.suffix.size() * 4);
+  return IXG_OK;
+}
+
+int ixg_p7_vit_tables(const ixg_p7 *p, int16_t *rwv, int16_t *tr8, int16_t *ddb) {
+  if (!p) return IXG_EINVAL;
+  if (rwv) memcpy(rwv, p->vit.rwv.data(), p->vit.rwv.size() * 2);
+  if (tr8) {
+    int16_t *o = tr8;
+    for (auto *a : {&p->vit.tmm, &p->vit.tim, &p->vit.tdm, &p->vit.vbm, &p->vit.tmi, &p->vit.tii, &p->vit.tmd,
+                    &p->vit.tdd}) {
+      memcpy(o, a->data(), a->size() * 2);
+      o += a->size();
+    }
+  }
+  if (ddb) *ddb = p->vit.ddbound_w;
+  return IXG_OK;
+}
+
+int ixg_p7_fwd_tables(const ixg_p7 *p, int32_t *nq, float *tmain, float *tddv, float *rfvv) {
+  if (!p) return IXG_EINVAL;
+  if (nq) *nq = p->fwd.nq;
+  if (tmain) memcpy(tmain, p->fwd.tmain.data(), p->fwd.tmain.size() * 4);
+  if (tddv) memcpy(tddv, p->fwd.tddv.data(), p->fwd.tddv.size() * 4);
+  if (rfvv) memcpy(rfvv, p->fwd.rfvv.data(), p->fwd.rfvv.size() * 4);
+  return IXG_OK;
+}
+
+static bool codes_ok_pub(const uint8_t *p, int64_t n) {
+  for (int64_t i = 0; i < n; i++)
+    if (p[i] >= KP) {
+      g_err = "digitized residue code >= 18 in the sequence";
+      return false;
+    }
+  return true;
+}
+
+int ixg_target_create(ixg_ctx *c, const uint8_t *dsq, int64_t L, ixg_target **out) {
+  if (!c || !dsq || L < 0 || !out) return IXG_EINVAL;
+  if (!codes_ok_pub(dsq + 1, L)) return IXG_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  auto t = std::make_unique<ixg_target>();
+  t->L = L;
+  t->bytes = (size_t)2 * (L + 2);
+  RCHK(dmalloc(c, (void **)&t->d_both, t->bytes));
+  std::vector<uint8_t> h(L + 2, 255);
+  if (L) memcpy(h.data() + 1, dsq + 1, L);
+  HIPCHK(hipMemcpyAsync(t->d_both, h.data(), L + 2, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(launch_revcomp(t->d_both, t->d_both + L + 2, L, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *out = t.release();
+  return IXG_OK;
+}
+
+int ixg_target_free(ixg_ctx *c, ixg_target *t) {
+  if (!c || !t) return IXG_EINVAL;
+  dfree(c, t->d_both, t->bytes);
+  delete t;
+  return IXG_OK;
+}
+
+int64_t ixg_window_tasks(int64_t L, int64_t maxw, int do_top, int do_bot, ixg_task *out, int64_t cap) {
+  // cm_search.rs:1774-1808 (CM_MAX_RESIDUE_COUNT = 100000, cm_pipeline.rs:858)
+  int64_t n = 0, new_start = 1;
+  bool first = true;
+  while (new_start <= L) {
+    int64_t ctx = first ? 0 : std::min(maxw, new_start - 1);
+    int64_t wgs = new_start - ctx;
+    int64_t new_count = std::min<int64_t>(100000, L - new_start + 1);
+    int64_t wge = wgs + ctx + new_count - 1;
+    for (int s = 0; s < 2; s++) {
+      if ((s == 0 && !do_top) || (s == 1 && !do_bot)) continue;
+      if (n < cap && out) {
+        out[n].start = wgs;
+        out[n].end = wge;
+        out[n].strand = s;
+        out[n].ctx = (int32_t)ctx;
+      }
+      n++;
+    }
+    new_start = wge + 1;
+    first = false;
+  }
+  return n;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// cascade internals
+// ---------------------------------------------------------------------------
+namespace {
+
+struct HostTask {
+  int64_t base;  // offset of the task's dsq view (1-indexed) in the device sequence buffer
+  int64_t L;
+};
+
+struct RawW {
+  int64_t n, length, k;
+  float score;
+};
+struct Win {
+  int64_t start, end;
+  float score;
+};
+
+// cm_pipeline.rs:722-793
+std::vector<Win> extend_and_merge(const MsvHost &f, std::vector<RawW> raws, int64_t target_len) {
+  std::vector<Win> out;
+  if (raws.empty()) return out;
+  const double maxl = (double)f.max_length;
+  for (auto &w : raws) {
+    int64_t ks = std::clamp<int64_t>(w.k - w.length + 1, 0, f.m);
+    int64_t ke = std::clamp<int64_t>(w.k, 0, f.m);
+    double pref = (double)f.prefix[ks], suff = (double)f.suffix[ke];
+    double ws = std::max((double)w.n - maxl * (0.1 + pref), 1.0);
+    double we = std::min((double)w.n + (double)w.length + maxl * (0.1 + suff), (double)target_len);
+    int64_t s = (int64_t)ws, e = (int64_t)we;
+    w.n = s;
+    w.length = e - s + 1;
+  }
+  std::vector<RawW> kept;
+  kept.reserve(raws.size());
+  kept.push_back(raws[0]);
+  for (size_t i = 1; i < raws.size(); i++) {
+    const RawW cur = raws[i];
+    RawW &prev = kept.back();
+    int64_t ws = std::max(prev.n, cur.n);
+    int64_t pe = prev.n + prev.length - 1, ce = cur.n + cur.length - 1;
+    int64_t wl = std::min(pe, ce) - ws + 1;
+    int64_t minlen = std::min(prev.length, cur.length);
+    if ((float)wl / (float)minlen > 0.0f) {
+      int64_t ms = std::min(prev.n, cur.n), me = std::max(pe, ce);
+      prev.n = ms;
+      prev.length = me - ms + 1;
+    } else {
+      kept.push_back(cur);
+    }
+  }
+  for (auto &k : kept) out.push_back({k.n, k.n + k.length - 1, k.score});
+  return out;
+}
+
+// cm_pipeline.rs:817-837
+std::vector<Win> split_windows(const std::vector<Win> &in, int64_t cmw) {
+  std::vector<Win> out;
+  for (const auto &w : in) {
+    int64_t wlen = w.end - w.start + 1;
+    if (wlen > 2 * cmw) {
+      int64_t nws = w.start, nwe = std::min(w.start + 2 * cmw - 1, w.end);
+      out.push_back({nws, nwe, w.score});
+      while (nwe < w.end) {
+        nws = std::min(nws + cmw, w.end);
+        nwe = std::min(nwe + cmw, w.end);
+        out.push_back({nws, nwe, w.score});
+      }
+    } else {
+      out.push_back(w);
+    }
+  }
+  return out;
+}
+
+struct Timer {
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+  double ms() const {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+};
+
+// Runs F1 for all tasks on the device; raws[t] gets task t's diagonals.
+int run_f1(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<HostTask> &tasks, double f1,
+           std::vector<std::vector<RawW>> &raws) {
+  const SsvDev sd = p->ssv_dev();
+  const int thresh = ssv_thresh(p->msv, f1);
+  std::vector<TaskDev> td(tasks.size());
+  std::vector<SweepItem> items;
+  int64_t mask_words = 0;
+  for (size_t t = 0; t < tasks.size(); t++) {
+    td[t].dsq = d_seq + tasks[t].base;
+    td[t].L = tasks[t].L;
+    td[t].mask_off = mask_words;
+    mask_words += (tasks[t].L + 63) / 64;
+    const int64_t nseg = (tasks[t].L + SWEEP_SEG - 1) / SWEEP_SEG;
+    for (int64_t s = 0; s < nseg; s++) items.push_back({(int32_t)t, (int32_t)s});
+  }
+  TaskDev *d_tasks;
+  SweepItem *d_items;
+  RCHK(upload(c, S_TASKS, td, &d_tasks));
+  RCHK(upload(c, S_ITEMS, items, &d_items));
+  uint64_t *d_mask;
+  RCHK(scratch(c, S_MASK, (size_t)std::max<int64_t>(mask_words, 1) * 8, (void **)&d_mask));
+  int64_t total_len = 0;
+  for (auto &t : tasks) total_len += t.L;
+  int64_t cap = total_len / 32 + 256 * (int64_t)tasks.size() + 1024;
+  unsigned long long *d_counter;
+  RCHK(scratch(c, S_COUNTER, 8, (void **)&d_counter));
+  HIPCHK(hipEventRecord(c->ev[0], c->stream));
+  HIPCHK(launch_ssv_sweep(sd, d_tasks, d_items, (int64_t)items.size(), d_mask, thresh, c->stream));
+  HIPCHK(hipEventRecord(c->ev[1], c->stream));
+  RawWinDev *d_raw = nullptr;
+  unsigned long long count = 0;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    RCHK(scratch(c, S_RAW, (size_t)cap * sizeof(RawWinDev), (void **)&d_raw));
+    static const unsigned long long zero = 0;
+    HIPCHK(hipMemcpyAsync(d_counter, &zero, 8, hipMemcpyHostToDevice, c->stream));
+    if (attempt) HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(launch_ssv_resolve(sd, d_tasks, (int64_t)tasks.size(), d_mask, thresh, d_raw, cap, d_counter,
+                              c->stream));
+    HIPCHK(hipEventRecord(c->ev[2], c->stream));
+    HIPCHK(hipMemcpyAsync(&count, d_counter, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    if ((int64_t)count <= cap) break;
+    cap = (int64_t)count + 1024;
+  }
+  std::vector<RawWinDev> h(count);
+  if (count) HIPCHK(hipMemcpy(h.data(), d_raw, count * sizeof(RawWinDev), hipMemcpyDeviceToHost));
+  raws.assign(tasks.size(), {});
+  for (const auto &r : h) raws[r.task].push_back({r.n, r.length, r.k, r.score});
+  float a = 0, b = 0;
+  HIPCHK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
+  HIPCHK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+  c->ms[0] += a;
+  c->ms[1] += b;
+  c->cells[0] += total_len * p->msv.m;
+  return IXG_OK;
+}
+
+struct WinRef {
+  int32_t task;
+  int64_t start, end;  // chunk-local window
+};
+
+// Scores for a list of windows: kind 0 fwd, 1 bias, 2 vit, 3 msv (F1b).
+int run_scores(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<HostTask> &tasks,
+               const std::vector<WinRef> &wl, int kind, std::vector<float> &sc, std::vector<uint8_t> &ovf) {
+  const int64_t n = (int64_t)wl.size();
+  sc.assign(n, 0.f);
+  ovf.assign(n, 0);
+  if (n == 0) return IXG_OK;
+  std::vector<WinDev> wd(n);
+  for (int64_t i = 0; i < n; i++) {
+    const WinRef &w = wl[i];
+    const int64_t len = w.end - w.start + 1;
+    wd[i].off = tasks[w.task].base + w.start - 1;
+    wd[i].len = (int32_t)len;
+    wd[i].idx = (int32_t)i;
+    wd[i].a = wd[i].b = 0.f;
+    if (kind == 1) {  // bias tail: L*ln(p1) and ln(1-p1) in f32 (cm_pipeline.rs:1773)
+      const float p1 = (float)len / ((float)len + 1.0f);
+      wd[i].a = (float)len * std::log(p1);
+      wd[i].b = std::log(1.0f - p1);
+    } else if (kind == 2) {  // xw_move (p7_vitfilter.rs:311-312)
+      const float pmove = 3.0f / ((float)len + 3.0f);
+      wd[i].a = (float)wordify(p->vit.scale_w, std::log(pmove));
+    } else if (kind == 3) {  // reconfigured tjb_b (cm_pipeline.rs:427) and tec_b
+      wd[i].a = (float)unbiased_byteify(p->msv.scale_b, (float)std::log(3.0 / ((double)len + 3.0)));
+      wd[i].b = (float)p->msv.tec_b;
+    }
+  }
+  // longest first, so waves that share windows see similar lengths
+  std::stable_sort(wd.begin(), wd.end(), [](const WinDev &a, const WinDev &b) { return a.len > b.len; });
+  WinDev *d_w;
+  RCHK(upload(c, S_WINS, wd, &d_w));
+  float *d_sc;
+  uint8_t *d_ovf;
+  RCHK(scratch(c, S_SC0, n * 4, (void **)&d_sc));
+  RCHK(scratch(c, S_OVF0, n, (void **)&d_ovf));
+  hipError_t e = hipSuccess;
+  int64_t cells = 0;
+  for (auto &w : wd) cells += w.len;
+  if (kind == 0) e = launch_fwd(p->fwd_dev(), d_seq, d_w, n, d_sc, c->stream);
+  else if (kind == 1) e = launch_bias(p->bias_dev(), d_seq, d_w, n, d_sc, c->stream);
+  else if (kind == 2) e = launch_vit(p->vit_dev(), d_seq, d_w, n, d_sc, d_ovf, c->stream);
+  else e = launch_msv(p->ssv_dev(), d_seq, d_w, n, d_sc, d_ovf, c->stream);
+  HIPCHK(e);
+  HIPCHK(hipMemcpyAsync(sc.data(), d_sc, n * 4, hipMemcpyDeviceToHost, c->stream));
+  if (kind >= 2) HIPCHK(hipMemcpyAsync(ovf.data(), d_ovf, n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->cells[1 + (kind == 0 ? 0 : (kind == 1 ? 1 : 2))] += cells * (kind == 1 ? 1 : p->msv.m);
+  return IXG_OK;
+}
+
+static const double LN2 = 0.693147180559945309417232121458176568;
+
+// Host side of f3_filter_sequence (cm_pipeline.rs:1863-2164) for many tasks.
+int cascade(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<HostTask> &tasks,
+            const ixg_f3opts &o, std::vector<std::vector<Win>> &res, std::vector<ixg_acct> &accts) {
+  const bool do_msv = o.opts & IXG_DO_MSV, do_fwd = o.opts & IXG_DO_FWD, do_fwdbias = o.opts & IXG_DO_FWDBIAS;
+  const bool do_vit = o.opts & IXG_DO_VIT, do_vitbias = o.opts & IXG_DO_VITBIAS,
+             do_msvbias = o.opts & IXG_DO_MSVBIAS;
+  const size_t nt = tasks.size();
+  std::vector<std::vector<Win>> wins(nt);
+  if (do_msv) {
+    std::vector<std::vector<RawW>> raws;
+    RCHK(run_f1(c, p, d_seq, tasks, o.f1, raws));
+    Timer tm;
+    for (size_t t = 0; t < nt; t++) wins[t] = split_windows(extend_and_merge(p->msv, raws[t], tasks[t].L), o.cmw);
+    c->ms[3] += tm.ms();
+  } else {  // deterministic 2*maxW tiling (cm_pipeline.rs:1893-1912)
+    for (size_t t = 0; t < nt; t++) {
+      const int64_t n = tasks[t].L, mw = o.maxw;
+      int64_t nwin = 1;
+      if (n > 2 * mw) {
+        nwin += (n - 2 * mw) / (mw + 1);
+        if ((n - 2 * mw) % (mw + 1) > 0) nwin++;
+      }
+      for (int64_t i = 0; i < nwin; i++) {
+        int64_t ws = 1 + i * (mw + 1);
+        wins[t].push_back({ws, std::min(ws + 2 * mw - 1, n), 0.f});
+      }
+    }
+  }
+  // flat window list + per-window survival flags
+  std::vector<WinRef> all;
+  std::vector<size_t> first(nt + 1, 0);
+  for (size_t t = 0; t < nt; t++) {
+    first[t] = all.size();
+    for (auto &w : wins[t]) all.push_back({(int32_t)t, w.start, w.end});
+  }
+  first[nt] = all.size();
+  const size_t N = all.size();
+  std::vector<uint8_t> s_f1b(N, 1), s_f2(N, 1), s_f2b(N, 1), s_f3(N, 0), s_f3b(N, 0), alive(N, 1);
+  std::vector<float> wb(N, -999.0f), biassc(N, 0.f), fwdsc(N, 0.f);
+  std::vector<uint8_t> have_bias(N, 0);
+  auto alive_list = [&](std::vector<size_t> &idx, std::vector<WinRef> &sub) {
+    idx.clear();
+    sub.clear();
+    for (size_t i = 0; i < N; i++)
+      if (alive[i]) {
+        idx.push_back(i);
+        sub.push_back(all[i]);
+      }
+  };
+  auto ensure_bias = [&](const std::vector<size_t> &idx) -> int {
+    std::vector<size_t> need;
+    std::vector<WinRef> sub;
+    for (size_t i : idx)
+      if (!have_bias[i]) {
+        need.push_back(i);
+        sub.push_back(all[i]);
+      }
+    std::vector<float> sc;
+    std::vector<uint8_t> ov;
+    RCHK(run_scores(c, p, d_seq, tasks, sub, 1, sc, ov));
+    for (size_t j = 0; j < need.size(); j++) {
+      biassc[need[j]] = sc[j];
+      have_bias[need[j]] = 1;
+    }
+    return IXG_OK;
+  };
+  std::vector<size_t> idx;
+  std::vector<WinRef> sub;
+  std::vector<float> sc;
+  std::vector<uint8_t> ov;
+  hipEvent_t e0 = c->ev[3], e1 = c->ev[4];
+  HIPCHK(hipEventRecord(e0, c->stream));
+  if (do_msvbias) {  // F1b (cm_pipeline.rs:1947-1958)
+    alive_list(idx, sub);
+    RCHK(run_scores(c, p, d_seq, tasks, sub, 3, sc, ov));
+    RCHK(ensure_bias(idx));
+    for (size_t j = 0; j < idx.size(); j++) {
+      if (ov[j]) continue;
+      const size_t i = idx[j];
+      double w = ((double)sc[j] - (double)biassc[i]) / LN2;
+      // the MSV-bias Gumbel uses the p7's local MSV parameters
+      if (gumbel_surv(w, p->msv.mmu, p->msv.mlambda) > o.f1b) {
+        s_f1b[i] = 0;
+        alive[i] = 0;
+      }
+    }
+  }
+  if (do_vit) {  // F2/F2b (cm_pipeline.rs:1966-1997)
+    alive_list(idx, sub);
+    RCHK(run_scores(c, p, d_seq, tasks, sub, 2, sc, ov));
+    std::vector<size_t> pass;
+    for (size_t j = 0; j < idx.size(); j++) {
+      const size_t i = idx[j];
+      if (ov[j]) continue;
+      const int64_t wlen = all[i].end - all[i].start + 1;
+      double w = ((double)sc[j] - bg_null_one(wlen)) / LN2;
+      if (gumbel_surv(w, p->vit_lvmu, p->vit_lvlambda) > o.f2) {
+        s_f2[i] = s_f2b[i] = 0;
+        alive[i] = 0;
+        continue;
+      }
+      if (do_vitbias) pass.push_back(j);
+    }
+    if (do_vitbias) {
+      std::vector<size_t> bidx;
+      for (size_t j : pass) bidx.push_back(idx[j]);
+      RCHK(ensure_bias(bidx));
+      for (size_t j : pass) {
+        const size_t i = idx[j];
+        double w = ((double)sc[j] - (double)biassc[i]) / LN2;
+        if (gumbel_surv(w, p->vit_lvmu, p->vit_lvlambda) > o.f2b) {
+          s_f2b[i] = 0;
+          alive[i] = 0;
+        }
+      }
+    }
+  }
+  if (do_fwd) {  // F3/F3b (cm_pipeline.rs:2013-2052)
+    alive_list(idx, sub);
+    RCHK(run_scores(c, p, d_seq, tasks, sub, 0, sc, ov));
+    std::vector<size_t> pass;
+    for (size_t j = 0; j < idx.size(); j++) {
+      const size_t i = idx[j];
+      const int64_t wlen = all[i].end - all[i].start + 1;
+      fwdsc[i] = sc[j];
+      double w = ((double)sc[j] - bg_null_one(wlen)) / LN2;
+      double pv = exp_surv(w, p->fwd.ftau, p->fwd.flambda);
+      wb[i] = (float)w;
+      if (pv > o.f3) {
+        alive[i] = 0;
+        continue;
+      }
+      s_f3[i] = 1;
+      pass.push_back(i);
+    }
+    if (do_fwdbias) {
+      RCHK(ensure_bias(pass));
+      for (size_t i : pass) {
+        double w = ((double)fwdsc[i] - (double)biassc[i]) / LN2;
+        double pv = exp_surv(w, p->fwd.ftau, p->fwd.flambda);
+        wb[i] = (float)w;
+        if (pv > o.f3b) {
+          alive[i] = 0;
+          continue;
+        }
+        s_f3b[i] = 1;
+      }
+    } else {
+      for (size_t i : pass) s_f3b[i] = 1;
+    }
+  } else {
+    for (size_t i = 0; i < N; i++)
+      if (alive[i]) s_f3[i] = s_f3b[i] = 1;
+  }
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  float wms = 0;
+  HIPCHK(hipEventElapsedTime(&wms, e0, e1));
+  c->ms[2] += wms;
+  Timer tm;
+  // survivors, merge of overlapping/abutting windows (cm_pipeline.rs:2056-2114)
+  res.assign(nt, {});
+  accts.assign(nt, ixg_acct{});
+  for (size_t t = 0; t < nt; t++) {
+    std::vector<Win> sv;
+    for (size_t i = first[t]; i < first[t + 1]; i++)
+      if (s_f3b[i]) sv.push_back({all[i].start, all[i].end, wb[i]});
+    size_t i = 0;
+    while (i < sv.size()) {
+      size_t i2 = i + 1;
+      while (i2 < sv.size() && sv[i].end + 1 >= sv[i2].start) {
+        sv[i].end = sv[i2].end;
+        sv[i].score = std::max(sv[i].score, sv[i2].score);
+        i2++;
+      }
+      res[t].push_back(sv[i]);
+      i = i2;
+    }
+    ixg_acct &a = accts[t];  // cm_pipeline.rs:2116-2164
+    a.n_past_msv = first[t + 1] - first[t];
+    for (size_t k = first[t]; k < first[t + 1]; k++) {
+      const uint64_t wl = (uint64_t)(all[k].end - all[k].start + 1);
+      a.pos_past_msv += wl;
+      if (s_f1b[k]) { a.n_past_msvbias++; a.pos_past_msvbias += wl; }
+      if (s_f2[k]) { a.n_past_vit++; a.pos_past_vit += wl; }
+      if (s_f2b[k]) { a.n_past_vitbias++; a.pos_past_vitbias += wl; }
+      if (s_f3[k]) { a.n_past_fwd++; a.pos_past_fwd += wl; }
+      if (s_f3b[k]) { a.n_past_fwdbias++; a.pos_past_fwdbias += wl; }
+      if (k > first[t]) {
+        const int64_t ovl = all[k - 1].end - all[k].start + 1;
+        if (ovl > 0) {
+          const uint64_t ovv = (uint64_t)ovl;
+          a.pos_past_msv -= ovv;
+          if (s_f1b[k] && s_f1b[k - 1]) a.pos_past_msvbias -= ovv;
+          if (s_f2[k] && s_f2[k - 1]) a.pos_past_vit -= ovv;
+          if (s_f2b[k] && s_f2b[k - 1]) a.pos_past_vitbias -= ovv;
+          if (s_f3[k] && s_f3[k - 1]) a.pos_past_fwd -= ovv;
+          if (s_f3b[k] && s_f3b[k - 1]) a.pos_past_fwdbias -= ovv;
+        }
+      }
+    }
+  }
+  c->ms[3] += tm.ms();
+  return IXG_OK;
+}
+
+// Residue codes must index the 18-symbol tables (easel/alphabet.rs:46-157).
+bool codes_ok(const uint8_t *p, int64_t n) {
+  uint8_t acc = 0;
+  for (int64_t i = 0; i < n; i++) acc |= (uint8_t)(p[i] >= KP);
+  if (acc) g_err = "digitized residue code >= 18 in the sequence";
+  return !acc;
+}
+
+// Uploads a host 1-indexed sequence into a scratch slot (sentinel-padded).
+int upload_seq(ixg_ctx *c, const uint8_t *dsq, int64_t L, uint8_t **out) {
+  if (!codes_ok(dsq + 1, L)) return IXG_EINVAL;
+  uint8_t *d;
+  RCHK(scratch(c, S_SEQ, (size_t)L + 2, (void **)&d));
+  std::vector<uint8_t> h(L + 2, 255);
+  if (L) memcpy(h.data() + 1, dsq + 1, L);
+  HIPCHK(hipMemcpyAsync(d, h.data(), L + 2, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *out = d;
+  return IXG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ixg_ssv_diagonals(ixg_ctx *c, const ixg_p7 *p, const uint8_t *dsq, int64_t L, double f1, ixg_rawwin *out,
+                      int64_t cap, int64_t *n) {
+  if (!c || !p || !dsq || L < 0 || !n || (cap > 0 && !out)) return IXG_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  uint8_t *d;
+  RCHK(upload_seq(c, dsq, L, &d));
+  std::vector<std::vector<RawW>> raws;
+  RCHK(run_f1(c, p, d, {{0, L}}, f1, raws));
+  const auto &r = raws[0];
+  *n = (int64_t)r.size();
+  for (int64_t i = 0; i < std::min<int64_t>(cap, r.size()); i++)
+    out[i] = {r[i].n, r[i].length, r[i].k, r[i].score, 0};
+  return (int64_t)r.size() > cap ? IXG_ERANGE : IXG_OK;
+}
+
+static int scores_entry(ixg_ctx *c, const ixg_p7 *p, const uint8_t *seq, int64_t seq_len, const int64_t *off,
+                        const int64_t *len, int64_t nwin, float *sc, uint8_t *ovf, int kind) {
+  if (!c || !p || !seq || !off || !len || !sc || nwin < 0) return IXG_EINVAL;
+  for (int64_t i = 0; i < nwin; i++)
+    if (off[i] < 0 || len[i] < 1 || off[i] + len[i] >= seq_len || len[i] > INT32_MAX) {
+      g_err = "window outside the sequence buffer";
+      return IXG_EINVAL;
+    }
+  if (!codes_ok(seq, seq_len)) {
+    // only window residues are read; check exactly those
+    for (int64_t i = 0; i < nwin; i++)
+      if (!codes_ok(seq + off[i] + 1, len[i])) return IXG_EINVAL;
+    g_err.clear();
+  }
+  HIPCHK(hipSetDevice(c->device));
+  uint8_t *d;
+  RCHK(scratch(c, S_SEQ, (size_t)seq_len + 2, (void **)&d));
+  HIPCHK(hipMemcpyAsync(d, seq, seq_len, hipMemcpyHostToDevice, c->stream));
+  std::vector<HostTask> tasks{{0, seq_len}};
+  std::vector<WinRef> wl(nwin);
+  for (int64_t i = 0; i < nwin; i++) wl[i] = {0, off[i] + 1, off[i] + len[i]};
+  std::vector<float> s;
+  std::vector<uint8_t> o;
+  RCHK(run_scores(c, p, d, tasks, wl, kind, s, o));
+  memcpy(sc, s.data(), nwin * 4);
+  if (ovf) memcpy(ovf, o.data(), nwin);
+  return IXG_OK;
+}
+
+int ixg_vit_scores(ixg_ctx *c, const ixg_p7 *p, const uint8_t *seq, int64_t seq_len, const int64_t *off,
+                   const int64_t *len, int64_t nwin, float *sc, uint8_t *ovf) {
+  return scores_entry(c, p, seq, seq_len, off, len, nwin, sc, ovf, 2);
+}
+int ixg_fwd_scores(ixg_ctx *c, const ixg_p7 *p, const uint8_t *seq, int64_t seq_len, const int64_t *off,
+                   const int64_t *len, int64_t nwin, float *sc) {
+  return scores_entry(c, p, seq, seq_len, off, len, nwin, sc, nullptr, 0);
+}
+int ixg_bias_scores(ixg_ctx *c, const ixg_p7 *p, const uint8_t *seq, int64_t seq_len, const int64_t *off,
+                    const int64_t *len, int64_t nwin, float *sc) {
+  return scores_entry(c, p, seq, seq_len, off, len, nwin, sc, nullptr, 1);
+}
+int ixg_msv_scores(ixg_ctx *c, const ixg_p7 *p, const uint8_t *seq, int64_t seq_len, const int64_t *off,
+                   const int64_t *len, int64_t nwin, float *sc, uint8_t *ovf) {
+  return scores_entry(c, p, seq, seq_len, off, len, nwin, sc, ovf, 3);
+}
+
+static int emit(const std::vector<std::vector<Win>> &res, ixg_window *out, int64_t cap, int64_t *n) {
+  int64_t k = 0;
+  for (size_t t = 0; t < res.size(); t++)
+    for (auto &w : res[t]) {
+      if (k < cap && out) out[k] = {w.start, w.end, w.score, (int32_t)t};
+      k++;
+    }
+  *n = k;
+  return k > cap ? IXG_ERANGE : IXG_OK;
+}
+
+int ixg_f3_filter_sequence(ixg_ctx *c, const ixg_p7 *p, const uint8_t *dsq, int64_t L, const ixg_f3opts *o,
+                           ixg_window *out, int64_t cap, int64_t *n, ixg_acct *acct) {
+  if (!c || !p || !dsq || !o || !n || L < 0) return IXG_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  uint8_t *d;
+  RCHK(upload_seq(c, dsq, L, &d));
+  std::vector<std::vector<Win>> res;
+  std::vector<ixg_acct> accts;
+  RCHK(cascade(c, p, d, {{0, L}}, *o, res, accts));
+  if (acct) *acct = accts[0];
+  return emit(res, out, cap, n);
+}
+
+int ixg_f3_filter_tasks(ixg_ctx *c, const ixg_p7 *p, const ixg_target *tgt, const ixg_task *tasks, int64_t nt,
+                        const ixg_f3opts *o, ixg_window *out, int64_t cap, int64_t *n, ixg_acct *acct) {
+  if (!c || !p || !tgt || (!tasks && nt) || !o || !n) return IXG_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  Timer total;
+  for (int i = 0; i < 5; i++) c->ms[i] = 0, c->cells[i] = 0;
+  std::vector<HostTask> ht(nt);
+  for (int64_t t = 0; t < nt; t++) {
+    const ixg_task &k = tasks[t];
+    if (k.start < 1 || k.end > tgt->L || k.end < k.start || (k.strand != 0 && k.strand != 1)) {
+      g_err = "task outside the target";
+      return IXG_EINVAL;
+    }
+    // top strand: dsq[1] = seq[start]; bottom: dsq[k] = comp(seq[end-k+1]) = rc[L-end+k]
+    ht[t].base = k.strand == 0 ? (k.start - 1) : (tgt->L + 2) + (tgt->L - k.end);
+    ht[t].L = k.end - k.start + 1;
+  }
+  std::vector<std::vector<Win>> res;
+  std::vector<ixg_acct> accts;
+  RCHK(cascade(c, p, tgt->d_both, ht, *o, res, accts));
+  if (acct)
+    for (int64_t t = 0; t < nt; t++) acct[t] = accts[t];
+  c->ms[4] = total.ms();
+  return emit(res, out, cap, n);
+}
+
+int ixg_last_timings(const ixg_ctx *c, double *ms5, int64_t *cells5) {
+  if (!c) return IXG_EINVAL;
+  if (ms5) memcpy(ms5, c->ms, sizeof(c->ms));
+  if (cells5) memcpy(cells5, c->cells, sizeof(c->cells));
+  return IXG_OK;
+}
+
+}  // extern "C"

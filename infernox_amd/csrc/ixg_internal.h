@@ -1,0 +1,121 @@
+// Internal declarations shared by the host glue and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+#include "../../include/ixg.h"
+
+namespace ixg {
+
+constexpr int KP = IXG_KP;
+constexpr int16_t NEGINF_W = -32768;
+
+// Host-built p7 filter profiles (natural order + striped packs).
+struct MsvHost {
+  int m = 0;
+  float scale_b = 0.f;
+  uint8_t base_b = 0, bias_b = 0, tbm_b = 0, tec_b = 0, tjb_b = 0;
+  std::vector<uint8_t> rbv;          // (m+1)*KP
+  std::vector<float> prefix, suffix; // m+1
+  double mmu = 0, mlambda = 0;
+  int max_length = 0;
+};
+struct VitHost {
+  int m = 0;
+  float scale_w = 0.f;
+  int16_t base_w = 0;
+  std::vector<int16_t> rwv;          // (m+1)*KP
+  std::vector<int16_t> tmm, tim, tdm, vbm, tmi, tii, tmd, tdd;
+  int16_t xw_e_move = 0, xw_e_loop = 0;
+  int16_t ddbound_w = 0;
+};
+struct FwdHost {
+  int m = 0, nq = 0;
+  std::vector<float> rfv;            // (m+1)*KP
+  std::vector<float> tbm, amm, aim, adm, tmi, tii, tmd, tdd;
+  std::vector<float> tmain, tddv, rfvv;
+  double ftau = 0, flambda = 0;
+};
+struct BiasHost {
+  float eo[KP][2];
+  float t1[3];
+  float pi[2];
+};
+
+void build_msv(const ixg_p7_hmm &h, int max_length, MsvHost &f);
+void build_vit(const ixg_p7_hmm &h, VitHost &f);
+void build_fwd(const ixg_p7_hmm &h, FwdHost &f);
+void build_bias(const float compo[4], int m, BiasHost &b);
+int ssv_thresh(const MsvHost &f, double f1);
+uint8_t unbiased_byteify(float scale_b, float sc);
+int16_t wordify(float scale_w, float sc);
+double gumbel_surv(double x, double mu, double lambda);
+double exp_surv(double x, double mu, double lambda);
+double bg_null_one(int64_t L);
+
+// Device-side views (all pointers are device pointers).
+struct SsvDev {
+  const uint32_t *cost4;   // (m+1): bytes = rbv[k][A,C,G,U]
+  const uint8_t *rbv;      // (m+1)*KP
+  int m;
+  uint8_t xb, bias, base_b, tjb_b, tbm_b;
+  float scale_b;
+};
+struct FwdDev {
+  const float *tmain, *tddv, *rfvv;
+  int m, nq;
+};
+struct VitDev {
+  const int16_t *rwv;      // (m+1)*KP
+  const int16_t *tr;       // 8*(m+1): tmm,tim,tdm,vbm,tmi,tii,tmd,tdd
+  int m;
+  int16_t base_w, xw_e_move, xw_e_loop;
+  float scale_w;
+};
+struct BiasDev {
+  float eo[KP][2];
+  float t1[3];
+  float pi[2];
+};
+
+// One F1 sweep work item: rows [s0, s0+seg) of task t, plus warm-up.
+struct SweepItem {
+  int32_t task;
+  int32_t seg;      // segment index within the task
+};
+struct TaskDev {
+  const uint8_t *dsq;  // 1-indexed view: dsq[1..L]
+  int64_t L;
+  int64_t mask_off;    // offset (in u64 words) of the task's candidate bitmask
+};
+struct RawWinDev {
+  int64_t n, length, k;
+  float score;
+  int32_t task;
+};
+struct WinDev {
+  int64_t off;   // seq + off is the window's 1-indexed dsq
+  int32_t len;
+  int32_t idx;   // index into the output score arrays
+  float a, b;    // per-window host-computed terms (length model pieces)
+};
+
+// Kernel launchers (p7_kernels.hip).
+constexpr int SWEEP_SEG = 4096;  // rows per F1 sweep work item (multiple of 64)
+hipError_t launch_ssv_sweep(const SsvDev &p, const TaskDev *tasks, const SweepItem *items,
+                            int64_t nitems, uint64_t *mask, int thresh, hipStream_t s);
+hipError_t launch_ssv_resolve(const SsvDev &p, const TaskDev *tasks, int64_t ntasks,
+                              const uint64_t *mask, int thresh, RawWinDev *out, int64_t cap,
+                              unsigned long long *counter, hipStream_t s);
+hipError_t launch_fwd(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin,
+                      float *sc, hipStream_t s);
+hipError_t launch_bias(const BiasDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin,
+                       float *sc, hipStream_t s);
+hipError_t launch_vit(const VitDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin,
+                      float *sc, uint8_t *ovf, hipStream_t s);
+hipError_t launch_msv(const SsvDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin,
+                      float *sc, uint8_t *ovf, hipStream_t s);
+hipError_t launch_revcomp(const uint8_t *fwd, uint8_t *rc, int64_t L, hipStream_t s);
+
+}  // namespace ixg

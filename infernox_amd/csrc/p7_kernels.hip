@@ -1,0 +1,637 @@
+// gfx950 kernels for the p7 filter stages: F1 SSV (speculative sweep +
+// exact per-chunk resolve), F2 Viterbi, F3 Forward (striped order), F3b
+// composition bias. All integer paths are exact; float paths reproduce the
+// reference's evaluation order (built with -ffp-contract=off).
+#include "ixg_internal.h"
+
+namespace ixg {
+
+#define DPP_WAVE_SHR1 0x138
+#define DPP_QUAD_RSZ 0x90   // quad_perm [0,0,1,2]
+#define DPP_QUAD_X1 0xB1    // quad_perm [1,0,3,2]
+#define DPP_QUAD_X2 0x4E    // quad_perm [2,3,0,1]
+
+__device__ __forceinline__ uint32_t ssv_cost(const SsvDev &p, uint32_t c4, int k, int x) {
+  if (x < 4) return (c4 >> (8 * x)) & 0xFFu;
+  return x < KP ? (uint32_t)p.rbv[k * KP + x] : 255u;
+}
+
+// One SSV cell: subs(adds(max(prev, xb), bias), cost) on u8 values held in u32.
+__device__ __forceinline__ uint32_t ssv_cell(uint32_t prev, uint32_t xb, uint32_t bias, uint32_t c) {
+  uint32_t t = prev > xb ? prev : xb;
+  t = t + bias;
+  t = t < 255u ? t : 255u;
+  return t > c ? t - c : 0u;
+}
+
+// ---------------------------------------------------------------------------
+// F1 sweep. Each 64-lane wave owns SWEEP_SEG rows of one task and computes,
+// for every row, the SSV row maximum WITHOUT the reference's reset-and-skip
+// (cm_pipeline.rs:703-712). A cell (i,k) depends only on residues
+// i-k+1..i (the diagonal enters at k=1 from the zero column), so starting the
+// sweep M-1 rows early with a zero row reproduces every row of the segment.
+// Resetting only lowers cells (max/adds/subs are monotone and the reset value
+// is the minimum), so rows whose reset-free maximum stays below the threshold
+// can never trigger; the others are flagged in a bitmask for the resolve
+// kernel. Lane = row; the diagonal dependency (i-1,k-1) -> (i,k) is a
+// one-lane wave shift, and lane 0 takes its input from the previous tile's
+// lane 63 through a small LDS carry row.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void ssv_sweep_kernel(SsvDev p, const TaskDev *tasks,
+                                                       const SweepItem *items, uint64_t *mask,
+                                                       int thresh) {
+  extern __shared__ uint8_t carry[];  // 2 * (m+1)
+  const SweepItem it = items[blockIdx.x];
+  const TaskDev t = tasks[it.task];
+  const int m = p.m;
+  const int lane = threadIdx.x;
+  const int64_t s0 = 1 + (int64_t)it.seg * SWEEP_SEG;
+  const int64_t s1 = min(s0 + (int64_t)SWEEP_SEG, t.L + 1);
+  const int64_t warm = (int64_t)((m - 1 + 63) / 64) * 64;
+  int64_t w0 = s0 - warm;
+  if (w0 < 1) w0 = 1;
+  for (int k = lane; k <= m; k += 64) carry[k] = 0;
+  __syncthreads();
+  const uint32_t xb = p.xb, bias = p.bias;
+  int cb = 0;
+  for (int64_t b = w0; b < s1; b += 64) {
+    const int64_t i = b + lane;
+    const bool valid = i < s1;
+    const int x = valid ? (int)t.dsq[i] : 0;
+    uint8_t *cprev = carry + cb * (m + 1);
+    uint8_t *cnext = carry + (cb ^ 1) * (m + 1);
+    uint32_t cur = 0, rowmax = 0;
+    for (int k = 1; k <= m; k++) {
+      const uint32_t cp = cprev[k - 1];
+      const uint32_t nb = (uint32_t)__builtin_amdgcn_update_dpp((int)cp, (int)cur, DPP_WAVE_SHR1, 0xF, 0xF, false);
+      const uint32_t c = ssv_cost(p, p.cost4[k], k, x);
+      cur = ssv_cell(nb, xb, bias, c);
+      rowmax = cur > rowmax ? cur : rowmax;
+      if (lane == 63) cnext[k] = (uint8_t)cur;
+    }
+    if (lane == 63) cnext[0] = 0;
+    __syncthreads();
+    cb ^= 1;
+    if (b >= s0) {
+      const uint64_t bits = __ballot(valid && (int)rowmax >= thresh);
+      if (lane == 0) mask[t.mask_off + (b - 1) / 64] = bits;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// F1 resolve: one wave per task walks the flagged rows in order and replays
+// the reference's sequential logic exactly (cm_pipeline.rs:521-717): rows are
+// recomputed from the last restart (or from M-1 rows back, which yields the
+// same row because no cell reaches further back), the first flagged row whose
+// true maximum reaches the threshold triggers, and the diagonal is recovered
+// backward and extended forward on lane 0. Lanes hold KPL consecutive model
+// positions each.
+// ---------------------------------------------------------------------------
+template <int KPL>
+__global__ __launch_bounds__(64) void ssv_resolve_kernel(SsvDev p, const TaskDev *tasks,
+                                                         const uint64_t *mask, int thresh,
+                                                         RawWinDev *out, int64_t cap,
+                                                         unsigned long long *counter) {
+  const TaskDev t = tasks[blockIdx.x];
+  const int m = p.m;
+  const int lane = threadIdx.x;
+  const int kb = lane * KPL + 1;
+  uint32_t c4[KPL], d[KPL];
+#pragma unroll
+  for (int j = 0; j < KPL; j++) {
+    c4[j] = (kb + j <= m) ? p.cost4[kb + j] : 0xFFFFFFFFu;
+    d[j] = 0;
+  }
+  const uint32_t xb = p.xb, bias = p.bias;
+  const int64_t L = t.L;
+  const int64_t nwords = (L + 63) / 64;
+  int64_t r = 1, pos = 1;
+  for (int64_t w = 0; w < nwords; w++) {
+    uint64_t bits = mask[t.mask_off + w];
+    while (bits) {
+      const int bi = __builtin_ctzll(bits);
+      bits &= bits - 1;
+      const int64_t c = 64 * w + 1 + bi;
+      if (c < r) continue;
+      if (c - pos + 1 > m) {
+#pragma unroll
+        for (int j = 0; j < KPL; j++) d[j] = 0;
+        pos = c - m + 1;
+      }
+      for (int64_t i = pos; i <= c; i++) {
+        const int x = t.dsq[i];
+        uint32_t prev = (uint32_t)__shfl_up((int)d[KPL - 1], 1);
+        if (lane == 0) prev = 0;
+#pragma unroll
+        for (int j = KPL - 1; j >= 0; j--) {
+          const uint32_t in = j ? d[j - 1] : prev;
+          const uint32_t cc = ssv_cost(p, c4[j], kb + j, x);
+          d[j] = (kb + j <= m) ? ssv_cell(in, xb, bias, cc) : 0u;
+        }
+      }
+      pos = c + 1;
+      // row maximum and its leftmost position
+      uint32_t bv = 0;
+      int bk = 0x7FFFFFFF;
+#pragma unroll
+      for (int j = 0; j < KPL; j++)
+        if (kb + j <= m && d[j] > bv) { bv = d[j]; bk = kb + j; }
+      if (bv == 0) bk = 0x7FFFFFFF;
+      for (int off = 32; off >= 1; off >>= 1) {
+        const uint32_t ov = (uint32_t)__shfl_xor((int)bv, off);
+        const int ok = __shfl_xor(bk, off);
+        if (ov > bv || (ov == bv && ok < bk)) { bv = ov; bk = ok; }
+      }
+      if ((int)bv >= thresh && bv > 0) {
+        int64_t max_end = 0;
+        if (lane == 0) {
+          const int end_k = bk;
+          const int baseline = (int)p.base_b - (int)p.tjb_b - (int)p.tbm_b;
+          int64_t start = end_k, tstart = c;
+          int rem = (int)bv;
+          while (rem > baseline && start >= 1 && tstart >= 1) {
+            const int x = t.dsq[tstart];
+            const int cst = x < KP ? (int)p.rbv[start * KP + x] : 255;
+            rem -= (int)p.bias - cst;
+            start--;
+            tstart--;
+          }
+          start++;
+          tstart++;
+          int sc2 = (int)bv, max_sc = (int)bv, since = 0;
+          max_end = c;
+          int64_t k = end_k + 1, n = c + 1;
+          while (k < m && n <= L) {
+            const int x = t.dsq[n];
+            const int cst = x < KP ? (int)p.rbv[k * KP + x] : 255;
+            sc2 += (int)p.bias - cst;
+            if (sc2 >= max_sc) { max_sc = sc2; max_end = n; since = 0; }
+            else if (++since == 5) break;
+            k++;
+            n++;
+          }
+          const int64_t end_model = end_k + (max_end - c);
+          // Records of one task are appended by this wave alone, in trigger
+          // order; the host regroups them by task.
+          const unsigned long long slot = atomicAdd(counter, 1ull);
+          if ((int64_t)slot < cap) {
+            RawWinDev rw;
+            rw.n = tstart;
+            rw.length = end_model - start + 1;
+            rw.k = end_model;
+            rw.score = ((float)(max_sc - (int)p.tjb_b) - (float)p.base_b) / p.scale_b - 3.0f;
+            rw.task = (int32_t)blockIdx.x;
+            out[slot] = rw;
+          }
+        }
+        max_end = __shfl(max_end, 0);
+        r = max_end + 1;
+        pos = r;
+#pragma unroll
+        for (int j = 0; j < KPL; j++) d[j] = 0;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// F3 Forward, striped order. Four lanes emulate the reference's four SSE
+// lanes for one window (cm_pipeline.rs:1268-1418): lane z owns model
+// positions q+1+z*nq and runs the q loop in the same order, so every add and
+// multiply happens in the same sequence as on the CPU. A wave serves 16
+// windows. The three DP columns live in LDS, one slot per lane.
+// ---------------------------------------------------------------------------
+template <int CTRL>
+__device__ __forceinline__ float quad_dpp(float v) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+__global__ __launch_bounds__(64) void fwd_kernel(FwdDev p, const uint8_t *seq, const WinDev *wins,
+                                                 int64_t nwin, float *sc) {
+  extern __shared__ float col[];  // [3][nq][64]
+  const int lane = threadIdx.x, z = lane & 3;
+  const int64_t widx = (int64_t)blockIdx.x * 16 + (lane >> 2);
+  const bool has = widx < nwin;
+  WinDev w;
+  w.off = 0;
+  w.len = 0;
+  w.idx = 0;
+  w.a = 0.f;
+  w.b = 0.f;
+  if (has) w = wins[widx];
+  const int nq = p.nq;
+  float *mm = col, *dm = col + nq * 64, *im = col + 2 * nq * 64;
+  for (int q = 0; q < nq; q++) {
+    mm[q * 64 + lane] = 0.f;
+    dm[q * 64 + lane] = 0.f;
+    im[q * 64 + lane] = 0.f;
+  }
+  int maxL = w.len;
+  for (int off = 32; off >= 1; off >>= 1) {
+    const int o = __shfl_xor(maxL, off);
+    maxL = o > maxL ? o : maxL;
+  }
+  const float L = (float)w.len;
+  const float pmove = (2.0f + 1.0f) / (L + 2.0f + 1.0f);
+  const float ploop = 1.0f - pmove;
+  float xn = 1.0f, xj = 0.0f, xb = pmove, xc = 0.0f, totscale = 0.0f;
+  const float *tm = p.tmain;
+  for (int i = 1; i <= maxL; i++) {
+    const bool act = i <= w.len;
+    const int x = act ? (int)seq[w.off + i] : 0;
+    const float *rp = p.rfvv + (size_t)x * (nq * 4);
+    float dcv = 0.f, xev = 0.f;
+    const float xbv = xb;
+    float mpv = z ? mm[(nq - 1) * 64 + lane - 1] : 0.f;
+    float dpv = z ? dm[(nq - 1) * 64 + lane - 1] : 0.f;
+    float ipv = z ? im[(nq - 1) * 64 + lane - 1] : 0.f;
+    __builtin_amdgcn_wave_barrier();
+    for (int q = 0; q < nq; q++) {
+      const float *tb = tm + q * 28 + z;
+      float sv = xbv * tb[0];
+      sv = sv + mpv * tb[4];
+      sv = sv + ipv * tb[8];
+      sv = sv + dpv * tb[12];
+      sv = sv * rp[q * 4 + z];
+      xev = xev + sv;
+      mpv = mm[q * 64 + lane];
+      dpv = dm[q * 64 + lane];
+      ipv = im[q * 64 + lane];
+      mm[q * 64 + lane] = sv;
+      dm[q * 64 + lane] = dcv;
+      dcv = sv * tb[16];
+      const float svi = mpv * tb[20];
+      im[q * 64 + lane] = svi + ipv * tb[24];
+    }
+    // DD paths: first obligatory pass
+    dcv = quad_dpp<DPP_QUAD_RSZ>(dcv);
+    if (z == 0) dcv = 0.f;
+    dm[lane] = 0.f;
+    for (int q = 0; q < nq; q++) {
+      const float d = dcv + dm[q * 64 + lane];
+      dm[q * 64 + lane] = d;
+      dcv = d * p.tddv[q * 4 + z];
+    }
+    if (p.m < 100) {
+      for (int j = 1; j < 4; j++) {
+        dcv = quad_dpp<DPP_QUAD_RSZ>(dcv);
+        if (z == 0) dcv = 0.f;
+        for (int q = 0; q < nq; q++) {
+          const float d = dcv + dm[q * 64 + lane];
+          dm[q * 64 + lane] = d;
+          dcv = dcv * p.tddv[q * 4 + z];
+        }
+      }
+    } else {
+      bool done = false;
+      for (int j = 1; j < 4; j++) {
+        // `done` is uniform within a quad, so the quad shuffles below always
+        // read lanes of the same window.
+        int cv = 0;
+        dcv = quad_dpp<DPP_QUAD_RSZ>(dcv);
+        if (z == 0) dcv = 0.f;
+        if (!done) {
+          for (int q = 0; q < nq; q++) {
+            const float old = dm[q * 64 + lane];
+            const float d = dcv + old;
+            cv |= d > old;
+            dm[q * 64 + lane] = d;
+            dcv = dcv * p.tddv[q * 4 + z];
+          }
+        }
+        int any = cv | __builtin_amdgcn_update_dpp(0, cv, DPP_QUAD_X1, 0xF, 0xF, false);
+        any = any | __builtin_amdgcn_update_dpp(0, any, DPP_QUAD_X2, 0xF, 0xF, false);
+        if (!any) done = true;
+      }
+    }
+    for (int q = 0; q < nq; q++) xev = xev + dm[q * 64 + lane];
+    const float p01 = xev + quad_dpp<DPP_QUAD_X1>(xev);
+    const float xe = p01 + quad_dpp<DPP_QUAD_X2>(p01);
+    if (act) {
+      xn = xn * ploop;
+      xc = (xc * ploop) + (xe * 0.5f);
+      xj = (xj * ploop) + (xe * 0.5f);
+      xb = (xj * pmove) + (xn * pmove);
+      if (xe > 1.0e4f) {
+        const float inv = 1.0f / xe;
+        xn *= inv;
+        xc *= inv;
+        xj *= inv;
+        xb *= inv;
+        for (int q = 0; q < nq; q++) {
+          mm[q * 64 + lane] *= inv;
+          dm[q * 64 + lane] *= inv;
+          im[q * 64 + lane] *= inv;
+        }
+        totscale += (float)log((double)xe);
+      }
+    }
+    __builtin_amdgcn_wave_barrier();
+  }
+  if (has && z == 0) sc[w.idx] = (float)((double)totscale + log((double)xc * (double)pmove));
+}
+
+// ---------------------------------------------------------------------------
+// F3b composition bias: the reference's 2-state filter-null Forward with
+// per-row max rescaling (cm_pipeline.rs:1722-1774), one lane per window. The
+// length-only tail (L*ln p1 + ln(1-p1)) arrives precomputed in (a, b).
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void bias_kernel(BiasDev p, const uint8_t *seq, const WinDev *wins,
+                                                  int64_t nwin, float *sc) {
+  const int64_t wi = (int64_t)blockIdx.x * 64 + threadIdx.x;
+  if (wi >= nwin) return;
+  const WinDev w = wins[wi];
+  const int64_t L = w.len;
+  if (L == 0) { sc[w.idx] = 0.f; return; }
+  const float p1 = (float)L / ((float)L + 1.0f);
+  const float t00 = p1, t01 = 1.0f - p1;
+  const float t10 = p.t1[0], t11 = p.t1[1], t12 = p.t1[2];
+  const uint8_t *s = seq + w.off;
+  float logsc = 0.0f;
+  int x = s[1];
+  float d0 = p.eo[x][0] * p.pi[0], d1 = p.eo[x][1] * p.pi[1];
+  float mx = d0 > d1 ? d0 : d1;
+  d0 /= mx;
+  d1 /= mx;
+  logsc += (float)log((double)mx);
+  for (int64_t i = 2; i <= L; i++) {
+    x = s[i];
+    float s0 = 0.0f;
+    s0 += d0 * t00;
+    s0 += d1 * t10;
+    float s1 = 0.0f;
+    s1 += d0 * t01;
+    s1 += d1 * t11;
+    const float n0 = s0 * p.eo[x][0], n1 = s1 * p.eo[x][1];
+    mx = n0 > n1 ? n0 : n1;
+    d0 = n0 / mx;
+    d1 = n1 / mx;
+    logsc += (float)log((double)mx);
+  }
+  const float end = d0 * 1.0f + d1 * t12;
+  logsc += (float)log((double)end);
+  sc[w.idx] = (logsc + w.a) + w.b;
+}
+
+// ---------------------------------------------------------------------------
+// F2 Viterbi (p7_vitfilter.rs:305-386), one wave per window, KPL model
+// positions per lane. M and I cells are independent within a row; the D row
+// is a max-plus prefix over k whose added terms are all <= 0, so saturating
+// at -32768 after every step equals saturating once at the end; the prefix
+// is computed exactly in 32-bit with a wave scan.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int adds16(int a, int b) {
+  int r = a + b;
+  r = r > 32767 ? 32767 : r;
+  return r < -32768 ? -32768 : r;
+}
+
+template <int KPL>
+__global__ __launch_bounds__(64) void vit_kernel(VitDev p, const uint8_t *seq, const WinDev *wins,
+                                                 int64_t nwin, float *sc, uint8_t *ovf) {
+  const WinDev w = wins[blockIdx.x];
+  const int m = p.m, lane = threadIdx.x, kb = lane * KPL + 1;
+  const int16_t *tmm = p.tr, *tim = p.tr + (m + 1), *tdm = p.tr + 2 * (m + 1), *vbm = p.tr + 3 * (m + 1);
+  const int16_t *tmi = p.tr + 4 * (m + 1), *tii = p.tr + 5 * (m + 1), *tmd = p.tr + 6 * (m + 1),
+                *tdd = p.tr + 7 * (m + 1);
+  int rmm[KPL], rim[KPL], rdm[KPL], rbm[KPL], rmi[KPL], rii[KPL], rmd[KPL], rdd[KPL];
+  int mp[KPL], ip[KPL], dp[KPL];
+  const int NEG = -32768;
+#pragma unroll
+  for (int j = 0; j < KPL; j++) {
+    const int k = kb + j;
+    const bool ok = k <= m;
+    rmm[j] = ok ? tmm[k - 1] : NEG;
+    rim[j] = ok ? tim[k - 1] : NEG;
+    rdm[j] = ok ? tdm[k - 1] : NEG;
+    rbm[j] = ok ? vbm[k] : NEG;
+    rmi[j] = ok ? tmi[k] : NEG;
+    rii[j] = ok ? tii[k] : NEG;
+    rmd[j] = ok ? tmd[k - 1] : NEG;
+    rdd[j] = ok ? tdd[k - 1] : NEG;
+    mp[j] = ip[j] = dp[j] = NEG;
+  }
+  const int xw_move = (int)w.a;
+  int xn = p.base_w, xb = adds16(xn, xw_move), xj = NEG, xc = NEG;
+  bool overflow = false;
+  const uint8_t *s = seq + w.off;
+  for (int64_t i = 1; i <= w.len; i++) {
+    const int x = s[i];
+    int mpl = __shfl_up(mp[KPL - 1], 1), ipl = __shfl_up(ip[KPL - 1], 1), dpl = __shfl_up(dp[KPL - 1], 1);
+    if (lane == 0) mpl = ipl = dpl = NEG;
+    int mc[KPL], ic[KPL];
+    int lmax = NEG;
+#pragma unroll
+    for (int j = 0; j < KPL; j++) {
+      const int pm = j ? mp[j - 1] : mpl, pi = j ? ip[j - 1] : ipl, pd = j ? dp[j - 1] : dpl;
+      int v = adds16(xb, rbm[j]);
+      v = max(v, adds16(pm, rmm[j]));
+      v = max(v, adds16(pi, rim[j]));
+      v = max(v, adds16(pd, rdm[j]));
+      const int k = kb + j;
+      const int e = k <= m ? (int)p.rwv[k * KP + x] : NEG;
+      mc[j] = k <= m ? adds16(v, e) : NEG;
+      ic[j] = k <= m ? max(adds16(mp[j], rmi[j]), adds16(ip[j], rii[j])) : NEG;
+      lmax = max(lmax, mc[j]);
+    }
+    for (int off = 32; off >= 1; off >>= 1) lmax = max(lmax, __shfl_xor(lmax, off));
+    // D row: dc[k] = max(mc[k-1] + tmd[k-1], dc[k-1] + tdd[k-1]), clamp once.
+    int mcl = __shfl_up(mc[KPL - 1], 1);
+    if (lane == 0) mcl = NEG;
+    int A[KPL];
+    int fa = -(1 << 30), fb = 0;
+#pragma unroll
+    for (int j = 0; j < KPL; j++) {
+      const int pmc = j ? mc[j - 1] : mcl;
+      A[j] = pmc + rmd[j];
+      fa = max(A[j], fa + rdd[j]);
+      fb = fb + rdd[j];
+    }
+    for (int off = 1; off < 64; off <<= 1) {
+      const int oa = __shfl_up(fa, off), ob = __shfl_up(fb, off);
+      if (lane >= off) {
+        fa = max(fa, oa + fb);
+        fb = ob + fb;
+      }
+    }
+    int cin = __shfl_up(max(fa, NEG + fb), 1);
+    if (lane == 0) cin = NEG;
+    int dcur = cin;
+#pragma unroll
+    for (int j = 0; j < KPL; j++) {
+      dcur = max(A[j], dcur + rdd[j]);
+      dp[j] = (kb + j <= m) ? max(dcur, NEG) : NEG;
+      dcur = max(dcur, NEG);
+      mp[j] = mc[j];
+      ip[j] = ic[j];
+    }
+    const int xe = lmax;
+    if (xe >= 32767) { overflow = true; break; }
+    xc = max(xc, adds16(xe, p.xw_e_move));
+    xj = max(xj, adds16(xe, p.xw_e_loop));
+    xb = max(adds16(xj, xw_move), adds16(xn, xw_move));
+  }
+  if (lane == 0) {
+    ovf[w.idx] = overflow ? 1 : 0;
+    float r = -INFINITY;
+    if (!overflow && xc > NEG) {
+      float t = (float)xc + (float)xw_move - (float)p.base_w;
+      t /= p.scale_w;
+      r = t - 3.0f;
+    }
+    sc[w.idx] = overflow ? 0.f : r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// F1b full MSV with the J state (cm_pipeline.rs:426-452), one wave per
+// window, KPL model positions per lane; overflow (the 255 ceiling) -> ovf.
+// ---------------------------------------------------------------------------
+template <int KPL>
+__global__ __launch_bounds__(64) void msv_kernel(SsvDev p, const uint8_t *seq, const WinDev *wins,
+                                                 int64_t nwin, float *sc, uint8_t *ovf) {
+  const WinDev w = wins[blockIdx.x];
+  const int m = p.m, lane = threadIdx.x, kb = lane * KPL + 1;
+  const uint32_t tjb_b = (uint32_t)w.a;               // reconfigured J->B cost
+  const uint32_t tjbm = (tjb_b + p.tbm_b) & 0xFFu;    // wrapping u8 add
+  uint32_t c4[KPL], d[KPL];
+#pragma unroll
+  for (int j = 0; j < KPL; j++) {
+    c4[j] = (kb + j <= m) ? p.cost4[kb + j] : 0xFFFFFFFFu;
+    d[j] = 0;
+  }
+  const uint32_t bias = p.bias, base = p.base_b, tec = (uint32_t)w.b;
+  uint32_t xj = 0;
+  uint32_t xb = base > tjbm ? base - tjbm : 0;
+  bool overflow = false;
+  const uint8_t *s = seq + w.off;
+  for (int64_t i = 1; i <= w.len; i++) {
+    const int x = s[i];
+    uint32_t prev = (uint32_t)__shfl_up((int)d[KPL - 1], 1);
+    if (lane == 0) prev = 0;
+    uint32_t lmax = 0;
+#pragma unroll
+    for (int j = KPL - 1; j >= 0; j--) {
+      const uint32_t in = j ? d[j - 1] : prev;
+      const uint32_t cc = ssv_cost(p, c4[j], kb + j, x);
+      d[j] = (kb + j <= m) ? ssv_cell(in, xb, bias, cc) : 0u;
+      lmax = d[j] > lmax ? d[j] : lmax;
+    }
+    for (int off = 32; off >= 1; off >>= 1) {
+      const uint32_t o = (uint32_t)__shfl_xor((int)lmax, off);
+      lmax = o > lmax ? o : lmax;
+    }
+    uint32_t xe = lmax;
+    if (min(xe + bias, 255u) == 255u) { overflow = true; break; }
+    xe = xe > tec ? xe - tec : 0;
+    xj = xj > xe ? xj : xe;
+    const uint32_t bm = base > xj ? base : xj;
+    xb = bm > tjbm ? bm - tjbm : 0;
+  }
+  if (lane == 0) {
+    ovf[w.idx] = overflow ? 1 : 0;
+    sc[w.idx] = overflow ? 0.f : ((float)((int)xj - (int)tjb_b) - (float)base) / p.scale_b - 3.0f;
+  }
+}
+
+__global__ void revcomp_kernel(const uint8_t *fwd, uint8_t *rc, int64_t L) {
+  // rc[j] = comp(fwd[L+1-j]) for j in 1..L (cm_search.rs:1822-1827); sentinels kept.
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j > L + 1) return;
+  if (j == 0 || j == L + 1) { rc[j] = 255; return; }
+  const uint8_t d = fwd[L + 1 - j];
+  rc[j] = d < 4 ? (uint8_t)(3 - d) : d;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+hipError_t launch_ssv_sweep(const SsvDev &p, const TaskDev *tasks, const SweepItem *items, int64_t nitems,
+                            uint64_t *mask, int thresh, hipStream_t s) {
+  if (nitems == 0) return hipSuccess;
+  hipLaunchKernelGGL(ssv_sweep_kernel, dim3((unsigned)nitems), dim3(64), 2 * (p.m + 1), s, p, tasks, items,
+                     mask, thresh);
+  return hipGetLastError();
+}
+
+template <int KPL>
+static void resolve_k(const SsvDev &p, const TaskDev *tasks, int64_t ntasks, const uint64_t *mask, int thresh,
+                      RawWinDev *out, int64_t cap, unsigned long long *counter, hipStream_t s) {
+  hipLaunchKernelGGL(ssv_resolve_kernel<KPL>, dim3((unsigned)ntasks), dim3(64), 0, s, p, tasks, mask, thresh,
+                     out, cap, counter);
+}
+hipError_t launch_ssv_resolve(const SsvDev &p, const TaskDev *tasks, int64_t ntasks, const uint64_t *mask,
+                              int thresh, RawWinDev *out, int64_t cap, unsigned long long *counter,
+                              hipStream_t s) {
+  if (ntasks == 0) return hipSuccess;
+  const int kpl = (p.m + 63) / 64;
+  if (kpl <= 1) resolve_k<1>(p, tasks, ntasks, mask, thresh, out, cap, counter, s);
+  else if (kpl <= 2) resolve_k<2>(p, tasks, ntasks, mask, thresh, out, cap, counter, s);
+  else if (kpl <= 4) resolve_k<4>(p, tasks, ntasks, mask, thresh, out, cap, counter, s);
+  else if (kpl <= 8) resolve_k<8>(p, tasks, ntasks, mask, thresh, out, cap, counter, s);
+  else if (kpl <= 16) resolve_k<16>(p, tasks, ntasks, mask, thresh, out, cap, counter, s);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_fwd(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin, float *sc,
+                      hipStream_t s) {
+  if (nwin == 0) return hipSuccess;
+  const unsigned nb = (unsigned)((nwin + 15) / 16);
+  const size_t lds = (size_t)3 * p.nq * 64 * sizeof(float);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(fwd_kernel, dim3(nb), dim3(64), lds, s, p, seq, wins, nwin, sc);
+  return hipGetLastError();
+}
+
+hipError_t launch_bias(const BiasDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin, float *sc,
+                       hipStream_t s) {
+  if (nwin == 0) return hipSuccess;
+  hipLaunchKernelGGL(bias_kernel, dim3((unsigned)((nwin + 63) / 64)), dim3(64), 0, s, p, seq, wins, nwin, sc);
+  return hipGetLastError();
+}
+
+template <int KPL>
+static void vit_k(const VitDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin, float *sc, uint8_t *ovf,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(vit_kernel<KPL>, dim3((unsigned)nwin), dim3(64), 0, s, p, seq, wins, nwin, sc, ovf);
+}
+hipError_t launch_vit(const VitDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin, float *sc,
+                      uint8_t *ovf, hipStream_t s) {
+  if (nwin == 0) return hipSuccess;
+  const int kpl = (p.m + 63) / 64;
+  if (kpl <= 1) vit_k<1>(p, seq, wins, nwin, sc, ovf, s);
+  else if (kpl <= 2) vit_k<2>(p, seq, wins, nwin, sc, ovf, s);
+  else if (kpl <= 4) vit_k<4>(p, seq, wins, nwin, sc, ovf, s);
+  else if (kpl <= 8) vit_k<8>(p, seq, wins, nwin, sc, ovf, s);
+  else if (kpl <= 16) vit_k<16>(p, seq, wins, nwin, sc, ovf, s);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+template <int KPL>
+static void msv_k(const SsvDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin, float *sc, uint8_t *ovf,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(msv_kernel<KPL>, dim3((unsigned)nwin), dim3(64), 0, s, p, seq, wins, nwin, sc, ovf);
+}
+hipError_t launch_msv(const SsvDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin, float *sc,
+                      uint8_t *ovf, hipStream_t s) {
+  if (nwin == 0) return hipSuccess;
+  const int kpl = (p.m + 63) / 64;
+  if (kpl <= 1) msv_k<1>(p, seq, wins, nwin, sc, ovf, s);
+  else if (kpl <= 2) msv_k<2>(p, seq, wins, nwin, sc, ovf, s);
+  else if (kpl <= 4) msv_k<4>(p, seq, wins, nwin, sc, ovf, s);
+  else if (kpl <= 8) msv_k<8>(p, seq, wins, nwin, sc, ovf, s);
+  else if (kpl <= 16) msv_k<16>(p, seq, wins, nwin, sc, ovf, s);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_revcomp(const uint8_t *fwd, uint8_t *rc, int64_t L, hipStream_t s) {
+  const int64_t n = L + 2;
+  hipLaunchKernelGGL(revcomp_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fwd, rc, L);
+  return hipGetLastError();
+}
+
+}  // namespace ixg

@@ -1,0 +1,286 @@
+"""ctypes binding of libixg.so (the C ABI declared in include/ixg.h).
+
+Thin plumbing for tests and the benchmark: every call goes straight to the
+HIP library. There is no CPU fallback; loading fails loudly when the built
+library is missing.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libixg.so")
+
+IXG_OK, IXG_EINVAL, IXG_ENOMEM, IXG_EDEVICE, IXG_ERANGE = 0, -1, -2, -3, -4
+DO_MSV, DO_FWD, DO_FWDBIAS, DO_VIT, DO_VITBIAS, DO_MSVBIAS = 1, 2, 4, 8, 16, 32
+
+
+class P7Hmm(C.Structure):
+    _fields_ = [("M", C.c_int32), ("mat", C.POINTER(C.c_float)), ("trans", C.POINTER(C.c_float)),
+                ("compo", C.c_float * 4), ("lmmu", C.c_double), ("lmlambda", C.c_double),
+                ("lvmu", C.c_double), ("lvlambda", C.c_double), ("lftau", C.c_double),
+                ("lflambda", C.c_double)]
+
+
+class RawWin(C.Structure):
+    _fields_ = [("n", C.c_int64), ("length", C.c_int64), ("k", C.c_int64), ("score", C.c_float),
+                ("task", C.c_int32)]
+
+
+class Window(C.Structure):
+    _fields_ = [("start", C.c_int64), ("end", C.c_int64), ("score", C.c_float), ("task", C.c_int32)]
+
+
+class Acct(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "n_past_msv", "pos_past_msv", "n_past_msvbias", "pos_past_msvbias",
+        "n_past_vit", "pos_past_vit", "n_past_vitbias", "pos_past_vitbias",
+        "n_past_fwd", "pos_past_fwd", "n_past_fwdbias", "pos_past_fwdbias")]
+
+
+class F3Opts(C.Structure):
+    _fields_ = [("f1", C.c_double), ("f1b", C.c_double), ("f2", C.c_double), ("f2b", C.c_double),
+                ("f3", C.c_double), ("f3b", C.c_double), ("cmw", C.c_int64), ("maxw", C.c_int64),
+                ("opts", C.c_int32)]
+
+
+class Task(C.Structure):
+    _fields_ = [("start", C.c_int64), ("end", C.c_int64), ("strand", C.c_int32), ("ctx", C.c_int32)]
+
+
+# every symbol include/ixg.h declares
+EXPORTS = [
+    "ixg_init", "ixg_shutdown", "ixg_last_error", "ixg_mem_in_use", "ixg_p7_create", "ixg_p7_free",
+    "ixg_p7_msv_tables", "ixg_p7_vit_tables", "ixg_p7_fwd_tables", "ixg_ssv_diagonals",
+    "ixg_vit_scores", "ixg_fwd_scores", "ixg_bias_scores", "ixg_msv_scores",
+    "ixg_f3_filter_sequence", "ixg_target_create", "ixg_target_free", "ixg_window_tasks",
+    "ixg_f3_filter_tasks", "ixg_last_timings",
+]
+
+_lib = None
+
+
+def lib():
+    """Load libixg.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = C.CDLL(LIB_PATH)
+        vp, i64, i32, f32p, u8p = C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_uint8)
+        i64p = C.POINTER(C.c_int64)
+        L.ixg_last_error.restype = C.c_char_p
+        L.ixg_init.argtypes = [C.c_int, C.c_size_t, C.POINTER(vp)]
+        L.ixg_shutdown.argtypes = [vp]
+        L.ixg_mem_in_use.argtypes = [vp]
+        L.ixg_mem_in_use.restype = C.c_size_t
+        L.ixg_p7_create.argtypes = [vp, C.POINTER(P7Hmm), i32, C.POINTER(vp)]
+        L.ixg_p7_free.argtypes = [vp, vp]
+        L.ixg_p7_msv_tables.argtypes = [vp, u8p, u8p, f32p, f32p]
+        L.ixg_p7_vit_tables.argtypes = [vp, C.POINTER(C.c_int16), C.POINTER(C.c_int16), C.POINTER(C.c_int16)]
+        L.ixg_p7_fwd_tables.argtypes = [vp, C.POINTER(C.c_int32), f32p, f32p, f32p]
+        L.ixg_ssv_diagonals.argtypes = [vp, vp, u8p, i64, C.c_double, C.POINTER(RawWin), i64, i64p]
+        for nm in ("ixg_vit_scores", "ixg_msv_scores"):
+            getattr(L, nm).argtypes = [vp, vp, u8p, i64, i64p, i64p, i64, f32p, u8p]
+        for nm in ("ixg_fwd_scores", "ixg_bias_scores"):
+            getattr(L, nm).argtypes = [vp, vp, u8p, i64, i64p, i64p, i64, f32p]
+        L.ixg_f3_filter_sequence.argtypes = [vp, vp, u8p, i64, C.POINTER(F3Opts), C.POINTER(Window), i64, i64p,
+                                             C.POINTER(Acct)]
+        L.ixg_target_create.argtypes = [vp, u8p, i64, C.POINTER(vp)]
+        L.ixg_target_free.argtypes = [vp, vp]
+        L.ixg_window_tasks.argtypes = [i64, i64, C.c_int, C.c_int, C.POINTER(Task), i64]
+        L.ixg_window_tasks.restype = i64
+        L.ixg_f3_filter_tasks.argtypes = [vp, vp, vp, C.POINTER(Task), i64, C.POINTER(F3Opts), C.POINTER(Window),
+                                          i64, i64p, C.POINTER(Acct)]
+        L.ixg_last_timings.argtypes = [vp, C.POINTER(C.c_double), i64p]
+        _lib = L
+    return _lib
+
+
+class IxgError(RuntimeError):
+    pass
+
+
+def check(rc, what=""):
+    if rc != IXG_OK:
+        raise IxgError(f"{what}: status {rc}: {lib().ixg_last_error().decode()}")
+
+
+def u8ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+def f32ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def i64ptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int64))
+
+
+class Context:
+    """One device, one HIP stream (ixg_init / ixg_shutdown)."""
+
+    def __init__(self, device=0, mem_budget=0):
+        self.h = C.c_void_p()
+        check(lib().ixg_init(device, mem_budget, C.byref(self.h)), "ixg_init")
+
+    def close(self):
+        if self.h:
+            check(lib().ixg_shutdown(self.h), "ixg_shutdown")
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def mem_in_use(self):
+        return lib().ixg_mem_in_use(self.h)
+
+    def timings(self):
+        ms = (C.c_double * 5)()
+        cells = (C.c_int64 * 5)()
+        check(lib().ixg_last_timings(self.h, ms, cells), "timings")
+        return list(ms), list(cells)
+
+
+def p7_struct(p7):
+    """ctypes P7Hmm view of a dict {M, mat (M+1,4), trans (M+1,7), compo, ev...}; keeps arrays alive."""
+    mat = np.ascontiguousarray(p7["mat"], dtype=np.float32)
+    trans = np.ascontiguousarray(p7["trans"], dtype=np.float32)
+    s = P7Hmm()
+    s.M = int(p7["M"])
+    s.mat = f32ptr(mat)
+    s.trans = f32ptr(trans)
+    for i in range(4):
+        s.compo[i] = float(p7["compo"][i])
+    for k in ("lmmu", "lmlambda", "lvmu", "lvlambda", "lftau", "lflambda"):
+        setattr(s, k, float(p7[k]))
+    s._keep = (mat, trans)
+    return s
+
+
+class P7:
+    """Device-resident p7 filter profile (ixg_p7_create)."""
+
+    def __init__(self, ctx, p7, max_length):
+        self.ctx = ctx
+        self.M = int(p7["M"])
+        self.s = p7_struct(p7)
+        self.h = C.c_void_p()
+        check(lib().ixg_p7_create(ctx.h, C.byref(self.s), int(max_length), C.byref(self.h)), "ixg_p7_create")
+
+    def free(self):
+        if self.h:
+            check(lib().ixg_p7_free(self.ctx.h, self.h), "ixg_p7_free")
+            self.h = C.c_void_p()
+
+    def msv_tables(self):
+        M = self.M
+        rbv = np.zeros((M + 1) * 18, np.uint8)
+        b = np.zeros(9, np.uint8)
+        pre = np.zeros(M + 1, np.float32)
+        suf = np.zeros(M + 1, np.float32)
+        check(lib().ixg_p7_msv_tables(self.h, u8ptr(rbv), u8ptr(b), f32ptr(pre), f32ptr(suf)))
+        return rbv, b, pre, suf
+
+    def vit_tables(self):
+        M = self.M
+        rwv = np.zeros((M + 1) * 18, np.int16)
+        tr = np.zeros(8 * (M + 1), np.int16)
+        ddb = np.zeros(1, np.int16)
+        P = C.POINTER(C.c_int16)
+        check(lib().ixg_p7_vit_tables(self.h, rwv.ctypes.data_as(P), tr.ctypes.data_as(P), ddb.ctypes.data_as(P)))
+        return rwv, tr, int(ddb[0])
+
+    def fwd_tables(self):
+        nq = C.c_int32()
+        check(lib().ixg_p7_fwd_tables(self.h, C.byref(nq), None, None, None))
+        n = nq.value
+        tm = np.zeros(n * 28, np.float32)
+        td = np.zeros(n * 4, np.float32)
+        rf = np.zeros(18 * n * 4, np.float32)
+        check(lib().ixg_p7_fwd_tables(self.h, C.byref(nq), f32ptr(tm), f32ptr(td), f32ptr(rf)))
+        return n, tm, td, rf
+
+    def ssv_diagonals(self, dsq, L, f1, cap=None):
+        """dsq: uint8 array, 1-indexed (len >= L+2). Returns list of (n, length, k, score)."""
+        dsq = np.ascontiguousarray(dsq, np.uint8)
+        cap = cap if cap is not None else max(16, L // 4 + 16)
+        out = (RawWin * cap)()
+        n = C.c_int64()
+        rc = lib().ixg_ssv_diagonals(self.ctx.h, self.h, u8ptr(dsq), int(L), float(f1), out, cap, C.byref(n))
+        check(rc, "ixg_ssv_diagonals")
+        return [(out[i].n, out[i].length, out[i].k, out[i].score) for i in range(n.value)]
+
+    def _scores(self, name, seq, offs, lens, with_ovf):
+        seq = np.ascontiguousarray(seq, np.uint8)
+        offs = np.ascontiguousarray(offs, np.int64)
+        lens = np.ascontiguousarray(lens, np.int64)
+        n = len(offs)
+        sc = np.zeros(n, np.float32)
+        args = [self.ctx.h, self.h, u8ptr(seq), len(seq), i64ptr(offs), i64ptr(lens), n, f32ptr(sc)]
+        ovf = np.zeros(n, np.uint8)
+        if with_ovf:
+            args.append(u8ptr(ovf))
+        check(getattr(lib(), name)(*args), name)
+        return (sc, ovf) if with_ovf else sc
+
+    def fwd_scores(self, seq, offs, lens):
+        return self._scores("ixg_fwd_scores", seq, offs, lens, False)
+
+    def bias_scores(self, seq, offs, lens):
+        return self._scores("ixg_bias_scores", seq, offs, lens, False)
+
+    def vit_scores(self, seq, offs, lens):
+        return self._scores("ixg_vit_scores", seq, offs, lens, True)
+
+    def msv_scores(self, seq, offs, lens):
+        return self._scores("ixg_msv_scores", seq, offs, lens, True)
+
+    def f3_filter_sequence(self, dsq, L, opts, cap=None):
+        dsq = np.ascontiguousarray(dsq, np.uint8)
+        cap = cap if cap is not None else max(64, L // 16 + 64)
+        out = (Window * cap)()
+        n = C.c_int64()
+        acct = Acct()
+        check(lib().ixg_f3_filter_sequence(self.ctx.h, self.h, u8ptr(dsq), int(L), C.byref(opts), out, cap,
+                                           C.byref(n), C.byref(acct)), "ixg_f3_filter_sequence")
+        return [(out[i].start, out[i].end, out[i].score) for i in range(n.value)], acct
+
+
+class Target:
+    """Device-resident digitized sequence with its reverse complement."""
+
+    def __init__(self, ctx, dsq, L):
+        self.ctx = ctx
+        self.L = int(L)
+        self.dsq = np.ascontiguousarray(dsq, np.uint8)
+        self.h = C.c_void_p()
+        check(lib().ixg_target_create(ctx.h, u8ptr(self.dsq), self.L, C.byref(self.h)), "ixg_target_create")
+
+    def free(self):
+        if self.h:
+            check(lib().ixg_target_free(self.ctx.h, self.h), "ixg_target_free")
+            self.h = C.c_void_p()
+
+
+def window_tasks(L, maxw, do_top=True, do_bot=True):
+    n = lib().ixg_window_tasks(int(L), int(maxw), int(do_top), int(do_bot), None, 0)
+    arr = (Task * max(n, 1))()
+    lib().ixg_window_tasks(int(L), int(maxw), int(do_top), int(do_bot), arr, n)
+    return arr, n
+
+
+def f3_filter_tasks(ctx, p7, tgt, tasks, ntasks, opts, cap=None):
+    cap = cap if cap is not None else max(1024, tgt.L // 8)
+    out = (Window * cap)()
+    n = C.c_int64()
+    accts = (Acct * max(ntasks, 1))()
+    check(lib().ixg_f3_filter_tasks(ctx.h, p7.h, tgt.h, tasks, int(ntasks), C.byref(opts), out, cap, C.byref(n),
+                                    accts), "ixg_f3_filter_tasks")
+    wins = [(out[i].task, out[i].start, out[i].end, out[i].score) for i in range(n.value)]
+    return wins, accts
