@@ -210,11 +210,15 @@ class P7:
         """dsq: uint8 array, 1-indexed (len >= L+2). Returns list of (n, length, k, score)."""
         dsq = np.ascontiguousarray(dsq, np.uint8)
         cap = cap if cap is not None else max(16, L // 4 + 16)
-        out = (RawWin * cap)()
-        n = C.c_int64()
-        rc = lib().ixg_ssv_diagonals(self.ctx.h, self.h, u8ptr(dsq), int(L), float(f1), out, cap, C.byref(n))
-        check(rc, "ixg_ssv_diagonals")
-        return [(out[i].n, out[i].length, out[i].k, out[i].score) for i in range(n.value)]
+        while True:
+            out = (RawWin * cap)()
+            n = C.c_int64()
+            rc = lib().ixg_ssv_diagonals(self.ctx.h, self.h, u8ptr(dsq), int(L), float(f1), out, cap, C.byref(n))
+            if rc == IXG_ERANGE and n.value > cap:
+                cap = n.value
+                continue
+            check(rc, "ixg_ssv_diagonals")
+            return [(out[i].n, out[i].length, out[i].k, out[i].score) for i in range(n.value)]
 
     def _scores(self, name, seq, offs, lens, with_ovf):
         seq = np.ascontiguousarray(seq, np.uint8)
@@ -244,12 +248,17 @@ class P7:
     def f3_filter_sequence(self, dsq, L, opts, cap=None):
         dsq = np.ascontiguousarray(dsq, np.uint8)
         cap = cap if cap is not None else max(64, L // 16 + 64)
-        out = (Window * cap)()
-        n = C.c_int64()
-        acct = Acct()
-        check(lib().ixg_f3_filter_sequence(self.ctx.h, self.h, u8ptr(dsq), int(L), C.byref(opts), out, cap,
-                                           C.byref(n), C.byref(acct)), "ixg_f3_filter_sequence")
-        return [(out[i].start, out[i].end, out[i].score) for i in range(n.value)], acct
+        while True:
+            out = (Window * cap)()
+            n = C.c_int64()
+            acct = Acct()
+            rc = lib().ixg_f3_filter_sequence(self.ctx.h, self.h, u8ptr(dsq), int(L), C.byref(opts), out, cap,
+                                              C.byref(n), C.byref(acct))
+            if rc == IXG_ERANGE and n.value > cap:
+                cap = n.value
+                continue
+            check(rc, "ixg_f3_filter_sequence")
+            return [(out[i].start, out[i].end, out[i].score) for i in range(n.value)], acct
 
 
 class Target:
@@ -277,10 +286,16 @@ def window_tasks(L, maxw, do_top=True, do_bot=True):
 
 def f3_filter_tasks(ctx, p7, tgt, tasks, ntasks, opts, cap=None):
     cap = cap if cap is not None else max(1024, tgt.L // 8)
-    out = (Window * cap)()
-    n = C.c_int64()
-    accts = (Acct * max(ntasks, 1))()
-    check(lib().ixg_f3_filter_tasks(ctx.h, p7.h, tgt.h, tasks, int(ntasks), C.byref(opts), out, cap, C.byref(n),
-                                    accts), "ixg_f3_filter_tasks")
+    while True:
+        out = (Window * cap)()
+        n = C.c_int64()
+        accts = (Acct * max(ntasks, 1))()
+        rc = lib().ixg_f3_filter_tasks(ctx.h, p7.h, tgt.h, tasks, int(ntasks), C.byref(opts), out, cap, C.byref(n),
+                                       accts)
+        if rc == IXG_ERANGE and n.value > cap:
+            cap = n.value
+            continue
+        check(rc, "ixg_f3_filter_tasks")
+        break
     wins = [(out[i].task, out[i].start, out[i].end, out[i].score) for i in range(n.value)]
     return wins, accts
