@@ -1,0 +1,49 @@
+// Device context and scratch helpers shared by the C ABI translation units.
+#pragma once
+#include <algorithm>
+#include <string>
+#include <vector>
+#include "ixg_internal.h"
+
+namespace ixg {
+
+extern thread_local std::string g_err;
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      ::ixg::g_err = std::string(#expr) + ": " + hipGetErrorString(e_);               \
+      return e_ == hipErrorOutOfMemory ? IXG_ENOMEM : IXG_EDEVICE;                    \
+    }                                                                                 \
+  } while (0)
+#define RCHK(expr)               \
+  do {                           \
+    int r_ = (expr);             \
+    if (r_ != IXG_OK) return r_; \
+  } while (0)
+
+struct DevBuf {
+  void *p = nullptr;
+  size_t cap = 0;
+};
+
+enum Slot { S_TASKS, S_ITEMS, S_MASK, S_RAW, S_COUNTER, S_WINS, S_SC0, S_SC1, S_SC2, S_OVF0, S_OVF1, S_SEQ,
+            S_CM_ENV, S_CM_BANDS, S_CM_ALPHA, S_CM_SEQ, S_NSLOTS };
+
+int dmalloc(ixg_ctx *c, void **p, size_t n);
+void dfree(ixg_ctx *c, void *p, size_t n);
+int scratch(ixg_ctx *c, int slot, size_t n, void **out);
+
+}  // namespace ixg
+
+struct ixg_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  size_t budget = 0;
+  size_t in_use = 0;
+  std::vector<ixg::DevBuf> bufs;  // grow-only scratch slots
+  hipEvent_t ev[8];
+  double ms[5] = {0, 0, 0, 0, 0};
+  int64_t cells[5] = {0, 0, 0, 0, 0};
+};

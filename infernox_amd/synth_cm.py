@@ -35,20 +35,16 @@ def wuss_to_ct(ss):
     return ct
 
 
-def cloverleaf(stems=(7, 4, 5, 5), loops=(8, 7, 9), seed=1):
-    """tRNA-like cloverleaf: acceptor stem, D/anticodon/T arms with loops of
-    7-9 nt, joined by short linkers; returns a WUSS string of 72 columns."""
-    r = np.random.default_rng(seed)
+def cloverleaf(stems=(7, 4, 5, 5), loops=(7, 7, 7), clen=72):
+    """tRNA-like cloverleaf of clen columns: acceptor stem, then D, anticodon
+    and T arms (stems 7/4/5/5 bp, loops of 7-9 nt) joined by 2-, 1- and
+    4-nt linkers, then an unpaired 3' tail."""
     acc, d, ac, t = stems
-    arms = []
-    for s, lp in ((d, loops[0]), (ac, loops[1]), (t, loops[2])):
-        arms.append("<" * s + "." * lp + ">" * s)
-    inner = "." * 2 + arms[0] + "." * 1 + arms[1] + "." * 5 + arms[2]
-    core = "<" * acc + inner + ">" * acc
-    tail = 72 - len(core)
+    arms = ["<" * s + "." * lp + ">" * s for s, lp in ((d, loops[0]), (ac, loops[1]), (t, loops[2]))]
+    core = "<" * acc + ".." + arms[0] + "." + arms[1] + "...." + arms[2] + ">" * acc
+    tail = clen - len(core)
     if tail < 0:
-        raise ValueError("cloverleaf does not fit 72 columns")
-    del r
+        raise ValueError("cloverleaf does not fit")
     return core + "." * tail
 
 

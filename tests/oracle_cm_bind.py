@@ -1,0 +1,135 @@
+"""ctypes binding of the CM part of the CPU oracle (oracle/liboracle.so).
+Test infrastructure only."""
+import ctypes as C
+
+import numpy as np
+
+from oracle_bind import lib as _lib
+
+_f32p = C.POINTER(C.c_float)
+_i32p = C.POINTER(C.c_int32)
+_i8p = C.POINTER(C.c_int8)
+
+
+class Cm(C.Structure):
+    _fields_ = [("M", C.c_int), ("nodes", C.c_int), ("clen", C.c_int), ("W", C.c_int), ("null", C.c_float * 4),
+                ("el_selfsc", C.c_float), ("pbegin", C.c_float), ("pend", C.c_float), ("n3_omega", C.c_double),
+                ("sttype", _i8p), ("ndtype", _i8p), ("nodemap", _i32p), ("cfirst", _i32p), ("cnum", _i32p),
+                ("t", _f32p), ("e", _f32p)]
+
+
+class CmSc(C.Structure):
+    _fields_ = [("M", C.c_int), ("local_begin", C.c_int), ("el_selfsc", C.c_float), ("tsc", _f32p),
+                ("oesc", _f32p), ("beginsc", _f32p), ("endsc", _f32p)]
+
+
+class Bands(C.Structure):
+    _fields_ = [("jmin", _i32p), ("jmax", _i32p), ("hd_off", _i32p), ("hdmin", _i32p), ("hdmax", _i32p)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("i", C.c_int32), ("j", C.c_int32), ("score", C.c_float), ("bias", C.c_float)]
+
+
+_ready = False
+
+
+def lib():
+    global _ready
+    L = _lib()
+    if not _ready:
+        L.ixo_cm_configure.argtypes = [C.POINTER(Cm), C.POINTER(CmSc)]
+        L.ixo_cm_free_scores.argtypes = [C.POINTER(CmSc)]
+        L.ixo_hb_ncells.argtypes = [C.POINTER(Bands), C.c_int]
+        L.ixo_hb_ncells.restype = C.c_int64
+        L.ixo_hb_scan.argtypes = [C.POINTER(Cm), C.POINTER(CmSc), C.POINTER(Bands), C.POINTER(C.c_uint8), C.c_int,
+                                  C.c_int, C.c_int, C.c_float, _f32p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                  _f32p, _f32p]
+        L.ixo_hb_scan.restype = C.c_int64
+        L.ixo_hb_report_hits.argtypes = [C.POINTER(Cm), C.POINTER(Bands), C.POINTER(C.c_uint8), C.c_int, C.c_int,
+                                         _f32p, C.c_float, C.c_int, C.POINTER(Hit), C.c_int64]
+        L.ixo_hb_report_hits.restype = C.c_int64
+        _ready = True
+    return L
+
+
+def cm_struct(cm):
+    keep = {k: np.ascontiguousarray(cm[k]) for k in ("sttype", "ndtype", "nodemap", "cfirst", "cnum", "t", "e")}
+    s = Cm()
+    s.M, s.nodes, s.clen, s.W = int(cm["M"]), int(cm["nodes"]), int(cm["clen"]), int(cm["W"])
+    for i in range(4):
+        s.null[i] = float(cm["null"][i])
+    s.el_selfsc, s.pbegin, s.pend, s.n3_omega = float(cm["el_selfsc"]), float(cm["pbegin"]), float(cm["pend"]), \
+        float(cm["n3_omega"])
+    s.sttype = keep["sttype"].astype(np.int8).ctypes.data_as(_i8p)
+    keep["sttype8"] = keep["sttype"].astype(np.int8)
+    s.sttype = keep["sttype8"].ctypes.data_as(_i8p)
+    keep["ndtype8"] = keep["ndtype"].astype(np.int8)
+    s.ndtype = keep["ndtype8"].ctypes.data_as(_i8p)
+    for k in ("nodemap", "cfirst", "cnum"):
+        keep[k] = keep[k].astype(np.int32)
+        setattr(s, k, keep[k].ctypes.data_as(_i32p))
+    keep["t"] = keep["t"].astype(np.float32)
+    keep["e"] = keep["e"].astype(np.float32)
+    s.t = keep["t"].ctypes.data_as(_f32p)
+    s.e = keep["e"].ctypes.data_as(_f32p)
+    s._keep = keep
+    return s
+
+
+def bands_struct(b):
+    keep = {k: np.ascontiguousarray(b[k], np.int32) for k in ("jmin", "jmax", "hd_off", "hdmin", "hdmax")}
+    s = Bands()
+    for k, v in keep.items():
+        setattr(s, k, v.ctypes.data_as(_i32p))
+    s._keep = keep
+    return s
+
+
+class OracleCM:
+    def __init__(self, cm):
+        self.cm = cm
+        self.s = cm_struct(cm)
+        self.sc = CmSc()
+        lib().ixo_cm_configure(C.byref(self.s), C.byref(self.sc))
+        M = cm["M"]
+        self.tsc = np.ctypeslib.as_array(self.sc.tsc, shape=(M, 6)).copy()
+        self.oesc = np.ctypeslib.as_array(self.sc.oesc, shape=(M, 18 * 18)).copy()
+        self.beginsc = np.ctypeslib.as_array(self.sc.beginsc, shape=(M,)).copy()
+        self.endsc = np.ctypeslib.as_array(self.sc.endsc, shape=(M,)).copy()
+        self.el_selfsc = self.sc.el_selfsc
+
+    def __del__(self):
+        try:
+            lib().ixo_cm_free_scores(C.byref(self.sc))
+        except Exception:
+            pass
+
+    def ncells(self, b):
+        return lib().ixo_hb_ncells(C.byref(bands_struct(b)), self.cm["M"])
+
+    def scan(self, b, dsq, i0, j0, inside, env_cutoff=0.0, want_alpha=False):
+        bs = bands_struct(b)
+        n = self.ncells(b)
+        root = np.zeros(max(1, n), np.float32)
+        alpha = np.zeros(max(1, n), np.float32) if want_alpha else None
+        vsc = C.c_float()
+        ei, ej = C.c_int64(), C.c_int64()
+        d = np.ascontiguousarray(dsq, np.uint8)
+        nr = lib().ixo_hb_scan(C.byref(self.s), C.byref(self.sc), C.byref(bs), d.ctypes.data_as(C.POINTER(C.c_uint8)),
+                               i0, j0, int(inside), env_cutoff, C.byref(vsc), C.byref(ei), C.byref(ej),
+                               root.ctypes.data_as(_f32p), alpha.ctypes.data_as(_f32p) if want_alpha else None)
+        out = dict(vsc=vsc.value, envi=ei.value, envj=ej.value, root=root[:nr].copy())
+        if want_alpha:
+            out["alpha"] = alpha[:n].copy()
+        return out
+
+    def hits(self, b, dsq, i0, j0, root, cutoff, do_null3=True):
+        bs = bands_struct(b)
+        d = np.ascontiguousarray(dsq, np.uint8)
+        r = np.ascontiguousarray(root, np.float32)
+        cap = 4096
+        out = (Hit * cap)()
+        n = lib().ixo_hb_report_hits(C.byref(self.s), C.byref(bs), d.ctypes.data_as(C.POINTER(C.c_uint8)), i0, j0,
+                                     r.ctypes.data_as(_f32p), cutoff, int(do_null3), out, cap)
+        return [(out[k].i, out[k].j, out[k].score, out[k].bias) for k in range(min(n, cap))]
