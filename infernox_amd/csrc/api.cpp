@@ -11,9 +11,119 @@
 #include <string>
 #include <vector>
 #include <chrono>
-#include "ixg_internal.h"
+#include "ctx.h"
 
 using namespace ixg;
+
+namespace ixg {
+
+thread_local std::string g_err;
+
+int dmalloc(ixg_ctx *c, void **p, size_t n) {
+  if (n == 0) n = 1;
+  if (c->budget && c->in_use + n > c->budget) {
+    g_err = "device memory budget exceeded";
+    return IXG_ENOMEM;
+  }
+  HIPCHK(hipMalloc(p, n));
+  c->in_use += n;
+  return IXG_OK;
+}
+
+void dfree(ixg_ctx *c, void *p, size_t n) {
+  if (!p) return;
+  (void)hipFree(p);
+  c->in_use -= std::min(c->in_use, n ? n : (size_t)1);
+}
+
+// Scratch slot `slot` grown to at least n bytes (contents not preserved).
+int scratch(ixg_ctx *c, int slot, size_t n, void **out) {
+  if ((int)c->bufs.size() <= slot) c->bufs.resize(slot + 1);
+  DevBuf &b = c->bufs[slot];
+  if (b.cap < n) {
+    dfree(c, b.p, b.cap);
+    b.p = nullptr;
+    b.cap = 0;
+    size_t want = std::max(n, (size_t)4096);
+    want = std::max(want, n + n / 4);
+    RCHK(dmalloc(c, &b.p, want));
+    b.cap = want;
+  }
+  *out = b.p;
+  return IXG_OK;
+}
+
+template <typename T>
+static int upload(ixg_ctx *c, int slot, const std::vector<T> &v, T **out) {
+  void *p;
+  RCHK(scratch(c, slot, v.size() * sizeof(T), &p));
+  if (!v.empty()) HIPCHK(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  *out = (T *)p;
+  return IXG_OK;
+}
+
+}  // namespace ixg
+
+struct ixg_p7 {
+  MsvHost msv;
+  VitHost vit;
+  FwdHost fwd;
+  BiasHost bias;
+  uint32_t *d_cost4 = nullptr;
+  uint8_t *d_rbv = nullptr;
+  float *d_fwd = nullptr;   // tmain | tddv | rfvv
+  int16_t *d_vit = nullptr; // rwv | tr(8*(m+1))
+  size_t bytes = 0;
+  double vit_lvmu = 0, vit_lvlambda = 0;
+  SsvDev ssv_dev() const {
+    SsvDev d;
+    d.cost4 = d_cost4;
+    d.rbv = d_rbv;
+    d.m = msv.m;
+    d.bias = msv.bias_b;
+    d.base_b = msv.base_b;
+    d.tjb_b = msv.tjb_b;
+    d.tbm_b = msv.tbm_b;
+    int tj = msv.tjb_b + msv.tbm_b;
+    tj = tj > 255 ? 255 : tj;
+    d.xb = (uint8_t)(msv.base_b > tj ? msv.base_b - tj : 0);
+    d.scale_b = msv.scale_b;
+    return d;
+  }
+  FwdDev fwd_dev() const {
+    FwdDev d;
+    d.tmain = d_fwd;
+    d.tddv = d_fwd + fwd.tmain.size();
+    d.rfvv = d_fwd + fwd.tmain.size() + fwd.tddv.size();
+    d.m = fwd.m;
+    d.nq = fwd.nq;
+    return d;
+  }
+  VitDev vit_dev() const {
+    VitDev d;
+    d.rwv = d_vit;
+    d.tr = d_vit + vit.rwv.size();
+    d.m = vit.m;
+    d.base_w = vit.base_w;
+    d.xw_e_move = vit.xw_e_move;
+    d.xw_e_loop = vit.xw_e_loop;
+    d.scale_w = vit.scale_w;
+    return d;
+  }
+  BiasDev bias_dev() const {
+    BiasDev d;
+    memcpy(d.eo, bias.eo, sizeof(d.eo));
+    memcpy(d.t1, bias.t1, sizeof(d.t1));
+    memcpy(d.pi, bias.pi, sizeof(d.pi));
+    return d;
+  }
+};
+struct ixg_target {
+  uint8_t *d_both = nullptr;  // [fwd (L+2) | revcomp (L+2)], sentinel-padded
+  int64_t L = 0;
+  size_t bytes = 0;
+};
+
 
 extern "C" {
 

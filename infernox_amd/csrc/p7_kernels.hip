@@ -207,9 +207,25 @@ __device__ __forceinline__ float quad_dpp(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 
-__global__ __launch_bounds__(64) void fwd_kernel(FwdDev p, const uint8_t *seq, const WinDev *wins,
-                                                 int64_t nwin, float *sc) {
-  extern __shared__ float col[];  // [3][nq][64]
+// Columns live in registers (NQ = compile-time upper bound of the stripe
+// count nq, loops guarded by q < nq so the order is exactly the reference's);
+// the profile is staged in LDS, transitions re-laid out per (q, z) so a lane
+// reads its seven values with two 16-byte loads.
+template <int NQ>
+__global__ __launch_bounds__(64) void fwd_kernel(FwdDev p, const uint8_t *seq, const WinDev *wins, int64_t nwin,
+                                                 float *sc) {
+  extern __shared__ float4 lds4[];
+  const int nq = p.nq;
+  float4 *tr4 = lds4;                                  // [nq][4 z][2] : BM MM IM DM | MD MI II DD
+  float *rf = (float *)(lds4 + nq * 8);                // [18][nq][4]
+  for (int k = threadIdx.x; k < nq * 4; k += 64) {
+    const int q = k >> 2, z = k & 3;
+    const float *t = p.tmain + q * 28 + z;
+    tr4[k * 2] = make_float4(t[0], t[4], t[8], t[12]);
+    tr4[k * 2 + 1] = make_float4(t[16], t[20], t[24], p.tddv[q * 4 + z]);
+  }
+  for (int k = threadIdx.x; k < KP * nq * 4; k += 64) rf[k] = p.rfvv[k];
+  __syncthreads();
   const int lane = threadIdx.x, z = lane & 3;
   const int64_t widx = (int64_t)blockIdx.x * 16 + (lane >> 2);
   const bool has = widx < nwin;
@@ -220,13 +236,9 @@ __global__ __launch_bounds__(64) void fwd_kernel(FwdDev p, const uint8_t *seq, c
   w.a = 0.f;
   w.b = 0.f;
   if (has) w = wins[widx];
-  const int nq = p.nq;
-  float *mm = col, *dm = col + nq * 64, *im = col + 2 * nq * 64;
-  for (int q = 0; q < nq; q++) {
-    mm[q * 64 + lane] = 0.f;
-    dm[q * 64 + lane] = 0.f;
-    im[q * 64 + lane] = 0.f;
-  }
+  float mm[NQ], dm[NQ], im[NQ];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) mm[q] = dm[q] = im[q] = 0.f;
   int maxL = w.len;
   for (int off = 32; off >= 1; off >>= 1) {
     const int o = __shfl_xor(maxL, off);
@@ -236,76 +248,87 @@ __global__ __launch_bounds__(64) void fwd_kernel(FwdDev p, const uint8_t *seq, c
   const float pmove = (2.0f + 1.0f) / (L + 2.0f + 1.0f);
   const float ploop = 1.0f - pmove;
   float xn = 1.0f, xj = 0.0f, xb = pmove, xc = 0.0f, totscale = 0.0f;
-  const float *tm = p.tmain;
+  const bool lazy = p.m >= 100;
+  int xnext = (w.len >= 1) ? (int)seq[w.off + 1] : 0;
   for (int i = 1; i <= maxL; i++) {
     const bool act = i <= w.len;
-    const int x = act ? (int)seq[w.off + i] : 0;
-    const float *rp = p.rfvv + (size_t)x * (nq * 4);
+    const int x = act ? xnext : 0;
+    if (i + 1 <= w.len) xnext = seq[w.off + i + 1];
+    const float *rp = rf + x * (nq * 4) + z;
+    float lastm = 0.f, lastd = 0.f, lasti = 0.f;
+#pragma unroll
+    for (int q = 0; q < NQ; q++)
+      if (q == nq - 1) {
+        lastm = mm[q];
+        lastd = dm[q];
+        lasti = im[q];
+      }
+    float mpv = quad_dpp<DPP_QUAD_RSZ>(lastm);
+    float dpv = quad_dpp<DPP_QUAD_RSZ>(lastd);
+    float ipv = quad_dpp<DPP_QUAD_RSZ>(lasti);
+    if (z == 0) mpv = dpv = ipv = 0.f;
     float dcv = 0.f, xev = 0.f;
     const float xbv = xb;
-    float mpv = z ? mm[(nq - 1) * 64 + lane - 1] : 0.f;
-    float dpv = z ? dm[(nq - 1) * 64 + lane - 1] : 0.f;
-    float ipv = z ? im[(nq - 1) * 64 + lane - 1] : 0.f;
-    __builtin_amdgcn_wave_barrier();
-    for (int q = 0; q < nq; q++) {
-      const float *tb = tm + q * 28 + z;
-      float sv = xbv * tb[0];
-      sv = sv + mpv * tb[4];
-      sv = sv + ipv * tb[8];
-      sv = sv + dpv * tb[12];
-      sv = sv * rp[q * 4 + z];
-      xev = xev + sv;
-      mpv = mm[q * 64 + lane];
-      dpv = dm[q * 64 + lane];
-      ipv = im[q * 64 + lane];
-      mm[q * 64 + lane] = sv;
-      dm[q * 64 + lane] = dcv;
-      dcv = sv * tb[16];
-      const float svi = mpv * tb[20];
-      im[q * 64 + lane] = svi + ipv * tb[24];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      if (q < nq) {
+        const float4 ta = tr4[(q * 4 + z) * 2], tb = tr4[(q * 4 + z) * 2 + 1];
+        float sv = xbv * ta.x;
+        sv = sv + mpv * ta.y;
+        sv = sv + ipv * ta.z;
+        sv = sv + dpv * ta.w;
+        sv = sv * rp[q * 4];
+        xev = xev + sv;
+        mpv = mm[q];
+        dpv = dm[q];
+        ipv = im[q];
+        mm[q] = sv;
+        dm[q] = dcv;
+        dcv = sv * tb.x;
+        const float svi = mpv * tb.y;
+        im[q] = svi + ipv * tb.z;
+      }
     }
     // DD paths: first obligatory pass
     dcv = quad_dpp<DPP_QUAD_RSZ>(dcv);
     if (z == 0) dcv = 0.f;
-    dm[lane] = 0.f;
-    for (int q = 0; q < nq; q++) {
-      const float d = dcv + dm[q * 64 + lane];
-      dm[q * 64 + lane] = d;
-      dcv = d * p.tddv[q * 4 + z];
-    }
-    if (p.m < 100) {
-      for (int j = 1; j < 4; j++) {
-        dcv = quad_dpp<DPP_QUAD_RSZ>(dcv);
-        if (z == 0) dcv = 0.f;
-        for (int q = 0; q < nq; q++) {
-          const float d = dcv + dm[q * 64 + lane];
-          dm[q * 64 + lane] = d;
-          dcv = dcv * p.tddv[q * 4 + z];
-        }
+    dm[0] = 0.f;
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+      if (q < nq) {
+        const float d = dcv + dm[q];
+        dm[q] = d;
+        dcv = d * tr4[(q * 4 + z) * 2 + 1].w;
       }
-    } else {
-      bool done = false;
-      for (int j = 1; j < 4; j++) {
-        // `done` is uniform within a quad, so the quad shuffles below always
-        // read lanes of the same window.
-        int cv = 0;
-        dcv = quad_dpp<DPP_QUAD_RSZ>(dcv);
-        if (z == 0) dcv = 0.f;
-        if (!done) {
-          for (int q = 0; q < nq; q++) {
-            const float old = dm[q * 64 + lane];
+    }
+    bool done = false;
+    for (int j = 1; j < 4; j++) {
+      // `done` is uniform within a quad, so the quad shuffles only mix lanes
+      // of one window.
+      dcv = quad_dpp<DPP_QUAD_RSZ>(dcv);
+      if (z == 0) dcv = 0.f;
+      int cv = 0;
+      if (!done) {
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+          if (q < nq) {
+            const float old = dm[q];
             const float d = dcv + old;
             cv |= d > old;
-            dm[q * 64 + lane] = d;
-            dcv = dcv * p.tddv[q * 4 + z];
+            dm[q] = d;
+            dcv = dcv * tr4[(q * 4 + z) * 2 + 1].w;
           }
         }
+      }
+      if (lazy) {
         int any = cv | __builtin_amdgcn_update_dpp(0, cv, DPP_QUAD_X1, 0xF, 0xF, false);
         any = any | __builtin_amdgcn_update_dpp(0, any, DPP_QUAD_X2, 0xF, 0xF, false);
         if (!any) done = true;
       }
     }
-    for (int q = 0; q < nq; q++) xev = xev + dm[q * 64 + lane];
+#pragma unroll
+    for (int q = 0; q < NQ; q++)
+      if (q < nq) xev = xev + dm[q];
     const float p01 = xev + quad_dpp<DPP_QUAD_X1>(xev);
     const float xe = p01 + quad_dpp<DPP_QUAD_X2>(p01);
     if (act) {
@@ -319,15 +342,15 @@ __global__ __launch_bounds__(64) void fwd_kernel(FwdDev p, const uint8_t *seq, c
         xc *= inv;
         xj *= inv;
         xb *= inv;
-        for (int q = 0; q < nq; q++) {
-          mm[q * 64 + lane] *= inv;
-          dm[q * 64 + lane] *= inv;
-          im[q * 64 + lane] *= inv;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+          mm[q] *= inv;
+          dm[q] *= inv;
+          im[q] *= inv;
         }
         totscale += (float)log((double)xe);
       }
     }
-    __builtin_amdgcn_wave_barrier();
   }
   if (has && z == 0) sc[w.idx] = (float)((double)totscale + log((double)xc * (double)pmove));
 }
@@ -575,13 +598,28 @@ hipError_t launch_ssv_resolve(const SsvDev &p, const TaskDev *tasks, int64_t nta
   return hipGetLastError();
 }
 
+template <int NQ>
+static void fwd_k(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin, float *sc, size_t lds,
+                  hipStream_t s) {
+  hipLaunchKernelGGL(fwd_kernel<NQ>, dim3((unsigned)((nwin + 15) / 16)), dim3(64), lds, s, p, seq, wins, nwin, sc);
+}
 hipError_t launch_fwd(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin, float *sc,
                       hipStream_t s) {
   if (nwin == 0) return hipSuccess;
-  const unsigned nb = (unsigned)((nwin + 15) / 16);
-  const size_t lds = (size_t)3 * p.nq * 64 * sizeof(float);
+  const size_t lds = (size_t)p.nq * 8 * sizeof(float4) + (size_t)KP * p.nq * 4 * sizeof(float);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(fwd_kernel, dim3(nb), dim3(64), lds, s, p, seq, wins, nwin, sc);
+  const int q = p.nq;
+  if (q <= 8) fwd_k<8>(p, seq, wins, nwin, sc, lds, s);
+  else if (q <= 16) fwd_k<16>(p, seq, wins, nwin, sc, lds, s);
+  else if (q <= 24) fwd_k<24>(p, seq, wins, nwin, sc, lds, s);
+  else if (q <= 32) fwd_k<32>(p, seq, wins, nwin, sc, lds, s);
+  else if (q <= 48) fwd_k<48>(p, seq, wins, nwin, sc, lds, s);
+  else if (q <= 64) fwd_k<64>(p, seq, wins, nwin, sc, lds, s);
+  else if (q <= 80) fwd_k<80>(p, seq, wins, nwin, sc, lds, s);
+  else if (q <= 100) fwd_k<100>(p, seq, wins, nwin, sc, lds, s);
+  else if (q <= 128) fwd_k<128>(p, seq, wins, nwin, sc, lds, s);
+  else if (q <= 256) fwd_k<256>(p, seq, wins, nwin, sc, lds, s);
+  else return hipErrorInvalidValue;
   return hipGetLastError();
 }
 
