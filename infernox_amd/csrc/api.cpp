@@ -431,13 +431,15 @@ int run_f1(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<
   RCHK(upload(c, S_ITEMS, items, &d_items));
   uint64_t *d_mask;
   RCHK(scratch(c, S_MASK, (size_t)std::max<int64_t>(mask_words, 1) * 8, (void **)&d_mask));
+  uint32_t *d_cinfo;  // per flagged row: reset-free maximum << 16 | leftmost argmax
+  RCHK(scratch(c, S_CINFO, (size_t)std::max<int64_t>(mask_words, 1) * 64 * 4, (void **)&d_cinfo));
   int64_t total_len = 0;
   for (auto &t : tasks) total_len += t.L;
   int64_t cap = total_len / 32 + 256 * (int64_t)tasks.size() + 1024;
   unsigned long long *d_counter;
   RCHK(scratch(c, S_COUNTER, 8, (void **)&d_counter));
   HIPCHK(hipEventRecord(c->ev[0], c->stream));
-  HIPCHK(launch_ssv_sweep(sd, d_tasks, d_items, (int64_t)items.size(), d_mask, thresh, c->stream));
+  HIPCHK(launch_ssv_sweep(sd, d_tasks, d_items, (int64_t)items.size(), d_mask, d_cinfo, thresh, c->stream));
   HIPCHK(hipEventRecord(c->ev[1], c->stream));
   RawWinDev *d_raw = nullptr;
   unsigned long long count = 0;
@@ -446,7 +448,7 @@ int run_f1(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<
     static const unsigned long long zero = 0;
     HIPCHK(hipMemcpyAsync(d_counter, &zero, 8, hipMemcpyHostToDevice, c->stream));
     if (attempt) HIPCHK(hipEventRecord(c->ev[1], c->stream));
-    HIPCHK(launch_ssv_resolve(sd, d_tasks, (int64_t)tasks.size(), d_mask, thresh, d_raw, cap, d_counter,
+    HIPCHK(launch_ssv_resolve(sd, d_tasks, (int64_t)tasks.size(), d_mask, d_cinfo, thresh, d_raw, cap, d_counter,
                               c->stream));
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
     HIPCHK(hipMemcpyAsync(&count, d_counter, 8, hipMemcpyDeviceToHost, c->stream));

@@ -104,10 +104,10 @@ struct WinDev {
 // Kernel launchers (p7_kernels.hip).
 constexpr int SWEEP_SEG = 4096;  // rows per F1 sweep work item (multiple of 64)
 hipError_t launch_ssv_sweep(const SsvDev &p, const TaskDev *tasks, const SweepItem *items,
-                            int64_t nitems, uint64_t *mask, int thresh, hipStream_t s);
+                            int64_t nitems, uint64_t *mask, uint32_t *cinfo, int thresh, hipStream_t s);
 hipError_t launch_ssv_resolve(const SsvDev &p, const TaskDev *tasks, int64_t ntasks,
-                              const uint64_t *mask, int thresh, RawWinDev *out, int64_t cap,
-                              unsigned long long *counter, hipStream_t s);
+                              const uint64_t *mask, const uint32_t *cinfo, int thresh, RawWinDev *out,
+                              int64_t cap, unsigned long long *counter, hipStream_t s);
 hipError_t launch_fwd(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin,
                       float *sc, hipStream_t s);
 hipError_t launch_bias(const BiasDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin,

@@ -39,58 +39,79 @@ __device__ __forceinline__ uint32_t ssv_cell(uint32_t prev, uint32_t xb, uint32_
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void ssv_sweep_kernel(SsvDev p, const TaskDev *tasks,
                                                        const SweepItem *items, uint64_t *mask,
-                                                       int thresh) {
-  extern __shared__ uint8_t carry[];  // 2 * (m+1)
+                                                       uint32_t *cinfo, int thresh) {
+  extern __shared__ uint32_t sh[];  // cost4[m+1] | carry 2 x (m+4) bytes
   const SweepItem it = items[blockIdx.x];
   const TaskDev t = tasks[it.task];
   const int m = p.m;
   const int lane = threadIdx.x;
+  uint32_t *c4s = sh;
+  const int cstride = ((m + 4) + 3) & ~3;
+  uint8_t *carry = (uint8_t *)(sh + m + 1);
+  for (int k = lane; k <= m; k += 64) c4s[k] = p.cost4[k];
+  for (int k = lane; k < 2 * cstride; k += 64) carry[k] = 0;
+  __syncthreads();
   const int64_t s0 = 1 + (int64_t)it.seg * SWEEP_SEG;
   const int64_t s1 = min(s0 + (int64_t)SWEEP_SEG, t.L + 1);
   const int64_t warm = (int64_t)((m - 1 + 63) / 64) * 64;
   int64_t w0 = s0 - warm;
   if (w0 < 1) w0 = 1;
-  for (int k = lane; k <= m; k += 64) carry[k] = 0;
-  __syncthreads();
   const uint32_t xb = p.xb, bias = p.bias;
   int cb = 0;
   for (int64_t b = w0; b < s1; b += 64) {
     const int64_t i = b + lane;
     const bool valid = i < s1;
     const int x = valid ? (int)t.dsq[i] : 0;
-    uint8_t *cprev = carry + cb * (m + 1);
-    uint8_t *cnext = carry + (cb ^ 1) * (m + 1);
+    const uint8_t *cprev = carry + cb * cstride;
+    uint8_t *cnext = carry + (cb ^ 1) * cstride;
     uint32_t cur = 0, rowmax = 0;
-    for (int k = 1; k <= m; k++) {
-      const uint32_t cp = cprev[k - 1];
-      const uint32_t nb = (uint32_t)__builtin_amdgcn_update_dpp((int)cp, (int)cur, DPP_WAVE_SHR1, 0xF, 0xF, false);
-      const uint32_t c = ssv_cost(p, p.cost4[k], k, x);
-      cur = ssv_cell(nb, xb, bias, c);
-      rowmax = cur > rowmax ? cur : rowmax;
-      if (lane == 63) cnext[k] = (uint8_t)cur;
+    int arg = 0;
+    const bool degenerate = __any(x >= 4);
+    if (!degenerate) {
+      const uint32_t sh8 = 8u * (uint32_t)x;
+      for (int k = 1; k <= m; k++) {
+        const uint32_t cp = cprev[k - 1];
+        const uint32_t nb = (uint32_t)__builtin_amdgcn_update_dpp((int)cp, (int)cur, DPP_WAVE_SHR1, 0xF, 0xF, false);
+        const uint32_t c = __builtin_amdgcn_ubfe(c4s[k], sh8, 8);
+        cur = ssv_cell(nb, xb, bias, c);
+        if (cur > rowmax) { rowmax = cur; arg = k; }
+        if (lane == 63) cnext[k] = (uint8_t)cur;
+      }
+    } else {
+      for (int k = 1; k <= m; k++) {
+        const uint32_t cp = cprev[k - 1];
+        const uint32_t nb = (uint32_t)__builtin_amdgcn_update_dpp((int)cp, (int)cur, DPP_WAVE_SHR1, 0xF, 0xF, false);
+        const uint32_t c = ssv_cost(p, c4s[k], k, x);
+        cur = ssv_cell(nb, xb, bias, c);
+        if (cur > rowmax) { rowmax = cur; arg = k; }
+        if (lane == 63) cnext[k] = (uint8_t)cur;
+      }
     }
     if (lane == 63) cnext[0] = 0;
     __syncthreads();
     cb ^= 1;
     if (b >= s0) {
-      const uint64_t bits = __ballot(valid && (int)rowmax >= thresh);
+      const bool cand = valid && (int)rowmax >= thresh;
+      const uint64_t bits = __ballot(cand);
       if (lane == 0) mask[t.mask_off + (b - 1) / 64] = bits;
+      // the reset-free row maximum and its leftmost position, for the resolve
+      if (cand) cinfo[t.mask_off * 64 + (i - 1)] = (rowmax << 16) | (uint32_t)arg;
     }
   }
 }
 
 // ---------------------------------------------------------------------------
 // F1 resolve: one wave per task walks the flagged rows in order and replays
-// the reference's sequential logic exactly (cm_pipeline.rs:521-717): rows are
-// recomputed from the last restart (or from M-1 rows back, which yields the
-// same row because no cell reaches further back), the first flagged row whose
-// true maximum reaches the threshold triggers, and the diagonal is recovered
-// backward and extended forward on lane 0. Lanes hold KPL consecutive model
-// positions each.
+// the reference's sequential logic exactly (cm_pipeline.rs:521-717). A flagged
+// row at least M-1 rows past the last restart has its reset-free values, so
+// it triggers with the maximum and leftmost argmax the sweep recorded; rows
+// closer to a restart are recomputed from the restart (lanes hold KPL
+// consecutive model positions each). The diagonal is then recovered backward
+// and extended forward on lane 0.
 // ---------------------------------------------------------------------------
 template <int KPL>
 __global__ __launch_bounds__(64) void ssv_resolve_kernel(SsvDev p, const TaskDev *tasks,
-                                                         const uint64_t *mask, int thresh,
+                                                         const uint64_t *mask, const uint32_t *cinfo, int thresh,
                                                          RawWinDev *out, int64_t cap,
                                                          unsigned long long *counter) {
   const TaskDev t = tasks[blockIdx.x];
@@ -114,36 +135,47 @@ __global__ __launch_bounds__(64) void ssv_resolve_kernel(SsvDev p, const TaskDev
       bits &= bits - 1;
       const int64_t c = 64 * w + 1 + bi;
       if (c < r) continue;
-      if (c - pos + 1 > m) {
+      uint32_t bv;
+      int bk;
+      // Before the first trigger the scan is reset-free, so the sweep's
+      // values hold as well.
+      if (c >= r + m - 1 || r == 1) {
+        const uint32_t ci = cinfo[t.mask_off * 64 + (c - 1)];
+        bv = ci >> 16;
+        bk = (int)(ci & 0xFFFF);
+      } else {
+        // rows pos..c from the restart state, residues fetched 64 at a time
+        for (int64_t base = pos; base <= c; base += 64) {
+          const int64_t ri = base + lane;
+          const int xr = ri <= c ? (int)t.dsq[ri] : 0;
+          const int nrow = (int)min((int64_t)64, c - base + 1);
+          for (int rr = 0; rr < nrow; rr++) {
+            const int x = __builtin_amdgcn_readlane(xr, rr);
+            const uint32_t prev =
+                (uint32_t)__builtin_amdgcn_update_dpp(0, (int)d[KPL - 1], DPP_WAVE_SHR1, 0xF, 0xF, false);
 #pragma unroll
-        for (int j = 0; j < KPL; j++) d[j] = 0;
-        pos = c - m + 1;
-      }
-      for (int64_t i = pos; i <= c; i++) {
-        const int x = t.dsq[i];
-        uint32_t prev = (uint32_t)__shfl_up((int)d[KPL - 1], 1);
-        if (lane == 0) prev = 0;
-#pragma unroll
-        for (int j = KPL - 1; j >= 0; j--) {
-          const uint32_t in = j ? d[j - 1] : prev;
-          const uint32_t cc = ssv_cost(p, c4[j], kb + j, x);
-          d[j] = (kb + j <= m) ? ssv_cell(in, xb, bias, cc) : 0u;
+            for (int j = KPL - 1; j >= 0; j--) {
+              const uint32_t in = j ? d[j - 1] : prev;
+              const uint32_t cc = ssv_cost(p, c4[j], kb + j, x);
+              d[j] = (kb + j <= m) ? ssv_cell(in, xb, bias, cc) : 0u;
+            }
+          }
         }
-      }
-      pos = c + 1;
-      // row maximum and its leftmost position
-      uint32_t bv = 0;
-      int bk = 0x7FFFFFFF;
+        pos = c + 1;
+        // row maximum and its leftmost position
+        bv = 0;
+        bk = 0x7FFFFFFF;
 #pragma unroll
-      for (int j = 0; j < KPL; j++)
-        if (kb + j <= m && d[j] > bv) { bv = d[j]; bk = kb + j; }
-      if (bv == 0) bk = 0x7FFFFFFF;
-      for (int off = 32; off >= 1; off >>= 1) {
-        const uint32_t ov = (uint32_t)__shfl_xor((int)bv, off);
-        const int ok = __shfl_xor(bk, off);
-        if (ov > bv || (ov == bv && ok < bk)) { bv = ov; bk = ok; }
+        for (int j = 0; j < KPL; j++)
+          if (kb + j <= m && d[j] > bv) { bv = d[j]; bk = kb + j; }
+        for (int off = 32; off >= 1; off >>= 1) {
+          const uint32_t ov = (uint32_t)__shfl_xor((int)bv, off);
+          const int ok = __shfl_xor(bk, off);
+          if (ov > bv || (ov == bv && ok < bk)) { bv = ov; bk = ok; }
+        }
+        if (bv == 0) bk = 0;
       }
-      if ((int)bv >= thresh && bv > 0) {
+      if ((int)bv >= thresh && bk > 0) {
         int64_t max_end = 0;
         if (lane == 0) {
           const int end_k = bk;
@@ -571,29 +603,31 @@ __global__ void revcomp_kernel(const uint8_t *fwd, uint8_t *rc, int64_t L) {
 // launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_ssv_sweep(const SsvDev &p, const TaskDev *tasks, const SweepItem *items, int64_t nitems,
-                            uint64_t *mask, int thresh, hipStream_t s) {
+                            uint64_t *mask, uint32_t *cinfo, int thresh, hipStream_t s) {
   if (nitems == 0) return hipSuccess;
-  hipLaunchKernelGGL(ssv_sweep_kernel, dim3((unsigned)nitems), dim3(64), 2 * (p.m + 1), s, p, tasks, items,
-                     mask, thresh);
+  const size_t lds = (size_t)(p.m + 1) * 4 + 2 * (size_t)(((p.m + 4) + 3) & ~3);
+  hipLaunchKernelGGL(ssv_sweep_kernel, dim3((unsigned)nitems), dim3(64), lds, s, p, tasks, items, mask, cinfo,
+                     thresh);
   return hipGetLastError();
 }
 
 template <int KPL>
-static void resolve_k(const SsvDev &p, const TaskDev *tasks, int64_t ntasks, const uint64_t *mask, int thresh,
-                      RawWinDev *out, int64_t cap, unsigned long long *counter, hipStream_t s) {
-  hipLaunchKernelGGL(ssv_resolve_kernel<KPL>, dim3((unsigned)ntasks), dim3(64), 0, s, p, tasks, mask, thresh,
-                     out, cap, counter);
+static void resolve_k(const SsvDev &p, const TaskDev *tasks, int64_t ntasks, const uint64_t *mask,
+                      const uint32_t *cinfo, int thresh, RawWinDev *out, int64_t cap, unsigned long long *counter,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(ssv_resolve_kernel<KPL>, dim3((unsigned)ntasks), dim3(64), 0, s, p, tasks, mask, cinfo,
+                     thresh, out, cap, counter);
 }
 hipError_t launch_ssv_resolve(const SsvDev &p, const TaskDev *tasks, int64_t ntasks, const uint64_t *mask,
-                              int thresh, RawWinDev *out, int64_t cap, unsigned long long *counter,
-                              hipStream_t s) {
+                              const uint32_t *cinfo, int thresh, RawWinDev *out, int64_t cap,
+                              unsigned long long *counter, hipStream_t s) {
   if (ntasks == 0) return hipSuccess;
   const int kpl = (p.m + 63) / 64;
-  if (kpl <= 1) resolve_k<1>(p, tasks, ntasks, mask, thresh, out, cap, counter, s);
-  else if (kpl <= 2) resolve_k<2>(p, tasks, ntasks, mask, thresh, out, cap, counter, s);
-  else if (kpl <= 4) resolve_k<4>(p, tasks, ntasks, mask, thresh, out, cap, counter, s);
-  else if (kpl <= 8) resolve_k<8>(p, tasks, ntasks, mask, thresh, out, cap, counter, s);
-  else if (kpl <= 16) resolve_k<16>(p, tasks, ntasks, mask, thresh, out, cap, counter, s);
+  if (kpl <= 1) resolve_k<1>(p, tasks, ntasks, mask, cinfo, thresh, out, cap, counter, s);
+  else if (kpl <= 2) resolve_k<2>(p, tasks, ntasks, mask, cinfo, thresh, out, cap, counter, s);
+  else if (kpl <= 4) resolve_k<4>(p, tasks, ntasks, mask, cinfo, thresh, out, cap, counter, s);
+  else if (kpl <= 8) resolve_k<8>(p, tasks, ntasks, mask, cinfo, thresh, out, cap, counter, s);
+  else if (kpl <= 16) resolve_k<16>(p, tasks, ntasks, mask, cinfo, thresh, out, cap, counter, s);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
