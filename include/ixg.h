@@ -153,6 +153,68 @@ int ixg_f3_filter_tasks(ixg_ctx *ctx, const ixg_p7 *p7, const ixg_target *tgt,
                         const ixg_task *tasks, int64_t ntasks, const ixg_f3opts *o,
                         ixg_window *out, int64_t cap, int64_t *n, ixg_acct *acct);
 
+/* ---- covariance models: HMM-banded CYK / Inside scanners (F6/F7) --------------- */
+typedef struct ixg_cm ixg_cm;   /* device-resident configured CM */
+/* A probability-parameterized CM (cm.rs:315-563): states 0..M-1 with types
+ * (constants.rs:9-18), children cfirst[v]..cfirst[v]+cnum[v]-1 (for a B state
+ * cfirst is the left S and cnum the right S), node types and first states,
+ * transition probabilities t[M][6] and emission probabilities e[M][16]
+ * (pair states use all 16, singlets the first 4). */
+typedef struct {
+  int32_t M, nodes, clen, W;
+  float null[4];
+  float el_selfsc;      /* log2 of the EL self-transition */
+  float pbegin, pend;   /* local begin/end totals */
+  double n3_omega;
+  const int8_t *sttype; /* M */
+  const int8_t *ndtype; /* nodes */
+  const int32_t *nodemap; /* nodes */
+  const int32_t *cfirst, *cnum; /* M */
+  const float *t;       /* M*6 */
+  const float *e;       /* M*16 */
+} ixg_cm_params;
+/* Configures the CM for local search as cp9.rs:4538 cm_configure_scores does
+ * (cm_config_local + cm_logoddsify + the EL self-score bound) and uploads the
+ * score tables; ctx == NULL builds the host tables only. */
+int ixg_cm_create(ixg_ctx *ctx, const ixg_cm_params *p, ixg_cm **out);
+int ixg_cm_free(ixg_ctx *ctx, ixg_cm *cm);
+/* tsc M*6, oesc M*324, beginsc M, endsc M, el_selfsc 1 (any may be NULL). */
+int ixg_cm_tables(const ixg_cm *cm, float *tsc, float *oesc, float *beginsc, float *endsc, float *el_selfsc);
+
+/* HMM bands of one envelope (CP9Bands, cp9.rs:1453-1470): j band per state,
+ * d band per (v, j) at hd_off[v] + (j - jmin[v]); hd_off has M+1 entries. */
+typedef struct {
+  const int32_t *jmin, *jmax, *hd_off, *hdmin, *hdmax;
+} ixg_bands;
+typedef struct { int32_t i, j; float score, bias; int32_t env; } ixg_hit;
+
+/* cp9.rs:4682 fast_cyk_scan_hb(&CM, &CP9Bands, &[u8], i0, j0, env_cutoff)
+ * -> (vsc_root, envi, envj); envi/envj = -1 when no cell reaches env_cutoff.
+ * dsq is 1-indexed with L residues. */
+int ixg_cyk_scan_hb(ixg_ctx *ctx, const ixg_cm *cm, const ixg_bands *b, const uint8_t *dsq, int64_t L,
+                    int32_t i0, int32_t j0, float env_cutoff, float *vsc, int64_t *envi, int64_t *envj);
+/* cp9.rs:5465 fast_finside_scan_hb (inside = 1) and cp9.rs:4704
+ * fast_fcyk_scan_hb (inside = 0): the scan plus ReportHitsGreedily with the
+ * null3 correction (cp9.rs:5950-5997). */
+int ixg_inside_scan_hb(ixg_ctx *ctx, const ixg_cm *cm, const ixg_bands *b, const uint8_t *dsq, int64_t L,
+                       int32_t i0, int32_t j0, int inside, float env_cutoff, float cutoff, int do_null3,
+                       float *vsc, int64_t *envi, int64_t *envj, ixg_hit *hits, int64_t cap, int64_t *nhits);
+
+/* Batched form: n envelopes of one CM over a device-resident target, all in
+ * one launch. Envelope k scans [i0, j0] of strand `strand` (1-based
+ * coordinates of that strand's sequence; strand 1 is the reverse complement).
+ * Per-envelope results go to vsc/envi/envj[k]; with report != 0 the greedy
+ * hits of every envelope are appended to hits (tagged with env = k, grouped
+ * by envelope in order). cells (may be NULL) receives the number of banded
+ * cells filled. */
+typedef struct {
+  int32_t i0, j0, strand, pad;
+  ixg_bands bands;
+} ixg_envelope;
+int ixg_hb_scan_batch(ixg_ctx *ctx, const ixg_cm *cm, const ixg_target *tgt, const ixg_envelope *env, int64_t n,
+                      int inside, float env_cutoff, int report, float cutoff, int do_null3, float *vsc,
+                      int64_t *envi, int64_t *envj, ixg_hit *hits, int64_t cap, int64_t *nhits, int64_t *cells);
+
 /* Per-stage device timings (milliseconds, HIP events) of the most recent
  * ixg_f3_filter_tasks call on ctx: [0] F1 sweep, [1] F1 resolve, [2] F2/F3/F3b
  * window kernels, [3] host glue, [4] total. Also the launch counts. */

@@ -1,0 +1,138 @@
+// Host-side covariance-model score configuration (once per model): local
+// begins/ends, log-odds transitions and emissions, EL self score. Mirrors
+// cp9.rs:4194-4545 (cm_config_local, cm_logoddsify, cm_configure_scores).
+#include <cmath>
+#include <cstring>
+#include "cm_internal.h"
+
+namespace ixg {
+
+static double sre_log2(double x) { return x > 0.0 ? std::log(x) * 1.44269504 : -1e36; }  // cp9.rs:961
+
+static const unsigned char DEGEN[KP][4] = {  // cm.rs:24-43
+    {1, 0, 0, 0}, {0, 1, 0, 0}, {0, 0, 1, 0}, {0, 0, 0, 1}, {0, 0, 0, 0}, {1, 0, 1, 0}, {0, 1, 0, 1},
+    {1, 1, 0, 0}, {0, 0, 1, 1}, {0, 1, 1, 0}, {1, 0, 0, 1}, {1, 1, 0, 1}, {0, 1, 1, 1}, {1, 1, 1, 0},
+    {1, 0, 1, 1}, {1, 1, 1, 1}, {0, 0, 0, 0}, {0, 0, 0, 0}};
+static int ndeg(int x) { return DEGEN[x][0] + DEGEN[x][1] + DEGEN[x][2] + DEGEN[x][3]; }
+static float frac(int x, int l) { return DEGEN[x][l] ? 1.0f / (float)ndeg(x) : 0.0f; }
+static float favg(int x, const float *sc) {
+  if (!(x < 4 || (x > 4 && x < KP - 2))) return 0.0f;
+  float r = 0.0f;
+  for (int i = 0; i < 4; i++)
+    if (DEGEN[x][i]) r += sc[i];
+  return r / (float)ndeg(x);
+}
+static float kahan_sum(const float *v, int n) {  // cp9.rs:451 esl_vec_fsum
+  float sum = 0.0f, c = 0.0f;
+  for (int i = 0; i < n; i++) {
+    float y = v[i] - c;
+    float t = sum + y;
+    c = (t - sum) - y;
+    sum = t;
+  }
+  return sum;
+}
+
+int configure_cm(const ixg_cm_params &p, CmHost &h) {
+  const int M = p.M;
+  h.M = M;
+  h.sttype.assign(p.sttype, p.sttype + M);
+  h.cfirst.assign(p.cfirst, p.cfirst + M);
+  h.cnum.assign(p.cnum, p.cnum + M);
+  memcpy(h.null, p.null, sizeof(h.null));
+  h.n3_omega = p.n3_omega;
+  std::vector<float> t(p.t, p.t + (size_t)M * MAXC), begin(M, 0.f), end(M, 0.f);
+  auto is_start = [&](int nd) {
+    int ty = p.ndtype[nd];
+    return ty == MATP_ND || ty == MATL_ND || ty == MATR_ND || ty == BIF_ND;
+  };
+  int nstarts = 0;
+  for (int nd = 2; nd < p.nodes; nd++) nstarts += is_start(nd);
+  begin[p.nodemap[1]] = 1.0f - p.pbegin;
+  const float pb = p.pbegin / (float)nstarts;
+  for (int nd = 2; nd < p.nodes; nd++)
+    if (is_start(nd)) begin[p.nodemap[nd]] = pb;
+  for (int y = 0; y < p.cnum[0]; y++) t[y] = 0.f;
+  auto is_exit = [&](int nd) {
+    int ty = p.ndtype[nd];
+    return (ty == MATP_ND || ty == MATL_ND || ty == MATR_ND || ty == BEGL_ND || ty == BEGR_ND) &&
+           p.ndtype[nd + 1] != END_ND;
+  };
+  int nexits = 0;
+  for (int nd = 1; nd < p.nodes; nd++) nexits += is_exit(nd);
+  for (int nd = 1; nd < p.nodes; nd++)
+    if (is_exit(nd)) {
+      const int v = p.nodemap[nd];
+      end[v] = p.pend / (float)nexits;
+      float denom = kahan_sum(&t[(size_t)v * MAXC], p.cnum[v]);
+      denom += end[v];
+      const float inv = 1.0f / denom;
+      for (int x = 0; x < p.cnum[v]; x++) t[(size_t)v * MAXC + x] *= inv;
+    }
+  h.local_begin = 1;
+  h.el_selfsc = p.el_selfsc;
+  if (h.el_selfsc * (float)p.W < IMPOSSIBLE) h.el_selfsc = IMPOSSIBLE / ((float)p.W + 1.0f);
+  h.tsc.assign((size_t)M * MAXC, 0.f);
+  h.oesc.assign((size_t)M * KP * KP, 0.f);
+  h.beginsc.assign(M, 0.f);
+  h.endsc.assign(M, 0.f);
+  for (int v = 0; v < M; v++) {
+    const int st = p.sttype[v];
+    if (st != B_ST && st != E_ST)
+      for (int x = 0; x < p.cnum[v]; x++) h.tsc[(size_t)v * MAXC + x] = (float)sre_log2((double)t[(size_t)v * MAXC + x]);
+    float *oe = &h.oesc[(size_t)v * KP * KP];
+    const float *e = p.e + (size_t)v * 16;
+    if (st == MP_ST) {
+      float esc[16];
+      for (int x = 0; x < 4; x++)
+        for (int y = 0; y < 4; y++) esc[x * 4 + y] = (float)sre_log2((double)(e[x * 4 + y] / (p.null[x] * p.null[y])));
+      for (int i = 0; i < KP * KP; i++) oe[i] = IMPOSSIBLE;
+      for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) oe[a * KP + b] = esc[a * 4 + b];
+      for (int a = 5; a < KP - 1; a++)
+        for (int b = 0; b < 4; b++) {
+          float sc = 0.f;
+          for (int l = 0; l < 4; l++) sc += esc[l * 4 + b] * frac(a, l);
+          oe[a * KP + b] = sc;
+        }
+      for (int a = 0; a < 4; a++)
+        for (int b = 5; b < KP - 1; b++) {
+          float sc = 0.f;
+          for (int r = 0; r < 4; r++) sc += esc[a * 4 + r] * frac(b, r);
+          oe[a * KP + b] = sc;
+        }
+      for (int a = 5; a < KP - 1; a++)
+        for (int b = 5; b < KP - 1; b++) {
+          float sc = 0.f;
+          for (int l = 0; l < 4; l++)
+            for (int r = 0; r < 4; r++) sc += esc[l * 4 + r] * frac(a, l) * frac(b, r);
+          oe[a * KP + b] = sc;
+        }
+    } else if (st == ML_ST || st == MR_ST || st == IL_ST || st == IR_ST) {
+      float esc[4];
+      for (int x = 0; x < 4; x++) esc[x] = (float)sre_log2((double)(e[x] / p.null[x]));
+      for (int a = 0; a < KP; a++) oe[a] = IMPOSSIBLE;
+      for (int a = 0; a < 4; a++) oe[a] = esc[a];
+      for (int a = 5; a < KP - 1; a++) oe[a] = favg(a, esc);
+    }
+    h.beginsc[v] = (float)sre_log2((double)begin[v]);
+    h.endsc[v] = (float)sre_log2((double)end[v]);
+  }
+  h.begin_list.clear();
+  for (int y = 1; y < M; y++)
+    if (h.beginsc[y] > -9.999e35f) h.begin_list.push_back(y);
+  return IXG_OK;
+}
+
+void flogsum_table(std::vector<float> &t) {  // cp9.rs:5391-5401
+  t.resize(FLOGSUM_TBL);
+  for (int i = 0; i < FLOGSUM_TBL; i++) t[i] = (float)sre_log2(1.0 + std::exp((-(double)i / 1000.0) * 0.69314718));
+}
+
+float host_flogsum(const std::vector<float> &t, float a, float b) {  // cp9.rs:5405
+  const float mx = a > b ? a : b, mn = a < b ? a : b;
+  if (mn == -INFINITY || (mx - mn) >= 23.0f) return mx;
+  return mx + t[(size_t)((mx - mn) * 1000.0f)];
+}
+
+}  // namespace ixg

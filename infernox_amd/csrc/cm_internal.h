@@ -1,0 +1,54 @@
+// Internal declarations of the covariance-model scanners.
+#pragma once
+#include "ixg_internal.h"
+
+namespace ixg {
+
+constexpr int MAXC = 6;
+constexpr float IMPOSSIBLE = -1e36f;  // cp9.rs:4189
+constexpr int FLOGSUM_TBL = 23000;     // cp9.rs:5388
+enum { D_ST = 0, MP_ST = 1, ML_ST = 2, MR_ST = 3, IL_ST = 4, IR_ST = 5, S_ST = 6, E_ST = 7, B_ST = 8 };
+enum { BIF_ND = 0, MATP_ND = 1, MATL_ND = 2, MATR_ND = 3, BEGL_ND = 4, BEGR_ND = 5, ROOT_ND = 6, END_ND = 7 };
+
+// Host copy of a configured CM (the tables the scanners read).
+struct CmHost {
+  int M = 0;
+  int local_begin = 1;
+  float el_selfsc = 0.f;
+  float null[4];
+  double n3_omega = 0;
+  std::vector<int8_t> sttype;
+  std::vector<int32_t> cfirst, cnum;
+  std::vector<float> tsc, oesc, beginsc, endsc;  // M*6, M*324, M, M
+  std::vector<int32_t> begin_list;               // states y >= 1 with a local begin
+};
+int configure_cm(const ixg_cm_params &p, CmHost &h);
+void flogsum_table(std::vector<float> &t);
+float host_flogsum(const std::vector<float> &t, float a, float b);
+
+struct CmDev {
+  const int8_t *sttype;
+  const int32_t *cfirst, *cnum;
+  const float *tsc, *oesc, *beginsc, *endsc;
+  const int32_t *begin_list;
+  int nbegin;
+  int M;
+  int local_begin;
+  float el_selfsc;
+};
+
+// One envelope of a batched banded scan. Band and layout arrays are indexed
+// like the reference's CP9Bands: jmin/jmax per state, hd_off per state (M+1),
+// and per (v, j - jmin[v]) at hd_off[v] + jp: hdmin, hdmax and roff, the
+// offset of the row's first cell in alpha.
+struct EnvDev {
+  const uint8_t *dsq;
+  const int32_t *jmin, *jmax, *hd_off, *hdmin, *hdmax, *roff;
+  float *alpha;
+  int32_t i0, j0;
+};
+
+hipError_t launch_hb_scan(const CmDev &cm, const EnvDev *envs, int64_t n, const float *ftbl, int inside,
+                          hipStream_t s);
+
+}  // namespace ixg

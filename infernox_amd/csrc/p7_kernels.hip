@@ -239,154 +239,6 @@ __device__ __forceinline__ float quad_dpp(float v) {
   return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 
-// Columns live in registers (NQ = compile-time upper bound of the stripe
-// count nq, loops guarded by q < nq so the order is exactly the reference's);
-// the profile is staged in LDS, transitions re-laid out per (q, z) so a lane
-// reads its seven values with two 16-byte loads.
-template <int NQ>
-__global__ __launch_bounds__(64) void fwd_kernel(FwdDev p, const uint8_t *seq, const WinDev *wins, int64_t nwin,
-                                                 float *sc) {
-  extern __shared__ float4 lds4[];
-  const int nq = p.nq;
-  float4 *tr4 = lds4;                                  // [nq][4 z][2] : BM MM IM DM | MD MI II DD
-  float *rf = (float *)(lds4 + nq * 8);                // [18][nq][4]
-  for (int k = threadIdx.x; k < nq * 4; k += 64) {
-    const int q = k >> 2, z = k & 3;
-    const float *t = p.tmain + q * 28 + z;
-    tr4[k * 2] = make_float4(t[0], t[4], t[8], t[12]);
-    tr4[k * 2 + 1] = make_float4(t[16], t[20], t[24], p.tddv[q * 4 + z]);
-  }
-  for (int k = threadIdx.x; k < KP * nq * 4; k += 64) rf[k] = p.rfvv[k];
-  __syncthreads();
-  const int lane = threadIdx.x, z = lane & 3;
-  const int64_t widx = (int64_t)blockIdx.x * 16 + (lane >> 2);
-  const bool has = widx < nwin;
-  WinDev w;
-  w.off = 0;
-  w.len = 0;
-  w.idx = 0;
-  w.a = 0.f;
-  w.b = 0.f;
-  if (has) w = wins[widx];
-  float mm[NQ], dm[NQ], im[NQ];
-#pragma unroll
-  for (int q = 0; q < NQ; q++) mm[q] = dm[q] = im[q] = 0.f;
-  int maxL = w.len;
-  for (int off = 32; off >= 1; off >>= 1) {
-    const int o = __shfl_xor(maxL, off);
-    maxL = o > maxL ? o : maxL;
-  }
-  const float L = (float)w.len;
-  const float pmove = (2.0f + 1.0f) / (L + 2.0f + 1.0f);
-  const float ploop = 1.0f - pmove;
-  float xn = 1.0f, xj = 0.0f, xb = pmove, xc = 0.0f, totscale = 0.0f;
-  const bool lazy = p.m >= 100;
-  int xnext = (w.len >= 1) ? (int)seq[w.off + 1] : 0;
-  for (int i = 1; i <= maxL; i++) {
-    const bool act = i <= w.len;
-    const int x = act ? xnext : 0;
-    if (i + 1 <= w.len) xnext = seq[w.off + i + 1];
-    const float *rp = rf + x * (nq * 4) + z;
-    float lastm = 0.f, lastd = 0.f, lasti = 0.f;
-#pragma unroll
-    for (int q = 0; q < NQ; q++)
-      if (q == nq - 1) {
-        lastm = mm[q];
-        lastd = dm[q];
-        lasti = im[q];
-      }
-    float mpv = quad_dpp<DPP_QUAD_RSZ>(lastm);
-    float dpv = quad_dpp<DPP_QUAD_RSZ>(lastd);
-    float ipv = quad_dpp<DPP_QUAD_RSZ>(lasti);
-    if (z == 0) mpv = dpv = ipv = 0.f;
-    float dcv = 0.f, xev = 0.f;
-    const float xbv = xb;
-#pragma unroll
-    for (int q = 0; q < NQ; q++) {
-      if (q < nq) {
-        const float4 ta = tr4[(q * 4 + z) * 2], tb = tr4[(q * 4 + z) * 2 + 1];
-        float sv = xbv * ta.x;
-        sv = sv + mpv * ta.y;
-        sv = sv + ipv * ta.z;
-        sv = sv + dpv * ta.w;
-        sv = sv * rp[q * 4];
-        xev = xev + sv;
-        mpv = mm[q];
-        dpv = dm[q];
-        ipv = im[q];
-        mm[q] = sv;
-        dm[q] = dcv;
-        dcv = sv * tb.x;
-        const float svi = mpv * tb.y;
-        im[q] = svi + ipv * tb.z;
-      }
-    }
-    // DD paths: first obligatory pass
-    dcv = quad_dpp<DPP_QUAD_RSZ>(dcv);
-    if (z == 0) dcv = 0.f;
-    dm[0] = 0.f;
-#pragma unroll
-    for (int q = 0; q < NQ; q++) {
-      if (q < nq) {
-        const float d = dcv + dm[q];
-        dm[q] = d;
-        dcv = d * tr4[(q * 4 + z) * 2 + 1].w;
-      }
-    }
-    bool done = false;
-    for (int j = 1; j < 4; j++) {
-      // `done` is uniform within a quad, so the quad shuffles only mix lanes
-      // of one window.
-      dcv = quad_dpp<DPP_QUAD_RSZ>(dcv);
-      if (z == 0) dcv = 0.f;
-      int cv = 0;
-      if (!done) {
-#pragma unroll
-        for (int q = 0; q < NQ; q++) {
-          if (q < nq) {
-            const float old = dm[q];
-            const float d = dcv + old;
-            cv |= d > old;
-            dm[q] = d;
-            dcv = dcv * tr4[(q * 4 + z) * 2 + 1].w;
-          }
-        }
-      }
-      if (lazy) {
-        int any = cv | __builtin_amdgcn_update_dpp(0, cv, DPP_QUAD_X1, 0xF, 0xF, false);
-        any = any | __builtin_amdgcn_update_dpp(0, any, DPP_QUAD_X2, 0xF, 0xF, false);
-        if (!any) done = true;
-      }
-    }
-#pragma unroll
-    for (int q = 0; q < NQ; q++)
-      if (q < nq) xev = xev + dm[q];
-    const float p01 = xev + quad_dpp<DPP_QUAD_X1>(xev);
-    const float xe = p01 + quad_dpp<DPP_QUAD_X2>(p01);
-    if (act) {
-      xn = xn * ploop;
-      xc = (xc * ploop) + (xe * 0.5f);
-      xj = (xj * ploop) + (xe * 0.5f);
-      xb = (xj * pmove) + (xn * pmove);
-      if (xe > 1.0e4f) {
-        const float inv = 1.0f / xe;
-        xn *= inv;
-        xc *= inv;
-        xj *= inv;
-        xb *= inv;
-#pragma unroll
-        for (int q = 0; q < NQ; q++) {
-          mm[q] *= inv;
-          dm[q] *= inv;
-          im[q] *= inv;
-        }
-        totscale += (float)log((double)xe);
-      }
-    }
-  }
-  if (has && z == 0) sc[w.idx] = (float)((double)totscale + log((double)xc * (double)pmove));
-}
-
 // ---------------------------------------------------------------------------
 // F3b composition bias: the reference's 2-state filter-null Forward with
 // per-row max rescaling (cm_pipeline.rs:1722-1774), one lane per window. The
@@ -628,31 +480,6 @@ hipError_t launch_ssv_resolve(const SsvDev &p, const TaskDev *tasks, int64_t nta
   else if (kpl <= 4) resolve_k<4>(p, tasks, ntasks, mask, cinfo, thresh, out, cap, counter, s);
   else if (kpl <= 8) resolve_k<8>(p, tasks, ntasks, mask, cinfo, thresh, out, cap, counter, s);
   else if (kpl <= 16) resolve_k<16>(p, tasks, ntasks, mask, cinfo, thresh, out, cap, counter, s);
-  else return hipErrorInvalidValue;
-  return hipGetLastError();
-}
-
-template <int NQ>
-static void fwd_k(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin, float *sc, size_t lds,
-                  hipStream_t s) {
-  hipLaunchKernelGGL(fwd_kernel<NQ>, dim3((unsigned)((nwin + 15) / 16)), dim3(64), lds, s, p, seq, wins, nwin, sc);
-}
-hipError_t launch_fwd(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin, float *sc,
-                      hipStream_t s) {
-  if (nwin == 0) return hipSuccess;
-  const size_t lds = (size_t)p.nq * 8 * sizeof(float4) + (size_t)KP * p.nq * 4 * sizeof(float);
-  if (lds > 160 * 1024) return hipErrorInvalidValue;
-  const int q = p.nq;
-  if (q <= 8) fwd_k<8>(p, seq, wins, nwin, sc, lds, s);
-  else if (q <= 16) fwd_k<16>(p, seq, wins, nwin, sc, lds, s);
-  else if (q <= 24) fwd_k<24>(p, seq, wins, nwin, sc, lds, s);
-  else if (q <= 32) fwd_k<32>(p, seq, wins, nwin, sc, lds, s);
-  else if (q <= 48) fwd_k<48>(p, seq, wins, nwin, sc, lds, s);
-  else if (q <= 64) fwd_k<64>(p, seq, wins, nwin, sc, lds, s);
-  else if (q <= 80) fwd_k<80>(p, seq, wins, nwin, sc, lds, s);
-  else if (q <= 100) fwd_k<100>(p, seq, wins, nwin, sc, lds, s);
-  else if (q <= 128) fwd_k<128>(p, seq, wins, nwin, sc, lds, s);
-  else if (q <= 256) fwd_k<256>(p, seq, wins, nwin, sc, lds, s);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
