@@ -122,7 +122,16 @@ struct ixg_target {
   uint8_t *d_both = nullptr;  // [fwd (L+2) | revcomp (L+2)], sentinel-padded
   int64_t L = 0;
   size_t bytes = 0;
+  std::vector<uint8_t> h_both;  // host copy of the same two strands
 };
+
+const uint8_t *ixg_target_device_base(const ixg_target *t, int64_t *L) {
+  *L = t->L;
+  return t->d_both;
+}
+const uint8_t *ixg_target_host_dsq(const ixg_target *t, int strand) {
+  return t->h_both.data() + (strand ? t->L + 2 : 0);
+}
 
 
 extern "C" {
@@ -283,9 +292,14 @@ int ixg_target_create(ixg_ctx *c, const uint8_t *dsq, int64_t L, ixg_target **ou
   t->L = L;
   t->bytes = (size_t)2 * (L + 2);
   RCHK(dmalloc(c, (void **)&t->d_both, t->bytes));
-  std::vector<uint8_t> h(L + 2, 255);
-  if (L) memcpy(h.data() + 1, dsq + 1, L);
-  HIPCHK(hipMemcpyAsync(t->d_both, h.data(), L + 2, hipMemcpyHostToDevice, c->stream));
+  t->h_both.assign(2 * (L + 2), 255);
+  uint8_t *h = t->h_both.data();
+  if (L) memcpy(h + 1, dsq + 1, L);
+  for (int64_t j = 1; j <= L; j++) {  // reverse complement (cm_search.rs:1822-1827)
+    const uint8_t d = dsq[L + 1 - j];
+    h[L + 2 + j] = d < 4 ? (uint8_t)(3 - d) : d;
+  }
+  HIPCHK(hipMemcpyAsync(t->d_both, h, L + 2, hipMemcpyHostToDevice, c->stream));
   HIPCHK(launch_revcomp(t->d_both, t->d_both + L + 2, L, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   *out = t.release();

@@ -1,1 +1,454 @@
-// placeholder
+// C ABI of the HMM-banded CYK / Inside scanners: resident CM score tables,
+// band validation and deck layout (cp9.rs:4741-4763), batched launches, and
+// the host side of the reference's reporting (vsc_root and the envelope,
+// cp9.rs:5305-5325; ReportHitsGreedily with null3, cp9.rs:5950-5997).
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include "cm_internal.h"
+#include "ctx.h"
+
+using namespace ixg;
+
+struct ixg_cm {
+  CmHost h;
+  std::vector<float> ftbl_host;
+  int8_t *d_st = nullptr;
+  int32_t *d_i32 = nullptr;  // cfirst | cnum | begin_list
+  float *d_f = nullptr;      // tsc | oesc | beginsc | endsc | flogsum table
+  size_t bytes = 0;
+  CmDev dev() const {
+    CmDev c;
+    const int M = h.M;
+    c.sttype = d_st;
+    c.cfirst = d_i32;
+    c.cnum = d_i32 + M;
+    c.begin_list = d_i32 + 2 * M;
+    c.nbegin = (int)h.begin_list.size();
+    c.tsc = d_f;
+    c.oesc = d_f + (size_t)M * MAXC;
+    c.beginsc = c.oesc + (size_t)M * KP * KP;
+    c.endsc = c.beginsc + M;
+    c.M = M;
+    c.local_begin = h.local_begin;
+    c.el_selfsc = h.el_selfsc;
+    return c;
+  }
+  const float *d_ftbl() const { return d_f + (size_t)h.M * MAXC + (size_t)h.M * KP * KP + 2 * (size_t)h.M; }
+};
+
+namespace {
+
+int sdelta_host(int st) { return st == MP_ST ? 2 : ((st >= ML_ST && st <= IR_ST) ? 1 : 0); }
+
+// Deck layout of one envelope: roff per (v, jp) and the cell count, after
+// checking that every cell stays inside [i0, j0] and that the d bands respect
+// the states' emission counts (as CP9-derived bands do, cp9.rs:3815-3850).
+int layout(const CmHost &h, const ixg_bands &b, int32_t i0, int32_t j0, std::vector<int32_t> &roff,
+           int64_t &ncells, int32_t &nrows) {
+  const int M = h.M;
+  if (!b.jmin || !b.jmax || !b.hd_off || !b.hdmin || !b.hdmax || i0 < 1 || j0 < i0 - 1) {
+    g_err = "bands: null array or bad envelope";
+    return IXG_EINVAL;
+  }
+  nrows = b.hd_off[M];
+  roff.assign(nrows > 0 ? nrows : 1, 0);
+  int64_t nc = 0;
+  for (int v = 0; v < M; v++) {
+    const int nj = b.jmax[v] >= b.jmin[v] ? b.jmax[v] - b.jmin[v] + 1 : 0;
+    if (b.hd_off[v] < 0 || b.hd_off[v] + nj > nrows || (v + 1 <= M && b.hd_off[v + 1] < b.hd_off[v] + nj)) {
+      g_err = "bands: hd_off inconsistent with the j bands";
+      return IXG_EINVAL;
+    }
+    if (nj && (b.jmin[v] < i0 - 1 || b.jmax[v] > j0)) {
+      g_err = "bands: j band outside the envelope";
+      return IXG_EINVAL;
+    }
+    const int st = h.sttype[v];
+    for (int jp = 0; jp < nj; jp++) {
+      const int r = b.hd_off[v] + jp, j = b.jmin[v] + jp;
+      roff[r] = (int32_t)nc;
+      const int lo = b.hdmin[r], hi = b.hdmax[r];
+      if (hi >= lo) {
+        if (lo < (st == E_ST ? 0 : sdelta_host(st)) || j - hi + 1 < i0 || (st == E_ST && (lo != 0 || hi != 0))) {
+          g_err = "bands: d band outside the envelope or below the state's emission count";
+          return IXG_EINVAL;
+        }
+        nc += hi - lo + 1;
+      } else if (st == E_ST) {
+        g_err = "bands: empty row of an end state";
+        return IXG_EINVAL;
+      }
+    }
+  }
+  if (nc >= INT32_MAX) {
+    g_err = "bands: deck larger than 2^31 cells";
+    return IXG_EINVAL;
+  }
+  ncells = nc;
+  return IXG_OK;
+}
+
+// vsc_root and the envelope bounds from the root deck (cp9.rs:5305-5325).
+void root_summary(const ixg_bands &b, const float *root, int32_t i0, int32_t j0, float env_cutoff, float &vsc,
+                  int64_t &envi, int64_t &envj) {
+  float vr = IMPOSSIBLE;
+  int64_t ei = (int64_t)j0 + 1, ej = (int64_t)i0 - 1;
+  int64_t off = 0;
+  for (int j = b.jmin[0]; j <= b.jmax[0]; j++) {
+    const int r = b.hd_off[0] + (j - b.jmin[0]);
+    for (int d = b.hdmin[r]; d <= b.hdmax[r]; d++) {
+      const float s = root[off + (d - b.hdmin[r])];
+      if (s > vr) vr = s;
+      if (s >= env_cutoff) {
+        ei = std::min<int64_t>(ei, j - d + 1);
+        ej = std::max<int64_t>(ej, j);
+      }
+    }
+    if (b.hdmax[r] >= b.hdmin[r]) off += b.hdmax[r] - b.hdmin[r] + 1;
+  }
+  vsc = vr;
+  envi = ei == (int64_t)j0 + 1 ? -1 : ei;
+  envj = ej == (int64_t)i0 - 1 ? -1 : ej;
+}
+
+double sre_log2(double x) { return x > 0.0 ? std::log(x) * 1.44269504 : -1e36; }
+
+float kahan(const float *v, int n) {
+  float sum = 0.f, c = 0.f;
+  for (int i = 0; i < n; i++) {
+    float y = v[i] - c;
+    float t = sum + y;
+    c = (t - sum) - y;
+    sum = t;
+  }
+  return sum;
+}
+
+// cp9.rs:6516-6529 score_correction_null3
+float null3(const ixg_cm &cm, const float *comp, int len) {
+  float score = 0.f;
+  for (int a = 0; a < 4; a++) {
+    double term = sre_log2((double)(comp[a] / cm.h.null[a])) * (double)comp[a] * (double)len;
+    score = (float)((double)score + term);
+  }
+  score = (float)((double)score + sre_log2((double)(float)cm.h.n3_omega));
+  return host_flogsum(cm.ftbl_host, 0.0f, score);
+}
+
+// ReportHitsGreedily over the root deck (cp9.rs:5887-5900, 5950-5997).
+void report_hits(const ixg_cm &cm, const ixg_bands &b, const uint8_t *dsq, int32_t i0, int32_t j0,
+                 const float *root, float cutoff, int do_null3, int32_t env, std::vector<ixg_hit> &out) {
+  const int W = j0 - i0 + 1;
+  std::vector<double> act;
+  if (do_null3) {
+    act.assign((size_t)(W + 1) * 4, 0.0);
+    for (int jp = 1; jp <= W; jp++) {
+      for (int a = 0; a < 4; a++) act[jp * 4 + a] = act[(jp - 1) * 4 + a];
+      const int r = dsq[i0 + jp - 1];
+      if (r < 4) act[jp * 4 + r] += 1.0;
+    }
+  }
+  int64_t off = 0;
+  for (int j = b.jmin[0]; j <= b.jmax[0]; j++) {
+    const int r = b.hd_off[0] + (j - b.jmin[0]);
+    const int dmin = b.hdmin[r], dmax = b.hdmax[r];
+    if (dmin <= dmax) {
+      float maxrep = IMPOSSIBLE;
+      for (int d = std::max(dmin, 1); d <= dmax; d++) {
+        const int i = j - d + 1;
+        float hs = root[off + (d - dmin)];
+        if (hs > maxrep && hs >= cutoff && hs > -9.999e35f) {
+          bool rep = true;
+          float bias = 0.f;
+          if (do_null3) {
+            float comp[4];
+            const int ip = i - i0 + 1, jj = j - i0 + 1;
+            for (int a = 0; a < 4; a++) comp[a] = (float)(act[jj * 4 + a] - act[(ip - 1) * 4 + a]);
+            const float s = kahan(comp, 4);
+            if (s != 0.f)
+              for (int a = 0; a < 4; a++) comp[a] /= s;
+            else
+              for (int a = 0; a < 4; a++) comp[a] = 1.0f / 4.0f;
+            const float corr = null3(cm, comp, d);
+            hs -= corr;
+            bias = corr;
+            rep = hs > maxrep && hs >= cutoff;
+          }
+          if (rep) {
+            out.push_back({i, j, hs, bias, env});
+            maxrep = hs;
+          }
+        }
+      }
+      off += dmax - dmin + 1;
+    }
+  }
+}
+
+struct EnvPlan {
+  int64_t ncells;
+  int32_t nrows;
+  int64_t nroot;
+  std::vector<int32_t> roff;
+};
+
+// One or more envelopes over a device sequence buffer: uploads bands and
+// layouts, launches the scan in groups that fit the cell budget, and returns
+// every envelope's root deck on the host.
+int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const std::vector<int64_t> &seq_off,
+                  const ixg_envelope *env, int64_t n, int inside, std::vector<std::vector<float>> &roots,
+                  int64_t *cells_out) {
+  const int M = cm->h.M;
+  std::vector<EnvPlan> plan(n);
+  int64_t total_cells = 0;
+  for (int64_t k = 0; k < n; k++) {
+    RCHK(layout(cm->h, env[k].bands, env[k].i0, env[k].j0, plan[k].roff, plan[k].ncells, plan[k].nrows));
+    const ixg_bands &b = env[k].bands;
+    int64_t nr = 0;
+    for (int j = b.jmin[0]; j <= b.jmax[0]; j++) {
+      const int r = b.hd_off[0] + (j - b.jmin[0]);
+      if (b.hdmax[r] >= b.hdmin[r]) nr += b.hdmax[r] - b.hdmin[r] + 1;
+    }
+    plan[k].nroot = nr;
+    total_cells += plan[k].ncells;
+  }
+  if (cells_out) *cells_out = total_cells;
+  roots.assign(n, {});
+  const int64_t budget = (int64_t)1 << 28;  // cells per launch group (1 GiB of deck)
+  int64_t k0 = 0;
+  while (k0 < n) {
+    int64_t k1 = k0, cells = 0;
+    while (k1 < n && (k1 == k0 || cells + plan[k1].ncells <= budget)) cells += plan[k1++].ncells;
+    // pack band arrays: per envelope jmin|jmax (M each), hd_off (M+1), hdmin|hdmax|roff (nrows each)
+    std::vector<int32_t> pack;
+    std::vector<int64_t> at(k1 - k0);
+    for (int64_t k = k0; k < k1; k++) {
+      const ixg_bands &b = env[k].bands;
+      const int nr = plan[k].nrows;
+      at[k - k0] = (int64_t)pack.size();
+      pack.insert(pack.end(), b.jmin, b.jmin + M);
+      pack.insert(pack.end(), b.jmax, b.jmax + M);
+      pack.insert(pack.end(), b.hd_off, b.hd_off + M + 1);
+      pack.insert(pack.end(), b.hdmin, b.hdmin + nr);
+      pack.insert(pack.end(), b.hdmax, b.hdmax + nr);
+      pack.insert(pack.end(), plan[k].roff.begin(), plan[k].roff.begin() + nr);
+    }
+    int32_t *d_pack;
+    float *d_alpha;
+    EnvDev *d_env;
+    RCHK(scratch(c, S_CM_BANDS, pack.size() * 4, (void **)&d_pack));
+    RCHK(scratch(c, S_CM_ALPHA, (size_t)std::max<int64_t>(cells, 1) * 4, (void **)&d_alpha));
+    RCHK(scratch(c, S_CM_ENV, (size_t)(k1 - k0) * sizeof(EnvDev), (void **)&d_env));
+    std::vector<EnvDev> ed(k1 - k0);
+    int64_t aoff = 0;
+    for (int64_t k = k0; k < k1; k++) {
+      const int nr = plan[k].nrows;
+      const int32_t *base = d_pack + at[k - k0];
+      EnvDev &e = ed[k - k0];
+      e.dsq = d_seq_base + seq_off[k];
+      e.jmin = base;
+      e.jmax = base + M;
+      e.hd_off = base + 2 * M;
+      e.hdmin = base + 3 * M + 1;
+      e.hdmax = e.hdmin + nr;
+      e.roff = e.hdmax + nr;
+      e.alpha = d_alpha + aoff;
+      e.i0 = env[k].i0;
+      e.j0 = env[k].j0;
+      aoff += plan[k].ncells;
+    }
+    HIPCHK(hipMemcpyAsync(d_pack, pack.data(), pack.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d_env, ed.data(), ed.size() * sizeof(EnvDev), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipEventRecord(c->ev[5], c->stream));
+    HIPCHK(launch_hb_scan(cm->dev(), d_env, k1 - k0, cm->d_ftbl(), inside, c->stream));
+    HIPCHK(hipEventRecord(c->ev[6], c->stream));
+    for (int64_t k = k0; k < k1; k++) {
+      roots[k].resize(plan[k].nroot);
+      if (plan[k].nroot)
+        HIPCHK(hipMemcpyAsync(roots[k].data(), ed[k - k0].alpha, plan[k].nroot * 4, hipMemcpyDeviceToHost,
+                              c->stream));
+    }
+    HIPCHK(hipStreamSynchronize(c->stream));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[5], c->ev[6]));
+    c->ms[2] += ms;
+    k0 = k1;
+  }
+  return IXG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ixg_cm_create(ixg_ctx *c, const ixg_cm_params *p, ixg_cm **out) {
+  if (!p || !out || p->M < 1 || p->nodes < 2 || !p->sttype || !p->ndtype || !p->nodemap || !p->cfirst || !p->cnum ||
+      !p->t || !p->e) {
+    g_err = "ixg_cm_create: bad arguments";
+    return IXG_EINVAL;
+  }
+  for (int v = 0; v < p->M; v++) {  // children must follow their parent
+    const int st = p->sttype[v];
+    if (st < 0 || st > B_ST) {
+      g_err = "ixg_cm_create: bad state type";
+      return IXG_EINVAL;
+    }
+    if (st == B_ST) {
+      if (p->cfirst[v] <= v || p->cnum[v] <= v || p->cfirst[v] >= p->M || p->cnum[v] >= p->M) {
+        g_err = "ixg_cm_create: bad bifurcation children";
+        return IXG_EINVAL;
+      }
+    } else if (st != E_ST) {
+      const bool self = st == IL_ST || st == IR_ST;
+      if (p->cnum[v] < 1 || p->cnum[v] > MAXC || p->cfirst[v] < (self ? v : v + 1) ||
+          p->cfirst[v] + p->cnum[v] > p->M) {
+        g_err = "ixg_cm_create: bad children";
+        return IXG_EINVAL;
+      }
+    }
+  }
+  auto cm = std::make_unique<ixg_cm>();
+  RCHK(configure_cm(*p, cm->h));
+  flogsum_table(cm->ftbl_host);
+  if (!c) {
+    *out = cm.release();
+    return IXG_OK;
+  }
+  HIPCHK(hipSetDevice(c->device));
+  const int M = p->M;
+  std::vector<int32_t> i32(cm->h.cfirst);
+  i32.insert(i32.end(), cm->h.cnum.begin(), cm->h.cnum.end());
+  i32.insert(i32.end(), cm->h.begin_list.begin(), cm->h.begin_list.end());
+  std::vector<float> f(cm->h.tsc);
+  f.insert(f.end(), cm->h.oesc.begin(), cm->h.oesc.end());
+  f.insert(f.end(), cm->h.beginsc.begin(), cm->h.beginsc.end());
+  f.insert(f.end(), cm->h.endsc.begin(), cm->h.endsc.end());
+  f.insert(f.end(), cm->ftbl_host.begin(), cm->ftbl_host.end());
+  RCHK(dmalloc(c, (void **)&cm->d_st, M));
+  RCHK(dmalloc(c, (void **)&cm->d_i32, i32.size() * 4));
+  RCHK(dmalloc(c, (void **)&cm->d_f, f.size() * 4));
+  cm->bytes = M + i32.size() * 4 + f.size() * 4;
+  HIPCHK(hipMemcpy(cm->d_st, cm->h.sttype.data(), M, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(cm->d_i32, i32.data(), i32.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(cm->d_f, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+  *out = cm.release();
+  return IXG_OK;
+}
+
+int ixg_cm_free(ixg_ctx *c, ixg_cm *cm) {
+  if (!cm) return IXG_EINVAL;
+  if (cm->d_st) {
+    if (!c) return IXG_EINVAL;
+    const int M = cm->h.M;
+    dfree(c, cm->d_st, M);
+    dfree(c, cm->d_i32, (2 * M + cm->h.begin_list.size()) * 4);
+    dfree(c, cm->d_f, cm->bytes - M - (2 * M + cm->h.begin_list.size()) * 4);
+  }
+  delete cm;
+  return IXG_OK;
+}
+
+int ixg_cm_tables(const ixg_cm *cm, float *tsc, float *oesc, float *beginsc, float *endsc, float *el) {
+  if (!cm) return IXG_EINVAL;
+  const CmHost &h = cm->h;
+  if (tsc) memcpy(tsc, h.tsc.data(), h.tsc.size() * 4);
+  if (oesc) memcpy(oesc, h.oesc.data(), h.oesc.size() * 4);
+  if (beginsc) memcpy(beginsc, h.beginsc.data(), h.beginsc.size() * 4);
+  if (endsc) memcpy(endsc, h.endsc.data(), h.endsc.size() * 4);
+  if (el) *el = h.el_selfsc;
+  return IXG_OK;
+}
+
+static int single_env(ixg_ctx *c, const ixg_cm *cm, const ixg_bands *b, const uint8_t *dsq, int64_t L, int32_t i0,
+                      int32_t j0, int inside, std::vector<float> &root) {
+  if (!c || !cm || !b || !dsq || !cm->d_st || i0 < 1 || j0 > L || j0 < i0) {
+    g_err = "scan: bad arguments (1 <= i0 <= j0 <= L, device CM)";
+    return IXG_EINVAL;
+  }
+  for (int64_t i = i0; i <= j0; i++)
+    if (dsq[i] >= KP) {
+      g_err = "digitized residue code >= 18 in the envelope";
+      return IXG_EINVAL;
+    }
+  HIPCHK(hipSetDevice(c->device));
+  uint8_t *d;
+  RCHK(scratch(c, S_CM_SEQ, (size_t)L + 2, (void **)&d));
+  HIPCHK(hipMemcpyAsync(d, dsq, (size_t)L + 1, hipMemcpyHostToDevice, c->stream));
+  ixg_envelope e;
+  e.i0 = i0;
+  e.j0 = j0;
+  e.strand = 0;
+  e.pad = 0;
+  e.bands = *b;
+  std::vector<std::vector<float>> roots;
+  RCHK(run_envelopes(c, cm, d, {0}, &e, 1, inside, roots, nullptr));
+  root.swap(roots[0]);
+  return IXG_OK;
+}
+
+int ixg_cyk_scan_hb(ixg_ctx *c, const ixg_cm *cm, const ixg_bands *b, const uint8_t *dsq, int64_t L, int32_t i0,
+                    int32_t j0, float env_cutoff, float *vsc, int64_t *envi, int64_t *envj) {
+  if (!vsc || !envi || !envj) return IXG_EINVAL;
+  std::vector<float> root;
+  RCHK(single_env(c, cm, b, dsq, L, i0, j0, 0, root));
+  root_summary(*b, root.data(), i0, j0, env_cutoff, *vsc, *envi, *envj);
+  return IXG_OK;
+}
+
+int ixg_inside_scan_hb(ixg_ctx *c, const ixg_cm *cm, const ixg_bands *b, const uint8_t *dsq, int64_t L, int32_t i0,
+                       int32_t j0, int inside, float env_cutoff, float cutoff, int do_null3, float *vsc,
+                       int64_t *envi, int64_t *envj, ixg_hit *hits, int64_t cap, int64_t *nhits) {
+  if (!vsc || !envi || !envj || !nhits || (cap > 0 && !hits)) return IXG_EINVAL;
+  std::vector<float> root;
+  RCHK(single_env(c, cm, b, dsq, L, i0, j0, inside ? 1 : 0, root));
+  root_summary(*b, root.data(), i0, j0, env_cutoff, *vsc, *envi, *envj);
+  std::vector<ixg_hit> out;
+  report_hits(*cm, *b, dsq, i0, j0, root.data(), cutoff, do_null3, 0, out);
+  *nhits = (int64_t)out.size();
+  for (int64_t k = 0; k < std::min<int64_t>(cap, out.size()); k++) hits[k] = out[k];
+  return (int64_t)out.size() > cap ? IXG_ERANGE : IXG_OK;
+}
+
+}  // extern "C"
+
+// Host view of a device target (defined with the target in api.cpp).
+const uint8_t *ixg_target_device_base(const ixg_target *t, int64_t *L);
+const uint8_t *ixg_target_host_dsq(const ixg_target *t, int strand);
+
+extern "C" int ixg_hb_scan_batch(ixg_ctx *c, const ixg_cm *cm, const ixg_target *tgt, const ixg_envelope *env,
+                                 int64_t n, int inside, float env_cutoff, int report, float cutoff, int do_null3,
+                                 float *vsc, int64_t *envi, int64_t *envj, ixg_hit *hits, int64_t cap,
+                                 int64_t *nhits, int64_t *cells) {
+  if (!c || !cm || !cm->d_st || !tgt || (n && !env) || !vsc || !envi || !envj || (report && !nhits)) {
+    g_err = "ixg_hb_scan_batch: bad arguments";
+    return IXG_EINVAL;
+  }
+  HIPCHK(hipSetDevice(c->device));
+  int64_t L = 0;
+  const uint8_t *base = ixg_target_device_base(tgt, &L);
+  std::vector<int64_t> off(n);
+  for (int64_t k = 0; k < n; k++) {
+    if (env[k].i0 < 1 || env[k].j0 > L || env[k].j0 < env[k].i0 || (env[k].strand != 0 && env[k].strand != 1)) {
+      g_err = "ixg_hb_scan_batch: envelope outside the target";
+      return IXG_EINVAL;
+    }
+    off[k] = env[k].strand ? L + 2 : 0;
+  }
+  c->ms[2] = 0;
+  std::vector<std::vector<float>> roots;
+  RCHK(run_envelopes(c, cm, base, off, env, n, inside ? 1 : 0, roots, cells));
+  std::vector<ixg_hit> out;
+  for (int64_t k = 0; k < n; k++) {
+    root_summary(env[k].bands, roots[k].data(), env[k].i0, env[k].j0, env_cutoff, vsc[k], envi[k], envj[k]);
+    if (report)
+      report_hits(*cm, env[k].bands, ixg_target_host_dsq(tgt, env[k].strand), env[k].i0, env[k].j0,
+                  roots[k].data(), cutoff, do_null3, (int32_t)k, out);
+  }
+  if (report) {
+    *nhits = (int64_t)out.size();
+    for (int64_t k = 0; k < std::min<int64_t>(cap, out.size()); k++) hits[k] = out[k];
+    if ((int64_t)out.size() > cap) return IXG_ERANGE;
+  }
+  return IXG_OK;
+}

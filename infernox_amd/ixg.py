@@ -55,8 +55,29 @@ EXPORTS = [
     "ixg_p7_msv_tables", "ixg_p7_vit_tables", "ixg_p7_fwd_tables", "ixg_ssv_diagonals",
     "ixg_vit_scores", "ixg_fwd_scores", "ixg_bias_scores", "ixg_msv_scores",
     "ixg_f3_filter_sequence", "ixg_target_create", "ixg_target_free", "ixg_window_tasks",
-    "ixg_f3_filter_tasks", "ixg_last_timings",
+    "ixg_f3_filter_tasks", "ixg_last_timings", "ixg_cm_create", "ixg_cm_free", "ixg_cm_tables",
+    "ixg_cyk_scan_hb", "ixg_inside_scan_hb", "ixg_hb_scan_batch",
 ]
+
+
+class CmParams(C.Structure):
+    _fields_ = [("M", C.c_int32), ("nodes", C.c_int32), ("clen", C.c_int32), ("W", C.c_int32),
+                ("null", C.c_float * 4), ("el_selfsc", C.c_float), ("pbegin", C.c_float), ("pend", C.c_float),
+                ("n3_omega", C.c_double), ("sttype", C.POINTER(C.c_int8)), ("ndtype", C.POINTER(C.c_int8)),
+                ("nodemap", C.POINTER(C.c_int32)), ("cfirst", C.POINTER(C.c_int32)), ("cnum", C.POINTER(C.c_int32)),
+                ("t", C.POINTER(C.c_float)), ("e", C.POINTER(C.c_float))]
+
+
+class Bands(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_int32)) for n in ("jmin", "jmax", "hd_off", "hdmin", "hdmax")]
+
+
+class Hit(C.Structure):
+    _fields_ = [("i", C.c_int32), ("j", C.c_int32), ("score", C.c_float), ("bias", C.c_float), ("env", C.c_int32)]
+
+
+class Envelope(C.Structure):
+    _fields_ = [("i0", C.c_int32), ("j0", C.c_int32), ("strand", C.c_int32), ("pad", C.c_int32), ("bands", Bands)]
 
 _lib = None
 
@@ -94,6 +115,14 @@ def lib():
         L.ixg_f3_filter_tasks.argtypes = [vp, vp, vp, C.POINTER(Task), i64, C.POINTER(F3Opts), C.POINTER(Window),
                                           i64, i64p, C.POINTER(Acct)]
         L.ixg_last_timings.argtypes = [vp, C.POINTER(C.c_double), i64p]
+        L.ixg_cm_create.argtypes = [vp, C.POINTER(CmParams), C.POINTER(vp)]
+        L.ixg_cm_free.argtypes = [vp, vp]
+        L.ixg_cm_tables.argtypes = [vp, f32p, f32p, f32p, f32p, f32p]
+        L.ixg_cyk_scan_hb.argtypes = [vp, vp, C.POINTER(Bands), u8p, i64, i32, i32, C.c_float, f32p, i64p, i64p]
+        L.ixg_inside_scan_hb.argtypes = [vp, vp, C.POINTER(Bands), u8p, i64, i32, i32, C.c_int, C.c_float, C.c_float,
+                                         C.c_int, f32p, i64p, i64p, C.POINTER(Hit), i64, i64p]
+        L.ixg_hb_scan_batch.argtypes = [vp, vp, vp, C.POINTER(Envelope), i64, C.c_int, C.c_float, C.c_int,
+                                        C.c_float, C.c_int, f32p, i64p, i64p, C.POINTER(Hit), i64, i64p, i64p]
         _lib = L
     return _lib
 
@@ -299,3 +328,99 @@ def f3_filter_tasks(ctx, p7, tgt, tasks, ntasks, opts, cap=None):
         break
     wins = [(out[i].task, out[i].start, out[i].end, out[i].score) for i in range(n.value)]
     return wins, accts
+
+
+def cm_params(cm):
+    """ctypes CmParams view of a synth_cm CM dict (keeps its arrays alive)."""
+    keep = dict(sttype=np.ascontiguousarray(cm["sttype"], np.int8), ndtype=np.ascontiguousarray(cm["ndtype"], np.int8),
+                nodemap=np.ascontiguousarray(cm["nodemap"], np.int32), cfirst=np.ascontiguousarray(cm["cfirst"], np.int32),
+                cnum=np.ascontiguousarray(cm["cnum"], np.int32), t=np.ascontiguousarray(cm["t"], np.float32),
+                e=np.ascontiguousarray(cm["e"], np.float32))
+    s = CmParams()
+    s.M, s.nodes, s.clen, s.W = int(cm["M"]), int(cm["nodes"]), int(cm["clen"]), int(cm["W"])
+    for i in range(4):
+        s.null[i] = float(cm["null"][i])
+    s.el_selfsc, s.pbegin, s.pend, s.n3_omega = float(cm["el_selfsc"]), float(cm["pbegin"]), float(cm["pend"]), \
+        float(cm["n3_omega"])
+    s.sttype = keep["sttype"].ctypes.data_as(C.POINTER(C.c_int8))
+    s.ndtype = keep["ndtype"].ctypes.data_as(C.POINTER(C.c_int8))
+    for k in ("nodemap", "cfirst", "cnum"):
+        setattr(s, k, keep[k].ctypes.data_as(C.POINTER(C.c_int32)))
+    s.t = f32ptr(keep["t"])
+    s.e = f32ptr(keep["e"])
+    s._keep = keep
+    return s
+
+
+def bands_struct(b):
+    keep = {k: np.ascontiguousarray(b[k], np.int32) for k in ("jmin", "jmax", "hd_off", "hdmin", "hdmax")}
+    s = Bands()
+    for k, v in keep.items():
+        setattr(s, k, v.ctypes.data_as(C.POINTER(C.c_int32)))
+    s._keep = keep
+    return s
+
+
+class CM:
+    """Device-resident configured CM (ixg_cm_create); ctx=None builds host tables only."""
+
+    def __init__(self, ctx, cm):
+        self.ctx = ctx
+        self.M = int(cm["M"])
+        self.p = cm_params(cm)
+        self.h = C.c_void_p()
+        check(lib().ixg_cm_create(ctx.h if ctx else None, C.byref(self.p), C.byref(self.h)), "ixg_cm_create")
+
+    def free(self):
+        if self.h:
+            check(lib().ixg_cm_free(self.ctx.h if self.ctx else None, self.h), "ixg_cm_free")
+            self.h = C.c_void_p()
+
+    def tables(self):
+        M = self.M
+        tsc, oesc = np.zeros(M * 6, np.float32), np.zeros(M * 324, np.float32)
+        bs, es, el = np.zeros(M, np.float32), np.zeros(M, np.float32), np.zeros(1, np.float32)
+        check(lib().ixg_cm_tables(self.h, f32ptr(tsc), f32ptr(oesc), f32ptr(bs), f32ptr(es), f32ptr(el)))
+        return tsc.reshape(M, 6), oesc.reshape(M, 324), bs, es, float(el[0])
+
+    def cyk_scan(self, bands, dsq, L, i0, j0, env_cutoff=0.0):
+        d = np.ascontiguousarray(dsq, np.uint8)
+        vsc, ei, ej = C.c_float(), C.c_int64(), C.c_int64()
+        b = bands_struct(bands)
+        check(lib().ixg_cyk_scan_hb(self.ctx.h, self.h, C.byref(b), u8ptr(d), int(L), int(i0), int(j0),
+                                    float(env_cutoff), C.byref(vsc), C.byref(ei), C.byref(ej)), "ixg_cyk_scan_hb")
+        return vsc.value, ei.value, ej.value
+
+    def scan_hits(self, bands, dsq, L, i0, j0, inside=True, env_cutoff=0.0, cutoff=0.0, null3=True, cap=4096):
+        d = np.ascontiguousarray(dsq, np.uint8)
+        vsc, ei, ej, n = C.c_float(), C.c_int64(), C.c_int64(), C.c_int64()
+        b = bands_struct(bands)
+        hits = (Hit * cap)()
+        check(lib().ixg_inside_scan_hb(self.ctx.h, self.h, C.byref(b), u8ptr(d), int(L), int(i0), int(j0), int(inside),
+                                       float(env_cutoff), float(cutoff), int(null3), C.byref(vsc), C.byref(ei),
+                                       C.byref(ej), hits, cap, C.byref(n)), "ixg_inside_scan_hb")
+        return vsc.value, ei.value, ej.value, [(hits[k].i, hits[k].j, hits[k].score, hits[k].bias)
+                                               for k in range(n.value)]
+
+
+def hb_scan_batch(ctx, cm, tgt, envs, inside=False, env_cutoff=0.0, report=False, cutoff=0.0, null3=True):
+    """envs: list of (i0, j0, strand, bands dict). Returns (vsc, envi, envj, hits, cells)."""
+    n = len(envs)
+    arr = (Envelope * max(n, 1))()
+    keep = []
+    for k, (i0, j0, strand, b) in enumerate(envs):
+        bs = bands_struct(b)
+        keep.append(bs)
+        arr[k].i0, arr[k].j0, arr[k].strand, arr[k].pad = int(i0), int(j0), int(strand), 0
+        arr[k].bands = bs
+    vsc = np.zeros(max(n, 1), np.float32)
+    ei = np.zeros(max(n, 1), np.int64)
+    ej = np.zeros(max(n, 1), np.int64)
+    cap = 1 << 16
+    hits = (Hit * cap)()
+    nh, cells = C.c_int64(), C.c_int64()
+    check(lib().ixg_hb_scan_batch(ctx.h, cm.h, tgt.h, arr, n, int(inside), float(env_cutoff), int(report),
+                                  float(cutoff), int(null3), f32ptr(vsc), i64ptr(ei), i64ptr(ej), hits, cap,
+                                  C.byref(nh), C.byref(cells)), "ixg_hb_scan_batch")
+    hl = [(hits[k].env, hits[k].i, hits[k].j, hits[k].score, hits[k].bias) for k in range(nh.value)] if report else []
+    return vsc[:n], ei[:n], ej[:n], hl, cells.value

@@ -98,3 +98,18 @@ def test_bad_arguments_are_rejected():
     s.M = 0
     assert L.ixg_p7_create(None, C.byref(s), 10, C.byref(h)) == ixg.IXG_EINVAL
     assert L.ixg_p7_free(None, None) == ixg.IXG_EINVAL
+
+
+@pytest.mark.parametrize("ss,seed", [("<<..>>.", 1), ("<.>.<.>", 2), (None, 3)])
+def test_cm_host_tables_equal_oracle(ss, seed):
+    from infernox_amd import synth_cm as S
+    from oracle_cm_bind import OracleCM
+    cm = S.build_cm(ss if ss else S.cloverleaf(), seed)
+    g = ixg.CM(None, cm)
+    tsc, oesc, bs, es, el = g.tables()
+    o = OracleCM(cm)
+    assert tsc.tobytes() == o.tsc.tobytes()
+    assert oesc.tobytes() == o.oesc.tobytes()
+    assert bs.tobytes() == o.beginsc.tobytes() and es.tobytes() == o.endsc.tobytes()
+    assert np.float32(el).tobytes() == np.float32(o.el_selfsc).tobytes()
+    g.free()
