@@ -19,7 +19,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from infernox_amd import ixg, synth  # noqa: E402
+from infernox_amd import ixg, synth, synth_cm  # noqa: E402
 
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # 256 CUs x 4 SIMD16 x 2.4 GHz, one lane-op per cycle
 
@@ -31,6 +31,49 @@ def workload():
     d = synth.markov2_dsq(L, 3, gc=0.508)
     planted = synth.plant(d, L, p, 100, 3)
     return p, d, L, M, maxw, planted
+
+
+def cm_workload(n_env=512, r=8, seed=3):
+    """HB scanner workload of the genome config: the 300-column, 12-stem CM
+    (about 960 states, W ~ 400), n_env envelopes each holding one emitted
+    homolog with 20 flanking residues per side, CP9-shaped bands of
+    half-width r (+0..2). Returns (cm, dsq, L, envelopes)."""
+    cm = synth_cm.build_cm(synth_cm.multibranch(300, 12, seed), seed)
+    rng = np.random.default_rng(seed + 1)
+    parts, envs, pos = [], [], 1
+    for k in range(n_env):
+        h = synth_cm.emit(cm, 1000 + k)
+        left, right = rng.integers(0, 4, 20).astype(np.uint8), rng.integers(0, 4, 20).astype(np.uint8)
+        parts += [left, h, right]
+        i0, j0 = pos, pos + 40 + len(h) - 1
+        envs.append((i0, j0, 0, synth_cm.bands(cm, i0, j0, r=r, seed=k)))
+        pos = j0 + 1
+    seq = np.concatenate(parts)
+    L = len(seq)
+    dsq = np.concatenate([[255], seq, [255]]).astype(np.uint8)
+    return cm, dsq, L, envs
+
+
+def time_cm(ctx, steps, warmup):
+    cm, dsq, L, envs = cm_workload()
+    g = ixg.CM(ctx, cm)
+    tgt = ixg.Target(ctx, dsq, L)
+    out = {}
+    for name, inside in (("cyk", False), ("inside", True)):
+        for _ in range(warmup):
+            ixg.hb_scan_batch(ctx, g, tgt, envs, inside=inside, report=inside)
+        kms, t0 = 0.0, time.perf_counter()
+        for _ in range(steps):
+            vsc, ei, ej, hits, cells = ixg.hb_scan_batch(ctx, g, tgt, envs, inside=inside, report=inside)
+            kms += ctx.timings()[0][2]
+        wall = (time.perf_counter() - t0) / steps
+        kms /= steps
+        out[name] = {"gcups": cells / (kms * 1e-3) / 1e9, "gcups_wall": cells / wall / 1e9, "kernel_ms": kms,
+                     "wall_ms": wall * 1e3, "cells": int(cells), "envelopes": len(envs),
+                     "states": int(cm["M"]), "positive": int((vsc > 0).sum())}
+    tgt.free()
+    g.free()
+    return out
 
 
 def cpu_baseline(p, d, L, M, maxw, tasks, nt, budget_s, threads):
@@ -74,6 +117,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cm", action="store_true")
     a = ap.parse_args()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (the path shards by "
@@ -101,6 +145,7 @@ def main():
         n_f3b = sum(accts[t].n_past_fwdbias for t in range(nt))
         tgt.free()
         g.free()
+        cm_line = time_cm(ctx, max(1, a.steps // 2), 1) if not a.no_cm else None
     # dominant device stage and its VALU roofline (ops per cell from the reference loop bodies)
     f1_ops = 4 * cells[0]                 # SSV: 4 u8 ops per (residue, k), cm_pipeline.rs:598-601
     fwd_ops = 16 * cells[1]               # Forward: ~16 FP32 ops per (i, k), cm_pipeline.rs:1324-1382
@@ -135,6 +180,8 @@ def main():
         "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": VALU_PEAK_TOPS,
                      "unit": "TOPS", "frac": achieved / VALU_PEAK_TOPS, "traffic": None},
     }
+    if cm_line:
+        line["hb_scanners"] = cm_line
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(p, d, L, M, maxw, tasks, nt, a.cpu_seconds, a.cpu_threads)
     print(json.dumps(line))

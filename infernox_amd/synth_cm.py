@@ -267,55 +267,45 @@ def state_right_delta(st):
 def ij_bands(cm, i0, j0, r, seed, jitter=2):
     """CP9-shaped i/j bands of every state for the envelope [i0, j0]: the
     node's consensus span [a, b] mapped linearly onto the envelope, widened by
-    r (+- a seeded jitter). Returns imin, imax, jmin, jmax (int32, per state)."""
+    r (+ a seeded jitter of 0..jitter). Returns imin, imax, jmin, jmax (int32)."""
     rng = np.random.default_rng(seed)
     M = cm["M"]
-    clen = cm["clen"]
-    scale = (j0 - i0 + 1) / clen
-    imin = np.zeros(M, np.int32)
-    imax = np.zeros(M, np.int32)
-    jmin = np.zeros(M, np.int32)
-    jmax = np.zeros(M, np.int32)
-    for v in range(M):
-        a, b = cm["node_lr"][cm["ndidx"][v]]
-        ic = i0 + int(round((a - 1) * scale))
-        jc = i0 + int(round(b * scale)) - 1
-        ri = r + int(rng.integers(0, jitter + 1))
-        rj = r + int(rng.integers(0, jitter + 1))
-        imin[v] = max(i0, ic - ri)
-        imax[v] = min(j0 + 1, ic + ri)
-        jmin[v] = max(i0 - 1, jc - rj)
-        jmax[v] = min(j0, jc + rj)
+    scale = (j0 - i0 + 1) / cm["clen"]
+    lr = cm["node_lr"][cm["ndidx"]]
+    ic = i0 + np.round((lr[:, 0] - 1) * scale).astype(np.int64)
+    jc = i0 + np.round(lr[:, 1] * scale).astype(np.int64) - 1
+    ri = r + rng.integers(0, jitter + 1, size=M)
+    rj = r + rng.integers(0, jitter + 1, size=M)
+    imin = np.maximum(i0, ic - ri).astype(np.int32)
+    imax = np.minimum(j0 + 1, ic + ri).astype(np.int32)
+    jmin = np.maximum(i0 - 1, jc - rj).astype(np.int32)
+    jmax = np.minimum(j0, jc + rj).astype(np.int32)
     return imin, imax, jmin, jmax
 
 
 def ij2d(cm, imin, imax, jmin, jmax, do_trunc=False):
     """cp9.rs:3815-3850 ij2d_bands: per (v, j) d ranges, flattened."""
     M = cm["M"]
+    st = np.asarray(cm["sttype"], np.int64)
+    n = np.maximum(0, np.asarray(jmax, np.int64) - np.asarray(jmin, np.int64) + 1)
     hd_off = np.zeros(M + 1, np.int32)
-    hmn, hmx = [], []
-    for v in range(M):
-        hd_off[v] = len(hmn)
-        st = int(cm["sttype"][v])
-        n = max(0, jmax[v] - jmin[v] + 1)
-        for jp in range(n):
-            if st == E_ST:
-                hmn.append(0)
-                hmx.append(0)
-                continue
-            sd = state_delta(st)
-            dn = max(state_left_delta(st), state_right_delta(st)) if do_trunc else sd
-            j = jp + jmin[v]
-            hdn, hdx = j - imax[v] + 1, j - imin[v] + 1
-            if hdx < dn:
-                hmn.append(-1)
-                hmx.append(-2)
-            else:
-                hmn.append(max(hdn, dn))
-                hmx.append(hdx)
-    hd_off[M] = len(hmn)
+    hd_off[1:] = np.cumsum(n)
+    v = np.repeat(np.arange(M), n)
+    jp = np.arange(int(n.sum())) - np.repeat(hd_off[:-1].astype(np.int64), n)
+    j = jp + np.asarray(jmin, np.int64)[v]
+    sd = np.where(st == MP_ST, 2, np.where((st >= ML_ST) & (st <= IR_ST), 1, 0))
+    sl = np.isin(st, (MP_ST, ML_ST, IL_ST)).astype(np.int64)
+    sr = np.isin(st, (MP_ST, MR_ST, IR_ST)).astype(np.int64)
+    dn = (np.maximum(sl, sr) if do_trunc else sd)[v]
+    hdn = j - np.asarray(imax, np.int64)[v] + 1
+    hdx = j - np.asarray(imin, np.int64)[v] + 1
+    hmn = np.where(hdx < dn, -1, np.maximum(hdn, dn))
+    hmx = np.where(hdx < dn, -2, hdx)
+    e = st[v] == E_ST
+    hmn = np.where(e, 0, hmn)
+    hmx = np.where(e, 0, hmx)
     return dict(jmin=np.asarray(jmin, np.int32), jmax=np.asarray(jmax, np.int32), hd_off=hd_off,
-                hdmin=np.array(hmn, np.int32), hdmax=np.array(hmx, np.int32))
+                hdmin=hmn.astype(np.int32), hdmax=hmx.astype(np.int32))
 
 
 def bands(cm, i0, j0, r=8, seed=0, do_trunc=False):
