@@ -215,6 +215,62 @@ int ixg_hb_scan_batch(ixg_ctx *ctx, const ixg_cm *cm, const ixg_target *tgt, con
                       int inside, float env_cutoff, int report, float cutoff, int do_null3, float *vsc,
                       int64_t *envi, int64_t *envj, ixg_hit *hits, int64_t cap, int64_t *nhits, int64_t *cells);
 
+/* ---- truncated HMM-banded scanners: TrCYK / FTrInside (F6/F7, truncated passes) -- */
+/* Pipeline pass indices (cm_trunc.rs:49-55) and marginal modes (:34-38). */
+enum { IXG_PASS_STD_ANY = 1, IXG_PASS_5P_ONLY_FORCE = 2, IXG_PASS_3P_ONLY_FORCE = 3,
+       IXG_PASS_5P_AND_3P_FORCE = 4, IXG_PASS_5P_AND_3P_ANY = 5 };
+enum { IXG_TRMODE_T = 0, IXG_TRMODE_R = 1, IXG_TRMODE_L = 2, IXG_TRMODE_J = 3, IXG_TRMODE_UNKNOWN = 4 };
+
+/* Truncated-begin penalties of the CM (TrPenalties, cm_trunc.rs:138-150): g
+ * (global) and l (local) are [3][M] log2 scores, first index 0 = 5'&3',
+ * 1 = 5' only, 2 = 3' only; <= -1e36 where a truncated entry is not allowed.
+ * Must be set once before the truncated scanners run. The marginal emission
+ * scores (cp9.rs:4441 fill_marginal_emissions) are built by ixg_cm_create. */
+int ixg_cm_set_trpenalties(ixg_ctx *ctx, ixg_cm *cm, const float *g, const float *l);
+/* Host copies: lmesc/rmesc M*18; lpos/rpos per state: the consensus span of
+ * the state's subtree (the emit map, cp9.rs:144-204, with the per-node-type
+ * offsets of cm_dpsearch_trunc.rs:2394-2403). Any may be NULL. */
+int ixg_cm_trtables(const ixg_cm *cm, float *lmesc, float *rmesc, int32_t *lpos, int32_t *rpos);
+
+/* Marginal-mode validity of one envelope's states (CP9Bands jvalid, lvalid,
+ * rvalid, tvalid, cp9.rs:1488-1491): M bytes each, nonzero = valid. */
+typedef struct {
+  const uint8_t *jvalid, *lvalid, *rvalid, *tvalid;
+} ixg_trvalid;
+/* A truncated hit (cm_dpsearch_trunc.rs:110-118 TrHit) + its envelope. */
+typedef struct { int32_t i, j; float score, bias; int32_t mode, root, env; } ixg_trhit;
+
+/* cm_dpsearch_trunc.rs:264 tr_cyk_scan_hb(cm, trp, cp9b, dsq, i0, j0, cutoff,
+ * do_null3, pass_idx, env_cutoff, local) -> (hits, vsc, vmode, envi, envj):
+ * the four marginal planes J/L/R/T over the HMM bands, truncated begins into
+ * the root, greedy reporting with allow_truncation and null3, and greedy
+ * overlap removal. pass_idx is one of IXG_PASS_5P_ONLY_FORCE ..
+ * IXG_PASS_5P_AND_3P_ANY. d bands may start at max(left, right) emission
+ * count (cp9.rs:3832-3836). dsq is 1-indexed with L residues. Hits beyond
+ * cap are counted in *nhits and IXG_ERANGE is returned. */
+int ixg_trcyk_scan_hb(ixg_ctx *ctx, const ixg_cm *cm, const ixg_bands *b, const ixg_trvalid *valid,
+                      const uint8_t *dsq, int64_t L, int32_t i0, int32_t j0, float cutoff, int do_null3,
+                      int pass_idx, float env_cutoff, int local, float *vsc, int32_t *vmode, int64_t *envi,
+                      int64_t *envj, ixg_trhit *hits, int64_t cap, int64_t *nhits);
+/* cm_dpsearch_trunc.rs:1055 ftr_inside_scan_hb: the same with FLogsum child
+ * and split combines. */
+int ixg_trinside_scan_hb(ixg_ctx *ctx, const ixg_cm *cm, const ixg_bands *b, const ixg_trvalid *valid,
+                         const uint8_t *dsq, int64_t L, int32_t i0, int32_t j0, float cutoff, int do_null3,
+                         int pass_idx, float env_cutoff, int local, float *vsc, int32_t *vmode, int64_t *envi,
+                         int64_t *envj, ixg_trhit *hits, int64_t cap, int64_t *nhits);
+/* Batched form over a device-resident target (as ixg_hb_scan_batch):
+ * inside selects FTrInside; per-envelope vsc/vmode/envi/envj; hits of every
+ * envelope appended in envelope order (env = k). */
+typedef struct {
+  int32_t i0, j0, strand, pad;
+  ixg_bands bands;
+  ixg_trvalid valid;
+} ixg_trenvelope;
+int ixg_trhb_scan_batch(ixg_ctx *ctx, const ixg_cm *cm, const ixg_target *tgt, const ixg_trenvelope *env,
+                        int64_t n, int inside, int pass_idx, int local, float env_cutoff, float cutoff,
+                        int do_null3, float *vsc, int32_t *vmode, int64_t *envi, int64_t *envj, ixg_trhit *hits,
+                        int64_t cap, int64_t *nhits, int64_t *cells);
+
 /* Per-stage device timings (milliseconds, HIP events) of the most recent
  * ixg_f3_filter_tasks call on ctx: [0] F1 sweep, [1] F1 resolve, [2] F2/F3/F3b
  * window kernels, [3] host glue, [4] total. Also the launch counts. */

@@ -15,8 +15,11 @@ struct ixg_cm {
   std::vector<float> ftbl_host;
   int8_t *d_st = nullptr;
   int32_t *d_i32 = nullptr;  // cfirst | cnum | begin_list
-  float *d_f = nullptr;      // tsc | oesc | beginsc | endsc | flogsum table
-  size_t bytes = 0;
+  float *d_f = nullptr;      // tsc | oesc | beginsc | endsc | flogsum table | lmesc | rmesc
+  float *d_tr = nullptr;     // truncated-begin penalties: g[3][M] | l[3][M]
+  int32_t *d_yl = nullptr;   // per (local, pass index): states with an allowed truncated begin
+  int32_t yoff[7] = {0, 0, 0, 0, 0, 0, 0};
+  size_t b_i32 = 0, b_f = 0, b_tr = 0, b_yl = 0;
   CmDev dev() const {
     CmDev c;
     const int M = h.M;
@@ -29,6 +32,8 @@ struct ixg_cm {
     c.oesc = d_f + (size_t)M * MAXC;
     c.beginsc = c.oesc + (size_t)M * KP * KP;
     c.endsc = c.beginsc + M;
+    c.lmesc = d_ftbl() + FLOGSUM_TBL;
+    c.rmesc = c.lmesc + (size_t)M * KP;
     c.M = M;
     c.local_begin = h.local_begin;
     c.el_selfsc = h.el_selfsc;
@@ -44,7 +49,17 @@ int sdelta_host(int st) { return st == MP_ST ? 2 : ((st >= ML_ST && st <= IR_ST)
 // Deck layout of one envelope: roff per (v, jp) and the cell count, after
 // checking that every cell stays inside [i0, j0] and that the d bands respect
 // the states' emission counts (as CP9-derived bands do, cp9.rs:3815-3850).
-int layout(const CmHost &h, const ixg_bands &b, int32_t i0, int32_t j0, std::vector<int32_t> &roff,
+int sldelta_host(int st) { return (st == MP_ST || st == ML_ST || st == IL_ST) ? 1 : 0; }
+int srdelta_host(int st) { return (st == MP_ST || st == MR_ST || st == IR_ST) ? 1 : 0; }
+
+// Smallest d a band of state type st may start at: the emission count, or for
+// a truncated scan the larger of the left and right counts.
+int dmin_host(int st, bool trunc) {
+  if (st == E_ST) return 0;
+  return trunc ? std::max(sldelta_host(st), srdelta_host(st)) : sdelta_host(st);
+}
+
+int layout(const CmHost &h, const ixg_bands &b, int32_t i0, int32_t j0, bool trunc, std::vector<int32_t> &roff,
            int64_t &ncells, int32_t &nrows) {
   const int M = h.M;
   if (!b.jmin || !b.jmax || !b.hd_off || !b.hdmin || !b.hdmax || i0 < 1 || j0 < i0 - 1) {
@@ -70,7 +85,7 @@ int layout(const CmHost &h, const ixg_bands &b, int32_t i0, int32_t j0, std::vec
       roff[r] = (int32_t)nc;
       const int lo = b.hdmin[r], hi = b.hdmax[r];
       if (hi >= lo) {
-        if (lo < (st == E_ST ? 0 : sdelta_host(st)) || j - hi + 1 < i0 || (st == E_ST && (lo != 0 || hi != 0))) {
+        if (lo < dmin_host(st, trunc) || j - hi + 1 < i0 || (st == E_ST && (lo != 0 || hi != 0))) {
           g_err = "bands: d band outside the envelope or below the state's emission count";
           return IXG_EINVAL;
         }
@@ -81,7 +96,7 @@ int layout(const CmHost &h, const ixg_bands &b, int32_t i0, int32_t j0, std::vec
       }
     }
   }
-  if (nc >= INT32_MAX) {
+  if (nc >= (trunc ? INT32_MAX / 4 : INT32_MAX)) {
     g_err = "bands: deck larger than 2^31 cells";
     return IXG_EINVAL;
   }
@@ -193,17 +208,38 @@ struct EnvPlan {
   std::vector<int32_t> roff;
 };
 
+// A truncated scan's pass: the mode bits it fills and its device penalties
+// and entry-state list.
+struct TrSel {
+  int fill;  // BL | BR | BT bits enabled by the pass
+  const float *d_pty;
+  const int32_t *d_yl;
+  int ny;
+};
+
+int mode_mask(const ixg_trvalid &val, int v, int fill) {
+  return (val.jvalid[v] ? 1 : 0) | ((val.lvalid[v] && (fill & 2)) ? 2 : 0) | ((val.rvalid[v] && (fill & 4)) ? 4 : 0) |
+         ((val.tvalid[v] && (fill & 8)) ? 8 : 0);
+}
+
 // One or more envelopes over a device sequence buffer: uploads bands and
-// layouts, launches the scan in groups that fit the cell budget, and returns
-// every envelope's root deck on the host.
+// layouts, launches the scan in groups that fit the deck budget, and returns
+// every envelope's root rows on the host (standard: nroot floats; truncated:
+// the J, L, R, T root planes then the best entry per root cell, nroot each).
 int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const std::vector<int64_t> &seq_off,
-                  const ixg_envelope *env, int64_t n, int inside, std::vector<std::vector<float>> &roots,
-                  int64_t *cells_out) {
+                  const ixg_envelope *env, int64_t n, int inside, const TrSel *tr, const ixg_trvalid *valid,
+                  std::vector<std::vector<float>> &roots, int64_t *cells_out) {
   const int M = cm->h.M;
+  const bool trunc = tr != nullptr;
+  const int planes = trunc ? 4 : 1, rper = trunc ? 5 : 1;
   std::vector<EnvPlan> plan(n);
   int64_t total_cells = 0;
   for (int64_t k = 0; k < n; k++) {
-    RCHK(layout(cm->h, env[k].bands, env[k].i0, env[k].j0, plan[k].roff, plan[k].ncells, plan[k].nrows));
+    RCHK(layout(cm->h, env[k].bands, env[k].i0, env[k].j0, trunc, plan[k].roff, plan[k].ncells, plan[k].nrows));
+    if (trunc && (!valid[k].jvalid || !valid[k].lvalid || !valid[k].rvalid || !valid[k].tvalid)) {
+      g_err = "truncated scan: null validity array";
+      return IXG_EINVAL;
+    }
     const ixg_bands &b = env[k].bands;
     int64_t nr = 0;
     for (int j = b.jmin[0]; j <= b.jmax[0]; j++) {
@@ -215,14 +251,15 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
   }
   if (cells_out) *cells_out = total_cells;
   roots.assign(n, {});
-  const int64_t budget = (int64_t)1 << 28;  // cells per launch group (1 GiB of deck)
+  const int64_t budget = (int64_t)1 << 28;  // deck floats per launch group (1 GiB)
   int64_t k0 = 0;
   while (k0 < n) {
-    int64_t k1 = k0, cells = 0;
-    while (k1 < n && (k1 == k0 || cells + plan[k1].ncells <= budget)) cells += plan[k1++].ncells;
-    // pack band arrays: per envelope jmin|jmax (M each), hd_off (M+1), hdmin|hdmax|roff (nrows each)
+    int64_t k1 = k0, floats = 0;
+    while (k1 < n && (k1 == k0 || floats + planes * plan[k1].ncells <= budget)) floats += planes * plan[k1++].ncells;
+    // pack per envelope: jmin|jmax (M each), hd_off (M+1), hdmin|hdmax|roff (nrows each), [mode mask (M)]
     std::vector<int32_t> pack;
-    std::vector<int64_t> at(k1 - k0);
+    std::vector<int64_t> at(k1 - k0), rat(k1 - k0);
+    int64_t rtotal = 0;
     for (int64_t k = k0; k < k1; k++) {
       const ixg_bands &b = env[k].bands;
       const int nr = plan[k].nrows;
@@ -233,13 +270,18 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
       pack.insert(pack.end(), b.hdmin, b.hdmin + nr);
       pack.insert(pack.end(), b.hdmax, b.hdmax + nr);
       pack.insert(pack.end(), plan[k].roff.begin(), plan[k].roff.begin() + nr);
+      if (trunc)
+        for (int v = 0; v < M; v++) pack.push_back(mode_mask(valid[k], v, tr->fill));
+      rat[k - k0] = rtotal;
+      rtotal += rper * plan[k].nroot;
     }
     int32_t *d_pack;
-    float *d_alpha;
+    float *d_alpha, *d_rout;
     EnvDev *d_env;
     RCHK(scratch(c, S_CM_BANDS, pack.size() * 4, (void **)&d_pack));
-    RCHK(scratch(c, S_CM_ALPHA, (size_t)std::max<int64_t>(cells, 1) * 4, (void **)&d_alpha));
+    RCHK(scratch(c, S_CM_ALPHA, (size_t)std::max<int64_t>(floats, 1) * 4, (void **)&d_alpha));
     RCHK(scratch(c, S_CM_ENV, (size_t)(k1 - k0) * sizeof(EnvDev), (void **)&d_env));
+    RCHK(scratch(c, S_CM_ROOT, (size_t)std::max<int64_t>(rtotal, 1) * 4, (void **)&d_rout));
     std::vector<EnvDev> ed(k1 - k0);
     int64_t aoff = 0;
     for (int64_t k = k0; k < k1; k++) {
@@ -253,29 +295,189 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
       e.hdmin = base + 3 * M + 1;
       e.hdmax = e.hdmin + nr;
       e.roff = e.hdmax + nr;
+      e.mask = trunc ? e.roff + nr : nullptr;
       e.alpha = d_alpha + aoff;
+      e.nc = (int32_t)plan[k].ncells;
+      e.rout = d_rout + rat[k - k0];
       e.i0 = env[k].i0;
       e.j0 = env[k].j0;
-      aoff += plan[k].ncells;
+      aoff += planes * plan[k].ncells;
     }
     HIPCHK(hipMemcpyAsync(d_pack, pack.data(), pack.size() * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(d_env, ed.data(), ed.size() * sizeof(EnvDev), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
-    HIPCHK(launch_hb_scan(cm->dev(), d_env, k1 - k0, cm->d_ftbl(), inside, c->stream));
-    HIPCHK(hipEventRecord(c->ev[6], c->stream));
-    for (int64_t k = k0; k < k1; k++) {
-      roots[k].resize(plan[k].nroot);
-      if (plan[k].nroot)
-        HIPCHK(hipMemcpyAsync(roots[k].data(), ed[k - k0].alpha, plan[k].nroot * 4, hipMemcpyDeviceToHost,
+    if (trunc)
+      HIPCHK(launch_trhb_scan(cm->dev(), d_env, k1 - k0, cm->d_ftbl(), tr->d_pty, tr->d_yl, tr->ny, inside,
                               c->stream));
-    }
+    else
+      HIPCHK(launch_hb_scan(cm->dev(), d_env, k1 - k0, cm->d_ftbl(), inside, c->stream));
+    HIPCHK(hipEventRecord(c->ev[6], c->stream));
+    std::vector<float> rhost(std::max<int64_t>(rtotal, 1));
+    if (rtotal) HIPCHK(hipMemcpyAsync(rhost.data(), d_rout, rtotal * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    for (int64_t k = k0; k < k1; k++)
+      roots[k].assign(rhost.begin() + rat[k - k0], rhost.begin() + rat[k - k0] + rper * plan[k].nroot);
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, c->ev[5], c->ev[6]));
     c->ms[2] += ms;
     k0 = k1;
   }
   return IXG_OK;
+}
+
+// ---- truncated scans: pass selection, root summary, reporting ----
+
+int pty_index(int pass_idx) {  // cm_trunc.rs:99-107
+  switch (pass_idx) {
+    case IXG_PASS_5P_ONLY_FORCE: return 1;
+    case IXG_PASS_3P_ONLY_FORCE: return 2;
+    case IXG_PASS_5P_AND_3P_FORCE:
+    case IXG_PASS_5P_AND_3P_ANY: return 0;
+    default: return -1;
+  }
+}
+int fill_bits(int pass_idx) {  // cm_dpsearch_trunc.rs:184-199
+  if (pass_idx == IXG_PASS_5P_AND_3P_FORCE || pass_idx == IXG_PASS_5P_AND_3P_ANY) return 2 | 4 | 8;
+  if (pass_idx == IXG_PASS_5P_ONLY_FORCE) return 4;
+  if (pass_idx == IXG_PASS_3P_ONLY_FORCE) return 2;
+  return 0;
+}
+
+int tr_select(const ixg_cm *cm, int pass_idx, int local, TrSel &sel) {
+  const int idx = pty_index(pass_idx);
+  if (idx < 0) {
+    g_err = "truncated scan: pass_idx must be a truncated pass (2..5)";
+    return IXG_EINVAL;
+  }
+  if (!cm->d_tr) {
+    g_err = "truncated scan: ixg_cm_set_trpenalties has not been called";
+    return IXG_EINVAL;
+  }
+  const int q = (local ? 3 : 0) + idx;
+  sel.fill = fill_bits(pass_idx);
+  sel.d_pty = cm->d_tr + (size_t)q * cm->h.M;
+  sel.d_yl = cm->d_yl + cm->yoff[q];
+  sel.ny = cm->yoff[q + 1] - cm->yoff[q];
+  return IXG_OK;
+}
+
+// vsc_root / vmode over the four root planes and the envelope bounds
+// (cm_dpsearch_trunc.rs:989-1022).
+void tr_root_summary(const ixg_bands &b, const float *rr, int64_t nroot, int m0, int32_t i0, int32_t j0,
+                     float env_cutoff, float &vsc, int32_t &vmode, int64_t &envi, int64_t &envj) {
+  static const int modes[4] = {IXG_TRMODE_J, IXG_TRMODE_L, IXG_TRMODE_R, IXG_TRMODE_T};
+  float vr = IMPOSSIBLE;
+  int32_t vm = IXG_TRMODE_UNKNOWN;
+  int64_t ei = (int64_t)j0 + 1, ej = (int64_t)i0 - 1;
+  int64_t off = 0;
+  for (int j = b.jmin[0]; j <= b.jmax[0]; j++) {
+    const int r = b.hd_off[0] + (j - b.jmin[0]);
+    for (int d = b.hdmin[r]; d <= b.hdmax[r]; d++) {
+      const int64_t c = off + (d - b.hdmin[r]);
+      bool in_env = false;
+      for (int p = 0; p < 4; p++) {
+        if (!(m0 & (1 << p))) continue;
+        const float s = rr[p * nroot + c];
+        if (s > vr) vr = s, vm = modes[p];
+        if (s >= env_cutoff) in_env = true;
+      }
+      if (in_env) {
+        ei = std::min<int64_t>(ei, j - d + 1);
+        ej = std::max<int64_t>(ej, j);
+      }
+    }
+    if (b.hdmax[r] >= b.hdmin[r]) off += b.hdmax[r] - b.hdmin[r] + 1;
+  }
+  vsc = vr;
+  vmode = vm;
+  envi = ei == (int64_t)j0 + 1 ? -1 : ei;
+  envj = ej == (int64_t)i0 - 1 ? -1 : ej;
+}
+
+// cm_dpsearch_trunc.rs:2376-2422 allow_truncation
+bool allow_truncation(const CmHost &h, int pass_idx, int local, int start, int stop, int i0, int j0, int mode, int b) {
+  if (pass_idx == IXG_PASS_STD_ANY || pass_idx == IXG_PASS_5P_AND_3P_ANY) return true;
+  if (start == i0 && stop == j0) return true;
+  const int lp = h.lpos[b], rp = h.rpos[b];
+  if (local) {
+    if (mode == IXG_TRMODE_J) return true;
+    if (mode == IXG_TRMODE_L) return stop == j0;
+    if (mode == IXG_TRMODE_R) return start == i0;
+    return false;
+  }
+  if (mode == IXG_TRMODE_J) return lp == 1 && rp == h.clen;
+  if (mode == IXG_TRMODE_L) return stop == j0 && lp == 1;
+  if (mode == IXG_TRMODE_R) return start == i0 && rp == h.clen;
+  return false;
+}
+
+// Greedy truncated reporting per j (cm_dpsearch_trunc.rs:2322-2372) over the
+// root planes, then greedy overlap removal (:2470-2495); appends to out.
+void tr_report(const ixg_cm &cm, const ixg_bands &b, const uint8_t *dsq, int32_t i0, int32_t j0, const float *rr,
+               int64_t nroot, int pass_idx, int local, float cutoff, int do_null3, int32_t env,
+               std::vector<ixg_trhit> &out) {
+  const int W = j0 - i0 + 1;
+  std::vector<double> act;
+  if (do_null3) {
+    act.assign((size_t)(W + 1) * 4, 0.0);
+    for (int jp = 1; jp <= W; jp++) {
+      for (int a = 0; a < 4; a++) act[jp * 4 + a] = act[(jp - 1) * 4 + a];
+      const int r = dsq[i0 + jp - 1];
+      if (r < 4) act[jp * 4 + r] += 1.0;
+    }
+  }
+  std::vector<ixg_trhit> tmp;
+  int64_t off = 0;
+  for (int j = b.jmin[0]; j <= b.jmax[0]; j++) {
+    const int r = b.hd_off[0] + (j - b.jmin[0]);
+    const int dmin = b.hdmin[r], dmax = b.hdmax[r];
+    if (dmin > dmax) continue;
+    float maxrep = IMPOSSIBLE;
+    for (int d = std::max(dmin, 1); d <= dmax; d++) {
+      const int64_t c = off + (d - dmin);
+      float hs = IMPOSSIBLE;
+      for (int p = 0; p < 4; p++) hs = std::max(hs, rr[p * nroot + c]);
+      int32_t code;
+      memcpy(&code, &rr[4 * nroot + c], 4);
+      const int mode = code & 7, root = code >> 3;
+      const int i = j - d + 1;
+      if (!(hs > maxrep && hs >= cutoff && hs > IMPOSSIBLE + 1.0f)) continue;
+      bool rep = allow_truncation(cm.h, pass_idx, local, i, j, i0, j0, mode, root);
+      float bias = 0.f;
+      if (rep && do_null3) {
+        float comp[4];
+        const int ip = i - i0 + 1, jj = j - i0 + 1;
+        for (int a = 0; a < 4; a++) comp[a] = (float)(act[jj * 4 + a] - act[(ip - 1) * 4 + a]);
+        float sum = 0.f;
+        for (int a = 0; a < 4; a++) sum += comp[a];
+        if (sum > 0.f)
+          for (int a = 0; a < 4; a++) comp[a] /= sum;
+        else
+          for (int a = 0; a < 4; a++) comp[a] = 1.0f / 4.0f;
+        const float corr = null3(cm, comp, d);
+        hs -= corr;
+        bias = corr;
+        rep = hs > maxrep && hs >= cutoff;
+      }
+      if (rep) {
+        tmp.push_back({i, j, hs, bias, mode, root, env});
+        maxrep = hs;
+      }
+    }
+    off += dmax - dmin + 1;
+  }
+  std::stable_sort(tmp.begin(), tmp.end(), [](const ixg_trhit &a, const ixg_trhit &b) {
+    if (a.score != b.score) return a.score > b.score;
+    return a.i < b.i;
+  });
+  std::vector<char> removed(tmp.size(), 0);
+  for (size_t a = 0; a < tmp.size(); a++) {
+    if (removed[a]) continue;
+    for (size_t k = a + 1; k < tmp.size(); k++)
+      if (!removed[k] && !(tmp[k].j < tmp[a].i) && !(tmp[a].j < tmp[k].i)) removed[k] = 1;
+  }
+  for (size_t a = 0; a < tmp.size(); a++)
+    if (!removed[a]) out.push_back(tmp[a]);
 }
 
 }  // namespace
@@ -287,6 +489,14 @@ int ixg_cm_create(ixg_ctx *c, const ixg_cm_params *p, ixg_cm **out) {
       !p->t || !p->e) {
     g_err = "ixg_cm_create: bad arguments";
     return IXG_EINVAL;
+  }
+  for (int nd = 0; nd < p->nodes; nd++) {  // nodes in order, each owning a run of states
+    if (p->ndtype[nd] < BIF_ND || p->ndtype[nd] > END_ND || p->nodemap[nd] < 0 || p->nodemap[nd] >= p->M ||
+        (nd == 0 && p->nodemap[nd] != 0) || (nd > 0 && p->nodemap[nd] <= p->nodemap[nd - 1]) ||
+        ((p->ndtype[nd] == BIF_ND) != (p->sttype[p->nodemap[nd]] == B_ST))) {
+      g_err = "ixg_cm_create: bad node types or node map";
+      return IXG_EINVAL;
+    }
   }
   for (int v = 0; v < p->M; v++) {  // children must follow their parent
     const int st = p->sttype[v];
@@ -310,6 +520,9 @@ int ixg_cm_create(ixg_ctx *c, const ixg_cm_params *p, ixg_cm **out) {
   }
   auto cm = std::make_unique<ixg_cm>();
   RCHK(configure_cm(*p, cm->h));
+  marginal_emissions(*p, cm->h);
+  RCHK(state_spans(*p, cm->h));
+  cm->h.clen = p->clen;
   flogsum_table(cm->ftbl_host);
   if (!c) {
     *out = cm.release();
@@ -325,10 +538,13 @@ int ixg_cm_create(ixg_ctx *c, const ixg_cm_params *p, ixg_cm **out) {
   f.insert(f.end(), cm->h.beginsc.begin(), cm->h.beginsc.end());
   f.insert(f.end(), cm->h.endsc.begin(), cm->h.endsc.end());
   f.insert(f.end(), cm->ftbl_host.begin(), cm->ftbl_host.end());
+  f.insert(f.end(), cm->h.lmesc.begin(), cm->h.lmesc.end());
+  f.insert(f.end(), cm->h.rmesc.begin(), cm->h.rmesc.end());
   RCHK(dmalloc(c, (void **)&cm->d_st, M));
   RCHK(dmalloc(c, (void **)&cm->d_i32, i32.size() * 4));
   RCHK(dmalloc(c, (void **)&cm->d_f, f.size() * 4));
-  cm->bytes = M + i32.size() * 4 + f.size() * 4;
+  cm->b_i32 = i32.size() * 4;
+  cm->b_f = f.size() * 4;
   HIPCHK(hipMemcpy(cm->d_st, cm->h.sttype.data(), M, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(cm->d_i32, i32.data(), i32.size() * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(cm->d_f, f.data(), f.size() * 4, hipMemcpyHostToDevice));
@@ -340,10 +556,11 @@ int ixg_cm_free(ixg_ctx *c, ixg_cm *cm) {
   if (!cm) return IXG_EINVAL;
   if (cm->d_st) {
     if (!c) return IXG_EINVAL;
-    const int M = cm->h.M;
-    dfree(c, cm->d_st, M);
-    dfree(c, cm->d_i32, (2 * M + cm->h.begin_list.size()) * 4);
-    dfree(c, cm->d_f, cm->bytes - M - (2 * M + cm->h.begin_list.size()) * 4);
+    dfree(c, cm->d_st, cm->h.M);
+    dfree(c, cm->d_i32, cm->b_i32);
+    dfree(c, cm->d_f, cm->b_f);
+    dfree(c, cm->d_tr, cm->b_tr);
+    dfree(c, cm->d_yl, cm->b_yl);
   }
   delete cm;
   return IXG_OK;
@@ -382,7 +599,7 @@ static int single_env(ixg_ctx *c, const ixg_cm *cm, const ixg_bands *b, const ui
   e.pad = 0;
   e.bands = *b;
   std::vector<std::vector<float>> roots;
-  RCHK(run_envelopes(c, cm, d, {0}, &e, 1, inside, roots, nullptr));
+  RCHK(run_envelopes(c, cm, d, {0}, &e, 1, inside, nullptr, nullptr, roots, nullptr));
   root.swap(roots[0]);
   return IXG_OK;
 }
@@ -437,7 +654,7 @@ extern "C" int ixg_hb_scan_batch(ixg_ctx *c, const ixg_cm *cm, const ixg_target 
   }
   c->ms[2] = 0;
   std::vector<std::vector<float>> roots;
-  RCHK(run_envelopes(c, cm, base, off, env, n, inside ? 1 : 0, roots, cells));
+  RCHK(run_envelopes(c, cm, base, off, env, n, inside ? 1 : 0, nullptr, nullptr, roots, cells));
   std::vector<ixg_hit> out;
   for (int64_t k = 0; k < n; k++) {
     root_summary(env[k].bands, roots[k].data(), env[k].i0, env[k].j0, env_cutoff, vsc[k], envi[k], envj[k]);
@@ -452,3 +669,153 @@ extern "C" int ixg_hb_scan_batch(ixg_ctx *c, const ixg_cm *cm, const ixg_target 
   }
   return IXG_OK;
 }
+
+extern "C" {
+
+int ixg_cm_set_trpenalties(ixg_ctx *c, ixg_cm *cm, const float *g, const float *l) {
+  if (!cm || !g || !l || (cm->d_st && !c)) {
+    g_err = "ixg_cm_set_trpenalties: bad arguments";
+    return IXG_EINVAL;
+  }
+  const int M = cm->h.M;
+  cm->h.trp_g.assign(g, g + 3 * (size_t)M);
+  cm->h.trp_l.assign(l, l + 3 * (size_t)M);
+  std::vector<float> all(cm->h.trp_g);
+  all.insert(all.end(), cm->h.trp_l.begin(), cm->h.trp_l.end());
+  std::vector<int32_t> yl;
+  for (int q = 0; q < 6; q++) {
+    cm->yoff[q] = (int32_t)yl.size();
+    for (int y = 1; y < M; y++)
+      if (all[(size_t)q * M + y] > IMPOSSIBLE + 1.0f) yl.push_back(y);
+  }
+  cm->yoff[6] = (int32_t)yl.size();
+  if (!cm->d_st) return IXG_OK;
+  HIPCHK(hipSetDevice(c->device));
+  dfree(c, cm->d_tr, cm->b_tr);
+  dfree(c, cm->d_yl, cm->b_yl);
+  cm->d_tr = nullptr;
+  cm->d_yl = nullptr;
+  cm->b_tr = all.size() * 4;
+  cm->b_yl = std::max<size_t>(yl.size(), 1) * 4;
+  RCHK(dmalloc(c, (void **)&cm->d_tr, cm->b_tr));
+  RCHK(dmalloc(c, (void **)&cm->d_yl, cm->b_yl));
+  HIPCHK(hipMemcpy(cm->d_tr, all.data(), cm->b_tr, hipMemcpyHostToDevice));
+  if (!yl.empty()) HIPCHK(hipMemcpy(cm->d_yl, yl.data(), yl.size() * 4, hipMemcpyHostToDevice));
+  return IXG_OK;
+}
+
+int ixg_cm_trtables(const ixg_cm *cm, float *lmesc, float *rmesc, int32_t *lpos, int32_t *rpos) {
+  if (!cm) return IXG_EINVAL;
+  const CmHost &h = cm->h;
+  if (lmesc) memcpy(lmesc, h.lmesc.data(), h.lmesc.size() * 4);
+  if (rmesc) memcpy(rmesc, h.rmesc.data(), h.rmesc.size() * 4);
+  if (lpos) memcpy(lpos, h.lpos.data(), h.lpos.size() * 4);
+  if (rpos) memcpy(rpos, h.rpos.data(), h.rpos.size() * 4);
+  return IXG_OK;
+}
+
+}  // extern "C"
+
+static int tr_single(ixg_ctx *c, const ixg_cm *cm, const ixg_bands *b, const ixg_trvalid *valid, const uint8_t *dsq,
+                     int64_t L, int32_t i0, int32_t j0, float cutoff, int do_null3, int pass_idx, float env_cutoff,
+                     int local, int inside, float *vsc, int32_t *vmode, int64_t *envi, int64_t *envj,
+                     ixg_trhit *hits, int64_t cap, int64_t *nhits) {
+  if (!c || !cm || !b || !valid || !dsq || !cm->d_st || i0 < 1 || j0 > L || j0 < i0 || !vsc || !vmode || !envi ||
+      !envj || !nhits || (cap > 0 && !hits)) {
+    g_err = "truncated scan: bad arguments (1 <= i0 <= j0 <= L, device CM)";
+    return IXG_EINVAL;
+  }
+  for (int64_t i = i0; i <= j0; i++)
+    if (dsq[i] >= KP) {
+      g_err = "digitized residue code >= 18 in the envelope";
+      return IXG_EINVAL;
+    }
+  TrSel sel;
+  RCHK(tr_select(cm, pass_idx, local, sel));
+  HIPCHK(hipSetDevice(c->device));
+  uint8_t *d;
+  RCHK(scratch(c, S_CM_SEQ, (size_t)L + 2, (void **)&d));
+  HIPCHK(hipMemcpyAsync(d, dsq, (size_t)L + 1, hipMemcpyHostToDevice, c->stream));
+  ixg_envelope e;
+  e.i0 = i0;
+  e.j0 = j0;
+  e.strand = 0;
+  e.pad = 0;
+  e.bands = *b;
+  std::vector<std::vector<float>> roots;
+  RCHK(run_envelopes(c, cm, d, {0}, &e, 1, inside, &sel, valid, roots, nullptr));
+  const int64_t nroot = (int64_t)roots[0].size() / 5;
+  tr_root_summary(*b, roots[0].data(), nroot, mode_mask(*valid, 0, sel.fill), i0, j0, env_cutoff, *vsc, *vmode,
+                  *envi, *envj);
+  std::vector<ixg_trhit> out;
+  tr_report(*cm, *b, dsq, i0, j0, roots[0].data(), nroot, pass_idx, local, cutoff, do_null3, 0, out);
+  *nhits = (int64_t)out.size();
+  for (int64_t k = 0; k < std::min<int64_t>(cap, out.size()); k++) hits[k] = out[k];
+  return (int64_t)out.size() > cap ? IXG_ERANGE : IXG_OK;
+}
+
+extern "C" {
+
+int ixg_trcyk_scan_hb(ixg_ctx *c, const ixg_cm *cm, const ixg_bands *b, const ixg_trvalid *valid,
+                      const uint8_t *dsq, int64_t L, int32_t i0, int32_t j0, float cutoff, int do_null3,
+                      int pass_idx, float env_cutoff, int local, float *vsc, int32_t *vmode, int64_t *envi,
+                      int64_t *envj, ixg_trhit *hits, int64_t cap, int64_t *nhits) {
+  return tr_single(c, cm, b, valid, dsq, L, i0, j0, cutoff, do_null3, pass_idx, env_cutoff, local, 0, vsc, vmode,
+                   envi, envj, hits, cap, nhits);
+}
+
+int ixg_trinside_scan_hb(ixg_ctx *c, const ixg_cm *cm, const ixg_bands *b, const ixg_trvalid *valid,
+                         const uint8_t *dsq, int64_t L, int32_t i0, int32_t j0, float cutoff, int do_null3,
+                         int pass_idx, float env_cutoff, int local, float *vsc, int32_t *vmode, int64_t *envi,
+                         int64_t *envj, ixg_trhit *hits, int64_t cap, int64_t *nhits) {
+  return tr_single(c, cm, b, valid, dsq, L, i0, j0, cutoff, do_null3, pass_idx, env_cutoff, local, 1, vsc, vmode,
+                   envi, envj, hits, cap, nhits);
+}
+
+int ixg_trhb_scan_batch(ixg_ctx *c, const ixg_cm *cm, const ixg_target *tgt, const ixg_trenvelope *env, int64_t n,
+                        int inside, int pass_idx, int local, float env_cutoff, float cutoff, int do_null3,
+                        float *vsc, int32_t *vmode, int64_t *envi, int64_t *envj, ixg_trhit *hits, int64_t cap,
+                        int64_t *nhits, int64_t *cells) {
+  if (!c || !cm || !cm->d_st || !tgt || (n && !env) || !vsc || !vmode || !envi || !envj || !nhits ||
+      (cap > 0 && !hits)) {
+    g_err = "ixg_trhb_scan_batch: bad arguments";
+    return IXG_EINVAL;
+  }
+  TrSel sel;
+  RCHK(tr_select(cm, pass_idx, local, sel));
+  HIPCHK(hipSetDevice(c->device));
+  int64_t L = 0;
+  const uint8_t *base = ixg_target_device_base(tgt, &L);
+  std::vector<int64_t> off(n);
+  std::vector<ixg_envelope> ev(n);
+  std::vector<ixg_trvalid> val(n);
+  for (int64_t k = 0; k < n; k++) {
+    if (env[k].i0 < 1 || env[k].j0 > L || env[k].j0 < env[k].i0 || (env[k].strand != 0 && env[k].strand != 1)) {
+      g_err = "ixg_trhb_scan_batch: envelope outside the target";
+      return IXG_EINVAL;
+    }
+    off[k] = env[k].strand ? L + 2 : 0;
+    ev[k].i0 = env[k].i0;
+    ev[k].j0 = env[k].j0;
+    ev[k].strand = env[k].strand;
+    ev[k].pad = 0;
+    ev[k].bands = env[k].bands;
+    val[k] = env[k].valid;
+  }
+  c->ms[2] = 0;
+  std::vector<std::vector<float>> roots;
+  RCHK(run_envelopes(c, cm, base, off, ev.data(), n, inside ? 1 : 0, &sel, val.data(), roots, cells));
+  std::vector<ixg_trhit> out;
+  for (int64_t k = 0; k < n; k++) {
+    const int64_t nroot = (int64_t)roots[k].size() / 5;
+    tr_root_summary(ev[k].bands, roots[k].data(), nroot, mode_mask(val[k], 0, sel.fill), ev[k].i0, ev[k].j0,
+                    env_cutoff, vsc[k], vmode[k], envi[k], envj[k]);
+    tr_report(*cm, ev[k].bands, ixg_target_host_dsq(tgt, ev[k].strand), ev[k].i0, ev[k].j0, roots[k].data(), nroot,
+              pass_idx, local, cutoff, do_null3, (int32_t)k, out);
+  }
+  *nhits = (int64_t)out.size();
+  for (int64_t k = 0; k < std::min<int64_t>(cap, out.size()); k++) hits[k] = out[k];
+  return (int64_t)out.size() > cap ? IXG_ERANGE : IXG_OK;
+}
+
+}  // extern "C"

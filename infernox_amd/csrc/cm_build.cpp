@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include "cm_internal.h"
+#include "ctx.h"
 
 namespace ixg {
 
@@ -133,6 +134,106 @@ float host_flogsum(const std::vector<float> &t, float a, float b) {  // cp9.rs:5
   const float mx = a > b ? a : b, mn = a < b ? a : b;
   if (mn == -INFINITY || (mx - mn) >= 23.0f) return mx;
   return mx + t[(size_t)((mx - mn) * 1000.0f)];
+}
+
+}  // namespace ixg
+
+namespace ixg {
+
+// Null-weighted mean score of a degenerate code (cp9.rs:4375-4388).
+static float expect_score(int x, const float *sc, const float *p) {
+  if (!(x < 4 || (x > 4 && x < KP - 2))) return 0.0f;
+  float r = 0.0f, den = 0.0f;
+  for (int i = 0; i < 4; i++)
+    if (DEGEN[x][i]) {
+      r += sc[i] * p[i];
+      den += p[i];
+    }
+  return r / den;
+}
+
+// Marginal emission scores of the truncated scanners (cp9.rs:4441-4527): a
+// pair state's left (right) score is the log-odds of its row (column) sum; a
+// singlet state's own side is its emission score, the other side 0.
+void marginal_emissions(const ixg_cm_params &p, CmHost &h) {
+  const int M = p.M;
+  h.lmesc.assign((size_t)M * KP, 0.f);
+  h.rmesc.assign((size_t)M * KP, 0.f);
+  for (int v = 0; v < M; v++) {
+    float *lm = &h.lmesc[(size_t)v * KP], *rm = &h.rmesc[(size_t)v * KP];
+    lm[4] = rm[4] = lm[KP - 2] = rm[KP - 2] = lm[KP - 1] = rm[KP - 1] = IMPOSSIBLE;
+    const int st = p.sttype[v];
+    const float *e = p.e + (size_t)v * 16;
+    float *sides[2] = {nullptr, nullptr};
+    if (st == MP_ST) {
+      float l4[4] = {0, 0, 0, 0}, r4[4] = {0, 0, 0, 0};
+      for (int x = 0; x < 4; x++)
+        for (int y = 0; y < 4; y++) {
+          l4[x] += e[x * 4 + y];
+          r4[y] += e[x * 4 + y];
+        }
+      for (int x = 0; x < 4; x++) {
+        lm[x] = (float)sre_log2((double)(l4[x] / p.null[x]));
+        rm[x] = (float)sre_log2((double)(r4[x] / p.null[x]));
+      }
+      sides[0] = lm;
+      sides[1] = rm;
+    } else if (st == ML_ST || st == IL_ST || st == MR_ST || st == IR_ST) {
+      float *own = (st == ML_ST || st == IL_ST) ? lm : rm;
+      for (int x = 0; x < 4; x++) own[x] = (float)sre_log2((double)(e[x] / p.null[x]));
+      sides[0] = own;
+    }
+    for (float *s : sides)
+      if (s)
+        for (int x = 5; x <= KP - 3; x++) s[x] = expect_score(x, s, p.null);
+  }
+}
+
+// Consensus span [lpos, rpos] of every state's subtree: a left-to-right walk
+// of the guide tree counting consensus columns (the emit map, cp9.rs:144-204),
+// with the per-node-type offsets of cm_dpsearch_trunc.rs:2394-2403.
+int state_spans(const ixg_cm_params &p, CmHost &h) {
+  const int n = p.nodes, M = p.M;
+  std::vector<int32_t> ndidx(M), lpos(n, -1), rpos(n, -1);
+  for (int nd = 0, v = 0; v < M; v++) {
+    while (nd + 1 < n && p.nodemap[nd + 1] <= v) nd++;
+    ndidx[v] = nd;
+  }
+  std::vector<int32_t> stack;  // (node << 1) | right side
+  stack.push_back(0);
+  int cpos = 0;
+  int64_t steps = 0;
+  while (!stack.empty()) {
+    const int it = stack.back(), nd = it >> 1, nt = p.ndtype[nd];
+    stack.pop_back();
+    if (++steps > 2 * (int64_t)n + 2 || (!(it & 1) && lpos[nd] != -1)) {
+      g_err = "ixg_cm_create: node types and bifurcation children do not form a guide tree";
+      return IXG_EINVAL;
+    }
+    if (it & 1) {
+      rpos[nd] = cpos + 1;
+      if (nt == MATP_ND || nt == MATR_ND) cpos++;
+      continue;
+    }
+    if (nt == MATP_ND || nt == MATL_ND) cpos++;
+    lpos[nd] = cpos;
+    stack.push_back((nd << 1) | 1);
+    if (nt == BIF_ND) {
+      const int v = p.nodemap[nd];
+      stack.push_back(ndidx[p.cnum[v]] << 1);
+      stack.push_back(ndidx[p.cfirst[v]] << 1);
+    } else if (nt != END_ND && nd + 1 < n) {
+      stack.push_back((nd + 1) << 1);
+    }
+  }
+  h.lpos.resize(M);
+  h.rpos.resize(M);
+  for (int v = 0; v < M; v++) {
+    const int nd = ndidx[v], nt = p.ndtype[nd];
+    h.lpos[v] = (nt == MATP_ND || nt == MATL_ND) ? lpos[nd] : lpos[nd] + 1;
+    h.rpos[v] = (nt == MATP_ND || nt == MATR_ND) ? rpos[nd] : rpos[nd] - 1;
+  }
+  return IXG_OK;
 }
 
 }  // namespace ixg

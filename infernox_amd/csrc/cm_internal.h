@@ -21,7 +21,14 @@ struct CmHost {
   std::vector<int32_t> cfirst, cnum;
   std::vector<float> tsc, oesc, beginsc, endsc;  // M*6, M*324, M, M
   std::vector<int32_t> begin_list;               // states y >= 1 with a local begin
+  // truncated scanning
+  int clen = 0;
+  std::vector<float> lmesc, rmesc;  // M*18 marginal emission scores
+  std::vector<int32_t> lpos, rpos;  // per state: consensus span of its subtree
+  std::vector<float> trp_g, trp_l;  // 3*M truncated-begin penalties (empty until set)
 };
+void marginal_emissions(const ixg_cm_params &p, CmHost &h);
+int state_spans(const ixg_cm_params &p, CmHost &h);
 int configure_cm(const ixg_cm_params &p, CmHost &h);
 void flogsum_table(std::vector<float> &t);
 float host_flogsum(const std::vector<float> &t, float a, float b);
@@ -30,6 +37,7 @@ struct CmDev {
   const int8_t *sttype;
   const int32_t *cfirst, *cnum;
   const float *tsc, *oesc, *beginsc, *endsc;
+  const float *lmesc, *rmesc;
   const int32_t *begin_list;
   int nbegin;
   int M;
@@ -41,14 +49,25 @@ struct CmDev {
 // like the reference's CP9Bands: jmin/jmax per state, hd_off per state (M+1),
 // and per (v, j - jmin[v]) at hd_off[v] + jp: hdmin, hdmax and roff, the
 // offset of the row's first cell in alpha.
+// The truncated scanners add four planes (J, L, R, T) of nc cells each at
+// alpha + p * nc, a per-state mode mask (bit 0 J, 1 L, 2 R, 3 T, with the
+// pass's fill flags applied) and, per root cell, the entry state and mode of
+// the best truncated begin (state * 8 + mode, int32 bits in rout).
 struct EnvDev {
   const uint8_t *dsq;
   const int32_t *jmin, *jmax, *hd_off, *hdmin, *hdmax, *roff;
   float *alpha;
   int32_t i0, j0;
+  int32_t nc;
+  const int32_t *mask;
+  float *rout;  // root rows out: standard nroot floats; truncated J|L|R|T|aux, nroot each
 };
 
 hipError_t launch_hb_scan(const CmDev &cm, const EnvDev *envs, int64_t n, const float *ftbl, int inside,
                           hipStream_t s);
+// pty: the pass's truncated-begin penalty per state; ylist: the ny states
+// y >= 1 with an allowed truncated begin, ascending.
+hipError_t launch_trhb_scan(const CmDev &cm, const EnvDev *envs, int64_t n, const float *ftbl, const float *pty,
+                            const int32_t *ylist, int ny, int inside, hipStream_t s);
 
 }  // namespace ixg

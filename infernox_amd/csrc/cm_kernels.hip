@@ -161,8 +161,8 @@ __global__ __launch_bounds__(256) void hb_scan_kernel(CmDev cm, const EnvDev *en
     }
     __syncthreads();
   }
-  // local begins into the root deck by MAX (cp9.rs:5264-5304, 5908-5937)
-  if (!cm.local_begin) return;
+  // local begins into the root deck by MAX (cp9.rs:5264-5304, 5908-5937);
+  // the root rows also go to the envelope's compact root output
   const int jmn = e.jmin[0], jmx = e.jmax[0];
   const int nj = jmx >= jmn ? jmx - jmn + 1 : 0;
   if (nj == 0) return;
@@ -180,15 +180,17 @@ __global__ __launch_bounds__(256) void hb_scan_kernel(CmDev cm, const EnvDev *en
     }
     const int d = hmn[lo] + (tgt - ro[lo]), j = jmn + lo;
     float cur = e.alpha[tgt];
-    for (int b = 0; b < cm.nbegin; b++) {
-      const int y = cm.begin_list[b];
-      float a;
-      if (child_cell(e, y, j, d, a)) {
-        const float s = a + cm.beginsc[y];
-        if (s > cur) cur = s;
+    if (cm.local_begin)
+      for (int b = 0; b < cm.nbegin; b++) {
+        const int y = cm.begin_list[b];
+        float a;
+        if (child_cell(e, y, j, d, a)) {
+          const float s = a + cm.beginsc[y];
+          if (s > cur) cur = s;
+        }
       }
-    }
     e.alpha[tgt] = cur;
+    e.rout[c] = cur;
   }
 }
 

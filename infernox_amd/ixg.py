@@ -56,7 +56,8 @@ EXPORTS = [
     "ixg_vit_scores", "ixg_fwd_scores", "ixg_bias_scores", "ixg_msv_scores",
     "ixg_f3_filter_sequence", "ixg_target_create", "ixg_target_free", "ixg_window_tasks",
     "ixg_f3_filter_tasks", "ixg_last_timings", "ixg_cm_create", "ixg_cm_free", "ixg_cm_tables",
-    "ixg_cyk_scan_hb", "ixg_inside_scan_hb", "ixg_hb_scan_batch",
+    "ixg_cyk_scan_hb", "ixg_inside_scan_hb", "ixg_hb_scan_batch", "ixg_cm_set_trpenalties", "ixg_cm_trtables",
+    "ixg_trcyk_scan_hb", "ixg_trinside_scan_hb", "ixg_trhb_scan_batch",
 ]
 
 
@@ -78,6 +79,20 @@ class Hit(C.Structure):
 
 class Envelope(C.Structure):
     _fields_ = [("i0", C.c_int32), ("j0", C.c_int32), ("strand", C.c_int32), ("pad", C.c_int32), ("bands", Bands)]
+
+
+class TrValid(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_uint8)) for n in ("jvalid", "lvalid", "rvalid", "tvalid")]
+
+
+class TrHit(C.Structure):
+    _fields_ = [("i", C.c_int32), ("j", C.c_int32), ("score", C.c_float), ("bias", C.c_float), ("mode", C.c_int32),
+                ("root", C.c_int32), ("env", C.c_int32)]
+
+
+class TrEnvelope(C.Structure):
+    _fields_ = [("i0", C.c_int32), ("j0", C.c_int32), ("strand", C.c_int32), ("pad", C.c_int32), ("bands", Bands),
+                ("valid", TrValid)]
 
 _lib = None
 
@@ -123,6 +138,16 @@ def lib():
                                          C.c_int, f32p, i64p, i64p, C.POINTER(Hit), i64, i64p]
         L.ixg_hb_scan_batch.argtypes = [vp, vp, vp, C.POINTER(Envelope), i64, C.c_int, C.c_float, C.c_int,
                                         C.c_float, C.c_int, f32p, i64p, i64p, C.POINTER(Hit), i64, i64p, i64p]
+        i32p = C.POINTER(C.c_int32)
+        L.ixg_cm_set_trpenalties.argtypes = [vp, vp, f32p, f32p]
+        L.ixg_cm_trtables.argtypes = [vp, f32p, f32p, i32p, i32p]
+        for nm in ("ixg_trcyk_scan_hb", "ixg_trinside_scan_hb"):
+            getattr(L, nm).argtypes = [vp, vp, C.POINTER(Bands), C.POINTER(TrValid), u8p, i64, i32, i32, C.c_float,
+                                       C.c_int, C.c_int, C.c_float, C.c_int, f32p, i32p, i64p, i64p,
+                                       C.POINTER(TrHit), i64, i64p]
+        L.ixg_trhb_scan_batch.argtypes = [vp, vp, vp, C.POINTER(TrEnvelope), i64, C.c_int, C.c_int, C.c_int,
+                                          C.c_float, C.c_float, C.c_int, f32p, i32p, i64p, i64p, C.POINTER(TrHit),
+                                          i64, i64p, i64p]
         _lib = L
     return _lib
 
@@ -402,6 +427,48 @@ class CM:
         return vsc.value, ei.value, ej.value, [(hits[k].i, hits[k].j, hits[k].score, hits[k].bias)
                                                for k in range(n.value)]
 
+    def set_trpenalties(self, g, l):
+        """Truncated-begin penalties g[3, M], l[3, M] (log2)."""
+        self._trp = (np.ascontiguousarray(g, np.float32), np.ascontiguousarray(l, np.float32))
+        check(lib().ixg_cm_set_trpenalties(self.ctx.h if self.ctx else None, self.h, f32ptr(self._trp[0]),
+                                           f32ptr(self._trp[1])), "ixg_cm_set_trpenalties")
+
+    def trtables(self):
+        M = self.M
+        lm, rm = np.zeros(M * 18, np.float32), np.zeros(M * 18, np.float32)
+        lp, rp = np.zeros(M, np.int32), np.zeros(M, np.int32)
+        i32p = C.POINTER(C.c_int32)
+        check(lib().ixg_cm_trtables(self.h, f32ptr(lm), f32ptr(rm), lp.ctypes.data_as(i32p),
+                                    rp.ctypes.data_as(i32p)))
+        return lm.reshape(M, 18), rm.reshape(M, 18), lp, rp
+
+    def trscan(self, bands, valid, dsq, L, i0, j0, pass_idx, local=True, inside=False, cutoff=0.0, null3=True,
+               env_cutoff=0.0, cap=4096):
+        """TrCYK (inside=False) / FTrInside of one envelope: (vsc, vmode, envi, envj, hits)."""
+        d = np.ascontiguousarray(dsq, np.uint8)
+        vsc, vm, ei, ej, n = C.c_float(), C.c_int32(), C.c_int64(), C.c_int64(), C.c_int64()
+        b, v = bands_struct(bands), valid_struct(valid)
+        hits = (TrHit * cap)()
+        fn = lib().ixg_trinside_scan_hb if inside else lib().ixg_trcyk_scan_hb
+        check(fn(self.ctx.h, self.h, C.byref(b), C.byref(v), u8ptr(d), int(L), int(i0), int(j0), float(cutoff),
+                 int(null3), int(pass_idx), float(env_cutoff), int(local), C.byref(vsc), C.byref(vm), C.byref(ei),
+                 C.byref(ej), hits, cap, C.byref(n)), "truncated scan")
+        return vsc.value, vm.value, ei.value, ej.value, [h[1:] for h in _trhits(hits, n.value)]
+
+
+def valid_struct(v):
+    keep = {k: np.ascontiguousarray(v[k], np.uint8) for k in ("jvalid", "lvalid", "rvalid", "tvalid")}
+    s = TrValid()
+    for k, a in keep.items():
+        setattr(s, k, a.ctypes.data_as(C.POINTER(C.c_uint8)))
+    s._keep = keep
+    return s
+
+
+def _trhits(hits, n):
+    return [(hits[k].env, hits[k].i, hits[k].j, hits[k].score, hits[k].bias, hits[k].mode, hits[k].root)
+            for k in range(n)]
+
 
 def hb_scan_batch(ctx, cm, tgt, envs, inside=False, env_cutoff=0.0, report=False, cutoff=0.0, null3=True):
     """envs: list of (i0, j0, strand, bands dict). Returns (vsc, envi, envj, hits, cells)."""
@@ -424,3 +491,29 @@ def hb_scan_batch(ctx, cm, tgt, envs, inside=False, env_cutoff=0.0, report=False
                                   C.byref(nh), C.byref(cells)), "ixg_hb_scan_batch")
     hl = [(hits[k].env, hits[k].i, hits[k].j, hits[k].score, hits[k].bias) for k in range(nh.value)] if report else []
     return vsc[:n], ei[:n], ej[:n], hl, cells.value
+
+
+def trhb_scan_batch(ctx, cm, tgt, envs, pass_idx, local=True, inside=False, env_cutoff=0.0, cutoff=0.0, null3=True):
+    """envs: list of (i0, j0, strand, bands dict, validity dict). Returns
+    (vsc, vmode, envi, envj, hits (env, i, j, score, bias, mode, root), cells)."""
+    n = len(envs)
+    arr = (TrEnvelope * max(n, 1))()
+    keep = []
+    for k, (i0, j0, strand, b, v) in enumerate(envs):
+        bs, vs = bands_struct(b), valid_struct(v)
+        keep.append((bs, vs))
+        arr[k].i0, arr[k].j0, arr[k].strand, arr[k].pad = int(i0), int(j0), int(strand), 0
+        arr[k].bands = bs
+        arr[k].valid = vs
+    vsc = np.zeros(max(n, 1), np.float32)
+    vm = np.zeros(max(n, 1), np.int32)
+    ei = np.zeros(max(n, 1), np.int64)
+    ej = np.zeros(max(n, 1), np.int64)
+    cap = 1 << 16
+    hits = (TrHit * cap)()
+    nh, cells = C.c_int64(), C.c_int64()
+    check(lib().ixg_trhb_scan_batch(ctx.h, cm.h, tgt.h, arr, n, int(inside), int(pass_idx), int(local),
+                                    float(env_cutoff), float(cutoff), int(null3), f32ptr(vsc),
+                                    vm.ctypes.data_as(C.POINTER(C.c_int32)), i64ptr(ei), i64ptr(ej), hits, cap,
+                                    C.byref(nh), C.byref(cells)), "ixg_trhb_scan_batch")
+    return vsc[:n], vm[:n], ei[:n], ej[:n], _trhits(hits, nh.value), cells.value

@@ -264,16 +264,21 @@ def state_right_delta(st):
     return 1 if st in (MP_ST, MR_ST, IR_ST) else 0
 
 
-def ij_bands(cm, i0, j0, r, seed, jitter=2):
+def ij_bands(cm, i0, j0, r, seed, jitter=2, c1=1, c2=None):
     """CP9-shaped i/j bands of every state for the envelope [i0, j0]: the
     node's consensus span [a, b] mapped linearly onto the envelope, widened by
-    r (+ a seeded jitter of 0..jitter). Returns imin, imax, jmin, jmax (int32)."""
+    r (+ a seeded jitter of 0..jitter). With c1/c2 only consensus columns
+    c1..c2 map onto the envelope (a truncated alignment); spans outside are
+    clipped to the envelope's ends. Returns imin, imax, jmin, jmax (int32)."""
     rng = np.random.default_rng(seed)
     M = cm["M"]
-    scale = (j0 - i0 + 1) / cm["clen"]
+    c2 = cm["clen"] if c2 is None else c2
+    scale = (j0 - i0 + 1) / (c2 - c1 + 1)
     lr = cm["node_lr"][cm["ndidx"]]
-    ic = i0 + np.round((lr[:, 0] - 1) * scale).astype(np.int64)
-    jc = i0 + np.round(lr[:, 1] * scale).astype(np.int64) - 1
+    ic = i0 + np.round((lr[:, 0] - c1) * scale).astype(np.int64)
+    jc = i0 + np.round((lr[:, 1] - c1 + 1) * scale).astype(np.int64) - 1
+    ic = np.clip(ic, i0, j0 + 1)
+    jc = np.clip(jc, i0 - 1, j0)
     ri = r + rng.integers(0, jitter + 1, size=M)
     rj = r + rng.integers(0, jitter + 1, size=M)
     imin = np.maximum(i0, ic - ri).astype(np.int32)
@@ -355,3 +360,92 @@ def emit(cm, seed):
     import sys
     sys.setrecursionlimit(100000)
     return np.array(go(0), np.uint8)
+
+
+# ---- truncated scanning inputs ----
+
+PASS_STD_ANY, PASS_5P_ONLY, PASS_3P_ONLY, PASS_5P_AND_3P_FORCE, PASS_5P_AND_3P_ANY = 1, 2, 3, 4, 5
+TRMODE_T, TRMODE_R, TRMODE_L, TRMODE_J, TRMODE_UNKNOWN = 0, 1, 2, 3, 4
+
+
+def fill_flags(pass_idx):
+    """(fill L, fill R, fill T) of a pipeline pass (cm_dpsearch_trunc.rs:184-199)."""
+    both = pass_idx in (PASS_5P_AND_3P_FORCE, PASS_5P_AND_3P_ANY)
+    return (both or pass_idx == PASS_3P_ONLY, both or pass_idx == PASS_5P_ONLY, both)
+
+
+def state_spans(cm):
+    """Consensus span (first, last column) of every state's subtree: the
+    node spans of the guide tree, which is what the emit map gives after its
+    per-node-type offsets (cp9.rs:144-204, 1934-1943)."""
+    lr = cm["node_lr"][cm["ndidx"]]
+    return lr[:, 0].astype(np.int64), lr[:, 1].astype(np.int64)
+
+
+def _tr_qualifies(cm):
+    st, nd = cm["sttype"], cm["ndtype"][cm["ndidx"]]
+    return ((st == MP_ST) | ((st == ML_ST) & (nd == MATL_ND)) | ((st == MR_ST) & (nd == MATR_ND)) |
+            (st == B_ST) | np.isin(st, (IL_ST, IR_ST)))
+
+
+def tr_penalties(cm, seed=0):
+    """Truncated-begin penalties g[3, M] (global) and l[3, M] (local), log2,
+    index 0 = 5'&3', 1 = 5' only, 2 = 3' only, -1e36 where entry is not
+    allowed. Shaped like cm_trunc.rs:192-384 (global: 2/(clen(clen+1)) or
+    1/clen spread over the qualifying states of a node; local: the local
+    begin probability over the subtree's fragment count), with a seeded
+    jitter: the scanners treat them as given."""
+    rng = np.random.default_rng(seed)
+    M, clen = cm["M"], cm["clen"]
+    q = _tr_qualifies(cm)
+    lp, rp = state_spans(cm)
+    g = np.full((3, M), -1e36, np.float32)
+    l = np.full((3, M), -1e36, np.float32)
+    jit = rng.uniform(0.8, 1.0, size=M)
+    g53 = 2.0 / (clen * (clen + 1))
+    g[0, q] = np.log2(g53 * jit[q])
+    m5 = q & (rp == clen)
+    m3 = q & (lp == 1)
+    g[1, m5] = np.log2(jit[m5] / clen)
+    g[2, m3] = np.log2(jit[m3] / clen)
+    sub = np.maximum(1, rp - lp + 1).astype(np.float64)
+    pb = 0.05 / max(1, int(q.sum()))
+    l[0, q] = np.log2(pb / (sub[q] * (sub[q] + 1) / 2) * jit[q])
+    l[1, q] = np.log2(pb / sub[q] * jit[q])
+    l[2, q] = np.log2(pb / sub[q] * jit[q])
+    return g, l
+
+
+def tr_validity(cm, pass_idx, sp1, sp2, ep1, ep2):
+    """jvalid/lvalid/rvalid/tvalid (uint8[M]) of an envelope whose alignment
+    may start in consensus columns sp1..sp2 and end in ep1..ep2
+    (cp9.rs:1925-1990)."""
+    al, ar, at = fill_flags(pass_idx)
+    st = cm["sttype"]
+    lp, rp = state_spans(cm)
+    jv = (lp >= sp1) & (rp <= ep1)
+    lv = al & (lp >= sp1) & (lp <= ep1) & (rp > ep2)
+    rv = ar & (rp <= ep1) & (rp >= sp1) & (lp < sp2)
+    tv = (st == B_ST) & at & (lp < sp2) & (rp > ep2)
+    pj = np.isin(st, (B_ST, MP_ST, ML_ST, IL_ST, MR_ST, IR_ST))
+    pl = np.isin(st, (B_ST, MP_ST, ML_ST, IL_ST))
+    pr = np.isin(st, (B_ST, MP_ST, MR_ST, IR_ST))
+    jv[0] |= bool((pj & jv).any())
+    lv[0] |= bool((pl & lv).any())
+    rv[0] |= bool((pr & rv).any())
+    tv[0] |= bool(((st == B_ST) & tv).any())
+    return dict(jvalid=jv.astype(np.uint8), lvalid=lv.astype(np.uint8), rvalid=rv.astype(np.uint8),
+                tvalid=tv.astype(np.uint8))
+
+
+def tr_bands(cm, i0, j0, r=8, seed=0, c1=1, c2=None):
+    """Bands of a truncated scan: d bands start at max(left, right) emission
+    count instead of the full count (cp9.rs:3832-3836); consensus columns
+    c1..c2 are the part of the model present in the envelope."""
+    return ij2d(cm, *ij_bands(cm, i0, j0, r, seed, c1=c1, c2=c2), do_trunc=True)
+
+
+def wide_tr_bands(cm, i0, j0):
+    M = cm["M"]
+    return ij2d(cm, np.full(M, i0, np.int32), np.full(M, j0 + 1, np.int32), np.full(M, i0 - 1, np.int32),
+                np.full(M, j0, np.int32), do_trunc=True)

@@ -417,7 +417,7 @@ int64_t ixo_hb_scan(const ixo_cm *cm, const ixo_cmsc *sc, const ixo_bands *b, co
 }
 
 /* cp9.rs:6516-6529 score_correction_null3 */
-static float null3_correction(const float *null, const float *comp, int len, float omega) {
+float ixo_null3_correction(const float *null, const float *comp, int len, float omega) {
   float score = 0.0f;
   for (int a = 0; a < K4; a++) {
     double term = sre_log2((double)(comp[a] / null[a])) * (double)comp[a] * (double)len;
@@ -464,7 +464,7 @@ int64_t ixo_hb_report_hits(const ixo_cm *cm, const ixo_bands *b, const uint8_t *
               for (int a = 0; a < 4; a++) comp[a] /= s;
             else
               for (int a = 0; a < 4; a++) comp[a] = 1.0f / 4.0f;
-            float corr = null3_correction(cm->null, comp, d, (float)cm->n3_omega);
+            float corr = ixo_null3_correction(cm->null, comp, d, (float)cm->n3_omega);
             hs -= corr;
             bias = corr;
             rep = hs > maxrep && hs >= cutoff;

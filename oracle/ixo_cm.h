@@ -68,6 +68,36 @@ int64_t ixo_hb_scan(const ixo_cm *cm, const ixo_cmsc *sc, const ixo_bands *b, co
 int64_t ixo_hb_report_hits(const ixo_cm *cm, const ixo_bands *b, const uint8_t *dsq, int i0, int j0,
                            const float *root, float cutoff, int do_null3, ixo_hit *out, int64_t cap);
 
+/* ---- truncated scanners (trcm.c) ---- */
+
+/* pipeline pass indices (cm_trunc.rs:49-55) and marginal modes (:34-38) */
+enum { IXO_PASS_STD_ANY = 1, IXO_PASS_5P_ONLY_FORCE = 2, IXO_PASS_3P_ONLY_FORCE = 3,
+       IXO_PASS_5P_AND_3P_FORCE = 4, IXO_PASS_5P_AND_3P_ANY = 5 };
+enum { IXO_TRMODE_T = 0, IXO_TRMODE_R = 1, IXO_TRMODE_L = 2, IXO_TRMODE_J = 3, IXO_TRMODE_UNKNOWN = 4 };
+
+/* Per-state marginal-mode validity of one envelope (CP9Bands jvalid..tvalid). */
+typedef struct { const uint8_t *jvalid, *lvalid, *rvalid, *tvalid; } ixo_trvalid;
+
+typedef struct { int32_t i, j; float score, bias; int32_t mode, root; } ixo_trhit;
+
+/* Marginal left/right emission scores lmesc/rmesc [M*18] (cp9.rs:4441-4527). */
+void ixo_cm_marginals(const ixo_cm *cm, float *lmesc, float *rmesc);
+/* Consensus emit map lpos/rpos per node (cp9.rs:144-204). */
+void ixo_emit_map(const ixo_cm *cm, int32_t *lpos, int32_t *rpos);
+/* Marginal-mode validity per state from the envelope's consensus span
+ * sp1..ep1 / sp2..ep2 (cp9.rs:1925-1990); arrays of M entries. */
+void ixo_tr_validity(const ixo_cm *cm, const int32_t *lpos, const int32_t *rpos, int pass_idx, int sp1, int sp2,
+                     int ep1, int ep2, uint8_t *jv, uint8_t *lv, uint8_t *rv, uint8_t *tv);
+/* HMM-banded TrCYK (inside = 0, cm_dpsearch_trunc.rs:264-1029) and FTrInside
+ * (inside = 1, :1055-1794) with the truncated greedy reporting and overlap
+ * removal (:2322-2495). pty: the pass's truncated-begin penalties [M].
+ * planes_out (optional): J|L|R|T planes, 4 * ncells floats. Returns the number
+ * of hits (hits beyond cap are counted, not written). */
+int64_t ixo_trhb_scan(const ixo_cm *cm, const ixo_cmsc *sc, const float *lmesc, const float *rmesc, const float *pty,
+                      const ixo_bands *b, const ixo_trvalid *val, const uint8_t *dsq, int i0, int j0, int pass_idx,
+                      int local, int inside, float cutoff, int do_null3, float env_cutoff, float *vsc, int *vmode,
+                      int64_t *envi, int64_t *envj, ixo_trhit *hits, int64_t cap, float *planes_out);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,18 @@ class Hit(C.Structure):
     _fields_ = [("i", C.c_int32), ("j", C.c_int32), ("score", C.c_float), ("bias", C.c_float)]
 
 
+_u8p = C.POINTER(C.c_uint8)
+
+
+class TrValid(C.Structure):
+    _fields_ = [("jvalid", _u8p), ("lvalid", _u8p), ("rvalid", _u8p), ("tvalid", _u8p)]
+
+
+class TrHit(C.Structure):
+    _fields_ = [("i", C.c_int32), ("j", C.c_int32), ("score", C.c_float), ("bias", C.c_float),
+                ("mode", C.c_int32), ("root", C.c_int32)]
+
+
 _ready = False
 
 
@@ -49,6 +61,15 @@ def lib():
         L.ixo_hb_report_hits.argtypes = [C.POINTER(Cm), C.POINTER(Bands), C.POINTER(C.c_uint8), C.c_int, C.c_int,
                                          _f32p, C.c_float, C.c_int, C.POINTER(Hit), C.c_int64]
         L.ixo_hb_report_hits.restype = C.c_int64
+        L.ixo_cm_marginals.argtypes = [C.POINTER(Cm), _f32p, _f32p]
+        L.ixo_emit_map.argtypes = [C.POINTER(Cm), _i32p, _i32p]
+        L.ixo_tr_validity.argtypes = [C.POINTER(Cm), _i32p, _i32p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      _u8p, _u8p, _u8p, _u8p]
+        L.ixo_trhb_scan.argtypes = [C.POINTER(Cm), C.POINTER(CmSc), _f32p, _f32p, _f32p, C.POINTER(Bands),
+                                    C.POINTER(TrValid), _u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_float,
+                                    C.c_int, C.c_float, _f32p, C.POINTER(C.c_int), C.POINTER(C.c_int64),
+                                    C.POINTER(C.c_int64), C.POINTER(TrHit), C.c_int64, _f32p]
+        L.ixo_trhb_scan.restype = C.c_int64
         _ready = True
     return L
 
@@ -133,3 +154,75 @@ class OracleCM:
         n = lib().ixo_hb_report_hits(C.byref(self.s), C.byref(bs), d.ctypes.data_as(C.POINTER(C.c_uint8)), i0, j0,
                                      r.ctypes.data_as(_f32p), cutoff, int(do_null3), out, cap)
         return [(out[k].i, out[k].j, out[k].score, out[k].bias) for k in range(min(n, cap))]
+
+
+def valid_struct(val):
+    keep = {k: np.ascontiguousarray(val[k], np.uint8) for k in ("jvalid", "lvalid", "rvalid", "tvalid")}
+    s = TrValid()
+    for k, v in keep.items():
+        setattr(s, k, v.ctypes.data_as(_u8p))
+    s._keep = keep
+    return s
+
+
+class OracleTrCM(OracleCM):
+    """OracleCM plus the truncated scanners' inputs and scan. `scores`
+    (optional) replaces the configured score tables by hand-made ones:
+    dict with tsc [M,6], oesc [M,324], beginsc, endsc [M], el_selfsc."""
+
+    def __init__(self, cm, scores=None):
+        super().__init__(cm)
+        M = cm["M"]
+        if scores is not None:
+            self._own = {k: np.ascontiguousarray(scores[k], np.float32) for k in ("tsc", "oesc", "beginsc", "endsc")}
+            self.sc2 = CmSc()
+            self.sc2.M, self.sc2.local_begin, self.sc2.el_selfsc = M, 1, float(scores["el_selfsc"])
+            for k, v in self._own.items():
+                setattr(self.sc2, k, v.ctypes.data_as(_f32p))
+        else:
+            self.sc2 = self.sc
+        self.lmesc = np.zeros((M, 18), np.float32)
+        self.rmesc = np.zeros((M, 18), np.float32)
+        lib().ixo_cm_marginals(C.byref(self.s), self.lmesc.ctypes.data_as(_f32p), self.rmesc.ctypes.data_as(_f32p))
+
+    def emit_map(self):
+        n = self.cm["nodes"]
+        lp = np.zeros(n, np.int32)
+        rp = np.zeros(n, np.int32)
+        lib().ixo_emit_map(C.byref(self.s), lp.ctypes.data_as(_i32p), rp.ctypes.data_as(_i32p))
+        return lp, rp
+
+    def validity(self, pass_idx, sp1, sp2, ep1, ep2):
+        lp, rp = self.emit_map()
+        M = self.cm["M"]
+        out = {k: np.zeros(M, np.uint8) for k in ("jvalid", "lvalid", "rvalid", "tvalid")}
+        lib().ixo_tr_validity(C.byref(self.s), lp.ctypes.data_as(_i32p), rp.ctypes.data_as(_i32p), pass_idx, sp1,
+                              sp2, ep1, ep2, *[out[k].ctypes.data_as(_u8p) for k in ("jvalid", "lvalid", "rvalid",
+                                                                                      "tvalid")])
+        return out
+
+    def trscan(self, b, val, pty, dsq, i0, j0, pass_idx, local, inside, cutoff=0.0, do_null3=True, env_cutoff=0.0,
+               want_planes=False, lmesc=None, rmesc=None):
+        bs = bands_struct(b)
+        vs = valid_struct(val)
+        n = self.ncells(b)
+        planes = np.zeros(max(4, 4 * n), np.float32) if want_planes else None
+        p = np.ascontiguousarray(pty, np.float32)
+        lm = np.ascontiguousarray(self.lmesc if lmesc is None else lmesc, np.float32)
+        rm = np.ascontiguousarray(self.rmesc if rmesc is None else rmesc, np.float32)
+        d = np.ascontiguousarray(dsq, np.uint8)
+        vsc, vmode = C.c_float(), C.c_int()
+        ei, ej = C.c_int64(), C.c_int64()
+        cap = 8192
+        hits = (TrHit * cap)()
+        nh = lib().ixo_trhb_scan(C.byref(self.s), C.byref(self.sc2), lm.ctypes.data_as(_f32p),
+                                 rm.ctypes.data_as(_f32p), p.ctypes.data_as(_f32p), C.byref(bs), C.byref(vs),
+                                 d.ctypes.data_as(_u8p), i0, j0, pass_idx, int(local), int(inside), cutoff,
+                                 int(do_null3), env_cutoff, C.byref(vsc), C.byref(vmode), C.byref(ei), C.byref(ej),
+                                 hits, cap, planes.ctypes.data_as(_f32p) if want_planes else None)
+        out = dict(vsc=vsc.value, vmode=vmode.value, envi=ei.value, envj=ej.value,
+                   hits=[(hits[k].i, hits[k].j, hits[k].score, hits[k].bias, hits[k].mode, hits[k].root)
+                         for k in range(min(nh, cap))])
+        if want_planes:
+            out["planes"] = planes[:4 * n].reshape(4, n).copy()
+        return out

@@ -113,3 +113,39 @@ def test_cm_host_tables_equal_oracle(ss, seed):
     assert bs.tobytes() == o.beginsc.tobytes() and es.tobytes() == o.endsc.tobytes()
     assert np.float32(el).tobytes() == np.float32(o.el_selfsc).tobytes()
     g.free()
+
+
+@pytest.mark.parametrize("ss,seed", [("<<..>>.", 1), ("<.>.<.>", 2), (None, 3)])
+def test_cm_truncation_tables_equal_oracle(ss, seed):
+    from infernox_amd import synth_cm as S
+    from oracle_cm_bind import OracleTrCM
+    cm = S.build_cm(ss if ss else S.multibranch(120, 4, seed), seed)
+    g = ixg.CM(None, cm)
+    lm, rm, lp, rp = g.trtables()
+    o = OracleTrCM(cm)
+    assert lm.tobytes() == o.lmesc.tobytes() and rm.tobytes() == o.rmesc.tobytes()
+    sl, sr = S.state_spans(cm)
+    assert np.array_equal(lp, sl) and np.array_equal(rp, sr)
+    gp, lpn = S.tr_penalties(cm, seed)
+    g.set_trpenalties(gp, lpn)  # host-only CM: stores the tables
+    g.free()
+
+
+def test_truncated_scan_rejects_bad_arguments():
+    from infernox_amd import synth_cm as S
+    L = ixg.lib()
+    cm = S.build_cm("<<..>>.", 1)
+    g = ixg.CM(None, cm)
+    vsc, vm, ei, ej, n = C.c_float(), C.c_int32(), C.c_int64(), C.c_int64(), C.c_int64()
+    # no context / host-only CM
+    rc = L.ixg_trcyk_scan_hb(None, g.h, None, None, None, 0, 1, 1, 0.0, 0, 2, 0.0, 1, C.byref(vsc), C.byref(vm),
+                             C.byref(ei), C.byref(ej), None, 0, C.byref(n))
+    assert rc == ixg.IXG_EINVAL
+    assert L.ixg_cm_set_trpenalties(None, None, None, None) == ixg.IXG_EINVAL
+    # a node list that is not a guide tree (bifurcation child node before its parent)
+    bad = dict(cm)
+    bad["ndtype"] = np.array([S.ROOT_ND] + [S.BIF_ND] * (cm["nodes"] - 1), np.int8)
+    p = ixg.cm_params(bad)
+    h = C.c_void_p()
+    assert L.ixg_cm_create(None, C.byref(p), C.byref(h)) == ixg.IXG_EINVAL
+    g.free()
