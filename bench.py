@@ -22,6 +22,9 @@ sys.path.insert(0, ROOT)
 from infernox_amd import ixg, synth, synth_cm  # noqa: E402
 
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # 256 CUs x 4 SIMD16 x 2.4 GHz, one lane-op per cycle
+# HBM bytes per launch of each kernel on this workload, from rocprofv3 FETCH_SIZE +
+# WRITE_SIZE passes scaled by the copy-kernel calibration (profiles/hbm_traffic_*.md)
+TRAFFIC = {}
 
 
 def workload():
@@ -76,6 +79,79 @@ def time_cm(ctx, steps, warmup):
     return out
 
 
+def trcm_workload(n_env=512, r=8, seed=3, cut=0.3):
+    """Truncated-pass workload on the same CM: n_env envelopes, each holding an
+    emitted homolog with its 5' (even k) or 3' (odd k) 30% missing at the
+    envelope's edge, bands over the consensus columns present, validity of
+    the 5'&3' pass (L, R and T planes filled)."""
+    cm = synth_cm.build_cm(synth_cm.multibranch(300, 12, seed), seed)
+    clen = cm["clen"]
+    rng = np.random.default_rng(seed + 2)
+    c1, c2 = int(cut * clen), int((1 - cut) * clen)
+    parts, envs, pos = [], [], 1
+    for k in range(n_env):
+        h = synth_cm.emit(cm, 2000 + k)
+        cutn = int(len(h) * cut)
+        if k % 2 == 0:
+            part, cols, vp = h[cutn:], (c1, clen), (1, c1 + 6, clen, clen - 2)
+        else:
+            part, cols, vp = h[:len(h) - cutn], (1, c2), (1, 3, c2 - 5, c2 + 5)
+        gap = rng.integers(0, 4, 10).astype(np.uint8)
+        parts += [part, gap]
+        i0, j0 = pos, pos + len(part) - 1
+        b = synth_cm.tr_bands(cm, i0, j0, r=r, seed=k, c1=cols[0], c2=cols[1])
+        val = synth_cm.tr_validity(cm, synth_cm.PASS_5P_AND_3P_FORCE, *vp)
+        envs.append((i0, j0, 0, b, val))
+        pos = j0 + 1 + len(gap)
+    seq = np.concatenate(parts)
+    L = len(seq)
+    dsq = np.concatenate([[255], seq, [255]]).astype(np.uint8)
+    return cm, dsq, L, envs
+
+
+def filled_cells(cm, envs):
+    """Banded cells summed over the marginal planes each state fills (J, L,
+    R; T for bifurcations)."""
+    st = cm["sttype"]
+    tot = 0
+    for (_, _, _, b, val) in envs:
+        M = cm["M"]
+        w = np.maximum(0, b["hdmax"] - b["hdmin"] + 1)
+        per_v = np.add.reduceat(np.concatenate([w, [0]]), b["hd_off"][:M]) if len(w) else np.zeros(M)
+        nj = b["hd_off"][1:] - b["hd_off"][:-1]
+        per_v = np.where(nj > 0, per_v, 0)
+        planes = (val["jvalid"].astype(np.int64) + val["lvalid"] + val["rvalid"] +
+                  ((st == synth_cm.B_ST) & (val["tvalid"] > 0)))
+        tot += int((per_v * planes)[1:].sum())
+    return tot
+
+
+def time_trcm(ctx, steps, warmup):
+    cm, dsq, L, envs = trcm_workload()
+    g = ixg.CM(ctx, cm)
+    gp, lp = synth_cm.tr_penalties(cm, 3)
+    g.set_trpenalties(gp, lp)
+    tgt = ixg.Target(ctx, dsq, L)
+    cells = filled_cells(cm, envs)
+    out = {}
+    for name, inside in (("trcyk", False), ("trinside", True)):
+        for _ in range(warmup):
+            ixg.trhb_scan_batch(ctx, g, tgt, envs, synth_cm.PASS_5P_AND_3P_FORCE, inside=inside)
+        kms, t0 = 0.0, time.perf_counter()
+        for _ in range(steps):
+            vsc, vm, ei, ej, hits, _ = ixg.trhb_scan_batch(ctx, g, tgt, envs, synth_cm.PASS_5P_AND_3P_FORCE,
+                                                           inside=inside)
+            kms += ctx.timings()[0][2]
+        wall = (time.perf_counter() - t0) / steps
+        kms /= steps
+        out[name] = {"gcups": cells / (kms * 1e-3) / 1e9, "gcups_wall": cells / wall / 1e9, "kernel_ms": kms,
+                     "wall_ms": wall * 1e3, "plane_cells": int(cells), "envelopes": len(envs),
+                     "positive": int((vsc > 0).sum()), "hits": len(hits)}
+    tgt.free()
+    g.free()
+    return out
+
+
 def cpu_baseline(p, d, L, M, maxw, tasks, nt, budget_s, threads):
     """Times the oracle (C restatement, not infernox) on a bounded sample of
     the same tasks: as many tasks as fit in budget_s, run on `threads`
@@ -118,7 +194,15 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cm", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the 256 MiB device copy probe (HBM counter calibration) and print its rate")
     a = ap.parse_args()
+    if a.copy_probe:
+        with ixg.Context(0) as ctx:
+            nb = 256 << 20
+            ms = ctx.copy_probe(nb, 2)
+        print(json.dumps({"copy_probe_bytes": nb, "ms_per_copy": ms, "GB/s": 2 * nb / (ms * 1e-3) / 1e9}))
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (the path shards by "
                                    "independent tasks, see DESIGN.md)"}))
@@ -133,6 +217,7 @@ def main():
         for _ in range(a.warmup):
             ixg.f3_filter_tasks(ctx, g, tgt, tasks, nt, opts)
         stage_ms = np.zeros(5)
+        ctx.kernel_timings(reset=True)
         t0 = time.perf_counter()
         for _ in range(a.steps):
             wins, accts = ixg.f3_filter_tasks(ctx, g, tgt, tasks, nt, opts)
@@ -141,22 +226,24 @@ def main():
         dt = (time.perf_counter() - t0) / a.steps
         stage_ms /= a.steps
         ms, cells = ctx.timings()
+        kt = ctx.kernel_timings(reset=True)
         n_f1 = sum(accts[t].n_past_msv for t in range(nt))
         n_f3b = sum(accts[t].n_past_fwdbias for t in range(nt))
         tgt.free()
         g.free()
         cm_line = time_cm(ctx, max(1, a.steps // 2), 1) if not a.no_cm else None
-    # dominant device stage and its VALU roofline (ops per cell from the reference loop bodies)
-    f1_ops = 4 * cells[0]                 # SSV: 4 u8 ops per (residue, k), cm_pipeline.rs:598-601
-    fwd_ops = 16 * cells[1]               # Forward: ~16 FP32 ops per (i, k), cm_pipeline.rs:1324-1382
-    stages = {"ssv_sweep": (stage_ms[0], f1_ops), "ssv_resolve": (stage_ms[1], 0),
-              "windows_f3_f3b": (stage_ms[2], fwd_ops)}
-    dom = max(stages, key=lambda k: stages[k][0])
-    dom_ms, dom_ops = stages[dom]
-    if dom_ops == 0:
-        dom = "ssv_sweep"
-        dom_ms, dom_ops = stages[dom]
-    achieved = dom_ops / (dom_ms * 1e-3) / 1e12 if dom_ms > 0 else 0.0
+        tr_line = time_trcm(ctx, max(1, a.steps // 2), 1) if not a.no_cm else None
+    # dominant kernel with a per-unit op count, and its VALU roofline: ops per
+    # launch (the last step's cells x ops per cell, from the reference loop
+    # bodies) over the kernel's mean launch time (HIP events around each launch)
+    ops_per_launch = {"ssv_sweep": 4 * cells[0],  # SSV: 4 u8 ops per (residue, k), cm_pipeline.rs:598-601
+                      "fwd": 16 * cells[1],        # Forward: ~16 FP32 ops per (i, k), cm_pipeline.rs:1324-1382
+                      "bias": 10 * cells[2]}       # bias filter: ~10 FP32 ops per residue, cm_pipeline.rs:1750-1763
+    kern = {k: {"ms_per_launch": v[0] / v[1], "launches_per_step": v[1] / a.steps}
+            for k, v in kt.items() if v[1] > 0}
+    dom = max(ops_per_launch, key=lambda k: kt[k][0])
+    dom_ms = kt[dom][0] / max(1, kt[dom][1])
+    achieved = ops_per_launch[dom] / (dom_ms * 1e-3) / 1e12 if dom_ms > 0 else 0.0
     line = {
         "metric": "Mres/s searched (F1-F3b filter cascade, both strands)",
         "value": nres / dt / 1e6,
@@ -177,11 +264,15 @@ def main():
         "stage_ms": {"ssv_sweep": stage_ms[0], "ssv_resolve": stage_ms[1], "f3_f3b_windows": stage_ms[2],
                      "host_glue": stage_ms[3], "total_call": stage_ms[4]},
         "survivors": {"f1_windows": int(n_f1), "f3b_windows": int(n_f3b), "merged": len(wins)},
+        "kernels": kern,
         "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": VALU_PEAK_TOPS,
-                     "unit": "TOPS", "frac": achieved / VALU_PEAK_TOPS, "traffic": None},
+                     "unit": "TOPS", "frac": achieved / VALU_PEAK_TOPS, "ms_per_launch": dom_ms,
+                     "ops_per_launch": ops_per_launch[dom], "traffic": TRAFFIC.get(dom)},
     }
     if cm_line:
         line["hb_scanners"] = cm_line
+    if tr_line:
+        line["hb_truncated_scanners"] = tr_line
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(p, d, L, M, maxw, tasks, nt, a.cpu_seconds, a.cpu_threads)
     print(json.dumps(line))

@@ -271,6 +271,21 @@ int ixg_trhb_scan_batch(ixg_ctx *ctx, const ixg_cm *cm, const ixg_target *tgt, c
                         int do_null3, float *vsc, int32_t *vmode, int64_t *envi, int64_t *envj, ixg_trhit *hits,
                         int64_t cap, int64_t *nhits, int64_t *cells);
 
+/* Per-kernel device time, cumulative since the last reset: HIP events around
+ * every launch of each kernel (IXG_K_* index), total milliseconds and launch
+ * counts; reset != 0 clears them after reading. A measurement hook with no
+ * counterpart in the reference. Timing synchronizes the context's stream
+ * after each launch. */
+enum { IXG_K_SSV_SWEEP = 0, IXG_K_SSV_RESOLVE = 1, IXG_K_FWD = 2, IXG_K_BIAS = 3, IXG_K_VIT = 4, IXG_K_MSV = 5,
+       IXG_K_HB_SCAN = 6, IXG_K_TRHB_SCAN = 7, IXG_K_COPY = 8, IXG_NKERNELS = 9 };
+int ixg_kernel_timings(ixg_ctx *ctx, double *ms, int64_t *launches, int reset);
+/* Measurement hook (no counterpart in the reference): copies `bytes`
+ * device-to-device with a plain kernel reps times after one warm-up copy;
+ * *ms_per_copy = mean kernel time. Its counters calibrate HBM traffic
+ * readings (FETCH_SIZE / WRITE_SIZE) and its rate is the achievable copy
+ * bandwidth. */
+int ixg_copy_probe(ixg_ctx *ctx, size_t bytes, int reps, double *ms_per_copy);
+
 /* Per-stage device timings (milliseconds, HIP events) of the most recent
  * ixg_f3_filter_tasks call on ctx: [0] F1 sweep, [1] F1 resolve, [2] F2/F3/F3b
  * window kernels, [3] host glue, [4] total. Also the launch counts. */

@@ -157,6 +157,7 @@ int ixg_init(int device, size_t mem_budget, ixg_ctx **out) {
     return IXG_EDEVICE;
   }
   for (auto &ev : c->ev) HIPCHK(hipEventCreate(&ev));
+  for (auto &ev : c->kev) HIPCHK(hipEventCreate(&ev));
   *out = c;
   return IXG_OK;
 }
@@ -167,6 +168,7 @@ int ixg_shutdown(ixg_ctx *c) {
   (void)hipStreamSynchronize(c->stream);
   for (auto &b : c->bufs) dfree(c, b.p, b.cap);
   for (auto &ev : c->ev) (void)hipEventDestroy(ev);
+  for (auto &ev : c->kev) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return IXG_OK;
@@ -462,6 +464,7 @@ int run_f1(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<
     static const unsigned long long zero = 0;
     HIPCHK(hipMemcpyAsync(d_counter, &zero, 8, hipMemcpyHostToDevice, c->stream));
     if (attempt) HIPCHK(hipEventRecord(c->ev[1], c->stream));
+    HIPCHK(hipEventRecord(c->ev[7], c->stream));
     HIPCHK(launch_ssv_resolve(sd, d_tasks, (int64_t)tasks.size(), d_mask, d_cinfo, thresh, d_raw, cap, d_counter,
                               c->stream));
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
@@ -474,11 +477,16 @@ int run_f1(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<
   if (count) HIPCHK(hipMemcpy(h.data(), d_raw, count * sizeof(RawWinDev), hipMemcpyDeviceToHost));
   raws.assign(tasks.size(), {});
   for (const auto &r : h) raws[r.task].push_back({r.n, r.length, r.k, r.score});
-  float a = 0, b = 0;
+  float a = 0, b = 0, kr = 0;
   HIPCHK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
   HIPCHK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
+  HIPCHK(hipEventElapsedTime(&kr, c->ev[7], c->ev[2]));
   c->ms[0] += a;
   c->ms[1] += b;
+  c->kms[IXG_K_SSV_SWEEP] += a;
+  c->kn[IXG_K_SSV_SWEEP] += 1;
+  c->kms[IXG_K_SSV_RESOLVE] += kr;
+  c->kn[IXG_K_SSV_RESOLVE] += 1;
   c->cells[0] += total_len * p->msv.m;
   return IXG_OK;
 }
@@ -526,11 +534,13 @@ int run_scores(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vec
   hipError_t e = hipSuccess;
   int64_t cells = 0;
   for (auto &w : wd) cells += w.len;
+  HIPCHK(ktime_begin(c));
   if (kind == 0) e = launch_fwd(p->fwd_dev(), d_seq, d_w, n, d_sc, c->stream);
   else if (kind == 1) e = launch_bias(p->bias_dev(), d_seq, d_w, n, d_sc, c->stream);
   else if (kind == 2) e = launch_vit(p->vit_dev(), d_seq, d_w, n, d_sc, d_ovf, c->stream);
   else e = launch_msv(p->ssv_dev(), d_seq, d_w, n, d_sc, d_ovf, c->stream);
   HIPCHK(e);
+  HIPCHK(ktime_end(c, kind == 0 ? IXG_K_FWD : kind == 1 ? IXG_K_BIAS : kind == 2 ? IXG_K_VIT : IXG_K_MSV));
   HIPCHK(hipMemcpyAsync(sc.data(), d_sc, n * 4, hipMemcpyDeviceToHost, c->stream));
   if (kind >= 2) HIPCHK(hipMemcpyAsync(ovf.data(), d_ovf, n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -880,6 +890,45 @@ int ixg_f3_filter_tasks(ixg_ctx *c, const ixg_p7 *p, const ixg_target *tgt, cons
     for (int64_t t = 0; t < nt; t++) acct[t] = accts[t];
   c->ms[4] = total.ms();
   return emit(res, out, cap, n);
+}
+
+int ixg_copy_probe(ixg_ctx *c, size_t bytes, int reps, double *ms_per_copy) {
+  if (!c || bytes < 16 || reps < 1 || !ms_per_copy) return IXG_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  const size_t n = bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  RCHK(dmalloc(c, &a, n * 16));
+  int rc = dmalloc(c, &b, n * 16);
+  if (rc != IXG_OK) {
+    dfree(c, a, n * 16);
+    return rc;
+  }
+  double tot = 0;
+  hipError_t e = hipMemsetAsync(a, 1, n * 16, c->stream);
+  for (int r = 0; r <= reps && e == hipSuccess; r++) {  // the first copy is a warm-up
+    if ((e = ktime_begin(c)) != hipSuccess) break;
+    if ((e = launch_copy_probe(a, b, (int64_t)n, c->stream)) != hipSuccess) break;
+    const double before = c->kms[IXG_K_COPY];
+    if ((e = ktime_end(c, IXG_K_COPY)) != hipSuccess) break;
+    if (r) tot += c->kms[IXG_K_COPY] - before;
+  }
+  dfree(c, a, n * 16);
+  dfree(c, b, n * 16);
+  if (e != hipSuccess) {
+    g_err = std::string("ixg_copy_probe: ") + hipGetErrorString(e);
+    return IXG_EDEVICE;
+  }
+  *ms_per_copy = tot / reps;
+  return IXG_OK;
+}
+
+int ixg_kernel_timings(ixg_ctx *c, double *ms, int64_t *launches, int reset) {
+  if (!c) return IXG_EINVAL;
+  if (ms) memcpy(ms, c->kms, sizeof(c->kms));
+  if (launches) memcpy(launches, c->kn, sizeof(c->kn));
+  if (reset)
+    for (int k = 0; k < IXG_NKERNELS; k++) c->kms[k] = 0, c->kn[k] = 0;
+  return IXG_OK;
 }
 
 int ixg_last_timings(const ixg_ctx *c, double *ms5, int64_t *cells5) {

@@ -14,7 +14,7 @@ struct ixg_cm {
   CmHost h;
   std::vector<float> ftbl_host;
   int8_t *d_st = nullptr;
-  int32_t *d_i32 = nullptr;  // cfirst | cnum | begin_list
+  int32_t *d_i32 = nullptr;  // cfirst | cnum | begin_list | lvl_states | lvl_off
   float *d_f = nullptr;      // tsc | oesc | beginsc | endsc | flogsum table | lmesc | rmesc
   float *d_tr = nullptr;     // truncated-begin penalties: g[3][M] | l[3][M]
   int32_t *d_yl = nullptr;   // per (local, pass index): states with an allowed truncated begin
@@ -28,6 +28,9 @@ struct ixg_cm {
     c.cnum = d_i32 + M;
     c.begin_list = d_i32 + 2 * M;
     c.nbegin = (int)h.begin_list.size();
+    c.lvl_states = c.begin_list + h.begin_list.size();
+    c.lvl_off = c.lvl_states + M;
+    c.nlev = (int)h.lvl_off.size() - 1;
     c.tsc = d_f;
     c.oesc = d_f + (size_t)M * MAXC;
     c.beginsc = c.oesc + (size_t)M * KP * KP;
@@ -222,6 +225,45 @@ int mode_mask(const ixg_trvalid &val, int v, int fill) {
          ((val.tvalid[v] && (fill & 8)) ? 8 : 0);
 }
 
+// Per state, the work units of its level step (cells; rows of an IL state;
+// i diagonals of an IR state; 0 for a state the scan skips), then the first
+// i of each IR state.
+void append_units(const CmHost &h, const ixg_bands &b, const ixg_trvalid *val, int fill,
+                  std::vector<int32_t> &pack) {
+  const int M = h.M;
+  std::vector<int32_t> ib(M, 0);
+  for (int v = 0; v < M; v++) {
+    const int nj = b.jmax[v] >= b.jmin[v] ? b.jmax[v] - b.jmin[v] + 1 : 0;
+    int64_t n = 0;
+    const bool skip = val && (v == 0 || mode_mask(*val, v, fill) == 0);
+    const int st = h.sttype[v];
+    if (nj > 0 && !skip) {
+      if (st == IL_ST) {
+        n = nj;
+      } else if (st == IR_ST) {
+        int imin = INT32_MAX, imax = INT32_MIN;
+        for (int jp = 0; jp < nj; jp++) {
+          const int r = b.hd_off[v] + jp;
+          if (b.hdmax[r] < b.hdmin[r]) continue;
+          imin = std::min(imin, b.jmin[v] + jp - b.hdmax[r] + 1);
+          imax = std::max(imax, b.jmin[v] + jp - b.hdmin[r] + 1);
+        }
+        if (imax >= imin) {
+          n = imax - imin + 1;
+          ib[v] = imin;
+        }
+      } else {
+        for (int jp = 0; jp < nj; jp++) {
+          const int r = b.hd_off[v] + jp;
+          if (b.hdmax[r] >= b.hdmin[r]) n += b.hdmax[r] - b.hdmin[r] + 1;
+        }
+      }
+    }
+    pack.push_back((int32_t)n);
+  }
+  pack.insert(pack.end(), ib.begin(), ib.end());
+}
+
 // One or more envelopes over a device sequence buffer: uploads bands and
 // layouts, launches the scan in groups that fit the deck budget, and returns
 // every envelope's root rows on the host (standard: nroot floats; truncated:
@@ -272,6 +314,7 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
       pack.insert(pack.end(), plan[k].roff.begin(), plan[k].roff.begin() + nr);
       if (trunc)
         for (int v = 0; v < M; v++) pack.push_back(mode_mask(valid[k], v, tr->fill));
+      append_units(cm->h, b, trunc ? &valid[k] : nullptr, trunc ? tr->fill : 0, pack);
       rat[k - k0] = rtotal;
       rtotal += rper * plan[k].nroot;
     }
@@ -296,6 +339,8 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
       e.hdmax = e.hdmin + nr;
       e.roff = e.hdmax + nr;
       e.mask = trunc ? e.roff + nr : nullptr;
+      e.units = e.roff + nr + (trunc ? M : 0);
+      e.ibase = e.units + M;
       e.alpha = d_alpha + aoff;
       e.nc = (int32_t)plan[k].ncells;
       e.rout = d_rout + rat[k - k0];
@@ -320,6 +365,8 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, c->ev[5], c->ev[6]));
     c->ms[2] += ms;
+    c->kms[trunc ? IXG_K_TRHB_SCAN : IXG_K_HB_SCAN] += ms;
+    c->kn[trunc ? IXG_K_TRHB_SCAN : IXG_K_HB_SCAN] += 1;
     k0 = k1;
   }
   return IXG_OK;
@@ -522,6 +569,7 @@ int ixg_cm_create(ixg_ctx *c, const ixg_cm_params *p, ixg_cm **out) {
   RCHK(configure_cm(*p, cm->h));
   marginal_emissions(*p, cm->h);
   RCHK(state_spans(*p, cm->h));
+  dependency_levels(*p, cm->h);
   cm->h.clen = p->clen;
   flogsum_table(cm->ftbl_host);
   if (!c) {
@@ -533,6 +581,8 @@ int ixg_cm_create(ixg_ctx *c, const ixg_cm_params *p, ixg_cm **out) {
   std::vector<int32_t> i32(cm->h.cfirst);
   i32.insert(i32.end(), cm->h.cnum.begin(), cm->h.cnum.end());
   i32.insert(i32.end(), cm->h.begin_list.begin(), cm->h.begin_list.end());
+  i32.insert(i32.end(), cm->h.lvl_states.begin(), cm->h.lvl_states.end());
+  i32.insert(i32.end(), cm->h.lvl_off.begin(), cm->h.lvl_off.end());
   std::vector<float> f(cm->h.tsc);
   f.insert(f.end(), cm->h.oesc.begin(), cm->h.oesc.end());
   f.insert(f.end(), cm->h.beginsc.begin(), cm->h.beginsc.end());

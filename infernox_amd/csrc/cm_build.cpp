@@ -1,6 +1,7 @@
 // Host-side covariance-model score configuration (once per model): local
 // begins/ends, log-odds transitions and emissions, EL self score. Mirrors
 // cp9.rs:4194-4545 (cm_config_local, cm_logoddsify, cm_configure_scores).
+#include <algorithm>
 #include <cmath>
 #include <cstring>
 #include "cm_internal.h"
@@ -234,6 +235,36 @@ int state_spans(const ixg_cm_params &p, CmHost &h) {
     h.rpos[v] = (nt == MATP_ND || nt == MATR_ND) ? rpos[nd] : rpos[nd] - 1;
   }
   return IXG_OK;
+}
+
+}  // namespace ixg
+
+namespace ixg {
+
+void dependency_levels(const ixg_cm_params &p, CmHost &h) {
+  const int M = p.M;
+  std::vector<int32_t> lvl(M, 0);
+  int nlev = 0;
+  for (int v = M - 1; v >= 0; v--) {
+    const int st = p.sttype[v];
+    int l = 0;
+    if (st == B_ST) {
+      l = 1 + std::max(lvl[p.cfirst[v]], lvl[p.cnum[v]]);
+    } else if (st != E_ST) {
+      for (int k = 0; k < p.cnum[v]; k++) {
+        const int y = p.cfirst[v] + k;
+        if (y != v) l = std::max(l, lvl[y] + 1);
+      }
+    }
+    lvl[v] = l;
+    nlev = std::max(nlev, l + 1);
+  }
+  h.lvl_off.assign(nlev + 1, 0);
+  for (int v = 0; v < M; v++) h.lvl_off[lvl[v] + 1]++;
+  for (int l = 0; l < nlev; l++) h.lvl_off[l + 1] += h.lvl_off[l];
+  h.lvl_states.assign(M, 0);
+  std::vector<int32_t> at(h.lvl_off.begin(), h.lvl_off.end() - 1);
+  for (int v = M - 1; v >= 0; v--) h.lvl_states[at[lvl[v]]++] = v;
 }
 
 }  // namespace ixg

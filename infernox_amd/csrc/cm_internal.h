@@ -21,6 +21,11 @@ struct CmHost {
   std::vector<int32_t> cfirst, cnum;
   std::vector<float> tsc, oesc, beginsc, endsc;  // M*6, M*324, M, M
   std::vector<int32_t> begin_list;               // states y >= 1 with a local begin
+  // dependency levels: a state's level is one more than its children's
+  // highest (self transitions excluded), so the states of one level are
+  // mutually independent; lvl_states lists states level by level, level l
+  // at lvl_off[l] .. lvl_off[l+1]-1
+  std::vector<int32_t> lvl_states, lvl_off;
   // truncated scanning
   int clen = 0;
   std::vector<float> lmesc, rmesc;  // M*18 marginal emission scores
@@ -30,6 +35,7 @@ struct CmHost {
 void marginal_emissions(const ixg_cm_params &p, CmHost &h);
 int state_spans(const ixg_cm_params &p, CmHost &h);
 int configure_cm(const ixg_cm_params &p, CmHost &h);
+void dependency_levels(const ixg_cm_params &p, CmHost &h);
 void flogsum_table(std::vector<float> &t);
 float host_flogsum(const std::vector<float> &t, float a, float b);
 
@@ -40,6 +46,8 @@ struct CmDev {
   const float *lmesc, *rmesc;
   const int32_t *begin_list;
   int nbegin;
+  const int32_t *lvl_states, *lvl_off;
+  int nlev;
   int M;
   int local_begin;
   float el_selfsc;
@@ -61,7 +69,13 @@ struct EnvDev {
   int32_t nc;
   const int32_t *mask;
   float *rout;  // root rows out: standard nroot floats; truncated J|L|R|T|aux, nroot each
+  // per state: work units of one level step (cells; rows of an IL state;
+  // i diagonals of an IR state; 0 to skip) and the first i of an IR state
+  const int32_t *units, *ibase;
 };
+
+constexpr int HB_THREADS = 512;      // lanes per envelope workgroup
+constexpr int HB_LDS_STATES = 3200;  // states whose per-state band scalars fit in LDS
 
 hipError_t launch_hb_scan(const CmDev &cm, const EnvDev *envs, int64_t n, const float *ftbl, int inside,
                           hipStream_t s);

@@ -1,13 +1,14 @@
 // gfx950 HMM-banded CYK / Inside scanners (F6/F7): one 256-lane workgroup per
-// envelope walks the states from M-1 down to 0 (children have larger
-// indices), filling each state's banded deck in parallel before the next
-// state starts. Per cell the children are combined in increasing child order
+// envelope walks the model's dependency levels (a state's level is one more
+// than its children's highest, so the two subtrees of a bifurcation and the
+// match states of a node proceed together), filling the banded decks of all
+// states of a level in parallel before the next level starts. Per cell the children are combined in increasing child order
 // (max for CYK, the FLogsum table for Inside), exactly the order of the
 // reference's scalar loops (cp9.rs:4788-5253, 5514-5885), so CYK is
 // bit-identical and Inside applies the same float operations in the same
 // order. The deck lives in HBM; every cell is written once, by the lane
 // that computes it.
-#include "cm_internal.h"
+#include "hb_sched.h"
 
 namespace ixg {
 
@@ -23,7 +24,7 @@ __device__ __forceinline__ float combine(float cur, float cand, const float *__r
   return mx + ftbl[(int)((mx - mn) * 1000.0f)];
 }
 
-// Value of child y's cell (j - sdr, d - sd) if it lies in y's bands.
+// Value of child y's cell (jy, dy) if it lies in y's bands.
 __device__ __forceinline__ bool child_cell(const EnvDev &e, int y, int jy, int dy, float &val) {
   const int jmn = e.jmin[y];
   if (jy < jmn || jy > e.jmax[y]) return false;
@@ -32,6 +33,24 @@ __device__ __forceinline__ bool child_cell(const EnvDev &e, int y, int jy, int d
   if (dy < lo || dy > e.hdmax[h]) return false;
   val = e.alpha[e.roff[h] + (dy - lo)];
   return true;
+}
+
+// Row jy of state y: cell d at base + d for lo <= d <= hi (lo > hi when jy
+// is outside y's j band). The loads are unconditional (index 0 stands in for
+// an absent row) so that the rows of several children are fetched together.
+struct RowRef {
+  int base, lo, hi;
+};
+__device__ __forceinline__ RowRef row_ref(const EnvDev &e, int y, int jy) {
+  const int jmn = e.jmin[y];
+  const bool in = jy >= jmn && jy <= e.jmax[y];
+  const int h = in ? e.hd_off[y] + (jy - jmn) : 0;
+  const int lo = e.hdmin[h], hi = e.hdmax[h], ro = e.roff[h];
+  RowRef r;
+  r.lo = in ? lo : 1;
+  r.hi = in ? hi : 0;
+  r.base = ro - lo;
+  return r;
 }
 
 template <bool INSIDE>
@@ -45,7 +64,8 @@ __device__ __forceinline__ float cell_init(const CmDev &cm, int v, int st, int d
   return IMPOSSIBLE;
 }
 
-// Non-insert, non-bifurcation states (S, D, ML, MR, MP) and E.
+// Non-insert, non-bifurcation states (S, D, ML, MR, MP) and E: all children's
+// cells are fetched first, then combined in child order.
 template <bool INSIDE>
 __device__ float cell_generic(const CmDev &cm, const EnvDev &e, int v, int st, int j, int d, int dp,
                               const float *__restrict__ ftbl) {
@@ -54,10 +74,19 @@ __device__ float cell_generic(const CmDev &cm, const EnvDev &e, int v, int st, i
   const int sd = sdelta(st), sdr = srdelta(st);
   const int cf = cm.cfirst[v], cn = cm.cnum[v];
   const float *tsc = cm.tsc + v * MAXC;
-  for (int yo = 0; yo < cn; yo++) {
-    float a;
-    if (child_cell(e, cf + yo, j - sdr, d - sd, a)) cur = combine<INSIDE>(cur, a + tsc[yo], ftbl);
+  float a[MAXC];
+  bool ok[MAXC];
+#pragma unroll
+  for (int yo = 0; yo < MAXC; yo++) {
+    const int y = min(cf + yo, cm.M - 1);
+    const RowRef r = row_ref(e, y, j - sdr);
+    const int dy = d - sd;
+    ok[yo] = yo < cn && dy >= r.lo && dy <= r.hi;
+    a[yo] = e.alpha[ok[yo] ? r.base + dy : 0];
   }
+#pragma unroll
+  for (int yo = 0; yo < MAXC; yo++)
+    if (ok[yo]) cur = combine<INSIDE>(cur, a[yo] + tsc[yo], ftbl);
   const float *oe = cm.oesc + (size_t)v * (KP * KP);
   const int i = j - d + 1;
   if (st == ML_ST) cur += oe[e.dsq[i]];
@@ -79,86 +108,166 @@ __device__ float cell_bif(const CmDev &cm, const EnvDev &e, int v, int j, int d,
   const int kx = jpy < e.hdmax[hz] ? jpy : e.hdmax[hz];
   const float *az = e.alpha + e.roff[hz] - hz0;
   const int hy0 = e.hd_off[y];
+#pragma unroll 4
   for (int k = kn; k <= kx; k++) {
     const int hy = hy0 + (jpy - k);
     const int lo = e.hdmin[hy];
-    if (k >= d - e.hdmax[hy] && k <= d - lo)
-      cur = combine<INSIDE>(cur, e.alpha[e.roff[hy] + (d - lo - k)] + az[k], ftbl);
+    const bool ok = k >= d - e.hdmax[hy] && k <= d - lo;
+    const float a = e.alpha[ok ? e.roff[hy] + (d - lo - k) : 0];
+    const float b = az[k];
+    if (ok) cur = combine<INSIDE>(cur, a + b, ftbl);
   }
   return cur;
 }
 
-// IL/IR cell: children in increasing order (the self transition included),
-// then the single emission and the clamp.
+constexpr int ICH = 4;  // cells of an insert chain fetched ahead of their serial combine
+
+// One j row of an IL state: the self transition reads (j, d-1), the previous
+// cell of the same row, so the row is a serial chain over d. The other
+// children's rows are looked up once per row; their cells are fetched ICH
+// cells ahead; the chain itself carries the previous cell in a register.
 template <bool INSIDE>
-__device__ __forceinline__ float cell_insert(const CmDev &cm, const EnvDev &e, int v, int st, int j, int d,
-                                             const float *__restrict__ ftbl) {
-  float cur = cell_init<INSIDE>(cm, v, st, d);
-  const int sdr = st == IR_ST ? 1 : 0;
+__device__ void il_row(const CmDev &cm, const EnvDev &e, int v, int u, const float *__restrict__ ftbl) {
+  const int j = e.jmin[v] + u, h = e.hd_off[v] + u;
+  const int lo = e.hdmin[h], hi = e.hdmax[h];
+  if (hi < lo) return;
+  float *row = e.alpha + e.roff[h] - lo;
   const int cf = cm.cfirst[v], cn = cm.cnum[v];
   const float *tsc = cm.tsc + v * MAXC;
-  for (int yo = 0; yo < cn; yo++) {
-    float a;
-    if (child_cell(e, cf + yo, j - sdr, d - 1, a)) cur = combine<INSIDE>(cur, a + tsc[yo], ftbl);
-  }
   const float *oe = cm.oesc + (size_t)v * (KP * KP);
-  cur += oe[st == IL_ST ? e.dsq[j - d + 1] : e.dsq[j]];
-  return cur < IMPOSSIBLE ? IMPOSSIBLE : cur;
+  RowRef rr[MAXC];
+  float t[MAXC];
+#pragma unroll
+  for (int yo = 0; yo < MAXC; yo++) {
+    rr[yo] = row_ref(e, min(cf + yo, cm.M - 1), j);
+    t[yo] = tsc[yo];
+  }
+  float prev = 0.f;
+  for (int d0 = lo; d0 <= hi; d0 += ICH) {
+    float a[ICH][MAXC];
+    bool ok[ICH][MAXC];
+    int x[ICH];
+#pragma unroll
+    for (int c = 0; c < ICH; c++) {
+      const int d = d0 + c;
+      x[c] = e.dsq[d <= hi ? j - d + 1 : j];
+#pragma unroll
+      for (int yo = 0; yo < MAXC; yo++) {
+        const int y = cf + yo;
+        ok[c][yo] = d <= hi && yo < cn && y != v && d - 1 >= rr[yo].lo && d - 1 <= rr[yo].hi;
+        a[c][yo] = e.alpha[ok[c][yo] ? rr[yo].base + d - 1 : 0];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < ICH; c++) {
+      const int d = d0 + c;
+      if (d > hi) break;
+      float cur = cell_init<INSIDE>(cm, v, IL_ST, d);
+#pragma unroll
+      for (int yo = 0; yo < MAXC; yo++) {
+        if (yo >= cn) break;
+        if (cf + yo == v) {
+          if (d > lo) cur = combine<INSIDE>(cur, prev + t[yo], ftbl);
+        } else if (ok[c][yo]) {
+          cur = combine<INSIDE>(cur, a[c][yo] + t[yo], ftbl);
+        }
+      }
+      cur += oe[x[c]];
+      cur = cur < IMPOSSIBLE ? IMPOSSIBLE : cur;
+      row[d] = cur;
+      prev = cur;
+    }
+  }
+}
+
+// One i diagonal of an IR state: the self transition reads (j-1, d-1), the
+// previous cell of the same diagonal, so the diagonal is a serial chain over
+// j. Cells ahead are located and their other children fetched ICH rows ahead.
+template <bool INSIDE>
+__device__ void ir_diag(const CmDev &cm, const EnvDev &e, int v, int u, const float *__restrict__ ftbl) {
+  const int i = e.ibase[v] + u;
+  const int jmn = e.jmin[v], nj = e.jmax[v] - jmn + 1, ho = e.hd_off[v];
+  const int cf = cm.cfirst[v], cn = cm.cnum[v];
+  const float *tsc = cm.tsc + v * MAXC;
+  const float *oe = cm.oesc + (size_t)v * (KP * KP);
+  float t[MAXC];
+#pragma unroll
+  for (int yo = 0; yo < MAXC; yo++) t[yo] = tsc[yo];
+  float prev = 0.f;
+  bool have_prev = false;
+  for (int jp0 = 0; jp0 < nj; jp0 += ICH) {
+    float a[ICH][MAXC];
+    bool ok[ICH][MAXC], on[ICH];
+    int cell[ICH], x[ICH];
+#pragma unroll
+    for (int c = 0; c < ICH; c++) {
+      const int jp = jp0 + c, j = jmn + jp, d = j - i + 1;
+      const int h = ho + (jp < nj ? jp : 0);
+      const int lo = e.hdmin[h], hi = e.hdmax[h];
+      on[c] = jp < nj && d >= lo && d <= hi;
+      cell[c] = e.roff[h] + (d - lo);
+      x[c] = e.dsq[on[c] ? j : i];
+#pragma unroll
+      for (int yo = 0; yo < MAXC; yo++) {
+        const int y = cf + yo;
+        const RowRef r = row_ref(e, min(y, cm.M - 1), j - 1);
+        ok[c][yo] = on[c] && yo < cn && y != v && d - 1 >= r.lo && d - 1 <= r.hi;
+        a[c][yo] = e.alpha[ok[c][yo] ? r.base + d - 1 : 0];
+      }
+    }
+#pragma unroll
+    for (int c = 0; c < ICH; c++) {
+      if (!on[c]) {
+        have_prev = false;
+        continue;
+      }
+      const int d = jmn + jp0 + c - i + 1;
+      float cur = cell_init<INSIDE>(cm, v, IR_ST, d);
+#pragma unroll
+      for (int yo = 0; yo < MAXC; yo++) {
+        if (yo >= cn) break;
+        if (cf + yo == v) {
+          if (have_prev) cur = combine<INSIDE>(cur, prev + t[yo], ftbl);
+        } else if (ok[c][yo]) {
+          cur = combine<INSIDE>(cur, a[c][yo] + t[yo], ftbl);
+        }
+      }
+      cur += oe[x[c]];
+      cur = cur < IMPOSSIBLE ? IMPOSSIBLE : cur;
+      e.alpha[cell[c]] = cur;
+      prev = cur;
+      have_prev = true;
+    }
+  }
 }
 
 template <bool INSIDE>
-__global__ __launch_bounds__(256) void hb_scan_kernel(CmDev cm, const EnvDev *envs, const float *ftbl) {
-  const EnvDev e = envs[blockIdx.x];
+__global__ __launch_bounds__(HB_THREADS) void hb_scan_kernel(CmDev cm, const EnvDev *envs, const float *ftbl) {
+  extern __shared__ int32_t smem[];
+  EnvDev e = envs[blockIdx.x];
   const int tid = threadIdx.x, nt = blockDim.x;
-  for (int v = cm.M - 1; v >= 0; v--) {
+  hb_stage_scalars(e, cm.M, smem);
+  auto unit = [&](int v, int u) {
     const int st = cm.sttype[v];
-    const int jmn = e.jmin[v], jmx = e.jmax[v];
-    const int nj = jmx >= jmn ? jmx - jmn + 1 : 0;
-    if (nj > 0) {
-      const int ho = e.hd_off[v];
-      const int32_t *hmn = e.hdmin + ho, *hmx = e.hdmax + ho, *ro = e.roff + ho;
-      if (st == IL_ST) {
-        // self transition reads (j, d-1): serial over d within a row
-        for (int jp = tid; jp < nj; jp += nt) {
-          const int j = jmn + jp;
-          float *row = e.alpha + ro[jp] - hmn[jp];
-          for (int d = hmn[jp]; d <= hmx[jp]; d++) row[d] = cell_insert<INSIDE>(cm, e, v, st, j, d, ftbl);
-        }
-      } else if (st == IR_ST) {
-        // self transition reads (j-1, d-1), the same i: serial along each i
-        int imin = 0x7FFFFFFF, imax = -0x7FFFFFFF;
-        for (int jp = 0; jp < nj; jp++)
-          if (hmx[jp] >= hmn[jp]) {
-            const int j = jmn + jp;
-            imin = min(imin, j - hmx[jp] + 1);
-            imax = max(imax, j - hmn[jp] + 1);
-          }
-        for (int i = imin + tid; i <= imax; i += nt)
-          for (int jp = 0; jp < nj; jp++) {
-            const int j = jmn + jp, d = j - i + 1;
-            if (d < hmn[jp] || d > hmx[jp]) continue;
-            e.alpha[ro[jp] + (d - hmn[jp])] = cell_insert<INSIDE>(cm, e, v, st, j, d, ftbl);
-          }
-      } else {
-        const int last = nj - 1;
-        const int wl = hmx[last] - hmn[last] + 1;
-        const int base = ro[0];
-        const int ncv = ro[last] + (wl > 0 ? wl : 0) - base;
-        for (int c = tid; c < ncv; c += nt) {
-          const int tgt = base + c;
-          int lo = 0, hi = last;  // last row whose first cell is at or before tgt
-          while (lo < hi) {
-            const int mid = (lo + hi + 1) >> 1;
-            if (ro[mid] <= tgt) lo = mid;
-            else hi = mid - 1;
-          }
-          const int dp = tgt - ro[lo];
-          const int d = hmn[lo] + dp, j = jmn + lo;
-          e.alpha[tgt] = st == B_ST ? cell_bif<INSIDE>(cm, e, v, j, d, ftbl)
-                                    : cell_generic<INSIDE>(cm, e, v, st, j, d, dp, ftbl);
-        }
-      }
+    const int jmn = e.jmin[v], nj = e.jmax[v] - jmn + 1;
+    const int ho = e.hd_off[v];
+    const int32_t *hmn = e.hdmin + ho, *hmx = e.hdmax + ho, *ro = e.roff + ho;
+    if (st == IL_ST) {
+      il_row<INSIDE>(cm, e, v, u, ftbl);
+    } else if (st == IR_ST) {
+      ir_diag<INSIDE>(cm, e, v, u, ftbl);
+    } else {
+      const int tgt = ro[0] + u;
+      const int r = hb_row_of(ro, nj - 1, tgt);
+      const int dp = tgt - ro[r];
+      const int d = hmn[r] + dp, j = jmn + r;
+      e.alpha[tgt] = st == B_ST ? cell_bif<INSIDE>(cm, e, v, j, d, ftbl)
+                                : cell_generic<INSIDE>(cm, e, v, st, j, d, dp, ftbl);
     }
+  };
+  // states level by level: the states of one level read only lower levels
+  for (int l = 0; l < cm.nlev; l++) {
+    hb_level_units(cm, e, l, unit);
     __syncthreads();
   }
   // local begins into the root deck by MAX (cp9.rs:5264-5304, 5908-5937);
@@ -172,12 +281,7 @@ __global__ __launch_bounds__(256) void hb_scan_kernel(CmDev cm, const EnvDev *en
   const int ncv = ro[last] + (wl > 0 ? wl : 0) - ro[0];
   for (int c = tid; c < ncv; c += nt) {
     const int tgt = ro[0] + c;
-    int lo = 0, hi = last;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (ro[mid] <= tgt) lo = mid;
-      else hi = mid - 1;
-    }
+    const int lo = hb_row_of(ro, last, tgt);
     const int d = hmn[lo] + (tgt - ro[lo]), j = jmn + lo;
     float cur = e.alpha[tgt];
     if (cm.local_begin)
@@ -197,8 +301,11 @@ __global__ __launch_bounds__(256) void hb_scan_kernel(CmDev cm, const EnvDev *en
 hipError_t launch_hb_scan(const CmDev &cm, const EnvDev *envs, int64_t n, const float *ftbl, int inside,
                           hipStream_t s) {
   if (n == 0) return hipSuccess;
-  if (inside) hipLaunchKernelGGL(hb_scan_kernel<true>, dim3((unsigned)n), dim3(256), 0, s, cm, envs, ftbl);
-  else hipLaunchKernelGGL(hb_scan_kernel<false>, dim3((unsigned)n), dim3(256), 0, s, cm, envs, ftbl);
+  const size_t lds = hb_lds_bytes(cm.M);
+  if (inside)
+    hipLaunchKernelGGL(hb_scan_kernel<true>, dim3((unsigned)n), dim3(HB_THREADS), lds, s, cm, envs, ftbl);
+  else
+    hipLaunchKernelGGL(hb_scan_kernel<false>, dim3((unsigned)n), dim3(HB_THREADS), lds, s, cm, envs, ftbl);
   return hipGetLastError();
 }
 

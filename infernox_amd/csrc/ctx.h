@@ -46,4 +46,25 @@ struct ixg_ctx {
   hipEvent_t ev[8];
   double ms[5] = {0, 0, 0, 0, 0};
   int64_t cells[5] = {0, 0, 0, 0, 0};
+  // per-kernel device time (HIP events around each launch), cumulative:
+  // IXG_K_* index; kev is the event pair used for it
+  hipEvent_t kev[2];
+  double kms[IXG_NKERNELS] = {};
+  int64_t kn[IXG_NKERNELS] = {};
 };
+
+namespace ixg {
+// Records the start event of a timed launch on the context's stream.
+inline hipError_t ktime_begin(ixg_ctx *c) { return hipEventRecord(c->kev[0], c->stream); }
+// Records the end event, waits for it and adds the launch to kernel k.
+inline hipError_t ktime_end(ixg_ctx *c, int k) {
+  hipError_t e = hipEventRecord(c->kev[1], c->stream);
+  if (e != hipSuccess) return e;
+  if ((e = hipEventSynchronize(c->kev[1])) != hipSuccess) return e;
+  float ms = 0;
+  if ((e = hipEventElapsedTime(&ms, c->kev[0], c->kev[1])) != hipSuccess) return e;
+  c->kms[k] += ms;
+  c->kn[k] += 1;
+  return hipSuccess;
+}
+}  // namespace ixg

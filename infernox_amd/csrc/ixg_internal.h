@@ -117,5 +117,6 @@ hipError_t launch_vit(const VitDev &p, const uint8_t *seq, const WinDev *wins, i
 hipError_t launch_msv(const SsvDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin,
                       float *sc, uint8_t *ovf, hipStream_t s);
 hipError_t launch_revcomp(const uint8_t *fwd, uint8_t *rc, int64_t L, hipStream_t s);
+hipError_t launch_copy_probe(const void *a, void *b, int64_t n16, hipStream_t s);
 
 }  // namespace ixg

@@ -527,6 +527,17 @@ hipError_t launch_msv(const SsvDev &p, const uint8_t *seq, const WinDev *wins, i
   return hipGetLastError();
 }
 
+// Plain device-to-device copy, 16 bytes per lane per step (HBM calibration).
+__global__ void copy_probe_kernel(const float4 *__restrict__ a, float4 *__restrict__ b, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) b[i] = a[i];
+}
+
+hipError_t launch_copy_probe(const void *a, void *b, int64_t n16, hipStream_t s) {
+  hipLaunchKernelGGL(copy_probe_kernel, dim3(8192), dim3(256), 0, s, (const float4 *)a, (float4 *)b, n16);
+  return hipGetLastError();
+}
+
 hipError_t launch_revcomp(const uint8_t *fwd, uint8_t *rc, int64_t L, hipStream_t s) {
   const int64_t n = L + 2;
   hipLaunchKernelGGL(revcomp_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fwd, rc, L);

@@ -1,15 +1,16 @@
 // gfx950 truncated HMM-banded scanners, TrCYK and FTrInside: four marginal
 // planes (J, L, R and, for bifurcations and the root, T) over the same banded
 // deck layout as the standard scanners. One 256-lane workgroup per envelope
-// walks the states from M-1 down to 1; within a state every cell is computed
-// by one lane, all of its valid planes at once. Per cell and plane the
+// walks the model's dependency levels (the states of a level read only lower
+// levels); every cell of a level's states is computed by one lane, all of its
+// valid planes at once. Per cell and plane the
 // children are combined in increasing child order, and split points in
 // increasing k, which is the order in which the reference's state loops
 // reach that cell (cm_dpsearch_trunc.rs:354-880, 1140-1640), so TrCYK is
 // bit-identical and FTrInside applies the same FLogsum sequence. The root
 // takes truncated begins by MAX with the per-cell best entry state and mode
 // (cm_dpsearch_trunc.rs:882-987); reporting runs on the host.
-#include "cm_internal.h"
+#include "hb_sched.h"
 
 namespace ixg {
 
@@ -221,63 +222,42 @@ __device__ void tr_cell_bif(const CmDev &cm, const EnvDev &e, int v, int j, int 
   if (dT) A[3 * nc + c] = T;
 }
 
-// Row of the flat cell index tgt among rows [0, last] of one state.
-__device__ __forceinline__ int row_of(const int32_t *ro, int last, int tgt) {
-  int lo = 0, hi = last;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (ro[mid] <= tgt) lo = mid;
-    else hi = mid - 1;
-  }
-  return lo;
-}
-
 template <bool INSIDE>
-__global__ __launch_bounds__(256) void trhb_scan_kernel(CmDev cm, const EnvDev *envs, const float *ftbl,
-                                                        const float *pty, const int32_t *ylist, int ny) {
-  const EnvDev e = envs[blockIdx.x];
+__global__ __launch_bounds__(HB_THREADS) void trhb_scan_kernel(CmDev cm, const EnvDev *envs, const float *ftbl,
+                                                               const float *pty, const int32_t *ylist, int ny) {
+  extern __shared__ int32_t smem[];
+  EnvDev e = envs[blockIdx.x];
   const int tid = threadIdx.x, nt = blockDim.x;
-  for (int v = cm.M - 1; v >= 1; v--) {
+  hb_stage_scalars(e, cm.M, smem);
+  auto unit = [&](int v, int u) {
     const int st = cm.sttype[v];
-    const int jmn = e.jmin[v], jmx = e.jmax[v];
-    const int nj = jmx >= jmn ? jmx - jmn + 1 : 0;
-    if (nj > 0 && (e.mask[v] & (BJ | BL | BR | BT))) {
-      const int ho = e.hd_off[v];
-      const int32_t *hmn = e.hdmin + ho, *hmx = e.hdmax + ho, *ro = e.roff + ho;
-      if (st == IL_ST) {
-        // J and L read their own (j, d-1): serial over d within a row
-        for (int jp = tid; jp < nj; jp += nt)
-          for (int d = hmn[jp]; d <= hmx[jp]; d++)
-            tr_cell<INSIDE>(cm, e, v, st, jmn + jp, d, d - hmn[jp], ro[jp] + (d - hmn[jp]), ftbl);
-      } else if (st == IR_ST) {
-        // J and R read their own (j-1, d-1), the same i: serial along each i
-        int imin = 0x7FFFFFFF, imax = -0x7FFFFFFF;
-        for (int jp = 0; jp < nj; jp++)
-          if (hmx[jp] >= hmn[jp]) {
-            imin = min(imin, jmn + jp - hmx[jp] + 1);
-            imax = max(imax, jmn + jp - hmn[jp] + 1);
-          }
-        for (int i = imin + tid; i <= imax; i += nt)
-          for (int jp = 0; jp < nj; jp++) {
-            const int j = jmn + jp, d = j - i + 1;
-            if (d < hmn[jp] || d > hmx[jp]) continue;
-            tr_cell<INSIDE>(cm, e, v, st, j, d, d - hmn[jp], ro[jp] + (d - hmn[jp]), ftbl);
-          }
-      } else {
-        const int last = nj - 1;
-        const int wl = hmx[last] - hmn[last] + 1;
-        const int base = ro[0];
-        const int ncv = ro[last] + (wl > 0 ? wl : 0) - base;
-        for (int c = tid; c < ncv; c += nt) {
-          const int tgt = base + c;
-          const int r = row_of(ro, last, tgt);
-          const int dp = tgt - ro[r];
-          const int d = hmn[r] + dp, j = jmn + r;
-          if (st == B_ST) tr_cell_bif<INSIDE>(cm, e, v, j, d, tgt, ftbl);
-          else tr_cell<INSIDE>(cm, e, v, st, j, d, dp, tgt, ftbl);
-        }
+    const int jmn = e.jmin[v], nj = e.jmax[v] - jmn + 1;
+    const int ho = e.hd_off[v];
+    const int32_t *hmn = e.hdmin + ho, *hmx = e.hdmax + ho, *ro = e.roff + ho;
+    if (st == IL_ST) {
+      // J and L read their own (j, d-1): serial over d within row u
+      for (int d = hmn[u]; d <= hmx[u]; d++)
+        tr_cell<INSIDE>(cm, e, v, st, jmn + u, d, d - hmn[u], ro[u] + (d - hmn[u]), ftbl);
+    } else if (st == IR_ST) {
+      // J and R read their own (j-1, d-1), the same i: serial along diagonal i
+      const int i = e.ibase[v] + u;
+      for (int jp = 0; jp < nj; jp++) {
+        const int j = jmn + jp, d = j - i + 1;
+        if (d < hmn[jp] || d > hmx[jp]) continue;
+        tr_cell<INSIDE>(cm, e, v, st, j, d, d - hmn[jp], ro[jp] + (d - hmn[jp]), ftbl);
       }
+    } else {
+      const int tgt = ro[0] + u;
+      const int r = hb_row_of(ro, nj - 1, tgt);
+      const int dp = tgt - ro[r];
+      const int d = hmn[r] + dp, j = jmn + r;
+      if (st == B_ST) tr_cell_bif<INSIDE>(cm, e, v, j, d, tgt, ftbl);
+      else tr_cell<INSIDE>(cm, e, v, st, j, d, dp, tgt, ftbl);
     }
+  };
+  // states level by level (the root, with no units, takes truncated begins below)
+  for (int l = 0; l < cm.nlev; l++) {
+    hb_level_units(cm, e, l, unit);
     __syncthreads();
   }
   // root: truncated begins by MAX, per cell best (entry state, mode)
@@ -292,7 +272,7 @@ __global__ __launch_bounds__(256) void trhb_scan_kernel(CmDev cm, const EnvDev *
   const float *A = e.alpha;
   for (int c = tid; c < ncv; c += nt) {
     const int tgt = ro[0] + c;
-    const int r = row_of(ro, last, tgt);
+    const int r = hb_row_of(ro, last, tgt);
     const int d = hmn[r] + (tgt - ro[r]), j = jmn + r;
     float J = IMPOSSIBLE, L = IMPOSSIBLE, R = IMPOSSIBLE, T = IMPOSSIBLE, best = IMPOSSIBLE;
     int bm = MODE_UNKNOWN, br = 0;
@@ -344,10 +324,13 @@ __global__ __launch_bounds__(256) void trhb_scan_kernel(CmDev cm, const EnvDev *
 hipError_t launch_trhb_scan(const CmDev &cm, const EnvDev *envs, int64_t n, const float *ftbl, const float *pty,
                             const int32_t *ylist, int ny, int inside, hipStream_t s) {
   if (n == 0) return hipSuccess;
+  const size_t lds = hb_lds_bytes(cm.M);
   if (inside)
-    hipLaunchKernelGGL(trhb_scan_kernel<true>, dim3((unsigned)n), dim3(256), 0, s, cm, envs, ftbl, pty, ylist, ny);
+    hipLaunchKernelGGL(trhb_scan_kernel<true>, dim3((unsigned)n), dim3(HB_THREADS), lds, s, cm, envs, ftbl, pty,
+                       ylist, ny);
   else
-    hipLaunchKernelGGL(trhb_scan_kernel<false>, dim3((unsigned)n), dim3(256), 0, s, cm, envs, ftbl, pty, ylist, ny);
+    hipLaunchKernelGGL(trhb_scan_kernel<false>, dim3((unsigned)n), dim3(HB_THREADS), lds, s, cm, envs, ftbl, pty,
+                       ylist, ny);
   return hipGetLastError();
 }
 

@@ -57,8 +57,10 @@ EXPORTS = [
     "ixg_f3_filter_sequence", "ixg_target_create", "ixg_target_free", "ixg_window_tasks",
     "ixg_f3_filter_tasks", "ixg_last_timings", "ixg_cm_create", "ixg_cm_free", "ixg_cm_tables",
     "ixg_cyk_scan_hb", "ixg_inside_scan_hb", "ixg_hb_scan_batch", "ixg_cm_set_trpenalties", "ixg_cm_trtables",
-    "ixg_trcyk_scan_hb", "ixg_trinside_scan_hb", "ixg_trhb_scan_batch",
+    "ixg_trcyk_scan_hb", "ixg_trinside_scan_hb", "ixg_trhb_scan_batch", "ixg_kernel_timings",
+    "ixg_copy_probe",
 ]
+KERNELS = ["ssv_sweep", "ssv_resolve", "fwd", "bias", "vit", "msv", "hb_scan", "trhb_scan", "copy_probe"]
 
 
 class CmParams(C.Structure):
@@ -130,6 +132,8 @@ def lib():
         L.ixg_f3_filter_tasks.argtypes = [vp, vp, vp, C.POINTER(Task), i64, C.POINTER(F3Opts), C.POINTER(Window),
                                           i64, i64p, C.POINTER(Acct)]
         L.ixg_last_timings.argtypes = [vp, C.POINTER(C.c_double), i64p]
+        L.ixg_kernel_timings.argtypes = [vp, C.POINTER(C.c_double), i64p, C.c_int]
+        L.ixg_copy_probe.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
         L.ixg_cm_create.argtypes = [vp, C.POINTER(CmParams), C.POINTER(vp)]
         L.ixg_cm_free.argtypes = [vp, vp]
         L.ixg_cm_tables.argtypes = [vp, f32p, f32p, f32p, f32p, f32p]
@@ -199,6 +203,19 @@ class Context:
         cells = (C.c_int64 * 5)()
         check(lib().ixg_last_timings(self.h, ms, cells), "timings")
         return list(ms), list(cells)
+
+    def copy_probe(self, nbytes, reps=3):
+        """Mean milliseconds of a plain device copy of nbytes."""
+        ms = C.c_double()
+        check(lib().ixg_copy_probe(self.h, int(nbytes), int(reps), C.byref(ms)), "copy probe")
+        return ms.value
+
+    def kernel_timings(self, reset=False):
+        """{kernel: (total ms, launches)} since the last reset."""
+        ms = (C.c_double * len(KERNELS))()
+        n = (C.c_int64 * len(KERNELS))()
+        check(lib().ixg_kernel_timings(self.h, ms, n, int(reset)), "kernel timings")
+        return {k: (ms[i], n[i]) for i, k in enumerate(KERNELS)}
 
 
 def p7_struct(p7):
