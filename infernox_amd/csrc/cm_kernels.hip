@@ -18,10 +18,13 @@ __device__ __forceinline__ int srdelta(int st) { return (st == MP_ST || st == MR
 template <bool INSIDE>
 __device__ __forceinline__ float combine(float cur, float cand, const float *__restrict__ ftbl) {
   if (!INSIDE) return cand > cur ? cand : cur;
-  // FLogsum (cp9.rs:5405-5413)
+  // FLogsum (cp9.rs:5405-5413), branch-free: the table is read at index 0
+  // when the smaller term is -inf or 23 or more below the larger
   const float mx = cur > cand ? cur : cand, mn = cur < cand ? cur : cand;
-  if (mn == -INFINITY || (mx - mn) >= 23.0f) return mx;
-  return mx + ftbl[(int)((mx - mn) * 1000.0f)];
+  const float diff = mx - mn;
+  const bool near = !(mn == -INFINITY) && diff < 23.0f;
+  const float tv = ftbl[near ? (int)(diff * 1000.0f) : 0];
+  return near ? mx + tv : mx;
 }
 
 // Value of child y's cell (jy, dy) if it lies in y's bands.
@@ -33,24 +36,6 @@ __device__ __forceinline__ bool child_cell(const EnvDev &e, int y, int jy, int d
   if (dy < lo || dy > e.hdmax[h]) return false;
   val = e.alpha[e.roff[h] + (dy - lo)];
   return true;
-}
-
-// Row jy of state y: cell d at base + d for lo <= d <= hi (lo > hi when jy
-// is outside y's j band). The loads are unconditional (index 0 stands in for
-// an absent row) so that the rows of several children are fetched together.
-struct RowRef {
-  int base, lo, hi;
-};
-__device__ __forceinline__ RowRef row_ref(const EnvDev &e, int y, int jy) {
-  const int jmn = e.jmin[y];
-  const bool in = jy >= jmn && jy <= e.jmax[y];
-  const int h = in ? e.hd_off[y] + (jy - jmn) : 0;
-  const int lo = e.hdmin[h], hi = e.hdmax[h], ro = e.roff[h];
-  RowRef r;
-  r.lo = in ? lo : 1;
-  r.hi = in ? hi : 0;
-  r.base = ro - lo;
-  return r;
 }
 
 template <bool INSIDE>

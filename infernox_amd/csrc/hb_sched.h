@@ -47,6 +47,24 @@ __device__ __forceinline__ void hb_level_units(const CmDev &cm, const EnvDev &e,
   }
 }
 
+// Row jy of state y: cell d at base + d for lo <= d <= hi (lo > hi when jy
+// is outside y's j band). The loads are unconditional (index 0 stands in for
+// an absent row) so that the rows of several children are fetched together.
+struct RowRef {
+  int base, lo, hi;
+};
+__device__ __forceinline__ RowRef row_ref(const EnvDev &e, int y, int jy) {
+  const int jmn = e.jmin[y];
+  const bool in = jy >= jmn && jy <= e.jmax[y];
+  const int h = in ? e.hd_off[y] + (jy - jmn) : 0;
+  const int lo = e.hdmin[h], hi = e.hdmax[h], ro = e.roff[h];
+  RowRef r;
+  r.lo = in ? lo : 1;
+  r.hi = in ? hi : 0;
+  r.base = ro - lo;
+  return r;
+}
+
 // Row of the flat cell index tgt among rows [0, last] of one state.
 __device__ __forceinline__ int hb_row_of(const int32_t *ro, int last, int tgt) {
   int lo = 0, hi = last;

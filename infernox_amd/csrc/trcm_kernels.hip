@@ -26,9 +26,13 @@ __device__ __forceinline__ int t_sdr(int st) { return (st == MP_ST || st == MR_S
 template <bool INSIDE>
 __device__ __forceinline__ float tcomb(float cur, float cand, const float *__restrict__ ftbl) {
   if (!INSIDE) return cand > cur ? cand : cur;
-  const float mx = cur > cand ? cur : cand, mn = cur < cand ? cur : cand;  // FLogsum, cp9.rs:5405-5413
-  if (mn == -INFINITY || (mx - mn) >= 23.0f) return mx;
-  return mx + ftbl[(int)((mx - mn) * 1000.0f)];
+  // FLogsum (cp9.rs:5405-5413), branch-free: the table is read at index 0
+  // when the smaller term is -inf or 23 or more below the larger
+  const float mx = cur > cand ? cur : cand, mn = cur < cand ? cur : cand;
+  const float diff = mx - mn;
+  const bool near = !(mn == -INFINITY) && diff < 23.0f;
+  const float tv = ftbl[near ? (int)(diff * 1000.0f) : 0];
+  return near ? mx + tv : mx;
 }
 
 __device__ __forceinline__ float clampi(float x) { return x < IMPOSSIBLE ? IMPOSSIBLE : x; }
@@ -51,7 +55,11 @@ __device__ __forceinline__ float tr_init(const CmDev &cm, int v, int d, int sdx)
   return IMPOSSIBLE;
 }
 
-// Cell (j, d) of a non-bifurcation state v: J, L and R planes.
+__device__ __forceinline__ bool in_row(const RowRef &r, int d) { return d >= r.lo && d <= r.hi; }
+
+// Cell (j, d) of a non-insert, non-bifurcation state v (E, ML, MR, MP, D,
+// S): J, L and R planes. Every child's cells are fetched first (rows j and,
+// for right emitters, j-1), then combined per plane in child order.
 template <bool INSIDE>
 __device__ void tr_cell(const CmDev &cm, const EnvDev &e, int v, int st, int j, int d, int dp, int c,
                         const float *__restrict__ ftbl) {
@@ -65,75 +73,113 @@ __device__ void tr_cell(const CmDev &cm, const EnvDev &e, int v, int st, int j, 
   const float *oe = cm.oesc + (size_t)v * (KP * KP);
   if (st == E_ST) {
     if (dp == 0) J = L = R = 0.0f;
-  } else if (st == ML_ST || st == IL_ST) {
+  } else if (st == ML_ST) {
+    // J, L from (j, d-1); R from (j, d)
+    float a1[MAXC], l1[MAXC], a0[MAXC], r0[MAXC];
+    int my[MAXC];
+    bool k1[MAXC], k0[MAXC];
+#pragma unroll
+    for (int yo = 0; yo < MAXC; yo++) {
+      const int y = min(cf + yo, cm.M - 1);
+      my[yo] = yo < cn ? e.mask[y] : 0;
+      const RowRef r = row_ref(e, y, j);
+      k1[yo] = in_row(r, d - 1);
+      k0[yo] = in_row(r, d);
+      const int i1 = k1[yo] ? r.base + d - 1 : 0, i0 = k0[yo] ? r.base + d : 0;
+      a1[yo] = A[i1];
+      l1[yo] = (dL && (my[yo] & BL)) ? A[nc + i1] : 0.f;
+      a0[yo] = A[i0];
+      r0[yo] = (dR && (my[yo] & BR)) ? A[2 * nc + i0] : 0.f;
+    }
+    const float em = oe[e.dsq[j - d + 1]];
     if (dJ || dL) {
-      for (int yo = 0; yo < cn; yo++) {
-        const int y = cf + yo, my = e.mask[y];
-        if (!(my & (BJ | BL))) continue;
-        const int cy = cell_at(e, y, j, d - 1);
-        if (cy < 0) continue;
-        if (dJ && (my & BJ)) J = tcomb<INSIDE>(J, A[cy] + tsc[yo], ftbl);
-        if (dL && (my & BL)) L = tcomb<INSIDE>(L, A[nc + cy] + tsc[yo], ftbl);
+#pragma unroll
+      for (int yo = 0; yo < MAXC; yo++) {
+        if (!(my[yo] & (BJ | BL)) || !k1[yo]) continue;
+        if (dJ && (my[yo] & BJ)) J = tcomb<INSIDE>(J, a1[yo] + tsc[yo], ftbl);
+        if (dL && (my[yo] & BL)) L = tcomb<INSIDE>(L, l1[yo] + tsc[yo], ftbl);
       }
-      const float em = oe[e.dsq[j - d + 1]];
       J = clampi(J + em);
       L = clampi(d >= 2 ? L + em : em);
     }
     if (dR) {
-      for (int yo = 0; yo < cn; yo++) {
-        const int y = cf + yo, my = e.mask[y];
-        if (y == v || !(my & (BJ | BR))) continue;
-        const int cy = cell_at(e, y, j, d);
-        if (cy < 0) continue;
-        if (my & BJ) R = tcomb<INSIDE>(R, A[cy] + tsc[yo], ftbl);
-        if (my & BR) R = tcomb<INSIDE>(R, A[2 * nc + cy] + tsc[yo], ftbl);
+#pragma unroll
+      for (int yo = 0; yo < MAXC; yo++) {
+        if (!(my[yo] & (BJ | BR)) || !k0[yo]) continue;
+        if (my[yo] & BJ) R = tcomb<INSIDE>(R, a0[yo] + tsc[yo], ftbl);
+        if (my[yo] & BR) R = tcomb<INSIDE>(R, r0[yo] + tsc[yo], ftbl);
       }
     }
-  } else if (st == MR_ST || st == IR_ST) {
+  } else if (st == MR_ST) {
+    // J, R from (j-1, d-1); L from (j, d)
+    float a1[MAXC], r1[MAXC], a0[MAXC], l0[MAXC];
+    int my[MAXC];
+    bool k1[MAXC], k0[MAXC];
+#pragma unroll
+    for (int yo = 0; yo < MAXC; yo++) {
+      const int y = min(cf + yo, cm.M - 1);
+      my[yo] = yo < cn ? e.mask[y] : 0;
+      const RowRef rp = row_ref(e, y, j - 1), rq = row_ref(e, y, j);
+      k1[yo] = in_row(rp, d - 1);
+      k0[yo] = in_row(rq, d);
+      const int i1 = k1[yo] ? rp.base + d - 1 : 0, i0 = k0[yo] ? rq.base + d : 0;
+      a1[yo] = A[i1];
+      r1[yo] = (dR && (my[yo] & BR)) ? A[2 * nc + i1] : 0.f;
+      a0[yo] = A[i0];
+      l0[yo] = (dL && (my[yo] & BL)) ? A[nc + i0] : 0.f;
+    }
+    const float em = oe[e.dsq[j]];
     if (dJ || dR) {
-      for (int yo = 0; yo < cn; yo++) {
-        const int y = cf + yo, my = e.mask[y];
-        if (!(my & (BJ | BR))) continue;
-        const int cy = cell_at(e, y, j - 1, d - 1);
-        if (cy < 0) continue;
-        if (dJ && (my & BJ)) J = tcomb<INSIDE>(J, A[cy] + tsc[yo], ftbl);
-        if (dR && (my & BR)) R = tcomb<INSIDE>(R, A[2 * nc + cy] + tsc[yo], ftbl);
+#pragma unroll
+      for (int yo = 0; yo < MAXC; yo++) {
+        if (!(my[yo] & (BJ | BR)) || !k1[yo]) continue;
+        if (dJ && (my[yo] & BJ)) J = tcomb<INSIDE>(J, a1[yo] + tsc[yo], ftbl);
+        if (dR && (my[yo] & BR)) R = tcomb<INSIDE>(R, r1[yo] + tsc[yo], ftbl);
       }
-      const float em = oe[e.dsq[j]];
       J = clampi(J + em);
       R = clampi(d >= 2 ? R + em : em);
     }
     if (dL) {
-      for (int yo = 0; yo < cn; yo++) {
-        const int y = cf + yo, my = e.mask[y];
-        if (y == v || !(my & (BJ | BL))) continue;
-        const int cy = cell_at(e, y, j, d);
-        if (cy < 0) continue;
-        if (my & BJ) L = tcomb<INSIDE>(L, A[cy] + tsc[yo], ftbl);
-        if (my & BL) L = tcomb<INSIDE>(L, A[nc + cy] + tsc[yo], ftbl);
+#pragma unroll
+      for (int yo = 0; yo < MAXC; yo++) {
+        if (!(my[yo] & (BJ | BL)) || !k0[yo]) continue;
+        if (my[yo] & BJ) L = tcomb<INSIDE>(L, a0[yo] + tsc[yo], ftbl);
+        if (my[yo] & BL) L = tcomb<INSIDE>(L, l0[yo] + tsc[yo], ftbl);
       }
     }
   } else if (st == MP_ST) {
-    for (int yo = 0; yo < cn; yo++) {
-      const int y = cf + yo, my = e.mask[y];
+    // J from (j-1, d-2); R from (j-1, d-1); L from (j, d-1)
+    float a2[MAXC], a1[MAXC], r1[MAXC], b1[MAXC], l1[MAXC];
+    int my[MAXC];
+    bool k2[MAXC], k1[MAXC], kq[MAXC];
+#pragma unroll
+    for (int yo = 0; yo < MAXC; yo++) {
+      const int y = min(cf + yo, cm.M - 1);
+      my[yo] = yo < cn ? e.mask[y] : 0;
+      const RowRef rp = row_ref(e, y, j - 1), rq = row_ref(e, y, j);
+      k2[yo] = in_row(rp, d - 2);
+      k1[yo] = in_row(rp, d - 1);
+      kq[yo] = in_row(rq, d - 1);
+      const int i2 = k2[yo] ? rp.base + d - 2 : 0, i1 = k1[yo] ? rp.base + d - 1 : 0;
+      const int iq = kq[yo] ? rq.base + d - 1 : 0;
+      a2[yo] = A[i2];
+      a1[yo] = A[i1];
+      r1[yo] = (dR && (my[yo] & BR)) ? A[2 * nc + i1] : 0.f;
+      b1[yo] = A[iq];
+      l1[yo] = (dL && (my[yo] & BL)) ? A[nc + iq] : 0.f;
+    }
+#pragma unroll
+    for (int yo = 0; yo < MAXC; yo++) {
+      const int my_ = my[yo];
       const float t = tsc[yo];
-      if (dJ && (my & BJ)) {
-        const int cy = cell_at(e, y, j - 1, d - 2);
-        if (cy >= 0) J = tcomb<INSIDE>(J, A[cy] + t, ftbl);
+      if (dJ && (my_ & BJ) && k2[yo]) J = tcomb<INSIDE>(J, a2[yo] + t, ftbl);
+      if (dR && (my_ & (BJ | BR)) && k1[yo]) {
+        if (my_ & BJ) R = tcomb<INSIDE>(R, a1[yo] + t, ftbl);
+        if (my_ & BR) R = tcomb<INSIDE>(R, r1[yo] + t, ftbl);
       }
-      if (dR && (my & (BJ | BR))) {
-        const int cy = cell_at(e, y, j - 1, d - 1);
-        if (cy >= 0) {
-          if (my & BJ) R = tcomb<INSIDE>(R, A[cy] + t, ftbl);
-          if (my & BR) R = tcomb<INSIDE>(R, A[2 * nc + cy] + t, ftbl);
-        }
-      }
-      if (dL && (my & (BJ | BL))) {
-        const int cy = cell_at(e, y, j, d - 1);
-        if (cy >= 0) {
-          if (my & BJ) L = tcomb<INSIDE>(L, A[cy] + t, ftbl);
-          if (my & BL) L = tcomb<INSIDE>(L, A[nc + cy] + t, ftbl);
-        }
+      if (dL && (my_ & (BJ | BL)) && kq[yo]) {
+        if (my_ & BJ) L = tcomb<INSIDE>(L, b1[yo] + t, ftbl);
+        if (my_ & BL) L = tcomb<INSIDE>(L, l1[yo] + t, ftbl);
       }
     }
     const int xi = e.dsq[j - d + 1], xj = e.dsq[j];
@@ -150,22 +196,219 @@ __device__ void tr_cell(const CmDev &cm, const EnvDev &e, int v, int st, int j, 
     L = clampi(L);
     R = clampi(R);
   } else {  // D, S: no emission; a zero-length L or R cell reached from a child is IMPOSSIBLE
-    for (int yo = 0; yo < cn; yo++) {
-      const int y = cf + yo, my = e.mask[y];
-      const bool uj = dJ && (my & BJ), ul = dL && (my & BL), ur = dR && (my & BR);
-      if (!(uj || ul || ur)) continue;
-      const int cy = cell_at(e, y, j, d);
-      if (cy < 0) continue;
+    float a0[MAXC], l0[MAXC], r0[MAXC];
+    int my[MAXC];
+    bool k0[MAXC];
+#pragma unroll
+    for (int yo = 0; yo < MAXC; yo++) {
+      const int y = min(cf + yo, cm.M - 1);
+      my[yo] = yo < cn ? e.mask[y] : 0;
+      const RowRef r = row_ref(e, y, j);
+      k0[yo] = in_row(r, d);
+      const int i0 = k0[yo] ? r.base + d : 0;
+      a0[yo] = A[i0];
+      l0[yo] = (dL && (my[yo] & BL)) ? A[nc + i0] : 0.f;
+      r0[yo] = (dR && (my[yo] & BR)) ? A[2 * nc + i0] : 0.f;
+    }
+#pragma unroll
+    for (int yo = 0; yo < MAXC; yo++) {
+      const bool uj = dJ && (my[yo] & BJ), ul = dL && (my[yo] & BL), ur = dR && (my[yo] & BR);
+      if (!(uj || ul || ur) || !k0[yo]) continue;
       const float t = tsc[yo];
-      if (uj) J = tcomb<INSIDE>(J, A[cy] + t, ftbl);
-      if (ul) L = tcomb<INSIDE>(L, A[nc + cy] + t, ftbl);
-      if (ur) R = tcomb<INSIDE>(R, A[2 * nc + cy] + t, ftbl);
+      if (uj) J = tcomb<INSIDE>(J, a0[yo] + t, ftbl);
+      if (ul) L = tcomb<INSIDE>(L, l0[yo] + t, ftbl);
+      if (ur) R = tcomb<INSIDE>(R, r0[yo] + t, ftbl);
       if (d == 0) L = R = IMPOSSIBLE;
     }
   }
   if (dJ) A[c] = J;
   if (dL) A[nc + c] = L;
   if (dR) A[2 * nc + c] = R;
+}
+
+constexpr int TCH = 1;  // cells of an insert chain fetched ahead of their serial combine
+
+// One j row of an IL state: J and L read their own (j, d-1), the previous
+// cell of the row, carried in registers; the other children's cells (rows
+// looked up once per row) are fetched TCH cells ahead. R has no self term.
+template <bool INSIDE>
+__device__ void tr_il_row(const CmDev &cm, const EnvDev &e, int v, int u, const float *__restrict__ ftbl) {
+  const int j = e.jmin[v] + u, h = e.hd_off[v] + u;
+  const int lo = e.hdmin[h], hi = e.hdmax[h];
+  if (hi < lo) return;
+  const int cbase = e.roff[h] - lo;
+  const int mv = e.mask[v];
+  const bool dJ = mv & BJ, dL = mv & BL, dR = mv & BR;
+  const int nc = e.nc;
+  float *A = e.alpha;
+  const int cf = cm.cfirst[v], cn = cm.cnum[v];
+  const float *tsc = cm.tsc + v * MAXC;
+  const float *oe = cm.oesc + (size_t)v * (KP * KP);
+  RowRef rr[MAXC];
+  int my[MAXC];
+  float t[MAXC];
+#pragma unroll
+  for (int yo = 0; yo < MAXC; yo++) {
+    const int y = min(cf + yo, cm.M - 1);
+    rr[yo] = row_ref(e, y, j);
+    my[yo] = (yo < cn && cf + yo != v) ? e.mask[y] : 0;  // the self child is handled by the chain
+    t[yo] = tsc[yo];
+  }
+  int selfo = -1;
+#pragma unroll
+  for (int yo = 0; yo < MAXC; yo++)
+    if (yo < cn && cf + yo == v) selfo = yo;
+  float pJ = 0.f, pL = 0.f;
+  for (int d0 = lo; d0 <= hi; d0 += TCH) {
+    float a1[TCH][MAXC], l1[TCH][MAXC], a0[TCH][MAXC], r0[TCH][MAXC];
+    int x[TCH];
+#pragma unroll
+    for (int q = 0; q < TCH; q++) {
+      const int d = d0 + q;
+      x[q] = e.dsq[d <= hi ? j - d + 1 : j];
+#pragma unroll
+      for (int yo = 0; yo < MAXC; yo++) {
+        const bool k1 = d <= hi && my[yo] && in_row(rr[yo], d - 1), k0 = d <= hi && my[yo] && in_row(rr[yo], d);
+        const int i1 = k1 ? rr[yo].base + d - 1 : 0, i0 = k0 ? rr[yo].base + d : 0;
+        a1[q][yo] = k1 ? A[i1] : IMPOSSIBLE;
+        l1[q][yo] = (k1 && dL && (my[yo] & BL)) ? A[nc + i1] : IMPOSSIBLE;
+        a0[q][yo] = k0 ? A[i0] : IMPOSSIBLE;
+        r0[q][yo] = (k0 && dR && (my[yo] & BR)) ? A[2 * nc + i0] : IMPOSSIBLE;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < TCH; q++) {
+      const int d = d0 + q;
+      if (d > hi) break;
+      float J = tr_init(cm, v, d, 1), L = tr_init(cm, v, d, 1), R = tr_init(cm, v, d, 0);
+      if (dJ || dL) {
+#pragma unroll
+        for (int yo = 0; yo < MAXC; yo++) {
+          if (yo == selfo) {
+            if (d > lo) {  // (j, d-1) of this state, valid in every plane this state fills
+              if (dJ) J = tcomb<INSIDE>(J, pJ + t[yo], ftbl);
+              if (dL) L = tcomb<INSIDE>(L, pL + t[yo], ftbl);
+            }
+            continue;
+          }
+          if (!(my[yo] & (BJ | BL)) || !in_row(rr[yo], d - 1)) continue;
+          if (dJ && (my[yo] & BJ)) J = tcomb<INSIDE>(J, a1[q][yo] + t[yo], ftbl);
+          if (dL && (my[yo] & BL)) L = tcomb<INSIDE>(L, l1[q][yo] + t[yo], ftbl);
+        }
+        const float em = oe[x[q]];
+        J = clampi(J + em);
+        L = clampi(d >= 2 ? L + em : em);
+      }
+      if (dR) {
+#pragma unroll
+        for (int yo = 0; yo < MAXC; yo++) {
+          if (!(my[yo] & (BJ | BR)) || !in_row(rr[yo], d)) continue;
+          if (my[yo] & BJ) R = tcomb<INSIDE>(R, a0[q][yo] + t[yo], ftbl);
+          if (my[yo] & BR) R = tcomb<INSIDE>(R, r0[q][yo] + t[yo], ftbl);
+        }
+      }
+      const int c = cbase + d;
+      if (dJ) A[c] = J;
+      if (dL) A[nc + c] = L;
+      if (dR) A[2 * nc + c] = R;
+      pJ = J;
+      pL = L;
+    }
+  }
+}
+
+// One i diagonal of an IR state: J and R read their own (j-1, d-1), the
+// previous cell of the diagonal, carried in registers; the other children's
+// cells are fetched TCH cells ahead. L has no self term.
+template <bool INSIDE>
+__device__ void tr_ir_diag(const CmDev &cm, const EnvDev &e, int v, int u, const float *__restrict__ ftbl) {
+  const int i = e.ibase[v] + u;
+  const int jmn = e.jmin[v], nj = e.jmax[v] - jmn + 1, ho = e.hd_off[v];
+  const int mv = e.mask[v];
+  const bool dJ = mv & BJ, dL = mv & BL, dR = mv & BR;
+  const int nc = e.nc;
+  float *A = e.alpha;
+  const int cf = cm.cfirst[v], cn = cm.cnum[v];
+  const float *tsc = cm.tsc + v * MAXC;
+  const float *oe = cm.oesc + (size_t)v * (KP * KP);
+  int my[MAXC], ys[MAXC];
+  float t[MAXC];
+  int selfo = -1;
+#pragma unroll
+  for (int yo = 0; yo < MAXC; yo++) {
+    ys[yo] = min(cf + yo, cm.M - 1);
+    my[yo] = (yo < cn && cf + yo != v) ? e.mask[ys[yo]] : 0;
+    t[yo] = tsc[yo];
+    if (yo < cn && cf + yo == v) selfo = yo;
+  }
+  float pJ = 0.f, pR = 0.f;
+  bool have = false;
+  for (int jp0 = 0; jp0 < nj; jp0 += TCH) {
+    float a1[TCH][MAXC], r1[TCH][MAXC], a0[TCH][MAXC], l0[TCH][MAXC];
+    bool k1[TCH][MAXC], k0[TCH][MAXC], on[TCH];
+    int cell[TCH], x[TCH];
+#pragma unroll
+    for (int q = 0; q < TCH; q++) {
+      const int jp = jp0 + q, j = jmn + jp, d = j - i + 1;
+      const int h = ho + (jp < nj ? jp : 0);
+      const int lo = e.hdmin[h], hi = e.hdmax[h];
+      on[q] = jp < nj && d >= lo && d <= hi;
+      cell[q] = e.roff[h] + (d - lo);
+      x[q] = e.dsq[on[q] ? j : i];
+#pragma unroll
+      for (int yo = 0; yo < MAXC; yo++) {
+        const RowRef rp = row_ref(e, ys[yo], j - 1), rq = row_ref(e, ys[yo], j);
+        k1[q][yo] = on[q] && my[yo] && in_row(rp, d - 1);
+        k0[q][yo] = on[q] && my[yo] && in_row(rq, d);
+        const int i1 = k1[q][yo] ? rp.base + d - 1 : 0, i0 = k0[q][yo] ? rq.base + d : 0;
+        a1[q][yo] = k1[q][yo] ? A[i1] : IMPOSSIBLE;
+        r1[q][yo] = (k1[q][yo] && dR && (my[yo] & BR)) ? A[2 * nc + i1] : IMPOSSIBLE;
+        a0[q][yo] = k0[q][yo] ? A[i0] : IMPOSSIBLE;
+        l0[q][yo] = (k0[q][yo] && dL && (my[yo] & BL)) ? A[nc + i0] : IMPOSSIBLE;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < TCH; q++) {
+      if (!on[q]) {
+        have = false;
+        continue;
+      }
+      const int d = jmn + jp0 + q - i + 1;
+      float J = tr_init(cm, v, d, 1), L = tr_init(cm, v, d, 0), R = tr_init(cm, v, d, 1);
+      if (dJ || dR) {
+#pragma unroll
+        for (int yo = 0; yo < MAXC; yo++) {
+          if (yo == selfo) {
+            if (have) {  // (j-1, d-1) of this state
+              if (dJ) J = tcomb<INSIDE>(J, pJ + t[yo], ftbl);
+              if (dR) R = tcomb<INSIDE>(R, pR + t[yo], ftbl);
+            }
+            continue;
+          }
+          if (!(my[yo] & (BJ | BR)) || !k1[q][yo]) continue;
+          if (dJ && (my[yo] & BJ)) J = tcomb<INSIDE>(J, a1[q][yo] + t[yo], ftbl);
+          if (dR && (my[yo] & BR)) R = tcomb<INSIDE>(R, r1[q][yo] + t[yo], ftbl);
+        }
+        const float em = oe[x[q]];
+        J = clampi(J + em);
+        R = clampi(d >= 2 ? R + em : em);
+      }
+      if (dL) {
+#pragma unroll
+        for (int yo = 0; yo < MAXC; yo++) {
+          if (!(my[yo] & (BJ | BL)) || !k0[q][yo]) continue;
+          if (my[yo] & BJ) L = tcomb<INSIDE>(L, a0[q][yo] + t[yo], ftbl);
+          if (my[yo] & BL) L = tcomb<INSIDE>(L, l0[q][yo] + t[yo], ftbl);
+        }
+      }
+      if (dJ) A[cell[q]] = J;
+      if (dL) A[nc + cell[q]] = L;
+      if (dR) A[2 * nc + cell[q]] = R;
+      pJ = J;
+      pR = R;
+      have = true;
+    }
+  }
 }
 
 // Cell (j, d) of a bifurcation: split point k = length of the right child's
@@ -235,17 +478,9 @@ __global__ __launch_bounds__(HB_THREADS) void trhb_scan_kernel(CmDev cm, const E
     const int ho = e.hd_off[v];
     const int32_t *hmn = e.hdmin + ho, *hmx = e.hdmax + ho, *ro = e.roff + ho;
     if (st == IL_ST) {
-      // J and L read their own (j, d-1): serial over d within row u
-      for (int d = hmn[u]; d <= hmx[u]; d++)
-        tr_cell<INSIDE>(cm, e, v, st, jmn + u, d, d - hmn[u], ro[u] + (d - hmn[u]), ftbl);
+      tr_il_row<INSIDE>(cm, e, v, u, ftbl);
     } else if (st == IR_ST) {
-      // J and R read their own (j-1, d-1), the same i: serial along diagonal i
-      const int i = e.ibase[v] + u;
-      for (int jp = 0; jp < nj; jp++) {
-        const int j = jmn + jp, d = j - i + 1;
-        if (d < hmn[jp] || d > hmx[jp]) continue;
-        tr_cell<INSIDE>(cm, e, v, st, j, d, d - hmn[jp], ro[jp] + (d - hmn[jp]), ftbl);
-      }
+      tr_ir_diag<INSIDE>(cm, e, v, u, ftbl);
     } else {
       const int tgt = ro[0] + u;
       const int r = hb_row_of(ro, nj - 1, tgt);
