@@ -180,6 +180,15 @@ int ixg_cm_create(ixg_ctx *ctx, const ixg_cm_params *p, ixg_cm **out);
 int ixg_cm_free(ixg_ctx *ctx, ixg_cm *cm);
 /* tsc M*6, oesc M*324, beginsc M, endsc M, el_selfsc 1 (any may be NULL). */
 int ixg_cm_tables(const ixg_cm *cm, float *tsc, float *oesc, float *beginsc, float *endsc, float *el_selfsc);
+/* As ixg_cm_create with a choice of configuration: local != 0 is
+ * cm_configure_scores (cp9.rs:4538), local == 0 the global configuration
+ * of cm_nohmm.rs:264 cm_configure_scores_global (no local begins or ends).
+ * Both also build the integer scores of cm_nohmm.rs:279
+ * build_integer_scores for the QDB Inside scanner. */
+int ixg_cm_create_cfg(ixg_ctx *ctx, const ixg_cm_params *p, int local, ixg_cm **out);
+/* Integer scores: itsc M*6, ioesc M*324, ibeginsc M, iendsc M, iel_selfsc 1. */
+int ixg_cm_itables(const ixg_cm *cm, int32_t *itsc, int32_t *ioesc, int32_t *ibeginsc, int32_t *iendsc,
+                   int32_t *iel_selfsc);
 
 /* HMM bands of one envelope (CP9Bands, cp9.rs:1453-1470): j band per state,
  * d band per (v, j) at hd_off[v] + (j - jmin[v]); hd_off has M+1 entries. */
@@ -271,13 +280,37 @@ int ixg_trhb_scan_batch(ixg_ctx *ctx, const ixg_cm *cm, const ixg_target *tgt, c
                         int do_null3, float *vsc, int32_t *vmode, int64_t *envi, int64_t *envj, ixg_trhit *hits,
                         int64_t cap, int64_t *nhits, int64_t *cells);
 
+/* ---- query-dependent-banded / non-banded scanners (cm_nohmm.rs) ------------------ */
+/* cm_nohmm.rs:406 generic_scan over dsq[i0..j0]: CYK on the float scores
+ * (inside = 0; cyk_seq_filter's scan, :855) or Inside on the integer scores
+ * (inside = 1; final_stage_inside_opt, :919), with per-state d bands
+ * dmin/dmax (the CM's dmin1/dmax1 or dmin2/dmax2; both NULL = non-banded,
+ * 0..W). Hits after greedy reporting with null3 and overlap removal, in
+ * window coordinates; IXG_ERANGE if more than cap. */
+int ixg_qdb_scan(ixg_ctx *ctx, const ixg_cm *cm, const uint8_t *dsq, int64_t L, int32_t i0, int32_t j0,
+                 const int32_t *dmin, const int32_t *dmax, int inside, float cutoff, int do_null3, ixg_hit *hits,
+                 int64_t cap, int64_t *nhits);
+/* Batched form: n windows of a device-resident target (strand 1 = reverse
+ * complement coordinates); hits tagged with env = window index, grouped by
+ * window in order (the per-envelope loop of final_stage_inside_opt). */
+typedef struct { int32_t i0, j0, strand, pad; } ixg_qdbwin;
+int ixg_qdb_scan_batch(ixg_ctx *ctx, const ixg_cm *cm, const ixg_target *tgt, const ixg_qdbwin *win, int64_t n,
+                       const int32_t *dmin, const int32_t *dmax, int inside, float cutoff, int do_null3,
+                       ixg_hit *hits, int64_t cap, int64_t *nhits);
+/* cm_nohmm.rs:855 cyk_seq_filter(cm, dsq, n, cutoff, do_null3): CYK over
+ * dsq[1..n] with dmin1/dmax1, hits padded by W-1 and merged into envelopes
+ * (es[k], ee[k]); *nenv = count (IXG_ERANGE above cap). */
+int ixg_cyk_seq_filter(ixg_ctx *ctx, const ixg_cm *cm, const uint8_t *dsq, int64_t n, const int32_t *dmin1,
+                       const int32_t *dmax1, float cutoff, int do_null3, int32_t *es, int32_t *ee, int64_t cap,
+                       int64_t *nenv);
+
 /* Per-kernel device time, cumulative since the last reset: HIP events around
  * every launch of each kernel (IXG_K_* index), total milliseconds and launch
  * counts; reset != 0 clears them after reading. A measurement hook with no
  * counterpart in the reference. Timing synchronizes the context's stream
  * after each launch. */
 enum { IXG_K_SSV_SWEEP = 0, IXG_K_SSV_RESOLVE = 1, IXG_K_FWD = 2, IXG_K_BIAS = 3, IXG_K_VIT = 4, IXG_K_MSV = 5,
-       IXG_K_HB_SCAN = 6, IXG_K_TRHB_SCAN = 7, IXG_K_COPY = 8, IXG_NKERNELS = 9 };
+       IXG_K_HB_SCAN = 6, IXG_K_TRHB_SCAN = 7, IXG_K_COPY = 8, IXG_K_QDB_SCAN = 9, IXG_NKERNELS = 10 };
 int ixg_kernel_timings(ixg_ctx *ctx, double *ms, int64_t *launches, int reset);
 /* Measurement hook (no counterpart in the reference): copies `bytes`
  * device-to-device with a plain kernel reps times after one warm-up copy;

@@ -7,43 +7,9 @@
 #include <memory>
 #include "cm_internal.h"
 #include "ctx.h"
+#include "cm_handle.h"
 
 using namespace ixg;
-
-struct ixg_cm {
-  CmHost h;
-  std::vector<float> ftbl_host;
-  int8_t *d_st = nullptr;
-  int32_t *d_i32 = nullptr;  // cfirst | cnum | begin_list | lvl_states | lvl_off
-  float *d_f = nullptr;      // tsc | oesc | beginsc | endsc | flogsum table | lmesc | rmesc
-  float *d_tr = nullptr;     // truncated-begin penalties: g[3][M] | l[3][M]
-  int32_t *d_yl = nullptr;   // per (local, pass index): states with an allowed truncated begin
-  int32_t yoff[7] = {0, 0, 0, 0, 0, 0, 0};
-  size_t b_i32 = 0, b_f = 0, b_tr = 0, b_yl = 0;
-  CmDev dev() const {
-    CmDev c;
-    const int M = h.M;
-    c.sttype = d_st;
-    c.cfirst = d_i32;
-    c.cnum = d_i32 + M;
-    c.begin_list = d_i32 + 2 * M;
-    c.nbegin = (int)h.begin_list.size();
-    c.lvl_states = c.begin_list + h.begin_list.size();
-    c.lvl_off = c.lvl_states + M;
-    c.nlev = (int)h.lvl_off.size() - 1;
-    c.tsc = d_f;
-    c.oesc = d_f + (size_t)M * MAXC;
-    c.beginsc = c.oesc + (size_t)M * KP * KP;
-    c.endsc = c.beginsc + M;
-    c.lmesc = d_ftbl() + FLOGSUM_TBL;
-    c.rmesc = c.lmesc + (size_t)M * KP;
-    c.M = M;
-    c.local_begin = h.local_begin;
-    c.el_selfsc = h.el_selfsc;
-    return c;
-  }
-  const float *d_ftbl() const { return d_f + (size_t)h.M * MAXC + (size_t)h.M * KP * KP + 2 * (size_t)h.M; }
-};
 
 namespace {
 
@@ -144,15 +110,7 @@ float kahan(const float *v, int n) {
 }
 
 // cp9.rs:6516-6529 score_correction_null3
-float null3(const ixg_cm &cm, const float *comp, int len) {
-  float score = 0.f;
-  for (int a = 0; a < 4; a++) {
-    double term = sre_log2((double)(comp[a] / cm.h.null[a])) * (double)comp[a] * (double)len;
-    score = (float)((double)score + term);
-  }
-  score = (float)((double)score + sre_log2((double)(float)cm.h.n3_omega));
-  return host_flogsum(cm.ftbl_host, 0.0f, score);
-}
+float null3(const ixg_cm &cm, const float *comp, int len) { return null3_score(cm.h, cm.ftbl_host, comp, len); }
 
 // ReportHitsGreedily over the root deck (cp9.rs:5887-5900, 5950-5997).
 void report_hits(const ixg_cm &cm, const ixg_bands &b, const uint8_t *dsq, int32_t i0, int32_t j0,
@@ -531,7 +489,9 @@ void tr_report(const ixg_cm &cm, const ixg_bands &b, const uint8_t *dsq, int32_t
 
 extern "C" {
 
-int ixg_cm_create(ixg_ctx *c, const ixg_cm_params *p, ixg_cm **out) {
+int ixg_cm_create(ixg_ctx *c, const ixg_cm_params *p, ixg_cm **out) { return ixg_cm_create_cfg(c, p, 1, out); }
+
+int ixg_cm_create_cfg(ixg_ctx *c, const ixg_cm_params *p, int local, ixg_cm **out) {
   if (!p || !out || p->M < 1 || p->nodes < 2 || !p->sttype || !p->ndtype || !p->nodemap || !p->cfirst || !p->cnum ||
       !p->t || !p->e) {
     g_err = "ixg_cm_create: bad arguments";
@@ -566,10 +526,11 @@ int ixg_cm_create(ixg_ctx *c, const ixg_cm_params *p, ixg_cm **out) {
     }
   }
   auto cm = std::make_unique<ixg_cm>();
-  RCHK(configure_cm(*p, cm->h));
+  RCHK(configure_cm(*p, local, cm->h));
   marginal_emissions(*p, cm->h);
   RCHK(state_spans(*p, cm->h));
   dependency_levels(*p, cm->h);
+  RCHK(qdb_tables(*p, cm->h));
   cm->h.clen = p->clen;
   flogsum_table(cm->ftbl_host);
   if (!c) {
@@ -598,6 +559,7 @@ int ixg_cm_create(ixg_ctx *c, const ixg_cm_params *p, ixg_cm **out) {
   HIPCHK(hipMemcpy(cm->d_st, cm->h.sttype.data(), M, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(cm->d_i32, i32.data(), i32.size() * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(cm->d_f, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+  RCHK(qdb_upload(c, cm.get()));
   *out = cm.release();
   return IXG_OK;
 }
@@ -611,6 +573,7 @@ int ixg_cm_free(ixg_ctx *c, ixg_cm *cm) {
     dfree(c, cm->d_f, cm->b_f);
     dfree(c, cm->d_tr, cm->b_tr);
     dfree(c, cm->d_yl, cm->b_yl);
+    dfree(c, cm->d_q, cm->b_q);
   }
   delete cm;
   return IXG_OK;

@@ -35,7 +35,75 @@ static float kahan_sum(const float *v, int n) {  // cp9.rs:451 esl_vec_fsum
   return sum;
 }
 
-int configure_cm(const ixg_cm_params &p, CmHost &h) {
+// cm_nohmm.rs:27-33 prob2score: floor(0.5 + 1000 log2(p / null)), -INFTY for p == 0
+static int32_t prob2score(float p, float null) {
+  if (p == 0.0f) return -INFTY_I;
+  return (int32_t)std::floor(0.5 + 1000.0 * std::log2((double)p / (double)null));
+}
+// cm.rs:104-120 abc_iavg_score: mean over a degenerate code, rounded half away from zero
+static int32_t iavg(int x, const int32_t *sc) {
+  if (!(x < 4 || (x > 4 && x < KP - 2))) return 0;
+  float r = 0.0f;
+  for (int i = 0; i < 4; i++)
+    if (DEGEN[x][i]) r += (float)sc[i];
+  r /= (float)ndeg(x);
+  return r < 0.0f ? (int32_t)(r - 0.5f) : (int32_t)(r + 0.5f);
+}
+
+// Integer scores of the QDB Inside scanner (cm_nohmm.rs:279-368
+// build_integer_scores) from the configured probabilities.
+static void integer_scores(const ixg_cm_params &p, const float *t, const float *begin, const float *end, CmHost &h) {
+  const int M = p.M;
+  h.itsc.assign((size_t)M * MAXC, 0);
+  h.ioesc.assign((size_t)M * KP * KP, 0);
+  h.ibeginsc.assign(M, 0);
+  h.iendsc.assign(M, 0);
+  for (int v = 0; v < M; v++) {
+    const int st = p.sttype[v];
+    if (st != B_ST && st != E_ST)
+      for (int x = 0; x < p.cnum[v]; x++) h.itsc[(size_t)v * MAXC + x] = prob2score(t[(size_t)v * MAXC + x], 1.0f);
+    int32_t *oe = &h.ioesc[(size_t)v * KP * KP];
+    const float *e = p.e + (size_t)v * 16;
+    if (st == MP_ST) {
+      int32_t ie[16];
+      for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) ie[a * 4 + b] = prob2score(e[a * 4 + b], p.null[a] * p.null[b]);
+      for (int i = 0; i < KP * KP; i++) oe[i] = -INFTY_I;
+      for (int a = 0; a < 4; a++)
+        for (int b = 0; b < 4; b++) oe[a * KP + b] = ie[a * 4 + b];
+      for (int a = 5; a < KP - 1; a++)
+        for (int b = 0; b < 4; b++) {
+          float sc = 0.f;
+          for (int l = 0; l < 4; l++) sc += (float)ie[l * 4 + b] * frac(a, l);
+          oe[a * KP + b] = (int32_t)sc;
+        }
+      for (int a = 0; a < 4; a++)
+        for (int b = 5; b < KP - 1; b++) {
+          float sc = 0.f;
+          for (int r = 0; r < 4; r++) sc += (float)ie[a * 4 + r] * frac(b, r);
+          oe[a * KP + b] = (int32_t)sc;
+        }
+      for (int a = 5; a < KP - 1; a++)
+        for (int b = 5; b < KP - 1; b++) {
+          float sc = 0.f;
+          for (int l = 0; l < 4; l++)
+            for (int r = 0; r < 4; r++) sc += (float)ie[l * 4 + r] * frac(a, l) * frac(b, r);
+          oe[a * KP + b] = (int32_t)sc;
+        }
+    } else if (st == ML_ST || st == MR_ST || st == IL_ST || st == IR_ST) {
+      int32_t ie[4];
+      for (int a = 0; a < 4; a++) ie[a] = prob2score(e[a], p.null[a]);
+      for (int a = 0; a < KP; a++) oe[a] = -INFTY_I;
+      for (int a = 0; a < 4; a++) oe[a] = ie[a];
+      for (int a = 5; a < KP - 1; a++) oe[a] = iavg(a, ie);
+    }
+    h.ibeginsc[v] = prob2score(begin[v], 1.0f);
+    h.iendsc[v] = prob2score(end[v], 1.0f);
+  }
+  h.iel_selfsc = prob2score(std::pow(2.0f, h.el_selfsc), 1.0f);
+}
+
+int configure_cm(const ixg_cm_params &p, int local, CmHost &h) {
   const int M = p.M;
   h.M = M;
   h.sttype.assign(p.sttype, p.sttype + M);
@@ -48,30 +116,32 @@ int configure_cm(const ixg_cm_params &p, CmHost &h) {
     int ty = p.ndtype[nd];
     return ty == MATP_ND || ty == MATL_ND || ty == MATR_ND || ty == BIF_ND;
   };
-  int nstarts = 0;
-  for (int nd = 2; nd < p.nodes; nd++) nstarts += is_start(nd);
-  begin[p.nodemap[1]] = 1.0f - p.pbegin;
-  const float pb = p.pbegin / (float)nstarts;
-  for (int nd = 2; nd < p.nodes; nd++)
-    if (is_start(nd)) begin[p.nodemap[nd]] = pb;
-  for (int y = 0; y < p.cnum[0]; y++) t[y] = 0.f;
-  auto is_exit = [&](int nd) {
-    int ty = p.ndtype[nd];
-    return (ty == MATP_ND || ty == MATL_ND || ty == MATR_ND || ty == BEGL_ND || ty == BEGR_ND) &&
-           p.ndtype[nd + 1] != END_ND;
-  };
-  int nexits = 0;
-  for (int nd = 1; nd < p.nodes; nd++) nexits += is_exit(nd);
-  for (int nd = 1; nd < p.nodes; nd++)
-    if (is_exit(nd)) {
-      const int v = p.nodemap[nd];
-      end[v] = p.pend / (float)nexits;
-      float denom = kahan_sum(&t[(size_t)v * MAXC], p.cnum[v]);
-      denom += end[v];
-      const float inv = 1.0f / denom;
-      for (int x = 0; x < p.cnum[v]; x++) t[(size_t)v * MAXC + x] *= inv;
-    }
-  h.local_begin = 1;
+  if (local) {  // cm_config_local: local begins and ends
+    int nstarts = 0;
+    for (int nd = 2; nd < p.nodes; nd++) nstarts += is_start(nd);
+    begin[p.nodemap[1]] = 1.0f - p.pbegin;
+    const float pb = p.pbegin / (float)nstarts;
+    for (int nd = 2; nd < p.nodes; nd++)
+      if (is_start(nd)) begin[p.nodemap[nd]] = pb;
+    for (int y = 0; y < p.cnum[0]; y++) t[y] = 0.f;
+    auto is_exit = [&](int nd) {
+      int ty = p.ndtype[nd];
+      return (ty == MATP_ND || ty == MATL_ND || ty == MATR_ND || ty == BEGL_ND || ty == BEGR_ND) &&
+             p.ndtype[nd + 1] != END_ND;
+    };
+    int nexits = 0;
+    for (int nd = 1; nd < p.nodes; nd++) nexits += is_exit(nd);
+    for (int nd = 1; nd < p.nodes; nd++)
+      if (is_exit(nd)) {
+        const int v = p.nodemap[nd];
+        end[v] = p.pend / (float)nexits;
+        float denom = kahan_sum(&t[(size_t)v * MAXC], p.cnum[v]);
+        denom += end[v];
+        const float inv = 1.0f / denom;
+        for (int x = 0; x < p.cnum[v]; x++) t[(size_t)v * MAXC + x] *= inv;
+      }
+  }
+  h.local_begin = local ? 1 : 0;
   h.el_selfsc = p.el_selfsc;
   if (h.el_selfsc * (float)p.W < IMPOSSIBLE) h.el_selfsc = IMPOSSIBLE / ((float)p.W + 1.0f);
   h.tsc.assign((size_t)M * MAXC, 0.f);
@@ -120,6 +190,7 @@ int configure_cm(const ixg_cm_params &p, CmHost &h) {
     h.beginsc[v] = (float)sre_log2((double)begin[v]);
     h.endsc[v] = (float)sre_log2((double)end[v]);
   }
+  integer_scores(p, t.data(), begin.data(), end.data(), h);
   h.begin_list.clear();
   for (int y = 1; y < M; y++)
     if (h.beginsc[y] > -9.999e35f) h.begin_list.push_back(y);
@@ -265,6 +336,95 @@ void dependency_levels(const ixg_cm_params &p, CmHost &h) {
   h.lvl_states.assign(M, 0);
   std::vector<int32_t> at(h.lvl_off.begin(), h.lvl_off.end() - 1);
   for (int v = M - 1; v >= 0; v--) h.lvl_states[at[lvl[v]]++] = v;
+}
+
+}  // namespace ixg
+
+namespace ixg {
+
+static int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+
+int qdb_tables(const ixg_cm_params &p, CmHost &h) {
+  const int M = p.M;
+  h.W = p.W;
+  if (h.W < 1) {
+    g_err = "ixg_cm_create: W must be at least 1";
+    return IXG_EINVAL;
+  }
+  std::vector<int32_t> ndidx(M);
+  for (int nd = 0, v = 0; v < M; v++) {
+    while (nd + 1 < p.nodes && p.nodemap[nd + 1] <= v) nd++;
+    ndidx[v] = nd;
+  }
+  h.qbegl.assign(M, -1);
+  int nbegl = 0;
+  for (int v = 0; v < M; v++)
+    if (p.sttype[v] == S_ST && p.ndtype[ndidx[v]] == BEGL_ND) h.qbegl[v] = nbegl++;
+  for (int v = 0; v < M; v++)
+    if (p.sttype[v] == B_ST && h.qbegl[p.cfirst[v]] < 0) {
+      g_err = "ixg_cm_create: a bifurcation's left child is not the start state of a BEGL node";
+      return IXG_EINVAL;
+    }
+  // el_sca[d] = el_self added d times (cm_nohmm.rs:466-470)
+  h.qfel.assign(h.W + 1, 0.f);
+  h.qiel.assign(h.W + 1, 0);
+  for (int d = 1; d <= h.W; d++) {
+    h.qfel[d] = h.qfel[d - 1] + h.el_selfsc;
+    h.qiel[d] = wadd(h.qiel[d - 1], h.iel_selfsc);
+  }
+  // d = 0 base cases, states in descending order (cm_nohmm.rs:489-532)
+  h.qfbase0.assign(M, IMPOSSIBLE);
+  h.qibase0.assign(M, -INFTY_I);
+  h.qfbbase0.assign(std::max(nbegl, 1), IMPOSSIBLE);
+  h.qibbase0.assign(std::max(nbegl, 1), -INFTY_I);
+  for (int v = M - 1; v >= 0; v--) {
+    const int st = p.sttype[v];
+    if (st == E_ST) {
+      h.qfbase0[v] = 0.f;
+      h.qibase0[v] = 0;
+    } else if (st == S_ST || st == D_ST) {
+      float fa = h.endsc[v];
+      int32_t ia = h.iendsc[v];
+      for (int k = 0; k < p.cnum[v]; k++) {
+        const int y = p.cfirst[v] + k;
+        const float fc = h.qfbase0[y] + h.tsc[(size_t)v * MAXC + k];
+        const int32_t ic = wadd(h.qibase0[y], h.itsc[(size_t)v * MAXC + k]);
+        fa = fa >= fc ? fa : fc;
+        ia = ia >= ic ? ia : ic;
+      }
+      fa = fa >= IMPOSSIBLE ? fa : IMPOSSIBLE;
+      ia = ia >= -INFTY_I ? ia : -INFTY_I;
+      if (h.qbegl[v] >= 0) {  // a BEGL_S: the value lives in its deck
+        h.qfbbase0[h.qbegl[v]] = fa;
+        h.qibbase0[h.qbegl[v]] = ia;
+      } else {
+        h.qfbase0[v] = fa;
+        h.qibase0[v] = ia;
+      }
+    } else if (st == B_ST) {
+      const int bl = h.qbegl[p.cfirst[v]], y = p.cnum[v];
+      h.qfbase0[v] = h.qfbbase0[bl] + h.qfbase0[y];
+      h.qibase0[v] = wadd(h.qibbase0[bl], h.qibase0[y]);
+    }
+  }
+  h.qblist.clear();
+  for (int y = 1; y < M; y++)
+    if (h.beginsc[y] > IMPOSSIBLE + 1.0f) h.qblist.push_back(y);
+  h.ilog.resize(FLOGSUM_TBL);  // cp9.rs:28-43 ILogsum table
+  for (int i = 0; i < FLOGSUM_TBL; i++)
+    h.ilog[i] = (int32_t)std::nearbyint(1000.0 * std::log2(1.0 + std::pow(2.0, -(double)i / 1000.0)));
+  return IXG_OK;
+}
+
+// cp9.rs:6516-6529 score_correction_null3
+float null3_score(const CmHost &h, const std::vector<float> &ftbl, const float *comp, int len) {
+  float score = 0.f;
+  for (int a = 0; a < 4; a++) {
+    const double term = sre_log2((double)(comp[a] / h.null[a])) * (double)comp[a] * (double)len;
+    score = (float)((double)score + term);
+  }
+  score = (float)((double)score + sre_log2((double)(float)h.n3_omega));
+  return host_flogsum(ftbl, 0.0f, score);
 }
 
 }  // namespace ixg

@@ -7,6 +7,7 @@ namespace ixg {
 constexpr int MAXC = 6;
 constexpr float IMPOSSIBLE = -1e36f;  // cp9.rs:4189
 constexpr int FLOGSUM_TBL = 23000;     // cp9.rs:5388
+constexpr int INFTY_I = 987654321;     // cp9.rs:21, the integer scores' -inf
 enum { D_ST = 0, MP_ST = 1, ML_ST = 2, MR_ST = 3, IL_ST = 4, IR_ST = 5, S_ST = 6, E_ST = 7, B_ST = 8 };
 enum { BIF_ND = 0, MATP_ND = 1, MATL_ND = 2, MATR_ND = 3, BEGL_ND = 4, BEGR_ND = 5, ROOT_ND = 6, END_ND = 7 };
 
@@ -20,6 +21,14 @@ struct CmHost {
   std::vector<int8_t> sttype;
   std::vector<int32_t> cfirst, cnum;
   std::vector<float> tsc, oesc, beginsc, endsc;  // M*6, M*324, M, M
+  std::vector<int32_t> itsc, ioesc, ibeginsc, iendsc;  // integer scores (same shapes)
+  int32_t iel_selfsc = 0;
+  // QDB scanner inputs (cm_nohmm.rs:406-532): W, BEGL_S deck index per state
+  // (-1 otherwise), per semiring the d = 0 base value of each state and deck
+  // and el_sca[0..W], and the states y >= 1 with a local begin
+  int W = 0;
+  std::vector<int32_t> qbegl, qibase0, qiel, qibbase0, qblist, ilog;
+  std::vector<float> qfbase0, qfel, qfbbase0;
   std::vector<int32_t> begin_list;               // states y >= 1 with a local begin
   // dependency levels: a state's level is one more than its children's
   // highest (self transitions excluded), so the states of one level are
@@ -34,8 +43,10 @@ struct CmHost {
 };
 void marginal_emissions(const ixg_cm_params &p, CmHost &h);
 int state_spans(const ixg_cm_params &p, CmHost &h);
-int configure_cm(const ixg_cm_params &p, CmHost &h);
+int configure_cm(const ixg_cm_params &p, int local, CmHost &h);
 void dependency_levels(const ixg_cm_params &p, CmHost &h);
+int qdb_tables(const ixg_cm_params &p, CmHost &h);
+float null3_score(const CmHost &h, const std::vector<float> &ftbl, const float *comp, int len);
 void flogsum_table(std::vector<float> &t);
 float host_flogsum(const std::vector<float> &t, float a, float b);
 
@@ -83,5 +94,38 @@ hipError_t launch_hb_scan(const CmDev &cm, const EnvDev *envs, int64_t n, const 
 // y >= 1 with an allowed truncated begin, ascending.
 hipError_t launch_trhb_scan(const CmDev &cm, const EnvDev *envs, int64_t n, const float *ftbl, const float *pty,
                             const int32_t *ylist, int ny, int inside, hipStream_t s);
+
+// ---- QDB / non-banded scanners (cm_nohmm.rs generic_scan) ----
+
+// Score tables of one semiring (float CYK or integer Inside): T is float or
+// int32; base0 holds each state's d = 0 value, el_sca[d] the EL score of d
+// residues (repeated additions), bbase0 the d = 0 value of each BEGL_S deck.
+template <typename T>
+struct QdbTabs {
+  const T *tsc, *oesc, *endsc, *beginsc, *base0, *el_sca, *bbase0;
+};
+struct QdbDev {
+  const int8_t *sttype;
+  const int32_t *cfirst, *cnum;
+  const int32_t *begl;  // per state: BEGL_S deck index, -1 otherwise
+  const int32_t *lvl_states, *lvl_off;
+  const int32_t *blist;  // states y >= 1 with a local begin, ascending
+  const int32_t *dn, *dxcap;  // per state, this call's bands: max(dmin, emissions), min(dmax, W)
+  const int32_t *dminw;       // per state, min(dmin, W): a bifurcation's split range
+  const int32_t *ilog;        // ILogsum table
+  int nlev, nb, M, W, nbegl, local;
+};
+struct QdbWinDev {
+  const uint8_t *dsq;
+  int32_t i0, j0, w, win;
+  void *rows;  // [2][M][w+1]
+  void *begl;  // [w+1][nbegl][w+1]
+};
+struct QdbCand {  // a root cell at or above the reporting cutoff
+  int32_t win, j, d;
+  float sc;
+};
+hipError_t launch_qdb_scan(const QdbDev &q, const void *tabs, int inside, const QdbWinDev *wins, int64_t n,
+                           float cutoff, QdbCand *cand, int64_t cap, unsigned long long *counter, hipStream_t s);
 
 }  // namespace ixg

@@ -1,0 +1,372 @@
+// C ABI of the query-dependent-banded (or non-banded) scanners
+// (cm_nohmm.rs generic_scan, cyk_seq_filter, final_stage_inside_opt): the
+// resident integer tables, per-call band vectors, windows launched in groups
+// that fit the row budget, and the host side of the reporting (greedy per j
+// with null3, cm_nohmm.rs:769-813; overlap removal, cp9.rs:6544; the
+// envelope padding and merge of cm_nohmm.rs:855-894).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include "cm_handle.h"
+
+using namespace ixg;
+
+int qdb_upload(ixg_ctx *c, ixg_cm *cm) {
+  const CmHost &h = cm->h;
+  std::vector<int32_t> q(h.itsc);
+  auto put = [&](const std::vector<int32_t> &x) { q.insert(q.end(), x.begin(), x.end()); };
+  auto putf = [&](const std::vector<float> &x) {
+    for (float f : x) {
+      int32_t b;
+      memcpy(&b, &f, 4);
+      q.push_back(b);
+    }
+  };
+  put(h.ioesc);
+  put(h.ibeginsc);
+  put(h.iendsc);
+  put(h.ilog);
+  put(h.qbegl);
+  put(h.qibase0);
+  put(h.qiel);
+  put(h.qibbase0);
+  putf(h.qfbase0);
+  putf(h.qfel);
+  putf(h.qfbbase0);
+  put(h.qblist);
+  q.push_back(0);
+  cm->b_q = q.size() * 4;
+  RCHK(dmalloc(c, (void **)&cm->d_q, cm->b_q));
+  HIPCHK(hipMemcpy(cm->d_q, q.data(), cm->b_q, hipMemcpyHostToDevice));
+  return IXG_OK;
+}
+
+namespace {
+
+struct QdbView {  // device pointers into cm->d_q
+  const int32_t *itsc, *ioesc, *ibeginsc, *iendsc, *ilog, *begl, *ibase0, *iel, *ibbase0, *blist;
+  const float *fbase0, *fel, *fbbase0;
+};
+QdbView qview(const ixg_cm *cm) {
+  const CmHost &h = cm->h;
+  const int M = h.M, nb = (int)h.qibbase0.size();
+  QdbView v;
+  const int32_t *p = cm->d_q;
+  v.itsc = p;
+  p += (size_t)M * MAXC;
+  v.ioesc = p;
+  p += (size_t)M * KP * KP;
+  v.ibeginsc = p;
+  p += M;
+  v.iendsc = p;
+  p += M;
+  v.ilog = p;
+  p += FLOGSUM_TBL;
+  v.begl = p;
+  p += M;
+  v.ibase0 = p;
+  p += M;
+  v.iel = p;
+  p += h.W + 1;
+  v.ibbase0 = p;
+  p += nb;
+  v.fbase0 = (const float *)p;
+  p += M;
+  v.fel = (const float *)p;
+  p += h.W + 1;
+  v.fbbase0 = (const float *)p;
+  p += nb;
+  v.blist = p;
+  return v;
+}
+
+struct HostCand {
+  int32_t j, d;
+  float sc;
+};
+
+// Greedy reporting per j over one window's candidates, null3, then overlap
+// removal (cm_nohmm.rs:769-813, cp9.rs:6544-6574); appends to out.
+void qdb_report(const ixg_cm &cm, const uint8_t *dsq, int32_t i0, int32_t j0, std::vector<HostCand> &cand,
+                float cutoff, int do_null3, int32_t env, std::vector<ixg_hit> &out) {
+  std::sort(cand.begin(), cand.end(), [](const HostCand &a, const HostCand &b) {
+    return a.j != b.j ? a.j < b.j : a.d < b.d;
+  });
+  std::vector<double> act;
+  if (do_null3) {
+    const int L = j0 - i0 + 1;
+    act.assign((size_t)(L + 1) * 4, 0.0);
+    for (int jp = 1; jp <= L; jp++) {
+      for (int a = 0; a < 4; a++) act[jp * 4 + a] = act[(jp - 1) * 4 + a];
+      if (dsq[i0 + jp - 1] < 4) act[jp * 4 + dsq[i0 + jp - 1]] += 1.0;
+    }
+  }
+  std::vector<ixg_hit> tmp;
+  size_t k = 0;
+  while (k < cand.size()) {
+    const int j = cand[k].j;
+    float maxrep = IMPOSSIBLE;
+    for (; k < cand.size() && cand[k].j == j; k++) {
+      const int d = cand[k].d, i = j - d + 1;
+      if (d < 1) continue;
+      float hs = cand[k].sc, bias = 0.f;
+      if (!(hs > maxrep && hs >= cutoff && hs > IMPOSSIBLE + 1.0f)) continue;
+      bool rep = true;
+      if (do_null3) {
+        float comp[4];
+        const int ip = i - i0 + 1, jj = j - i0 + 1;
+        for (int a = 0; a < 4; a++) comp[a] = (float)(act[jj * 4 + a] - act[(ip - 1) * 4 + a]);
+        float s = 0.f;
+        for (int a = 0; a < 4; a++) s += comp[a];
+        if (s > 0.f)
+          for (int a = 0; a < 4; a++) comp[a] /= s;
+        else
+          for (int a = 0; a < 4; a++) comp[a] = 0.25f;
+        const float corr = null3_score(cm.h, cm.ftbl_host, comp, d);
+        hs -= corr;
+        bias = corr;
+        rep = hs > maxrep && hs >= cutoff;
+      }
+      if (rep) {
+        tmp.push_back({i, j, hs, bias, env});
+        maxrep = hs;
+      }
+    }
+  }
+  std::stable_sort(tmp.begin(), tmp.end(), [](const ixg_hit &a, const ixg_hit &b) {
+    if (a.score != b.score) return a.score > b.score;
+    return a.i < b.i;
+  });
+  std::vector<char> removed(tmp.size(), 0);
+  for (size_t a = 0; a < tmp.size(); a++) {
+    if (removed[a]) continue;
+    for (size_t b = a + 1; b < tmp.size(); b++)
+      if (!removed[b] && !(tmp[b].j < tmp[a].i) && !(tmp[a].j < tmp[b].i)) removed[b] = 1;
+  }
+  for (size_t a = 0; a < tmp.size(); a++)
+    if (!removed[a]) out.push_back(tmp[a]);
+}
+
+struct QWin {
+  int32_t i0, j0;
+  int64_t seq_off;
+  const uint8_t *host_dsq;
+};
+
+// Runs the windows (over the device sequence base + seq_off) and appends
+// every window's hits (env = window index) to out.
+int qdb_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vector<QWin> &wins,
+            const int32_t *dmin, const int32_t *dmax, int inside, float cutoff, int do_null3,
+            std::vector<ixg_hit> &out) {
+  const CmHost &h = cm->h;
+  const int M = h.M, W = h.W, nbegl = (int)std::count_if(h.qbegl.begin(), h.qbegl.end(), [](int b) { return b >= 0; });
+  // per-state bands of this call (cm_nohmm.rs:444-455)
+  std::vector<int32_t> bands(3 * (size_t)M);
+  for (int v = 0; v < M; v++) {
+    const int mn = dmin ? dmin[v] : 0, mx = dmax ? dmax[v] : W;
+    if (mn < 0 || mx < 0) {
+      g_err = "QDB scan: negative band";
+      return IXG_EINVAL;
+    }
+    const int base = h.sttype[v] == MP_ST ? 2 : 1;
+    bands[v] = std::min(std::max(mn, base), W);
+    bands[M + v] = std::min(mx, W);
+    bands[2 * M + v] = std::min(mn, W);
+  }
+  int32_t *d_bands;
+  RCHK(scratch(c, S_CM_BANDS, bands.size() * 4, (void **)&d_bands));
+  HIPCHK(hipMemcpyAsync(d_bands, bands.data(), bands.size() * 4, hipMemcpyHostToDevice, c->stream));
+  const QdbView qv = qview(cm);
+  const CmDev cd = cm->dev();
+  QdbDev q;
+  q.sttype = cd.sttype;
+  q.cfirst = cd.cfirst;
+  q.cnum = cd.cnum;
+  q.begl = qv.begl;
+  q.lvl_states = cd.lvl_states;
+  q.lvl_off = cd.lvl_off;
+  q.nlev = cd.nlev;
+  q.blist = qv.blist;
+  q.nb = (int)h.qblist.size();
+  q.dn = d_bands;
+  q.dxcap = d_bands + M;
+  q.dminw = d_bands + 2 * M;
+  q.ilog = qv.ilog;
+  q.M = M;
+  q.W = W;
+  q.nbegl = nbegl;
+  q.local = h.local_begin;
+  QdbTabs<float> tf{cd.tsc, cd.oesc, cd.endsc, cd.beginsc, qv.fbase0, qv.fel, qv.fbbase0};
+  QdbTabs<int32_t> ti{qv.itsc, qv.ioesc, qv.iendsc, qv.ibeginsc, qv.ibase0, qv.iel, qv.ibbase0};
+  const void *tabs = inside ? (const void *)&ti : (const void *)&tf;
+  const int64_t n = (int64_t)wins.size();
+  const int64_t budget = (int64_t)1 << 28;  // row and deck elements per launch group (1 GiB)
+  int64_t k0 = 0;
+  while (k0 < n) {
+    int64_t k1 = k0, elems = 0, res = 0;
+    std::vector<int64_t> at;
+    while (k1 < n) {
+      const int L = wins[k1].j0 - wins[k1].i0 + 1, w = std::min(W, L), W1 = w + 1;
+      const int64_t need = 2 * (int64_t)M * W1 + (int64_t)W1 * std::max(nbegl, 1) * W1;
+      if (k1 > k0 && elems + need > budget) break;
+      at.push_back(elems);
+      elems += need;
+      res += L;
+      k1++;
+    }
+    void *d_mx;
+    QdbWinDev *d_w;
+    RCHK(scratch(c, S_CM_ALPHA, (size_t)elems * 4, &d_mx));
+    RCHK(scratch(c, S_CM_ENV, (size_t)(k1 - k0) * sizeof(QdbWinDev), (void **)&d_w));
+    std::vector<QdbWinDev> wd(k1 - k0);
+    for (int64_t k = k0; k < k1; k++) {
+      QdbWinDev &x = wd[k - k0];
+      const int L = wins[k].j0 - wins[k].i0 + 1, w = std::min(W, L), W1 = w + 1;
+      x.dsq = d_base + wins[k].seq_off;
+      x.i0 = wins[k].i0;
+      x.j0 = wins[k].j0;
+      x.w = w;
+      x.win = (int32_t)k;
+      x.rows = (int32_t *)d_mx + at[k - k0];
+      x.begl = (int32_t *)d_mx + at[k - k0] + 2 * (int64_t)M * W1;
+    }
+    HIPCHK(hipMemcpyAsync(d_w, wd.data(), wd.size() * sizeof(QdbWinDev), hipMemcpyHostToDevice, c->stream));
+    int64_t cap = res / 8 + 4096;
+    std::vector<QdbCand> hc;
+    for (int attempt = 0; attempt < 2; attempt++) {
+      QdbCand *d_c;
+      unsigned long long *d_cnt;
+      RCHK(scratch(c, S_CM_ROOT, (size_t)cap * sizeof(QdbCand), (void **)&d_c));
+      RCHK(scratch(c, S_COUNTER, 8, (void **)&d_cnt));
+      static const unsigned long long zero = 0;
+      HIPCHK(hipMemcpyAsync(d_cnt, &zero, 8, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(ktime_begin(c));
+      HIPCHK(launch_qdb_scan(q, tabs, inside, d_w, k1 - k0, cutoff, d_c, cap, d_cnt, c->stream));
+      HIPCHK(ktime_end(c, IXG_K_QDB_SCAN));
+      unsigned long long cnt = 0;
+      HIPCHK(hipMemcpy(&cnt, d_cnt, 8, hipMemcpyDeviceToHost));
+      if ((int64_t)cnt > cap) {
+        cap = (int64_t)cnt + 1024;
+        continue;
+      }
+      hc.resize(cnt);
+      if (cnt) HIPCHK(hipMemcpy(hc.data(), d_c, cnt * sizeof(QdbCand), hipMemcpyDeviceToHost));
+      break;
+    }
+    std::vector<std::vector<HostCand>> per(k1 - k0);
+    for (const QdbCand &x : hc) per[x.win - k0].push_back({x.j, x.d, x.sc});
+    for (int64_t k = k0; k < k1; k++)
+      qdb_report(*cm, wins[k].host_dsq, wins[k].i0, wins[k].j0, per[k - k0], cutoff, do_null3, (int32_t)k, out);
+    k0 = k1;
+  }
+  return IXG_OK;
+}
+
+int check_window(const ixg_cm *cm, const uint8_t *dsq, int64_t L, int32_t i0, int32_t j0) {
+  if (!cm || !cm->d_q || !dsq || i0 < 1 || j0 > L || j0 < i0) {
+    g_err = "QDB scan: bad arguments (1 <= i0 <= j0 <= L, device CM)";
+    return IXG_EINVAL;
+  }
+  for (int64_t i = i0; i <= j0; i++)
+    if (dsq[i] >= KP) {
+      g_err = "digitized residue code >= 18 in the window";
+      return IXG_EINVAL;
+    }
+  return IXG_OK;
+}
+
+}  // namespace
+
+const uint8_t *ixg_target_device_base(const ixg_target *t, int64_t *L);
+const uint8_t *ixg_target_host_dsq(const ixg_target *t, int strand);
+
+extern "C" {
+
+int ixg_cm_itables(const ixg_cm *cm, int32_t *itsc, int32_t *ioesc, int32_t *ibeginsc, int32_t *iendsc,
+                   int32_t *iel_selfsc) {
+  if (!cm) return IXG_EINVAL;
+  const CmHost &h = cm->h;
+  if (itsc) memcpy(itsc, h.itsc.data(), h.itsc.size() * 4);
+  if (ioesc) memcpy(ioesc, h.ioesc.data(), h.ioesc.size() * 4);
+  if (ibeginsc) memcpy(ibeginsc, h.ibeginsc.data(), h.ibeginsc.size() * 4);
+  if (iendsc) memcpy(iendsc, h.iendsc.data(), h.iendsc.size() * 4);
+  if (iel_selfsc) *iel_selfsc = h.iel_selfsc;
+  return IXG_OK;
+}
+
+int ixg_qdb_scan(ixg_ctx *c, const ixg_cm *cm, const uint8_t *dsq, int64_t L, int32_t i0, int32_t j0,
+                 const int32_t *dmin, const int32_t *dmax, int inside, float cutoff, int do_null3, ixg_hit *hits,
+                 int64_t cap, int64_t *nhits) {
+  if (!c || !nhits || (cap > 0 && !hits)) return IXG_EINVAL;
+  RCHK(check_window(cm, dsq, L, i0, j0));
+  HIPCHK(hipSetDevice(c->device));
+  uint8_t *d;
+  RCHK(scratch(c, S_CM_SEQ, (size_t)L + 2, (void **)&d));
+  HIPCHK(hipMemcpyAsync(d, dsq, (size_t)L + 1, hipMemcpyHostToDevice, c->stream));
+  std::vector<ixg_hit> out;
+  RCHK(qdb_run(c, cm, d, {QWin{i0, j0, 0, dsq}}, dmin, dmax, inside ? 1 : 0, cutoff, do_null3, out));
+  *nhits = (int64_t)out.size();
+  for (int64_t k = 0; k < std::min<int64_t>(cap, out.size()); k++) hits[k] = out[k];
+  return (int64_t)out.size() > cap ? IXG_ERANGE : IXG_OK;
+}
+
+int ixg_qdb_scan_batch(ixg_ctx *c, const ixg_cm *cm, const ixg_target *tgt, const ixg_qdbwin *win, int64_t n,
+                       const int32_t *dmin, const int32_t *dmax, int inside, float cutoff, int do_null3,
+                       ixg_hit *hits, int64_t cap, int64_t *nhits) {
+  if (!c || !cm || !cm->d_q || !tgt || (n && !win) || !nhits || (cap > 0 && !hits)) {
+    g_err = "ixg_qdb_scan_batch: bad arguments";
+    return IXG_EINVAL;
+  }
+  HIPCHK(hipSetDevice(c->device));
+  int64_t L = 0;
+  const uint8_t *base = ixg_target_device_base(tgt, &L);
+  std::vector<QWin> w(n);
+  for (int64_t k = 0; k < n; k++) {
+    if (win[k].i0 < 1 || win[k].j0 > L || win[k].j0 < win[k].i0 || (win[k].strand != 0 && win[k].strand != 1)) {
+      g_err = "ixg_qdb_scan_batch: window outside the target";
+      return IXG_EINVAL;
+    }
+    w[k] = QWin{win[k].i0, win[k].j0, win[k].strand ? L + 2 : 0, ixg_target_host_dsq(tgt, win[k].strand)};
+  }
+  std::vector<ixg_hit> out;
+  RCHK(qdb_run(c, cm, base, w, dmin, dmax, inside ? 1 : 0, cutoff, do_null3, out));
+  *nhits = (int64_t)out.size();
+  for (int64_t k = 0; k < std::min<int64_t>(cap, out.size()); k++) hits[k] = out[k];
+  return (int64_t)out.size() > cap ? IXG_ERANGE : IXG_OK;
+}
+
+int ixg_cyk_seq_filter(ixg_ctx *c, const ixg_cm *cm, const uint8_t *dsq, int64_t n, const int32_t *dmin1,
+                       const int32_t *dmax1, float cutoff, int do_null3, int32_t *es, int32_t *ee, int64_t cap,
+                       int64_t *nenv) {
+  if (!c || !nenv || (cap > 0 && (!es || !ee)) || n < 1) return IXG_EINVAL;
+  RCHK(check_window(cm, dsq, n, 1, (int32_t)n));
+  HIPCHK(hipSetDevice(c->device));
+  uint8_t *d;
+  RCHK(scratch(c, S_CM_SEQ, (size_t)n + 2, (void **)&d));
+  HIPCHK(hipMemcpyAsync(d, dsq, (size_t)n + 1, hipMemcpyHostToDevice, c->stream));
+  std::vector<ixg_hit> hits;
+  RCHK(qdb_run(c, cm, d, {QWin{1, (int32_t)n, 0, dsq}}, dmin1, dmax1, 0, cutoff, do_null3, hits));
+  const int W = cm->h.W;
+  std::vector<std::pair<int32_t, int32_t>> pad;  // (end, start) for the sort
+  for (const ixg_hit &h : hits)
+    pad.push_back({std::min<int32_t>((int32_t)n, h.i + (W - 1)), std::max<int32_t>(1, h.j - (W - 1))});
+  std::sort(pad.begin(), pad.end());
+  int64_t no = 0;
+  for (size_t k = 0; k < pad.size(); k++) {
+    const int32_t iw = pad[k].second;
+    int32_t jw = pad[k].first;
+    while (k + 1 < pad.size() && pad[k + 1].second <= jw) {
+      k++;
+      jw = std::max(jw, pad[k].first);
+    }
+    if (no < cap) {
+      es[no] = iw;
+      ee[no] = jw;
+    }
+    no++;
+  }
+  *nenv = no;
+  return no > cap ? IXG_ERANGE : IXG_OK;
+}
+
+}  // extern "C"

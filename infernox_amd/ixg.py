@@ -58,9 +58,11 @@ EXPORTS = [
     "ixg_f3_filter_tasks", "ixg_last_timings", "ixg_cm_create", "ixg_cm_free", "ixg_cm_tables",
     "ixg_cyk_scan_hb", "ixg_inside_scan_hb", "ixg_hb_scan_batch", "ixg_cm_set_trpenalties", "ixg_cm_trtables",
     "ixg_trcyk_scan_hb", "ixg_trinside_scan_hb", "ixg_trhb_scan_batch", "ixg_kernel_timings",
-    "ixg_copy_probe",
+    "ixg_copy_probe", "ixg_cm_create_cfg", "ixg_cm_itables", "ixg_qdb_scan", "ixg_qdb_scan_batch",
+    "ixg_cyk_seq_filter",
 ]
-KERNELS = ["ssv_sweep", "ssv_resolve", "fwd", "bias", "vit", "msv", "hb_scan", "trhb_scan", "copy_probe"]
+KERNELS = ["ssv_sweep", "ssv_resolve", "fwd", "bias", "vit", "msv", "hb_scan", "trhb_scan", "copy_probe",
+           "qdb_scan"]
 
 
 class CmParams(C.Structure):
@@ -81,6 +83,10 @@ class Hit(C.Structure):
 
 class Envelope(C.Structure):
     _fields_ = [("i0", C.c_int32), ("j0", C.c_int32), ("strand", C.c_int32), ("pad", C.c_int32), ("bands", Bands)]
+
+
+class QdbWin(C.Structure):
+    _fields_ = [("i0", C.c_int32), ("j0", C.c_int32), ("strand", C.c_int32), ("pad", C.c_int32)]
 
 
 class TrValid(C.Structure):
@@ -108,6 +114,7 @@ def lib():
         L = C.CDLL(LIB_PATH)
         vp, i64, i32, f32p, u8p = C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_uint8)
         i64p = C.POINTER(C.c_int64)
+        i32p = C.POINTER(C.c_int32)
         L.ixg_last_error.restype = C.c_char_p
         L.ixg_init.argtypes = [C.c_int, C.c_size_t, C.POINTER(vp)]
         L.ixg_shutdown.argtypes = [vp]
@@ -134,6 +141,13 @@ def lib():
         L.ixg_last_timings.argtypes = [vp, C.POINTER(C.c_double), i64p]
         L.ixg_kernel_timings.argtypes = [vp, C.POINTER(C.c_double), i64p, C.c_int]
         L.ixg_copy_probe.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
+        L.ixg_cm_create_cfg.argtypes = [vp, C.POINTER(CmParams), C.c_int, C.POINTER(vp)]
+        L.ixg_cm_itables.argtypes = [vp, i32p, i32p, i32p, i32p, i32p]
+        L.ixg_qdb_scan.argtypes = [vp, vp, u8p, i64, i32, i32, i32p, i32p, C.c_int, C.c_float, C.c_int,
+                                   C.POINTER(Hit), i64, i64p]
+        L.ixg_qdb_scan_batch.argtypes = [vp, vp, vp, C.POINTER(QdbWin), i64, i32p, i32p, C.c_int, C.c_float,
+                                         C.c_int, C.POINTER(Hit), i64, i64p]
+        L.ixg_cyk_seq_filter.argtypes = [vp, vp, u8p, i64, i32p, i32p, C.c_float, C.c_int, i32p, i32p, i64, i64p]
         L.ixg_cm_create.argtypes = [vp, C.POINTER(CmParams), C.POINTER(vp)]
         L.ixg_cm_free.argtypes = [vp, vp]
         L.ixg_cm_tables.argtypes = [vp, f32p, f32p, f32p, f32p, f32p]
@@ -142,7 +156,6 @@ def lib():
                                          C.c_int, f32p, i64p, i64p, C.POINTER(Hit), i64, i64p]
         L.ixg_hb_scan_batch.argtypes = [vp, vp, vp, C.POINTER(Envelope), i64, C.c_int, C.c_float, C.c_int,
                                         C.c_float, C.c_int, f32p, i64p, i64p, C.POINTER(Hit), i64, i64p, i64p]
-        i32p = C.POINTER(C.c_int32)
         L.ixg_cm_set_trpenalties.argtypes = [vp, vp, f32p, f32p]
         L.ixg_cm_trtables.argtypes = [vp, f32p, f32p, i32p, i32p]
         for nm in ("ixg_trcyk_scan_hb", "ixg_trinside_scan_hb"):
@@ -406,12 +419,13 @@ def bands_struct(b):
 class CM:
     """Device-resident configured CM (ixg_cm_create); ctx=None builds host tables only."""
 
-    def __init__(self, ctx, cm):
+    def __init__(self, ctx, cm, local=True):
         self.ctx = ctx
         self.M = int(cm["M"])
         self.p = cm_params(cm)
         self.h = C.c_void_p()
-        check(lib().ixg_cm_create(ctx.h if ctx else None, C.byref(self.p), C.byref(self.h)), "ixg_cm_create")
+        check(lib().ixg_cm_create_cfg(ctx.h if ctx else None, C.byref(self.p), int(local), C.byref(self.h)),
+              "ixg_cm_create_cfg")
 
     def free(self):
         if self.h:
@@ -443,6 +457,38 @@ class CM:
                                        C.byref(ej), hits, cap, C.byref(n)), "ixg_inside_scan_hb")
         return vsc.value, ei.value, ej.value, [(hits[k].i, hits[k].j, hits[k].score, hits[k].bias)
                                                for k in range(n.value)]
+
+    def itables(self):
+        M = self.M
+        i32p = C.POINTER(C.c_int32)
+        itsc, ioesc = np.zeros(M * 6, np.int32), np.zeros(M * 324, np.int32)
+        ib, ie, el = np.zeros(M, np.int32), np.zeros(M, np.int32), np.zeros(1, np.int32)
+        check(lib().ixg_cm_itables(self.h, *[x.ctypes.data_as(i32p) for x in (itsc, ioesc, ib, ie, el)]))
+        return itsc.reshape(M, 6), ioesc.reshape(M, 324), ib, ie, int(el[0])
+
+    def qdb_scan(self, dsq, L, i0, j0, inside=False, dmin=None, dmax=None, cutoff=0.0, null3=True, cap=1 << 16):
+        """QDB (dmin/dmax) or non-banded CYK / integer Inside of one window: hits (i, j, score, bias)."""
+        d = np.ascontiguousarray(dsq, np.uint8)
+        i32p = C.POINTER(C.c_int32)
+        keep = [np.ascontiguousarray(x, np.int32) if x is not None else None for x in (dmin, dmax)]
+        ptr = [k.ctypes.data_as(i32p) if k is not None else None for k in keep]
+        hits = (Hit * cap)()
+        n = C.c_int64()
+        check(lib().ixg_qdb_scan(self.ctx.h, self.h, u8ptr(d), int(L), int(i0), int(j0), ptr[0], ptr[1], int(inside),
+                                 float(cutoff), int(null3), hits, cap, C.byref(n)), "ixg_qdb_scan")
+        return [(hits[k].i, hits[k].j, hits[k].score, hits[k].bias) for k in range(n.value)]
+
+    def cyk_seq_filter(self, dsq, n, dmin, dmax, cutoff, null3=True, cap=4096):
+        d = np.ascontiguousarray(dsq, np.uint8)
+        i32p = C.POINTER(C.c_int32)
+        keep = [np.ascontiguousarray(x, np.int32) if x is not None else None for x in (dmin, dmax)]
+        ptr = [k.ctypes.data_as(i32p) if k is not None else None for k in keep]
+        es, ee = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        k = C.c_int64()
+        check(lib().ixg_cyk_seq_filter(self.ctx.h, self.h, u8ptr(d), int(n), ptr[0], ptr[1], float(cutoff),
+                                       int(null3), es.ctypes.data_as(i32p), ee.ctypes.data_as(i32p), cap,
+                                       C.byref(k)), "ixg_cyk_seq_filter")
+        return list(zip(es[:k.value].tolist(), ee[:k.value].tolist()))
 
     def set_trpenalties(self, g, l):
         """Truncated-begin penalties g[3, M], l[3, M] (log2)."""
@@ -534,3 +580,20 @@ def trhb_scan_batch(ctx, cm, tgt, envs, pass_idx, local=True, inside=False, env_
                                     vm.ctypes.data_as(C.POINTER(C.c_int32)), i64ptr(ei), i64ptr(ej), hits, cap,
                                     C.byref(nh), C.byref(cells)), "ixg_trhb_scan_batch")
     return vsc[:n], vm[:n], ei[:n], ej[:n], _trhits(hits, nh.value), cells.value
+
+
+def qdb_scan_batch(ctx, cm, tgt, wins, inside=False, dmin=None, dmax=None, cutoff=0.0, null3=True):
+    """wins: list of (i0, j0, strand). Returns hits (env, i, j, score, bias)."""
+    n = len(wins)
+    arr = (QdbWin * max(n, 1))()
+    for k, (i0, j0, strand) in enumerate(wins):
+        arr[k].i0, arr[k].j0, arr[k].strand, arr[k].pad = int(i0), int(j0), int(strand), 0
+    i32p = C.POINTER(C.c_int32)
+    keep = [np.ascontiguousarray(x, np.int32) if x is not None else None for x in (dmin, dmax)]
+    ptr = [k.ctypes.data_as(i32p) if k is not None else None for k in keep]
+    cap = 1 << 16
+    hits = (Hit * cap)()
+    nh = C.c_int64()
+    check(lib().ixg_qdb_scan_batch(ctx.h, cm.h, tgt.h, arr, n, ptr[0], ptr[1], int(inside), float(cutoff), int(null3),
+                                   hits, cap, C.byref(nh)), "ixg_qdb_scan_batch")
+    return [(hits[k].env, hits[k].i, hits[k].j, hits[k].score, hits[k].bias) for k in range(nh.value)]

@@ -54,9 +54,15 @@ static int is_exit_node(const ixo_cm *cm, int nd) { /* cp9.rs:4231-4236 */
          cm->ndtype[nd + 1] != IXO_END_ND;
 }
 
-/* cp9.rs:4194-4256 cm_config_local + cp9.rs:4263-4386 cm_logoddsify +
- * cp9.rs:4538-4545 cm_configure_scores. */
-int ixo_cm_configure(const ixo_cm *cm, ixo_cmsc *s) {
+static void int_scores(const ixo_cm *cm, const float *t, const float *begin, const float *end, float el,
+                       ixo_cmisc *is);
+
+/* cp9.rs:4194-4256 cm_config_local (local != 0) + cp9.rs:4263-4386
+ * cm_logoddsify + the EL bound of cp9.rs:4538-4545 cm_configure_scores; with
+ * local == 0 the global configuration of cm_nohmm.rs:264
+ * cm_configure_scores_global (no local begins or ends). With is != NULL also
+ * the integer scores of cm_nohmm.rs:279 build_integer_scores. */
+int ixo_cm_configure_full(const ixo_cm *cm, int local, ixo_cmsc *s, ixo_cmisc *is) {
   const int M = cm->M;
   memset(s, 0, sizeof(*s));
   s->M = M;
@@ -68,34 +74,36 @@ int ixo_cm_configure(const ixo_cm *cm, ixo_cmsc *s) {
   float *begin = (float *)calloc((size_t)M, sizeof(float));
   float *end = (float *)calloc((size_t)M, sizeof(float));
   memcpy(t, cm->t, sizeof(float) * (size_t)M * MAXC);
-  /* local begins */
-  int nstarts = 0;
-  for (int nd = 2; nd < cm->nodes; nd++) {
-    int ty = cm->ndtype[nd];
-    if (ty == IXO_MATP_ND || ty == IXO_MATL_ND || ty == IXO_MATR_ND || ty == IXO_BIF_ND) nstarts++;
-  }
-  begin[cm->nodemap[1]] = 1.0f - cm->pbegin;
-  float pb = cm->pbegin / (float)nstarts;
-  for (int nd = 2; nd < cm->nodes; nd++) {
-    int ty = cm->ndtype[nd];
-    if (ty == IXO_MATP_ND || ty == IXO_MATL_ND || ty == IXO_MATR_ND || ty == IXO_BIF_ND) begin[cm->nodemap[nd]] = pb;
-  }
-  for (int y = 0; y < cm->cnum[0]; y++) t[y] = 0.0f;
-  /* local ends */
-  int nexits = 0;
-  for (int nd = 1; nd < cm->nodes; nd++)
-    if (is_exit_node(cm, nd)) nexits++;
-  for (int nd = 1; nd < cm->nodes; nd++)
-    if (is_exit_node(cm, nd)) {
-      int v = cm->nodemap[nd];
-      end[v] = cm->pend / (float)nexits;
-      int cn = cm->cnum[v];
-      float denom = vec_fsum(t + (size_t)v * MAXC, cn);
-      denom += end[v];
-      float inv = 1.0f / denom;
-      for (int x = 0; x < cn; x++) t[(size_t)v * MAXC + x] *= inv;
+  if (local) {
+    /* local begins */
+    int nstarts = 0;
+    for (int nd = 2; nd < cm->nodes; nd++) {
+      int ty = cm->ndtype[nd];
+      if (ty == IXO_MATP_ND || ty == IXO_MATL_ND || ty == IXO_MATR_ND || ty == IXO_BIF_ND) nstarts++;
     }
-  s->local_begin = 1;
+    begin[cm->nodemap[1]] = 1.0f - cm->pbegin;
+    float pb = cm->pbegin / (float)nstarts;
+    for (int nd = 2; nd < cm->nodes; nd++) {
+      int ty = cm->ndtype[nd];
+      if (ty == IXO_MATP_ND || ty == IXO_MATL_ND || ty == IXO_MATR_ND || ty == IXO_BIF_ND) begin[cm->nodemap[nd]] = pb;
+    }
+    for (int y = 0; y < cm->cnum[0]; y++) t[y] = 0.0f;
+    /* local ends */
+    int nexits = 0;
+    for (int nd = 1; nd < cm->nodes; nd++)
+      if (is_exit_node(cm, nd)) nexits++;
+    for (int nd = 1; nd < cm->nodes; nd++)
+      if (is_exit_node(cm, nd)) {
+        int v = cm->nodemap[nd];
+        end[v] = cm->pend / (float)nexits;
+        int cn = cm->cnum[v];
+        float denom = vec_fsum(t + (size_t)v * MAXC, cn);
+        denom += end[v];
+        float inv = 1.0f / denom;
+        for (int x = 0; x < cn; x++) t[(size_t)v * MAXC + x] *= inv;
+      }
+  }
+  s->local_begin = local ? 1 : 0;
   /* el_selfsc adjustment (cp9.rs:4541-4543) */
   s->el_selfsc = cm->el_selfsc;
   if (s->el_selfsc * (float)cm->W < IMPOSSIBLE) s->el_selfsc = IMPOSSIBLE / ((float)cm->W + 1.0f);
@@ -146,10 +154,88 @@ int ixo_cm_configure(const ixo_cm *cm, ixo_cmsc *s) {
     s->beginsc[v] = (float)sre_log2((double)begin[v]);
     s->endsc[v] = (float)sre_log2((double)end[v]);
   }
+  if (is) int_scores(cm, t, begin, end, s->el_selfsc, is);
   free(t);
   free(begin);
   free(end);
   return 0;
+}
+
+int ixo_cm_configure(const ixo_cm *cm, ixo_cmsc *s) { return ixo_cm_configure_full(cm, 1, s, NULL); }
+
+#define INFTY 987654321 /* cp9.rs:21 */
+/* cm_nohmm.rs:27-33 prob2score: floor(0.5 + 1000 log2(p / null)), -INFTY for p == 0 */
+static int32_t prob2score(float p, float null) {
+  if (p == 0.0f) return -INFTY;
+  return (int32_t)floor(0.5 + 1000.0 * log2((double)p / (double)null));
+}
+/* cm.rs:104-120 abc_iavg_score: mean over a degenerate code, rounded half away */
+static int32_t iavg_score(int x, const int32_t *sc) {
+  if (!is_residue(x)) return 0;
+  float r = 0.0f;
+  for (int i = 0; i < K4; i++)
+    if (DEGEN[x][i]) r += (float)sc[i];
+  r /= (float)ndegen(x);
+  return r < 0.0f ? (int32_t)(r - 0.5f) : (int32_t)(r + 0.5f);
+}
+
+/* cm_nohmm.rs:279-368 build_integer_scores */
+static void int_scores(const ixo_cm *cm, const float *t, const float *begin, const float *end, float el,
+                       ixo_cmisc *is) {
+  const int M = cm->M;
+  is->itsc = (int32_t *)calloc((size_t)M * MAXC, sizeof(int32_t));
+  is->ioesc = (int32_t *)malloc(sizeof(int32_t) * (size_t)M * KP * KP);
+  is->ibeginsc = (int32_t *)malloc(sizeof(int32_t) * (size_t)M);
+  is->iendsc = (int32_t *)malloc(sizeof(int32_t) * (size_t)M);
+  for (int v = 0; v < M; v++) {
+    const int st = cm->sttype[v];
+    int32_t *oe = is->ioesc + (size_t)v * KP * KP;
+    for (int i = 0; i < KP * KP; i++) oe[i] = 0;
+    if (st != IXO_B_ST && st != IXO_E_ST)
+      for (int x = 0; x < cm->cnum[v]; x++) is->itsc[(size_t)v * MAXC + x] = prob2score(t[(size_t)v * MAXC + x], 1.0f);
+    const float *e = cm->e + (size_t)v * 16;
+    if (st == IXO_MP_ST) {
+      int32_t ie[16];
+      for (int a = 0; a < K4; a++)
+        for (int b = 0; b < K4; b++) ie[a * 4 + b] = prob2score(e[a * 4 + b], cm->null[a] * cm->null[b]);
+      for (int i = 0; i < KP * KP; i++) oe[i] = -INFTY;
+      for (int a = 0; a < K4; a++)
+        for (int b = 0; b < K4; b++) oe[a * KP + b] = ie[a * 4 + b];
+      for (int a = K4 + 1; a < KP - 1; a++)
+        for (int b = 0; b < K4; b++) {
+          float sc = 0.0f;
+          for (int l = 0; l < K4; l++) sc += (float)ie[l * 4 + b] * fcount_frac(a, l);
+          oe[a * KP + b] = (int32_t)sc;
+        }
+      for (int a = 0; a < K4; a++)
+        for (int b = K4 + 1; b < KP - 1; b++) {
+          float sc = 0.0f;
+          for (int r = 0; r < K4; r++) sc += (float)ie[a * 4 + r] * fcount_frac(b, r);
+          oe[a * KP + b] = (int32_t)sc;
+        }
+      for (int a = K4 + 1; a < KP - 1; a++)
+        for (int b = K4 + 1; b < KP - 1; b++) {
+          float sc = 0.0f;
+          for (int l = 0; l < K4; l++)
+            for (int r = 0; r < K4; r++) sc += (float)ie[l * 4 + r] * fcount_frac(a, l) * fcount_frac(b, r);
+          oe[a * KP + b] = (int32_t)sc;
+        }
+    } else if (st == IXO_ML_ST || st == IXO_MR_ST || st == IXO_IL_ST || st == IXO_IR_ST) {
+      int32_t ie[4];
+      for (int a = 0; a < K4; a++) ie[a] = prob2score(e[a], cm->null[a]);
+      for (int a = 0; a < KP; a++) oe[a] = -INFTY;
+      for (int a = 0; a < K4; a++) oe[a] = ie[a];
+      for (int a = K4 + 1; a < KP - 1; a++) oe[a] = iavg_score(a, ie);
+    }
+    is->ibeginsc[v] = prob2score(begin[v], 1.0f);
+    is->iendsc[v] = prob2score(end[v], 1.0f);
+  }
+  is->iel_selfsc = prob2score(powf(2.0f, el), 1.0f);
+}
+
+void ixo_cm_free_iscores(ixo_cmisc *is) {
+  free(is->itsc); free(is->ioesc); free(is->ibeginsc); free(is->iendsc);
+  memset(is, 0, sizeof(*is));
 }
 
 void ixo_cm_free_scores(ixo_cmsc *s) {

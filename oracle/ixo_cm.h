@@ -55,6 +55,17 @@ typedef struct {
 typedef struct { int32_t i, j; float score, bias; } ixo_hit;
 
 int  ixo_cm_configure(const ixo_cm *cm, ixo_cmsc *out);
+/* Integer scores of the QDB Inside scanner (cm_nohmm.rs:279 build_integer_scores). */
+typedef struct {
+  int32_t *itsc;     /* M*6 */
+  int32_t *ioesc;    /* M*18*18 */
+  int32_t *ibeginsc, *iendsc; /* M */
+  int32_t iel_selfsc;
+} ixo_cmisc;
+/* local != 0: the local configuration of ixo_cm_configure; local == 0: the
+ * global one (cm_nohmm.rs:264). is (may be NULL) receives integer scores. */
+int  ixo_cm_configure_full(const ixo_cm *cm, int local, ixo_cmsc *out, ixo_cmisc *is);
+void ixo_cm_free_iscores(ixo_cmisc *is);
 void ixo_cm_free_scores(ixo_cmsc *s);
 void ixo_ij2d_bands(const int8_t *sttype, int M, const int32_t *imin, const int32_t *imax, const int32_t *jmin,
                     const int32_t *jmax, int do_trunc, int32_t *hd_off, int32_t *hdmin, int32_t *hdmax);
@@ -97,6 +108,21 @@ int64_t ixo_trhb_scan(const ixo_cm *cm, const ixo_cmsc *sc, const float *lmesc, 
                       const ixo_bands *b, const ixo_trvalid *val, const uint8_t *dsq, int i0, int j0, int pass_idx,
                       int local, int inside, float cutoff, int do_null3, float env_cutoff, float *vsc, int *vmode,
                       int64_t *envi, int64_t *envj, ixo_trhit *hits, int64_t cap, float *planes_out);
+
+/* ---- QDB / non-banded scanners (qdb.c) ---- */
+
+/* cm_nohmm.rs:406 generic_scan over dsq[i0..j0]: CYK on the float scores
+ * (inside = 0) or Inside on the integer scores (inside = 1), with per-state
+ * d bands dmin/dmax (NULL: non-banded, 0..W). Hits after greedy reporting and
+ * overlap removal; returns their number (beyond cap: counted only).
+ * root_out (optional): root scores in bits per (j, d), [(j-i0)*(W+1) + d]. */
+int64_t ixo_qdb_scan(const ixo_cm *cm, const ixo_cmsc *fs, const ixo_cmisc *is, const int32_t *dmin,
+                     const int32_t *dmax, const uint8_t *dsq, int i0, int j0, int inside, float cutoff, int do_null3,
+                     ixo_hit *out, int64_t cap, float *root_out);
+/* cm_nohmm.rs:855 cyk_seq_filter over dsq[1..n]: envelopes (es, ee). */
+int64_t ixo_cyk_seq_filter(const ixo_cm *cm, const ixo_cmsc *fs, const int32_t *dmin1, const int32_t *dmax1,
+                           const uint8_t *dsq, int n, float cutoff, int do_null3, int32_t *es, int32_t *ee,
+                           int64_t cap);
 
 #ifdef __cplusplus
 }

@@ -23,6 +23,10 @@ class CmSc(C.Structure):
                 ("oesc", _f32p), ("beginsc", _f32p), ("endsc", _f32p)]
 
 
+class CmISc(C.Structure):
+    _fields_ = [("itsc", _i32p), ("ioesc", _i32p), ("ibeginsc", _i32p), ("iendsc", _i32p), ("iel_selfsc", C.c_int32)]
+
+
 class Bands(C.Structure):
     _fields_ = [("jmin", _i32p), ("jmax", _i32p), ("hd_off", _i32p), ("hdmin", _i32p), ("hdmax", _i32p)]
 
@@ -70,6 +74,14 @@ def lib():
                                     C.c_int, C.c_float, _f32p, C.POINTER(C.c_int), C.POINTER(C.c_int64),
                                     C.POINTER(C.c_int64), C.POINTER(TrHit), C.c_int64, _f32p]
         L.ixo_trhb_scan.restype = C.c_int64
+        L.ixo_cm_configure_full.argtypes = [C.POINTER(Cm), C.c_int, C.POINTER(CmSc), C.POINTER(CmISc)]
+        L.ixo_cm_free_iscores.argtypes = [C.POINTER(CmISc)]
+        L.ixo_qdb_scan.argtypes = [C.POINTER(Cm), C.POINTER(CmSc), C.POINTER(CmISc), _i32p, _i32p, _u8p, C.c_int,
+                                   C.c_int, C.c_int, C.c_float, C.c_int, C.POINTER(Hit), C.c_int64, _f32p]
+        L.ixo_qdb_scan.restype = C.c_int64
+        L.ixo_cyk_seq_filter.argtypes = [C.POINTER(Cm), C.POINTER(CmSc), _i32p, _i32p, _u8p, C.c_int, C.c_float,
+                                         C.c_int, _i32p, _i32p, C.c_int64]
+        L.ixo_cyk_seq_filter.restype = C.c_int64
         _ready = True
     return L
 
@@ -226,3 +238,61 @@ class OracleTrCM(OracleCM):
         if want_planes:
             out["planes"] = planes[:4 * n].reshape(4, n).copy()
         return out
+
+
+class OracleQdbCM:
+    """Float and integer scores of a CM in local or global configuration and
+    the QDB / non-banded scanners."""
+
+    def __init__(self, cm, local=False):
+        self.cm = cm
+        self.s = cm_struct(cm)
+        self.fs, self.isc = CmSc(), CmISc()
+        lib().ixo_cm_configure_full(C.byref(self.s), int(local), C.byref(self.fs), C.byref(self.isc))
+        M = cm["M"]
+        self.tsc = np.ctypeslib.as_array(self.fs.tsc, shape=(M, 6)).copy()
+        self.oesc = np.ctypeslib.as_array(self.fs.oesc, shape=(M, 324)).copy()
+        self.beginsc = np.ctypeslib.as_array(self.fs.beginsc, shape=(M,)).copy()
+        self.endsc = np.ctypeslib.as_array(self.fs.endsc, shape=(M,)).copy()
+        self.el_selfsc = self.fs.el_selfsc
+        self.itsc = np.ctypeslib.as_array(self.isc.itsc, shape=(M, 6)).copy()
+        self.ioesc = np.ctypeslib.as_array(self.isc.ioesc, shape=(M, 324)).copy()
+        self.ibeginsc = np.ctypeslib.as_array(self.isc.ibeginsc, shape=(M,)).copy()
+        self.iendsc = np.ctypeslib.as_array(self.isc.iendsc, shape=(M,)).copy()
+        self.iel_selfsc = self.isc.iel_selfsc
+
+    def __del__(self):
+        try:
+            lib().ixo_cm_free_scores(C.byref(self.fs))
+            lib().ixo_cm_free_iscores(C.byref(self.isc))
+        except Exception:
+            pass
+
+    def scan(self, dsq, i0, j0, inside, dmin=None, dmax=None, cutoff=0.0, do_null3=True, want_root=False):
+        d = np.ascontiguousarray(dsq, np.uint8)
+        W = int(self.cm["W"])
+        root = np.zeros((j0 - i0 + 1) * (W + 1), np.float32) if want_root else None
+        keep = [np.ascontiguousarray(x, np.int32) if x is not None else None for x in (dmin, dmax)]
+        cap = 1 << 16
+        out = (Hit * cap)()
+        n = lib().ixo_qdb_scan(C.byref(self.s), C.byref(self.fs), C.byref(self.isc),
+                               keep[0].ctypes.data_as(_i32p) if keep[0] is not None else None,
+                               keep[1].ctypes.data_as(_i32p) if keep[1] is not None else None,
+                               d.ctypes.data_as(_u8p), i0, j0, int(inside), cutoff, int(do_null3), out, cap,
+                               root.ctypes.data_as(_f32p) if want_root else None)
+        hits = [(out[k].i, out[k].j, out[k].score, out[k].bias) for k in range(min(n, cap))]
+        if want_root:
+            return hits, root.reshape(j0 - i0 + 1, W + 1)
+        return hits
+
+    def seq_filter(self, dsq, n, dmin, dmax, cutoff, do_null3=True):
+        d = np.ascontiguousarray(dsq, np.uint8)
+        keep = [np.ascontiguousarray(x, np.int32) if x is not None else None for x in (dmin, dmax)]
+        cap = 4096
+        es, ee = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        k = lib().ixo_cyk_seq_filter(C.byref(self.s), C.byref(self.fs),
+                                     keep[0].ctypes.data_as(_i32p) if keep[0] is not None else None,
+                                     keep[1].ctypes.data_as(_i32p) if keep[1] is not None else None,
+                                     d.ctypes.data_as(_u8p), n, cutoff, int(do_null3), es.ctypes.data_as(_i32p),
+                                     ee.ctypes.data_as(_i32p), cap)
+        return list(zip(es[:k].tolist(), ee[:k].tolist()))

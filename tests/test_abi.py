@@ -149,3 +149,19 @@ def test_truncated_scan_rejects_bad_arguments():
     h = C.c_void_p()
     assert L.ixg_cm_create(None, C.byref(p), C.byref(h)) == ixg.IXG_EINVAL
     g.free()
+
+
+@pytest.mark.parametrize("local", [True, False])
+def test_cm_integer_and_global_tables_equal_oracle(local):
+    from infernox_amd import synth_cm as S
+    from oracle_cm_bind import OracleQdbCM
+    cm = S.build_cm(S.multibranch(120, 4, 2), 2)
+    g = ixg.CM(None, cm, local=local)
+    tsc, oesc, bs, es, el = g.tables()
+    itsc, ioesc, ib, ie, iel = g.itables()
+    o = OracleQdbCM(cm, local=local)
+    assert tsc.tobytes() == o.tsc.tobytes() and oesc.tobytes() == o.oesc.tobytes()
+    assert bs.tobytes() == o.beginsc.tobytes() and es.tobytes() == o.endsc.tobytes()
+    assert np.array_equal(itsc, o.itsc) and np.array_equal(ioesc, o.ioesc)
+    assert np.array_equal(ib, o.ibeginsc) and np.array_equal(ie, o.iendsc) and iel == o.iel_selfsc
+    g.free()
