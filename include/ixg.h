@@ -106,6 +106,22 @@ int ixg_bias_scores(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *seq, int64_t 
 int ixg_msv_scores(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *seq, int64_t seq_len,
                    const int64_t *off, const int64_t *len, int64_t nwin, float *sc, uint8_t *ovf);
 
+/* p7_fwdback.rs:301 p7_forward(&ForwardFilter, &XFactors, &[u8], l) -> Omx and
+ * :529 forward_score: the full-matrix odds-space Forward over dsq[1..L]
+ * (1-indexed, codes < 18) in the multihit (unihit = 0, XFactors::multihit) or
+ * unihit (XFactors::unihit) length model. Writes the retained rows 0..L in
+ * natural order, (L+1)*(M+1) floats each for mmx, imx, dmx, the specials
+ * E N J B C SCALE per row ((L+1)*6 floats), totscale and the score (nats). */
+int ixg_p7_forward(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *dsq, int64_t L, int unihit, float *mmx,
+                   float *imx, float *dmx, float *xmx, float *totscale, float *sc);
+/* Batched form: n regions of seq (region r = seq[off[r]+1 .. off[r]+len[r]],
+ * off[r] + len[r] < seq_len); the matrices of region r follow those of the
+ * regions before it in mat (M | I | D, 3*(len+1)*(M+1) floats per region)
+ * and its specials in xmx (6*(len+1) floats per region). */
+int ixg_p7_forward_batch(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *seq, int64_t seq_len, const int64_t *off,
+                         const int64_t *len, int64_t n, int unihit, float *mat, float *xmx, float *sc,
+                         float *totscale);
+
 /* ---- the filter cascade -------------------------------------------------------- */
 #define IXG_DO_MSV      (1 << 0)
 #define IXG_DO_FWD      (1 << 1)
@@ -310,7 +326,8 @@ int ixg_cyk_seq_filter(ixg_ctx *ctx, const ixg_cm *cm, const uint8_t *dsq, int64
  * counterpart in the reference. Timing synchronizes the context's stream
  * after each launch. */
 enum { IXG_K_SSV_SWEEP = 0, IXG_K_SSV_RESOLVE = 1, IXG_K_FWD = 2, IXG_K_BIAS = 3, IXG_K_VIT = 4, IXG_K_MSV = 5,
-       IXG_K_HB_SCAN = 6, IXG_K_TRHB_SCAN = 7, IXG_K_COPY = 8, IXG_K_QDB_SCAN = 9, IXG_NKERNELS = 10 };
+       IXG_K_HB_SCAN = 6, IXG_K_TRHB_SCAN = 7, IXG_K_COPY = 8, IXG_K_QDB_SCAN = 9, IXG_K_FWD_FULL = 10,
+       IXG_NKERNELS = 11 };
 int ixg_kernel_timings(ixg_ctx *ctx, double *ms, int64_t *launches, int reset);
 /* Measurement hook (no counterpart in the reference): copies `bytes`
  * device-to-device with a plain kernel reps times after one warm-up copy;

@@ -842,6 +842,77 @@ int ixg_msv_scores(ixg_ctx *c, const ixg_p7 *p, const uint8_t *seq, int64_t seq_
   return scores_entry(c, p, seq, seq_len, off, len, nwin, sc, ovf, 3);
 }
 
+int ixg_p7_forward_batch(ixg_ctx *c, const ixg_p7 *p, const uint8_t *seq, int64_t seq_len, const int64_t *off,
+                         const int64_t *len, int64_t n, int unihit, float *mat, float *xmx, float *sc,
+                         float *totscale) {
+  if (!c || !p || !seq || (n && (!off || !len)) || n < 0 || !mat || !xmx || !sc || !totscale) return IXG_EINVAL;
+  if (!p->d_fwd) {
+    g_err = "ixg_p7_forward: host-only profile";
+    return IXG_EINVAL;
+  }
+  const int M = p->fwd.m;
+  std::vector<int64_t> moff(n), xoff(n);
+  int64_t mtot = 0, xtot = 0;
+  for (int64_t i = 0; i < n; i++) {
+    if (off[i] < 0 || len[i] < 1 || off[i] + len[i] >= seq_len || len[i] > INT32_MAX) {
+      g_err = "region outside the sequence buffer";
+      return IXG_EINVAL;
+    }
+    if (!codes_ok(seq + off[i] + 1, len[i])) return IXG_EINVAL;
+    moff[i] = mtot;
+    xoff[i] = xtot;
+    mtot += 3 * (len[i] + 1) * (int64_t)(M + 1);
+    xtot += 6 * (len[i] + 1);
+  }
+  HIPCHK(hipSetDevice(c->device));
+  uint8_t *d;
+  RCHK(scratch(c, S_SEQ, (size_t)seq_len + 2, (void **)&d));
+  HIPCHK(hipMemcpyAsync(d, seq, seq_len, hipMemcpyHostToDevice, c->stream));
+  std::vector<WinDev> wd(n);
+  for (int64_t i = 0; i < n; i++) {
+    wd[i].off = off[i];
+    wd[i].len = (int32_t)len[i];
+    wd[i].idx = (int32_t)i;
+    wd[i].a = wd[i].b = 0.f;
+  }
+  std::vector<int64_t> offs(moff);
+  offs.insert(offs.end(), xoff.begin(), xoff.end());
+  WinDev *d_w;
+  int64_t *d_off;
+  float *d_mat, *d_x, *d_out;
+  RCHK(upload(c, S_WINS, wd, &d_w));
+  RCHK(upload(c, S_TASKS, offs, &d_off));
+  RCHK(scratch(c, S_CM_ALPHA, (size_t)std::max<int64_t>(mtot, 1) * 4, (void **)&d_mat));
+  RCHK(scratch(c, S_CM_ROOT, (size_t)std::max<int64_t>(xtot, 1) * 4, (void **)&d_x));
+  RCHK(scratch(c, S_SC0, (size_t)std::max<int64_t>(2 * n, 1) * 4, (void **)&d_out));
+  HIPCHK(ktime_begin(c));
+  HIPCHK(launch_fwd_full(p->fwd_dev(), d, d_w, n, d_off, d_off + n, d_mat, d_x, d_out, unihit ? 1 : 0, c->stream));
+  HIPCHK(ktime_end(c, IXG_K_FWD_FULL));
+  std::vector<float> o(2 * n);
+  if (n) HIPCHK(hipMemcpyAsync(o.data(), d_out, 2 * n * 4, hipMemcpyDeviceToHost, c->stream));
+  if (mtot) HIPCHK(hipMemcpyAsync(mat, d_mat, mtot * 4, hipMemcpyDeviceToHost, c->stream));
+  if (xtot) HIPCHK(hipMemcpyAsync(xmx, d_x, xtot * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int64_t i = 0; i < n; i++) {
+    sc[i] = o[2 * i];
+    totscale[i] = o[2 * i + 1];
+  }
+  return IXG_OK;
+}
+
+int ixg_p7_forward(ixg_ctx *c, const ixg_p7 *p, const uint8_t *dsq, int64_t L, int unihit, float *mmx, float *imx,
+                   float *dmx, float *xmx, float *totscale, float *sc) {
+  if (!c || !p || !dsq || L < 1 || !mmx || !imx || !dmx || !xmx || !totscale || !sc) return IXG_EINVAL;
+  const int64_t off = 0, len = L;
+  const size_t plane = (size_t)(L + 1) * (p->fwd.m + 1);
+  std::vector<float> mat(3 * plane);
+  RCHK(ixg_p7_forward_batch(c, p, dsq, L + 1, &off, &len, 1, unihit, mat.data(), xmx, sc, totscale));
+  memcpy(mmx, mat.data(), plane * 4);
+  memcpy(imx, mat.data() + plane, plane * 4);
+  memcpy(dmx, mat.data() + 2 * plane, plane * 4);
+  return IXG_OK;
+}
+
 static int emit(const std::vector<std::vector<Win>> &res, ixg_window *out, int64_t cap, int64_t *n) {
   int64_t k = 0;
   for (size_t t = 0; t < res.size(); t++)

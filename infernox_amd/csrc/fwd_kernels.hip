@@ -245,6 +245,161 @@ __global__ __launch_bounds__(64) void fwd_generic(FwdDev p, const uint8_t *seq, 
   if (widx < nwin && z == 0) sc[w.idx] = (float)((double)totscale + log((double)xc * (double)pmove));
 }
 
+// Full-matrix Forward (p7_fwdback.rs:301-532 p7_forward + forward_score):
+// the fwd_generic recurrence with the length model of either the multihit or
+// the unihit configuration, every row's M, I and D (after rescaling) stored
+// in natural order k = 0..M at mat + moff[w] (M | I | D, (len+1)*(M+1)
+// each), the specials E N J B C SCALE per row at xmx + xoff[w], and score
+// and totscale at out[2*idx], out[2*idx+1].
+__global__ __launch_bounds__(64) void fwd_full(FwdDev p, const uint8_t *seq, const WinDev *wins, int64_t nwin,
+                                               const int64_t *moff, const int64_t *xoff, float *mat, float *xmx,
+                                               float *out, int unihit) {
+  extern __shared__ float col[];  // [3][nq][64]
+  const int lane = threadIdx.x, z = lane & 3;
+  const int64_t widx = (int64_t)blockIdx.x * 16 + (lane >> 2);
+  WinDev w;
+  w.off = 0;
+  w.len = 0;
+  w.idx = 0;
+  w.a = 0.f;
+  w.b = 0.f;
+  if (widx < nwin) w = wins[widx];
+  const int nq = p.nq, M = p.m;
+  float *mm = col, *dm = col + nq * 64, *im = col + 2 * nq * 64;
+  for (int q = 0; q < nq; q++) mm[q * 64 + lane] = dm[q * 64 + lane] = im[q * 64 + lane] = 0.f;
+  int maxL = w.len;
+  for (int off = 32; off >= 1; off >>= 1) maxL = max(maxL, __shfl_xor(maxL, off));
+  const float L = (float)w.len;
+  const float nj = unihit ? 0.0f : 1.0f;
+  const float pmove = (2.0f + nj) / (L + 2.0f + nj);  // p7_fwdback.rs:212-252
+  const float ploop = 1.0f - pmove;
+  const float emove = unihit ? 1.0f : 0.5f, eloop = unihit ? 0.0f : 0.5f;
+  float xn = 1.0f, xj = 0.0f, xb = pmove, xc = 0.0f, totscale = 0.0f;
+  const size_t rowlen = (size_t)M + 1, plane = ((size_t)w.len + 1) * rowlen;
+  float *om = mat + (widx < nwin ? moff[w.idx] : 0), *oi = om + plane, *od = om + 2 * plane;
+  float *ox = xmx + (widx < nwin ? xoff[w.idx] : 0);
+  if (widx < nwin) {  // row 0 (p7_fwdback.rs:314-322)
+    for (int k = z; k <= M; k += 4) om[k] = oi[k] = od[k] = 0.f;
+    if (z == 0) {
+      ox[0] = 0.f;
+      ox[1] = 1.f;
+      ox[2] = 0.f;
+      ox[3] = pmove;
+      ox[4] = 0.f;
+      ox[5] = 1.f;
+    }
+  }
+  const float *tm = p.tmain;
+  for (int i = 1; i <= maxL; i++) {
+    const bool act = i <= w.len;
+    const int x = act ? (int)seq[w.off + i] : 0;
+    const float *rp = p.rfvv + (size_t)x * (nq * 4);
+    float dcv = 0.f, xev = 0.f;
+    const float xbv = xb;
+    float mpv = quad_rsz(mm[(nq - 1) * 64 + lane], z);
+    float dpv = quad_rsz(dm[(nq - 1) * 64 + lane], z);
+    float ipv = quad_rsz(im[(nq - 1) * 64 + lane], z);
+    for (int q = 0; q < nq; q++) {
+      const float *tb = tm + q * 28 + z;
+      float sv = xbv * tb[0];
+      sv = sv + mpv * tb[4];
+      sv = sv + ipv * tb[8];
+      sv = sv + dpv * tb[12];
+      sv = sv * rp[q * 4 + z];
+      xev = xev + sv;
+      mpv = mm[q * 64 + lane];
+      dpv = dm[q * 64 + lane];
+      ipv = im[q * 64 + lane];
+      mm[q * 64 + lane] = sv;
+      dm[q * 64 + lane] = dcv;
+      dcv = sv * tb[16];
+      const float svi = mpv * tb[20];
+      im[q * 64 + lane] = svi + ipv * tb[24];
+    }
+    dcv = quad_rsz(dcv, z);
+    dm[lane] = 0.f;
+    for (int q = 0; q < nq; q++) {
+      const float d = dcv + dm[q * 64 + lane];
+      dm[q * 64 + lane] = d;
+      dcv = d * p.tddv[q * 4 + z];
+    }
+    bool done = false;
+    for (int j = 1; j < 4; j++) {
+      dcv = quad_rsz(dcv, z);
+      int cv = 0;
+      if (!done) {
+        for (int q = 0; q < nq; q++) {
+          const float old = dm[q * 64 + lane];
+          const float d = dcv + old;
+          cv |= d > old;
+          dm[q * 64 + lane] = d;
+          dcv = dcv * p.tddv[q * 4 + z];
+        }
+      }
+      if (p.m >= 100 && !quad_any(cv)) done = true;
+    }
+    for (int q = 0; q < nq; q++) xev = xev + dm[q * 64 + lane];
+    const float p01 = xev + quad_dpp<DPP_QUAD_X1>(xev);
+    float xe = p01 + quad_dpp<DPP_QUAD_X2>(p01);
+    if (act) {
+      xn = xn * ploop;
+      xc = (xc * ploop) + (xe * emove);
+      xj = (xj * ploop) + (xe * eloop);
+      xb = (xj * pmove) + (xn * pmove);
+      float scale = 1.0f;
+      if (xe > 1.0e4f) {
+        const float inv = 1.0f / xe;
+        xn *= inv;
+        xc *= inv;
+        xj *= inv;
+        xb *= inv;
+        for (int q = 0; q < nq; q++) {
+          mm[q * 64 + lane] *= inv;
+          dm[q * 64 + lane] *= inv;
+          im[q * 64 + lane] *= inv;
+        }
+        totscale += (float)log((double)xe);
+        scale = xe;
+        xe = 1.0f;
+      }
+      const size_t r = (size_t)i * rowlen;
+      if (z == 0) {
+        om[r] = oi[r] = od[r] = 0.f;
+        float *xr = ox + (size_t)i * 6;
+        xr[0] = xe;
+        xr[1] = xn;
+        xr[2] = xj;
+        xr[3] = xb;
+        xr[4] = xc;
+        xr[5] = scale;
+      }
+      for (int q = 0; q < nq; q++) {
+        const int k = z * nq + q + 1;
+        if (k <= M) {
+          om[r + k] = mm[q * 64 + lane];
+          oi[r + k] = im[q * 64 + lane];
+          od[r + k] = dm[q * 64 + lane];
+        }
+      }
+    }
+  }
+  if (widx < nwin && z == 0) {
+    out[2 * w.idx] = (float)((double)totscale + log((double)xc * (double)pmove));
+    out[2 * w.idx + 1] = totscale;
+  }
+}
+
+hipError_t launch_fwd_full(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin,
+                           const int64_t *moff, const int64_t *xoff, float *mat, float *xmx, float *out, int unihit,
+                           hipStream_t s) {
+  if (nwin == 0) return hipSuccess;
+  const size_t lds = (size_t)3 * p.nq * 64 * sizeof(float);
+  if (lds > 160 * 1024) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(fwd_full, dim3((unsigned)((nwin + 15) / 16)), dim3(64), lds, s, p, seq, wins, nwin, moff, xoff,
+                     mat, xmx, out, unihit);
+  return hipGetLastError();
+}
+
 template <int NQ>
 static hipError_t fwd_exact_launch(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin, float *sc,
                                    hipStream_t s) {

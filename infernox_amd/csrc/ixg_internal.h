@@ -110,6 +110,9 @@ hipError_t launch_ssv_resolve(const SsvDev &p, const TaskDev *tasks, int64_t nta
                               int64_t cap, unsigned long long *counter, hipStream_t s);
 hipError_t launch_fwd(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin,
                       float *sc, hipStream_t s);
+hipError_t launch_fwd_full(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin,
+                           const int64_t *moff, const int64_t *xoff, float *mat, float *xmx, float *out, int unihit,
+                           hipStream_t s);
 hipError_t launch_bias(const BiasDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin,
                        float *sc, hipStream_t s);
 hipError_t launch_vit(const VitDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin,

@@ -59,10 +59,10 @@ EXPORTS = [
     "ixg_cyk_scan_hb", "ixg_inside_scan_hb", "ixg_hb_scan_batch", "ixg_cm_set_trpenalties", "ixg_cm_trtables",
     "ixg_trcyk_scan_hb", "ixg_trinside_scan_hb", "ixg_trhb_scan_batch", "ixg_kernel_timings",
     "ixg_copy_probe", "ixg_cm_create_cfg", "ixg_cm_itables", "ixg_qdb_scan", "ixg_qdb_scan_batch",
-    "ixg_cyk_seq_filter",
+    "ixg_cyk_seq_filter", "ixg_p7_forward", "ixg_p7_forward_batch",
 ]
 KERNELS = ["ssv_sweep", "ssv_resolve", "fwd", "bias", "vit", "msv", "hb_scan", "trhb_scan", "copy_probe",
-           "qdb_scan"]
+           "qdb_scan", "fwd_full"]
 
 
 class CmParams(C.Structure):
@@ -141,6 +141,8 @@ def lib():
         L.ixg_last_timings.argtypes = [vp, C.POINTER(C.c_double), i64p]
         L.ixg_kernel_timings.argtypes = [vp, C.POINTER(C.c_double), i64p, C.c_int]
         L.ixg_copy_probe.argtypes = [vp, C.c_size_t, C.c_int, C.POINTER(C.c_double)]
+        L.ixg_p7_forward.argtypes = [vp, vp, u8p, i64, C.c_int, f32p, f32p, f32p, f32p, f32p, f32p]
+        L.ixg_p7_forward_batch.argtypes = [vp, vp, u8p, i64, i64p, i64p, i64, C.c_int, f32p, f32p, f32p, f32p]
         L.ixg_cm_create_cfg.argtypes = [vp, C.POINTER(CmParams), C.c_int, C.POINTER(vp)]
         L.ixg_cm_itables.argtypes = [vp, i32p, i32p, i32p, i32p, i32p]
         L.ixg_qdb_scan.argtypes = [vp, vp, u8p, i64, i32, i32, i32p, i32p, C.c_int, C.c_float, C.c_int,
@@ -328,6 +330,45 @@ class P7:
 
     def msv_scores(self, seq, offs, lens):
         return self._scores("ixg_msv_scores", seq, offs, lens, True)
+
+    def p7_forward(self, dsq, L, unihit=False):
+        """Full-matrix Forward: dict(score, totscale, mmx, imx, dmx [(L+1), M+1], xmx [(L+1), 6])."""
+        M = self.M
+        d = np.ascontiguousarray(dsq, np.uint8)
+        mm, im, dm = (np.zeros((L + 1) * (M + 1), np.float32) for _ in range(3))
+        xm = np.zeros((L + 1) * 6, np.float32)
+        ts, sc = np.zeros(1, np.float32), np.zeros(1, np.float32)
+        check(lib().ixg_p7_forward(self.ctx.h, self.h, u8ptr(d), int(L), int(unihit), f32ptr(mm), f32ptr(im),
+                                   f32ptr(dm), f32ptr(xm), f32ptr(ts), f32ptr(sc)), "ixg_p7_forward")
+        shp = (L + 1, M + 1)
+        return dict(score=float(sc[0]), totscale=float(ts[0]), mmx=mm.reshape(shp), imx=im.reshape(shp),
+                    dmx=dm.reshape(shp), xmx=xm.reshape(L + 1, 6))
+
+    def p7_forward_batch(self, seq, regions, unihit=False):
+        """Full-matrix Forward over regions [(off, len)] of one buffer: list of
+        dicts as p7_forward returns."""
+        M = self.M
+        s = np.ascontiguousarray(seq, np.uint8)
+        off = np.int64([r[0] for r in regions])
+        ln = np.int64([r[1] for r in regions])
+        n = len(regions)
+        mat = np.zeros(max(1, int(np.sum(3 * (ln + 1) * (M + 1)))), np.float32)
+        xm = np.zeros(max(1, int(np.sum(6 * (ln + 1)))), np.float32)
+        sc, ts = np.zeros(max(1, n), np.float32), np.zeros(max(1, n), np.float32)
+        check(lib().ixg_p7_forward_batch(self.ctx.h, self.h, u8ptr(s), len(s), i64ptr(off), i64ptr(ln), n,
+                                         int(unihit), f32ptr(mat), f32ptr(xm), f32ptr(sc), f32ptr(ts)),
+              "ixg_p7_forward_batch")
+        out, mo, xo = [], 0, 0
+        for r in range(n):
+            L = int(ln[r])
+            pl = (L + 1) * (M + 1)
+            shp = (L + 1, M + 1)
+            out.append(dict(score=float(sc[r]), totscale=float(ts[r]), mmx=mat[mo:mo + pl].reshape(shp),
+                            imx=mat[mo + pl:mo + 2 * pl].reshape(shp), dmx=mat[mo + 2 * pl:mo + 3 * pl].reshape(shp),
+                            xmx=xm[xo:xo + 6 * (L + 1)].reshape(L + 1, 6)))
+            mo += 3 * pl
+            xo += 6 * (L + 1)
+        return out
 
     def f3_filter_sequence(self, dsq, L, opts, cap=None):
         dsq = np.ascontiguousarray(dsq, np.uint8)

@@ -113,6 +113,11 @@ int     ixo_vit_score_scalar(const ixo_vit *f, const uint8_t *dsq, int64_t L, fl
 int     ixo_vit_score_striped(const ixo_vit *f, const uint8_t *dsq, int64_t L, float *sc);
 float   ixo_fwd_score_striped(const ixo_fwd *f, const uint8_t *dsq, int64_t L);
 float   ixo_fwd_score_scalar(const ixo_fwd *f, const uint8_t *dsq, int64_t L);
+/* p7_fwdback.rs:301 p7_forward + :529 forward_score: full odds-space
+ * matrices mmx/imx/dmx [(L+1)*(m+1)], specials xmx [(L+1)*6] (E N J B C
+ * SCALE), totscale; returns the score in nats. unihit: the unihit length model. */
+float   ixo_p7_forward(const ixo_fwd *f, const uint8_t *dsq, int64_t L, int unihit, float *mmx, float *imx,
+                       float *dmx, float *xmx, float *totscale);
 float   ixo_bias_score(const ixo_bias *b, const uint8_t *dsq, int64_t L);
 
 /* filter cascade (cm_pipeline.rs:1863-2164). opts bit0 do_msv, bit1 do_fwd,

@@ -799,6 +799,104 @@ float ixo_fwd_score_striped(const ixo_fwd *f, const uint8_t *dsq, int64_t L) {
   return (float)sc;
 }
 
+/* p7_fwdback.rs:194-288 XFactors and :301-523 p7_forward: the full-matrix
+ * odds-space Forward (rows 0..L kept, natural order k = 0..m), M and I per
+ * cell from the previous row, the DD paths in the striped lane layout with
+ * the M >= 100 lazy early exit, the 4-lane xE reduction, specials and sparse
+ * rescaling; p7_fwdback.rs:529-532 forward_score. unihit selects the unihit
+ * length model (:236-252). xmx rows hold E, N, J, B, C, SCALE. */
+float ixo_p7_forward(const ixo_fwd *f, const uint8_t *dsq, int64_t L, int unihit, float *mmx, float *imx,
+                     float *dmx, float *xmx, float *totscale_out) {
+  const int m = f->m, q = f->nq;
+  const float nj = unihit ? 0.0f : 1.0f;
+  const float pmove = (2.0f + nj) / ((float)L + 2.0f + nj), ploop = 1.0f - pmove;
+  const float e_move = unihit ? 1.0f : 0.5f, e_loop = unihit ? 0.0f : 0.5f;
+  const size_t W = (size_t)m + 1;
+  for (size_t k = 0; k < W; k++) mmx[k] = imx[k] = dmx[k] = 0.0f;
+  xmx[0] = 0.0f; xmx[1] = 1.0f; xmx[2] = 0.0f; xmx[3] = pmove; xmx[4] = 0.0f; xmx[5] = 1.0f;
+  float xn = 1.0f, xj = 0.0f, xb = pmove, xc = 0.0f, totscale = 0.0f;
+  float *dmp = (float *)malloc(sizeof(float) * (size_t)q * 4);
+  for (int64_t i = 1; i <= L; i++) {
+    const int x = dsq[i];
+    const float *mp = mmx + (size_t)(i - 1) * W, *ip = imx + (size_t)(i - 1) * W, *dp = dmx + (size_t)(i - 1) * W;
+    float *mc = mmx + (size_t)i * W, *ic = imx + (size_t)i * W, *dc = dmx + (size_t)i * W;
+    mc[0] = ic[0] = dc[0] = 0.0f;
+    for (int k = 1; k <= m; k++) {
+      const float sv = xb * f->tbm[k] + mp[k - 1] * f->amm[k] + ip[k - 1] * f->aim[k] + dp[k - 1] * f->adm[k];
+      mc[k] = sv * f->rfv[k * KP + x];
+    }
+    for (int k = 1; k <= m; k++) ic[k] = mp[k] * f->tmi[k] + ip[k] * f->tii[k];
+    /* striped DD: lane z of quad qi is position k = z*q + qi + 1 */
+#define AT(arr, k) (((k) >= 1 && (k) <= m) ? (arr)[k] : 0.0f)
+    float dcv[4] = {0, 0, 0, 0};
+    for (int qi = 0; qi < q; qi++) {
+      for (int z = 0; z < 4; z++) dmp[qi * 4 + z] = dcv[z];
+      for (int z = 0; z < 4; z++) {
+        const int k = z * q + qi + 1;
+        dcv[z] = AT(mc, k) * AT(f->tmd, k);
+      }
+    }
+    { float r[4] = {0.0f, dcv[0], dcv[1], dcv[2]}; memcpy(dcv, r, sizeof r); }
+    for (int z = 0; z < 4; z++) dmp[z] = 0.0f;
+    for (int qi = 0; qi < q; qi++)
+      for (int z = 0; z < 4; z++) {
+        const int k = z * q + qi + 1;
+        const float d = dcv[z] + dmp[qi * 4 + z];
+        dmp[qi * 4 + z] = d;
+        dcv[z] = d * AT(f->tdd, k);
+      }
+    for (int j = 1; j < 4; j++) {
+      int changed = 0;
+      { float r[4] = {0.0f, dcv[0], dcv[1], dcv[2]}; memcpy(dcv, r, sizeof r); }
+      for (int qi = 0; qi < q; qi++)
+        for (int z = 0; z < 4; z++) {
+          const int k = z * q + qi + 1;
+          const float old = dmp[qi * 4 + z], d = dcv[z] + old;
+          if (d > old) changed = 1;
+          dmp[qi * 4 + z] = d;
+          dcv[z] = dcv[z] * AT(f->tdd, k);
+        }
+      if (m >= 100 && !changed) break;
+    }
+#undef AT
+    for (int qi = 0; qi < q; qi++)
+      for (int z = 0; z < 4; z++) {
+        const int k = z * q + qi + 1;
+        if (k <= m) dc[k] = dmp[qi * 4 + z];
+      }
+    float lane[4] = {0, 0, 0, 0};
+    for (int r = 0; r < 4; r++)
+      for (int qi = 0; qi < q; qi++) {
+        const int k = r * q + qi + 1;
+        if (k <= m) lane[r] += mc[k];
+      }
+    for (int r = 0; r < 4; r++)
+      for (int qi = 0; qi < q; qi++) {
+        const int k = r * q + qi + 1;
+        if (k <= m) lane[r] += dc[k];
+      }
+    float xe = (lane[0] + lane[1]) + (lane[2] + lane[3]);
+    xn *= ploop;
+    xc = (xc * ploop) + (xe * e_move);
+    xj = (xj * ploop) + (xe * e_loop);
+    xb = (xj * pmove) + (xn * pmove);
+    float scale = 1.0f;
+    if (xe > 1.0e4f) {
+      const float inv = 1.0f / xe;
+      xn *= inv; xc *= inv; xj *= inv; xb *= inv;
+      for (int k = 0; k <= m; k++) { mc[k] *= inv; dc[k] *= inv; ic[k] *= inv; }
+      scale = xe;
+      totscale += (float)log((double)xe);
+      xe = 1.0f;
+    }
+    float *xr = xmx + (size_t)i * 6;
+    xr[0] = xe; xr[1] = xn; xr[2] = xj; xr[3] = xb; xr[4] = xc; xr[5] = scale;
+  }
+  free(dmp);
+  if (totscale_out) *totscale_out = totscale;
+  return (float)((double)totscale + log((double)xc * (double)pmove));
+}
+
 /* cm_pipeline.rs:1420-1657 forward_filter_score, scalar order (the --strict
  * path: M/I per element, serial DD sweep, left-to-right xE sums). */
 float ixo_fwd_score_scalar(const ixo_fwd *f, const uint8_t *dsq, int64_t L) {

@@ -95,6 +95,9 @@ def lib():
         L.ixo_fwd_score_striped.restype = C.c_float
         L.ixo_fwd_score_scalar.argtypes = [C.POINTER(Fwd), u8p, i64]
         L.ixo_fwd_score_scalar.restype = C.c_float
+        f32p = C.POINTER(C.c_float)
+        L.ixo_p7_forward.argtypes = [C.POINTER(Fwd), u8p, i64, C.c_int, f32p, f32p, f32p, f32p, f32p]
+        L.ixo_p7_forward.restype = C.c_float
         L.ixo_bias_score.argtypes = [C.POINTER(Bias), u8p, i64]
         L.ixo_bias_score.restype = C.c_float
         L.ixo_f3_filter_sequence.argtypes = [C.POINTER(Msv), C.POINTER(Vit), C.POINTER(Fwd), C.POINTER(Bias),
@@ -191,6 +194,20 @@ class Oracle:
 
     def fwd(self, dsq, L):
         return lib().ixo_fwd_score_striped(C.byref(self.sfwd), u8(dsq), L)
+
+    def p7_forward(self, dsq, L, unihit=False):
+        """Full-matrix Forward: dict(score, totscale, mmx, imx, dmx [(L+1), M+1], xmx [(L+1), 6])."""
+        M = self.M
+        f32p = C.POINTER(C.c_float)
+        mm, im, dm = (np.zeros((L + 1) * (M + 1), np.float32) for _ in range(3))
+        xm = np.zeros((L + 1) * 6, np.float32)
+        ts = C.c_float()
+        sc = lib().ixo_p7_forward(C.byref(self.sfwd), u8(dsq), L, int(unihit), mm.ctypes.data_as(f32p),
+                                  im.ctypes.data_as(f32p), dm.ctypes.data_as(f32p), xm.ctypes.data_as(f32p),
+                                  C.byref(ts))
+        shp = (L + 1, M + 1)
+        return dict(score=sc, totscale=ts.value, mmx=mm.reshape(shp), imx=im.reshape(shp), dmx=dm.reshape(shp),
+                    xmx=xm.reshape(L + 1, 6))
 
     def fwd_scalar(self, dsq, L):
         return lib().ixo_fwd_score_scalar(C.byref(self.sfwd), u8(dsq), L)
