@@ -465,8 +465,8 @@ int run_f1(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<
     HIPCHK(hipMemcpyAsync(d_counter, &zero, 8, hipMemcpyHostToDevice, c->stream));
     if (attempt) HIPCHK(hipEventRecord(c->ev[1], c->stream));
     HIPCHK(hipEventRecord(c->ev[7], c->stream));
-    HIPCHK(launch_ssv_resolve(sd, d_tasks, (int64_t)tasks.size(), d_mask, d_cinfo, thresh, d_raw, cap, d_counter,
-                              c->stream));
+    HIPCHK(launch_ssv_resolve(sd, d_tasks, d_items, (int64_t)items.size(), d_mask, d_cinfo, thresh, d_raw, cap,
+                              d_counter, c->stream));
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
     HIPCHK(hipMemcpyAsync(&count, d_counter, 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
@@ -475,6 +475,9 @@ int run_f1(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<
   }
   std::vector<RawWinDev> h(count);
   if (count) HIPCHK(hipMemcpy(h.data(), d_raw, count * sizeof(RawWinDev), hipMemcpyDeviceToHost));
+  std::sort(h.begin(), h.end(), [](const RawWinDev &a, const RawWinDev &b) {
+    return a.task != b.task ? a.task < b.task : a.row < b.row;
+  });
   raws.assign(tasks.size(), {});
   for (const auto &r : h) raws[r.task].push_back({r.n, r.length, r.k, r.score});
   float a = 0, b = 0, kr = 0;

@@ -91,6 +91,7 @@ struct TaskDev {
 };
 struct RawWinDev {
   int64_t n, length, k;
+  int64_t row;  // the row that triggered the diagonal
   float score;
   int32_t task;
 };
@@ -105,7 +106,7 @@ struct WinDev {
 constexpr int SWEEP_SEG = 4096;  // rows per F1 sweep work item (multiple of 64)
 hipError_t launch_ssv_sweep(const SsvDev &p, const TaskDev *tasks, const SweepItem *items,
                             int64_t nitems, uint64_t *mask, uint32_t *cinfo, int thresh, hipStream_t s);
-hipError_t launch_ssv_resolve(const SsvDev &p, const TaskDev *tasks, int64_t ntasks,
+hipError_t launch_ssv_resolve(const SsvDev &p, const TaskDev *tasks, const SweepItem *items, int64_t nitems,
                               const uint64_t *mask, const uint32_t *cinfo, int thresh, RawWinDev *out,
                               int64_t cap, unsigned long long *counter, hipStream_t s);
 hipError_t launch_fwd(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin,

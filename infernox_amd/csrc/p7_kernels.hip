@@ -101,20 +101,30 @@ __global__ __launch_bounds__(64) void ssv_sweep_kernel(SsvDev p, const TaskDev *
 }
 
 // ---------------------------------------------------------------------------
-// F1 resolve: one wave per task walks the flagged rows in order and replays
-// the reference's sequential logic exactly (cm_pipeline.rs:521-717). A flagged
-// row at least M-1 rows past the last restart has its reset-free values, so
-// it triggers with the maximum and leftmost argmax the sweep recorded; rows
-// closer to a restart are recomputed from the restart (lanes hold KPL
-// consecutive model positions each). The diagonal is then recovered backward
-// and extended forward on lane 0.
+// F1 resolve: replays the reference's sequential reset-and-skip logic
+// (cm_pipeline.rs:521-717) exactly, one wave per SWEEP_SEG-row block of a
+// task. A trigger at row c' ends its forward extension at most M-2 rows
+// later, and after a restart at r only rows below r+M-1 differ from the
+// reset-free sweep. Hence a flagged row c with no flagged row in
+// [c-2M, c) sees the same state as at the start of the task: every earlier
+// restart r satisfies c >= r+M-1, so c triggers with the sweep's values and
+// everything after it depends only on c ("fresh" row). The wave of a block
+// starts at the block's first fresh row and stops at the first fresh row
+// past the block's end, where the next block's wave takes over; flagged rows
+// before the first fresh row belong to the wave of an earlier block.
+// Within a wave, a flagged row at least M-1 rows past the last restart has
+// its reset-free values, so it triggers with the maximum and leftmost argmax
+// the sweep recorded; rows closer to a restart are recomputed from the
+// restart (lanes hold KPL consecutive model positions each). The diagonal
+// is then recovered backward and extended forward on lane 0.
 // ---------------------------------------------------------------------------
 template <int KPL>
-__global__ __launch_bounds__(64) void ssv_resolve_kernel(SsvDev p, const TaskDev *tasks,
+__global__ __launch_bounds__(64) void ssv_resolve_kernel(SsvDev p, const TaskDev *tasks, const SweepItem *items,
                                                          const uint64_t *mask, const uint32_t *cinfo, int thresh,
                                                          RawWinDev *out, int64_t cap,
                                                          unsigned long long *counter) {
-  const TaskDev t = tasks[blockIdx.x];
+  const SweepItem it = items[blockIdx.x];
+  const TaskDev t = tasks[it.task];
   const int m = p.m;
   const int lane = threadIdx.x;
   const int kb = lane * KPL + 1;
@@ -127,18 +137,36 @@ __global__ __launch_bounds__(64) void ssv_resolve_kernel(SsvDev p, const TaskDev
   const uint32_t xb = p.xb, bias = p.bias;
   const int64_t L = t.L;
   const int64_t nwords = (L + 63) / 64;
+  const int64_t bs = 1 + (int64_t)it.seg * SWEEP_SEG, be = bs + SWEEP_SEG;
+  const int64_t gap = 2 * (int64_t)m;
+  const int64_t s0 = bs - gap > 1 ? bs - gap : 1;
+  int64_t lastf = -1;  // last flagged row seen (-1: none since s0)
+  bool active = false;
   int64_t r = 1, pos = 1;
-  for (int64_t w = 0; w < nwords; w++) {
+  for (int64_t w = (s0 - 1) / 64; w < nwords; w++) {
     uint64_t bits = mask[t.mask_off + w];
+    if (w == (s0 - 1) / 64) bits &= ~0ull << ((s0 - 1) % 64);
     while (bits) {
       const int bi = __builtin_ctzll(bits);
       bits &= bits - 1;
       const int64_t c = 64 * w + 1 + bi;
+      const bool fresh = lastf < 0 || c - lastf > gap;
+      if (!active) {
+        if (c >= be) return;  // no fresh row in this block
+        if (c < bs || !fresh) {
+          lastf = c;
+          continue;
+        }
+        active = true;
+      } else if (c >= be && fresh) {
+        return;
+      }
+      lastf = c;
       if (c < r) continue;
       uint32_t bv;
       int bk;
-      // Before the first trigger the scan is reset-free, so the sweep's
-      // values hold as well.
+      // Before the wave's first trigger the scan is reset-free, so the
+      // sweep's values hold as well.
       if (c >= r + m - 1 || r == 1) {
         const uint32_t ci = cinfo[t.mask_off * 64 + (c - 1)];
         bv = ci >> 16;
@@ -204,16 +232,17 @@ __global__ __launch_bounds__(64) void ssv_resolve_kernel(SsvDev p, const TaskDev
             n++;
           }
           const int64_t end_model = end_k + (max_end - c);
-          // Records of one task are appended by this wave alone, in trigger
-          // order; the host regroups them by task.
+          // Records of one block are appended in trigger order; the host
+          // orders them by (task, trigger row).
           const unsigned long long slot = atomicAdd(counter, 1ull);
           if ((int64_t)slot < cap) {
             RawWinDev rw;
             rw.n = tstart;
             rw.length = end_model - start + 1;
             rw.k = end_model;
+            rw.row = c;
             rw.score = ((float)(max_sc - (int)p.tjb_b) - (float)p.base_b) / p.scale_b - 3.0f;
-            rw.task = (int32_t)blockIdx.x;
+            rw.task = it.task;
             out[slot] = rw;
           }
         }
@@ -464,22 +493,24 @@ hipError_t launch_ssv_sweep(const SsvDev &p, const TaskDev *tasks, const SweepIt
 }
 
 template <int KPL>
-static void resolve_k(const SsvDev &p, const TaskDev *tasks, int64_t ntasks, const uint64_t *mask,
-                      const uint32_t *cinfo, int thresh, RawWinDev *out, int64_t cap, unsigned long long *counter,
-                      hipStream_t s) {
-  hipLaunchKernelGGL(ssv_resolve_kernel<KPL>, dim3((unsigned)ntasks), dim3(64), 0, s, p, tasks, mask, cinfo,
+static void resolve_k(const SsvDev &p, const TaskDev *tasks, const SweepItem *items, int64_t nitems,
+                      const uint64_t *mask, const uint32_t *cinfo, int thresh, RawWinDev *out, int64_t cap,
+                      unsigned long long *counter, hipStream_t s) {
+  hipLaunchKernelGGL(ssv_resolve_kernel<KPL>, dim3((unsigned)nitems), dim3(64), 0, s, p, tasks, items, mask, cinfo,
                      thresh, out, cap, counter);
 }
-hipError_t launch_ssv_resolve(const SsvDev &p, const TaskDev *tasks, int64_t ntasks, const uint64_t *mask,
-                              const uint32_t *cinfo, int thresh, RawWinDev *out, int64_t cap,
+hipError_t launch_ssv_resolve(const SsvDev &p, const TaskDev *tasks, const SweepItem *items, int64_t nitems,
+                              const uint64_t *mask, const uint32_t *cinfo, int thresh, RawWinDev *out, int64_t cap,
                               unsigned long long *counter, hipStream_t s) {
-  if (ntasks == 0) return hipSuccess;
+  if (nitems == 0) return hipSuccess;
   const int kpl = (p.m + 63) / 64;
-  if (kpl <= 1) resolve_k<1>(p, tasks, ntasks, mask, cinfo, thresh, out, cap, counter, s);
-  else if (kpl <= 2) resolve_k<2>(p, tasks, ntasks, mask, cinfo, thresh, out, cap, counter, s);
-  else if (kpl <= 4) resolve_k<4>(p, tasks, ntasks, mask, cinfo, thresh, out, cap, counter, s);
-  else if (kpl <= 8) resolve_k<8>(p, tasks, ntasks, mask, cinfo, thresh, out, cap, counter, s);
-  else if (kpl <= 16) resolve_k<16>(p, tasks, ntasks, mask, cinfo, thresh, out, cap, counter, s);
+#define IXG_RES(K) resolve_k<K>(p, tasks, items, nitems, mask, cinfo, thresh, out, cap, counter, s)
+  if (kpl <= 1) IXG_RES(1);
+  else if (kpl <= 2) IXG_RES(2);
+  else if (kpl <= 4) IXG_RES(4);
+  else if (kpl <= 8) IXG_RES(8);
+  else if (kpl <= 16) IXG_RES(16);
+#undef IXG_RES
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
