@@ -71,7 +71,7 @@ struct ixg_p7 {
   BiasHost bias;
   uint32_t *d_cost4 = nullptr;
   uint8_t *d_rbv = nullptr;
-  float *d_fwd = nullptr;   // tmain | tddv | rfvv
+  float *d_fwd = nullptr;   // tmain | tddv | rfvv | tz
   int16_t *d_vit = nullptr; // rwv | tr(8*(m+1))
   size_t bytes = 0;
   double vit_lvmu = 0, vit_lvlambda = 0;
@@ -95,6 +95,7 @@ struct ixg_p7 {
     d.tmain = d_fwd;
     d.tddv = d_fwd + fwd.tmain.size();
     d.rfvv = d_fwd + fwd.tmain.size() + fwd.tddv.size();
+    d.tz = d_fwd + fwd.tmain.size() + fwd.tddv.size() + fwd.rfvv.size();
     d.m = fwd.m;
     d.nq = fwd.nq;
     return d;
@@ -204,6 +205,16 @@ int ixg_p7_create(ixg_ctx *c, const ixg_p7_hmm *h, int32_t max_length, ixg_p7 **
   fwd.insert(fwd.end(), p->fwd.tmain.begin(), p->fwd.tmain.end());
   fwd.insert(fwd.end(), p->fwd.tddv.begin(), p->fwd.tddv.end());
   fwd.insert(fwd.end(), p->fwd.rfvv.begin(), p->fwd.rfvv.end());
+  // The striped transitions regrouped per stripe lane z ([4][nq][8]), then
+  // the DD transitions alone per lane, zero-padded to a multiple of 8.
+  const int nq = p->fwd.nq, nq8 = (nq + 7) & ~7;
+  for (int z = 0; z < 4; z++)
+    for (int q = 0; q < nq; q++) {
+      for (int t = 0; t < 7; t++) fwd.push_back(p->fwd.tmain[q * 28 + t * 4 + z]);
+      fwd.push_back(p->fwd.tddv[q * 4 + z]);
+    }
+  for (int z = 0; z < 4; z++)
+    for (int q = 0; q < nq8; q++) fwd.push_back(q < nq ? p->fwd.tddv[q * 4 + z] : 0.f);
   std::vector<int16_t> vit(p->vit.rwv);
   for (auto *a : {&p->vit.tmm, &p->vit.tim, &p->vit.tdm, &p->vit.vbm, &p->vit.tmi, &p->vit.tii, &p->vit.tmd,
                   &p->vit.tdd})

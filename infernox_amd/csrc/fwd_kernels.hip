@@ -425,6 +425,7 @@ using FwdAll = MakeFwdTable<FWD_NQ_EXACT_MAX>::type;  // stripe counts 2..FWD_NQ
 hipError_t launch_fwd(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin, float *sc,
                       hipStream_t s) {
   if (nwin == 0) return hipSuccess;
+  if (p.nq >= 2 && p.nq <= FWD_ZW_NQ_MAX && p.tz) return launch_fwd_zw(p, seq, wins, nwin, sc, s);
   if (p.nq >= 2 && p.nq <= FWD_NQ_EXACT_MAX) return FwdAll::fns[p.nq - 2](p, seq, wins, nwin, sc, s);
   const size_t lds = (size_t)3 * p.nq * 64 * sizeof(float);
   if (lds > 160 * 1024) return hipErrorInvalidValue;

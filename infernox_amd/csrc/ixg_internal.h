@@ -64,6 +64,8 @@ struct SsvDev {
 };
 struct FwdDev {
   const float *tmain, *tddv, *rfvv;
+  const float *tz;  // [4 z][nq][8]: BM MM IM DM MD MI II DD of stripe position q+1+z*nq,
+                    // then [4 z][nq rounded up to 8]: DD alone, zero-padded
   int m, nq;
 };
 struct VitDev {
@@ -111,6 +113,9 @@ hipError_t launch_ssv_resolve(const SsvDev &p, const TaskDev *tasks, const Sweep
                               int64_t cap, unsigned long long *counter, hipStream_t s);
 hipError_t launch_fwd(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin,
                       float *sc, hipStream_t s);
+constexpr int FWD_ZW_NQ_MAX = 80;  // largest stripe count of the stripe-per-wave Forward kernel
+hipError_t launch_fwd_zw(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin, float *sc,
+                         hipStream_t s);
 hipError_t launch_fwd_full(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin,
                            const int64_t *moff, const int64_t *xoff, float *mat, float *xmx, float *out, int unihit,
                            hipStream_t s);
