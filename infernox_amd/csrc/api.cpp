@@ -53,6 +53,19 @@ int scratch(ixg_ctx *c, int slot, size_t n, void **out) {
   return IXG_OK;
 }
 
+int pinned(ixg_ctx *c, size_t n, void **out) {
+  if (c->hpin_cap < n) {
+    if (c->hpin) HIPCHK(hipHostFree(c->hpin));
+    c->hpin = nullptr;
+    c->hpin_cap = 0;
+    const size_t want = std::max(n + n / 4, (size_t)1 << 20);
+    HIPCHK(hipHostMalloc(&c->hpin, want, hipHostMallocDefault));
+    c->hpin_cap = want;
+  }
+  *out = c->hpin;
+  return IXG_OK;
+}
+
 template <typename T>
 static int upload(ixg_ctx *c, int slot, const std::vector<T> &v, T **out) {
   void *p;
@@ -168,6 +181,7 @@ int ixg_shutdown(ixg_ctx *c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (auto &b : c->bufs) dfree(c, b.p, b.cap);
+  if (c->hpin) (void)hipHostFree(c->hpin);
   for (auto &ev : c->ev) (void)hipEventDestroy(ev);
   for (auto &ev : c->kev) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(c->stream);

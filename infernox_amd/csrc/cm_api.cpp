@@ -28,11 +28,14 @@ int dmin_host(int st, bool trunc) {
   return trunc ? std::max(sldelta_host(st), srdelta_host(st)) : sdelta_host(st);
 }
 
+// Checks one envelope's bands and lays out its deck: roff[r] = first cell
+// of band row r (rows in state order), ncells, nrows. On error returns
+// IXG_EINVAL with *err set (g_err is left alone: this runs on host workers).
 int layout(const CmHost &h, const ixg_bands &b, int32_t i0, int32_t j0, bool trunc, std::vector<int32_t> &roff,
-           int64_t &ncells, int32_t &nrows) {
+           int64_t &ncells, int32_t &nrows, const char **err) {
   const int M = h.M;
   if (!b.jmin || !b.jmax || !b.hd_off || !b.hdmin || !b.hdmax || i0 < 1 || j0 < i0 - 1) {
-    g_err = "bands: null array or bad envelope";
+    *err = "bands: null array or bad envelope";
     return IXG_EINVAL;
   }
   nrows = b.hd_off[M];
@@ -41,11 +44,11 @@ int layout(const CmHost &h, const ixg_bands &b, int32_t i0, int32_t j0, bool tru
   for (int v = 0; v < M; v++) {
     const int nj = b.jmax[v] >= b.jmin[v] ? b.jmax[v] - b.jmin[v] + 1 : 0;
     if (b.hd_off[v] < 0 || b.hd_off[v] + nj > nrows || (v + 1 <= M && b.hd_off[v + 1] < b.hd_off[v] + nj)) {
-      g_err = "bands: hd_off inconsistent with the j bands";
+      *err = "bands: hd_off inconsistent with the j bands";
       return IXG_EINVAL;
     }
     if (nj && (b.jmin[v] < i0 - 1 || b.jmax[v] > j0)) {
-      g_err = "bands: j band outside the envelope";
+      *err = "bands: j band outside the envelope";
       return IXG_EINVAL;
     }
     const int st = h.sttype[v];
@@ -55,18 +58,18 @@ int layout(const CmHost &h, const ixg_bands &b, int32_t i0, int32_t j0, bool tru
       const int lo = b.hdmin[r], hi = b.hdmax[r];
       if (hi >= lo) {
         if (lo < dmin_host(st, trunc) || j - hi + 1 < i0 || (st == E_ST && (lo != 0 || hi != 0))) {
-          g_err = "bands: d band outside the envelope or below the state's emission count";
+          *err = "bands: d band outside the envelope or below the state's emission count";
           return IXG_EINVAL;
         }
         nc += hi - lo + 1;
       } else if (st == E_ST) {
-        g_err = "bands: empty row of an end state";
+        *err = "bands: empty row of an end state";
         return IXG_EINVAL;
       }
     }
   }
   if (nc >= (trunc ? INT32_MAX / 4 : INT32_MAX)) {
-    g_err = "bands: deck larger than 2^31 cells";
+    *err = "bands: deck larger than 2^31 cells";
     return IXG_EINVAL;
   }
   ncells = nc;
@@ -185,11 +188,11 @@ int mode_mask(const ixg_trvalid &val, int v, int fill) {
 
 // Per state, the work units of its level step (cells; rows of an IL state;
 // i diagonals of an IR state; 0 for a state the scan skips), then the first
-// i of each IR state.
-void append_units(const CmHost &h, const ixg_bands &b, const ixg_trvalid *val, int fill,
-                  std::vector<int32_t> &pack) {
+// i of each IR state: 2*M values written at units.
+void append_units(const CmHost &h, const ixg_bands &b, const ixg_trvalid *val, int fill, int32_t *units) {
   const int M = h.M;
-  std::vector<int32_t> ib(M, 0);
+  int32_t *ib = units + M;
+  std::fill(ib, ib + M, 0);
   for (int v = 0; v < M; v++) {
     const int nj = b.jmax[v] >= b.jmin[v] ? b.jmax[v] - b.jmin[v] + 1 : 0;
     int64_t n = 0;
@@ -217,9 +220,8 @@ void append_units(const CmHost &h, const ixg_bands &b, const ixg_trvalid *val, i
         }
       }
     }
-    pack.push_back((int32_t)n);
+    units[v] = (int32_t)n;
   }
-  pack.insert(pack.end(), ib.begin(), ib.end());
 }
 
 // One or more envelopes over a device sequence buffer: uploads bands and
@@ -233,12 +235,15 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
   const bool trunc = tr != nullptr;
   const int planes = trunc ? 4 : 1, rper = trunc ? 5 : 1;
   std::vector<EnvPlan> plan(n);
-  int64_t total_cells = 0;
-  for (int64_t k = 0; k < n; k++) {
-    RCHK(layout(cm->h, env[k].bands, env[k].i0, env[k].j0, trunc, plan[k].roff, plan[k].ncells, plan[k].nrows));
+  std::vector<const char *> err(n, nullptr);
+  // per envelope, on host workers: band checks, deck layout, root row count
+  par_for(n, [&](int64_t k) {
+    if (layout(cm->h, env[k].bands, env[k].i0, env[k].j0, trunc, plan[k].roff, plan[k].ncells, plan[k].nrows,
+               &err[k]) != IXG_OK)
+      return;
     if (trunc && (!valid[k].jvalid || !valid[k].lvalid || !valid[k].rvalid || !valid[k].tvalid)) {
-      g_err = "truncated scan: null validity array";
-      return IXG_EINVAL;
+      err[k] = "truncated scan: null validity array";
+      return;
     }
     const ixg_bands &b = env[k].bands;
     int64_t nr = 0;
@@ -247,6 +252,13 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
       if (b.hdmax[r] >= b.hdmin[r]) nr += b.hdmax[r] - b.hdmin[r] + 1;
     }
     plan[k].nroot = nr;
+  });
+  int64_t total_cells = 0;
+  for (int64_t k = 0; k < n; k++) {
+    if (err[k]) {
+      g_err = err[k];
+      return IXG_EINVAL;
+    }
     total_cells += plan[k].ncells;
   }
   if (cells_out) *cells_out = total_cells;
@@ -256,30 +268,39 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
   while (k0 < n) {
     int64_t k1 = k0, floats = 0;
     while (k1 < n && (k1 == k0 || floats + planes * plan[k1].ncells <= budget)) floats += planes * plan[k1++].ncells;
-    // pack per envelope: jmin|jmax (M each), hd_off (M+1), hdmin|hdmax|roff (nrows each), [mode mask (M)]
-    std::vector<int32_t> pack;
-    std::vector<int64_t> at(k1 - k0), rat(k1 - k0);
+    // pack per envelope: jmin|jmax (M each), hd_off (M+1), hdmin|hdmax|roff
+    // (nrows each), [mode mask (M)], units (M), first IR i (M); staged in
+    // pinned memory, filled on host workers
+    std::vector<int64_t> at(k1 - k0 + 1), rat(k1 - k0);
     int64_t rtotal = 0;
+    at[0] = 0;
     for (int64_t k = k0; k < k1; k++) {
-      const ixg_bands &b = env[k].bands;
-      const int nr = plan[k].nrows;
-      at[k - k0] = (int64_t)pack.size();
-      pack.insert(pack.end(), b.jmin, b.jmin + M);
-      pack.insert(pack.end(), b.jmax, b.jmax + M);
-      pack.insert(pack.end(), b.hd_off, b.hd_off + M + 1);
-      pack.insert(pack.end(), b.hdmin, b.hdmin + nr);
-      pack.insert(pack.end(), b.hdmax, b.hdmax + nr);
-      pack.insert(pack.end(), plan[k].roff.begin(), plan[k].roff.begin() + nr);
-      if (trunc)
-        for (int v = 0; v < M; v++) pack.push_back(mode_mask(valid[k], v, tr->fill));
-      append_units(cm->h, b, trunc ? &valid[k] : nullptr, trunc ? tr->fill : 0, pack);
+      at[k - k0 + 1] = at[k - k0] + 3 * (int64_t)M + 1 + 3 * (int64_t)plan[k].nrows + (trunc ? M : 0) + 2 * M;
       rat[k - k0] = rtotal;
       rtotal += rper * plan[k].nroot;
     }
+    const int64_t pack_n = at[k1 - k0];
+    int32_t *pack;
+    RCHK(pinned(c, (size_t)pack_n * 4, (void **)&pack));
+    par_for(k1 - k0, [&](int64_t kk) {
+      const int64_t k = k0 + kk;
+      const ixg_bands &b = env[k].bands;
+      const int nr = plan[k].nrows;
+      int32_t *o = pack + at[kk];
+      o = std::copy(b.jmin, b.jmin + M, o);
+      o = std::copy(b.jmax, b.jmax + M, o);
+      o = std::copy(b.hd_off, b.hd_off + M + 1, o);
+      o = std::copy(b.hdmin, b.hdmin + nr, o);
+      o = std::copy(b.hdmax, b.hdmax + nr, o);
+      o = std::copy(plan[k].roff.begin(), plan[k].roff.begin() + nr, o);
+      if (trunc)
+        for (int v = 0; v < M; v++) *o++ = mode_mask(valid[k], v, tr->fill);
+      append_units(cm->h, b, trunc ? &valid[k] : nullptr, trunc ? tr->fill : 0, o);
+    });
     int32_t *d_pack;
     float *d_alpha, *d_rout;
     EnvDev *d_env;
-    RCHK(scratch(c, S_CM_BANDS, pack.size() * 4, (void **)&d_pack));
+    RCHK(scratch(c, S_CM_BANDS, (size_t)pack_n * 4, (void **)&d_pack));
     RCHK(scratch(c, S_CM_ALPHA, (size_t)std::max<int64_t>(floats, 1) * 4, (void **)&d_alpha));
     RCHK(scratch(c, S_CM_ENV, (size_t)(k1 - k0) * sizeof(EnvDev), (void **)&d_env));
     RCHK(scratch(c, S_CM_ROOT, (size_t)std::max<int64_t>(rtotal, 1) * 4, (void **)&d_rout));
@@ -306,7 +327,7 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
       e.j0 = env[k].j0;
       aoff += planes * plan[k].ncells;
     }
-    HIPCHK(hipMemcpyAsync(d_pack, pack.data(), pack.size() * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d_pack, pack, (size_t)pack_n * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(d_env, ed.data(), ed.size() * sizeof(EnvDev), hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipEventRecord(c->ev[5], c->stream));
     if (trunc)
@@ -668,13 +689,15 @@ extern "C" int ixg_hb_scan_batch(ixg_ctx *c, const ixg_cm *cm, const ixg_target 
   c->ms[2] = 0;
   std::vector<std::vector<float>> roots;
   RCHK(run_envelopes(c, cm, base, off, env, n, inside ? 1 : 0, nullptr, nullptr, roots, cells));
-  std::vector<ixg_hit> out;
-  for (int64_t k = 0; k < n; k++) {
+  std::vector<std::vector<ixg_hit>> per(n);
+  par_for(n, [&](int64_t k) {
     root_summary(env[k].bands, roots[k].data(), env[k].i0, env[k].j0, env_cutoff, vsc[k], envi[k], envj[k]);
     if (report)
       report_hits(*cm, env[k].bands, ixg_target_host_dsq(tgt, env[k].strand), env[k].i0, env[k].j0,
-                  roots[k].data(), cutoff, do_null3, (int32_t)k, out);
-  }
+                  roots[k].data(), cutoff, do_null3, (int32_t)k, per[k]);
+  });
+  std::vector<ixg_hit> out;
+  for (auto &h : per) out.insert(out.end(), h.begin(), h.end());
   if (report) {
     *nhits = (int64_t)out.size();
     for (int64_t k = 0; k < std::min<int64_t>(cap, out.size()); k++) hits[k] = out[k];
@@ -818,14 +841,16 @@ int ixg_trhb_scan_batch(ixg_ctx *c, const ixg_cm *cm, const ixg_target *tgt, con
   c->ms[2] = 0;
   std::vector<std::vector<float>> roots;
   RCHK(run_envelopes(c, cm, base, off, ev.data(), n, inside ? 1 : 0, &sel, val.data(), roots, cells));
-  std::vector<ixg_trhit> out;
-  for (int64_t k = 0; k < n; k++) {
+  std::vector<std::vector<ixg_trhit>> per(n);
+  par_for(n, [&](int64_t k) {
     const int64_t nroot = (int64_t)roots[k].size() / 5;
     tr_root_summary(ev[k].bands, roots[k].data(), nroot, mode_mask(val[k], 0, sel.fill), ev[k].i0, ev[k].j0,
                     env_cutoff, vsc[k], vmode[k], envi[k], envj[k]);
     tr_report(*cm, ev[k].bands, ixg_target_host_dsq(tgt, ev[k].strand), ev[k].i0, ev[k].j0, roots[k].data(), nroot,
-              pass_idx, local, cutoff, do_null3, (int32_t)k, out);
-  }
+              pass_idx, local, cutoff, do_null3, (int32_t)k, per[k]);
+  });
+  std::vector<ixg_trhit> out;
+  for (auto &h : per) out.insert(out.end(), h.begin(), h.end());
   *nhits = (int64_t)out.size();
   for (int64_t k = 0; k < std::min<int64_t>(cap, out.size()); k++) hits[k] = out[k];
   return (int64_t)out.size() > cap ? IXG_ERANGE : IXG_OK;

@@ -1,7 +1,9 @@
 // Device context and scratch helpers shared by the C ABI translation units.
 #pragma once
 #include <algorithm>
+#include <atomic>
 #include <string>
+#include <thread>
 #include <vector>
 #include "ixg_internal.h"
 
@@ -34,6 +36,35 @@ enum Slot { S_TASKS, S_ITEMS, S_MASK, S_RAW, S_COUNTER, S_WINS, S_SC0, S_SC1, S_
 int dmalloc(ixg_ctx *c, void **p, size_t n);
 void dfree(ixg_ctx *c, void *p, size_t n);
 int scratch(ixg_ctx *c, int slot, size_t n, void **out);
+// Pinned host staging buffer of the context grown to at least n bytes
+// (contents not preserved; reused once the stream has drained).
+int pinned(ixg_ctx *c, size_t n, void **out);
+
+// Runs f(k) for k in [0, n) on up to 16 host threads, in chunks of 4 items
+// (serially when n is small). f must not touch g_err; return codes are the
+// caller's business.
+template <class F>
+void par_for(int64_t n, F &&f) {
+  const unsigned hw = std::thread::hardware_concurrency();
+  int nt = (int)std::min<unsigned>(16u, hw ? hw : 1u);
+  if (n < 16) nt = 1;
+  if (nt <= 1) {
+    for (int64_t k = 0; k < n; k++) f(k);
+    return;
+  }
+  std::atomic<int64_t> next(0);
+  auto work = [&]() {
+    for (;;) {
+      const int64_t k0 = next.fetch_add(4);
+      if (k0 >= n) break;
+      for (int64_t k = k0; k < std::min<int64_t>(n, k0 + 4); k++) f(k);
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; t++) th.emplace_back(work);
+  work();
+  for (auto &t : th) t.join();
+}
 
 }  // namespace ixg
 
@@ -43,6 +74,8 @@ struct ixg_ctx {
   size_t budget = 0;
   size_t in_use = 0;
   std::vector<ixg::DevBuf> bufs;  // grow-only scratch slots
+  void *hpin = nullptr;           // pinned host staging (hipHostMalloc), grow-only
+  size_t hpin_cap = 0;
   hipEvent_t ev[8];
   double ms[5] = {0, 0, 0, 0, 0};
   int64_t cells[5] = {0, 0, 0, 0, 0};
