@@ -23,8 +23,11 @@ from infernox_amd import ixg, synth, synth_cm  # noqa: E402
 
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # 256 CUs x 4 SIMD16 x 2.4 GHz, one lane-op per cycle
 # HBM bytes per launch of each kernel on this workload, from rocprofv3 FETCH_SIZE +
-# WRITE_SIZE passes scaled by the copy-kernel calibration (profiles/hbm_traffic_*.md)
-TRAFFIC = {}
+# WRITE_SIZE passes (KiB per dispatch) scaled by what the same counters read for
+# a 256 MiB copy kernel (FETCH x2.0, WRITE x1.0); the measurements are in
+# profiles/hbm_traffic_p7_cascade_genome4.6Mb_M300.md. They hold for the
+# default workload only and are reported as measured, not recomputed.
+TRAFFIC = {"ssv_sweep": 12360715, "ssv_resolve": 6054781, "fwd": 6628113, "bias": 189719}
 
 
 def workload():
