@@ -209,7 +209,9 @@ int ixg_p7_create(ixg_ctx *c, const ixg_p7_hmm *h, int32_t max_length, ixg_p7 **
   }
   HIPCHK(hipSetDevice(c->device));
   const int m = h->M;
-  std::vector<uint32_t> cost4(m + 1, 0xFFFFFFFFu);
+  // cost words for k = 0..m, padded with cost 255 to a multiple of 4 (+4) so
+  // the sweep reads whole groups of four positions
+  std::vector<uint32_t> cost4((size_t)((m + 1 + 3) & ~3) + 4, 0xFFFFFFFFu);
   for (int k = 0; k <= m; k++) {
     uint32_t v = 0;
     for (int x = 0; x < 4; x++) v |= (uint32_t)p->msv.rbv[k * KP + x] << (8 * x);

@@ -56,7 +56,7 @@ double bg_null_one(int64_t L);
 
 // Device-side views (all pointers are device pointers).
 struct SsvDev {
-  const uint32_t *cost4;   // (m+1): bytes = rbv[k][A,C,G,U]
+  const uint32_t *cost4;   // (m+1) rounded up to 4, + 4: bytes = rbv[k][A,C,G,U]; 0xFFFFFFFF past m
   const uint8_t *rbv;      // (m+1)*KP
   int m;
   uint8_t xb, bias, base_b, tjb_b, tbm_b;
@@ -106,6 +106,7 @@ struct WinDev {
 
 // Kernel launchers (p7_kernels.hip).
 constexpr int SWEEP_SEG = 4096;  // rows per F1 sweep work item (multiple of 64)
+constexpr int SWEEP_TILES = 8;   // 64-row tiles a sweep wave carries in registers
 hipError_t launch_ssv_sweep(const SsvDev &p, const TaskDev *tasks, const SweepItem *items,
                             int64_t nitems, uint64_t *mask, uint32_t *cinfo, int thresh, hipStream_t s);
 hipError_t launch_ssv_resolve(const SsvDev &p, const TaskDev *tasks, const SweepItem *items, int64_t nitems,

@@ -33,69 +33,119 @@ __device__ __forceinline__ uint32_t ssv_cell(uint32_t prev, uint32_t xb, uint32_
 // Resetting only lowers cells (max/adds/subs are monotone and the reset value
 // is the minimum), so rows whose reset-free maximum stays below the threshold
 // can never trigger; the others are flagged in a bitmask for the resolve
-// kernel. Lane = row; the diagonal dependency (i-1,k-1) -> (i,k) is a
-// one-lane wave shift, and lane 0 takes its input from the previous tile's
-// lane 63 through a small LDS carry row.
+// kernel.
+//
+// Layout: lane = row within a 64-row tile, and a wave carries SWEEP_TILES
+// tiles (SWEEP_TILES*64 consecutive rows) in registers, all advancing one
+// model position k per step. The diagonal dependency (i-1,k-1) -> (i,k) is a
+// one-lane wave shift inside a tile; lane 0 of tile t takes lane 63 of tile
+// t-1 (still holding step k-1, as tiles are updated from the last to the
+// first) through a wave rotate, and lane 0 of the first tile takes the
+// previous tile group's last row from a small LDS carry row. The row maximum
+// and its leftmost position are tracked as one packed key
+// (value << 16 | 0xFFFF - k), whose maximum is the largest value at the
+// smallest k. Costs come from wave-uniform scalar loads, four positions at a
+// time.
 // ---------------------------------------------------------------------------
+#define DPP_WAVE_ROR1 0x13C
+typedef __attribute__((address_space(4))) const uint32_t CU32;
+
+__device__ __forceinline__ uint32_t ssv_in(uint32_t prev_tile, uint32_t cur_tile) {
+  // lanes 1..63: cur_tile of lane-1; lane 0: prev_tile of lane 63
+  const int rot = __builtin_amdgcn_update_dpp(0, (int)prev_tile, DPP_WAVE_ROR1, 0xF, 0xF, false);
+  return (uint32_t)__builtin_amdgcn_update_dpp(rot, (int)cur_tile, DPP_WAVE_SHR1, 0xF, 0xF, false);
+}
+
+template <int R, bool DEGEN>
+__device__ __forceinline__ void ssv_group(const SsvDev &p, const CU32 *c4g, const uint8_t *cprev, uint8_t *cnext,
+                                          const int (&x)[R], uint32_t (&key)[R]) {
+  const int m = p.m, lane = threadIdx.x;
+  const uint32_t xb = p.xb, bias = p.bias;
+  uint32_t a[R];
+#pragma unroll
+  for (int t = 0; t < R; t++) a[t] = 0;
+  const uint32_t *cp32 = (const uint32_t *)cprev;
+  uint32_t *cn32 = (uint32_t *)cnext;
+  // Carry byte b holds the last row's value at position b (byte 0: the zero
+  // column). Word j = bytes 4j..4j+3 is read for steps 4j+1..4j+4 and written
+  // after them, holding position 4j from the step before (lastb).
+  uint32_t lastb = 0;
+  for (int k0 = 1; k0 <= m; k0 += 4) {
+    const uint32_t carry4 = cp32[(k0 - 1) >> 2];  // carry[k0-1 .. k0+2]
+    uint32_t out4 = 0;
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const int k = k0 + u;
+      const uint32_t c4 = c4g[k];
+      const uint32_t cin = (carry4 >> (8 * u)) & 0xFFu;
+#pragma unroll
+      for (int t = R - 1; t >= 0; t--) {
+        const uint32_t in = t ? ssv_in(a[t - 1], a[t])
+                              : (uint32_t)__builtin_amdgcn_update_dpp((int)cin, (int)a[0], DPP_WAVE_SHR1, 0xF, 0xF,
+                                                                      false);
+        uint32_t c;
+        if (DEGEN) c = k <= m ? ssv_cost(p, c4, k, x[t]) : 255u;  // positions past m cost 255
+        else c = __builtin_amdgcn_ubfe(c4, 8u * (uint32_t)x[t], 8);
+        a[t] = ssv_cell(in, xb, bias, c);
+        key[t] = max(key[t], (a[t] << 16) | (uint32_t)(0xFFFF - k));
+      }
+      out4 |= (a[R - 1] & 0xFFu) << (8 * u);
+    }
+    if (lane == 63) cn32[(k0 - 1) >> 2] = lastb | (out4 << 8);
+    lastb = out4 >> 24;
+  }
+}
+
 __global__ __launch_bounds__(64) void ssv_sweep_kernel(SsvDev p, const TaskDev *tasks,
                                                        const SweepItem *items, uint64_t *mask,
                                                        uint32_t *cinfo, int thresh) {
-  extern __shared__ uint32_t sh[];  // cost4[m+1] | carry 2 x (m+4) bytes
+  constexpr int R = SWEEP_TILES;
+  extern __shared__ uint32_t sh[];  // carry rows 2 x cstride bytes
   const SweepItem it = items[blockIdx.x];
   const TaskDev t = tasks[it.task];
   const int m = p.m;
   const int lane = threadIdx.x;
-  uint32_t *c4s = sh;
-  const int cstride = ((m + 4) + 3) & ~3;
-  uint8_t *carry = (uint8_t *)(sh + m + 1);
-  for (int k = lane; k <= m; k += 64) c4s[k] = p.cost4[k];
+  const int cstride = ((m + 8) + 3) & ~3;
+  uint8_t *carry = (uint8_t *)sh;
   for (int k = lane; k < 2 * cstride; k += 64) carry[k] = 0;
   __syncthreads();
+  const CU32 *c4g = (const CU32 *)p.cost4;
   const int64_t s0 = 1 + (int64_t)it.seg * SWEEP_SEG;
   const int64_t s1 = min(s0 + (int64_t)SWEEP_SEG, t.L + 1);
   const int64_t warm = (int64_t)((m - 1 + 63) / 64) * 64;
   int64_t w0 = s0 - warm;
   if (w0 < 1) w0 = 1;
-  const uint32_t xb = p.xb, bias = p.bias;
   int cb = 0;
-  for (int64_t b = w0; b < s1; b += 64) {
-    const int64_t i = b + lane;
-    const bool valid = i < s1;
-    const int x = valid ? (int)t.dsq[i] : 0;
+  for (int64_t g = w0; g < s1; g += 64 * R) {
+    int x[R];
+    bool degenerate = false;
+#pragma unroll
+    for (int u = 0; u < R; u++) {
+      const int64_t i = g + 64 * u + lane;
+      x[u] = i < s1 ? (int)t.dsq[i] : 0;
+      degenerate |= x[u] >= 4;
+    }
+    uint32_t key[R];
+#pragma unroll
+    for (int u = 0; u < R; u++) key[u] = 0;
     const uint8_t *cprev = carry + cb * cstride;
     uint8_t *cnext = carry + (cb ^ 1) * cstride;
-    uint32_t cur = 0, rowmax = 0;
-    int arg = 0;
-    const bool degenerate = __any(x >= 4);
-    if (!degenerate) {
-      const uint32_t sh8 = 8u * (uint32_t)x;
-      for (int k = 1; k <= m; k++) {
-        const uint32_t cp = cprev[k - 1];
-        const uint32_t nb = (uint32_t)__builtin_amdgcn_update_dpp((int)cp, (int)cur, DPP_WAVE_SHR1, 0xF, 0xF, false);
-        const uint32_t c = __builtin_amdgcn_ubfe(c4s[k], sh8, 8);
-        cur = ssv_cell(nb, xb, bias, c);
-        if (cur > rowmax) { rowmax = cur; arg = k; }
-        if (lane == 63) cnext[k] = (uint8_t)cur;
-      }
-    } else {
-      for (int k = 1; k <= m; k++) {
-        const uint32_t cp = cprev[k - 1];
-        const uint32_t nb = (uint32_t)__builtin_amdgcn_update_dpp((int)cp, (int)cur, DPP_WAVE_SHR1, 0xF, 0xF, false);
-        const uint32_t c = ssv_cost(p, c4s[k], k, x);
-        cur = ssv_cell(nb, xb, bias, c);
-        if (cur > rowmax) { rowmax = cur; arg = k; }
-        if (lane == 63) cnext[k] = (uint8_t)cur;
-      }
-    }
-    if (lane == 63) cnext[0] = 0;
+    if (__any(degenerate)) ssv_group<R, true>(p, c4g, cprev, cnext, x, key);
+    else ssv_group<R, false>(p, c4g, cprev, cnext, x, key);
     __syncthreads();
     cb ^= 1;
-    if (b >= s0) {
-      const bool cand = valid && (int)rowmax >= thresh;
-      const uint64_t bits = __ballot(cand);
-      if (lane == 0) mask[t.mask_off + (b - 1) / 64] = bits;
-      // the reset-free row maximum and its leftmost position, for the resolve
-      if (cand) cinfo[t.mask_off * 64 + (i - 1)] = (rowmax << 16) | (uint32_t)arg;
+#pragma unroll
+    for (int u = 0; u < R; u++) {
+      const int64_t b = g + 64 * u;
+      if (b >= s0 && b < s1) {
+        const int64_t i = b + lane;
+        const uint32_t rowmax = key[u] >> 16;
+        const bool cand = i < s1 && (int)rowmax >= thresh;
+        const uint64_t bits = __ballot(cand);
+        if (lane == 0) mask[t.mask_off + (b - 1) / 64] = bits;
+        // the reset-free row maximum and its leftmost position, for the resolve
+        if (cand) cinfo[t.mask_off * 64 + (i - 1)] = (rowmax << 16) | (0xFFFFu - (key[u] & 0xFFFFu));
+      }
     }
   }
 }
@@ -486,7 +536,7 @@ __global__ void revcomp_kernel(const uint8_t *fwd, uint8_t *rc, int64_t L) {
 hipError_t launch_ssv_sweep(const SsvDev &p, const TaskDev *tasks, const SweepItem *items, int64_t nitems,
                             uint64_t *mask, uint32_t *cinfo, int thresh, hipStream_t s) {
   if (nitems == 0) return hipSuccess;
-  const size_t lds = (size_t)(p.m + 1) * 4 + 2 * (size_t)(((p.m + 4) + 3) & ~3);
+  const size_t lds = 2 * (size_t)(((p.m + 8) + 3) & ~3);
   hipLaunchKernelGGL(ssv_sweep_kernel, dim3((unsigned)nitems), dim3(64), lds, s, p, tasks, items, mask, cinfo,
                      thresh);
   return hipGetLastError();
