@@ -168,6 +168,36 @@ __global__ __launch_bounds__(64) void ssv_sweep_kernel(SsvDev p, const TaskDev *
 // restart (lanes hold KPL consecutive model positions each). The diagonal
 // is then recovered backward and extended forward on lane 0.
 // ---------------------------------------------------------------------------
+// SSV cost of model position k (1..m) for residue code x.
+__device__ __forceinline__ int ssv_cost_at(const SsvDev &p, int k, int x) {
+  return x < KP ? (int)p.rbv[k * KP + x] : 255;
+}
+// Wave-wide inclusive prefix sum / maximum over lanes 0..63, exclusive maximum.
+__device__ __forceinline__ int wave_incl_sum(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(v, off);
+    if (lane >= off) v += t;
+  }
+  return v;
+}
+__device__ __forceinline__ int wave_incl_max(int v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    const int t = __shfl_up(v, off);
+    if (lane >= off) v = max(v, t);
+  }
+  return v;
+}
+__device__ __forceinline__ int wave_excl_max(int v) {
+  const int lane = threadIdx.x & 63;
+  const int in = wave_incl_max(v);
+  const int t = __shfl_up(in, 1);
+  return lane ? t : INT_MIN;
+}
+
 template <int KPL>
 __global__ __launch_bounds__(64) void ssv_resolve_kernel(SsvDev p, const TaskDev *tasks, const SweepItem *items,
                                                          const uint64_t *mask, const uint32_t *cinfo, int thresh,
@@ -193,8 +223,14 @@ __global__ __launch_bounds__(64) void ssv_resolve_kernel(SsvDev p, const TaskDev
   int64_t lastf = -1;  // last flagged row seen (-1: none since s0)
   bool active = false;
   int64_t r = 1, pos = 1;
+  // mask words are fetched 64 at a time (one per lane) and read back with
+  // readlane, so the walk over a block waits on one load per 4096 rows
+  uint64_t wbuf = 0;
   for (int64_t w = (s0 - 1) / 64; w < nwords; w++) {
-    uint64_t bits = mask[t.mask_off + w];
+    const int wl = (int)((w - (s0 - 1) / 64) & 63);
+    if (wl == 0) wbuf = w + lane < nwords ? mask[t.mask_off + w + lane] : 0ull;
+    uint64_t bits = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(wbuf >> 32), wl) << 32) |
+                    (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)wbuf, wl);
     if (w == (s0 - 1) / 64) bits &= ~0ull << ((s0 - 1) % 64);
     while (bits) {
       const int bi = __builtin_ctzll(bits);
@@ -231,56 +267,105 @@ __global__ __launch_bounds__(64) void ssv_resolve_kernel(SsvDev p, const TaskDev
             const int x = __builtin_amdgcn_readlane(xr, rr);
             const uint32_t prev =
                 (uint32_t)__builtin_amdgcn_update_dpp(0, (int)d[KPL - 1], DPP_WAVE_SHR1, 0xF, 0xF, false);
+            if (x < 4) {
+              // one uniform branch per row; positions past m cost 255
+              // (c4 = 0xFFFFFFFF there), which makes their cells 0
+              const uint32_t sh8 = 8u * (uint32_t)x;
 #pragma unroll
-            for (int j = KPL - 1; j >= 0; j--) {
-              const uint32_t in = j ? d[j - 1] : prev;
-              const uint32_t cc = ssv_cost(p, c4[j], kb + j, x);
-              d[j] = (kb + j <= m) ? ssv_cell(in, xb, bias, cc) : 0u;
+              for (int j = KPL - 1; j >= 0; j--) {
+                const uint32_t in = j ? d[j - 1] : prev;
+                d[j] = ssv_cell(in, xb, bias, __builtin_amdgcn_ubfe(c4[j], sh8, 8));
+              }
+            } else {
+#pragma unroll
+              for (int j = KPL - 1; j >= 0; j--) {
+                const uint32_t in = j ? d[j - 1] : prev;
+                const uint32_t cc = kb + j <= m ? ssv_cost(p, c4[j], kb + j, x) : 255u;
+                d[j] = (kb + j <= m) ? ssv_cell(in, xb, bias, cc) : 0u;
+              }
             }
           }
         }
         pos = c + 1;
-        // row maximum and its leftmost position
-        bv = 0;
-        bk = 0x7FFFFFFF;
+        // The row triggers only if some cell reaches the threshold; a ballot
+        // of the lanes' maxima decides that, and only then is the row maximum
+        // and its leftmost position reduced (one packed key per lane:
+        // value << 16 | 0xFFFF - k, whose maximum is the largest value at the
+        // smallest k).
+        uint32_t lmax = 0;
 #pragma unroll
         for (int j = 0; j < KPL; j++)
-          if (kb + j <= m && d[j] > bv) { bv = d[j]; bk = kb + j; }
-        for (int off = 32; off >= 1; off >>= 1) {
-          const uint32_t ov = (uint32_t)__shfl_xor((int)bv, off);
-          const int ok = __shfl_xor(bk, off);
-          if (ov > bv || (ov == bv && ok < bk)) { bv = ov; bk = ok; }
-        }
-        if (bv == 0) bk = 0;
+          if (kb + j <= m) lmax = max(lmax, d[j]);
+        if (!__any((int)lmax >= thresh)) continue;
+        uint32_t key = 0;
+#pragma unroll
+        for (int j = 0; j < KPL; j++)
+          if (kb + j <= m) key = max(key, (d[j] << 16) | (uint32_t)(0xFFFF - (kb + j)));
+        for (int off = 32; off >= 1; off >>= 1) key = max(key, (uint32_t)__shfl_xor((int)key, off));
+        bv = key >> 16;
+        bk = bv ? (int)(0xFFFFu - (key & 0xFFFFu)) : 0;
       }
       if ((int)bv >= thresh && bk > 0) {
-        int64_t max_end = 0;
+        const int end_k = bk;
+        // backward diagonal recovery (cm_pipeline.rs:657-672), 64 steps per
+        // chunk: step j reads (end_k - j, c - j) and runs while the remaining
+        // score stays above the baseline and both indices are >= 1
+        const int baseline = (int)p.base_b - (int)p.tjb_b - (int)p.tbm_b;
+        int64_t nit = 0;
+        {
+          int rem0 = (int)bv;
+          for (int64_t j0 = 0;; j0 += 64) {
+            const int64_t j = j0 + lane;
+            const bool inb = end_k - j >= 1 && c - j >= 1;
+            const int term = inb ? (int)p.bias - ssv_cost_at(p, end_k - j, t.dsq[c - j]) : 0;
+            const int incl = wave_incl_sum(term);
+            const bool ok = inb && rem0 - (incl - term) > baseline;
+            const uint64_t fail = __ballot(!ok);
+            if (fail) {
+              nit = j0 + __builtin_ctzll(fail);
+              break;
+            }
+            rem0 -= __shfl(incl, 63);
+          }
+        }
+        const int64_t start = end_k - nit + 1, tstart = c - nit + 1;
+        // forward extension (cm_pipeline.rs:674-690), 64 steps per chunk:
+        // step j scores (end_k+1+j, c+1+j) while k < m and n <= L, and stops
+        // after the fifth consecutive step below the running maximum
+        int max_sc = (int)bv, since = 0;
+        int64_t max_end = c;
+        {
+          int sc0 = (int)bv;
+          for (int64_t j0 = 0;; j0 += 64) {
+            const int64_t j = j0 + lane, k = end_k + 1 + j, n = c + 1 + j;
+            const bool valid = k < m && n <= L;
+            const int term = valid ? (int)p.bias - ssv_cost_at(p, (int)k, t.dsq[n]) : 0;
+            const int sc2 = sc0 + wave_incl_sum(term);
+            const int before = max(max_sc, wave_excl_max(sc2));  // running maximum before step j
+            const bool imp = valid && sc2 >= before;
+            const int last = wave_incl_max(imp ? lane : -1);
+            const int sinc = last >= 0 ? lane - last : since + lane + 1;
+            const uint64_t stop = __ballot(!valid || sinc == 5);
+            // steps taken in this chunk: up to the first invalid one
+            // (excluded) or the one that reaches five (included)
+            int taken = 64;
+            if (stop) {
+              const int s = __builtin_ctzll(stop);
+              taken = __shfl((int)valid, s) ? s + 1 : s;
+            }
+            if (taken > 0) {
+              const int lt = __shfl(last, taken - 1);
+              if (lt >= 0) {
+                max_end = c + 1 + j0 + lt;
+                max_sc = __shfl(sc2, lt);
+              }
+              since = __shfl(sinc, taken - 1);
+              sc0 = __shfl(sc2, taken - 1);
+            }
+            if (stop) break;
+          }
+        }
         if (lane == 0) {
-          const int end_k = bk;
-          const int baseline = (int)p.base_b - (int)p.tjb_b - (int)p.tbm_b;
-          int64_t start = end_k, tstart = c;
-          int rem = (int)bv;
-          while (rem > baseline && start >= 1 && tstart >= 1) {
-            const int x = t.dsq[tstart];
-            const int cst = x < KP ? (int)p.rbv[start * KP + x] : 255;
-            rem -= (int)p.bias - cst;
-            start--;
-            tstart--;
-          }
-          start++;
-          tstart++;
-          int sc2 = (int)bv, max_sc = (int)bv, since = 0;
-          max_end = c;
-          int64_t k = end_k + 1, n = c + 1;
-          while (k < m && n <= L) {
-            const int x = t.dsq[n];
-            const int cst = x < KP ? (int)p.rbv[k * KP + x] : 255;
-            sc2 += (int)p.bias - cst;
-            if (sc2 >= max_sc) { max_sc = sc2; max_end = n; since = 0; }
-            else if (++since == 5) break;
-            k++;
-            n++;
-          }
           const int64_t end_model = end_k + (max_end - c);
           // Records of one block are appended in trigger order; the host
           // orders them by (task, trigger row).
@@ -296,7 +381,6 @@ __global__ __launch_bounds__(64) void ssv_resolve_kernel(SsvDev p, const TaskDev
             out[slot] = rw;
           }
         }
-        max_end = __shfl(max_end, 0);
         r = max_end + 1;
         pos = r;
 #pragma unroll
