@@ -502,11 +502,20 @@ int run_f1(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<
   }
   std::vector<RawWinDev> h(count);
   if (count) HIPCHK(hipMemcpy(h.data(), d_raw, count * sizeof(RawWinDev), hipMemcpyDeviceToHost));
-  std::sort(h.begin(), h.end(), [](const RawWinDev &a, const RawWinDev &b) {
-    return a.task != b.task ? a.task < b.task : a.row < b.row;
-  });
+  // records by task, then by trigger row within a task (blocks append in
+  // trigger order, but blocks of one task interleave)
   raws.assign(tasks.size(), {});
-  for (const auto &r : h) raws[r.task].push_back({r.n, r.length, r.k, r.score});
+  std::vector<std::vector<std::pair<int64_t, int32_t>>> by(tasks.size());
+  for (int32_t i = 0; i < (int32_t)h.size(); i++) by[h[i].task].push_back({h[i].row, i});
+  for (size_t t = 0; t < tasks.size(); t++) {
+    auto &v = by[t];
+    std::sort(v.begin(), v.end());
+    raws[t].reserve(v.size());
+    for (const auto &e : v) {
+      const RawWinDev &r = h[e.second];
+      raws[t].push_back({r.n, r.length, r.k, r.score});
+    }
+  }
   float a = 0, b = 0, kr = 0;
   HIPCHK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
   HIPCHK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
@@ -553,8 +562,18 @@ int run_scores(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vec
       wd[i].b = (float)p->msv.tec_b;
     }
   }
-  // longest first, so waves that share windows see similar lengths
-  std::stable_sort(wd.begin(), wd.end(), [](const WinDev &a, const WinDev &b) { return a.len > b.len; });
+  // longest first (a stable counting sort by length), so windows that share
+  // a wave or workgroup have similar lengths
+  {
+    int32_t lmax = 0;
+    for (const auto &w : wd) lmax = std::max(lmax, w.len);
+    std::vector<int64_t> cnt((size_t)lmax + 2, 0);
+    for (const auto &w : wd) cnt[(size_t)(lmax - w.len) + 1]++;
+    for (size_t i = 1; i < cnt.size(); i++) cnt[i] += cnt[i - 1];
+    std::vector<WinDev> sorted(wd.size());
+    for (const auto &w : wd) sorted[cnt[(size_t)(lmax - w.len)]++] = w;
+    wd.swap(sorted);
+  }
   WinDev *d_w;
   RCHK(upload(c, S_WINS, wd, &d_w));
   float *d_sc;
