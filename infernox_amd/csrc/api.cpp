@@ -53,16 +53,16 @@ int scratch(ixg_ctx *c, int slot, size_t n, void **out) {
   return IXG_OK;
 }
 
-int pinned(ixg_ctx *c, size_t n, void **out) {
-  if (c->hpin_cap < n) {
-    if (c->hpin) HIPCHK(hipHostFree(c->hpin));
-    c->hpin = nullptr;
-    c->hpin_cap = 0;
+int pinned(ixg_ctx *c, int slot, size_t n, void **out) {
+  if (c->hpin_cap[slot] < n) {
+    if (c->hpin[slot]) HIPCHK(hipHostFree(c->hpin[slot]));
+    c->hpin[slot] = nullptr;
+    c->hpin_cap[slot] = 0;
     const size_t want = std::max(n + n / 4, (size_t)1 << 20);
-    HIPCHK(hipHostMalloc(&c->hpin, want, hipHostMallocDefault));
-    c->hpin_cap = want;
+    HIPCHK(hipHostMalloc(&c->hpin[slot], want, hipHostMallocDefault));
+    c->hpin_cap[slot] = want;
   }
-  *out = c->hpin;
+  *out = c->hpin[slot];
   return IXG_OK;
 }
 
@@ -181,7 +181,8 @@ int ixg_shutdown(ixg_ctx *c) {
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (auto &b : c->bufs) dfree(c, b.p, b.cap);
-  if (c->hpin) (void)hipHostFree(c->hpin);
+  for (auto *h : c->hpin)
+    if (h) (void)hipHostFree(h);
   for (auto &ev : c->ev) (void)hipEventDestroy(ev);
   for (auto &ev : c->kev) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(c->stream);
@@ -500,13 +501,17 @@ int run_f1(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<
     if ((int64_t)count <= cap) break;
     cap = (int64_t)count + 1024;
   }
-  std::vector<RawWinDev> h(count);
-  if (count) HIPCHK(hipMemcpy(h.data(), d_raw, count * sizeof(RawWinDev), hipMemcpyDeviceToHost));
+  RawWinDev *h = nullptr;
+  RCHK(pinned(c, P_DOWN, std::max<size_t>(count, 1) * sizeof(RawWinDev), (void **)&h));
+  if (count) {
+    HIPCHK(hipMemcpyAsync(h, d_raw, count * sizeof(RawWinDev), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
   // records by task, then by trigger row within a task (blocks append in
   // trigger order, but blocks of one task interleave)
   raws.assign(tasks.size(), {});
   std::vector<std::vector<std::pair<int64_t, int32_t>>> by(tasks.size());
-  for (int32_t i = 0; i < (int32_t)h.size(); i++) by[h[i].task].push_back({h[i].row, i});
+  for (int32_t i = 0; i < (int32_t)count; i++) by[h[i].task].push_back({h[i].row, i});
   for (size_t t = 0; t < tasks.size(); t++) {
     auto &v = by[t];
     std::sort(v.begin(), v.end());
@@ -574,8 +579,11 @@ int run_scores(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vec
     for (const auto &w : wd) sorted[cnt[(size_t)(lmax - w.len)]++] = w;
     wd.swap(sorted);
   }
-  WinDev *d_w;
-  RCHK(upload(c, S_WINS, wd, &d_w));
+  WinDev *d_w, *h_w;
+  RCHK(scratch(c, S_WINS, wd.size() * sizeof(WinDev), (void **)&d_w));
+  RCHK(pinned(c, P_UP, wd.size() * sizeof(WinDev), (void **)&h_w));
+  std::copy(wd.begin(), wd.end(), h_w);
+  HIPCHK(hipMemcpyAsync(d_w, h_w, wd.size() * sizeof(WinDev), hipMemcpyHostToDevice, c->stream));
   float *d_sc;
   uint8_t *d_ovf;
   RCHK(scratch(c, S_SC0, n * 4, (void **)&d_sc));
@@ -590,9 +598,13 @@ int run_scores(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vec
   else e = launch_msv(p->ssv_dev(), d_seq, d_w, n, d_sc, d_ovf, c->stream);
   HIPCHK(e);
   HIPCHK(ktime_end(c, kind == 0 ? IXG_K_FWD : kind == 1 ? IXG_K_BIAS : kind == 2 ? IXG_K_VIT : IXG_K_MSV));
-  HIPCHK(hipMemcpyAsync(sc.data(), d_sc, n * 4, hipMemcpyDeviceToHost, c->stream));
-  if (kind >= 2) HIPCHK(hipMemcpyAsync(ovf.data(), d_ovf, n, hipMemcpyDeviceToHost, c->stream));
+  uint8_t *h_out;
+  RCHK(pinned(c, P_DOWN, (size_t)n * 5, (void **)&h_out));
+  HIPCHK(hipMemcpyAsync(h_out, d_sc, n * 4, hipMemcpyDeviceToHost, c->stream));
+  if (kind >= 2) HIPCHK(hipMemcpyAsync(h_out + n * 4, d_ovf, n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
+  std::copy((const float *)h_out, (const float *)h_out + n, sc.begin());
+  if (kind >= 2) std::copy(h_out + n * 4, h_out + n * 5, ovf.begin());
   c->cells[1 + (kind == 0 ? 0 : (kind == 1 ? 1 : 2))] += cells * (kind == 1 ? 1 : p->msv.m);
   return IXG_OK;
 }

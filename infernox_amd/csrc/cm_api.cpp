@@ -281,7 +281,7 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
     }
     const int64_t pack_n = at[k1 - k0];
     int32_t *pack;
-    RCHK(pinned(c, (size_t)pack_n * 4, (void **)&pack));
+    RCHK(pinned(c, P_PACK, (size_t)pack_n * 4, (void **)&pack));
     par_for(k1 - k0, [&](int64_t kk) {
       const int64_t k = k0 + kk;
       const ixg_bands &b = env[k].bands;

@@ -36,9 +36,11 @@ enum Slot { S_TASKS, S_ITEMS, S_MASK, S_RAW, S_COUNTER, S_WINS, S_SC0, S_SC1, S_
 int dmalloc(ixg_ctx *c, void **p, size_t n);
 void dfree(ixg_ctx *c, void *p, size_t n);
 int scratch(ixg_ctx *c, int slot, size_t n, void **out);
-// Pinned host staging buffer of the context grown to at least n bytes
-// (contents not preserved; reused once the stream has drained).
-int pinned(ixg_ctx *c, size_t n, void **out);
+// Pinned host staging buffer `slot` of the context grown to at least n bytes
+// (contents not preserved; a slot is reused only after the stream has
+// drained past its previous copy).
+enum PinSlot { P_PACK, P_UP, P_DOWN, P_NSLOTS };
+int pinned(ixg_ctx *c, int slot, size_t n, void **out);
 
 // Runs f(k) for k in [0, n) on up to 16 host threads, in chunks of 4 items
 // (serially when n is small). f must not touch g_err; return codes are the
@@ -74,8 +76,8 @@ struct ixg_ctx {
   size_t budget = 0;
   size_t in_use = 0;
   std::vector<ixg::DevBuf> bufs;  // grow-only scratch slots
-  void *hpin = nullptr;           // pinned host staging (hipHostMalloc), grow-only
-  size_t hpin_cap = 0;
+  void *hpin[ixg::P_NSLOTS] = {};   // pinned host staging (hipHostMalloc), grow-only
+  size_t hpin_cap[ixg::P_NSLOTS] = {};
   hipEvent_t ev[8];
   double ms[5] = {0, 0, 0, 0, 0};
   int64_t cells[5] = {0, 0, 0, 0, 0};
