@@ -91,8 +91,8 @@ __global__ __launch_bounds__(256) void fwd_zw(FwdDev p, const float *__restrict_
     // value is the reference's expression evaluated in its order; only the
     // order in which independent positions are visited differs. Transitions
     // are fetched two positions ahead and emissions one group of four ahead;
-    // the scheduling barriers keep the steps in order so that only the three
-    // columns and the prefetched values are live.
+    // scheduling barriers between pairs of steps keep the steps in order so
+    // that only the three columns and the prefetched values are live.
     float dcv = 0.f;  // D carry out of this stripe: M(NQ-1)*MD(NQ-1)
     float t0[8], t1[8], t2[8];
 #pragma unroll
@@ -133,12 +133,14 @@ __global__ __launch_bounds__(256) void fwd_zw(FwdDev p, const float *__restrict_
         t1[k] = t2[k];
       }
       if ((q & 3) == 0) e4 = e4n;
-      __builtin_amdgcn_sched_barrier(0);
+      if ((q & 1) == 0) __builtin_amdgcn_sched_barrier(0);  // pairs of steps may interleave
     }
     dm[0] = 0.f;
-    float xev = 0.f;  // lane z's xE partial: M over q ascending, then D (below)
-#pragma unroll
-    for (int q = 0; q < NQ; q++) xev = xev + mm[q];
+    // Lane z's xE partial is M over q ascending, then D over q ascending with
+    // the final D values. The M sum runs inside DD pass 1 and the D sum
+    // inside every lazy pass (the last pass that runs leaves the final
+    // values), each as a second dependency chain beside the D carry.
+    float xev_m = 0.f, xev = 0.f;
     // DD paths: first obligatory pass on the carry shifted in from stripe z-1
     x1[tid] = dcv;
     __syncthreads();
@@ -158,6 +160,7 @@ __global__ __launch_bounds__(256) void fwd_zw(FwdDev p, const float *__restrict_
         for (int u = 0; u < 8; u++) {
           const int q = c * 8 + u;
           if (q < NQ) {
+            xev_m = xev_m + mm[q];
             const float d = dcv + dm[q];
             dm[q] = d;
             dcv = d * dd0[u];
@@ -188,6 +191,7 @@ __global__ __launch_bounds__(256) void fwd_zw(FwdDev p, const float *__restrict_
       // keeps the pass free of divergent branches.
       if (done) dcv = 0.f;
       cv = 0;
+      xev = xev_m;
       {
         const CFloat *td = dd_table(tdd_z);
         float dd0[8], dd1[8];
@@ -207,6 +211,7 @@ __global__ __launch_bounds__(256) void fwd_zw(FwdDev p, const float *__restrict_
               const float d = dcv + old;
               cv |= d > old;
               dm[q] = d;
+              xev = xev + d;
               dcv = dcv * dd0[u];
             }
           }
@@ -216,8 +221,6 @@ __global__ __launch_bounds__(256) void fwd_zw(FwdDev p, const float *__restrict_
         }
       }
     }
-#pragma unroll
-    for (int q = 0; q < NQ; q++) xev = xev + dm[q];
     xx[tid] = make_float4(xev, mm[NQ - 1], dm[NQ - 1], im[NQ - 1]);
     __syncthreads();
     const float xe = (xx[lane].x + xx[64 + lane].x) + (xx[128 + lane].x + xx[192 + lane].x);
