@@ -388,32 +388,58 @@ def _tr_qualifies(cm):
             (st == B_ST) | np.isin(st, (IL_ST, IR_ST)))
 
 
-def tr_penalties(cm, seed=0):
-    """Truncated-begin penalties g[3, M] (global) and l[3, M] (local), log2,
-    index 0 = 5'&3', 1 = 5' only, 2 = 3' only, -1e36 where entry is not
-    allowed. Shaped like cm_trunc.rs:192-384 (global: 2/(clen(clen+1)) or
-    1/clen spread over the qualifying states of a node; local: the local
+def _tr_penalty_probs(cm, seed):
+    """Truncated-begin probabilities pg[3, M] (global) and pl[3, M] (local) as
+    float32, 0 where the entry is not allowed; index 0 = 5'&3', 1 = 5' only,
+    2 = 3' only. Shaped like cm_trunc.rs:192-384 (global: 2/(clen(clen+1))
+    or 1/clen spread over the qualifying states of a node; local: the local
     begin probability over the subtree's fragment count), with a seeded
     jitter: the scanners treat them as given."""
     rng = np.random.default_rng(seed)
     M, clen = cm["M"], cm["clen"]
     q = _tr_qualifies(cm)
     lp, rp = state_spans(cm)
-    g = np.full((3, M), -1e36, np.float32)
-    l = np.full((3, M), -1e36, np.float32)
+    pg = np.zeros((3, M), np.float64)
+    pl = np.zeros((3, M), np.float64)
     jit = rng.uniform(0.8, 1.0, size=M)
     g53 = 2.0 / (clen * (clen + 1))
-    g[0, q] = np.log2(g53 * jit[q])
+    pg[0, q] = g53 * jit[q]
     m5 = q & (rp == clen)
     m3 = q & (lp == 1)
-    g[1, m5] = np.log2(jit[m5] / clen)
-    g[2, m3] = np.log2(jit[m3] / clen)
+    pg[1, m5] = jit[m5] / clen
+    pg[2, m3] = jit[m3] / clen
     sub = np.maximum(1, rp - lp + 1).astype(np.float64)
     pb = 0.05 / max(1, int(q.sum()))
-    l[0, q] = np.log2(pb / (sub[q] * (sub[q] + 1) / 2) * jit[q])
-    l[1, q] = np.log2(pb / sub[q] * jit[q])
-    l[2, q] = np.log2(pb / sub[q] * jit[q])
-    return g, l
+    pl[0, q] = pb / (sub[q] * (sub[q] + 1) / 2) * jit[q]
+    pl[1, q] = pb / sub[q] * jit[q]
+    pl[2, q] = pb / sub[q] * jit[q]
+    return pg.astype(np.float32), pl.astype(np.float32)
+
+
+def tr_penalties(cm, seed=0):
+    """Truncated-begin penalties g[3, M] (global) and l[3, M] (local), log2
+    of the probabilities of _tr_penalty_probs, -1e36 where the entry is not
+    allowed (cm_trunc.rs:349-361 converts with sreLOG2)."""
+    out = []
+    for p in _tr_penalty_probs(cm, seed):
+        lg = np.full(p.shape, -1e36, np.float32)
+        ok = p > 0
+        lg[ok] = (np.log(p[ok].astype(np.float64)) * 1.44269504).astype(np.float32)
+        out.append(lg)
+    return out[0], out[1]
+
+
+def tr_ipenalties(cm, seed=0):
+    """Integer truncated-begin penalties ig[3, M] and il[3, M] of the same
+    probabilities: Prob2Score(p, 1) = floor(0.5 + 1000 log2 p), -INFTY where
+    not allowed (cm_trunc.rs:349-374)."""
+    out = []
+    for p in _tr_penalty_probs(cm, seed):
+        ip = np.full(p.shape, -987654321, np.int32)
+        ok = p > 0
+        ip[ok] = np.floor(0.5 + 1000.0 * np.log2(p[ok].astype(np.float64))).astype(np.int32)
+        out.append(ip)
+    return out[0], out[1]
 
 
 def tr_validity(cm, pass_idx, sp1, sp2, ep1, ep2):

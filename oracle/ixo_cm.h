@@ -109,6 +109,29 @@ int64_t ixo_trhb_scan(const ixo_cm *cm, const ixo_cmsc *sc, const float *lmesc, 
                       int local, int inside, float cutoff, int do_null3, float env_cutoff, float *vsc, int *vmode,
                       int64_t *envi, int64_t *envj, ixo_trhit *hits, int64_t cap, float *planes_out);
 
+/* Integer marginal emission scores ilmesc/irmesc [M*18] (cp9.rs:4441-4527). */
+void ixo_cm_imarginals(const ixo_cm *cm, int32_t *ilmesc, int32_t *irmesc);
+/* cm_hit_AllowTruncation (cm_dpsearch_trunc.rs:2376-2422) for a hit start..stop
+ * of mode `mode` entered at state b. */
+int ixo_tr_allow(const ixo_cm *cm, const int32_t *lpos, const int32_t *rpos, int pass_idx, int local, int start,
+                 int stop, int i0, int j0, int mode, int b);
+/* remove_overlaps_greedy_tr (cm_dpsearch_trunc.rs:2470-2495). */
+int64_t ixo_tr_remove_overlaps(ixo_trhit *tmp, int64_t ntmp, ixo_trhit *hits, int64_t cap);
+
+/* cm_dpsearch_trunc.rs:1800 generic_tr_scan over dsq[i0..j0] (the QDB /
+ * non-banded truncated scanners ref_tr_cyk_scan :207 and ref_itr_inside_scan
+ * :226): truncated CYK on the float tables (inside = 0: sc, lmesc, rmesc, pty)
+ * or truncated Inside on the integer ones (inside = 1: is, ilmesc, irmesc,
+ * ipty), with per-state dmax (NULL: non-banded, W). Writes vsc_root, its mode
+ * and the full-window best (bsc_full); returns the number of hits after
+ * greedy reporting and overlap removal. best_out (optional): the per-(j, d)
+ * best root score in bits, [(j-i0)*(W+1) + d]. */
+int64_t ixo_trqdb_scan(const ixo_cm *cm, const ixo_cmsc *sc, const ixo_cmisc *is, const float *lmesc,
+                       const float *rmesc, const int32_t *ilmesc, const int32_t *irmesc, const float *pty,
+                       const int32_t *ipty, const int32_t *dmax, const uint8_t *dsq, int i0, int j0, int inside,
+                       int pass_idx, int local, float cutoff, int do_null3, float *vsc, int *vmode, float *bsc_full,
+                       ixo_trhit *hits, int64_t cap, float *best_out);
+
 /* ---- QDB / non-banded scanners (qdb.c) ---- */
 
 /* cm_nohmm.rs:406 generic_scan over dsq[i0..j0]: CYK on the float scores

@@ -74,6 +74,59 @@ void ixo_cm_marginals(const ixo_cm *cm, float *lmesc, float *rmesc) {
   }
 }
 
+/* cm_nohmm.rs:27-33 prob2score: floor(0.5 + 1000 log2(p / null)), -INFTY for p == 0 */
+#define TR_INFTY 987654321
+static int32_t tr_prob2score(float p, float null) {
+  if (p == 0.0f) return -TR_INFTY;
+  return (int32_t)floor(0.5 + 1000.0 * log2((double)p / (double)null));
+}
+/* cp9.rs:4391-4408 i_expect_score (esl_abc_IExpectScore): the null-weighted
+ * mean of the canonical integer scores over a degenerate code, rounded half
+ * away from zero; 0 for non-residue codes. */
+static int32_t i_expect_score(int x, const int32_t *sc, const float *p) {
+  if (!is_residue(x)) return 0;
+  float r = 0.0f, den = 0.0f;
+  for (int i = 0; i < K4; i++)
+    if (DEGEN[x][i]) {
+      r += (float)sc[i] * p[i];
+      den += p[i];
+    }
+  r /= den;
+  return r < 0.0f ? (int32_t)(r - 0.5f) : (int32_t)(r + 0.5f);
+}
+static void i_expect_vec(int32_t *sc, const float *p) { /* cp9.rs:4423-4429 */
+  for (int x = K4 + 1; x <= KP - 3; x++) sc[x] = i_expect_score(x, sc, p);
+}
+
+/* cp9.rs:4441-4527 fill_marginal_emissions (integer part ilmesc/irmesc). */
+void ixo_cm_imarginals(const ixo_cm *cm, int32_t *ilmesc, int32_t *irmesc) {
+  for (int v = 0; v < cm->M; v++) {
+    int32_t *lm = ilmesc + (size_t)v * KP, *rm = irmesc + (size_t)v * KP;
+    for (int x = 0; x < KP; x++) lm[x] = rm[x] = 0;
+    lm[K4] = rm[K4] = lm[KP - 2] = rm[KP - 2] = lm[KP - 1] = rm[KP - 1] = -TR_INFTY;
+    const int st = cm->sttype[v];
+    const float *e = cm->e + (size_t)v * 16;
+    if (st == IXO_MP_ST) {
+      float l4[4] = {0, 0, 0, 0}, r4[4] = {0, 0, 0, 0};
+      for (int x = 0; x < K4; x++)
+        for (int y = 0; y < K4; y++) {
+          l4[x] += e[x * 4 + y];
+          r4[y] += e[x * 4 + y];
+        }
+      for (int x = 0; x < K4; x++) {
+        lm[x] = tr_prob2score(l4[x], cm->null[x]);
+        rm[x] = tr_prob2score(r4[x], cm->null[x]);
+      }
+      i_expect_vec(lm, cm->null);
+      i_expect_vec(rm, cm->null);
+    } else if (st == IXO_ML_ST || st == IXO_IL_ST || st == IXO_MR_ST || st == IXO_IR_ST) {
+      int32_t *emit = (st == IXO_ML_ST || st == IXO_IL_ST) ? lm : rm;
+      for (int x = 0; x < K4; x++) emit[x] = tr_prob2score(e[x], cm->null[x]);
+      i_expect_vec(emit, cm->null);
+    }
+  }
+}
+
 static int node_of_state(const ixo_cm *cm, int v) {
   int nd = 0;
   while (nd + 1 < cm->nodes && cm->nodemap[nd + 1] <= v) nd++;
@@ -169,6 +222,11 @@ static int allow_truncation(const ixo_cm *cm, const int32_t *lpos, const int32_t
   return 0;
 }
 
+int ixo_tr_allow(const ixo_cm *cm, const int32_t *lpos, const int32_t *rpos, int pass_idx, int local, int start,
+                 int stop, int i0, int j0, int mode, int b) {
+  return allow_truncation(cm, lpos, rpos, pass_idx, local, start, stop, i0, j0, mode, b);
+}
+
 typedef struct {
   int64_t *start, *roff, ncells;
 } lay_t;
@@ -210,6 +268,36 @@ static int cmp_hits(const void *pa, const void *pb) {
   if (a->score < b->score) return 1;
   if (a->i != b->i) return a->i < b->i ? -1 : 1;
   return 0;
+}
+
+/* remove_overlaps_greedy_tr (cm_dpsearch_trunc.rs:2470-2495): score
+ * descending, then start ascending (a stable order: insertion sort keeps equal
+ * keys in place); then every hit overlapping a kept one is dropped. tmp is
+ * reordered; returns the number kept (beyond cap: counted only). */
+int64_t ixo_tr_remove_overlaps(ixo_trhit *tmp, int64_t ntmp, ixo_trhit *hits, int64_t cap) {
+  for (int64_t a = 1; a < ntmp; a++) {
+    ixo_trhit h = tmp[a];
+    int64_t k = a;
+    while (k > 0 && cmp_hits(&tmp[k - 1], &h) > 0) {
+      tmp[k] = tmp[k - 1];
+      k--;
+    }
+    tmp[k] = h;
+  }
+  char *rem = (char *)calloc((size_t)(ntmp + 1), 1);
+  int64_t nk = 0;
+  for (int64_t a = 0; a < ntmp; a++) {
+    if (rem[a]) continue;
+    for (int64_t k = a + 1; k < ntmp; k++)
+      if (!rem[k] && !(tmp[k].j < tmp[a].i) && !(tmp[a].j < tmp[k].i)) rem[k] = 1;
+  }
+  for (int64_t a = 0; a < ntmp; a++)
+    if (!rem[a]) {
+      if (nk < cap) hits[nk] = tmp[a];
+      nk++;
+    }
+  free(rem);
+  return nk;
 }
 
 int64_t ixo_trhb_scan(const ixo_cm *cm, const ixo_cmsc *sc, const float *lmesc, const float *rmesc, const float *pty,
@@ -609,31 +697,8 @@ int64_t ixo_trhb_scan(const ixo_cm *cm, const ixo_cmsc *sc, const float *lmesc, 
   *vmode = vm;
   *envi = ei == (int64_t)j0 + 1 ? -1 : ei;
   *envj = ej == (int64_t)i0 - 1 ? -1 : ej;
-  /* remove_overlaps_greedy_tr (:2470-2495): score descending, then start
-   * ascending (a stable order: insertion sort keeps equal keys in place) */
-  for (int64_t a = 1; a < ntmp; a++) {
-    ixo_trhit h = tmp[a];
-    int64_t k = a;
-    while (k > 0 && cmp_hits(&tmp[k - 1], &h) > 0) {
-      tmp[k] = tmp[k - 1];
-      k--;
-    }
-    tmp[k] = h;
-  }
-  char *rem = (char *)calloc((size_t)(ntmp + 1), 1);
-  int64_t nk = 0;
-  for (int64_t a = 0; a < ntmp; a++) {
-    if (rem[a]) continue;
-    for (int64_t k = a + 1; k < ntmp; k++)
-      if (!rem[k] && !(tmp[k].j < tmp[a].i) && !(tmp[a].j < tmp[k].i)) rem[k] = 1;
-  }
-  for (int64_t a = 0; a < ntmp; a++)
-    if (!rem[a]) {
-      if (nk < cap) hits[nk] = tmp[a];
-      nk++;
-    }
+  const int64_t nk = ixo_tr_remove_overlaps(tmp, ntmp, hits, cap);
   if (planes_out) memcpy(planes_out, ja, sizeof(float) * (size_t)(4 * nc));
-  free(rem);
   free(tmp);
   free(bestsc);
   free(bestr);

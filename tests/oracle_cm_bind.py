@@ -82,6 +82,12 @@ def lib():
         L.ixo_cyk_seq_filter.argtypes = [C.POINTER(Cm), C.POINTER(CmSc), _i32p, _i32p, _u8p, C.c_int, C.c_float,
                                          C.c_int, _i32p, _i32p, C.c_int64]
         L.ixo_cyk_seq_filter.restype = C.c_int64
+        L.ixo_cm_imarginals.argtypes = [C.POINTER(Cm), _i32p, _i32p]
+        L.ixo_trqdb_scan.argtypes = [C.POINTER(Cm), C.POINTER(CmSc), C.POINTER(CmISc), _f32p, _f32p, _i32p, _i32p,
+                                     _f32p, _i32p, _i32p, _u8p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_float, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_float), C.POINTER(TrHit), C.c_int64, _f32p]
+        L.ixo_trqdb_scan.restype = C.c_int64
         _ready = True
     return L
 
@@ -296,3 +302,54 @@ class OracleQdbCM:
                                      d.ctypes.data_as(_u8p), n, cutoff, int(do_null3), es.ctypes.data_as(_i32p),
                                      ee.ctypes.data_as(_i32p), cap)
         return list(zip(es[:k].tolist(), ee[:k].tolist()))
+
+
+PTY_INDEX = {2: 1, 3: 2, 4: 0, 5: 0}  # pass -> penalty row (cm_trunc.rs:99-107)
+
+
+class OracleTrQdbCM(OracleQdbCM):
+    """OracleQdbCM plus the QDB / non-banded truncated scanners
+    (generic_tr_scan): float and integer marginal emissions and the
+    truncated-begin penalties (float g/l and integer ig/il, [3, M])."""
+
+    def __init__(self, cm, local=False, seed=0, pen=None):
+        super().__init__(cm, local=local)
+        M = cm["M"]
+        self.lmesc = np.zeros((M, 18), np.float32)
+        self.rmesc = np.zeros((M, 18), np.float32)
+        self.ilmesc = np.zeros((M, 18), np.int32)
+        self.irmesc = np.zeros((M, 18), np.int32)
+        lib().ixo_cm_marginals(C.byref(self.s), self.lmesc.ctypes.data_as(_f32p), self.rmesc.ctypes.data_as(_f32p))
+        lib().ixo_cm_imarginals(C.byref(self.s), self.ilmesc.ctypes.data_as(_i32p), self.irmesc.ctypes.data_as(_i32p))
+        if pen is None:
+            from infernox_amd import synth_cm as S
+            g, l = S.tr_penalties(cm, seed)
+            ig, il = S.tr_ipenalties(cm, seed)
+            pen = dict(g=g, l=l, ig=ig, il=il)
+        self.pen = pen
+
+    def trscan(self, dsq, i0, j0, inside, pass_idx, local, dmax=None, cutoff=0.0, do_null3=True, want_best=False):
+        d = np.ascontiguousarray(dsq, np.uint8)
+        W = int(self.cm["W"])
+        row = PTY_INDEX[pass_idx]
+        pty = np.ascontiguousarray(self.pen["l" if local else "g"][row], np.float32)
+        ipty = np.ascontiguousarray(self.pen["il" if local else "ig"][row], np.int32)
+        dm = np.ascontiguousarray(dmax, np.int32) if dmax is not None else None
+        best = np.zeros((j0 - i0 + 1) * (W + 1), np.float32) if want_best else None
+        vsc, vmode, bfull = C.c_float(), C.c_int(), C.c_float()
+        cap = 1 << 14
+        hits = (TrHit * cap)()
+        n = lib().ixo_trqdb_scan(C.byref(self.s), C.byref(self.fs), C.byref(self.isc),
+                                 self.lmesc.ctypes.data_as(_f32p), self.rmesc.ctypes.data_as(_f32p),
+                                 self.ilmesc.ctypes.data_as(_i32p), self.irmesc.ctypes.data_as(_i32p),
+                                 pty.ctypes.data_as(_f32p), ipty.ctypes.data_as(_i32p),
+                                 dm.ctypes.data_as(_i32p) if dm is not None else None, d.ctypes.data_as(_u8p),
+                                 int(i0), int(j0), int(inside), int(pass_idx), int(local), float(cutoff),
+                                 int(do_null3), C.byref(vsc), C.byref(vmode), C.byref(bfull), hits, cap,
+                                 best.ctypes.data_as(_f32p) if want_best else None)
+        out = dict(vsc=vsc.value, vmode=vmode.value, bsc_full=bfull.value,
+                   hits=[(hits[k].i, hits[k].j, hits[k].score, hits[k].bias, hits[k].mode, hits[k].root)
+                         for k in range(min(n, cap))])
+        if want_best:
+            out["best"] = best.reshape(j0 - i0 + 1, W + 1)
+        return out
