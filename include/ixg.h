@@ -320,6 +320,37 @@ int ixg_cyk_seq_filter(ixg_ctx *ctx, const ixg_cm *cm, const uint8_t *dsq, int64
                        const int32_t *dmax1, float cutoff, int do_null3, int32_t *es, int32_t *ee, int64_t cap,
                        int64_t *nenv);
 
+/* ---- QDB / non-banded truncated scanners (cm_dpsearch_trunc.rs generic_tr_scan) -- */
+/* Integer truncated-begin penalties (TrPenalties ig/il, cm_trunc.rs:145-150,
+ * Prob2Score of the same probabilities as g/l): [3][M] each, same row order
+ * as ixg_cm_set_trpenalties, -987654321 where a truncated begin is not
+ * allowed. Required by the integer Inside scanner. */
+int ixg_cm_set_itrpenalties(ixg_ctx *ctx, ixg_cm *cm, const int32_t *ig, const int32_t *il);
+/* Host copies of the integer marginal emission scores ilmesc/irmesc (M*18
+ * each, cp9.rs:4441 fill_marginal_emissions). Either may be NULL. */
+int ixg_cm_imtables(const ixg_cm *cm, int32_t *ilmesc, int32_t *irmesc);
+/* cm_dpsearch_trunc.rs:207 ref_tr_cyk_scan (inside = 0: float scores and
+ * penalties) and :226 ref_itr_inside_scan (inside = 1: integer scores,
+ * integer marginals and penalties, ILogsum) over dsq[i0..j0], i.e.
+ * generic_tr_scan (:1800): the J/L/R/T planes with per-state dmax (the CM's
+ * dmax1 or dmax2; NULL = non-banded), truncated begins into root rows kept
+ * across j, greedy reporting with cm_hit_AllowTruncation and null3, overlap
+ * removal. Writes vsc_root and its mode, and bsc_full (the best root score
+ * of the full window, reported by the reference as the fourth result;
+ * may be NULL). pass_idx: a truncated pass (IXG_PASS_5P_ONLY_FORCE ..
+ * IXG_PASS_5P_AND_3P_ANY); local selects the local penalties and truncation
+ * rules. Hits beyond cap are counted in *nhits and IXG_ERANGE is returned. */
+int ixg_trqdb_scan(ixg_ctx *ctx, const ixg_cm *cm, const uint8_t *dsq, int64_t L, int32_t i0, int32_t j0,
+                   const int32_t *dmax, int inside, int pass_idx, int local, float cutoff, int do_null3, float *vsc,
+                   int32_t *vmode, float *bsc_full, ixg_trhit *hits, int64_t cap, int64_t *nhits);
+/* Batched form over a device-resident target: per-window vsc, vmode,
+ * bsc_full (may be NULL); hits tagged with env = window index, grouped by
+ * window in order. */
+int ixg_trqdb_scan_batch(ixg_ctx *ctx, const ixg_cm *cm, const ixg_target *tgt, const ixg_qdbwin *win, int64_t n,
+                         const int32_t *dmax, int inside, int pass_idx, int local, float cutoff, int do_null3,
+                         float *vsc, int32_t *vmode, float *bsc_full, ixg_trhit *hits, int64_t cap,
+                         int64_t *nhits);
+
 /* Per-kernel device time, cumulative since the last reset: HIP events around
  * every launch of each kernel (IXG_K_* index), total milliseconds and launch
  * counts; reset != 0 clears them after reading. A measurement hook with no
@@ -327,7 +358,7 @@ int ixg_cyk_seq_filter(ixg_ctx *ctx, const ixg_cm *cm, const uint8_t *dsq, int64
  * after each launch. */
 enum { IXG_K_SSV_SWEEP = 0, IXG_K_SSV_RESOLVE = 1, IXG_K_FWD = 2, IXG_K_BIAS = 3, IXG_K_VIT = 4, IXG_K_MSV = 5,
        IXG_K_HB_SCAN = 6, IXG_K_TRHB_SCAN = 7, IXG_K_COPY = 8, IXG_K_QDB_SCAN = 9, IXG_K_FWD_FULL = 10,
-       IXG_NKERNELS = 11 };
+       IXG_K_TRQDB_SCAN = 11, IXG_NKERNELS = 12 };
 int ixg_kernel_timings(ixg_ctx *ctx, double *ms, int64_t *launches, int reset);
 /* Measurement hook (no counterpart in the reference): copies `bytes`
  * device-to-device with a plain kernel reps times after one warm-up copy;

@@ -595,6 +595,7 @@ int ixg_cm_free(ixg_ctx *c, ixg_cm *cm) {
     dfree(c, cm->d_tr, cm->b_tr);
     dfree(c, cm->d_yl, cm->b_yl);
     dfree(c, cm->d_q, cm->b_q);
+    dfree(c, cm->d_tq, cm->b_tq);
   }
   delete cm;
   return IXG_OK;
@@ -857,3 +858,9 @@ int ixg_trhb_scan_batch(ixg_ctx *c, const ixg_cm *cm, const ixg_target *tgt, con
 }
 
 }  // extern "C"
+
+namespace ixg {
+bool tr_allow(const CmHost &h, int pass_idx, int local, int start, int stop, int i0, int j0, int mode, int b) {
+  return allow_truncation(h, pass_idx, local, start, stop, i0, j0, mode, b);
+}
+}  // namespace ixg

@@ -224,10 +224,61 @@ static float expect_score(int x, const float *sc, const float *p) {
   return r / den;
 }
 
+// cp9.rs:4391-4408 i_expect_score (esl_abc_IExpectScore): the null-weighted
+// mean of the canonical integer scores over a degenerate code, rounded half
+// away from zero; 0 for non-residue codes.
+static int32_t i_expect_score(int x, const int32_t *sc, const float *null) {
+  if (!(x < 4 || (x > 4 && x < KP - 2))) return 0;
+  float r = 0.0f, den = 0.0f;
+  for (int i = 0; i < 4; i++)
+    if (DEGEN[x][i]) {
+      r += (float)sc[i] * null[i];
+      den += null[i];
+    }
+  r /= den;
+  return r < 0.0f ? (int32_t)(r - 0.5f) : (int32_t)(r + 0.5f);
+}
+
+// cp9.rs:4441-4527 fill_marginal_emissions, integer part (ilmesc/irmesc).
+static void imarginal_emissions(const ixg_cm_params &p, CmHost &h) {
+  const int M = p.M;
+  h.ilmesc.assign((size_t)M * KP, 0);
+  h.irmesc.assign((size_t)M * KP, 0);
+  for (int v = 0; v < M; v++) {
+    int32_t *lm = &h.ilmesc[(size_t)v * KP], *rm = &h.irmesc[(size_t)v * KP];
+    lm[4] = rm[4] = lm[KP - 2] = rm[KP - 2] = lm[KP - 1] = rm[KP - 1] = -INFTY_I;
+    const int st = p.sttype[v];
+    const float *e = p.e + (size_t)v * 16;
+    int32_t *sides[2] = {nullptr, nullptr};
+    if (st == MP_ST) {
+      float l4[4] = {0, 0, 0, 0}, r4[4] = {0, 0, 0, 0};
+      for (int x = 0; x < 4; x++)
+        for (int y = 0; y < 4; y++) {
+          l4[x] += e[x * 4 + y];
+          r4[y] += e[x * 4 + y];
+        }
+      for (int x = 0; x < 4; x++) {
+        lm[x] = prob2score(l4[x], p.null[x]);
+        rm[x] = prob2score(r4[x], p.null[x]);
+      }
+      sides[0] = lm;
+      sides[1] = rm;
+    } else if (st == ML_ST || st == IL_ST || st == MR_ST || st == IR_ST) {
+      int32_t *own = (st == ML_ST || st == IL_ST) ? lm : rm;
+      for (int x = 0; x < 4; x++) own[x] = prob2score(e[x], p.null[x]);
+      sides[0] = own;
+    }
+    for (int32_t *s : sides)
+      if (s)
+        for (int x = 5; x <= KP - 3; x++) s[x] = i_expect_score(x, s, p.null);
+  }
+}
+
 // Marginal emission scores of the truncated scanners (cp9.rs:4441-4527): a
 // pair state's left (right) score is the log-odds of its row (column) sum; a
 // singlet state's own side is its emission score, the other side 0.
 void marginal_emissions(const ixg_cm_params &p, CmHost &h) {
+  imarginal_emissions(p, h);
   const int M = p.M;
   h.lmesc.assign((size_t)M * KP, 0.f);
   h.rmesc.assign((size_t)M * KP, 0.f);

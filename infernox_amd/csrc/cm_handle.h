@@ -15,6 +15,8 @@ struct ixg_cm {
   int32_t *d_q = nullptr;    // QDB tables: itsc | ioesc | ibeginsc | iendsc | ilogsum table |
                              // begl index | base0 (int) | el_sca (int) | base0 (f32 bits) | el_sca (f32 bits) | begin list
   size_t b_q = 0;
+  int32_t *d_tq = nullptr;   // truncated QDB tables: ilmesc | irmesc (M*18 each) | ig[3][M] | il[3][M]
+  size_t b_tq = 0;
   int32_t *d_yl = nullptr;   // per (local, pass index): states with an allowed truncated begin
   int32_t yoff[7] = {0, 0, 0, 0, 0, 0, 0};
   size_t b_i32 = 0, b_f = 0, b_tr = 0, b_yl = 0;
@@ -43,6 +45,13 @@ struct ixg_cm {
   const float *d_ftbl() const { return d_f + (size_t)h.M * ixg::MAXC + (size_t)h.M * ixg::KP * ixg::KP + 2 * (size_t)h.M; }
 };
 
-// Uploads the QDB scanner's integer scores and tables (cm->d_q).
+// Uploads the QDB scanner's integer scores and tables (cm->d_q) and the
+// truncated QDB scanner's integer marginal emissions (cm->d_tq).
 int qdb_upload(ixg_ctx *c, ixg_cm *cm);
+
+namespace ixg {
+// cm_dpsearch_trunc.rs:2376-2422 cm_hit_AllowTruncation for a hit start..stop
+// of marginal mode `mode` entered at state b.
+bool tr_allow(const CmHost &h, int pass_idx, int local, int start, int stop, int i0, int j0, int mode, int b);
+}  // namespace ixg
 

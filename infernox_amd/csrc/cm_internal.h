@@ -40,6 +40,8 @@ struct CmHost {
   std::vector<float> lmesc, rmesc;  // M*18 marginal emission scores
   std::vector<int32_t> lpos, rpos;  // per state: consensus span of its subtree
   std::vector<float> trp_g, trp_l;  // 3*M truncated-begin penalties (empty until set)
+  std::vector<int32_t> ilmesc, irmesc;  // M*18 integer marginal emission scores
+  std::vector<int32_t> trp_ig, trp_il;  // 3*M integer truncated-begin penalties (empty until set)
 };
 void marginal_emissions(const ixg_cm_params &p, CmHost &h);
 int state_spans(const ixg_cm_params &p, CmHost &h);
@@ -127,5 +129,37 @@ struct QdbCand {  // a root cell at or above the reporting cutoff
 };
 hipError_t launch_qdb_scan(const QdbDev &q, const void *tabs, int inside, const QdbWinDev *wins, int64_t n,
                            float cutoff, QdbCand *cand, int64_t cap, unsigned long long *counter, hipStream_t s);
+
+// The QDB / non-banded truncated scanners (generic_tr_scan).
+template <typename T>
+struct TrqTabs {
+  const T *tsc, *oesc, *endsc, *el_sca, *base0, *bbase0;
+  const T *lmesc, *rmesc;  // marginal emission scores [M][18]
+  const T *pty;            // the pass's truncated-begin penalties [M]
+};
+struct TrqDev {
+  const int8_t *sttype;
+  const int32_t *cfirst, *cnum;
+  const int32_t *begl;               // per state: BEGL_S deck index, -1 otherwise
+  const int32_t *lvl_states, *lvl_off;
+  const int32_t *dxcap;              // per state: min(dmax, W)
+  const int32_t *ilog;               // ILogsum table
+  int nlev, M, W, nbegl;
+  int fill;       // bit 1 L, bit 2 R, bit 3 T planes filled by the pass
+  int nonbanded;  // no dmax given: bifurcation splits over every k
+};
+struct TrqCand {  // a root cell whose best truncated begin is at or above the cutoff
+  int32_t win, j, d;
+  float sc;
+  int32_t root, mode;
+};
+struct TrqSummary {  // per window: best root score (first in (j, d) order), its mode, full-window best
+  float vsc;
+  int32_t vmode;
+  float bsc_full;
+};
+hipError_t launch_trqdb_scan(const TrqDev &q, const void *tabs, int inside, const QdbWinDev *wins, int64_t n,
+                             float cutoff, TrqCand *cand, int64_t cap, unsigned long long *counter, TrqSummary *summ,
+                             hipStream_t s);
 
 }  // namespace ixg

@@ -38,6 +38,14 @@ int qdb_upload(ixg_ctx *c, ixg_cm *cm) {
   cm->b_q = q.size() * 4;
   RCHK(dmalloc(c, (void **)&cm->d_q, cm->b_q));
   HIPCHK(hipMemcpy(cm->d_q, q.data(), cm->b_q, hipMemcpyHostToDevice));
+  // truncated QDB: integer marginal emissions, then integer penalties
+  // (-INFTY, i.e. no truncated begin, until ixg_cm_set_itrpenalties)
+  std::vector<int32_t> t(h.ilmesc);
+  t.insert(t.end(), h.irmesc.begin(), h.irmesc.end());
+  t.resize(t.size() + 6 * (size_t)h.M, -INFTY_I);
+  cm->b_tq = t.size() * 4;
+  RCHK(dmalloc(c, (void **)&cm->d_tq, cm->b_tq));
+  HIPCHK(hipMemcpy(cm->d_tq, t.data(), cm->b_tq, hipMemcpyHostToDevice));
   return IXG_OK;
 }
 
@@ -262,6 +270,202 @@ int qdb_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
   return IXG_OK;
 }
 
+// ---- truncated QDB scanners (cm_dpsearch_trunc.rs:1800 generic_tr_scan) ----
+
+struct TrqHostCand {
+  int32_t j, d;
+  float sc;
+  int32_t root, mode;
+};
+
+// Greedy reporting per j (report_hits_greedily with cm_hit_AllowTruncation
+// and null3, cm_dpsearch_trunc.rs:2322-2422), then overlap removal
+// (remove_overlaps_greedy_tr, :2470-2495); appends to out.
+void trq_report(const ixg_cm &cm, const uint8_t *dsq, int32_t i0, int32_t j0, std::vector<TrqHostCand> &cand,
+                int pass_idx, int local, float cutoff, int do_null3, int32_t env, std::vector<ixg_trhit> &out) {
+  std::sort(cand.begin(), cand.end(), [](const TrqHostCand &a, const TrqHostCand &b) {
+    return a.j != b.j ? a.j < b.j : a.d < b.d;
+  });
+  std::vector<double> act;
+  if (do_null3) {
+    const int L = j0 - i0 + 1;
+    act.assign((size_t)(L + 1) * 4, 0.0);
+    for (int jp = 1; jp <= L; jp++) {
+      for (int a = 0; a < 4; a++) act[jp * 4 + a] = act[(jp - 1) * 4 + a];
+      if (dsq[i0 + jp - 1] < 4) act[jp * 4 + dsq[i0 + jp - 1]] += 1.0;
+    }
+  }
+  std::vector<ixg_trhit> tmp;
+  size_t k = 0;
+  while (k < cand.size()) {
+    const int j = cand[k].j;
+    float maxrep = IMPOSSIBLE;
+    for (; k < cand.size() && cand[k].j == j; k++) {
+      const int d = cand[k].d, i = j - d + 1;
+      float hs = cand[k].sc, bias = 0.f;
+      if (!(hs > maxrep && hs >= cutoff && hs > IMPOSSIBLE + 1.0f)) continue;
+      bool rep = tr_allow(cm.h, pass_idx, local, i, j, i0, j0, cand[k].mode, cand[k].root);
+      if (rep && do_null3) {
+        float comp[4];
+        const int ip = i - i0 + 1, jj = j - i0 + 1;
+        for (int a = 0; a < 4; a++) comp[a] = (float)(act[jj * 4 + a] - act[(ip - 1) * 4 + a]);
+        float sum = 0.f;
+        for (int a = 0; a < 4; a++) sum += comp[a];
+        if (sum > 0.f)
+          for (int a = 0; a < 4; a++) comp[a] /= sum;
+        else
+          for (int a = 0; a < 4; a++) comp[a] = 0.25f;
+        const float corr = null3_score(cm.h, cm.ftbl_host, comp, d);
+        hs -= corr;
+        bias = corr;
+        rep = hs > maxrep && hs >= cutoff;
+      }
+      if (rep) {
+        tmp.push_back({i, j, hs, bias, cand[k].mode, cand[k].root, env});
+        maxrep = hs;
+      }
+    }
+  }
+  std::stable_sort(tmp.begin(), tmp.end(), [](const ixg_trhit &a, const ixg_trhit &b) {
+    if (a.score != b.score) return a.score > b.score;
+    return a.i < b.i;
+  });
+  std::vector<char> removed(tmp.size(), 0);
+  for (size_t a = 0; a < tmp.size(); a++) {
+    if (removed[a]) continue;
+    for (size_t b = a + 1; b < tmp.size(); b++)
+      if (!removed[b] && !(tmp[b].j < tmp[a].i) && !(tmp[a].j < tmp[b].i)) removed[b] = 1;
+  }
+  for (size_t a = 0; a < tmp.size(); a++)
+    if (!removed[a]) out.push_back(tmp[a]);
+}
+
+// Runs the truncated scan of every window (over the device sequence base +
+// seq_off); appends each window's hits (env = window index) to out and its
+// summary to summ.
+int trq_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vector<QWin> &wins, const int32_t *dmax,
+            int inside, int pass_idx, int local, float cutoff, int do_null3, std::vector<ixg_trhit> &out,
+            std::vector<TrqSummary> &summ) {
+  const CmHost &h = cm->h;
+  const int M = h.M, W = h.W;
+  const int nbegl = (int)std::count_if(h.qbegl.begin(), h.qbegl.end(), [](int b) { return b >= 0; });
+  int row, fill;
+  switch (pass_idx) {  // cm_trunc.rs:99-107, cm_dpsearch_trunc.rs:184-199
+    case IXG_PASS_5P_ONLY_FORCE: row = 1, fill = 4; break;
+    case IXG_PASS_3P_ONLY_FORCE: row = 2, fill = 2; break;
+    case IXG_PASS_5P_AND_3P_FORCE:
+    case IXG_PASS_5P_AND_3P_ANY: row = 0, fill = 2 | 4 | 8; break;
+    default:
+      g_err = "truncated QDB scan: pass_idx must be a truncated pass";
+      return IXG_EINVAL;
+  }
+  if (inside ? h.trp_ig.empty() : h.trp_g.empty()) {
+    g_err = inside ? "truncated QDB Inside: ixg_cm_set_itrpenalties has not been called"
+                   : "truncated QDB CYK: ixg_cm_set_trpenalties has not been called";
+    return IXG_EINVAL;
+  }
+  std::vector<int32_t> dxcap(M);
+  for (int v = 0; v < M; v++) {
+    if (dmax && dmax[v] < 0) {
+      g_err = "truncated QDB scan: negative band";
+      return IXG_EINVAL;
+    }
+    dxcap[v] = std::min(dmax ? dmax[v] : W, W);
+  }
+  int32_t *d_dx;
+  RCHK(scratch(c, S_CM_BANDS, (size_t)M * 4, (void **)&d_dx));
+  HIPCHK(hipMemcpyAsync(d_dx, dxcap.data(), (size_t)M * 4, hipMemcpyHostToDevice, c->stream));
+  const QdbView qv = qview(cm);
+  const CmDev cd = cm->dev();
+  TrqDev q;
+  q.sttype = cd.sttype;
+  q.cfirst = cd.cfirst;
+  q.cnum = cd.cnum;
+  q.begl = qv.begl;
+  q.lvl_states = cd.lvl_states;
+  q.lvl_off = cd.lvl_off;
+  q.dxcap = d_dx;
+  q.ilog = qv.ilog;
+  q.nlev = cd.nlev;
+  q.M = M;
+  q.W = W;
+  q.nbegl = nbegl;
+  q.fill = fill;
+  q.nonbanded = dmax ? 0 : 1;
+  const size_t pq = (size_t)((local ? 3 : 0) + row) * M;
+  TrqTabs<float> tf{cd.tsc, cd.oesc, cd.endsc, qv.fel, qv.fbase0, qv.fbbase0, cd.lmesc, cd.rmesc, cm->d_tr + pq};
+  const int32_t *tq = cm->d_tq;
+  TrqTabs<int32_t> ti{qv.itsc, qv.ioesc, qv.iendsc, qv.iel, qv.ibase0, qv.ibbase0,
+                      tq, tq + (size_t)M * KP, tq + 2 * (size_t)M * KP + pq};
+  const void *tabs = inside ? (const void *)&ti : (const void *)&tf;
+  const int64_t n = (int64_t)wins.size();
+  summ.assign(n, TrqSummary{IMPOSSIBLE, IXG_TRMODE_UNKNOWN, IMPOSSIBLE});
+  const int64_t budget = (int64_t)1 << 28;  // plane and deck elements per launch group (1 GiB)
+  int64_t k0 = 0;
+  while (k0 < n) {
+    int64_t k1 = k0, elems = 0, res = 0;
+    std::vector<int64_t> at;
+    while (k1 < n) {
+      const int L = wins[k1].j0 - wins[k1].i0 + 1, w = std::min(W, L), W1 = w + 1;
+      const int64_t need = 8 * (int64_t)M * W1 + 3 * (int64_t)W1 * std::max(nbegl, 1) * W1;
+      if (k1 > k0 && elems + need > budget) break;
+      at.push_back(elems);
+      elems += need;
+      res += L;
+      k1++;
+    }
+    void *d_mx;
+    QdbWinDev *d_w;
+    TrqSummary *d_s;
+    RCHK(scratch(c, S_CM_ALPHA, (size_t)elems * 4, &d_mx));
+    RCHK(scratch(c, S_CM_ENV, (size_t)(k1 - k0) * sizeof(QdbWinDev), (void **)&d_w));
+    RCHK(scratch(c, S_SC1, (size_t)(k1 - k0) * sizeof(TrqSummary), (void **)&d_s));
+    std::vector<QdbWinDev> wd(k1 - k0);
+    for (int64_t k = k0; k < k1; k++) {
+      QdbWinDev &x = wd[k - k0];
+      const int L = wins[k].j0 - wins[k].i0 + 1, w = std::min(W, L), W1 = w + 1;
+      x.dsq = d_base + wins[k].seq_off;
+      x.i0 = wins[k].i0;
+      x.j0 = wins[k].j0;
+      x.w = w;
+      x.win = (int32_t)(k - k0);
+      x.rows = (int32_t *)d_mx + at[k - k0];
+      x.begl = (int32_t *)d_mx + at[k - k0] + 8 * (int64_t)M * W1;
+    }
+    HIPCHK(hipMemcpyAsync(d_w, wd.data(), wd.size() * sizeof(QdbWinDev), hipMemcpyHostToDevice, c->stream));
+    int64_t cap = res / 8 + 4096;
+    std::vector<TrqCand> hc;
+    for (int attempt = 0; attempt < 2; attempt++) {
+      TrqCand *d_c;
+      unsigned long long *d_cnt;
+      RCHK(scratch(c, S_CM_ROOT, (size_t)cap * sizeof(TrqCand), (void **)&d_c));
+      RCHK(scratch(c, S_COUNTER, 8, (void **)&d_cnt));
+      static const unsigned long long zero = 0;
+      HIPCHK(hipMemcpyAsync(d_cnt, &zero, 8, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(ktime_begin(c));
+      HIPCHK(launch_trqdb_scan(q, tabs, inside, d_w, k1 - k0, cutoff, d_c, cap, d_cnt, d_s, c->stream));
+      HIPCHK(ktime_end(c, IXG_K_TRQDB_SCAN));
+      unsigned long long cnt = 0;
+      HIPCHK(hipMemcpy(&cnt, d_cnt, 8, hipMemcpyDeviceToHost));
+      if ((int64_t)cnt > cap) {
+        cap = (int64_t)cnt + 1024;
+        continue;
+      }
+      hc.resize(cnt);
+      if (cnt) HIPCHK(hipMemcpy(hc.data(), d_c, cnt * sizeof(TrqCand), hipMemcpyDeviceToHost));
+      break;
+    }
+    HIPCHK(hipMemcpy(summ.data() + k0, d_s, (size_t)(k1 - k0) * sizeof(TrqSummary), hipMemcpyDeviceToHost));
+    std::vector<std::vector<TrqHostCand>> per(k1 - k0);
+    for (const TrqCand &x : hc) per[x.win].push_back({x.j, x.d, x.sc, x.root, x.mode});
+    for (int64_t k = k0; k < k1; k++)
+      trq_report(*cm, wins[k].host_dsq, wins[k].i0, wins[k].j0, per[k - k0], pass_idx, local, cutoff, do_null3,
+                 (int32_t)k, out);
+    k0 = k1;
+  }
+  return IXG_OK;
+}
+
 int check_window(const ixg_cm *cm, const uint8_t *dsq, int64_t L, int32_t i0, int32_t j0) {
   if (!cm || !cm->d_q || !dsq || i0 < 1 || j0 > L || j0 < i0) {
     g_err = "QDB scan: bad arguments (1 <= i0 <= j0 <= L, device CM)";
@@ -330,6 +534,81 @@ int ixg_qdb_scan_batch(ixg_ctx *c, const ixg_cm *cm, const ixg_target *tgt, cons
   }
   std::vector<ixg_hit> out;
   RCHK(qdb_run(c, cm, base, w, dmin, dmax, inside ? 1 : 0, cutoff, do_null3, out));
+  *nhits = (int64_t)out.size();
+  for (int64_t k = 0; k < std::min<int64_t>(cap, out.size()); k++) hits[k] = out[k];
+  return (int64_t)out.size() > cap ? IXG_ERANGE : IXG_OK;
+}
+
+int ixg_cm_set_itrpenalties(ixg_ctx *c, ixg_cm *cm, const int32_t *ig, const int32_t *il) {
+  if (!c || !cm || !cm->d_tq || !ig || !il) {
+    g_err = "ixg_cm_set_itrpenalties: bad arguments (device CM required)";
+    return IXG_EINVAL;
+  }
+  const int M = cm->h.M;
+  cm->h.trp_ig.assign(ig, ig + 3 * (size_t)M);
+  cm->h.trp_il.assign(il, il + 3 * (size_t)M);
+  HIPCHK(hipSetDevice(c->device));
+  int32_t *dst = cm->d_tq + 2 * (size_t)M * KP;
+  HIPCHK(hipMemcpy(dst, ig, 3 * (size_t)M * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(dst + 3 * (size_t)M, il, 3 * (size_t)M * 4, hipMemcpyHostToDevice));
+  return IXG_OK;
+}
+
+int ixg_cm_imtables(const ixg_cm *cm, int32_t *ilmesc, int32_t *irmesc) {
+  if (!cm) return IXG_EINVAL;
+  if (ilmesc) memcpy(ilmesc, cm->h.ilmesc.data(), cm->h.ilmesc.size() * 4);
+  if (irmesc) memcpy(irmesc, cm->h.irmesc.data(), cm->h.irmesc.size() * 4);
+  return IXG_OK;
+}
+
+int ixg_trqdb_scan(ixg_ctx *c, const ixg_cm *cm, const uint8_t *dsq, int64_t L, int32_t i0, int32_t j0,
+                   const int32_t *dmax, int inside, int pass_idx, int local, float cutoff, int do_null3, float *vsc,
+                   int32_t *vmode, float *bsc_full, ixg_trhit *hits, int64_t cap, int64_t *nhits) {
+  if (!c || !vsc || !vmode || !nhits || (cap > 0 && !hits)) return IXG_EINVAL;
+  RCHK(check_window(cm, dsq, L, i0, j0));
+  HIPCHK(hipSetDevice(c->device));
+  uint8_t *d;
+  RCHK(scratch(c, S_CM_SEQ, (size_t)L + 2, (void **)&d));
+  HIPCHK(hipMemcpyAsync(d, dsq, (size_t)L + 1, hipMemcpyHostToDevice, c->stream));
+  std::vector<ixg_trhit> out;
+  std::vector<TrqSummary> sm;
+  RCHK(trq_run(c, cm, d, {QWin{i0, j0, 0, dsq}}, dmax, inside ? 1 : 0, pass_idx, local ? 1 : 0, cutoff, do_null3,
+               out, sm));
+  *vsc = sm[0].vsc;
+  *vmode = sm[0].vmode;
+  if (bsc_full) *bsc_full = sm[0].bsc_full;
+  *nhits = (int64_t)out.size();
+  for (int64_t k = 0; k < std::min<int64_t>(cap, out.size()); k++) hits[k] = out[k];
+  return (int64_t)out.size() > cap ? IXG_ERANGE : IXG_OK;
+}
+
+int ixg_trqdb_scan_batch(ixg_ctx *c, const ixg_cm *cm, const ixg_target *tgt, const ixg_qdbwin *win, int64_t n,
+                         const int32_t *dmax, int inside, int pass_idx, int local, float cutoff, int do_null3,
+                         float *vsc, int32_t *vmode, float *bsc_full, ixg_trhit *hits, int64_t cap,
+                         int64_t *nhits) {
+  if (!c || !cm || !cm->d_q || !tgt || (n && (!win || !vsc || !vmode)) || !nhits || (cap > 0 && !hits)) {
+    g_err = "ixg_trqdb_scan_batch: bad arguments";
+    return IXG_EINVAL;
+  }
+  HIPCHK(hipSetDevice(c->device));
+  int64_t L = 0;
+  const uint8_t *base = ixg_target_device_base(tgt, &L);
+  std::vector<QWin> w(n);
+  for (int64_t k = 0; k < n; k++) {
+    if (win[k].i0 < 1 || win[k].j0 > L || win[k].j0 < win[k].i0 || (win[k].strand != 0 && win[k].strand != 1)) {
+      g_err = "ixg_trqdb_scan_batch: window outside the target";
+      return IXG_EINVAL;
+    }
+    w[k] = QWin{win[k].i0, win[k].j0, win[k].strand ? L + 2 : 0, ixg_target_host_dsq(tgt, win[k].strand)};
+  }
+  std::vector<ixg_trhit> out;
+  std::vector<TrqSummary> sm;
+  RCHK(trq_run(c, cm, base, w, dmax, inside ? 1 : 0, pass_idx, local ? 1 : 0, cutoff, do_null3, out, sm));
+  for (int64_t k = 0; k < n; k++) {
+    vsc[k] = sm[k].vsc;
+    vmode[k] = sm[k].vmode;
+    if (bsc_full) bsc_full[k] = sm[k].bsc_full;
+  }
   *nhits = (int64_t)out.size();
   for (int64_t k = 0; k < std::min<int64_t>(cap, out.size()); k++) hits[k] = out[k];
   return (int64_t)out.size() > cap ? IXG_ERANGE : IXG_OK;

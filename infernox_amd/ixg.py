@@ -59,10 +59,11 @@ EXPORTS = [
     "ixg_cyk_scan_hb", "ixg_inside_scan_hb", "ixg_hb_scan_batch", "ixg_cm_set_trpenalties", "ixg_cm_trtables",
     "ixg_trcyk_scan_hb", "ixg_trinside_scan_hb", "ixg_trhb_scan_batch", "ixg_kernel_timings",
     "ixg_copy_probe", "ixg_cm_create_cfg", "ixg_cm_itables", "ixg_qdb_scan", "ixg_qdb_scan_batch",
-    "ixg_cyk_seq_filter", "ixg_p7_forward", "ixg_p7_forward_batch",
+    "ixg_cyk_seq_filter", "ixg_p7_forward", "ixg_p7_forward_batch", "ixg_cm_set_itrpenalties", "ixg_cm_imtables",
+    "ixg_trqdb_scan", "ixg_trqdb_scan_batch",
 ]
 KERNELS = ["ssv_sweep", "ssv_resolve", "fwd", "bias", "vit", "msv", "hb_scan", "trhb_scan", "copy_probe",
-           "qdb_scan", "fwd_full"]
+           "qdb_scan", "fwd_full", "trqdb_scan"]
 
 
 class CmParams(C.Structure):
@@ -159,6 +160,12 @@ def lib():
         L.ixg_hb_scan_batch.argtypes = [vp, vp, vp, C.POINTER(Envelope), i64, C.c_int, C.c_float, C.c_int,
                                         C.c_float, C.c_int, f32p, i64p, i64p, C.POINTER(Hit), i64, i64p, i64p]
         L.ixg_cm_set_trpenalties.argtypes = [vp, vp, f32p, f32p]
+        L.ixg_cm_set_itrpenalties.argtypes = [vp, vp, i32p, i32p]
+        L.ixg_cm_imtables.argtypes = [vp, i32p, i32p]
+        L.ixg_trqdb_scan.argtypes = [vp, vp, u8p, i64, i32, i32, i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int,
+                                     f32p, i32p, f32p, C.POINTER(TrHit), i64, i64p]
+        L.ixg_trqdb_scan_batch.argtypes = [vp, vp, vp, C.POINTER(QdbWin), i64, i32p, C.c_int, C.c_int, C.c_int,
+                                           C.c_float, C.c_int, f32p, i32p, f32p, C.POINTER(TrHit), i64, i64p]
         L.ixg_cm_trtables.argtypes = [vp, f32p, f32p, i32p, i32p]
         for nm in ("ixg_trcyk_scan_hb", "ixg_trinside_scan_hb"):
             getattr(L, nm).argtypes = [vp, vp, C.POINTER(Bands), C.POINTER(TrValid), u8p, i64, i32, i32, C.c_float,
@@ -537,6 +544,36 @@ class CM:
         check(lib().ixg_cm_set_trpenalties(self.ctx.h if self.ctx else None, self.h, f32ptr(self._trp[0]),
                                            f32ptr(self._trp[1])), "ixg_cm_set_trpenalties")
 
+    def set_itrpenalties(self, ig, il):
+        """Integer truncated-begin penalties ig[3, M], il[3, M]."""
+        self._itrp = (np.ascontiguousarray(ig, np.int32), np.ascontiguousarray(il, np.int32))
+        i32p = C.POINTER(C.c_int32)
+        check(lib().ixg_cm_set_itrpenalties(self.ctx.h, self.h, self._itrp[0].ctypes.data_as(i32p),
+                                            self._itrp[1].ctypes.data_as(i32p)), "ixg_cm_set_itrpenalties")
+
+    def imtables(self):
+        M = self.M
+        lm, rm = np.zeros(M * 18, np.int32), np.zeros(M * 18, np.int32)
+        i32p = C.POINTER(C.c_int32)
+        check(lib().ixg_cm_imtables(self.h, lm.ctypes.data_as(i32p), rm.ctypes.data_as(i32p)))
+        return lm.reshape(M, 18), rm.reshape(M, 18)
+
+    def trqdb_scan(self, dsq, L, i0, j0, pass_idx, local=True, inside=False, dmax=None, cutoff=0.0, null3=True,
+                   cap=1 << 14):
+        """QDB / non-banded TrCYK (float) or TrInside (integer) of one window:
+        dict(vsc, vmode, bsc_full, hits [(i, j, score, bias, mode, root)])."""
+        d = np.ascontiguousarray(dsq, np.uint8)
+        i32p = C.POINTER(C.c_int32)
+        dm = np.ascontiguousarray(dmax, np.int32) if dmax is not None else None
+        vsc, vm, bf, n = np.zeros(1, np.float32), np.zeros(1, np.int32), np.zeros(1, np.float32), C.c_int64()
+        hits = (TrHit * cap)()
+        check(lib().ixg_trqdb_scan(self.ctx.h, self.h, u8ptr(d), int(L), int(i0), int(j0),
+                                   dm.ctypes.data_as(i32p) if dm is not None else None, int(inside), int(pass_idx),
+                                   int(local), float(cutoff), int(null3), f32ptr(vsc), vm.ctypes.data_as(i32p),
+                                   f32ptr(bf), hits, cap, C.byref(n)), "ixg_trqdb_scan")
+        return dict(vsc=float(vsc[0]), vmode=int(vm[0]), bsc_full=float(bf[0]),
+                    hits=[h[1:] for h in _trhits(hits, n.value)])
+
     def trtables(self):
         M = self.M
         lm, rm = np.zeros(M * 18, np.float32), np.zeros(M * 18, np.float32)
@@ -621,6 +658,25 @@ def trhb_scan_batch(ctx, cm, tgt, envs, pass_idx, local=True, inside=False, env_
                                     vm.ctypes.data_as(C.POINTER(C.c_int32)), i64ptr(ei), i64ptr(ej), hits, cap,
                                     C.byref(nh), C.byref(cells)), "ixg_trhb_scan_batch")
     return vsc[:n], vm[:n], ei[:n], ej[:n], _trhits(hits, nh.value), cells.value
+
+
+def trqdb_scan_batch(ctx, cm, tgt, wins, pass_idx, local=True, inside=False, dmax=None, cutoff=0.0, null3=True):
+    """wins: list of (i0, j0, strand). Returns (vsc, vmode, bsc_full, hits (env, i, j, score, bias, mode, root))."""
+    n = len(wins)
+    arr = (QdbWin * max(n, 1))()
+    for k, (i0, j0, strand) in enumerate(wins):
+        arr[k].i0, arr[k].j0, arr[k].strand, arr[k].pad = int(i0), int(j0), int(strand), 0
+    i32p = C.POINTER(C.c_int32)
+    dm = np.ascontiguousarray(dmax, np.int32) if dmax is not None else None
+    vsc, vm, bf = np.zeros(max(n, 1), np.float32), np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1), np.float32)
+    cap = 1 << 16
+    hits = (TrHit * cap)()
+    nh = C.c_int64()
+    check(lib().ixg_trqdb_scan_batch(ctx.h, cm.h, tgt.h, arr, n, dm.ctypes.data_as(i32p) if dm is not None else None,
+                                     int(inside), int(pass_idx), int(local), float(cutoff), int(null3), f32ptr(vsc),
+                                     vm.ctypes.data_as(i32p), f32ptr(bf), hits, cap, C.byref(nh)),
+          "ixg_trqdb_scan_batch")
+    return vsc[:n], vm[:n], bf[:n], _trhits(hits, nh.value)
 
 
 def qdb_scan_batch(ctx, cm, tgt, wins, inside=False, dmin=None, dmax=None, cutoff=0.0, null3=True):

@@ -118,7 +118,7 @@ def test_cm_host_tables_equal_oracle(ss, seed):
 @pytest.mark.parametrize("ss,seed", [("<<..>>.", 1), ("<.>.<.>", 2), (None, 3)])
 def test_cm_truncation_tables_equal_oracle(ss, seed):
     from infernox_amd import synth_cm as S
-    from oracle_cm_bind import OracleTrCM
+    from oracle_cm_bind import OracleTrCM, OracleTrQdbCM
     cm = S.build_cm(ss if ss else S.multibranch(120, 4, seed), seed)
     g = ixg.CM(None, cm)
     lm, rm, lp, rp = g.trtables()
@@ -128,6 +128,9 @@ def test_cm_truncation_tables_equal_oracle(ss, seed):
     assert np.array_equal(lp, sl) and np.array_equal(rp, sr)
     gp, lpn = S.tr_penalties(cm, seed)
     g.set_trpenalties(gp, lpn)  # host-only CM: stores the tables
+    ilm, irm = g.imtables()
+    oq = OracleTrQdbCM(cm)
+    assert ilm.tobytes() == oq.ilmesc.tobytes() and irm.tobytes() == oq.irmesc.tobytes()
     g.free()
 
 
@@ -142,6 +145,11 @@ def test_truncated_scan_rejects_bad_arguments():
                              C.byref(ei), C.byref(ej), None, 0, C.byref(n))
     assert rc == ixg.IXG_EINVAL
     assert L.ixg_cm_set_trpenalties(None, None, None, None) == ixg.IXG_EINVAL
+    assert L.ixg_cm_set_itrpenalties(None, g.h, None, None) == ixg.IXG_EINVAL
+    f, m = C.c_float(), C.c_int32()
+    rc = L.ixg_trqdb_scan(None, g.h, None, 0, 1, 1, None, 0, 2, 1, 0.0, 0, C.byref(f), C.byref(m), None, None, 0,
+                          C.byref(n))
+    assert rc == ixg.IXG_EINVAL
     # a node list that is not a guide tree (bifurcation child node before its parent)
     bad = dict(cm)
     bad["ndtype"] = np.array([S.ROOT_ND] + [S.BIF_ND] * (cm["nodes"] - 1), np.int8)
