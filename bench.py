@@ -155,6 +155,106 @@ def time_trcm(ctx, steps, warmup):
     return out
 
 
+def qdb_workload(n_win=48, seed=4, slack=8):
+    """QDB / non-banded scanner workload: the 120-column, 3-stem CM of the
+    sequence-database config, n_win windows each holding one emitted homolog
+    with 20 random flanking residues per side (both strands alternate), and
+    per-state QDB bands of the consensus subtree span +- slack."""
+    cm = synth_cm.build_cm(synth_cm.multibranch(120, 3, seed), seed)
+    rng = np.random.default_rng(seed + 5)
+    parts, wins, pos = [], [], 1
+    for k in range(n_win):
+        h = synth_cm.emit(cm, 3000 + k)
+        left, right = rng.integers(0, 4, 20).astype(np.uint8), rng.integers(0, 4, 20).astype(np.uint8)
+        parts += [left, h, right]
+        i0, j0 = pos, pos + 40 + len(h) - 1
+        wins.append((i0, j0, 0))
+        pos = j0 + 1
+    seq = np.concatenate(parts)
+    L = len(seq)
+    dsq = np.concatenate([[255], seq, [255]]).astype(np.uint8)
+    lp, rp = synth_cm.state_spans(cm)
+    span = np.maximum(0, rp - lp + 1)
+    dmin = np.maximum(0, span - slack).astype(np.int32)
+    dmax = (span + slack).astype(np.int32)
+    return cm, dsq, L, wins, dmin, dmax
+
+
+def qdb_cells(cm, wins, dmin, dmax, trunc=False):
+    """(state, j, d) cells a QDB / non-banded scan fills: per window and j,
+    d from the state's lower bound to min(j - i0 + 1, dmax, W)."""
+    st = np.asarray(cm["sttype"])
+    W = int(cm["W"])
+    dx_cap = np.minimum(W, dmax if dmax is not None else W)
+    base = np.where(st == synth_cm.MP_ST, 2, 1)
+    dn = np.ones_like(base) if trunc else np.minimum(np.maximum(dmin if dmin is not None else 0, base), W)
+    live = st != synth_cm.E_ST
+    live[0] = False
+    n = 0
+    for i0, j0, _ in wins:
+        for jrow in range(1, j0 - i0 + 2):
+            dx = np.minimum(min(jrow, W), dx_cap)
+            n += int(np.maximum(0, dx - dn + 1)[live].sum())
+    return n
+
+
+def time_qdb(ctx, steps):
+    cm, dsq, L, wins, dmin, dmax = qdb_workload()
+    tgt = ixg.Target(ctx, dsq, L)
+    out = {}
+    g = ixg.CM(ctx, cm, local=True)
+    for name, inside, band in (("cyk_qdb", False, True), ("inside_qdb", True, True), ("cyk_nonbanded", False, False)):
+        dmn, dmx = (dmin, dmax) if band else (None, None)
+        cells = qdb_cells(cm, wins, dmn, dmx)
+        ixg.qdb_scan_batch(ctx, g, tgt, wins, inside=inside, dmin=dmn, dmax=dmx, cutoff=0.0)
+        ctx.kernel_timings(reset=True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            hits = ixg.qdb_scan_batch(ctx, g, tgt, wins, inside=inside, dmin=dmn, dmax=dmx, cutoff=0.0)
+        wall = (time.perf_counter() - t0) / steps
+        kms = ctx.kernel_timings(reset=True)["qdb_scan"][0] / steps
+        out[name] = {"gcups": cells / (kms * 1e-3) / 1e9, "kernel_ms": kms, "wall_ms": wall * 1e3,
+                     "cells": cells, "windows": len(wins), "states": int(cm["M"]), "hits": len(hits)}
+    gp, lp = synth_cm.tr_penalties(cm, 4)
+    ig, il = synth_cm.tr_ipenalties(cm, 4)
+    g.set_trpenalties(gp, lp)
+    g.set_itrpenalties(ig, il)
+    cells = qdb_cells(cm, wins, None, None, trunc=True)
+    for name, inside in (("trcyk_nonbanded", False), ("trinside_nonbanded", True)):
+        ixg.trqdb_scan_batch(ctx, g, tgt, wins, synth_cm.PASS_5P_AND_3P_FORCE, inside=inside, cutoff=0.0)
+        ctx.kernel_timings(reset=True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            vsc, vm, bf, hits = ixg.trqdb_scan_batch(ctx, g, tgt, wins, synth_cm.PASS_5P_AND_3P_FORCE,
+                                                     inside=inside, cutoff=0.0)
+        wall = (time.perf_counter() - t0) / steps
+        kms = ctx.kernel_timings(reset=True)["trqdb_scan"][0] / steps
+        out[name] = {"gcups_per_plane": cells / (kms * 1e-3) / 1e9, "kernel_ms": kms, "wall_ms": wall * 1e3,
+                     "plane_cells": cells, "windows": len(wins), "pass": "5'&3'", "hits": len(hits)}
+    g.free()
+    tgt.free()
+    return out
+
+
+def time_p7_forward(ctx, p, d, steps, n=256, length=400):
+    """Full-matrix Forward (retained rows, multihit) over n windows of the
+    genome: kernel time and DP cells per second."""
+    g = ixg.P7(ctx, p, 400)
+    seq = np.ascontiguousarray(d[:n * length + 2], np.uint8)
+    regions = [(k * length, length) for k in range(n)]
+    g.p7_forward_batch(seq, regions)
+    ctx.kernel_timings(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.p7_forward_batch(seq, regions)
+    wall = (time.perf_counter() - t0) / steps
+    kms = ctx.kernel_timings(reset=True)["fwd_full"][0] / steps
+    cells = n * length * int(p["M"])
+    g.free()
+    return {"gcups": cells / (kms * 1e-3) / 1e9, "kernel_ms": kms, "wall_ms": wall * 1e3, "windows": n,
+            "length": length, "M": int(p["M"]), "matrix_MB": 3 * n * (length + 1) * (int(p["M"]) + 1) * 4 / 1e6}
+
+
 def cpu_baseline(p, d, L, M, maxw, tasks, nt, budget_s, threads):
     """Times the oracle (C restatement, not infernox) on a bounded sample of
     the same tasks: as many tasks as fit in budget_s, run on `threads`
@@ -236,6 +336,8 @@ def main():
         g.free()
         cm_line = time_cm(ctx, max(1, a.steps // 2), 1) if not a.no_cm else None
         tr_line = time_trcm(ctx, max(1, a.steps // 2), 1) if not a.no_cm else None
+        qdb_line = time_qdb(ctx, 2) if not a.no_cm else None
+        pf_line = time_p7_forward(ctx, p, d, 2) if not a.no_cm else None
     # dominant kernel with a per-unit op count, and its VALU roofline: ops per
     # launch (the last step's cells x ops per cell, from the reference loop
     # bodies) over the kernel's mean launch time (HIP events around each launch)
@@ -276,6 +378,10 @@ def main():
         line["hb_scanners"] = cm_line
     if tr_line:
         line["hb_truncated_scanners"] = tr_line
+    if qdb_line:
+        line["qdb_scanners"] = qdb_line
+    if pf_line:
+        line["p7_forward_full"] = pf_line
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(p, d, L, M, maxw, tasks, nt, a.cpu_seconds, a.cpu_threads)
     print(json.dumps(line))
