@@ -106,22 +106,41 @@ template <typename T>
 struct QdbTabs {
   const T *tsc, *oesc, *endsc, *beginsc, *base0, *el_sca, *bbase0;
 };
+// The wavefront schedule of one scan call: at step t every state v works on
+// the end position j = i0 + t - lvl[v] (lvl: one more than its highest
+// child). All of a step's cells read only cells written at earlier steps, so
+// one barrier per step suffices. Per state the rows of the j's still to be
+// read live in a ring (a power of two deep); each BEGL_S deck is a ring of
+// w+1+gap rows; local begins are folded into per-(j, slot, d) maxima that
+// the root reads and clears.
 struct QdbDev {
   const int8_t *sttype;
   const int32_t *cfirst, *cnum;
   const int32_t *begl;  // per state: BEGL_S deck index, -1 otherwise
-  const int32_t *lvl_states, *lvl_off;
-  const int32_t *blist;  // states y >= 1 with a local begin, ascending
   const int32_t *dn, *dxcap;  // per state, this call's bands: max(dmin, emissions), min(dmax, W)
   const int32_t *dminw;       // per state, min(dmin, W): a bifurcation's split range
   const int32_t *ilog;        // ILogsum table
-  int nlev, nb, M, W, nbegl, local;
+  const int32_t *lvl;         // per state: dependency level
+  const int32_t *acc;         // per state: local-begin slot, -1 without a local begin
+  const int2 *ring;           // per state: {first ring row, ring depth - 1}
+  const int32_t *row_state;   // per ring row: its state
+  const int2 *deck;           // per BEGL_S deck: {extra rows before it, extra depth beyond w+1}
+  const int4 *item;           // per item of 64 units: {lowest level, highest level, lowest d, 0}
+  const int2 *unit;           // [item][64]: {kind << 28 | state, d}
+  const int32_t *wave_item;   // per wave: its items wave_item[w] .. wave_item[w+1]-1
+  int M, W, nbegl, local, nw;
+  int nrows;       // ring rows over all states
+  int deck_extra;  // extra deck rows over all decks
+  int nslot, amask;  // local-begin slots per j, ring depth - 1 of those rows
+  int lroot;         // the root's level
 };
+enum QdbUnitKind { QU_NONE = 0, QU_CELL = 1, QU_IL = 2, QU_ROOT = 3 };
+constexpr int QDB_MAX_WAVES = 16, QDB_MAX_THREADS = 64 * QDB_MAX_WAVES;
 struct QdbWinDev {
   const uint8_t *dsq;
   int32_t i0, j0, w, win;
-  void *rows;  // [2][M][w+1]
-  void *begl;  // [w+1][nbegl][w+1]
+  void *rows;  // standard scanners: state rings, decks, local-begin rows; truncated: plane rows
+  void *begl;  // truncated scanners: the BEGL_S decks
 };
 struct QdbCand {  // a root cell at or above the reporting cutoff
   int32_t win, j, d;

@@ -155,6 +155,180 @@ void qdb_report(const ixg_cm &cm, const uint8_t *dsq, int32_t i0, int32_t j0, st
     if (!removed[a]) out.push_back(tmp[a]);
 }
 
+// The wavefront schedule of one call (the per-call bands decide which cells
+// exist): per-state levels, ring depths and local-begin slots, the deck
+// gaps, and the (state, d) cells dealt as 64-lane items to the waves.
+struct QdbSched {
+  std::vector<int32_t> blob;  // lvl | acc | row_state | wave_item | ring | deck | item | unit
+  size_t o_lvl, o_acc, o_row, o_wave, o_ring, o_deck, o_item, o_unit;
+  int nw, nrows, deck_extra, nslot, amask, lroot, nitems;
+};
+
+static int pow2_at_least(int n) {
+  int r = 1;
+  while (r < n) r <<= 1;
+  return r;
+}
+
+void qdb_schedule(const CmHost &h, const std::vector<int32_t> &bands, int local, QdbSched &s) {
+  const int M = h.M, W = h.W;
+  const int32_t *dn = bands.data(), *dx = dn + M, *dminw = dn + 2 * M;
+  const int nbegl = (int)h.qibbase0.size();
+  std::vector<int32_t> lvl(M, 0);
+  for (size_t l = 0; l + 1 < h.lvl_off.size(); l++)
+    for (int k = h.lvl_off[l]; k < h.lvl_off[l + 1]; k++) lvl[h.lvl_states[k]] = (int32_t)l;
+  s.lroot = lvl[0];
+  // a cell of state y written at step j + lvl[y] is read at latest by a
+  // parent p at step j + delta + lvl[p] (delta = 1 for right emitters), so
+  // its ring must not come round before then
+  std::vector<int32_t> need(M, 1), gap(std::max(nbegl, 1), 0);
+  for (int p = 0; p < M; p++) {
+    const int st = h.sttype[p];
+    if (st == E_ST) continue;
+    if (st == B_ST) {
+      const int y = h.cnum[p], wl = h.cfirst[p];
+      need[y] = std::max(need[y], lvl[p] - lvl[y] + 1);
+      gap[h.qbegl[wl]] = std::max(gap[h.qbegl[wl]], lvl[p] - lvl[wl]);
+      continue;
+    }
+    const int dl = (st == MP_ST || st == MR_ST || st == IR_ST) ? 1 : 0;
+    for (int k = 0; k < h.cnum[p]; k++) {
+      const int y = h.cfirst[p] + k;
+      need[y] = std::max(need[y], lvl[p] - lvl[y] + dl + 1);
+    }
+  }
+  std::vector<int32_t> ring(2 * (size_t)M), row_state;
+  for (int v = 0; v < M; v++) {
+    const int r = pow2_at_least(need[v]);
+    ring[2 * v] = (int32_t)row_state.size();
+    ring[2 * v + 1] = r - 1;
+    row_state.insert(row_state.end(), r, v);
+  }
+  s.nrows = (int)row_state.size();
+  std::vector<int32_t> deck(2 * (size_t)std::max(nbegl, 1), 0);
+  s.deck_extra = 0;
+  for (int b = 0; b < nbegl; b++) {
+    deck[2 * b] = s.deck_extra;
+    deck[2 * b + 1] = gap[b];
+    s.deck_extra += gap[b];
+  }
+  // local begins: one slot per begin state of a level; a (j, slot) row is
+  // first written at step j + (lowest begin level) and cleared by the root at
+  // step j + lroot
+  std::vector<int32_t> acc(M, -1);
+  s.nslot = 0;
+  s.amask = 0;
+  if (local && !h.qblist.empty()) {
+    std::vector<int32_t> per(s.lroot + 1, 0);
+    int amin = s.lroot;
+    for (int y : h.qblist) {
+      acc[y] = per[lvl[y]]++;
+      s.nslot = std::max(s.nslot, per[lvl[y]]);
+      amin = std::min(amin, (int)lvl[y]);
+    }
+    s.amask = pow2_at_least(s.lroot - amin + 1) - 1;
+  }
+  // the cells: class 0 ordinary cells and the root, 1 bifurcations, 2 IL rows
+  struct U {
+    int cls, lvl, v, d, cost, code;
+  };
+  std::vector<U> us;
+  for (int v = 1; v < M; v++) {
+    const int st = h.sttype[v], cn = h.cnum[v];
+    if (st == E_ST) continue;
+    const int a = dn[v], b = std::min(W, (int)dx[v]);
+    if (b < a) continue;
+    if (st == IL_ST) {
+      us.push_back({2, lvl[v], v, a, (b - a + 1) * (cn + 4), QU_IL << 28 | v});
+    } else if (st == B_ST) {
+      const int wl = h.cfirst[v], y = cn;
+      for (int d = a; d <= b; d++) {
+        const int kmin = std::max(0, std::max((int)dminw[y], d - (int)dx[wl])), kmax = std::min((int)dx[y], d - (int)dminw[wl]);
+        us.push_back({1, lvl[v], v, d, std::max(0, kmax - kmin + 1) * 3 + 4, QU_CELL << 28 | v});
+      }
+    } else {
+      for (int d = a; d <= b; d++) us.push_back({0, lvl[v], v, d, cn + 4 + (acc[v] >= 0 ? 2 : 0), QU_CELL << 28 | v});
+    }
+  }
+  for (int d = dn[0], b = std::min(W, (int)dx[0]); d <= b; d++)
+    us.push_back({0, s.lroot, 0, d, h.cnum[0] + 4 + s.nslot, QU_ROOT << 28});
+  std::sort(us.begin(), us.end(), [](const U &x, const U &y) {
+    if (x.cls != y.cls) return x.cls < y.cls;
+    if (x.lvl != y.lvl) return x.lvl < y.lvl;
+    if (x.v != y.v) return x.v < y.v;
+    return x.d < y.d;
+  });
+  struct It {
+    int lmin, lmax, dmin, cost;
+    size_t u0, n;
+  };
+  std::vector<It> its;
+  for (size_t k = 0; k < us.size(); k++) {
+    if (its.empty() || its.back().n == 64 || us[its.back().u0].cls != us[k].cls)
+      its.push_back({us[k].lvl, us[k].lvl, us[k].d, 0, k, 0});
+    It &t = its.back();
+    t.lmin = std::min(t.lmin, us[k].lvl);
+    t.lmax = std::max(t.lmax, us[k].lvl);
+    t.dmin = std::min(t.dmin, us[k].d);
+    t.cost = std::max(t.cost, us[k].cost);
+    t.n++;
+  }
+  s.nitems = (int)its.size();
+  // waves: enough that the heaviest item is about one wave's share per step
+  int64_t total = 0, heavy = 1;
+  for (const It &t : its) {
+    total += t.cost;
+    heavy = std::max<int64_t>(heavy, t.cost);
+  }
+  s.nw = (int)std::min<int64_t>(QDB_MAX_WAVES, std::max<int64_t>(4, (total + heavy - 1) / heavy));
+  // deal items in level order to the least-loaded wave, which keeps every
+  // level (and so the steps where only some levels are live) spread out
+  std::vector<size_t> order(its.size());
+  for (size_t k = 0; k < order.size(); k++) order[k] = k;
+  std::stable_sort(order.begin(), order.end(), [&](size_t a, size_t b) { return its[a].lmin < its[b].lmin; });
+  std::vector<int64_t> load(s.nw, 0);
+  std::vector<std::vector<size_t>> per_wave(s.nw);
+  for (size_t k : order) {
+    const int wv = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[wv] += its[k].cost;
+    per_wave[wv].push_back(k);
+  }
+  std::vector<int32_t> wave_item(s.nw + 1, 0), item, unit;
+  for (int wv = 0; wv < s.nw; wv++) {
+    for (size_t k : per_wave[wv]) {
+      const It &t = its[k];
+      item.insert(item.end(), {t.lmin, t.lmax, t.dmin, 0});
+      for (size_t l = 0; l < 64; l++) {
+        if (l < t.n) {
+          unit.push_back(us[t.u0 + l].code);
+          unit.push_back(us[t.u0 + l].d);
+        } else {
+          unit.push_back(QU_NONE << 28);
+          unit.push_back(0);
+        }
+      }
+    }
+    wave_item[wv + 1] = wave_item[wv] + (int32_t)per_wave[wv].size();
+  }
+  // one blob, 16-byte aligned parts
+  s.blob.clear();
+  auto part = [&](const std::vector<int32_t> &x) {
+    while (s.blob.size() % 4) s.blob.push_back(0);
+    const size_t o = s.blob.size();
+    s.blob.insert(s.blob.end(), x.begin(), x.end());
+    return o;
+  };
+  s.o_lvl = part(lvl);
+  s.o_acc = part(acc);
+  s.o_row = part(row_state);
+  s.o_wave = part(wave_item);
+  s.o_ring = part(ring);
+  s.o_deck = part(deck);
+  s.o_item = part(item);
+  s.o_unit = part(unit);
+  s.blob.push_back(0);
+}
+
 struct QWin {
   int32_t i0, j0;
   int64_t seq_off;
@@ -186,24 +360,38 @@ int qdb_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
   HIPCHK(hipMemcpyAsync(d_bands, bands.data(), bands.size() * 4, hipMemcpyHostToDevice, c->stream));
   const QdbView qv = qview(cm);
   const CmDev cd = cm->dev();
+  QdbSched sc;
+  qdb_schedule(h, bands, h.local_begin, sc);
+  int32_t *d_sched;
+  RCHK(scratch(c, S_CM_SCHED, sc.blob.size() * 4, (void **)&d_sched));
+  HIPCHK(hipMemcpyAsync(d_sched, sc.blob.data(), sc.blob.size() * 4, hipMemcpyHostToDevice, c->stream));
   QdbDev q;
   q.sttype = cd.sttype;
   q.cfirst = cd.cfirst;
   q.cnum = cd.cnum;
   q.begl = qv.begl;
-  q.lvl_states = cd.lvl_states;
-  q.lvl_off = cd.lvl_off;
-  q.nlev = cd.nlev;
-  q.blist = qv.blist;
-  q.nb = (int)h.qblist.size();
   q.dn = d_bands;
   q.dxcap = d_bands + M;
   q.dminw = d_bands + 2 * M;
   q.ilog = qv.ilog;
+  q.lvl = d_sched + sc.o_lvl;
+  q.acc = d_sched + sc.o_acc;
+  q.ring = (const int2 *)(d_sched + sc.o_ring);
+  q.row_state = d_sched + sc.o_row;
+  q.deck = (const int2 *)(d_sched + sc.o_deck);
+  q.item = (const int4 *)(d_sched + sc.o_item);
+  q.unit = (const int2 *)(d_sched + sc.o_unit);
+  q.wave_item = d_sched + sc.o_wave;
   q.M = M;
   q.W = W;
   q.nbegl = nbegl;
   q.local = h.local_begin;
+  q.nw = sc.nw;
+  q.nrows = sc.nrows;
+  q.deck_extra = sc.deck_extra;
+  q.nslot = sc.nslot;
+  q.amask = sc.amask;
+  q.lroot = sc.lroot;
   QdbTabs<float> tf{cd.tsc, cd.oesc, cd.endsc, cd.beginsc, qv.fbase0, qv.fel, qv.fbbase0};
   QdbTabs<int32_t> ti{qv.itsc, qv.ioesc, qv.iendsc, qv.ibeginsc, qv.ibase0, qv.iel, qv.ibbase0};
   const void *tabs = inside ? (const void *)&ti : (const void *)&tf;
@@ -215,7 +403,8 @@ int qdb_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
     std::vector<int64_t> at;
     while (k1 < n) {
       const int L = wins[k1].j0 - wins[k1].i0 + 1, w = std::min(W, L), W1 = w + 1;
-      const int64_t need = 2 * (int64_t)M * W1 + (int64_t)W1 * std::max(nbegl, 1) * W1;
+      const int64_t need =
+          ((int64_t)sc.nrows + (int64_t)nbegl * W1 + sc.deck_extra + (int64_t)(sc.amask + 1) * sc.nslot) * W1;
       if (k1 > k0 && elems + need > budget) break;
       at.push_back(elems);
       elems += need;
@@ -229,14 +418,14 @@ int qdb_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
     std::vector<QdbWinDev> wd(k1 - k0);
     for (int64_t k = k0; k < k1; k++) {
       QdbWinDev &x = wd[k - k0];
-      const int L = wins[k].j0 - wins[k].i0 + 1, w = std::min(W, L), W1 = w + 1;
+      const int L = wins[k].j0 - wins[k].i0 + 1, w = std::min(W, L);
       x.dsq = d_base + wins[k].seq_off;
       x.i0 = wins[k].i0;
       x.j0 = wins[k].j0;
       x.w = w;
       x.win = (int32_t)k;
       x.rows = (int32_t *)d_mx + at[k - k0];
-      x.begl = (int32_t *)d_mx + at[k - k0] + 2 * (int64_t)M * W1;
+      x.begl = nullptr;
     }
     HIPCHK(hipMemcpyAsync(d_w, wd.data(), wd.size() * sizeof(QdbWinDev), hipMemcpyHostToDevice, c->stream));
     int64_t cap = res / 8 + 4096;

@@ -1,22 +1,30 @@
 // gfx950 query-dependent-banded (or non-banded) scanners: CYK on float
 // scores and Inside on integer scores with the ILogsum table
-// (cm_nohmm.rs:406-767 generic_scan). One 256-lane workgroup per window
-// walks the end positions j in order; for each j it fills the rows of the
-// model's dependency levels one level at a time (a level's states read only
-// lower levels at the same j, or the previous j), each lane computing whole
-// cells of one state. Rows roll over two j's ([2][M][w+1]); BEGL_S decks keep
-// w+1 rows for the bifurcations' splits, as the reference's matrices do.
-// Children are combined in the reference's unrolled order (reduce_c_order),
-// so both semirings are bit-identical to it. Root cells at or above the
-// reporting cutoff are appended to a candidate list; greedy reporting, null3
-// and overlap removal run on the host.
+// (cm_nohmm.rs:406-767 generic_scan). One workgroup per window runs a
+// wavefront over the model's dependency levels: at step t each state v fills
+// its cells of end position j = i0 + t - lvl[v]. A state reads lower-level
+// states at the same j, or itself and lower levels at j - 1, so everything a
+// step reads was written at an earlier step and one barrier per step is the
+// only synchronisation; per step every level is busy at once instead of one
+// level per barrier. The host deals the step's (state, d) cells as 64-lane
+// items to the waves. Rows live in per-state rings deep enough that a cell is
+// read before its slot comes round again; cells outside a state's band are
+// never written, so they keep their initial values exactly as in the
+// reference's rolling rows. BEGL_S decks keep w+1 rows plus the steps between
+// the deck's writer and its bifurcation. Local begins are combined by max
+// into per-(j, slot, d) rows, one slot per begin state of a level, which the
+// root reads and clears; max is order-free, so the result equals the
+// reference's scan over the begin list. Children are combined in the
+// reference's unrolled order (reduce_c_order), so both semirings are
+// bit-identical to it. Root cells at or above the reporting cutoff are
+// appended to a candidate list; greedy reporting, null3 and overlap removal
+// run on the host.
 #include "cm_internal.h"
 
 namespace ixg {
 
 namespace {
 
-constexpr int QDB_THREADS = 256;
 
 __device__ __forceinline__ int q_sd(int st) { return st == MP_ST ? 2 : ((st >= ML_ST && st <= IR_ST) ? 1 : 0); }
 __device__ __forceinline__ int q_sdr(int st) { return (st == MP_ST || st == MR_ST || st == IR_ST) ? 1 : 0; }
@@ -89,141 +97,168 @@ __device__ __forceinline__ T reduce_c(int cn, const T *t, T init, const int32_t 
 }
 
 template <typename T>
-struct Win {
-  T *rows, *bg;
-  int M, W1, nbegl;
-  __device__ __forceinline__ T &r(int par, int v, int d) const { return rows[((size_t)par * M + v) * W1 + d]; }
-  __device__ __forceinline__ T &b(int jr, int bi, int d) const { return bg[((size_t)jr * nbegl + bi) * W1 + d]; }
-};
-
-template <typename T>
 __device__ __forceinline__ T init_sc(const QdbTabs<T> &tb, int v, int dp) {
   const T es = tb.endsc[v];
   return Sr<T>::possible(es) ? Sr<T>::add(tb.el_sca[dp], es) : Sr<T>::zero();
 }
 
-// One cell (j, d) of a state v other than IL and the root.
+// the row of end position j in state v's ring
 template <typename T>
-__device__ void qdb_cell(const QdbDev &q, const QdbTabs<T> &tb, const Win<T> &w, const uint8_t *dsq, int v, int st,
-                         int j, int d, int cur, int prv) {
+__device__ __forceinline__ T *ring_row(T *rows, int2 ri, int j, int W1) {
+  return rows + (size_t)(ri.x + (j & ri.y)) * W1;
+}
+
+struct QdbGeo {  // one window's arena
+  int i0, j0, w, W1;
+};
+
+// One cell (j, d) of a state v other than IL and the root; returns its value
+// (for the local-begin rows).
+template <typename T>
+__device__ __forceinline__ T qdb_cell(const QdbDev &q, const QdbTabs<T> &tb, T *rows, T *deck, const uint8_t *dsq,
+                                      const QdbGeo &g, int v, int st, int y0, int cn, int j, int d) {
   using R = Sr<T>;
   const int32_t *tbl = q.ilog;
+  const int W1 = g.W1;
   const int sd = q_sd(st);
-  if (st == B_ST) {  // cm_nohmm.rs:589-606
-    const int wl = q.cfirst[v], y = q.cnum[v], bi = q.begl[wl];
+  if (st == B_ST) {  // cm_nohmm.rs:589-606; cn holds the right child
+    const int wl = y0, y = cn, bi = q.begl[wl];
     const int dn_y = q.dminw[y], dx_y = q.dxcap[y], dn_w = q.dminw[wl], dx_w = q.dxcap[wl];
     const int kmin = max(0, max(dn_y, d - dx_w)), kmax = min(dx_y, d - dn_w);
-    T sc = init_sc(tb, v, d - sd);
+    const int2 dk = q.deck[bi];
+    const int DR = W1 + dk.y;
+    const T *db = deck + ((size_t)bi * W1 + dk.x) * W1;
+    const T *yr = ring_row(rows, q.ring[y], j, W1);
+    T sc = init_sc(tb, v, d);
+    int jr = (j - kmin) % DR;  // deck row of j - k (j - kmin >= i0 - 1 >= 0)
     for (int k = kmin; k <= kmax; k++) {
-      const int jr = ((j - k) % w.W1 + w.W1) % w.W1;
-      sc = R::comb(sc, R::add(w.b(jr, bi, d - k), w.r(cur, y, k)), tbl);
+      sc = R::comb(sc, R::add(db[(size_t)jr * W1 + d - k], yr[k]), tbl);
+      jr = jr == 0 ? DR - 1 : jr - 1;
     }
-    w.r(cur, v, d) = sc;
-    return;
+    ring_row(rows, q.ring[v], j, W1)[d] = sc;
+    return sc;
   }
-  const int y0 = q.cfirst[v], cn = q.cnum[v];
-  const int jp_y = q_sdr(st) ? prv : cur;
+  const int jy = q_sdr(st) ? j - 1 : j;
   const T *ts = tb.tsc + v * MAXC;
-  if (q.begl[v] >= 0) {  // BEGL_S: natural child order, into its deck (cm_nohmm.rs:607-616)
+  const int bv = q.begl[v];
+  if (bv >= 0) {  // BEGL_S: natural child order, into its deck (cm_nohmm.rs:607-616)
     T sc = init_sc(tb, v, d - sd);
-    for (int yo = 0; yo < cn; yo++) sc = R::comb(sc, R::add(w.r(jp_y, y0 + yo, d - sd), ts[yo]), tbl);
-    w.b(j % w.W1, q.begl[v], d) = sc;
-    return;
+    for (int yo = 0; yo < cn; yo++)
+      sc = R::comb(sc, R::add(ring_row(rows, q.ring[y0 + yo], jy, W1)[d - sd], ts[yo]), tbl);
+    const int2 dk = q.deck[bv];
+    deck[((size_t)bv * W1 + dk.x + j % (W1 + dk.y)) * W1 + d] = sc;
+    return sc;
   }
   T terms[MAXC];
 #pragma unroll
   for (int yo = 0; yo < MAXC; yo++)
-    if (yo < cn) terms[yo] = R::add(w.r(jp_y, y0 + yo, d - sd), ts[yo]);
+    if (yo < cn) terms[yo] = R::add(ring_row(rows, q.ring[y0 + yo], jy, W1)[d - sd], ts[yo]);
   T sc = reduce_c<T>(cn, terms, init_sc(tb, v, d - sd), tbl);
   const T *oe = tb.oesc + (size_t)v * (KP * KP);
   const int i = j - d + 1;
   if (st == ML_ST) sc = R::add(sc, oe[dsq[i]]);
   else if (st == MR_ST || st == IR_ST) sc = R::add(sc, oe[dsq[j]]);
   else if (st == MP_ST) sc = R::add(sc, oe[dsq[i] * KP + dsq[j]]);
-  w.r(cur, v, d) = sc;
+  ring_row(rows, q.ring[v], j, W1)[d] = sc;
+  return sc;
 }
 
 // The d row dn..dx of an IL state at j: the self transition reads (j, d-1),
 // carried in a register along the row (cm_nohmm.rs:617-647).
 template <typename T>
-__device__ void qdb_il_row(const QdbDev &q, const QdbTabs<T> &tb, const Win<T> &w, const uint8_t *dsq, int v, int j,
-                           int dn, int dx, int cur) {
+__device__ void qdb_il_row(const QdbDev &q, const QdbTabs<T> &tb, T *rows, const uint8_t *dsq, int W1, int v,
+                           int y0, int cn, int j, int dn, int dx) {
   using R = Sr<T>;
   const int32_t *tbl = q.ilog;
-  const int y0 = q.cfirst[v], cn = q.cnum[v];
   const T *ts = tb.tsc + v * MAXC;
   const T *oe = tb.oesc + (size_t)v * (KP * KP);
-  T prev = w.r(cur, v, dn - 1);
+  T *out = ring_row(rows, q.ring[v], j, W1);
+  const T *cr[MAXC];
+#pragma unroll
+  for (int yo = 0; yo < MAXC; yo++)
+    if (yo < cn) cr[yo] = ring_row(rows, q.ring[y0 + yo], j, W1);
+  T prev = out[dn - 1];
   for (int d = dn; d <= dx; d++) {
     T terms[MAXC];
 #pragma unroll
     for (int yo = 0; yo < MAXC; yo++)
-      if (yo < cn) terms[yo] = R::add(y0 + yo == v ? prev : w.r(cur, y0 + yo, d - 1), ts[yo]);
+      if (yo < cn) terms[yo] = R::add(y0 + yo == v ? prev : cr[yo][d - 1], ts[yo]);
     T sc = reduce_c<T>(cn, terms, init_sc(tb, v, d - 1), tbl);
     sc = R::add(sc, oe[dsq[j - d + 1]]);
-    w.r(cur, v, d) = sc;
+    out[d] = sc;
     prev = sc;
   }
 }
 
 template <typename T>
-__global__ __launch_bounds__(QDB_THREADS) void qdb_kernel(QdbDev q, QdbTabs<T> tb, const QdbWinDev *wins,
-                                                          float cutoff, QdbCand *cand, int64_t cap,
-                                                          unsigned long long *counter) {
+__global__ __launch_bounds__(QDB_MAX_THREADS) void qdb_kernel(QdbDev q, QdbTabs<T> tb, const QdbWinDev *wins,
+                                                              float cutoff, QdbCand *cand, int64_t cap,
+                                                              unsigned long long *counter) {
   using R = Sr<T>;
   const QdbWinDev wd = wins[blockIdx.x];
-  const int tid = threadIdx.x, nt = blockDim.x, M = q.M;
-  Win<T> w;
-  w.rows = (T *)wd.rows;
-  w.bg = (T *)wd.begl;
-  w.M = M;
-  w.W1 = wd.w + 1;
-  w.nbegl = q.nbegl;
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6;
+  QdbGeo g;
+  g.i0 = wd.i0;
+  g.j0 = wd.j0;
+  g.w = wd.w;
+  g.W1 = wd.w + 1;
+  const int W1 = g.W1;
+  T *rows = (T *)wd.rows;
+  T *deck = rows + (size_t)q.nrows * W1;
+  T *acc = deck + ((size_t)q.nbegl * W1 + q.deck_extra) * W1;
   const uint8_t *dsq = wd.dsq;
   // every cell starts at the semiring's zero, d = 0 at its base case
   // (cm_nohmm.rs:489-532)
-  for (int64_t x = tid; x < 2 * (int64_t)M * w.W1; x += nt) {
-    const int d = (int)(x % w.W1), v = (int)((x / w.W1) % M);
-    w.rows[x] = (d == 0 && q.begl[v] < 0) ? tb.base0[v] : R::zero();
+  for (int64_t x = tid; x < (int64_t)q.nrows * W1; x += nt) {
+    const int d = (int)(x % W1), v = q.row_state[x / W1];
+    rows[x] = (d == 0 && q.begl[v] < 0) ? tb.base0[v] : R::zero();
   }
-  for (int64_t x = tid; x < (int64_t)w.W1 * q.nbegl * w.W1; x += nt) {
-    const int d = (int)(x % w.W1), bi = (int)((x / w.W1) % q.nbegl);
-    w.bg[x] = d == 0 ? tb.bbase0[bi] : R::zero();
+  for (int b = 0; b < q.nbegl; b++) {
+    const int2 dk = q.deck[b];
+    T *db = deck + ((size_t)b * W1 + dk.x) * W1;
+    for (int64_t x = tid; x < (int64_t)(W1 + dk.y) * W1; x += nt) db[x] = (x % W1) == 0 ? tb.bbase0[b] : R::zero();
   }
+  if (q.local)
+    for (int64_t x = tid; x < (int64_t)(q.amask + 1) * q.nslot * W1; x += nt) acc[x] = R::zero();
   __syncthreads();
+  const int L = g.j0 - g.i0 + 1, nsteps = L + q.lroot;
+  const int ib = wave < q.nw ? q.wave_item[wave] : 0, ie = wave < q.nw ? q.wave_item[wave + 1] : 0;
   const int r0 = q.cfirst[0], cn0 = q.cnum[0];
-  for (int j = wd.i0; j <= wd.j0; j++) {
-    const int jrow = min(j - wd.i0 + 1, wd.w), cur = j & 1, prv = (j - 1) & 1;
-    for (int l = 0; l < q.nlev; l++) {
-      int off = 0;  // units dealt so far in this level, modulo nt
-      for (int s = q.lvl_off[l]; s < q.lvl_off[l + 1]; s++) {
-        const int v = q.lvl_states[s];
-        const int st = q.sttype[v];
-        if (v == 0 || st == E_ST) continue;
-        const int dn = q.dn[v], dx = min(jrow, q.dxcap[v]);
-        if (dx < dn) continue;
-        const int n = st == IL_ST ? 1 : dx - dn + 1;
-        int r = (tid - off) % nt;
-        if (r < 0) r += nt;
-        for (int u = r; u < n; u += nt) {
-          if (st == IL_ST) qdb_il_row<T>(q, tb, w, dsq, v, j, dn, dx, cur);
-          else qdb_cell<T>(q, tb, w, dsq, v, st, j, dn + u, cur, prv);
-        }
-        off = (off + n) % nt;
+  for (int t = 0; t < nsteps; t++) {
+    for (int it = ib; it < ie; it++) {
+      const int4 im = q.item[it];  // wave-uniform: skip items with no live cell at this step
+      if (im.y < t - (L - 1) || im.x > t || im.z > min(t - im.x + 1, g.w)) continue;
+      const int2 u = q.unit[(size_t)it * 64 + lane];
+      const int kind = u.x >> 28, v = u.x & 0x0FFFFFFF, d = u.y;
+      if (kind == QU_NONE) continue;
+      const int j = g.i0 + t - q.lvl[v];
+      if (j < g.i0 || j > g.j0) continue;
+      const int jrow = min(j - g.i0 + 1, g.w);
+      if (kind == QU_IL) {
+        const int dx = min(jrow, q.dxcap[v]);
+        if (dx >= d) qdb_il_row<T>(q, tb, rows, dsq, W1, v, q.cfirst[v], q.cnum[v], j, d, dx);
+        continue;
       }
-      __syncthreads();
-    }
-    // the root: its own transitions, then local begins by MAX (cm_nohmm.rs:686-746)
-    const int dn0 = q.dn[0], dx0 = min(jrow, q.dxcap[0]);
-    for (int d = dn0 + tid; d <= dx0; d += nt) {
-      T a = R::comb(R::zero(), R::add(w.r(cur, r0, d), tb.tsc[0]), q.ilog);
-      for (int yo = 1; yo < cn0; yo++) a = R::comb(a, R::add(w.r(cur, r0 + yo, d), tb.tsc[yo]), q.ilog);
-      if (q.local)
-        for (int b = 0; b < q.nb; b++) {
-          const int y = q.blist[b];
-          if (d < q.dn[y] || d > min(jrow, q.dxcap[y])) continue;
-          const T cnd = R::add(q.begl[y] >= 0 ? w.b(j % w.W1, q.begl[y], d) : w.r(cur, y, d), tb.beginsc[y]);
+      if (d > jrow) continue;
+      if (kind == QU_CELL) {
+        const T sc = qdb_cell<T>(q, tb, rows, deck, dsq, g, v, q.sttype[v], q.cfirst[v], q.cnum[v], j, d);
+        const int slot = q.acc[v];
+        if (q.local && slot >= 0) {  // this begin's candidate for the root at (j, d)
+          T &a = acc[((size_t)(j & q.amask) * q.nslot + slot) * W1 + d];
+          const T cnd = R::add(sc, tb.beginsc[v]);
           if (a < cnd) a = cnd;
+        }
+        continue;
+      }
+      // the root: its own transitions, then local begins by MAX (cm_nohmm.rs:686-746)
+      T a = R::comb(R::zero(), R::add(ring_row(rows, q.ring[r0], j, W1)[d], tb.tsc[0]), q.ilog);
+      for (int yo = 1; yo < cn0; yo++)
+        a = R::comb(a, R::add(ring_row(rows, q.ring[r0 + yo], j, W1)[d], tb.tsc[yo]), q.ilog);
+      if (q.local)
+        for (int s = 0; s < q.nslot; s++) {
+          T &b = acc[((size_t)(j & q.amask) * q.nslot + s) * W1 + d];
+          if (a < b) a = b;
+          b = R::zero();
         }
       const float sc = R::bits(a);
       if (sc >= cutoff && sc > IMPOSSIBLE + 1.0f) {
@@ -238,6 +273,7 @@ __global__ __launch_bounds__(QDB_THREADS) void qdb_kernel(QdbDev q, QdbTabs<T> t
         }
       }
     }
+    __syncthreads();
   }
 }
 
@@ -247,10 +283,10 @@ hipError_t launch_qdb_scan(const QdbDev &q, const void *tabs, int inside, const 
                            float cutoff, QdbCand *cand, int64_t cap, unsigned long long *counter, hipStream_t s) {
   if (n == 0) return hipSuccess;
   if (inside)
-    hipLaunchKernelGGL(qdb_kernel<int32_t>, dim3((unsigned)n), dim3(QDB_THREADS), 0, s, q,
+    hipLaunchKernelGGL(qdb_kernel<int32_t>, dim3((unsigned)n), dim3(64 * q.nw), 0, s, q,
                        *(const QdbTabs<int32_t> *)tabs, wins, cutoff, cand, cap, counter);
   else
-    hipLaunchKernelGGL(qdb_kernel<float>, dim3((unsigned)n), dim3(QDB_THREADS), 0, s, q,
+    hipLaunchKernelGGL(qdb_kernel<float>, dim3((unsigned)n), dim3(64 * q.nw), 0, s, q,
                        *(const QdbTabs<float> *)tabs, wins, cutoff, cand, cap, counter);
   return hipGetLastError();
 }
