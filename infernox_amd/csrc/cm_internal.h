@@ -106,6 +106,25 @@ template <typename T>
 struct QdbTabs {
   const T *tsc, *oesc, *endsc, *beginsc, *base0, *el_sca, *bbase0;
 };
+enum QdbUnitKind { QU_NONE = 0, QU_CELL = 1, QU_IL = 2, QU_ROOT = 3 };
+// One lane's work at a step: a cell (state, d), an IL row, or a root cell,
+// with everything the lane needs to address it (no dependent metadata loads).
+// A ring is packed as first row | log2(depth) << 27.
+struct QdbUnit {
+  int32_t code;      // kind << 28 | state
+  int32_t d;         // the cell's d (IL: the row's first d)
+  int32_t lvl;       // the state's level: it works on j = i0 + t - lvl at step t
+  int32_t info;      // sttype | child count << 4 | (local-begin slot + 1) << 8
+  int32_t y0;        // first child (bifurcation: the left child, a BEGL_S)
+  int32_t y1;        // bifurcation: the right child; IL: the row's last d
+  int32_t ring;      // the state's own ring
+  int32_t deck;      // BEGL_S and bifurcation: the deck index; -1 otherwise
+  int32_t cring[6];  // the children's rings; bifurcation: right child's ring, deck offset, deck gap
+  int32_t k0, k1;    // bifurcation: split range; BEGL_S: deck offset, deck gap
+};
+static_assert(sizeof(QdbUnit) == 64, "QdbUnit is four 16-byte loads");
+constexpr int RING_SHIFT = 27;
+
 // The wavefront schedule of one scan call: at step t every state v works on
 // the end position j = i0 + t - lvl[v] (lvl: one more than its highest
 // child). All of a step's cells read only cells written at earlier steps, so
@@ -126,7 +145,7 @@ struct QdbDev {
   const int32_t *row_state;   // per ring row: its state
   const int2 *deck;           // per BEGL_S deck: {extra rows before it, extra depth beyond w+1}
   const int4 *item;           // per item of 64 units: {lowest level, highest level, lowest d, 0}
-  const int2 *unit;           // [item][64]: {kind << 28 | state, d}
+  const QdbUnit *unit;        // [item][64]
   const int32_t *wave_item;   // per wave: its items wave_item[w] .. wave_item[w+1]-1
   int M, W, nbegl, local, nw;
   int nrows;       // ring rows over all states
@@ -134,7 +153,6 @@ struct QdbDev {
   int nslot, amask;  // local-begin slots per j, ring depth - 1 of those rows
   int lroot;         // the root's level
 };
-enum QdbUnitKind { QU_NONE = 0, QU_CELL = 1, QU_IL = 2, QU_ROOT = 3 };
 constexpr int QDB_MAX_WAVES = 16, QDB_MAX_THREADS = 64 * QDB_MAX_WAVES;
 struct QdbWinDev {
   const uint8_t *dsq;
@@ -156,16 +174,25 @@ struct TrqTabs {
   const T *lmesc, *rmesc;  // marginal emission scores [M][18]
   const T *pty;            // the pass's truncated-begin penalties [M]
 };
+// The truncated scanners' wavefront schedule: as QdbDev, with four planes
+// (J, L, R, T) per ring row and three (J, L, R) per deck row; the root keeps
+// its two parity rows of each plane for the whole window.
 struct TrqDev {
   const int8_t *sttype;
   const int32_t *cfirst, *cnum;
-  const int32_t *begl;               // per state: BEGL_S deck index, -1 otherwise
-  const int32_t *lvl_states, *lvl_off;
-  const int32_t *dxcap;              // per state: min(dmax, W)
-  const int32_t *ilog;               // ILogsum table
-  int nlev, M, W, nbegl;
+  const int32_t *begl;   // per state: BEGL_S deck index, -1 otherwise
+  const int32_t *dxcap;  // per state: min(dmax, W)
+  const int32_t *ilog;   // ILogsum table
+  const int32_t *lvl, *row_state, *wave_item;
+  const int2 *ring, *deck;
+  const QdbUnit *unit;
+  const int4 *item;
+  const int4 *tblist;  // the root's begin states: {ring, penalty bits, dxcap, state | bifurcation << 30}
+  int ntb;
+  int M, W, nbegl;
   int fill;       // bit 1 L, bit 2 R, bit 3 T planes filled by the pass
   int nonbanded;  // no dmax given: bifurcation splits over every k
+  int nw, nrows, deck_extra, lroot;
 };
 struct TrqCand {  // a root cell whose best truncated begin is at or above the cutoff
   int32_t win, j, d;

@@ -160,8 +160,8 @@ void qdb_report(const ixg_cm &cm, const uint8_t *dsq, int32_t i0, int32_t j0, st
 // gaps, and the (state, d) cells dealt as 64-lane items to the waves.
 struct QdbSched {
   std::vector<int32_t> blob;  // lvl | acc | row_state | wave_item | ring | deck | item | unit
-  size_t o_lvl, o_acc, o_row, o_wave, o_ring, o_deck, o_item, o_unit;
-  int nw, nrows, deck_extra, nslot, amask, lroot, nitems;
+  size_t o_lvl, o_acc, o_row, o_wave, o_ring, o_deck, o_item, o_unit, o_tb;
+  int nw, nrows, deck_extra, nslot, amask, lroot, nitems, ntb;
 };
 
 static int pow2_at_least(int n) {
@@ -170,7 +170,12 @@ static int pow2_at_least(int n) {
   return r;
 }
 
-void qdb_schedule(const CmHost &h, const std::vector<int32_t> &bands, int local, QdbSched &s) {
+// tbegin (truncated scanners): per state, whether the pass gives it a
+// truncated begin (tpen: the penalties' bits); the root then reads those
+// states at its own j, keeps two parity rows of its own, and has no
+// transitions of its own.
+void qdb_schedule(const CmHost &h, const std::vector<int32_t> &bands, int local, const std::vector<char> *tbegin,
+                  const int32_t *tpen, int nonbanded, QdbSched &s) {
   const int M = h.M, W = h.W;
   const int32_t *dn = bands.data(), *dx = dn + M, *dminw = dn + 2 * M;
   const int nbegl = (int)h.qibbase0.size();
@@ -182,7 +187,12 @@ void qdb_schedule(const CmHost &h, const std::vector<int32_t> &bands, int local,
   // parent p at step j + delta + lvl[p] (delta = 1 for right emitters), so
   // its ring must not come round before then
   std::vector<int32_t> need(M, 1), gap(std::max(nbegl, 1), 0);
-  for (int p = 0; p < M; p++) {
+  if (tbegin) {
+    need[0] = 2;
+    for (int y = 1; y < M; y++)
+      if ((*tbegin)[y]) need[y] = std::max(need[y], s.lroot - lvl[y] + 1);
+  }
+  for (int p = tbegin ? 1 : 0; p < M; p++) {
     const int st = h.sttype[p];
     if (st == E_ST) continue;
     if (st == B_ST) {
@@ -218,7 +228,7 @@ void qdb_schedule(const CmHost &h, const std::vector<int32_t> &bands, int local,
   std::vector<int32_t> acc(M, -1);
   s.nslot = 0;
   s.amask = 0;
-  if (local && !h.qblist.empty()) {
+  if (!tbegin && local && !h.qblist.empty()) {
     std::vector<int32_t> per(s.lroot + 1, 0);
     int amin = s.lroot;
     for (int y : h.qblist) {
@@ -233,25 +243,41 @@ void qdb_schedule(const CmHost &h, const std::vector<int32_t> &bands, int local,
     int cls, lvl, v, d, cost, code;
   };
   std::vector<U> us;
+  const int pw = tbegin ? 3 : 1;  // cost weight: planes per cell
+  int ntb = 0;
+  if (tbegin)
+    for (int y = 1; y < M; y++) ntb += (*tbegin)[y] ? 1 : 0;
   for (int v = 1; v < M; v++) {
     const int st = h.sttype[v], cn = h.cnum[v];
     if (st == E_ST) continue;
+    // truncated scanners: an emitter followed by an END node's E state is
+    // detached (cm_dpsearch_trunc.rs:88)
+    if (tbegin && (st == ML_ST || st == IL_ST || st == MR_ST || st == IR_ST) && v + 1 < M && h.sttype[v + 1] == E_ST)
+      continue;
     const int a = dn[v], b = std::min(W, (int)dx[v]);
     if (b < a) continue;
     if (st == IL_ST) {
-      us.push_back({2, lvl[v], v, a, (b - a + 1) * (cn + 4), QU_IL << 28 | v});
+      us.push_back({2, lvl[v], v, a, (b - a + 1) * (cn + 4) * pw, QU_IL << 28 | v});
     } else if (st == B_ST) {
       const int wl = h.cfirst[v], y = cn;
       for (int d = a; d <= b; d++) {
-        const int kmin = std::max(0, std::max((int)dminw[y], d - (int)dx[wl])), kmax = std::min((int)dx[y], d - (int)dminw[wl]);
-        us.push_back({1, lvl[v], v, d, std::max(0, kmax - kmin + 1) * 3 + 4, QU_CELL << 28 | v});
+        int kmin, kmax;
+        if (!tbegin) {
+          kmin = std::max(0, std::max((int)dminw[y], d - (int)dx[wl]));
+          kmax = std::min((int)dx[y], d - (int)dminw[wl]);
+        } else {
+          kmin = nonbanded ? 0 : std::max(0, d - (int)dx[wl]);
+          kmax = nonbanded ? d : std::min((int)dx[y], d);
+        }
+        us.push_back({1, lvl[v], v, d, (std::max(0, kmax - kmin + 1) * 3 + 4) * pw, QU_CELL << 28 | v});
       }
     } else {
-      for (int d = a; d <= b; d++) us.push_back({0, lvl[v], v, d, cn + 4 + (acc[v] >= 0 ? 2 : 0), QU_CELL << 28 | v});
+      for (int d = a; d <= b; d++)
+        us.push_back({0, lvl[v], v, d, (cn + 4 + (acc[v] >= 0 ? 2 : 0)) * pw, QU_CELL << 28 | v});
     }
   }
   for (int d = dn[0], b = std::min(W, (int)dx[0]); d <= b; d++)
-    us.push_back({0, s.lroot, 0, d, h.cnum[0] + 4 + s.nslot, QU_ROOT << 28});
+    us.push_back({0, s.lroot, 0, d, tbegin ? ntb * 4 * 3 + M : h.cnum[0] + 4 + s.nslot, QU_ROOT << 28});
   std::sort(us.begin(), us.end(), [](const U &x, const U &y) {
     if (x.cls != y.cls) return x.cls < y.cls;
     if (x.lvl != y.lvl) return x.lvl < y.lvl;
@@ -280,7 +306,7 @@ void qdb_schedule(const CmHost &h, const std::vector<int32_t> &bands, int local,
     total += t.cost;
     heavy = std::max<int64_t>(heavy, t.cost);
   }
-  s.nw = (int)std::min<int64_t>(QDB_MAX_WAVES, std::max<int64_t>(4, (total + heavy - 1) / heavy));
+  s.nw = (int)std::min<int64_t>(QDB_MAX_WAVES, std::max<int64_t>({4, (total + heavy - 1) / heavy, s.nitems / 2}));
   // deal items in level order to the least-loaded wave, which keeps every
   // level (and so the steps where only some levels are live) spread out
   std::vector<size_t> order(its.size());
@@ -293,20 +319,59 @@ void qdb_schedule(const CmHost &h, const std::vector<int32_t> &bands, int local,
     load[wv] += its[k].cost;
     per_wave[wv].push_back(k);
   }
+  auto pack = [&](int v) {
+    int lg = 0;
+    while ((1 << lg) <= ring[2 * v + 1]) lg++;
+    return ring[2 * v] | lg << RING_SHIFT;
+  };
+  // one lane's record (QdbUnit, sixteen words)
+  auto record = [&](const U &x, std::vector<int32_t> &out) {
+    QdbUnit r;
+    memset(&r, 0, sizeof r);
+    r.deck = -1;
+    r.code = x.code;
+    r.d = x.d;
+    const int v = x.v, kind = x.code >> 28;
+    if (kind != QU_NONE) {
+      const int st = h.sttype[v], cn = st == B_ST ? 0 : h.cnum[v];
+      r.lvl = lvl[v];
+      r.info = st | (cn & 15) << 4 | (acc[v] + 1) << 8;
+      r.y0 = h.cfirst[v];
+      r.ring = pack(v);
+      if (st == B_ST) {
+        const int wl = h.cfirst[v], y = h.cnum[v], bi = h.qbegl[wl];
+        r.y1 = y;
+        r.deck = bi;
+        r.cring[0] = pack(y);
+        r.cring[1] = deck[2 * bi];
+        r.cring[2] = deck[2 * bi + 1];
+        if (!tbegin) {
+          r.k0 = std::max(0, std::max((int)dminw[y], x.d - (int)dx[wl]));
+          r.k1 = std::min((int)dx[y], x.d - (int)dminw[wl]);
+        } else {
+          r.k0 = nonbanded ? 0 : std::max(0, x.d - (int)dx[wl]);
+          r.k1 = nonbanded ? x.d : std::min((int)dx[y], x.d);
+        }
+      } else {
+        if (st != E_ST && !(tbegin && v == 0))
+          for (int k = 0; k < cn && k < MAXC; k++) r.cring[k] = pack(h.cfirst[v] + k);
+        if (h.qbegl[v] >= 0) {
+          r.deck = h.qbegl[v];
+          r.k0 = deck[2 * r.deck];
+          r.k1 = deck[2 * r.deck + 1];
+        }
+        if (st == IL_ST) r.y1 = std::min(W, (int)dx[v]);
+      }
+    }
+    const int32_t *w = (const int32_t *)&r;
+    out.insert(out.end(), w, w + 16);
+  };
   std::vector<int32_t> wave_item(s.nw + 1, 0), item, unit;
   for (int wv = 0; wv < s.nw; wv++) {
     for (size_t k : per_wave[wv]) {
       const It &t = its[k];
       item.insert(item.end(), {t.lmin, t.lmax, t.dmin, 0});
-      for (size_t l = 0; l < 64; l++) {
-        if (l < t.n) {
-          unit.push_back(us[t.u0 + l].code);
-          unit.push_back(us[t.u0 + l].d);
-        } else {
-          unit.push_back(QU_NONE << 28);
-          unit.push_back(0);
-        }
-      }
+      for (size_t l = 0; l < 64; l++) record(l < t.n ? us[t.u0 + l] : U{0, 0, 0, 0, 0, QU_NONE << 28}, unit);
     }
     wave_item[wv + 1] = wave_item[wv] + (int32_t)per_wave[wv].size();
   }
@@ -326,7 +391,25 @@ void qdb_schedule(const CmHost &h, const std::vector<int32_t> &bands, int local,
   s.o_deck = part(deck);
   s.o_item = part(item);
   s.o_unit = part(unit);
+  // the truncated root's begin states in increasing order:
+  // {ring, penalty bits, dxcap, state | bifurcation << 30}
+  std::vector<int32_t> tbl;
+  if (tbegin)
+    for (int y = 1; y < M; y++)
+      if ((*tbegin)[y]) tbl.insert(tbl.end(), {pack(y), tpen[y], dx[y], y | (h.sttype[y] == B_ST ? 1 << 30 : 0)});
+  s.ntb = (int)(tbl.size() / 4);
+  s.o_tb = part(tbl);
   s.blob.push_back(0);
+}
+
+// The kernels hold the ILogsum table in LDS as 16-bit entries.
+int check_ilog16(const CmHost &h) {
+  for (int32_t x : h.ilog)
+    if (x < 0 || x > 65535) {
+      g_err = "ILogsum table entry outside 0..65535";
+      return IXG_EINVAL;
+    }
+  return IXG_OK;
 }
 
 struct QWin {
@@ -342,6 +425,7 @@ int qdb_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
             std::vector<ixg_hit> &out) {
   const CmHost &h = cm->h;
   const int M = h.M, W = h.W, nbegl = (int)std::count_if(h.qbegl.begin(), h.qbegl.end(), [](int b) { return b >= 0; });
+  if (inside) RCHK(check_ilog16(h));
   // per-state bands of this call (cm_nohmm.rs:444-455)
   std::vector<int32_t> bands(3 * (size_t)M);
   for (int v = 0; v < M; v++) {
@@ -361,7 +445,7 @@ int qdb_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
   const QdbView qv = qview(cm);
   const CmDev cd = cm->dev();
   QdbSched sc;
-  qdb_schedule(h, bands, h.local_begin, sc);
+  qdb_schedule(h, bands, h.local_begin, nullptr, nullptr, 0, sc);
   int32_t *d_sched;
   RCHK(scratch(c, S_CM_SCHED, sc.blob.size() * 4, (void **)&d_sched));
   HIPCHK(hipMemcpyAsync(d_sched, sc.blob.data(), sc.blob.size() * 4, hipMemcpyHostToDevice, c->stream));
@@ -380,7 +464,7 @@ int qdb_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
   q.row_state = d_sched + sc.o_row;
   q.deck = (const int2 *)(d_sched + sc.o_deck);
   q.item = (const int4 *)(d_sched + sc.o_item);
-  q.unit = (const int2 *)(d_sched + sc.o_unit);
+  q.unit = (const QdbUnit *)(d_sched + sc.o_unit);
   q.wave_item = d_sched + sc.o_wave;
   q.M = M;
   q.W = W;
@@ -548,6 +632,7 @@ int trq_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
       g_err = "truncated QDB scan: pass_idx must be a truncated pass";
       return IXG_EINVAL;
   }
+  if (inside) RCHK(check_ilog16(h));
   if (inside ? h.trp_ig.empty() : h.trp_g.empty()) {
     g_err = inside ? "truncated QDB Inside: ixg_cm_set_itrpenalties has not been called"
                    : "truncated QDB CYK: ixg_cm_set_trpenalties has not been called";
@@ -564,6 +649,31 @@ int trq_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
   int32_t *d_dx;
   RCHK(scratch(c, S_CM_BANDS, (size_t)M * 4, (void **)&d_dx));
   HIPCHK(hipMemcpyAsync(d_dx, dxcap.data(), (size_t)M * 4, hipMemcpyHostToDevice, c->stream));
+  const size_t pq = (size_t)((local ? 3 : 0) + row) * M;
+  // the states with a truncated begin in this pass (cm_dpsearch_trunc.rs:2220-2289)
+  std::vector<char> tbegin(M, 0);
+  std::vector<int32_t> tpen(M, 0);
+  for (int y = 1; y < M; y++) {
+    if (inside) {
+      const std::vector<int32_t> &pt = local ? h.trp_il : h.trp_ig;
+      tpen[y] = pt[(size_t)row * M + y];
+      tbegin[y] = tpen[y] != -INFTY_I;
+    } else {
+      const std::vector<float> &pt = local ? h.trp_l : h.trp_g;
+      memcpy(&tpen[y], &pt[(size_t)row * M + y], 4);
+      tbegin[y] = pt[(size_t)row * M + y] > IMPOSSIBLE + 1.0f;
+    }
+  }
+  std::vector<int32_t> bands(3 * (size_t)M, 0);
+  for (int v = 0; v < M; v++) {
+    bands[v] = 1;
+    bands[M + v] = dxcap[v];
+  }
+  QdbSched sc;
+  qdb_schedule(h, bands, local, &tbegin, tpen.data(), dmax ? 0 : 1, sc);
+  int32_t *d_sched;
+  RCHK(scratch(c, S_CM_SCHED, sc.blob.size() * 4, (void **)&d_sched));
+  HIPCHK(hipMemcpyAsync(d_sched, sc.blob.data(), sc.blob.size() * 4, hipMemcpyHostToDevice, c->stream));
   const QdbView qv = qview(cm);
   const CmDev cd = cm->dev();
   TrqDev q;
@@ -571,17 +681,26 @@ int trq_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
   q.cfirst = cd.cfirst;
   q.cnum = cd.cnum;
   q.begl = qv.begl;
-  q.lvl_states = cd.lvl_states;
-  q.lvl_off = cd.lvl_off;
   q.dxcap = d_dx;
   q.ilog = qv.ilog;
-  q.nlev = cd.nlev;
+  q.lvl = d_sched + sc.o_lvl;
+  q.ring = (const int2 *)(d_sched + sc.o_ring);
+  q.row_state = d_sched + sc.o_row;
+  q.deck = (const int2 *)(d_sched + sc.o_deck);
+  q.item = (const int4 *)(d_sched + sc.o_item);
+  q.unit = (const QdbUnit *)(d_sched + sc.o_unit);
+  q.wave_item = d_sched + sc.o_wave;
   q.M = M;
   q.W = W;
   q.nbegl = nbegl;
   q.fill = fill;
   q.nonbanded = dmax ? 0 : 1;
-  const size_t pq = (size_t)((local ? 3 : 0) + row) * M;
+  q.nw = sc.nw;
+  q.nrows = sc.nrows;
+  q.deck_extra = sc.deck_extra;
+  q.lroot = sc.lroot;
+  q.tblist = (const int4 *)(d_sched + sc.o_tb);
+  q.ntb = sc.ntb;
   TrqTabs<float> tf{cd.tsc, cd.oesc, cd.endsc, qv.fel, qv.fbase0, qv.fbbase0, cd.lmesc, cd.rmesc, cm->d_tr + pq};
   const int32_t *tq = cm->d_tq;
   TrqTabs<int32_t> ti{qv.itsc, qv.ioesc, qv.iendsc, qv.iel, qv.ibase0, qv.ibbase0,
@@ -596,7 +715,7 @@ int trq_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
     std::vector<int64_t> at;
     while (k1 < n) {
       const int L = wins[k1].j0 - wins[k1].i0 + 1, w = std::min(W, L), W1 = w + 1;
-      const int64_t need = 8 * (int64_t)M * W1 + 3 * (int64_t)W1 * std::max(nbegl, 1) * W1;
+      const int64_t need = (4 * (int64_t)sc.nrows + 3 * ((int64_t)nbegl * W1 + sc.deck_extra)) * W1;
       if (k1 > k0 && elems + need > budget) break;
       at.push_back(elems);
       elems += need;
@@ -619,7 +738,7 @@ int trq_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
       x.w = w;
       x.win = (int32_t)(k - k0);
       x.rows = (int32_t *)d_mx + at[k - k0];
-      x.begl = (int32_t *)d_mx + at[k - k0] + 8 * (int64_t)M * W1;
+      x.begl = (int32_t *)d_mx + at[k - k0] + 4 * (int64_t)sc.nrows * W1;
     }
     HIPCHK(hipMemcpyAsync(d_w, wd.data(), wd.size() * sizeof(QdbWinDev), hipMemcpyHostToDevice, c->stream));
     int64_t cap = res / 8 + 4096;

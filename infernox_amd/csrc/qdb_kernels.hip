@@ -19,6 +19,7 @@
 // bit-identical to it. Root cells at or above the reporting cutoff are
 // appended to a candidate list; greedy reporting, null3 and overlap removal
 // run on the host.
+#include <type_traits>
 #include "cm_internal.h"
 
 namespace ixg {
@@ -35,7 +36,7 @@ struct Sr;
 template <>
 struct Sr<float> {
   static __device__ __forceinline__ float zero() { return IMPOSSIBLE; }
-  static __device__ __forceinline__ float comb(float a, float b, const int32_t *) { return a >= b ? a : b; }
+  static __device__ __forceinline__ float comb(float a, float b, const uint16_t *) { return a >= b ? a : b; }
   static __device__ __forceinline__ float add(float a, float b) { return a + b; }
   static __device__ __forceinline__ float bits(float a) { return a; }
   static __device__ __forceinline__ bool possible(float a) { return a > IMPOSSIBLE + 1.0f; }
@@ -43,8 +44,9 @@ struct Sr<float> {
 template <>
 struct Sr<int32_t> {
   static __device__ __forceinline__ int32_t zero() { return -INFTY_I; }
-  // ILogsum (cp9.rs:62-74), branch-free: the table is read at 0 when unused
-  static __device__ __forceinline__ int32_t comb(int32_t a, int32_t b, const int32_t *tbl) {
+  // ILogsum (cp9.rs:62-74), branch-free: the table (in LDS, its entries
+  // 0..1000 held as 16-bit) is read at 0 when unused
+  static __device__ __forceinline__ int32_t comb(int32_t a, int32_t b, const uint16_t *tbl) {
     const int32_t mx = max(-INFTY_I, max(a, b)), mn = min(a, b);
     const int32_t diff = mx - mn;
     const bool use = mn > -INFTY_I && diff < FLOGSUM_TBL;
@@ -61,7 +63,7 @@ struct Sr<int32_t> {
 // cm_nohmm.rs:187-244 reduce_c_order: init and the child terms folded in C's
 // hand-unrolled order
 template <typename T>
-__device__ __forceinline__ T reduce_c(int cn, const T *t, T init, const int32_t *tbl) {
+__device__ __forceinline__ T reduce_c(int cn, const T *t, T init, const uint16_t *tbl) {
   using R = Sr<T>;
   T s;
   switch (cn) {
@@ -89,9 +91,11 @@ __device__ __forceinline__ T reduce_c(int cn, const T *t, T init, const int32_t 
       s = R::comb(s, t[2], tbl);
       s = R::comb(s, t[1], tbl);
       return R::comb(s, t[0], tbl);
-    default:
+    default:  // not reached: a state has 1..MAXC children
       s = init;
-      for (int k = 0; k < cn; k++) s = R::comb(s, t[k], tbl);
+#pragma unroll
+      for (int k = 0; k < MAXC; k++)
+        if (k < cn) s = R::comb(s, t[k], tbl);
       return s;
   }
 }
@@ -102,88 +106,97 @@ __device__ __forceinline__ T init_sc(const QdbTabs<T> &tb, int v, int dp) {
   return Sr<T>::possible(es) ? Sr<T>::add(tb.el_sca[dp], es) : Sr<T>::zero();
 }
 
-// the row of end position j in state v's ring
-template <typename T>
-__device__ __forceinline__ T *ring_row(T *rows, int2 ri, int j, int W1) {
-  return rows + (size_t)(ri.x + (j & ri.y)) * W1;
+// first cell of the row of end position j in a packed ring
+__device__ __forceinline__ size_t ring_at(int32_t ring, int j, int W1) {
+  const int base = ring & ((1 << RING_SHIFT) - 1), mask = (1 << (ring >> RING_SHIFT)) - 1;
+  return (size_t)(base + (j & mask)) * W1;
 }
 
-struct QdbGeo {  // one window's arena
-  int i0, j0, w, W1;
-};
+__device__ __forceinline__ QdbUnit load_unit(const QdbUnit *u) {
+  const int4 *p = (const int4 *)u;
+  QdbUnit r;
+  int4 *o = (int4 *)&r;
+  o[0] = p[0];
+  o[1] = p[1];
+  o[2] = p[2];
+  o[3] = p[3];
+  return r;
+}
 
-// One cell (j, d) of a state v other than IL and the root; returns its value
+// One cell (j, d) of a state other than IL and the root; returns its value
 // (for the local-begin rows).
 template <typename T>
-__device__ __forceinline__ T qdb_cell(const QdbDev &q, const QdbTabs<T> &tb, T *rows, T *deck, const uint8_t *dsq,
-                                      const QdbGeo &g, int v, int st, int y0, int cn, int j, int d) {
+__device__ __forceinline__ T qdb_cell(const QdbTabs<T> &tb, const uint16_t *tbl, T *__restrict__ rows,
+                                      T *__restrict__ deck, const uint8_t *__restrict__ dsq, int W1,
+                                      const QdbUnit &u, int st, int cn, int j, int d) {
   using R = Sr<T>;
-  const int32_t *tbl = q.ilog;
-  const int W1 = g.W1;
+  const int v = u.code & 0x0FFFFFFF;
   const int sd = q_sd(st);
-  if (st == B_ST) {  // cm_nohmm.rs:589-606; cn holds the right child
-    const int wl = y0, y = cn, bi = q.begl[wl];
-    const int dn_y = q.dminw[y], dx_y = q.dxcap[y], dn_w = q.dminw[wl], dx_w = q.dxcap[wl];
-    const int kmin = max(0, max(dn_y, d - dx_w)), kmax = min(dx_y, d - dn_w);
-    const int2 dk = q.deck[bi];
-    const int DR = W1 + dk.y;
-    const T *db = deck + ((size_t)bi * W1 + dk.x) * W1;
-    const T *yr = ring_row(rows, q.ring[y], j, W1);
+  if (st == B_ST) {  // cm_nohmm.rs:589-606
+    const int kmin = u.k0, kmax = u.k1, DR = W1 + u.cring[2];
+    const T *db = deck + ((size_t)u.deck * W1 + u.cring[1]) * W1;
+    const T *yr = rows + ring_at(u.cring[0], j, W1);
     T sc = init_sc(tb, v, d);
     int jr = (j - kmin) % DR;  // deck row of j - k (j - kmin >= i0 - 1 >= 0)
+#pragma unroll 4
     for (int k = kmin; k <= kmax; k++) {
       sc = R::comb(sc, R::add(db[(size_t)jr * W1 + d - k], yr[k]), tbl);
       jr = jr == 0 ? DR - 1 : jr - 1;
     }
-    ring_row(rows, q.ring[v], j, W1)[d] = sc;
+    rows[ring_at(u.ring, j, W1) + d] = sc;
     return sc;
   }
   const int jy = q_sdr(st) ? j - 1 : j;
   const T *ts = tb.tsc + v * MAXC;
-  const int bv = q.begl[v];
-  if (bv >= 0) {  // BEGL_S: natural child order, into its deck (cm_nohmm.rs:607-616)
-    T sc = init_sc(tb, v, d - sd);
-    for (int yo = 0; yo < cn; yo++)
-      sc = R::comb(sc, R::add(ring_row(rows, q.ring[y0 + yo], jy, W1)[d - sd], ts[yo]), tbl);
-    const int2 dk = q.deck[bv];
-    deck[((size_t)bv * W1 + dk.x + j % (W1 + dk.y)) * W1 + d] = sc;
-    return sc;
-  }
   T terms[MAXC];
 #pragma unroll
   for (int yo = 0; yo < MAXC; yo++)
-    if (yo < cn) terms[yo] = R::add(ring_row(rows, q.ring[y0 + yo], jy, W1)[d - sd], ts[yo]);
-  T sc = reduce_c<T>(cn, terms, init_sc(tb, v, d - sd), tbl);
+    if (yo < cn) terms[yo] = R::add(rows[ring_at(u.cring[yo], jy, W1) + d - sd], ts[yo]);
+  const T init = init_sc(tb, v, d - sd);
+  if (u.deck >= 0) {  // BEGL_S: natural child order, into its deck (cm_nohmm.rs:607-616)
+    T sc = init;
+    for (int yo = 0; yo < cn; yo++) sc = R::comb(sc, terms[yo], tbl);
+    deck[((size_t)u.deck * W1 + u.k0 + j % (W1 + u.k1)) * W1 + d] = sc;
+    return sc;
+  }
   const T *oe = tb.oesc + (size_t)v * (KP * KP);
   const int i = j - d + 1;
-  if (st == ML_ST) sc = R::add(sc, oe[dsq[i]]);
-  else if (st == MR_ST || st == IR_ST) sc = R::add(sc, oe[dsq[j]]);
-  else if (st == MP_ST) sc = R::add(sc, oe[dsq[i] * KP + dsq[j]]);
-  ring_row(rows, q.ring[v], j, W1)[d] = sc;
+  const int ei = st == ML_ST ? dsq[i] : st == MP_ST ? dsq[i] * KP + dsq[j] : dsq[j];
+  T sc = reduce_c<T>(cn, terms, init, tbl);
+  if (st == ML_ST || st == MR_ST || st == IR_ST || st == MP_ST) sc = R::add(sc, oe[ei]);
+  rows[ring_at(u.ring, j, W1) + d] = sc;
   return sc;
 }
 
 // The d row dn..dx of an IL state at j: the self transition reads (j, d-1),
 // carried in a register along the row (cm_nohmm.rs:617-647).
 template <typename T>
-__device__ void qdb_il_row(const QdbDev &q, const QdbTabs<T> &tb, T *rows, const uint8_t *dsq, int W1, int v,
-                           int y0, int cn, int j, int dn, int dx) {
+__device__ void qdb_il_row(const QdbTabs<T> &tb, const uint16_t *tbl, T *__restrict__ rows,
+                           const uint8_t *__restrict__ dsq, int W1, const QdbUnit &u, int cn, int j, int dn, int dx) {
   using R = Sr<T>;
-  const int32_t *tbl = q.ilog;
+  const int v = u.code & 0x0FFFFFFF;
   const T *ts = tb.tsc + v * MAXC;
   const T *oe = tb.oesc + (size_t)v * (KP * KP);
-  T *out = ring_row(rows, q.ring[v], j, W1);
+  T *out = rows + ring_at(u.ring, j, W1);
   const T *cr[MAXC];
+  T tsr[MAXC];
 #pragma unroll
   for (int yo = 0; yo < MAXC; yo++)
-    if (yo < cn) cr[yo] = ring_row(rows, q.ring[y0 + yo], j, W1);
+    if (yo < cn) {
+      cr[yo] = rows + ring_at(u.cring[yo], j, W1);
+      tsr[yo] = ts[yo];
+    }
+  const int self = v - u.y0;  // the self transition's child index
+  const T es = tb.endsc[v];
   T prev = out[dn - 1];
+#pragma unroll 2
   for (int d = dn; d <= dx; d++) {
     T terms[MAXC];
 #pragma unroll
     for (int yo = 0; yo < MAXC; yo++)
-      if (yo < cn) terms[yo] = R::add(y0 + yo == v ? prev : cr[yo][d - 1], ts[yo]);
-    T sc = reduce_c<T>(cn, terms, init_sc(tb, v, d - 1), tbl);
+      if (yo < cn) terms[yo] = R::add(yo == self ? prev : cr[yo][d - 1], tsr[yo]);
+    const T init = R::possible(es) ? R::add(tb.el_sca[d - 1], es) : R::zero();
+    T sc = reduce_c<T>(cn, terms, init, tbl);
     sc = R::add(sc, oe[dsq[j - d + 1]]);
     out[d] = sc;
     prev = sc;
@@ -197,63 +210,73 @@ __global__ __launch_bounds__(QDB_MAX_THREADS) void qdb_kernel(QdbDev q, QdbTabs<
   using R = Sr<T>;
   const QdbWinDev wd = wins[blockIdx.x];
   const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6;
-  QdbGeo g;
-  g.i0 = wd.i0;
-  g.j0 = wd.j0;
-  g.w = wd.w;
-  g.W1 = wd.w + 1;
-  const int W1 = g.W1;
+  const int W1 = wd.w + 1;
   T *rows = (T *)wd.rows;
   T *deck = rows + (size_t)q.nrows * W1;
   T *acc = deck + ((size_t)q.nbegl * W1 + q.deck_extra) * W1;
   const uint8_t *dsq = wd.dsq;
+  const uint16_t *tbl = nullptr;
+  if constexpr (sizeof(T) == 4 && !std::is_same<T, float>::value) {
+    __shared__ uint16_t s_ilog[FLOGSUM_TBL];
+    for (int x = tid; x < FLOGSUM_TBL; x += nt) s_ilog[x] = (uint16_t)q.ilog[x];
+    tbl = s_ilog;
+  }
   // every cell starts at the semiring's zero, d = 0 at its base case
   // (cm_nohmm.rs:489-532)
-  for (int64_t x = tid; x < (int64_t)q.nrows * W1; x += nt) {
-    const int d = (int)(x % W1), v = q.row_state[x / W1];
-    rows[x] = (d == 0 && q.begl[v] < 0) ? tb.base0[v] : R::zero();
+  for (int r = wave; r < q.nrows; r += nt >> 6) {
+    const int v = q.row_state[r];
+    const T b0 = q.begl[v] < 0 ? tb.base0[v] : R::zero();
+    for (int d = lane; d < W1; d += 64) rows[(size_t)r * W1 + d] = d == 0 ? b0 : R::zero();
   }
   for (int b = 0; b < q.nbegl; b++) {
     const int2 dk = q.deck[b];
     T *db = deck + ((size_t)b * W1 + dk.x) * W1;
-    for (int64_t x = tid; x < (int64_t)(W1 + dk.y) * W1; x += nt) db[x] = (x % W1) == 0 ? tb.bbase0[b] : R::zero();
+    for (int r = wave; r < W1 + dk.y; r += nt >> 6)
+      for (int d = lane; d < W1; d += 64) db[(size_t)r * W1 + d] = d == 0 ? tb.bbase0[b] : R::zero();
   }
   if (q.local)
-    for (int64_t x = tid; x < (int64_t)(q.amask + 1) * q.nslot * W1; x += nt) acc[x] = R::zero();
+    for (int x = tid; x < (q.amask + 1) * q.nslot * W1; x += nt) acc[x] = R::zero();
   __syncthreads();
-  const int L = g.j0 - g.i0 + 1, nsteps = L + q.lroot;
+  const int L = wd.j0 - wd.i0 + 1, nsteps = L + q.lroot;
   const int ib = wave < q.nw ? q.wave_item[wave] : 0, ie = wave < q.nw ? q.wave_item[wave + 1] : 0;
-  const int r0 = q.cfirst[0], cn0 = q.cnum[0];
   for (int t = 0; t < nsteps; t++) {
+    // items are in increasing lowest level: the live ones at this step start
+    // where the highest level reaches t - (L - 1) and end before lowest > t
+    QdbUnit nxt;
+    if (ib < ie) nxt = load_unit(q.unit + (size_t)ib * 64 + lane);
     for (int it = ib; it < ie; it++) {
-      const int4 im = q.item[it];  // wave-uniform: skip items with no live cell at this step
-      if (im.y < t - (L - 1) || im.x > t || im.z > min(t - im.x + 1, g.w)) continue;
-      const int2 u = q.unit[(size_t)it * 64 + lane];
-      const int kind = u.x >> 28, v = u.x & 0x0FFFFFFF, d = u.y;
+      const int4 im = q.item[it];
+      if (im.x > t) break;
+      const QdbUnit u = nxt;
+      if (it + 1 < ie) nxt = load_unit(q.unit + (size_t)(it + 1) * 64 + lane);
+      if (im.y < t - (L - 1) || im.z > min(t - im.x + 1, wd.w)) continue;
+      const int kind = u.code >> 28, d = u.d;
       if (kind == QU_NONE) continue;
-      const int j = g.i0 + t - q.lvl[v];
-      if (j < g.i0 || j > g.j0) continue;
-      const int jrow = min(j - g.i0 + 1, g.w);
+      const int j = wd.i0 + t - u.lvl;
+      if (j < wd.i0 || j > wd.j0) continue;
+      const int jrow = min(j - wd.i0 + 1, wd.w);
+      const int st = u.info & 15, cn = (u.info >> 4) & 15;
       if (kind == QU_IL) {
-        const int dx = min(jrow, q.dxcap[v]);
-        if (dx >= d) qdb_il_row<T>(q, tb, rows, dsq, W1, v, q.cfirst[v], q.cnum[v], j, d, dx);
+        const int dx = min(jrow, u.y1);
+        if (dx >= d) qdb_il_row<T>(tb, tbl, rows, dsq, W1, u, cn, j, d, dx);
         continue;
       }
       if (d > jrow) continue;
       if (kind == QU_CELL) {
-        const T sc = qdb_cell<T>(q, tb, rows, deck, dsq, g, v, q.sttype[v], q.cfirst[v], q.cnum[v], j, d);
-        const int slot = q.acc[v];
+        const T sc = qdb_cell<T>(tb, tbl, rows, deck, dsq, W1, u, st, cn, j, d);
+        const int slot = (u.info >> 8) - 1;
         if (q.local && slot >= 0) {  // this begin's candidate for the root at (j, d)
           T &a = acc[((size_t)(j & q.amask) * q.nslot + slot) * W1 + d];
-          const T cnd = R::add(sc, tb.beginsc[v]);
+          const T cnd = R::add(sc, tb.beginsc[u.code & 0x0FFFFFFF]);
           if (a < cnd) a = cnd;
         }
         continue;
       }
       // the root: its own transitions, then local begins by MAX (cm_nohmm.rs:686-746)
-      T a = R::comb(R::zero(), R::add(ring_row(rows, q.ring[r0], j, W1)[d], tb.tsc[0]), q.ilog);
-      for (int yo = 1; yo < cn0; yo++)
-        a = R::comb(a, R::add(ring_row(rows, q.ring[r0 + yo], j, W1)[d], tb.tsc[yo]), q.ilog);
+      T a = R::comb(R::zero(), R::add(rows[ring_at(u.cring[0], j, W1) + d], tb.tsc[0]), tbl);
+#pragma unroll
+      for (int yo = 1; yo < MAXC; yo++)
+        if (yo < cn) a = R::comb(a, R::add(rows[ring_at(u.cring[yo], j, W1) + d], tb.tsc[yo]), tbl);
       if (q.local)
         for (int s = 0; s < q.nslot; s++) {
           T &b = acc[((size_t)(j & q.amask) * q.nslot + s) * W1 + d];

@@ -1,26 +1,31 @@
 // gfx950 QDB / non-banded truncated scanners: truncated CYK on float
 // scores and truncated Inside on integer scores with the ILogsum table
-// (cm_dpsearch_trunc.rs:1800-2316 generic_tr_scan). One 256-lane workgroup
-// per window walks the end positions j in order; for each j it fills the
-// four marginal planes J/L/R/T (rows rolling over two j's, [2][M][w+1] per
-// plane) and the BEGL_S decks of J/L/R ([w+1][nbegl][w+1]) one dependency
-// level at a time, each lane computing whole cells (d = 1..min(jrow, dmax, W)
-// for every state); an IL state's row is one lane's serial walk along its
-// self transition. Children are folded in increasing order, the bifurcation
-// and pair-state marginal cells take the maximum of their J and L (or R)
-// terms exactly where the reference does, and the root rows are never reset
-// between j's: a truncated begin must beat what the same-parity row holds at
-// d. Per root cell the best begin (score, state, mode) is kept; cells at or
-// above the reporting cutoff go to a candidate list, and the window's best
-// root score, its mode and the full-window best to a summary. Greedy
-// reporting, truncation legality, null3 and overlap removal run on the host.
+// (cm_dpsearch_trunc.rs:1800-2316 generic_tr_scan). One workgroup per
+// window runs the same wavefront as the standard scanners: at step t each
+// state v fills the four marginal planes J/L/R/T of its cells at end position
+// j = i0 + t - lvl[v] (d = 1..min(jrow, dmax, W)); every read goes to lower
+// levels at j or to j - 1, written at earlier steps, so one barrier per step
+// suffices. Rows live in per-state rings (four planes per row) deep enough
+// for their last reader; the states the pass gives a truncated begin keep
+// their rows until the root has read them at the same j. BEGL_S decks keep
+// J/L/R over w+1 rows plus the steps between writer and bifurcation. An IL
+// state's row is one lane's serial walk along its self transition. Children
+// are folded in increasing order, the bifurcation and pair-state marginal
+// cells take the maximum of their J and L (or R) terms exactly where the
+// reference does, and the root's two parity rows are never reset between
+// j's: a truncated begin must beat what the same-parity row holds at d. The
+// root scans the begin states in increasing order, as the reference does,
+// and keeps per cell the best begin (score, state, mode); cells at or above
+// the reporting cutoff go to a candidate list, and the window's best root
+// score, its mode and the full-window best to a summary. Greedy reporting,
+// truncation legality, null3 and overlap removal run on the host.
+#include <type_traits>
 #include "cm_internal.h"
 
 namespace ixg {
 
 namespace {
 
-constexpr int TRQ_THREADS = 256;
 constexpr int MODE_T = 0, MODE_R = 1, MODE_L = 2, MODE_J = 3, MODE_UNKNOWN = 4;
 
 template <typename T>
@@ -28,7 +33,7 @@ struct Tq;
 template <>
 struct Tq<float> {
   static __device__ __forceinline__ float zero() { return IMPOSSIBLE; }
-  static __device__ __forceinline__ float comb(float a, float b, const int32_t *) { return a >= b ? a : b; }
+  static __device__ __forceinline__ float comb(float a, float b, const uint16_t *) { return a >= b ? a : b; }
   static __device__ __forceinline__ float max2(float a, float b) { return a >= b ? a : b; }
   static __device__ __forceinline__ float add(float a, float b) { return a + b; }
   static __device__ __forceinline__ float bits(float a) { return a; }
@@ -37,8 +42,9 @@ struct Tq<float> {
 template <>
 struct Tq<int32_t> {
   static __device__ __forceinline__ int32_t zero() { return -INFTY_I; }
-  // ILogsum (cp9.rs:62-74), branch-free: the table is read at 0 when unused
-  static __device__ __forceinline__ int32_t comb(int32_t a, int32_t b, const int32_t *tbl) {
+  // ILogsum (cp9.rs:62-74), branch-free: the table (in LDS, its entries
+  // 0..1000 held as 16-bit) is read at 0 when unused
+  static __device__ __forceinline__ int32_t comb(int32_t a, int32_t b, const uint16_t *tbl) {
     const int32_t mx = max(-INFTY_I, max(a, b)), mn = min(a, b);
     const int32_t diff = mx - mn;
     const bool use = mn > -INFTY_I && diff < FLOGSUM_TBL;
@@ -53,17 +59,22 @@ struct Tq<int32_t> {
   static __device__ __forceinline__ bool possible(int32_t a) { return a != -INFTY_I; }
 };
 
-template <typename T>
-struct Mx {
-  T *rows, *dk;  // planes J|L|R|T of [2][M][W1]; decks J|L|R of [W1][nbegl][W1]
-  int M, W1, nbegl;
-  __device__ __forceinline__ T &p(int pl, int par, int v, int d) const {
-    return rows[(((size_t)pl * 2 + par) * M + v) * W1 + d];
-  }
-  __device__ __forceinline__ T &b(int pl, int jr, int bi, int d) const {
-    return dk[(((size_t)pl * W1 + jr) * nbegl + bi) * W1 + d];
-  }
-};
+// first cell of plane 0 of the row of end position j in a packed ring (four planes per row)
+__device__ __forceinline__ size_t tring_at(int32_t ring, int j, int W1) {
+  const int base = ring & ((1 << RING_SHIFT) - 1), mask = (1 << (ring >> RING_SHIFT)) - 1;
+  return (size_t)(base + (j & mask)) * 4 * W1;
+}
+
+__device__ __forceinline__ QdbUnit load_unit(const QdbUnit *u) {
+  const int4 *p = (const int4 *)u;
+  QdbUnit r;
+  int4 *o = (int4 *)&r;
+  o[0] = p[0];
+  o[1] = p[1];
+  o[2] = p[2];
+  o[3] = p[3];
+  return r;
+}
 
 template <typename T>
 __device__ __forceinline__ T tinit(const TrqTabs<T> &tb, int v, int dp) {
@@ -72,252 +83,306 @@ __device__ __forceinline__ T tinit(const TrqTabs<T> &tb, int v, int dp) {
 }
 
 __device__ __forceinline__ int t_sd(int st) { return st == MP_ST ? 2 : ((st >= ML_ST && st <= IR_ST) ? 1 : 0); }
-__device__ __forceinline__ int t_sdr(int st) { return (st == MP_ST || st == MR_ST || st == IR_ST) ? 1 : 0; }
 
 // One cell (j, d) of a state other than IL and the root, every filled plane
-// (cm_dpsearch_trunc.rs:1978-2217).
+// (cm_dpsearch_trunc.rs:1978-2217). P: plane stride (W1); cj / cp: each
+// child's row of j and of j - 1.
 template <typename T>
-__device__ void trq_cell(const TrqDev &q, const TrqTabs<T> &tb, const Mx<T> &x, const uint8_t *dsq, int v, int st,
-                         int j, int d, int cur, int prv) {
+__device__ void trq_cell(const TrqDev &q, const TrqTabs<T> &tb, const uint16_t *tbl, T *__restrict__ rows,
+                         T *__restrict__ dk, const uint8_t *__restrict__ dsq, int W1, const QdbUnit &u, int st,
+                         int cn, int j, int d) {
   using R = Tq<T>;
-  const int32_t *tbl = q.ilog;
   const bool fl = q.fill & 2, fr = q.fill & 4, ft = q.fill & 8;
+  const int v = u.code & 0x0FFFFFFF;
   const int sd = t_sd(st);
   const T z = R::zero();
+  const size_t P = W1;
+  T *out = rows + tring_at(u.ring, j, W1) + d;
   if (st == B_ST) {
-    const int wl = q.cfirst[v], y = q.cnum[v], bi = q.begl[wl];
-    int kmin = 0, kmax = d;
-    if (!q.nonbanded) {
-      kmin = max(0, d - q.dxcap[wl]);
-      kmax = min(q.dxcap[y], d);
-    }
+    const int kmin = u.k0, kmax = u.k1, DR = W1 + u.cring[2];
+    const T *db = dk + ((size_t)u.deck * W1 + u.cring[1]) * 3 * P;
+    const size_t rs = 3 * P;  // deck row stride
+    const T *yj = rows + tring_at(u.cring[0], j, W1), *yl = yj + P;
     T jsc = tinit(tb, v, d - sd);
     T lsc = fl ? jsc : z, rsc = fr ? jsc : z, tsc = ft ? jsc : z;
+    int jr = (j - kmin) % DR;  // deck row of j - k (j - kmin >= i0 - 1 >= 0)
+#pragma unroll 4
     for (int k = kmin; k <= kmax; k++) {
-      const int jr = ((j - k) % x.W1 + x.W1) % x.W1;
-      const T lft = x.b(0, jr, bi, d - k);
-      jsc = R::comb(jsc, R::add(lft, x.p(0, cur, y, k)), tbl);
-      if (fl) lsc = R::comb(lsc, R::add(lft, x.p(1, cur, y, k)), tbl);
-      if (fr) rsc = R::comb(rsc, R::add(x.b(2, jr, bi, d - k), x.p(0, cur, y, k)), tbl);
+      const T *dr = db + jr * rs + (d - k);
+      const T lft = dr[0];
+      jsc = R::comb(jsc, R::add(lft, yj[k]), tbl);
+      if (fl) lsc = R::comb(lsc, R::add(lft, yl[k]), tbl);
+      if (fr) rsc = R::comb(rsc, R::add(dr[2 * P], yj[k]), tbl);
+      jr = jr == 0 ? DR - 1 : jr - 1;
     }
-    if (ft)
-      for (int k = max(1, kmin); k <= min(d - 1, kmax); k++) {
-        const int jr = ((j - k) % x.W1 + x.W1) % x.W1;
-        tsc = R::comb(tsc, R::add(x.b(2, jr, bi, d - k), x.p(1, cur, y, k)), tbl);
+    if (ft) {
+      const int k0 = max(1, kmin);
+      jr = (j - k0) % DR;
+      for (int k = k0; k <= min(d - 1, kmax); k++) {
+        tsc = R::comb(tsc, R::add(db[jr * rs + 2 * P + (d - k)], yl[k]), tbl);
+        jr = jr == 0 ? DR - 1 : jr - 1;
       }
-    x.p(0, cur, v, d) = jsc;
-    if (ft) x.p(3, cur, v, d) = tsc;
-    const int j0r = j % x.W1;
-    if (fl) x.p(1, cur, v, d) = kmin == 0 ? R::comb(lsc, R::max2(x.b(0, j0r, bi, d), x.b(1, j0r, bi, d)), tbl) : lsc;
-    if (fr) x.p(2, cur, v, d) = kmax == d ? R::comb(rsc, R::max2(x.p(0, cur, y, d), x.p(2, cur, y, d)), tbl) : rsc;
+    }
+    out[0] = jsc;
+    if (ft) out[3 * P] = tsc;
+    const T *dj = db + (size_t)(j % DR) * rs + d;
+    if (fl) out[P] = kmin == 0 ? R::comb(lsc, R::max2(dj[0], dj[P]), tbl) : lsc;
+    if (fr) out[2 * P] = kmax == d ? R::comb(rsc, R::max2(yj[d], yj[2 * P + d]), tbl) : rsc;
     return;
   }
-  const int y0 = q.cfirst[v], cn = q.cnum[v];
-  const int jp_y = t_sdr(st) ? prv : cur, jq_y = t_sdr(st) ? cur : prv;
   const T *ts = tb.tsc + v * MAXC;
-  if (q.begl[v] >= 0) {  // BEGL_S: into its J/L/R decks (:2046-2067)
+  const T *cj[MAXC], *cp[MAXC];  // children's rows of j and j - 1
+  T tsr[MAXC];
+#pragma unroll
+  for (int yo = 0; yo < MAXC; yo++)
+    if (yo < cn) {
+      cj[yo] = rows + tring_at(u.cring[yo], j, W1);
+      cp[yo] = rows + tring_at(u.cring[yo], j - 1, W1);
+      tsr[yo] = ts[yo];
+    }
+  if (u.deck >= 0) {  // BEGL_S: into its J/L/R decks (:2046-2067)
     T jsc = tinit(tb, v, d - sd);
     T lsc = fl ? jsc : z, rsc = fr ? jsc : z;
-    for (int yo = 0; yo < cn; yo++) {
-      jsc = R::comb(jsc, R::add(x.p(0, jp_y, y0 + yo, d - sd), ts[yo]), tbl);
-      if (fl) lsc = R::comb(lsc, R::add(x.p(1, jp_y, y0 + yo, d - sd), ts[yo]), tbl);
-      if (fr) rsc = R::comb(rsc, R::add(x.p(2, jp_y, y0 + yo, d - sd), ts[yo]), tbl);
-    }
-    const int jr = j % x.W1, bi = q.begl[v];
-    x.b(0, jr, bi, d) = jsc;
-    if (fl) x.b(1, jr, bi, d) = lsc;
-    if (fr) x.b(2, jr, bi, d) = rsc;
+#pragma unroll
+    for (int yo = 0; yo < MAXC; yo++)
+      if (yo < cn) {
+        jsc = R::comb(jsc, R::add(cj[yo][d - sd], tsr[yo]), tbl);
+        if (fl) lsc = R::comb(lsc, R::add(cj[yo][P + d - sd], tsr[yo]), tbl);
+        if (fr) rsc = R::comb(rsc, R::add(cj[yo][2 * P + d - sd], tsr[yo]), tbl);
+      }
+    T *dr = dk + ((size_t)u.deck * W1 + u.k0 + j % (W1 + u.k1)) * 3 * P + d;
+    dr[0] = jsc;
+    if (fl) dr[P] = lsc;
+    if (fr) dr[2 * P] = rsc;
     return;
   }
   const T *oe = tb.oesc + (size_t)v * (KP * KP);
   const int i = j - d + 1;
   if (st == ML_ST) {  // EMITLEFT, non-self (:2068-2106)
+    const T e = oe[dsq[i]];
     T jsc = tinit(tb, v, d - sd);
     T lsc = fl ? jsc : z;
-    for (int yo = 0; yo < cn; yo++) {
-      jsc = R::comb(jsc, R::add(x.p(0, jp_y, y0 + yo, d - sd), ts[yo]), tbl);
-      if (fl) lsc = R::comb(lsc, R::add(x.p(1, jp_y, y0 + yo, d - sd), ts[yo]), tbl);
-    }
-    const T e = oe[dsq[i]];
-    x.p(0, cur, v, d) = R::add(jsc, e);
-    if (fl) x.p(1, cur, v, d) = d >= 2 ? R::add(lsc, e) : e;
+#pragma unroll
+    for (int yo = 0; yo < MAXC; yo++)
+      if (yo < cn) {
+        jsc = R::comb(jsc, R::add(cj[yo][d - sd], tsr[yo]), tbl);
+        if (fl) lsc = R::comb(lsc, R::add(cj[yo][P + d - sd], tsr[yo]), tbl);
+      }
+    out[0] = R::add(jsc, e);
+    if (fl) out[P] = d >= 2 ? R::add(lsc, e) : e;
     if (fr) {
       T rsc = tinit(tb, v, d);
-      for (int yo = 0; yo < cn; yo++)
-        rsc = R::comb(rsc, R::max2(R::add(x.p(0, jp_y, y0 + yo, d), ts[yo]), R::add(x.p(2, jp_y, y0 + yo, d), ts[yo])),
-                      tbl);
-      x.p(2, cur, v, d) = rsc;
+#pragma unroll
+      for (int yo = 0; yo < MAXC; yo++)
+        if (yo < cn) rsc = R::comb(rsc, R::max2(R::add(cj[yo][d], tsr[yo]), R::add(cj[yo][2 * P + d], tsr[yo])), tbl);
+      out[2 * P] = rsc;
     }
   } else if (st == MR_ST || st == IR_ST) {  // EMITRIGHT (:2107-2141)
     const T esc_j = oe[dsq[j]];
     T jsc = tinit(tb, v, d - sd);
     T rsc = fr ? jsc : z;
-    for (int yo = 0; yo < cn; yo++) {
-      jsc = R::comb(jsc, R::add(x.p(0, jp_y, y0 + yo, d - sd), ts[yo]), tbl);
-      if (fr) rsc = R::comb(rsc, R::add(x.p(2, jp_y, y0 + yo, d - sd), ts[yo]), tbl);
-    }
-    x.p(0, cur, v, d) = R::add(jsc, esc_j);
-    if (fr) x.p(2, cur, v, d) = d >= 2 ? R::add(rsc, esc_j) : esc_j;
+#pragma unroll
+    for (int yo = 0; yo < MAXC; yo++)
+      if (yo < cn) {
+        jsc = R::comb(jsc, R::add(cp[yo][d - sd], tsr[yo]), tbl);
+        if (fr) rsc = R::comb(rsc, R::add(cp[yo][2 * P + d - sd], tsr[yo]), tbl);
+      }
+    out[0] = R::add(jsc, esc_j);
+    if (fr) out[2 * P] = d >= 2 ? R::add(rsc, esc_j) : esc_j;
     if (fl) {
       T lsc = tinit(tb, v, d);
-      for (int yo = st == IR_ST ? 1 : 0; yo < cn; yo++)
-        lsc = R::comb(lsc, R::max2(R::add(x.p(0, jq_y, y0 + yo, d), ts[yo]), R::add(x.p(1, jq_y, y0 + yo, d), ts[yo])),
-                      tbl);
-      x.p(1, cur, v, d) = lsc;
+      const int y1 = st == IR_ST ? 1 : 0;
+#pragma unroll
+      for (int yo = 0; yo < MAXC; yo++)
+        if (yo >= y1 && yo < cn)
+          lsc = R::comb(lsc, R::max2(R::add(cj[yo][d], tsr[yo]), R::add(cj[yo][P + d], tsr[yo])), tbl);
+      out[P] = lsc;
     }
   } else if (st == MP_ST) {  // EMITPAIR (:2142-2190)
     T jsc = d >= 2 ? tinit(tb, v, d - sd) : z;
     T lsc = fl ? tinit(tb, v, d - 1) : z, rsc = fr ? tinit(tb, v, d - 1) : z;
-    for (int yo = 0; yo < cn; yo++) {
-      if (d >= 2) jsc = R::comb(jsc, R::add(x.p(0, jp_y, y0 + yo, d - 2), ts[yo]), tbl);
-      if (fl)
-        lsc = R::comb(lsc, R::max2(R::add(x.p(0, jq_y, y0 + yo, d - 1), ts[yo]), R::add(x.p(1, jq_y, y0 + yo, d - 1), ts[yo])),
-                      tbl);
-      if (fr)
-        rsc = R::comb(rsc, R::max2(R::add(x.p(0, jp_y, y0 + yo, d - 1), ts[yo]), R::add(x.p(2, jp_y, y0 + yo, d - 1), ts[yo])),
-                      tbl);
-    }
-    x.p(0, cur, v, d) = d >= 2 ? R::add(jsc, oe[dsq[i] * KP + dsq[j]]) : z;
+#pragma unroll
+    for (int yo = 0; yo < MAXC; yo++)
+      if (yo < cn) {
+        if (d >= 2) jsc = R::comb(jsc, R::add(cp[yo][d - 2], tsr[yo]), tbl);
+        if (fl) lsc = R::comb(lsc, R::max2(R::add(cj[yo][d - 1], tsr[yo]), R::add(cj[yo][P + d - 1], tsr[yo])), tbl);
+        if (fr) rsc = R::comb(rsc, R::max2(R::add(cp[yo][d - 1], tsr[yo]), R::add(cp[yo][2 * P + d - 1], tsr[yo])), tbl);
+      }
+    out[0] = d >= 2 ? R::add(jsc, oe[dsq[i] * KP + dsq[j]]) : z;
     const T lm = tb.lmesc[v * KP + dsq[i]], rm = tb.rmesc[v * KP + dsq[j]];
-    if (fl) x.p(1, cur, v, d) = d >= 2 ? R::add(lsc, lm) : lm;
-    if (fr) x.p(2, cur, v, d) = d >= 2 ? R::add(rsc, rm) : rm;
+    if (fl) out[P] = d >= 2 ? R::add(lsc, lm) : lm;
+    if (fr) out[2 * P] = d >= 2 ? R::add(rsc, rm) : rm;
   } else {  // EMITNONE: D, S (:2191-2217)
     T jsc = tinit(tb, v, d - sd);
     T lsc = fl ? jsc : z, rsc = fr ? jsc : z;
-    for (int yo = 0; yo < cn; yo++) {
-      jsc = R::comb(jsc, R::add(x.p(0, jp_y, y0 + yo, d - sd), ts[yo]), tbl);
-      if (fl) lsc = R::comb(lsc, R::add(x.p(1, jp_y, y0 + yo, d - sd), ts[yo]), tbl);
-      if (fr) rsc = R::comb(rsc, R::add(x.p(2, jp_y, y0 + yo, d - sd), ts[yo]), tbl);
-    }
-    x.p(0, cur, v, d) = jsc;
-    if (fl) x.p(1, cur, v, d) = lsc;
-    if (fr) x.p(2, cur, v, d) = rsc;
+#pragma unroll
+    for (int yo = 0; yo < MAXC; yo++)
+      if (yo < cn) {
+        jsc = R::comb(jsc, R::add(cj[yo][d - sd], tsr[yo]), tbl);
+        if (fl) lsc = R::comb(lsc, R::add(cj[yo][P + d - sd], tsr[yo]), tbl);
+        if (fr) rsc = R::comb(rsc, R::add(cj[yo][2 * P + d - sd], tsr[yo]), tbl);
+      }
+    out[0] = jsc;
+    if (fl) out[P] = lsc;
+    if (fr) out[2 * P] = rsc;
   }
 }
 
 // The row d = 1..dx of an IL state at j: its J and L cells read the same
-// state at (j, d-1), so one lane walks the row (:2068-2106).
+// state at (j, d-1), carried in registers along the row (:2068-2106).
 template <typename T>
-__device__ void trq_il_row(const TrqDev &q, const TrqTabs<T> &tb, const Mx<T> &x, const uint8_t *dsq, int v, int j,
-                           int dx, int cur) {
+__device__ void trq_il_row(const TrqDev &q, const TrqTabs<T> &tb, const uint16_t *tbl, T *__restrict__ rows,
+                           const uint8_t *__restrict__ dsq, int W1, const QdbUnit &u, int cn, int j, int dx) {
   using R = Tq<T>;
-  const int32_t *tbl = q.ilog;
   const bool fl = q.fill & 2, fr = q.fill & 4;
   const T z = R::zero();
-  const int y0 = q.cfirst[v], cn = q.cnum[v];
+  const int v = u.code & 0x0FFFFFFF;
+  const size_t P = W1;
   const T *ts = tb.tsc + v * MAXC;
   const T *oe = tb.oesc + (size_t)v * (KP * KP);
+  T *out = rows + tring_at(u.ring, j, W1);
+  const T *cj[MAXC];
+  T tsr[MAXC];
+#pragma unroll
+  for (int yo = 0; yo < MAXC; yo++)
+    if (yo < cn) {
+      cj[yo] = rows + tring_at(u.cring[yo], j, W1);
+      tsr[yo] = ts[yo];
+    }
+  const int self = v - u.y0;  // the self transition's child index
+  T pj = out[0], pl = out[P];  // the state's own J and L at (j, d-1)
+#pragma unroll 2
   for (int d = 1; d <= dx; d++) {
     T jsc = tinit(tb, v, d - 1);
     T lsc = fl ? jsc : z;
-    for (int yo = 0; yo < cn; yo++) {
-      jsc = R::comb(jsc, R::add(x.p(0, cur, y0 + yo, d - 1), ts[yo]), tbl);
-      if (fl) lsc = R::comb(lsc, R::add(x.p(1, cur, y0 + yo, d - 1), ts[yo]), tbl);
-    }
+#pragma unroll
+    for (int yo = 0; yo < MAXC; yo++)
+      if (yo < cn) {
+        jsc = R::comb(jsc, R::add(yo == self ? pj : cj[yo][d - 1], tsr[yo]), tbl);
+        if (fl) lsc = R::comb(lsc, R::add(yo == self ? pl : cj[yo][P + d - 1], tsr[yo]), tbl);
+      }
     const T e = oe[dsq[j - d + 1]];
-    x.p(0, cur, v, d) = R::add(jsc, e);
-    if (fl) x.p(1, cur, v, d) = d >= 2 ? R::add(lsc, e) : e;
+    pj = R::add(jsc, e);
+    out[d] = pj;
+    if (fl) {
+      pl = d >= 2 ? R::add(lsc, e) : e;
+      out[P + d] = pl;
+    }
     if (fr) {
       T rsc = tinit(tb, v, d);
-      for (int yo = 1; yo < cn; yo++)
-        rsc = R::comb(rsc, R::max2(R::add(x.p(0, cur, y0 + yo, d), ts[yo]), R::add(x.p(2, cur, y0 + yo, d), ts[yo])),
-                      tbl);
-      x.p(2, cur, v, d) = rsc;
+#pragma unroll
+      for (int yo = 1; yo < MAXC; yo++)
+        if (yo < cn) rsc = R::comb(rsc, R::max2(R::add(cj[yo][d], tsr[yo]), R::add(cj[yo][2 * P + d], tsr[yo])), tbl);
+      out[2 * P + d] = rsc;
     }
   }
 }
 
 template <typename T>
-__global__ __launch_bounds__(TRQ_THREADS) void trqdb_kernel(TrqDev q, TrqTabs<T> tb, const QdbWinDev *wins,
-                                                             float cutoff, TrqCand *cand, int64_t cap,
-                                                             unsigned long long *counter, TrqSummary *summ) {
+__global__ __launch_bounds__(QDB_MAX_THREADS) void trqdb_kernel(TrqDev q, TrqTabs<T> tb, const QdbWinDev *wins,
+                                                                float cutoff, TrqCand *cand, int64_t cap,
+                                                                unsigned long long *counter, TrqSummary *summ) {
   using R = Tq<T>;
-  __shared__ float s_sc[TRQ_THREADS];
-  __shared__ int s_j[TRQ_THREADS], s_d[TRQ_THREADS], s_m[TRQ_THREADS];
+  __shared__ float s_sc[QDB_MAX_THREADS];
+  __shared__ int s_j[QDB_MAX_THREADS], s_d[QDB_MAX_THREADS], s_m[QDB_MAX_THREADS];
   __shared__ float s_full;
   const QdbWinDev wd = wins[blockIdx.x];
-  const int tid = threadIdx.x, nt = blockDim.x, M = q.M;
-  Mx<T> x;
-  x.rows = (T *)wd.rows;
-  x.dk = (T *)wd.begl;
-  x.M = M;
-  x.W1 = wd.w + 1;
-  x.nbegl = q.nbegl;
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6;
+  const int W1 = wd.w + 1;
+  const size_t P = W1;
+  T *rows = (T *)wd.rows, *dk = (T *)wd.begl;
   const uint8_t *dsq = wd.dsq;
+  const uint16_t *tbl = nullptr;
+  if constexpr (!std::is_same<T, float>::value) {
+    __shared__ uint16_t s_ilog[FLOGSUM_TBL];
+    for (int x = tid; x < FLOGSUM_TBL; x += nt) s_ilog[x] = (uint16_t)q.ilog[x];
+    tbl = s_ilog;
+  }
   // every cell starts at the semiring's zero; d = 0 at the base cases, set
   // once (:1873-1922): J of non-BEGL states, L and R of E states, the J decks
-  const int64_t plane = 2 * (int64_t)M * x.W1;
-  for (int64_t k = tid; k < 4 * plane; k += nt) {
-    const int pl = (int)(k / plane);
-    const int64_t r = k % plane;
-    const int d = (int)(r % x.W1), v = (int)((r / x.W1) % M);
-    T val = R::zero();
-    if (d == 0 && q.begl[v] < 0) {
-      if (pl == 0) val = tb.base0[v];
-      else if ((pl == 1 || pl == 2) && q.sttype[v] == E_ST) val = (T)0;
+  for (int r = wave; r < 4 * q.nrows; r += nt >> 6) {
+    const int v = q.row_state[r >> 2], pl = r & 3;
+    T b0 = R::zero();
+    if (q.begl[v] < 0) {
+      if (pl == 0) b0 = tb.base0[v];
+      else if ((pl == 1 || pl == 2) && q.sttype[v] == E_ST) b0 = (T)0;
     }
-    x.rows[k] = val;
+    for (int d = lane; d < W1; d += 64) rows[(size_t)r * W1 + d] = d == 0 ? b0 : R::zero();
   }
-  const int64_t deck = (int64_t)x.W1 * q.nbegl * x.W1;
-  for (int64_t k = tid; k < 3 * deck; k += nt) {
-    const int64_t r = k % deck;
-    const int d = (int)(r % x.W1), bi = (int)((r / x.W1) % q.nbegl);
-    x.dk[k] = (k < deck && d == 0) ? tb.bbase0[bi] : R::zero();
+  for (int b = 0; b < q.nbegl; b++) {
+    const int2 k = q.deck[b];
+    T *db = dk + ((size_t)b * W1 + k.x) * 3 * P;
+    for (int r = wave; r < 3 * (W1 + k.y); r += nt >> 6)
+      for (int d = lane; d < W1; d += 64) db[(size_t)r * W1 + d] = (r % 3 == 0 && d == 0) ? tb.bbase0[b] : R::zero();
   }
   if (tid == 0) s_full = IMPOSSIBLE;
   __syncthreads();
   const bool fl = q.fill & 2, fr = q.fill & 4, ft = q.fill & 8;
   float best_sc = IMPOSSIBLE;
   int best_j = 0, best_d = 0, best_m = MODE_UNKNOWN;
-  for (int j = wd.i0; j <= wd.j0; j++) {
-    const int jrow = min(j - wd.i0 + 1, wd.w), cur = j & 1, prv = (j - 1) & 1;
-    for (int l = 0; l < q.nlev; l++) {
-      int off = 0;  // units dealt so far in this level, modulo nt
-      for (int s = q.lvl_off[l]; s < q.lvl_off[l + 1]; s++) {
-        const int v = q.lvl_states[s];
-        const int st = q.sttype[v];
-        if (v == 0 || st == E_ST) continue;
-        // an emitter followed by an END node's E state is detached (:88)
-        if ((st == ML_ST || st == IL_ST || st == MR_ST || st == IR_ST) && v + 1 < M && q.sttype[v + 1] == E_ST)
-          continue;
-        const int dx = min(jrow, q.dxcap[v]);
-        if (dx < 1) continue;
-        const int n = st == IL_ST ? 1 : dx;
-        int r = (tid - off) % nt;
-        if (r < 0) r += nt;
-        for (int u = r; u < n; u += nt) {
-          if (st == IL_ST) trq_il_row<T>(q, tb, x, dsq, v, j, dx, cur);
-          else trq_cell<T>(q, tb, x, dsq, v, st, j, 1 + u, cur, prv);
-        }
-        off = (off + n) % nt;
+  const int L = wd.j0 - wd.i0 + 1, nsteps = L + q.lroot;
+  const int ib = wave < q.nw ? q.wave_item[wave] : 0, ie = wave < q.nw ? q.wave_item[wave + 1] : 0;
+  for (int t = 0; t < nsteps; t++) {
+    // items are in increasing lowest level: stop at the first one whose
+    // lowest level is above t
+    QdbUnit nxt;
+    if (ib < ie) nxt = load_unit(q.unit + (size_t)ib * 64 + lane);
+    for (int it = ib; it < ie; it++) {
+      const int4 im = q.item[it];
+      if (im.x > t) break;
+      const QdbUnit u = nxt;
+      if (it + 1 < ie) nxt = load_unit(q.unit + (size_t)(it + 1) * 64 + lane);
+      if (im.y < t - (L - 1) || im.z > min(t - im.x + 1, wd.w)) continue;
+      const int kind = u.code >> 28, d = u.d;
+      if (kind == QU_NONE) continue;
+      const int j = wd.i0 + t - u.lvl;
+      if (j < wd.i0 || j > wd.j0) continue;
+      const int jrow = min(j - wd.i0 + 1, wd.w);
+      const int st = u.info & 15, cn = (u.info >> 4) & 15;
+      if (kind == QU_IL) {
+        const int dx = min(jrow, u.y1);
+        if (dx >= 1) trq_il_row<T>(q, tb, tbl, rows, dsq, W1, u, cn, j, dx);
+        continue;
       }
-      __syncthreads();
-    }
-    // the root: truncated begins into the persistent root rows (:2220-2289)
-    const int dx0 = min(jrow, q.dxcap[0]);
-    for (int d = 1 + tid; d <= dx0; d += nt) {
+      if (d > jrow) continue;
+      if (kind == QU_CELL) {
+        trq_cell<T>(q, tb, tbl, rows, dk, dsq, W1, u, st, cn, j, d);
+        continue;
+      }
+      // the root: truncated begins into the persistent root rows (:2220-2289),
+      // the begin states in increasing order, the row held in registers
       float bsc = IMPOSSIBLE;
       int br = 0, bm = MODE_UNKNOWN;
-      for (int y = 1; y < M; y++) {
-        const T tp = tb.pty[y];
-        if (!R::possible(tp) || d > min(jrow, q.dxcap[y])) continue;
-        const int ny = (ft && q.sttype[y] == B_ST) ? 4 : 3;
-        for (int pl = 0; pl < ny; pl++) {
-          if ((pl == 1 && !fl) || (pl == 2 && !fr)) continue;
-          const T sc = R::add(x.p(pl, cur, y, d), tp);
-          T &root = x.p(pl, cur, 0, d);
-          if (sc > root) {
-            root = sc;
+      T *root = rows + tring_at(u.ring, j, W1) + d;
+      T rt[4] = {root[0], root[P], root[2 * P], root[3 * P]};
+#pragma unroll 4
+      for (int e = 0; e < q.ntb; e++) {
+        const int4 te = q.tblist[e];
+        if (d > min(jrow, te.z)) continue;
+        T tp;
+        memcpy(&tp, &te.y, 4);
+        const T *yc = rows + tring_at(te.x, j, W1) + d;
+        const int ny = (ft && (te.w >> 30)) ? 4 : 3;
+#pragma unroll
+        for (int pl = 0; pl < 4; pl++) {
+          if (pl >= ny || (pl == 1 && !fl) || (pl == 2 && !fr)) continue;
+          const T sc = R::add(yc[pl * P], tp);
+          if (sc > rt[pl]) {
+            rt[pl] = sc;
             const float fsc = R::bits(sc);
             if (fsc > bsc) {
               bsc = fsc;
               bm = pl == 0 ? MODE_J : pl == 1 ? MODE_L : pl == 2 ? MODE_R : MODE_T;
-              br = y;
+              br = te.w & 0x3FFFFFFF;
             }
           }
         }
       }
-      if (bsc > best_sc) {
+      root[0] = rt[0];
+      root[P] = rt[1];
+      root[2 * P] = rt[2];
+      root[3 * P] = rt[3];
+      if (bsc > best_sc || (bsc == best_sc && bsc > IMPOSSIBLE && (j < best_j || (j == best_j && d < best_d)))) {
         best_sc = bsc;
         best_j = j;
         best_d = d;
@@ -338,6 +403,7 @@ __global__ __launch_bounds__(TRQ_THREADS) void trqdb_kernel(TrqDev q, TrqTabs<T>
         }
       }
     }
+    __syncthreads();
   }
   // the window's best root score: the first maximum in (j, d) order
   s_sc[tid] = best_sc;
@@ -372,10 +438,10 @@ hipError_t launch_trqdb_scan(const TrqDev &q, const void *tabs, int inside, cons
                              hipStream_t s) {
   if (n == 0) return hipSuccess;
   if (inside)
-    hipLaunchKernelGGL(trqdb_kernel<int32_t>, dim3((unsigned)n), dim3(TRQ_THREADS), 0, s, q,
+    hipLaunchKernelGGL(trqdb_kernel<int32_t>, dim3((unsigned)n), dim3(64 * q.nw), 0, s, q,
                        *(const TrqTabs<int32_t> *)tabs, wins, cutoff, cand, cap, counter, summ);
   else
-    hipLaunchKernelGGL(trqdb_kernel<float>, dim3((unsigned)n), dim3(TRQ_THREADS), 0, s, q,
+    hipLaunchKernelGGL(trqdb_kernel<float>, dim3((unsigned)n), dim3(64 * q.nw), 0, s, q,
                        *(const TrqTabs<float> *)tabs, wins, cutoff, cand, cap, counter, summ);
   return hipGetLastError();
 }
