@@ -263,7 +263,7 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
   }
   if (cells_out) *cells_out = total_cells;
   roots.assign(n, {});
-  const int64_t budget = (int64_t)1 << 28;  // deck floats per launch group (1 GiB)
+  const int64_t budget = arena_elems(c, S_CM_ALPHA);  // deck floats per launch group
   int64_t k0 = 0;
   while (k0 < n) {
     int64_t k1 = k0, floats = 0;

@@ -40,6 +40,7 @@ int scratch(ixg_ctx *c, int slot, size_t n, void **out);
 // (contents not preserved; a slot is reused only after the stream has
 // drained past its previous copy).
 enum PinSlot { P_PACK, P_UP, P_DOWN, P_NSLOTS };
+int64_t arena_elems(const ixg_ctx *c, int slot);
 int pinned(ixg_ctx *c, int slot, size_t n, void **out);
 
 // Runs f(k) for k in [0, n) on up to 16 host threads, in chunks of 4 items

@@ -480,7 +480,7 @@ int qdb_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
   QdbTabs<int32_t> ti{qv.itsc, qv.ioesc, qv.iendsc, qv.ibeginsc, qv.ibase0, qv.iel, qv.ibbase0};
   const void *tabs = inside ? (const void *)&ti : (const void *)&tf;
   const int64_t n = (int64_t)wins.size();
-  const int64_t budget = (int64_t)1 << 28;  // row and deck elements per launch group (1 GiB)
+  const int64_t budget = arena_elems(c, S_CM_ALPHA);  // row and deck elements per launch group
   int64_t k0 = 0;
   while (k0 < n) {
     int64_t k1 = k0, elems = 0, res = 0;
@@ -708,7 +708,7 @@ int trq_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
   const void *tabs = inside ? (const void *)&ti : (const void *)&tf;
   const int64_t n = (int64_t)wins.size();
   summ.assign(n, TrqSummary{IMPOSSIBLE, IXG_TRMODE_UNKNOWN, IMPOSSIBLE});
-  const int64_t budget = (int64_t)1 << 28;  // plane and deck elements per launch group (1 GiB)
+  const int64_t budget = arena_elems(c, S_CM_ALPHA);  // plane and deck elements per launch group
   int64_t k0 = 0;
   while (k0 < n) {
     int64_t k1 = k0, elems = 0, res = 0;
