@@ -416,6 +416,13 @@ int qdb_tables(const ixg_cm_params &p, CmHost &h) {
       g_err = "ixg_cm_create: a bifurcation's left child is not the start state of a BEGL node";
       return IXG_EINVAL;
     }
+  // an insert state's first child is itself (its self transition), as the
+  // scanners' serial IL and IR walks assume
+  for (int v = 0; v < M; v++)
+    if ((p.sttype[v] == IL_ST || p.sttype[v] == IR_ST) && (p.cnum[v] < 1 || p.cfirst[v] != v)) {
+      g_err = "ixg_cm_create: an insert state's first child is not itself";
+      return IXG_EINVAL;
+    }
   // el_sca[d] = el_self added d times (cm_nohmm.rs:466-470)
   h.qfel.assign(h.W + 1, 0.f);
   h.qiel.assign(h.W + 1, 0);

@@ -164,8 +164,11 @@ struct QdbCand {  // a root cell at or above the reporting cutoff
   int32_t win, j, d;
   float sc;
 };
+// maxlen: the longest window of the launch (windows up to 32768 residues are
+// staged in LDS)
 hipError_t launch_qdb_scan(const QdbDev &q, const void *tabs, int inside, const QdbWinDev *wins, int64_t n,
-                           float cutoff, QdbCand *cand, int64_t cap, unsigned long long *counter, hipStream_t s);
+                           float cutoff, QdbCand *cand, int64_t cap, unsigned long long *counter, int maxlen,
+                           hipStream_t s);
 
 // The QDB / non-banded truncated scanners (generic_tr_scan).
 template <typename T>
@@ -206,6 +209,6 @@ struct TrqSummary {  // per window: best root score (first in (j, d) order), its
 };
 hipError_t launch_trqdb_scan(const TrqDev &q, const void *tabs, int inside, const QdbWinDev *wins, int64_t n,
                              float cutoff, TrqCand *cand, int64_t cap, unsigned long long *counter, TrqSummary *summ,
-                             hipStream_t s);
+                             int maxlen, hipStream_t s);
 
 }  // namespace ixg

@@ -207,12 +207,13 @@ void qdb_schedule(const CmHost &h, const std::vector<int32_t> &bands, int local,
       need[y] = std::max(need[y], lvl[p] - lvl[y] + dl + 1);
     }
   }
+  // (the standard scanners' IL states keep one scratch row past the ring)
   std::vector<int32_t> ring(2 * (size_t)M), row_state;
   for (int v = 0; v < M; v++) {
     const int r = pow2_at_least(need[v]);
     ring[2 * v] = (int32_t)row_state.size();
     ring[2 * v + 1] = r - 1;
-    row_state.insert(row_state.end(), r, v);
+    row_state.insert(row_state.end(), r + (!tbegin && h.sttype[v] == IL_ST ? 1 : 0), v);
   }
   s.nrows = (int)row_state.size();
   std::vector<int32_t> deck(2 * (size_t)std::max(nbegl, 1), 0);
@@ -513,6 +514,8 @@ int qdb_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
     }
     HIPCHK(hipMemcpyAsync(d_w, wd.data(), wd.size() * sizeof(QdbWinDev), hipMemcpyHostToDevice, c->stream));
     int64_t cap = res / 8 + 4096;
+    int maxlen = 0;
+    for (int64_t k = k0; k < k1; k++) maxlen = std::max(maxlen, wins[k].j0 - wins[k].i0 + 1);
     std::vector<QdbCand> hc;
     for (int attempt = 0; attempt < 2; attempt++) {
       QdbCand *d_c;
@@ -522,7 +525,7 @@ int qdb_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
       static const unsigned long long zero = 0;
       HIPCHK(hipMemcpyAsync(d_cnt, &zero, 8, hipMemcpyHostToDevice, c->stream));
       HIPCHK(ktime_begin(c));
-      HIPCHK(launch_qdb_scan(q, tabs, inside, d_w, k1 - k0, cutoff, d_c, cap, d_cnt, c->stream));
+      HIPCHK(launch_qdb_scan(q, tabs, inside, d_w, k1 - k0, cutoff, d_c, cap, d_cnt, maxlen, c->stream));
       HIPCHK(ktime_end(c, IXG_K_QDB_SCAN));
       unsigned long long cnt = 0;
       HIPCHK(hipMemcpy(&cnt, d_cnt, 8, hipMemcpyDeviceToHost));
@@ -742,6 +745,8 @@ int trq_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
     }
     HIPCHK(hipMemcpyAsync(d_w, wd.data(), wd.size() * sizeof(QdbWinDev), hipMemcpyHostToDevice, c->stream));
     int64_t cap = res / 8 + 4096;
+    int maxlen = 0;
+    for (int64_t k = k0; k < k1; k++) maxlen = std::max(maxlen, wins[k].j0 - wins[k].i0 + 1);
     std::vector<TrqCand> hc;
     for (int attempt = 0; attempt < 2; attempt++) {
       TrqCand *d_c;
@@ -751,7 +756,7 @@ int trq_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
       static const unsigned long long zero = 0;
       HIPCHK(hipMemcpyAsync(d_cnt, &zero, 8, hipMemcpyHostToDevice, c->stream));
       HIPCHK(ktime_begin(c));
-      HIPCHK(launch_trqdb_scan(q, tabs, inside, d_w, k1 - k0, cutoff, d_c, cap, d_cnt, d_s, c->stream));
+      HIPCHK(launch_trqdb_scan(q, tabs, inside, d_w, k1 - k0, cutoff, d_c, cap, d_cnt, d_s, maxlen, c->stream));
       HIPCHK(ktime_end(c, IXG_K_TRQDB_SCAN));
       unsigned long long cnt = 0;
       HIPCHK(hipMemcpy(&cnt, d_cnt, 8, hipMemcpyDeviceToHost));

@@ -112,6 +112,22 @@ __device__ __forceinline__ size_t ring_at(int32_t ring, int j, int W1) {
   return (size_t)(base + (j & mask)) * W1;
 }
 
+// The window's residues: staged in LDS when the window fits, else read from
+// the device sequence.
+constexpr int QDB_SEQ_LDS = 32768;
+template <bool LDS>
+struct Seq {
+  const uint8_t *g;
+  const uint8_t *l;
+  int i0;
+  __device__ __forceinline__ int operator[](int i) const { return LDS ? (int)l[i - i0] : (int)g[i]; }
+};
+
+// an IL state's scratch row: the row just past its ring
+__device__ __forceinline__ size_t ring_scratch(int32_t ring) {
+  return (size_t)((ring & ((1 << RING_SHIFT) - 1)) + (1 << (ring >> RING_SHIFT)));
+}
+
 __device__ __forceinline__ QdbUnit load_unit(const QdbUnit *u) {
   const int4 *p = (const int4 *)u;
   QdbUnit r;
@@ -125,9 +141,9 @@ __device__ __forceinline__ QdbUnit load_unit(const QdbUnit *u) {
 
 // One cell (j, d) of a state other than IL and the root; returns its value
 // (for the local-begin rows).
-template <typename T>
+template <typename T, bool LS>
 __device__ __forceinline__ T qdb_cell(const QdbTabs<T> &tb, const uint16_t *tbl, T *__restrict__ rows,
-                                      T *__restrict__ deck, const uint8_t *__restrict__ dsq, int W1,
+                                      T *__restrict__ deck, const Seq<LS> &dsq, int W1,
                                       const QdbUnit &u, int st, int cn, int j, int d) {
   using R = Sr<T>;
   const int v = u.code & 0x0FFFFFFF;
@@ -168,42 +184,107 @@ __device__ __forceinline__ T qdb_cell(const QdbTabs<T> &tb, const uint16_t *tbl,
   return sc;
 }
 
-// The d row dn..dx of an IL state at j: the self transition reads (j, d-1),
-// carried in a register along the row (cm_nohmm.rs:617-647).
+// reduce_c_order with the first child left out: every unrolled order above
+// combines t[0] last, so reduce_c(cn, t, init) == comb(reduce_c_rest(...), t[0]).
 template <typename T>
-__device__ void qdb_il_row(const QdbTabs<T> &tb, const uint16_t *tbl, T *__restrict__ rows,
-                           const uint8_t *__restrict__ dsq, int W1, const QdbUnit &u, int cn, int j, int dn, int dx) {
+__device__ __forceinline__ T reduce_c_rest(int cn, const T *t, T init, const uint16_t *tbl) {
   using R = Sr<T>;
+  T s;
+  switch (cn) {
+    case 1: return init;
+    case 2: return R::comb(t[1], init, tbl);
+    case 3:
+      s = R::comb(t[2], t[1], tbl);
+      return R::comb(s, init, tbl);
+    case 4:
+      s = R::comb(t[3], t[2], tbl);
+      s = R::comb(s, init, tbl);
+      return R::comb(s, t[1], tbl);
+    case 5:
+      s = R::comb(t[4], t[3], tbl);
+      s = R::comb(s, init, tbl);
+      s = R::comb(s, t[1], tbl);
+      return R::comb(s, t[2], tbl);
+    default:  // 6
+      s = R::comb(t[5], init, tbl);
+      s = R::comb(s, t[4], tbl);
+      s = R::comb(s, t[3], tbl);
+      s = R::comb(s, t[2], tbl);
+      return R::comb(s, t[1], tbl);
+  }
+}
+
+// The d row dn..dx of an IL state at j (cm_nohmm.rs:617-647). Its first
+// child is itself, read at (j, d-1), and reduce_c_order combines that term
+// last; so a first pass, with no dependence between d's, folds the other
+// children, the local end and the emission into the row and a scratch row,
+// and a second pass adds the self term along d with nothing but that chain.
+template <typename T, bool LS>
+__device__ void qdb_il_row(const QdbTabs<T> &tb, const uint16_t *tbl, T *__restrict__ rows, const Seq<LS> &dsq,
+                           int W1, const QdbUnit &u, int cn, int j, int dn, int dx) {
+  using R = Sr<T>;
+  constexpr int B1 = 4, B2 = 8;  // d's whose loads are issued together
   const int v = u.code & 0x0FFFFFFF;
   const T *ts = tb.tsc + v * MAXC;
   const T *oe = tb.oesc + (size_t)v * (KP * KP);
   T *out = rows + ring_at(u.ring, j, W1);
+  T *scr = rows + ring_scratch(u.ring) * W1;
   const T *cr[MAXC];
   T tsr[MAXC];
 #pragma unroll
-  for (int yo = 0; yo < MAXC; yo++)
+  for (int yo = 1; yo < MAXC; yo++)
     if (yo < cn) {
       cr[yo] = rows + ring_at(u.cring[yo], j, W1);
       tsr[yo] = ts[yo];
     }
-  const int self = v - u.y0;  // the self transition's child index
-  const T es = tb.endsc[v];
-  T prev = out[dn - 1];
-#pragma unroll 2
-  for (int d = dn; d <= dx; d++) {
-    T terms[MAXC];
+  const T es = tb.endsc[v], tself = ts[0];
+  for (int d0 = dn; d0 <= dx; d0 += B1) {
+    T c[MAXC][B1], el[B1], e[B1];
+    int x[B1];
 #pragma unroll
-    for (int yo = 0; yo < MAXC; yo++)
-      if (yo < cn) terms[yo] = R::add(yo == self ? prev : cr[yo][d - 1], tsr[yo]);
-    const T init = R::possible(es) ? R::add(tb.el_sca[d - 1], es) : R::zero();
-    T sc = reduce_c<T>(cn, terms, init, tbl);
-    sc = R::add(sc, oe[dsq[j - d + 1]]);
-    out[d] = sc;
-    prev = sc;
+    for (int k = 0; k < B1; k++) {
+      const int d = min(d0 + k, dx);
+#pragma unroll
+      for (int yo = 1; yo < MAXC; yo++)
+        if (yo < cn) c[yo][k] = cr[yo][d - 1];
+      el[k] = tb.el_sca[d - 1];
+      x[k] = dsq[j - d + 1];
+    }
+#pragma unroll
+    for (int k = 0; k < B1; k++) e[k] = oe[x[k]];
+#pragma unroll
+    for (int k = 0; k < B1; k++)
+      if (d0 + k <= dx) {
+        T terms[MAXC];
+        terms[0] = R::zero();
+#pragma unroll
+        for (int yo = 1; yo < MAXC; yo++)
+          if (yo < cn) terms[yo] = R::add(c[yo][k], tsr[yo]);
+        const T init = R::possible(es) ? R::add(el[k], es) : R::zero();
+        out[d0 + k] = reduce_c_rest<T>(cn, terms, init, tbl);
+        scr[d0 + k] = e[k];
+      }
+  }
+  T prev = out[dn - 1];
+  for (int d0 = dn; d0 <= dx; d0 += B2) {
+    T a[B2], e[B2];
+#pragma unroll
+    for (int k = 0; k < B2; k++) {
+      const int d = min(d0 + k, dx);
+      a[k] = out[d];
+      e[k] = scr[d];
+    }
+#pragma unroll
+    for (int k = 0; k < B2; k++)
+      if (d0 + k <= dx) {
+        const T sc = R::add(R::comb(a[k], R::add(prev, tself), tbl), e[k]);
+        out[d0 + k] = sc;
+        prev = sc;
+      }
   }
 }
 
-template <typename T>
+template <typename T, bool LS>
 __global__ __launch_bounds__(QDB_MAX_THREADS) void qdb_kernel(QdbDev q, QdbTabs<T> tb, const QdbWinDev *wins,
                                                               float cutoff, QdbCand *cand, int64_t cap,
                                                               unsigned long long *counter) {
@@ -214,7 +295,15 @@ __global__ __launch_bounds__(QDB_MAX_THREADS) void qdb_kernel(QdbDev q, QdbTabs<
   T *rows = (T *)wd.rows;
   T *deck = rows + (size_t)q.nrows * W1;
   T *acc = deck + ((size_t)q.nbegl * W1 + q.deck_extra) * W1;
-  const uint8_t *dsq = wd.dsq;
+  Seq<LS> dsq;
+  dsq.g = wd.dsq;
+  dsq.i0 = wd.i0;
+  dsq.l = nullptr;
+  if constexpr (LS) {
+    __shared__ uint8_t s_seq[QDB_SEQ_LDS];
+    for (int x = tid; x < wd.j0 - wd.i0 + 1; x += nt) s_seq[x] = wd.dsq[wd.i0 + x];
+    dsq.l = s_seq;
+  }
   const uint16_t *tbl = nullptr;
   if constexpr (sizeof(T) == 4 && !std::is_same<T, float>::value) {
     __shared__ uint16_t s_ilog[FLOGSUM_TBL];
@@ -258,12 +347,12 @@ __global__ __launch_bounds__(QDB_MAX_THREADS) void qdb_kernel(QdbDev q, QdbTabs<
       const int st = u.info & 15, cn = (u.info >> 4) & 15;
       if (kind == QU_IL) {
         const int dx = min(jrow, u.y1);
-        if (dx >= d) qdb_il_row<T>(tb, tbl, rows, dsq, W1, u, cn, j, d, dx);
+        if (dx >= d) qdb_il_row<T, LS>(tb, tbl, rows, dsq, W1, u, cn, j, d, dx);
         continue;
       }
       if (d > jrow) continue;
       if (kind == QU_CELL) {
-        const T sc = qdb_cell<T>(tb, tbl, rows, deck, dsq, W1, u, st, cn, j, d);
+        const T sc = qdb_cell<T, LS>(tb, tbl, rows, deck, dsq, W1, u, st, cn, j, d);
         const int slot = (u.info >> 8) - 1;
         if (q.local && slot >= 0) {  // this begin's candidate for the root at (j, d)
           T &a = acc[((size_t)(j & q.amask) * q.nslot + slot) * W1 + d];
@@ -303,14 +392,20 @@ __global__ __launch_bounds__(QDB_MAX_THREADS) void qdb_kernel(QdbDev q, QdbTabs<
 }  // namespace
 
 hipError_t launch_qdb_scan(const QdbDev &q, const void *tabs, int inside, const QdbWinDev *wins, int64_t n,
-                           float cutoff, QdbCand *cand, int64_t cap, unsigned long long *counter, hipStream_t s) {
+                           float cutoff, QdbCand *cand, int64_t cap, unsigned long long *counter, int maxlen,
+                           hipStream_t s) {
   if (n == 0) return hipSuccess;
-  if (inside)
-    hipLaunchKernelGGL(qdb_kernel<int32_t>, dim3((unsigned)n), dim3(64 * q.nw), 0, s, q,
-                       *(const QdbTabs<int32_t> *)tabs, wins, cutoff, cand, cap, counter);
-  else
-    hipLaunchKernelGGL(qdb_kernel<float>, dim3((unsigned)n), dim3(64 * q.nw), 0, s, q,
-                       *(const QdbTabs<float> *)tabs, wins, cutoff, cand, cap, counter);
+  const dim3 grid((unsigned)n), block(64 * q.nw);
+  const bool ls = maxlen <= QDB_SEQ_LDS;
+  if (inside) {
+    const QdbTabs<int32_t> &t = *(const QdbTabs<int32_t> *)tabs;
+    if (ls) hipLaunchKernelGGL((qdb_kernel<int32_t, true>), grid, block, 0, s, q, t, wins, cutoff, cand, cap, counter);
+    else hipLaunchKernelGGL((qdb_kernel<int32_t, false>), grid, block, 0, s, q, t, wins, cutoff, cand, cap, counter);
+  } else {
+    const QdbTabs<float> &t = *(const QdbTabs<float> *)tabs;
+    if (ls) hipLaunchKernelGGL((qdb_kernel<float, true>), grid, block, 0, s, q, t, wins, cutoff, cand, cap, counter);
+    else hipLaunchKernelGGL((qdb_kernel<float, false>), grid, block, 0, s, q, t, wins, cutoff, cand, cap, counter);
+  }
   return hipGetLastError();
 }
 

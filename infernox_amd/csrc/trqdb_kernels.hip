@@ -65,6 +65,17 @@ __device__ __forceinline__ size_t tring_at(int32_t ring, int j, int W1) {
   return (size_t)(base + (j & mask)) * 4 * W1;
 }
 
+// The window's residues: staged in LDS when the window fits, else read from
+// the device sequence.
+constexpr int TRQ_SEQ_LDS = 32768;
+template <bool LDS>
+struct TSeq {
+  const uint8_t *g;
+  const uint8_t *l;
+  int i0;
+  __device__ __forceinline__ int operator[](int i) const { return LDS ? (int)l[i - i0] : (int)g[i]; }
+};
+
 __device__ __forceinline__ QdbUnit load_unit(const QdbUnit *u) {
   const int4 *p = (const int4 *)u;
   QdbUnit r;
@@ -87,9 +98,9 @@ __device__ __forceinline__ int t_sd(int st) { return st == MP_ST ? 2 : ((st >= M
 // One cell (j, d) of a state other than IL and the root, every filled plane
 // (cm_dpsearch_trunc.rs:1978-2217). P: plane stride (W1); cj / cp: each
 // child's row of j and of j - 1.
-template <typename T>
+template <typename T, bool LS>
 __device__ void trq_cell(const TrqDev &q, const TrqTabs<T> &tb, const uint16_t *tbl, T *__restrict__ rows,
-                         T *__restrict__ dk, const uint8_t *__restrict__ dsq, int W1, const QdbUnit &u, int st,
+                         T *__restrict__ dk, const TSeq<LS> &dsq, int W1, const QdbUnit &u, int st,
                          int cn, int j, int d) {
   using R = Tq<T>;
   const bool fl = q.fill & 2, fr = q.fill & 4, ft = q.fill & 8;
@@ -229,11 +240,15 @@ __device__ void trq_cell(const TrqDev &q, const TrqTabs<T> &tb, const uint16_t *
 }
 
 // The row d = 1..dx of an IL state at j: its J and L cells read the same
-// state at (j, d-1), carried in registers along the row (:2068-2106).
-template <typename T>
+// state at (j, d-1), carried in registers along the row (:2068-2106). The R
+// plane does not read the state itself and is filled first, with no
+// dependence between d's; the J/L walk then fetches four d's of its other
+// inputs at a time.
+template <typename T, bool LS>
 __device__ void trq_il_row(const TrqDev &q, const TrqTabs<T> &tb, const uint16_t *tbl, T *__restrict__ rows,
-                           const uint8_t *__restrict__ dsq, int W1, const QdbUnit &u, int cn, int j, int dx) {
+                           const TSeq<LS> &dsq, int W1, const QdbUnit &u, int cn, int j, int dx) {
   using R = Tq<T>;
+  constexpr int B = 4;
   const bool fl = q.fill & 2, fr = q.fill & 4;
   const T z = R::zero();
   const int v = u.code & 0x0FFFFFFF;
@@ -249,36 +264,73 @@ __device__ void trq_il_row(const TrqDev &q, const TrqTabs<T> &tb, const uint16_t
       cj[yo] = rows + tring_at(u.cring[yo], j, W1);
       tsr[yo] = ts[yo];
     }
-  const int self = v - u.y0;  // the self transition's child index
-  T pj = out[0], pl = out[P];  // the state's own J and L at (j, d-1)
-#pragma unroll 2
-  for (int d = 1; d <= dx; d++) {
-    T jsc = tinit(tb, v, d - 1);
-    T lsc = fl ? jsc : z;
+  if (fr)
+    for (int d0 = 1; d0 <= dx; d0 += B) {
+      T a[MAXC][B], b[MAXC][B], in[B];
 #pragma unroll
-    for (int yo = 0; yo < MAXC; yo++)
-      if (yo < cn) {
-        jsc = R::comb(jsc, R::add(yo == self ? pj : cj[yo][d - 1], tsr[yo]), tbl);
-        if (fl) lsc = R::comb(lsc, R::add(yo == self ? pl : cj[yo][P + d - 1], tsr[yo]), tbl);
+      for (int k = 0; k < B; k++) {
+        const int d = min(d0 + k, dx);
+#pragma unroll
+        for (int yo = 1; yo < MAXC; yo++)
+          if (yo < cn) {
+            a[yo][k] = cj[yo][d];
+            b[yo][k] = cj[yo][2 * P + d];
+          }
+        in[k] = tinit(tb, v, d);
       }
-    const T e = oe[dsq[j - d + 1]];
-    pj = R::add(jsc, e);
-    out[d] = pj;
-    if (fl) {
-      pl = d >= 2 ? R::add(lsc, e) : e;
-      out[P + d] = pl;
+#pragma unroll
+      for (int k = 0; k < B; k++)
+        if (d0 + k <= dx) {
+          T rsc = in[k];
+#pragma unroll
+          for (int yo = 1; yo < MAXC; yo++)
+            if (yo < cn) rsc = R::comb(rsc, R::max2(R::add(a[yo][k], tsr[yo]), R::add(b[yo][k], tsr[yo])), tbl);
+          out[2 * P + d0 + k] = rsc;
+        }
     }
-    if (fr) {
-      T rsc = tinit(tb, v, d);
+  T pj = out[0], pl = out[P];  // the state's own J and L at (j, d-1)
+  for (int d0 = 1; d0 <= dx; d0 += B) {
+    T cJ[MAXC][B], cL[MAXC][B], in[B], e[B];
+    int x[B];
+#pragma unroll
+    for (int k = 0; k < B; k++) {
+      const int d = min(d0 + k, dx);
 #pragma unroll
       for (int yo = 1; yo < MAXC; yo++)
-        if (yo < cn) rsc = R::comb(rsc, R::max2(R::add(cj[yo][d], tsr[yo]), R::add(cj[yo][2 * P + d], tsr[yo])), tbl);
-      out[2 * P + d] = rsc;
+        if (yo < cn) {
+          cJ[yo][k] = cj[yo][d - 1];
+          if (fl) cL[yo][k] = cj[yo][P + d - 1];
+        }
+      in[k] = tinit(tb, v, d - 1);
+      x[k] = dsq[j - d + 1];
     }
+#pragma unroll
+    for (int k = 0; k < B; k++) e[k] = oe[x[k]];
+#pragma unroll
+    for (int k = 0; k < B; k++)
+      if (d0 + k <= dx) {
+        const int d = d0 + k;
+        T jsc = in[k];
+        T lsc = fl ? jsc : z;
+        jsc = R::comb(jsc, R::add(pj, tsr[0]), tbl);
+        if (fl) lsc = R::comb(lsc, R::add(pl, tsr[0]), tbl);
+#pragma unroll
+        for (int yo = 1; yo < MAXC; yo++)
+          if (yo < cn) {
+            jsc = R::comb(jsc, R::add(cJ[yo][k], tsr[yo]), tbl);
+            if (fl) lsc = R::comb(lsc, R::add(cL[yo][k], tsr[yo]), tbl);
+          }
+        pj = R::add(jsc, e[k]);
+        out[d] = pj;
+        if (fl) {
+          pl = d >= 2 ? R::add(lsc, e[k]) : e[k];
+          out[P + d] = pl;
+        }
+      }
   }
 }
 
-template <typename T>
+template <typename T, bool LS>
 __global__ __launch_bounds__(QDB_MAX_THREADS) void trqdb_kernel(TrqDev q, TrqTabs<T> tb, const QdbWinDev *wins,
                                                                 float cutoff, TrqCand *cand, int64_t cap,
                                                                 unsigned long long *counter, TrqSummary *summ) {
@@ -291,7 +343,15 @@ __global__ __launch_bounds__(QDB_MAX_THREADS) void trqdb_kernel(TrqDev q, TrqTab
   const int W1 = wd.w + 1;
   const size_t P = W1;
   T *rows = (T *)wd.rows, *dk = (T *)wd.begl;
-  const uint8_t *dsq = wd.dsq;
+  TSeq<LS> dsq;
+  dsq.g = wd.dsq;
+  dsq.i0 = wd.i0;
+  dsq.l = nullptr;
+  if constexpr (LS) {
+    __shared__ uint8_t s_seq[TRQ_SEQ_LDS];
+    for (int x = tid; x < wd.j0 - wd.i0 + 1; x += nt) s_seq[x] = wd.dsq[wd.i0 + x];
+    dsq.l = s_seq;
+  }
   const uint16_t *tbl = nullptr;
   if constexpr (!std::is_same<T, float>::value) {
     __shared__ uint16_t s_ilog[FLOGSUM_TBL];
@@ -341,12 +401,12 @@ __global__ __launch_bounds__(QDB_MAX_THREADS) void trqdb_kernel(TrqDev q, TrqTab
       const int st = u.info & 15, cn = (u.info >> 4) & 15;
       if (kind == QU_IL) {
         const int dx = min(jrow, u.y1);
-        if (dx >= 1) trq_il_row<T>(q, tb, tbl, rows, dsq, W1, u, cn, j, dx);
+        if (dx >= 1) trq_il_row<T, LS>(q, tb, tbl, rows, dsq, W1, u, cn, j, dx);
         continue;
       }
       if (d > jrow) continue;
       if (kind == QU_CELL) {
-        trq_cell<T>(q, tb, tbl, rows, dk, dsq, W1, u, st, cn, j, d);
+        trq_cell<T, LS>(q, tb, tbl, rows, dk, dsq, W1, u, st, cn, j, d);
         continue;
       }
       // the root: truncated begins into the persistent root rows (:2220-2289),
@@ -435,14 +495,23 @@ __global__ __launch_bounds__(QDB_MAX_THREADS) void trqdb_kernel(TrqDev q, TrqTab
 
 hipError_t launch_trqdb_scan(const TrqDev &q, const void *tabs, int inside, const QdbWinDev *wins, int64_t n,
                              float cutoff, TrqCand *cand, int64_t cap, unsigned long long *counter, TrqSummary *summ,
-                             hipStream_t s) {
+                             int maxlen, hipStream_t s) {
   if (n == 0) return hipSuccess;
-  if (inside)
-    hipLaunchKernelGGL(trqdb_kernel<int32_t>, dim3((unsigned)n), dim3(64 * q.nw), 0, s, q,
-                       *(const TrqTabs<int32_t> *)tabs, wins, cutoff, cand, cap, counter, summ);
-  else
-    hipLaunchKernelGGL(trqdb_kernel<float>, dim3((unsigned)n), dim3(64 * q.nw), 0, s, q,
-                       *(const TrqTabs<float> *)tabs, wins, cutoff, cand, cap, counter, summ);
+  const dim3 grid((unsigned)n), block(64 * q.nw);
+  const bool ls = maxlen <= TRQ_SEQ_LDS;
+  if (inside) {
+    const TrqTabs<int32_t> &t = *(const TrqTabs<int32_t> *)tabs;
+    if (ls)
+      hipLaunchKernelGGL((trqdb_kernel<int32_t, true>), grid, block, 0, s, q, t, wins, cutoff, cand, cap, counter, summ);
+    else
+      hipLaunchKernelGGL((trqdb_kernel<int32_t, false>), grid, block, 0, s, q, t, wins, cutoff, cand, cap, counter, summ);
+  } else {
+    const TrqTabs<float> &t = *(const TrqTabs<float> *)tabs;
+    if (ls)
+      hipLaunchKernelGGL((trqdb_kernel<float, true>), grid, block, 0, s, q, t, wins, cutoff, cand, cap, counter, summ);
+    else
+      hipLaunchKernelGGL((trqdb_kernel<float, false>), grid, block, 0, s, q, t, wins, cutoff, cand, cap, counter, summ);
+  }
   return hipGetLastError();
 }
 
