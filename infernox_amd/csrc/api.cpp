@@ -184,6 +184,7 @@ int ixg_init(int device, size_t mem_budget, ixg_ctx **out) {
   }
   for (auto &ev : c->ev) HIPCHK(hipEventCreate(&ev));
   for (auto &ev : c->kev) HIPCHK(hipEventCreate(&ev));
+  for (auto &ev : c->sev) HIPCHK(hipEventCreate(&ev));
   *out = c;
   return IXG_OK;
 }
@@ -197,6 +198,7 @@ int ixg_shutdown(ixg_ctx *c) {
     if (h) (void)hipHostFree(h);
   for (auto &ev : c->ev) (void)hipEventDestroy(ev);
   for (auto &ev : c->kev) (void)hipEventDestroy(ev);
+  for (auto &ev : c->sev) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return IXG_OK;

@@ -282,21 +282,6 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
     const int64_t pack_n = at[k1 - k0];
     int32_t *pack;
     RCHK(pinned(c, P_PACK, (size_t)pack_n * 4, (void **)&pack));
-    par_for(k1 - k0, [&](int64_t kk) {
-      const int64_t k = k0 + kk;
-      const ixg_bands &b = env[k].bands;
-      const int nr = plan[k].nrows;
-      int32_t *o = pack + at[kk];
-      o = std::copy(b.jmin, b.jmin + M, o);
-      o = std::copy(b.jmax, b.jmax + M, o);
-      o = std::copy(b.hd_off, b.hd_off + M + 1, o);
-      o = std::copy(b.hdmin, b.hdmin + nr, o);
-      o = std::copy(b.hdmax, b.hdmax + nr, o);
-      o = std::copy(plan[k].roff.begin(), plan[k].roff.begin() + nr, o);
-      if (trunc)
-        for (int v = 0; v < M; v++) *o++ = mode_mask(valid[k], v, tr->fill);
-      append_units(cm->h, b, trunc ? &valid[k] : nullptr, trunc ? tr->fill : 0, o);
-    });
     int32_t *d_pack;
     float *d_alpha, *d_rout;
     EnvDev *d_env;
@@ -327,22 +312,50 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
       e.j0 = env[k].j0;
       aoff += planes * plan[k].ncells;
     }
-    HIPCHK(hipMemcpyAsync(d_pack, pack, (size_t)pack_n * 4, hipMemcpyHostToDevice, c->stream));
     HIPCHK(hipMemcpyAsync(d_env, ed.data(), ed.size() * sizeof(EnvDev), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipEventRecord(c->ev[5], c->stream));
+    // the band upload in parts of about equal size: the host packs part s+1
+    // (into its own part of the pinned buffer) while part s is copied
+    constexpr int NSUB = 4;
+    std::vector<int64_t> sub{0};
+    for (int64_t kk = 0; kk < k1 - k0; kk++)
+      if ((int)sub.size() < NSUB && at[kk + 1] * NSUB >= pack_n * (int64_t)sub.size() && kk + 1 < k1 - k0)
+        sub.push_back(kk + 1);
+    sub.push_back(k1 - k0);
+    for (size_t g = 0; g + 1 < sub.size(); g++) {
+      const int64_t a0 = sub[g], a1 = sub[g + 1];
+      par_for(a1 - a0, [&](int64_t t) {
+        const int64_t kk = a0 + t, k = k0 + kk;
+        const ixg_bands &b = env[k].bands;
+        const int nr = plan[k].nrows;
+        int32_t *o = pack + at[kk];
+        o = std::copy(b.jmin, b.jmin + M, o);
+        o = std::copy(b.jmax, b.jmax + M, o);
+        o = std::copy(b.hd_off, b.hd_off + M + 1, o);
+        o = std::copy(b.hdmin, b.hdmin + nr, o);
+        o = std::copy(b.hdmax, b.hdmax + nr, o);
+        o = std::copy(plan[k].roff.begin(), plan[k].roff.begin() + nr, o);
+        if (trunc)
+          for (int v = 0; v < M; v++) *o++ = mode_mask(valid[k], v, tr->fill);
+        append_units(cm->h, b, trunc ? &valid[k] : nullptr, trunc ? tr->fill : 0, o);
+      });
+      HIPCHK(hipMemcpyAsync(d_pack + at[a0], pack + at[a0], (size_t)(at[a1] - at[a0]) * 4, hipMemcpyHostToDevice,
+                            c->stream));
+    }
+    HIPCHK(hipEventRecord(c->sev[0], c->stream));
     if (trunc)
       HIPCHK(launch_trhb_scan(cm->dev(), d_env, k1 - k0, cm->d_ftbl(), tr->d_pty, tr->d_yl, tr->ny, inside,
                               c->stream));
     else
       HIPCHK(launch_hb_scan(cm->dev(), d_env, k1 - k0, cm->d_ftbl(), inside, c->stream));
-    HIPCHK(hipEventRecord(c->ev[6], c->stream));
-    std::vector<float> rhost(std::max<int64_t>(rtotal, 1));
-    if (rtotal) HIPCHK(hipMemcpyAsync(rhost.data(), d_rout, rtotal * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipEventRecord(c->sev[1], c->stream));
+    float *rhost;
+    RCHK(pinned(c, P_DOWN, (size_t)std::max<int64_t>(rtotal, 1) * 4, (void **)&rhost));
+    if (rtotal) HIPCHK(hipMemcpyAsync(rhost, d_rout, rtotal * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
     for (int64_t k = k0; k < k1; k++)
-      roots[k].assign(rhost.begin() + rat[k - k0], rhost.begin() + rat[k - k0] + rper * plan[k].nroot);
+      roots[k].assign(rhost + rat[k - k0], rhost + rat[k - k0] + rper * plan[k].nroot);
     float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev[5], c->ev[6]));
+    HIPCHK(hipEventElapsedTime(&ms, c->sev[0], c->sev[1]));
     c->ms[2] += ms;
     c->kms[trunc ? IXG_K_TRHB_SCAN : IXG_K_HB_SCAN] += ms;
     c->kn[trunc ? IXG_K_TRHB_SCAN : IXG_K_HB_SCAN] += 1;

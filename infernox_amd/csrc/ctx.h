@@ -85,6 +85,7 @@ struct ixg_ctx {
   // per-kernel device time (HIP events around each launch), cumulative:
   // IXG_K_* index; kev is the event pair used for it
   hipEvent_t kev[2];
+  hipEvent_t sev[2];  // scanner batch kernel: start/end
   double kms[IXG_NKERNELS] = {};
   int64_t kn[IXG_NKERNELS] = {};
 };

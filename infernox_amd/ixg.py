@@ -611,16 +611,36 @@ def _trhits(hits, n):
             for k in range(n)]
 
 
+class EnvBatch:
+    """The ixg_envelope (or ixg_trenvelope) array of a list of envelopes,
+    marshalled once so that repeated scans of the same batch pass it as is:
+    (i0, j0, strand, bands dict) or, truncated, (..., validity dict)."""
+
+    def __init__(self, envs, trunc=False):
+        self.n, self.trunc = len(envs), trunc
+        self.arr = ((TrEnvelope if trunc else Envelope) * max(self.n, 1))()
+        self.keep = []
+        for k, e in enumerate(envs):
+            i0, j0, strand, b = e[:4]
+            a = self.arr[k]
+            a.i0, a.j0, a.strand, a.pad = int(i0), int(j0), int(strand), 0
+            bs = bands_struct(b)  # holds the arrays its pointers refer to
+            a.bands = bs
+            self.keep.append(bs)
+            if trunc:
+                vs = valid_struct(e[4])
+                a.valid = vs
+                self.keep.append(vs)
+
+    def __len__(self):
+        return self.n
+
+
 def hb_scan_batch(ctx, cm, tgt, envs, inside=False, env_cutoff=0.0, report=False, cutoff=0.0, null3=True):
-    """envs: list of (i0, j0, strand, bands dict). Returns (vsc, envi, envj, hits, cells)."""
-    n = len(envs)
-    arr = (Envelope * max(n, 1))()
-    keep = []
-    for k, (i0, j0, strand, b) in enumerate(envs):
-        bs = bands_struct(b)
-        keep.append(bs)
-        arr[k].i0, arr[k].j0, arr[k].strand, arr[k].pad = int(i0), int(j0), int(strand), 0
-        arr[k].bands = bs
+    """envs: list of (i0, j0, strand, bands dict), or an EnvBatch of them.
+    Returns (vsc, envi, envj, hits, cells)."""
+    batch = envs if isinstance(envs, EnvBatch) else EnvBatch(envs)
+    n, arr = batch.n, batch.arr
     vsc = np.zeros(max(n, 1), np.float32)
     ei = np.zeros(max(n, 1), np.int64)
     ej = np.zeros(max(n, 1), np.int64)
@@ -635,17 +655,11 @@ def hb_scan_batch(ctx, cm, tgt, envs, inside=False, env_cutoff=0.0, report=False
 
 
 def trhb_scan_batch(ctx, cm, tgt, envs, pass_idx, local=True, inside=False, env_cutoff=0.0, cutoff=0.0, null3=True):
-    """envs: list of (i0, j0, strand, bands dict, validity dict). Returns
+    """envs: list of (i0, j0, strand, bands dict, validity dict), or an
+    EnvBatch(..., trunc=True) of them. Returns
     (vsc, vmode, envi, envj, hits (env, i, j, score, bias, mode, root), cells)."""
-    n = len(envs)
-    arr = (TrEnvelope * max(n, 1))()
-    keep = []
-    for k, (i0, j0, strand, b, v) in enumerate(envs):
-        bs, vs = bands_struct(b), valid_struct(v)
-        keep.append((bs, vs))
-        arr[k].i0, arr[k].j0, arr[k].strand, arr[k].pad = int(i0), int(j0), int(strand), 0
-        arr[k].bands = bs
-        arr[k].valid = vs
+    batch = envs if isinstance(envs, EnvBatch) else EnvBatch(envs, trunc=True)
+    n, arr = batch.n, batch.arr
     vsc = np.zeros(max(n, 1), np.float32)
     vm = np.zeros(max(n, 1), np.int32)
     ei = np.zeros(max(n, 1), np.int64)
