@@ -60,11 +60,12 @@ def cm_workload(n_env=512, r=8, seed=3):
     return cm, dsq, L, envs
 
 
-def time_cm(ctx, steps, warmup):
-    cm, dsq, L, envs = cm_workload()
+def time_cm(ctx, steps, warmup, cpu=None):
+    cm, dsq, L, envl = cm_workload()
     g = ixg.CM(ctx, cm)
     tgt = ixg.Target(ctx, dsq, L)
-    envs = ixg.EnvBatch(envs)  # marshalled once, as a caller holding its band arrays would
+    envs = ixg.EnvBatch(envl)  # marshalled once, as a caller holding its band arrays would
+    work = np.array([hb_work(cm, e[3]) for e in envl], np.int64).sum(0)
     out = {}
     for name, inside in (("cyk", False), ("inside", True)):
         for _ in range(warmup):
@@ -75,9 +76,23 @@ def time_cm(ctx, steps, warmup):
             kms += ctx.timings()[0][2]
         wall = (time.perf_counter() - t0) / steps
         kms /= steps
+        c, u, sp = (int(x) for x in work)
+        # CYK: add + max per child update and per split, emission + clamp per cell; Inside: ~9 per update
+        ops = (9 * (u + sp) + 2 * c) if inside else (2 * u + 2 * c + 2 * sp)
         out[name] = {"gcups": cells / (kms * 1e-3) / 1e9, "gcups_wall": cells / wall / 1e9, "kernel_ms": kms,
-                     "wall_ms": wall * 1e3, "cells": int(cells), "envelopes": len(envs),
-                     "states": int(cm["M"]), "positive": int((vsc > 0).sum())}
+                     "wall_ms": wall * 1e3, "cells": int(cells), "updates": u, "b_splits": sp,
+                     "envelopes": len(envs), "states": int(cm["M"]), "positive": int((vsc > 0).sum()),
+                     "valu_roofline": valu_line(ops, kms)}
+        if cpu:
+            ob = oracle_cm_bind()
+
+            def run(o, e, inside=inside):
+                r = o.scan(e[3], dsq, int(e[0]), int(e[1]), int(inside))
+                if inside:
+                    o.hits(e[3], dsq, int(e[0]), int(e[1]), r["root"], 0.0)
+                return o.ncells(e[3])
+            done, w, dt = cpu_rate(lambda: ob.OracleCM(cm), run, envl, *cpu)
+            out[name]["cpu"] = cpu_line(done, len(envl), w, dt, cpu[1], "envelopes")
     tgt.free()
     g.free()
     return out
@@ -130,14 +145,101 @@ def filled_cells(cm, envs):
     return tot
 
 
-def time_trcm(ctx, steps, warmup):
-    cm, dsq, L, envs = trcm_workload()
+VALU_PEAK = 39.3216  # T lane-ops/s: 256 CU x 4 SIMD16 x 16 lanes x 2.4 GHz
+
+
+def hb_work(cm, b, val=None):
+    """Algorithmic work of one banded scan (the counts the per-unit op figures
+    multiply): cells, child updates (cells x children, per filled plane when
+    val is given) and bifurcation split terms (standard scan only), from the
+    bands as the reference's loops visit them (cp9.rs:4818-5253)."""
+    M = int(cm["M"])
+    st, cfirst, cnum = cm["sttype"], cm["cfirst"], cm["cnum"]
+    hmn, hmx, off = b["hdmin"], b["hdmax"], b["hd_off"]
+    w = np.maximum(0, hmx - hmn + 1)
+    per_v = np.add.reduceat(np.concatenate([w, [0]]), off[:M]) if len(w) else np.zeros(M, np.int64)
+    per_v = np.where(off[1:M + 1] > off[:M], per_v, 0).astype(np.int64)
+    live = (st != synth_cm.E_ST) & (st != synth_cm.B_ST)
+    if val is not None:
+        planes = (val["jvalid"].astype(np.int64) + val["lvalid"] + val["rvalid"] +
+                  ((st == synth_cm.B_ST) & (val["tvalid"] > 0)))
+        cells = int((per_v * planes)[1:].sum())
+        upd = int((per_v * planes * np.where(live, cnum, 0))[1:].sum())
+        return cells, upd, 0
+    cells = int(per_v.sum())
+    upd = int((per_v * np.where(live, cnum, 0)).sum())
+    splits = 0
+    for v in np.nonzero(st == synth_cm.B_ST)[0]:
+        y, z = int(cfirst[v]), int(cnum[v])
+        jn, jx = max(b["jmin"][v], b["jmin"][z]), min(b["jmax"][v], b["jmax"][z])
+        if jx < jn:
+            continue
+        j = np.arange(jn, jx + 1)
+        jpv, jpy, jpz = j - b["jmin"][v], j - b["jmin"][y], j - b["jmin"][z]
+        dn, dx = hmn[off[v] + jpv], hmx[off[v] + jpv]
+        kn = np.maximum(np.maximum(j - b["jmax"][y], hmn[off[z] + jpz]), 0)
+        kx = np.minimum(jpy, hmx[off[z] + jpz])
+        nd, nk = int(max(0, (dx - dn).max() + 1)), int(max(0, (kx - kn).max() + 1))
+        if nd == 0 or nk == 0:
+            continue
+        d = dn[:, None, None] + np.arange(nd)[None, :, None]
+        k = kn[:, None, None] + np.arange(nk)[None, None, :]
+        r = np.clip(jpy[:, None, None] - k, 0, off[y + 1] - off[y] - 1)
+        ok = ((d <= dx[:, None, None]) & (k <= kx[:, None, None]) & (k >= d - hmx[off[y] + r]) &
+              (k <= d - hmn[off[y] + r]) & (jpy[:, None, None] - k >= 0))
+        splits += int(ok.sum())
+    return cells, upd, splits
+
+
+def valu_line(ops, ms):
+    """VALU roofline entry: algorithmic lane-ops of one launch over its time."""
+    t = ops / (ms * 1e-3) / 1e12
+    return {"ops": int(ops), "achieved_tops": t, "peak_tops": VALU_PEAK, "frac": t / VALU_PEAK}
+
+
+def cpu_rate(make, run, items, budget_s, threads):
+    """Oracle (C restatement, not infernox) rate over as many items as fit in
+    budget_s on `threads` threads (one oracle object per thread; ctypes
+    releases the GIL): (items done, work done, seconds). run(o, item) returns
+    the item's work units."""
+    import threading
+    from concurrent.futures import ThreadPoolExecutor
+    tl = threading.local()
+
+    def one(it):
+        if not hasattr(tl, "o"):
+            tl.o = make()
+        return run(tl.o, it)
+
+    done, work, t0 = 0, 0, time.time()
+    with ThreadPoolExecutor(threads) as ex:
+        while done < len(items) and time.time() - t0 < budget_s:
+            batch = items[done:done + threads]
+            work += sum(ex.map(one, batch))
+            done += len(batch)
+    return done, work, time.time() - t0
+
+
+def cpu_line(done, total, work, dt, threads, what):
+    return {"gcups": work / dt / 1e9, "cores": threads, "kind": "port",
+            "sample": f"{done} of {total} {what}, oracle (C restatement, not infernox), {threads} threads, {dt:.1f} s"}
+
+
+def oracle_cm_bind():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_cm_bind
+    return oracle_cm_bind
+
+
+def time_trcm(ctx, steps, warmup, cpu=None):
+    cm, dsq, L, envl = trcm_workload()
     g = ixg.CM(ctx, cm)
     gp, lp = synth_cm.tr_penalties(cm, 3)
     g.set_trpenalties(gp, lp)
     tgt = ixg.Target(ctx, dsq, L)
-    cells = filled_cells(cm, envs)
-    envs = ixg.EnvBatch(envs, trunc=True)  # marshalled once, as a caller holding its band arrays would
+    cells = filled_cells(cm, envl)
+    envs = ixg.EnvBatch(envl, trunc=True)  # marshalled once, as a caller holding its band arrays would
+    work = np.array([hb_work(cm, e[3], e[4]) for e in envl], np.int64).sum(0)
     out = {}
     for name, inside in (("trcyk", False), ("trinside", True)):
         for _ in range(warmup):
@@ -149,15 +251,26 @@ def time_trcm(ctx, steps, warmup):
             kms += ctx.timings()[0][2]
         wall = (time.perf_counter() - t0) / steps
         kms /= steps
+        c, u = int(work[0]), int(work[1])
+        ops = (9 * u + 2 * c) if inside else (2 * u + 2 * c)  # bifurcation splits not counted: a lower bound
         out[name] = {"gcups": cells / (kms * 1e-3) / 1e9, "gcups_wall": cells / wall / 1e9, "kernel_ms": kms,
-                     "wall_ms": wall * 1e3, "plane_cells": int(cells), "envelopes": len(envs),
-                     "positive": int((vsc > 0).sum()), "hits": len(hits)}
+                     "wall_ms": wall * 1e3, "plane_cells": int(cells), "plane_updates": u, "envelopes": len(envs),
+                     "positive": int((vsc > 0).sum()), "hits": len(hits), "valu_roofline": valu_line(ops, kms)}
+        if cpu:
+            ob = oracle_cm_bind()
+            pty = lp[ob.PTY_INDEX[synth_cm.PASS_5P_AND_3P_FORCE]]
+
+            def run(o, e, inside=inside):
+                o.trscan(e[3], e[4], pty, dsq, int(e[0]), int(e[1]), synth_cm.PASS_5P_AND_3P_FORCE, True, int(inside))
+                return filled_cells(cm, [e])
+            done, w, dt = cpu_rate(lambda: ob.OracleTrCM(cm), run, envl, *cpu)
+            out[name]["cpu"] = cpu_line(done, len(envl), w, dt, cpu[1], "envelopes (plane cells)")
     tgt.free()
     g.free()
     return out
 
 
-def qdb_workload(n_win=48, seed=4, slack=8):
+def qdb_workload(n_win=512, seed=4, slack=8):
     """QDB / non-banded scanner workload: the 120-column, 3-stem CM of the
     sequence-database config, n_win windows each holding one emitted homolog
     with 20 random flanking residues per side (both strands alternate), and
@@ -200,14 +313,49 @@ def qdb_cells(cm, wins, dmin, dmax, trunc=False):
     return n
 
 
-def time_qdb(ctx, steps):
-    cm, dsq, L, wins, dmin, dmax = qdb_workload()
+def qdb_work(cm, wins, dmin, dmax):
+    """(cells, child updates, bifurcation split terms) of a QDB / non-banded
+    scan (cm_nohmm.rs:582-616): per window and j, state v covers d from
+    max(dmin, emissions) to min(jrow, dmax, W); a bifurcation cell d splits
+    over k in [max(0, dmin_right, d - dmax_left), min(dmax_right, d - dmin_left)]."""
+    st, cf, cn = np.asarray(cm["sttype"]), np.asarray(cm["cfirst"]), np.asarray(cm["cnum"])
+    W, M = int(cm["W"]), int(cm["M"])
+    base = np.where(st == synth_cm.MP_ST, 2, 1)
+    dmn = np.zeros(M, np.int64) if dmin is None else np.asarray(dmin, np.int64)
+    dxc = np.minimum(W, W if dmax is None else np.asarray(dmax, np.int64)) * np.ones(M, np.int64)
+    dn = np.minimum(np.maximum(dmn, base), W)
+    dmw = np.minimum(dmn, W)
+    live = (st != synth_cm.E_ST)
+    live[0] = False
+    r = np.arange(W + 1)[:, None]
+    cells_r = np.maximum(0, np.minimum(r, dxc[None, :]) - dn[None, :] + 1) * live[None, :]
+    F_c = cells_r.sum(1)
+    F_u = (cells_r * np.where(st == synth_cm.B_ST, 0, cn)[None, :]).sum(1)
+    F_s = np.zeros(W + 1, np.int64)
+    for v in np.nonzero(st == synth_cm.B_ST)[0]:
+        y, wl = int(cn[v]), int(cf[v])
+        d = np.arange(W + 1)
+        kmin = np.maximum(0, np.maximum(dmw[y], d - dxc[wl]))
+        kmax = np.minimum(dxc[y], d - dmw[wl])
+        sv = np.where((d >= dn[v]) & (d <= dxc[v]), np.maximum(0, kmax - kmin + 1), 0)
+        F_s += np.cumsum(sv)
+    tot = np.zeros(3, np.int64)
+    for i0, j0, _ in wins:
+        L = j0 - i0 + 1
+        jr = np.minimum(np.arange(1, L + 1), min(W, L))
+        tot += [F_c[jr].sum(), F_u[jr].sum(), F_s[jr].sum()]
+    return tuple(int(x) for x in tot)
+
+
+def time_qdb(ctx, steps, cpu=None):
+    cm, dsq, L, wins, dmin, dmax = qdb_workload(n_win=512)
     tgt = ixg.Target(ctx, dsq, L)
     out = {}
     g = ixg.CM(ctx, cm, local=True)
     for name, inside, band in (("cyk_qdb", False, True), ("inside_qdb", True, True), ("cyk_nonbanded", False, False)):
         dmn, dmx = (dmin, dmax) if band else (None, None)
         cells = qdb_cells(cm, wins, dmn, dmx)
+        c, u, sp = qdb_work(cm, wins, dmn, dmx)
         ixg.qdb_scan_batch(ctx, g, tgt, wins, inside=inside, dmin=dmn, dmax=dmx, cutoff=0.0)
         ctx.kernel_timings(reset=True)
         t0 = time.perf_counter()
@@ -215,12 +363,28 @@ def time_qdb(ctx, steps):
             hits = ixg.qdb_scan_batch(ctx, g, tgt, wins, inside=inside, dmin=dmn, dmax=dmx, cutoff=0.0)
         wall = (time.perf_counter() - t0) / steps
         kms = ctx.kernel_timings(reset=True)["qdb_scan"][0] / steps
+        # CYK: add + max per child update and per split, emission + clamp per
+        # cell; integer Inside: ~7 int ops + 1 table read per update
+        ops = (7 * (u + sp) + 2 * c) if inside else (2 * u + 2 * c + 2 * sp)
         out[name] = {"gcups": cells / (kms * 1e-3) / 1e9, "kernel_ms": kms, "wall_ms": wall * 1e3,
-                     "cells": cells, "windows": len(wins), "states": int(cm["M"]), "hits": len(hits)}
+                     "cells": cells, "updates": u, "b_splits": sp, "windows": len(wins), "states": int(cm["M"]),
+                     "hits": len(hits), "valu_roofline": valu_line(ops, kms)}
+        if cpu:
+            ob = oracle_cm_bind()
+
+            def run(o, w, inside=inside, dmn=dmn, dmx=dmx):
+                o.scan(dsq, int(w[0]), int(w[1]), int(inside), dmn, dmx, cutoff=0.0)
+                return qdb_cells(cm, [w], dmn, dmx)
+            done, wk, dt = cpu_rate(lambda: ob.OracleQdbCM(cm, local=True), run, wins, *cpu)
+            out[name]["cpu"] = cpu_line(done, len(wins), wk, dt, cpu[1], "windows")
     gp, lp = synth_cm.tr_penalties(cm, 4)
     ig, il = synth_cm.tr_ipenalties(cm, 4)
     g.set_trpenalties(gp, lp)
     g.set_itrpenalties(ig, il)
+    # the truncated scanners (a test oracle in the reference) on the first 256
+    # windows: their rows for the root's begin states take ~80 MB per window,
+    # so the batch runs in launch groups of ~100 windows
+    wins = wins[:256]
     cells = qdb_cells(cm, wins, None, None, trunc=True)
     for name, inside in (("trcyk_nonbanded", False), ("trinside_nonbanded", True)):
         ixg.trqdb_scan_batch(ctx, g, tgt, wins, synth_cm.PASS_5P_AND_3P_FORCE, inside=inside, cutoff=0.0)
@@ -233,12 +397,20 @@ def time_qdb(ctx, steps):
         kms = ctx.kernel_timings(reset=True)["trqdb_scan"][0] / steps
         out[name] = {"gcups_per_plane": cells / (kms * 1e-3) / 1e9, "kernel_ms": kms, "wall_ms": wall * 1e3,
                      "plane_cells": cells, "windows": len(wins), "pass": "5'&3'", "hits": len(hits)}
+        if cpu:
+            ob = oracle_cm_bind()
+
+            def run(o, w, inside=inside):
+                o.trscan(dsq, int(w[0]), int(w[1]), int(inside), synth_cm.PASS_5P_AND_3P_FORCE, True, cutoff=0.0)
+                return qdb_cells(cm, [w], None, None, trunc=True)
+            done, wk, dt = cpu_rate(lambda: ob.OracleTrQdbCM(cm, local=True, seed=4), run, wins, *cpu)
+            out[name]["cpu"] = cpu_line(done, len(wins), wk, dt, cpu[1], "windows (per-plane cells)")
     g.free()
     tgt.free()
     return out
 
 
-def time_p7_forward(ctx, p, d, steps, n=256, length=400):
+def time_p7_forward(ctx, p, d, steps, n=1024, length=400, cpu=None):
     """Full-matrix Forward (retained rows, multihit) over n windows of the
     genome: kernel time and DP cells per second."""
     g = ixg.P7(ctx, p, 400)
@@ -253,8 +425,20 @@ def time_p7_forward(ctx, p, d, steps, n=256, length=400):
     kms = ctx.kernel_timings(reset=True)["fwd_full"][0] / steps
     cells = n * length * int(p["M"])
     g.free()
-    return {"gcups": cells / (kms * 1e-3) / 1e9, "kernel_ms": kms, "wall_ms": wall * 1e3, "windows": n,
-            "length": length, "M": int(p["M"]), "matrix_MB": 3 * n * (length + 1) * (int(p["M"]) + 1) * 4 / 1e6}
+    out = {"gcups": cells / (kms * 1e-3) / 1e9, "kernel_ms": kms, "wall_ms": wall * 1e3, "windows": n,
+           "length": length, "M": int(p["M"]), "matrix_MB": 3 * n * (length + 1) * (int(p["M"]) + 1) * 4 / 1e6,
+           "valu_roofline": valu_line(16 * cells, kms)}  # ~16 FP32 ops per (i, k)
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_bind import Oracle
+
+        def run(o, r):
+            sub = np.concatenate([[255], seq[r[0] + 1:r[0] + 1 + r[1]], [255]]).astype(np.uint8)
+            o.p7_forward(sub, r[1])
+            return r[1] * int(p["M"])
+        done, wk, dt = cpu_rate(lambda: Oracle(p, 400), run, regions, *cpu)
+        out["cpu"] = cpu_line(done, n, wk, dt, cpu[1], "windows")
+    return out
 
 
 def cpu_baseline(p, d, L, M, maxw, tasks, nt, budget_s, threads):
@@ -297,6 +481,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-section-seconds", type=float, default=2.0,
+                    help="oracle time budget per scanner section (CPU rate beside each GPU rate)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cm", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
@@ -336,10 +522,11 @@ def main():
         n_f3b = sum(accts[t].n_past_fwdbias for t in range(nt))
         tgt.free()
         g.free()
-        cm_line = time_cm(ctx, max(1, a.steps // 2), 1) if not a.no_cm else None
-        tr_line = time_trcm(ctx, max(1, a.steps // 2), 1) if not a.no_cm else None
-        qdb_line = time_qdb(ctx, 2) if not a.no_cm else None
-        pf_line = time_p7_forward(ctx, p, d, 2) if not a.no_cm else None
+        cpu = None if a.no_cpu else (a.cpu_section_seconds, a.cpu_threads)
+        cm_line = time_cm(ctx, max(1, a.steps // 2), 1, cpu) if not a.no_cm else None
+        tr_line = time_trcm(ctx, max(1, a.steps // 2), 1, cpu) if not a.no_cm else None
+        qdb_line = time_qdb(ctx, 2, cpu) if not a.no_cm else None
+        pf_line = time_p7_forward(ctx, p, d, 2, cpu=cpu) if not a.no_cm else None
     # dominant kernel with a per-unit op count, and its VALU roofline: ops per
     # launch (the last step's cells x ops per cell, from the reference loop
     # bodies) over the kernel's mean launch time (HIP events around each launch)

@@ -278,7 +278,7 @@ void qdb_schedule(const CmHost &h, const std::vector<int32_t> &bands, int local,
     }
   }
   for (int d = dn[0], b = std::min(W, (int)dx[0]); d <= b; d++)
-    us.push_back({0, s.lroot, 0, d, tbegin ? ntb * 4 * 3 + M : h.cnum[0] + 4 + s.nslot, QU_ROOT << 28});
+    us.push_back({0, s.lroot, 0, d, tbegin ? ntb * 4 * 12 : h.cnum[0] + 4 + s.nslot, QU_ROOT << 28});
   std::sort(us.begin(), us.end(), [](const U &x, const U &y) {
     if (x.cls != y.cls) return x.cls < y.cls;
     if (x.lvl != y.lvl) return x.lvl < y.lvl;

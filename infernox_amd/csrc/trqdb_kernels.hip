@@ -415,25 +415,36 @@ __global__ __launch_bounds__(QDB_MAX_THREADS) void trqdb_kernel(TrqDev q, TrqTab
       int br = 0, bm = MODE_UNKNOWN;
       T *root = rows + tring_at(u.ring, j, W1) + d;
       T rt[4] = {root[0], root[P], root[2 * P], root[3 * P]};
-#pragma unroll 4
-      for (int e = 0; e < q.ntb; e++) {
-        const int4 te = q.tblist[e];
-        if (d > min(jrow, te.z)) continue;
-        T tp;
-        memcpy(&tp, &te.y, 4);
-        const T *yc = rows + tring_at(te.x, j, W1) + d;
-        const int ny = (ft && (te.w >> 30)) ? 4 : 3;
+      constexpr int TB = 8;  // begin states whose cells are fetched together
+      for (int e0 = 0; e0 < q.ntb; e0 += TB) {
+        T val[TB][4];
+        int4 tev[TB];
 #pragma unroll
-        for (int pl = 0; pl < 4; pl++) {
-          if (pl >= ny || (pl == 1 && !fl) || (pl == 2 && !fr)) continue;
-          const T sc = R::add(yc[pl * P], tp);
-          if (sc > rt[pl]) {
-            rt[pl] = sc;
-            const float fsc = R::bits(sc);
-            if (fsc > bsc) {
-              bsc = fsc;
-              bm = pl == 0 ? MODE_J : pl == 1 ? MODE_L : pl == 2 ? MODE_R : MODE_T;
-              br = te.w & 0x3FFFFFFF;
+        for (int k = 0; k < TB; k++) {
+          tev[k] = q.tblist[min(e0 + k, q.ntb - 1)];
+          const T *yc = rows + tring_at(tev[k].x, j, W1) + d;
+#pragma unroll
+          for (int pl = 0; pl < 4; pl++) val[k][pl] = yc[pl * P];
+        }
+#pragma unroll
+        for (int k = 0; k < TB; k++) {
+          const int4 te = tev[k];
+          if (e0 + k >= q.ntb || d > min(jrow, te.z)) continue;
+          T tp;
+          memcpy(&tp, &te.y, 4);
+          const int ny = (ft && (te.w >> 30)) ? 4 : 3;
+#pragma unroll
+          for (int pl = 0; pl < 4; pl++) {
+            if (pl >= ny || (pl == 1 && !fl) || (pl == 2 && !fr)) continue;
+            const T sc = R::add(val[k][pl], tp);
+            if (sc > rt[pl]) {
+              rt[pl] = sc;
+              const float fsc = R::bits(sc);
+              if (fsc > bsc) {
+                bsc = fsc;
+                bm = pl == 0 ? MODE_J : pl == 1 ? MODE_L : pl == 2 ? MODE_R : MODE_T;
+                br = te.w & 0x3FFFFFFF;
+              }
             }
           }
         }
