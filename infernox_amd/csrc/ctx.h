@@ -88,6 +88,9 @@ struct ixg_ctx {
   hipEvent_t sev[2];  // scanner batch kernel: start/end
   double kms[IXG_NKERNELS] = {};
   int64_t kn[IXG_NKERNELS] = {};
+  // candidate root cells per residue seen by the last QDB scans (sizes the
+  // next call's candidate list so that it rarely has to be rerun)
+  double qdb_cand_rate = 0.125, trq_cand_rate = 0.125;
 };
 
 namespace ixg {

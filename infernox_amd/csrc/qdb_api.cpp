@@ -513,7 +513,7 @@ int qdb_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
       x.begl = nullptr;
     }
     HIPCHK(hipMemcpyAsync(d_w, wd.data(), wd.size() * sizeof(QdbWinDev), hipMemcpyHostToDevice, c->stream));
-    int64_t cap = res / 8 + 4096;
+    int64_t cap = (int64_t)((double)res * c->qdb_cand_rate * 1.25) + 4096;
     int maxlen = 0;
     for (int64_t k = k0; k < k1; k++) maxlen = std::max(maxlen, wins[k].j0 - wins[k].i0 + 1);
     std::vector<QdbCand> hc;
@@ -529,6 +529,7 @@ int qdb_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
       HIPCHK(ktime_end(c, IXG_K_QDB_SCAN));
       unsigned long long cnt = 0;
       HIPCHK(hipMemcpy(&cnt, d_cnt, 8, hipMemcpyDeviceToHost));
+      c->qdb_cand_rate = std::max(c->qdb_cand_rate, (double)cnt / (double)std::max<int64_t>(res, 1));
       if ((int64_t)cnt > cap) {
         cap = (int64_t)cnt + 1024;
         continue;
@@ -744,7 +745,7 @@ int trq_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
       x.begl = (int32_t *)d_mx + at[k - k0] + 4 * (int64_t)sc.nrows * W1;
     }
     HIPCHK(hipMemcpyAsync(d_w, wd.data(), wd.size() * sizeof(QdbWinDev), hipMemcpyHostToDevice, c->stream));
-    int64_t cap = res / 8 + 4096;
+    int64_t cap = (int64_t)((double)res * c->trq_cand_rate * 1.25) + 4096;
     int maxlen = 0;
     for (int64_t k = k0; k < k1; k++) maxlen = std::max(maxlen, wins[k].j0 - wins[k].i0 + 1);
     std::vector<TrqCand> hc;
@@ -760,6 +761,7 @@ int trq_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
       HIPCHK(ktime_end(c, IXG_K_TRQDB_SCAN));
       unsigned long long cnt = 0;
       HIPCHK(hipMemcpy(&cnt, d_cnt, 8, hipMemcpyDeviceToHost));
+      c->trq_cand_rate = std::max(c->trq_cand_rate, (double)cnt / (double)std::max<int64_t>(res, 1));
       if ((int64_t)cnt > cap) {
         cap = (int64_t)cnt + 1024;
         continue;
