@@ -99,8 +99,6 @@ void root_summary(const ixg_bands &b, const float *root, int32_t i0, int32_t j0,
   envj = ej == (int64_t)i0 - 1 ? -1 : ej;
 }
 
-double sre_log2(double x) { return x > 0.0 ? std::log(x) * 1.44269504 : -1e36; }
-
 float kahan(const float *v, int n) {
   float sum = 0.f, c = 0.f;
   for (int i = 0; i < n; i++) {

@@ -236,7 +236,7 @@ __global__ __launch_bounds__(HB_THREADS) void hb_scan_kernel(CmDev cm, const Env
     const int st = cm.sttype[v];
     const int jmn = e.jmin[v], nj = e.jmax[v] - jmn + 1;
     const int ho = e.hd_off[v];
-    const int32_t *hmn = e.hdmin + ho, *hmx = e.hdmax + ho, *ro = e.roff + ho;
+    const int32_t *hmn = e.hdmin + ho, *ro = e.roff + ho;
     if (st == IL_ST) {
       il_row<INSIDE>(cm, e, v, u, ftbl);
     } else if (st == IR_ST) {

@@ -147,6 +147,28 @@ int64_t ixo_cyk_seq_filter(const ixo_cm *cm, const ixo_cmsc *fs, const int32_t *
                            const uint8_t *dsq, int n, float cutoff, int do_null3, int32_t *es, int32_t *ee,
                            int64_t cap);
 
+/* A configured CP9 HMM (cp9.rs:871-896): integer scores in the DP's order.
+ * otsc [(M+1)][12] in cp9O order MM IM DM BM MI II DI MD ID DD ME MEL;
+ * msc/isc [(M+1)][18]; EL: has_el [M+1], and per node k = 0..M+1 the
+ * el_from_ct[k] source nodes el_from_idx[el_from_off[k] + c]. */
+typedef struct {
+  int M;
+  int el; /* CPLAN9_EL set */
+  int32_t el_selfsc;
+  const int32_t *otsc, *msc, *isc;
+  const uint8_t *has_el;
+  const int32_t *el_from_ct, *el_from_off, *el_from_idx;
+} ixo_cp9;
+
+float ixo_cp9_forward(const ixo_cp9 *h, const uint8_t *dsq, int i0, int j0, int do_scan, int doing_align,
+                      int32_t *mmx, int32_t *imx, int32_t *dmx, int32_t *elmx, int32_t *sca, int64_t *best_pos);
+float ixo_cp9_backward(const ixo_cp9 *h, const uint8_t *dsq, int i0, int j0, int do_scan, int doing_align,
+                       int32_t *mmx, int32_t *imx, int32_t *dmx, int32_t *elmx, int32_t *sca, int64_t *best_pos);
+int32_t ixo_cp9_band_thresh(double p_thresh);
+void ixo_cp9_fb2hmmbands(const ixo_cp9 *h, const uint8_t *dsq, const int32_t *fm, const int32_t *fi,
+                         const int32_t *fd, const int32_t *bm, const int32_t *bi, const int32_t *bd, int i0, int j0,
+                         double p_thresh, int did_fwd_scan, int32_t *pn);
+
 #ifdef __cplusplus
 }
 #endif
