@@ -351,6 +351,43 @@ int ixg_trqdb_scan_batch(ixg_ctx *ctx, const ixg_cm *cm, const ixg_target *tgt, 
                          float *vsc, int32_t *vmode, float *bsc_full, ixg_trhit *hits, int64_t cap,
                          int64_t *nhits);
 
+/* ---- CP9 HMM banding (cp9.rs:1275 cp9_forward, :1995 cp9_backward, :1517
+ * cp9_fb2hmmbands): the band-producing HMM's integer Forward and Backward
+ * over a window and its posterior HMM bands ---- */
+typedef struct ixg_cp9 ixg_cp9;
+/* A configured CP9 HMM (cp9.rs:871-896 CP9 after cp9_logoddsify): M nodes;
+ * otsc [(M+1)][12] in the reference's cp9O order MM IM DM BM MI II DI MD ID
+ * DD ME MEL (cp9.rs:906-918); msc / isc [(M+1)][18] over the digitized
+ * alphabet; el != 0 when local ends (CPLAN9_EL) are on, with el_selfsc,
+ * has_el [M+1] and, per node k = 0..M+1, the el_from_ct[k] source nodes
+ * el_from_idx[el_from_off[k] + c] (cp9.rs:1042-1076). Scores are scaled
+ * log2-odds x 1000, -987654321 for impossible. */
+typedef struct {
+  int32_t M, el, el_selfsc, pad;
+  const int32_t *otsc, *msc, *isc;
+  const uint8_t *has_el;
+  const int32_t *el_from_ct, *el_from_off, *el_from_idx;
+} ixg_cp9_params;
+/* Uploads the HMM (device-resident until ixg_cp9_free). */
+int ixg_cp9_create(ixg_ctx *ctx, const ixg_cp9_params *p, ixg_cp9 **out);
+int ixg_cp9_free(ixg_ctx *ctx, ixg_cp9 *h);
+/* One window dsq[i0..j0] (1-based, sentinel-padded, length L): Forward and
+ * Backward with do_scan as cp9_seq2bands passes it (1 in search, 0 in
+ * alignment, where doing_align = !do_scan), their matrices (each
+ * [(j0-i0+2)][(M+1)], row ip = position i0+ip-1; any may be NULL), the two
+ * best scores (bits) and the posterior HMM bands at tau: pn [6][M+1] =
+ * pn_min_m, pn_max_m, pn_min_i, pn_max_i, pn_min_d, pn_max_d, -1 where the
+ * state has no band (the CP9Bands fields cp9_fb2hmmbands fills). */
+int ixg_cp9_bands(ixg_ctx *ctx, const ixg_cp9 *h, const uint8_t *dsq, int64_t L, int32_t i0, int32_t j0,
+                  double tau, int do_scan, int32_t *fmx, int32_t *fix, int32_t *fdx, int32_t *bmx, int32_t *bix,
+                  int32_t *bdx, float *fwd_sc, float *bwd_sc, int32_t *pn);
+/* Batched form over a device-resident target (the per-envelope loop that
+ * calls cp9_seq2bands, cm_search.rs:2740 / :2816): pn [n][6][M+1];
+ * fwd_sc / bwd_sc [n] may be NULL. */
+typedef struct { int32_t i0, j0, strand, pad; } ixg_cp9win;
+int ixg_cp9_bands_batch(ixg_ctx *ctx, const ixg_cp9 *h, const ixg_target *tgt, const ixg_cp9win *win, int64_t n,
+                        double tau, int do_scan, float *fwd_sc, float *bwd_sc, int32_t *pn);
+
 /* Per-kernel device time, cumulative since the last reset: HIP events around
  * every launch of each kernel (IXG_K_* index), total milliseconds and launch
  * counts; reset != 0 clears them after reading. A measurement hook with no
@@ -358,7 +395,7 @@ int ixg_trqdb_scan_batch(ixg_ctx *ctx, const ixg_cm *cm, const ixg_target *tgt, 
  * after each launch. */
 enum { IXG_K_SSV_SWEEP = 0, IXG_K_SSV_RESOLVE = 1, IXG_K_FWD = 2, IXG_K_BIAS = 3, IXG_K_VIT = 4, IXG_K_MSV = 5,
        IXG_K_HB_SCAN = 6, IXG_K_TRHB_SCAN = 7, IXG_K_COPY = 8, IXG_K_QDB_SCAN = 9, IXG_K_FWD_FULL = 10,
-       IXG_K_TRQDB_SCAN = 11, IXG_NKERNELS = 12 };
+       IXG_K_TRQDB_SCAN = 11, IXG_K_CP9 = 12, IXG_NKERNELS = 13 };
 int ixg_kernel_timings(ixg_ctx *ctx, double *ms, int64_t *launches, int reset);
 /* Measurement hook (no counterpart in the reference): copies `bytes`
  * device-to-device with a plain kernel reps times after one warm-up copy;

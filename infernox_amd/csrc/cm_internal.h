@@ -211,4 +211,27 @@ hipError_t launch_trqdb_scan(const TrqDev &q, const void *tabs, int inside, cons
                              float cutoff, TrqCand *cand, int64_t cap, unsigned long long *counter, TrqSummary *summ,
                              int maxlen, hipStream_t s);
 
+// ---- CP9 HMM banding (cp9.rs:1275 cp9_forward, :1995 cp9_backward, :1517
+// cp9_fb2hmmbands) ----
+struct Cp9Dev {
+  int M, el, el_selfsc;
+  const int32_t *otsc, *msc, *isc;  // [(M+1)][12] cp9O order, [(M+1)][18], [(M+1)][18]
+  const uint8_t *has_el;            // [M+1]
+  const int32_t *el_ct, *el_off, *el_idx;  // local-end sources per node 0..M+1
+  const int32_t *ilog;                     // ILogsum table
+};
+struct Cp9WinDev {
+  const uint8_t *dsq;
+  int32_t i0, j0;
+  int64_t out;  // the window's index in the caller's order
+};
+// One wave's matrices, interleaved over its 64 lanes ([row][k][lane]):
+// Forward and Backward M/I/D [(lmax+1)][(M+1)][64], the two rolling local-end
+// rows of each [2][(M+1)][64], and the posterior masses [3][(M+1)][64].
+struct Cp9GrpDev {
+  int32_t *fm, *fi, *fd, *bm, *bi, *bd, *fel, *bel, *post;
+};
+hipError_t launch_cp9_bands(const Cp9Dev &h, const Cp9WinDev *wins, int64_t n, const Cp9GrpDev *grps, int do_scan,
+                            int32_t thresh, float *fsc, float *bsc, int32_t *pn, hipStream_t s);
+
 }  // namespace ixg

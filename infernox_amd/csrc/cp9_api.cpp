@@ -1,0 +1,252 @@
+// C ABI of the CP9 HMM banding DPs (cp9.rs:1275 cp9_forward, :1995
+// cp9_backward, :1517 cp9_fb2hmmbands): the resident HMM, windows sorted by
+// length and dealt 64 to a wave, launch groups that fit the matrix budget.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include "cm_handle.h"
+
+using namespace ixg;
+
+struct ixg_cp9 {
+  int M = 0, el = 0, el_selfsc = 0;
+  int32_t *d = nullptr;  // otsc | msc | isc | el_ct | el_off | el_idx | ilog | has_el (bytes)
+  size_t bytes = 0;
+  Cp9Dev dev;
+};
+
+const uint8_t *ixg_target_device_base(const ixg_target *t, int64_t *L);
+
+namespace {
+
+// cp9.rs:977-998 Prob2Score, the band threshold of cp9_fb2hmmbands (:1531)
+int32_t band_thresh(double tau) {
+  const float p = (float)((1.0 - (1.0 - tau)) / 2.0);
+  if (p == 0.0f) return -INFTY_I;
+  const double lg = (double)p > 0.0 ? std::log((double)p) * 1.44269504 : -1e36;
+  const double v = std::floor(0.5 + 1000.0 * lg);
+  if (!std::isfinite(v)) return -INFTY_I;
+  return std::max((int32_t)v, -INFTY_I);
+}
+
+struct CWin {
+  const uint8_t *dsq;  // device
+  int32_t i0, j0;
+};
+
+// Runs Forward, Backward and the posterior bands of every window; fsc, bsc,
+// pn are host arrays in the windows' order. mats (optional, n == 1): the six
+// matrices of the single window in [(L+1)][(M+1)] order.
+int cp9_run(ixg_ctx *c, const ixg_cp9 *h, const std::vector<CWin> &wins, double tau, int do_scan, float *fsc,
+            float *bsc, int32_t *pn, int32_t **mats) {
+  const int M = h->M, M1 = M + 1;
+  const int64_t n = (int64_t)wins.size();
+  if (!(tau > 0.0 && tau < 1.0)) {
+    g_err = "CP9 bands: tau must be in (0, 1)";
+    return IXG_EINVAL;
+  }
+  const int32_t thresh = band_thresh(tau);
+  std::vector<int64_t> order(n);
+  std::iota(order.begin(), order.end(), 0);
+  std::stable_sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+    return wins[a].j0 - wins[a].i0 > wins[b].j0 - wins[b].i0;
+  });
+  const int64_t budget = arena_elems(c, S_CM_ALPHA);
+  std::vector<float> hf(std::max<int64_t>(n, 1)), hb(std::max<int64_t>(n, 1));
+  std::vector<int32_t> hp((size_t)std::max<int64_t>(n, 1) * 6 * M1);
+  int64_t g0 = 0;  // first window (in sorted order) of the launch group
+  while (g0 < n) {
+    // whole waves of 64 windows while the matrices fit
+    int64_t g1 = g0, elems = 0;
+    std::vector<int64_t> goff;
+    while (g1 < n) {
+      const int lmax = wins[order[g1]].j0 - wins[order[g1]].i0 + 1;
+      const int64_t need = (6 * (int64_t)(lmax + 1) * M1 + 4 * (int64_t)M1 + 3 * (int64_t)M1) * 64;
+      if (g1 > g0 && elems + need > budget) break;
+      goff.push_back(elems);
+      elems += need;
+      g1 = std::min(n, g1 + 64);
+    }
+    const int64_t nw = g1 - g0, ng = (int64_t)goff.size();
+    int32_t *d_mx;
+    RCHK(scratch(c, S_CM_ALPHA, (size_t)elems * 4, (void **)&d_mx));
+    std::vector<Cp9WinDev> wd(nw);
+    std::vector<Cp9GrpDev> gd(ng);
+    for (int64_t k = 0; k < nw; k++) {
+      const CWin &w = wins[order[g0 + k]];
+      wd[k] = Cp9WinDev{w.dsq, w.i0, w.j0, k};
+    }
+    for (int64_t g = 0; g < ng; g++) {
+      const int lmax = wins[order[g0 + 64 * g]].j0 - wins[order[g0 + 64 * g]].i0 + 1;
+      const int64_t mat = (int64_t)(lmax + 1) * M1 * 64;
+      int32_t *b = d_mx + goff[g];
+      gd[g] = Cp9GrpDev{b, b + mat, b + 2 * mat, b + 3 * mat, b + 4 * mat, b + 5 * mat, b + 6 * mat,
+                        b + 6 * mat + 2 * (int64_t)M1 * 64, b + 6 * mat + 4 * (int64_t)M1 * 64};
+    }
+    const size_t wbytes = nw * sizeof(Cp9WinDev), gbytes = ng * sizeof(Cp9GrpDev);
+    const size_t obytes = (size_t)nw * (2 * 4 + 6 * M1 * 4);
+    char *d_meta;
+    RCHK(scratch(c, S_CM_ENV, wbytes + gbytes + obytes + 64, (void **)&d_meta));
+    Cp9WinDev *d_w = (Cp9WinDev *)d_meta;
+    Cp9GrpDev *d_g = (Cp9GrpDev *)(d_meta + ((wbytes + 15) & ~(size_t)15));
+    float *d_f = (float *)(d_meta + ((wbytes + 15) & ~(size_t)15) + ((gbytes + 15) & ~(size_t)15));
+    float *d_b = d_f + nw;
+    int32_t *d_pn = (int32_t *)(d_b + nw);
+    HIPCHK(hipMemcpyAsync(d_w, wd.data(), wbytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemcpyAsync(d_g, gd.data(), gbytes, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(ktime_begin(c));
+    HIPCHK(launch_cp9_bands(h->dev, d_w, nw, d_g, do_scan, thresh, d_f, d_b, d_pn, c->stream));
+    HIPCHK(ktime_end(c, IXG_K_CP9));
+    std::vector<float> tf(nw), tb(nw);
+    std::vector<int32_t> tp((size_t)nw * 6 * M1);
+    HIPCHK(hipMemcpy(tf.data(), d_f, nw * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(tb.data(), d_b, nw * 4, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(tp.data(), d_pn, tp.size() * 4, hipMemcpyDeviceToHost));
+    for (int64_t k = 0; k < nw; k++) {
+      const int64_t o = order[g0 + k];
+      hf[o] = tf[k];
+      hb[o] = tb[k];
+      std::copy(tp.begin() + k * 6 * M1, tp.begin() + (k + 1) * 6 * M1, hp.begin() + o * 6 * M1);
+    }
+    if (mats && n == 1) {  // de-interleave lane 0 of the single window's group
+      const int L = wins[0].j0 - wins[0].i0 + 1;
+      const size_t cells = (size_t)(L + 1) * M1;
+      std::vector<int32_t> buf(cells * 64);
+      int32_t *src[6] = {gd[0].fm, gd[0].fi, gd[0].fd, gd[0].bm, gd[0].bi, gd[0].bd};
+      for (int m = 0; m < 6; m++) {
+        if (!mats[m]) continue;
+        HIPCHK(hipMemcpy(buf.data(), src[m], cells * 64 * 4, hipMemcpyDeviceToHost));
+        for (size_t x = 0; x < cells; x++) mats[m][x] = buf[x * 64];
+      }
+    }
+    g0 = g1;
+  }
+  if (fsc) std::copy(hf.begin(), hf.begin() + n, fsc);
+  if (bsc) std::copy(hb.begin(), hb.begin() + n, bsc);
+  if (pn) std::copy(hp.begin(), hp.begin() + n * 6 * M1, pn);
+  return IXG_OK;
+}
+
+int check_cp9_window(const ixg_cp9 *h, const uint8_t *dsq, int64_t L, int32_t i0, int32_t j0) {
+  if (!h || !dsq || i0 < 1 || j0 > L || j0 < i0) {
+    g_err = "CP9 bands: bad arguments (1 <= i0 <= j0 <= L)";
+    return IXG_EINVAL;
+  }
+  for (int64_t i = i0; i <= j0; i++)
+    if (dsq[i] >= KP) {
+      g_err = "digitized residue code >= 18 in the window";
+      return IXG_EINVAL;
+    }
+  return IXG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ixg_cp9_create(ixg_ctx *c, const ixg_cp9_params *p, ixg_cp9 **out) {
+  if (!c || !p || !out || p->M < 1 || !p->otsc || !p->msc || !p->isc || !p->has_el || !p->el_from_ct ||
+      !p->el_from_off || (p->el && !p->el_from_idx)) {
+    g_err = "ixg_cp9_create: bad arguments";
+    return IXG_EINVAL;
+  }
+  const int M = p->M, M1 = M + 1;
+  const int nidx = p->el_from_off[M + 2];
+  for (int k = 0; k <= M + 1; k++)
+    if (p->el_from_ct[k] < 0 || p->el_from_off[k] + p->el_from_ct[k] > nidx) {
+      g_err = "ixg_cp9_create: local-end lists out of range";
+      return IXG_EINVAL;
+    }
+  for (int x = 0; x < nidx; x++)
+    if (p->el_from_idx[x] < 0 || p->el_from_idx[x] > M) {
+      g_err = "ixg_cp9_create: local-end source outside 0..M";
+      return IXG_EINVAL;
+    }
+  std::vector<int32_t> ilog(FLOGSUM_TBL);
+  for (int i = 0; i < FLOGSUM_TBL; i++)  // cp9.rs:28-43 ILogsum table
+    ilog[i] = (int32_t)std::nearbyint(1000.0 * std::log2(1.0 + std::pow(2.0, -(double)i / 1000.0)));
+  std::vector<int32_t> blob;
+  auto put = [&](const int32_t *x, size_t k) {
+    const size_t o = blob.size();
+    blob.insert(blob.end(), x, x + k);
+    while (blob.size() % 4) blob.push_back(0);
+    return o;
+  };
+  const size_t o_t = put(p->otsc, (size_t)M1 * 12), o_m = put(p->msc, (size_t)M1 * KP),
+               o_i = put(p->isc, (size_t)M1 * KP), o_c = put(p->el_from_ct, M + 2), o_o = put(p->el_from_off, M + 3);
+  const int32_t zero = 0;
+  const size_t o_x = nidx ? put(p->el_from_idx, nidx) : put(&zero, 1), o_l = put(ilog.data(), ilog.size());
+  const size_t o_h = blob.size();
+  blob.resize(o_h + (M1 + 3) / 4 + 1, 0);
+  std::memcpy(blob.data() + o_h, p->has_el, M1);
+  ixg_cp9 *h = new ixg_cp9;
+  h->M = M;
+  h->el = p->el ? 1 : 0;
+  h->el_selfsc = p->el_selfsc;
+  h->bytes = blob.size() * 4;
+  if (int r = dmalloc(c, (void **)&h->d, h->bytes)) {
+    delete h;
+    return r;
+  }
+  if (hipMemcpy(h->d, blob.data(), h->bytes, hipMemcpyHostToDevice) != hipSuccess) {
+    dfree(c, h->d, h->bytes);
+    delete h;
+    g_err = "ixg_cp9_create: upload failed";
+    return IXG_EDEVICE;
+  }
+  Cp9Dev &d = h->dev;
+  d.M = M;
+  d.el = h->el;
+  d.el_selfsc = h->el_selfsc;
+  d.otsc = h->d + o_t;
+  d.msc = h->d + o_m;
+  d.isc = h->d + o_i;
+  d.el_ct = h->d + o_c;
+  d.el_off = h->d + o_o;
+  d.el_idx = h->d + o_x;
+  d.ilog = h->d + o_l;
+  d.has_el = (const uint8_t *)(h->d + o_h);
+  *out = h;
+  return IXG_OK;
+}
+
+int ixg_cp9_free(ixg_ctx *c, ixg_cp9 *h) {
+  if (!h) return IXG_OK;
+  if (c) dfree(c, h->d, h->bytes);
+  delete h;
+  return IXG_OK;
+}
+
+int ixg_cp9_bands(ixg_ctx *c, const ixg_cp9 *h, const uint8_t *dsq, int64_t L, int32_t i0, int32_t j0, double tau,
+                  int do_scan, int32_t *fmx, int32_t *fix, int32_t *fdx, int32_t *bmx, int32_t *bix, int32_t *bdx,
+                  float *fwd_sc, float *bwd_sc, int32_t *pn) {
+  if (!c) return IXG_EINVAL;
+  RCHK(check_cp9_window(h, dsq, L, i0, j0));
+  uint8_t *d;
+  RCHK(scratch(c, S_CM_SEQ, (size_t)(L + 2), (void **)&d));
+  HIPCHK(hipMemcpyAsync(d, dsq, (size_t)(L + 2), hipMemcpyHostToDevice, c->stream));
+  int32_t *mats[6] = {fmx, fix, fdx, bmx, bix, bdx};
+  return cp9_run(c, h, {CWin{d, i0, j0}}, tau, do_scan ? 1 : 0, fwd_sc, bwd_sc, pn, mats);
+}
+
+int ixg_cp9_bands_batch(ixg_ctx *c, const ixg_cp9 *h, const ixg_target *tgt, const ixg_cp9win *win, int64_t n,
+                        double tau, int do_scan, float *fwd_sc, float *bwd_sc, int32_t *pn) {
+  if (!c || !h || !tgt || (n > 0 && (!win || !pn)) || n < 0) {
+    g_err = "ixg_cp9_bands_batch: bad arguments";
+    return IXG_EINVAL;
+  }
+  int64_t L;
+  const uint8_t *base = ixg_target_device_base(tgt, &L);
+  std::vector<CWin> w(n);
+  for (int64_t k = 0; k < n; k++) {
+    if (win[k].i0 < 1 || win[k].j0 > L || win[k].j0 < win[k].i0 || (win[k].strand != 0 && win[k].strand != 1)) {
+      g_err = "ixg_cp9_bands_batch: window outside the target";
+      return IXG_EINVAL;
+    }
+    w[k] = CWin{base + (win[k].strand ? L + 2 : 0), win[k].i0, win[k].j0};
+  }
+  return cp9_run(c, h, w, tau, do_scan ? 1 : 0, fwd_sc, bwd_sc, pn, nullptr);
+}
+
+}  // extern "C"

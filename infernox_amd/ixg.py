@@ -60,10 +60,11 @@ EXPORTS = [
     "ixg_trcyk_scan_hb", "ixg_trinside_scan_hb", "ixg_trhb_scan_batch", "ixg_kernel_timings",
     "ixg_copy_probe", "ixg_cm_create_cfg", "ixg_cm_itables", "ixg_qdb_scan", "ixg_qdb_scan_batch",
     "ixg_cyk_seq_filter", "ixg_p7_forward", "ixg_p7_forward_batch", "ixg_cm_set_itrpenalties", "ixg_cm_imtables",
-    "ixg_trqdb_scan", "ixg_trqdb_scan_batch",
+    "ixg_trqdb_scan", "ixg_trqdb_scan_batch", "ixg_cp9_create", "ixg_cp9_free", "ixg_cp9_bands",
+    "ixg_cp9_bands_batch",
 ]
 KERNELS = ["ssv_sweep", "ssv_resolve", "fwd", "bias", "vit", "msv", "hb_scan", "trhb_scan", "copy_probe",
-           "qdb_scan", "fwd_full", "trqdb_scan"]
+           "qdb_scan", "fwd_full", "trqdb_scan", "cp9"]
 
 
 class CmParams(C.Structure):
@@ -708,3 +709,88 @@ def qdb_scan_batch(ctx, cm, tgt, wins, inside=False, dmin=None, dmax=None, cutof
     check(lib().ixg_qdb_scan_batch(ctx.h, cm.h, tgt.h, arr, n, ptr[0], ptr[1], int(inside), float(cutoff), int(null3),
                                    hits, cap, C.byref(nh)), "ixg_qdb_scan_batch")
     return [(hits[k].env, hits[k].i, hits[k].j, hits[k].score, hits[k].bias) for k in range(nh.value)]
+
+
+# ---- CP9 HMM banding (cp9.rs:1275 cp9_forward, :1995 cp9_backward, :1517 cp9_fb2hmmbands) ----
+
+class Cp9Params(C.Structure):
+    _fields_ = [("M", C.c_int32), ("el", C.c_int32), ("el_selfsc", C.c_int32), ("pad", C.c_int32),
+                ("otsc", C.POINTER(C.c_int32)), ("msc", C.POINTER(C.c_int32)), ("isc", C.POINTER(C.c_int32)),
+                ("has_el", C.POINTER(C.c_uint8)), ("el_from_ct", C.POINTER(C.c_int32)),
+                ("el_from_off", C.POINTER(C.c_int32)), ("el_from_idx", C.POINTER(C.c_int32))]
+
+
+class Cp9Win(C.Structure):
+    _fields_ = [("i0", C.c_int32), ("j0", C.c_int32), ("strand", C.c_int32), ("pad", C.c_int32)]
+
+
+def _i32p(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+_cp9_ready = False
+
+
+def _cp9_lib():
+    global _cp9_ready
+    L = lib()
+    if not _cp9_ready:
+        vp, i32, i64 = C.c_void_p, C.c_int32, C.c_int64
+        i32p, f32p, u8p = C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_uint8)
+        L.ixg_cp9_create.argtypes = [vp, C.POINTER(Cp9Params), C.POINTER(vp)]
+        L.ixg_cp9_free.argtypes = [vp, vp]
+        L.ixg_cp9_bands.argtypes = [vp, vp, u8p, i64, i32, i32, C.c_double, C.c_int] + [i32p] * 6 + [f32p, f32p, i32p]
+        L.ixg_cp9_bands_batch.argtypes = [vp, vp, vp, C.POINTER(Cp9Win), i64, C.c_double, C.c_int, f32p, f32p, i32p]
+        _cp9_ready = True
+    return L
+
+
+class CP9:
+    """Device-resident configured CP9 HMM (ixg_cp9_create) from a dict with
+    M, el, el_selfsc, otsc, msc, isc, has_el, el_from_ct, el_from_off,
+    el_from_idx (synth_cp9.build_cp9 makes one)."""
+
+    def __init__(self, ctx, h):
+        self.ctx, self.M = ctx, int(h["M"])
+        self.keep = {k: np.ascontiguousarray(h[k], np.uint8 if k == "has_el" else np.int32)
+                     for k in ("otsc", "msc", "isc", "has_el", "el_from_ct", "el_from_off", "el_from_idx")}
+        p = Cp9Params()
+        p.M, p.el, p.el_selfsc, p.pad = self.M, int(bool(h["el"])), int(h["el_selfsc"]), 0
+        for k, v in self.keep.items():
+            setattr(p, k, v.ctypes.data_as(C.POINTER(C.c_uint8 if k == "has_el" else C.c_int32)))
+        self.h = C.c_void_p()
+        check(_cp9_lib().ixg_cp9_create(ctx.h, C.byref(p), C.byref(self.h)), "ixg_cp9_create")
+
+    def free(self):
+        if self.h:
+            check(_cp9_lib().ixg_cp9_free(self.ctx.h, self.h), "ixg_cp9_free")
+            self.h = C.c_void_p()
+
+    def bands(self, dsq, L, i0, j0, tau, scan=True, matrices=False):
+        """One window: dict(pn [6, M+1], fsc, bsc[, fm, fi, fd, bm, bi, bd [(L'+1), M+1]])."""
+        M1, Lw = self.M + 1, j0 - i0 + 1
+        d = np.ascontiguousarray(dsq, np.uint8)
+        mats = [np.zeros((Lw + 1) * M1, np.int32) for _ in range(6)] if matrices else None
+        pn = np.zeros(6 * M1, np.int32)
+        fs, bs = C.c_float(), C.c_float()
+        mp = [(_i32p(m) if matrices else None) for m in (mats or [None] * 6)]
+        check(_cp9_lib().ixg_cp9_bands(self.ctx.h, self.h, u8ptr(d), int(L), int(i0), int(j0), float(tau), int(scan),
+                                  *mp, C.byref(fs), C.byref(bs), _i32p(pn)), "ixg_cp9_bands")
+        out = dict(pn=pn.reshape(6, M1), fsc=fs.value, bsc=bs.value)
+        if matrices:
+            for k, m in zip(("fm", "fi", "fd", "bm", "bi", "bd"), mats):
+                out[k] = m.reshape(Lw + 1, M1)
+        return out
+
+
+def cp9_bands_batch(ctx, cp9, tgt, wins, tau, scan=True):
+    """wins: list of (i0, j0, strand). Returns (pn [n, 6, M+1], fsc [n], bsc [n])."""
+    n, M1 = len(wins), cp9.M + 1
+    arr = (Cp9Win * max(n, 1))()
+    for k, (i0, j0, strand) in enumerate(wins):
+        arr[k].i0, arr[k].j0, arr[k].strand, arr[k].pad = int(i0), int(j0), int(strand), 0
+    pn = np.zeros(max(n, 1) * 6 * M1, np.int32)
+    fs, bs = np.zeros(max(n, 1), np.float32), np.zeros(max(n, 1), np.float32)
+    check(_cp9_lib().ixg_cp9_bands_batch(ctx.h, cp9.h, tgt.h, arr, n, float(tau), int(scan), f32ptr(fs), f32ptr(bs),
+                                    _i32p(pn)), "ixg_cp9_bands_batch")
+    return pn[:n * 6 * M1].reshape(n, 6, M1), fs[:n], bs[:n]

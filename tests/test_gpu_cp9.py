@@ -1,0 +1,92 @@
+"""GPU parity of the CP9 banding DPs (cp9.rs:1275 cp9_forward, :1995
+cp9_backward, :1517 cp9_fb2hmmbands) against the CPU oracle, through the C
+ABI: the six Forward/Backward matrices, both best scores (bits) and the
+posterior HMM bands identical, in search (scan) and alignment mode, with and
+without local ends, single window and batched over both strands of a target
+(windows of many lengths, several waves)."""
+import numpy as np
+import pytest
+
+from infernox_amd import ixg, synth_cm as S, synth_cp9 as P
+from oracle_cp9_bind import OracleCP9
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _bits(x):
+    return np.float32(x).tobytes()
+
+
+def _model(ss, seed, el=True):
+    cm = S.build_cm(ss, seed)
+    return cm, P.build_cp9(cm, seed, el=el)
+
+
+def _window(cm, seed, flank):
+    r = np.random.default_rng(seed)
+    h = S.emit(cm, seed + 70)
+    seq = np.concatenate([r.integers(0, 4, flank), h, r.integers(0, 4, flank)])
+    return np.concatenate([[255], seq, [255]]).astype(np.uint8), len(seq)
+
+
+MODELS = [(".<<<..>>>..<<..>>.", 1, True), (S.cloverleaf(), 2, True), (S.cloverleaf(), 3, False),
+          (S.multibranch(120, 3, 4), 4, True)]
+
+
+@pytest.mark.parametrize("ss,seed,el", MODELS)
+@pytest.mark.parametrize("scan", [True, False])
+def test_cp9_single_window_matrices_scores_bands(ctx, ss, seed, el, scan):
+    cm, h = _model(ss, seed, el)
+    dsq, L = _window(cm, seed, 15)
+    o = OracleCP9(h)
+    g = ixg.CP9(ctx, h)
+    try:
+        for i0, j0 in ((1, L), (4, L - 3), (7, 7)):
+            _say("cp9 single", ss[:20], seed, el, scan, i0, j0)
+            f, b = o.forward(dsq, i0, j0, scan, not scan), o.backward(dsq, i0, j0, scan, not scan)
+            for tau in (1e-7, 1e-3):
+                got = g.bands(dsq, L, i0, j0, tau, scan, matrices=True)
+                want = o.bands(dsq, i0, j0, tau, f, b, scan)
+                for k, m in (("fm", f["mmx"]), ("fi", f["imx"]), ("fd", f["dmx"]), ("bm", b["mmx"]),
+                             ("bi", b["imx"]), ("bd", b["dmx"])):
+                    assert np.array_equal(got[k], m), (k, i0, j0, tau)
+                assert _bits(got["fsc"]) == _bits(f["sc"]) and _bits(got["bsc"]) == _bits(b["sc"]), (i0, j0)
+                assert np.array_equal(got["pn"], want), (i0, j0, tau)
+    finally:
+        g.free()
+
+
+def test_cp9_batch_both_strands(ctx):
+    cm, h = _model(S.cloverleaf(), 5, True)
+    r = np.random.default_rng(11)
+    parts, wins, pos = [], [], 1
+    for k in range(150):  # three waves of windows of many lengths
+        hh = S.emit(cm, 500 + k)[: 20 + (k * 7) % 60]
+        fl = int(r.integers(0, 30))
+        part = np.concatenate([r.integers(0, 4, fl), hh]).astype(np.uint8)
+        parts.append(part)
+        wins.append((pos, pos + len(part) - 1, k % 2))
+        pos += len(part)
+    seq = np.concatenate(parts)
+    L = len(seq)
+    dsq = np.concatenate([[255], seq, [255]]).astype(np.uint8)
+    rc = np.concatenate([[255], np.where(seq < 4, 3 - seq, seq)[::-1], [255]]).astype(np.uint8)
+    tgt = ixg.Target(ctx, dsq, L)
+    g = ixg.CP9(ctx, h)
+    o = OracleCP9(h)
+    try:
+        for scan in (True, False):
+            _say("cp9 batch", scan)
+            pn, fs, bs = ixg.cp9_bands_batch(ctx, g, tgt, wins, 1e-7, scan)
+            for k, (i0, j0, strand) in enumerate(wins):
+                d = rc if strand else dsq
+                f, b = o.forward(d, i0, j0, scan, not scan), o.backward(d, i0, j0, scan, not scan)
+                assert np.array_equal(pn[k], o.bands(d, i0, j0, 1e-7, f, b, scan)), (k, scan)
+                assert _bits(fs[k]) == _bits(f["sc"]) and _bits(bs[k]) == _bits(b["sc"]), (k, scan)
+    finally:
+        g.free()
+        tgt.free()
