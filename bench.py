@@ -19,7 +19,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from infernox_amd import ixg, synth, synth_cm  # noqa: E402
+from infernox_amd import ixg, synth, synth_cm, synth_cp9  # noqa: E402
 
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # 256 CUs x 4 SIMD16 x 2.4 GHz, one lane-op per cycle
 # HBM bytes per launch of each kernel on this workload, from rocprofv3 FETCH_SIZE +
@@ -60,8 +60,8 @@ def cm_workload(n_env=512, r=8, seed=3):
     return cm, dsq, L, envs
 
 
-def time_cm(ctx, steps, warmup, cpu=None):
-    cm, dsq, L, envl = cm_workload()
+def time_cm(ctx, steps, warmup, cpu=None, wl=None):
+    cm, dsq, L, envl = wl or cm_workload()
     g = ixg.CM(ctx, cm)
     tgt = ixg.Target(ctx, dsq, L)
     envs = ixg.EnvBatch(envl)  # marshalled once, as a caller holding its band arrays would
@@ -95,6 +95,42 @@ def time_cm(ctx, steps, warmup, cpu=None):
             out[name]["cpu"] = cpu_line(done, len(envl), w, dt, cpu[1], "envelopes")
     tgt.free()
     g.free()
+    return out
+
+
+def time_cp9(ctx, steps, cpu=None, wl=None, tau=1e-7):
+    """CP9 banding (Forward, Backward, posterior HMM bands) of the HB
+    workload's envelopes with a CP9 HMM over the CM's consensus: cells are
+    (window position, HMM node) pairs per DP."""
+    cm, dsq, L, envl = wl or cm_workload()
+    h = synth_cp9.build_cp9(cm, 3)
+    g = ixg.CP9(ctx, h)
+    tgt = ixg.Target(ctx, dsq, L)
+    wins = [(e[0], e[1], 0) for e in envl]
+    cells = sum((j0 - i0 + 2) * (h["M"] + 1) for i0, j0, _ in wins)
+    ixg.cp9_bands_batch(ctx, g, tgt, wins, tau)
+    ctx.kernel_timings(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ixg.cp9_bands_batch(ctx, g, tgt, wins, tau)
+    wall = (time.perf_counter() - t0) / steps
+    kms = ctx.kernel_timings(reset=True)["cp9"][0] / steps
+    # per cell: Forward ~9 and Backward ~9 ILogsums, the two posterior scans
+    # ~6; an ILogsum is ~7 integer ops and one table read
+    out = {"gcups": cells / (kms * 1e-3) / 1e9, "kernel_ms": kms, "wall_ms": wall * 1e3, "cells": int(cells),
+           "windows": len(wins), "hmm_nodes": int(h["M"]), "tau": tau,
+           "valu_roofline": valu_line(24 * 7 * cells, kms)}
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_cp9_bind import OracleCP9
+
+        def run(o, w):
+            o.bands(dsq, int(w[0]), int(w[1]), tau)
+            return (w[1] - w[0] + 2) * (h["M"] + 1)
+        done, wk, dt = cpu_rate(lambda: OracleCP9(h), run, wins, *cpu)
+        out["cpu"] = cpu_line(done, len(wins), wk, dt, cpu[1], "windows")
+    g.free()
+    tgt.free()
     return out
 
 
@@ -523,7 +559,9 @@ def main():
         tgt.free()
         g.free()
         cpu = None if a.no_cpu else (a.cpu_section_seconds, a.cpu_threads)
-        cm_line = time_cm(ctx, max(1, a.steps // 2), 1, cpu) if not a.no_cm else None
+        wl = cm_workload() if not a.no_cm else None
+        cm_line = time_cm(ctx, max(1, a.steps // 2), 1, cpu, wl) if not a.no_cm else None
+        cp9_line = time_cp9(ctx, 2, cpu, wl) if not a.no_cm else None
         tr_line = time_trcm(ctx, max(1, a.steps // 2), 1, cpu) if not a.no_cm else None
         qdb_line = time_qdb(ctx, 2, cpu) if not a.no_cm else None
         pf_line = time_p7_forward(ctx, p, d, 2, cpu=cpu) if not a.no_cm else None
@@ -571,6 +609,8 @@ def main():
         line["qdb_scanners"] = qdb_line
     if pf_line:
         line["p7_forward_full"] = pf_line
+    if cp9_line:
+        line["cp9_bands"] = cp9_line
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(p, d, L, M, maxw, tasks, nt, a.cpu_seconds, a.cpu_threads)
     print(json.dumps(line))

@@ -173,3 +173,25 @@ def test_cm_integer_and_global_tables_equal_oracle(local):
     assert np.array_equal(itsc, o.itsc) and np.array_equal(ioesc, o.ioesc)
     assert np.array_equal(ib, o.ibeginsc) and np.array_equal(ie, o.iendsc) and iel == o.iel_selfsc
     g.free()
+
+
+def test_cp9_entry_points_reject_bad_arguments():
+    from infernox_amd import synth_cm as S, synth_cp9 as P
+    L = ixg._cp9_lib()
+    cm = S.build_cm(S.cloverleaf(), 1)
+    h = P.build_cp9(cm, 1)
+    keep = {k: np.ascontiguousarray(h[k], np.uint8 if k == "has_el" else np.int32)
+            for k in ("otsc", "msc", "isc", "has_el", "el_from_ct", "el_from_off", "el_from_idx")}
+    p = ixg.Cp9Params()
+    p.M, p.el, p.el_selfsc = h["M"], 1, h["el_selfsc"]
+    for k, v in keep.items():
+        setattr(p, k, v.ctypes.data_as(C.POINTER(C.c_uint8 if k == "has_el" else C.c_int32)))
+    out = C.c_void_p()
+    assert L.ixg_cp9_create(None, C.byref(p), C.byref(out)) == ixg.IXG_EINVAL  # no context
+    p.M = 0
+    assert L.ixg_cp9_create(None, C.byref(p), C.byref(out)) == ixg.IXG_EINVAL
+    assert L.ixg_cp9_free(None, None) == ixg.IXG_OK
+    pn = np.zeros(6 * 80, np.int32)
+    assert L.ixg_cp9_bands(None, None, None, 0, 1, 1, 1e-7, 1, None, None, None, None, None, None, None, None,
+                           pn.ctypes.data_as(C.POINTER(C.c_int32))) == ixg.IXG_EINVAL
+    assert L.ixg_cp9_bands_batch(None, None, None, None, 1, 1e-7, 1, None, None, None) == ixg.IXG_EINVAL
