@@ -215,23 +215,26 @@ hipError_t launch_trqdb_scan(const TrqDev &q, const void *tabs, int inside, cons
 // cp9_fb2hmmbands) ----
 struct Cp9Dev {
   int M, el, el_selfsc;
+  int nz;                           // ILogsum table entries up to the last nonzero one
   const int32_t *otsc, *msc, *isc;  // [(M+1)][12] cp9O order, [(M+1)][18], [(M+1)][18]
   const uint8_t *has_el;            // [M+1]
   const int32_t *el_ct, *el_off, *el_idx;  // local-end sources per node 0..M+1
-  const int32_t *ilog;                     // ILogsum table
+  // The same lists per source node y, for the Backward's local-end cells:
+  // how often y ends at node M, whether it ends at M+1 (a scan resets the
+  // cell there), and the other nodes it ends at, in decreasing order.
+  const int32_t *bel_nM, *bel_reset, *bel_off, *bel_k;  // [M+1], [M+1], [M+2], list
+  const int32_t *ilog;                                  // ILogsum table
 };
 struct Cp9WinDev {
   const uint8_t *dsq;
   int32_t i0, j0;
   int64_t out;  // the window's index in the caller's order
+  int32_t *mx;  // Forward M, I, D then Backward M, I, D, each [(L+1)][(M+1)]
 };
-// One wave's matrices, interleaved over its 64 lanes ([row][k][lane]):
-// Forward and Backward M/I/D [(lmax+1)][(M+1)][64], the two rolling local-end
-// rows of each [2][(M+1)][64], and the posterior masses [3][(M+1)][64].
-struct Cp9GrpDev {
-  int32_t *fm, *fi, *fd, *bm, *bi, *bd, *fel, *bel, *post;
-};
-hipError_t launch_cp9_bands(const Cp9Dev &h, const Cp9WinDev *wins, int64_t n, const Cp9GrpDev *grps, int do_scan,
-                            int32_t thresh, float *fsc, float *bsc, int32_t *pn, hipStream_t s);
+// Dynamic LDS of the CP9 kernels: two local-end rows, the two serial chains'
+// terms, a transition column, the ILogsum table as 16-bit entries.
+inline size_t cp9_lds_bytes(int M, int nz) { return (size_t)5 * (M + 1) * 4 + (((size_t)nz * 2 + 15) & ~(size_t)15); }
+hipError_t launch_cp9_bands(const Cp9Dev &h, const Cp9WinDev *wins, int64_t n, int do_scan, int32_t thresh,
+                            float *fsc, float *bsc, int32_t *pn, hipStream_t s);
 
 }  // namespace ixg

@@ -1,6 +1,7 @@
 // C ABI of the CP9 HMM banding DPs (cp9.rs:1275 cp9_forward, :1995
-// cp9_backward, :1517 cp9_fb2hmmbands): the resident HMM, windows sorted by
-// length and dealt 64 to a wave, launch groups that fit the matrix budget.
+// cp9_backward, :1517 cp9_fb2hmmbands): the resident HMM with its local-end
+// lists in both directions, one wave per window, launch groups of windows
+// whose six matrices fit the matrix budget.
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -11,7 +12,7 @@ using namespace ixg;
 
 struct ixg_cp9 {
   int M = 0, el = 0, el_selfsc = 0;
-  int32_t *d = nullptr;  // otsc | msc | isc | el_ct | el_off | el_idx | ilog | has_el (bytes)
+  int32_t *d = nullptr;  // otsc | msc | isc | el_ct | el_off | el_idx | bel_* | ilog | has_el (bytes)
   size_t bytes = 0;
   Cp9Dev dev;
 };
@@ -57,46 +58,36 @@ int cp9_run(ixg_ctx *c, const ixg_cp9 *h, const std::vector<CWin> &wins, double 
   std::vector<int32_t> hp((size_t)std::max<int64_t>(n, 1) * 6 * M1);
   int64_t g0 = 0;  // first window (in sorted order) of the launch group
   while (g0 < n) {
-    // whole waves of 64 windows while the matrices fit
+    // windows, longest first, while their matrices fit
     int64_t g1 = g0, elems = 0;
-    std::vector<int64_t> goff;
+    std::vector<int64_t> woff;
     while (g1 < n) {
-      const int lmax = wins[order[g1]].j0 - wins[order[g1]].i0 + 1;
-      const int64_t need = (6 * (int64_t)(lmax + 1) * M1 + 4 * (int64_t)M1 + 3 * (int64_t)M1) * 64;
+      const int L = wins[order[g1]].j0 - wins[order[g1]].i0 + 1;
+      const int64_t need = 6 * (int64_t)(L + 1) * M1;
       if (g1 > g0 && elems + need > budget) break;
-      goff.push_back(elems);
+      woff.push_back(elems);
       elems += need;
-      g1 = std::min(n, g1 + 64);
+      g1++;
     }
-    const int64_t nw = g1 - g0, ng = (int64_t)goff.size();
+    const int64_t nw = g1 - g0;
     int32_t *d_mx;
     RCHK(scratch(c, S_CM_ALPHA, (size_t)elems * 4, (void **)&d_mx));
     std::vector<Cp9WinDev> wd(nw);
-    std::vector<Cp9GrpDev> gd(ng);
     for (int64_t k = 0; k < nw; k++) {
       const CWin &w = wins[order[g0 + k]];
-      wd[k] = Cp9WinDev{w.dsq, w.i0, w.j0, k};
+      wd[k] = Cp9WinDev{w.dsq, w.i0, w.j0, k, d_mx + woff[k]};
     }
-    for (int64_t g = 0; g < ng; g++) {
-      const int lmax = wins[order[g0 + 64 * g]].j0 - wins[order[g0 + 64 * g]].i0 + 1;
-      const int64_t mat = (int64_t)(lmax + 1) * M1 * 64;
-      int32_t *b = d_mx + goff[g];
-      gd[g] = Cp9GrpDev{b, b + mat, b + 2 * mat, b + 3 * mat, b + 4 * mat, b + 5 * mat, b + 6 * mat,
-                        b + 6 * mat + 2 * (int64_t)M1 * 64, b + 6 * mat + 4 * (int64_t)M1 * 64};
-    }
-    const size_t wbytes = nw * sizeof(Cp9WinDev), gbytes = ng * sizeof(Cp9GrpDev);
+    const size_t wbytes = nw * sizeof(Cp9WinDev);
     const size_t obytes = (size_t)nw * (2 * 4 + 6 * M1 * 4);
     char *d_meta;
-    RCHK(scratch(c, S_CM_ENV, wbytes + gbytes + obytes + 64, (void **)&d_meta));
+    RCHK(scratch(c, S_CM_ENV, wbytes + obytes + 64, (void **)&d_meta));
     Cp9WinDev *d_w = (Cp9WinDev *)d_meta;
-    Cp9GrpDev *d_g = (Cp9GrpDev *)(d_meta + ((wbytes + 15) & ~(size_t)15));
-    float *d_f = (float *)(d_meta + ((wbytes + 15) & ~(size_t)15) + ((gbytes + 15) & ~(size_t)15));
+    float *d_f = (float *)(d_meta + ((wbytes + 15) & ~(size_t)15));
     float *d_b = d_f + nw;
     int32_t *d_pn = (int32_t *)(d_b + nw);
     HIPCHK(hipMemcpyAsync(d_w, wd.data(), wbytes, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipMemcpyAsync(d_g, gd.data(), gbytes, hipMemcpyHostToDevice, c->stream));
     HIPCHK(ktime_begin(c));
-    HIPCHK(launch_cp9_bands(h->dev, d_w, nw, d_g, do_scan, thresh, d_f, d_b, d_pn, c->stream));
+    HIPCHK(launch_cp9_bands(h->dev, d_w, nw, do_scan, thresh, d_f, d_b, d_pn, c->stream));
     HIPCHK(ktime_end(c, IXG_K_CP9));
     std::vector<float> tf(nw), tb(nw);
     std::vector<int32_t> tp((size_t)nw * 6 * M1);
@@ -109,16 +100,11 @@ int cp9_run(ixg_ctx *c, const ixg_cp9 *h, const std::vector<CWin> &wins, double 
       hb[o] = tb[k];
       std::copy(tp.begin() + k * 6 * M1, tp.begin() + (k + 1) * 6 * M1, hp.begin() + o * 6 * M1);
     }
-    if (mats && n == 1) {  // de-interleave lane 0 of the single window's group
+    if (mats && n == 1) {  // the single window's matrices, [(L+1)][(M+1)] each
       const int L = wins[0].j0 - wins[0].i0 + 1;
       const size_t cells = (size_t)(L + 1) * M1;
-      std::vector<int32_t> buf(cells * 64);
-      int32_t *src[6] = {gd[0].fm, gd[0].fi, gd[0].fd, gd[0].bm, gd[0].bi, gd[0].bd};
-      for (int m = 0; m < 6; m++) {
-        if (!mats[m]) continue;
-        HIPCHK(hipMemcpy(buf.data(), src[m], cells * 64 * 4, hipMemcpyDeviceToHost));
-        for (size_t x = 0; x < cells; x++) mats[m][x] = buf[x * 64];
-      }
+      for (int m = 0; m < 6; m++)
+        if (mats[m]) HIPCHK(hipMemcpy(mats[m], d_mx + m * cells, cells * 4, hipMemcpyDeviceToHost));
     }
     g0 = g1;
   }
@@ -163,9 +149,39 @@ int ixg_cp9_create(ixg_ctx *c, const ixg_cp9_params *p, ixg_cp9 **out) {
       g_err = "ixg_cp9_create: local-end source outside 0..M";
       return IXG_EINVAL;
     }
+  // A local end's source node precedes the node it ends at (cp9.rs:1042-1076
+  // takes them from a CM node's left and right consensus positions); the
+  // Backward completes each local-end cell before its node's match cell.
+  for (int k = 1; k <= M + 1; k++)
+    for (int c = 0; c < p->el_from_ct[k]; c++)
+      if (p->el_from_idx[p->el_from_off[k] + c] >= k) {
+        g_err = "ixg_cp9_create: a local-end source must precede the node it ends at";
+        return IXG_EINVAL;
+      }
   std::vector<int32_t> ilog(FLOGSUM_TBL);
-  for (int i = 0; i < FLOGSUM_TBL; i++)  // cp9.rs:28-43 ILogsum table
+  int nz = 1;
+  for (int i = 0; i < FLOGSUM_TBL; i++) {  // cp9.rs:28-43 ILogsum table
     ilog[i] = (int32_t)std::nearbyint(1000.0 * std::log2(1.0 + std::pow(2.0, -(double)i / 1000.0)));
+    if (ilog[i] != 0) nz = i + 1;
+  }
+  if (cp9_lds_bytes(M, nz) + 64 > 65536) {
+    g_err = "ixg_cp9_create: the HMM has more nodes than the kernels' LDS holds (M <= 2100)";
+    return IXG_EINVAL;
+  }
+  // Backward local-end lists per source node y (used only with local ends)
+  std::vector<int32_t> bel_nM(M1, 0), bel_reset(M1, 0), bel_off(M1 + 1, 0), bel_k;
+  if (p->el) {
+    for (int c = 0; c < p->el_from_ct[M]; c++) bel_nM[p->el_from_idx[p->el_from_off[M] + c]]++;
+    for (int c = 0; c < p->el_from_ct[M + 1]; c++) bel_reset[p->el_from_idx[p->el_from_off[M + 1] + c]] = 1;
+    std::vector<std::vector<int32_t>> per(M1);
+    for (int k = M - 1; k >= 1; k--)
+      for (int c = 0; c < p->el_from_ct[k]; c++) per[p->el_from_idx[p->el_from_off[k] + c]].push_back(k);
+    for (int y = 0; y <= M; y++) {
+      bel_off[y] = (int32_t)bel_k.size();
+      bel_k.insert(bel_k.end(), per[y].begin(), per[y].end());
+    }
+    bel_off[M1] = (int32_t)bel_k.size();
+  }
   std::vector<int32_t> blob;
   auto put = [&](const int32_t *x, size_t k) {
     const size_t o = blob.size();
@@ -176,7 +192,10 @@ int ixg_cp9_create(ixg_ctx *c, const ixg_cp9_params *p, ixg_cp9 **out) {
   const size_t o_t = put(p->otsc, (size_t)M1 * 12), o_m = put(p->msc, (size_t)M1 * KP),
                o_i = put(p->isc, (size_t)M1 * KP), o_c = put(p->el_from_ct, M + 2), o_o = put(p->el_from_off, M + 3);
   const int32_t zero = 0;
-  const size_t o_x = nidx ? put(p->el_from_idx, nidx) : put(&zero, 1), o_l = put(ilog.data(), ilog.size());
+  const size_t o_x = nidx ? put(p->el_from_idx, nidx) : put(&zero, 1);
+  const size_t o_bn = put(bel_nM.data(), M1), o_br = put(bel_reset.data(), M1), o_bo = put(bel_off.data(), M1 + 1);
+  const size_t o_bk = bel_k.empty() ? put(&zero, 1) : put(bel_k.data(), bel_k.size());
+  const size_t o_l = put(ilog.data(), ilog.size());
   const size_t o_h = blob.size();
   blob.resize(o_h + (M1 + 3) / 4 + 1, 0);
   std::memcpy(blob.data() + o_h, p->has_el, M1);
@@ -205,6 +224,11 @@ int ixg_cp9_create(ixg_ctx *c, const ixg_cp9_params *p, ixg_cp9 **out) {
   d.el_ct = h->d + o_c;
   d.el_off = h->d + o_o;
   d.el_idx = h->d + o_x;
+  d.bel_nM = h->d + o_bn;
+  d.bel_reset = h->d + o_br;
+  d.bel_off = h->d + o_bo;
+  d.bel_k = h->d + o_bk;
+  d.nz = nz;
   d.ilog = h->d + o_l;
   d.has_el = (const uint8_t *)(h->d + o_h);
   *out = h;
