@@ -34,7 +34,7 @@ __device__ __forceinline__ bool child_cell(const EnvDev &e, int y, int jy, int d
   const int h = e.hd_off[y] + (jy - jmn);
   const int lo = e.hdmin[h];
   if (dy < lo || dy > e.hdmax[h]) return false;
-  val = e.alpha[e.roff[h] + (dy - lo)];
+  val = gmem(e.alpha)[e.roff[h] + (dy - lo)];
   return true;
 }
 
@@ -67,16 +67,16 @@ __device__ float cell_generic(const CmDev &cm, const EnvDev &e, int v, int st, i
     const RowRef r = row_ref(e, y, j - sdr);
     const int dy = d - sd;
     ok[yo] = yo < cn && dy >= r.lo && dy <= r.hi;
-    a[yo] = e.alpha[ok[yo] ? r.base + dy : 0];
+    a[yo] = gmem(e.alpha)[ok[yo] ? r.base + dy : 0];
   }
 #pragma unroll
   for (int yo = 0; yo < MAXC; yo++)
     if (ok[yo]) cur = combine<INSIDE>(cur, a[yo] + tsc[yo], ftbl);
   const float *oe = cm.oesc + (size_t)v * (KP * KP);
   const int i = j - d + 1;
-  if (st == ML_ST) cur += oe[e.dsq[i]];
-  else if (st == MR_ST) cur += oe[e.dsq[j]];
-  else if (st == MP_ST) cur += oe[e.dsq[i] * KP + e.dsq[j]];
+  if (st == ML_ST) cur += oe[gmem(e.dsq)[i]];
+  else if (st == MR_ST) cur += oe[gmem(e.dsq)[j]];
+  else if (st == MP_ST) cur += oe[gmem(e.dsq)[i] * KP + gmem(e.dsq)[j]];
   return cur < IMPOSSIBLE ? IMPOSSIBLE : cur;
 }
 
@@ -91,14 +91,14 @@ __device__ float cell_bif(const CmDev &cm, const EnvDev &e, int v, int j, int d,
   int kn = j - e.jmax[y] > hz0 ? j - e.jmax[y] : hz0;
   kn = kn > 0 ? kn : 0;
   const int kx = jpy < e.hdmax[hz] ? jpy : e.hdmax[hz];
-  const float *az = e.alpha + e.roff[hz] - hz0;
+  const IXG_GLOBAL float *az = gmem(e.alpha) + e.roff[hz] - hz0;
   const int hy0 = e.hd_off[y];
 #pragma unroll 4
   for (int k = kn; k <= kx; k++) {
     const int hy = hy0 + (jpy - k);
     const int lo = e.hdmin[hy];
     const bool ok = k >= d - e.hdmax[hy] && k <= d - lo;
-    const float a = e.alpha[ok ? e.roff[hy] + (d - lo - k) : 0];
+    const float a = gmem(e.alpha)[ok ? e.roff[hy] + (d - lo - k) : 0];
     const float b = az[k];
     if (ok) cur = combine<INSIDE>(cur, a + b, ftbl);
   }
@@ -116,7 +116,7 @@ __device__ void il_row(const CmDev &cm, const EnvDev &e, int v, int u, const flo
   const int j = e.jmin[v] + u, h = e.hd_off[v] + u;
   const int lo = e.hdmin[h], hi = e.hdmax[h];
   if (hi < lo) return;
-  float *row = e.alpha + e.roff[h] - lo;
+  IXG_GLOBAL float *row = gmem(e.alpha) + e.roff[h] - lo;
   const int cf = cm.cfirst[v], cn = cm.cnum[v];
   const float *tsc = cm.tsc + v * MAXC;
   const float *oe = cm.oesc + (size_t)v * (KP * KP);
@@ -135,12 +135,12 @@ __device__ void il_row(const CmDev &cm, const EnvDev &e, int v, int u, const flo
 #pragma unroll
     for (int c = 0; c < ICH; c++) {
       const int d = d0 + c;
-      x[c] = e.dsq[d <= hi ? j - d + 1 : j];
+      x[c] = gmem(e.dsq)[d <= hi ? j - d + 1 : j];
 #pragma unroll
       for (int yo = 0; yo < MAXC; yo++) {
         const int y = cf + yo;
         ok[c][yo] = d <= hi && yo < cn && y != v && d - 1 >= rr[yo].lo && d - 1 <= rr[yo].hi;
-        a[c][yo] = e.alpha[ok[c][yo] ? rr[yo].base + d - 1 : 0];
+        a[c][yo] = gmem(e.alpha)[ok[c][yo] ? rr[yo].base + d - 1 : 0];
       }
     }
 #pragma unroll
@@ -191,13 +191,13 @@ __device__ void ir_diag(const CmDev &cm, const EnvDev &e, int v, int u, const fl
       const int lo = e.hdmin[h], hi = e.hdmax[h];
       on[c] = jp < nj && d >= lo && d <= hi;
       cell[c] = e.roff[h] + (d - lo);
-      x[c] = e.dsq[on[c] ? j : i];
+      x[c] = gmem(e.dsq)[on[c] ? j : i];
 #pragma unroll
       for (int yo = 0; yo < MAXC; yo++) {
         const int y = cf + yo;
         const RowRef r = row_ref(e, min(y, cm.M - 1), j - 1);
         ok[c][yo] = on[c] && yo < cn && y != v && d - 1 >= r.lo && d - 1 <= r.hi;
-        a[c][yo] = e.alpha[ok[c][yo] ? r.base + d - 1 : 0];
+        a[c][yo] = gmem(e.alpha)[ok[c][yo] ? r.base + d - 1 : 0];
       }
     }
 #pragma unroll
@@ -219,7 +219,7 @@ __device__ void ir_diag(const CmDev &cm, const EnvDev &e, int v, int u, const fl
       }
       cur += oe[x[c]];
       cur = cur < IMPOSSIBLE ? IMPOSSIBLE : cur;
-      e.alpha[cell[c]] = cur;
+      gmem(e.alpha)[cell[c]] = cur;
       prev = cur;
       have_prev = true;
     }
@@ -246,7 +246,7 @@ __global__ __launch_bounds__(HB_THREADS) void hb_scan_kernel(CmDev cm, const Env
       const int r = hb_row_of(ro, nj - 1, tgt);
       const int dp = tgt - ro[r];
       const int d = hmn[r] + dp, j = jmn + r;
-      e.alpha[tgt] = st == B_ST ? cell_bif<INSIDE>(cm, e, v, j, d, ftbl)
+      gmem(e.alpha)[tgt] = st == B_ST ? cell_bif<INSIDE>(cm, e, v, j, d, ftbl)
                                 : cell_generic<INSIDE>(cm, e, v, st, j, d, dp, ftbl);
     }
   };
@@ -268,7 +268,7 @@ __global__ __launch_bounds__(HB_THREADS) void hb_scan_kernel(CmDev cm, const Env
     const int tgt = ro[0] + c;
     const int lo = hb_row_of(ro, last, tgt);
     const int d = hmn[lo] + (tgt - ro[lo]), j = jmn + lo;
-    float cur = e.alpha[tgt];
+    float cur = gmem(e.alpha)[tgt];
     if (cm.local_begin)
       for (int b = 0; b < cm.nbegin; b++) {
         const int y = cm.begin_list[b];
@@ -278,8 +278,8 @@ __global__ __launch_bounds__(HB_THREADS) void hb_scan_kernel(CmDev cm, const Env
           if (s > cur) cur = s;
         }
       }
-    e.alpha[tgt] = cur;
-    e.rout[c] = cur;
+    gmem(e.alpha)[tgt] = cur;
+    gmem(e.rout)[c] = cur;
   }
 }
 

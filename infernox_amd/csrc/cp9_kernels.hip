@@ -50,25 +50,36 @@ struct Lds {
   uint16_t *tbl;
 };
 
-__device__ Lds lds_setup(const Cp9Dev &h, int zcol, int zshift) {
+__device__ __forceinline__ Lds lds_setup(const Cp9Dev &h, int zcol, int zshift) {
   extern __shared__ int32_t lds[];
   const int M1 = h.M + 1;
   Lds s{lds, lds + 2 * M1, lds + 3 * M1, lds + 4 * M1, (uint16_t *)(lds + 5 * M1)};
-  for (int x = threadIdx.x; x < h.nz; x += 64) s.tbl[x] = (uint16_t)h.ilog[x];
+  const IXG_GLOBAL int32_t *ilog = gmem(h.ilog), *otsc = gmem(h.otsc);
+  for (int x = threadIdx.x; x < h.nz; x += 64) s.tbl[x] = (uint16_t)ilog[x];
   // z[k] = the transition zcol of node k - zshift
-  for (int k = threadIdx.x; k <= h.M; k += 64) s.z[k] = k >= zshift ? h.otsc[(size_t)(k - zshift) * O_NT + zcol] : 0;
+  for (int k = threadIdx.x; k <= h.M; k += 64) s.z[k] = k >= zshift ? otsc[(size_t)(k - zshift) * O_NT + zcol] : 0;
   return s;
 }
 
-#define TS(k, x) h.otsc[(size_t)(k) * O_NT + (x)]
-#define MSC(k, x) h.msc[(size_t)(k) * KP + (x)]
-#define ISC(k, x) h.isc[(size_t)(k) * KP + (x)]
+#define TS(k, x) g.otsc[(size_t)(k) * O_NT + (x)]
+#define MSC(k, x) g.msc[(size_t)(k) * KP + (x)]
+#define ISC(k, x) g.isc[(size_t)(k) * KP + (x)]
+
+// The HMM's tables as global pointers.
+struct G {
+  const IXG_GLOBAL int32_t *otsc, *msc, *isc, *el_ct, *el_off, *el_idx, *bel_nM, *bel_reset, *bel_off, *bel_k;
+  const IXG_GLOBAL uint8_t *has_el, *dsq;
+};
+__device__ __forceinline__ G gtabs(const Cp9Dev &h, const Cp9WinDev &w) {
+  return G{gmem(h.otsc),   gmem(h.msc),    gmem(h.isc),       gmem(h.el_ct),   gmem(h.el_off), gmem(h.el_idx),
+           gmem(h.bel_nM), gmem(h.bel_reset), gmem(h.bel_off), gmem(h.bel_k), gmem(h.has_el), gmem(w.dsq)};
+}
 #define LS(a, b) lsum((a), (b), s.tbl, nz)
 
 // acc = ILogsum(u[k], acc + z[k]) over k = k0, k0 + dk, ... (n steps); lane 0
 // stores each acc to out[k] when out is set. z is used by lane 0 only.
 __device__ __forceinline__ int32_t chain(const int32_t *u, const int32_t *z, int32_t acc, int k0, int dk, int n,
-                                        int32_t *out, const uint16_t *tbl, int nz) {
+                                        IXG_GLOBAL int32_t *out, const uint16_t *tbl, int nz) {
   int k = k0;
   int c = 0;
   for (; c + 4 <= n; c += 4) {
@@ -99,7 +110,8 @@ __global__ __launch_bounds__(64) void cp9_fwd_kernel(Cp9Dev h, const Cp9WinDev *
   const Cp9WinDev w = wins[blockIdx.x];
   const int i0 = w.i0, j0 = w.j0, L = j0 - i0 + 1;
   const size_t cells = (size_t)(L + 1) * M1;
-  int32_t *gm = w.mx, *gi = gm + cells, *gd = gi + cells;
+  const G g = gtabs(h, w);
+  IXG_GLOBAL int32_t *gm = gmem(w.mx), *gi = gm + cells, *gd = gi + cells;
   const bool el = h.el != 0;
   // row 0, position i0 - 1 (cp9.rs:1309-1333)
   for (int k = t; k <= M; k += 64) {
@@ -121,9 +133,10 @@ __global__ __launch_bounds__(64) void cp9_fwd_kernel(Cp9Dev h, const Cp9WinDev *
   int32_t sca = wa(gd[M], TS(M, O_DM));
   if (t == 1) best = scorify(sca);
   for (int j = i0; j <= j0; j++) {
-    const int dj = w.dsq[j], cur = j - i0 + 1;
-    const int32_t *mp = gm + (size_t)(cur - 1) * M1, *ip = gi + (size_t)(cur - 1) * M1, *dp = gd + (size_t)(cur - 1) * M1;
-    int32_t *mc = gm + (size_t)cur * M1, *ic = gi + (size_t)cur * M1, *dc = gd + (size_t)cur * M1;
+    const int dj = g.dsq[j], cur = j - i0 + 1;
+    const IXG_GLOBAL int32_t *mp = gm + (size_t)(cur - 1) * M1, *ip = gi + (size_t)(cur - 1) * M1,
+                             *dp = gd + (size_t)(cur - 1) * M1;
+    IXG_GLOBAL int32_t *mc = gm + (size_t)cur * M1, *ic = gi + (size_t)cur * M1, *dc = gd + (size_t)cur * M1;
     const int32_t *ep = s.el + ((cur - 1) & 1) * M1;
     int32_t *ec = s.el + (cur & 1) * M1;
     const int32_t mp0 = mp[0];
@@ -139,12 +152,12 @@ __global__ __launch_bounds__(64) void cp9_fwd_kernel(Cp9Dev h, const Cp9WinDev *
       } else {
         int32_t sc = LS(LS(wa(mp[k - 1], TS(k - 1, O_MM)), wa(ip[k - 1], TS(k - 1, O_IM))),
                         LS(wa(dp[k - 1], TS(k - 1, O_DM)), wa(mp0, TS(k, O_BM))));
-        const int n = h.el_ct[k], o = h.el_off[k];
-        for (int c = 0; c < n; c++) sc = LS(sc, ep[h.el_idx[o + c]]);
+        const int n = g.el_ct[k], o = g.el_off[k];
+        for (int c = 0; c < n; c++) sc = LS(sc, ep[g.el_idx[o + c]]);
         m = max(wa(sc, MSC(k, dj)), NINF);
         sc = LS(LS(wa(mp[k], TS(k, O_MI)), wa(ip[k], TS(k, O_II))), wa(dp[k], TS(k, O_DI)));
         in = max(wa(sc, ISC(k, dj)), NINF);
-        ec[k] = (el && h.has_el[k]) ? LS(wa(m, TS(k, O_MEL)), wa(ep[k], h.el_selfsc)) : NINF;
+        ec[k] = (el && g.has_el[k]) ? LS(wa(m, TS(k, O_MEL)), wa(ep[k], h.el_selfsc)) : NINF;
         s.ub[k] = max(wa(m, TS(k, O_ME)), NINF);  // ILogsum(x, -INFTY)
       }
       mc[k] = m;
@@ -161,8 +174,8 @@ __global__ __launch_bounds__(64) void cp9_fwd_kernel(Cp9Dev h, const Cp9WinDev *
     const int32_t dM = __shfl(acc, 0, 64);
     if (t == 1) {  // cp9.rs:1420-1430
       int32_t endsc = LS(LS(acc, wa(dM, TS(M, O_DM))), wa(ic[M], TS(M, O_IM)));
-      const int n = h.el_ct[M + 1], o = h.el_off[M + 1];
-      for (int c = 0; c < n; c++) endsc = LS(endsc, ec[h.el_idx[o + c]]);
+      const int n = g.el_ct[M + 1], o = g.el_off[M + 1];
+      for (int c = 0; c < n; c++) endsc = LS(endsc, ec[g.el_idx[o + c]]);
       sca = endsc;
       if (scorify(endsc) > best) best = scorify(endsc);
     }
@@ -178,15 +191,17 @@ __global__ __launch_bounds__(64) void cp9_bwd_kernel(Cp9Dev h, const Cp9WinDev *
   const Cp9WinDev w = wins[blockIdx.x];
   const int i0 = w.i0, j0 = w.j0, L = j0 - i0 + 1;
   const size_t cells = (size_t)(L + 1) * M1;
-  int32_t *gm = w.mx + 3 * cells, *gi = gm + cells, *gd = gi + cells;
+  const G g = gtabs(h, w);
+  IXG_GLOBAL int32_t *gm = gmem(w.mx) + 3 * cells, *gi = gm + cells, *gd = gi + cells;
   const bool el = h.el != 0;
   __syncthreads();
   if (t == 0) {  // row L, position j0 (cp9.rs:2029-2055)
-    const int dj0 = w.dsq[j0];
-    int32_t *e = s.el + (L & 1) * M1, *m = gm + (size_t)L * M1, *i = gi + (size_t)L * M1, *d = gd + (size_t)L * M1;
+    const int dj0 = g.dsq[j0];
+    int32_t *e = s.el + (L & 1) * M1;
+    IXG_GLOBAL int32_t *m = gm + (size_t)L * M1, *i = gi + (size_t)L * M1, *d = gd + (size_t)L * M1;
     for (int k = 1; k <= M; k++) e[k] = NINF;
     if (el)
-      for (int c = 0; c < h.el_ct[M + 1]; c++) e[h.el_idx[h.el_off[M + 1] + c]] = 0;
+      for (int c = 0; c < g.el_ct[M + 1]; c++) e[g.el_idx[g.el_off[M + 1] + c]] = 0;
     e[0] = NINF;
     m[M] = wa(LS(wa(e[M], TS(M, O_MEL)), TS(M, O_ME)), MSC(M, dj0));
     i[M] = wa(TS(M, O_IM), ISC(M, dj0));
@@ -211,9 +226,9 @@ __global__ __launch_bounds__(64) void cp9_bwd_kernel(Cp9Dev h, const Cp9WinDev *
   // rows L-1 .. 1 (positions j0-1 .. i0, cp9.rs:2057-2169), then row 0
   // (position i0 - 1, cp9.rs:2171-2200)
   for (int cur = L - 1; cur >= 0; cur--) {
-    const int di = cur ? w.dsq[i0 + cur - 1] : 0;
-    const int32_t *mp = gm + (size_t)(cur + 1) * M1, *ip = gi + (size_t)(cur + 1) * M1;
-    int32_t *mc = gm + (size_t)cur * M1, *ic = gi + (size_t)cur * M1, *dc = gd + (size_t)cur * M1;
+    const int di = cur ? g.dsq[i0 + cur - 1] : 0;
+    const IXG_GLOBAL int32_t *mp = gm + (size_t)(cur + 1) * M1, *ip = gi + (size_t)(cur + 1) * M1;
+    IXG_GLOBAL int32_t *mc = gm + (size_t)cur * M1, *ic = gi + (size_t)cur * M1, *dc = gd + (size_t)cur * M1;
     const int32_t *ep = s.el + ((cur + 1) & 1) * M1;
     int32_t *ec = s.el + (cur & 1) * M1;
     // EL cells (cp9.rs:2068-2119), and the terms of the delete chain and the
@@ -221,13 +236,13 @@ __global__ __launch_bounds__(64) void cp9_bwd_kernel(Cp9Dev h, const Cp9WinDev *
     for (int y = t; y <= M; y += 64) {
       if (cur > 0) {
         int32_t v = NINF;
-        if (y == M && el && h.has_el[M]) v = wa(v, h.el_selfsc);
-        const int nM = h.bel_nM[y];
+        if (y == M && el && g.has_el[M]) v = wa(v, h.el_selfsc);
+        const int nM = g.bel_nM[y];
         for (int c = 0; c < nM; c++) v = LS(v, mp[M]);
-        if (do_scan && h.bel_reset[y]) v = 0;
-        const int o = h.bel_off[y], n = h.bel_off[y + 1] - o;
-        for (int c = 0; c < n; c++) v = LS(v, mp[h.bel_k[o + c]]);
-        if (el && y >= 1 && y < M && h.has_el[y]) v = LS(v, wa(ep[y], h.el_selfsc));
+        if (do_scan && g.bel_reset[y]) v = 0;
+        const int o = g.bel_off[y], n = g.bel_off[y + 1] - o;
+        for (int c = 0; c < n; c++) v = LS(v, mp[g.bel_k[o + c]]);
+        if (el && y >= 1 && y < M && g.has_el[y]) v = LS(v, wa(ep[y], h.el_selfsc));
         ec[y] = y ? v : NINF;
       }
       if (y >= 1 && y < M) s.ua[y] = LS(wa(mp[y + 1], TS(y, O_DM)), wa(ip[y], TS(y, O_DI)));
@@ -259,7 +274,7 @@ __global__ __launch_bounds__(64) void cp9_bwd_kernel(Cp9Dev h, const Cp9WinDev *
         m = NINF;  // the begin cell, written by lane 1 below
         in = wa(LS(LS(wa(mp[1], TS(0, O_IM)), wa(ip[0], TS(0, O_II))), wa(dc[1], TS(0, O_ID))), ISC(0, di));
       } else if (k == M) {
-        const bool sel = el && h.has_el[M];
+        const bool sel = el && g.has_el[M];
         m = wa(wa(ip[M], TS(M, O_MI)), MSC(M, di));
         if (sel) m = LS(m, wa(wa(NINF, h.el_selfsc), TS(M, O_MEL)));
         in = wa(wa(ip[M], TS(M, O_II)), ISC(M, di));
@@ -269,7 +284,7 @@ __global__ __launch_bounds__(64) void cp9_bwd_kernel(Cp9Dev h, const Cp9WinDev *
         }
       } else {
         int32_t mv = LS(LS(wa(mp[k + 1], TS(k, O_MM)), wa(ip[k], TS(k, O_MI))), wa(dc[k + 1], TS(k, O_MD)));
-        if (el && h.has_el[k]) mv = LS(mv, wa(ec[k], TS(k, O_MEL)));
+        if (el && g.has_el[k]) mv = LS(mv, wa(ec[k], TS(k, O_MEL)));
         m = wa(mv, MSC(k, di));
         in = wa(LS(LS(wa(mp[k + 1], TS(k, O_IM)), wa(ip[k], TS(k, O_II))), wa(dc[k + 1], TS(k, O_ID))), ISC(k, di));
         if (do_scan) m = LS(m, TS(k, O_ME));
@@ -299,7 +314,9 @@ __global__ __launch_bounds__(64) void cp9_post_kernel(Cp9Dev h, const Cp9WinDev 
   const Cp9WinDev w = wins[blockIdx.x];
   const int i0 = w.i0, j0 = w.j0, L = j0 - i0 + 1;
   const size_t cells = (size_t)(L + 1) * M1;
-  const int32_t *fm = w.mx, *fi = fm + cells, *fd = fi + cells, *bm = fd + cells, *bi = bm + cells, *bd = bi + cells;
+  const G g = gtabs(h, w);
+  const IXG_GLOBAL int32_t *fm = gmem(w.mx), *fi = fm + cells, *fd = fi + cells, *bm = fd + cells, *bi = bm + cells,
+                           *bd = bi + cells;
   __syncthreads();
   if (t == 0) {
     int32_t sc;
@@ -313,7 +330,7 @@ __global__ __launch_bounds__(64) void cp9_post_kernel(Cp9Dev h, const Cp9WinDev 
   }
   __syncthreads();
   const int32_t sc = s_sc;
-  int32_t *pn = pn_out + (size_t)w.out * 6 * M1;
+  IXG_GLOBAL int32_t *pn = gmem(pn_out) + (size_t)w.out * 6 * M1;
   const int32_t first = i0 - 1 < 0 ? 0 : i0 - 1;
   for (int k = t; k <= M; k += 64) {
     int32_t mm = NINF, mi = NINF, md = NINF;
@@ -327,7 +344,7 @@ __global__ __launch_bounds__(64) void cp9_post_kernel(Cp9Dev h, const Cp9WinDev 
       if (md > thresh) nd = first;
     }
     for (int r = 1; r <= L; r++) {
-      const int i = i0 + r - 1, x = w.dsq[i];
+      const int i = i0 + r - 1, x = g.dsq[i];
       const size_t o = (size_t)r * M1 + k;
       if (nm < 0) {
         mm = LS(mm, max(ws(ws(wa(fm[o], bm[o]), k ? MSC(k, x) : 0), sc), NINF));
@@ -345,7 +362,7 @@ __global__ __launch_bounds__(64) void cp9_post_kernel(Cp9Dev h, const Cp9WinDev 
     // maximum scan: rows L .. 1, then row 0
     mm = mi = md = NINF;
     for (int r = L; r >= 1; r--) {
-      const int i = i0 + r - 1, x = w.dsq[i];
+      const int i = i0 + r - 1, x = g.dsq[i];
       const size_t o = (size_t)r * M1 + k;
       if (xm < 0) {
         mm = LS(mm, max(ws(ws(wa(fm[o], bm[o]), k ? MSC(k, x) : 0), sc), NINF));

@@ -11,6 +11,24 @@ namespace ixg {
 constexpr int KP = IXG_KP;
 constexpr int16_t NEGINF_W = -32768;
 
+// A device pointer the compiler cannot trace to a kernel argument (one read
+// from a descriptor in memory) is generic: its loads and stores become flat
+// instructions, which count against the LDS counter too, so every LDS wait
+// after one also waits for the global access. gmem() states that the pointer
+// addresses global memory.
+#define IXG_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ IXG_GLOBAL T *gmem(T *p) {
+  return (IXG_GLOBAL T *)p;
+}
+// The same, handed back as a plain pointer: the compiler still sees where it
+// came from and emits global instructions through it, so it can flow into
+// code written for plain pointers.
+template <class T>
+__device__ __forceinline__ T *gplain(T *p) {
+  return (T *)(IXG_GLOBAL T *)p;
+}
+
 // Host-built p7 filter profiles (natural order + striped packs).
 struct MsvHost {
   int m = 0;

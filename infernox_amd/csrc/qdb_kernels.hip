@@ -115,9 +115,14 @@ __device__ __forceinline__ size_t ring_at(int32_t ring, int j, int W1) {
 // The window's residues: staged in LDS when the window fits, else read from
 // the device sequence.
 constexpr int QDB_SEQ_LDS = 32768;
+// Ring, deck and local-begin rows, read from a window descriptor, in global
+// memory.
+template <class T>
+using Gp = IXG_GLOBAL T;
+
 template <bool LDS>
 struct Seq {
-  const uint8_t *g;
+  const Gp<uint8_t> *g;
   const uint8_t *l;
   int i0;
   __device__ __forceinline__ int operator[](int i) const { return LDS ? (int)l[i - i0] : (int)g[i]; }
@@ -142,16 +147,16 @@ __device__ __forceinline__ QdbUnit load_unit(const QdbUnit *u) {
 // One cell (j, d) of a state other than IL and the root; returns its value
 // (for the local-begin rows).
 template <typename T, bool LS>
-__device__ __forceinline__ T qdb_cell(const QdbTabs<T> &tb, const uint16_t *tbl, T *__restrict__ rows,
-                                      T *__restrict__ deck, const Seq<LS> &dsq, int W1,
+__device__ __forceinline__ T qdb_cell(const QdbTabs<T> &tb, const uint16_t *tbl, Gp<T> *__restrict__ rows,
+                                      Gp<T> *__restrict__ deck, const Seq<LS> &dsq, int W1,
                                       const QdbUnit &u, int st, int cn, int j, int d) {
   using R = Sr<T>;
   const int v = u.code & 0x0FFFFFFF;
   const int sd = q_sd(st);
   if (st == B_ST) {  // cm_nohmm.rs:589-606
     const int kmin = u.k0, kmax = u.k1, DR = W1 + u.cring[2];
-    const T *db = deck + ((size_t)u.deck * W1 + u.cring[1]) * W1;
-    const T *yr = rows + ring_at(u.cring[0], j, W1);
+    const Gp<T> *db = deck + ((size_t)u.deck * W1 + u.cring[1]) * W1;
+    const Gp<T> *yr = rows + ring_at(u.cring[0], j, W1);
     T sc = init_sc(tb, v, d);
     int jr = (j - kmin) % DR;  // deck row of j - k (j - kmin >= i0 - 1 >= 0)
 #pragma unroll 4
@@ -220,16 +225,16 @@ __device__ __forceinline__ T reduce_c_rest(int cn, const T *t, T init, const uin
 // children, the local end and the emission into the row and a scratch row,
 // and a second pass adds the self term along d with nothing but that chain.
 template <typename T, bool LS>
-__device__ void qdb_il_row(const QdbTabs<T> &tb, const uint16_t *tbl, T *__restrict__ rows, const Seq<LS> &dsq,
+__device__ void qdb_il_row(const QdbTabs<T> &tb, const uint16_t *tbl, Gp<T> *__restrict__ rows, const Seq<LS> &dsq,
                            int W1, const QdbUnit &u, int cn, int j, int dn, int dx) {
   using R = Sr<T>;
   constexpr int B1 = 4, B2 = 8;  // d's whose loads are issued together
   const int v = u.code & 0x0FFFFFFF;
   const T *ts = tb.tsc + v * MAXC;
   const T *oe = tb.oesc + (size_t)v * (KP * KP);
-  T *out = rows + ring_at(u.ring, j, W1);
-  T *scr = rows + ring_scratch(u.ring) * W1;
-  const T *cr[MAXC];
+  Gp<T> *out = rows + ring_at(u.ring, j, W1);
+  Gp<T> *scr = rows + ring_scratch(u.ring) * W1;
+  const Gp<T> *cr[MAXC];
   T tsr[MAXC];
 #pragma unroll
   for (int yo = 1; yo < MAXC; yo++)
@@ -292,11 +297,11 @@ __global__ __launch_bounds__(QDB_MAX_THREADS) void qdb_kernel(QdbDev q, QdbTabs<
   const QdbWinDev wd = wins[blockIdx.x];
   const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6;
   const int W1 = wd.w + 1;
-  T *rows = (T *)wd.rows;
-  T *deck = rows + (size_t)q.nrows * W1;
-  T *acc = deck + ((size_t)q.nbegl * W1 + q.deck_extra) * W1;
+  Gp<T> *rows = gmem((T *)wd.rows);
+  Gp<T> *deck = rows + (size_t)q.nrows * W1;
+  Gp<T> *acc = deck + ((size_t)q.nbegl * W1 + q.deck_extra) * W1;
   Seq<LS> dsq;
-  dsq.g = wd.dsq;
+  dsq.g = gmem(wd.dsq);
   dsq.i0 = wd.i0;
   dsq.l = nullptr;
   if constexpr (LS) {
@@ -319,7 +324,7 @@ __global__ __launch_bounds__(QDB_MAX_THREADS) void qdb_kernel(QdbDev q, QdbTabs<
   }
   for (int b = 0; b < q.nbegl; b++) {
     const int2 dk = q.deck[b];
-    T *db = deck + ((size_t)b * W1 + dk.x) * W1;
+    Gp<T> *db = deck + ((size_t)b * W1 + dk.x) * W1;
     for (int r = wave; r < W1 + dk.y; r += nt >> 6)
       for (int d = lane; d < W1; d += 64) db[(size_t)r * W1 + d] = d == 0 ? tb.bbase0[b] : R::zero();
   }
@@ -355,7 +360,7 @@ __global__ __launch_bounds__(QDB_MAX_THREADS) void qdb_kernel(QdbDev q, QdbTabs<
         const T sc = qdb_cell<T, LS>(tb, tbl, rows, deck, dsq, W1, u, st, cn, j, d);
         const int slot = (u.info >> 8) - 1;
         if (q.local && slot >= 0) {  // this begin's candidate for the root at (j, d)
-          T &a = acc[((size_t)(j & q.amask) * q.nslot + slot) * W1 + d];
+          Gp<T> &a = acc[((size_t)(j & q.amask) * q.nslot + slot) * W1 + d];
           const T cnd = R::add(sc, tb.beginsc[u.code & 0x0FFFFFFF]);
           if (a < cnd) a = cnd;
         }
@@ -368,7 +373,7 @@ __global__ __launch_bounds__(QDB_MAX_THREADS) void qdb_kernel(QdbDev q, QdbTabs<
         if (yo < cn) a = R::comb(a, R::add(rows[ring_at(u.cring[yo], j, W1) + d], tb.tsc[yo]), tbl);
       if (q.local)
         for (int s = 0; s < q.nslot; s++) {
-          T &b = acc[((size_t)(j & q.amask) * q.nslot + s) * W1 + d];
+          Gp<T> &b = acc[((size_t)(j & q.amask) * q.nslot + s) * W1 + d];
           if (a < b) a = b;
           b = R::zero();
         }

@@ -66,7 +66,7 @@ __device__ void tr_cell(const CmDev &cm, const EnvDev &e, int v, int st, int j, 
   const int mv = e.mask[v];
   const bool dJ = mv & BJ, dL = mv & BL, dR = mv & BR;
   const int nc = e.nc;
-  float *A = e.alpha;
+  IXG_GLOBAL float *A = gmem(e.alpha);
   float J = tr_init(cm, v, d, t_sd(st)), L = tr_init(cm, v, d, t_sdl(st)), R = tr_init(cm, v, d, t_sdr(st));
   const int cf = cm.cfirst[v], cn = cm.cnum[v];
   const float *tsc = cm.tsc + v * MAXC;
@@ -91,7 +91,7 @@ __device__ void tr_cell(const CmDev &cm, const EnvDev &e, int v, int st, int j, 
       a0[yo] = A[i0];
       r0[yo] = (dR && (my[yo] & BR)) ? A[2 * nc + i0] : 0.f;
     }
-    const float em = oe[e.dsq[j - d + 1]];
+    const float em = oe[gmem(e.dsq)[j - d + 1]];
     if (dJ || dL) {
 #pragma unroll
       for (int yo = 0; yo < MAXC; yo++) {
@@ -128,7 +128,7 @@ __device__ void tr_cell(const CmDev &cm, const EnvDev &e, int v, int st, int j, 
       a0[yo] = A[i0];
       l0[yo] = (dL && (my[yo] & BL)) ? A[nc + i0] : 0.f;
     }
-    const float em = oe[e.dsq[j]];
+    const float em = oe[gmem(e.dsq)[j]];
     if (dJ || dR) {
 #pragma unroll
       for (int yo = 0; yo < MAXC; yo++) {
@@ -182,7 +182,7 @@ __device__ void tr_cell(const CmDev &cm, const EnvDev &e, int v, int st, int j, 
         if (my_ & BL) L = tcomb<INSIDE>(L, l1[yo] + t, ftbl);
       }
     }
-    const int xi = e.dsq[j - d + 1], xj = e.dsq[j];
+    const int xi = gmem(e.dsq)[j - d + 1], xj = gmem(e.dsq)[j];
     if (d >= 2) {
       J += oe[xi * KP + xj];
       L += cm.lmesc[v * KP + xi];
@@ -240,7 +240,7 @@ __device__ void tr_il_row(const CmDev &cm, const EnvDev &e, int v, int u, const 
   const int mv = e.mask[v];
   const bool dJ = mv & BJ, dL = mv & BL, dR = mv & BR;
   const int nc = e.nc;
-  float *A = e.alpha;
+  IXG_GLOBAL float *A = gmem(e.alpha);
   const int cf = cm.cfirst[v], cn = cm.cnum[v];
   const float *tsc = cm.tsc + v * MAXC;
   const float *oe = cm.oesc + (size_t)v * (KP * KP);
@@ -265,7 +265,7 @@ __device__ void tr_il_row(const CmDev &cm, const EnvDev &e, int v, int u, const 
 #pragma unroll
     for (int q = 0; q < TCH; q++) {
       const int d = d0 + q;
-      x[q] = e.dsq[d <= hi ? j - d + 1 : j];
+      x[q] = gmem(e.dsq)[d <= hi ? j - d + 1 : j];
 #pragma unroll
       for (int yo = 0; yo < MAXC; yo++) {
         const bool k1 = d <= hi && my[yo] && in_row(rr[yo], d - 1), k0 = d <= hi && my[yo] && in_row(rr[yo], d);
@@ -327,7 +327,7 @@ __device__ void tr_ir_diag(const CmDev &cm, const EnvDev &e, int v, int u, const
   const int mv = e.mask[v];
   const bool dJ = mv & BJ, dL = mv & BL, dR = mv & BR;
   const int nc = e.nc;
-  float *A = e.alpha;
+  IXG_GLOBAL float *A = gmem(e.alpha);
   const int cf = cm.cfirst[v], cn = cm.cnum[v];
   const float *tsc = cm.tsc + v * MAXC;
   const float *oe = cm.oesc + (size_t)v * (KP * KP);
@@ -354,7 +354,7 @@ __device__ void tr_ir_diag(const CmDev &cm, const EnvDev &e, int v, int u, const
       const int lo = e.hdmin[h], hi = e.hdmax[h];
       on[q] = jp < nj && d >= lo && d <= hi;
       cell[q] = e.roff[h] + (d - lo);
-      x[q] = e.dsq[on[q] ? j : i];
+      x[q] = gmem(e.dsq)[on[q] ? j : i];
 #pragma unroll
       for (int yo = 0; yo < MAXC; yo++) {
         const RowRef rp = row_ref(e, ys[yo], j - 1), rq = row_ref(e, ys[yo], j);
@@ -420,7 +420,7 @@ __device__ void tr_cell_bif(const CmDev &cm, const EnvDev &e, int v, int j, int 
   const int mv = e.mask[v];
   const bool dJ = mv & BJ, dL = mv & BL, dR = mv & BR, dT = mv & BT;
   const int nc = e.nc;
-  float *A = e.alpha;
+  IXG_GLOBAL float *A = gmem(e.alpha);
   float J = tr_init(cm, v, d, 0), L = J, R = J, T = IMPOSSIBLE;
   const int y = cm.cfirst[v], z = cm.cnum[v];
   const int my = e.mask[y], mz = e.mask[z];
@@ -504,7 +504,7 @@ __global__ __launch_bounds__(HB_THREADS) void trhb_scan_kernel(CmDev cm, const E
   const int wl = hmx[last] - hmn[last] + 1;
   const int ncv = ro[last] + (wl > 0 ? wl : 0) - ro[0];
   const int m0 = e.mask[0], nc = e.nc;
-  const float *A = e.alpha;
+  const IXG_GLOBAL float *A = gmem(e.alpha);
   for (int c = tid; c < ncv; c += nt) {
     const int tgt = ro[0] + c;
     const int r = hb_row_of(ro, last, tgt);
@@ -546,11 +546,11 @@ __global__ __launch_bounds__(HB_THREADS) void trhb_scan_kernel(CmDev cm, const E
         }
       }
     }
-    e.rout[c] = J;
-    e.rout[ncv + c] = L;
-    e.rout[2 * ncv + c] = R;
-    e.rout[3 * ncv + c] = T;
-    e.rout[4 * ncv + c] = __int_as_float(br * 8 + bm);
+    gmem(e.rout)[c] = J;
+    gmem(e.rout)[ncv + c] = L;
+    gmem(e.rout)[2 * ncv + c] = R;
+    gmem(e.rout)[3 * ncv + c] = T;
+    gmem(e.rout)[4 * ncv + c] = __int_as_float(br * 8 + bm);
   }
 }
 

@@ -68,9 +68,13 @@ __device__ __forceinline__ size_t tring_at(int32_t ring, int j, int W1) {
 // The window's residues: staged in LDS when the window fits, else read from
 // the device sequence.
 constexpr int TRQ_SEQ_LDS = 32768;
+// Plane rows and decks, read from a window descriptor, in global memory.
+template <class T>
+using Gp = IXG_GLOBAL T;
+
 template <bool LDS>
 struct TSeq {
-  const uint8_t *g;
+  const Gp<uint8_t> *g;
   const uint8_t *l;
   int i0;
   __device__ __forceinline__ int operator[](int i) const { return LDS ? (int)l[i - i0] : (int)g[i]; }
@@ -99,8 +103,8 @@ __device__ __forceinline__ int t_sd(int st) { return st == MP_ST ? 2 : ((st >= M
 // (cm_dpsearch_trunc.rs:1978-2217). P: plane stride (W1); cj / cp: each
 // child's row of j and of j - 1.
 template <typename T, bool LS>
-__device__ void trq_cell(const TrqDev &q, const TrqTabs<T> &tb, const uint16_t *tbl, T *__restrict__ rows,
-                         T *__restrict__ dk, const TSeq<LS> &dsq, int W1, const QdbUnit &u, int st,
+__device__ void trq_cell(const TrqDev &q, const TrqTabs<T> &tb, const uint16_t *tbl, Gp<T> *__restrict__ rows,
+                         Gp<T> *__restrict__ dk, const TSeq<LS> &dsq, int W1, const QdbUnit &u, int st,
                          int cn, int j, int d) {
   using R = Tq<T>;
   const bool fl = q.fill & 2, fr = q.fill & 4, ft = q.fill & 8;
@@ -108,18 +112,18 @@ __device__ void trq_cell(const TrqDev &q, const TrqTabs<T> &tb, const uint16_t *
   const int sd = t_sd(st);
   const T z = R::zero();
   const size_t P = W1;
-  T *out = rows + tring_at(u.ring, j, W1) + d;
+  Gp<T> *out = rows + tring_at(u.ring, j, W1) + d;
   if (st == B_ST) {
     const int kmin = u.k0, kmax = u.k1, DR = W1 + u.cring[2];
-    const T *db = dk + ((size_t)u.deck * W1 + u.cring[1]) * 3 * P;
+    const Gp<T> *db = dk + ((size_t)u.deck * W1 + u.cring[1]) * 3 * P;
     const size_t rs = 3 * P;  // deck row stride
-    const T *yj = rows + tring_at(u.cring[0], j, W1), *yl = yj + P;
+    const Gp<T> *yj = rows + tring_at(u.cring[0], j, W1), *yl = yj + P;
     T jsc = tinit(tb, v, d - sd);
     T lsc = fl ? jsc : z, rsc = fr ? jsc : z, tsc = ft ? jsc : z;
     int jr = (j - kmin) % DR;  // deck row of j - k (j - kmin >= i0 - 1 >= 0)
 #pragma unroll 4
     for (int k = kmin; k <= kmax; k++) {
-      const T *dr = db + jr * rs + (d - k);
+      const Gp<T> *dr = db + jr * rs + (d - k);
       const T lft = dr[0];
       jsc = R::comb(jsc, R::add(lft, yj[k]), tbl);
       if (fl) lsc = R::comb(lsc, R::add(lft, yl[k]), tbl);
@@ -136,13 +140,13 @@ __device__ void trq_cell(const TrqDev &q, const TrqTabs<T> &tb, const uint16_t *
     }
     out[0] = jsc;
     if (ft) out[3 * P] = tsc;
-    const T *dj = db + (size_t)(j % DR) * rs + d;
+    const Gp<T> *dj = db + (size_t)(j % DR) * rs + d;
     if (fl) out[P] = kmin == 0 ? R::comb(lsc, R::max2(dj[0], dj[P]), tbl) : lsc;
     if (fr) out[2 * P] = kmax == d ? R::comb(rsc, R::max2(yj[d], yj[2 * P + d]), tbl) : rsc;
     return;
   }
   const T *ts = tb.tsc + v * MAXC;
-  const T *cj[MAXC], *cp[MAXC];  // children's rows of j and j - 1
+  const Gp<T> *cj[MAXC], *cp[MAXC];  // children's rows of j and j - 1
   T tsr[MAXC];
 #pragma unroll
   for (int yo = 0; yo < MAXC; yo++)
@@ -161,7 +165,7 @@ __device__ void trq_cell(const TrqDev &q, const TrqTabs<T> &tb, const uint16_t *
         if (fl) lsc = R::comb(lsc, R::add(cj[yo][P + d - sd], tsr[yo]), tbl);
         if (fr) rsc = R::comb(rsc, R::add(cj[yo][2 * P + d - sd], tsr[yo]), tbl);
       }
-    T *dr = dk + ((size_t)u.deck * W1 + u.k0 + j % (W1 + u.k1)) * 3 * P + d;
+    Gp<T> *dr = dk + ((size_t)u.deck * W1 + u.k0 + j % (W1 + u.k1)) * 3 * P + d;
     dr[0] = jsc;
     if (fl) dr[P] = lsc;
     if (fr) dr[2 * P] = rsc;
@@ -245,7 +249,7 @@ __device__ void trq_cell(const TrqDev &q, const TrqTabs<T> &tb, const uint16_t *
 // dependence between d's; the J/L walk then fetches four d's of its other
 // inputs at a time.
 template <typename T, bool LS>
-__device__ void trq_il_row(const TrqDev &q, const TrqTabs<T> &tb, const uint16_t *tbl, T *__restrict__ rows,
+__device__ void trq_il_row(const TrqDev &q, const TrqTabs<T> &tb, const uint16_t *tbl, Gp<T> *__restrict__ rows,
                            const TSeq<LS> &dsq, int W1, const QdbUnit &u, int cn, int j, int dx) {
   using R = Tq<T>;
   constexpr int B = 4;
@@ -255,8 +259,8 @@ __device__ void trq_il_row(const TrqDev &q, const TrqTabs<T> &tb, const uint16_t
   const size_t P = W1;
   const T *ts = tb.tsc + v * MAXC;
   const T *oe = tb.oesc + (size_t)v * (KP * KP);
-  T *out = rows + tring_at(u.ring, j, W1);
-  const T *cj[MAXC];
+  Gp<T> *out = rows + tring_at(u.ring, j, W1);
+  const Gp<T> *cj[MAXC];
   T tsr[MAXC];
 #pragma unroll
   for (int yo = 0; yo < MAXC; yo++)
@@ -342,9 +346,9 @@ __global__ __launch_bounds__(QDB_MAX_THREADS) void trqdb_kernel(TrqDev q, TrqTab
   const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wave = tid >> 6;
   const int W1 = wd.w + 1;
   const size_t P = W1;
-  T *rows = (T *)wd.rows, *dk = (T *)wd.begl;
+  Gp<T> *rows = gmem((T *)wd.rows), *dk = gmem((T *)wd.begl);
   TSeq<LS> dsq;
-  dsq.g = wd.dsq;
+  dsq.g = gmem(wd.dsq);
   dsq.i0 = wd.i0;
   dsq.l = nullptr;
   if constexpr (LS) {
@@ -371,7 +375,7 @@ __global__ __launch_bounds__(QDB_MAX_THREADS) void trqdb_kernel(TrqDev q, TrqTab
   }
   for (int b = 0; b < q.nbegl; b++) {
     const int2 k = q.deck[b];
-    T *db = dk + ((size_t)b * W1 + k.x) * 3 * P;
+    Gp<T> *db = dk + ((size_t)b * W1 + k.x) * 3 * P;
     for (int r = wave; r < 3 * (W1 + k.y); r += nt >> 6)
       for (int d = lane; d < W1; d += 64) db[(size_t)r * W1 + d] = (r % 3 == 0 && d == 0) ? tb.bbase0[b] : R::zero();
   }
@@ -413,7 +417,7 @@ __global__ __launch_bounds__(QDB_MAX_THREADS) void trqdb_kernel(TrqDev q, TrqTab
       // the begin states in increasing order, the row held in registers
       float bsc = IMPOSSIBLE;
       int br = 0, bm = MODE_UNKNOWN;
-      T *root = rows + tring_at(u.ring, j, W1) + d;
+      Gp<T> *root = rows + tring_at(u.ring, j, W1) + d;
       T rt[4] = {root[0], root[P], root[2 * P], root[3 * P]};
       constexpr int TB = 8;  // begin states whose cells are fetched together
       for (int e0 = 0; e0 < q.ntb; e0 += TB) {
@@ -422,7 +426,7 @@ __global__ __launch_bounds__(QDB_MAX_THREADS) void trqdb_kernel(TrqDev q, TrqTab
 #pragma unroll
         for (int k = 0; k < TB; k++) {
           tev[k] = q.tblist[min(e0 + k, q.ntb - 1)];
-          const T *yc = rows + tring_at(tev[k].x, j, W1) + d;
+          const Gp<T> *yc = rows + tring_at(tev[k].x, j, W1) + d;
 #pragma unroll
           for (int pl = 0; pl < 4; pl++) val[k][pl] = yc[pl * P];
         }
