@@ -53,12 +53,13 @@ int scratch(ixg_ctx *c, int slot, size_t n, void **out) {
   return IXG_OK;
 }
 
-// 4-byte elements one launch group may put in scratch slot `slot`: 8 GiB
-// (HBM is 288 GB), or under a memory budget half of what the budget leaves
+// 4-byte elements one launch group may put in scratch slot `slot`: 64 GiB
+// (HBM is 288 GB, and a launch of fewer windows than the chip has CUs leaves
+// most of it idle), or under a memory budget half of what the budget leaves
 // (counting the slot's current allocation, which a regrow frees), at least
 // 64 MiB.
 int64_t arena_elems(const ixg_ctx *c, int slot) {
-  const int64_t cap = (int64_t)1 << 31;
+  const int64_t cap = (int64_t)1 << 34;
   if (!c->budget) return cap;
   const size_t held = (int)c->bufs.size() > slot ? c->bufs[slot].cap : 0;
   const int64_t left = (int64_t)c->budget - (int64_t)c->in_use + (int64_t)held;
