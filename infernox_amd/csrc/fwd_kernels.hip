@@ -250,13 +250,22 @@ __global__ __launch_bounds__(64) void fwd_generic(FwdDev p, const uint8_t *seq, 
 // the unihit configuration, every row's M, I and D (after rescaling) stored
 // in natural order k = 0..M at mat + moff[w] (M | I | D, (len+1)*(M+1)
 // each), the specials E N J B C SCALE per row at xmx + xoff[w], and score
-// and totscale at out[2*idx], out[2*idx+1].
+// and totscale at out[2*idx], out[2*idx+1]. A workgroup of 4 * wpw lanes
+// holds wpw windows (4 lanes each): a batch of few windows is spread over
+// more SIMDs instead of packed 16 to a wave.
 __global__ __launch_bounds__(64) void fwd_full(FwdDev p, const uint8_t *seq, const WinDev *wins, int64_t nwin,
                                                const int64_t *moff, const int64_t *xoff, float *mat, float *xmx,
                                                float *out, int unihit) {
-  extern __shared__ float col[];  // [3][nq][64]
-  const int lane = threadIdx.x, z = lane & 3;
-  const int64_t widx = (int64_t)blockIdx.x * 16 + (lane >> 2);
+  // LDS: the transitions [nq][28], DD [nq][4] and emissions [KP][nq][4]
+  // staged once, then the M, D, I columns [3][nq][S]
+  extern __shared__ float lds[];
+  const int lane = threadIdx.x, z = lane & 3, S = blockDim.x;
+  const int nq = p.nq, M = p.m;
+  float *tm = lds, *tdd = tm + nq * 28, *rf = tdd + nq * 4, *col = rf + KP * nq * 4;
+  for (int k = lane; k < nq * 28; k += S) tm[k] = p.tmain[k];
+  for (int k = lane; k < nq * 4; k += S) tdd[k] = p.tddv[k];
+  for (int k = lane; k < KP * nq * 4; k += S) rf[k] = p.rfvv[k];
+  const int64_t widx = (int64_t)blockIdx.x * (S >> 2) + (lane >> 2);
   WinDev w;
   w.off = 0;
   w.len = 0;
@@ -264,11 +273,10 @@ __global__ __launch_bounds__(64) void fwd_full(FwdDev p, const uint8_t *seq, con
   w.a = 0.f;
   w.b = 0.f;
   if (widx < nwin) w = wins[widx];
-  const int nq = p.nq, M = p.m;
-  float *mm = col, *dm = col + nq * 64, *im = col + 2 * nq * 64;
-  for (int q = 0; q < nq; q++) mm[q * 64 + lane] = dm[q * 64 + lane] = im[q * 64 + lane] = 0.f;
+  float *mm = col, *dm = col + nq * S, *im = col + 2 * nq * S;
+  for (int q = 0; q < nq; q++) mm[q * S + lane] = dm[q * S + lane] = im[q * S + lane] = 0.f;
   int maxL = w.len;
-  for (int off = 32; off >= 1; off >>= 1) maxL = max(maxL, __shfl_xor(maxL, off));
+  for (int off = S >> 1; off >= 1; off >>= 1) maxL = max(maxL, __shfl_xor(maxL, off));
   const float L = (float)w.len;
   const float nj = unihit ? 0.0f : 1.0f;
   const float pmove = (2.0f + nj) / (L + 2.0f + nj);  // p7_fwdback.rs:212-252
@@ -289,16 +297,16 @@ __global__ __launch_bounds__(64) void fwd_full(FwdDev p, const uint8_t *seq, con
       ox[5] = 1.f;
     }
   }
-  const float *tm = p.tmain;
+  __syncthreads();
   for (int i = 1; i <= maxL; i++) {
     const bool act = i <= w.len;
     const int x = act ? (int)seq[w.off + i] : 0;
-    const float *rp = p.rfvv + (size_t)x * (nq * 4);
+    const float *rp = rf + (size_t)x * (nq * 4);
     float dcv = 0.f, xev = 0.f;
     const float xbv = xb;
-    float mpv = quad_rsz(mm[(nq - 1) * 64 + lane], z);
-    float dpv = quad_rsz(dm[(nq - 1) * 64 + lane], z);
-    float ipv = quad_rsz(im[(nq - 1) * 64 + lane], z);
+    float mpv = quad_rsz(mm[(nq - 1) * S + lane], z);
+    float dpv = quad_rsz(dm[(nq - 1) * S + lane], z);
+    float ipv = quad_rsz(im[(nq - 1) * S + lane], z);
     for (int q = 0; q < nq; q++) {
       const float *tb = tm + q * 28 + z;
       float sv = xbv * tb[0];
@@ -307,21 +315,21 @@ __global__ __launch_bounds__(64) void fwd_full(FwdDev p, const uint8_t *seq, con
       sv = sv + dpv * tb[12];
       sv = sv * rp[q * 4 + z];
       xev = xev + sv;
-      mpv = mm[q * 64 + lane];
-      dpv = dm[q * 64 + lane];
-      ipv = im[q * 64 + lane];
-      mm[q * 64 + lane] = sv;
-      dm[q * 64 + lane] = dcv;
+      mpv = mm[q * S + lane];
+      dpv = dm[q * S + lane];
+      ipv = im[q * S + lane];
+      mm[q * S + lane] = sv;
+      dm[q * S + lane] = dcv;
       dcv = sv * tb[16];
       const float svi = mpv * tb[20];
-      im[q * 64 + lane] = svi + ipv * tb[24];
+      im[q * S + lane] = svi + ipv * tb[24];
     }
     dcv = quad_rsz(dcv, z);
     dm[lane] = 0.f;
     for (int q = 0; q < nq; q++) {
-      const float d = dcv + dm[q * 64 + lane];
-      dm[q * 64 + lane] = d;
-      dcv = d * p.tddv[q * 4 + z];
+      const float d = dcv + dm[q * S + lane];
+      dm[q * S + lane] = d;
+      dcv = d * tdd[q * 4 + z];
     }
     bool done = false;
     for (int j = 1; j < 4; j++) {
@@ -329,16 +337,16 @@ __global__ __launch_bounds__(64) void fwd_full(FwdDev p, const uint8_t *seq, con
       int cv = 0;
       if (!done) {
         for (int q = 0; q < nq; q++) {
-          const float old = dm[q * 64 + lane];
+          const float old = dm[q * S + lane];
           const float d = dcv + old;
           cv |= d > old;
-          dm[q * 64 + lane] = d;
-          dcv = dcv * p.tddv[q * 4 + z];
+          dm[q * S + lane] = d;
+          dcv = dcv * tdd[q * 4 + z];
         }
       }
       if (p.m >= 100 && !quad_any(cv)) done = true;
     }
-    for (int q = 0; q < nq; q++) xev = xev + dm[q * 64 + lane];
+    for (int q = 0; q < nq; q++) xev = xev + dm[q * S + lane];
     const float p01 = xev + quad_dpp<DPP_QUAD_X1>(xev);
     float xe = p01 + quad_dpp<DPP_QUAD_X2>(p01);
     if (act) {
@@ -354,9 +362,9 @@ __global__ __launch_bounds__(64) void fwd_full(FwdDev p, const uint8_t *seq, con
         xj *= inv;
         xb *= inv;
         for (int q = 0; q < nq; q++) {
-          mm[q * 64 + lane] *= inv;
-          dm[q * 64 + lane] *= inv;
-          im[q * 64 + lane] *= inv;
+          mm[q * S + lane] *= inv;
+          dm[q * S + lane] *= inv;
+          im[q * S + lane] *= inv;
         }
         totscale += (float)log((double)xe);
         scale = xe;
@@ -376,9 +384,9 @@ __global__ __launch_bounds__(64) void fwd_full(FwdDev p, const uint8_t *seq, con
       for (int q = 0; q < nq; q++) {
         const int k = z * nq + q + 1;
         if (k <= M) {
-          om[r + k] = mm[q * 64 + lane];
-          oi[r + k] = im[q * 64 + lane];
-          od[r + k] = dm[q * 64 + lane];
+          om[r + k] = mm[q * S + lane];
+          oi[r + k] = im[q * S + lane];
+          od[r + k] = dm[q * S + lane];
         }
       }
     }
@@ -393,10 +401,14 @@ hipError_t launch_fwd_full(const FwdDev &p, const uint8_t *seq, const WinDev *wi
                            const int64_t *moff, const int64_t *xoff, float *mat, float *xmx, float *out, int unihit,
                            hipStream_t s) {
   if (nwin == 0) return hipSuccess;
-  const size_t lds = (size_t)3 * p.nq * 64 * sizeof(float);
+  // windows per workgroup: 16 once the batch gives every SIMD (1024) two
+  // waves of 16, fewer (a power of two) below that
+  int wpw = 16;
+  while (wpw > 1 && nwin < (int64_t)wpw * 2048) wpw >>= 1;
+  const size_t lds = ((size_t)p.nq * (28 + 4 + KP * 4) + (size_t)3 * p.nq * 4 * wpw) * sizeof(float);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
-  hipLaunchKernelGGL(fwd_full, dim3((unsigned)((nwin + 15) / 16)), dim3(64), lds, s, p, seq, wins, nwin, moff, xoff,
-                     mat, xmx, out, unihit);
+  hipLaunchKernelGGL(fwd_full, dim3((unsigned)((nwin + wpw - 1) / wpw)), dim3(4 * wpw), lds, s, p, seq, wins, nwin,
+                     moff, xoff, mat, xmx, out, unihit);
   return hipGetLastError();
 }
 
