@@ -368,7 +368,9 @@ typedef struct {
   const uint8_t *has_el;
   const int32_t *el_from_ct, *el_from_off, *el_from_idx;
 } ixg_cp9_params;
-/* Uploads the HMM (device-resident until ixg_cp9_free). */
+/* Uploads the HMM (device-resident until ixg_cp9_free). IXG_EINVAL when a
+ * local-end source node is not below the node it ends at (a CP9 built from
+ * a CM never has one) or M exceeds 2100 (the kernels' per-window LDS). */
 int ixg_cp9_create(ixg_ctx *ctx, const ixg_cp9_params *p, ixg_cp9 **out);
 int ixg_cp9_free(ixg_ctx *ctx, ixg_cp9 *h);
 /* One window dsq[i0..j0] (1-based, sentinel-padded, length L): Forward and

@@ -90,3 +90,20 @@ def test_cp9_batch_both_strands(ctx):
     finally:
         g.free()
         tgt.free()
+
+
+def test_cp9_create_rejects_a_local_end_that_does_not_run_forward(ctx):
+    """A local end whose source node is not below its end node has no
+    counterpart in a CP9 built from a CM; the library refuses the model
+    instead of computing its Backward local-end cells in another order."""
+    cm, h = _model(S.cloverleaf(), 6, True)
+    M = h["M"]
+    ct, off, idx = h["el_from_ct"].copy(), h["el_from_off"].copy(), h["el_from_idx"].copy()
+    k = int(np.nonzero(ct[1:M + 1])[0][0]) + 1  # the first end node with a source
+    idx[off[k]] = k  # the source becomes the end node itself
+    bad = dict(h, el_from_idx=idx)
+    _say("cp9 create bad local end", k)
+    with pytest.raises(ixg.IxgError, match="precede"):
+        ixg.CP9(ctx, bad)
+    g = ixg.CP9(ctx, h)  # the untouched model is accepted
+    g.free()
