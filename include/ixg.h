@@ -373,22 +373,32 @@ typedef struct {
  * a CM never has one) or M exceeds 2100 (the kernels' per-window LDS). */
 int ixg_cp9_create(ixg_ctx *ctx, const ixg_cp9_params *p, ixg_cp9 **out);
 int ixg_cp9_free(ixg_ctx *ctx, ixg_cp9 *h);
+/* mode bits: the Forward scans (a parse may start at any position,
+ * do_fwd_scan), the Backward scans (do_bck_scan), the reported scores are the
+ * whole window's (doing_align) rather than the best over positions.
+ * cp9_seq2bands passes FWD_SCAN | BCK_SCAN in search and ALIGN in alignment
+ * (cp9.rs:4063-4066); cp9_seq2bands_trunc passes each scan unless the pass
+ * forces the first (Forward) or final (Backward) residue (cp9.rs:4147-4152).
+ * The posterior pass sums the Backward's begin cells over every row when the
+ * Forward scanned (did_fwd_scan). */
+#define IXG_CP9_FWD_SCAN 1
+#define IXG_CP9_BCK_SCAN 2
+#define IXG_CP9_ALIGN 4
 /* One window dsq[i0..j0] (1-based, sentinel-padded, length L): Forward and
- * Backward with do_scan as cp9_seq2bands passes it (1 in search, 0 in
- * alignment, where doing_align = !do_scan), their matrices (each
+ * Backward in the given mode, their matrices (each
  * [(j0-i0+2)][(M+1)], row ip = position i0+ip-1; any may be NULL), the two
  * best scores (bits) and the posterior HMM bands at tau: pn [6][M+1] =
  * pn_min_m, pn_max_m, pn_min_i, pn_max_i, pn_min_d, pn_max_d, -1 where the
  * state has no band (the CP9Bands fields cp9_fb2hmmbands fills). */
 int ixg_cp9_bands(ixg_ctx *ctx, const ixg_cp9 *h, const uint8_t *dsq, int64_t L, int32_t i0, int32_t j0,
-                  double tau, int do_scan, int32_t *fmx, int32_t *fix, int32_t *fdx, int32_t *bmx, int32_t *bix,
+                  double tau, int mode, int32_t *fmx, int32_t *fix, int32_t *fdx, int32_t *bmx, int32_t *bix,
                   int32_t *bdx, float *fwd_sc, float *bwd_sc, int32_t *pn);
 /* Batched form over a device-resident target (the per-envelope loop that
  * calls cp9_seq2bands, cm_search.rs:2740 / :2816): pn [n][6][M+1];
  * fwd_sc / bwd_sc [n] may be NULL. */
 typedef struct { int32_t i0, j0, strand, pad; } ixg_cp9win;
 int ixg_cp9_bands_batch(ixg_ctx *ctx, const ixg_cp9 *h, const ixg_target *tgt, const ixg_cp9win *win, int64_t n,
-                        double tau, int do_scan, float *fwd_sc, float *bwd_sc, int32_t *pn);
+                        double tau, int mode, float *fwd_sc, float *bwd_sc, int32_t *pn);
 
 /* Per-kernel device time, cumulative since the last reset: HIP events around
  * every launch of each kernel (IXG_K_* index), total milliseconds and launch

@@ -234,7 +234,7 @@ struct Cp9WinDev {
 // Dynamic LDS of the CP9 kernels: two local-end rows, the two serial chains'
 // terms, a transition column, the ILogsum table as 16-bit entries.
 inline size_t cp9_lds_bytes(int M, int nz) { return (size_t)5 * (M + 1) * 4 + (((size_t)nz * 2 + 15) & ~(size_t)15); }
-hipError_t launch_cp9_bands(const Cp9Dev &h, const Cp9WinDev *wins, int64_t n, int do_scan, int32_t thresh,
+hipError_t launch_cp9_bands(const Cp9Dev &h, const Cp9WinDev *wins, int64_t n, int mode, int32_t thresh,
                             float *fsc, float *bsc, int32_t *pn, hipStream_t s);
 
 }  // namespace ixg

@@ -39,12 +39,16 @@ struct CWin {
 // Runs Forward, Backward and the posterior bands of every window; fsc, bsc,
 // pn are host arrays in the windows' order. mats (optional, n == 1): the six
 // matrices of the single window in [(L+1)][(M+1)] order.
-int cp9_run(ixg_ctx *c, const ixg_cp9 *h, const std::vector<CWin> &wins, double tau, int do_scan, float *fsc,
+int cp9_run(ixg_ctx *c, const ixg_cp9 *h, const std::vector<CWin> &wins, double tau, int mode, float *fsc,
             float *bsc, int32_t *pn, int32_t **mats) {
   const int M = h->M, M1 = M + 1;
   const int64_t n = (int64_t)wins.size();
   if (!(tau > 0.0 && tau < 1.0)) {
     g_err = "CP9 bands: tau must be in (0, 1)";
+    return IXG_EINVAL;
+  }
+  if (mode & ~(IXG_CP9_FWD_SCAN | IXG_CP9_BCK_SCAN | IXG_CP9_ALIGN)) {
+    g_err = "CP9 bands: unknown mode bits";
     return IXG_EINVAL;
   }
   const int32_t thresh = band_thresh(tau);
@@ -87,7 +91,7 @@ int cp9_run(ixg_ctx *c, const ixg_cp9 *h, const std::vector<CWin> &wins, double 
     int32_t *d_pn = (int32_t *)(d_b + nw);
     HIPCHK(hipMemcpyAsync(d_w, wd.data(), wbytes, hipMemcpyHostToDevice, c->stream));
     HIPCHK(ktime_begin(c));
-    HIPCHK(launch_cp9_bands(h->dev, d_w, nw, do_scan, thresh, d_f, d_b, d_pn, c->stream));
+    HIPCHK(launch_cp9_bands(h->dev, d_w, nw, mode, thresh, d_f, d_b, d_pn, c->stream));
     HIPCHK(ktime_end(c, IXG_K_CP9));
     std::vector<float> tf(nw), tb(nw);
     std::vector<int32_t> tp((size_t)nw * 6 * M1);
@@ -243,7 +247,7 @@ int ixg_cp9_free(ixg_ctx *c, ixg_cp9 *h) {
 }
 
 int ixg_cp9_bands(ixg_ctx *c, const ixg_cp9 *h, const uint8_t *dsq, int64_t L, int32_t i0, int32_t j0, double tau,
-                  int do_scan, int32_t *fmx, int32_t *fix, int32_t *fdx, int32_t *bmx, int32_t *bix, int32_t *bdx,
+                  int mode, int32_t *fmx, int32_t *fix, int32_t *fdx, int32_t *bmx, int32_t *bix, int32_t *bdx,
                   float *fwd_sc, float *bwd_sc, int32_t *pn) {
   if (!c) return IXG_EINVAL;
   RCHK(check_cp9_window(h, dsq, L, i0, j0));
@@ -251,11 +255,11 @@ int ixg_cp9_bands(ixg_ctx *c, const ixg_cp9 *h, const uint8_t *dsq, int64_t L, i
   RCHK(scratch(c, S_CM_SEQ, (size_t)(L + 2), (void **)&d));
   HIPCHK(hipMemcpyAsync(d, dsq, (size_t)(L + 2), hipMemcpyHostToDevice, c->stream));
   int32_t *mats[6] = {fmx, fix, fdx, bmx, bix, bdx};
-  return cp9_run(c, h, {CWin{d, i0, j0}}, tau, do_scan ? 1 : 0, fwd_sc, bwd_sc, pn, mats);
+  return cp9_run(c, h, {CWin{d, i0, j0}}, tau, mode, fwd_sc, bwd_sc, pn, mats);
 }
 
 int ixg_cp9_bands_batch(ixg_ctx *c, const ixg_cp9 *h, const ixg_target *tgt, const ixg_cp9win *win, int64_t n,
-                        double tau, int do_scan, float *fwd_sc, float *bwd_sc, int32_t *pn) {
+                        double tau, int mode, float *fwd_sc, float *bwd_sc, int32_t *pn) {
   if (!c || !h || !tgt || (n > 0 && (!win || !pn)) || n < 0) {
     g_err = "ixg_cp9_bands_batch: bad arguments";
     return IXG_EINVAL;
@@ -270,7 +274,7 @@ int ixg_cp9_bands_batch(ixg_ctx *c, const ixg_cp9 *h, const ixg_target *tgt, con
     }
     w[k] = CWin{base + (win[k].strand ? L + 2 : 0), win[k].i0, win[k].j0};
   }
-  return cp9_run(c, h, w, tau, do_scan ? 1 : 0, fwd_sc, bwd_sc, pn, nullptr);
+  return cp9_run(c, h, w, tau, mode, fwd_sc, bwd_sc, pn, nullptr);
 }
 
 }  // extern "C"

@@ -103,7 +103,8 @@ __device__ __forceinline__ int32_t chain(const int32_t *u, const int32_t *z, int
   return acc;
 }
 
-__global__ __launch_bounds__(64) void cp9_fwd_kernel(Cp9Dev h, const Cp9WinDev *wins, int do_scan, float *best_out) {
+__global__ __launch_bounds__(64) void cp9_fwd_kernel(Cp9Dev h, const Cp9WinDev *wins, int do_scan, int align,
+                                                    float *best_out) {
   // z[k] = DD(k - 1): the delete chain's self term into node k
   const Lds s = lds_setup(h, O_DD, 1);
   const int t = threadIdx.x, M = h.M, M1 = M + 1, nz = h.nz;
@@ -181,10 +182,11 @@ __global__ __launch_bounds__(64) void cp9_fwd_kernel(Cp9Dev h, const Cp9WinDev *
     }
     __syncthreads();
   }
-  if (t == 1) best_out[w.out] = do_scan ? best : scorify(sca);  // doing_align: the whole window's score
+  if (t == 1) best_out[w.out] = align ? scorify(sca) : best;  // doing_align: the whole window's score
 }
 
-__global__ __launch_bounds__(64) void cp9_bwd_kernel(Cp9Dev h, const Cp9WinDev *wins, int do_scan, float *best_out) {
+__global__ __launch_bounds__(64) void cp9_bwd_kernel(Cp9Dev h, const Cp9WinDev *wins, int do_scan, int align,
+                                                    float *best_out) {
   // z[k] = DD(k): the delete chain's self term out of node k
   const Lds s = lds_setup(h, O_DD, 0);
   const int t = threadIdx.x, M = h.M, M1 = M + 1, nz = h.nz;
@@ -301,7 +303,7 @@ __global__ __launch_bounds__(64) void cp9_bwd_kernel(Cp9Dev h, const Cp9WinDev *
     }
     __syncthreads();
   }
-  if (t == 1) best_out[w.out] = do_scan ? best : scorify(b);  // doing_align: the whole window's score
+  if (t == 1) best_out[w.out] = align ? scorify(b) : best;  // doing_align: the whole window's score
 }
 
 // cp9.rs:1517-1737: posterior mass crept in from both ends, one lane per
@@ -404,14 +406,16 @@ __global__ __launch_bounds__(64) void cp9_post_kernel(Cp9Dev h, const Cp9WinDev 
 
 }  // namespace
 
-hipError_t launch_cp9_bands(const Cp9Dev &h, const Cp9WinDev *wins, int64_t n, int do_scan, int32_t thresh,
+hipError_t launch_cp9_bands(const Cp9Dev &h, const Cp9WinDev *wins, int64_t n, int mode, int32_t thresh,
                             float *fsc, float *bsc, int32_t *pn, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const dim3 grid((unsigned)n), block(64);
   const size_t lds = cp9_lds_bytes(h.M, h.nz);
-  hipLaunchKernelGGL(cp9_fwd_kernel, grid, block, lds, s, h, wins, do_scan, fsc);
-  hipLaunchKernelGGL(cp9_bwd_kernel, grid, block, lds, s, h, wins, do_scan, bsc);
-  hipLaunchKernelGGL(cp9_post_kernel, grid, block, lds, s, h, wins, do_scan, thresh, pn);
+  const int fscan = mode & IXG_CP9_FWD_SCAN ? 1 : 0, bscan = mode & IXG_CP9_BCK_SCAN ? 1 : 0;
+  const int align = mode & IXG_CP9_ALIGN ? 1 : 0;
+  hipLaunchKernelGGL(cp9_fwd_kernel, grid, block, lds, s, h, wins, fscan, align, fsc);
+  hipLaunchKernelGGL(cp9_bwd_kernel, grid, block, lds, s, h, wins, bscan, align, bsc);
+  hipLaunchKernelGGL(cp9_post_kernel, grid, block, lds, s, h, wins, fscan, thresh, pn);
   return hipGetLastError();
 }
 

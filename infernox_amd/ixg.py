@@ -766,16 +766,18 @@ class CP9:
             check(_cp9_lib().ixg_cp9_free(self.ctx.h, self.h), "ixg_cp9_free")
             self.h = C.c_void_p()
 
-    def bands(self, dsq, L, i0, j0, tau, scan=True, matrices=False):
-        """One window: dict(pn [6, M+1], fsc, bsc[, fm, fi, fd, bm, bi, bd [(L'+1), M+1]])."""
+    def bands(self, dsq, L, i0, j0, tau, scan=True, matrices=False, mode=None):
+        """One window: dict(pn [6, M+1], fsc, bsc[, fm, fi, fd, bm, bi, bd [(L'+1), M+1]]).
+        mode: IXG_CP9_* bits; by default search (both scans) or, with
+        scan=False, alignment."""
         M1, Lw = self.M + 1, j0 - i0 + 1
         d = np.ascontiguousarray(dsq, np.uint8)
         mats = [np.zeros((Lw + 1) * M1, np.int32) for _ in range(6)] if matrices else None
         pn = np.zeros(6 * M1, np.int32)
         fs, bs = C.c_float(), C.c_float()
         mp = [(_i32p(m) if matrices else None) for m in (mats or [None] * 6)]
-        check(_cp9_lib().ixg_cp9_bands(self.ctx.h, self.h, u8ptr(d), int(L), int(i0), int(j0), float(tau), int(scan),
-                                  *mp, C.byref(fs), C.byref(bs), _i32p(pn)), "ixg_cp9_bands")
+        check(_cp9_lib().ixg_cp9_bands(self.ctx.h, self.h, u8ptr(d), int(L), int(i0), int(j0), float(tau),
+                                  cp9_mode(scan, mode), *mp, C.byref(fs), C.byref(bs), _i32p(pn)), "ixg_cp9_bands")
         out = dict(pn=pn.reshape(6, M1), fsc=fs.value, bsc=bs.value)
         if matrices:
             for k, m in zip(("fm", "fi", "fd", "bm", "bi", "bd"), mats):
@@ -783,7 +785,18 @@ class CP9:
         return out
 
 
-def cp9_bands_batch(ctx, cp9, tgt, wins, tau, scan=True):
+IXG_CP9_FWD_SCAN, IXG_CP9_BCK_SCAN, IXG_CP9_ALIGN = 1, 2, 4
+
+
+def cp9_mode(scan=True, mode=None):
+    """The mode bits of cp9_seq2bands: search scans both DPs, alignment
+    reports whole-window scores; an explicit mode wins."""
+    if mode is not None:
+        return int(mode)
+    return IXG_CP9_FWD_SCAN | IXG_CP9_BCK_SCAN if scan else IXG_CP9_ALIGN
+
+
+def cp9_bands_batch(ctx, cp9, tgt, wins, tau, scan=True, mode=None):
     """wins: list of (i0, j0, strand). Returns (pn [n, 6, M+1], fsc [n], bsc [n])."""
     n, M1 = len(wins), cp9.M + 1
     arr = (Cp9Win * max(n, 1))()
@@ -791,6 +804,6 @@ def cp9_bands_batch(ctx, cp9, tgt, wins, tau, scan=True):
         arr[k].i0, arr[k].j0, arr[k].strand, arr[k].pad = int(i0), int(j0), int(strand), 0
     pn = np.zeros(max(n, 1) * 6 * M1, np.int32)
     fs, bs = np.zeros(max(n, 1), np.float32), np.zeros(max(n, 1), np.float32)
-    check(_cp9_lib().ixg_cp9_bands_batch(ctx.h, cp9.h, tgt.h, arr, n, float(tau), int(scan), f32ptr(fs), f32ptr(bs),
+    check(_cp9_lib().ixg_cp9_bands_batch(ctx.h, cp9.h, tgt.h, arr, n, float(tau), cp9_mode(scan, mode), f32ptr(fs), f32ptr(bs),
                                     _i32p(pn)), "ixg_cp9_bands_batch")
     return pn[:n * 6 * M1].reshape(n, 6, M1), fs[:n], bs[:n]

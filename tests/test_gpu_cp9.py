@@ -107,3 +107,31 @@ def test_cp9_create_rejects_a_local_end_that_does_not_run_forward(ctx):
         ixg.CP9(ctx, bad)
     g = ixg.CP9(ctx, h)  # the untouched model is accepted
     g.free()
+
+
+@pytest.mark.parametrize("mode", [ixg.IXG_CP9_FWD_SCAN, ixg.IXG_CP9_BCK_SCAN, 0,
+                                  ixg.IXG_CP9_FWD_SCAN | ixg.IXG_CP9_ALIGN])
+def test_cp9_truncated_pass_modes(ctx, mode):
+    """cp9_seq2bands_trunc (cp9.rs:4130-4156) scans the Forward unless the pass
+    forces the first residue and the Backward unless it forces the final one,
+    and always reports best-over-position scores; every combination of the
+    mode bits matches the oracle run with the same flags."""
+    fscan, bscan = bool(mode & ixg.IXG_CP9_FWD_SCAN), bool(mode & ixg.IXG_CP9_BCK_SCAN)
+    align = bool(mode & ixg.IXG_CP9_ALIGN)
+    for ss, seed, el in ((S.cloverleaf(), 7, True), (".<<<..>>>..<<..>>.", 8, False)):
+        cm, h = _model(ss, seed, el)
+        dsq, L = _window(cm, seed, 12)
+        o = OracleCP9(h)
+        g = ixg.CP9(ctx, h)
+        try:
+            for i0, j0 in ((1, L), (5, L - 2)):
+                _say("cp9 mode", mode, seed, i0, j0)
+                f, b = o.forward(dsq, i0, j0, fscan, align), o.backward(dsq, i0, j0, bscan, align)
+                got = g.bands(dsq, L, i0, j0, 1e-7, matrices=True, mode=mode)
+                for k, m in (("fm", f["mmx"]), ("fi", f["imx"]), ("fd", f["dmx"]), ("bm", b["mmx"]),
+                             ("bi", b["imx"]), ("bd", b["dmx"])):
+                    assert np.array_equal(got[k], m), (k, mode, i0, j0)
+                assert _bits(got["fsc"]) == _bits(f["sc"]) and _bits(got["bsc"]) == _bits(b["sc"]), (mode, i0)
+                assert np.array_equal(got["pn"], o.bands(dsq, i0, j0, 1e-7, f, b, fscan)), (mode, i0, j0)
+        finally:
+            g.free()
