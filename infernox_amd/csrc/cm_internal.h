@@ -224,6 +224,16 @@ struct Cp9Dev {
   // cell there), and the other nodes it ends at, in decreasing order.
   const int32_t *bel_nM, *bel_reset, *bel_off, *bel_k;  // [M+1], [M+1], [M+2], list
   const int32_t *ilog;                                  // ILogsum table
+  // LDS slots of the wavefront kernels: per node, the slot holding its
+  // local-end cell while a later node of the next row may read it (Forward)
+  // and the slot holding its match cell while a local-end cell of the next
+  // row may read it (Backward); -1 for nodes that need none
+  const int32_t *fslot, *bslot;  // [M+1] each
+  int nsf, nsb;                  // slots in use
+  // per entry of the local-end lists: the slot of el_idx's source (Forward)
+  // and of bel_k's end node (Backward); list lengths
+  const int32_t *el_slot, *bel_slot;
+  int nel, nbel;
 };
 struct Cp9WinDev {
   const uint8_t *dsq;
@@ -234,6 +244,13 @@ struct Cp9WinDev {
 // Dynamic LDS of the CP9 kernels: two local-end rows, the two serial chains'
 // terms, a transition column, the ILogsum table as 16-bit entries.
 inline size_t cp9_lds_bytes(int M, int nz) { return (size_t)5 * (M + 1) * 4 + (((size_t)nz * 2 + 15) & ~(size_t)15); }
+// Dynamic LDS of the wavefront kernels: nslots slots of 65 lanes, two
+// local-end rows, the nlist entries of the local-end lists (node and slot,
+// 16 bits each), the ILogsum table.
+inline size_t cp9_wave_lds_bytes(int M, int nz, int nslots, int nlist) {
+  return (size_t)nslots * 65 * 4 + (size_t)2 * (M + 1) * 4 + (size_t)nlist * 4 + (((size_t)nz * 2 + 15) & ~(size_t)15);
+}
+constexpr size_t CP9_LDS_MAX = 65536 - 64;
 hipError_t launch_cp9_bands(const Cp9Dev &h, const Cp9WinDev *wins, int64_t n, int mode, int32_t thresh,
                             float *fsc, float *bsc, int32_t *pn, hipStream_t s);
 

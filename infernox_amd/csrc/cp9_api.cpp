@@ -186,6 +186,42 @@ int ixg_cp9_create(ixg_ctx *c, const ixg_cp9_params *p, ixg_cp9 **out) {
     }
     bel_off[M1] = (int32_t)bel_k.size();
   }
+  // LDS slots of the wavefront kernels, coloured in node order. The reader of
+  // a slot runs one lane (one step) behind its writer, so a slot is free for a
+  // new occupant two nodes after its occupant's last reader.
+  std::vector<int32_t> fslot(M1, -1), bslot(M1, -1);
+  int nsf = 0, nsb = 0;
+  {  // Forward: a source node's local-end cell, read up to the last node it ends at (M+1: the row end)
+    std::vector<int> last(M1, -1), ends;
+    for (int k = 1; k <= M + 1; k++)
+      for (int c = 0; c < p->el_from_ct[k]; c++) {
+        const int y = p->el_from_idx[p->el_from_off[k] + c];
+        last[y] = std::max(last[y], k);
+      }
+    for (int y = 0; y <= M; y++) {
+      if (last[y] < 0) continue;
+      int q = 0;
+      while (q < (int)ends.size() && ends[q] + 2 > y) q++;
+      if (q == (int)ends.size()) ends.push_back(0);
+      ends[q] = last[y];
+      fslot[y] = q;
+    }
+    nsf = (int)ends.size();
+  }
+  if (p->el) {  // Backward: an end node's match cell, read down to the lowest source ending there
+    std::vector<int> low(M1, M1), ends;
+    for (int k = 1; k <= M - 1; k++)
+      for (int c = 0; c < p->el_from_ct[k]; c++) low[k] = std::min(low[k], p->el_from_idx[p->el_from_off[k] + c]);
+    for (int e = M - 1; e >= 1; e--) {
+      if (low[e] == M1) continue;
+      int q = 0;
+      while (q < (int)ends.size() && ends[q] - 2 < e) q++;
+      if (q == (int)ends.size()) ends.push_back(0);
+      ends[q] = low[e];
+      bslot[e] = q;
+    }
+    nsb = (int)ends.size();
+  }
   std::vector<int32_t> blob;
   auto put = [&](const int32_t *x, size_t k) {
     const size_t o = blob.size();
@@ -199,6 +235,15 @@ int ixg_cp9_create(ixg_ctx *c, const ixg_cp9_params *p, ixg_cp9 **out) {
   const size_t o_x = nidx ? put(p->el_from_idx, nidx) : put(&zero, 1);
   const size_t o_bn = put(bel_nM.data(), M1), o_br = put(bel_reset.data(), M1), o_bo = put(bel_off.data(), M1 + 1);
   const size_t o_bk = bel_k.empty() ? put(&zero, 1) : put(bel_k.data(), bel_k.size());
+  std::vector<int32_t> el_slot(std::max(nidx, 1), -1), bel_slot(std::max<size_t>(bel_k.size(), 1), -1);
+  for (int x = 0; x < nidx; x++) el_slot[x] = fslot[p->el_from_idx[x]];
+  for (size_t x = 0; x < bel_k.size(); x++) bel_slot[x] = bslot[bel_k[x]];
+  if (M > 32767) {  // list entries are staged in LDS as 16-bit node numbers
+    g_err = "ixg_cp9_create: more than 32767 nodes";
+    return IXG_EINVAL;
+  }
+  const size_t o_fs = put(fslot.data(), M1), o_bs = put(bslot.data(), M1);
+  const size_t o_es = put(el_slot.data(), el_slot.size()), o_bes = put(bel_slot.data(), bel_slot.size());
   const size_t o_l = put(ilog.data(), ilog.size());
   const size_t o_h = blob.size();
   blob.resize(o_h + (M1 + 3) / 4 + 1, 0);
@@ -233,6 +278,14 @@ int ixg_cp9_create(ixg_ctx *c, const ixg_cp9_params *p, ixg_cp9 **out) {
   d.bel_off = h->d + o_bo;
   d.bel_k = h->d + o_bk;
   d.nz = nz;
+  d.fslot = h->d + o_fs;
+  d.bslot = h->d + o_bs;
+  d.nsf = nsf;
+  d.nsb = nsb;
+  d.el_slot = h->d + o_es;
+  d.bel_slot = h->d + o_bes;
+  d.nel = nidx;
+  d.nbel = (int)bel_k.size();
   d.ilog = h->d + o_l;
   d.has_el = (const uint8_t *)(h->d + o_h);
   *out = h;

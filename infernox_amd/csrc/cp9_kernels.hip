@@ -399,6 +399,323 @@ __global__ __launch_bounds__(64) void cp9_post_kernel(Cp9Dev h, const Cp9WinDev 
   }
 }
 
+// ---- anti-diagonal wavefront forms -----------------------------------------
+// One wave per window, lane p holding one row of a 64-row strip: the Forward
+// visits row s0 + 1 + p at node k = step - p, the Backward row L - 1 - s0 - p
+// at node k = M - step + p. A cell's neighbours in the row before (the row
+// after, for the Backward) were computed by lane p - 1 one step earlier and
+// arrive by a one-lane DPP shift; lane 0 reads them from the matrices in
+// global memory (the previous strip's last row) and from a local-end row that
+// lane 63 left in LDS. The delete chain and the end/begin sum run along each
+// lane's own row, one node per step, so a strip takes M + 64 steps instead of
+// the row-at-a-time form's M dependent steps per row. Values a lane needs
+// from the row before at an earlier node (the local-end cells a Forward match
+// cell sums; the match cells a Backward local-end cell sums) go through LDS
+// slots, one column per lane, that the host assigns so that no slot is
+// rewritten before its last reader has read it.
+
+// the row before's cell from lane p - 1; lane 0 gets `own`
+__device__ __forceinline__ int32_t from_prev_lane(int32_t own, int32_t v) {
+  return __builtin_amdgcn_update_dpp(own, v, 0x138 /* wave_shr:1 */, 0xF, 0xF, false);
+}
+
+struct WaveLds {
+  int32_t *slot;    // [nslots][65]: column c holds lane c - 1's values
+  int32_t *elb;     // [2][M+1]: the local-end row of a strip's last row, for the next strip's lane 0
+  int16_t *lnode;   // the local-end list's entries: node ...
+  int16_t *lslot;   // ... and its slot
+  uint16_t *tbl;
+};
+__device__ __forceinline__ WaveLds wave_lds(const Cp9Dev &h, int nslots, const int32_t *node, const int32_t *slot,
+                                            int nlist) {
+  extern __shared__ int32_t lds[];
+  int16_t *ln = (int16_t *)(lds + nslots * 65 + 2 * (h.M + 1));
+  WaveLds s{lds, lds + nslots * 65, ln, ln + nlist, (uint16_t *)(ln + 2 * nlist)};
+  const IXG_GLOBAL int32_t *ilog = gmem(h.ilog), *gn = gmem(node), *gs = gmem(slot);
+  for (int x = threadIdx.x; x < h.nz; x += 64) s.tbl[x] = (uint16_t)ilog[x];
+  for (int x = threadIdx.x; x < nlist; x += 64) {
+    s.lnode[x] = (int16_t)gn[x];
+    s.lslot[x] = (int16_t)gs[x];
+  }
+  return s;
+}
+
+// One node's transitions (cp9O order: MM IM DM BM | MI II DI MD | ID DD ME
+// MEL) and emission scores, loaded a step ahead of their use.
+struct NodeW {
+  int4 a, b, c;
+  int32_t ms, is;
+};
+__device__ __forceinline__ NodeW node_w(const G &g, int k, int x) {
+  const IXG_GLOBAL int32_t *t = g.otsc + (size_t)k * O_NT;
+  return NodeW{make_int4(t[0], t[1], t[2], t[3]), make_int4(t[4], t[5], t[6], t[7]),
+               make_int4(t[8], t[9], t[10], t[11]), g.msc[(size_t)k * KP + x], g.isc[(size_t)k * KP + x]};
+}
+#define N_MM(n) (n).a.x
+#define N_IM(n) (n).a.y
+#define N_DM(n) (n).a.z
+#define N_BM(n) (n).a.w
+#define N_MI(n) (n).b.x
+#define N_II(n) (n).b.y
+#define N_DI(n) (n).b.z
+#define N_MD(n) (n).b.w
+#define N_ID(n) (n).c.x
+#define N_DD(n) (n).c.y
+#define N_ME(n) (n).c.z
+#define N_MEL(n) (n).c.w
+
+#define LW(a, b) lsum((a), (b), s.tbl, nz)
+
+__global__ __launch_bounds__(64) void cp9_fwd_wave(Cp9Dev h, const Cp9WinDev *wins, int do_scan, int align,
+                                                  float *best_out) {
+  const WaveLds s = wave_lds(h, h.nsf, h.el_idx, h.el_slot, h.nel);
+  const int t = threadIdx.x, M = h.M, M1 = M + 1, nz = h.nz;
+  const Cp9WinDev w = wins[blockIdx.x];
+  const G g = gtabs(h, w);
+  const IXG_GLOBAL int32_t *fslot = gmem(h.fslot);
+  const int i0 = w.i0, j0 = w.j0, L = j0 - i0 + 1;
+  const size_t cells = (size_t)(L + 1) * M1;
+  IXG_GLOBAL int32_t *gm = gmem(w.mx), *gi = gm + cells, *gd = gi + cells;
+  const bool el = h.el != 0;
+  // row 0, position i0 - 1 (cp9.rs:1309-1333); its local-end row is -INFTY
+  for (int k = t; k <= M; k += 64) {
+    gm[k] = k ? NINF : 0;
+    gi[k] = NINF;
+    s.elb[k] = NINF;
+  }
+  __syncthreads();
+  if (t == 0) {
+    int32_t d = NINF;
+    gd[0] = NINF;
+    for (int k = 1; k <= M; k++) {
+      d = LW(LW(wa(k == 1 ? 0 : NINF, TS(k - 1, O_MD)), wa(NINF, TS(k - 1, O_ID))), wa(d, TS(k - 1, O_DD)));
+      gd[k] = d;
+    }
+  }
+  __syncthreads();
+  float best = t == 0 ? scorify(wa(gd[M], TS(M, O_DM))) : -1e36f;
+  for (int s0 = 0; s0 < L; s0 += 64) {
+    const int r = s0 + 1 + t, par = (s0 >> 6) & 1;
+    const bool live = r <= L;
+    const int x = live ? g.dsq[i0 + r - 1] : 0;
+    const IXG_GLOBAL int32_t *um = gm + (size_t)s0 * M1, *ui = gi + (size_t)s0 * M1, *ud = gd + (size_t)s0 * M1;
+    IXG_GLOBAL int32_t *om = gm + (size_t)(live ? r : 0) * M1, *oi = gi + (size_t)(live ? r : 0) * M1,
+                       *od = gd + (size_t)(live ? r : 0) * M1;
+    const int32_t *uel = s.elb + par * M1;
+    int32_t *nel = s.elb + (par ^ 1) * M1;
+    int32_t oM = NINF, oI = NINF, oD = NINF, oE = NINF;  // this lane's cells at its previous node
+    int32_t lM = NINF, lI = NINF, lD = NINF;             // the row before at the previous node
+    int32_t mp0 = NINF, endsc = NINF;
+    NodeW pn = node_w(g, 0, x), nx = node_w(g, min(max(-t, 0), M), x);  // the previous and next nodes
+    for (int step = 0; step <= M + 63; step++) {
+      const int k = step - t;
+      const bool act = live && k >= 0 && k <= M;
+      const NodeW cn = nx;
+      nx = node_w(g, min(max(k + 1, 0), M), x);
+      int32_t gM = NINF, gI = NINF, gD = NINF, gE = NINF;
+      if (t == 0 && act) {
+        gM = um[k];
+        gI = ui[k];
+        gD = ud[k];
+        gE = uel[k];
+      }
+      const int32_t uM = from_prev_lane(gM, oM), uI = from_prev_lane(gI, oI), uD = from_prev_lane(gD, oD),
+                    uE = from_prev_lane(gE, oE);
+      if (act) {
+        int32_t cM, cI, cD, cE;
+        if (k == 0) {  // cp9.rs:1362-1371
+          cM = do_scan ? 0 : NINF;
+          cI = max(wa(LW(LW(wa(uM, N_MI(cn)), wa(uI, N_II(cn))), wa(uD, N_DI(cn))), cn.is), NINF);
+          cD = NINF;
+          cE = NINF;
+          mp0 = uM;
+        } else {  // cp9.rs:1377-1418
+          int32_t sc = LW(LW(wa(lM, N_MM(pn)), wa(lI, N_IM(pn))), LW(wa(lD, N_DM(pn)), wa(mp0, N_BM(cn))));
+          const int n = g.el_ct[k], o = g.el_off[k];
+          for (int c = 0; c < n; c++)
+            sc = LW(sc, t == 0 ? uel[s.lnode[o + c]] : s.slot[s.lslot[o + c] * 65 + t]);
+          cM = max(wa(sc, cn.ms), NINF);
+          cI = max(wa(LW(LW(wa(uM, N_MI(cn)), wa(uI, N_II(cn))), wa(uD, N_DI(cn))), cn.is), NINF);
+          cD = LW(LW(wa(oM, N_MD(pn)), wa(oI, N_ID(pn))), wa(oD, N_DD(pn)));
+          cE = (el && g.has_el[k]) ? LW(wa(cM, N_MEL(cn)), wa(uE, h.el_selfsc)) : NINF;
+          endsc = LW(endsc, wa(cM, N_ME(cn)));
+        }
+        om[k] = cM;
+        oi[k] = cI;
+        od[k] = cD;
+        const int sl = fslot[k];
+        if (sl >= 0) s.slot[sl * 65 + t + 1] = cE;
+        if (t == 63) nel[k] = cE;
+        if (k == M) {  // cp9.rs:1420-1430
+          endsc = LW(LW(endsc, wa(cD, N_DM(cn))), wa(cI, N_IM(cn)));
+          const int n = g.el_ct[M + 1], o = g.el_off[M + 1];
+          for (int c = 0; c < n; c++) endsc = LW(endsc, s.slot[s.lslot[o + c] * 65 + t + 1]);
+          if (scorify(endsc) > best) best = scorify(endsc);
+          if (align && r == L) best_out[w.out] = scorify(endsc);  // doing_align: the whole window's score
+        }
+        oM = cM;
+        oI = cI;
+        oD = cD;
+        oE = cE;
+        lM = uM;
+        lI = uI;
+        lD = uD;
+        pn = cn;
+      }
+    }
+    __syncthreads();
+  }
+  for (int off = 32; off >= 1; off >>= 1) best = fmaxf(best, __shfl_xor(best, off));
+  if (!align && t == 0) best_out[w.out] = best;
+}
+
+__global__ __launch_bounds__(64) void cp9_bwd_wave(Cp9Dev h, const Cp9WinDev *wins, int do_scan, int align,
+                                                  float *best_out) {
+  const WaveLds s = wave_lds(h, h.nsb, h.bel_k, h.bel_slot, h.nbel);
+  const int t = threadIdx.x, M = h.M, M1 = M + 1, nz = h.nz;
+  const Cp9WinDev w = wins[blockIdx.x];
+  const G g = gtabs(h, w);
+  const IXG_GLOBAL int32_t *bslot = gmem(h.bslot);
+  const int i0 = w.i0, j0 = w.j0, L = j0 - i0 + 1;
+  const size_t cells = (size_t)(L + 1) * M1;
+  IXG_GLOBAL int32_t *gm = gmem(w.mx) + 3 * cells, *gi = gm + cells, *gd = gi + cells;
+  const bool el = h.el != 0;
+  __syncthreads();
+  if (t == 0) {  // row L, position j0 (cp9.rs:2029-2055); its local-end row for strip 0's lane 0
+    const int dj0 = g.dsq[j0];
+    int32_t *e = s.elb;
+    IXG_GLOBAL int32_t *m = gm + (size_t)L * M1, *i = gi + (size_t)L * M1, *d = gd + (size_t)L * M1;
+    for (int k = 1; k <= M; k++) e[k] = NINF;
+    if (el)
+      for (int c = 0; c < g.el_ct[M + 1]; c++) e[g.el_idx[g.el_off[M + 1] + c]] = 0;
+    e[0] = NINF;
+    m[M] = wa(LW(wa(e[M], TS(M, O_MEL)), TS(M, O_ME)), MSC(M, dj0));
+    i[M] = wa(TS(M, O_IM), ISC(M, dj0));
+    int32_t dn = TS(M, O_DM);
+    d[M] = dn;
+    for (int k = M - 1; k >= 1; k--) {
+      int32_t v = TS(k, O_ME);
+      v = LW(v, wa(dn, TS(k, O_MD)));
+      if (el) v = LW(v, wa(e[k], TS(k, O_MEL)));
+      m[k] = wa(v, MSC(k, dj0));
+      i[k] = wa(wa(dn, TS(k, O_ID)), ISC(k, dj0));
+      dn = wa(dn, TS(k, O_DD));
+      d[k] = dn;
+    }
+    m[0] = wa(dn, TS(0, O_MD));
+    i[0] = wa(wa(dn, TS(0, O_ID)), ISC(0, dj0));
+    d[0] = NINF;
+  }
+  __syncthreads();
+  float best = -1e36f;
+  // rows L-1 .. 1 (positions j0-1 .. i0, cp9.rs:2057-2169), then row 0
+  // (position i0 - 1, cp9.rs:2171-2200), 64 to a strip
+  for (int s0 = 0; s0 < L; s0 += 64) {
+    const int cur = L - 1 - s0 - t, par = (s0 >> 6) & 1;
+    const bool live = cur >= 0;
+    const int x = cur > 0 ? g.dsq[i0 + cur - 1] : 0;
+    const int up = L - s0;  // lane 0's row after
+    const IXG_GLOBAL int32_t *um = gm + (size_t)up * M1, *ui = gi + (size_t)up * M1;
+    IXG_GLOBAL int32_t *om = gm + (size_t)(live ? cur : 0) * M1, *oi = gi + (size_t)(live ? cur : 0) * M1,
+                       *od = gd + (size_t)(live ? cur : 0) * M1;
+    const int32_t *uel = s.elb + par * M1;
+    int32_t *nel = s.elb + (par ^ 1) * M1;
+    int32_t oM = NINF, oI = NINF, oD = NINF, oE = NINF;  // this lane's cells at its previous node (k + 1)
+    int32_t lM = NINF;                                   // the row after's match cell at k + 1
+    int32_t mM = NINF, b = NINF;                         // the row after's M(M); the begin sum
+    NodeW nx = node_w(g, min(M + t, M), x);              // the next node
+    for (int step = 0; step <= M + 63; step++) {
+      const int k = M - step + t;
+      const bool act = live && k >= 0 && k <= M;
+      const NodeW cn = nx;
+      nx = node_w(g, min(max(k - 1, 0), M), x);
+      int32_t gM = NINF, gI = NINF, gE = NINF;
+      if (t == 0 && act) {
+        gM = um[k];
+        gI = ui[k];
+        gE = uel[k];
+      }
+      const int32_t uM = from_prev_lane(gM, oM), uI = from_prev_lane(gI, oI), uE = from_prev_lane(gE, oE);
+      if (act) {
+        int32_t cM = NINF, cI = NINF, cD, cE = NINF;
+        if (k == M) mM = uM;
+        // local-end cell (cp9.rs:2068-2119): its end nodes' next-row match
+        // cells in decreasing order, the scan reset, its self loop
+        if (cur > 0 && k >= 1) {
+          int32_t v = (k == M && el && g.has_el[M]) ? wa(NINF, h.el_selfsc) : NINF;
+          const int nM = g.bel_nM[k];
+          for (int c = 0; c < nM; c++) v = LW(v, mM);
+          if (do_scan && g.bel_reset[k]) v = 0;
+          const int o = g.bel_off[k], n = g.bel_off[k + 1] - o;
+          for (int c = 0; c < n; c++) v = LW(v, t == 0 ? um[s.lnode[o + c]] : s.slot[s.lslot[o + c] * 65 + t]);
+          if (el && k < M && g.has_el[k]) v = LW(v, wa(uE, h.el_selfsc));
+          cE = v;
+        }
+        if (k == M) {
+          cD = wa(uI, N_DI(cn));
+          if (do_scan) cD = LW(cD, N_DM(cn));
+          if (cur > 0) {
+            cM = wa(wa(uI, N_MI(cn)), cn.ms);
+            if (el && g.has_el[M]) cM = LW(cM, wa(wa(NINF, h.el_selfsc), N_MEL(cn)));
+            cI = wa(wa(uI, N_II(cn)), cn.is);
+            if (do_scan) {
+              cM = LW(cM, LW(wa(cE, N_MEL(cn)), N_ME(cn)));
+              cI = LW(cI, N_IM(cn));
+            }
+          }
+        } else if (k >= 1) {
+          cD = LW(LW(wa(lM, N_DM(cn)), wa(uI, N_DI(cn))), wa(oD, N_DD(cn)));
+          if (cur > 0) {
+            int32_t mv = LW(LW(wa(lM, N_MM(cn)), wa(uI, N_MI(cn))), wa(oD, N_MD(cn)));
+            if (el && g.has_el[k]) mv = LW(mv, wa(cE, N_MEL(cn)));
+            cM = wa(mv, cn.ms);
+            cI = wa(LW(LW(wa(lM, N_IM(cn)), wa(uI, N_II(cn))), wa(oD, N_ID(cn))), cn.is);
+            if (do_scan) cM = LW(cM, N_ME(cn));
+          }
+        } else {  // k = 0 (cp9.rs:2147-2167): I(0), D(0), the begin cell
+          cD = NINF;
+          if (cur > 0)
+            cI = wa(LW(LW(wa(lM, N_IM(cn)), wa(uI, N_II(cn))), wa(oD, N_ID(cn))), cn.is);
+          b = LW(b, wa(uI, N_MI(cn)));
+          b = LW(b, wa(oD, N_MD(cn)));
+          cM = b;
+          if (scorify(b) > best) best = scorify(b);
+          if (align && cur == 0) best_out[w.out] = scorify(b);  // doing_align: the whole window's score
+        }
+        if (k >= 1) b = LW(b, wa(uM, N_BM(cn)));
+        om[k] = cM;
+        oi[k] = cI;
+        od[k] = cD;
+        const int sl = bslot[k];
+        if (sl >= 0) s.slot[sl * 65 + t + 1] = cM;
+        if (t == 63) nel[k] = cE;
+        oM = cM;
+        oI = cI;
+        oD = cD;
+        oE = cE;
+        lM = uM;
+      }
+    }
+    __syncthreads();
+  }
+  for (int off = 32; off >= 1; off >>= 1) best = fmaxf(best, __shfl_xor(best, off));
+  if (!align && t == 0) best_out[w.out] = best;
+}
+
+#undef LW
+#undef N_MM
+#undef N_IM
+#undef N_DM
+#undef N_BM
+#undef N_MI
+#undef N_II
+#undef N_DI
+#undef N_MD
+#undef N_ID
+#undef N_DD
+#undef N_ME
+#undef N_MEL
+
 #undef TS
 #undef MSC
 #undef ISC
@@ -413,8 +730,16 @@ hipError_t launch_cp9_bands(const Cp9Dev &h, const Cp9WinDev *wins, int64_t n, i
   const size_t lds = cp9_lds_bytes(h.M, h.nz);
   const int fscan = mode & IXG_CP9_FWD_SCAN ? 1 : 0, bscan = mode & IXG_CP9_BCK_SCAN ? 1 : 0;
   const int align = mode & IXG_CP9_ALIGN ? 1 : 0;
-  hipLaunchKernelGGL(cp9_fwd_kernel, grid, block, lds, s, h, wins, fscan, align, fsc);
-  hipLaunchKernelGGL(cp9_bwd_kernel, grid, block, lds, s, h, wins, bscan, align, bsc);
+  // the wavefront forms when their local-end slots fit in LDS
+  const size_t lf = cp9_wave_lds_bytes(h.M, h.nz, h.nsf, h.nel), lb = cp9_wave_lds_bytes(h.M, h.nz, h.nsb, h.nbel);
+  if (lf <= CP9_LDS_MAX)
+    hipLaunchKernelGGL(cp9_fwd_wave, grid, block, lf, s, h, wins, fscan, align, fsc);
+  else
+    hipLaunchKernelGGL(cp9_fwd_kernel, grid, block, lds, s, h, wins, fscan, align, fsc);
+  if (lb <= CP9_LDS_MAX)
+    hipLaunchKernelGGL(cp9_bwd_wave, grid, block, lb, s, h, wins, bscan, align, bsc);
+  else
+    hipLaunchKernelGGL(cp9_bwd_kernel, grid, block, lds, s, h, wins, bscan, align, bsc);
   hipLaunchKernelGGL(cp9_post_kernel, grid, block, lds, s, h, wins, fscan, thresh, pn);
   return hipGetLastError();
 }
