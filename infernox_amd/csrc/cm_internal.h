@@ -234,21 +234,30 @@ struct Cp9Dev {
   // and of bel_k's end node (Backward); list lengths
   const int32_t *el_slot, *bel_slot;
   int nel, nbel;
+  // per node, two packed words staged in LDS by the wavefront kernels.
+  // Forward (nodes 0..M+1): list offset << 16 | list length; slot (16 bits,
+  // 0xFFFF = none) | has_el << 16. Backward (nodes 0..M): list offset << 16 |
+  // list length; slot | has_el << 16 | ends at M+1 << 17 | times it ends at
+  // M << 18.
+  const uint32_t *finfo, *binfo;  // [2][M+2] each
+  int wave;                       // bit 0: the Forward wavefront may run, bit 1: the Backward
 };
 struct Cp9WinDev {
   const uint8_t *dsq;
   int32_t i0, j0;
   int64_t out;  // the window's index in the caller's order
-  int32_t *mx;  // Forward M, I, D then Backward M, I, D, each [(L+1)][(M+1)]
+  int32_t *mx;  // Forward M, I, D then Backward M, I, D, each [(L+1)][(M+1)], then 64 words
+                // the wavefront kernels' idle lanes store to
 };
 // Dynamic LDS of the CP9 kernels: two local-end rows, the two serial chains'
 // terms, a transition column, the ILogsum table as 16-bit entries.
 inline size_t cp9_lds_bytes(int M, int nz) { return (size_t)5 * (M + 1) * 4 + (((size_t)nz * 2 + 15) & ~(size_t)15); }
 // Dynamic LDS of the wavefront kernels: nslots slots of 65 lanes, two
-// local-end rows, the nlist entries of the local-end lists (node and slot,
+// local-end rows and two match rows, the packed node words, the nlist entries of the local-end lists (node and slot,
 // 16 bits each), the ILogsum table.
 inline size_t cp9_wave_lds_bytes(int M, int nz, int nslots, int nlist) {
-  return (size_t)nslots * 65 * 4 + (size_t)2 * (M + 1) * 4 + (size_t)nlist * 4 + (((size_t)nz * 2 + 15) & ~(size_t)15);
+  return (size_t)nslots * 65 * 4 + (size_t)4 * (M + 1) * 4 + (size_t)2 * (M + 2) * 4 + (size_t)nlist * 4 +
+         (((size_t)nz * 2 + 15) & ~(size_t)15);
 }
 constexpr size_t CP9_LDS_MAX = 65536 - 64;
 hipError_t launch_cp9_bands(const Cp9Dev &h, const Cp9WinDev *wins, int64_t n, int mode, int32_t thresh,

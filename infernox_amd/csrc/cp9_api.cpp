@@ -67,7 +67,7 @@ int cp9_run(ixg_ctx *c, const ixg_cp9 *h, const std::vector<CWin> &wins, double 
     std::vector<int64_t> woff;
     while (g1 < n) {
       const int L = wins[order[g1]].j0 - wins[order[g1]].i0 + 1;
-      const int64_t need = 6 * (int64_t)(L + 1) * M1;
+      const int64_t need = 6 * (int64_t)(L + 1) * M1 + 64;
       if (g1 > g0 && elems + need > budget) break;
       woff.push_back(elems);
       elems += need;
@@ -242,7 +242,24 @@ int ixg_cp9_create(ixg_ctx *c, const ixg_cp9_params *p, ixg_cp9 **out) {
     g_err = "ixg_cp9_create: more than 32767 nodes";
     return IXG_EINVAL;
   }
+  // packed per-node words of the wavefront kernels (16-bit fields)
+  std::vector<int32_t> finfo(2 * (M + 2), 0), binfo(2 * (M + 2), 0);
+  int wave = (nidx < 65536 && nsf < 32768 ? 1 : 0) | (bel_k.size() < 65536 && nsb < 32768 ? 2 : 0);
+  for (int k = 0; k <= M + 1; k++) {
+    finfo[k] = (int32_t)((uint32_t)p->el_from_off[k] << 16 | (uint32_t)(p->el_from_ct[k] & 0xFFFF));
+    const int sl = k <= M ? fslot[k] : -1, he = k <= M ? (p->has_el[k] != 0) : 0;
+    finfo[M + 2 + k] = (int32_t)(((uint32_t)sl & 0xFFFF) | (uint32_t)he << 16);
+    if (p->el_from_ct[k] > 0xFFFF) wave &= ~1;
+  }
+  for (int k = 0; k <= M; k++) {
+    const int n = bel_k.empty() ? 0 : bel_off[k + 1] - bel_off[k];
+    binfo[k] = (int32_t)((uint32_t)(bel_k.empty() ? 0 : bel_off[k]) << 16 | (uint32_t)n);
+    binfo[M + 2 + k] = (int32_t)(((uint32_t)bslot[k] & 0xFFFF) | (uint32_t)(p->has_el[k] != 0) << 16 |
+                                 (uint32_t)(bel_reset[k] != 0) << 17 | (uint32_t)bel_nM[k] << 18);
+    if (bel_nM[k] >= (1 << 14)) wave &= ~2;
+  }
   const size_t o_fs = put(fslot.data(), M1), o_bs = put(bslot.data(), M1);
+  const size_t o_fi = put(finfo.data(), finfo.size()), o_bi = put(binfo.data(), binfo.size());
   const size_t o_es = put(el_slot.data(), el_slot.size()), o_bes = put(bel_slot.data(), bel_slot.size());
   const size_t o_l = put(ilog.data(), ilog.size());
   const size_t o_h = blob.size();
@@ -286,6 +303,9 @@ int ixg_cp9_create(ixg_ctx *c, const ixg_cp9_params *p, ixg_cp9 **out) {
   d.bel_slot = h->d + o_bes;
   d.nel = nidx;
   d.nbel = (int)bel_k.size();
+  d.finfo = (const uint32_t *)(h->d + o_fi);
+  d.binfo = (const uint32_t *)(h->d + o_bi);
+  d.wave = wave;
   d.ilog = h->d + o_l;
   d.has_el = (const uint8_t *)(h->d + o_h);
   *out = h;

@@ -422,16 +422,21 @@ __device__ __forceinline__ int32_t from_prev_lane(int32_t own, int32_t v) {
 struct WaveLds {
   int32_t *slot;    // [nslots][65]: column c holds lane c - 1's values
   int32_t *elb;     // [2][M+1]: the local-end row of a strip's last row, for the next strip's lane 0
+  int32_t *mb;      // [2][M+1]: the same row's match cells (Backward)
+  uint32_t *info;   // [2][M+2]: the packed per-node words
   int16_t *lnode;   // the local-end list's entries: node ...
   int16_t *lslot;   // ... and its slot
   uint16_t *tbl;
 };
 __device__ __forceinline__ WaveLds wave_lds(const Cp9Dev &h, int nslots, const int32_t *node, const int32_t *slot,
-                                            int nlist) {
+                                            int nlist, const uint32_t *info) {
   extern __shared__ int32_t lds[];
-  int16_t *ln = (int16_t *)(lds + nslots * 65 + 2 * (h.M + 1));
-  WaveLds s{lds, lds + nslots * 65, ln, ln + nlist, (uint16_t *)(ln + 2 * nlist)};
+  uint32_t *in = (uint32_t *)(lds + nslots * 65 + 4 * (h.M + 1));
+  int16_t *ln = (int16_t *)(in + 2 * (h.M + 2));
+  WaveLds s{lds, lds + nslots * 65, lds + nslots * 65 + 2 * (h.M + 1), in, ln, ln + nlist, (uint16_t *)(ln + 2 * nlist)};
   const IXG_GLOBAL int32_t *ilog = gmem(h.ilog), *gn = gmem(node), *gs = gmem(slot);
+  const IXG_GLOBAL uint32_t *gi = gmem(info);
+  for (int x = threadIdx.x; x < 2 * (h.M + 2); x += 64) s.info[x] = gi[x];
   for (int x = threadIdx.x; x < h.nz; x += 64) s.tbl[x] = (uint16_t)ilog[x];
   for (int x = threadIdx.x; x < nlist; x += 64) {
     s.lnode[x] = (int16_t)gn[x];
@@ -468,11 +473,10 @@ __device__ __forceinline__ NodeW node_w(const G &g, int k, int x) {
 
 __global__ __launch_bounds__(64) void cp9_fwd_wave(Cp9Dev h, const Cp9WinDev *wins, int do_scan, int align,
                                                   float *best_out) {
-  const WaveLds s = wave_lds(h, h.nsf, h.el_idx, h.el_slot, h.nel);
+  const WaveLds s = wave_lds(h, h.nsf, h.el_idx, h.el_slot, h.nel, h.finfo);
   const int t = threadIdx.x, M = h.M, M1 = M + 1, nz = h.nz;
   const Cp9WinDev w = wins[blockIdx.x];
   const G g = gtabs(h, w);
-  const IXG_GLOBAL int32_t *fslot = gmem(h.fslot);
   const int i0 = w.i0, j0 = w.j0, L = j0 - i0 + 1;
   const size_t cells = (size_t)(L + 1) * M1;
   IXG_GLOBAL int32_t *gm = gmem(w.mx), *gi = gm + cells, *gd = gi + cells;
@@ -507,22 +511,29 @@ __global__ __launch_bounds__(64) void cp9_fwd_wave(Cp9Dev h, const Cp9WinDev *wi
     int32_t lM = NINF, lI = NINF, lD = NINF;             // the row before at the previous node
     int32_t mp0 = NINF, endsc = NINF;
     NodeW pn = node_w(g, 0, x), nx = node_w(g, min(max(-t, 0), M), x);  // the previous and next nodes
+    // lane 0: the row before's cells at its next node, read a step ahead
+    // (every lane loads, at a clamped node, so that the step's memory
+    // operations are the same on every path and a wait for them never has
+    // to wait for the step's stores)
+    int32_t nM = um[0], nI = ui[0], nD = ud[0];
+    IXG_GLOBAL int32_t *sink = gm + 6 * cells + t;
     for (int step = 0; step <= M + 63; step++) {
       const int k = step - t;
       const bool act = live && k >= 0 && k <= M;
       const NodeW cn = nx;
       nx = node_w(g, min(max(k + 1, 0), M), x);
-      int32_t gM = NINF, gI = NINF, gD = NINF, gE = NINF;
-      if (t == 0 && act) {
-        gM = um[k];
-        gI = ui[k];
-        gD = ud[k];
-        gE = uel[k];
+      const int32_t gM = nM, gI = nI, gD = nD, gE = (t == 0 && act) ? uel[k] : NINF;
+      {
+        const int kn = min(max(k + 1, 0), M);
+        nM = um[kn];
+        nI = ui[kn];
+        nD = ud[kn];
       }
+      int32_t cM = NINF, cI = NINF, cD = NINF;
       const int32_t uM = from_prev_lane(gM, oM), uI = from_prev_lane(gI, oI), uD = from_prev_lane(gD, oD),
                     uE = from_prev_lane(gE, oE);
       if (act) {
-        int32_t cM, cI, cD, cE;
+        int32_t cE;
         if (k == 0) {  // cp9.rs:1362-1371
           cM = do_scan ? 0 : NINF;
           cI = max(wa(LW(LW(wa(uM, N_MI(cn)), wa(uI, N_II(cn))), wa(uD, N_DI(cn))), cn.is), NINF);
@@ -531,24 +542,23 @@ __global__ __launch_bounds__(64) void cp9_fwd_wave(Cp9Dev h, const Cp9WinDev *wi
           mp0 = uM;
         } else {  // cp9.rs:1377-1418
           int32_t sc = LW(LW(wa(lM, N_MM(pn)), wa(lI, N_IM(pn))), LW(wa(lD, N_DM(pn)), wa(mp0, N_BM(cn))));
-          const int n = g.el_ct[k], o = g.el_off[k];
+          const uint32_t i0w = s.info[k];
+          const int n = i0w & 0xFFFF, o = i0w >> 16;
           for (int c = 0; c < n; c++)
             sc = LW(sc, t == 0 ? uel[s.lnode[o + c]] : s.slot[s.lslot[o + c] * 65 + t]);
           cM = max(wa(sc, cn.ms), NINF);
           cI = max(wa(LW(LW(wa(uM, N_MI(cn)), wa(uI, N_II(cn))), wa(uD, N_DI(cn))), cn.is), NINF);
           cD = LW(LW(wa(oM, N_MD(pn)), wa(oI, N_ID(pn))), wa(oD, N_DD(pn)));
-          cE = (el && g.has_el[k]) ? LW(wa(cM, N_MEL(cn)), wa(uE, h.el_selfsc)) : NINF;
+          cE = (el && (s.info[M + 2 + k] >> 16 & 1)) ? LW(wa(cM, N_MEL(cn)), wa(uE, h.el_selfsc)) : NINF;
           endsc = LW(endsc, wa(cM, N_ME(cn)));
         }
-        om[k] = cM;
-        oi[k] = cI;
-        od[k] = cD;
-        const int sl = fslot[k];
+        const int sl = (int16_t)(s.info[M + 2 + k] & 0xFFFF);
         if (sl >= 0) s.slot[sl * 65 + t + 1] = cE;
         if (t == 63) nel[k] = cE;
         if (k == M) {  // cp9.rs:1420-1430
           endsc = LW(LW(endsc, wa(cD, N_DM(cn))), wa(cI, N_IM(cn)));
-          const int n = g.el_ct[M + 1], o = g.el_off[M + 1];
+          const uint32_t iw = s.info[M + 1];
+          const int n = iw & 0xFFFF, o = iw >> 16;
           for (int c = 0; c < n; c++) endsc = LW(endsc, s.slot[s.lslot[o + c] * 65 + t + 1]);
           if (scorify(endsc) > best) best = scorify(endsc);
           if (align && r == L) best_out[w.out] = scorify(endsc);  // doing_align: the whole window's score
@@ -562,6 +572,9 @@ __global__ __launch_bounds__(64) void cp9_fwd_wave(Cp9Dev h, const Cp9WinDev *wi
         lD = uD;
         pn = cn;
       }
+      *(act ? om + k : sink) = cM;
+      *(act ? oi + k : sink) = cI;
+      *(act ? od + k : sink) = cD;
     }
     __syncthreads();
   }
@@ -571,11 +584,10 @@ __global__ __launch_bounds__(64) void cp9_fwd_wave(Cp9Dev h, const Cp9WinDev *wi
 
 __global__ __launch_bounds__(64) void cp9_bwd_wave(Cp9Dev h, const Cp9WinDev *wins, int do_scan, int align,
                                                   float *best_out) {
-  const WaveLds s = wave_lds(h, h.nsb, h.bel_k, h.bel_slot, h.nbel);
+  const WaveLds s = wave_lds(h, h.nsb, h.bel_k, h.bel_slot, h.nbel, h.binfo);
   const int t = threadIdx.x, M = h.M, M1 = M + 1, nz = h.nz;
   const Cp9WinDev w = wins[blockIdx.x];
   const G g = gtabs(h, w);
-  const IXG_GLOBAL int32_t *bslot = gmem(h.bslot);
   const int i0 = w.i0, j0 = w.j0, L = j0 - i0 + 1;
   const size_t cells = (size_t)(L + 1) * M1;
   IXG_GLOBAL int32_t *gm = gmem(w.mx) + 3 * cells, *gi = gm + cells, *gd = gi + cells;
@@ -589,7 +601,8 @@ __global__ __launch_bounds__(64) void cp9_bwd_wave(Cp9Dev h, const Cp9WinDev *wi
     if (el)
       for (int c = 0; c < g.el_ct[M + 1]; c++) e[g.el_idx[g.el_off[M + 1] + c]] = 0;
     e[0] = NINF;
-    m[M] = wa(LW(wa(e[M], TS(M, O_MEL)), TS(M, O_ME)), MSC(M, dj0));
+    int32_t *mb = s.mb;  // row L's match cells for strip 0's lane 0
+    m[M] = mb[M] = wa(LW(wa(e[M], TS(M, O_MEL)), TS(M, O_ME)), MSC(M, dj0));
     i[M] = wa(TS(M, O_IM), ISC(M, dj0));
     int32_t dn = TS(M, O_DM);
     d[M] = dn;
@@ -597,12 +610,12 @@ __global__ __launch_bounds__(64) void cp9_bwd_wave(Cp9Dev h, const Cp9WinDev *wi
       int32_t v = TS(k, O_ME);
       v = LW(v, wa(dn, TS(k, O_MD)));
       if (el) v = LW(v, wa(e[k], TS(k, O_MEL)));
-      m[k] = wa(v, MSC(k, dj0));
+      m[k] = mb[k] = wa(v, MSC(k, dj0));
       i[k] = wa(wa(dn, TS(k, O_ID)), ISC(k, dj0));
       dn = wa(dn, TS(k, O_DD));
       d[k] = dn;
     }
-    m[0] = wa(dn, TS(0, O_MD));
+    m[0] = mb[0] = wa(dn, TS(0, O_MD));
     i[0] = wa(wa(dn, TS(0, O_ID)), ISC(0, dj0));
     d[0] = NINF;
   }
@@ -618,37 +631,45 @@ __global__ __launch_bounds__(64) void cp9_bwd_wave(Cp9Dev h, const Cp9WinDev *wi
     const IXG_GLOBAL int32_t *um = gm + (size_t)up * M1, *ui = gi + (size_t)up * M1;
     IXG_GLOBAL int32_t *om = gm + (size_t)(live ? cur : 0) * M1, *oi = gi + (size_t)(live ? cur : 0) * M1,
                        *od = gd + (size_t)(live ? cur : 0) * M1;
-    const int32_t *uel = s.elb + par * M1;
-    int32_t *nel = s.elb + (par ^ 1) * M1;
+    const int32_t *uel = s.elb + par * M1, *umb = s.mb + par * M1;
+    int32_t *nel = s.elb + (par ^ 1) * M1, *nmb = s.mb + (par ^ 1) * M1;
     int32_t oM = NINF, oI = NINF, oD = NINF, oE = NINF;  // this lane's cells at its previous node (k + 1)
     int32_t lM = NINF;                                   // the row after's match cell at k + 1
     int32_t mM = NINF, b = NINF;                         // the row after's M(M); the begin sum
     NodeW nx = node_w(g, min(M + t, M), x);              // the next node
+    // lane 0: the row after's cells at its next node, read a step ahead
+    // (every lane loads, at a clamped node, as in the Forward)
+    int32_t nM = um[M], nI = ui[M];
+    IXG_GLOBAL int32_t *sink = gmem(w.mx) + 6 * cells + t;
     for (int step = 0; step <= M + 63; step++) {
       const int k = M - step + t;
       const bool act = live && k >= 0 && k <= M;
       const NodeW cn = nx;
       nx = node_w(g, min(max(k - 1, 0), M), x);
-      int32_t gM = NINF, gI = NINF, gE = NINF;
-      if (t == 0 && act) {
-        gM = um[k];
-        gI = ui[k];
-        gE = uel[k];
+      const int32_t gM = nM, gI = nI, gE = (t == 0 && act) ? uel[k] : NINF;
+      {
+        const int kn = min(max(k - 1, 0), M);
+        nM = um[kn];
+        nI = ui[kn];
       }
+      int32_t cM = NINF, cI = NINF, cD = NINF;
       const int32_t uM = from_prev_lane(gM, oM), uI = from_prev_lane(gI, oI), uE = from_prev_lane(gE, oE);
       if (act) {
-        int32_t cM = NINF, cI = NINF, cD, cE = NINF;
+        int32_t cE = NINF;
         if (k == M) mM = uM;
         // local-end cell (cp9.rs:2068-2119): its end nodes' next-row match
         // cells in decreasing order, the scan reset, its self loop
+        const uint32_t i1w = s.info[M + 2 + k];
+        const bool hel = i1w >> 16 & 1;
         if (cur > 0 && k >= 1) {
-          int32_t v = (k == M && el && g.has_el[M]) ? wa(NINF, h.el_selfsc) : NINF;
-          const int nM = g.bel_nM[k];
+          int32_t v = (k == M && el && hel) ? wa(NINF, h.el_selfsc) : NINF;
+          const int nM = i1w >> 18;
           for (int c = 0; c < nM; c++) v = LW(v, mM);
-          if (do_scan && g.bel_reset[k]) v = 0;
-          const int o = g.bel_off[k], n = g.bel_off[k + 1] - o;
-          for (int c = 0; c < n; c++) v = LW(v, t == 0 ? um[s.lnode[o + c]] : s.slot[s.lslot[o + c] * 65 + t]);
-          if (el && k < M && g.has_el[k]) v = LW(v, wa(uE, h.el_selfsc));
+          if (do_scan && (i1w >> 17 & 1)) v = 0;
+          const uint32_t i0w = s.info[k];
+          const int o = i0w >> 16, n = i0w & 0xFFFF;
+          for (int c = 0; c < n; c++) v = LW(v, t == 0 ? umb[s.lnode[o + c]] : s.slot[s.lslot[o + c] * 65 + t]);
+          if (el && k < M && hel) v = LW(v, wa(uE, h.el_selfsc));
           cE = v;
         }
         if (k == M) {
@@ -656,7 +677,7 @@ __global__ __launch_bounds__(64) void cp9_bwd_wave(Cp9Dev h, const Cp9WinDev *wi
           if (do_scan) cD = LW(cD, N_DM(cn));
           if (cur > 0) {
             cM = wa(wa(uI, N_MI(cn)), cn.ms);
-            if (el && g.has_el[M]) cM = LW(cM, wa(wa(NINF, h.el_selfsc), N_MEL(cn)));
+            if (el && hel) cM = LW(cM, wa(wa(NINF, h.el_selfsc), N_MEL(cn)));
             cI = wa(wa(uI, N_II(cn)), cn.is);
             if (do_scan) {
               cM = LW(cM, LW(wa(cE, N_MEL(cn)), N_ME(cn)));
@@ -667,7 +688,7 @@ __global__ __launch_bounds__(64) void cp9_bwd_wave(Cp9Dev h, const Cp9WinDev *wi
           cD = LW(LW(wa(lM, N_DM(cn)), wa(uI, N_DI(cn))), wa(oD, N_DD(cn)));
           if (cur > 0) {
             int32_t mv = LW(LW(wa(lM, N_MM(cn)), wa(uI, N_MI(cn))), wa(oD, N_MD(cn)));
-            if (el && g.has_el[k]) mv = LW(mv, wa(cE, N_MEL(cn)));
+            if (el && hel) mv = LW(mv, wa(cE, N_MEL(cn)));
             cM = wa(mv, cn.ms);
             cI = wa(LW(LW(wa(lM, N_IM(cn)), wa(uI, N_II(cn))), wa(oD, N_ID(cn))), cn.is);
             if (do_scan) cM = LW(cM, N_ME(cn));
@@ -683,18 +704,21 @@ __global__ __launch_bounds__(64) void cp9_bwd_wave(Cp9Dev h, const Cp9WinDev *wi
           if (align && cur == 0) best_out[w.out] = scorify(b);  // doing_align: the whole window's score
         }
         if (k >= 1) b = LW(b, wa(uM, N_BM(cn)));
-        om[k] = cM;
-        oi[k] = cI;
-        od[k] = cD;
-        const int sl = bslot[k];
+        const int sl = (int16_t)(i1w & 0xFFFF);
         if (sl >= 0) s.slot[sl * 65 + t + 1] = cM;
-        if (t == 63) nel[k] = cE;
+        if (t == 63) {
+          nel[k] = cE;
+          nmb[k] = cM;
+        }
         oM = cM;
         oI = cI;
         oD = cD;
         oE = cE;
         lM = uM;
       }
+      *(act ? om + k : sink) = cM;
+      *(act ? oi + k : sink) = cI;
+      *(act ? od + k : sink) = cD;
     }
     __syncthreads();
   }
@@ -732,11 +756,11 @@ hipError_t launch_cp9_bands(const Cp9Dev &h, const Cp9WinDev *wins, int64_t n, i
   const int align = mode & IXG_CP9_ALIGN ? 1 : 0;
   // the wavefront forms when their local-end slots fit in LDS
   const size_t lf = cp9_wave_lds_bytes(h.M, h.nz, h.nsf, h.nel), lb = cp9_wave_lds_bytes(h.M, h.nz, h.nsb, h.nbel);
-  if (lf <= CP9_LDS_MAX)
+  if ((h.wave & 1) && lf <= CP9_LDS_MAX)
     hipLaunchKernelGGL(cp9_fwd_wave, grid, block, lf, s, h, wins, fscan, align, fsc);
   else
     hipLaunchKernelGGL(cp9_fwd_kernel, grid, block, lds, s, h, wins, fscan, align, fsc);
-  if (lb <= CP9_LDS_MAX)
+  if ((h.wave & 2) && lb <= CP9_LDS_MAX)
     hipLaunchKernelGGL(cp9_bwd_wave, grid, block, lb, s, h, wins, bscan, align, bsc);
   else
     hipLaunchKernelGGL(cp9_bwd_kernel, grid, block, lds, s, h, wins, bscan, align, bsc);
