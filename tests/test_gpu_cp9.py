@@ -135,3 +135,72 @@ def test_cp9_truncated_pass_modes(ctx, mode):
                 assert np.array_equal(got["pn"], o.bands(dsq, i0, j0, 1e-7, f, b, fscan)), (mode, i0, j0)
         finally:
             g.free()
+
+
+def _slots_needed(h):
+    """The wavefront kernels' LDS slots for this HMM (the host's interval
+    colouring): Forward, over local-end sources; Backward, over end nodes."""
+    M, ct, off, idx = h["M"], h["el_from_ct"], h["el_from_off"], h["el_from_idx"]
+    last, low = {}, {}
+    for k in range(1, M + 2):
+        for c in range(ct[k]):
+            y = int(idx[off[k] + c])
+            last[y] = max(last.get(y, -1), k)
+            if k < M:
+                low[k] = min(low.get(k, 1 << 30), y)
+
+    def colour(items, free):
+        ends = []
+        for a, b in items:
+            for q, e in enumerate(ends):
+                if free(e, a):
+                    ends[q] = b
+                    break
+            else:
+                ends.append(b)
+        return len(ends)
+    nsf = colour(sorted(last.items()), lambda e, a: e + 2 <= a)
+    nsb = colour(sorted(low.items(), reverse=True), lambda e, a: e - 2 >= a)
+    return nsf, nsb
+
+
+def test_cp9_row_parallel_fallback_on_deeply_nested_local_ends(ctx):
+    """A 200-pair stem nests 200 local ends inside one another, more than the
+    wavefront kernels' LDS slots hold in either direction, so the
+    row-parallel kernels (lanes over nodes, serial delete chain) run; they
+    match the oracle in search and alignment mode, single and batched."""
+    ss = "<" * 200 + "." * 6 + ">" * 200 + "." * 30
+    cm, h = _model(ss, 9, True)
+    M = h["M"]
+    nsf, nsb = _slots_needed(h)
+    nz, nlist = 11495, len(h["el_from_idx"])
+    fixed = 4 * (M + 1) * 4 + 2 * (M + 2) * 4 + nlist * 4 + ((nz * 2 + 15) // 16) * 16
+    assert nsf * 65 * 4 + fixed > 65472 and nsb * 65 * 4 + fixed > 65472, (nsf, nsb)
+    dsq, L = _window(cm, 9, 10)
+    o = OracleCP9(h)
+    g = ixg.CP9(ctx, h)
+    try:
+        for scan in (True, False):
+            _say("cp9 fallback", M, nsf, nsb, scan)
+            f, b = o.forward(dsq, 1, L, scan, not scan), o.backward(dsq, 1, L, scan, not scan)
+            got = g.bands(dsq, L, 1, L, 1e-7, scan, matrices=True)
+            for k, m in (("fm", f["mmx"]), ("fi", f["imx"]), ("fd", f["dmx"]), ("bm", b["mmx"]),
+                         ("bi", b["imx"]), ("bd", b["dmx"])):
+                assert np.array_equal(got[k], m), (k, scan)
+            assert _bits(got["fsc"]) == _bits(f["sc"]) and _bits(got["bsc"]) == _bits(b["sc"]), scan
+            assert np.array_equal(got["pn"], o.bands(dsq, 1, L, 1e-7, f, b, scan)), scan
+        tgt = ixg.Target(ctx, dsq, L)
+        try:
+            wins = [(1, L, 0), (5, L - 9, 1), (30, L - 40, 0)]
+            rc = np.concatenate([[255], np.where(dsq[1:L + 1] < 4, 3 - dsq[1:L + 1], dsq[1:L + 1])[::-1], [255]])
+            rc = rc.astype(np.uint8)
+            pn, fs, bs = ixg.cp9_bands_batch(ctx, g, tgt, wins, 1e-7, True)
+            for k, (i0, j0, strand) in enumerate(wins):
+                d = rc if strand else dsq
+                f, b = o.forward(d, i0, j0, True, False), o.backward(d, i0, j0, True, False)
+                assert np.array_equal(pn[k], o.bands(d, i0, j0, 1e-7, f, b, True)), k
+                assert _bits(fs[k]) == _bits(f["sc"]) and _bits(bs[k]) == _bits(b["sc"]), k
+        finally:
+            tgt.free()
+    finally:
+        g.free()
