@@ -2,6 +2,7 @@
 infrastructure only: the checker, never the thing measured."""
 import ctypes as C
 import os
+import threading
 
 import numpy as np
 
@@ -66,9 +67,21 @@ class F3Opts(C.Structure):
 
 
 _lib = None
+# Lazy set-up runs once under this lock: ctypes creates a function object on
+# first attribute access, so two threads configuring at once can leave a
+# caller holding one whose argtypes are not yet set.
+INIT_LOCK = threading.RLock()
 
 
 def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    with INIT_LOCK:
+        return _init()
+
+
+def _init():
     global _lib
     if _lib is None:
         if not os.path.exists(LIB):

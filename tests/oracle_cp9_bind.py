@@ -5,7 +5,7 @@ import ctypes as C
 
 import numpy as np
 
-from oracle_bind import lib as _lib
+from oracle_bind import INIT_LOCK, lib as _lib
 
 _i32p = C.POINTER(C.c_int32)
 _i64p = C.POINTER(C.c_int64)
@@ -22,6 +22,13 @@ _ready = False
 
 
 def lib():
+    if _ready:
+        return _lib()
+    with INIT_LOCK:
+        return _init()
+
+
+def _init():
     global _ready
     L = _lib()
     if not _ready:
