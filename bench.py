@@ -12,6 +12,7 @@ import json
 import math
 import os
 import sys
+import threading
 import time
 
 import numpy as np
@@ -510,6 +511,122 @@ def cpu_baseline(p, d, L, M, maxw, tasks, nt, budget_s, threads):
                        f"f3_filter_sequence (C restatement, not infernox), {threads} threads, {dt:.1f} s")
 
 
+def serving_workload(n_req=1200, n_models=200, seed=5, median=5000.0, sigma=1.5):
+    """Request stream for the serving loop: (model, sequence) pairs with lengths
+    log-normal (median 5 kb, sigma 1.5) clipped to [100, 1e6] nt, iid ACGU,
+    models drawn uniformly from a pool of 200 p7 profiles with consensus
+    lengths uniform in [50, 400]."""
+    rng = np.random.default_rng(seed)
+    Ms = rng.integers(50, 401, n_models)
+    lens = np.clip(np.exp(np.log(median) + sigma * rng.standard_normal(n_req)), 100, 1_000_000).astype(np.int64)
+    mids = rng.integers(0, n_models, n_req)
+    base = synth.iid_dsq(int(lens.max()), seed + 1)
+    # each request's sequence is a window of one long iid sequence (cheap to make)
+    starts = [int(rng.integers(0, int(lens.max()) - int(Lq) + 1)) for Lq in lens]
+    seqs = []
+    for st, Lq in zip(starts, lens):
+        dq = np.empty(int(Lq) + 2, np.uint8)
+        dq[0] = dq[-1] = 255
+        dq[1:-1] = base[1 + st:1 + st + int(Lq)]
+        seqs.append(dq)
+    return Ms, mids, lens, seqs
+
+
+def time_serving(ctx, nstreams=4, qps_fracs=(0.5, 1.0, 2.0, 4.0)):
+    """Serving loop (continuous batching over nstreams streams) against plain
+    per-request calls (ixg_target_create + ixg_f3_filter_tasks, one request at
+    a time) on the same requests. Plain latency under an arrival process is a
+    single FIFO server with the measured per-request service times."""
+    Ms, mids, lens, seqs = serving_workload()
+    models = []
+    for k, M in enumerate(Ms):
+        pk = synth.consensus_profile(int(M), 500 + k)
+        pk.update(dict(lmmu=-8.5, lmlambda=0.69, lvmu=-9.0, lvlambda=0.69, lftau=-5.5, lflambda=0.69))
+        maxw = int(1.25 * M) + 20
+        o = ixg.F3Opts(f1=0.35, f1b=1.0, f2=1.0, f2b=1.0, f3=0.02, f3b=0.02, cmw=int(M), maxw=maxw, opts=7)
+        models.append((ixg.P7(ctx, pk, maxw), o))
+    n = len(lens)
+    res = 2 * int(lens.sum())  # residues searched: both strands
+
+    def plain_one(k):
+        g, o = models[mids[k]]
+        tasks, nt = ixg.window_tasks(int(lens[k]), o.maxw)
+        tgt = ixg.Target(ctx, seqs[k], int(lens[k]))
+        ixg.f3_filter_tasks(ctx, g, tgt, tasks, nt, o)
+        tgt.free()
+
+    for k in range(min(20, n)):  # warm-up (scratch growth)
+        plain_one(k)
+    svc = np.zeros(n)
+    for k in range(n):
+        t0 = time.perf_counter()
+        plain_one(k)
+        svc[k] = time.perf_counter() - t0
+    plain_rate = res / svc.sum() / 1e6
+    cap_qps = n / svc.sum()
+
+    srv = ixg.Server(ctx, nstreams=nstreams, max_inflight=1 << 26, max_batch=1 << 22)
+    try:
+        for k in range(min(40, n)):  # warm-up
+            srv.result(srv.submit(models[mids[k]][0], seqs[k], int(lens[k]), models[mids[k]][1]))
+        t0 = time.perf_counter()
+        tickets = [srv.submit(models[mids[k]][0], seqs[k], int(lens[k]), models[mids[k]][1]) for k in range(n)]
+        for t in tickets:
+            srv.result(t)
+        burst = time.perf_counter() - t0
+        served_peak = res / burst / 1e6
+        sweep = []
+        rng = np.random.default_rng(55)
+        for frac in qps_fracs:
+            qps = frac * cap_qps
+            arr = np.cumsum(rng.exponential(1.0 / qps, n))
+            # plain: one FIFO server with the measured service times
+            fin, t = np.zeros(n), 0.0
+            for k in range(n):
+                t = max(t, arr[k]) + svc[k]
+                fin[k] = t
+            plat = (fin - arr) * 1e3
+            # served: open-loop submission at the arrival times, results collected in a second thread
+            lat = np.zeros(n)
+            sub_wall = np.zeros(n)
+            tick = [0] * n
+            def collect():
+                for k in range(n):
+                    while tick[k] == 0:
+                        time.sleep(0.0002)
+                    tl, w, acc, tm = srv.result(tick[k])
+                    lat[k] = (sub_wall[k] - arr[k]) * 1e3 + tm[2]
+
+            th = threading.Thread(target=collect)
+            th.start()
+            t0 = time.perf_counter()
+            for k in range(n):
+                dt = arr[k] - (time.perf_counter() - t0)
+                if dt > 0:
+                    time.sleep(dt)
+                sub_wall[k] = time.perf_counter() - t0
+                tick[k] = srv.submit(models[mids[k]][0], seqs[k], int(lens[k]), models[mids[k]][1])
+            th.join()
+            span = (time.perf_counter() - t0)
+            sweep.append({"qps": qps, "served": {"p50_ms": float(np.percentile(lat, 50)),
+                                                 "p99_ms": float(np.percentile(lat, 99)),
+                                                 "mres_s": res / span / 1e6},
+                          "plain": {"p50_ms": float(np.percentile(plat, 50)), "p99_ms": float(np.percentile(plat, 99)),
+                                    "mres_s": res / (fin[-1]) / 1e6}})
+        st = srv.stats()
+    finally:
+        srv.free()
+        for g, _ in models:
+            g.free()
+    return {"workload": f"{n} requests, lengths log-normal median 5 kb sigma 1.5 clipped to [100, 1e6] "
+                        f"(seed 5), both strands, models uniform from 200 p7 profiles M in [50, 400], "
+                        f"F1 0.35 / F3 0.02 / F3b 0.02",
+            "residues": res, "streams": nstreams,
+            "plain_mres_s": plain_rate, "plain_capacity_qps": cap_qps,
+            "served_peak_mres_s": served_peak, "served_vs_plain": served_peak / plain_rate,
+            "sweep": sweep, "server_stats": st}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -521,6 +638,8 @@ def main():
                     help="oracle time budget per scanner section (CPU rate beside each GPU rate)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-cm", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-only", action="store_true", help="only run the serving-loop section")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the 256 MiB device copy probe (HBM counter calibration) and print its rate")
     a = ap.parse_args()
@@ -529,6 +648,10 @@ def main():
             nb = 256 << 20
             ms = ctx.copy_probe(nb, 2)
         print(json.dumps({"copy_probe_bytes": nb, "ms_per_copy": ms, "GB/s": 2 * nb / (ms * 1e-3) / 1e9}))
+        return
+    if a.serve_only:
+        with ixg.Context(0) as ctx:
+            print(json.dumps({"serving": time_serving(ctx)}))
         return
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (the path shards by "
@@ -565,6 +688,7 @@ def main():
         tr_line = time_trcm(ctx, max(1, a.steps // 2), 1, cpu) if not a.no_cm else None
         qdb_line = time_qdb(ctx, 2, cpu) if not a.no_cm else None
         pf_line = time_p7_forward(ctx, p, d, 2, cpu=cpu) if not a.no_cm else None
+        serve_line = time_serving(ctx) if not a.no_serve else None
     # dominant kernel with a per-unit op count, and its VALU roofline: ops per
     # launch (the last step's cells x ops per cell, from the reference loop
     # bodies) over the kernel's mean launch time (HIP events around each launch)
@@ -611,6 +735,8 @@ def main():
         line["p7_forward_full"] = pf_line
     if cp9_line:
         line["cp9_bands"] = cp9_line
+    if serve_line:
+        line["serving"] = serve_line
     if not a.no_cpu:
         line["cpu_baseline"] = cpu_baseline(p, d, L, M, maxw, tasks, nt, a.cpu_seconds, a.cpu_threads)
     print(json.dumps(line))

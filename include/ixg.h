@@ -421,6 +421,39 @@ int ixg_copy_probe(ixg_ctx *ctx, size_t bytes, int reps, double *ms_per_copy);
  * window kernels, [3] host glue, [4] total. Also the launch counts. */
 int ixg_last_timings(const ixg_ctx *ctx, double *ms5, int64_t *cells5);
 
+/* ---- serving: continuous batching of search requests --------------------------
+ * No counterpart in the reference, which runs each (model, sequence) search to
+ * completion. A request is the filter stage of one search: the sequence's
+ * (chunk x strand) tasks on both strands (cm_search.rs:1774-1808) filtered
+ * with f3_filter_sequence (cm_pipeline.rs:1863) against a resident p7 model;
+ * its result equals ixg_window_tasks(L, o->maxw, 1, 1) + ixg_f3_filter_tasks
+ * on the sequence alone. The server owns nstreams worker contexts on owner's
+ * device (mem_budget/nstreams bytes each, 0 = no cap); each idle worker takes
+ * the oldest queued request and every later queued one for the same model and
+ * options, up to max_batch residues, as one batch. The head of the queue is
+ * admitted once the residues in flight plus its own fit max_inflight (alone
+ * when larger). Models stay owned by the caller and must outlive the server. */
+typedef struct ixg_server ixg_server;
+int ixg_server_create(ixg_ctx *owner, int nstreams, size_t mem_budget, int64_t max_inflight, int64_t max_batch,
+                      ixg_server **out);
+/* Copies dsq[1..L] (1-indexed) and queues the request; returns its ticket. */
+int ixg_server_submit(ixg_server *s, const ixg_p7 *p7, const uint8_t *dsq, int64_t L, const ixg_f3opts *o,
+                      uint64_t *ticket);
+/* *done = 1 once the request has finished; waits up to timeout_us (0 polls,
+ * negative waits until done). */
+int ixg_server_wait(ixg_server *s, uint64_t ticket, int64_t timeout_us, int *done);
+/* Waits for the request, then writes its tasks, windows (task index local to
+ * the request, grouped by task), per-task acct and times_ms[4] = {queue wait,
+ * run, submit-to-done latency, requests in its batch}, and retires the
+ * ticket. IXG_ERANGE (ticket kept) if tcap or wcap is too small; a failed
+ * batch's status is returned for each of its requests. */
+int ixg_server_result(ixg_server *s, uint64_t ticket, ixg_task *tasks, int64_t tcap, int64_t *ntasks,
+                      ixg_window *wins, int64_t wcap, int64_t *nwins, ixg_acct *acct, double *times_ms);
+/* stats4 = {batches, requests, largest batch in requests, residues searched}. */
+int ixg_server_stats(ixg_server *s, int64_t *stats4);
+/* Runs every queued request to completion, then stops the workers. */
+int ixg_server_free(ixg_server *s);
+
 #ifdef __cplusplus
 }
 #endif

@@ -855,6 +855,30 @@ int upload_seq(ixg_ctx *c, const uint8_t *dsq, int64_t L, uint8_t **out) {
 
 }  // namespace
 
+namespace ixg {
+
+// The cascade over tasks given as (base, L) views of a device sequence
+// buffer; windows come back grouped by task with their task index, first[t]
+// is where task t's windows begin (first[nt] = total).
+int cascade_views(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const int64_t *base, const int64_t *len,
+                  int64_t nt, const ixg_f3opts &o, std::vector<ixg_window> &out, std::vector<int64_t> &first,
+                  std::vector<ixg_acct> &acct) {
+  std::vector<HostTask> ht(nt);
+  for (int64_t t = 0; t < nt; t++) ht[t] = {base[t], len[t]};
+  std::vector<std::vector<Win>> res;
+  RCHK(cascade(c, p, d_seq, ht, o, res, acct));
+  out.clear();
+  first.assign(nt + 1, 0);
+  for (int64_t t = 0; t < nt; t++) {
+    first[t] = (int64_t)out.size();
+    for (auto &w : res[t]) out.push_back({w.start, w.end, w.score, (int32_t)t});
+  }
+  first[nt] = (int64_t)out.size();
+  return IXG_OK;
+}
+
+}  // namespace ixg
+
 extern "C" {
 
 int ixg_ssv_diagonals(ixg_ctx *c, const ixg_p7 *p, const uint8_t *dsq, int64_t L, double f1, ixg_rawwin *out,

@@ -31,7 +31,7 @@ struct DevBuf {
 };
 
 enum Slot { S_TASKS, S_ITEMS, S_MASK, S_RAW, S_COUNTER, S_WINS, S_SC0, S_SC1, S_SC2, S_OVF0, S_OVF1, S_SEQ,
-            S_CM_ENV, S_CM_BANDS, S_CM_ALPHA, S_CM_SEQ, S_CINFO, S_CM_ROOT, S_CM_SCHED, S_NSLOTS };
+            S_CM_ENV, S_CM_BANDS, S_CM_ALPHA, S_CM_SEQ, S_CINFO, S_CM_ROOT, S_CM_SCHED, S_SERVE_SEQ, S_NSLOTS };
 
 int dmalloc(ixg_ctx *c, void **p, size_t n);
 void dfree(ixg_ctx *c, void *p, size_t n);
@@ -39,7 +39,7 @@ int scratch(ixg_ctx *c, int slot, size_t n, void **out);
 // Pinned host staging buffer `slot` of the context grown to at least n bytes
 // (contents not preserved; a slot is reused only after the stream has
 // drained past its previous copy).
-enum PinSlot { P_PACK, P_UP, P_DOWN, P_NSLOTS };
+enum PinSlot { P_PACK, P_UP, P_DOWN, P_SERVE, P_NSLOTS };
 int64_t arena_elems(const ixg_ctx *c, int slot);
 int pinned(ixg_ctx *c, int slot, size_t n, void **out);
 

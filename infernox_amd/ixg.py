@@ -61,7 +61,8 @@ EXPORTS = [
     "ixg_copy_probe", "ixg_cm_create_cfg", "ixg_cm_itables", "ixg_qdb_scan", "ixg_qdb_scan_batch",
     "ixg_cyk_seq_filter", "ixg_p7_forward", "ixg_p7_forward_batch", "ixg_cm_set_itrpenalties", "ixg_cm_imtables",
     "ixg_trqdb_scan", "ixg_trqdb_scan_batch", "ixg_cp9_create", "ixg_cp9_free", "ixg_cp9_bands",
-    "ixg_cp9_bands_batch",
+    "ixg_cp9_bands_batch", "ixg_server_create", "ixg_server_submit", "ixg_server_wait", "ixg_server_result",
+    "ixg_server_stats", "ixg_server_free",
 ]
 KERNELS = ["ssv_sweep", "ssv_resolve", "fwd", "bias", "vit", "msv", "hb_scan", "trhb_scan", "copy_probe",
            "qdb_scan", "fwd_full", "trqdb_scan", "cp9"]
@@ -175,6 +176,13 @@ def lib():
         L.ixg_trhb_scan_batch.argtypes = [vp, vp, vp, C.POINTER(TrEnvelope), i64, C.c_int, C.c_int, C.c_int,
                                           C.c_float, C.c_float, C.c_int, f32p, i32p, i64p, i64p, C.POINTER(TrHit),
                                           i64, i64p, i64p]
+        L.ixg_server_create.argtypes = [vp, C.c_int, C.c_size_t, i64, i64, C.POINTER(vp)]
+        L.ixg_server_submit.argtypes = [vp, vp, u8p, i64, C.POINTER(F3Opts), C.POINTER(C.c_uint64)]
+        L.ixg_server_wait.argtypes = [vp, C.c_uint64, i64, C.POINTER(C.c_int)]
+        L.ixg_server_result.argtypes = [vp, C.c_uint64, C.POINTER(Task), i64, i64p, C.POINTER(Window), i64, i64p,
+                                        C.POINTER(Acct), C.POINTER(C.c_double)]
+        L.ixg_server_stats.argtypes = [vp, i64p]
+        L.ixg_server_free.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -432,6 +440,59 @@ def f3_filter_tasks(ctx, p7, tgt, tasks, ntasks, opts, cap=None):
         break
     wins = [(out[i].task, out[i].start, out[i].end, out[i].score) for i in range(n.value)]
     return wins, accts
+
+
+class Server:
+    """Continuous-batching request loop over nstreams worker streams (ixg_server_*).
+
+    submit() queues (model, sequence) and returns a ticket; result(ticket)
+    waits and returns (tasks, windows, accts, times_ms) where windows are
+    (task, start, end, score) with the task index local to the request and
+    times_ms = (queue wait, run, latency, requests in the batch)."""
+
+    def __init__(self, ctx, nstreams=4, mem_budget=0, max_inflight=1 << 26, max_batch=1 << 24):
+        self.ctx = ctx
+        self.h = C.c_void_p()
+        check(lib().ixg_server_create(ctx.h, int(nstreams), int(mem_budget), int(max_inflight), int(max_batch),
+                                      C.byref(self.h)), "ixg_server_create")
+
+    def submit(self, p7, dsq, L, opts):
+        d = np.ascontiguousarray(dsq, np.uint8)
+        t = C.c_uint64()
+        check(lib().ixg_server_submit(self.h, p7.h, u8ptr(d), int(L), C.byref(opts), C.byref(t)),
+              "ixg_server_submit")
+        return t.value
+
+    def done(self, ticket, timeout_us=0):
+        d = C.c_int()
+        check(lib().ixg_server_wait(self.h, ticket, int(timeout_us), C.byref(d)), "ixg_server_wait")
+        return bool(d.value)
+
+    def result(self, ticket, tcap=64, wcap=1024):
+        nt, nw = C.c_int64(), C.c_int64()
+        while True:
+            tasks, wins, accts = (Task * tcap)(), (Window * wcap)(), (Acct * tcap)()
+            times = (C.c_double * 4)()
+            rc = lib().ixg_server_result(self.h, ticket, tasks, tcap, C.byref(nt), wins, wcap, C.byref(nw), accts,
+                                         times)
+            if rc == IXG_ERANGE:
+                tcap, wcap = max(tcap, nt.value), max(wcap, nw.value)
+                continue
+            check(rc, "ixg_server_result")
+            break
+        w = [(wins[i].task, wins[i].start, wins[i].end, wins[i].score) for i in range(nw.value)]
+        tl = [(tasks[i].start, tasks[i].end, tasks[i].strand) for i in range(nt.value)]
+        return tl, w, [accts[i] for i in range(nt.value)], tuple(times)
+
+    def stats(self):
+        s = (C.c_int64 * 4)()
+        check(lib().ixg_server_stats(self.h, s), "ixg_server_stats")
+        return dict(batches=s[0], requests=s[1], largest_batch=s[2], residues=s[3])
+
+    def free(self):
+        if self.h:
+            check(lib().ixg_server_free(self.h), "ixg_server_free")
+            self.h = C.c_void_p()
 
 
 def cm_params(cm):
