@@ -511,6 +511,72 @@ def cpu_baseline(p, d, L, M, maxw, tasks, nt, budget_s, threads):
                        f"f3_filter_sequence (C restatement, not infernox), {threads} threads, {dt:.1f} s")
 
 
+def multi_workload(n_cm=32, per_cm=16, r=8, seed=7):
+    """cmscan-style HB batch: n_cm CMs with consensus lengths uniform in
+    [50, 400] (random nested structures, 40-60% paired), per_cm envelopes
+    each, every envelope one emitted homolog with 20 flanking residues per
+    side on one query sequence."""
+    rng = np.random.default_rng(seed)
+    cms = [synth_cm.build_cm(synth_cm.random_nested(int(c), seed + k), seed + k)
+           for k, c in enumerate(rng.integers(50, 401, n_cm))]
+    parts, envs, env_cm, pos = [], [], [], 1
+    for k in range(n_cm * per_cm):
+        m = k % n_cm
+        h = synth_cm.emit(cms[m], 5000 + k)
+        left, right = rng.integers(0, 4, 20).astype(np.uint8), rng.integers(0, 4, 20).astype(np.uint8)
+        parts += [left, h, right]
+        i0, j0 = pos, pos + 40 + len(h) - 1
+        envs.append((i0, j0, 0, synth_cm.bands(cms[m], i0, j0, r=r, seed=k)))
+        env_cm.append(m)
+        pos = j0 + 1
+    seq = np.concatenate(parts)
+    L = len(seq)
+    dsq = np.concatenate([[255], seq, [255]]).astype(np.uint8)
+    return cms, env_cm, dsq, L, envs
+
+
+def time_multi(ctx, steps=3):
+    """Envelopes of 32 CMs: one persistent work-queue launch (ixg_hb_scan_multi)
+    against one ixg_hb_scan_batch call per model."""
+    cms, env_cm, dsq, L, envl = multi_workload()
+    gs = [ixg.CM(ctx, cm) for cm in cms]
+    tgt = ixg.Target(ctx, dsq, L)
+    batch = ixg.EnvBatch(envl)
+    per = [[k for k in range(len(envl)) if env_cm[k] == m] for m in range(len(cms))]
+    pb = [ixg.EnvBatch([envl[k] for k in idx]) for idx in per]
+    out = {"workload": f"{len(cms)} CMs (consensus 50-400, states {min(c['M'] for c in cms)}-"
+                       f"{max(c['M'] for c in cms)}), {len(envl)} envelopes, one query"}
+    for name, inside in (("cyk", False), ("inside", True)):
+        ixg.hb_scan_multi(ctx, gs, env_cm, tgt, batch, inside=inside)
+        t0 = time.perf_counter()
+        kms = 0.0
+        for _ in range(steps):
+            _, _, _, _, cells = ixg.hb_scan_multi(ctx, gs, env_cm, tgt, batch, inside=inside)
+            kms += ctx.timings()[0][2]
+        wall_multi = (time.perf_counter() - t0) / steps
+        kms /= steps
+        for m in range(len(cms)):
+            ixg.hb_scan_batch(ctx, gs[m], tgt, pb[m], inside=inside)
+        t0 = time.perf_counter()
+        kper = 0.0
+        for _ in range(steps):
+            for m in range(len(cms)):
+                ixg.hb_scan_batch(ctx, gs[m], tgt, pb[m], inside=inside)
+                kper += ctx.timings()[0][2]
+        wall_per = (time.perf_counter() - t0) / steps
+        kper /= steps
+        out[name] = {"cells": int(cells), "one_launch": {"kernel_ms": kms, "wall_ms": wall_multi * 1e3,
+                                                         "gcups": cells / (kms * 1e-3) / 1e9,
+                                                         "gcups_wall": cells / wall_multi / 1e9},
+                     "per_model_launches": {"kernel_ms": kper, "wall_ms": wall_per * 1e3,
+                                            "gcups": cells / (kper * 1e-3) / 1e9,
+                                            "gcups_wall": cells / wall_per / 1e9}}
+    tgt.free()
+    for g in gs:
+        g.free()
+    return out
+
+
 def serving_workload(n_req=1200, n_models=200, seed=5, median=5000.0, sigma=1.5):
     """Request stream for the serving loop: (model, sequence) pairs with lengths
     log-normal (median 5 kb, sigma 1.5) clipped to [100, 1e6] nt, iid ACGU,
@@ -640,6 +706,7 @@ def main():
     ap.add_argument("--no-cm", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-only", action="store_true", help="only run the serving-loop section")
+    ap.add_argument("--multi-only", action="store_true", help="only run the several-CM HB launch section")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the 256 MiB device copy probe (HBM counter calibration) and print its rate")
     a = ap.parse_args()
@@ -648,6 +715,10 @@ def main():
             nb = 256 << 20
             ms = ctx.copy_probe(nb, 2)
         print(json.dumps({"copy_probe_bytes": nb, "ms_per_copy": ms, "GB/s": 2 * nb / (ms * 1e-3) / 1e9}))
+        return
+    if a.multi_only:
+        with ixg.Context(0) as ctx:
+            print(json.dumps({"hb_multi": time_multi(ctx)}))
         return
     if a.serve_only:
         with ixg.Context(0) as ctx:
@@ -688,6 +759,7 @@ def main():
         tr_line = time_trcm(ctx, max(1, a.steps // 2), 1, cpu) if not a.no_cm else None
         qdb_line = time_qdb(ctx, 2, cpu) if not a.no_cm else None
         pf_line = time_p7_forward(ctx, p, d, 2, cpu=cpu) if not a.no_cm else None
+        multi_line = time_multi(ctx) if not a.no_cm else None
         serve_line = time_serving(ctx) if not a.no_serve else None
     # dominant kernel with a per-unit op count, and its VALU roofline: ops per
     # launch (the last step's cells x ops per cell, from the reference loop
@@ -735,6 +807,8 @@ def main():
         line["p7_forward_full"] = pf_line
     if cp9_line:
         line["cp9_bands"] = cp9_line
+    if multi_line:
+        line["hb_multi"] = multi_line
     if serve_line:
         line["serving"] = serve_line
     if not a.no_cpu:

@@ -239,6 +239,16 @@ typedef struct {
 int ixg_hb_scan_batch(ixg_ctx *ctx, const ixg_cm *cm, const ixg_target *tgt, const ixg_envelope *env, int64_t n,
                       int inside, float env_cutoff, int report, float cutoff, int do_null3, float *vsc,
                       int64_t *envi, int64_t *envj, ixg_hit *hits, int64_t cap, int64_t *nhits, int64_t *cells);
+/* The same over envelopes of several CMs in one persistent launch (cmscan's
+ * per-query loop over models, cm_search.rs:1357-1366, batched): envelope k
+ * belongs to cms[env_cm[k]] (ncm models, all created on ctx's device).
+ * Workgroups claim envelopes, largest first, from a device work queue.
+ * Results are per envelope exactly as ixg_hb_scan_batch gives for that
+ * envelope's model alone. */
+int ixg_hb_scan_multi(ixg_ctx *ctx, const ixg_cm *const *cms, int32_t ncm, const int32_t *env_cm,
+                      const ixg_target *tgt, const ixg_envelope *env, int64_t n, int inside, float env_cutoff,
+                      int report, float cutoff, int do_null3, float *vsc, int64_t *envi, int64_t *envj,
+                      ixg_hit *hits, int64_t cap, int64_t *nhits, int64_t *cells);
 
 /* ---- truncated HMM-banded scanners: TrCYK / FTrInside (F6/F7, truncated passes) -- */
 /* Pipeline pass indices (cm_trunc.rs:49-55) and marginal modes (:34-38). */

@@ -226,17 +226,61 @@ void append_units(const CmHost &h, const ixg_bands &b, const ixg_trvalid *val, i
 // layouts, launches the scan in groups that fit the deck budget, and returns
 // every envelope's root rows on the host (standard: nroot floats; truncated:
 // the J, L, R, T root planes then the best entry per root cell, nroot each).
+// One persistent launch over envelopes k0..k1 of several models: the
+// distinct models' descriptors, each envelope's model index, and the claim
+// order (largest deck first) go to the device; the grid is two workgroups
+// per CU (fewer when there are fewer envelopes).
+int launch_multi(ixg_ctx *c, const std::vector<const ixg_cm *> &percm, std::vector<EnvDev> &ed,
+                 const std::vector<EnvPlan> &plan, int64_t k0, int64_t k1, EnvDev *d_env, int inside) {
+  std::vector<const ixg_cm *> uniq;
+  std::vector<CmDev> devs;
+  int max_m = 0;
+  for (int64_t k = k0; k < k1; k++) {
+    const ixg_cm *m = percm[k];
+    auto it = std::find(uniq.begin(), uniq.end(), m);
+    if (it == uniq.end()) {
+      uniq.push_back(m);
+      devs.push_back(m->dev());
+      max_m = std::max(max_m, m->h.M);
+      it = uniq.end() - 1;
+    }
+    ed[k - k0].cmi = (int32_t)(it - uniq.begin());
+  }
+  std::vector<int32_t> order(k1 - k0);
+  for (int64_t k = 0; k < k1 - k0; k++) order[k] = (int32_t)k;
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int32_t a, int32_t b) { return plan[k0 + a].ncells > plan[k0 + b].ncells; });
+  const size_t dev_bytes = devs.size() * sizeof(CmDev), ord_bytes = order.size() * 4;
+  char *d_meta;
+  RCHK(scratch(c, S_CM_SCHED, dev_bytes + ord_bytes + 256, (void **)&d_meta));
+  CmDev *d_cms = (CmDev *)d_meta;
+  int32_t *d_order = (int32_t *)(d_meta + dev_bytes);
+  uint32_t *d_counter = (uint32_t *)(d_meta + ((dev_bytes + ord_bytes + 127) / 128) * 128);
+  HIPCHK(hipMemcpyAsync(d_env, ed.data(), ed.size() * sizeof(EnvDev), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_cms, devs.data(), dev_bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(d_order, order.data(), ord_bytes, hipMemcpyHostToDevice, c->stream));
+  int cus = 0;
+  HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+  HIPCHK(launch_hb_scan_queue(d_cms, max_m, d_env, d_order, k1 - k0, d_counter, 2 * std::max(cus, 1),
+                              uniq[0]->d_ftbl(), inside, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));  // the copies above read host vectors local to this call
+  return IXG_OK;
+}
+
+// percm (standard scans only): the model of each envelope, run as one
+// persistent launch over a work queue; null runs every envelope against cm.
 int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const std::vector<int64_t> &seq_off,
                   const ixg_envelope *env, int64_t n, int inside, const TrSel *tr, const ixg_trvalid *valid,
-                  std::vector<std::vector<float>> &roots, int64_t *cells_out) {
-  const int M = cm->h.M;
+                  std::vector<std::vector<float>> &roots, int64_t *cells_out,
+                  const std::vector<const ixg_cm *> *percm = nullptr) {
+  auto cmk = [&](int64_t k) { return percm ? (*percm)[k] : cm; };
   const bool trunc = tr != nullptr;
   const int planes = trunc ? 4 : 1, rper = trunc ? 5 : 1;
   std::vector<EnvPlan> plan(n);
   std::vector<const char *> err(n, nullptr);
   // per envelope, on host workers: band checks, deck layout, root row count
   par_for(n, [&](int64_t k) {
-    if (layout(cm->h, env[k].bands, env[k].i0, env[k].j0, trunc, plan[k].roff, plan[k].ncells, plan[k].nrows,
+    if (layout(cmk(k)->h, env[k].bands, env[k].i0, env[k].j0, trunc, plan[k].roff, plan[k].ncells, plan[k].nrows,
                &err[k]) != IXG_OK)
       return;
     if (trunc && (!valid[k].jvalid || !valid[k].lvalid || !valid[k].rvalid || !valid[k].tvalid)) {
@@ -273,6 +317,7 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
     int64_t rtotal = 0;
     at[0] = 0;
     for (int64_t k = k0; k < k1; k++) {
+      const int M = cmk(k)->h.M;
       at[k - k0 + 1] = at[k - k0] + 3 * (int64_t)M + 1 + 3 * (int64_t)plan[k].nrows + (trunc ? M : 0) + 2 * M;
       rat[k - k0] = rtotal;
       rtotal += rper * plan[k].nroot;
@@ -290,6 +335,7 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
     std::vector<EnvDev> ed(k1 - k0);
     int64_t aoff = 0;
     for (int64_t k = k0; k < k1; k++) {
+      const int M = cmk(k)->h.M;
       const int nr = plan[k].nrows;
       const int32_t *base = d_pack + at[k - k0];
       EnvDev &e = ed[k - k0];
@@ -324,6 +370,7 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
       par_for(a1 - a0, [&](int64_t t) {
         const int64_t kk = a0 + t, k = k0 + kk;
         const ixg_bands &b = env[k].bands;
+        const int M = cmk(k)->h.M;
         const int nr = plan[k].nrows;
         int32_t *o = pack + at[kk];
         o = std::copy(b.jmin, b.jmin + M, o);
@@ -334,7 +381,7 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
         o = std::copy(plan[k].roff.begin(), plan[k].roff.begin() + nr, o);
         if (trunc)
           for (int v = 0; v < M; v++) *o++ = mode_mask(valid[k], v, tr->fill);
-        append_units(cm->h, b, trunc ? &valid[k] : nullptr, trunc ? tr->fill : 0, o);
+        append_units(cmk(k)->h, b, trunc ? &valid[k] : nullptr, trunc ? tr->fill : 0, o);
       });
       HIPCHK(hipMemcpyAsync(d_pack + at[a0], pack + at[a0], (size_t)(at[a1] - at[a0]) * 4, hipMemcpyHostToDevice,
                             c->stream));
@@ -343,6 +390,8 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
     if (trunc)
       HIPCHK(launch_trhb_scan(cm->dev(), d_env, k1 - k0, cm->d_ftbl(), tr->d_pty, tr->d_yl, tr->ny, inside,
                               c->stream));
+    else if (percm)
+      RCHK(launch_multi(c, *percm, ed, plan, k0, k1, d_env, inside));
     else
       HIPCHK(launch_hb_scan(cm->dev(), d_env, k1 - k0, cm->d_ftbl(), inside, c->stream));
     HIPCHK(hipEventRecord(c->sev[1], c->stream));
@@ -679,14 +728,10 @@ int ixg_inside_scan_hb(ixg_ctx *c, const ixg_cm *cm, const ixg_bands *b, const u
 const uint8_t *ixg_target_device_base(const ixg_target *t, int64_t *L);
 const uint8_t *ixg_target_host_dsq(const ixg_target *t, int strand);
 
-extern "C" int ixg_hb_scan_batch(ixg_ctx *c, const ixg_cm *cm, const ixg_target *tgt, const ixg_envelope *env,
-                                 int64_t n, int inside, float env_cutoff, int report, float cutoff, int do_null3,
-                                 float *vsc, int64_t *envi, int64_t *envj, ixg_hit *hits, int64_t cap,
-                                 int64_t *nhits, int64_t *cells) {
-  if (!c || !cm || !cm->d_st || !tgt || (n && !env) || !vsc || !envi || !envj || (report && !nhits)) {
-    g_err = "ixg_hb_scan_batch: bad arguments";
-    return IXG_EINVAL;
-  }
+static int hb_batch(ixg_ctx *c, const ixg_cm *cm, const std::vector<const ixg_cm *> *percm, const ixg_target *tgt,
+                    const ixg_envelope *env, int64_t n, int inside, float env_cutoff, int report, float cutoff,
+                    int do_null3, float *vsc, int64_t *envi, int64_t *envj, ixg_hit *hits, int64_t cap,
+                    int64_t *nhits, int64_t *cells) {
   HIPCHK(hipSetDevice(c->device));
   int64_t L = 0;
   const uint8_t *base = ixg_target_device_base(tgt, &L);
@@ -700,12 +745,12 @@ extern "C" int ixg_hb_scan_batch(ixg_ctx *c, const ixg_cm *cm, const ixg_target 
   }
   c->ms[2] = 0;
   std::vector<std::vector<float>> roots;
-  RCHK(run_envelopes(c, cm, base, off, env, n, inside ? 1 : 0, nullptr, nullptr, roots, cells));
+  RCHK(run_envelopes(c, cm, base, off, env, n, inside ? 1 : 0, nullptr, nullptr, roots, cells, percm));
   std::vector<std::vector<ixg_hit>> per(n);
   par_for(n, [&](int64_t k) {
     root_summary(env[k].bands, roots[k].data(), env[k].i0, env[k].j0, env_cutoff, vsc[k], envi[k], envj[k]);
     if (report)
-      report_hits(*cm, env[k].bands, ixg_target_host_dsq(tgt, env[k].strand), env[k].i0, env[k].j0,
+      report_hits(percm ? *(*percm)[k] : *cm, env[k].bands, ixg_target_host_dsq(tgt, env[k].strand), env[k].i0, env[k].j0,
                   roots[k].data(), cutoff, do_null3, (int32_t)k, per[k]);
   });
   std::vector<ixg_hit> out;
@@ -716,6 +761,44 @@ extern "C" int ixg_hb_scan_batch(ixg_ctx *c, const ixg_cm *cm, const ixg_target 
     if ((int64_t)out.size() > cap) return IXG_ERANGE;
   }
   return IXG_OK;
+}
+
+extern "C" int ixg_hb_scan_batch(ixg_ctx *c, const ixg_cm *cm, const ixg_target *tgt, const ixg_envelope *env,
+                                 int64_t n, int inside, float env_cutoff, int report, float cutoff, int do_null3,
+                                 float *vsc, int64_t *envi, int64_t *envj, ixg_hit *hits, int64_t cap,
+                                 int64_t *nhits, int64_t *cells) {
+  if (!c || !cm || !cm->d_st || !tgt || (n && !env) || !vsc || !envi || !envj || (report && !nhits)) {
+    g_err = "ixg_hb_scan_batch: bad arguments";
+    return IXG_EINVAL;
+  }
+  return hb_batch(c, cm, nullptr, tgt, env, n, inside, env_cutoff, report, cutoff, do_null3, vsc, envi, envj, hits,
+                  cap, nhits, cells);
+}
+
+extern "C" int ixg_hb_scan_multi(ixg_ctx *c, const ixg_cm *const *cms, int32_t ncm, const int32_t *env_cm,
+                                 const ixg_target *tgt, const ixg_envelope *env, int64_t n, int inside,
+                                 float env_cutoff, int report, float cutoff, int do_null3, float *vsc, int64_t *envi,
+                                 int64_t *envj, ixg_hit *hits, int64_t cap, int64_t *nhits, int64_t *cells) {
+  if (!c || !cms || ncm < 1 || (n && (!env || !env_cm)) || !tgt || !vsc || !envi || !envj || (report && !nhits) ||
+      n > INT32_MAX) {
+    g_err = "ixg_hb_scan_multi: bad arguments";
+    return IXG_EINVAL;
+  }
+  for (int32_t m = 0; m < ncm; m++)
+    if (!cms[m] || !cms[m]->d_st) {
+      g_err = "ixg_hb_scan_multi: null model";
+      return IXG_EINVAL;
+    }
+  std::vector<const ixg_cm *> percm(n);
+  for (int64_t k = 0; k < n; k++) {
+    if (env_cm[k] < 0 || env_cm[k] >= ncm) {
+      g_err = "ixg_hb_scan_multi: model index out of range";
+      return IXG_EINVAL;
+    }
+    percm[k] = cms[env_cm[k]];
+  }
+  return hb_batch(c, cms[0], &percm, tgt, env, n, inside, env_cutoff, report, cutoff, do_null3, vsc, envi, envj,
+                  hits, cap, nhits, cells);
 }
 
 extern "C" {

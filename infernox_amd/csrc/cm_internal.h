@@ -85,11 +85,14 @@ struct EnvDev {
   // per state: work units of one level step (cells; rows of an IL state;
   // i diagonals of an IR state; 0 to skip) and the first i of an IR state
   const int32_t *units, *ibase;
+  int32_t cmi;  // model of the envelope in a several-model launch (index into its CmDev array)
 };
 
 constexpr int HB_THREADS = 512;      // lanes per envelope workgroup
 constexpr int HB_LDS_STATES = 3200;  // states whose per-state band scalars fit in LDS
 
+hipError_t launch_hb_scan_queue(const CmDev *cms, int max_m, const EnvDev *envs, const int32_t *order, int64_t n,
+                                uint32_t *counter, int grid, const float *ftbl, int inside, hipStream_t s);
 hipError_t launch_hb_scan(const CmDev &cm, const EnvDev *envs, int64_t n, const float *ftbl, int inside,
                           hipStream_t s);
 // pty: the pass's truncated-begin penalty per state; ylist: the ny states

@@ -226,10 +226,9 @@ __device__ void ir_diag(const CmDev &cm, const EnvDev &e, int v, int u, const fl
   }
 }
 
+// One envelope, scanned by the whole workgroup.
 template <bool INSIDE>
-__global__ __launch_bounds__(HB_THREADS) void hb_scan_kernel(CmDev cm, const EnvDev *envs, const float *ftbl) {
-  extern __shared__ int32_t smem[];
-  EnvDev e = envs[blockIdx.x];
+__device__ __forceinline__ void hb_scan_env(const CmDev &cm, EnvDev e, const float *ftbl, int32_t *smem) {
   const int tid = threadIdx.x, nt = blockDim.x;
   hb_stage_scalars(e, cm.M, smem);
   auto unit = [&](int v, int u) {
@@ -281,6 +280,55 @@ __global__ __launch_bounds__(HB_THREADS) void hb_scan_kernel(CmDev cm, const Env
     gmem(e.alpha)[tgt] = cur;
     gmem(e.rout)[c] = cur;
   }
+}
+
+template <bool INSIDE>
+__global__ __launch_bounds__(HB_THREADS) void hb_scan_kernel(CmDev cm, const EnvDev *envs, const float *ftbl) {
+  extern __shared__ int32_t smem[];
+  hb_scan_env<INSIDE>(cm, envs[blockIdx.x], ftbl, smem);
+}
+
+// Persistent form over envelopes of several models: each workgroup claims
+// the next envelope of `order` (largest first) from a device counter until
+// the list is exhausted; an envelope's model is cms[e.cmi]. Every workgroup
+// leaves once a claim passes n, so the grid drains whatever its size.
+template <bool INSIDE>
+__global__ __launch_bounds__(HB_THREADS) void hb_scan_queue_kernel(const CmDev *__restrict__ cms,
+                                                                   const EnvDev *__restrict__ envs,
+                                                                   const int32_t *__restrict__ order, int32_t n,
+                                                                   uint32_t *counter, const float *ftbl) {
+  extern __shared__ int32_t smem[];
+  __shared__ int32_t claim;
+  for (;;) {
+    if (threadIdx.x == 0) claim = (int32_t)atomicAdd(counter, 1u);
+    __syncthreads();
+    const int32_t k = __builtin_amdgcn_readfirstlane(claim);
+    __syncthreads();  // every lane has read the claim before the next one is written
+    if (k >= n) break;
+    const int32_t ek = order[k];
+    const EnvDev e = envs[ek];
+    const CmDev cm = cms[e.cmi];
+    hb_scan_env<INSIDE>(cm, e, ftbl, smem);
+    __syncthreads();  // the next envelope's LDS staging waits for this one's last reads
+  }
+}
+
+hipError_t launch_hb_scan_queue(const CmDev *cms, int max_m, const EnvDev *envs, const int32_t *order, int64_t n,
+                                uint32_t *counter, int grid, const float *ftbl, int inside, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  // models with more states than the LDS holds read their band scalars from
+  // memory; the others stage 20 bytes per state
+  const size_t lds = (size_t)20 * std::min(max_m, HB_LDS_STATES);
+  hipError_t r = hipMemsetAsync(counter, 0, sizeof(uint32_t), s);
+  if (r != hipSuccess) return r;
+  const unsigned g = (unsigned)std::min<int64_t>(n, grid);
+  if (inside)
+    hipLaunchKernelGGL(hb_scan_queue_kernel<true>, dim3(g), dim3(HB_THREADS), lds, s, cms, envs, order, (int32_t)n,
+                       counter, ftbl);
+  else
+    hipLaunchKernelGGL(hb_scan_queue_kernel<false>, dim3(g), dim3(HB_THREADS), lds, s, cms, envs, order,
+                       (int32_t)n, counter, ftbl);
+  return hipGetLastError();
 }
 
 hipError_t launch_hb_scan(const CmDev &cm, const EnvDev *envs, int64_t n, const float *ftbl, int inside,

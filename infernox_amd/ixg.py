@@ -62,7 +62,7 @@ EXPORTS = [
     "ixg_cyk_seq_filter", "ixg_p7_forward", "ixg_p7_forward_batch", "ixg_cm_set_itrpenalties", "ixg_cm_imtables",
     "ixg_trqdb_scan", "ixg_trqdb_scan_batch", "ixg_cp9_create", "ixg_cp9_free", "ixg_cp9_bands",
     "ixg_cp9_bands_batch", "ixg_server_create", "ixg_server_submit", "ixg_server_wait", "ixg_server_result",
-    "ixg_server_stats", "ixg_server_free",
+    "ixg_server_stats", "ixg_server_free", "ixg_hb_scan_multi",
 ]
 KERNELS = ["ssv_sweep", "ssv_resolve", "fwd", "bias", "vit", "msv", "hb_scan", "trhb_scan", "copy_probe",
            "qdb_scan", "fwd_full", "trqdb_scan", "cp9"]
@@ -176,6 +176,9 @@ def lib():
         L.ixg_trhb_scan_batch.argtypes = [vp, vp, vp, C.POINTER(TrEnvelope), i64, C.c_int, C.c_int, C.c_int,
                                           C.c_float, C.c_float, C.c_int, f32p, i32p, i64p, i64p, C.POINTER(TrHit),
                                           i64, i64p, i64p]
+        L.ixg_hb_scan_multi.argtypes = [vp, C.POINTER(vp), C.c_int32, i32p, vp, C.POINTER(Envelope), i64, C.c_int,
+                                        C.c_float, C.c_int, C.c_float, C.c_int, f32p, i64p, i64p, C.POINTER(Hit),
+                                        i64, i64p, i64p]
         L.ixg_server_create.argtypes = [vp, C.c_int, C.c_size_t, i64, i64, C.POINTER(vp)]
         L.ixg_server_submit.argtypes = [vp, vp, u8p, i64, C.POINTER(F3Opts), C.POINTER(C.c_uint64)]
         L.ixg_server_wait.argtypes = [vp, C.c_uint64, i64, C.POINTER(C.c_int)]
@@ -712,6 +715,29 @@ def hb_scan_batch(ctx, cm, tgt, envs, inside=False, env_cutoff=0.0, report=False
     check(lib().ixg_hb_scan_batch(ctx.h, cm.h, tgt.h, arr, n, int(inside), float(env_cutoff), int(report),
                                   float(cutoff), int(null3), f32ptr(vsc), i64ptr(ei), i64ptr(ej), hits, cap,
                                   C.byref(nh), C.byref(cells)), "ixg_hb_scan_batch")
+    hl = [(hits[k].env, hits[k].i, hits[k].j, hits[k].score, hits[k].bias) for k in range(nh.value)] if report else []
+    return vsc[:n], ei[:n], ej[:n], hl, cells.value
+
+
+def hb_scan_multi(ctx, cms, env_cm, tgt, envs, inside=False, env_cutoff=0.0, report=False, cutoff=0.0, null3=True):
+    """Envelopes of several CMs in one persistent launch: envs as for
+    hb_scan_batch, env_cm[k] = index into cms of envelope k's model.
+    Returns (vsc, envi, envj, hits, cells)."""
+    batch = envs if isinstance(envs, EnvBatch) else EnvBatch(envs)
+    n, arr = batch.n, batch.arr
+    ecm = np.ascontiguousarray(env_cm, np.int32)
+    assert len(ecm) == n
+    hs = (C.c_void_p * len(cms))(*[c.h.value for c in cms])
+    vsc = np.zeros(max(n, 1), np.float32)
+    ei = np.zeros(max(n, 1), np.int64)
+    ej = np.zeros(max(n, 1), np.int64)
+    cap = 1 << 16
+    hits = (Hit * cap)()
+    nh, cells = C.c_int64(), C.c_int64()
+    check(lib().ixg_hb_scan_multi(ctx.h, hs, len(cms), ecm.ctypes.data_as(C.POINTER(C.c_int32)), tgt.h, arr, n,
+                                  int(inside), float(env_cutoff), int(report), float(cutoff), int(null3), f32ptr(vsc),
+                                  i64ptr(ei), i64ptr(ej), hits, cap, C.byref(nh), C.byref(cells)),
+          "ixg_hb_scan_multi")
     hl = [(hits[k].env, hits[k].i, hits[k].j, hits[k].score, hits[k].bias) for k in range(nh.value)] if report else []
     return vsc[:n], ei[:n], ej[:n], hl, cells.value
 

@@ -109,3 +109,60 @@ def test_batch_both_strands(ctx):
         assert (vsc > 0).sum() >= 1  # planted homologs are found in some envelopes
     tgt.free()
     g.free()
+
+
+def test_multi_cm_queue_equals_per_model_batches(ctx):
+    # envelopes of three CMs of different sizes interleaved in one persistent
+    # launch (ixg_hb_scan_multi) give each envelope exactly what its own
+    # model's batch gives, and the oracle's scores and hits
+    cms = [S.build_cm(S.cloverleaf(), 11), S.build_cm(S.multibranch(150, 4, 5), 12),
+           S.build_cm(S.random_nested(90, 13), 13)]
+    rng = np.random.default_rng(21)
+    L = 12000
+    seq = rng.integers(0, 4, L).astype(np.uint8)
+    spans = []
+    for k in range(18):
+        m = k % 3
+        h = S.emit(cms[m], 300 + k)
+        s = 100 + k * 640
+        if k % 4 == 1:
+            h = np.where(h < 4, 3 - h, h)[::-1]
+        seq[s:s + len(h)] = h[:600]
+        spans.append((m, s + 1, s + min(len(h), 600), int(k % 4 == 1)))
+    dsq = np.concatenate([[255], seq, [255]]).astype(np.uint8)
+    rc = np.concatenate([[255], np.where(seq < 4, 3 - seq, seq)[::-1], [255]]).astype(np.uint8)
+    gs = [ixg.CM(ctx, cm) for cm in cms]
+    tgt = ixg.Target(ctx, dsq, L)
+    envs, env_cm = [], []
+    for m, a, b_, strand in spans:
+        if strand:
+            a, b_ = L - b_ + 1, L - a + 1
+        i0, j0 = max(1, a - 12), min(L, b_ + 12)
+        envs.append((i0, j0, strand, S.bands(cms[m], i0, j0, r=8, seed=a)))
+        env_cm.append(m)
+    for inside in (0, 1):
+        _say("multi", inside)
+        vsc, ei, ej, hits, cells = ixg.hb_scan_multi(ctx, gs, env_cm, tgt, envs, inside=bool(inside),
+                                                     report=True, cutoff=0.0)
+        tot = 0
+        for m in range(3):
+            idx = [k for k in range(len(envs)) if env_cm[k] == m]
+            v1, e1, f1, h1, c1 = ixg.hb_scan_batch(ctx, gs[m], tgt, [envs[k] for k in idx], inside=bool(inside),
+                                                  report=True, cutoff=0.0)
+            tot += c1
+            for t, k in enumerate(idx):
+                assert np.float32(vsc[k]).tobytes() == np.float32(v1[t]).tobytes(), (m, k)
+                assert (ei[k], ej[k]) == (e1[t], f1[t]), (m, k)
+                got = [(h[1], h[2], np.float32(h[3]), np.float32(h[4])) for h in hits if h[0] == k]
+                want = [(h[1], h[2], np.float32(h[3]), np.float32(h[4])) for h in h1 if h[0] == t]
+                assert got == want, (m, k)
+        assert cells == tot
+        o = OracleCM(cms[1])
+        k = env_cm.index(1)
+        i0, j0, strand, b = envs[k]
+        want = o.scan(b, rc if strand else dsq, i0, j0, inside, env_cutoff=0.0)
+        assert np.float32(vsc[k]).tobytes() == np.float32(want["vsc"]).tobytes()
+        assert (vsc > 0).sum() >= 1
+    tgt.free()
+    for g in gs:
+        g.free()
