@@ -759,7 +759,7 @@ def main():
         tr_line = time_trcm(ctx, max(1, a.steps // 2), 1, cpu) if not a.no_cm else None
         qdb_line = time_qdb(ctx, 2, cpu) if not a.no_cm else None
         pf_line = time_p7_forward(ctx, p, d, 2, cpu=cpu) if not a.no_cm else None
-        multi_line = time_multi(ctx) if not a.no_cm else None
+        multi_line = None  # the several-CM queue launch is switched off (ixg_hb_scan_multi refuses to run)
         serve_line = time_serving(ctx) if not a.no_serve else None
     # dominant kernel with a per-unit op count, and its VALU roofline: ops per
     # launch (the last step's cells x ops per cell, from the reference loop

@@ -779,6 +779,13 @@ extern "C" int ixg_hb_scan_multi(ixg_ctx *c, const ixg_cm *const *cms, int32_t n
                                  const ixg_target *tgt, const ixg_envelope *env, int64_t n, int inside,
                                  float env_cutoff, int report, float cutoff, int do_null3, float *vsc, int64_t *envi,
                                  int64_t *envj, ixg_hit *hits, int64_t cap, int64_t *nhits, int64_t *cells) {
+  // Switched off: GPU test-suite runs that included this launch faulted the
+  // GPU later, in other kernels; until a cause is shown it refuses to run.
+  constexpr bool kEnabled = false;
+  if (!kEnabled) {
+    g_err = "ixg_hb_scan_multi: switched off (under investigation for a GPU fault)";
+    return IXG_EINVAL;
+  }
   if (!c || !cms || ncm < 1 || (n && (!env || !env_cm)) || !tgt || !vsc || !envi || !envj || (report && !nhits) ||
       n > INT32_MAX) {
     g_err = "ixg_hb_scan_multi: bad arguments";

@@ -111,6 +111,8 @@ def test_batch_both_strands(ctx):
     g.free()
 
 
+@pytest.mark.skip(reason="the several-CM queue launch is switched off: GPU suite runs that included it faulted "
+                         "later in the window-score kernels; cause not found")
 def test_multi_cm_queue_equals_per_model_batches(ctx):
     # envelopes of three CMs of different sizes interleaved in one persistent
     # launch (ixg_hb_scan_multi) give each envelope exactly what its own
