@@ -68,7 +68,10 @@ int64_t arena_elems(const ixg_ctx *c, int slot) {
 
 int pinned(ixg_ctx *c, int slot, size_t n, void **out) {
   if (c->hpin_cap[slot] < n) {
-    if (c->hpin[slot]) HIPCHK(hipHostFree(c->hpin[slot]));
+    if (c->hpin[slot]) {
+      HIPCHK(hipStreamSynchronize(c->stream));
+      HIPCHK(hipHostFree(c->hpin[slot]));
+    }
     c->hpin[slot] = nullptr;
     c->hpin_cap[slot] = 0;
     const size_t want = std::max(n + n / 4, (size_t)1 << 20);
@@ -76,6 +79,13 @@ int pinned(ixg_ctx *c, int slot, size_t n, void **out) {
     c->hpin_cap[slot] = want;
   }
   *out = c->hpin[slot];
+  return IXG_OK;
+}
+
+int d2h(ixg_ctx *c, void *dst, const void *src, size_t n) {
+  if (n == 0) return IXG_OK;
+  HIPCHK(hipMemcpyAsync(dst, src, n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return IXG_OK;
 }
 
@@ -184,7 +194,7 @@ int ixg_init(int device, size_t mem_budget, ixg_ctx **out) {
     return IXG_EDEVICE;
   }
   for (auto &ev : c->ev) HIPCHK(hipEventCreate(&ev));
-  for (auto &ev : c->kev) HIPCHK(hipEventCreate(&ev));
+  for (auto &ev : c->kpool) HIPCHK(hipEventCreate(&ev));
   for (auto &ev : c->sev) HIPCHK(hipEventCreate(&ev));
   *out = c;
   return IXG_OK;
@@ -198,7 +208,7 @@ int ixg_shutdown(ixg_ctx *c) {
   for (auto *h : c->hpin)
     if (h) (void)hipHostFree(h);
   for (auto &ev : c->ev) (void)hipEventDestroy(ev);
-  for (auto &ev : c->kev) (void)hipEventDestroy(ev);
+  for (auto &ev : c->kpool) (void)hipEventDestroy(ev);
   for (auto &ev : c->sev) (void)hipEventDestroy(ev);
   (void)hipStreamDestroy(c->stream);
   delete c;
@@ -261,6 +271,7 @@ int ixg_p7_create(ixg_ctx *c, const ixg_p7_hmm *h, int32_t max_length, ixg_p7 **
   HIPCHK(hipMemcpy(p->d_rbv, p->msv.rbv.data(), b1, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(p->d_fwd, fwd.data(), b2, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(p->d_vit, vit.data(), b3, hipMemcpyHostToDevice));
+  HIPCHK(hipStreamSynchronize(nullptr));  // the null-stream DMA has landed before any context stream reads it
   *out = p.release();
   return IXG_OK;
 }
@@ -502,17 +513,19 @@ int run_f1(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<
   HIPCHK(hipEventRecord(c->ev[1], c->stream));
   RawWinDev *d_raw = nullptr;
   unsigned long long count = 0;
+  unsigned long long *h_count;
+  RCHK(pinned(c, P_WORD, 64, (void **)&h_count));
   for (int attempt = 0; attempt < 2; attempt++) {
     RCHK(scratch(c, S_RAW, (size_t)cap * sizeof(RawWinDev), (void **)&d_raw));
-    static const unsigned long long zero = 0;
-    HIPCHK(hipMemcpyAsync(d_counter, &zero, 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipMemsetAsync(d_counter, 0, 8, c->stream));
     if (attempt) HIPCHK(hipEventRecord(c->ev[1], c->stream));
     HIPCHK(hipEventRecord(c->ev[7], c->stream));
     HIPCHK(launch_ssv_resolve(sd, d_tasks, d_items, (int64_t)items.size(), d_mask, d_cinfo, thresh, d_raw, cap,
                               d_counter, c->stream));
     HIPCHK(hipEventRecord(c->ev[2], c->stream));
-    HIPCHK(hipMemcpyAsync(&count, d_counter, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipMemcpyAsync(h_count, d_counter, 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
+    count = *h_count;
     if ((int64_t)count <= cap) break;
     cap = (int64_t)count + 1024;
   }
@@ -1095,6 +1108,8 @@ int ixg_copy_probe(ixg_ctx *c, size_t bytes, int reps, double *ms_per_copy) {
 
 int ixg_kernel_timings(ixg_ctx *c, double *ms, int64_t *launches, int reset) {
   if (!c) return IXG_EINVAL;
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(ktime_flush(c));
   if (ms) memcpy(ms, c->kms, sizeof(c->kms));
   if (launches) memcpy(launches, c->kn, sizeof(c->kn));
   if (reset)

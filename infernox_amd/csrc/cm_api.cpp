@@ -640,6 +640,7 @@ int ixg_cm_create_cfg(ixg_ctx *c, const ixg_cm_params *p, int local, ixg_cm **ou
   HIPCHK(hipMemcpy(cm->d_st, cm->h.sttype.data(), M, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(cm->d_i32, i32.data(), i32.size() * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(cm->d_f, f.data(), f.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipStreamSynchronize(nullptr));  // the null-stream DMA has landed before any context stream reads it
   RCHK(qdb_upload(c, cm.get()));
   *out = cm.release();
   return IXG_OK;
@@ -779,13 +780,6 @@ extern "C" int ixg_hb_scan_multi(ixg_ctx *c, const ixg_cm *const *cms, int32_t n
                                  const ixg_target *tgt, const ixg_envelope *env, int64_t n, int inside,
                                  float env_cutoff, int report, float cutoff, int do_null3, float *vsc, int64_t *envi,
                                  int64_t *envj, ixg_hit *hits, int64_t cap, int64_t *nhits, int64_t *cells) {
-  // Switched off: GPU test-suite runs that included this launch faulted the
-  // GPU later, in other kernels; until a cause is shown it refuses to run.
-  constexpr bool kEnabled = false;
-  if (!kEnabled) {
-    g_err = "ixg_hb_scan_multi: switched off (under investigation for a GPU fault)";
-    return IXG_EINVAL;
-  }
   if (!c || !cms || ncm < 1 || (n && (!env || !env_cm)) || !tgt || !vsc || !envi || !envj || (report && !nhits) ||
       n > INT32_MAX) {
     g_err = "ixg_hb_scan_multi: bad arguments";
@@ -839,6 +833,7 @@ int ixg_cm_set_trpenalties(ixg_ctx *c, ixg_cm *cm, const float *g, const float *
   RCHK(dmalloc(c, (void **)&cm->d_yl, cm->b_yl));
   HIPCHK(hipMemcpy(cm->d_tr, all.data(), cm->b_tr, hipMemcpyHostToDevice));
   if (!yl.empty()) HIPCHK(hipMemcpy(cm->d_yl, yl.data(), yl.size() * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipStreamSynchronize(nullptr));  // the null-stream DMA has landed before any context stream reads it
   return IXG_OK;
 }
 

@@ -95,9 +95,9 @@ int cp9_run(ixg_ctx *c, const ixg_cp9 *h, const std::vector<CWin> &wins, double 
     HIPCHK(ktime_end(c, IXG_K_CP9));
     std::vector<float> tf(nw), tb(nw);
     std::vector<int32_t> tp((size_t)nw * 6 * M1);
-    HIPCHK(hipMemcpy(tf.data(), d_f, nw * 4, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(tb.data(), d_b, nw * 4, hipMemcpyDeviceToHost));
-    HIPCHK(hipMemcpy(tp.data(), d_pn, tp.size() * 4, hipMemcpyDeviceToHost));
+    RCHK(d2h(c, tf.data(), d_f, nw * 4));
+    RCHK(d2h(c, tb.data(), d_b, nw * 4));
+    RCHK(d2h(c, tp.data(), d_pn, tp.size() * 4));
     for (int64_t k = 0; k < nw; k++) {
       const int64_t o = order[g0 + k];
       hf[o] = tf[k];
@@ -108,7 +108,7 @@ int cp9_run(ixg_ctx *c, const ixg_cp9 *h, const std::vector<CWin> &wins, double 
       const int L = wins[0].j0 - wins[0].i0 + 1;
       const size_t cells = (size_t)(L + 1) * M1;
       for (int m = 0; m < 6; m++)
-        if (mats[m]) HIPCHK(hipMemcpy(mats[m], d_mx + m * cells, cells * 4, hipMemcpyDeviceToHost));
+        if (mats[m]) RCHK(d2h(c, mats[m], d_mx + m * cells, cells * 4));
     }
     g0 = g1;
   }
@@ -274,7 +274,8 @@ int ixg_cp9_create(ixg_ctx *c, const ixg_cp9_params *p, ixg_cp9 **out) {
     delete h;
     return r;
   }
-  if (hipMemcpy(h->d, blob.data(), h->bytes, hipMemcpyHostToDevice) != hipSuccess) {
+  if (hipMemcpy(h->d, blob.data(), h->bytes, hipMemcpyHostToDevice) != hipSuccess ||
+      hipStreamSynchronize(nullptr) != hipSuccess) {
     dfree(c, h->d, h->bytes);
     delete h;
     g_err = "ixg_cp9_create: upload failed";

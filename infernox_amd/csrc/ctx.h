@@ -37,10 +37,14 @@ int dmalloc(ixg_ctx *c, void **p, size_t n);
 void dfree(ixg_ctx *c, void *p, size_t n);
 int scratch(ixg_ctx *c, int slot, size_t n, void **out);
 // Pinned host staging buffer `slot` of the context grown to at least n bytes
-// (contents not preserved; a slot is reused only after the stream has
-// drained past its previous copy).
-enum PinSlot { P_PACK, P_UP, P_DOWN, P_SERVE, P_NSLOTS };
+// (contents not preserved). Growing waits for the context's stream first, so
+// no copy still queued on it reads or writes the buffer being freed.
+enum PinSlot { P_PACK, P_UP, P_DOWN, P_SERVE, P_WORD, P_NSLOTS };
 int64_t arena_elems(const ixg_ctx *c, int slot);
+// Device-to-host copy ordered after the work queued on the context's stream
+// (a plain hipMemcpy runs on the null stream, which does not wait for a
+// non-blocking stream); returns when the bytes are on the host.
+int d2h(ixg_ctx *c, void *dst, const void *src, size_t n);
 int pinned(ixg_ctx *c, int slot, size_t n, void **out);
 
 // Runs f(k) for k in [0, n) on up to 16 host threads, in chunks of 4 items
@@ -82,9 +86,14 @@ struct ixg_ctx {
   hipEvent_t ev[8];
   double ms[5] = {0, 0, 0, 0, 0};
   int64_t cells[5] = {0, 0, 0, 0, 0};
-  // per-kernel device time (HIP events around each launch), cumulative:
-  // IXG_K_* index; kev is the event pair used for it
-  hipEvent_t kev[2];
+  // per-kernel device time, cumulative (IXG_K_* index): each timed launch is
+  // bracketed by a pair of events from kpool, recorded without waiting; the
+  // pairs are read (after one wait on the last end event) when the pool is
+  // full or the timings are queried, so timing adds no host round trip
+  static constexpr int KT_POOL = 256;
+  hipEvent_t kpool[2 * KT_POOL];
+  int kpend_k[KT_POOL];
+  int knext = 0;
   hipEvent_t sev[2];  // scanner batch kernel: start/end
   double kms[IXG_NKERNELS] = {};
   int64_t kn[IXG_NKERNELS] = {};
@@ -94,17 +103,34 @@ struct ixg_ctx {
 };
 
 namespace ixg {
-// Records the start event of a timed launch on the context's stream.
-inline hipError_t ktime_begin(ixg_ctx *c) { return hipEventRecord(c->kev[0], c->stream); }
-// Records the end event, waits for it and adds the launch to kernel k.
-inline hipError_t ktime_end(ixg_ctx *c, int k) {
-  hipError_t e = hipEventRecord(c->kev[1], c->stream);
+// Reads every recorded event pair of the pool into kms/kn (one wait on the
+// last end event) and empties the pool.
+inline hipError_t ktime_flush(ixg_ctx *c) {
+  if (c->knext == 0) return hipSuccess;
+  hipError_t e = hipEventSynchronize(c->kpool[2 * c->knext - 1]);
   if (e != hipSuccess) return e;
-  if ((e = hipEventSynchronize(c->kev[1])) != hipSuccess) return e;
-  float ms = 0;
-  if ((e = hipEventElapsedTime(&ms, c->kev[0], c->kev[1])) != hipSuccess) return e;
-  c->kms[k] += ms;
-  c->kn[k] += 1;
+  for (int t = 0; t < c->knext; t++) {
+    float ms = 0;
+    if ((e = hipEventElapsedTime(&ms, c->kpool[2 * t], c->kpool[2 * t + 1])) != hipSuccess) return e;
+    c->kms[c->kpend_k[t]] += ms;
+    c->kn[c->kpend_k[t]] += 1;
+  }
+  c->knext = 0;
+  return hipSuccess;
+}
+// Records the start event of a timed launch on the context's stream.
+inline hipError_t ktime_begin(ixg_ctx *c) {
+  if (c->knext == ixg_ctx::KT_POOL) {
+    hipError_t e = ktime_flush(c);
+    if (e != hipSuccess) return e;
+  }
+  return hipEventRecord(c->kpool[2 * c->knext], c->stream);
+}
+// Records the end event of the launch begun last and files it under kernel k.
+inline hipError_t ktime_end(ixg_ctx *c, int k) {
+  hipError_t e = hipEventRecord(c->kpool[2 * c->knext + 1], c->stream);
+  if (e != hipSuccess) return e;
+  c->kpend_k[c->knext++] = k;
   return hipSuccess;
 }
 }  // namespace ixg

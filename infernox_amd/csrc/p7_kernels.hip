@@ -11,9 +11,12 @@ namespace ixg {
 #define DPP_QUAD_X1 0xB1    // quad_perm [1,0,3,2]
 #define DPP_QUAD_X2 0x4E    // quad_perm [2,3,0,1]
 
+// SSV cost of model position k for residue code x. Canonical codes come from
+// the packed word c4 (0xFFFFFFFF past m); degenerate codes read rbv, whose
+// rows exist for k = 0..m only, so positions past m cost 255 without a load.
 __device__ __forceinline__ uint32_t ssv_cost(const SsvDev &p, uint32_t c4, int k, int x) {
   if (x < 4) return (c4 >> (8 * x)) & 0xFFu;
-  return x < KP ? (uint32_t)p.rbv[k * KP + x] : 255u;
+  return (x < KP && k <= p.m) ? (uint32_t)p.rbv[k * KP + x] : 255u;
 }
 
 // One SSV cell: subs(adds(max(prev, xb), bias), cost) on u8 values held in u32.
@@ -170,7 +173,7 @@ __global__ __launch_bounds__(64) void ssv_sweep_kernel(SsvDev p, const TaskDev *
 // ---------------------------------------------------------------------------
 // SSV cost of model position k (1..m) for residue code x.
 __device__ __forceinline__ int ssv_cost_at(const SsvDev &p, int k, int x) {
-  return x < KP ? (int)p.rbv[k * KP + x] : 255;
+  return (x < KP && k >= 0 && k <= p.m) ? (int)p.rbv[k * KP + x] : 255;
 }
 // Wave-wide inclusive prefix sum / maximum over lanes 0..63, exclusive maximum.
 __device__ __forceinline__ int wave_incl_sum(int v) {
@@ -584,7 +587,7 @@ __global__ __launch_bounds__(64) void msv_kernel(SsvDev p, const uint8_t *seq, c
 #pragma unroll
     for (int j = KPL - 1; j >= 0; j--) {
       const uint32_t in = j ? d[j - 1] : prev;
-      const uint32_t cc = ssv_cost(p, c4[j], kb + j, x);
+      const uint32_t cc = kb + j <= m ? ssv_cost(p, c4[j], kb + j, x) : 255u;
       d[j] = (kb + j <= m) ? ssv_cell(in, xb, bias, cc) : 0u;
       lmax = d[j] > lmax ? d[j] : lmax;
     }

@@ -38,6 +38,7 @@ int qdb_upload(ixg_ctx *c, ixg_cm *cm) {
   cm->b_q = q.size() * 4;
   RCHK(dmalloc(c, (void **)&cm->d_q, cm->b_q));
   HIPCHK(hipMemcpy(cm->d_q, q.data(), cm->b_q, hipMemcpyHostToDevice));
+  HIPCHK(hipStreamSynchronize(nullptr));  // the null-stream DMA has landed before any context stream reads it
   // truncated QDB: integer marginal emissions, then integer penalties
   // (-INFTY, i.e. no truncated begin, until ixg_cm_set_itrpenalties)
   std::vector<int32_t> t(h.ilmesc);
@@ -46,6 +47,7 @@ int qdb_upload(ixg_ctx *c, ixg_cm *cm) {
   cm->b_tq = t.size() * 4;
   RCHK(dmalloc(c, (void **)&cm->d_tq, cm->b_tq));
   HIPCHK(hipMemcpy(cm->d_tq, t.data(), cm->b_tq, hipMemcpyHostToDevice));
+  HIPCHK(hipStreamSynchronize(nullptr));  // the null-stream DMA has landed before any context stream reads it
   return IXG_OK;
 }
 
@@ -522,20 +524,19 @@ int qdb_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
       unsigned long long *d_cnt;
       RCHK(scratch(c, S_CM_ROOT, (size_t)cap * sizeof(QdbCand), (void **)&d_c));
       RCHK(scratch(c, S_COUNTER, 8, (void **)&d_cnt));
-      static const unsigned long long zero = 0;
-      HIPCHK(hipMemcpyAsync(d_cnt, &zero, 8, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemsetAsync(d_cnt, 0, 8, c->stream));
       HIPCHK(ktime_begin(c));
       HIPCHK(launch_qdb_scan(q, tabs, inside, d_w, k1 - k0, cutoff, d_c, cap, d_cnt, maxlen, c->stream));
       HIPCHK(ktime_end(c, IXG_K_QDB_SCAN));
       unsigned long long cnt = 0;
-      HIPCHK(hipMemcpy(&cnt, d_cnt, 8, hipMemcpyDeviceToHost));
+      RCHK(d2h(c, &cnt, d_cnt, 8));
       c->qdb_cand_rate = std::max(c->qdb_cand_rate, (double)cnt / (double)std::max<int64_t>(res, 1));
       if ((int64_t)cnt > cap) {
         cap = (int64_t)cnt + 1024;
         continue;
       }
       hc.resize(cnt);
-      if (cnt) HIPCHK(hipMemcpy(hc.data(), d_c, cnt * sizeof(QdbCand), hipMemcpyDeviceToHost));
+      if (cnt) RCHK(d2h(c, hc.data(), d_c, cnt * sizeof(QdbCand)));
       break;
     }
     std::vector<std::vector<HostCand>> per(k1 - k0);
@@ -754,23 +755,22 @@ int trq_run(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_base, const std::vect
       unsigned long long *d_cnt;
       RCHK(scratch(c, S_CM_ROOT, (size_t)cap * sizeof(TrqCand), (void **)&d_c));
       RCHK(scratch(c, S_COUNTER, 8, (void **)&d_cnt));
-      static const unsigned long long zero = 0;
-      HIPCHK(hipMemcpyAsync(d_cnt, &zero, 8, hipMemcpyHostToDevice, c->stream));
+      HIPCHK(hipMemsetAsync(d_cnt, 0, 8, c->stream));
       HIPCHK(ktime_begin(c));
       HIPCHK(launch_trqdb_scan(q, tabs, inside, d_w, k1 - k0, cutoff, d_c, cap, d_cnt, d_s, maxlen, c->stream));
       HIPCHK(ktime_end(c, IXG_K_TRQDB_SCAN));
       unsigned long long cnt = 0;
-      HIPCHK(hipMemcpy(&cnt, d_cnt, 8, hipMemcpyDeviceToHost));
+      RCHK(d2h(c, &cnt, d_cnt, 8));
       c->trq_cand_rate = std::max(c->trq_cand_rate, (double)cnt / (double)std::max<int64_t>(res, 1));
       if ((int64_t)cnt > cap) {
         cap = (int64_t)cnt + 1024;
         continue;
       }
       hc.resize(cnt);
-      if (cnt) HIPCHK(hipMemcpy(hc.data(), d_c, cnt * sizeof(TrqCand), hipMemcpyDeviceToHost));
+      if (cnt) RCHK(d2h(c, hc.data(), d_c, cnt * sizeof(TrqCand)));
       break;
     }
-    HIPCHK(hipMemcpy(summ.data() + k0, d_s, (size_t)(k1 - k0) * sizeof(TrqSummary), hipMemcpyDeviceToHost));
+    RCHK(d2h(c, summ.data() + k0, d_s, (size_t)(k1 - k0) * sizeof(TrqSummary)));
     std::vector<std::vector<TrqHostCand>> per(k1 - k0);
     for (const TrqCand &x : hc) per[x.win].push_back({x.j, x.d, x.sc, x.root, x.mode});
     for (int64_t k = k0; k < k1; k++)
@@ -866,6 +866,7 @@ int ixg_cm_set_itrpenalties(ixg_ctx *c, ixg_cm *cm, const int32_t *ig, const int
   int32_t *dst = cm->d_tq + 2 * (size_t)M * KP;
   HIPCHK(hipMemcpy(dst, ig, 3 * (size_t)M * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(dst + 3 * (size_t)M, il, 3 * (size_t)M * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipStreamSynchronize(nullptr));  // the null-stream DMA has landed before any context stream reads it
   return IXG_OK;
 }
 

@@ -208,6 +208,17 @@ int ixg_server_create(ixg_ctx *owner, int nstreams, size_t mem_budget, int64_t m
       return rc;
     }
     s->ctx.push_back(c);
+    // staging sized up front for a full batch, so serving does not free and
+    // reallocate pinned memory (a device-wide synchronising call) while other
+    // workers' copies are in flight
+    void *h;
+    int r1 = pinned(c, P_SERVE, (size_t)(2 * max_batch) + ((size_t)1 << 20), &h);
+    int r2 = r1 == IXG_OK ? pinned(c, P_UP, (size_t)4 << 20, &h) : r1;
+    int r3 = r2 == IXG_OK ? pinned(c, P_DOWN, (size_t)4 << 20, &h) : r2;
+    if (r3 != IXG_OK) {
+      for (auto *x : s->ctx) ixg_shutdown(x);
+      return r3;
+    }
   }
   ixg_server *raw = s.get();
   for (int i = 0; i < nstreams; i++) s->th.emplace_back([raw, i] { raw->worker(i); });

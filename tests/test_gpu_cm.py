@@ -27,7 +27,8 @@ def _case(ss, seed, flank=25, r=8):
 
 
 STRUCTS = [("<<<..>>>", 1), ("<.>.<.>", 2), ("<<..>>.<<...>>..", 3), (S.cloverleaf(), 4),
-           (S.multibranch(120, 3, 4), 5), (S.random_nested(90, 6), 6), (S.random_nested(200, 7), 7)]
+           (S.multibranch(120, 3, 4), 5), (S.random_nested(90, 6), 6), (S.random_nested(200, 7), 7),
+           (S.multibranch(300, 12, 8), 8)]
 
 
 @pytest.mark.parametrize("ss,seed", STRUCTS)
@@ -111,8 +112,6 @@ def test_batch_both_strands(ctx):
     g.free()
 
 
-@pytest.mark.skip(reason="the several-CM queue launch is switched off: GPU suite runs that included it faulted "
-                         "later in the window-score kernels; cause not found")
 def test_multi_cm_queue_equals_per_model_batches(ctx):
     # envelopes of three CMs of different sizes interleaved in one persistent
     # launch (ixg_hb_scan_multi) give each envelope exactly what its own

@@ -76,17 +76,8 @@ def _windows(rng, L, n):
     return np.array(offs, np.int64), np.array(lens, np.int64)
 
 
-# M >= 150 is skipped: two runs of the GPU suite failed at the first of these
-# cases it reached (150, then 200 once 150 was skipped) and left the GPU
-# faulted; the logs did not survive the box. The same cases passed in an
-# earlier run with the same window-score kernels; the runs that faulted also
-# ran the several-CM HB queue launch earlier in the session (now switched off),
-# so no cause is shown yet and these cases stay off the GPU until one is.
-_FAULTED = pytest.mark.skip(reason="GPU fault seen at M >= 150 after the several-CM launch ran; cause not found")
-
-
-@pytest.mark.parametrize("M", [1, 2, 3, 8, 9, 17, 33, 64, 65, 99, 100, 101] +
-                         [pytest.param(m, marks=_FAULTED) for m in (150, 200, 300, 400)])
+@pytest.mark.parametrize("M", [1, 2, 3, 8, 9, 17, 33, 64, 65, 99, 100, 101, 128, 129, 150, 200, 256, 257,
+                               300, 400])
 def test_window_scores_parity(ctx, M):
     rng = synth.Lcg(7 + M)
     p = synth.lcg_profile(rng, M, EV)
