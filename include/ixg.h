@@ -99,6 +99,20 @@ int ixg_vit_scores(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *seq, int64_t s
 /* cm_pipeline.rs:1420 forward_filter_score(&ForwardFilter, &[u8], usize) -> f32 (nats) */
 int ixg_fwd_scores(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *seq, int64_t seq_len,
                    const int64_t *off, const int64_t *len, int64_t nwin, float *sc);
+/* No counterpart in the reference. ixg_fwd_scores without any device
+ * allocation, for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc). The sequence, the
+ * window list and the scores are staged in a fixed device array of
+ * IXG_NOALLOC_BYTES that each GPU holds until the process ends; its address
+ * is looked up once per device by ixg_init. Limits: seq_len <=
+ * IXG_NOALLOC_MAX_SEQ and nwin <= IXG_NOALLOC_MAX_WINDOWS (else IXG_ERANGE).
+ * Calls on one GPU share the array: calls that overlap on one GPU (from
+ * several threads or contexts) race. Scores equal ixg_fwd_scores'. */
+#define IXG_NOALLOC_BYTES (24 << 20)
+#define IXG_NOALLOC_MAX_SEQ (16 << 20)
+#define IXG_NOALLOC_MAX_WINDOWS (1 << 18)
+int ixg_fwd_scores_noalloc(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *seq, int64_t seq_len,
+                           const int64_t *off, const int64_t *len, int64_t nwin, float *sc);
 /* cm_pipeline.rs:1722 bias_filter_score(&BiasFilter, &[u8], usize) -> f32 (nats) */
 int ixg_bias_scores(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *seq, int64_t seq_len,
                     const int64_t *off, const int64_t *len, int64_t nwin, float *sc);

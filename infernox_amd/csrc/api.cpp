@@ -155,6 +155,13 @@ struct ixg_p7 {
     return d;
   }
 };
+namespace ixg {
+FwdDev p7_fwd_dev(const ixg_p7 *p) {
+  if (!p->d_fwd) return FwdDev{};
+  return p->fwd_dev();
+}
+}  // namespace ixg
+
 struct ixg_target {
   uint8_t *d_both = nullptr;  // [fwd (L+2) | revcomp (L+2)], sentinel-padded
   int64_t L = 0;
@@ -196,6 +203,7 @@ int ixg_init(int device, size_t mem_budget, ixg_ctx **out) {
   for (auto &ev : c->ev) HIPCHK(hipEventCreate(&ev));
   for (auto &ev : c->kpool) HIPCHK(hipEventCreate(&ev));
   for (auto &ev : c->sev) HIPCHK(hipEventCreate(&ev));
+  RCHK(noalloc_setup(device));
   *out = c;
   return IXG_OK;
 }

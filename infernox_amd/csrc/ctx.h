@@ -45,6 +45,9 @@ int64_t arena_elems(const ixg_ctx *c, int slot);
 // (a plain hipMemcpy runs on the null stream, which does not wait for a
 // non-blocking stream); returns when the bytes are on the host.
 int d2h(ixg_ctx *c, void *dst, const void *src, size_t n);
+// Looks up the device address of the no-allocation scratch array for
+// `device` (current) and keeps it; called by ixg_init.
+int noalloc_setup(int device);
 int pinned(ixg_ctx *c, int slot, size_t n, void **out);
 
 // Runs f(k) for k in [0, n) on up to 16 host threads, in chunks of 4 items

@@ -131,7 +131,7 @@ def lib():
         L.ixg_ssv_diagonals.argtypes = [vp, vp, u8p, i64, C.c_double, C.POINTER(RawWin), i64, i64p]
         for nm in ("ixg_vit_scores", "ixg_msv_scores"):
             getattr(L, nm).argtypes = [vp, vp, u8p, i64, i64p, i64p, i64, f32p, u8p]
-        for nm in ("ixg_fwd_scores", "ixg_bias_scores"):
+        for nm in ("ixg_fwd_scores", "ixg_bias_scores", "ixg_fwd_scores_noalloc"):
             getattr(L, nm).argtypes = [vp, vp, u8p, i64, i64p, i64p, i64, f32p]
         L.ixg_f3_filter_sequence.argtypes = [vp, vp, u8p, i64, C.POINTER(F3Opts), C.POINTER(Window), i64, i64p,
                                              C.POINTER(Acct)]
@@ -340,6 +340,10 @@ class P7:
 
     def fwd_scores(self, seq, offs, lens):
         return self._scores("ixg_fwd_scores", seq, offs, lens, False)
+
+    def fwd_scores_noalloc(self, seq, offs, lens):
+        """fwd_scores through the entry point that allocates no device memory."""
+        return self._scores("ixg_fwd_scores_noalloc", seq, offs, lens, False)
 
     def bias_scores(self, seq, offs, lens):
         return self._scores("ixg_bias_scores", seq, offs, lens, False)
