@@ -424,14 +424,56 @@ typedef struct { int32_t i0, j0, strand, pad; } ixg_cp9win;
 int ixg_cp9_bands_batch(ixg_ctx *ctx, const ixg_cp9 *h, const ixg_target *tgt, const ixg_cp9win *win, int64_t n,
                         double tau, int mode, float *fwd_sc, float *bwd_sc, int32_t *pn);
 
+/* ---- F4/F5: glocal envelope definition (STD pass) -----------------------------
+ * The glocal profile (p7_generic.rs:156 build_glocal_profile, bg 0.25) is
+ * built from the same ixg_p7_hmm by ixg_p7_create and kept resident.
+ *
+ * ixg_gfb_scores replaces p7_generic.rs:519 p7_gforward and :939
+ * p7_gbackward (nats, the 16,000-entry FLogsum table) over a batch of items
+ * seq[off[i]+1 .. off[i]+len[i]]: the profile is configured multihit (or
+ * unihit with IXG_GFB_UNIHIT) for length cfg_len[i] (len[i] when cfg_len is
+ * NULL; p7_generic.rs:379 reconfig_length). sc[i]: the score; xmx (optional):
+ * the special states [(len[i]+1)][E N J B C] of every item, concatenated;
+ * mat (optional, Forward only, IXG_GFB_MATRIX): [(len[i]+1)][M | I | D][M+1]
+ * per item, concatenated. */
+#define IXG_GFB_BACKWARD 1
+#define IXG_GFB_UNIHIT 2
+#define IXG_GFB_MATRIX 4
+int ixg_gfb_scores(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *seq, int64_t seq_len, const int64_t *off,
+                   const int64_t *len, const int32_t *cfg_len, int64_t n, int mode, float *sc, float *xmx,
+                   float *mat);
+/* Thresholds of the envelope definition (cm_search.rs:2449 trunc_env_def,
+ * STD pass): F4 / F4b / F5 / F5b P-values against the glocal Forward tail
+ * (gfmu, gflambda); rt1 rt2 rt3 and ns of p7_generic.rs:1765
+ * p7_domaindef_glocal (the multidomain traces use Easel's fast RNG, seed 181). */
+typedef struct {
+  double f4, f4b, f5, f5b;
+  double gfmu, gflambda;
+  float rt1, rt2, rt3;
+  int32_t ns, do_gfwdbias, do_edefbias;
+} ixg_envopts;
+/* Per-pass accounting of the stage (cm_search.rs add_gfwd / add_gfwdbias /
+ * add_edef / add_edefbias). */
+typedef struct {
+  uint64_t n_past_gfwd, pos_past_gfwd, n_past_gfwdbias, pos_past_gfwdbias;
+  uint64_t n_past_edef, pos_past_edef, n_past_edefbias, pos_past_edefbias;
+} ixg_envacct;
+/* Replaces the STD-pass call of cm_search.rs:2449 trunc_env_def(dsq, L,
+ * merged windows, PLI_PASS_STD_ANY, ...) -> Vec<(es, ee)>: the F3b-surviving
+ * windows of one chunk (dsq 1-indexed, L residues) to envelopes in chunk
+ * coordinates, in the reference's order. */
+int ixg_env_def(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *dsq, int64_t L, const ixg_window *wins,
+                int64_t nwin, const ixg_envopts *o, int32_t *es, int32_t *ee, int64_t cap, int64_t *nenv,
+                ixg_envacct *acct);
+
 /* Per-kernel device time, cumulative since the last reset: HIP events around
  * every launch of each kernel (IXG_K_* index), total milliseconds and launch
  * counts; reset != 0 clears them after reading. A measurement hook with no
- * counterpart in the reference. Timing synchronizes the context's stream
- * after each launch. */
+ * counterpart in the reference. The events are recorded without waiting and
+ * read when the timings are queried (or a pool of 256 launches fills). */
 enum { IXG_K_SSV_SWEEP = 0, IXG_K_SSV_RESOLVE = 1, IXG_K_FWD = 2, IXG_K_BIAS = 3, IXG_K_VIT = 4, IXG_K_MSV = 5,
        IXG_K_HB_SCAN = 6, IXG_K_TRHB_SCAN = 7, IXG_K_COPY = 8, IXG_K_QDB_SCAN = 9, IXG_K_FWD_FULL = 10,
-       IXG_K_TRQDB_SCAN = 11, IXG_K_CP9 = 12, IXG_NKERNELS = 13 };
+       IXG_K_TRQDB_SCAN = 11, IXG_K_CP9 = 12, IXG_K_GFWD = 13, IXG_K_GBWD = 14, IXG_NKERNELS = 15 };
 int ixg_kernel_timings(ixg_ctx *ctx, double *ms, int64_t *launches, int reset);
 /* Measurement hook (no counterpart in the reference): copies `bytes`
  * device-to-device with a plain kernel reps times after one warm-up copy;

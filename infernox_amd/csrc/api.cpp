@@ -105,6 +105,8 @@ struct ixg_p7 {
   VitHost vit;
   FwdHost fwd;
   BiasHost bias;
+  GmHost gm;                // glocal profile of the F4/F5 envelope definition
+  float *d_gm = nullptr;    // tsc | msc | isc | nats logsum table
   uint32_t *d_cost4 = nullptr;
   uint8_t *d_rbv = nullptr;
   float *d_fwd = nullptr;   // tmain | tddv | rfvv | tz
@@ -136,6 +138,16 @@ struct ixg_p7 {
     d.nq = fwd.nq;
     return d;
   }
+  GmDev gm_dev() const {
+    GmDev d;
+    const size_t M1 = (size_t)gm.M + 1;
+    d.tsc = d_gm;
+    d.msc = d_gm + 8 * M1;
+    d.isc = d.msc + KP * M1;
+    d.tbl = d.isc + KP * M1;
+    d.m = gm.M;
+    return d;
+  }
   VitDev vit_dev() const {
     VitDev d;
     d.rwv = d_vit;
@@ -156,6 +168,9 @@ struct ixg_p7 {
   }
 };
 namespace ixg {
+GmDev p7_gm_dev(const ixg_p7 *p) { return p->gm_dev(); }
+const GmHost &p7_gm_host(const ixg_p7 *p) { return p->gm; }
+bool p7_has_device(const ixg_p7 *p) { return p->d_gm != nullptr; }
 FwdDev p7_fwd_dev(const ixg_p7 *p) {
   if (!p->d_fwd) return FwdDev{};
   return p->fwd_dev();
@@ -235,6 +250,7 @@ int ixg_p7_create(ixg_ctx *c, const ixg_p7_hmm *h, int32_t max_length, ixg_p7 **
   build_vit(*h, p->vit);
   build_fwd(*h, p->fwd);
   build_bias(h->compo, h->M, p->bias);
+  build_glocal(*h, p->gm);
   p->vit_lvmu = h->lvmu;
   p->vit_lvlambda = h->lvlambda;
   if (!c) {  // host tables only (inspection); no device copy
@@ -269,12 +285,19 @@ int ixg_p7_create(ixg_ctx *c, const ixg_p7_hmm *h, int32_t max_length, ixg_p7 **
   for (auto *a : {&p->vit.tmm, &p->vit.tim, &p->vit.tdm, &p->vit.vbm, &p->vit.tmi, &p->vit.tii, &p->vit.tmd,
                   &p->vit.tdd})
     vit.insert(vit.end(), a->begin(), a->end());
+  std::vector<float> gmv(p->gm.tsc);
+  gmv.insert(gmv.end(), p->gm.msc.begin(), p->gm.msc.end());
+  gmv.insert(gmv.end(), p->gm.isc.begin(), p->gm.isc.end());
+  gmv.insert(gmv.end(), p7_logsum_table().begin(), p7_logsum_table().end());
+  const size_t b4 = gmv.size() * 4;
+  RCHK(dmalloc(c, (void **)&p->d_gm, b4));
+  HIPCHK(hipMemcpy(p->d_gm, gmv.data(), b4, hipMemcpyHostToDevice));
   size_t b0 = cost4.size() * 4, b1 = p->msv.rbv.size(), b2 = fwd.size() * 4, b3 = vit.size() * 2;
   RCHK(dmalloc(c, (void **)&p->d_cost4, b0));
   RCHK(dmalloc(c, (void **)&p->d_rbv, b1));
   RCHK(dmalloc(c, (void **)&p->d_fwd, b2));
   RCHK(dmalloc(c, (void **)&p->d_vit, b3));
-  p->bytes = b0 + b1 + b2 + b3;
+  p->bytes = b0 + b1 + b2 + b3 + b4;
   HIPCHK(hipMemcpy(p->d_cost4, cost4.data(), b0, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(p->d_rbv, p->msv.rbv.data(), b1, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(p->d_fwd, fwd.data(), b2, hipMemcpyHostToDevice));
@@ -295,6 +318,7 @@ int ixg_p7_free(ixg_ctx *c, ixg_p7 *p) {
   dfree(c, p->d_rbv, p->msv.rbv.size());
   dfree(c, p->d_fwd, (p->fwd.tmain.size() + p->fwd.tddv.size() + p->fwd.rfvv.size()) * 4);
   dfree(c, p->d_vit, (p->vit.rwv.size() + 8 * (p->vit.m + 1)) * 2);
+  dfree(c, p->d_gm, (p->gm.tsc.size() + p->gm.msc.size() + p->gm.isc.size() + GFB_TBL) * 4);
   delete p;
   return IXG_OK;
 }
@@ -409,10 +433,6 @@ int64_t ixg_window_tasks(int64_t L, int64_t maxw, int do_top, int do_bot, ixg_ta
 // ---------------------------------------------------------------------------
 namespace {
 
-struct HostTask {
-  int64_t base;  // offset of the task's dsq view (1-indexed) in the device sequence buffer
-  int64_t L;
-};
 
 struct RawW {
   int64_t n, length, k;
@@ -571,14 +591,10 @@ int run_f1(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<
   return IXG_OK;
 }
 
-struct WinRef {
-  int32_t task;
-  int64_t start, end;  // chunk-local window
-};
 
 // Scores for a list of windows: kind 0 fwd, 1 bias, 2 vit, 3 msv (F1b).
-int run_scores(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<HostTask> &tasks,
-               const std::vector<WinRef> &wl, int kind, std::vector<float> &sc, std::vector<uint8_t> &ovf) {
+int scores_impl(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<HostTask> &tasks,
+                const std::vector<WinRef> &wl, int kind, std::vector<float> &sc, std::vector<uint8_t> &ovf) {
   const int64_t n = (int64_t)wl.size();
   sc.assign(n, 0.f);
   ovf.assign(n, 0);
@@ -706,7 +722,7 @@ int cascade(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector
       }
     std::vector<float> sc;
     std::vector<uint8_t> ov;
-    RCHK(run_scores(c, p, d_seq, tasks, sub, 1, sc, ov));
+    RCHK(scores_impl(c, p, d_seq, tasks, sub, 1, sc, ov));
     for (size_t j = 0; j < need.size(); j++) {
       biassc[need[j]] = sc[j];
       have_bias[need[j]] = 1;
@@ -721,7 +737,7 @@ int cascade(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector
   HIPCHK(hipEventRecord(e0, c->stream));
   if (do_msvbias) {  // F1b (cm_pipeline.rs:1947-1958)
     alive_list(idx, sub);
-    RCHK(run_scores(c, p, d_seq, tasks, sub, 3, sc, ov));
+    RCHK(scores_impl(c, p, d_seq, tasks, sub, 3, sc, ov));
     RCHK(ensure_bias(idx));
     for (size_t j = 0; j < idx.size(); j++) {
       if (ov[j]) continue;
@@ -736,7 +752,7 @@ int cascade(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector
   }
   if (do_vit) {  // F2/F2b (cm_pipeline.rs:1966-1997)
     alive_list(idx, sub);
-    RCHK(run_scores(c, p, d_seq, tasks, sub, 2, sc, ov));
+    RCHK(scores_impl(c, p, d_seq, tasks, sub, 2, sc, ov));
     std::vector<size_t> pass;
     for (size_t j = 0; j < idx.size(); j++) {
       const size_t i = idx[j];
@@ -766,7 +782,7 @@ int cascade(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector
   }
   if (do_fwd) {  // F3/F3b (cm_pipeline.rs:2013-2052)
     alive_list(idx, sub);
-    RCHK(run_scores(c, p, d_seq, tasks, sub, 0, sc, ov));
+    RCHK(scores_impl(c, p, d_seq, tasks, sub, 0, sc, ov));
     std::vector<size_t> pass;
     for (size_t j = 0; j < idx.size(); j++) {
       const size_t i = idx[j];
@@ -877,6 +893,10 @@ int upload_seq(ixg_ctx *c, const uint8_t *dsq, int64_t L, uint8_t **out) {
 }  // namespace
 
 namespace ixg {
+int run_scores(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<HostTask> &tasks,
+               const std::vector<WinRef> &wl, int kind, std::vector<float> &sc, std::vector<uint8_t> &ovf) {
+  return scores_impl(c, p, d_seq, tasks, wl, kind, sc, ovf);
+}
 
 // The cascade over tasks given as (base, L) views of a device sequence
 // buffer; windows come back grouped by task with their task index, first[t]
@@ -940,7 +960,7 @@ static int scores_entry(ixg_ctx *c, const ixg_p7 *p, const uint8_t *seq, int64_t
   for (int64_t i = 0; i < nwin; i++) wl[i] = {0, off[i] + 1, off[i] + len[i]};
   std::vector<float> s;
   std::vector<uint8_t> o;
-  RCHK(run_scores(c, p, d, tasks, wl, kind, s, o));
+  RCHK(scores_impl(c, p, d, tasks, wl, kind, s, o));
   memcpy(sc, s.data(), nwin * 4);
   if (ovf) memcpy(ovf, o.data(), nwin);
   return IXG_OK;

@@ -31,7 +31,8 @@ struct DevBuf {
 };
 
 enum Slot { S_TASKS, S_ITEMS, S_MASK, S_RAW, S_COUNTER, S_WINS, S_SC0, S_SC1, S_SC2, S_OVF0, S_OVF1, S_SEQ,
-            S_CM_ENV, S_CM_BANDS, S_CM_ALPHA, S_CM_SEQ, S_CINFO, S_CM_ROOT, S_CM_SCHED, S_SERVE_SEQ, S_NSLOTS };
+            S_CM_ENV, S_CM_BANDS, S_CM_ALPHA, S_CM_SEQ, S_CINFO, S_CM_ROOT, S_CM_SCHED, S_SERVE_SEQ,
+            S_GFB_ITEMS, S_GFB_X, S_GFB_M, S_GFB_SC, S_NSLOTS };
 
 int dmalloc(ixg_ctx *c, void **p, size_t n);
 void dfree(ixg_ctx *c, void *p, size_t n);
@@ -45,6 +46,21 @@ int64_t arena_elems(const ixg_ctx *c, int slot);
 // (a plain hipMemcpy runs on the null stream, which does not wait for a
 // non-blocking stream); returns when the bytes are on the host.
 int d2h(ixg_ctx *c, void *dst, const void *src, size_t n);
+struct HostTask {
+  int64_t base;  // offset of the task's dsq view (1-indexed) in the device sequence buffer
+  int64_t L;
+};
+struct WinRef {
+  int32_t task;
+  int64_t start, end;  // chunk-local window
+};
+// Per-window F1b/F2/F3/F3b scores (kind 1 bias, 0 Forward, 2 Viterbi, 3
+// MSV) of windows wl over tasks whose dsq views sit in d_seq.
+int run_scores(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<HostTask> &tasks,
+               const std::vector<WinRef> &wl, int kind, std::vector<float> &sc, std::vector<uint8_t> &ovf);
+GmDev p7_gm_dev(const ixg_p7 *p);
+const GmHost &p7_gm_host(const ixg_p7 *p);
+bool p7_has_device(const ixg_p7 *p);
 // Looks up the device address of the no-allocation scratch array for
 // `device` (current) and keeps it; called by ixg_init.
 int noalloc_setup(int device);

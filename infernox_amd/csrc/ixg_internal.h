@@ -61,6 +61,27 @@ struct BiasHost {
   float pi[2];
 };
 
+// Glocal p7 profile (p7_generic.rs:134-147): tsc [(M+1)][8] in the order
+// below, msc / isc [KP][M+1], special states [E N J C][LOOP MOVE], nats.
+struct GmHost {
+  enum { MM = 0, IM = 1, DM = 2, BM = 3, MD = 4, DD = 5, MI = 6, II = 7 };
+  enum { XE = 0, XN = 1, XJ = 2, XC = 3 };
+  int M = 0;
+  std::vector<float> tsc, msc, isc;
+  float xsc[4][2] = {};
+  float nj = 0.f;
+  int len = 0;
+  void set_length(int L);
+  void set_unihit();
+  void set_multihit();
+  void xsc8(float out[8]) const;
+};
+void build_glocal(const ixg_p7_hmm &h, GmHost &g);
+const std::vector<float> &p7_logsum_table();
+float p7_logsum(float a, float b);
+void domain_decoding(const GmHost &g, int L, const float *fx, const float *bx, float *btot, float *etot,
+                     float *mocc);
+
 void build_msv(const ixg_p7_hmm &h, int max_length, MsvHost &f);
 void build_vit(const ixg_p7_hmm &h, VitHost &f);
 void build_fwd(const ixg_p7_hmm &h, FwdHost &f);
@@ -121,6 +142,28 @@ struct WinDev {
   int32_t idx;   // index into the output score arrays
   float a, b;    // per-window host-computed terms (length model pieces)
 };
+
+// Glocal p7 profile for the F4/F5 envelope definition (p7_generic.rs:134-147),
+// device form: tsc [(M+1)][8] (MM IM DM BM MD DD MI II, nats), msc / isc
+// [KP][M+1], and the 16,000-entry nats FLogsum table.
+constexpr int GFB_TBL = 16000;
+struct GmDev {
+  const float *tsc, *msc, *isc, *tbl;
+  int m;
+};
+// One glocal Forward / Backward item: the dsq at seq + off (1-indexed, len
+// residues), its special-state scores [E N J C][LOOP MOVE], flags bit 0 =
+// local ends (esc = 0), bit 1 = keep the Forward matrix; the specials go to
+// xout + xoff [(len+1)][E N J B C], the matrix to mout + moff
+// [(len+1)][M | I | D][M+1].
+struct GfbItem {
+  int64_t off;
+  int32_t len, flags;
+  float xsc[8];
+  int64_t xoff, moff;
+};
+hipError_t launch_gfb(const GmDev &g, const uint8_t *seq, const GfbItem *items, int64_t n, float *xout, float *mout,
+                      float *sc, bool backward, hipStream_t s);
 
 // Kernel launchers (p7_kernels.hip).
 constexpr int SWEEP_SEG = 4096;  // rows per F1 sweep work item (multiple of 64)

@@ -62,10 +62,11 @@ EXPORTS = [
     "ixg_cyk_seq_filter", "ixg_p7_forward", "ixg_p7_forward_batch", "ixg_cm_set_itrpenalties", "ixg_cm_imtables",
     "ixg_trqdb_scan", "ixg_trqdb_scan_batch", "ixg_cp9_create", "ixg_cp9_free", "ixg_cp9_bands",
     "ixg_cp9_bands_batch", "ixg_server_create", "ixg_server_submit", "ixg_server_wait", "ixg_server_result",
-    "ixg_server_stats", "ixg_server_free", "ixg_hb_scan_multi",
+    "ixg_server_stats", "ixg_server_free", "ixg_hb_scan_multi", "ixg_fwd_scores_noalloc", "ixg_gfb_scores",
+    "ixg_env_def",
 ]
 KERNELS = ["ssv_sweep", "ssv_resolve", "fwd", "bias", "vit", "msv", "hb_scan", "trhb_scan", "copy_probe",
-           "qdb_scan", "fwd_full", "trqdb_scan", "cp9"]
+           "qdb_scan", "fwd_full", "trqdb_scan", "cp9", "gfwd", "gbwd"]
 
 
 class CmParams(C.Structure):
@@ -898,3 +899,93 @@ def cp9_bands_batch(ctx, cp9, tgt, wins, tau, scan=True, mode=None):
     check(_cp9_lib().ixg_cp9_bands_batch(ctx.h, cp9.h, tgt.h, arr, n, float(tau), cp9_mode(scan, mode), f32ptr(fs), f32ptr(bs),
                                     _i32p(pn)), "ixg_cp9_bands_batch")
     return pn[:n * 6 * M1].reshape(n, 6, M1), fs[:n], bs[:n]
+
+
+# ---- F4/F5 envelope definition ----------------------------------------------
+
+IXG_GFB_BACKWARD, IXG_GFB_UNIHIT, IXG_GFB_MATRIX = 1, 2, 4
+
+
+class EnvOpts(C.Structure):
+    _fields_ = [("f4", C.c_double), ("f4b", C.c_double), ("f5", C.c_double), ("f5b", C.c_double),
+                ("gfmu", C.c_double), ("gflambda", C.c_double), ("rt1", C.c_float), ("rt2", C.c_float),
+                ("rt3", C.c_float), ("ns", C.c_int32), ("do_gfwdbias", C.c_int32), ("do_edefbias", C.c_int32)]
+
+
+class EnvAcct(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "n_past_gfwd", "pos_past_gfwd", "n_past_gfwdbias", "pos_past_gfwdbias",
+        "n_past_edef", "pos_past_edef", "n_past_edefbias", "pos_past_edefbias")]
+
+
+def env_opts(f4=0.005, f4b=0.005, f5=0.005, f5b=0.005, gfmu=-5.0, gflambda=0.69, rt1=0.25, rt2=0.10, rt3=0.20,
+             ns=200, do_gfwdbias=True, do_edefbias=False):
+    o = EnvOpts()
+    o.f4, o.f4b, o.f5, o.f5b, o.gfmu, o.gflambda = f4, f4b, f5, f5b, gfmu, gflambda
+    o.rt1, o.rt2, o.rt3, o.ns = rt1, rt2, rt3, ns
+    o.do_gfwdbias, o.do_edefbias = int(bool(do_gfwdbias)), int(bool(do_edefbias))
+    return o
+
+
+_env_ready = False
+
+
+def _env_lib():
+    global _env_ready
+    L = lib()
+    if not _env_ready:
+        vp, i64 = C.c_void_p, C.c_int64
+        i32p, i64p, f32p, u8p = C.POINTER(C.c_int32), C.POINTER(C.c_int64), C.POINTER(C.c_float), C.POINTER(C.c_uint8)
+        L.ixg_gfb_scores.argtypes = [vp, vp, u8p, i64, i64p, i64p, i32p, i64, C.c_int, f32p, f32p, f32p]
+        L.ixg_env_def.argtypes = [vp, vp, u8p, i64, C.POINTER(Window), i64, C.POINTER(EnvOpts), i32p, i32p, i64, i64p,
+                                  C.POINTER(EnvAcct)]
+        _env_ready = True
+    return L
+
+
+def gfb_scores(p7, seq, offs, lens, cfg_len=None, backward=False, unihit=False, matrix=False):
+    """Glocal Forward / Backward over items seq[off+1 .. off+len] (ixg_gfb_scores).
+    Returns (scores, [specials (len+1, 5) per item], [matrix (len+1, 3, M+1) per item] or None)."""
+    seq = np.ascontiguousarray(seq, np.uint8)
+    offs = np.ascontiguousarray(offs, np.int64)
+    lens = np.ascontiguousarray(lens, np.int64)
+    n, M = len(offs), p7.M
+    sc = np.zeros(max(n, 1), np.float32)
+    xm = np.zeros(max(int(5 * (lens + 1).sum()), 1), np.float32)
+    mat = np.zeros(max(int(3 * (M + 1) * (lens + 1).sum()), 1), np.float32) if matrix else None
+    cl = None if cfg_len is None else np.ascontiguousarray(cfg_len, np.int32)
+    mode = (IXG_GFB_BACKWARD if backward else 0) | (IXG_GFB_UNIHIT if unihit else 0) | (IXG_GFB_MATRIX if matrix else 0)
+    check(_env_lib().ixg_gfb_scores(p7.ctx.h, p7.h, u8ptr(seq), len(seq), i64ptr(offs), i64ptr(lens),
+                                    None if cl is None else cl.ctypes.data_as(C.POINTER(C.c_int32)), n, mode,
+                                    f32ptr(sc), f32ptr(xm), None if mat is None else f32ptr(mat)), "ixg_gfb_scores")
+    xs, ms, xo, mo = [], [], 0, 0
+    for ln in lens:
+        xs.append(xm[xo:xo + 5 * (ln + 1)].reshape(ln + 1, 5))
+        xo += 5 * (ln + 1)
+        if matrix:
+            ms.append(mat[mo:mo + 3 * (M + 1) * (ln + 1)].reshape(ln + 1, 3, M + 1))
+            mo += 3 * (M + 1) * (ln + 1)
+    return sc[:n], xs, (ms if matrix else None)
+
+
+def env_def(p7, dsq, L, windows, opts):
+    """STD-pass envelope definition of one chunk (ixg_env_def): windows
+    [(start, end)] -> ([(es, ee)], EnvAcct)."""
+    d = np.ascontiguousarray(dsq, np.uint8)
+    n = len(windows)
+    arr = (Window * max(n, 1))()
+    for k, (a, b) in enumerate(windows):
+        arr[k].start, arr[k].end, arr[k].score, arr[k].task = int(a), int(b), 0.0, 0
+    cap = 64 + 8 * n
+    while True:
+        es, ee = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
+        nenv = C.c_int64()
+        acct = EnvAcct()
+        rc = _env_lib().ixg_env_def(p7.ctx.h, p7.h, u8ptr(d), int(L), arr, n, C.byref(opts),
+                                    es.ctypes.data_as(C.POINTER(C.c_int32)), ee.ctypes.data_as(C.POINTER(C.c_int32)),
+                                    cap, C.byref(nenv), C.byref(acct))
+        if rc == IXG_ERANGE and nenv.value > cap:
+            cap = nenv.value
+            continue
+        check(rc, "ixg_env_def")
+        return [(int(es[i]), int(ee[i])) for i in range(nenv.value)], acct
