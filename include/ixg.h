@@ -466,6 +466,27 @@ int ixg_env_def(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *dsq, int64_t L, c
                 int64_t nwin, const ixg_envopts *o, int32_t *es, int32_t *ee, int64_t cap, int64_t *nenv,
                 ixg_envacct *acct);
 
+/* ---- HMM -> CM band conversion (cp9.rs:2893-3879) ---------------------------
+ * A host map of one CM and its CP9 (ixg_cm_params / ixg_cp9_params, the CP9
+ * having one node per consensus column): the guide-tree walk and the CP9
+ * terms the conversion reads. cm_local / cp9_local: local begins and ends on
+ * (cmsearch's default configuration of both). */
+typedef struct ixg_hbmap ixg_hbmap;
+int ixg_hbmap_create(const ixg_cm_params *cm, int cm_local, const ixg_cp9_params *cp9, int cp9_local,
+                     ixg_hbmap **out);
+int ixg_hbmap_free(ixg_hbmap *m);
+/* cp9.rs:3380 cp9_hmm2ijbands (non-truncated) with
+ * hmm_bands_enforce_valid_parse: pn [6][hmm_m+1] (as ixg_cp9_bands returns
+ * it; doctored in place when local begins/ends are off) -> per-state i and j
+ * bands imin/imax/jmin/jmax [M] (-1/-2 empty). */
+int ixg_hmm2ij_bands(const ixg_hbmap *m, int32_t *pn, int32_t i0, int32_t j0, int doing_search, int32_t *imin,
+                     int32_t *imax, int32_t *jmin, int32_t *jmax);
+/* cp9.rs:3815 ij2d_bands into the ixg_bands layout: hd_off [M+1], rows
+ * hdmin/hdmax; *nrows = rows needed (IXG_ERANGE above cap). */
+int ixg_ij2d_bands(const ixg_hbmap *m, const int32_t *imin, const int32_t *imax, const int32_t *jmin,
+                   const int32_t *jmax, int do_trunc, int32_t *hd_off, int32_t *hdmin, int32_t *hdmax, int64_t cap,
+                   int64_t *nrows);
+
 /* Per-kernel device time, cumulative since the last reset: HIP events around
  * every launch of each kernel (IXG_K_* index), total milliseconds and launch
  * counts; reset != 0 clears them after reading. A measurement hook with no

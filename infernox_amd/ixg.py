@@ -63,7 +63,7 @@ EXPORTS = [
     "ixg_trqdb_scan", "ixg_trqdb_scan_batch", "ixg_cp9_create", "ixg_cp9_free", "ixg_cp9_bands",
     "ixg_cp9_bands_batch", "ixg_server_create", "ixg_server_submit", "ixg_server_wait", "ixg_server_result",
     "ixg_server_stats", "ixg_server_free", "ixg_hb_scan_multi", "ixg_fwd_scores_noalloc", "ixg_gfb_scores",
-    "ixg_env_def",
+    "ixg_env_def", "ixg_hbmap_create", "ixg_hbmap_free", "ixg_hmm2ij_bands", "ixg_ij2d_bands",
 ]
 KERNELS = ["ssv_sweep", "ssv_resolve", "fwd", "bias", "vit", "msv", "hb_scan", "trhb_scan", "copy_probe",
            "qdb_scan", "fwd_full", "trqdb_scan", "cp9", "gfwd", "gbwd"]
@@ -989,3 +989,72 @@ def env_def(p7, dsq, L, windows, opts):
             continue
         check(rc, "ixg_env_def")
         return [(int(es[i]), int(ee[i])) for i in range(nenv.value)], acct
+
+
+# ---- HMM -> CM band conversion ------------------------------------------------
+
+def cp9_params(h):
+    """Cp9Params view of a CP9 dict (keeps its arrays alive)."""
+    keep = {k: np.ascontiguousarray(h[k], np.uint8 if k == "has_el" else np.int32)
+            for k in ("otsc", "msc", "isc", "has_el", "el_from_ct", "el_from_off", "el_from_idx")}
+    p = Cp9Params()
+    p.M, p.el, p.el_selfsc, p.pad = int(h["M"]), int(bool(h["el"])), int(h["el_selfsc"]), 0
+    for k, v in keep.items():
+        setattr(p, k, v.ctypes.data_as(C.POINTER(C.c_uint8 if k == "has_el" else C.c_int32)))
+    p._keep = keep
+    return p
+
+
+_hb_ready = False
+
+
+def _hb_lib():
+    global _hb_ready
+    L = lib()
+    if not _hb_ready:
+        vp, i32p, i64p = C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int64)
+        L.ixg_hbmap_create.argtypes = [C.POINTER(CmParams), C.c_int, C.POINTER(Cp9Params), C.c_int, C.POINTER(vp)]
+        L.ixg_hbmap_free.argtypes = [vp]
+        L.ixg_hmm2ij_bands.argtypes = [vp, i32p, C.c_int32, C.c_int32, C.c_int, i32p, i32p, i32p, i32p]
+        L.ixg_ij2d_bands.argtypes = [vp, i32p, i32p, i32p, i32p, C.c_int, i32p, i32p, i32p, C.c_int64, i64p]
+        _hb_ready = True
+    return L
+
+
+def _i32(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class HbMap:
+    """Host map of a CM and its CP9 for the HMM -> CM band conversion."""
+
+    def __init__(self, cm, cp9, cm_local=True, cp9_local=True):
+        self.M = int(cm["M"])
+        self._c, self._p = cm_params(cm), cp9_params(cp9)
+        self.h = C.c_void_p()
+        check(_hb_lib().ixg_hbmap_create(C.byref(self._c), int(cm_local), C.byref(self._p), int(cp9_local),
+                                         C.byref(self.h)), "ixg_hbmap_create")
+
+    def free(self):
+        if self.h:
+            _hb_lib().ixg_hbmap_free(self.h)
+            self.h = C.c_void_p()
+
+    def hmm2ij(self, pn, i0, j0, search=True):
+        """Returns (imin, imax, jmin, jmax, pn after doctoring)."""
+        pn = np.ascontiguousarray(pn, np.int32).copy()
+        out = [np.zeros(self.M, np.int32) for _ in range(4)]
+        check(_hb_lib().ixg_hmm2ij_bands(self.h, _i32(pn), int(i0), int(j0), int(search), *[_i32(a) for a in out]),
+              "ixg_hmm2ij_bands")
+        return (*out, pn)
+
+    def ij2d(self, imin, imax, jmin, jmax, trunc=False):
+        """Bands dict in the scanners' layout (jmin, jmax, hd_off, hdmin, hdmax)."""
+        a = [np.ascontiguousarray(x, np.int32) for x in (imin, imax, jmin, jmax)]
+        hd_off = np.zeros(self.M + 1, np.int32)
+        n = C.c_int64()
+        _hb_lib().ixg_ij2d_bands(self.h, *[_i32(x) for x in a], int(trunc), _i32(hd_off), None, None, 0, C.byref(n))
+        hdmin, hdmax = np.zeros(max(n.value, 1), np.int32), np.zeros(max(n.value, 1), np.int32)
+        check(_hb_lib().ixg_ij2d_bands(self.h, *[_i32(x) for x in a], int(trunc), _i32(hd_off), _i32(hdmin),
+                                       _i32(hdmax), n.value, C.byref(n)), "ixg_ij2d_bands")
+        return dict(jmin=a[2], jmax=a[3], hd_off=hd_off, hdmin=hdmin[:n.value], hdmax=hdmax[:n.value])

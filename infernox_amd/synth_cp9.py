@@ -103,3 +103,36 @@ def build_cp9(cm, seed=0, el=True, pbegin=0.05, pend=0.05, gap=0.05):
     idx = np.array([y for x in srcs for y in x] or [0], np.int32)
     return dict(M=M, el=bool(el), el_selfsc=int(prob2score(np.float32(0.94))), otsc=otsc, msc=msc, isc=isc,
                 has_el=has_el, el_from_ct=ct, el_from_off=off, el_from_idx=idx)
+
+
+def build_cp9_like(cm, seed=0, el=True, pbegin=0.05, pend=0.05):
+    """A configured CP9 whose match emissions follow the CM's consensus
+    columns (the marginal of the CM state that emits each column: MATL ML,
+    MATR MR, or the left / right marginal of MATP MP) and whose transitions
+    mostly stay on the match path, so its posterior bands track the CM's
+    homologs. A stand-in for the reference's CM -> CP9 construction
+    (cp9.rs:2212), which is not reproduced; the local-end lists are the
+    guide tree's, as in build_cp9."""
+    h = build_cp9(cm, seed=seed, el=el, pbegin=pbegin, pend=pend)
+    lpos, rpos = emit_map(cm)
+    M = h["M"]
+    mat = np.full((M + 1, 4), 0.25)
+    for nd in range(cm["nodes"]):
+        t, v = cm["ndtype"][nd], cm["nodemap"][nd]
+        e = np.asarray(cm["e"][v], np.float64)
+        if t == S.MATL_ND:
+            mat[lpos[nd]] = e[:4]
+        elif t == S.MATR_ND:
+            mat[rpos[nd]] = e[:4]
+        elif t == S.MATP_ND:
+            pe = e.reshape(4, 4)
+            mat[lpos[nd]] = pe.sum(axis=1)
+            mat[rpos[nd]] = pe.sum(axis=0)
+    msc = h["msc"].copy()
+    for k in range(1, M + 1):
+        p = mat[k] / mat[k].sum()
+        sc = prob2score(p.astype(np.float32), 0.25)
+        msc[k, :4] = sc
+        msc[k, 5:16] = int(np.mean(sc))
+    h["msc"] = msc
+    return h

@@ -87,3 +87,76 @@ class OracleCP9:
         lib().ixo_cp9_fb2hmmbands(C.byref(self.s), _p(d, _u8p), *(_p(m) for m in mats), int(i0), int(j0),
                                   1.0 - tau, int(scan), _p(pn))
         return pn.reshape(6, self.h["M"] + 1)
+
+
+# ---- HMM -> CM band conversion (oracle/hmmband.c) ----------------------------
+
+class HbCm(C.Structure):
+    _fields_ = [("M", C.c_int), ("nodes", C.c_int), ("sttype", C.POINTER(C.c_int8)), ("ndtype", C.POINTER(C.c_int8)),
+                ("nodemap", _i32p), ("cfirst", _i32p), ("cnum", _i32p), ("local_begin", C.c_int),
+                ("local_end", C.c_int)]
+
+
+class HbCp9(C.Structure):
+    _fields_ = [("hmm_m", C.c_int), ("local_begin", C.c_int), ("local_end", C.c_int), ("el", C.c_int),
+                ("esc", _i32p), ("tdm", _i32p), ("tim", _i32p), ("el_from_ct", _i32p), ("el_from_off", _i32p),
+                ("el_from_idx", _i32p)]
+
+
+class OracleHb:
+    """The oracle's band conversion for a CM dict and a CP9 dict."""
+
+    def __init__(self, cm, h, cm_local=True, cp9_local=True):
+        L = lib()
+        L.ixo_hmm2ij_bands.argtypes = [C.POINTER(HbCm), C.POINTER(HbCp9), _i32p, C.c_int, C.c_int, C.c_int] + [_i32p] * 4
+        L.ixo_hb_ij2d.argtypes = [C.POINTER(HbCm)] + [_i32p] * 4 + [C.c_int, _i32p, _i32p, _i32p, C.c_int64]
+        L.ixo_hb_ij2d.restype = C.c_int64
+        L.ixo_hb_mx_mb.argtypes = [C.POINTER(HbCm)] + [_i32p] * 5 + [C.c_int, C.POINTER(C.c_int64)]
+        L.ixo_hb_mx_mb.restype = C.c_double
+        self.M = int(cm["M"])
+        k = self.keep = dict(sttype=np.ascontiguousarray(cm["sttype"], np.int8),
+                             ndtype=np.ascontiguousarray(cm["ndtype"], np.int8),
+                             nodemap=np.ascontiguousarray(cm["nodemap"], np.int32),
+                             cfirst=np.ascontiguousarray(cm["cfirst"], np.int32),
+                             cnum=np.ascontiguousarray(cm["cnum"], np.int32))
+        c = HbCm()
+        c.M, c.nodes = self.M, int(cm["nodes"])
+        c.sttype = k["sttype"].ctypes.data_as(C.POINTER(C.c_int8))
+        c.ndtype = k["ndtype"].ctypes.data_as(C.POINTER(C.c_int8))
+        for f in ("nodemap", "cfirst", "cnum"):
+            setattr(c, f, _p(k[f]))
+        c.local_begin = c.local_end = int(cm_local)
+        otsc = np.asarray(h["otsc"], np.int32).reshape(-1, 12)
+        for f, col in (("esc", 10), ("tdm", 2), ("tim", 1)):
+            k[f] = np.ascontiguousarray(otsc[:, col], np.int32)
+        for f in ("el_from_ct", "el_from_off", "el_from_idx"):
+            k[f] = np.ascontiguousarray(h[f], np.int32)
+        p = HbCp9()
+        p.hmm_m = int(h["M"])
+        p.local_begin = p.local_end = int(cp9_local)
+        p.el = int(bool(h["el"]))
+        for f in ("esc", "tdm", "tim", "el_from_ct", "el_from_off", "el_from_idx"):
+            setattr(p, f, _p(k[f]))
+        self.c, self.p = c, p
+
+    def hmm2ij(self, pn, i0, j0, search=True):
+        pn = np.ascontiguousarray(pn, np.int32).copy()
+        out = [np.zeros(self.M, np.int32) for _ in range(4)]
+        lib().ixo_hmm2ij_bands(C.byref(self.c), C.byref(self.p), _p(pn), int(i0), int(j0), int(search),
+                               *[_p(a) for a in out])
+        return (*out, pn)
+
+    def ij2d(self, imin, imax, jmin, jmax, trunc=False):
+        a = [np.ascontiguousarray(x, np.int32) for x in (imin, imax, jmin, jmax)]
+        hd_off = np.zeros(self.M + 1, np.int32)
+        n = lib().ixo_hb_ij2d(C.byref(self.c), *[_p(x) for x in a], int(trunc), _p(hd_off), None, None, 0)
+        hdmin, hdmax = np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1), np.int32)
+        lib().ixo_hb_ij2d(C.byref(self.c), *[_p(x) for x in a], int(trunc), _p(hd_off), _p(hdmin), _p(hdmax), n)
+        return dict(jmin=a[2], jmax=a[3], hd_off=hd_off, hdmin=hdmin[:n], hdmax=hdmax[:n])
+
+    def mx_mb(self, b, L):
+        cells = C.c_int64()
+        mb = lib().ixo_hb_mx_mb(C.byref(self.c), *[_p(np.ascontiguousarray(b[f], np.int32))
+                                                   for f in ("jmin", "jmax", "hd_off", "hdmin", "hdmax")], int(L),
+                                C.byref(cells))
+        return mb, cells.value
