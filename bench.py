@@ -1,11 +1,22 @@
-"""Benchmark of the MI355X filter cascade (F1 SSV -> F3 Forward -> F3b bias)
-on the genome-scale workload: a 4.6 Mb order-2 Markov genome at 50.8% GC
-(seed 3) with 100 planted homologs, searched on both strands in the
-reference's (100 kb chunk + maxW context) x strand tasks with a 300-column
-profile (maxW 400), thresholds of the 2-20 Mb tier (F1 0.35, F3 = F3b 0.005).
+"""Benchmark of the MI355X search path. Prints one JSON line.
 
-One step = one pass of ixg_f3_filter_tasks over every task of the genome,
-inputs resident in HBM. Prints one JSON line.
+Headline: the small plain cmsearch workload -- a 72-column tRNA-like
+cloverleaf CM (stems 7/4/5/5 bp, loops 7-9 nt) estimated from a seeded
+40-sequence alignment (seed 1), its CP9 and p7 filter HMM, searched against
+a 1 Mb iid uniform ACGU target (seed 2) with 20 CM-sampled homologs planted
+on random strands, both strands, thresholds of the 2-20 Mb tier (F1 0.35,
+F3 = F3b = F4 = F4b = F5 = 0.005, F6 1e-4). One step = one ixg_search_std
+call over the whole target with its sequence resident in HBM: the STD pass
+F1 SSV -> F3 Forward -> F3b bias -> F4/F4b glocal Forward -> glocal Backward
+and domain definition -> F5 -> CP9 bands (tau iteration) -> F6 HB CYK with
+the CYK envelope redefinition -> CP9 bands at the final tau -> F7 HB Inside
+with null3 and greedy hit reporting -> genome coordinates, E-values,
+overlap removal and sort. Not in the step: the truncated-hit passes and
+hit alignment.
+
+Secondary sections, each a rate beside the oracle's: the F1-F3b cascade on
+a 4.6 Mb genome with a 300-column profile, the HB / truncated HB / QDB
+scanners, CP9 banding, full p7 Forward, and the serving loop.
 """
 import argparse
 import json
@@ -20,7 +31,7 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-from infernox_amd import ixg, synth, synth_cm, synth_cp9  # noqa: E402
+from infernox_amd import ixg, models, synth, synth_cm, synth_cp9  # noqa: E402
 
 VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # 256 CUs x 4 SIMD16 x 2.4 GHz, one lane-op per cycle
 # HBM bytes per launch of each kernel on this workload, from rocprofv3 FETCH_SIZE +
@@ -38,6 +49,157 @@ def workload():
     d = synth.markov2_dsq(L, 3, gc=0.508)
     planted = synth.plant(d, L, p, 100, 3)
     return p, d, L, M, maxw, planted
+
+
+FP32_PEAK_TFLOPS = 157.3  # MI355X published vector FP32 (packed FMA: 2 flops x 2 lanes per lane-cycle)
+
+
+def search_workload():
+    """The small plain cmsearch workload: (p7, cm, cp9, cm_ev, dsq, L,
+    planted)."""
+    p7, cm, cp9, cmev = models.config1_models()
+    dsq, L, planted = models.target_with_homologs(cm, 1_000_000, 20, seed=2)
+    return p7, cm, cp9, cmev, dsq, L, planted
+
+
+def search_opts(p7, cm, cmev, L):
+    """Options of the 2-20 Mb tier (Z = 2L residues, both strands); E-values
+    from the calibrated tails, which are per 2W-residue window, so the
+    effective database size is Z / 2W."""
+    W = int(cm["W"])
+    f3 = ixg.F3Opts(f1=0.35, f1b=1.0, f2=1.0, f2b=1.0, f3=0.005, f3b=0.005, cmw=W, maxw=W,
+                    opts=ixg.DO_MSV | ixg.DO_FWD | ixg.DO_FWDBIAS)
+    env = ixg.env_opts(f4=0.005, f4b=0.005, f5=0.005, gfmu=p7["gfmu"], gflambda=p7["gflambda"])
+    return ixg.search_opts(f3, env, 1e-4, cmev["lc"], cmev["li"], 2.0 * L / (2 * W), W=W)
+
+
+def cpu_model():
+    """The host CPU's model name (lscpu, else /proc/cpuinfo)."""
+    import subprocess
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for ln in out.splitlines():
+            if ln.startswith("Model name:"):
+                return ln.split(":", 1)[1].strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    return ln.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_quota():
+    """CPUs this process may use: (nproc, affinity count, cgroup quota or None)."""
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    q = None
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as f:
+            a, b = f.read().split()
+            if a != "max":
+                q = float(a) / float(b)
+    except (OSError, ValueError):
+        pass
+    return os.cpu_count(), aff, q
+
+
+def search_cpu_baseline(wl, opts, threads, budget_s):
+    """The oracle's STD-pass search (C restatement of the same pipeline, not
+    infernox) over the whole workload on `threads` threads, repeated while
+    under budget_s: Mres/s."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_pipe_bind import OracleSearch
+    p7, cm, cp9, _, dsq, L, _ = wl
+    o = OracleSearch(p7, cm, cp9, int(cm["W"]))
+    runs, res, t0 = 0, 0, time.time()
+    hits = None
+    while runs == 0 or time.time() - t0 < budget_s / 2:
+        hits, a = o.search(dsq, L, opts, threads=threads)
+        res += a.nres
+        runs += 1
+    dt = time.time() - t0
+    nproc, aff, q = cpu_quota()
+    return dict(value=res / dt / 1e6, unit="Mres/s", cores=threads, kind="port", nproc=nproc, affinity=aff,
+                cgroup_cpus=q, cpu_model=cpu_model(),
+                sample=f"the whole 1 Mb x 2 strands workload, {runs} run(s), oracle ixo_search_std_mt (C "
+                       f"restatement of the STD-pass pipeline, not infernox), {threads} threads, {dt:.1f} s; "
+                       f"{len(hits)} hits"), hits
+
+
+def time_search(ctx, steps, warmup, wl):
+    """The headline step: ixg_search_std over the whole target. Returns the
+    line's fields and the last step's hits."""
+    p7, cm, cp9, cmev, dsq, L, planted = wl
+    W = int(cm["W"])
+    opts = search_opts(p7, cm, cmev, L)
+    g7 = ixg.P7(ctx, p7, W)
+    gcm = ixg.CM(ctx, cm)
+    g9 = ixg.CP9(ctx, cp9)
+    hb = ixg.HbMap(cm, cp9)
+    tgt = ixg.Target(ctx, dsq, L)
+    for _ in range(warmup):
+        ixg.search_std(ctx, g7, gcm, g9, hb, tgt, opts)
+    ctx.kernel_timings(reset=True)
+    st = dict.fromkeys(ixg.SEARCH_STAGES, 0.0)
+    cyk_cells = ins_cells = 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        hits, acct = ixg.search_std(ctx, g7, gcm, g9, hb, tgt, opts)
+        ms, cells = ixg.search_stats(ctx)
+        for k in st:
+            st[k] += ms[k] / steps
+        cyk_cells, ins_cells = cells
+    dt = (time.perf_counter() - t0) / steps
+    kt = ctx.kernel_timings(reset=True)
+    f3ms, f3cells = ctx.timings()
+    found = sum(any(min(h[0], h[1]) <= e and max(h[0], h[1]) >= s and h[2] == sd for h in hits)
+                for s, e, sd in planted)
+    M7 = int(p7["M"])
+    # algorithmic lane-ops per launch of the kernels with a per-unit figure
+    # (the reference loop bodies): SSV 4 u8 ops per (residue, node)
+    # (cm_pipeline.rs:598-601); Forward ~16 FP32 ops per (residue, node)
+    # (cm_pipeline.rs:1324-1382); glocal Forward / Backward ~20 per (residue,
+    # node) (p7_generic.rs:519-640, :939-1060, 3 nats FLogsums of ~4 ops + 8
+    # adds/max); HB CYK 8 per cell (the mean of the cloverleaf's 2-3
+    # children x add+max, emission, clamp); HB Inside 24 per cell (the same
+    # with FLogsum for max)
+    ops = {"ssv_sweep": 4 * f3cells[0], "fwd": 16 * f3cells[1],
+           "gfwd": 20 * acct.f.pos_past_fwdbias * M7, "gbwd": 20 * acct.e.pos_past_gfwdbias * M7,
+           "hb_scan": 8 * cyk_cells + 24 * ins_cells}
+    kern = {k: {"ms_per_step": v[0] / steps, "launches_per_step": v[1] / steps,
+                "ms_per_launch": v[0] / max(1, v[1])} for k, v in kt.items() if v[1] > 0}
+    dom = max(kern, key=lambda k: kern[k]["ms_per_step"])
+    roof = None
+    if dom in ops:
+        ms_l = kern[dom]["ms_per_launch"]
+        per_launch = ops[dom] / max(1.0, kern[dom]["launches_per_step"])
+        a = per_launch / (ms_l * 1e-3) / 1e12
+        roof = {"bound": "valu", "kernel": dom, "achieved": a, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": a / FP32_PEAK_TFLOPS, "ops_per_launch": per_launch, "ms_per_launch": ms_l,
+                "lane_op_ceiling": VALU_PEAK_TOPS, "frac_of_lane_op_ceiling": a / VALU_PEAK_TOPS,
+                "why_two_peaks": "the published 157.3 TFLOP/s counts packed FP32 FMA (2 flops x 2 lanes per "
+                                 "lane-cycle); the max-plus / log-sum DP ops are single unpacked non-FMA lane "
+                                 "ops, built without packed FP32, so 39.3 T lane-op/s is their ceiling",
+                "traffic": None}
+    out = {"value": acct.nres / dt / 1e6, "ms_per_step": dt * 1e3, "residues": int(acct.nres),
+           "stage_ms": st, "kernels": kern, "roofline": roof,
+           "cyk_gcups": cyk_cells / (st["f6_cyk"] * 1e-3) / 1e9 if st["f6_cyk"] > 0 else None,
+           "inside_gcups": ins_cells / (st["f7_inside"] * 1e-3) / 1e9 if st["f7_inside"] > 0 else None,
+           "hb_cells": {"cyk": int(cyk_cells), "inside": int(ins_cells)},
+           "survivors": {"f1": int(acct.f.n_past_msv), "f3b": int(acct.f.n_past_fwdbias),
+                         "f4b": int(acct.e.n_past_gfwdbias), "f5_envelopes": int(acct.e.n_past_edef),
+                         "f6": int(acct.n_past_cyk), "hits": int(acct.n_hits)},
+           "planted_found": f"{found} of {len(planted)}"}
+    tgt.free()
+    g9.free()
+    gcm.free()
+    g7.free()
+    hb.free()
+    return out, opts, hits
 
 
 def cm_workload(n_env=512, r=8, seed=3):
@@ -693,23 +855,95 @@ def time_serving(ctx, nstreams=4, qps_fracs=(0.5, 1.0, 2.0, 4.0)):
             "sweep": sweep, "server_stats": st}
 
 
+def time_cascade(ctx, steps, warmup, cpu_s, cpu_threads):
+    """The F1-F3b cascade alone on the 4.6 Mb genome with the 300-column
+    profile: one ixg_f3_filter_tasks call over every task per step."""
+    p, d, L, M, maxw, planted = workload()
+    tasks, nt = ixg.window_tasks(L, maxw)
+    nres = sum(tasks[t].end - tasks[t].start + 1 - tasks[t].ctx for t in range(nt))
+    opts = ixg.F3Opts(f1=0.35, f1b=1.0, f2=1.0, f2b=1.0, f3=0.005, f3b=0.005, cmw=M, maxw=maxw, opts=7)
+    g = ixg.P7(ctx, p, maxw)
+    tgt = ixg.Target(ctx, d, L)
+    for _ in range(warmup):
+        ixg.f3_filter_tasks(ctx, g, tgt, tasks, nt, opts)
+    stage_ms = np.zeros(5)
+    ctx.kernel_timings(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        wins, accts = ixg.f3_filter_tasks(ctx, g, tgt, tasks, nt, opts)
+        ms, cells = ctx.timings()
+        stage_ms += np.array(ms)
+    dt = (time.perf_counter() - t0) / steps
+    stage_ms /= steps
+    ms, cells = ctx.timings()
+    kt = ctx.kernel_timings(reset=True)
+    n_f1 = sum(accts[t].n_past_msv for t in range(nt))
+    n_f3b = sum(accts[t].n_past_fwdbias for t in range(nt))
+    tgt.free()
+    g.free()
+    # VALU roofline of the dominant kernel: ops per launch (the last step's
+    # cells x ops per cell, from the reference loop bodies) over its mean
+    # launch time (HIP events around each launch)
+    ops_per_launch = {"ssv_sweep": 4 * cells[0],  # SSV: 4 u8 ops per (residue, k), cm_pipeline.rs:598-601
+                      "fwd": 16 * cells[1],        # Forward: ~16 FP32 ops per (i, k), cm_pipeline.rs:1324-1382
+                      "bias": 10 * cells[2]}       # bias filter: ~10 FP32 ops per residue, cm_pipeline.rs:1750-1763
+    kern = {k: {"ms_per_launch": v[0] / v[1], "launches_per_step": v[1] / steps}
+            for k, v in kt.items() if v[1] > 0}
+    dom = max(ops_per_launch, key=lambda k: kt[k][0])
+    dom_ms = kt[dom][0] / max(1, kt[dom][1])
+    achieved = ops_per_launch[dom] / (dom_ms * 1e-3) / 1e12 if dom_ms > 0 else 0.0
+    out = {"value": nres / dt / 1e6, "unit": "Mres/s", "ms_per_step": dt * 1e3,
+           "workload": "4.6 Mb order-2 Markov 50.8% GC (seed 3), 100 planted homologs, both strands, 100 kb "
+                       "chunks + maxW 400 context, p7 M=300, F1 0.35 / F3 0.005 / F3b 0.005; one "
+                       "ixg_f3_filter_tasks call per step",
+           "tasks": nt, "residues": nres,
+           "stage_ms": {"ssv_sweep": stage_ms[0], "ssv_resolve": stage_ms[1], "f3_f3b_windows": stage_ms[2],
+                        "host_glue": stage_ms[3], "total_call": stage_ms[4]},
+           "survivors": {"f1_windows": int(n_f1), "f3b_windows": int(n_f3b), "merged": len(wins)},
+           "kernels": kern,
+           "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": FP32_PEAK_TFLOPS,
+                        "unit": "TFLOP/s", "frac": achieved / FP32_PEAK_TFLOPS,
+                        "lane_op_ceiling": VALU_PEAK_TOPS, "frac_of_lane_op_ceiling": achieved / VALU_PEAK_TOPS,
+                        "ms_per_launch": dom_ms, "ops_per_launch": ops_per_launch[dom],
+                        "traffic": TRAFFIC.get(dom)}}
+    if cpu_s:
+        out["cpu"] = cpu_baseline(p, d, L, M, maxw, tasks, nt, cpu_s, cpu_threads)
+    return out
+
+
+def default_threads():
+    """Host threads for the CPU baselines: the CPUs this process may run on,
+    capped by the cgroup CPU quota when one is set."""
+    nproc, aff, q = cpu_quota()
+    n = aff or nproc or 1
+    if q:
+        n = min(n, max(1, int(q)))
+    return n
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-seconds", type=float, default=20.0,
+                    help="oracle time budget of the headline cpu_baseline")
+    ap.add_argument("--cpu-threads", type=int, default=default_threads())
     ap.add_argument("--cpu-section-seconds", type=float, default=2.0,
-                    help="oracle time budget per scanner section (CPU rate beside each GPU rate)")
+                    help="oracle time budget per secondary section (CPU rate beside each GPU rate)")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-cm", action="store_true")
+    ap.add_argument("--no-cm", action="store_true", help="skip the secondary scanner sections")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--headline-only", action="store_true", help="only the headline search step")
     ap.add_argument("--serve-only", action="store_true", help="only run the serving-loop section")
     ap.add_argument("--multi-only", action="store_true", help="only run the several-CM HB launch section")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the 256 MiB device copy probe (HBM counter calibration) and print its rate")
     a = ap.parse_args()
+    if a.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (the search shards by "
+                                   "independent (chunk x strand) tasks, see DESIGN.md)"}))
+        sys.exit(2)
     if a.copy_probe:
         with ixg.Context(0) as ctx:
             nb = 256 << 20
@@ -724,95 +958,65 @@ def main():
         with ixg.Context(0) as ctx:
             print(json.dumps({"serving": time_serving(ctx)}))
         return
-    if a.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (the path shards by "
-                                   "independent tasks, see DESIGN.md)"}))
-        sys.exit(2)
-    p, d, L, M, maxw, planted = workload()
-    tasks, nt = ixg.window_tasks(L, maxw)
-    nres = sum(tasks[t].end - tasks[t].start + 1 - tasks[t].ctx for t in range(nt))
-    opts = ixg.F3Opts(f1=0.35, f1b=1.0, f2=1.0, f2b=1.0, f3=0.005, f3b=0.005, cmw=M, maxw=maxw, opts=7)
+    wl = search_workload()
+    secondary = not (a.headline_only or a.no_cm)
+    cpu = None if a.no_cpu else (a.cpu_section_seconds, a.cpu_threads)
     with ixg.Context(0) as ctx:
-        g = ixg.P7(ctx, p, maxw)
-        tgt = ixg.Target(ctx, d, L)
-        for _ in range(a.warmup):
-            ixg.f3_filter_tasks(ctx, g, tgt, tasks, nt, opts)
-        stage_ms = np.zeros(5)
-        ctx.kernel_timings(reset=True)
-        t0 = time.perf_counter()
-        for _ in range(a.steps):
-            wins, accts = ixg.f3_filter_tasks(ctx, g, tgt, tasks, nt, opts)
-            ms, cells = ctx.timings()
-            stage_ms += np.array(ms)
-        dt = (time.perf_counter() - t0) / a.steps
-        stage_ms /= a.steps
-        ms, cells = ctx.timings()
-        kt = ctx.kernel_timings(reset=True)
-        n_f1 = sum(accts[t].n_past_msv for t in range(nt))
-        n_f3b = sum(accts[t].n_past_fwdbias for t in range(nt))
-        tgt.free()
-        g.free()
-        cpu = None if a.no_cpu else (a.cpu_section_seconds, a.cpu_threads)
-        wl = cm_workload() if not a.no_cm else None
-        cm_line = time_cm(ctx, max(1, a.steps // 2), 1, cpu, wl) if not a.no_cm else None
-        cp9_line = time_cp9(ctx, 2, cpu, wl) if not a.no_cm else None
-        tr_line = time_trcm(ctx, max(1, a.steps // 2), 1, cpu) if not a.no_cm else None
-        qdb_line = time_qdb(ctx, 2, cpu) if not a.no_cm else None
-        pf_line = time_p7_forward(ctx, p, d, 2, cpu=cpu) if not a.no_cm else None
-        multi_line = None  # the several-CM queue launch is switched off (ixg_hb_scan_multi refuses to run)
-        serve_line = time_serving(ctx) if not a.no_serve else None
-    # dominant kernel with a per-unit op count, and its VALU roofline: ops per
-    # launch (the last step's cells x ops per cell, from the reference loop
-    # bodies) over the kernel's mean launch time (HIP events around each launch)
-    ops_per_launch = {"ssv_sweep": 4 * cells[0],  # SSV: 4 u8 ops per (residue, k), cm_pipeline.rs:598-601
-                      "fwd": 16 * cells[1],        # Forward: ~16 FP32 ops per (i, k), cm_pipeline.rs:1324-1382
-                      "bias": 10 * cells[2]}       # bias filter: ~10 FP32 ops per residue, cm_pipeline.rs:1750-1763
-    kern = {k: {"ms_per_launch": v[0] / v[1], "launches_per_step": v[1] / a.steps}
-            for k, v in kt.items() if v[1] > 0}
-    dom = max(ops_per_launch, key=lambda k: kt[k][0])
-    dom_ms = kt[dom][0] / max(1, kt[dom][1])
-    achieved = ops_per_launch[dom] / (dom_ms * 1e-3) / 1e12 if dom_ms > 0 else 0.0
+        sr, opts, gpu_hits = time_search(ctx, a.steps, a.warmup, wl)
+        print(json.dumps({"progress": "headline search done", "ms_per_step": sr["ms_per_step"]}),
+              file=sys.stderr, flush=True)
+        extra = {}
+        if secondary:
+            extra["filter_cascade_genome"] = time_cascade(ctx, a.steps, a.warmup, None if a.no_cpu else
+                                                          a.cpu_section_seconds * 4, a.cpu_threads)
+            hwl = cm_workload()
+            extra["hb_scanners"] = time_cm(ctx, max(1, a.steps // 2), 1, cpu, hwl)
+            extra["cp9_bands"] = time_cp9(ctx, 2, cpu, hwl)
+            extra["hb_truncated_scanners"] = time_trcm(ctx, max(1, a.steps // 2), 1, cpu)
+            extra["qdb_scanners"] = time_qdb(ctx, 2, cpu)
+            p, d, _, _, _, _ = workload()
+            extra["p7_forward_full"] = time_p7_forward(ctx, p, d, 2, cpu=cpu)
+            extra["hb_multi"] = time_multi(ctx)
+        if secondary and not a.no_serve:
+            extra["serving"] = time_serving(ctx)
+    p7, cm = wl[0], wl[1]
     line = {
-        "metric": "Mres/s searched (F1-F3b filter cascade, both strands)",
-        "value": nres / dt / 1e6,
+        "metric": "Mres/s searched + CYK GCUPS at 1 GPU; p50/p99 latency at fixed QPS",
+        "value": sr["value"],
         "unit": "Mres/s",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": dt * 1e3,
+        "ms_per_step": sr["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u8+f32",
+        "dtype": "u8+i32+f32",
         "data": "synthetic",
-        "config": {"workload": "genome-scale single profile: 4.6 Mb order-2 Markov 50.8% GC (seed 3), 100 planted "
-                               "homologs, both strands, 100 kb chunks + maxW 400 context, p7 M=300, "
-                               "F1 0.35 / F3 0.005 / F3b 0.005",
-                   "tasks": nt, "residues": nres},
-        "stage_ms": {"ssv_sweep": stage_ms[0], "ssv_resolve": stage_ms[1], "f3_f3b_windows": stage_ms[2],
-                     "host_glue": stage_ms[3], "total_call": stage_ms[4]},
-        "survivors": {"f1_windows": int(n_f1), "f3b_windows": int(n_f3b), "merged": len(wins)},
-        "kernels": kern,
-        "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": VALU_PEAK_TOPS,
-                     "unit": "TOPS", "frac": achieved / VALU_PEAK_TOPS, "ms_per_launch": dom_ms,
-                     "ops_per_launch": ops_per_launch[dom], "traffic": TRAFFIC.get(dom)},
+        "config": {"workload": "plain cmsearch, small: 72-column tRNA-like cloverleaf CM (stems 7/4/5/5, loops "
+                               "7-9; alignment 40 seqs, seed 1), 1 Mb iid ACGU target (seed 2) + 20 planted "
+                               "CM samples, both strands, tier thresholds F1 0.35, F3=F3b=F4=F4b=F5 0.005, F6 "
+                               "1e-4",
+                   "timed_step": "one ixg_search_std call: STD pass F1 SSV, F3 Forward, F3b bias, F4/F4b glocal "
+                                 "Forward, glocal Backward + domain definition, F5, CP9 bands (tau iteration), "
+                                 "F6 HB CYK + envelope redefinition, CP9 bands at final tau, F7 HB Inside + "
+                                 "null3 + greedy hits, coordinates, E-values, overlap removal, sort; not the "
+                                 "truncated-hit passes, not alignment",
+                   "cm_states": int(cm["M"]), "cm_W": int(cm["W"]), "p7_M": int(p7["M"]),
+                   "residues": sr["residues"]},
+        "cyk_gcups": sr["cyk_gcups"],
+        "inside_gcups": sr["inside_gcups"],
+        "stage_ms": sr["stage_ms"],
+        "survivors": sr["survivors"],
+        "planted_found": sr["planted_found"],
+        "kernels": sr["kernels"],
+        "roofline": sr["roofline"],
     }
-    if cm_line:
-        line["hb_scanners"] = cm_line
-    if tr_line:
-        line["hb_truncated_scanners"] = tr_line
-    if qdb_line:
-        line["qdb_scanners"] = qdb_line
-    if pf_line:
-        line["p7_forward_full"] = pf_line
-    if cp9_line:
-        line["cp9_bands"] = cp9_line
-    if multi_line:
-        line["hb_multi"] = multi_line
-    if serve_line:
-        line["serving"] = serve_line
+    line.update(extra)
     if not a.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(p, d, L, M, maxw, tasks, nt, a.cpu_seconds, a.cpu_threads)
+        cb, cpu_hits = search_cpu_baseline(wl, opts, a.cpu_threads, a.cpu_seconds)
+        cb["same_hits_as_gpu"] = [h[:3] for h in cpu_hits] == [h[:3] for h in gpu_hits]
+        line["cpu_baseline"] = cb
     print(json.dumps(line))
 
 

@@ -487,6 +487,53 @@ int ixg_ij2d_bands(const ixg_hbmap *m, const int32_t *imin, const int32_t *imax,
                    const int32_t *jmax, int do_trunc, int32_t *hd_off, int32_t *hdmin, int32_t *hdmax, int64_t cap,
                    int64_t *nrows);
 
+/* ---- the STD pass of a search, F1 through F7 ----------------------------------
+ * Replaces FaithfulSearcher::search (cm_search.rs:1300) for one resident
+ * sequence with the standard pass only (no truncated passes, as with
+ * --notrunc): the (chunk x strand) tasks of cm_search.rs:1774, the filter
+ * cascade (cm_pipeline.rs:1863), the envelope definition (cm_search.rs:2449),
+ * cm_std_stage (cm_search.rs:2707: CP9 bands iterated to the size limit
+ * (cp9.rs:4075), HB CYK filter with its P-value gate and envelope
+ * redefinition, HB Inside with greedy reporting, null3 and per-envelope
+ * overlap removal) and finalize (cm_search.rs:1915: genome coordinates,
+ * E-values, global overlap removal, the E-value sort, the reporting
+ * threshold). The per-hit alignment (mdl from/to) is not computed. */
+typedef struct {
+  ixg_f3opts f3;                   /* F1..F3b thresholds, cmw, maxw, IXG_DO_* */
+  ixg_envopts env;                 /* F4..F5b */
+  double f6, cyk_mu, cyk_lambda;   /* F6 P-value against the local CYK tail */
+  float cyk_env_cutoff;            /* CYK envelope redefinition cutoff (bits) */
+  float pli_t;                     /* final reporting threshold (bits) */
+  double fin_mu, fin_lambda, ez;   /* E-value = exp_surv(score, fin_mu, fin_lambda) * ez */
+  double e_report;                 /* hits with E > e_report are dropped */
+  double fcyk_tau, final_tau, maxtau;
+  float size_limit;                /* HB matrix limit in Mb (cm_search.rs:890-896) */
+  int32_t do_fcyk, do_fcykenv, do_null3, do_cyk, do_top, do_bot;
+} ixg_searchopts;
+typedef struct {
+  int64_t start, stop;  /* genome coordinates (start > stop on the bottom strand) */
+  float score, bias;
+  double evalue;
+  int32_t strand, pad;
+} ixg_searchhit;
+typedef struct {
+  uint64_t nres;        /* residues searched: both strands, overlap context excluded */
+  ixg_acct f;           /* F1..F3b, summed over tasks */
+  ixg_envacct e;        /* F4..F5b */
+  uint64_t n_past_cyk, pos_past_cyk, n_hits;
+} ixg_searchacct;
+int ixg_search_std(ixg_ctx *ctx, const ixg_p7 *p7, const ixg_cm *cm, const ixg_cp9 *cp9, const ixg_hbmap *map,
+                   const ixg_target *tgt, const ixg_searchopts *o, ixg_searchhit *out, int64_t cap, int64_t *n,
+                   ixg_searchacct *acct);
+/* Host wall time (ms) of each stage of the last ixg_search_std call on the
+ * context (IXG_SS_* index; a stage's time includes its launches, waits and
+ * host work) and the HB cells its CYK (cells[0]) and Inside (cells[1])
+ * launches computed. A measurement hook with no counterpart in the
+ * reference. */
+enum { IXG_SS_F1_F3B = 0, IXG_SS_F4_F5 = 1, IXG_SS_F6_BANDS = 2, IXG_SS_F6_CYK = 3, IXG_SS_F7_BANDS = 4,
+       IXG_SS_F7_INSIDE = 5, IXG_SS_FINALIZE = 6, IXG_SS_TOTAL = 7, IXG_SS_N = 8 };
+int ixg_search_stats(ixg_ctx *ctx, double *ms, int64_t *cells);
+
 /* Per-kernel device time, cumulative since the last reset: HIP events around
  * every launch of each kernel (IXG_K_* index), total milliseconds and launch
  * counts; reset != 0 clears them after reading. A measurement hook with no

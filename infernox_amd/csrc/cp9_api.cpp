@@ -352,3 +352,19 @@ int ixg_cp9_bands_batch(ixg_ctx *c, const ixg_cp9 *h, const ixg_target *tgt, con
 }
 
 }  // extern "C"
+
+namespace ixg {
+// Posterior HMM bands pn [n][6][M+1] of windows dsq[k][i0[k]..j0[k]] (device
+// sequences), the batched DP of cp9_seq2bands (cp9.rs:4053).
+int cp9_posterior_bands(ixg_ctx *c, const ixg_cp9 *h, const std::vector<const uint8_t *> &dsq,
+                        const std::vector<int32_t> &i0, const std::vector<int32_t> &j0, double tau, int mode,
+                        std::vector<int32_t> &pn) {
+  const size_t n = dsq.size();
+  std::vector<CWin> w(n);
+  for (size_t k = 0; k < n; k++) w[k] = CWin{dsq[k], i0[k], j0[k]};
+  pn.assign(n * 6 * (size_t)(h->M + 1), 0);
+  if (n == 0) return IXG_OK;
+  return cp9_run(c, h, w, tau, mode, nullptr, nullptr, pn.data(), nullptr);
+}
+int cp9_nodes(const ixg_cp9 *h) { return h->M; }
+}  // namespace ixg

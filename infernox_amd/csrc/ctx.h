@@ -119,6 +119,10 @@ struct ixg_ctx {
   // candidate root cells per residue seen by the last QDB scans (sizes the
   // next call's candidate list so that it rarely has to be rerun)
   double qdb_cand_rate = 0.125, trq_cand_rate = 0.125;
+  // the last ixg_search_std call: host wall time per stage (IXG_SS_*) and
+  // the HB cells its CYK and Inside launches computed
+  double search_ms[8] = {};
+  int64_t search_cells[2] = {};
 };
 
 namespace ixg {

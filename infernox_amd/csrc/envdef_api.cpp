@@ -37,6 +37,11 @@ int run_gfb(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, std::vector<GfbIt
     if (it.flags & 2) mtot += 3 * ((int64_t)it.len + 1) * (M + 1);
   }
   const int64_t n = (int64_t)items.size();
+  for (const auto &it : items)
+    if ((it.flags & 1) != (items[0].flags & 1)) {
+      g_err = "glocal Forward/Backward: items of one launch differ in local ends";
+      return IXG_EINVAL;
+    }
   sc.assign(n, 0.f);
   xall.assign(xtot, 0.f);
   if (mats) mats->assign(mtot, 0.f);
@@ -49,7 +54,8 @@ int run_gfb(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, std::vector<GfbIt
   RCHK(scratch(c, S_GFB_SC, (size_t)n * 4, (void **)&d_sc));
   HIPCHK(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(GfbItem), hipMemcpyHostToDevice, c->stream));
   HIPCHK(ktime_begin(c));
-  HIPCHK(launch_gfb(p7_gm_dev(p), d_seq, d_items, n, d_x, d_m, d_sc, backward, c->stream));
+  HIPCHK(launch_gfb(p7_gm_dev(p), d_seq, d_items, n, d_x, d_m, d_sc, backward, (items[0].flags & 1) != 0,
+                    c->stream));
   HIPCHK(ktime_end(c, backward ? IXG_K_GBWD : IXG_K_GFWD));
   HIPCHK(hipMemcpyAsync(sc.data(), d_sc, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(xall.data(), d_x, (size_t)xtot * 4, hipMemcpyDeviceToHost, c->stream));

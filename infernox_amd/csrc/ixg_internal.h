@@ -153,7 +153,8 @@ struct GmDev {
 };
 // One glocal Forward / Backward item: the dsq at seq + off (1-indexed, len
 // residues), its special-state scores [E N J C][LOOP MOVE], flags bit 0 =
-// local ends (esc = 0), bit 1 = keep the Forward matrix; the specials go to
+// local ends (esc = 0; the same for every item of a launch), bit 1 = keep
+// the Forward matrix; the specials go to
 // xout + xoff [(len+1)][E N J B C], the matrix to mout + moff
 // [(len+1)][M | I | D][M+1].
 struct GfbItem {
@@ -163,7 +164,7 @@ struct GfbItem {
   int64_t xoff, moff;
 };
 hipError_t launch_gfb(const GmDev &g, const uint8_t *seq, const GfbItem *items, int64_t n, float *xout, float *mout,
-                      float *sc, bool backward, hipStream_t s);
+                      float *sc, bool backward, bool local_end, hipStream_t s);
 
 // Kernel launchers (p7_kernels.hip).
 constexpr int SWEEP_SEG = 4096;  // rows per F1 sweep work item (multiple of 64)

@@ -63,7 +63,7 @@ EXPORTS = [
     "ixg_trqdb_scan", "ixg_trqdb_scan_batch", "ixg_cp9_create", "ixg_cp9_free", "ixg_cp9_bands",
     "ixg_cp9_bands_batch", "ixg_server_create", "ixg_server_submit", "ixg_server_wait", "ixg_server_result",
     "ixg_server_stats", "ixg_server_free", "ixg_hb_scan_multi", "ixg_fwd_scores_noalloc", "ixg_gfb_scores",
-    "ixg_env_def", "ixg_hbmap_create", "ixg_hbmap_free", "ixg_hmm2ij_bands", "ixg_ij2d_bands",
+    "ixg_env_def", "ixg_hbmap_create", "ixg_hbmap_free", "ixg_hmm2ij_bands", "ixg_ij2d_bands", "ixg_search_std", "ixg_search_stats",
 ]
 KERNELS = ["ssv_sweep", "ssv_resolve", "fwd", "bias", "vit", "msv", "hb_scan", "trhb_scan", "copy_probe",
            "qdb_scan", "fwd_full", "trqdb_scan", "cp9", "gfwd", "gbwd"]
@@ -1058,3 +1058,85 @@ class HbMap:
         check(_hb_lib().ixg_ij2d_bands(self.h, *[_i32(x) for x in a], int(trunc), _i32(hd_off), _i32(hdmin),
                                        _i32(hdmax), n.value, C.byref(n)), "ixg_ij2d_bands")
         return dict(jmin=a[2], jmax=a[3], hd_off=hd_off, hdmin=hdmin[:n.value], hdmax=hdmax[:n.value])
+
+
+# ---- the STD pass of a search, F1 through F7 ------------------------------------
+
+class SearchOpts(C.Structure):
+    _fields_ = [("f3", F3Opts), ("env", EnvOpts), ("f6", C.c_double), ("cyk_mu", C.c_double),
+                ("cyk_lambda", C.c_double), ("cyk_env_cutoff", C.c_float), ("pli_t", C.c_float),
+                ("fin_mu", C.c_double), ("fin_lambda", C.c_double), ("ez", C.c_double), ("e_report", C.c_double),
+                ("fcyk_tau", C.c_double), ("final_tau", C.c_double), ("maxtau", C.c_double),
+                ("size_limit", C.c_float), ("do_fcyk", C.c_int32), ("do_fcykenv", C.c_int32),
+                ("do_null3", C.c_int32), ("do_cyk", C.c_int32), ("do_top", C.c_int32), ("do_bot", C.c_int32)]
+
+
+class SearchHit(C.Structure):
+    _fields_ = [("start", C.c_int64), ("stop", C.c_int64), ("score", C.c_float), ("bias", C.c_float),
+                ("evalue", C.c_double), ("strand", C.c_int32), ("pad", C.c_int32)]
+
+
+class SearchAcct(C.Structure):
+    _fields_ = [("nres", C.c_uint64), ("f", Acct), ("e", EnvAcct), ("n_past_cyk", C.c_uint64),
+                ("pos_past_cyk", C.c_uint64), ("n_hits", C.c_uint64)]
+
+
+def search_opts(f3, env, f6, cyk, fin, ez, e_report=10.0, pli_t=None, cykenvx=10, fcyk_tau=1e-4, final_tau=5e-6,
+                maxtau=0.05, W=None, do_fcyk=True, do_fcykenv=True, do_null3=True, do_cyk=False, top=True, bot=True):
+    """Options of ixg_search_std. cyk / fin: (mu, lambda) exponential tails of
+    the local CYK filter and the final round; the CYK envelope cutoff and the
+    reporting threshold follow them as cm_search.rs:1656-1721 derives them
+    (pli_t from e_report and ez unless given); the size limit from W
+    (cm_search.rs:890-896)."""
+    o = SearchOpts()
+    o.f3, o.env = f3, env
+    o.f6, o.cyk_mu, o.cyk_lambda = f6, cyk[0], cyk[1]
+    f6env = min(1.0, float(np.float32(f6) * np.float32(cykenvx)))
+    o.cyk_env_cutoff = float(np.float32(cyk[0] + np.log(f6env) / (-1.0 * cyk[1])))
+    o.fin_mu, o.fin_lambda, o.ez, o.e_report = fin[0], fin[1], ez, e_report
+    o.pli_t = float(np.float32(fin[0] + np.log(e_report / ez) / (-1.0 * fin[1]))) if pli_t is None else pli_t
+    o.fcyk_tau, o.final_tau, o.maxtau = fcyk_tau, final_tau, maxtau
+    w = float(W if W is not None else 1000)
+    o.size_limit = float(min(1024.0, max(256.0, (w - 1000.0) / (3000.0 - 1000.0) * (1024.0 - 256.0) + 256.0)))
+    o.do_fcyk, o.do_fcykenv, o.do_null3, o.do_cyk = int(do_fcyk), int(do_fcykenv), int(do_null3), int(do_cyk)
+    o.do_top, o.do_bot = int(top), int(bot)
+    return o
+
+
+_search_ready = False
+
+
+def search_std(ctx, p7, cm, cp9, hbmap, tgt, opts):
+    """ixg_search_std: ([(start, stop, strand, score, bias, evalue)], SearchAcct)."""
+    global _search_ready
+    L = lib()
+    if not _search_ready:
+        vp = C.c_void_p
+        L.ixg_search_std.argtypes = [vp, vp, vp, vp, vp, vp, C.POINTER(SearchOpts), C.POINTER(SearchHit), C.c_int64,
+                                     C.POINTER(C.c_int64), C.POINTER(SearchAcct)]
+        _search_ready = True
+    cap = 256
+    while True:
+        out = (SearchHit * cap)()
+        n = C.c_int64()
+        acct = SearchAcct()
+        rc = L.ixg_search_std(ctx.h, p7.h, cm.h, cp9.h, hbmap.h, tgt.h, C.byref(opts), out, cap, C.byref(n),
+                              C.byref(acct))
+        if rc == IXG_ERANGE and n.value > cap:
+            cap = n.value
+            continue
+        check(rc, "ixg_search_std")
+        return [(out[i].start, out[i].stop, out[i].strand, out[i].score, out[i].bias, out[i].evalue)
+                for i in range(n.value)], acct
+
+
+SEARCH_STAGES = ("f1_f3b", "f4_f5", "f6_bands", "f6_cyk", "f7_bands", "f7_inside", "finalize", "total")
+
+
+def search_stats(ctx):
+    """Stage wall times (ms, by SEARCH_STAGES name) and the CYK / Inside HB
+    cell counts of the context's last search_std call."""
+    ms = (C.c_double * len(SEARCH_STAGES))()
+    cells = (C.c_int64 * 2)()
+    check(lib().ixg_search_stats(ctx.h, ms, cells), "ixg_search_stats")
+    return dict(zip(SEARCH_STAGES, list(ms))), (cells[0], cells[1])

@@ -1,0 +1,372 @@
+/*
+ * pipeline.c -- CPU oracle (test infrastructure only) for the STD pass of a
+ * search over one sequence, F1 through F7: FaithfulSearcher::search
+ * (cm_search.rs:1300) with window_tasks (:1774), run_window_task (:1813),
+ * trunc_pipeline_one_strand's STD pass (:2214), trunc_env_def (:2449),
+ * cm_std_stage (:2707) with cp9_iterate_seq2bands (cp9.rs:4075) and
+ * remove_overlaps_greedy (cp9.rs:6544), and finalize (cm_search.rs:1915).
+ * Each task is searched on its own copy of the chunk (reverse-complemented
+ * for the bottom strand), as the reference does. Not linked into the product.
+ */
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+#include "ixo_pipe.h"
+
+#define CM_MAX_RESIDUE_COUNT 100000
+
+typedef struct {
+  int64_t start, stop;
+  float score, bias;
+  double evalue;
+  int strand, removed;
+} phit;
+
+/* cp9.rs:4053 cp9_seq2bands + :4075 the tau iteration; fills the bands of
+ * dsq[i0..j0]; returns the final size in Mb. The arrays are allocated here. */
+static double iterate_bands(const ixo_pipe *P, const uint8_t *dsq, int i0, int j0, double tau, double maxtau,
+                            float limit, int32_t **jmin, int32_t **jmax, int32_t **hd_off, int32_t **hdmin,
+                            int32_t **hdmax) {
+  const int H = P->cp9->M, M = P->hbcm->M, L = j0 - i0 + 1;
+  const size_t cells = (size_t)(L + 1) * (H + 1);
+  int32_t *mx[8];
+  for (int q = 0; q < 8; q++) mx[q] = (int32_t *)malloc(sizeof(int32_t) * cells);
+  int32_t *sca = (int32_t *)malloc(sizeof(int32_t) * (size_t)(L + 1));
+  int32_t *pn = (int32_t *)malloc(sizeof(int32_t) * 6 * (size_t)(H + 1));
+  int32_t *imin = (int32_t *)malloc(sizeof(int32_t) * M), *imax = (int32_t *)malloc(sizeof(int32_t) * M);
+  *jmin = (int32_t *)malloc(sizeof(int32_t) * M);
+  *jmax = (int32_t *)malloc(sizeof(int32_t) * M);
+  *hd_off = (int32_t *)malloc(sizeof(int32_t) * (M + 1));
+  *hdmin = *hdmax = NULL;
+  int at_limit = 0;
+  double mb = 0.0;
+  for (;;) {
+    int64_t bp;
+    ixo_cp9_forward(P->cp9, dsq, i0, j0, 1, 0, mx[0], mx[1], mx[2], mx[3], sca, &bp);
+    ixo_cp9_backward(P->cp9, dsq, i0, j0, 1, 0, mx[4], mx[5], mx[6], mx[7], sca, &bp);
+    ixo_cp9_fb2hmmbands(P->cp9, dsq, mx[0], mx[1], mx[2], mx[4], mx[5], mx[6], i0, j0, 1.0 - tau, 1, pn);
+    ixo_hmm2ij_bands(P->hbcm, P->hbcp9, pn, i0, j0, 1, imin, imax, *jmin, *jmax);
+    const int64_t rows = ixo_hb_ij2d(P->hbcm, imin, imax, *jmin, *jmax, 0, *hd_off, NULL, NULL, 0);
+    free(*hdmin);
+    free(*hdmax);
+    *hdmin = (int32_t *)malloc(sizeof(int32_t) * (size_t)(rows + 1));
+    *hdmax = (int32_t *)malloc(sizeof(int32_t) * (size_t)(rows + 1));
+    ixo_hb_ij2d(P->hbcm, imin, imax, *jmin, *jmax, 0, *hd_off, *hdmin, *hdmax, rows);
+    mb = ixo_hb_mx_mb(P->hbcm, *jmin, *jmax, *hd_off, *hdmin, *hdmax, L, NULL);
+    if (mb < limit || at_limit) break;
+    tau *= 2.0;
+    if (tau >= maxtau) {
+      tau = maxtau;
+      at_limit = 1;
+    }
+  }
+  for (int q = 0; q < 8; q++) free(mx[q]);
+  free(sca);
+  free(pn);
+  free(imin);
+  free(imax);
+  return mb;
+}
+
+static void free_bands(int32_t *a, int32_t *b, int32_t *c, int32_t *d, int32_t *e) {
+  free(a);
+  free(b);
+  free(c);
+  free(d);
+  free(e);
+}
+
+/* cp9.rs:6544 remove_overlaps_greedy: score descending, then start ascending;
+ * the reference's sort is stable, so the reporting order breaks ties */
+typedef struct { ixo_hit h; int64_t at; } ohit;
+static int hit_cmp(const void *x, const void *y) {
+  const ohit *a = (const ohit *)x, *b = (const ohit *)y;
+  if (a->h.score != b->h.score) return a->h.score > b->h.score ? -1 : 1;
+  if (a->h.i != b->h.i) return (a->h.i > b->h.i) - (a->h.i < b->h.i);
+  return (a->at > b->at) - (a->at < b->at);
+}
+
+static void push(phit **v, int64_t *n, int64_t *cap, phit h) {
+  if (*n == *cap) {
+    *cap = *cap ? 2 * *cap : 64;
+    *v = (phit *)realloc(*v, sizeof(phit) * (size_t)*cap);
+  }
+  (*v)[(*n)++] = h;
+}
+
+/* the STD pass of one (chunk x strand) task on its chunk dsq[1..wl] */
+static void one_task(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *dsq, int64_t wl, int64_t wgs,
+                     int64_t wge, int strand, ixo_pipeacct *A, phit **hits, int64_t *nh, int64_t *hcap) {
+  const int64_t wcap = 1 << 16;
+  ixo_window *wins = (ixo_window *)malloc(sizeof(ixo_window) * wcap);
+  ixo_acct fa;
+  memset(&fa, 0, sizeof(fa));
+  int64_t nw = ixo_f3_filter_sequence(P->msv, P->vit, P->fwd, P->bias, P->p7, dsq, wl, &o->f3, wins, wcap, &fa);
+  if (nw > wcap) nw = wcap;
+  {
+    const uint64_t *s = (const uint64_t *)&fa;
+    uint64_t *d = (uint64_t *)&A->f;
+    for (size_t q = 0; q < sizeof(ixo_acct) / 8; q++) d[q] += s[q];
+  }
+  if (nw == 0) {
+    free(wins);
+    return;
+  }
+  const int64_t ecap = 4096;
+  int32_t *es = (int32_t *)malloc(sizeof(int32_t) * ecap), *ee = (int32_t *)malloc(sizeof(int32_t) * ecap);
+  ixo_envacct ea;
+  int64_t ne = ixo_env_def_std(P->gm, P->bias, dsq, wl, wins, nw, &o->env, es, ee, ecap, &ea);
+  if (ne > ecap) ne = ecap;
+  {
+    const uint64_t *s = (const uint64_t *)&ea;
+    uint64_t *d = (uint64_t *)&A->e;
+    for (size_t q = 0; q < sizeof(ixo_envacct) / 8; q++) d[q] += s[q];
+  }
+  /* F6 */
+  int32_t *ss = (int32_t *)malloc(sizeof(int32_t) * (size_t)(ne + 1)), *se = (int32_t *)malloc(sizeof(int32_t) * (size_t)(ne + 1));
+  int64_t ns = 0;
+  for (int64_t k = 0; k < ne; k++) {
+    int e0 = es[k], e1 = ee[k];
+    if (!o->do_fcyk) {
+      ss[ns] = e0, se[ns] = e1, ns++;
+      continue;
+    }
+    int32_t *jmin, *jmax, *hd_off, *hdmin, *hdmax;
+    const double mb = iterate_bands(P, dsq, e0, e1, o->fcyk_tau, o->maxtau, o->size_limit, &jmin, &jmax, &hd_off,
+                                    &hdmin, &hdmax);
+    if (mb > o->size_limit) {
+      free_bands(jmin, jmax, hd_off, hdmin, hdmax);
+      continue;
+    }
+    ixo_bands b = {jmin, jmax, hd_off, hdmin, hdmax};
+    float vsc;
+    int64_t ei, ej;
+    ixo_hb_scan(P->cm, P->sc, &b, dsq, e0, e1, 0, o->cyk_env_cutoff, &vsc, &ei, &ej, NULL, NULL);
+    free_bands(jmin, jmax, hd_off, hdmin, hdmax);
+    if (ixo_exp_surv((double)vsc, o->cyk_mu, o->cyk_lambda) > o->f6) continue;
+    if (o->do_fcykenv && ei != -1 && ej != -1) e0 = (int)ei, e1 = (int)ej;
+    A->n_past_cyk++;
+    A->pos_past_cyk += (uint64_t)(e1 - e0 + 1);
+    ss[ns] = e0, se[ns] = e1, ns++;
+  }
+  /* F7 */
+  for (int64_t k = 0; k < ns; k++) {
+    const int e0 = ss[k], e1 = se[k];
+    int32_t *jmin, *jmax, *hd_off, *hdmin, *hdmax;
+    const double mb = iterate_bands(P, dsq, e0, e1, o->final_tau, o->maxtau, o->size_limit, &jmin, &jmax, &hd_off,
+                                    &hdmin, &hdmax);
+    if (mb > o->size_limit) {
+      free_bands(jmin, jmax, hd_off, hdmin, hdmax);
+      continue;
+    }
+    ixo_bands b = {jmin, jmax, hd_off, hdmin, hdmax};
+    const int64_t nc = ixo_hb_ncells(&b, P->cm->M);
+    float *root = (float *)malloc(sizeof(float) * (size_t)(nc + 1));
+    float vsc;
+    int64_t ei, ej;
+    ixo_hb_scan(P->cm, P->sc, &b, dsq, e0, e1, o->do_cyk ? 0 : 1, 0.0f, &vsc, &ei, &ej, root, NULL);
+    const int64_t rcap = 4096;
+    ixo_hit *raw = (ixo_hit *)malloc(sizeof(ixo_hit) * rcap);
+    int64_t nr = ixo_hb_report_hits(P->cm, &b, dsq, e0, e1, root, o->pli_t, o->do_null3, raw, rcap);
+    if (nr > rcap) nr = rcap;
+    ohit *srt = (ohit *)malloc(sizeof(ohit) * (size_t)(nr + 1));
+    for (int64_t a = 0; a < nr; a++) srt[a].h = raw[a], srt[a].at = a;
+    qsort(srt, (size_t)nr, sizeof(ohit), hit_cmp);
+    for (int64_t a = 0; a < nr; a++) raw[a] = srt[a].h;
+    free(srt);
+    char *gone = (char *)calloc((size_t)(nr + 1), 1);
+    for (int64_t a = 0; a < nr; a++) {
+      if (gone[a]) continue;
+      for (int64_t c = a + 1; c < nr; c++)
+        if (!gone[c] && !(raw[c].j < raw[a].i) && !(raw[a].j < raw[c].i)) gone[c] = 1;
+    }
+    for (int64_t a = 0; a < nr; a++) {
+      if (gone[a]) continue;
+      phit h;
+      memset(&h, 0, sizeof(h));
+      if (strand) h.start = wge - raw[a].i + 1, h.stop = wge - raw[a].j + 1;
+      else h.start = wgs + raw[a].i - 1, h.stop = wgs + raw[a].j - 1;
+      h.score = raw[a].score;
+      h.bias = raw[a].bias;
+      h.evalue = ixo_exp_surv((double)h.score, o->fin_mu, o->fin_lambda) * o->ez;
+      h.strand = strand;
+      push(hits, nh, hcap, h);
+    }
+    free(gone);
+    free(raw);
+    free(root);
+    free_bands(jmin, jmax, hd_off, hdmin, hdmax);
+  }
+  free(ss);
+  free(se);
+  free(es);
+  free(ee);
+  free(wins);
+}
+
+static const phit *g_h;
+static int ov_cmp(const void *x, const void *y) {
+  const phit *a = &g_h[*(const int64_t *)x], *b = &g_h[*(const int64_t *)y];
+  if (a->strand != b->strand) return a->strand - b->strand;
+  if (a->score != b->score) return a->score > b->score ? -1 : 1;
+  if (a->start != b->start) return a->start < b->start ? -1 : 1;
+  return (a->stop > b->stop) - (a->stop < b->stop);
+}
+static int fin_cmp(const void *x, const void *y) {
+  const phit *a = (const phit *)x, *b = (const phit *)y;
+  if (a->evalue != b->evalue) return a->evalue < b->evalue ? -1 : 1;
+  if (a->score != b->score) return a->score > b->score ? -1 : 1;
+  if (a->start != b->start) return a->start < b->start ? -1 : 1;
+  return (a->stop > b->stop) - (a->stop < b->stop);
+}
+
+typedef struct {
+  int64_t wgs, wge, wl, ctx;
+  int strand;
+} otask;
+
+/* cm_search.rs:1774-1808 window_tasks for one sequence */
+static int64_t make_tasks(const ixo_pipeopts *o, int64_t L, otask *out) {
+  const int64_t maxw = o->f3.maxw;
+  int64_t n = 0, new_start = 1;
+  int first = 1;
+  while (new_start <= L) {
+    const int64_t ctx = first ? 0 : (maxw < new_start - 1 ? maxw : new_start - 1);
+    const int64_t wgs = new_start - ctx;
+    const int64_t cnt = CM_MAX_RESIDUE_COUNT < L - new_start + 1 ? CM_MAX_RESIDUE_COUNT : L - new_start + 1;
+    const int64_t wl = ctx + cnt, wge = wgs + wl - 1;
+    for (int strand = 0; strand < 2; strand++) {
+      if ((strand == 0 && !o->do_top) || (strand == 1 && !o->do_bot)) continue;
+      if (out) out[n] = (otask){wgs, wge, wl, ctx, strand};
+      n++;
+    }
+    new_start = wge + 1;
+    first = 0;
+  }
+  return n;
+}
+
+static void run_task(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *seq, const otask *t, uint8_t *dsq,
+                     ixo_pipeacct *A, phit **hits, int64_t *nh, int64_t *hcap) {
+  dsq[0] = dsq[t->wl + 1] = 255;
+  for (int64_t k = 1; k <= t->wl; k++) {
+    const uint8_t x = t->strand ? seq[t->wge - (k - 1)] : seq[t->wgs + k - 1];
+    dsq[k] = t->strand ? (x < 4 ? (uint8_t)(3 - x) : x) : x;
+  }
+  A->nres += (uint64_t)(t->wl - t->ctx);
+  one_task(P, o, dsq, t->wl, t->wgs, t->wge, t->strand, A, hits, nh, hcap);
+}
+
+/* finalize (cm_search.rs:1915): global overlap removal, E-value sort,
+ * reporting threshold; returns the number reported */
+static int64_t finalize(const ixo_pipeopts *o, phit *hits, int64_t nh, ixo_pipehit *out, int64_t cap) {
+  int64_t *ord = (int64_t *)malloc(sizeof(int64_t) * (size_t)(nh + 1));
+  for (int64_t q = 0; q < nh; q++) ord[q] = q;
+  g_h = hits;
+  qsort(ord, (size_t)nh, sizeof(int64_t), ov_cmp);
+  for (int64_t a = 0; a < nh; a++) {
+    phit *ha = &hits[ord[a]];
+    if (ha->removed) continue;
+    for (int64_t b = a + 1; b < nh; b++) {
+      phit *hb = &hits[ord[b]];
+      if (hb->strand != ha->strand) break;
+      if (hb->removed) continue;
+      const int ov = ha->strand == 0 ? (!(hb->stop < ha->start) && !(ha->stop < hb->start))
+                                     : (!(hb->start < ha->stop) && !(ha->start < hb->stop));
+      if (ov) hb->removed = 1;
+    }
+  }
+  free(ord);
+  int64_t nk = 0;
+  for (int64_t q = 0; q < nh; q++)
+    if (!hits[q].removed) hits[nk++] = hits[q];
+  qsort(hits, (size_t)nk, sizeof(phit), fin_cmp);
+  int64_t n = 0;
+  for (int64_t q = 0; q < nk; q++) {
+    if (!(hits[q].evalue <= o->e_report)) continue;
+    if (n < cap) {
+      out[n].start = hits[q].start;
+      out[n].stop = hits[q].stop;
+      out[n].score = hits[q].score;
+      out[n].bias = hits[q].bias;
+      out[n].evalue = hits[q].evalue;
+      out[n].strand = hits[q].strand;
+    }
+    n++;
+  }
+  return n;
+}
+
+int64_t ixo_search_std(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *seq, int64_t L, ixo_pipehit *out,
+                       int64_t cap, ixo_pipeacct *A) {
+  return ixo_search_std_mt(P, o, seq, L, out, cap, A, 1);
+}
+
+/* the tasks run on nthreads threads (the reference's par_iter over tasks,
+ * cm_search.rs:1312); per-thread hit lists and counters are merged before
+ * finalize, so the result does not depend on the thread count */
+typedef struct {
+  const ixo_pipe *P;
+  const ixo_pipeopts *o;
+  const uint8_t *seq;
+  const otask *tasks;
+  int64_t nt;
+  int64_t *next;
+  pthread_mutex_t *mu;
+  ixo_pipeacct A;
+  phit *hits;
+  int64_t nh, hcap;
+} worker_t;
+
+static void *worker(void *arg) {
+  worker_t *w = (worker_t *)arg;
+  uint8_t *dsq = (uint8_t *)malloc((size_t)(CM_MAX_RESIDUE_COUNT + w->o->f3.maxw + 2));
+  for (;;) {
+    pthread_mutex_lock(w->mu);
+    const int64_t t = (*w->next)++;
+    pthread_mutex_unlock(w->mu);
+    if (t >= w->nt) break;
+    run_task(w->P, w->o, w->seq, &w->tasks[t], dsq, &w->A, &w->hits, &w->nh, &w->hcap);
+  }
+  free(dsq);
+  return NULL;
+}
+
+int64_t ixo_search_std_mt(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *seq, int64_t L,
+                          ixo_pipehit *out, int64_t cap, ixo_pipeacct *A, int nthreads) {
+  memset(A, 0, sizeof(*A));
+  (void)ixo_flogsum(0.f, 0.f); /* the lazily built tables, before any thread reads them */
+  (void)ixo_ilogsum(0, 0);
+  (void)ixo_p7_flogsum(0.f, 0.f);
+  const int64_t nt = make_tasks(o, L, NULL);
+  otask *tasks = (otask *)malloc(sizeof(otask) * (size_t)(nt + 1));
+  make_tasks(o, L, tasks);
+  if (nthreads < 1) nthreads = 1;
+  worker_t *w = (worker_t *)calloc((size_t)nthreads, sizeof(worker_t));
+  pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
+  pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
+  int64_t next = 0;
+  for (int q = 0; q < nthreads; q++) {
+    w[q].P = P, w[q].o = o, w[q].seq = seq, w[q].tasks = tasks, w[q].nt = nt, w[q].next = &next, w[q].mu = &mu;
+    if (q) pthread_create(&th[q], NULL, worker, &w[q]);
+  }
+  worker(&w[0]);
+  for (int q = 1; q < nthreads; q++) pthread_join(th[q], NULL);
+  phit *hits = NULL;
+  int64_t nh = 0, hcap = 0;
+  for (int q = 0; q < nthreads; q++) {
+    const uint64_t *s = (const uint64_t *)&w[q].A;
+    uint64_t *d = (uint64_t *)A;
+    for (size_t k = 0; k < sizeof(ixo_pipeacct) / 8; k++) d[k] += s[k];
+    for (int64_t k = 0; k < w[q].nh; k++) push(&hits, &nh, &hcap, w[q].hits[k]);
+    free(w[q].hits);
+  }
+  const int64_t n = finalize(o, hits, nh, out, cap);
+  A->n_hits = (uint64_t)n;
+  free(hits);
+  free(w);
+  free(th);
+  free(tasks);
+  return n;
+}

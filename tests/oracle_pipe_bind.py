@@ -1,0 +1,58 @@
+"""ctypes binding of the oracle's STD-pass search (oracle/pipeline.c).
+Test infrastructure only: the checker, never the thing measured."""
+import ctypes as C
+
+import numpy as np
+
+import oracle_bind as OB
+import oracle_cm_bind as OC
+import oracle_cp9_bind as O9
+from oracle_env_bind import Gm, OracleEnv
+from infernox_amd import ixg
+
+
+class Pipe(C.Structure):
+    _fields_ = [("msv", C.POINTER(OB.Msv)), ("vit", C.POINTER(OB.Vit)), ("fwd", C.POINTER(OB.Fwd)),
+                ("bias", C.POINTER(OB.Bias)), ("p7", C.POINTER(OB.P7)), ("gm", C.POINTER(Gm)),
+                ("cm", C.POINTER(OC.Cm)), ("sc", C.POINTER(OC.CmSc)), ("cp9", C.POINTER(O9.Cp9)),
+                ("hbcm", C.POINTER(O9.HbCm)), ("hbcp9", C.POINTER(O9.HbCp9))]
+
+
+class OracleSearch:
+    """The oracle's STD-pass search for a p7 HMM dict, a CM dict and a CP9
+    dict. The options are ixg.SearchOpts (the oracle's struct has the same
+    layout)."""
+
+    def __init__(self, p7, cm, cp9, max_length):
+        self.env = OracleEnv(p7, max_length)
+        self.base = self.env.base
+        self.ocm = OC.OracleCM(cm)
+        self.o9 = O9.OracleCP9(cp9)
+        self.hb = O9.OracleHb(cm, cp9)
+        P = Pipe()
+        b = self.base
+        P.msv, P.vit, P.fwd, P.bias, P.p7 = (C.pointer(b.smsv), C.pointer(b.svit), C.pointer(b.sfwd),
+                                             C.pointer(b.sbias), C.pointer(b.p7))
+        P.gm = C.pointer(self.env.gm)
+        P.cm, P.sc = C.pointer(self.ocm.s), C.pointer(self.ocm.sc)
+        P.cp9 = C.pointer(self.o9.s)
+        P.hbcm, P.hbcp9 = C.pointer(self.hb.c), C.pointer(self.hb.p)
+        self.P = P
+        L = OB.lib()
+        L.ixo_search_std.argtypes = [C.POINTER(Pipe), C.POINTER(ixg.SearchOpts), C.POINTER(C.c_uint8), C.c_int64,
+                                     C.POINTER(ixg.SearchHit), C.c_int64, C.POINTER(ixg.SearchAcct)]
+        L.ixo_search_std.restype = C.c_int64
+        L.ixo_search_std_mt.argtypes = L.ixo_search_std.argtypes + [C.c_int]
+        L.ixo_search_std_mt.restype = C.c_int64
+
+    def search(self, dsq, L, opts, threads=1):
+        """Hits (start, stop, strand, score, bias, evalue) and the counters;
+        threads > 1 runs the tasks on that many threads (same result)."""
+        d = np.ascontiguousarray(dsq, np.uint8)
+        cap = 4096
+        out = (ixg.SearchHit * cap)()
+        acct = ixg.SearchAcct()
+        n = OB.lib().ixo_search_std_mt(C.byref(self.P), C.byref(opts), d.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                       int(L), out, cap, C.byref(acct), int(threads))
+        return [(out[i].start, out[i].stop, out[i].strand, out[i].score, out[i].bias, out[i].evalue)
+                for i in range(min(n, cap))], acct
