@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <queue>
 #include "cm_internal.h"
 #include "ctx.h"
 #include "cm_handle.h"
@@ -164,11 +165,135 @@ void report_hits(const ixg_cm &cm, const ixg_bands &b, const uint8_t *dsq, int32
 }
 
 struct EnvPlan {
-  int64_t ncells;
+  int64_t ncells;  // band cells (what a scan computes and reports)
   int32_t nrows;
   int64_t nroot;
   std::vector<int32_t> roff;
+  // standard scans: the deck's state rectangles, root row offsets, lane
+  // slots, deck floats and wavefront steps
+  std::vector<int32_t> rect, rootoff, slots;
+  int64_t deck = 0;
+  int32_t nsteps = 0;
 };
+
+// The standard scan's deck and schedule for one envelope (see cm_kernels.hip):
+// per state the smaller of its (j, i) and (j, d) bounding rectangles, the
+// number of wavefront steps (the largest d + 1 plus the root's level), and
+// the lane slots: one per state with cells, then the remaining lanes dealt
+// one at a time to the state with the most cells per step per lane (its
+// band cells over its d range; a bifurcation's cell weighted by its right
+// child's mean row width over the 8 split terms fetched per batch).
+int rect_plan(const CmHost &h, const ixg_bands &b, EnvPlan &p, const char **err) {
+  const int M = h.M;
+  p.rect.assign((size_t)8 * M, 0);
+  std::vector<double> width(M, 0.0), perstep(M, 0.0);
+  std::vector<int> nxs(M, 0);
+  int64_t base = 0;
+  int dmax = -1, maxlvl = 0;
+  for (int v = 0; v < M; v++) {
+    const int nj = b.jmax[v] >= b.jmin[v] ? b.jmax[v] - b.jmin[v] + 1 : 0;
+    int ilo = INT32_MAX, ihi = INT32_MIN, dlo = INT32_MAX, dhi = INT32_MIN;
+    int64_t tc = 0;
+    for (int jp = 0; jp < nj; jp++) {
+      const int r = b.hd_off[v] + jp, j = b.jmin[v] + jp, lo = b.hdmin[r], hi = b.hdmax[r];
+      if (hi < lo) continue;
+      ilo = std::min(ilo, j - hi + 1);
+      ihi = std::max(ihi, j - lo + 1);
+      dlo = std::min(dlo, lo);
+      dhi = std::max(dhi, hi);
+      tc += hi - lo + 1;
+    }
+    int mode = 0, xlo = 0, nx = 0;
+    if (tc > 0) {
+      const int64_t ai = (int64_t)nj * (ihi - ilo + 1), ad = (int64_t)nj * (dhi - dlo + 1);
+      if (ad < ai) mode = 1, xlo = dlo, nx = dhi - dlo + 1;
+      else xlo = ilo, nx = ihi - ilo + 1;
+    }
+    const int njr = nx > 0 ? nj : 0;
+    int32_t *R = p.rect.data() + (size_t)8 * v;
+    R[0] = b.jmin[v];
+    R[1] = njr;
+    R[2] = xlo;
+    R[3] = nx;
+    R[4] = (int32_t)base;
+    R[5] = h.lvl[v] * 2 + mode;
+    R[6] = b.hd_off[v];
+    R[7] = h.sttype[v];
+    base += (int64_t)njr * nx;
+    if (base >= INT32_MAX) {
+      *err = "bands: deck larger than 2^31 cells";
+      return IXG_EINVAL;
+    }
+    if (njr > 0) {
+      dmax = std::max(dmax, dhi);
+      maxlvl = std::max(maxlvl, h.lvl[v]);
+      width[v] = (double)tc / nj;
+      perstep[v] = (double)tc / (dhi - dlo + 1);
+      nxs[v] = std::min(nx, nj);
+    }
+  }
+  p.deck = base;
+  p.nsteps = dmax >= 0 ? dmax + 1 + maxlvl : 0;
+  const int nj0 = p.rect[1];
+  p.rootoff.assign(std::max(nj0, 1), 0);
+  for (int jp = 0; jp < nj0; jp++) p.rootoff[jp] = p.roff[b.hd_off[0] + jp] - p.roff[b.hd_off[0]];
+  // lane slots
+  std::vector<int> nsub(M, 0), kind(M, HB_SLOT_GENERIC);
+  std::vector<double> cost(M, 0.0);
+  int active = 0;
+  for (int v = 0; v < M; v++) {
+    if (nxs[v] == 0) continue;
+    kind[v] = h.sttype[v] == B_ST ? HB_SLOT_BIF : HB_SLOT_GENERIC;
+    nsub[v] = 1;
+    active++;
+    cost[v] = perstep[v];
+    if (h.sttype[v] == B_ST) cost[v] *= std::max(1.0, width[h.cnum[v]] / 8.0);
+  }
+  int spare = HB_THREADS - active;
+  using QE = std::pair<double, int>;
+  std::priority_queue<QE> q;
+  auto cap = [&](int v) { return std::min(64, nxs[v]); };
+  for (int v = 0; v < M; v++)
+    if (nsub[v] && nsub[v] < cap(v)) q.push({cost[v], v});
+  while (spare > 0 && !q.empty()) {
+    const int v = q.top().second;
+    q.pop();
+    nsub[v]++;
+    spare--;
+    if (nsub[v] < cap(v)) q.push({cost[v] / nsub[v], v});
+  }
+  p.slots.clear();
+  for (int k : {HB_SLOT_GENERIC, HB_SLOT_BIF})
+    for (int v = 0; v < M; v++)
+      if (nsub[v] && kind[v] == k)
+        for (int sub = 0; sub < nsub[v]; sub++)
+          p.slots.push_back(v | sub << 16 | (nsub[v] - 1) << 22 | k << 28);
+  if (M > 65535) {
+    *err = "standard scan: more than 65535 states";
+    return IXG_EINVAL;
+  }
+  return IXG_OK;
+}
+
+// Every band row's x range in its state's rectangle (xa > xb for an empty
+// row), 2 ints per row at out.
+void rect_rows(const CmHost &h, const ixg_bands &b, const EnvPlan &p, int32_t *out) {
+  for (int v = 0; v < h.M; v++) {
+    const int nj = b.jmax[v] >= b.jmin[v] ? b.jmax[v] - b.jmin[v] + 1 : 0;
+    const int32_t *R = p.rect.data() + (size_t)8 * v;
+    const int xlo = R[2], mode = R[5] & 1;
+    for (int jp = 0; jp < nj; jp++) {
+      const int r = b.hd_off[v] + jp, j = b.jmin[v] + jp, lo = b.hdmin[r], hi = b.hdmax[r];
+      int xa = 1, xb = 0;
+      if (hi >= lo) {
+        xa = mode ? lo - xlo : (j - hi + 1) - xlo;
+        xb = mode ? hi - xlo : (j - lo + 1) - xlo;
+      }
+      out[(size_t)2 * r] = xa;
+      out[(size_t)2 * r + 1] = xb;
+    }
+  }
+}
 
 // A truncated scan's pass: the mode bits it fills and its device penalties
 // and entry-state list.
@@ -231,17 +356,16 @@ void append_units(const CmHost &h, const ixg_bands &b, const ixg_trvalid *val, i
 // order (largest deck first) go to the device; the grid is two workgroups
 // per CU (fewer when there are fewer envelopes).
 int launch_multi(ixg_ctx *c, const std::vector<const ixg_cm *> &percm, std::vector<EnvDev> &ed,
-                 const std::vector<EnvPlan> &plan, int64_t k0, int64_t k1, EnvDev *d_env, int inside) {
+                 const std::vector<EnvPlan> &plan, int64_t k0, int64_t k1, EnvDev *d_env, int inside,
+                 size_t lds_ints) {
   std::vector<const ixg_cm *> uniq;
   std::vector<CmDev> devs;
-  int max_m = 0;
   for (int64_t k = k0; k < k1; k++) {
     const ixg_cm *m = percm[k];
     auto it = std::find(uniq.begin(), uniq.end(), m);
     if (it == uniq.end()) {
       uniq.push_back(m);
       devs.push_back(m->dev());
-      max_m = std::max(max_m, m->h.M);
       it = uniq.end() - 1;
     }
     ed[k - k0].cmi = (int32_t)(it - uniq.begin());
@@ -261,8 +385,8 @@ int launch_multi(ixg_ctx *c, const std::vector<const ixg_cm *> &percm, std::vect
   HIPCHK(hipMemcpyAsync(d_order, order.data(), ord_bytes, hipMemcpyHostToDevice, c->stream));
   int cus = 0;
   HIPCHK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
-  HIPCHK(launch_hb_scan_queue(d_cms, max_m, d_env, d_order, k1 - k0, d_counter, 2 * std::max(cus, 1),
-                              uniq[0]->d_ftbl(), inside, c->stream));
+  HIPCHK(launch_hb_scan_queue(d_cms, d_env, d_order, k1 - k0, d_counter, std::max(cus, 1), uniq[0]->d_ftbl(),
+                              inside, lds_ints, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));  // the copies above read host vectors local to this call
   return IXG_OK;
 }
@@ -283,6 +407,7 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
     if (layout(cmk(k)->h, env[k].bands, env[k].i0, env[k].j0, trunc, plan[k].roff, plan[k].ncells, plan[k].nrows,
                &err[k]) != IXG_OK)
       return;
+    if (!trunc && rect_plan(cmk(k)->h, env[k].bands, plan[k], &err[k]) != IXG_OK) return;
     if (trunc && (!valid[k].jvalid || !valid[k].lvalid || !valid[k].rvalid || !valid[k].tvalid)) {
       err[k] = "truncated scan: null validity array";
       return;
@@ -308,20 +433,33 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
   const int64_t budget = arena_elems(c, S_CM_ALPHA);  // deck floats per launch group
   int64_t k0 = 0;
   while (k0 < n) {
+    auto deck = [&](int64_t k) { return trunc ? planes * plan[k].ncells : plan[k].deck; };
     int64_t k1 = k0, floats = 0;
-    while (k1 < n && (k1 == k0 || floats + planes * plan[k1].ncells <= budget)) floats += planes * plan[k1++].ncells;
-    // pack per envelope: jmin|jmax (M each), hd_off (M+1), hdmin|hdmax|roff
-    // (nrows each), [mode mask (M)], units (M), first IR i (M); staged in
-    // pinned memory, filled on host workers
+    while (k1 < n && (k1 == k0 || floats + deck(k1) <= budget)) floats += deck(k1++);
+    // pack per envelope, staged in pinned memory and filled on host workers.
+    // Truncated: jmin|jmax (M each), hd_off (M+1), hdmin|hdmax|roff (nrows
+    // each), mode mask (M), units (M), first IR i (M). Standard: rectangles
+    // (8M), row x ranges (2 nrows), root row offsets (root rows), slots.
     std::vector<int64_t> at(k1 - k0 + 1), rat(k1 - k0);
     int64_t rtotal = 0;
+    size_t lds_ints = 0;
     at[0] = 0;
     for (int64_t k = k0; k < k1; k++) {
       const int M = cmk(k)->h.M;
-      at[k - k0 + 1] = at[k - k0] + 3 * (int64_t)M + 1 + 3 * (int64_t)plan[k].nrows + (trunc ? M : 0) + 2 * M;
+      const int64_t nr = trunc ? plan[k].nrows : std::max<int64_t>(plan[k].nrows, 1);
+      const int64_t nb = cmk(k)->h.local_begin ? (int64_t)cmk(k)->h.begin_list.size() : 0;
+      at[k - k0 + 1] = at[k - k0] + (trunc ? 3 * (int64_t)M + 1 + 3 * nr + M + 2 * M
+                                           : 8 * (int64_t)M + (int64_t)plan[k].slots.size() + 2 * nb +
+                                                 (int64_t)plan[k].rootoff.size() + 2 * nr);
       rat[k - k0] = rtotal;
       rtotal += rper * plan[k].nroot;
+      if (!trunc)
+        lds_ints = std::max(lds_ints, (size_t)(8 * M + plan[k].slots.size() + 2 * nb) + (inside ? FLOGSUM_TBL : 0));
     }
+    // LDS holds the Inside FLogsum table, the rectangle table, slots and
+    // begin list when they fit (else the kernel reads them from memory)
+    constexpr size_t kMaxLdsInts = 156 * 1024 / 4;
+    if (lds_ints > kMaxLdsInts) lds_ints = kMaxLdsInts;
     const int64_t pack_n = at[k1 - k0];
     int32_t *pack;
     RCHK(pinned(c, P_PACK, (size_t)pack_n * 4, (void **)&pack));
@@ -336,25 +474,37 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
     int64_t aoff = 0;
     for (int64_t k = k0; k < k1; k++) {
       const int M = cmk(k)->h.M;
-      const int nr = plan[k].nrows;
+      const int nr = trunc ? plan[k].nrows : std::max(plan[k].nrows, 1);
       const int32_t *base = d_pack + at[k - k0];
       EnvDev &e = ed[k - k0];
+      e = EnvDev{};
       e.dsq = d_seq_base + seq_off[k];
-      e.jmin = base;
-      e.jmax = base + M;
-      e.hd_off = base + 2 * M;
-      e.hdmin = base + 3 * M + 1;
-      e.hdmax = e.hdmin + nr;
-      e.roff = e.hdmax + nr;
-      e.mask = trunc ? e.roff + nr : nullptr;
-      e.units = e.roff + nr + (trunc ? M : 0);
-      e.ibase = e.units + M;
+      if (trunc) {
+        e.jmin = base;
+        e.jmax = base + M;
+        e.hd_off = base + 2 * M;
+        e.hdmin = base + 3 * M + 1;
+        e.hdmax = e.hdmin + nr;
+        e.roff = e.hdmax + nr;
+        e.mask = e.roff + nr;
+        e.units = e.roff + nr + M;
+        e.ibase = e.units + M;
+      } else {
+        const int nb = cmk(k)->h.local_begin ? (int)cmk(k)->h.begin_list.size() : 0;
+        e.rect = base;
+        e.slots = base + 8 * M;
+        e.nslots = (int32_t)plan[k].slots.size();
+        e.nbegin = nb;
+        e.rootoff = e.slots + e.nslots + 2 * nb;
+        e.rowx = e.rootoff + plan[k].rootoff.size();
+        e.nsteps = plan[k].nsteps;
+      }
       e.alpha = d_alpha + aoff;
       e.nc = (int32_t)plan[k].ncells;
       e.rout = d_rout + rat[k - k0];
       e.i0 = env[k].i0;
       e.j0 = env[k].j0;
-      aoff += planes * plan[k].ncells;
+      aoff += deck(k);
     }
     HIPCHK(hipMemcpyAsync(d_env, ed.data(), ed.size() * sizeof(EnvDev), hipMemcpyHostToDevice, c->stream));
     // the band upload in parts of about equal size: the host packs part s+1
@@ -373,15 +523,30 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
         const int M = cmk(k)->h.M;
         const int nr = plan[k].nrows;
         int32_t *o = pack + at[kk];
+        if (!trunc) {
+          const EnvPlan &pk = plan[k];
+          const CmHost &h = cmk(k)->h;
+          o = std::copy(pk.rect.begin(), pk.rect.end(), o);
+          o = std::copy(pk.slots.begin(), pk.slots.end(), o);
+          if (h.local_begin) {
+            o = std::copy(h.begin_list.begin(), h.begin_list.end(), o);
+            for (int32_t y : h.begin_list) {
+              const float sc = h.beginsc[y];
+              std::memcpy(o++, &sc, 4);
+            }
+          }
+          o = std::copy(pk.rootoff.begin(), pk.rootoff.end(), o);
+          rect_rows(h, b, pk, o);
+          return;
+        }
         o = std::copy(b.jmin, b.jmin + M, o);
         o = std::copy(b.jmax, b.jmax + M, o);
         o = std::copy(b.hd_off, b.hd_off + M + 1, o);
         o = std::copy(b.hdmin, b.hdmin + nr, o);
         o = std::copy(b.hdmax, b.hdmax + nr, o);
         o = std::copy(plan[k].roff.begin(), plan[k].roff.begin() + nr, o);
-        if (trunc)
-          for (int v = 0; v < M; v++) *o++ = mode_mask(valid[k], v, tr->fill);
-        append_units(cmk(k)->h, b, trunc ? &valid[k] : nullptr, trunc ? tr->fill : 0, o);
+        for (int v = 0; v < M; v++) *o++ = mode_mask(valid[k], v, tr->fill);
+        append_units(cmk(k)->h, b, &valid[k], tr->fill, o);
       });
       HIPCHK(hipMemcpyAsync(d_pack + at[a0], pack + at[a0], (size_t)(at[a1] - at[a0]) * 4, hipMemcpyHostToDevice,
                             c->stream));
@@ -391,9 +556,9 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
       HIPCHK(launch_trhb_scan(cm->dev(), d_env, k1 - k0, cm->d_ftbl(), tr->d_pty, tr->d_yl, tr->ny, inside,
                               c->stream));
     else if (percm)
-      RCHK(launch_multi(c, *percm, ed, plan, k0, k1, d_env, inside));
+      RCHK(launch_multi(c, *percm, ed, plan, k0, k1, d_env, inside, lds_ints));
     else
-      HIPCHK(launch_hb_scan(cm->dev(), d_env, k1 - k0, cm->d_ftbl(), inside, c->stream));
+      HIPCHK(launch_hb_scan(cm->dev(), d_env, k1 - k0, cm->d_ftbl(), inside, lds_ints, c->stream));
     HIPCHK(hipEventRecord(c->sev[1], c->stream));
     float *rhost;
     RCHK(pinned(c, P_DOWN, (size_t)std::max<int64_t>(rtotal, 1) * 4, (void **)&rhost));

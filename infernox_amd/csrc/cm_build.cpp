@@ -387,6 +387,7 @@ void dependency_levels(const ixg_cm_params &p, CmHost &h) {
   h.lvl_states.assign(M, 0);
   std::vector<int32_t> at(h.lvl_off.begin(), h.lvl_off.end() - 1);
   for (int v = M - 1; v >= 0; v--) h.lvl_states[at[lvl[v]]++] = v;
+  h.lvl = lvl;
 }
 
 }  // namespace ixg

@@ -34,7 +34,7 @@ struct CmHost {
   // highest (self transitions excluded), so the states of one level are
   // mutually independent; lvl_states lists states level by level, level l
   // at lvl_off[l] .. lvl_off[l+1]-1
-  std::vector<int32_t> lvl_states, lvl_off;
+  std::vector<int32_t> lvl_states, lvl_off, lvl;  // lvl: each state's level
   // truncated scanning
   int clen = 0;
   std::vector<float> lmesc, rmesc;  // M*18 marginal emission scores
@@ -86,15 +86,31 @@ struct EnvDev {
   // i diagonals of an IR state; 0 to skip) and the first i of an IR state
   const int32_t *units, *ibase;
   int32_t cmi;  // model of the envelope in a several-model launch (index into its CmDev array)
+  // standard scans (wavefront over d), packed contiguously from rect: per
+  // state 8 ints (jlo, nj, xlo, nx, deck base, level * 2 + (1 when x counts
+  // d rather than i), first band row, state type); the lane slots (state |
+  // sub << 16 | (nsub - 1) << 22 | kind << 28); the nbegin local-begin states
+  // and their begin scores (float bits); per root row the offset of its first
+  // cell in rout; per band row the true x range (xa, xb; xa > xb when
+  // empty). nsteps: the wavefront's steps.
+  const int32_t *rect, *slots, *rootoff, *rowx;
+  int32_t nslots, nbegin, nsteps;
 };
 
-constexpr int HB_THREADS = 512;      // lanes per envelope workgroup
+// Slot kinds of the standard scan: a non-bifurcation state, a bifurcation.
+enum { HB_SLOT_GENERIC = 0, HB_SLOT_BIF = 2 };
+
+constexpr int HB_THREADS = 1024;     // lanes per envelope workgroup (standard scans)
+constexpr int TRHB_THREADS = 512;    // lanes per envelope workgroup (truncated scans)
 constexpr int HB_LDS_STATES = 3200;  // states whose per-state band scalars fit in LDS
 
-hipError_t launch_hb_scan_queue(const CmDev *cms, int max_m, const EnvDev *envs, const int32_t *order, int64_t n,
-                                uint32_t *counter, int grid, const float *ftbl, int inside, hipStream_t s);
+// lds_ints: LDS for the rectangle table and slots (8 * M + nslots ints of the
+// largest envelope); an envelope that does not fit reads them from memory.
+hipError_t launch_hb_scan_queue(const CmDev *cms, const EnvDev *envs, const int32_t *order, int64_t n,
+                                uint32_t *counter, int grid, const float *ftbl, int inside, size_t lds_ints,
+                                hipStream_t s);
 hipError_t launch_hb_scan(const CmDev &cm, const EnvDev *envs, int64_t n, const float *ftbl, int inside,
-                          hipStream_t s);
+                          size_t lds_ints, hipStream_t s);
 // pty: the pass's truncated-begin penalty per state; ylist: the ny states
 // y >= 1 with an allowed truncated begin, ascending.
 hipError_t launch_trhb_scan(const CmDev &cm, const EnvDev *envs, int64_t n, const float *ftbl, const float *pty,

@@ -466,7 +466,7 @@ __device__ void tr_cell_bif(const CmDev &cm, const EnvDev &e, int v, int j, int 
 }
 
 template <bool INSIDE>
-__global__ __launch_bounds__(HB_THREADS) void trhb_scan_kernel(CmDev cm, const EnvDev *envs, const float *ftbl,
+__global__ __launch_bounds__(TRHB_THREADS) void trhb_scan_kernel(CmDev cm, const EnvDev *envs, const float *ftbl,
                                                                const float *pty, const int32_t *ylist, int ny) {
   extern __shared__ int32_t smem[];
   EnvDev e = envs[blockIdx.x];
@@ -561,10 +561,10 @@ hipError_t launch_trhb_scan(const CmDev &cm, const EnvDev *envs, int64_t n, cons
   if (n == 0) return hipSuccess;
   const size_t lds = hb_lds_bytes(cm.M);
   if (inside)
-    hipLaunchKernelGGL(trhb_scan_kernel<true>, dim3((unsigned)n), dim3(HB_THREADS), lds, s, cm, envs, ftbl, pty,
+    hipLaunchKernelGGL(trhb_scan_kernel<true>, dim3((unsigned)n), dim3(TRHB_THREADS), lds, s, cm, envs, ftbl, pty,
                        ylist, ny);
   else
-    hipLaunchKernelGGL(trhb_scan_kernel<false>, dim3((unsigned)n), dim3(HB_THREADS), lds, s, cm, envs, ftbl, pty,
+    hipLaunchKernelGGL(trhb_scan_kernel<false>, dim3((unsigned)n), dim3(TRHB_THREADS), lds, s, cm, envs, ftbl, pty,
                        ylist, ny);
   return hipGetLastError();
 }
