@@ -5,7 +5,6 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
-#include <queue>
 #include "cm_internal.h"
 #include "ctx.h"
 #include "cm_handle.h"
@@ -169,25 +168,19 @@ struct EnvPlan {
   int32_t nrows;
   int64_t nroot;
   std::vector<int32_t> roff;
-  // standard scans: the deck's state rectangles, root row offsets, lane
-  // slots, deck floats and wavefront steps
-  std::vector<int32_t> rect, rootoff, slots;
+  // standard scans: the deck's state rectangles, root row offsets, deck
+  // floats and wavefront steps
+  std::vector<int32_t> rect, rootoff;
   int64_t deck = 0;
   int32_t nsteps = 0;
 };
 
 // The standard scan's deck and schedule for one envelope (see cm_kernels.hip):
-// per state the smaller of its (j, i) and (j, d) bounding rectangles, the
-// number of wavefront steps (the largest d + 1 plus the root's level), and
-// the lane slots: one per state with cells, then the remaining lanes dealt
-// one at a time to the state with the most cells per step per lane (its
-// band cells over its d range; a bifurcation's cell weighted by its right
-// child's mean row width over the 8 split terms fetched per batch).
+// per state the smaller of its (j, i) and (j, d) bounding rectangles and the
+// number of wavefront steps (the largest d + 1 plus the root's level).
 int rect_plan(const CmHost &h, const ixg_bands &b, EnvPlan &p, const char **err) {
   const int M = h.M;
   p.rect.assign((size_t)8 * M, 0);
-  std::vector<double> width(M, 0.0), perstep(M, 0.0);
-  std::vector<int> nxs(M, 0);
   int64_t base = 0;
   int dmax = -1, maxlvl = 0;
   for (int v = 0; v < M; v++) {
@@ -218,7 +211,10 @@ int rect_plan(const CmHost &h, const ixg_bands &b, EnvPlan &p, const char **err)
     R[4] = (int32_t)base;
     R[5] = h.lvl[v] * 2 + mode;
     R[6] = b.hd_off[v];
-    R[7] = h.sttype[v];
+    // type | child count << 4 | first child << 8 (a bifurcation's fields are
+    // not used: the kernel reads its two children from the model)
+    R[7] = (int32_t)((uint32_t)h.sttype[v] | (h.sttype[v] == B_ST ? 0u : (uint32_t)h.cnum[v] << 4 |
+                                                                        (uint32_t)h.cfirst[v] << 8));
     base += (int64_t)njr * nx;
     if (base >= INT32_MAX) {
       *err = "bands: deck larger than 2^31 cells";
@@ -227,9 +223,6 @@ int rect_plan(const CmHost &h, const ixg_bands &b, EnvPlan &p, const char **err)
     if (njr > 0) {
       dmax = std::max(dmax, dhi);
       maxlvl = std::max(maxlvl, h.lvl[v]);
-      width[v] = (double)tc / nj;
-      perstep[v] = (double)tc / (dhi - dlo + 1);
-      nxs[v] = std::min(nx, nj);
     }
   }
   p.deck = base;
@@ -237,37 +230,6 @@ int rect_plan(const CmHost &h, const ixg_bands &b, EnvPlan &p, const char **err)
   const int nj0 = p.rect[1];
   p.rootoff.assign(std::max(nj0, 1), 0);
   for (int jp = 0; jp < nj0; jp++) p.rootoff[jp] = p.roff[b.hd_off[0] + jp] - p.roff[b.hd_off[0]];
-  // lane slots
-  std::vector<int> nsub(M, 0), kind(M, HB_SLOT_GENERIC);
-  std::vector<double> cost(M, 0.0);
-  int active = 0;
-  for (int v = 0; v < M; v++) {
-    if (nxs[v] == 0) continue;
-    kind[v] = h.sttype[v] == B_ST ? HB_SLOT_BIF : HB_SLOT_GENERIC;
-    nsub[v] = 1;
-    active++;
-    cost[v] = perstep[v];
-    if (h.sttype[v] == B_ST) cost[v] *= std::max(1.0, width[h.cnum[v]] / 8.0);
-  }
-  int spare = HB_THREADS - active;
-  using QE = std::pair<double, int>;
-  std::priority_queue<QE> q;
-  auto cap = [&](int v) { return std::min(64, nxs[v]); };
-  for (int v = 0; v < M; v++)
-    if (nsub[v] && nsub[v] < cap(v)) q.push({cost[v], v});
-  while (spare > 0 && !q.empty()) {
-    const int v = q.top().second;
-    q.pop();
-    nsub[v]++;
-    spare--;
-    if (nsub[v] < cap(v)) q.push({cost[v] / nsub[v], v});
-  }
-  p.slots.clear();
-  for (int k : {HB_SLOT_GENERIC, HB_SLOT_BIF})
-    for (int v = 0; v < M; v++)
-      if (nsub[v] && kind[v] == k)
-        for (int sub = 0; sub < nsub[v]; sub++)
-          p.slots.push_back(v | sub << 16 | (nsub[v] - 1) << 22 | k << 28);
   if (M > 65535) {
     *err = "standard scan: more than 65535 states";
     return IXG_EINVAL;
@@ -439,27 +401,34 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
     // pack per envelope, staged in pinned memory and filled on host workers.
     // Truncated: jmin|jmax (M each), hd_off (M+1), hdmin|hdmax|roff (nrows
     // each), mode mask (M), units (M), first IR i (M). Standard: rectangles
-    // (8M), row x ranges (2 nrows), root row offsets (root rows), slots.
+    // (8M), begin states and scores (2 nbegin), root row offsets (root rows),
+    // row x ranges (2 nrows).
     std::vector<int64_t> at(k1 - k0 + 1), rat(k1 - k0);
     int64_t rtotal = 0;
-    size_t lds_ints = 0;
+    size_t lds_ints = 0, scan_ints = 0;
     at[0] = 0;
     for (int64_t k = k0; k < k1; k++) {
       const int M = cmk(k)->h.M;
       const int64_t nr = trunc ? plan[k].nrows : std::max<int64_t>(plan[k].nrows, 1);
       const int64_t nb = cmk(k)->h.local_begin ? (int64_t)cmk(k)->h.begin_list.size() : 0;
       at[k - k0 + 1] = at[k - k0] + (trunc ? 3 * (int64_t)M + 1 + 3 * nr + M + 2 * M
-                                           : 8 * (int64_t)M + (int64_t)plan[k].slots.size() + 2 * nb +
-                                                 (int64_t)plan[k].rootoff.size() + 2 * nr);
+                                           : 8 * (int64_t)M + 2 * nb + (int64_t)plan[k].rootoff.size() + 2 * nr);
       rat[k - k0] = rtotal;
       rtotal += rper * plan[k].nroot;
-      if (!trunc)
-        lds_ints = std::max(lds_ints, (size_t)(8 * M + plan[k].slots.size() + 2 * nb) + (inside ? FLOGSUM_TBL : 0));
+      if (!trunc) {
+        scan_ints = std::max(scan_ints, (size_t)(2 * M + 32));
+        lds_ints = std::max(lds_ints, (size_t)(2 * M + 32 + 8 * M + 2 * nb) + (inside ? FLOGSUM_TBL : 0));
+      }
     }
-    // LDS holds the Inside FLogsum table, the rectangle table, slots and
-    // begin list when they fit (else the kernel reads them from memory)
+    // LDS always holds the per-step scan (2M + 32 ints); the Inside FLogsum
+    // table, rectangle table and begin list too when they fit (else the
+    // kernel reads them from memory)
     constexpr size_t kMaxLdsInts = 156 * 1024 / 4;
-    if (lds_ints > kMaxLdsInts) lds_ints = kMaxLdsInts;
+    if (!trunc && scan_ints > kMaxLdsInts) {
+      g_err = "standard scan: too many states for the LDS step scan";
+      return IXG_EINVAL;
+    }
+    lds_ints = std::min(lds_ints, kMaxLdsInts);
     const int64_t pack_n = at[k1 - k0];
     int32_t *pack;
     RCHK(pinned(c, P_PACK, (size_t)pack_n * 4, (void **)&pack));
@@ -492,10 +461,8 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
       } else {
         const int nb = cmk(k)->h.local_begin ? (int)cmk(k)->h.begin_list.size() : 0;
         e.rect = base;
-        e.slots = base + 8 * M;
-        e.nslots = (int32_t)plan[k].slots.size();
         e.nbegin = nb;
-        e.rootoff = e.slots + e.nslots + 2 * nb;
+        e.rootoff = base + 8 * M + 2 * nb;
         e.rowx = e.rootoff + plan[k].rootoff.size();
         e.nsteps = plan[k].nsteps;
       }
@@ -527,7 +494,6 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
           const EnvPlan &pk = plan[k];
           const CmHost &h = cmk(k)->h;
           o = std::copy(pk.rect.begin(), pk.rect.end(), o);
-          o = std::copy(pk.slots.begin(), pk.slots.end(), o);
           if (h.local_begin) {
             o = std::copy(h.begin_list.begin(), h.begin_list.end(), o);
             for (int32_t y : h.begin_list) {

@@ -88,24 +88,21 @@ struct EnvDev {
   int32_t cmi;  // model of the envelope in a several-model launch (index into its CmDev array)
   // standard scans (wavefront over d), packed contiguously from rect: per
   // state 8 ints (jlo, nj, xlo, nx, deck base, level * 2 + (1 when x counts
-  // d rather than i), first band row, state type); the lane slots (state |
-  // sub << 16 | (nsub - 1) << 22 | kind << 28); the nbegin local-begin states
-  // and their begin scores (float bits); per root row the offset of its first
-  // cell in rout; per band row the true x range (xa, xb; xa > xb when
-  // empty). nsteps: the wavefront's steps.
-  const int32_t *rect, *slots, *rootoff, *rowx;
-  int32_t nslots, nbegin, nsteps;
+  // d rather than i), first band row, state type | child count << 4 | first
+  // child << 8); the nbegin local-begin
+  // states and their begin scores (float bits); per root row the offset of
+  // its first cell in rout; per band row the true x range (xa, xb; xa > xb
+  // when empty). nsteps: the wavefront's steps.
+  const int32_t *rect, *rootoff, *rowx;
+  int32_t nbegin, nsteps;
 };
-
-// Slot kinds of the standard scan: a non-bifurcation state, a bifurcation.
-enum { HB_SLOT_GENERIC = 0, HB_SLOT_BIF = 2 };
 
 constexpr int HB_THREADS = 1024;     // lanes per envelope workgroup (standard scans)
 constexpr int TRHB_THREADS = 512;    // lanes per envelope workgroup (truncated scans)
 constexpr int HB_LDS_STATES = 3200;  // states whose per-state band scalars fit in LDS
 
-// lds_ints: LDS for the rectangle table and slots (8 * M + nslots ints of the
-// largest envelope); an envelope that does not fit reads them from memory.
+// lds_ints: LDS ints per workgroup: the per-step scan (2M + 32) and, when
+// they fit, the Inside FLogsum table, rectangle table and begin list.
 hipError_t launch_hb_scan_queue(const CmDev *cms, const EnvDev *envs, const int32_t *order, int64_t n,
                                 uint32_t *counter, int grid, const float *ftbl, int inside, size_t lds_ints,
                                 hipStream_t s);

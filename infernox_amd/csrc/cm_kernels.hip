@@ -17,18 +17,16 @@
 //   in a lower level (S, D, and the k = 0 / k = d splits), so it was
 //   finished at an earlier step. One barrier per step, no serial chain
 //   inside a step except a bifurcation cell's splits.
-// * Lanes: the host deals each state one or more lane slots for the
-//   envelope in proportion to its cells per step (a bifurcation weighted by
-//   its splits), grouped by kind so that a wave runs one code path. A slot
-//   with nsub lanes takes every nsub-th cell of its state's d diagonal.
+// * Lanes: each step the states' cell counts at their d are prefix-summed
+//   in LDS and the step's cells are dealt round-robin over all lanes.
 //
 // Per cell the children are combined in increasing child order (max for
 // CYK, the FLogsum table for Inside) and a bifurcation's splits in
 // increasing k, the order of the reference's loops (cp9.rs:4788-5253,
 // 5514-5885), so CYK is bit-identical and Inside applies the same float
-// operations in the same order. Local begins are folded into the root by
-// MAX as each root row is finished (cp9.rs:5264-5304, 5908-5937); root rows
-// also go to the envelope's compact root output.
+// operations in the same order. Local begins are folded into each root cell
+// by MAX as it is computed (cp9.rs:5264-5304, 5908-5937); root cells also go
+// to the envelope's compact root output.
 #include "cm_internal.h"
 
 namespace ixg {
@@ -62,15 +60,18 @@ __device__ __forceinline__ float cell_init(const CmDev &cm, int v, int st, int d
 
 // A state's rectangle: rows jlo .. jlo+nj-1, columns x = 0 .. nx-1 (x = i -
 // xlo, or d - xlo in d mode), row-major at alpha + base.
+// The last word packs the state type, child count and first child (a
+// bifurcation's children are read from the model instead).
 struct Rect {
-  int jlo, nj, xlo, nx, base, lvl, mode, rowbase, st;
+  int jlo, nj, xlo, nx, base, lvl, mode, rowbase, st, cn, cf;
 };
 __device__ __forceinline__ Rect rect_of(const int32_t *rt, int v) {
   const int4 a = *reinterpret_cast<const int4 *>(rt + 8 * v);
   const int4 b = *reinterpret_cast<const int4 *>(rt + 8 * v + 4);
   Rect r;
   r.jlo = a.x, r.nj = a.y, r.xlo = a.z, r.nx = a.w;
-  r.base = b.x, r.lvl = b.y >> 1, r.mode = b.y & 1, r.rowbase = b.z, r.st = b.w;
+  r.base = b.x, r.lvl = b.y >> 1, r.mode = b.y & 1, r.rowbase = b.z;
+  r.st = b.w & 15, r.cn = (b.w >> 4) & 15, r.cf = (int)((unsigned)b.w >> 8);
   return r;
 }
 // Deck index of cell (j, d) of a state, or -1 outside its rectangle.
@@ -81,19 +82,18 @@ __device__ __forceinline__ int rect_cell(const Rect &r, int j, int d) {
 }
 __device__ __forceinline__ int rect_cell(const int32_t *rt, int y, int j, int d) { return rect_cell(rect_of(rt, y), j, d); }
 
-// d of column x of row j.
-__device__ __forceinline__ int d_of(const Rect &r, int j, int x) { return r.mode ? r.xlo + x : j - r.xlo - x + 1; }
-
-// One cell of a non-insert, non-bifurcation state (S, D, ML, MR, MP, E) or
-// of an IR state (its self transition reads its own row j-1): children
-// fetched first, then combined in child order.
+// One cell of a non-bifurcation state (S, D, ML, MR, MP, IL, IR, E; an
+// insert's self transition reads its own cell at d-1): children fetched
+// first, then combined in child order. Called for every rectangle cell; the
+// caller keeps the value only for band cells (every load stays in bounds).
 template <bool INSIDE>
-__device__ __forceinline__ float cell_generic(const CmDev &cm, const EnvDev &e, const int32_t *rt, int v, int st,
-                                              int j, int d, const float *__restrict__ ftbl) {
+__device__ __forceinline__ float cell_generic(const CmDev &cm, const EnvDev &e, const int32_t *rt, const Rect &r,
+                                              int v, int j, int d, const float *__restrict__ ftbl) {
+  const int st = r.st;
   if (st == E_ST) return 0.0f;  // an end state's band is d = 0
   float cur = cell_init(cm, v, st, d);
   const int sd = sdelta(st), sdr = srdelta(st);
-  const int cf = gmem(cm.cfirst)[v], cn = gmem(cm.cnum)[v];
+  const int cf = r.cf, cn = r.cn;
   const IXG_GLOBAL float *tsc = gmem(cm.tsc) + v * MAXC;
   const IXG_GLOBAL float *al = gmem(e.alpha);
   float a[MAXC];
@@ -105,7 +105,9 @@ __device__ __forceinline__ float cell_generic(const CmDev &cm, const EnvDev &e, 
     a[yo] = al[ok[yo] ? c : 0];
   }
   const IXG_GLOBAL uint8_t *dsq = gmem(e.dsq);
-  const int i = j - d + 1;
+  // i clamped into the envelope: a cell outside the band is computed for
+  // nothing (its value is discarded) and may have i < 1
+  const int i = max(j - d + 1, e.i0);
   const int xl = dsq[st == MR_ST || st == IR_ST ? j : i], xr = dsq[j];
 #pragma unroll
   for (int yo = 0; yo < MAXC; yo++)
@@ -136,177 +138,179 @@ __device__ __forceinline__ float cell_bif(const CmDev &cm, const EnvDev &e, cons
   const int zrow = rz.base + jpz * rz.nx;
   constexpr int KB = 8;
   for (int k0 = kn; k0 <= kx; k0 += KB) {
+    // a term whose left cell is outside y's rectangle (or past kx) is -inf,
+    // which max and FLogsum ignore exactly as the reference skips it
     float a[KB], b[KB];
-    bool ok[KB];
 #pragma unroll
     for (int u = 0; u < KB; u++) {
-      const int k = k0 + u;
+      const int k = k0 + u, kc = k <= kx ? k : kx;
       const int cy = k <= kx ? rect_cell(ry, j - k, d - k) : -1;
-      const int xz = rz.mode ? k - rz.xlo : (j - k + 1) - rz.xlo;
-      ok[u] = cy >= 0;
-      a[u] = al[ok[u] ? cy : 0];
-      b[u] = al[ok[u] ? zrow + xz : 0];
+      const int xz = rz.mode ? kc - rz.xlo : (j - kc + 1) - rz.xlo;
+      const float av = al[cy >= 0 ? cy : 0];
+      a[u] = cy >= 0 ? av : -INFINITY;
+      b[u] = al[zrow + xz];
     }
 #pragma unroll
-    for (int u = 0; u < KB; u++)
-      if (ok[u]) cur = combine<INSIDE>(cur, a[u] + b[u], ftbl);
+    for (int u = 0; u < KB; u++) cur = combine<INSIDE>(cur, a[u] + b[u], ftbl);
   }
   return cur;
 }
 
-__device__ __forceinline__ void slot_decode(int32_t code, int &v, int &sub, int &nsub, int &kind) {
-  // state (bits 0-15), sub (16-21), nsub - 1 (22-27), kind (28-29)
-  v = code & 0xFFFF, sub = (code >> 16) & 63, nsub = ((code >> 22) & 63) + 1, kind = (code >> 28) & 3;
+// One rectangle cell of state v at (j, d): its band cell or -inf, and its
+// deck index.
+template <bool INSIDE>
+__device__ __forceinline__ float hb_cell(const CmDev &cm, const EnvDev &e, const int32_t *rt, const Rect &r, int v,
+                                         int j, int d, const float *ftbl, int &idx) {
+  const int jp = j - r.jlo;
+  const int x = r.mode ? d - r.xlo : (j - d + 1) - r.xlo;
+  const IXG_GLOBAL int32_t *rx = gmem(e.rowx) + 2 * (r.rowbase + jp);
+  const int xa = rx[0], xb = rx[1];
+  const bool in = x >= xa && x <= xb;
+  idx = r.base + jp * r.nx + x;
+  if (r.st == B_ST) return in ? cell_bif<INSIDE>(cm, e, rt, v, j, d, ftbl) : -INFINITY;
+  const float cur = cell_generic<INSIDE>(cm, e, rt, r, v, j, d, ftbl);  // loads overlap the band range's
+  return in ? cur : -INFINITY;
 }
 
-// The slot's share of the cells of state v at subsequence length d: every
-// rectangle cell with that d, band cells computed, the others set to -inf.
+// One envelope, scanned by the whole workgroup: a wavefront over d (step t:
+// state v fills its rectangle cells of length d = t - level(v)). Each step
+// the states' cell counts are prefix-summed in LDS (pre[v], first row
+// jst[v]) and every lane takes every nt-th cell of the step (one cell per
+// lane at a time keeps the most cells in flight). rt / bl / bs:
+// the rectangle table (8 ints per state), the begin states and their begin
+// scores; ftbl: the FLogsum table.
 template <bool INSIDE>
-__device__ __forceinline__ void hb_diag(const CmDev &cm, const EnvDev &e, const int32_t *rt, const Rect &r, int v,
-                                        int kind, int sub, int nsub, int d, const float *ftbl) {
-  // rows whose rectangle holds a cell with this d
-  int ja = r.jlo, jb = r.jlo + r.nj - 1;
-  if (r.mode) {
-    if ((unsigned)(d - r.xlo) >= (unsigned)r.nx) return;
-  } else {
-    ja = max(ja, d - 1 + r.xlo);
-    jb = min(jb, d - 2 + r.xlo + r.nx);
-  }
-  IXG_GLOBAL float *al = gmem(e.alpha);
-  const IXG_GLOBAL int32_t *rowx = gmem(e.rowx) + 2 * r.rowbase;
-  for (int j = ja + sub; j <= jb; j += nsub) {
-    const int jp = j - r.jlo;
-    const int x = r.mode ? d - r.xlo : (j - d + 1) - r.xlo;
-    const int xa = rowx[2 * jp], xb = rowx[2 * jp + 1];
-    float cur = -INFINITY;
-    if (x >= xa && x <= xb)
-      cur = kind == HB_SLOT_BIF ? cell_bif<INSIDE>(cm, e, rt, v, j, d, ftbl)
-                                : cell_generic<INSIDE>(cm, e, rt, v, r.st, j, d, ftbl);
-    al[r.base + jp * r.nx + x] = cur;
-  }
-}
-
-// One envelope, scanned by the whole workgroup. rt / sl / bl / bs: the
-// rectangle table (8 ints per state), the lane slots, the begin states and
-// their begin scores; ftbl: the FLogsum table. At step t state v fills its
-// cells of length d = t - level(v).
-template <bool INSIDE>
-__device__ __forceinline__ void hb_scan_body(const CmDev &cm, const EnvDev &e, const int32_t *rt, const int32_t *sl,
-                                             const int32_t *bl, const float *bs, int nb, const float *ftbl) {
-  const int tid = threadIdx.x, nt = blockDim.x;
-  IXG_GLOBAL float *al = gmem(e.alpha);
-  auto row_range = [&](const Rect &r, int jp, int &xa, int &xb) {
-    const IXG_GLOBAL int32_t *rx = gmem(e.rowx) + 2 * (r.rowbase + jp);
-    xa = rx[0];
-    xb = rx[1];
-  };
-  if (e.nslots <= nt) {
-    int v = 0, sub = 0, nsub = 1, kind = 0;
-    Rect r{};
-    const bool mine = tid < e.nslots;
-    if (mine) {
-      slot_decode(sl[tid], v, sub, nsub, kind);
-      r = rect_of(rt, v);
-    }
-    for (int t = 0; t < e.nsteps; t++) {
+__device__ __forceinline__ void hb_scan_body(const CmDev &cm, const EnvDev &e, const int32_t *rt, const int32_t *bl,
+                                             const float *bs, int nb, const float *ftbl, int32_t *pre, int32_t *jst,
+                                             int32_t *wsum) {
+  const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
+  const int M = cm.M;
+  const int per = (M + nt - 1) / nt, v0 = tid * per;
+  for (int t = 0; t < e.nsteps; t++) {
+    // this thread's states: first row and cell count at this step
+    int my = 0;
+    for (int q = 0; q < per; q++) {
+      const int v = v0 + q;
+      if (v >= M) break;
+      const Rect r = rect_of(rt, v);
       const int d = t - r.lvl;
-      if (mine && d >= 0) hb_diag<INSIDE>(cm, e, rt, r, v, kind, sub, nsub, d, ftbl);
-      __syncthreads();
-    }
-  } else {
-    for (int t = 0; t < e.nsteps; t++) {
-      for (int s = tid; s < e.nslots; s += nt) {
-        int v, sub, nsub, kind;
-        slot_decode(sl[s], v, sub, nsub, kind);
-        const Rect r = rect_of(rt, v);
-        const int d = t - r.lvl;
-        if (d >= 0) hb_diag<INSIDE>(cm, e, rt, r, v, kind, sub, nsub, d, ftbl);
+      int n = 0, ja = 0;
+      if (d >= 0 && r.nx > 0) {
+        if (r.mode) {
+          if ((unsigned)(d - r.xlo) < (unsigned)r.nx) ja = r.jlo, n = r.nj;
+        } else {
+          ja = max(r.jlo, d - 1 + r.xlo);
+          n = max(0, min(r.jlo + r.nj - 1, d - 2 + r.xlo + r.nx) - ja + 1);
+        }
       }
-      __syncthreads();
+      jst[v] = ja;
+      pre[v] = my;
+      my += n;
     }
+    // block exclusive scan of the threads' counts
+    int inc = my;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_up(inc, o);
+      if (lane >= o) inc += u;
+    }
+    if (lane == 63) wsum[wv] = inc;
+    __syncthreads();
+    int off = inc - my, total = 0;
+    for (int w = 0; w < nw; w++) {
+      const int s = wsum[w];
+      if (w < wv) off += s;
+      total += s;
+    }
+    for (int q = 0; q < per; q++) {
+      const int v = v0 + q;
+      if (v >= M) break;
+      pre[v] += off;
+    }
+    __syncthreads();
+    // the step's cells, every nt-th to a lane
+    IXG_GLOBAL float *al = gmem(e.alpha);
+    for (int c = tid; c < total; c += nt) {
+      int lo = 0, hi = M - 1;  // the last state whose first cell is at or before c
+      while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= c) lo = mid;
+        else hi = mid - 1;
+      }
+      const Rect r = rect_of(rt, lo);
+      int ix;
+      const float val = hb_cell<INSIDE>(cm, e, rt, r, lo, jst[lo] + (c - pre[lo]), t - r.lvl, ftbl, ix);
+      al[ix] = val;
+    }
+    __syncthreads();
   }
-  // root: local begins by MAX in begin-list order (cp9.rs:5264-5304,
-  // 5908-5937), then the compact root rows (increasing d per row). Each lane
-  // owns RB root cells at a time; their begin terms are fetched together.
+  // local begins into the root by MAX (cp9.rs:5264-5304, 5908-5937), begin
+  // state by begin state in list order (the reference's strict comparison
+  // per cell, so ties resolve as there): all lanes stream through the
+  // state's rectangle rows that overlap the root's rows; band and rectangle
+  // checks on both sides (a cell outside either band contributes nothing)
+  IXG_GLOBAL float *al = gmem(e.alpha);
   const Rect r0 = rect_of(rt, 0);
+  for (int b = 0; b < nb; b++) {
+    const int y = bl[b];
+    const float sc = bs[b];
+    const Rect ry = rect_of(rt, y);
+    const int ja = max(ry.jlo, r0.jlo), jb = min(ry.jlo + ry.nj, r0.jlo + r0.nj) - 1;
+    const int n = jb >= ja ? (jb - ja + 1) * ry.nx : 0;
+    for (int c = tid; c < n; c += nt) {
+      const int jr = c / ry.nx, xy = c - jr * ry.nx, j = ja + jr;
+      const int d = ry.mode ? ry.xlo + xy : j - ry.xlo - xy + 1;
+      const int c0 = rect_cell(r0, j, d);
+      if (c0 < 0) continue;
+      const float a = al[ry.base + (j - ry.jlo) * ry.nx + xy];  // -inf outside y's band
+      const float sv = a + sc;
+      if (sv > al[c0]) al[c0] = sv;  // -inf root cells (outside its band) are rewritten below
+    }
+    __syncthreads();
+  }
+  // the root rows out, and -inf back into the root's out-of-band cells
   const int n0 = r0.nj * r0.nx;
-  constexpr int RB = 4;
-  for (int c0 = 0; c0 < n0; c0 += RB * nt) {
-    float cur[RB];
-    int jq[RB], dq[RB], oq[RB];
-#pragma unroll
-    for (int q = 0; q < RB; q++) {
-      const int c = c0 + q * nt + tid;
-      cur[q] = -INFINITY;
-      oq[q] = -1;
-      if (c < n0) {
-        const int jp = c / r0.nx, x = c - jp * r0.nx;
-        int xa, xb;
-        row_range(r0, jp, xa, xb);
-        jq[q] = r0.jlo + jp;
-        dq[q] = d_of(r0, jq[q], x);
-        if (x >= xa && x <= xb) {
-          cur[q] = al[r0.base + c];
-          oq[q] = e.rootoff[jp] + (r0.mode ? x - xa : xb - x);
-        }
-      }
-    }
-    for (int b = 0; b < nb; b++) {
-      const int y = bl[b];
-      const float sc = bs[b];
-      const Rect ry = rect_of(rt, y);
-      float a[RB];
-      int ci[RB];
-#pragma unroll
-      for (int q = 0; q < RB; q++) {
-        ci[q] = oq[q] >= 0 ? rect_cell(ry, jq[q], dq[q]) : -1;
-        a[q] = al[ci[q] >= 0 ? ci[q] : 0];
-      }
-#pragma unroll
-      for (int q = 0; q < RB; q++)
-        if (ci[q] >= 0) {
-          const float sv = a[q] + sc;
-          if (sv > cur[q]) cur[q] = sv;
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < RB; q++)
-      if (oq[q] >= 0) {
-        al[r0.base + c0 + q * nt + tid] = cur[q];
-        gmem(e.rout)[oq[q]] = cur[q];
-      }
+  for (int c = tid; c < n0; c += nt) {
+    const int jp = c / r0.nx, x = c - jp * r0.nx;
+    const IXG_GLOBAL int32_t *rx = gmem(e.rowx) + 2 * (r0.rowbase + jp);
+    const int xa = rx[0], xb = rx[1];
+    if (x >= xa && x <= xb) gmem(e.rout)[e.rootoff[jp] + (r0.mode ? x - xa : xb - x)] = al[r0.base + c];
+    else if (nb) al[r0.base + c] = -INFINITY;
   }
 }
 
-// Scans one envelope with its rectangle table, slots and begin list in LDS
-// (RL) or read from memory, and the Inside FLogsum table in LDS (TL) or
-// memory; the choice is a template parameter so that every access compiles
-// to LDS or global instructions.
+// Scans one envelope with its rectangle table and begin list in LDS (RL) or
+// read from memory, and the Inside FLogsum table in LDS (TL) or memory.
+// LDS: [FLogsum table (Inside, TL)] [pre (M), jst (M), wsum (32)]
+// [rectangle table (8M), begin states (nb), begin scores (nb)] (RL); the
+// choice is a template parameter so that every access compiles to LDS or
+// global instructions.
 template <bool INSIDE, bool RL, bool TL>
 __device__ __forceinline__ void hb_scan_staged(const CmDev &cm, const EnvDev &e, const float *ftbl, int32_t *smem) {
   const int M = cm.M, nb = e.nbegin;
-  const int32_t *base = RL ? smem + (TL ? FLOGSUM_TBL : 0) : (const int32_t *)gmem(e.rect);
-  const int32_t *sl = base + 8 * M, *bl = sl + e.nslots;
+  int32_t *pre = smem + (TL ? FLOGSUM_TBL : 0), *jst = pre + M, *wsum = jst + M;
+  const int32_t *base = RL ? wsum + 32 : (const int32_t *)gmem(e.rect);
+  const int32_t *bl = base + 8 * M;
   const float *bs = reinterpret_cast<const float *>(bl + nb);
   const float *tb = TL ? reinterpret_cast<const float *>(smem) : (const float *)gmem(ftbl);
-  hb_scan_body<INSIDE>(cm, e, base, sl, bl, bs, nb, tb);
+  hb_scan_body<INSIDE>(cm, e, base, bl, bs, nb, tb, pre, jst, wsum);
 }
 
-// Stages what fits in LDS (the FLogsum table for Inside, then the envelope's
-// rectangle table, slots, begin list and begin scores, which the host packs
-// contiguously at e.rect) and scans the envelope.
+// Stages what fits in LDS and scans the envelope; the host sizes lds_ints
+// for the step scan at least (2M + 32 ints).
 template <bool INSIDE>
 __device__ __forceinline__ void hb_scan_env(const CmDev &cm, const EnvDev &e, const float *ftbl, int32_t *smem,
                                             int lds_ints) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int M = cm.M, nb = e.nbegin;
-  const int need = 8 * M + e.nslots + 2 * nb;
-  const bool rl = need <= lds_ints, tl = INSIDE && FLOGSUM_TBL + need <= lds_ints;
-  int32_t *at = smem + (tl ? FLOGSUM_TBL : 0);
+  const int scan = 2 * M + 32, need = 8 * M + 2 * nb;
+  const bool rl = scan + need <= lds_ints, tl = INSIDE && rl && FLOGSUM_TBL + scan + need <= lds_ints;
   if (tl)
     for (int q = tid; q < FLOGSUM_TBL; q += nt) smem[q] = __float_as_int(ftbl[q]);
-  if (rl)
+  if (rl) {
+    int32_t *at = smem + (tl ? FLOGSUM_TBL : 0) + scan;
     for (int q = tid; q < need; q += nt) at[q] = e.rect[q];
+  }
   __syncthreads();
   if (tl) hb_scan_staged<INSIDE, true, INSIDE>(cm, e, ftbl, smem);
   else if (rl) hb_scan_staged<INSIDE, true, false>(cm, e, ftbl, smem);

@@ -269,7 +269,15 @@ extern "C" int ixg_search_std(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, con
     std::vector<float> vsc(run.size());
     std::vector<int64_t> ei(run.size()), ej(run.size());
     int64_t nh = 0, cells = 0;
-    std::vector<ixg_hit> hb(1024);
+    // hits buffer as large as the root cells, the most ReportHitsGreedily
+    // can report, so the scan never has to run again for room
+    int64_t rcells = 0;
+    for (const ixg_envelope &x : run)
+      for (int j = x.bands.jmin[0]; j <= x.bands.jmax[0]; j++) {
+        const int r = x.bands.hd_off[0] + (j - x.bands.jmin[0]);
+        rcells += std::max(0, x.bands.hdmax[r] - x.bands.hdmin[r] + 1);
+      }
+    std::vector<ixg_hit> hb(std::max<int64_t>(rcells, 1));
     if (run.empty()) hb.clear();
     else rc = ixg_hb_scan_batch(c, cm, tgt, run.data(), (int64_t)run.size(), o->do_cyk ? 0 : 1, 0.0f, 1, o->pli_t,
                            o->do_null3, vsc.data(), ei.data(), ej.data(), hb.data(), (int64_t)hb.size(), &nh, &cells);
