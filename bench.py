@@ -937,6 +937,8 @@ def main():
     ap.add_argument("--headline-only", action="store_true", help="only the headline search step")
     ap.add_argument("--serve-only", action="store_true", help="only run the serving-loop section")
     ap.add_argument("--multi-only", action="store_true", help="only run the several-CM HB launch section")
+    ap.add_argument("--hb-only", action="store_true",
+                    help="only run the HB CYK / Inside section (959-state CM, 512 envelopes)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the 256 MiB device copy probe (HBM counter calibration) and print its rate")
     a = ap.parse_args()
@@ -948,11 +950,19 @@ def main():
         with ixg.Context(0) as ctx:
             nb = 256 << 20
             ms = ctx.copy_probe(nb, 2)
-        print(json.dumps({"copy_probe_bytes": nb, "ms_per_copy": ms, "GB/s": 2 * nb / (ms * 1e-3) / 1e9}))
+        # a profiler's counter pass serialises the launches and the events
+        # can read 0: the rate is then unknown, not infinite
+        print(json.dumps({"copy_probe_bytes": nb, "ms_per_copy": ms,
+                          "GB/s": 2 * nb / (ms * 1e-3) / 1e9 if ms > 0 else None}))
         return
     if a.multi_only:
         with ixg.Context(0) as ctx:
             print(json.dumps({"hb_multi": time_multi(ctx)}))
+        return
+    if a.hb_only:
+        with ixg.Context(0) as ctx:
+            cpu = None if a.no_cpu else (a.cpu_section_seconds, a.cpu_threads)
+            print(json.dumps({"hb_scanners": time_cm(ctx, a.steps, a.warmup, cpu, cm_workload())}))
         return
     if a.serve_only:
         with ixg.Context(0) as ctx:

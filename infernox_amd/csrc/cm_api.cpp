@@ -176,17 +176,17 @@ struct EnvPlan {
 };
 
 // The standard scan's deck and schedule for one envelope (see cm_kernels.hip):
-// per state the smaller of its (j, i) and (j, d) bounding rectangles and the
-// number of wavefront steps (the largest d + 1 plus the root's level).
+// per state its band's rectangle, diagonal-major (d, x) with x counting j or
+// i, whichever band width is smaller, and the number of wavefront steps (the
+// largest d + 1 plus the root's level).
 int rect_plan(const CmHost &h, const ixg_bands &b, EnvPlan &p, const char **err) {
   const int M = h.M;
-  p.rect.assign((size_t)8 * M, 0);
+  p.rect.assign((size_t)HB_RECT * M, 0);
   int64_t base = 0;
   int dmax = -1, maxlvl = 0;
   for (int v = 0; v < M; v++) {
     const int nj = b.jmax[v] >= b.jmin[v] ? b.jmax[v] - b.jmin[v] + 1 : 0;
     int ilo = INT32_MAX, ihi = INT32_MIN, dlo = INT32_MAX, dhi = INT32_MIN;
-    int64_t tc = 0;
     for (int jp = 0; jp < nj; jp++) {
       const int r = b.hd_off[v] + jp, j = b.jmin[v] + jp, lo = b.hdmin[r], hi = b.hdmax[r];
       if (hi < lo) continue;
@@ -194,36 +194,33 @@ int rect_plan(const CmHost &h, const ixg_bands &b, EnvPlan &p, const char **err)
       ihi = std::max(ihi, j - lo + 1);
       dlo = std::min(dlo, lo);
       dhi = std::max(dhi, hi);
-      tc += hi - lo + 1;
     }
-    int mode = 0, xlo = 0, nx = 0;
-    if (tc > 0) {
-      const int64_t ai = (int64_t)nj * (ihi - ilo + 1), ad = (int64_t)nj * (dhi - dlo + 1);
-      if (ad < ai) mode = 1, xlo = dlo, nx = dhi - dlo + 1;
-      else xlo = ilo, nx = ihi - ilo + 1;
-    }
-    const int njr = nx > 0 ? nj : 0;
-    int32_t *R = p.rect.data() + (size_t)8 * v;
+    int32_t *R = p.rect.data() + (size_t)HB_RECT * v;
     R[0] = b.jmin[v];
-    R[1] = njr;
-    R[2] = xlo;
-    R[3] = nx;
-    R[4] = (int32_t)base;
-    R[5] = h.lvl[v] * 2 + mode;
     R[6] = b.hd_off[v];
     // type | child count << 4 | first child << 8 (a bifurcation's fields are
     // not used: the kernel reads its two children from the model)
     R[7] = (int32_t)((uint32_t)h.sttype[v] | (h.sttype[v] == B_ST ? 0u : (uint32_t)h.cnum[v] << 4 |
                                                                         (uint32_t)h.cfirst[v] << 8));
-    base += (int64_t)njr * nx;
+    if (dhi < dlo) {  // no band cells: no deck
+      R[5] = h.lvl[v] * 2;
+      continue;
+    }
+    const int ni = ihi - ilo + 1, mode = nj <= ni ? 1 : 0, nx = mode ? nj : ni, nd = dhi - dlo + 1;
+    R[1] = nj;
+    R[2] = ilo;
+    R[3] = nx;
+    R[4] = (int32_t)base;
+    R[5] = h.lvl[v] * 2 + mode;
+    R[8] = dlo;
+    R[9] = nd;
+    base += (int64_t)nd * nx;
     if (base >= INT32_MAX) {
       *err = "bands: deck larger than 2^31 cells";
       return IXG_EINVAL;
     }
-    if (njr > 0) {
-      dmax = std::max(dmax, dhi);
-      maxlvl = std::max(maxlvl, h.lvl[v]);
-    }
+    dmax = std::max(dmax, dhi);
+    maxlvl = std::max(maxlvl, h.lvl[v]);
   }
   p.deck = base;
   p.nsteps = dmax >= 0 ? dmax + 1 + maxlvl : 0;
@@ -237,25 +234,14 @@ int rect_plan(const CmHost &h, const ixg_bands &b, EnvPlan &p, const char **err)
   return IXG_OK;
 }
 
-// Every band row's x range in its state's rectangle (xa > xb for an empty
-// row), 2 ints per row at out.
-void rect_rows(const CmHost &h, const ixg_bands &b, const EnvPlan &p, int32_t *out) {
-  for (int v = 0; v < h.M; v++) {
-    const int nj = b.jmax[v] >= b.jmin[v] ? b.jmax[v] - b.jmin[v] + 1 : 0;
-    const int32_t *R = p.rect.data() + (size_t)8 * v;
-    const int xlo = R[2], mode = R[5] & 1;
-    for (int jp = 0; jp < nj; jp++) {
-      const int r = b.hd_off[v] + jp, j = b.jmin[v] + jp, lo = b.hdmin[r], hi = b.hdmax[r];
-      int xa = 1, xb = 0;
-      if (hi >= lo) {
-        xa = mode ? lo - xlo : (j - hi + 1) - xlo;
-        xb = mode ? hi - xlo : (j - lo + 1) - xlo;
-      }
-      out[(size_t)2 * r] = xa;
-      out[(size_t)2 * r + 1] = xb;
-    }
+// Every band row's d band (hdmin, hdmax), 2 ints per row at out.
+void rect_rows(const ixg_bands &b, int nrows, int32_t *out) {
+  for (int r = 0; r < nrows; r++) {
+    out[2 * (size_t)r] = b.hdmin[r];
+    out[2 * (size_t)r + 1] = b.hdmax[r];
   }
 }
+
 
 // A truncated scan's pass: the mode bits it fills and its device penalties
 // and entry-state list.
@@ -401,8 +387,8 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
     // pack per envelope, staged in pinned memory and filled on host workers.
     // Truncated: jmin|jmax (M each), hd_off (M+1), hdmin|hdmax|roff (nrows
     // each), mode mask (M), units (M), first IR i (M). Standard: rectangles
-    // (8M), begin states and scores (2 nbegin), root row offsets (root rows),
-    // row x ranges (2 nrows).
+    // (HB_RECT ints per state), begin states and scores (2 nbegin), root row
+    // offsets (root rows), row d bands (2 nrows).
     std::vector<int64_t> at(k1 - k0 + 1), rat(k1 - k0);
     int64_t rtotal = 0;
     size_t lds_ints = 0, scan_ints = 0;
@@ -412,15 +398,15 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
       const int64_t nr = trunc ? plan[k].nrows : std::max<int64_t>(plan[k].nrows, 1);
       const int64_t nb = cmk(k)->h.local_begin ? (int64_t)cmk(k)->h.begin_list.size() : 0;
       at[k - k0 + 1] = at[k - k0] + (trunc ? 3 * (int64_t)M + 1 + 3 * nr + M + 2 * M
-                                           : 8 * (int64_t)M + 2 * nb + (int64_t)plan[k].rootoff.size() + 2 * nr);
+                                           : HB_RECT * (int64_t)M + 2 * nb + (int64_t)plan[k].rootoff.size() + 2 * nr);
       rat[k - k0] = rtotal;
       rtotal += rper * plan[k].nroot;
       if (!trunc) {
-        scan_ints = std::max(scan_ints, (size_t)(2 * M + 32));
-        lds_ints = std::max(lds_ints, (size_t)(2 * M + 32 + 8 * M + 2 * nb) + (inside ? FLOGSUM_TBL : 0));
+        scan_ints = std::max(scan_ints, (size_t)hb_scan_ints(M));
+        lds_ints = std::max(lds_ints, (size_t)(hb_scan_ints(M) + HB_RECT * M + 2 * nb) + (inside ? FLOGSUM_TBL : 0));
       }
     }
-    // LDS always holds the per-step scan (2M + 32 ints); the Inside FLogsum
+    // LDS always holds the per-step scan (hb_scan_ints); the Inside FLogsum
     // table, rectangle table and begin list too when they fit (else the
     // kernel reads them from memory)
     constexpr size_t kMaxLdsInts = 156 * 1024 / 4;
@@ -462,7 +448,7 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
         const int nb = cmk(k)->h.local_begin ? (int)cmk(k)->h.begin_list.size() : 0;
         e.rect = base;
         e.nbegin = nb;
-        e.rootoff = base + 8 * M + 2 * nb;
+        e.rootoff = base + HB_RECT * M + 2 * nb;
         e.rowx = e.rootoff + plan[k].rootoff.size();
         e.nsteps = plan[k].nsteps;
       }
@@ -502,7 +488,7 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
             }
           }
           o = std::copy(pk.rootoff.begin(), pk.rootoff.end(), o);
-          rect_rows(h, b, pk, o);
+          rect_rows(b, nr, o);
           return;
         }
         o = std::copy(b.jmin, b.jmin + M, o);

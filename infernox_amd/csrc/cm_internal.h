@@ -87,12 +87,12 @@ struct EnvDev {
   const int32_t *units, *ibase;
   int32_t cmi;  // model of the envelope in a several-model launch (index into its CmDev array)
   // standard scans (wavefront over d), packed contiguously from rect: per
-  // state 8 ints (jlo, nj, xlo, nx, deck base, level * 2 + (1 when x counts
-  // d rather than i), first band row, state type | child count << 4 | first
-  // child << 8); the nbegin local-begin
-  // states and their begin scores (float bits); per root row the offset of
-  // its first cell in rout; per band row the true x range (xa, xb; xa > xb
-  // when empty). nsteps: the wavefront's steps.
+  // state HB_RECT ints (jlo, nj, xlo, nx, deck base, level * 2 + (1 when x
+  // counts j, 0 when it counts i), first band row, state type | child count
+  // << 4 | first child << 8, dlo, nd, 0, 0); the nbegin local-begin states and
+  // their begin scores (float bits); per root row the offset of its first
+  // cell in rout; per band row its d band (hdmin, hdmax). nsteps: the
+  // wavefront's steps.
   const int32_t *rect, *rootoff, *rowx;
   int32_t nbegin, nsteps;
 };
@@ -101,7 +101,12 @@ constexpr int HB_THREADS = 1024;     // lanes per envelope workgroup (standard s
 constexpr int TRHB_THREADS = 512;    // lanes per envelope workgroup (truncated scans)
 constexpr int HB_LDS_STATES = 3200;  // states whose per-state band scalars fit in LDS
 
-// lds_ints: LDS ints per workgroup: the per-step scan (2M + 32) and, when
+constexpr int HB_RECT = 12;  // ints per state of a standard scan's rectangle table
+// LDS ints of a standard scan's per-step prefix sum (pre and jst, M each, and
+// 32 wave sums), rounded up to 4 so that the rectangle table after it is
+// 16-byte aligned for its int4 reads
+__host__ __device__ constexpr int hb_scan_ints(int M) { return (2 * M + 32 + 3) & ~3; }
+// lds_ints: LDS ints per workgroup: the per-step scan (hb_scan_ints) and, when
 // they fit, the Inside FLogsum table, rectangle table and begin list.
 hipError_t launch_hb_scan_queue(const CmDev *cms, const EnvDev *envs, const int32_t *order, int64_t n,
                                 uint32_t *counter, int grid, const float *ftbl, int inside, size_t lds_ints,

@@ -1,22 +1,22 @@
 // gfx950 HMM-banded CYK / Inside scanners (F6/F7), one workgroup of 1024
-// lanes per envelope, as a wavefront over end positions j:
+// lanes per envelope, as a wavefront over the subsequence length d:
 //
-// * Deck: each state's band cells sit in a dense rectangle over (j, x),
-//   x = i - xlo (or d - xlo for states whose band is narrower in d, such as
-//   end states), so a child cell's address needs only the child's rectangle
-//   (in LDS), never a per-row lookup. Rectangle cells outside the true band
-//   hold -inf, the identity of max and of FLogsum, so reading one is the
-//   same as the reference skipping the out-of-band child. Every rectangle
-//   cell with 0 <= d <= the envelope's largest d is written; no read asks
-//   for another d.
-// * Schedule: a wavefront over the subsequence length d. At step t state v
-//   fills its cells of length d = t - level(v) (all j at once), where a
-//   state's level is one more than its children's highest. Every child cell
-//   is then at a smaller d (ML, MR, MP, IL, IR, including the inserts' self
-//   transitions, and a bifurcation's splits with 0 < k < d) or at the same d
-//   in a lower level (S, D, and the k = 0 / k = d splits), so it was
-//   finished at an earlier step. One barrier per step, no serial chain
-//   inside a step except a bifurcation cell's splits.
+// * Deck: each state's cells sit in a rectangle laid out diagonal-major,
+//   (d, x) with x = j - jlo or x = i - ilo, whichever of the state's j and
+//   i band widths is smaller, d over the state's band's d range. The cells
+//   of one d are contiguous, which is what a wavefront step reads and writes;
+//   a child cell's address is arithmetic on the child's rectangle (12 ints
+//   per state, in LDS), never a per-row lookup. Rectangle cells outside the
+//   true band hold -inf, the identity of max and of FLogsum, so reading one
+//   is the same as the reference skipping the out-of-band child; the cells
+//   whose j is outside the state's j band are never read and never written.
+// * Schedule: at step t state v fills its cells of length d = t - level(v)
+//   (all j at once), where a state's level is one more than its children's
+//   highest. Every child cell is then at a smaller d (ML, MR, MP, IL, IR,
+//   including the inserts' self transitions, and a bifurcation's splits
+//   with 0 < k < d) or at the same d in a lower level (S, D, and the k = 0 /
+//   k = d splits), so it was finished at an earlier step. One barrier per
+//   step, no serial chain inside a step except a bifurcation cell's splits.
 // * Lanes: each step the states' cell counts at their d are prefix-summed
 //   in LDS and the step's cells are dealt round-robin over all lanes.
 //
@@ -24,8 +24,8 @@
 // CYK, the FLogsum table for Inside) and a bifurcation's splits in
 // increasing k, the order of the reference's loops (cp9.rs:4788-5253,
 // 5514-5885), so CYK is bit-identical and Inside applies the same float
-// operations in the same order. Local begins are folded into each root cell
-// by MAX as it is computed (cp9.rs:5264-5304, 5908-5937); root cells also go
+// operations in the same order. Local begins are folded into the root by
+// MAX after the wavefront (cp9.rs:5264-5304, 5908-5937); root cells also go
 // to the envelope's compact root output.
 #include "cm_internal.h"
 
@@ -58,27 +58,32 @@ __device__ __forceinline__ float cell_init(const CmDev &cm, int v, int st, int d
   return IMPOSSIBLE;
 }
 
-// A state's rectangle: rows jlo .. jlo+nj-1, columns x = 0 .. nx-1 (x = i -
-// xlo, or d - xlo in d mode), row-major at alpha + base.
-// The last word packs the state type, child count and first child (a
-// bifurcation's children are read from the model instead).
+// A state's rectangle: band rows jlo .. jlo+nj-1, d = dlo .. dlo+nd-1,
+// column x = j - jlo (mode 1) or i - xlo (mode 0) in 0 .. nx-1, cell (d, x)
+// at alpha + base + (d - dlo) * nx + x. The record's word 7 packs the state
+// type, child count and first child (a bifurcation's children are read from
+// the model instead).
 struct Rect {
-  int jlo, nj, xlo, nx, base, lvl, mode, rowbase, st, cn, cf;
+  int jlo, nj, xlo, nx, base, lvl, mode, rowbase, st, cn, cf, dlo, nd;
 };
 __device__ __forceinline__ Rect rect_of(const int32_t *rt, int v) {
-  const int4 a = *reinterpret_cast<const int4 *>(rt + 8 * v);
-  const int4 b = *reinterpret_cast<const int4 *>(rt + 8 * v + 4);
+  const int4 a = *reinterpret_cast<const int4 *>(rt + HB_RECT * v);
+  const int4 b = *reinterpret_cast<const int4 *>(rt + HB_RECT * v + 4);
+  const int2 c = *reinterpret_cast<const int2 *>(rt + HB_RECT * v + 8);
   Rect r;
   r.jlo = a.x, r.nj = a.y, r.xlo = a.z, r.nx = a.w;
   r.base = b.x, r.lvl = b.y >> 1, r.mode = b.y & 1, r.rowbase = b.z;
   r.st = b.w & 15, r.cn = (b.w >> 4) & 15, r.cf = (int)((unsigned)b.w >> 8);
+  r.dlo = c.x, r.nd = c.y;
   return r;
 }
 // Deck index of cell (j, d) of a state, or -1 outside its rectangle.
 __device__ __forceinline__ int rect_cell(const Rect &r, int j, int d) {
-  const int jp = j - r.jlo;
-  const int x = r.mode ? d - r.xlo : (j - d + 1) - r.xlo;
-  return ((unsigned)jp < (unsigned)r.nj && (unsigned)x < (unsigned)r.nx) ? r.base + jp * r.nx + x : -1;
+  const int jp = j - r.jlo, dp = d - r.dlo;
+  const int x = r.mode ? jp : (j - d + 1) - r.xlo;
+  return ((unsigned)jp < (unsigned)r.nj && (unsigned)dp < (unsigned)r.nd && (unsigned)x < (unsigned)r.nx)
+             ? r.base + dp * r.nx + x
+             : -1;
 }
 __device__ __forceinline__ int rect_cell(const int32_t *rt, int y, int j, int d) { return rect_cell(rect_of(rt, y), j, d); }
 
@@ -113,45 +118,40 @@ __device__ __forceinline__ float cell_generic(const CmDev &cm, const EnvDev &e, 
   for (int yo = 0; yo < MAXC; yo++)
     if (ok[yo]) cur = combine<INSIDE>(cur, a[yo] + tsc[yo], ftbl);
   const IXG_GLOBAL float *oe = gmem(cm.oesc) + (size_t)v * (KP * KP);
-  if (st == ML_ST || st == MR_ST || st == IR_ST) cur += oe[xl];
+  if (st == ML_ST || st == MR_ST || st == IL_ST || st == IR_ST) cur += oe[xl];
   else if (st == MP_ST) cur += oe[xl * KP + xr];
   return cur < IMPOSSIBLE ? IMPOSSIBLE : cur;
 }
 
 // One cell of a bifurcation: left child y at (j-k, d-k), right child z at
-// (j, k), k increasing over z's rectangle at row j (terms outside either
-// band read -inf).
+// (j, k), k increasing over z's d range (terms outside either band read
+// -inf, which max and FLogsum ignore exactly as the reference skips them).
 template <bool INSIDE>
 __device__ __forceinline__ float cell_bif(const CmDev &cm, const EnvDev &e, const int32_t *rt, int v, int j, int d,
                                           const float *__restrict__ ftbl) {
   float cur = cell_init(cm, v, B_ST, d);
   const int y = gmem(cm.cfirst)[v], z = gmem(cm.cnum)[v];
   const Rect ry = rect_of(rt, y), rz = rect_of(rt, z);
-  const int jpz = j - rz.jlo;
-  if ((unsigned)jpz >= (unsigned)rz.nj || rz.nx == 0 || ry.nj == 0) return cur;
-  // z's d range at row j, and j - k within y's rows
-  int kn = rz.mode ? rz.xlo : j - (rz.xlo + rz.nx - 1) + 1;
-  int kx = rz.mode ? rz.xlo + rz.nx - 1 : j - rz.xlo + 1;
-  kn = max(kn, max(j - (ry.jlo + ry.nj - 1), 0));
-  kx = min(min(kx, j - ry.jlo), d);  // k > d puts the left child below d = 0, outside every band
+  if ((unsigned)(j - rz.jlo) >= (unsigned)rz.nj || rz.nd == 0 || ry.nj == 0) return cur;
+  const int kn = max(rz.dlo, max(j - (ry.jlo + ry.nj - 1), 0));
+  const int kx = min(min(rz.dlo + rz.nd - 1, j - ry.jlo), d);  // k > d: the left child below d = 0
   const IXG_GLOBAL float *al = gmem(e.alpha);
-  const int zrow = rz.base + jpz * rz.nx;
   constexpr int KB = 8;
   for (int k0 = kn; k0 <= kx; k0 += KB) {
-    // a term whose left cell is outside y's rectangle (or past kx) is -inf,
-    // which max and FLogsum ignore exactly as the reference skips it
-    float a[KB], b[KB];
+    // a term with either cell outside its rectangle is -inf as a whole: the
+    // placeholder load (cell 0, which may not be written yet in this scan)
+    // never enters the sum
+    float s[KB];
 #pragma unroll
     for (int u = 0; u < KB; u++) {
-      const int k = k0 + u, kc = k <= kx ? k : kx;
+      const int k = k0 + u;
       const int cy = k <= kx ? rect_cell(ry, j - k, d - k) : -1;
-      const int xz = rz.mode ? kc - rz.xlo : (j - kc + 1) - rz.xlo;
-      const float av = al[cy >= 0 ? cy : 0];
-      a[u] = cy >= 0 ? av : -INFINITY;
-      b[u] = al[zrow + xz];
+      const int cz = k <= kx ? rect_cell(rz, j, k) : -1;
+      const float av = al[cy >= 0 ? cy : 0], bv = al[cz >= 0 ? cz : 0];
+      s[u] = cy >= 0 && cz >= 0 ? av + bv : -INFINITY;
     }
 #pragma unroll
-    for (int u = 0; u < KB; u++) cur = combine<INSIDE>(cur, a[u] + b[u], ftbl);
+    for (int u = 0; u < KB; u++) cur = combine<INSIDE>(cur, s[u], ftbl);
   }
   return cur;
 }
@@ -162,13 +162,13 @@ template <bool INSIDE>
 __device__ __forceinline__ float hb_cell(const CmDev &cm, const EnvDev &e, const int32_t *rt, const Rect &r, int v,
                                          int j, int d, const float *ftbl, int &idx) {
   const int jp = j - r.jlo;
-  const int x = r.mode ? d - r.xlo : (j - d + 1) - r.xlo;
-  const IXG_GLOBAL int32_t *rx = gmem(e.rowx) + 2 * (r.rowbase + jp);
-  const int xa = rx[0], xb = rx[1];
-  const bool in = x >= xa && x <= xb;
-  idx = r.base + jp * r.nx + x;
+  const int x = r.mode ? jp : (j - d + 1) - r.xlo;
+  const IXG_GLOBAL int32_t *rb = gmem(e.rowx) + 2 * (r.rowbase + jp);
+  const int lo = rb[0], hi = rb[1];  // the row's d band
+  const bool in = d >= lo && d <= hi;
+  idx = r.base + (d - r.dlo) * r.nx + x;
   if (r.st == B_ST) return in ? cell_bif<INSIDE>(cm, e, rt, v, j, d, ftbl) : -INFINITY;
-  const float cur = cell_generic<INSIDE>(cm, e, rt, r, v, j, d, ftbl);  // loads overlap the band range's
+  const float cur = cell_generic<INSIDE>(cm, e, rt, r, v, j, d, ftbl);  // loads overlap the band's
   return in ? cur : -INFINITY;
 }
 
@@ -176,8 +176,8 @@ __device__ __forceinline__ float hb_cell(const CmDev &cm, const EnvDev &e, const
 // state v fills its rectangle cells of length d = t - level(v)). Each step
 // the states' cell counts are prefix-summed in LDS (pre[v], first row
 // jst[v]) and every lane takes every nt-th cell of the step (one cell per
-// lane at a time keeps the most cells in flight). rt / bl / bs:
-// the rectangle table (8 ints per state), the begin states and their begin
+// lane at a time keeps the most cells in flight). rt / bl / bs: the
+// rectangle table (HB_RECT ints per state), the begin states and their begin
 // scores; ftbl: the FLogsum table.
 template <bool INSIDE>
 __device__ __forceinline__ void hb_scan_body(const CmDev &cm, const EnvDev &e, const int32_t *rt, const int32_t *bl,
@@ -186,6 +186,7 @@ __device__ __forceinline__ void hb_scan_body(const CmDev &cm, const EnvDev &e, c
   const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
   const int M = cm.M;
   const int per = (M + nt - 1) / nt, v0 = tid * per;
+  IXG_GLOBAL float *al = gmem(e.alpha);
   for (int t = 0; t < e.nsteps; t++) {
     // this thread's states: first row and cell count at this step
     int my = 0;
@@ -195,9 +196,9 @@ __device__ __forceinline__ void hb_scan_body(const CmDev &cm, const EnvDev &e, c
       const Rect r = rect_of(rt, v);
       const int d = t - r.lvl;
       int n = 0, ja = 0;
-      if (d >= 0 && r.nx > 0) {
+      if ((unsigned)(d - r.dlo) < (unsigned)r.nd) {
         if (r.mode) {
-          if ((unsigned)(d - r.xlo) < (unsigned)r.nx) ja = r.jlo, n = r.nj;
+          ja = r.jlo, n = r.nj;
         } else {
           ja = max(r.jlo, d - 1 + r.xlo);
           n = max(0, min(r.jlo + r.nj - 1, d - 2 + r.xlo + r.nx) - ja + 1);
@@ -229,7 +230,6 @@ __device__ __forceinline__ void hb_scan_body(const CmDev &cm, const EnvDev &e, c
     }
     __syncthreads();
     // the step's cells, every nt-th to a lane
-    IXG_GLOBAL float *al = gmem(e.alpha);
     for (int c = tid; c < total; c += nt) {
       int lo = 0, hi = M - 1;  // the last state whose first cell is at or before c
       while (lo < hi) {
@@ -247,34 +247,35 @@ __device__ __forceinline__ void hb_scan_body(const CmDev &cm, const EnvDev &e, c
   // local begins into the root by MAX (cp9.rs:5264-5304, 5908-5937), begin
   // state by begin state in list order (the reference's strict comparison
   // per cell, so ties resolve as there): all lanes stream through the
-  // state's rectangle rows that overlap the root's rows; band and rectangle
-  // checks on both sides (a cell outside either band contributes nothing)
-  IXG_GLOBAL float *al = gmem(e.alpha);
+  // state's rectangle, a cell outside either band contributing nothing (the
+  // root's own out-of-band cells are reset below)
   const Rect r0 = rect_of(rt, 0);
   for (int b = 0; b < nb; b++) {
     const int y = bl[b];
     const float sc = bs[b];
     const Rect ry = rect_of(rt, y);
-    const int ja = max(ry.jlo, r0.jlo), jb = min(ry.jlo + ry.nj, r0.jlo + r0.nj) - 1;
-    const int n = jb >= ja ? (jb - ja + 1) * ry.nx : 0;
+    const int n = ry.nd * ry.nx;
     for (int c = tid; c < n; c += nt) {
-      const int jr = c / ry.nx, xy = c - jr * ry.nx, j = ja + jr;
-      const int d = ry.mode ? ry.xlo + xy : j - ry.xlo - xy + 1;
+      const int dp = c / ry.nx, x = c - dp * ry.nx, d = ry.dlo + dp;
+      const int j = ry.mode ? ry.jlo + x : x + ry.xlo + d - 1;
+      if ((unsigned)(j - ry.jlo) >= (unsigned)ry.nj) continue;  // never written
       const int c0 = rect_cell(r0, j, d);
       if (c0 < 0) continue;
-      const float a = al[ry.base + (j - ry.jlo) * ry.nx + xy];  // -inf outside y's band
-      const float sv = a + sc;
-      if (sv > al[c0]) al[c0] = sv;  // -inf root cells (outside its band) are rewritten below
+      const float sv = al[ry.base + c] + sc;
+      if (sv > al[c0]) al[c0] = sv;
     }
     __syncthreads();
   }
-  // the root rows out, and -inf back into the root's out-of-band cells
-  const int n0 = r0.nj * r0.nx;
+  // the root rows out (increasing d per row), and -inf back into the root's
+  // out-of-band cells
+  const int n0 = r0.nd * r0.nx;
   for (int c = tid; c < n0; c += nt) {
-    const int jp = c / r0.nx, x = c - jp * r0.nx;
-    const IXG_GLOBAL int32_t *rx = gmem(e.rowx) + 2 * (r0.rowbase + jp);
-    const int xa = rx[0], xb = rx[1];
-    if (x >= xa && x <= xb) gmem(e.rout)[e.rootoff[jp] + (r0.mode ? x - xa : xb - x)] = al[r0.base + c];
+    const int dp = c / r0.nx, x = c - dp * r0.nx, d = r0.dlo + dp;
+    const int j = r0.mode ? r0.jlo + x : x + r0.xlo + d - 1, jp = j - r0.jlo;
+    if ((unsigned)jp >= (unsigned)r0.nj) continue;
+    const IXG_GLOBAL int32_t *rb = gmem(e.rowx) + 2 * (r0.rowbase + jp);
+    const int lo = rb[0], hi = rb[1];
+    if (d >= lo && d <= hi) gmem(e.rout)[e.rootoff[jp] + (d - lo)] = al[r0.base + c];
     else if (nb) al[r0.base + c] = -INFINITY;
   }
 }
@@ -282,28 +283,28 @@ __device__ __forceinline__ void hb_scan_body(const CmDev &cm, const EnvDev &e, c
 // Scans one envelope with its rectangle table and begin list in LDS (RL) or
 // read from memory, and the Inside FLogsum table in LDS (TL) or memory.
 // LDS: [FLogsum table (Inside, TL)] [pre (M), jst (M), wsum (32)]
-// [rectangle table (8M), begin states (nb), begin scores (nb)] (RL); the
+// [rectangle table (HB_RECT * M), begin states (nb), begin scores (nb)] (RL); the
 // choice is a template parameter so that every access compiles to LDS or
 // global instructions.
 template <bool INSIDE, bool RL, bool TL>
 __device__ __forceinline__ void hb_scan_staged(const CmDev &cm, const EnvDev &e, const float *ftbl, int32_t *smem) {
   const int M = cm.M, nb = e.nbegin;
   int32_t *pre = smem + (TL ? FLOGSUM_TBL : 0), *jst = pre + M, *wsum = jst + M;
-  const int32_t *base = RL ? wsum + 32 : (const int32_t *)gmem(e.rect);
-  const int32_t *bl = base + 8 * M;
+  const int32_t *base = RL ? pre + hb_scan_ints(M) : (const int32_t *)gmem(e.rect);
+  const int32_t *bl = base + HB_RECT * M;
   const float *bs = reinterpret_cast<const float *>(bl + nb);
   const float *tb = TL ? reinterpret_cast<const float *>(smem) : (const float *)gmem(ftbl);
   hb_scan_body<INSIDE>(cm, e, base, bl, bs, nb, tb, pre, jst, wsum);
 }
 
 // Stages what fits in LDS and scans the envelope; the host sizes lds_ints
-// for the step scan at least (2M + 32 ints).
+// for the step scan at least (hb_scan_ints(M)).
 template <bool INSIDE>
 __device__ __forceinline__ void hb_scan_env(const CmDev &cm, const EnvDev &e, const float *ftbl, int32_t *smem,
                                             int lds_ints) {
   const int tid = threadIdx.x, nt = blockDim.x;
   const int M = cm.M, nb = e.nbegin;
-  const int scan = 2 * M + 32, need = 8 * M + 2 * nb;
+  const int scan = hb_scan_ints(M), need = HB_RECT * M + 2 * nb;
   const bool rl = scan + need <= lds_ints, tl = INSIDE && rl && FLOGSUM_TBL + scan + need <= lds_ints;
   if (tl)
     for (int q = tid; q < FLOGSUM_TBL; q += nt) smem[q] = __float_as_int(ftbl[q]);

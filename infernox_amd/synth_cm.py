@@ -333,6 +333,16 @@ def hb_cells(b):
     return int(w[w > 0].sum())
 
 
+def skew_inserts(cm, seed, alpha=0.7):
+    """A copy of cm whose IL / IR emissions are drawn from a Dirichlet(alpha)
+    instead of the background, so that insert emission scores are not 0."""
+    rng = np.random.default_rng(seed)
+    e = cm["e"].copy()
+    for v in np.flatnonzero(np.isin(cm["sttype"], (IL_ST, IR_ST))):
+        e[v, :4] = rng.dirichlet(np.full(4, alpha))
+    return dict(cm, e=e.astype(np.float32))
+
+
 def emit(cm, seed):
     """Samples one sequence from the CM from its root (global parse)."""
     r = np.random.default_rng(seed)

@@ -56,6 +56,24 @@ def test_cyk_inside_parity(ctx, ss, seed):
     g.free()
 
 
+@pytest.mark.parametrize("ss,seed", [(S.cloverleaf(), 21), (S.random_nested(90, 22), 22)])
+def test_insert_emissions(ctx, ss, seed):
+    cm, dsq, L, h0, h1 = _case(ss, seed)
+    cm = S.skew_inserts(cm, seed)
+    o = OracleCM(cm)
+    g = ixg.CM(ctx, cm)
+    for r in (3, 12):
+        b = S.bands(cm, h0, h1, r=r, seed=seed + r)
+        for inside in (0, 1):
+            _say("inserts", len(ss), seed, r, inside)
+            want = o.scan(b, dsq, h0, h1, inside, env_cutoff=-5.0)
+            vsc, ei, ej = g.cyk_scan(b, dsq, L, h0, h1, env_cutoff=-5.0) if inside == 0 else \
+                g.scan_hits(b, dsq, L, h0, h1, inside=True, env_cutoff=-5.0, cutoff=-10.0)[:3]
+            assert np.float32(vsc).tobytes() == np.float32(want["vsc"]).tobytes(), (vsc, want["vsc"])
+            assert (ei, ej) == (want["envi"], want["envj"])
+    g.free()
+
+
 def test_wide_bands_small(ctx):
     cm, dsq, L, h0, h1 = _case("<<.>.<..>>", 9, flank=3)
     o = OracleCM(cm)

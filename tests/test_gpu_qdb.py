@@ -42,9 +42,13 @@ STRUCTS = [("<<<..>>>", 1), ("<.>.<.>", 2), (S.cloverleaf(), 4), (S.random_neste
            (S.multibranch(120, 3, 4), 5)]
 
 
-@pytest.mark.parametrize("ss,seed", STRUCTS)
-def test_qdb_parity(ctx, ss, seed):
+# the last case: insert emissions away from the background (non-zero IL / IR
+# emission scores)
+@pytest.mark.parametrize("ss,seed,skew", [c + (False,) for c in STRUCTS] + [(S.random_nested(90, 24), 24, True)])
+def test_qdb_parity(ctx, ss, seed, skew):
     cm = S.build_cm(ss, seed)
+    if skew:
+        cm = S.skew_inserts(cm, seed)
     dsq, L = _seq(cm, seed)
     nh = 0
     for local in (True, False):

@@ -22,12 +22,14 @@ def _pty_index(pass_idx):
     return {S.PASS_5P_ONLY: 1, S.PASS_3P_ONLY: 2}.get(pass_idx, 0)
 
 
-def _truncated_case(ss, seed, cut=0.3, flank=20):
+def _truncated_case(ss, seed, cut=0.3, flank=20, skew=False):
     """A homolog whose 5' part is missing at the start of the sequence and a
     second whose 3' part is missing at the end. Returns the CM, dsq, L and
     the scan cases (i0, j0, consensus columns present c1..c2, validity
     parameters sp1, sp2, ep1, ep2)."""
     cm = S.build_cm(ss, seed)
+    if skew:
+        cm = S.skew_inserts(cm, seed)
     rng = np.random.default_rng(seed)
     h1, h2 = S.emit(cm, seed + 100), S.emit(cm, seed + 200)
     k1, k2 = int(len(h1) * cut), int(len(h2) * cut)
@@ -58,9 +60,11 @@ STRUCTS = [("<<<..>>>", 1), ("<.>.<.>", 2), (S.cloverleaf(), 4), (S.multibranch(
            (S.random_nested(90, 6), 6)]
 
 
-@pytest.mark.parametrize("ss,seed", STRUCTS)
-def test_truncated_parity(ctx, ss, seed):
-    cm, dsq, L, cases = _truncated_case(ss, seed)
+# the last case: insert emissions away from the background (non-zero IL / IR
+# emission scores)
+@pytest.mark.parametrize("ss,seed,skew", [c + (False,) for c in STRUCTS] + [(S.cloverleaf(), 25, True)])
+def test_truncated_parity(ctx, ss, seed, skew):
+    cm, dsq, L, cases = _truncated_case(ss, seed, skew=skew)
     o = OracleTrCM(cm)
     g = ixg.CM(ctx, cm)
     gp, lp = S.tr_penalties(cm, seed)

@@ -34,8 +34,10 @@ def _same(got, want, tag):
     assert np.array_equal(np.float32([h[3] for h in gh]), np.float32([h[3] for h in wh])), tag
 
 
-def _case(ss, seed, cut5, cut3, flank=12):
+def _case(ss, seed, cut5, cut3, flank=12, skew=False):
     cm = S.build_cm(ss, seed)
+    if skew:
+        cm = S.skew_inserts(cm, seed)
     r = np.random.default_rng(seed)
     h = S.emit(cm, seed + 100)
     h = h[cut5:len(h) - cut3] if cut3 else h[cut5:]
@@ -57,9 +59,11 @@ CASES = [("<<<..>>>.<<..>>", 1, 3, 0), (S.cloverleaf(), 3, 12, 0), (S.cloverleaf
          (S.random_nested(80, 6), 6, 20, 0)]
 
 
-@pytest.mark.parametrize("ss,seed,cut5,cut3", CASES)
-def test_trqdb_parity(ctx, ss, seed, cut5, cut3):
-    cm, dsq, L = _case(ss, seed, cut5, cut3)
+# the last case: insert emissions away from the background (non-zero IL / IR
+# emission scores)
+@pytest.mark.parametrize("ss,seed,cut5,cut3,skew", [c + (False,) for c in CASES] + [(S.cloverleaf(), 26, 8, 0, True)])
+def test_trqdb_parity(ctx, ss, seed, cut5, cut3, skew):
+    cm, dsq, L = _case(ss, seed, cut5, cut3, skew=skew)
     lp, rp = S.state_spans(cm)
     dmax = (np.maximum(0, rp - lp + 1) + 6).astype(np.int32)
     nh = 0
