@@ -197,23 +197,35 @@ int rect_plan(const CmHost &h, const ixg_bands &b, EnvPlan &p, const char **err)
     }
     int32_t *R = p.rect.data() + (size_t)HB_RECT * v;
     R[0] = b.jmin[v];
-    R[6] = b.hd_off[v];
+    R[8] = b.hd_off[v];
     // type | child count << 4 | first child << 8 (a bifurcation's fields are
     // not used: the kernel reads its two children from the model)
-    R[7] = (int32_t)((uint32_t)h.sttype[v] | (h.sttype[v] == B_ST ? 0u : (uint32_t)h.cnum[v] << 4 |
+    R[9] = (int32_t)((uint32_t)h.sttype[v] | (h.sttype[v] == B_ST ? 0u : (uint32_t)h.cnum[v] << 4 |
                                                                         (uint32_t)h.cfirst[v] << 8));
     if (dhi < dlo) {  // no band cells: no deck
-      R[5] = h.lvl[v] * 2;
+      R[5] = h.lvl[v] * 4;
       continue;
     }
     const int ni = ihi - ilo + 1, mode = nj <= ni ? 1 : 0, nx = mode ? nj : ni, nd = dhi - dlo + 1;
+    // a box band: every row's d band is its (i, j) box row clipped to
+    // dlo..dhi (what ij2d_bands makes of i and j bands); then a rectangle
+    // cell is a band cell exactly when its i is in ilo..ihi, with no row
+    // lookup
+    bool box = true;
+    for (int jp = 0; jp < nj && box; jp++) {
+      const int r = b.hd_off[v] + jp, j = b.jmin[v] + jp, lo = b.hdmin[r], hi = b.hdmax[r];
+      const int elo = std::max(j - ihi + 1, dlo), ehi = std::min(j - ilo + 1, dhi);
+      const bool emp = hi < lo, eemp = ehi < elo;
+      box = emp == eemp && (emp || (lo == elo && hi == ehi));
+    }
     R[1] = nj;
     R[2] = ilo;
     R[3] = nx;
     R[4] = (int32_t)base;
-    R[5] = h.lvl[v] * 2 + mode;
-    R[8] = dlo;
-    R[9] = nd;
+    R[5] = h.lvl[v] * 4 + (box ? 2 : 0) + mode;
+    R[6] = dlo;
+    R[7] = nd;
+    R[10] = ni;
     base += (int64_t)nd * nx;
     if (base >= INT32_MAX) {
       *err = "bands: deck larger than 2^31 cells";
@@ -402,11 +414,12 @@ int run_envelopes(ixg_ctx *c, const ixg_cm *cm, const uint8_t *d_seq_base, const
       rat[k - k0] = rtotal;
       rtotal += rper * plan[k].nroot;
       if (!trunc) {
-        scan_ints = std::max(scan_ints, (size_t)hb_scan_ints(M));
-        lds_ints = std::max(lds_ints, (size_t)(hb_scan_ints(M) + HB_RECT * M + 2 * nb) + (inside ? FLOGSUM_TBL : 0));
+        const size_t sc = (size_t)hb_scan_ints(M) + hb_seq_ints(env[k].j0 - env[k].i0 + 1);
+        scan_ints = std::max(scan_ints, sc);
+        lds_ints = std::max(lds_ints, sc + HB_RECT * M + 2 * nb + (inside ? FLOGSUM_TBL : 0));
       }
     }
-    // LDS always holds the per-step scan (hb_scan_ints); the Inside FLogsum
+    // LDS always holds the per-step scan and the residues; the Inside FLogsum
     // table, rectangle table and begin list too when they fit (else the
     // kernel reads them from memory)
     constexpr size_t kMaxLdsInts = 156 * 1024 / 4;

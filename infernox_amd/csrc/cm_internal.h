@@ -87,9 +87,10 @@ struct EnvDev {
   const int32_t *units, *ibase;
   int32_t cmi;  // model of the envelope in a several-model launch (index into its CmDev array)
   // standard scans (wavefront over d), packed contiguously from rect: per
-  // state HB_RECT ints (jlo, nj, xlo, nx, deck base, level * 2 + (1 when x
-  // counts j, 0 when it counts i), first band row, state type | child count
-  // << 4 | first child << 8, dlo, nd, 0, 0); the nbegin local-begin states and
+  // state HB_RECT ints (jlo, nj, ilo, nx, deck base, level * 4 + 2 when the
+  // band is its (i, j) box + 1 when x counts j (0: x counts i), dlo, nd,
+  // first band row, state type | child count << 4 | first child << 8, i band
+  // width, 0); the nbegin local-begin states and
   // their begin scores (float bits); per root row the offset of its first
   // cell in rout; per band row its d band (hdmin, hdmax). nsteps: the
   // wavefront's steps.
@@ -106,8 +107,11 @@ constexpr int HB_RECT = 12;  // ints per state of a standard scan's rectangle ta
 // 32 wave sums), rounded up to 4 so that the rectangle table after it is
 // 16-byte aligned for its int4 reads
 __host__ __device__ constexpr int hb_scan_ints(int M) { return (2 * M + 32 + 3) & ~3; }
-// lds_ints: LDS ints per workgroup: the per-step scan (hb_scan_ints) and, when
-// they fit, the Inside FLogsum table, rectangle table and begin list.
+// LDS ints of an envelope's residues i0..j0 (bytes), rounded up to 16 bytes
+__host__ __device__ constexpr int hb_seq_ints(int len) { return ((len + 15) / 16) * 4; }
+// lds_ints: LDS ints per workgroup: the per-step scan (hb_scan_ints), the
+// residues (hb_seq_ints) and, when they fit, the Inside FLogsum table,
+// rectangle table and begin list.
 hipError_t launch_hb_scan_queue(const CmDev *cms, const EnvDev *envs, const int32_t *order, int64_t n,
                                 uint32_t *counter, int grid, const float *ftbl, int inside, size_t lds_ints,
                                 hipStream_t s);

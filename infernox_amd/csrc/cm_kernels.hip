@@ -48,78 +48,102 @@ __device__ __forceinline__ float combine(float cur, float cand, const float *__r
   return near ? mx + tv : mx;
 }
 
+// The EL start value of a cell (branch-free: nothing waits on the endsc load)
 __device__ __forceinline__ float cell_init(const CmDev &cm, int v, int st, int d) {
   const float es = gmem(cm.endsc)[v];
-  if (es > -9.999e35f) {
-    const int sd = sdelta(st);
-    const int dp = d - sd > 0 ? d - sd : 0;
-    return cm.el_selfsc * (float)dp + es;
-  }
-  return IMPOSSIBLE;
+  const int sd = sdelta(st);
+  const int dp = d - sd > 0 ? d - sd : 0;
+  const float el = cm.el_selfsc * (float)dp + es;
+  return es > -9.999e35f ? el : IMPOSSIBLE;
 }
 
 // A state's rectangle: band rows jlo .. jlo+nj-1, d = dlo .. dlo+nd-1,
 // column x = j - jlo (mode 1) or i - xlo (mode 0) in 0 .. nx-1, cell (d, x)
-// at alpha + base + (d - dlo) * nx + x. The record's word 7 packs the state
-// type, child count and first child (a bifurcation's children are read from
-// the model instead).
+// at alpha + base + (d - dlo) * nx + x. Words 0-7 are what a cell address
+// needs (two 16-byte reads); word 9 packs the state type, child count and
+// first child (a bifurcation's children are read from the model instead).
+// box: the band is the (i, j) box i = xlo .. xlo+ni-1 clipped to the
+// rectangle's d range, so a rectangle cell is a band cell exactly when its
+// i is in that range (always true in mode 0).
 struct Rect {
-  int jlo, nj, xlo, nx, base, lvl, mode, rowbase, st, cn, cf, dlo, nd;
+  int jlo, nj, xlo, nx, base, lvl, mode, box, dlo, nd, rowbase, st, cn, cf, ni;
 };
-__device__ __forceinline__ Rect rect_of(const int32_t *rt, int v) {
+__device__ __forceinline__ Rect rect_addr(const int32_t *rt, int v) {
   const int4 a = *reinterpret_cast<const int4 *>(rt + HB_RECT * v);
   const int4 b = *reinterpret_cast<const int4 *>(rt + HB_RECT * v + 4);
-  const int2 c = *reinterpret_cast<const int2 *>(rt + HB_RECT * v + 8);
   Rect r;
   r.jlo = a.x, r.nj = a.y, r.xlo = a.z, r.nx = a.w;
-  r.base = b.x, r.lvl = b.y >> 1, r.mode = b.y & 1, r.rowbase = b.z;
-  r.st = b.w & 15, r.cn = (b.w >> 4) & 15, r.cf = (int)((unsigned)b.w >> 8);
-  r.dlo = c.x, r.nd = c.y;
+  r.base = b.x, r.lvl = b.y >> 2, r.box = (b.y >> 1) & 1, r.mode = b.y & 1, r.dlo = b.z, r.nd = b.w;
+  r.rowbase = 0, r.st = 0, r.cn = 0, r.cf = 0, r.ni = 0;
   return r;
 }
-// Deck index of cell (j, d) of a state, or -1 outside its rectangle.
+__device__ __forceinline__ Rect rect_of(const int32_t *rt, int v) {
+  Rect r = rect_addr(rt, v);
+  const int4 c = *reinterpret_cast<const int4 *>(rt + HB_RECT * v + 8);
+  r.rowbase = c.x;
+  r.st = c.y & 15, r.cn = (c.y >> 4) & 15, r.cf = (int)((unsigned)c.y >> 8);
+  r.ni = c.z;
+  return r;
+}
+// Deck index of cell (j, d) of a state, or -1 outside its rectangle
+// (branch-free).
 __device__ __forceinline__ int rect_cell(const Rect &r, int j, int d) {
   const int jp = j - r.jlo, dp = d - r.dlo;
   const int x = r.mode ? jp : (j - d + 1) - r.xlo;
-  return ((unsigned)jp < (unsigned)r.nj && (unsigned)dp < (unsigned)r.nd && (unsigned)x < (unsigned)r.nx)
-             ? r.base + dp * r.nx + x
-             : -1;
+  const bool in = ((unsigned)jp < (unsigned)r.nj) & ((unsigned)dp < (unsigned)r.nd) & ((unsigned)x < (unsigned)r.nx);
+  return in ? r.base + dp * r.nx + x : -1;
 }
-__device__ __forceinline__ int rect_cell(const int32_t *rt, int y, int j, int d) { return rect_cell(rect_of(rt, y), j, d); }
 
 // One cell of a non-bifurcation state (S, D, ML, MR, MP, IL, IR, E; an
-// insert's self transition reads its own cell at d-1): children fetched
-// first, then combined in child order. Called for every rectangle cell; the
-// caller keeps the value only for band cells (every load stays in bounds).
+// insert's self transition reads its own cell at d-1): every operand is
+// fetched first (the children's rectangles from LDS, then their cells, the
+// transition, end and emission scores from memory), then the children are
+// combined in child order. Called for every rectangle cell; the caller keeps
+// the value only for band cells (every load stays in bounds). sq: the
+// envelope's residues i0..j0.
 template <bool INSIDE>
 __device__ __forceinline__ float cell_generic(const CmDev &cm, const EnvDev &e, const int32_t *rt, const Rect &r,
-                                              int v, int j, int d, const float *__restrict__ ftbl) {
+                                              int v, int j, int d, const uint8_t *sq, const float *__restrict__ ftbl) {
   const int st = r.st;
   if (st == E_ST) return 0.0f;  // an end state's band is d = 0
-  float cur = cell_init(cm, v, st, d);
   const int sd = sdelta(st), sdr = srdelta(st);
-  const int cf = r.cf, cn = r.cn;
-  const IXG_GLOBAL float *tsc = gmem(cm.tsc) + v * MAXC;
-  const IXG_GLOBAL float *al = gmem(e.alpha);
-  float a[MAXC];
-  bool ok[MAXC];
+  const int cf = r.cf, cn = r.cn, last = cm.M - 1;
+  // LDS first: the children's rectangles (a slot past cn reads a valid
+  // record and is masked) and the residues; i and j are clamped into the
+  // envelope (a cell outside the band is computed for nothing, its value
+  // discarded, and may reach past the envelope)
+  Rect ry[MAXC];
+#pragma unroll
+  for (int yo = 0; yo < MAXC; yo++) ry[yo] = rect_addr(rt, min(cf + yo, last));
+  const int i = min(max(j - d + 1, e.i0), e.j0), jc = min(max(j, e.i0), e.j0);
+  const int xl = sq[(st == MR_ST || st == IR_ST ? jc : i) - e.i0], xr = sq[jc - e.i0];
+  __builtin_amdgcn_sched_barrier(0);
+  // then every memory operand, all in flight together
+  int c[MAXC];
 #pragma unroll
   for (int yo = 0; yo < MAXC; yo++) {
-    const int c = yo < cn ? rect_cell(rt, cf + yo, j - sdr, d - sd) : -1;
-    ok[yo] = c >= 0;
-    a[yo] = al[ok[yo] ? c : 0];
+    const int cc = rect_cell(ry[yo], j - sdr, d - sd);
+    c[yo] = yo < cn ? cc : -1;
   }
-  const IXG_GLOBAL uint8_t *dsq = gmem(e.dsq);
-  // i clamped into the envelope: a cell outside the band is computed for
-  // nothing (its value is discarded) and may have i < 1
-  const int i = max(j - d + 1, e.i0);
-  const int xl = dsq[st == MR_ST || st == IR_ST ? j : i], xr = dsq[j];
+  const IXG_GLOBAL float *al = gmem(e.alpha);
+  const IXG_GLOBAL float *tsc = gmem(cm.tsc) + v * MAXC;
+  float a[MAXC], t[MAXC];
+#pragma unroll
+  for (int yo = 0; yo < MAXC; yo++) {
+    a[yo] = al[c[yo] >= 0 ? c[yo] : 0];
+    t[yo] = tsc[yo];
+  }
+  const float es = gmem(cm.endsc)[v];
+  const float em = gmem(cm.oesc)[(size_t)v * (KP * KP) + (st == MP_ST ? xl * KP + xr : xl)];
+  __builtin_amdgcn_sched_barrier(0);
+  // EL start (cell_init, on the loaded end score), then the children in order
+  const int dpe = d - sd > 0 ? d - sd : 0;
+  const float el = cm.el_selfsc * (float)dpe + es;
+  float cur = es > -9.999e35f ? el : IMPOSSIBLE;
 #pragma unroll
   for (int yo = 0; yo < MAXC; yo++)
-    if (ok[yo]) cur = combine<INSIDE>(cur, a[yo] + tsc[yo], ftbl);
-  const IXG_GLOBAL float *oe = gmem(cm.oesc) + (size_t)v * (KP * KP);
-  if (st == ML_ST || st == MR_ST || st == IL_ST || st == IR_ST) cur += oe[xl];
-  else if (st == MP_ST) cur += oe[xl * KP + xr];
+    if (c[yo] >= 0) cur = combine<INSIDE>(cur, a[yo] + t[yo], ftbl);
+  if (st == ML_ST || st == MR_ST || st == IL_ST || st == IR_ST || st == MP_ST) cur += em;
   return cur < IMPOSSIBLE ? IMPOSSIBLE : cur;
 }
 
@@ -131,7 +155,7 @@ __device__ __forceinline__ float cell_bif(const CmDev &cm, const EnvDev &e, cons
                                           const float *__restrict__ ftbl) {
   float cur = cell_init(cm, v, B_ST, d);
   const int y = gmem(cm.cfirst)[v], z = gmem(cm.cnum)[v];
-  const Rect ry = rect_of(rt, y), rz = rect_of(rt, z);
+  const Rect ry = rect_addr(rt, y), rz = rect_addr(rt, z);
   if ((unsigned)(j - rz.jlo) >= (unsigned)rz.nj || rz.nd == 0 || ry.nj == 0) return cur;
   const int kn = max(rz.dlo, max(j - (ry.jlo + ry.nj - 1), 0));
   const int kx = min(min(rz.dlo + rz.nd - 1, j - ry.jlo), d);  // k > d: the left child below d = 0
@@ -141,17 +165,20 @@ __device__ __forceinline__ float cell_bif(const CmDev &cm, const EnvDev &e, cons
     // a term with either cell outside its rectangle is -inf as a whole: the
     // placeholder load (cell 0, which may not be written yet in this scan)
     // never enters the sum
-    float s[KB];
+    // the batch's 2 KB loads all issued before any is used
+    float av[KB], bv[KB];
+    bool ok[KB];
 #pragma unroll
     for (int u = 0; u < KB; u++) {
       const int k = k0 + u;
-      const int cy = k <= kx ? rect_cell(ry, j - k, d - k) : -1;
-      const int cz = k <= kx ? rect_cell(rz, j, k) : -1;
-      const float av = al[cy >= 0 ? cy : 0], bv = al[cz >= 0 ? cz : 0];
-      s[u] = cy >= 0 && cz >= 0 ? av + bv : -INFINITY;
+      const int cy = rect_cell(ry, j - k, d - k), cz = rect_cell(rz, j, k);
+      ok[u] = (k <= kx) & (cy >= 0) & (cz >= 0);
+      av[u] = al[ok[u] ? cy : 0];
+      bv[u] = al[ok[u] ? cz : 0];
     }
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int u = 0; u < KB; u++) cur = combine<INSIDE>(cur, s[u], ftbl);
+    for (int u = 0; u < KB; u++) cur = combine<INSIDE>(cur, ok[u] ? av[u] + bv[u] : -INFINITY, ftbl);
   }
   return cur;
 }
@@ -160,15 +187,19 @@ __device__ __forceinline__ float cell_bif(const CmDev &cm, const EnvDev &e, cons
 // deck index.
 template <bool INSIDE>
 __device__ __forceinline__ float hb_cell(const CmDev &cm, const EnvDev &e, const int32_t *rt, const Rect &r, int v,
-                                         int j, int d, const float *ftbl, int &idx) {
+                                         int j, int d, const uint8_t *sq, const float *ftbl, int &idx) {
   const int jp = j - r.jlo;
   const int x = r.mode ? jp : (j - d + 1) - r.xlo;
-  const IXG_GLOBAL int32_t *rb = gmem(e.rowx) + 2 * (r.rowbase + jp);
-  const int lo = rb[0], hi = rb[1];  // the row's d band
-  const bool in = d >= lo && d <= hi;
+  bool in;
+  if (r.box) {
+    in = r.mode ? (unsigned)((j - d + 1) - r.xlo) < (unsigned)r.ni : true;
+  } else {
+    const IXG_GLOBAL int32_t *rb = gmem(e.rowx) + 2 * (r.rowbase + jp);
+    in = d >= rb[0] && d <= rb[1];  // the row's d band
+  }
   idx = r.base + (d - r.dlo) * r.nx + x;
   if (r.st == B_ST) return in ? cell_bif<INSIDE>(cm, e, rt, v, j, d, ftbl) : -INFINITY;
-  const float cur = cell_generic<INSIDE>(cm, e, rt, r, v, j, d, ftbl);  // loads overlap the band's
+  const float cur = cell_generic<INSIDE>(cm, e, rt, r, v, j, d, sq, ftbl);  // loads overlap the band's
   return in ? cur : -INFINITY;
 }
 
@@ -182,7 +213,7 @@ __device__ __forceinline__ float hb_cell(const CmDev &cm, const EnvDev &e, const
 template <bool INSIDE>
 __device__ __forceinline__ void hb_scan_body(const CmDev &cm, const EnvDev &e, const int32_t *rt, const int32_t *bl,
                                              const float *bs, int nb, const float *ftbl, int32_t *pre, int32_t *jst,
-                                             int32_t *wsum) {
+                                             int32_t *wsum, const uint8_t *sq) {
   const int tid = threadIdx.x, nt = blockDim.x, lane = tid & 63, wv = tid >> 6, nw = nt >> 6;
   const int M = cm.M;
   const int per = (M + nt - 1) / nt, v0 = tid * per;
@@ -193,7 +224,7 @@ __device__ __forceinline__ void hb_scan_body(const CmDev &cm, const EnvDev &e, c
     for (int q = 0; q < per; q++) {
       const int v = v0 + q;
       if (v >= M) break;
-      const Rect r = rect_of(rt, v);
+      const Rect r = rect_addr(rt, v);
       const int d = t - r.lvl;
       int n = 0, ja = 0;
       if ((unsigned)(d - r.dlo) < (unsigned)r.nd) {
@@ -239,7 +270,7 @@ __device__ __forceinline__ void hb_scan_body(const CmDev &cm, const EnvDev &e, c
       }
       const Rect r = rect_of(rt, lo);
       int ix;
-      const float val = hb_cell<INSIDE>(cm, e, rt, r, lo, jst[lo] + (c - pre[lo]), t - r.lvl, ftbl, ix);
+      const float val = hb_cell<INSIDE>(cm, e, rt, r, lo, jst[lo] + (c - pre[lo]), t - r.lvl, sq, ftbl, ix);
       al[ix] = val;
     }
     __syncthreads();
@@ -253,7 +284,7 @@ __device__ __forceinline__ void hb_scan_body(const CmDev &cm, const EnvDev &e, c
   for (int b = 0; b < nb; b++) {
     const int y = bl[b];
     const float sc = bs[b];
-    const Rect ry = rect_of(rt, y);
+    const Rect ry = rect_addr(rt, y);
     const int n = ry.nd * ry.nx;
     for (int c = tid; c < n; c += nt) {
       const int dp = c / ry.nx, x = c - dp * ry.nx, d = ry.dlo + dp;
@@ -282,32 +313,38 @@ __device__ __forceinline__ void hb_scan_body(const CmDev &cm, const EnvDev &e, c
 
 // Scans one envelope with its rectangle table and begin list in LDS (RL) or
 // read from memory, and the Inside FLogsum table in LDS (TL) or memory.
-// LDS: [FLogsum table (Inside, TL)] [pre (M), jst (M), wsum (32)]
-// [rectangle table (HB_RECT * M), begin states (nb), begin scores (nb)] (RL); the
-// choice is a template parameter so that every access compiles to LDS or
-// global instructions.
+// LDS: [FLogsum table (Inside, TL)] [pre (M), jst (M), wsum (32), padded]
+// [residues i0..j0, padded] [rectangle table (HB_RECT * M), begin states
+// (nb), begin scores (nb)] (RL); the choice is a template parameter so that
+// every access compiles to LDS or global instructions.
 template <bool INSIDE, bool RL, bool TL>
 __device__ __forceinline__ void hb_scan_staged(const CmDev &cm, const EnvDev &e, const float *ftbl, int32_t *smem) {
   const int M = cm.M, nb = e.nbegin;
   int32_t *pre = smem + (TL ? FLOGSUM_TBL : 0), *jst = pre + M, *wsum = jst + M;
-  const int32_t *base = RL ? pre + hb_scan_ints(M) : (const int32_t *)gmem(e.rect);
+  int32_t *sqi = pre + hb_scan_ints(M);
+  const int32_t *base = RL ? sqi + hb_seq_ints(e.j0 - e.i0 + 1) : (const int32_t *)gmem(e.rect);
   const int32_t *bl = base + HB_RECT * M;
   const float *bs = reinterpret_cast<const float *>(bl + nb);
   const float *tb = TL ? reinterpret_cast<const float *>(smem) : (const float *)gmem(ftbl);
-  hb_scan_body<INSIDE>(cm, e, base, bl, bs, nb, tb, pre, jst, wsum);
+  hb_scan_body<INSIDE>(cm, e, base, bl, bs, nb, tb, pre, jst, wsum, reinterpret_cast<const uint8_t *>(sqi));
 }
 
 // Stages what fits in LDS and scans the envelope; the host sizes lds_ints
-// for the step scan at least (hb_scan_ints(M)).
+// for the step scan and the residues at least (hb_scan_ints(M) +
+// hb_seq_ints(j0 - i0 + 1)).
 template <bool INSIDE>
 __device__ __forceinline__ void hb_scan_env(const CmDev &cm, const EnvDev &e, const float *ftbl, int32_t *smem,
                                             int lds_ints) {
   const int tid = threadIdx.x, nt = blockDim.x;
-  const int M = cm.M, nb = e.nbegin;
-  const int scan = hb_scan_ints(M), need = HB_RECT * M + 2 * nb;
+  const int M = cm.M, nb = e.nbegin, len = e.j0 - e.i0 + 1;
+  const int scan = hb_scan_ints(M) + hb_seq_ints(len), need = HB_RECT * M + 2 * nb;
   const bool rl = scan + need <= lds_ints, tl = INSIDE && rl && FLOGSUM_TBL + scan + need <= lds_ints;
   if (tl)
     for (int q = tid; q < FLOGSUM_TBL; q += nt) smem[q] = __float_as_int(ftbl[q]);
+  int32_t *sqi = smem + (tl ? FLOGSUM_TBL : 0) + hb_scan_ints(M);
+  uint8_t *sq = reinterpret_cast<uint8_t *>(sqi);
+  const IXG_GLOBAL uint8_t *dsq = gmem(e.dsq);
+  for (int q = tid; q < len; q += nt) sq[q] = dsq[e.i0 + q];
   if (rl) {
     int32_t *at = smem + (tl ? FLOGSUM_TBL : 0) + scan;
     for (int q = tid; q < need; q += nt) at[q] = e.rect[q];
@@ -318,11 +355,15 @@ __device__ __forceinline__ void hb_scan_env(const CmDev &cm, const EnvDev &e, co
   else hb_scan_staged<INSIDE, false, false>(cm, e, ftbl, smem);
 }
 
+// amdgpu_waves_per_eu: a workgroup of HB_THREADS lanes fills a CU at 4 waves
+// per SIMD; telling the compiler so lets its scheduler keep loads in flight
+// instead of trading them for an occupancy the launch cannot use.
 template <bool INSIDE>
-__global__ __launch_bounds__(HB_THREADS) void hb_scan_kernel(CmDev cm, const EnvDev *envs, const float *ftbl,
+__global__ __launch_bounds__(HB_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4))) void hb_scan_kernel(CmDev cm, const EnvDev *envs, const float *ftbl,
                                                              int lds_ints) {
   extern __shared__ int32_t smem[];
-  hb_scan_env<INSIDE>(cm, envs[blockIdx.x], ftbl, smem, lds_ints);
+  const EnvDev e = envs[blockIdx.x];  // by value: its fields stay in registers
+  hb_scan_env<INSIDE>(cm, e, ftbl, smem, lds_ints);
 }
 
 // Persistent form over envelopes of several models: each workgroup claims
@@ -330,7 +371,7 @@ __global__ __launch_bounds__(HB_THREADS) void hb_scan_kernel(CmDev cm, const Env
 // the list is exhausted; an envelope's model is cms[e.cmi]. Every workgroup
 // leaves once a claim passes n, so the grid drains whatever its size.
 template <bool INSIDE>
-__global__ __launch_bounds__(HB_THREADS) void hb_scan_queue_kernel(const CmDev *__restrict__ cms,
+__global__ __launch_bounds__(HB_THREADS) __attribute__((amdgpu_waves_per_eu(4, 4))) void hb_scan_queue_kernel(const CmDev *__restrict__ cms,
                                                                    const EnvDev *__restrict__ envs,
                                                                    const int32_t *__restrict__ order, int32_t n,
                                                                    uint32_t *counter, const float *ftbl,
