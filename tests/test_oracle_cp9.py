@@ -10,8 +10,7 @@ cp9_backward, :1517 cp9_fb2hmmbands) without running the reference:
   end transition after the current residue's emission (cp9.rs:2130-2132)
   where Forward ends after the emission, so the two scan totals differ by
   design; the scan-only terms (:1358, :2095-2104, :2130-2132, :2171-2173)
-  are transcribed and their parity is unpinned beyond the cell-wise check
-  below;
+  are pinned against explicit path sums in test_oracle_cp9_paths.py;
 - the oracle's integer matrices agree with the double ones cell by cell
   within the table's rounding (the oracle is then the same recurrence);
 - in alignment mode without local ends every residue is emitted by exactly
