@@ -276,6 +276,13 @@ enum { IXG_TRMODE_T = 0, IXG_TRMODE_R = 1, IXG_TRMODE_L = 2, IXG_TRMODE_J = 3, I
  * Must be set once before the truncated scanners run. The marginal emission
  * scores (cp9.rs:4441 fill_marginal_emissions) are built by ixg_cm_create. */
 int ixg_cm_set_trpenalties(ixg_ctx *ctx, ixg_cm *cm, const float *g, const float *l);
+/* Host builder of those penalties from the probability CM, replacing
+ * cm_trunc.rs:192 TrPenalties::new(&cm_global, &emap, &psi) as
+ * cm_search.rs:927-930 calls it (psi: cm_expected_state_occupancy,
+ * cp9.rs:752, of the un-localized CM): g, l [3][M] for
+ * ixg_cm_set_trpenalties and ig, il [3][M] (Prob2Score, -987654321 where
+ * not allowed) for ixg_cm_set_itrpenalties. */
+int ixg_cm_trpenalties(const ixg_cm_params *p, float *g, float *l, int32_t *ig, int32_t *il);
 /* Host copies: lmesc/rmesc M*18; lpos/rpos per state: the consensus span of
  * the state's subtree (the emit map, cp9.rs:144-204, with the per-node-type
  * offsets of cm_dpsearch_trunc.rs:2394-2403). Any may be NULL. */

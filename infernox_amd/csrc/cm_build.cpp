@@ -359,7 +359,151 @@ int state_spans(const ixg_cm_params &p, CmHost &h) {
   return IXG_OK;
 }
 
+// cp9.rs:752-837 cm_expected_state_occupancy of the un-localized CM: each
+// state's expected visits, summed over its parents in increasing order (an
+// insert's own self transition taken out and applied as a geometric factor
+// afterwards); a parent x reaches child v with t[x][v - cfirst[x]].
+void expected_occupancy(const ixg_cm_params &p, std::vector<double> &psi) {
+  const int M = p.M;
+  psi.assign(M, 0.0);
+  // parents: every non-B, non-E state lists its children as a run
+  std::vector<std::vector<int32_t>> par(M);
+  for (int x = 0; x < M; x++) {
+    const int st = p.sttype[x];
+    if (st == B_ST || st == E_ST) continue;
+    for (int y = p.cfirst[x]; y < p.cfirst[x] + p.cnum[x]; y++)
+      if (y != x) par[y].push_back(x);
+  }
+  for (int v = 0; v < M; v++) {
+    const int st = p.sttype[v];
+    if (st == S_ST) {
+      psi[v] = 1.0;
+      continue;
+    }
+    double a = 0.0;
+    for (int32_t x : par[v]) a += psi[x] * (double)p.t[(size_t)x * 6 + (v - p.cfirst[x])];
+    psi[v] = a;
+    if (st == IL_ST || st == IR_ST) {
+      const double ts = (double)p.t[(size_t)v * 6];
+      psi[v] += psi[v] * (ts / (1.0 - ts));
+    }
+  }
+}
+
+// cm_trunc.rs:164-182: log2 as ln x 1.44269504 and Prob2Score over it (a
+// float-to-int conversion saturates there)
+static float trunc_log2(float x) { return x > 0.0f ? (float)(std::log((double)x) * 1.44269504) : -1e36f; }
+static int32_t trunc_prob2score(float x) {
+  if (x == 0.0f) return -INFTY_I;
+  const double r = std::floor(0.5 + 1000.0 * (x > 0.0f ? std::log((double)x) * 1.44269504 : -1e36));
+  return r < (double)INT32_MIN ? INT32_MIN : (r > (double)INT32_MAX ? INT32_MAX : (int32_t)r);
+}
+
+// cm_trunc.rs:192-384 TrPenalties::new on the global CM: per node, the
+// truncated-begin probability of a MATP/MATL/MATR/BIF node (global: uniform
+// over the 5'&3' fragments, or over the 5'/3' fragments when the node's span
+// reaches the 3'/5' end; local: the node's local begin probability over its
+// subtree's fragments plus the running value carried down MATL / MATR
+// chains) is split between the node's first state and the inserts before it
+// by their occupancies, then taken to log2 and to integer scores.
+int tr_penalties(const ixg_cm_params &p, float *g, float *l, int32_t *ig, int32_t *il) {
+  const int M = p.M, n = p.nodes, clen = p.clen;
+  CmHost h;
+  RCHK(state_spans(p, h));
+  std::vector<double> psi;
+  expected_occupancy(p, psi);
+  std::vector<int32_t> ndidx(M);
+  for (int nd = 0, v = 0; v < M; v++) {
+    while (nd + 1 < n && p.nodemap[nd + 1] <= v) nd++;
+    ndidx[v] = nd;
+  }
+  for (int q = 0; q < 3 * M; q++) g[q] = l[q] = IMPOSSIBLE, ig[q] = il[q] = -INFTY_I;
+  const float g53 = (float)(2.0 / (double)(clen * (clen + 1))), g5or3 = (float)(1.0 / (double)clen);
+  auto starts = [&](int nd) {
+    const int t = p.ndtype[nd];
+    return t == MATP_ND || t == MATL_ND || t == MATR_ND || t == BIF_ND;
+  };
+  std::vector<float> begin(M, 0.0f);  // local begin probabilities (cm_modelconfig.c:397)
+  int nstarts = 0;
+  for (int nd = 2; nd < n; nd++) nstarts += starts(nd);
+  if (n > 1) begin[p.nodemap[1]] = (float)(1.0 - (double)p.pbegin);
+  const float pb = p.pbegin / (float)nstarts;
+  for (int nd = 2; nd < n; nd++)
+    if (starts(nd)) begin[p.nodemap[nd]] = pb;
+  float prv5 = 0.f, prv3 = 0.f, prv53 = 0.f;
+  for (int nd = 0; nd < n; nd++) {
+    const int t = p.ndtype[nd], m = p.nodemap[nd];
+    if (t == END_ND) {
+      prv5 = prv3 = prv53 = 0.f;
+    } else if (t == BEGL_ND || t == BEGR_ND) {
+      int b = -1;  // the bifurcation whose child this S state is
+      for (int x = 0; x < m; x++)
+        if (p.sttype[x] == B_ST && (p.cfirst[x] == m || p.cnum[x] == m)) b = x;
+      if (b < 0) {
+        g_err = "ixg_cm_trpenalties: a BEGL/BEGR node without its bifurcation";
+        return IXG_EINVAL;
+      }
+      prv5 = t == BEGL_ND ? 0.f : l[1 * M + b];
+      prv3 = t == BEGR_ND ? 0.f : l[2 * M + b];
+      prv53 = l[0 * M + b];
+    } else if (starts(nd)) {
+      int i1 = -1, i2 = -1;  // inserts_given_node(nd - 1), cm_trunc.rs:111
+      if (nd >= 1) {
+        const int u = p.nodemap[nd - 1], tu = p.ndtype[nd - 1];
+        if (tu == MATP_ND) i1 = u + 4, i2 = u + 5;
+        else if (tu == MATL_ND || tu == MATR_ND) i1 = u + 2;
+        else if (tu == BEGR_ND) i1 = u + 1;
+        else if (tu == ROOT_ND) i1 = u + 1, i2 = u + 2;
+      }
+      const float mp = (float)psi[m], p1 = i1 >= 0 ? (float)psi[i1] : 0.f, p2 = i2 >= 0 ? (float)psi[i2] : 0.f;
+      const float sum = mp + p1 + p2;
+      auto put = [&](float *a, int q, float prob) {
+        a[q * M + m] = (mp / sum) * prob;
+        if (i1 >= 0) a[q * M + i1] = (p1 / sum) * prob;
+        if (i2 >= 0) a[q * M + i2] = (p2 / sum) * prob;
+      };
+      const int lp = h.lpos[m], rp = h.rpos[m];
+      put(g, 0, g53);
+      if (rp == clen) put(g, 1, g5or3);
+      if (lp == 1) put(g, 2, g5or3);
+      const int sub = rp - lp + 1, nf53 = (sub * (sub + 1)) / 2;
+      const float c5 = begin[m] / (float)sub + prv5, c3 = begin[m] / (float)sub + prv3;
+      const float c53 = begin[m] / (float)nf53 + prv53;
+      put(l, 0, c53);
+      put(l, 1, c5);
+      put(l, 2, c3);
+      prv5 = t == MATL_ND ? c5 : 0.f;
+      prv3 = t == MATR_ND ? c3 : 0.f;
+      prv53 = c53;
+    }
+  }
+  for (int v = 0; v < M; v++) {
+    const int st = p.sttype[v], nt = p.ndtype[ndidx[v]];
+    const bool detached = v + 1 < M && p.sttype[v + 1] == E_ST;
+    const bool q = st == MP_ST || (st == ML_ST && nt == MATL_ND) || (st == MR_ST && nt == MATR_ND) || st == B_ST ||
+                   ((st == IL_ST || st == IR_ST) && !detached);
+    if (!q) continue;
+    if (st == IL_ST && nt == MATP_ND && ndidx[v] + 1 < n && p.ndtype[ndidx[v] + 1] == END_ND) continue;
+    for (int k = 0; k < 3; k++) {
+      float &gv = g[k * M + v], &lv = l[k * M + v];
+      if (gv > -0.5e36f) ig[k * M + v] = trunc_prob2score(gv), gv = trunc_log2(gv);
+      il[k * M + v] = trunc_prob2score(lv);
+      lv = trunc_log2(lv);
+    }
+  }
+  return IXG_OK;
+}
+
 }  // namespace ixg
+
+extern "C" int ixg_cm_trpenalties(const ixg_cm_params *p, float *g, float *l, int32_t *ig, int32_t *il) {
+  if (!p || !g || !l || !ig || !il || p->M < 1 || p->nodes < 2 || p->clen < 1 || !p->sttype || !p->ndtype ||
+      !p->nodemap || !p->cfirst || !p->cnum || !p->t) {
+    ixg::g_err = "ixg_cm_trpenalties: bad arguments";
+    return IXG_EINVAL;
+  }
+  return ixg::tr_penalties(*p, g, l, ig, il);
+}
 
 namespace ixg {
 

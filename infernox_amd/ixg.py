@@ -525,6 +525,19 @@ def cm_params(cm):
     return s
 
 
+def cm_trpenalties(cm):
+    """TrPenalties::new of the CM (ixg_cm_trpenalties, host only): (g, l,
+    ig, il), each [3, M]."""
+    p = cm_params(cm)
+    M = int(cm["M"])
+    g, l = np.zeros((3, M), np.float32), np.zeros((3, M), np.float32)
+    ig, il = np.zeros((3, M), np.int32), np.zeros((3, M), np.int32)
+    i32p = C.POINTER(C.c_int32)
+    check(lib().ixg_cm_trpenalties(C.byref(p), f32ptr(g), f32ptr(l), ig.ctypes.data_as(i32p), il.ctypes.data_as(i32p)),
+          "ixg_cm_trpenalties")
+    return g, l, ig, il
+
+
 def bands_struct(b):
     keep = {k: np.ascontiguousarray(b[k], np.int32) for k in ("jmin", "jmax", "hd_off", "hdmin", "hdmax")}
     s = Bands()

@@ -385,3 +385,31 @@ void ixo_cp9_fb2hmmbands(const ixo_cp9 *h, const uint8_t *dsq, const int32_t *fm
   free(xi);
   free(xd);
 }
+
+/* cp9.rs:1765-1776 / :1805-1814 (inside cp9_predict_start_and_end_positions):
+ * the posterior occupancy of every node, occ[k] = ILogsum over rows ip =
+ * 0..L of ILogsum(pmx_m[ip][k], pmx_d[ip][k]), the posterior cells as
+ * cp9_fb2hmmbands (cp9.rs:1583-1640) defines them; occ[0] = -INFTY. */
+void ixo_cp9_occupancy(const ixo_cp9 *h, const uint8_t *dsq, const int32_t *fm, const int32_t *fd,
+                       const int32_t *bm, const int32_t *bd, int i0, int j0, int did_fwd_scan, int32_t *occ) {
+  const int M = h->M, L = j0 - i0 + 1;
+  const int32_t ninf = -INFTY;
+  int32_t sc;
+  if (did_fwd_scan) {
+    sc = ninf;
+    for (int ip = 0; ip <= L; ip++) sc = lsum(sc, AT(bm, ip, 0));
+  } else {
+    sc = AT(bm, 0, 0);
+  }
+  occ[0] = ninf;
+  for (int k = 1; k <= M; k++) {
+    int32_t o = lsum(ninf, lsum(ninf, ws(wa(AT(fd, 0, k), AT(bd, 0, k)), sc)));
+    for (int ip = 1; ip <= L; ip++) {
+      const int x = dsq[i0 + ip - 1];
+      const int32_t m = imax(ws(ws(wa(AT(fm, ip, k), AT(bm, ip, k)), MSC(k, x)), sc), ninf);
+      const int32_t d = imax(ws(wa(AT(fd, ip, k), AT(bd, ip, k)), sc), ninf);
+      o = lsum(o, lsum(m, d));
+    }
+    occ[k] = o;
+  }
+}

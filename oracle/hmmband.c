@@ -7,6 +7,7 @@
  * (:3815) and cm_hb_mx_size_needed (:3854). Not linked into the product.
  */
 #include <limits.h>
+#include <math.h>
 #include <stdlib.h>
 #include <string.h>
 #include "ixo_band.h"
@@ -331,9 +332,10 @@ static int state_left_delta(int st) { return st == MP_ST || st == ML_ST || st ==
 static int state_right_delta(int st) { return st == MP_ST || st == MR_ST || st == IR_ST; }
 static int state_delta(int st) { return st == MP_ST ? 2 : (st == ML_ST || st == MR_ST || st == IL_ST || st == IR_ST); }
 
-/* cp9.rs:3380-3794 cp9_hmm2ijbands, do_trunc = FALSE */
-void ixo_hmm2ij_bands(const ixo_hbcm *cm, const ixo_hbcp9 *h, int32_t *pn, int i0, int j0, int doing_search,
-                      int32_t *imin, int32_t *imax, int32_t *jmin, int32_t *jmax) {
+/* cp9.rs:3380-3794 cp9_hmm2ijbands; tb == NULL: do_trunc = FALSE */
+void ixo_hmm2ij_bands_tr(const ixo_hbcm *cm, const ixo_hbcp9 *h, int32_t *pn, int i0, int j0, int doing_search,
+                         ixo_trband *tb, int32_t *imin, int32_t *imax, int32_t *jmin, int32_t *jmax) {
+  const int do_trunc = tb != NULL;
   const int M = cm->M, hmm_m = h->hmm_m;
   const int hmm_localized = h->local_begin || h->local_end || h->el;
   const int cm_localized = cm->local_begin && cm->local_end;
@@ -365,7 +367,8 @@ void ixo_hmm2ij_bands(const ixo_hbcm *cm, const ixo_hbcp9 *h, int32_t *pn, int i
         imax[v] = imax[w] != -2 ? imax[w] : imax[y];
         jmin[v] = jmin[y] != -1 ? jmin[y] : jmin[w];
         jmax[v] = jmax[y] != -2 ? jmax[y] : jmax[w];
-        if (imin[w] == -1 || jmin[w] == -1 || imin[y] == -1 || jmin[y] == -1 || imax[w] == -2 || jmax[y] == -2) {
+        if (!do_trunc &&
+            (imin[w] == -1 || jmin[w] == -1 || imin[y] == -1 || jmin[y] == -1 || imax[w] == -2 || jmax[y] == -2)) {
           imin[v] = imin[w] = imin[y] = -1;
           jmin[v] = jmin[w] = jmin[y] = -1;
           imax[v] = imax[w] = imax[y] = -2;
@@ -499,6 +502,17 @@ void ixo_hmm2ij_bands(const ixo_hbcm *cm, const ixo_hbcp9 *h, int32_t *pn, int i
       }
     }
   }
+  if (do_trunc) /* cp9.rs:3646-3661: L / R / T marginal alignments widen the bands */
+    for (int v = 0; v < M; v++) {
+      if (tb->lvalid[v] || tb->tvalid[v]) {
+        jmin[v] = jmin[v] == -1 ? tb->lmarg_jmin : MIN(jmin[v], tb->lmarg_jmin);
+        jmax[v] = jmax[v] == -2 ? tb->lmarg_jmax : MAX(jmax[v], tb->lmarg_jmax);
+      }
+      if (tb->rvalid[v] || tb->tvalid[v]) {
+        imin[v] = imin[v] == -1 ? tb->rmarg_imin : MIN(imin[v], tb->rmarg_imin);
+        imax[v] = imax[v] == -2 ? tb->rmarg_imax : MAX(imax[v], tb->rmarg_imax);
+      }
+    }
   if (!doing_search) {
     imin[0] = i0;
     if (imin[1] != -1) imin[1] = i0;
@@ -510,6 +524,7 @@ void ixo_hmm2ij_bands(const ixo_hbcm *cm, const ixo_hbcp9 *h, int32_t *pn, int i
     if (imin[v] == -1 || jmin[v] == -1) imin[v] = jmin[v] = -1, imax[v] = jmax[v] = -2;
     if (v + 1 < M && cm->sttype[v + 1] == E_ST) /* detached (cp9.rs:446) */
       imin[v] = jmin[v] = -1, imax[v] = jmax[v] = -2;
+    if (do_trunc) continue;
     if (cm->sttype[v] == MP_ST) {
       if (jmax[v] == i0) imin[v] = jmin[v] = -1, imax[v] = jmax[v] = -2;
       else if (jmin[v] == i0) jmin[v] += 1;
@@ -520,12 +535,14 @@ void ixo_hmm2ij_bands(const ixo_hbcm *cm, const ixo_hbcp9 *h, int32_t *pn, int i
     }
   }
   if (hmm_localized && cm_localized) {
+    if (do_trunc) tb->jvalid[0] = 1;
     if (imin[0] == -1) imin[0] = imax[0] = i0, jmin[0] = jmax[0] = j0;
     int nd1 = 1;
     if (i0 == j0)
       while (nd1 < cm->nodes && cm->ndtype[nd1] == MATP_ND) nd1++;
     if (cm->ndtype[nd1] == BIF_ND) {
       const int v = cm->nodemap[nd1], w = cm->cfirst[v], y = cm->cnum[v];
+      if (do_trunc) tb->jvalid[v] = tb->jvalid[w] = tb->jvalid[y] = 1;
       if (imin[v] != -1 && imin[w] != -1 && imin[y] != -1) {
         if (!doing_search) {
           imin[v] = MIN(imin[v], i0), imax[v] = MAX(imax[v], i0);
@@ -551,6 +568,7 @@ void ixo_hmm2ij_bands(const ixo_hbcm *cm, const ixo_hbcp9 *h, int32_t *pn, int i
       }
     } else {
       const int v = cm->nodemap[nd1];
+      if (do_trunc) tb->jvalid[v] = 1;
       if (!doing_search) {
         if (imin[v] == -1) imin[v] = imax[v] = i0, jmin[v] = jmax[v] = j0;
         else {
@@ -572,6 +590,133 @@ void ixo_hmm2ij_bands(const ixo_hbcm *cm, const ixo_hbcp9 *h, int32_t *pn, int i
   free(nd2r);
   free(ndidx);
   (void)state_right_delta;
+}
+
+void ixo_hmm2ij_bands(const ixo_hbcm *cm, const ixo_hbcp9 *h, int32_t *pn, int i0, int j0, int doing_search,
+                      int32_t *imin, int32_t *imax, int32_t *jmin, int32_t *jmax) {
+  ixo_hmm2ij_bands_tr(cm, h, pn, i0, j0, doing_search, NULL, imin, imax, jmin, jmax);
+}
+
+/* Score2Prob(sc, 1.0) (cp9.rs:1741) */
+static float score2prob(int32_t sc) {
+  return sc == -987654321 ? 0.0f : (float)exp((double)((float)sc / 1000.0f) * 0.69314718);
+}
+
+/* cp9.rs:1755-1916 cp9_predict_start_and_end_positions */
+void ixo_tr_predict_start_end(const int32_t *pn, const int32_t *occ, int hmm_m, int i0, int j0, float thresh1,
+                              float thresh2, ixo_trband *tb) {
+  const int H1 = hmm_m + 1;
+  const int32_t *pmin_m = pn, *pmax_m = pn + H1, *pmin_i = pn + 2 * H1, *pmax_i = pn + 3 * H1, *pmin_d = pn + 4 * H1,
+                *pmax_d = pn + 5 * H1;
+  int sp1 = -1, sp2 = -1, ep1, ep2, k = 1;
+  while (k <= hmm_m && (sp1 == -1 || sp2 == -1)) {
+    if (!(pmin_m[k] == -1 && pmin_i[k] == -1 && pmin_d[k] == -1)) {
+      const float p = score2prob(occ[k]);
+      if (sp1 == -1 && p > thresh1) sp1 = k;
+      if (sp2 == -1 && p > thresh2) sp2 = k;
+    }
+    k++;
+  }
+  if (k == hmm_m + 1) {
+    if (sp1 == -1) sp1 = hmm_m + 1;
+    if (sp2 == -1) sp2 = hmm_m + 1;
+  }
+  if (sp1 == hmm_m + 1 && sp2 == hmm_m + 1) {
+    ep1 = ep2 = 0;
+  } else {
+    ep1 = ep2 = -1;
+    k = hmm_m;
+    while (k >= 1 && (ep1 == -1 || ep2 == -1)) {
+      if (!(pmin_m[k] == -1 && pmin_i[k] == -1 && pmin_d[k] == -1)) {
+        const float p = score2prob(occ[k]);
+        if (ep1 == -1 && p > thresh1) ep1 = k;
+        if (ep2 == -1 && p > thresh2) ep2 = k;
+      }
+      k--;
+    }
+    if (k == 0) {
+      if (ep1 == -1) ep1 = 0;
+      if (ep2 == -1) ep2 = 0;
+    }
+  }
+  tb->sp1 = sp1, tb->sp2 = sp2, tb->ep1 = ep1, tb->ep2 = ep2;
+  const int mm1 = hmm_m + 1;
+  /* the R marginal i band from the start nodes, the L marginal j band from
+   * the end nodes (a delete's j is one less: cp9.rs:1866-1912) */
+  if (sp1 == mm1) {
+    tb->rmarg_imin = i0;
+    tb->rmarg_imax = j0;
+  } else {
+    int lo = INT_MAX, hi = INT_MIN;
+    const int s[2] = {sp1, sp2};
+    for (int q = 0; q < 2; q++) {
+      if (s[q] == mm1) continue;
+      if (pmin_m[s[q]] >= 0) lo = MIN(lo, pmin_m[s[q]]);
+      if (pmin_i[s[q]] >= 0) lo = MIN(lo, pmin_i[s[q]]);
+      if (pmin_d[s[q]] >= 0) lo = MIN(lo, pmin_d[s[q]]);
+      if (pmax_m[s[q]] >= 0) hi = MAX(hi, pmax_m[s[q]]);
+      if (pmax_i[s[q]] >= 0) hi = MAX(hi, pmax_i[s[q]]);
+      if (pmax_d[s[q]] >= 0) hi = MAX(hi, pmax_d[s[q]]);
+    }
+    if (lo == INT_MAX || sp2 == mm1) lo = i0;
+    if (hi == INT_MIN || sp2 == mm1) hi = j0 + 1;
+    tb->rmarg_imin = MIN(j0 + 1, MAX(i0, lo));
+    tb->rmarg_imax = MIN(j0 + 1, MAX(i0, hi));
+  }
+  if (ep1 == 0) {
+    tb->lmarg_jmin = i0 - 1;
+    tb->lmarg_jmax = j0;
+  } else {
+    int lo = INT_MAX, hi = INT_MIN;
+    const int e[2] = {ep1, ep2};
+    for (int q = 0; q < 2; q++) {
+      if (e[q] == 0) continue;
+      if (pmin_m[e[q]] >= 0) lo = MIN(lo, pmin_m[e[q]]);
+      if (pmin_i[e[q]] >= 0) lo = MIN(lo, pmin_i[e[q]]);
+      if (pmin_d[e[q]] >= 0) lo = MIN(lo, pmin_d[e[q]] - 1);
+      if (pmax_m[e[q]] >= 0) hi = MAX(hi, pmax_m[e[q]]);
+      if (pmax_i[e[q]] >= 0) hi = MAX(hi, pmax_i[e[q]]);
+      if (pmax_d[e[q]] >= 0) hi = MAX(hi, pmax_d[e[q]] - 1);
+    }
+    if (lo == INT_MAX || ep2 == 0) lo = i0 - 1;
+    if (hi == INT_MIN || ep2 == 0) hi = j0;
+    tb->lmarg_jmin = MIN(j0, MAX(i0 - 1, lo));
+    tb->lmarg_jmax = MIN(j0, MAX(i0 - 1, hi));
+  }
+}
+
+/* cp9.rs:1918-1990 cp9_marginal_candidates_from_start_end_positions */
+void ixo_tr_candidates(const ixo_hbcm *cm, int pass_idx, ixo_trband *tb) {
+  const int al = pass_idx == 3 || pass_idx == 4 || pass_idx == 5; /* fill L: 3P, 5P&3P */
+  const int ar = pass_idx == 2 || pass_idx == 4 || pass_idx == 5; /* fill R: 5P, 5P&3P */
+  const int at = pass_idx == 4 || pass_idx == 5;
+  int *ndidx = node_index(cm);
+  int32_t *lpos = (int32_t *)malloc(sizeof(int32_t) * (size_t)cm->nodes),
+          *rpos = (int32_t *)malloc(sizeof(int32_t) * (size_t)cm->nodes);
+  ixo_hb_emit_map(cm, lpos, rpos);
+  for (int v = 0; v < cm->M; v++) {
+    const int nd = ndidx[v], nt = cm->ndtype[nd];
+    const int lp = (nt == MATP_ND || nt == MATL_ND) ? lpos[nd] : lpos[nd] + 1;
+    const int rp = (nt == MATP_ND || nt == MATR_ND) ? rpos[nd] : rpos[nd] - 1;
+    tb->jvalid[v] = lp >= tb->sp1 && rp <= tb->ep1;
+    tb->lvalid[v] = al && lp >= tb->sp1 && lp <= tb->ep1 && rp > tb->ep2;
+    tb->rvalid[v] = ar && rp <= tb->ep1 && rp >= tb->sp1 && lp < tb->sp2;
+    tb->tvalid[v] = cm->sttype[v] == B_ST ? (at && lp < tb->sp2 && rp > tb->ep2) : 0;
+  }
+  for (int v = 0; v < cm->M; v++) {
+    const int st = cm->sttype[v];
+    if (st == B_ST || st == MP_ST || st == ML_ST || st == IL_ST || st == MR_ST || st == IR_ST)
+      if (tb->jvalid[v]) tb->jvalid[0] = 1;
+    if (st == B_ST || st == MP_ST || st == ML_ST || st == IL_ST)
+      if (tb->lvalid[v]) tb->lvalid[0] = 1;
+    if (st == B_ST || st == MP_ST || st == MR_ST || st == IR_ST)
+      if (tb->rvalid[v]) tb->rvalid[0] = 1;
+    if (st == B_ST && tb->tvalid[v]) tb->tvalid[0] = 1;
+    if (tb->jvalid[0] && tb->lvalid[0] && tb->rvalid[0] && tb->tvalid[0]) break;
+  }
+  free(lpos);
+  free(rpos);
+  free(ndidx);
 }
 
 /* cp9.rs:3815-3850 ij2d_bands; hd_off[v] = first row of state v (M+1

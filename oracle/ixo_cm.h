@@ -97,6 +97,12 @@ void ixo_cm_marginals(const ixo_cm *cm, float *lmesc, float *rmesc);
 void ixo_emit_map(const ixo_cm *cm, int32_t *lpos, int32_t *rpos);
 /* Marginal-mode validity per state from the envelope's consensus span
  * sp1..ep1 / sp2..ep2 (cp9.rs:1925-1990); arrays of M entries. */
+/* cp9.rs:752 cm_expected_state_occupancy of the un-localized CM (psi[M]) */
+void ixo_cm_psi(const ixo_cm *cm, double *psi);
+/* cm_trunc.rs:192 TrPenalties::new on the global CM: g, l [3][M] log2
+ * (0 = 5'&3', 1 = 5' only, 2 = 3' only; -1e36 = not allowed), ig, il the
+ * Prob2Score integers (-987654321 = not allowed) */
+void ixo_trpenalties(const ixo_cm *cm, float *g, float *l, int32_t *ig, int32_t *il);
 void ixo_tr_validity(const ixo_cm *cm, const int32_t *lpos, const int32_t *rpos, int pass_idx, int sp1, int sp2,
                      int ep1, int ep2, uint8_t *jv, uint8_t *lv, uint8_t *rv, uint8_t *tv);
 /* HMM-banded TrCYK (inside = 0, cm_dpsearch_trunc.rs:264-1029) and FTrInside
@@ -168,6 +174,10 @@ int32_t ixo_cp9_band_thresh(double p_thresh);
 void ixo_cp9_fb2hmmbands(const ixo_cp9 *h, const uint8_t *dsq, const int32_t *fm, const int32_t *fi,
                          const int32_t *fd, const int32_t *bm, const int32_t *bi, const int32_t *bd, int i0, int j0,
                          double p_thresh, int did_fwd_scan, int32_t *pn);
+/* per-node posterior occupancy (M + D posterior summed over the window's
+ * rows, the quantity cp9_predict_start_and_end_positions thresholds) */
+void ixo_cp9_occupancy(const ixo_cp9 *h, const uint8_t *dsq, const int32_t *fm, const int32_t *fd,
+                       const int32_t *bm, const int32_t *bd, int i0, int j0, int did_fwd_scan, int32_t *occ);
 
 #ifdef __cplusplus
 }

@@ -95,8 +95,29 @@ def _init():
                                      C.c_float, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int),
                                      C.POINTER(C.c_float), C.POINTER(TrHit), C.c_int64, _f32p]
         L.ixo_trqdb_scan.restype = C.c_int64
+        L.ixo_cm_psi.argtypes = [C.POINTER(Cm), C.POINTER(C.c_double)]
+        L.ixo_trpenalties.argtypes = [C.POINTER(Cm), _f32p, _f32p, _i32p, _i32p]
         _ready = True
     return L
+
+
+def cm_psi(cm):
+    """cm_expected_state_occupancy of the un-localized CM (float64 [M])."""
+    s = cm_struct(cm)
+    psi = np.zeros(cm["M"], np.float64)
+    lib().ixo_cm_psi(C.byref(s), psi.ctypes.data_as(C.POINTER(C.c_double)))
+    return psi
+
+
+def trpenalties(cm):
+    """TrPenalties::new: (g, l, ig, il), each [3, M]."""
+    s = cm_struct(cm)
+    M = cm["M"]
+    g, l = np.zeros((3, M), np.float32), np.zeros((3, M), np.float32)
+    ig, il = np.zeros((3, M), np.int32), np.zeros((3, M), np.int32)
+    lib().ixo_trpenalties(C.byref(s), g.ctypes.data_as(_f32p), l.ctypes.data_as(_f32p), ig.ctypes.data_as(_i32p),
+                          il.ctypes.data_as(_i32p))
+    return g, l, ig, il
 
 
 def cm_struct(cm):
