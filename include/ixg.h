@@ -449,15 +449,20 @@ int ixg_cp9_bands_batch(ixg_ctx *ctx, const ixg_cp9 *h, const ixg_target *tgt, c
 int ixg_gfb_scores(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *seq, int64_t seq_len, const int64_t *off,
                    const int64_t *len, const int32_t *cfg_len, int64_t n, int mode, float *sc, float *xmx,
                    float *mat);
-/* Thresholds of the envelope definition (cm_search.rs:2449 trunc_env_def,
- * STD pass): F4 / F4b / F5 / F5b P-values against the glocal Forward tail
- * (gfmu, gflambda); rt1 rt2 rt3 and ns of p7_generic.rs:1765
- * p7_domaindef_glocal (the multidomain traces use Easel's fast RNG, seed 181). */
+/* Thresholds of the envelope definition (cm_search.rs:2449 trunc_env_def):
+ * F4 / F4b / F5 / F5b P-values against the glocal Forward tail (gfmu,
+ * gflambda) in the STD pass and the local Forward tail (lftau, lflambda) in
+ * the truncated passes; rt1 rt2 rt3 and ns of p7_generic.rs:1765
+ * p7_domaindef_glocal (the multidomain traces use Easel's fast RNG, seed
+ * 181); clen, the CM's consensus length, sets the 3' pass's score
+ * correction ln(1/clen) (cm_search.rs:2476-2478). */
 typedef struct {
   double f4, f4b, f5, f5b;
   double gfmu, gflambda;
   float rt1, rt2, rt3;
   int32_t ns, do_gfwdbias, do_edefbias;
+  double lftau, lflambda;
+  int32_t clen, pad;
 } ixg_envopts;
 /* Per-pass accounting of the stage (cm_search.rs add_gfwd / add_gfwdbias /
  * add_edef / add_edefbias). */
@@ -472,6 +477,16 @@ typedef struct {
 int ixg_env_def(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *dsq, int64_t L, const ixg_window *wins,
                 int64_t nwin, const ixg_envopts *o, int32_t *es, int32_t *ee, int64_t cap, int64_t *nenv,
                 ixg_envacct *acct);
+/* The same for any pass of trunc_env_def (cm_search.rs:2449-2600):
+ * IXG_PASS_STD_ANY (gm), IXG_PASS_5P_ONLY_FORCE (Rgm, windows must start at
+ * residue 1), IXG_PASS_3P_ONLY_FORCE (Lgm, must end at L, score correction
+ * ln(1/o->clen)), IXG_PASS_5P_AND_3P_FORCE (Tgm, both); the truncated
+ * passes' statistics are o->lftau / o->lflambda. The windows are one
+ * chunk's in the reference's order: a truncated pass's profile carries
+ * over from window to window as there. */
+int ixg_env_def_pass(ixg_ctx *ctx, const ixg_p7 *p7, const uint8_t *dsq, int64_t L, const ixg_window *wins,
+                     int64_t nwin, int pass_idx, const ixg_envopts *o, int32_t *es, int32_t *ee, int64_t cap,
+                     int64_t *nenv, ixg_envacct *acct);
 
 /* ---- HMM -> CM band conversion (cp9.rs:2893-3879) ---------------------------
  * A host map of one CM and its CP9 (ixg_cm_params / ixg_cp9_params, the CP9
@@ -490,21 +505,36 @@ int ixg_hmm2ij_bands(const ixg_hbmap *m, int32_t *pn, int32_t i0, int32_t j0, in
                      int32_t *imax, int32_t *jmin, int32_t *jmax);
 /* cp9.rs:3815 ij2d_bands into the ixg_bands layout: hd_off [M+1], rows
  * hdmin/hdmax; *nrows = rows needed (IXG_ERANGE above cap). */
+/* The truncated form (cp9_seq2bands_trunc's steps 3-4, cp9.rs:4130-4175):
+ * from pn and the per-node posterior occupancy occ [hmm_m+1], the predicted
+ * start / end nodes and marginal bands (cp9.rs:1755), the J/L/R/T
+ * candidacy of every state for the pass (:1918), then cp9_hmm2ijbands with
+ * do_trunc; jvalid..tvalid [M]; spep (may be NULL) receives sp1 sp2 ep1 ep2
+ * rmarg_imin rmarg_imax lmarg_jmin lmarg_jmax. */
+int ixg_hmm2ij_bands_trunc(const ixg_hbmap *m, int32_t *pn, const int32_t *occ, int32_t i0, int32_t j0,
+                           int pass_idx, int32_t *imin, int32_t *imax, int32_t *jmin, int32_t *jmax,
+                           uint8_t *jvalid, uint8_t *lvalid, uint8_t *rvalid, uint8_t *tvalid, int32_t *spep);
 int ixg_ij2d_bands(const ixg_hbmap *m, const int32_t *imin, const int32_t *imax, const int32_t *jmin,
                    const int32_t *jmax, int do_trunc, int32_t *hd_off, int32_t *hdmin, int32_t *hdmax, int64_t cap,
                    int64_t *nrows);
 
-/* ---- the STD pass of a search, F1 through F7 ----------------------------------
+/* ---- a search, F1 through F7 ------------------------------------------------
  * Replaces FaithfulSearcher::search (cm_search.rs:1300) for one resident
- * sequence with the standard pass only (no truncated passes, as with
- * --notrunc): the (chunk x strand) tasks of cm_search.rs:1774, the filter
- * cascade (cm_pipeline.rs:1863), the envelope definition (cm_search.rs:2449),
- * cm_std_stage (cm_search.rs:2707: CP9 bands iterated to the size limit
- * (cp9.rs:4075), HB CYK filter with its P-value gate and envelope
- * redefinition, HB Inside with greedy reporting, null3 and per-envelope
- * overlap removal) and finalize (cm_search.rs:1915: genome coordinates,
- * E-values, global overlap removal, the E-value sort, the reporting
- * threshold). The per-hit alignment (mdl from/to) is not computed. */
+ * sequence: the (chunk x strand) tasks of cm_search.rs:1774, per task
+ * trunc_pipeline_one_strand (:2214) with the STD pass (the filter cascade,
+ * cm_pipeline.rs:1863; the envelope definition, cm_search.rs:2449;
+ * cm_std_stage, :2707: CP9 bands iterated to the size limit (cp9.rs:4075),
+ * HB CYK filter with its P-value gate and envelope redefinition, HB Inside
+ * with greedy reporting, null3 and per-envelope overlap removal) and, with
+ * trunc_mode 1 (cmsearch's default), the terminal truncated passes: 5' on
+ * the first maxW residues of a chunk that holds the sequence's 5' end, 3'
+ * on the last maxW residues of one that holds its 3' end, 5'&3' on a chunk
+ * of at most maxW residues holding both (cm_search.rs:2265-2280, :2380-2420),
+ * each with its own filter cascade, envelope definition (Rgm / Lgm / Tgm)
+ * and cm_trunc_stage (:2863: cp9_seq2bands_trunc, TrCYK filter, FTrInside);
+ * then finalize (cm_search.rs:1915: genome coordinates, E-values, global
+ * overlap removal, the E-value sort, the reporting threshold). The per-hit
+ * alignment (mdl from/to, trunc) is not computed. */
 typedef struct {
   ixg_f3opts f3;                   /* F1..F3b thresholds, cmw, maxw, IXG_DO_* */
   ixg_envopts env;                 /* F4..F5b */
@@ -516,22 +546,40 @@ typedef struct {
   double fcyk_tau, final_tau, maxtau;
   float size_limit;                /* HB matrix limit in Mb (cm_search.rs:890-896) */
   int32_t do_fcyk, do_fcykenv, do_null3, do_cyk, do_top, do_bot;
+  int32_t trunc_mode;              /* 0: STD pass only (--notrunc); 1: + the terminal truncated passes */
+  int32_t pad;
 } ixg_searchopts;
 typedef struct {
   int64_t start, stop;  /* genome coordinates (start > stop on the bottom strand) */
   float score, bias;
   double evalue;
-  int32_t strand, pad;
+  int32_t strand, pass; /* pass: IXG_PASS_* that found the hit */
 } ixg_searchhit;
 typedef struct {
   uint64_t nres;        /* residues searched: both strands, overlap context excluded */
   ixg_acct f;           /* F1..F3b, summed over tasks */
   ixg_envacct e;        /* F4..F5b */
   uint64_t n_past_cyk, pos_past_cyk, n_hits;
+  /* the truncated passes, [0] 5', [1] 3', [2] 5'&3': residues charged
+   * (min(maxW, chunk) per chunk that runs the pass), filter, envelope and
+   * CYK accounting (cm_search.rs:2283-2300) and the hits they contribute
+   * before the global overlap removal */
+  uint64_t tr_nres[3];
+  ixg_acct tr_f[3];
+  ixg_envacct tr_e[3];
+  uint64_t tr_n_past_cyk[3], tr_pos_past_cyk[3], tr_hits[3];
 } ixg_searchacct;
 int ixg_search_std(ixg_ctx *ctx, const ixg_p7 *p7, const ixg_cm *cm, const ixg_cp9 *cp9, const ixg_hbmap *map,
                    const ixg_target *tgt, const ixg_searchopts *o, ixg_searchhit *out, int64_t cap, int64_t *n,
                    ixg_searchacct *acct);
+/* The same with the truncated passes when o->trunc_mode is 1: trcp9[k] and
+ * trmap[k] (k = 0 5', 1 3', 2 5'&3') are the pass's CP9 (Rcp9 / Lcp9 /
+ * Tcp9 of cm_search.rs:953-960, local ends as the search's configuration)
+ * and its band map; the CM's truncation penalties must be set
+ * (ixg_cm_set_trpenalties). ixg_search_std is this with trunc_mode 0. */
+int ixg_search(ixg_ctx *ctx, const ixg_p7 *p7, const ixg_cm *cm, const ixg_cp9 *cp9, const ixg_hbmap *map,
+               const ixg_cp9 *const *trcp9, const ixg_hbmap *const *trmap, const ixg_target *tgt,
+               const ixg_searchopts *o, ixg_searchhit *out, int64_t cap, int64_t *n, ixg_searchacct *acct);
 /* Host wall time (ms) of each stage of the last ixg_search_std call on the
  * context (IXG_SS_* index; a stage's time includes its launches, waits and
  * host work) and the HB cells its CYK (cells[0]) and Inside (cells[1])

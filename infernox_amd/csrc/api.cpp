@@ -106,7 +106,8 @@ struct ixg_p7 {
   FwdHost fwd;
   BiasHost bias;
   GmHost gm;                // glocal profile of the F4/F5 envelope definition
-  float *d_gm = nullptr;    // tsc | msc | isc | nats logsum table
+  GmHost gmt[3];            // the truncated passes' profiles (5', 3', 5'&3')
+  float *d_gm = nullptr;    // tsc x 4 (gm, gmt[0..2]) | msc | isc | nats logsum table
   uint32_t *d_cost4 = nullptr;
   uint8_t *d_rbv = nullptr;
   float *d_fwd = nullptr;   // tmain | tddv | rfvv | tz
@@ -142,7 +143,7 @@ struct ixg_p7 {
     GmDev d;
     const size_t M1 = (size_t)gm.M + 1;
     d.tsc = d_gm;
-    d.msc = d_gm + 8 * M1;
+    d.msc = d_gm + 4 * 8 * M1;
     d.isc = d.msc + KP * M1;
     d.tbl = d.isc + KP * M1;
     d.m = gm.M;
@@ -170,6 +171,7 @@ struct ixg_p7 {
 namespace ixg {
 GmDev p7_gm_dev(const ixg_p7 *p) { return p->gm_dev(); }
 const GmHost &p7_gm_host(const ixg_p7 *p) { return p->gm; }
+const GmHost &p7_gm_trunc(const ixg_p7 *p, int k) { return p->gmt[k]; }
 bool p7_has_device(const ixg_p7 *p) { return p->d_gm != nullptr; }
 FwdDev p7_fwd_dev(const ixg_p7 *p) {
   if (!p->d_fwd) return FwdDev{};
@@ -251,6 +253,7 @@ int ixg_p7_create(ixg_ctx *c, const ixg_p7_hmm *h, int32_t max_length, ixg_p7 **
   build_fwd(*h, p->fwd);
   build_bias(h->compo, h->M, p->bias);
   build_glocal(*h, p->gm);
+  build_trunc_profiles(*h, p->gm, p->gmt);
   p->vit_lvmu = h->lvmu;
   p->vit_lvlambda = h->lvlambda;
   if (!c) {  // host tables only (inspection); no device copy
@@ -286,6 +289,7 @@ int ixg_p7_create(ixg_ctx *c, const ixg_p7_hmm *h, int32_t max_length, ixg_p7 **
                   &p->vit.tdd})
     vit.insert(vit.end(), a->begin(), a->end());
   std::vector<float> gmv(p->gm.tsc);
+  for (const GmHost &t : p->gmt) gmv.insert(gmv.end(), t.tsc.begin(), t.tsc.end());
   gmv.insert(gmv.end(), p->gm.msc.begin(), p->gm.msc.end());
   gmv.insert(gmv.end(), p->gm.isc.begin(), p->gm.isc.end());
   gmv.insert(gmv.end(), p7_logsum_table().begin(), p7_logsum_table().end());
@@ -318,7 +322,7 @@ int ixg_p7_free(ixg_ctx *c, ixg_p7 *p) {
   dfree(c, p->d_rbv, p->msv.rbv.size());
   dfree(c, p->d_fwd, (p->fwd.tmain.size() + p->fwd.tddv.size() + p->fwd.rfvv.size()) * 4);
   dfree(c, p->d_vit, (p->vit.rwv.size() + 8 * (p->vit.m + 1)) * 2);
-  dfree(c, p->d_gm, (p->gm.tsc.size() + p->gm.msc.size() + p->gm.isc.size() + GFB_TBL) * 4);
+  dfree(c, p->d_gm, (4 * p->gm.tsc.size() + p->gm.msc.size() + p->gm.isc.size() + GFB_TBL) * 4);
   delete p;
   return IXG_OK;
 }

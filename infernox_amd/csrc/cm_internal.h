@@ -286,6 +286,6 @@ inline size_t cp9_wave_lds_bytes(int M, int nz, int nslots, int nlist) {
 }
 constexpr size_t CP9_LDS_MAX = 65536 - 64;
 hipError_t launch_cp9_bands(const Cp9Dev &h, const Cp9WinDev *wins, int64_t n, int mode, int32_t thresh,
-                            float *fsc, float *bsc, int32_t *pn, hipStream_t s);
+                            float *fsc, float *bsc, int32_t *pn, int32_t *occ, hipStream_t s);
 
 }  // namespace ixg

@@ -40,7 +40,7 @@ struct CWin {
 // pn are host arrays in the windows' order. mats (optional, n == 1): the six
 // matrices of the single window in [(L+1)][(M+1)] order.
 int cp9_run(ixg_ctx *c, const ixg_cp9 *h, const std::vector<CWin> &wins, double tau, int mode, float *fsc,
-            float *bsc, int32_t *pn, int32_t **mats) {
+            float *bsc, int32_t *pn, int32_t **mats, int32_t *occ = nullptr) {
   const int M = h->M, M1 = M + 1;
   const int64_t n = (int64_t)wins.size();
   if (!(tau > 0.0 && tau < 1.0)) {
@@ -82,27 +82,31 @@ int cp9_run(ixg_ctx *c, const ixg_cp9 *h, const std::vector<CWin> &wins, double 
       wd[k] = Cp9WinDev{w.dsq, w.i0, w.j0, k, d_mx + woff[k]};
     }
     const size_t wbytes = nw * sizeof(Cp9WinDev);
-    const size_t obytes = (size_t)nw * (2 * 4 + 6 * M1 * 4);
+    const size_t obytes = (size_t)nw * (2 * 4 + 6 * M1 * 4) + (occ ? (size_t)nw * M1 * 4 : 0);
     char *d_meta;
     RCHK(scratch(c, S_CM_ENV, wbytes + obytes + 64, (void **)&d_meta));
     Cp9WinDev *d_w = (Cp9WinDev *)d_meta;
     float *d_f = (float *)(d_meta + ((wbytes + 15) & ~(size_t)15));
     float *d_b = d_f + nw;
     int32_t *d_pn = (int32_t *)(d_b + nw);
+    int32_t *d_occ = occ ? d_pn + (size_t)nw * 6 * M1 : nullptr;
     HIPCHK(hipMemcpyAsync(d_w, wd.data(), wbytes, hipMemcpyHostToDevice, c->stream));
     HIPCHK(ktime_begin(c));
-    HIPCHK(launch_cp9_bands(h->dev, d_w, nw, mode, thresh, d_f, d_b, d_pn, c->stream));
+    HIPCHK(launch_cp9_bands(h->dev, d_w, nw, mode, thresh, d_f, d_b, d_pn, d_occ, c->stream));
     HIPCHK(ktime_end(c, IXG_K_CP9));
     std::vector<float> tf(nw), tb(nw);
     std::vector<int32_t> tp((size_t)nw * 6 * M1);
     RCHK(d2h(c, tf.data(), d_f, nw * 4));
     RCHK(d2h(c, tb.data(), d_b, nw * 4));
     RCHK(d2h(c, tp.data(), d_pn, tp.size() * 4));
+    std::vector<int32_t> to(occ ? (size_t)nw * M1 : 0);
+    if (occ) RCHK(d2h(c, to.data(), d_occ, to.size() * 4));
     for (int64_t k = 0; k < nw; k++) {
       const int64_t o = order[g0 + k];
       hf[o] = tf[k];
       hb[o] = tb[k];
       std::copy(tp.begin() + k * 6 * M1, tp.begin() + (k + 1) * 6 * M1, hp.begin() + o * 6 * M1);
+      if (occ) std::copy(to.begin() + k * M1, to.begin() + (k + 1) * M1, occ + o * M1);
     }
     if (mats && n == 1) {  // the single window's matrices, [(L+1)][(M+1)] each
       const int L = wins[0].j0 - wins[0].i0 + 1;
@@ -356,15 +360,17 @@ int ixg_cp9_bands_batch(ixg_ctx *c, const ixg_cp9 *h, const ixg_target *tgt, con
 namespace ixg {
 // Posterior HMM bands pn [n][6][M+1] of windows dsq[k][i0[k]..j0[k]] (device
 // sequences), the batched DP of cp9_seq2bands (cp9.rs:4053).
+// occ (optional): the per-node posterior occupancy [n][M+1] as well.
 int cp9_posterior_bands(ixg_ctx *c, const ixg_cp9 *h, const std::vector<const uint8_t *> &dsq,
                         const std::vector<int32_t> &i0, const std::vector<int32_t> &j0, double tau, int mode,
-                        std::vector<int32_t> &pn) {
+                        std::vector<int32_t> &pn, std::vector<int32_t> *occ) {
   const size_t n = dsq.size();
   std::vector<CWin> w(n);
   for (size_t k = 0; k < n; k++) w[k] = CWin{dsq[k], i0[k], j0[k]};
   pn.assign(n * 6 * (size_t)(h->M + 1), 0);
+  if (occ) occ->assign(n * (size_t)(h->M + 1), 0);
   if (n == 0) return IXG_OK;
-  return cp9_run(c, h, w, tau, mode, nullptr, nullptr, pn.data(), nullptr);
+  return cp9_run(c, h, w, tau, mode, nullptr, nullptr, pn.data(), nullptr, occ ? occ->data() : nullptr);
 }
 int cp9_nodes(const ixg_cp9 *h) { return h->M; }
 }  // namespace ixg

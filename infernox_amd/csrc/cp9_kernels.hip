@@ -308,8 +308,12 @@ __global__ __launch_bounds__(64) void cp9_bwd_kernel(Cp9Dev h, const Cp9WinDev *
 
 // cp9.rs:1517-1737: posterior mass crept in from both ends, one lane per
 // node; the posterior cells are recomputed in the maximum scan.
+// occ_out (may be null): per node the posterior occupancy ILogsum over rows
+// 0..L of ILogsum(M, D) posterior (cp9.rs:1765-1776, the quantity the
+// truncated bands' start / end prediction thresholds), [n][M+1], node 0
+// -INFTY.
 __global__ __launch_bounds__(64) void cp9_post_kernel(Cp9Dev h, const Cp9WinDev *wins, int did_scan, int32_t thresh,
-                                                     int32_t *pn_out) {
+                                                     int32_t *pn_out, int32_t *occ_out) {
   const Lds s = lds_setup(h, O_DD, 0);
   __shared__ int32_t s_sc;
   const int t = threadIdx.x, M = h.M, M1 = M + 1, nz = h.nz;
@@ -396,6 +400,20 @@ __global__ __launch_bounds__(64) void cp9_post_kernel(Cp9Dev h, const Cp9WinDev 
     pn[3 * M1 + k] = xi;
     pn[4 * M1 + k] = nd;
     pn[5 * M1 + k] = xd;
+    if (occ_out) {
+      int32_t o = NINF;
+      if (k > 0) {
+        o = LS(NINF, LS(NINF, ws(wa(fd[k], bd[k]), sc)));
+        for (int r = 1; r <= L; r++) {
+          const int x = g.dsq[i0 + r - 1];
+          const size_t c = (size_t)r * M1 + k;
+          const int32_t m = max(ws(ws(wa(fm[c], bm[c]), MSC(k, x)), sc), NINF);
+          const int32_t d = max(ws(wa(fd[c], bd[c]), sc), NINF);
+          o = LS(o, LS(m, d));
+        }
+      }
+      gmem(occ_out)[(size_t)w.out * M1 + k] = o;
+    }
   }
 }
 
@@ -748,7 +766,7 @@ __global__ __launch_bounds__(64) void cp9_bwd_wave(Cp9Dev h, const Cp9WinDev *wi
 }  // namespace
 
 hipError_t launch_cp9_bands(const Cp9Dev &h, const Cp9WinDev *wins, int64_t n, int mode, int32_t thresh,
-                            float *fsc, float *bsc, int32_t *pn, hipStream_t s) {
+                            float *fsc, float *bsc, int32_t *pn, int32_t *occ, hipStream_t s) {
   if (n == 0) return hipSuccess;
   const dim3 grid((unsigned)n), block(64);
   const size_t lds = cp9_lds_bytes(h.M, h.nz);
@@ -764,7 +782,7 @@ hipError_t launch_cp9_bands(const Cp9Dev &h, const Cp9WinDev *wins, int64_t n, i
     hipLaunchKernelGGL(cp9_bwd_wave, grid, block, lb, s, h, wins, bscan, align, bsc);
   else
     hipLaunchKernelGGL(cp9_bwd_kernel, grid, block, lds, s, h, wins, bscan, align, bsc);
-  hipLaunchKernelGGL(cp9_post_kernel, grid, block, lds, s, h, wins, fscan, thresh, pn);
+  hipLaunchKernelGGL(cp9_post_kernel, grid, block, lds, s, h, wins, fscan, thresh, pn, occ);
   return hipGetLastError();
 }
 

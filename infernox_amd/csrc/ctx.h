@@ -54,12 +54,30 @@ struct WinRef {
   int32_t task;
   int64_t start, end;  // chunk-local window
 };
+// The truncated-band fields of CP9Bands (cp9.rs:1471-1491): predicted
+// start / end nodes, the R / L marginal i / j bands, per state the J, L, R, T
+// candidacy (cp9_seq2bands_trunc's steps 3 and 4).
+struct TrBand {
+  float thresh1 = 0.01f, thresh2 = 0.98f;  // DEFAULT_CP9BANDS_THRESH1/2 (cp9.rs:1495)
+  int32_t sp1 = -1, sp2 = -1, ep1 = -1, ep2 = -1;
+  int32_t rmarg_imin = -1, rmarg_imax = -2, lmarg_jmin = -1, lmarg_jmax = -2;
+  std::vector<uint8_t> jv, lv, rv, tv;
+};
+// A window of the envelope definition: its task (a sequence view), its
+// coordinates in that view and its pass (IXG_PASS_*).
+struct EnvWin {
+  int32_t task;
+  int64_t start, end;
+  int32_t pass;
+};
 // Per-window F1b/F2/F3/F3b scores (kind 1 bias, 0 Forward, 2 Viterbi, 3
 // MSV) of windows wl over tasks whose dsq views sit in d_seq.
 int run_scores(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<HostTask> &tasks,
                const std::vector<WinRef> &wl, int kind, std::vector<float> &sc, std::vector<uint8_t> &ovf);
 GmDev p7_gm_dev(const ixg_p7 *p);
 const GmHost &p7_gm_host(const ixg_p7 *p);
+// the truncated passes' profiles: 0 = 5', 1 = 3', 2 = 5'&3'
+const GmHost &p7_gm_trunc(const ixg_p7 *p, int k);
 bool p7_has_device(const ixg_p7 *p);
 // Looks up the device address of the no-allocation scratch array for
 // `device` (current) and keeps it; called by ixg_init.

@@ -79,6 +79,67 @@ void GmHost::set_length(int L) {
   }
   len = L;
 }
+// p7_generic.rs:350-376: the truncated length models, in which only the C
+// (5') or only the N (3') state, and J, bear the length
+void GmHost::set_length_5p(int L) {
+  const float pmove = (1.0f + nj) / ((float)L + 1.0f + nj);
+  const float ploop = 1.0f - pmove;
+  const float lo = (float)std::log((double)ploop), mv = (float)std::log((double)pmove);
+  xsc[XC][0] = xsc[XJ][0] = lo;
+  xsc[XC][1] = xsc[XJ][1] = mv;
+  len = L;
+}
+void GmHost::set_length_3p(int L) {
+  const float pmove = (1.0f + nj) / ((float)L + 1.0f + nj);
+  const float ploop = 1.0f - pmove;
+  const float lo = (float)std::log((double)ploop), mv = (float)std::log((double)pmove);
+  xsc[XN][0] = xsc[XJ][0] = lo;
+  xsc[XN][1] = xsc[XJ][1] = mv;
+  len = L;
+}
+
+// p7_generic.rs:259-348 and the L = 100 configuration of cm_search.rs:1011-1016
+void build_trunc_profiles(const ixg_p7_hmm &h, const GmHost &gm, GmHost out[3]) {
+  enum { H_MM = 0, H_MI = 1, H_DM = 5 };
+  const int M = gm.M;
+  GmHost base = gm;  // build_glocal_profile(p7, 100): multihit, length 100
+  base.set_multihit();
+  base.set_length(100);
+  // 5' (Rgm, p7_UNIGLOCAL): uniform begins log(1/M); N and E loops off
+  GmHost &r = out[0];
+  r = base;
+  const float inv = (float)std::log(1.0 / (double)M);
+  for (int k = 1; k <= M; k++) r.tsc[(size_t)(k - 1) * 8 + GmHost::BM] = inv;
+  r.local_end = false;
+  r.xsc[GmHost::XN][1] = 0.0f, r.xsc[GmHost::XE][1] = 0.0f;
+  r.xsc[GmHost::XN][0] = -INFINITY, r.xsc[GmHost::XE][0] = -INFINITY;
+  r.set_length_5p(100);
+  // 3' (Lgm, p7_UNILOCAL): glocal begins, local ends; C and E loops off
+  GmHost &l = out[1];
+  l = base;
+  l.local_end = true;
+  l.xsc[GmHost::XC][1] = 0.0f, l.xsc[GmHost::XE][1] = 0.0f;
+  l.xsc[GmHost::XC][0] = -INFINITY, l.xsc[GmHost::XE][0] = -INFINITY;
+  l.set_length_3p(100);
+  // 5'&3' (Tgm): the local profile's occupancy-weighted begins, local ends,
+  // every loop off, nj = 0, no length model
+  GmHost &t = out[2];
+  t = base;
+  auto tr = [&](int k, int s) { return h.trans[k * 7 + s]; };
+  std::vector<float> mocc(M + 1, 0.0f);
+  mocc[1] = tr(0, H_MI) + tr(0, H_MM);
+  for (int k = 2; k <= M; k++)
+    mocc[k] = mocc[k - 1] * (tr(k - 1, H_MM) + tr(k - 1, H_MI)) + (1.0f - mocc[k - 1]) * tr(k - 1, H_DM);
+  float z = 0.0f;
+  for (int k = 1; k <= M; k++) z += mocc[k] * (float)(M - k + 1);
+  for (int k = 1; k <= M; k++) t.tsc[(size_t)(k - 1) * 8 + GmHost::BM] = (float)std::log((double)(mocc[k] / z));
+  t.local_end = true;
+  t.xsc[GmHost::XN][1] = t.xsc[GmHost::XC][1] = t.xsc[GmHost::XE][1] = 0.0f;
+  t.xsc[GmHost::XN][0] = t.xsc[GmHost::XC][0] = t.xsc[GmHost::XE][0] = -INFINITY;
+  t.nj = 0.0f;
+  t.len = 100;
+}
+
 // p7_generic.rs:1075-1094
 void GmHost::set_unihit() {
   xsc[XE][1] = 0.0f;

@@ -37,11 +37,6 @@ int run_gfb(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, std::vector<GfbIt
     if (it.flags & 2) mtot += 3 * ((int64_t)it.len + 1) * (M + 1);
   }
   const int64_t n = (int64_t)items.size();
-  for (const auto &it : items)
-    if ((it.flags & 1) != (items[0].flags & 1)) {
-      g_err = "glocal Forward/Backward: items of one launch differ in local ends";
-      return IXG_EINVAL;
-    }
   sc.assign(n, 0.f);
   xall.assign(xtot, 0.f);
   if (mats) mats->assign(mtot, 0.f);
@@ -54,8 +49,7 @@ int run_gfb(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, std::vector<GfbIt
   RCHK(scratch(c, S_GFB_SC, (size_t)n * 4, (void **)&d_sc));
   HIPCHK(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(GfbItem), hipMemcpyHostToDevice, c->stream));
   HIPCHK(ktime_begin(c));
-  HIPCHK(launch_gfb(p7_gm_dev(p), d_seq, d_items, n, d_x, d_m, d_sc, backward, (items[0].flags & 1) != 0,
-                    c->stream));
+  HIPCHK(launch_gfb(p7_gm_dev(p), d_seq, d_items, n, d_x, d_m, d_sc, backward, c->stream));
   HIPCHK(ktime_end(c, backward ? IXG_K_GBWD : IXG_K_GFWD));
   HIPCHK(hipMemcpyAsync(sc.data(), d_sc, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(xall.data(), d_x, (size_t)xtot * 4, hipMemcpyDeviceToHost, c->stream));
@@ -64,11 +58,12 @@ int run_gfb(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, std::vector<GfbIt
   return IXG_OK;
 }
 
-GfbItem make_item(int64_t off, int len, const GmHost &g, int flags) {
+// prof: the profile index (0 glocal, 1 5', 2 3', 3 5'&3')
+GfbItem make_item(int64_t off, int len, const GmHost &g, int flags, int prof = 0) {
   GfbItem it{};
   it.off = off;
   it.len = len;
-  it.flags = flags;
+  it.flags = flags | (g.local_end ? 1 : 0) | (prof << 2);
   g.xsc8(it.xsc);
   return it;
 }
@@ -347,82 +342,124 @@ std::vector<std::pair<int, int>> cluster_domains(const std::vector<Seg> &s, int 
 
 }  // namespace
 
-// The STD-pass envelope definition over windows of several tasks (windows
-// in chunk coordinates of their task). envs[w]: the window's surviving
-// envelopes (chunk coordinates), in the reference's order.
-int env_def_batch(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<HostTask> &tasks,
-                  const std::vector<WinRef> &wins, const ixg_envopts &o,
-                  std::vector<std::vector<std::pair<int32_t, int32_t>>> &envs, std::vector<ixg_envacct> &acct) {
-  const double LN2v = 0.693147180559945309417232121458176568;
-  const GmHost &proto = p7_gm_host(p);
-  const size_t nw = wins.size();
-  envs.assign(nw, {});
-  acct.assign(tasks.size(), ixg_envacct{});
-  if (nw == 0) return IXG_OK;
-  auto wlen = [&](size_t w) { return (int)(wins[w].end - wins[w].start + 1); };
-  auto woff = [&](size_t w) { return tasks[wins[w].task].base + wins[w].start - 1; };
-  // F4: glocal Forward of every window, multihit, configured for its length
-  std::vector<GfbItem> items(nw);
-  for (size_t w = 0; w < nw; w++) {
-    GmHost g = proto;
-    g.set_length(wlen(w));
-    items[w] = make_item(woff(w), wlen(w), g, 0);
+namespace {
+
+// A pass's profile as trunc_env_def configures it for one window
+// (cm_search.rs:2475-2520): the glocal gm with the standard length model in
+// the STD pass; the truncated passes' Rgm / Lgm with their 5' / 3' length
+// models and Tgm unchanged. A truncated pass's profile is one mutable copy
+// per (chunk, pass): once a window has resolved a multidomain region, the
+// trace ensemble has left it unihit with the standard length model at that
+// window's length (p7_generic.rs:1733-1760), and later windows start from
+// there (dirty_len, 0 while clean).
+struct PassProfile {
+  GmHost g;
+  int prof;
+};
+PassProfile window_profile(const ixg_p7 *p, int pass, int wlen, int dirty_len) {
+  PassProfile r;
+  if (pass == IXG_PASS_STD_ANY) {
+    r.g = p7_gm_host(p);
+    r.g.set_length(wlen);
+    r.prof = 0;
+    return r;
   }
+  const int k = pass == IXG_PASS_5P_ONLY_FORCE ? 0 : (pass == IXG_PASS_3P_ONLY_FORCE ? 1 : 2);
+  r.g = p7_gm_trunc(p, k);
+  r.prof = k + 1;
+  if (dirty_len) {
+    r.g.set_unihit();
+    r.g.set_length(dirty_len);
+  }
+  if (pass == IXG_PASS_5P_ONLY_FORCE) r.g.set_length_5p(wlen);
+  else if (pass == IXG_PASS_3P_ONLY_FORCE) r.g.set_length_3p(wlen);
+  return r;
+}
+
+// One wave of the envelope definition: every window w of `todo` with the
+// profile its group's dirty length gives. Per window: envelopes, accounting
+// and, for truncated passes, whether it resolved a multidomain region (and
+// at which length).
+int env_def_wave(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<HostTask> &tasks,
+                 const std::vector<EnvWin> &wins, const std::vector<size_t> &todo, const std::vector<int> &dirty,
+                 const ixg_envopts &o, std::vector<std::vector<std::pair<int32_t, int32_t>>> &envs,
+                 std::vector<ixg_envacct> &wacct, std::vector<int> &dirtied) {
+  const double LN2v = 0.693147180559945309417232121458176568;
+  const size_t nw = todo.size();
+  auto W = [&](size_t t) -> const EnvWin & { return wins[todo[t]]; };
+  auto wlen = [&](size_t t) { return (int)(W(t).end - W(t).start + 1); };
+  auto woff = [&](size_t t) { return tasks[W(t).task].base + W(t).start - 1; };
+  auto is_std = [&](size_t t) { return W(t).pass == IXG_PASS_STD_ANY; };
+  // per-pass statistics and score correction (cm_search.rs:2475-2489)
+  auto corr = [&](size_t t) -> float {
+    if (W(t).pass == IXG_PASS_5P_ONLY_FORCE) return 0.0f;
+    if (W(t).pass == IXG_PASS_3P_ONLY_FORCE) return logf(1.0f / (float)o.clen);
+    return NAN;  // none
+  };
+  auto mu = [&](size_t t) { return is_std(t) ? o.gfmu : o.lftau; };
+  auto lam = [&](size_t t) { return is_std(t) ? o.gflambda : o.lflambda; };
+  std::vector<PassProfile> pp(nw);
+  std::vector<GfbItem> items(nw);
+  for (size_t t = 0; t < nw; t++) {
+    pp[t] = window_profile(p, W(t).pass, wlen(t), dirty[todo[t]]);
+    items[t] = make_item(woff(t), wlen(t), pp[t].g, 0, pp[t].prof);
+  }
+  // F4: the Forward of every window
   std::vector<float> fsc, fx;
   RCHK(run_gfb(c, p, d_seq, items, false, fsc, fx, nullptr));
-  std::vector<int64_t> fxo(nw);
-  for (size_t w = 0; w < nw; w++) fxo[w] = items[w].xoff;
   std::vector<double> nullsc(nw);
   std::vector<size_t> s4;
-  for (size_t w = 0; w < nw; w++) {
-    nullsc[w] = bg_null_one(wlen(w));
-    const double s = ((double)fsc[w] - nullsc[w]) / LN2v;
-    if (exp_surv(s, o.gfmu, o.gflambda) > o.f4) continue;
-    acct[wins[w].task].n_past_gfwd++;
-    acct[wins[w].task].pos_past_gfwd += (uint64_t)wlen(w);
-    s4.push_back(w);
+  for (size_t t = 0; t < nw; t++) {
+    nullsc[t] = bg_null_one(wlen(t));
+    const float cr = corr(t);
+    const double s = std::isnan(cr) ? ((double)fsc[t] - nullsc[t]) / LN2v : ((double)(fsc[t] + cr) - nullsc[t]) / LN2v;
+    if (exp_surv(s, mu(t), lam(t)) > o.f4) continue;
+    wacct[todo[t]].n_past_gfwd++;
+    wacct[todo[t]].pos_past_gfwd += (uint64_t)wlen(t);
+    s4.push_back(t);
   }
   // F4b (and F5b): the window's composition-bias filter score
   std::vector<float> bias(nw, 0.f);
   if ((o.do_gfwdbias || o.do_edefbias) && !s4.empty()) {
     std::vector<WinRef> sub;
-    for (size_t w : s4) sub.push_back(wins[w]);
+    for (size_t t : s4) sub.push_back({W(t).task, W(t).start, W(t).end});
     std::vector<float> sc;
     std::vector<uint8_t> ov;
     RCHK(run_scores(c, p, d_seq, tasks, sub, 1, sc, ov));
-    for (size_t t = 0; t < s4.size(); t++) bias[s4[t]] = sc[t];
+    for (size_t q = 0; q < s4.size(); q++) bias[s4[q]] = sc[q];
   }
   std::vector<size_t> s4b;
-  for (size_t w : s4) {
+  for (size_t t : s4) {
     if (o.do_gfwdbias) {
-      const double s = ((double)fsc[w] - (double)bias[w]) / LN2v;
-      if (exp_surv(s, o.gfmu, o.gflambda) > o.f4b) continue;
-      acct[wins[w].task].n_past_gfwdbias++;
-      acct[wins[w].task].pos_past_gfwdbias += (uint64_t)wlen(w);
+      const float cr = corr(t);
+      const double s =
+          std::isnan(cr) ? ((double)fsc[t] - (double)bias[t]) / LN2v : ((double)(fsc[t] + cr) - nullsc[t]) / LN2v;
+      if (exp_surv(s, mu(t), lam(t)) > o.f4b) continue;
+      wacct[todo[t]].n_past_gfwdbias++;
+      wacct[todo[t]].pos_past_gfwdbias += (uint64_t)wlen(t);
     }
-    s4b.push_back(w);
+    s4b.push_back(t);
   }
   if (s4b.empty()) return IXG_OK;
-  // glocal Backward of the survivors (same configuration)
+  // the Backward of the survivors (same configuration)
   std::vector<GfbItem> bitems;
-  for (size_t w : s4b) bitems.push_back(items[w]);
+  for (size_t t : s4b) bitems.push_back(items[t]);
   std::vector<float> bsc, bx;
   RCHK(run_gfb(c, p, d_seq, bitems, true, bsc, bx, nullptr));
   // decoding and region finding (p7_generic.rs:1765-1836); each region is a
   // single domain or goes through the trace ensemble
   struct Region {
-    size_t w;
+    size_t t;
     int i, j;
     bool multi;
   };
   std::vector<Region> regions;
-  for (size_t t = 0; t < s4b.size(); t++) {
-    const size_t w = s4b[t];
-    const int L = wlen(w);
-    GmHost g = proto;
-    g.set_length(L);
+  for (size_t q = 0; q < s4b.size(); q++) {
+    const size_t t = s4b[q];
+    const int L = wlen(t);
     std::vector<float> btot(L + 1), etot(L + 1), mocc(L + 1);
-    domain_decoding(g, L, fx.data() + fxo[w], bx.data() + bitems[t].xoff, btot.data(), etot.data(), mocc.data());
+    domain_decoding(pp[t].g, L, fx.data() + items[t].xoff, bx.data() + bitems[q].xoff, btot.data(), etot.data(),
+                    mocc.data());
     int i = -1;
     bool trig = false;
     for (int j = 1; j <= L; j++) {
@@ -436,21 +473,27 @@ int env_def_batch(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::
           const float e = etot[z] - etot[i - 1], b = btot[j] - btot[z - 1];
           mx = std::max(mx, e < b ? e : b);
         }
-        regions.push_back({w, i, j, mx >= o.rt3});
+        regions.push_back({t, i, j, mx >= o.rt3});
         i = -1;
         trig = false;
       }
     }
   }
-  // multidomain regions: region Forward (multihit, configured for the
-  // window's length) with its matrix, then the traces on the host
+  // multidomain regions: the region Forward (multihit at the profile's
+  // length, with its matrix), then the traces on the host
+  auto multihit_cfg = [&](size_t t) {
+    GmHost g = pp[t].g;
+    g.set_multihit();
+    g.set_length(pp[t].g.len);
+    return g;
+  };
   std::vector<GfbItem> ritems;
   std::vector<size_t> ridx;
   for (size_t r = 0; r < regions.size(); r++)
     if (regions[r].multi) {
-      GmHost g = proto;
-      g.set_length(wlen(regions[r].w));
-      ritems.push_back(make_item(woff(regions[r].w) + regions[r].i - 1, regions[r].j - regions[r].i + 1, g, 2));
+      const size_t t = regions[r].t;
+      ritems.push_back(make_item(woff(t) + regions[r].i - 1, regions[r].j - regions[r].i + 1, multihit_cfg(t), 2,
+                                 pp[t].prof));
       ridx.push_back(r);
     }
   std::vector<std::vector<std::pair<int, int>>> rdoms(regions.size());
@@ -459,8 +502,7 @@ int env_def_batch(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::
     RCHK(run_gfb(c, p, d_seq, ritems, false, rsc, rx, &rm));
     for (size_t q = 0; q < ritems.size(); q++) {
       const Region &R = regions[ridx[q]];
-      GmHost g = proto;
-      g.set_length(wlen(R.w));
+      const GmHost g = multihit_cfg(R.t);
       FastRng rng(181u);  // re-seeded per region
       std::vector<Seg> segs;
       for (int t = 0; t < o.ns; t++) {
@@ -473,40 +515,124 @@ int env_def_batch(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::
   }
   for (size_t r = 0; r < regions.size(); r++)
     if (!regions[r].multi) rdoms[r].push_back({regions[r].i, regions[r].j});
-  // rescoring: unihit Forward of each domain, configured for the window's length
+  // rescoring: the Forward of each domain with the profile as it stands
+  // then: unihit at the profile's length after the STD pass's reconfiguration
+  // or after a trace ensemble; a truncated pass's own (unihit) profile before
   std::vector<GfbItem> ditems;
   struct Dom {
-    size_t w;
+    size_t t;
     int i, j;
   };
   std::vector<Dom> doms;
-  for (size_t r = 0; r < regions.size(); r++)
-    for (auto &d : rdoms[r]) {
-      GmHost g = proto;
-      g.set_unihit();
-      g.set_length(wlen(regions[r].w));
-      ditems.push_back(make_item(woff(regions[r].w) + d.first - 1, d.second - d.first + 1, g, 0));
-      doms.push_back({regions[r].w, d.first, d.second});
+  size_t cur = SIZE_MAX;
+  bool after_multi = false;
+  for (size_t r = 0; r < regions.size(); r++) {
+    const size_t t = regions[r].t;
+    if (t != cur) cur = t, after_multi = false;
+    if (regions[r].multi) {
+      after_multi = true;
+      if (!is_std(t)) dirtied[todo[t]] = pp[t].g.len;
     }
+    GmHost g = pp[t].g;
+    if (is_std(t) || after_multi) {
+      g.set_unihit();
+      g.set_length(pp[t].g.len);
+    }
+    for (auto &d : rdoms[r]) {
+      ditems.push_back(make_item(woff(t) + d.first - 1, d.second - d.first + 1, g, 0, pp[t].prof));
+      doms.push_back({t, d.first, d.second});
+    }
+  }
   std::vector<float> dsc, dx;
   RCHK(run_gfb(c, p, d_seq, ditems, false, dsc, dx, nullptr));
   // F5 / F5b in domain order
   for (size_t q = 0; q < doms.size(); q++) {
-    const size_t w = doms[q].w;
-    const int L = wlen(w), env_len = doms[q].j - doms[q].i + 1;
+    const size_t t = doms[q].t;
+    const int L = wlen(t), env_len = doms[q].j - doms[q].i + 1;
     const float env_sc = dsc[q] + ((float)L - (float)env_len) * logf((float)L / ((float)L + 3.0f));
-    const double pv = ((double)env_sc - (nullsc[w] + 0.0)) / LN2v;
-    if (exp_surv(pv, o.gfmu, o.gflambda) > o.f5) continue;
-    ixg_envacct &a = acct[wins[w].task];
+    const float cr = corr(t);
+    double pv = ((double)env_sc - (nullsc[t] + 0.0)) / LN2v;
+    if (!std::isnan(cr)) pv += (double)cr / LN2v;
+    if (exp_surv(pv, mu(t), lam(t)) > o.f5) continue;
+    ixg_envacct &a = wacct[todo[t]];
     a.n_past_edef++;
     a.pos_past_edef += (uint64_t)env_len;
     if (o.do_edefbias) {
-      const double sb = ((double)env_sc - (double)bias[w]) / LN2v;
-      if (exp_surv(sb, o.gfmu, o.gflambda) > o.f5b) continue;
+      double sb = ((double)env_sc - (double)bias[t]) / LN2v;
+      if (!std::isnan(cr)) sb += (double)cr / LN2v;
+      if (exp_surv(sb, mu(t), lam(t)) > o.f5b) continue;
       a.n_past_edefbias++;
       a.pos_past_edefbias += (uint64_t)env_len;
     }
-    envs[w].push_back({(int32_t)(doms[q].i + wins[w].start - 1), (int32_t)(doms[q].j + wins[w].start - 1)});
+    envs[todo[t]].push_back(
+        {(int32_t)(doms[q].i + W(t).start - 1), (int32_t)(doms[q].j + W(t).start - 1)});
+  }
+  return IXG_OK;
+}
+
+}  // namespace
+
+// trunc_env_def (cm_search.rs:2449) over the windows of several (chunk,
+// pass) groups, each group's windows consecutive and in the reference's
+// order, windows in the coordinates of their task's sequence view. The
+// enforce rules of the 5' / 3' passes (the window must start at the view's
+// first / end at its last residue, cm_search.rs:2497-2502) are applied here.
+// envs[w]: the window's surviving envelopes; wacct[w]: its accounting.
+// Every window runs in one wave; a group whose truncated-pass profile a
+// multidomain region changed runs its later windows again in the next wave
+// with the changed profile.
+int env_def_passes(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<HostTask> &tasks,
+                   const std::vector<EnvWin> &wins, const ixg_envopts &o,
+                   std::vector<std::vector<std::pair<int32_t, int32_t>>> &envs, std::vector<ixg_envacct> &wacct) {
+  const size_t nw = wins.size();
+  envs.assign(nw, {});
+  wacct.assign(nw, ixg_envacct{});
+  std::vector<size_t> todo;
+  for (size_t w = 0; w < nw; w++) {
+    const EnvWin &x = wins[w];
+    const bool first = x.pass == IXG_PASS_5P_ONLY_FORCE || x.pass == IXG_PASS_5P_AND_3P_FORCE;
+    const bool final = x.pass == IXG_PASS_3P_ONLY_FORCE || x.pass == IXG_PASS_5P_AND_3P_FORCE;
+    if (first && x.start != 1) continue;
+    if (final && x.end != tasks[x.task].L) continue;
+    todo.push_back(w);
+  }
+  std::vector<int> dirty(nw, 0), dirtied(nw, 0);
+  auto same_group = [&](size_t a, size_t b) { return wins[a].task == wins[b].task && wins[a].pass == wins[b].pass; };
+  while (!todo.empty()) {
+    for (size_t w : todo) envs[w].clear(), wacct[w] = ixg_envacct{}, dirtied[w] = 0;
+    RCHK(env_def_wave(c, p, d_seq, tasks, wins, todo, dirty, o, envs, wacct, dirtied));
+    // the windows after a group's first dirtying window go again
+    std::vector<size_t> next;
+    for (size_t q = 0; q < todo.size(); q++) {
+      const size_t w = todo[q];
+      if (!dirtied[w]) continue;
+      size_t r = q + 1;
+      for (; r < todo.size() && same_group(todo[r], w); r++) {
+        dirty[todo[r]] = dirtied[w];
+        next.push_back(todo[r]);
+      }
+      q = r - 1;
+    }
+    todo.swap(next);
+  }
+  return IXG_OK;
+}
+
+// The STD-pass envelope definition over windows of several tasks (windows
+// in chunk coordinates of their task). envs[w]: the window's surviving
+// envelopes (chunk coordinates), in the reference's order; acct per task.
+int env_def_batch(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<HostTask> &tasks,
+                  const std::vector<WinRef> &wins, const ixg_envopts &o,
+                  std::vector<std::vector<std::pair<int32_t, int32_t>>> &envs, std::vector<ixg_envacct> &acct) {
+  std::vector<EnvWin> ew(wins.size());
+  for (size_t w = 0; w < wins.size(); w++) ew[w] = {wins[w].task, wins[w].start, wins[w].end, IXG_PASS_STD_ANY};
+  std::vector<ixg_envacct> wa;
+  RCHK(env_def_passes(c, p, d_seq, tasks, ew, o, envs, wa));
+  acct.assign(tasks.size(), ixg_envacct{});
+  for (size_t w = 0; w < wins.size(); w++) {
+    const uint64_t *s = (const uint64_t *)&wa[w];
+    uint64_t *d = (uint64_t *)&acct[wins[w].task];
+    for (size_t q = 0; q < sizeof(ixg_envacct) / 8; q++) d[q] += s[q];
   }
   return IXG_OK;
 }
@@ -557,9 +683,12 @@ int ixg_gfb_scores(ixg_ctx *c, const ixg_p7 *p, const uint8_t *seq, int64_t seq_
   return IXG_OK;
 }
 
-int ixg_env_def(ixg_ctx *c, const ixg_p7 *p, const uint8_t *dsq, int64_t L, const ixg_window *wins, int64_t nwin,
-                const ixg_envopts *o, int32_t *es, int32_t *ee, int64_t cap, int64_t *nenv, ixg_envacct *acct) {
-  if (!c || !p || !dsq || (nwin && !wins) || nwin < 0 || !o || !nenv || (cap && (!es || !ee))) {
+int ixg_env_def_pass(ixg_ctx *c, const ixg_p7 *p, const uint8_t *dsq, int64_t L, const ixg_window *wins,
+                     int64_t nwin, int pass_idx, const ixg_envopts *o, int32_t *es, int32_t *ee, int64_t cap,
+                     int64_t *nenv, ixg_envacct *acct) {
+  if (!c || !p || !dsq || (nwin && !wins) || nwin < 0 || !o || !nenv || (cap && (!es || !ee)) ||
+      pass_idx < IXG_PASS_STD_ANY || pass_idx > IXG_PASS_5P_AND_3P_FORCE ||
+      (pass_idx == IXG_PASS_3P_ONLY_FORCE && o->clen < 1)) {
     g_err = "ixg_env_def: bad arguments";
     return IXG_EINVAL;
   }
@@ -582,20 +711,30 @@ int ixg_env_def(ixg_ctx *c, const ixg_p7 *p, const uint8_t *dsq, int64_t L, cons
   RCHK(scratch(c, S_SEQ, (size_t)L + 2, (void **)&d));
   HIPCHK(hipMemcpyAsync(d, dsq, (size_t)L + 2, hipMemcpyHostToDevice, c->stream));
   std::vector<HostTask> tasks{{0, L}};
-  std::vector<WinRef> wl(nwin);
-  for (int64_t w = 0; w < nwin; w++) wl[w] = {0, wins[w].start, wins[w].end};
+  std::vector<EnvWin> ew(nwin);
+  for (int64_t w = 0; w < nwin; w++) ew[w] = {0, wins[w].start, wins[w].end, pass_idx};
   std::vector<std::vector<std::pair<int32_t, int32_t>>> envs;
-  std::vector<ixg_envacct> ac;
-  RCHK(env_def_batch(c, p, d, tasks, wl, *o, envs, ac));
+  std::vector<ixg_envacct> wa;
+  RCHK(env_def_passes(c, p, d, tasks, ew, *o, envs, wa));
   int64_t n = 0;
-  for (auto &v : envs)
-    for (auto &e : v) {
+  ixg_envacct tot{};
+  for (int64_t w = 0; w < nwin; w++) {
+    for (auto &e : envs[w]) {
       if (n < cap) es[n] = e.first, ee[n] = e.second;
       n++;
     }
+    const uint64_t *s = (const uint64_t *)&wa[w];
+    uint64_t *dd = (uint64_t *)&tot;
+    for (size_t q = 0; q < sizeof(ixg_envacct) / 8; q++) dd[q] += s[q];
+  }
   *nenv = n;
-  if (acct) *acct = ac[0];
+  if (acct) *acct = tot;
   return n > cap ? IXG_ERANGE : IXG_OK;
+}
+
+int ixg_env_def(ixg_ctx *c, const ixg_p7 *p, const uint8_t *dsq, int64_t L, const ixg_window *wins, int64_t nwin,
+                const ixg_envopts *o, int32_t *es, int32_t *ee, int64_t cap, int64_t *nenv, ixg_envacct *acct) {
+  return ixg_env_def_pass(c, p, dsq, L, wins, nwin, IXG_PASS_STD_ANY, o, es, ee, cap, nenv, acct);
 }
 
 }  // extern "C"

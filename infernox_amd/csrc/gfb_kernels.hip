@@ -41,6 +41,14 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// The transition table of an item's profile: the profiles of one model
+// (glocal, and the truncated passes' 5', 3' and 5'&3' profiles) differ only
+// in their begin transitions and sit one after another; flags bits 2-3 pick
+// one. MD and DD, staged in LDS, are the same in all of them.
+__device__ __forceinline__ const float *item_tsc(const GmDev &g, const GfbItem &item) {
+  return g.tsc + (size_t)((item.flags >> 2) & 3) * (size_t)(g.m + 1) * 8;
+}
+
 struct GfbShared {
   float *tbl, *md, *dd, *bm;  // table; per k = 0..M: MD, DD, BM (tsc row k)
   float *wave;                // per wave: 3 rows of M+2 floats
@@ -81,18 +89,14 @@ struct LaneTsc {
 // the chain in the reference's order. The next row's emission scores are
 // fetched while the chain runs.
 template <int KPL, bool LE>
-__global__ __launch_bounds__(64 * GFB_WAVES) void gfwd_kernel(GmDev g, const uint8_t *seq, const GfbItem *items,
-                                                              int32_t n, float *xout, float *mout, float *sc) {
-  extern __shared__ float smem[];
-  const GfbShared S = gfb_stage(g, smem);
+__device__ __forceinline__ void gfwd_item(const GmDev &g, const GfbShared &S, const uint8_t *seq,
+                                          const GfbItem &item, int64_t it, float *xout, float *mout, float *sc) {
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t it = (int64_t)blockIdx.x * GFB_WAVES + wv;
-  if (it >= n) return;
   const int M = g.m, W = M + 2;
+  const float *tsc = item_tsc(g, item);
   // sA[k] = M(i,k-1) + MD(k-1) (k = 1..M), sD[k] = D(i,k), sM[k] = M(i,k),
   // sM[M+1] = the row's B for the next row
   float *sA = S.wave + (size_t)wv * 3 * W, *sD = sA + W, *sM = sD + W;
-  const GfbItem item = items[it];
   const int L = item.len;
   const bool keep = item.flags & 2;
   float xs[8];
@@ -107,7 +111,7 @@ __global__ __launch_bounds__(64 * GFB_WAVES) void gfwd_kernel(GmDev g, const uin
 #pragma unroll
   for (int j = 0; j < KPL; j++) {
     kk[j] = kb + j <= M ? kb + j : M;
-    const float *t = g.tsc + (size_t)(kk[j] - 1) * 8, *tk = g.tsc + (size_t)kk[j] * 8;
+    const float *t = tsc + (size_t)(kk[j] - 1) * 8, *tk = tsc + (size_t)kk[j] * 8;
     T.mm[j] = t[T_MM];
     T.im[j] = t[T_IM];
     T.dm[j] = t[T_DM];
@@ -226,18 +230,14 @@ __global__ __launch_bounds__(64 * GFB_WAVES) void gfwd_kernel(GmDev g, const uin
 // ends (LE) the inner FLogsums with E stay, in the reference's order. The
 // next row's emission scores are fetched while lane 0 works.
 template <int KPL, bool LE>
-__global__ __launch_bounds__(64 * GFB_WAVES) void gbwd_kernel(GmDev g, const uint8_t *seq, const GfbItem *items,
-                                                              int32_t n, float *xout, float *sc) {
-  extern __shared__ float smem[];
-  const GfbShared S = gfb_stage(g, smem);
+__device__ __forceinline__ void gbwd_item(const GmDev &g, const GfbShared &S, const uint8_t *seq,
+                                          const GfbItem &item, int64_t it, float *xout, float *sc) {
   const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-  const int64_t it = (int64_t)blockIdx.x * GFB_WAVES + wv;
-  if (it >= n) return;
   const int M = g.m, W = M + 2;
+  const float *tsc = item_tsc(g, item);
   // sT: the B terms, then D(i,k) (sT[M] = E); sP1 / sP2: the M / D parts
   // from row i+1
   float *sT = S.wave + (size_t)wv * 3 * W, *sP1 = sT + W, *sP2 = sP1 + W;
-  const GfbItem item = items[it];
   const int L = item.len;
   const float esc = LE ? 0.0f : -INFINITY;
   float xs[8];
@@ -253,14 +253,14 @@ __global__ __launch_bounds__(64 * GFB_WAVES) void gbwd_kernel(GmDev g, const uin
   for (int j = 0; j < KPL; j++) {
     kk[j] = kb + j <= M ? kb + j : M;
     kn[j] = kb + j < M ? kb + j + 1 : M;  // k + 1, clamped
-    const float *t = g.tsc + (size_t)kk[j] * 8;
+    const float *t = tsc + (size_t)kk[j] * 8;
     tmm[j] = t[T_MM];
     tmi[j] = t[T_MI];
     tim[j] = t[T_IM];
     tii[j] = t[T_II];
     tdm[j] = t[T_DM];
     tmd[j] = t[T_MD];
-    tbm[j] = g.tsc[(size_t)(kk[j] - 1) * 8 + T_BM];
+    tbm[j] = tsc[(size_t)(kk[j] - 1) * 8 + T_BM];
   }
   // row L (p7_generic.rs:946-962)
   float xJ = -INFINITY, xB = -INFINITY, xN = -INFINITY, xC = xs[XS_CM];
@@ -397,41 +397,64 @@ __global__ __launch_bounds__(64 * GFB_WAVES) void gbwd_kernel(GmDev g, const uin
   }
 }
 
-template <int KPL, bool LE>
+// One item per wave; an item's local-end mode (flags bit 0) selects the
+// body, uniformly over the wave.
+template <int KPL>
+__global__ __launch_bounds__(64 * GFB_WAVES) void gfwd_kernel(GmDev g, const uint8_t *seq, const GfbItem *items,
+                                                              int32_t n, float *xout, float *mout, float *sc) {
+  extern __shared__ float smem[];
+  const GfbShared S = gfb_stage(g, smem);
+  const int64_t it = (int64_t)blockIdx.x * GFB_WAVES + (threadIdx.x >> 6);
+  if (it >= n) return;
+  const GfbItem item = items[it];
+  if (item.flags & 1) gfwd_item<KPL, true>(g, S, seq, item, it, xout, mout, sc);
+  else gfwd_item<KPL, false>(g, S, seq, item, it, xout, mout, sc);
+}
+template <int KPL>
+__global__ __launch_bounds__(64 * GFB_WAVES) void gbwd_kernel(GmDev g, const uint8_t *seq, const GfbItem *items,
+                                                              int32_t n, float *xout, float *sc) {
+  extern __shared__ float smem[];
+  const GfbShared S = gfb_stage(g, smem);
+  const int64_t it = (int64_t)blockIdx.x * GFB_WAVES + (threadIdx.x >> 6);
+  if (it >= n) return;
+  const GfbItem item = items[it];
+  if (item.flags & 1) gbwd_item<KPL, true>(g, S, seq, item, it, xout, sc);
+  else gbwd_item<KPL, false>(g, S, seq, item, it, xout, sc);
+}
+
+template <int KPL>
 hipError_t gfb_launch(const GmDev &g, const uint8_t *seq, const GfbItem *items, int64_t n, float *xout,
                       float *mout, float *sc, bool backward, hipStream_t s) {
   const size_t lds = ((size_t)GFB_TBL + 3 * (size_t)(g.m + 1) + (size_t)GFB_WAVES * 3 * (g.m + 2)) * sizeof(float);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)((n + GFB_WAVES - 1) / GFB_WAVES);
   if (backward)
-    hipLaunchKernelGGL((gbwd_kernel<KPL, LE>), dim3(grid), dim3(64 * GFB_WAVES), lds, s, g, seq, items, (int32_t)n, xout,
+    hipLaunchKernelGGL((gbwd_kernel<KPL>), dim3(grid), dim3(64 * GFB_WAVES), lds, s, g, seq, items, (int32_t)n, xout,
                        sc);
   else
-    hipLaunchKernelGGL((gfwd_kernel<KPL, LE>), dim3(grid), dim3(64 * GFB_WAVES), lds, s, g, seq, items, (int32_t)n, xout,
+    hipLaunchKernelGGL((gfwd_kernel<KPL>), dim3(grid), dim3(64 * GFB_WAVES), lds, s, g, seq, items, (int32_t)n, xout,
                        mout, sc);
   return hipGetLastError();
 }
 
-template <bool LE>
 hipError_t launch_gfb_kpl(const GmDev &g, const uint8_t *seq, const GfbItem *items, int64_t n, float *xout,
                           float *mout, float *sc, bool backward, hipStream_t s) {
   if (n == 0) return hipSuccess;
   if (g.m < 2 || n > INT32_MAX) return hipErrorInvalidValue;
   const int kpl = (g.m + 63) / 64;
-  if (kpl <= 1) return gfb_launch<1, LE>(g, seq, items, n, xout, mout, sc, backward, s);
-  if (kpl <= 2) return gfb_launch<2, LE>(g, seq, items, n, xout, mout, sc, backward, s);
-  if (kpl <= 4) return gfb_launch<4, LE>(g, seq, items, n, xout, mout, sc, backward, s);
-  if (kpl <= 8) return gfb_launch<8, LE>(g, seq, items, n, xout, mout, sc, backward, s);
-  if (kpl <= 16) return gfb_launch<16, LE>(g, seq, items, n, xout, mout, sc, backward, s);
+  if (kpl <= 1) return gfb_launch<1>(g, seq, items, n, xout, mout, sc, backward, s);
+  if (kpl <= 2) return gfb_launch<2>(g, seq, items, n, xout, mout, sc, backward, s);
+  if (kpl <= 4) return gfb_launch<4>(g, seq, items, n, xout, mout, sc, backward, s);
+  if (kpl <= 8) return gfb_launch<8>(g, seq, items, n, xout, mout, sc, backward, s);
+  if (kpl <= 16) return gfb_launch<16>(g, seq, items, n, xout, mout, sc, backward, s);
   return hipErrorInvalidValue;
 }
 
 }  // namespace
 
 hipError_t launch_gfb(const GmDev &g, const uint8_t *seq, const GfbItem *items, int64_t n, float *xout, float *mout,
-                      float *sc, bool backward, bool local_end, hipStream_t s) {
-  if (local_end) return launch_gfb_kpl<true>(g, seq, items, n, xout, mout, sc, backward, s);
-  return launch_gfb_kpl<false>(g, seq, items, n, xout, mout, sc, backward, s);
+                      float *sc, bool backward, hipStream_t s) {
+  return launch_gfb_kpl(g, seq, items, n, xout, mout, sc, backward, s);
 }
 
 }  // namespace ixg

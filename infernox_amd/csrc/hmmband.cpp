@@ -34,6 +34,7 @@ struct ixg_hbmap {
     int32_t lpos, rpos;  // the consensus positions this step reads
   };
   std::vector<Step> walk;
+  std::vector<int32_t> slp, srp;  // per state: consensus span (emit map with the per-node-type offsets)
 };
 
 namespace ixg {
@@ -294,10 +295,12 @@ int rdelta_of(int st) { return st == MP_ST || st == MR_ST || st == IR_ST; }
 
 }  // namespace
 
-// cp9.rs:3380-3794 cp9_hmm2ijbands (do_trunc = false)
+// cp9.rs:3380-3794 cp9_hmm2ijbands; tb != null: do_trunc, with the L/R/T
+// band widening and the truncated special cases (tb's validity updated)
 void hmm2ij(const ixg_hbmap &m, int32_t *pn, int i0, int j0, bool search, int32_t *imin, int32_t *imax,
-            int32_t *jmin, int32_t *jmax) {
+            int32_t *jmin, int32_t *jmax, TrBand *tb) {
   const int M = m.M, H = m.hmm_m;
+  const bool trunc = tb != nullptr;
   for (int v = 0; v < M; v++) imin[v] = jmin[v] = -1, imax[v] = jmax[v] = -2;
   Reach r;
   reach(m, pn, i0, j0, search, r);
@@ -329,7 +332,8 @@ void hmm2ij(const ixg_hbmap &m, int32_t *pn, int i0, int j0, bool search, int32_
         imax[v0] = imax[w] != -2 ? imax[w] : imax[y];
         jmin[v0] = jmin[y] != -1 ? jmin[y] : jmin[w];
         jmax[v0] = jmax[y] != -2 ? jmax[y] : jmax[w];
-        if (imin[w] == -1 || jmin[w] == -1 || imin[y] == -1 || jmin[y] == -1 || imax[w] == -2 || jmax[y] == -2) {
+        if (!trunc &&
+            (imin[w] == -1 || jmin[w] == -1 || imin[y] == -1 || jmin[y] == -1 || imax[w] == -2 || jmax[y] == -2)) {
           clear(v0), clear(w), clear(y), clear(y + 1);
         }
         break;
@@ -398,6 +402,17 @@ void hmm2ij(const ixg_hbmap &m, int32_t *pn, int i0, int j0, bool search, int32_
       default: break;
     }
   }
+  if (trunc)  // cp9.rs:3646-3661: L / R / T alignments widen the j / i bands
+    for (int v = 0; v < M; v++) {
+      if (tb->lv[v] || tb->tv[v]) {
+        jmin[v] = jmin[v] == -1 ? tb->lmarg_jmin : std::min(jmin[v], tb->lmarg_jmin);
+        jmax[v] = jmax[v] == -2 ? tb->lmarg_jmax : std::max(jmax[v], tb->lmarg_jmax);
+      }
+      if (tb->rv[v] || tb->tv[v]) {
+        imin[v] = imin[v] == -1 ? tb->rmarg_imin : std::min(imin[v], tb->rmarg_imin);
+        imax[v] = imax[v] == -2 ? tb->rmarg_imax : std::max(imax[v], tb->rmarg_imax);
+      }
+    }
   if (!search) {
     imin[0] = i0;
     if (imin[1] != -1) imin[1] = i0;
@@ -408,6 +423,7 @@ void hmm2ij(const ixg_hbmap &m, int32_t *pn, int i0, int j0, bool search, int32_
   for (int v = 0; v < M; v++) {
     if (imin[v] == -1 || jmin[v] == -1) clear(v);
     if (v + 1 < M && m.sttype[v + 1] == E_ST) clear(v);  // detached insert
+    if (trunc) continue;
     const int st = m.sttype[v];
     if (st == MP_ST) {
       if (jmax[v] == i0) clear(v);
@@ -420,13 +436,16 @@ void hmm2ij(const ixg_hbmap &m, int32_t *pn, int i0, int j0, bool search, int32_
   }
   // the localized case keeps at least one parse (cp9.rs:3655-3791)
   if ((m.cp9_lb || m.cp9_le || m.cp9_el) && m.cm_local) {
+    if (trunc) tb->jv[0] = 1;
     if (imin[0] == -1) set(0, i0, i0, j0, j0);
     int nd1 = 1;
     if (i0 == j0)
       while (nd1 < m.nodes && m.ndtype[nd1] == MATP_ND) nd1++;
     const int v = m.nodemap[nd1];
+    if (trunc) tb->jv[v] = 1;  // cp9.rs:3710-3714 (the B node's S children too) and :3756
     if (m.ndtype[nd1] == BIF_ND) {
       const int w = m.cfirst[v], y = m.cnum[v];
+      if (trunc) tb->jv[w] = tb->jv[y] = 1;
       if (imin[v] != -1 && imin[w] != -1 && imin[y] != -1) {
         if (!search) {
           imin[v] = std::min(imin[v], i0), imax[v] = std::max(imax[v], i0);
@@ -453,6 +472,108 @@ void hmm2ij(const ixg_hbmap &m, int32_t *pn, int i0, int j0, bool search, int32_
       set(0, std::min(imin[0], imin[v]), std::max(imax[0], imax[v]), std::min(jmin[0], jmin[v]),
           std::max(jmax[0], jmax[v]));
     }
+  }
+}
+
+// cp9.rs:1755-1916 cp9_predict_start_and_end_positions: the first nodes
+// (from the left) whose posterior occupancy passes thresh1 / thresh2 start
+// the alignment, the first from the right end it; the R marginal i band
+// comes from the start nodes' posterior bands, the L marginal j band from
+// the end nodes' (a delete's j is one less). pn as the CP9 DPs return it.
+void tr_predict(const int32_t *pn, const int32_t *occ, int H, int i0, int j0, TrBand &tb) {
+  const int H1 = H + 1;
+  const int32_t *mn = pn, *mx = pn + H1, *in = pn + 2 * H1, *ix = pn + 3 * H1, *dn = pn + 4 * H1, *dx = pn + 5 * H1;
+  auto prob = [](int32_t sc) {  // Score2Prob(sc, 1.0), cp9.rs:1741
+    return sc == NINF_I ? 0.0f : (float)std::exp((double)((float)sc / 1000.0f) * 0.69314718);
+  };
+  auto banded = [&](int k) { return !(mn[k] == -1 && in[k] == -1 && dn[k] == -1); };
+  int sp1 = -1, sp2 = -1, ep1, ep2, k = 1;
+  for (; k <= H && (sp1 == -1 || sp2 == -1); k++)
+    if (banded(k)) {
+      const float p = prob(occ[k]);
+      if (sp1 == -1 && p > tb.thresh1) sp1 = k;
+      if (sp2 == -1 && p > tb.thresh2) sp2 = k;
+    }
+  if (k == H + 1) {
+    if (sp1 == -1) sp1 = H + 1;
+    if (sp2 == -1) sp2 = H + 1;
+  }
+  if (sp1 == H + 1 && sp2 == H + 1) {
+    ep1 = ep2 = 0;
+  } else {
+    ep1 = ep2 = -1;
+    for (k = H; k >= 1 && (ep1 == -1 || ep2 == -1); k--)
+      if (banded(k)) {
+        const float p = prob(occ[k]);
+        if (ep1 == -1 && p > tb.thresh1) ep1 = k;
+        if (ep2 == -1 && p > tb.thresh2) ep2 = k;
+      }
+    if (k == 0) {
+      if (ep1 == -1) ep1 = 0;
+      if (ep2 == -1) ep2 = 0;
+    }
+  }
+  tb.sp1 = sp1, tb.sp2 = sp2, tb.ep1 = ep1, tb.ep2 = ep2;
+  if (sp1 == H + 1) {
+    tb.rmarg_imin = i0, tb.rmarg_imax = j0;
+  } else {
+    int lo = INT_MAX, hi = INT_MIN;
+    for (int s : {sp1, sp2}) {
+      if (s == H + 1) continue;
+      for (const int32_t *a : {mn, in, dn})
+        if (a[s] >= 0) lo = std::min(lo, a[s]);
+      for (const int32_t *a : {mx, ix, dx})
+        if (a[s] >= 0) hi = std::max(hi, a[s]);
+    }
+    if (lo == INT_MAX || sp2 == H + 1) lo = i0;
+    if (hi == INT_MIN || sp2 == H + 1) hi = j0 + 1;
+    tb.rmarg_imin = std::min(j0 + 1, std::max(i0, lo));
+    tb.rmarg_imax = std::min(j0 + 1, std::max(i0, hi));
+  }
+  if (ep1 == 0) {
+    tb.lmarg_jmin = i0 - 1, tb.lmarg_jmax = j0;
+  } else {
+    int lo = INT_MAX, hi = INT_MIN;
+    for (int e : {ep1, ep2}) {
+      if (e == 0) continue;
+      if (mn[e] >= 0) lo = std::min(lo, mn[e]);
+      if (in[e] >= 0) lo = std::min(lo, in[e]);
+      if (dn[e] >= 0) lo = std::min(lo, dn[e] - 1);
+      if (mx[e] >= 0) hi = std::max(hi, mx[e]);
+      if (ix[e] >= 0) hi = std::max(hi, ix[e]);
+      if (dx[e] >= 0) hi = std::max(hi, dx[e] - 1);
+    }
+    if (lo == INT_MAX || ep2 == 0) lo = i0 - 1;
+    if (hi == INT_MIN || ep2 == 0) hi = j0;
+    tb.lmarg_jmin = std::min(j0, std::max(i0 - 1, lo));
+    tb.lmarg_jmax = std::min(j0, std::max(i0 - 1, hi));
+  }
+}
+
+// cp9.rs:1918-1990 cp9_marginal_candidates_from_start_end_positions: which
+// marginal modes may pass through each state given the predicted start and
+// end nodes and the pass's fill flags; the root takes every mode a child
+// of its kind allows.
+void tr_candidates(const ixg_hbmap &m, int pass, TrBand &tb) {
+  const bool al = pass == IXG_PASS_3P_ONLY_FORCE || pass == IXG_PASS_5P_AND_3P_FORCE || pass == IXG_PASS_5P_AND_3P_ANY;
+  const bool ar = pass == IXG_PASS_5P_ONLY_FORCE || pass == IXG_PASS_5P_AND_3P_FORCE || pass == IXG_PASS_5P_AND_3P_ANY;
+  const bool at = pass == IXG_PASS_5P_AND_3P_FORCE || pass == IXG_PASS_5P_AND_3P_ANY;
+  const int M = m.M;
+  tb.jv.assign(M, 0), tb.lv.assign(M, 0), tb.rv.assign(M, 0), tb.tv.assign(M, 0);
+  for (int v = 0; v < M; v++) {
+    const int lp = m.slp[v], rp = m.srp[v];
+    tb.jv[v] = lp >= tb.sp1 && rp <= tb.ep1;
+    tb.lv[v] = al && lp >= tb.sp1 && lp <= tb.ep1 && rp > tb.ep2;
+    tb.rv[v] = ar && rp <= tb.ep1 && rp >= tb.sp1 && lp < tb.sp2;
+    tb.tv[v] = m.sttype[v] == B_ST && at && lp < tb.sp2 && rp > tb.ep2;
+  }
+  for (int v = 0; v < M; v++) {
+    const int st = m.sttype[v];
+    if ((st == B_ST || st == MP_ST || st == ML_ST || st == IL_ST || st == MR_ST || st == IR_ST) && tb.jv[v]) tb.jv[0] = 1;
+    if ((st == B_ST || st == MP_ST || st == ML_ST || st == IL_ST) && tb.lv[v]) tb.lv[0] = 1;
+    if ((st == B_ST || st == MP_ST || st == MR_ST || st == IR_ST) && tb.rv[v]) tb.rv[0] = 1;
+    if (st == B_ST && tb.tv[v]) tb.tv[0] = 1;
+    if (tb.jv[0] && tb.lv[0] && tb.rv[0] && tb.tv[0]) break;
   }
 }
 
@@ -546,6 +667,20 @@ int ixg_hbmap_create(const ixg_cm_params *cm, int cm_local, const ixg_cp9_params
       return IXG_EINVAL;
     }
   build_walk(*m);
+  {
+    std::vector<int32_t> ndidx(m->M), lp, rp;
+    for (int nd = 0, v = 0; v < m->M; v++) {
+      while (nd + 1 < m->nodes && m->nodemap[nd + 1] <= v) nd++;
+      ndidx[v] = nd;
+    }
+    emit_positions(*m, ndidx, lp, rp);
+    m->slp.resize(m->M), m->srp.resize(m->M);
+    for (int v = 0; v < m->M; v++) {
+      const int nd = ndidx[v], t = m->ndtype[nd];
+      m->slp[v] = (t == MATP_ND || t == MATL_ND) ? lp[nd] : lp[nd] + 1;
+      m->srp[v] = (t == MATP_ND || t == MATR_ND) ? rp[nd] : rp[nd] - 1;
+    }
+  }
   int clen = 0;
   for (int nd = 0; nd < m->nodes; nd++) clen += m->ndtype[nd] == MATP_ND ? 2 : (m->ndtype[nd] == MATL_ND || m->ndtype[nd] == MATR_ND);
   if (clen != H) {
@@ -567,7 +702,30 @@ int ixg_hmm2ij_bands(const ixg_hbmap *m, int32_t *pn, int32_t i0, int32_t j0, in
     g_err = "ixg_hmm2ij_bands: bad arguments";
     return IXG_EINVAL;
   }
-  hmm2ij(*m, pn, i0, j0, doing_search != 0, imin, imax, jmin, jmax);
+  hmm2ij(*m, pn, i0, j0, doing_search != 0, imin, imax, jmin, jmax, nullptr);
+  return IXG_OK;
+}
+
+int ixg_hmm2ij_bands_trunc(const ixg_hbmap *m, int32_t *pn, const int32_t *occ, int32_t i0, int32_t j0, int pass_idx,
+                           int32_t *imin, int32_t *imax, int32_t *jmin, int32_t *jmax, uint8_t *jvalid,
+                           uint8_t *lvalid, uint8_t *rvalid, uint8_t *tvalid, int32_t *spep) {
+  if (!m || !pn || !occ || !imin || !imax || !jmin || !jmax || !jvalid || !lvalid || !rvalid || !tvalid || i0 < 1 ||
+      j0 < i0 || pass_idx < IXG_PASS_5P_ONLY_FORCE || pass_idx > IXG_PASS_5P_AND_3P_ANY) {
+    g_err = "ixg_hmm2ij_bands_trunc: bad arguments";
+    return IXG_EINVAL;
+  }
+  TrBand tb;
+  tr_predict(pn, occ, m->hmm_m, i0, j0, tb);
+  tr_candidates(*m, pass_idx, tb);
+  hmm2ij(*m, pn, i0, j0, true, imin, imax, jmin, jmax, &tb);
+  std::copy(tb.jv.begin(), tb.jv.end(), jvalid);
+  std::copy(tb.lv.begin(), tb.lv.end(), lvalid);
+  std::copy(tb.rv.begin(), tb.rv.end(), rvalid);
+  std::copy(tb.tv.begin(), tb.tv.end(), tvalid);
+  if (spep) {
+    const int32_t v[8] = {tb.sp1, tb.sp2, tb.ep1, tb.ep2, tb.rmarg_imin, tb.rmarg_imax, tb.lmarg_jmin, tb.lmarg_jmax};
+    std::copy(v, v + 8, spep);
+  }
   return IXG_OK;
 }
 

@@ -71,12 +71,19 @@ struct GmHost {
   float xsc[4][2] = {};
   float nj = 0.f;
   int len = 0;
+  bool local_end = false;  // exits from every position (esc = 0)
   void set_length(int L);
+  void set_length_5p(int L);
+  void set_length_3p(int L);
   void set_unihit();
   void set_multihit();
   void xsc8(float out[8]) const;
 };
 void build_glocal(const ixg_p7_hmm &h, GmHost &g);
+// The truncated passes' profiles from the glocal one (p7_generic.rs:259-348,
+// as cm_search.rs:1011-1016 configures them at L = 100): [0] 5' (Rgm), [1]
+// 3' (Lgm), [2] 5'&3' (Tgm).
+void build_trunc_profiles(const ixg_p7_hmm &h, const GmHost &gm, GmHost out[3]);
 const std::vector<float> &p7_logsum_table();
 float p7_logsum(float a, float b);
 void domain_decoding(const GmHost &g, int L, const float *fx, const float *bx, float *btot, float *etot,
@@ -153,8 +160,9 @@ struct GmDev {
 };
 // One glocal Forward / Backward item: the dsq at seq + off (1-indexed, len
 // residues), its special-state scores [E N J C][LOOP MOVE], flags bit 0 =
-// local ends (esc = 0; the same for every item of a launch), bit 1 = keep
-// the Forward matrix; the specials go to
+// local ends (esc = 0), bit 1 = keep the Forward matrix, bits 2-3 = the
+// profile (0 glocal, 1 5', 2 3', 3 5'&3'; GmDev.tsc holds the four transition
+// tables one after another); the specials go to
 // xout + xoff [(len+1)][E N J B C], the matrix to mout + moff
 // [(len+1)][M | I | D][M+1].
 struct GfbItem {
@@ -164,7 +172,7 @@ struct GfbItem {
   int64_t xoff, moff;
 };
 hipError_t launch_gfb(const GmDev &g, const uint8_t *seq, const GfbItem *items, int64_t n, float *xout, float *mout,
-                      float *sc, bool backward, bool local_end, hipStream_t s);
+                      float *sc, bool backward, hipStream_t s);
 
 // Kernel launchers (p7_kernels.hip).
 constexpr int SWEEP_SEG = 4096;  // rows per F1 sweep work item (multiple of 64)

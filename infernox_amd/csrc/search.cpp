@@ -1,12 +1,14 @@
-// The STD pass of a cmsearch over one resident sequence, F1 through F7, as
+// A cmsearch over one resident sequence, F1 through F7, as
 // FaithfulSearcher::search runs it for every (chunk x strand) task
 // (cm_search.rs:1300, :1774 window_tasks, :1813 run_window_task, :2214
-// trunc_pipeline_one_strand with the STD pass, :2707 cm_std_stage, :1915
+// trunc_pipeline_one_strand, :2707 cm_std_stage, :2863 cm_trunc_stage, :1915
 // finalize), with every task's stage batched into one device pass:
 //
-//   F1-F3b   ixg_f3_filter_tasks: one SSV sweep over all tasks, one
-//            Viterbi/Forward/bias launch over all surviving windows
-//   F4-F5    env_def_batch: glocal Forward/Backward launches over all windows,
+//   F1-F3b   ixg_f3_filter_tasks: one SSV sweep over all chunks and the
+//            truncated passes' subsequences, one Viterbi/Forward/bias launch
+//            over all surviving windows
+//   F4-F5    env_def_passes: glocal Forward/Backward launches over all
+//            windows of every pass (each item with its pass's profile),
 //            domain definition on the host
 //   F6       CP9 Forward/Backward/posterior bands of every envelope (one launch
 //            per tau round), HMM->CM band conversion on host workers, tau
@@ -15,8 +17,10 @@
 //   F7       the same band step at the final tau, one HB Inside launch with
 //            greedy hit reporting and null3; per-envelope overlap removal
 //
-// then genome coordinates, E-values, global overlap removal and the final
-// sort on the host. Envelope bands and scans work in the coordinates of the
+// then, per truncated pass, cp9_seq2bands_trunc, the TrCYK filter and
+// FTrInside, each one launch over the pass's envelopes of every chunk; then
+// genome coordinates, E-values, global overlap removal and the final sort
+// on the host. Envelope bands and scans work in the coordinates of the
 // task's strand buffer on the device, so no sequence is copied per task.
 #include <algorithm>
 #include <chrono>
@@ -27,15 +31,17 @@
 using namespace ixg;
 
 namespace ixg {
-int env_def_batch(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<HostTask> &tasks,
-                  const std::vector<WinRef> &wins, const ixg_envopts &o,
-                  std::vector<std::vector<std::pair<int32_t, int32_t>>> &envs, std::vector<ixg_envacct> &acct);
+int env_def_passes(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<HostTask> &tasks,
+                   const std::vector<EnvWin> &wins, const ixg_envopts &o,
+                   std::vector<std::vector<std::pair<int32_t, int32_t>>> &envs, std::vector<ixg_envacct> &wacct);
 int cp9_posterior_bands(ixg_ctx *c, const ixg_cp9 *h, const std::vector<const uint8_t *> &dsq,
                         const std::vector<int32_t> &i0, const std::vector<int32_t> &j0, double tau, int mode,
-                        std::vector<int32_t> &pn);
+                        std::vector<int32_t> &pn, std::vector<int32_t> *occ = nullptr);
 int cp9_nodes(const ixg_cp9 *h);
 void hmm2ij(const ixg_hbmap &m, int32_t *pn, int i0, int j0, bool search, int32_t *imin, int32_t *imax,
-            int32_t *jmin, int32_t *jmax);
+            int32_t *jmin, int32_t *jmax, TrBand *tb);
+void tr_predict(const int32_t *pn, const int32_t *occ, int H, int i0, int j0, TrBand &tb);
+void tr_candidates(const ixg_hbmap &m, int pass, TrBand &tb);
 int64_t ij2d(const ixg_hbmap &m, const int32_t *imin, const int32_t *imax, const int32_t *jmin, const int32_t *jmax,
              bool trunc, int32_t *hd_off, std::vector<int32_t> &hdmin, std::vector<int32_t> &hdmax);
 float hb_mx_mb(const ixg_hbmap &m, const int32_t *jmin, const int32_t *jmax, const int32_t *hd_off,
@@ -100,7 +106,7 @@ int iterate_bands(ixg_ctx *c, const ixg_cp9 *cp9, const ixg_hbmap *map, const ui
         b.jmax.assign(M, 0);
         b.hd_off.assign(M + 1, 0);
         hmm2ij(*map, pn.data() + g * 6 * (size_t)(H + 1), i0[g], j0[g], true, imin.data(), imax.data(),
-               b.jmin.data(), b.jmax.data());
+               b.jmin.data(), b.jmax.data(), nullptr);
         ij2d(*map, imin.data(), imax.data(), b.jmin.data(), b.jmax.data(), false, b.hd_off.data(), b.hdmin, b.hdmax);
         mb[g] = hb_mx_mb(*map, b.jmin.data(), b.jmax.data(), b.hd_off.data(), b.hdmin.data(), b.hdmax.data(),
                          j0[g] - i0[g] + 1, nullptr);
@@ -121,28 +127,90 @@ int iterate_bands(ixg_ctx *c, const ixg_cp9 *cp9, const ixg_hbmap *map, const ui
   return IXG_OK;
 }
 
+// cp9_seq2bands_trunc (cp9.rs:4130) over a batch of envelopes of one pass:
+// the pass's CP9 Forward / Backward (each scanning unless the pass forces
+// its first / final residue) with posterior bands and occupancy on the
+// device, then on host workers the start / end prediction, the J/L/R/T
+// candidacy, cp9_hmm2ijbands and ij2d_bands with do_trunc. No size limit
+// applies (cm_trunc_stage calls it once per envelope and tau).
+struct TrEnvBands {
+  EnvBands b;
+  std::vector<uint8_t> jv, lv, rv, tv;
+  ixg_trvalid valid() const { return ixg_trvalid{jv.data(), lv.data(), rv.data(), tv.data()}; }
+};
+int trunc_bands(ixg_ctx *c, const ixg_cp9 *cp9, const ixg_hbmap *map, const uint8_t *dbase, int64_t L,
+                const std::vector<ixg_envelope> &env, int pass, double tau, std::vector<TrEnvBands> &out) {
+  const size_t n = env.size();
+  const int M = hbmap_M(map), H = cp9_nodes(cp9);
+  out.assign(n, {});
+  if (n == 0) return IXG_OK;
+  const bool first = pass == IXG_PASS_5P_ONLY_FORCE || pass == IXG_PASS_5P_AND_3P_FORCE;
+  const bool final = pass == IXG_PASS_3P_ONLY_FORCE || pass == IXG_PASS_5P_AND_3P_FORCE;
+  const int mode = (first ? 0 : IXG_CP9_FWD_SCAN) | (final ? 0 : IXG_CP9_BCK_SCAN);
+  std::vector<const uint8_t *> dsq(n);
+  std::vector<int32_t> i0(n), j0(n), pn, occ;
+  for (size_t k = 0; k < n; k++) {
+    dsq[k] = dbase + (env[k].strand ? L + 2 : 0);
+    i0[k] = env[k].i0;
+    j0[k] = env[k].j0;
+  }
+  RCHK(cp9_posterior_bands(c, cp9, dsq, i0, j0, tau, mode, pn, &occ));
+  par_for((int64_t)n, [&](int64_t k) {
+    TrEnvBands &t = out[k];
+    int32_t *pk = pn.data() + k * 6 * (size_t)(H + 1);
+    TrBand tb;
+    tr_predict(pk, occ.data() + k * (size_t)(H + 1), H, i0[k], j0[k], tb);
+    tr_candidates(*map, pass, tb);
+    std::vector<int32_t> imin(M), imax(M);
+    t.b.jmin.assign(M, 0);
+    t.b.jmax.assign(M, 0);
+    t.b.hd_off.assign(M + 1, 0);
+    hmm2ij(*map, pk, i0[k], j0[k], true, imin.data(), imax.data(), t.b.jmin.data(), t.b.jmax.data(), &tb);
+    ij2d(*map, imin.data(), imax.data(), t.b.jmin.data(), t.b.jmax.data(), true, t.b.hd_off.data(), t.b.hdmin,
+         t.b.hdmax);
+    t.jv = tb.jv, t.lv = tb.lv, t.rv = tb.rv, t.tv = tb.tv;
+  });
+  return IXG_OK;
+}
+
 struct Hit {
   int32_t task;
   int64_t start, stop;
   float score, bias;
   double evalue;
-  int32_t strand;
+  int32_t strand, pass;
   bool removed = false;
 };
 
+// Genome coordinates of a hit at chunk-local ws..we of task k (cm_search.rs:1877-1883).
+void genome_coords(const ixg_task &k, int64_t ws, int64_t we, Hit &h) {
+  h.strand = k.strand;
+  if (k.strand) h.start = k.end - ws + 1, h.stop = k.end - we + 1;
+  else h.start = k.start + ws - 1, h.stop = k.start + we - 1;
+}
+
 }  // namespace
 
-extern "C" int ixg_search_std(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const ixg_cp9 *cp9,
-                              const ixg_hbmap *map, const ixg_target *tgt, const ixg_searchopts *o,
-                              ixg_searchhit *out, int64_t cap, int64_t *nout, ixg_searchacct *acct) {
-  if (!c || !p || !cm || !cp9 || !map || !tgt || !o || !nout || (cap && !out)) {
-    g_err = "ixg_search_std: bad arguments";
+extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const ixg_cp9 *cp9, const ixg_hbmap *map,
+                          const ixg_cp9 *const *trcp9, const ixg_hbmap *const *trmap, const ixg_target *tgt,
+                          const ixg_searchopts *o, ixg_searchhit *out, int64_t cap, int64_t *nout,
+                          ixg_searchacct *acct) {
+  if (!c || !p || !cm || !cp9 || !map || !tgt || !o || !nout || (cap && !out) || o->trunc_mode < 0 ||
+      o->trunc_mode > 1 || (o->trunc_mode && (!trcp9 || !trmap))) {
+    g_err = "ixg_search: bad arguments";
     return IXG_EINVAL;
   }
   if (cp9_nodes(cp9) != hbmap_hmm_m(map)) {
-    g_err = "ixg_search_std: the CP9 is not the band map's";
+    g_err = "ixg_search: the CP9 is not the band map's";
     return IXG_EINVAL;
   }
+  const bool do_trunc = o->trunc_mode == 1;
+  if (do_trunc)
+    for (int q = 0; q < 3; q++)
+      if (!trcp9[q] || !trmap[q] || cp9_nodes(trcp9[q]) != hbmap_hmm_m(trmap[q]) || hbmap_M(trmap[q]) != hbmap_M(map)) {
+        g_err = "ixg_search: a truncated pass's CP9 or band map is missing or does not match";
+        return IXG_EINVAL;
+      }
   HIPCHK(hipSetDevice(c->device));
   using clk = std::chrono::steady_clock;
   const clk::time_point t_begin = clk::now();
@@ -157,61 +225,117 @@ extern "C" int ixg_search_std(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, con
   int64_t L = 0;
   const uint8_t *dbase = ixg_target_device_base(tgt, &L);
   ixg_searchacct A{};
+  const int64_t maxw = o->f3.maxw;
   // (chunk x strand) tasks (cm_search.rs:1774)
-  const int64_t nt = ixg_window_tasks(L, o->f3.maxw, o->do_top, o->do_bot, nullptr, 0);
+  const int64_t nt = ixg_window_tasks(L, maxw, o->do_top, o->do_bot, nullptr, 0);
   std::vector<ixg_task> tasks(nt);
-  ixg_window_tasks(L, o->f3.maxw, o->do_top, o->do_bot, tasks.data(), nt);
-  std::vector<HostTask> ht(nt);
-  for (int64_t t = 0; t < nt; t++) {
+  ixg_window_tasks(L, maxw, o->do_top, o->do_bot, tasks.data(), nt);
+  // the truncated passes a task runs (cm_search.rs:2265-2280) and the
+  // sequence each searches (:2380-2395): the chunk when it is at most maxW
+  // long (or for 5'&3'), else its first (5') / last (3') maxW residues
+  struct TrRun {
+    int32_t task, pass, sub;  // sub: its task in the filter's task list
+    int64_t start_offset;
+  };
+  std::vector<ixg_task> ftasks(tasks);  // chunk tasks, then the truncated passes' subsequences
+  std::vector<TrRun> truns;
+  for (int64_t t = 0; t < nt && do_trunc; t++) {
     const ixg_task &k = tasks[t];
+    const int64_t wl = k.end - k.start + 1;
+    const bool h5 = k.strand ? k.end == L : k.start == 1, h3 = k.strand ? k.start == 1 : k.end == L;
+    const bool run[3] = {h5, h3, h5 && h3 && wl <= maxw};
+    for (int q = 0; q < 3; q++) {
+      if (!run[q]) continue;
+      const int pass = IXG_PASS_5P_ONLY_FORCE + q;
+      A.tr_nres[q] += (uint64_t)std::min<int64_t>(maxw, wl);
+      if (wl <= maxw || pass == IXG_PASS_5P_AND_3P_FORCE) {
+        truns.push_back({(int32_t)t, pass, (int32_t)t, 0});
+        continue;
+      }
+      // a view of the chunk's first / last maxW residues on its strand
+      ixg_task sk{};
+      sk.strand = k.strand;
+      sk.ctx = 0;
+      const bool front = pass == IXG_PASS_5P_ONLY_FORCE;  // first maxW residues of the strand's chunk
+      if ((k.strand == 0) == front) sk.start = k.start, sk.end = k.start + maxw - 1;
+      else sk.start = k.end - maxw + 1, sk.end = k.end;
+      truns.push_back({(int32_t)t, pass, (int32_t)ftasks.size(), front ? 0 : wl - maxw});
+      ftasks.push_back(sk);
+    }
+  }
+  const int64_t nft = (int64_t)ftasks.size();
+  std::vector<HostTask> ht(nft);
+  for (int64_t t = 0; t < nft; t++) {
+    const ixg_task &k = ftasks[t];
     ht[t].base = k.strand == 0 ? (k.start - 1) : (L + 2) + (L - k.end);
     ht[t].L = k.end - k.start + 1;
-    A.nres += (uint64_t)(ht[t].L - k.ctx);  // STD pass: new residues (cm_search.rs:2306-2311)
+    if (t < nt) A.nres += (uint64_t)(ht[t].L - k.ctx);  // STD pass: new residues (cm_search.rs:2306-2311)
   }
-  // F1-F3b
-  std::vector<ixg_acct> fa(nt);
+  // F1-F3b over every chunk and truncated-pass subsequence in one call
+  std::vector<ixg_acct> fa(nft);
   int64_t nw = 0;
   std::vector<ixg_window> wins(4096);
-  int rc = ixg_f3_filter_tasks(c, p, tgt, tasks.data(), nt, &o->f3, wins.data(), (int64_t)wins.size(), &nw, fa.data());
+  int rc = ixg_f3_filter_tasks(c, p, tgt, ftasks.data(), nft, &o->f3, wins.data(), (int64_t)wins.size(), &nw,
+                               fa.data());
   if (rc == IXG_ERANGE) {
     wins.resize(nw);
-    rc = ixg_f3_filter_tasks(c, p, tgt, tasks.data(), nt, &o->f3, wins.data(), nw, &nw, fa.data());
+    rc = ixg_f3_filter_tasks(c, p, tgt, ftasks.data(), nft, &o->f3, wins.data(), nw, &nw, fa.data());
   }
   RCHK(rc);
   wins.resize(nw);
   stage(IXG_SS_F1_F3B);
-  for (const ixg_acct &a : fa) {
-    const uint64_t *s = (const uint64_t *)&a;
-    uint64_t *d = (uint64_t *)&A.f;
-    for (size_t q = 0; q < sizeof(ixg_acct) / 8; q++) d[q] += s[q];
+  auto add_acct = [](const void *src, void *dst, size_t bytes) {
+    const uint64_t *s = (const uint64_t *)src;
+    uint64_t *d = (uint64_t *)dst;
+    for (size_t q = 0; q < bytes / 8; q++) d[q] += s[q];
+  };
+  std::vector<int64_t> nwin_of(nft, 0);
+  for (const ixg_window &w : wins) nwin_of[w.task]++;
+  for (int64_t t = 0; t < nt; t++) add_acct(&fa[t], &A.f, sizeof(ixg_acct));
+  // envelope definition of the STD pass and of every truncated pass whose
+  // chunk had STD survivors (cm_search.rs:2333-2335) and its own
+  std::vector<EnvWin> ew;
+  std::vector<int32_t> ew_run;  // -1: STD (task = the chunk), else the index into truns
+  for (const ixg_window &w : wins)
+    if (w.task < nt) ew.push_back({w.task, w.start, w.end, IXG_PASS_STD_ANY}), ew_run.push_back(-1);
+  for (size_t r = 0; r < truns.size(); r++) {
+    const TrRun &R = truns[r];
+    if (nwin_of[R.task] == 0) continue;  // no STD survivors: the truncated passes are skipped
+    add_acct(&fa[R.sub], &A.tr_f[R.pass - IXG_PASS_5P_ONLY_FORCE], sizeof(ixg_acct));
+    for (const ixg_window &w : wins)
+      if (w.task == R.sub) ew.push_back({R.sub, w.start, w.end, R.pass}), ew_run.push_back((int32_t)r);
   }
-  // F4-F5
-  std::vector<WinRef> wl(nw);
-  for (int64_t w = 0; w < nw; w++) wl[w] = {wins[w].task, wins[w].start, wins[w].end};
   std::vector<std::vector<std::pair<int32_t, int32_t>>> envs;
   std::vector<ixg_envacct> ea;
-  RCHK(env_def_batch(c, p, dbase, ht, wl, o->env, envs, ea));
+  RCHK(env_def_passes(c, p, dbase, ht, ew, o->env, envs, ea));
   stage(IXG_SS_F4_F5);
-  for (const ixg_envacct &a : ea) {
-    const uint64_t *s = (const uint64_t *)&a;
-    uint64_t *d = (uint64_t *)&A.e;
-    for (size_t q = 0; q < sizeof(ixg_envacct) / 8; q++) d[q] += s[q];
-  }
-  // envelopes in the coordinates of their task's strand buffer
+  for (size_t w = 0; w < ew.size(); w++)
+    add_acct(&ea[w], ew_run[w] < 0 ? &A.e : &A.tr_e[truns[ew_run[w]].pass - IXG_PASS_5P_ONLY_FORCE],
+             sizeof(ixg_envacct));
+  // envelopes in the coordinates of their sequence view's strand buffer
+  auto view_off = [&](int32_t t) {
+    const ixg_task &k = ftasks[t];
+    return k.strand == 0 ? k.start - 1 : L - k.end;
+  };
   std::vector<ixg_envelope> env;
   std::vector<int32_t> etask;
-  for (int64_t w = 0; w < nw; w++) {
-    const ixg_task &k = tasks[wins[w].task];
-    const int64_t off = k.strand == 0 ? k.start - 1 : L - k.end;
+  std::vector<std::vector<ixg_envelope>> tenv(truns.size());
+  for (size_t w = 0; w < ew.size(); w++) {
+    const int64_t off = view_off(ew[w].task);
     for (auto &e : envs[w]) {
       ixg_envelope x{};
       x.i0 = (int32_t)(off + e.first);
       x.j0 = (int32_t)(off + e.second);
-      x.strand = k.strand;
-      env.push_back(x);
-      etask.push_back(wins[w].task);
+      x.strand = ftasks[ew[w].task].strand;
+      if (ew_run[w] < 0) {
+        env.push_back(x);
+        etask.push_back(ew[w].task);
+      } else {
+        tenv[ew_run[w]].push_back(x);
+      }
     }
   }
+  std::vector<Hit> hits;
   // F6: CYK filter over HMM bands at the filter tau
   std::vector<ixg_envelope> surv;
   std::vector<int32_t> stask;
@@ -251,7 +375,6 @@ extern "C" int ixg_search_std(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, con
     }
   }
   // F7: Inside (or CYK) at the final tau, hits above pli_t
-  std::vector<Hit> hits;
   if (!surv.empty()) {
     std::vector<EnvBands> eb;
     std::vector<char> skip;
@@ -306,21 +429,105 @@ extern "C" int ixg_search_std(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, con
           if (!gone[b] && !(v[b].j < v[a].i) && !(v[a].j < v[b].i)) gone[b] = 1;
       }
       const int32_t t = stask[ridx[r]];
-      const ixg_task &k = tasks[t];
-      const int64_t off = k.strand == 0 ? k.start - 1 : L - k.end;
+      const int64_t off = view_off(t);
       for (size_t a = 0; a < v.size(); a++) {
         if (gone[a]) continue;
-        const int64_t ws = v[a].i - off, we = v[a].j - off;  // chunk-local
         Hit h;
         h.task = t;
-        h.strand = k.strand;
-        if (k.strand) h.start = k.end - ws + 1, h.stop = k.end - we + 1;
-        else h.start = k.start + ws - 1, h.stop = k.start + we - 1;
+        h.pass = IXG_PASS_STD_ANY;
+        genome_coords(tasks[t], v[a].i - off, v[a].j - off, h);
         h.score = v[a].score;
         h.bias = v[a].bias;
         h.evalue = exp_surv((double)h.score, o->fin_mu, o->fin_lambda) * o->ez;
         hits.push_back(h);
       }
+    }
+  }
+  // the truncated passes' CM stages (cm_search.rs:2863 cm_trunc_stage), one
+  // pass at a time (the pass selects the CP9 and the scanners' planes):
+  // TrCYK over bands at the filter tau with its gate and envelope
+  // redefinition (not at an enforced end), then FTrInside (or TrCYK) at the
+  // final tau; hits in the chunk's coordinates shifted by the pass's offset
+  for (int q = 0; q < 3 && do_trunc; q++) {
+    const int pass = IXG_PASS_5P_ONLY_FORCE + q;
+    const bool ef = pass != IXG_PASS_3P_ONLY_FORCE, el = pass != IXG_PASS_5P_ONLY_FORCE;  // enforced first / final
+    std::vector<ixg_envelope> pe;
+    std::vector<int32_t> prun;
+    for (size_t r = 0; r < truns.size(); r++)
+      if (truns[r].pass == pass)
+        for (const ixg_envelope &x : tenv[r]) pe.push_back(x), prun.push_back((int32_t)r);
+    if (pe.empty()) continue;
+    std::vector<ixg_envelope> ps;
+    std::vector<int32_t> psrun;
+    if (!o->do_fcyk) {
+      ps = pe, psrun = prun;
+    } else {
+      std::vector<TrEnvBands> tb;
+      RCHK(trunc_bands(c, trcp9[q], trmap[q], dbase, L, pe, pass, o->fcyk_tau, tb));
+      std::vector<ixg_trenvelope> te(pe.size());
+      for (size_t k = 0; k < pe.size(); k++) {
+        te[k].i0 = pe[k].i0, te[k].j0 = pe[k].j0, te[k].strand = pe[k].strand, te[k].pad = 0;
+        te[k].bands = tb[k].b.view();
+        te[k].valid = tb[k].valid();
+      }
+      std::vector<float> vsc(te.size());
+      std::vector<int32_t> vmode(te.size());
+      std::vector<int64_t> ei(te.size()), ej(te.size());
+      int64_t nh = 0;
+      // the filter's hits are not used: no room for them (IXG_ERANGE)
+      rc = ixg_trhb_scan_batch(c, cm, tgt, te.data(), (int64_t)te.size(), 0, pass, 1, o->cyk_env_cutoff, 0.0f,
+                               o->do_null3, vsc.data(), vmode.data(), ei.data(), ej.data(), nullptr, 0, &nh, nullptr);
+      if (rc != IXG_ERANGE) RCHK(rc);
+      for (size_t k = 0; k < te.size(); k++) {
+        if (exp_surv((double)vsc[k], o->cyk_mu, o->cyk_lambda) > o->f6) continue;
+        ixg_envelope x = pe[k];
+        if (o->do_fcykenv && ei[k] != -1 && ej[k] != -1) {
+          if (!ef) x.i0 = (int32_t)ei[k];
+          if (!el) x.j0 = (int32_t)ej[k];
+        }
+        A.tr_n_past_cyk[q]++;
+        A.tr_pos_past_cyk[q] += (uint64_t)(x.j0 - x.i0 + 1);
+        ps.push_back(x);
+        psrun.push_back(prun[k]);
+      }
+    }
+    if (ps.empty()) continue;
+    std::vector<TrEnvBands> tb;
+    RCHK(trunc_bands(c, trcp9[q], trmap[q], dbase, L, ps, pass, o->final_tau, tb));
+    std::vector<ixg_trenvelope> te(ps.size());
+    for (size_t k = 0; k < ps.size(); k++) {
+      te[k].i0 = ps[k].i0, te[k].j0 = ps[k].j0, te[k].strand = ps[k].strand, te[k].pad = 0;
+      te[k].bands = tb[k].b.view();
+      te[k].valid = tb[k].valid();
+    }
+    std::vector<float> vsc(te.size());
+    std::vector<int32_t> vmode(te.size());
+    std::vector<int64_t> ei(te.size()), ej(te.size());
+    int64_t nh = 0;
+    std::vector<ixg_trhit> th(1024);
+    rc = ixg_trhb_scan_batch(c, cm, tgt, te.data(), (int64_t)te.size(), o->do_cyk ? 0 : 1, pass, 1, 0.0f, o->pli_t,
+                             o->do_null3, vsc.data(), vmode.data(), ei.data(), ej.data(), th.data(),
+                             (int64_t)th.size(), &nh, nullptr);
+    if (rc == IXG_ERANGE) {
+      th.resize(nh);
+      rc = ixg_trhb_scan_batch(c, cm, tgt, te.data(), (int64_t)te.size(), o->do_cyk ? 0 : 1, pass, 1, 0.0f,
+                               o->pli_t, o->do_null3, vsc.data(), vmode.data(), ei.data(), ej.data(), th.data(), nh,
+                               &nh, nullptr);
+    }
+    RCHK(rc);
+    th.resize(nh);
+    for (const ixg_trhit &x : th) {
+      const TrRun &R = truns[psrun[x.env]];
+      const int64_t off = view_off(R.task);  // the chunk's offset in its strand buffer
+      Hit h;
+      h.task = R.task;
+      h.pass = pass;
+      genome_coords(tasks[R.task], x.i - off, x.j - off, h);
+      h.score = x.score;
+      h.bias = x.bias;
+      h.evalue = exp_surv((double)h.score, o->fin_mu, o->fin_lambda) * o->ez;
+      hits.push_back(h);
+      A.tr_hits[q]++;
     }
   }
   // finalize (cm_search.rs:1915): global overlap removal per strand, then the
@@ -355,6 +562,7 @@ extern "C" int ixg_search_std(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, con
     if (a.evalue != b.evalue) return a.evalue < b.evalue;
     if (a.score != b.score) return a.score > b.score;
     if (a.start != b.start) return a.start < b.start;
+    if (a.pass != b.pass) return a.pass > b.pass;
     return a.stop < b.stop;
   });
   int64_t n = 0;
@@ -368,7 +576,7 @@ extern "C" int ixg_search_std(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, con
       x.bias = h.bias;
       x.evalue = h.evalue;
       x.strand = h.strand;
-      x.pad = 0;
+      x.pass = h.pass;
     }
     n++;
   }
@@ -378,6 +586,18 @@ extern "C" int ixg_search_std(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, con
   stage(IXG_SS_FINALIZE);
   c->search_ms[IXG_SS_TOTAL] = std::chrono::duration<double, std::milli>(clk::now() - t_begin).count();
   return n > cap ? IXG_ERANGE : IXG_OK;
+}
+
+extern "C" int ixg_search_std(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const ixg_cp9 *cp9,
+                              const ixg_hbmap *map, const ixg_target *tgt, const ixg_searchopts *o,
+                              ixg_searchhit *out, int64_t cap, int64_t *nout, ixg_searchacct *acct) {
+  if (!o) {
+    g_err = "ixg_search_std: bad arguments";
+    return IXG_EINVAL;
+  }
+  ixg_searchopts so = *o;
+  so.trunc_mode = 0;
+  return ixg_search(c, p, cm, cp9, map, nullptr, nullptr, tgt, &so, out, cap, nout, acct);
 }
 
 extern "C" int ixg_search_stats(ixg_ctx *c, double *ms, int64_t *cells) {

@@ -922,7 +922,8 @@ IXG_GFB_BACKWARD, IXG_GFB_UNIHIT, IXG_GFB_MATRIX = 1, 2, 4
 class EnvOpts(C.Structure):
     _fields_ = [("f4", C.c_double), ("f4b", C.c_double), ("f5", C.c_double), ("f5b", C.c_double),
                 ("gfmu", C.c_double), ("gflambda", C.c_double), ("rt1", C.c_float), ("rt2", C.c_float),
-                ("rt3", C.c_float), ("ns", C.c_int32), ("do_gfwdbias", C.c_int32), ("do_edefbias", C.c_int32)]
+                ("rt3", C.c_float), ("ns", C.c_int32), ("do_gfwdbias", C.c_int32), ("do_edefbias", C.c_int32),
+                ("lftau", C.c_double), ("lflambda", C.c_double), ("clen", C.c_int32), ("pad", C.c_int32)]
 
 
 class EnvAcct(C.Structure):
@@ -932,11 +933,12 @@ class EnvAcct(C.Structure):
 
 
 def env_opts(f4=0.005, f4b=0.005, f5=0.005, f5b=0.005, gfmu=-5.0, gflambda=0.69, rt1=0.25, rt2=0.10, rt3=0.20,
-             ns=200, do_gfwdbias=True, do_edefbias=False):
+             ns=200, do_gfwdbias=True, do_edefbias=False, lftau=-4.0, lflambda=0.69, clen=1):
     o = EnvOpts()
     o.f4, o.f4b, o.f5, o.f5b, o.gfmu, o.gflambda = f4, f4b, f5, f5b, gfmu, gflambda
     o.rt1, o.rt2, o.rt3, o.ns = rt1, rt2, rt3, ns
     o.do_gfwdbias, o.do_edefbias = int(bool(do_gfwdbias)), int(bool(do_edefbias))
+    o.lftau, o.lflambda, o.clen = lftau, lflambda, int(clen)
     return o
 
 
@@ -952,6 +954,8 @@ def _env_lib():
         L.ixg_gfb_scores.argtypes = [vp, vp, u8p, i64, i64p, i64p, i32p, i64, C.c_int, f32p, f32p, f32p]
         L.ixg_env_def.argtypes = [vp, vp, u8p, i64, C.POINTER(Window), i64, C.POINTER(EnvOpts), i32p, i32p, i64, i64p,
                                   C.POINTER(EnvAcct)]
+        L.ixg_env_def_pass.argtypes = [vp, vp, u8p, i64, C.POINTER(Window), i64, C.c_int, C.POINTER(EnvOpts), i32p,
+                                       i32p, i64, i64p, C.POINTER(EnvAcct)]
         _env_ready = True
     return L
 
@@ -981,9 +985,9 @@ def gfb_scores(p7, seq, offs, lens, cfg_len=None, backward=False, unihit=False, 
     return sc[:n], xs, (ms if matrix else None)
 
 
-def env_def(p7, dsq, L, windows, opts):
-    """STD-pass envelope definition of one chunk (ixg_env_def): windows
-    [(start, end)] -> ([(es, ee)], EnvAcct)."""
+def env_def(p7, dsq, L, windows, opts, pass_idx=1):
+    """Envelope definition of one chunk (ixg_env_def_pass; pass 1 STD, 2 5',
+    3 3', 4 5'&3'): windows [(start, end)] -> ([(es, ee)], EnvAcct)."""
     d = np.ascontiguousarray(dsq, np.uint8)
     n = len(windows)
     arr = (Window * max(n, 1))()
@@ -994,9 +998,9 @@ def env_def(p7, dsq, L, windows, opts):
         es, ee = np.zeros(cap, np.int32), np.zeros(cap, np.int32)
         nenv = C.c_int64()
         acct = EnvAcct()
-        rc = _env_lib().ixg_env_def(p7.ctx.h, p7.h, u8ptr(d), int(L), arr, n, C.byref(opts),
-                                    es.ctypes.data_as(C.POINTER(C.c_int32)), ee.ctypes.data_as(C.POINTER(C.c_int32)),
-                                    cap, C.byref(nenv), C.byref(acct))
+        rc = _env_lib().ixg_env_def_pass(p7.ctx.h, p7.h, u8ptr(d), int(L), arr, n, int(pass_idx), C.byref(opts),
+                                         es.ctypes.data_as(C.POINTER(C.c_int32)),
+                                         ee.ctypes.data_as(C.POINTER(C.c_int32)), cap, C.byref(nenv), C.byref(acct))
         if rc == IXG_ERANGE and nenv.value > cap:
             cap = nenv.value
             continue
@@ -1030,6 +1034,9 @@ def _hb_lib():
         L.ixg_hbmap_free.argtypes = [vp]
         L.ixg_hmm2ij_bands.argtypes = [vp, i32p, C.c_int32, C.c_int32, C.c_int, i32p, i32p, i32p, i32p]
         L.ixg_ij2d_bands.argtypes = [vp, i32p, i32p, i32p, i32p, C.c_int, i32p, i32p, i32p, C.c_int64, i64p]
+        u8p = C.POINTER(C.c_uint8)
+        L.ixg_hmm2ij_bands_trunc.argtypes = [vp, i32p, i32p, C.c_int32, C.c_int32, C.c_int, i32p, i32p, i32p, i32p,
+                                             u8p, u8p, u8p, u8p, i32p]
         _hb_ready = True
     return L
 
@@ -1061,6 +1068,21 @@ class HbMap:
               "ixg_hmm2ij_bands")
         return (*out, pn)
 
+    def hmm2ij_trunc(self, pn, occ, i0, j0, pass_idx):
+        """ixg_hmm2ij_bands_trunc: (imin, imax, jmin, jmax, jvalid, lvalid,
+        rvalid, tvalid, [sp1 sp2 ep1 ep2 rmarg_imin rmarg_imax lmarg_jmin
+        lmarg_jmax])."""
+        L = _hb_lib()
+        pn = np.ascontiguousarray(pn, np.int32).copy()
+        occ = np.ascontiguousarray(occ, np.int32)
+        out = [np.zeros(self.M, np.int32) for _ in range(4)]
+        val = [np.zeros(self.M, np.uint8) for _ in range(4)]
+        spep = np.zeros(8, np.int32)
+        check(L.ixg_hmm2ij_bands_trunc(self.h, _i32(pn), _i32(occ), int(i0), int(j0), int(pass_idx),
+                                       *[_i32(a) for a in out], *[u8ptr(v) for v in val], _i32(spep)),
+              "ixg_hmm2ij_bands_trunc")
+        return (*out, *val, spep.tolist())
+
     def ij2d(self, imin, imax, jmin, jmax, trunc=False):
         """Bands dict in the scanners' layout (jmin, jmax, hd_off, hdmin, hdmax)."""
         a = [np.ascontiguousarray(x, np.int32) for x in (imin, imax, jmin, jmax)]
@@ -1081,21 +1103,25 @@ class SearchOpts(C.Structure):
                 ("fin_mu", C.c_double), ("fin_lambda", C.c_double), ("ez", C.c_double), ("e_report", C.c_double),
                 ("fcyk_tau", C.c_double), ("final_tau", C.c_double), ("maxtau", C.c_double),
                 ("size_limit", C.c_float), ("do_fcyk", C.c_int32), ("do_fcykenv", C.c_int32),
-                ("do_null3", C.c_int32), ("do_cyk", C.c_int32), ("do_top", C.c_int32), ("do_bot", C.c_int32)]
+                ("do_null3", C.c_int32), ("do_cyk", C.c_int32), ("do_top", C.c_int32), ("do_bot", C.c_int32),
+                ("trunc_mode", C.c_int32), ("pad", C.c_int32)]
 
 
 class SearchHit(C.Structure):
     _fields_ = [("start", C.c_int64), ("stop", C.c_int64), ("score", C.c_float), ("bias", C.c_float),
-                ("evalue", C.c_double), ("strand", C.c_int32), ("pad", C.c_int32)]
+                ("evalue", C.c_double), ("strand", C.c_int32), ("pass_idx", C.c_int32)]
 
 
 class SearchAcct(C.Structure):
     _fields_ = [("nres", C.c_uint64), ("f", Acct), ("e", EnvAcct), ("n_past_cyk", C.c_uint64),
-                ("pos_past_cyk", C.c_uint64), ("n_hits", C.c_uint64)]
+                ("pos_past_cyk", C.c_uint64), ("n_hits", C.c_uint64), ("tr_nres", C.c_uint64 * 3),
+                ("tr_f", Acct * 3), ("tr_e", EnvAcct * 3), ("tr_n_past_cyk", C.c_uint64 * 3),
+                ("tr_pos_past_cyk", C.c_uint64 * 3), ("tr_hits", C.c_uint64 * 3)]
 
 
 def search_opts(f3, env, f6, cyk, fin, ez, e_report=10.0, pli_t=None, cykenvx=10, fcyk_tau=1e-4, final_tau=5e-6,
-                maxtau=0.05, W=None, do_fcyk=True, do_fcykenv=True, do_null3=True, do_cyk=False, top=True, bot=True):
+                maxtau=0.05, W=None, do_fcyk=True, do_fcykenv=True, do_null3=True, do_cyk=False, top=True, bot=True,
+                trunc=False):
     """Options of ixg_search_std. cyk / fin: (mu, lambda) exponential tails of
     the local CYK filter and the final round; the CYK envelope cutoff and the
     reporting threshold follow them as cm_search.rs:1656-1721 derives them
@@ -1113,34 +1139,59 @@ def search_opts(f3, env, f6, cyk, fin, ez, e_report=10.0, pli_t=None, cykenvx=10
     o.size_limit = float(min(1024.0, max(256.0, (w - 1000.0) / (3000.0 - 1000.0) * (1024.0 - 256.0) + 256.0)))
     o.do_fcyk, o.do_fcykenv, o.do_null3, o.do_cyk = int(do_fcyk), int(do_fcykenv), int(do_null3), int(do_cyk)
     o.do_top, o.do_bot = int(top), int(bot)
+    o.trunc_mode = int(bool(trunc))
     return o
 
 
 _search_ready = False
 
 
-def search_std(ctx, p7, cm, cp9, hbmap, tgt, opts):
-    """ixg_search_std: ([(start, stop, strand, score, bias, evalue)], SearchAcct)."""
+def _search_lib():
     global _search_ready
     L = lib()
     if not _search_ready:
         vp = C.c_void_p
         L.ixg_search_std.argtypes = [vp, vp, vp, vp, vp, vp, C.POINTER(SearchOpts), C.POINTER(SearchHit), C.c_int64,
                                      C.POINTER(C.c_int64), C.POINTER(SearchAcct)]
+        L.ixg_search.argtypes = [vp, vp, vp, vp, vp, C.POINTER(vp), C.POINTER(vp), vp, C.POINTER(SearchOpts),
+                                 C.POINTER(SearchHit), C.c_int64, C.POINTER(C.c_int64), C.POINTER(SearchAcct)]
         _search_ready = True
+    return L
+
+
+def _search_call(call, name, with_pass=False):
     cap = 256
     while True:
         out = (SearchHit * cap)()
         n = C.c_int64()
         acct = SearchAcct()
-        rc = L.ixg_search_std(ctx.h, p7.h, cm.h, cp9.h, hbmap.h, tgt.h, C.byref(opts), out, cap, C.byref(n),
-                              C.byref(acct))
+        rc = call(out, cap, n, acct)
         if rc == IXG_ERANGE and n.value > cap:
             cap = n.value
             continue
-        check(rc, "ixg_search_std")
-        return [(out[i].start, out[i].stop, out[i].strand, out[i].score, out[i].bias, out[i].evalue)
-                for i in range(n.value)], acct
+        check(rc, name)
+        return [(out[i].start, out[i].stop, out[i].strand, out[i].score, out[i].bias, out[i].evalue) +
+                ((out[i].pass_idx,) if with_pass else ()) for i in range(n.value)], acct
+
+
+def search_std(ctx, p7, cm, cp9, hbmap, tgt, opts):
+    """ixg_search_std: ([(start, stop, strand, score, bias, evalue)], SearchAcct)."""
+    L = _search_lib()
+    return _search_call(lambda out, cap, n, acct: L.ixg_search_std(ctx.h, p7.h, cm.h, cp9.h, hbmap.h, tgt.h,
+                                                                   C.byref(opts), out, cap, C.byref(n), C.byref(acct)),
+                        "ixg_search_std")
+
+
+def search(ctx, p7, cm, cp9, hbmap, trcp9, trmap, tgt, opts):
+    """ixg_search with the truncated passes' CP9s and band maps (5', 3',
+    5'&3'): ([(start, stop, strand, score, bias, evalue, pass)], SearchAcct)."""
+    L = _search_lib()
+    vp = C.c_void_p
+    tc = (vp * 3)(*[x.h.value if hasattr(x.h, 'value') else x.h for x in trcp9])
+    tm = (vp * 3)(*[x.h.value if hasattr(x.h, 'value') else x.h for x in trmap])
+    return _search_call(lambda out, cap, n, acct: L.ixg_search(ctx.h, p7.h, cm.h, cp9.h, hbmap.h, tc, tm, tgt.h,
+                                                               C.byref(opts), out, cap, C.byref(n), C.byref(acct)),
+                        "ixg_search", with_pass=True)
 
 
 SEARCH_STAGES = ("f1_f3b", "f4_f5", "f6_bands", "f6_cyk", "f7_bands", "f7_inside", "finalize", "total")

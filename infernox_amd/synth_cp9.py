@@ -51,10 +51,24 @@ def prob2score(p, null=1.0):
     return np.where(p == 0, -INFTY, np.maximum(v, -INFTY)).astype(np.int32)
 
 
-def build_cp9(cm, seed=0, el=True, pbegin=0.05, pend=0.05, gap=0.05):
+def _nexits(cm):
+    """Nodes with a local end (cp9.rs:1121-1130)."""
+    nt = cm["ndtype"]
+    return sum(1 for nd in range(1, cm["nodes"])
+               if nt[nd] in (S.MATP_ND, S.MATL_ND, S.MATR_ND, S.BEGL_ND, S.BEGR_ND) and nt[nd + 1] != S.END_ND)
+
+
+def build_cp9(cm, seed=0, el=True, pbegin=0.05, pend=0.05, gap=0.05, trunc=None):
     """A configured CP9 for the CM's consensus (M = clen): dict with M, el,
     el_selfsc, otsc [(M+1), 12], msc / isc [(M+1), 18], has_el [M+1],
-    el_from_ct [M+2], el_from_off [M+3], el_from_idx."""
+    el_from_ct [M+2], el_from_off [M+3], el_from_idx.
+
+    trunc = (pentry, pexit) configures the begins and ends of the same draws
+    as cp9_sw_config does for the truncated passes' CP9s (cp9.rs:1097-1119:
+    begin[1] = (1 - pentry)(1 - s), begin[k >= 2] = pentry (1 - s) / (M - 1),
+    end[k < M] = b / (1 - b (k - 1)) with b = pexit / (M - 1), the match
+    transitions rescaled to the rest; cp9.rs:2235-2330), then local ends
+    (cp9.rs:1121-1152, pend over the exit nodes) when el is set."""
     rng = np.random.default_rng(seed)
     lpos, rpos = emit_map(cm)
     M = int(rpos[0] - 1)
@@ -64,6 +78,8 @@ def build_cp9(cm, seed=0, el=True, pbegin=0.05, pend=0.05, gap=0.05):
         m = rng.dirichlet([40.0, 1.0, 1.0])
         end = pend if k >= 1 else 0.0
         elp = 0.02 if (el and k >= 1) else 0.0
+        if trunc is not None:
+            end, elp = 0.0, 0.0
         t[k, MM], t[k, MI], t[k, MD] = m * (1 - end - elp)
         t[k, ME], t[k, MEL] = end, elp
         t[k, IM], t[k, II], t[k, ID] = rng.dirichlet([6.0, 3.0, 0.5])
@@ -77,6 +93,28 @@ def build_cp9(cm, seed=0, el=True, pbegin=0.05, pend=0.05, gap=0.05):
     if M >= 1:
         t[M, MM] = t[M, MD] = t[M, ID] = t[M, DD] = 0.0
         t[M, ME] = 1.0 - t[M, MI] - t[M, MEL]
+    if trunc is not None and M >= 2:
+        pentry, pexit = trunc
+        s0 = t[0, MI] + t[0, MD]
+        t[1:, BM] = pentry * (1.0 - s0) / (M - 1)
+        t[1, BM] = (1.0 - pentry) * (1.0 - s0)
+        b = pexit / (M - 1)
+        for k in range(1, M):
+            t[k, ME] = b / (1.0 - b * (k - 1))
+            tot = t[k, MM] + t[k, MI] + t[k, MD] + t[k, MEL]
+            for x in (MM, MI, MD, MEL):
+                t[k, x] *= (1.0 - t[k, ME]) / tot
+        if el:
+            to_el = float(cm["pend"]) / max(1, _nexits(cm))
+            for nd in range(cm["nodes"]):
+                if cm["ndtype"][nd] in (S.MATP_ND, S.MATL_ND, S.MATR_ND, S.BEGL_ND, S.BEGR_ND) and \
+                        cm["ndtype"][nd + 1] != S.END_ND:
+                    k = int(lpos[nd])
+                    if 1 <= k <= M and t[k, MEL] == 0.0:
+                        t[k, MEL] = to_el
+                        f = 1.0 - to_el / (1.0 - t[k, ME])
+                        for x in (MM, MI, MD):
+                            t[k, x] *= f
     otsc = prob2score(t.astype(np.float32))
     msc = np.zeros((M + 1, 18), np.int32)
     isc = np.zeros((M + 1, 18), np.int32)
@@ -105,7 +143,7 @@ def build_cp9(cm, seed=0, el=True, pbegin=0.05, pend=0.05, gap=0.05):
                 has_el=has_el, el_from_ct=ct, el_from_off=off, el_from_idx=idx)
 
 
-def build_cp9_like(cm, seed=0, el=True, pbegin=0.05, pend=0.05):
+def build_cp9_like(cm, seed=0, el=True, pbegin=0.05, pend=0.05, trunc=None):
     """A configured CP9 whose match emissions follow the CM's consensus
     columns (the marginal of the CM state that emits each column: MATL ML,
     MATR MR, or the left / right marginal of MATP MP) and whose transitions
@@ -113,7 +151,7 @@ def build_cp9_like(cm, seed=0, el=True, pbegin=0.05, pend=0.05):
     homologs. A stand-in for the reference's CM -> CP9 construction
     (cp9.rs:2212), which is not reproduced; the local-end lists are the
     guide tree's, as in build_cp9."""
-    h = build_cp9(cm, seed=seed, el=el, pbegin=pbegin, pend=pend)
+    h = build_cp9(cm, seed=seed, el=el, pbegin=pbegin, pend=pend, trunc=trunc)
     lpos, rpos = emit_map(cm)
     M = h["M"]
     mat = np.full((M + 1, 4), 0.25)
@@ -136,3 +174,14 @@ def build_cp9_like(cm, seed=0, el=True, pbegin=0.05, pend=0.05):
         msc[k, 5:16] = int(np.mean(sc))
     h["msc"] = msc
     return h
+
+
+def build_trunc_cp9s(cm, seed=0, el=True):
+    """The truncated passes' CP9s (Rcp9, Lcp9, Tcp9 of cp9.rs:2235-2330) for
+    the CM: the same draws as build_cp9_like configured with (pentry, pexit)
+    = ((M-1)/M, 0) for the 5' pass, (0, (M-1)/M) for the 3' pass, both for
+    5'&3'; local ends when el is set (the local search's configuration)."""
+    lpos, rpos = emit_map(cm)
+    M = int(rpos[0] - 1)
+    full = float(np.float32(M - 1) / np.float32(M))
+    return [build_cp9_like(cm, seed, el=el, trunc=cfg) for cfg in ((full, 0.0), (0.0, full), (full, full))]

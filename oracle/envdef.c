@@ -83,6 +83,24 @@ void ixo_gm_reconfig_multihit(ixo_gm *gm, int L) {
   ixo_gm_reconfig_length(gm, L);
 }
 
+/* p7_generic.rs:350-376 the 5' and 3' truncated length models */
+void ixo_gm_reconfig_length_5p(ixo_gm *gm, int L) {
+  const float pmove = (1.0f + gm->nj) / ((float)L + 1.0f + gm->nj);
+  const float ploop = 1.0f - pmove;
+  const float ll = (float)log((double)ploop), lm = (float)log((double)pmove);
+  gm->xsc[X_C][LOOP] = gm->xsc[X_J][LOOP] = ll;
+  gm->xsc[X_C][MOVE] = gm->xsc[X_J][MOVE] = lm;
+  gm->L = L;
+}
+void ixo_gm_reconfig_length_3p(ixo_gm *gm, int L) {
+  const float pmove = (1.0f + gm->nj) / ((float)L + 1.0f + gm->nj);
+  const float ploop = 1.0f - pmove;
+  const float ll = (float)log((double)ploop), lm = (float)log((double)pmove);
+  gm->xsc[X_N][LOOP] = gm->xsc[X_J][LOOP] = ll;
+  gm->xsc[X_N][MOVE] = gm->xsc[X_J][MOVE] = lm;
+  gm->L = L;
+}
+
 /* p7_generic.rs:156-257 build_glocal_profile (multihit glocal, bg 0.25) */
 int ixo_gm_build(const ixo_p7 *p7, int L, ixo_gm *gm) {
   const int M = p7->M;
@@ -142,6 +160,48 @@ int ixo_gm_build(const ixo_p7 *p7, int L, ixo_gm *gm) {
   gm->nj = 1.0f;
   gm->local_end = 0;
   ixo_gm_reconfig_length(gm, L);
+  return 0;
+}
+
+/* The truncated passes' profiles as cm_search.rs:1011-1016 builds them at
+ * L = 100: which 0 = Rgm (p7_generic.rs:289, 5'), 1 = Lgm (:311, 3'),
+ * 2 = Tgm (build_local_profile :259 + :336, 5'&3'). */
+int ixo_gm_build_trunc(const ixo_p7 *p7, int which, ixo_gm *gm) {
+  if (ixo_gm_build(p7, 100, gm)) return -1;
+  const int M = gm->M;
+  if (which == 0) {
+    const float inv = (float)log(1.0 / (double)M);
+    for (int k = 1; k <= M; k++) TSC(gm, P_BM, k - 1) = inv;
+    gm->local_end = 0;
+    gm->xsc[X_N][MOVE] = 0.0f, gm->xsc[X_E][MOVE] = 0.0f;
+    gm->xsc[X_N][LOOP] = NEGINF, gm->xsc[X_E][LOOP] = NEGINF;
+    gm->L = 0;
+    ixo_gm_reconfig_length_5p(gm, 100);
+  } else if (which == 1) {
+    /* the glocal entry, recomputed as the reference does: identical */
+    gm->local_end = 1;
+    gm->xsc[X_C][MOVE] = 0.0f, gm->xsc[X_E][MOVE] = 0.0f;
+    gm->xsc[X_C][LOOP] = NEGINF, gm->xsc[X_E][LOOP] = NEGINF;
+    gm->L = 0;
+    ixo_gm_reconfig_length_3p(gm, 100);
+  } else {
+    const float *t = p7->trans;
+    float *mocc = (float *)calloc((size_t)M + 1, sizeof(float));
+    mocc[1] = t[H_MI] + t[H_MM];
+    for (int k = 2; k <= M; k++) {
+      const float *tk = t + (size_t)(k - 1) * 7;
+      mocc[k] = mocc[k - 1] * (tk[H_MM] + tk[H_MI]) + (1.0f - mocc[k - 1]) * tk[H_DM];
+    }
+    float z = 0.0f;
+    for (int k = 1; k <= M; k++) z += mocc[k] * (float)(M - k + 1);
+    for (int k = 1; k <= M; k++) TSC(gm, P_BM, k - 1) = (float)log((double)(mocc[k] / z));
+    free(mocc);
+    gm->local_end = 1;
+    gm->xsc[X_N][MOVE] = gm->xsc[X_C][MOVE] = gm->xsc[X_E][MOVE] = 0.0f;
+    gm->xsc[X_N][LOOP] = gm->xsc[X_C][LOOP] = gm->xsc[X_E][LOOP] = NEGINF;
+    gm->nj = 0.0f;
+    gm->L = 100;
+  }
   return 0;
 }
 
@@ -711,35 +771,47 @@ int ixo_domaindef_glocal(ixo_gm *gm, const uint8_t *dsq, int L, const float *fx,
   return nd;
 }
 
-/* cm_search.rs:2449-2600 trunc_env_def, STD pass (gm, glocal statistics
- * gfmu/gflambda, no safe correction, neither residue enforced). windows are
- * (start, end) in dsq coordinates; returns the envelope count. */
-int64_t ixo_env_def_std(const ixo_gm *gm_proto, const ixo_bias *bf, const uint8_t *dsq, int64_t L,
-                        const ixo_window *wins, int64_t nwin, const ixo_envopts *o, int32_t *es, int32_t *ee,
-                        int64_t cap, ixo_envacct *acct) {
+/* cm_search.rs:2449-2600 trunc_env_def for pass `pass` (1 STD with gm, 2
+ * 5' with Rgm, 3 3' with Lgm, 4 5'&3' with Tgm: gm_proto is that profile):
+ * glocal statistics gfmu/gflambda in the STD pass, the local tail
+ * lftau/lflambda otherwise; the score correction 0 (5') or ln(1/clen) (3');
+ * the first / final residue of dsq[1..L] enforced by the 5' / 3' passes.
+ * The profile is one copy carried through the windows in order, as there.
+ * windows are (start, end) in dsq coordinates; returns the envelope count. */
+int64_t ixo_env_def_pass(const ixo_gm *gm_proto, int pass, const ixo_bias *bf, const uint8_t *dsq, int64_t L,
+                         const ixo_window *wins, int64_t nwin, const ixo_envopts *o, int32_t *es, int32_t *ee,
+                         int64_t cap, ixo_envacct *acct) {
   const double ln2 = 0.69314718055994530942;
   memset(acct, 0, sizeof(*acct));
   ixo_gm gm;
   ixo_gm_copy(gm_proto, &gm);
   int64_t n = 0;
-  (void)L;
+  const int is_std = pass == 1, safe = pass == 2 || pass == 3;
+  const float corr = pass == 3 ? logf(1.0f / (float)o->clen) : 0.0f;
+  const double mu = is_std ? o->gfmu : o->lftau, lam = is_std ? o->gflambda : o->lflambda;
+  const int enf_first = pass == 2 || pass == 4, enf_final = pass == 3 || pass == 4;
   for (int64_t w = 0; w < nwin; w++) {
     const int ws = (int)wins[w].start, we = (int)wins[w].end;
+    if (enf_first && ws != 1) continue;
+    if (enf_final && we != L) continue;
     const int wlen = we - ws + 1;
     uint8_t *sub = subseq(dsq, ws, we);
     const double nullsc = ixo_bg_null_one(wlen);
-    ixo_gm_reconfig_length(&gm, wlen);
+    if (pass == 2) ixo_gm_reconfig_length_5p(&gm, wlen);
+    else if (pass == 3) ixo_gm_reconfig_length_3p(&gm, wlen);
+    else if (pass == 1) ixo_gm_reconfig_length(&gm, wlen);
     float *fdp = (float *)malloc(sizeof(float) * (size_t)(wlen + 1) * (gm.M + 1) * 3);
     float *fxm = (float *)malloc(sizeof(float) * (size_t)(wlen + 1) * G_NX);
     const float fwdsc = ixo_gforward(sub, wlen, &gm, fdp, fxm);
-    const double sc_f4 = ((double)fwdsc - nullsc) / ln2;
-    if (ixo_exp_surv(sc_f4, o->gfmu, o->gflambda) > o->f4) goto next;
+    const float safe_lfwdsc = fwdsc + corr;
+    const double sc_f4 = safe ? ((double)safe_lfwdsc - nullsc) / ln2 : ((double)fwdsc - nullsc) / ln2;
+    if (ixo_exp_surv(sc_f4, mu, lam) > o->f4) goto next;
     acct->n_past_gfwd++;
     acct->pos_past_gfwd += (uint64_t)wlen;
     if (o->do_gfwdbias) {
       const float filtersc = ixo_bias_score(bf, sub, wlen);
-      const double sc_f4b = ((double)fwdsc - (double)filtersc) / ln2;
-      if (ixo_exp_surv(sc_f4b, o->gfmu, o->gflambda) > o->f4b) goto next;
+      const double sc_f4b = safe ? ((double)safe_lfwdsc - nullsc) / ln2 : ((double)fwdsc - (double)filtersc) / ln2;
+      if (ixo_exp_surv(sc_f4b, mu, lam) > o->f4b) goto next;
       acct->n_past_gfwdbias++;
       acct->pos_past_gfwdbias += (uint64_t)wlen;
     }
@@ -748,21 +820,23 @@ int64_t ixo_env_def_std(const ixo_gm *gm_proto, const ixo_bias *bf, const uint8_
       float *bxm = (float *)malloc(sizeof(float) * (size_t)(wlen + 1) * G_NX);
       ixo_gbackward(sub, wlen, &gm, bdp, bxm);
       ixo_domain doms[256];
-      int nd = ixo_domaindef_glocal(&gm, sub, wlen, fxm, bxm, o->rt1, o->rt2, o->rt3, o->ns, 0, doms, 256);
+      int nd = ixo_domaindef_glocal(&gm, sub, wlen, fxm, bxm, o->rt1, o->rt2, o->rt3, o->ns, !is_std, doms, 256);
       if (nd > 256) nd = 256;
       for (int d = 0; d < nd; d++) {
         const int env_len = doms[d].jenv - doms[d].ienv + 1;
         const float env_sc =
             doms[d].envsc + ((float)wlen - (float)env_len) * logf((float)wlen / ((float)wlen + 3.0f));
-        const double pv = ((double)env_sc - (nullsc + 0.0)) / ln2;
-        if (ixo_exp_surv(pv, o->gfmu, o->gflambda) <= o->f5) {
+        double pv = ((double)env_sc - (nullsc + 0.0)) / ln2;
+        if (safe) pv += (double)corr / ln2;
+        if (ixo_exp_surv(pv, mu, lam) <= o->f5) {
           acct->n_past_edef++;
           acct->pos_past_edef += (uint64_t)env_len;
           int keep = 1;
           if (o->do_edefbias) {
             const float filtersc = ixo_bias_score(bf, sub, wlen);
-            const double scb = ((double)env_sc - (double)filtersc) / ln2;
-            if (ixo_exp_surv(scb, o->gfmu, o->gflambda) > o->f5b) keep = 0;
+            double scb = ((double)env_sc - (double)filtersc) / ln2;
+            if (safe) scb += (double)corr / ln2;
+            if (ixo_exp_surv(scb, mu, lam) > o->f5b) keep = 0;
             else {
               acct->n_past_edefbias++;
               acct->pos_past_edefbias += (uint64_t)env_len;
@@ -787,4 +861,10 @@ int64_t ixo_env_def_std(const ixo_gm *gm_proto, const ixo_bias *bf, const uint8_
   }
   ixo_gm_free(&gm);
   return n;
+}
+
+int64_t ixo_env_def_std(const ixo_gm *gm_proto, const ixo_bias *bf, const uint8_t *dsq, int64_t L,
+                        const ixo_window *wins, int64_t nwin, const ixo_envopts *o, int32_t *es, int32_t *ee,
+                        int64_t cap, ixo_envacct *acct) {
+  return ixo_env_def_pass(gm_proto, 1, bf, dsq, L, wins, nwin, o, es, ee, cap, acct);
 }

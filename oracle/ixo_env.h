@@ -37,6 +37,8 @@ typedef struct {
   float rt1, rt2, rt3;       /* domain definition thresholds */
   int ns;                    /* stochastic traces per multidomain region */
   int do_gfwdbias, do_edefbias;
+  double lftau, lflambda;    /* local Forward tail (truncated passes) */
+  int clen, pad;             /* consensus length (the 3' pass's correction) */
 } ixo_envopts;
 
 typedef struct {
@@ -52,6 +54,10 @@ void  ixo_gm_copy(const ixo_gm *src, ixo_gm *dst);
 void  ixo_gm_reconfig_length(ixo_gm *gm, int L);
 void  ixo_gm_reconfig_unihit(ixo_gm *gm, int L);
 void  ixo_gm_reconfig_multihit(ixo_gm *gm, int L);
+void  ixo_gm_reconfig_length_5p(ixo_gm *gm, int L);
+void  ixo_gm_reconfig_length_3p(ixo_gm *gm, int L);
+/* the truncated passes' profiles: which 0 = 5' (Rgm), 1 = 3' (Lgm), 2 = 5'&3' (Tgm) */
+int   ixo_gm_build_trunc(const ixo_p7 *p7, int which, ixo_gm *gm);
 /* dp [(L+1)*(M+1)*3] (row i: M[0..M] I[0..M] D[0..M]), xm [(L+1)*5] (E N J B C) */
 float ixo_gforward(const uint8_t *dsq, int L, const ixo_gm *gm, float *dp, float *xm);
 float ixo_gbackward(const uint8_t *dsq, int L, const ixo_gm *gm, float *dp, float *xm);
@@ -59,6 +65,11 @@ void  ixo_gdomain_decoding(const ixo_gm *gm, int L, const float *fx, const float
                            float *mocc);
 int   ixo_domaindef_glocal(ixo_gm *gm, const uint8_t *dsq, int L, const float *fx, const float *bx, float rt1,
                            float rt2, float rt3, int ns, int is_unihit, ixo_domain *out, int cap);
+/* trunc_env_def for pass 1 (STD), 2 (5'), 3 (3') or 4 (5'&3') with that
+ * pass's profile */
+int64_t ixo_env_def_pass(const ixo_gm *gm_proto, int pass, const ixo_bias *bf, const uint8_t *dsq, int64_t L,
+                         const ixo_window *wins, int64_t nwin, const ixo_envopts *o, int32_t *es, int32_t *ee,
+                         int64_t cap, ixo_envacct *acct);
 int64_t ixo_env_def_std(const ixo_gm *gm_proto, const ixo_bias *bf, const uint8_t *dsq, int64_t L,
                         const ixo_window *wins, int64_t nwin, const ixo_envopts *o, int32_t *es, int32_t *ee,
                         int64_t cap, ixo_envacct *acct);

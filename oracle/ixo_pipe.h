@@ -29,6 +29,13 @@ typedef struct {
   const ixo_cp9 *cp9;
   const ixo_hbcm *hbcm;
   const ixo_hbcp9 *hbcp9;
+  /* the truncated passes ([0] 5', [1] 3', [2] 5'&3'): their p7 profiles
+   * (Rgm, Lgm, Tgm), CP9s and band-conversion views; the CM's marginal
+   * emission scores and local truncated-begin penalties l [3][M] */
+  const ixo_gm *gmt[3];
+  const ixo_cp9 *trcp9[3];
+  const ixo_hbcp9 *trhbcp9[3];
+  const float *lmesc, *rmesc, *trp_l;
 } ixo_pipe;
 
 typedef struct {
@@ -40,13 +47,14 @@ typedef struct {
   double fcyk_tau, final_tau, maxtau;
   float size_limit;
   int do_fcyk, do_fcykenv, do_null3, do_cyk, do_top, do_bot;
+  int trunc_mode, pad; /* 1: the terminal truncated passes as well */
 } ixo_pipeopts;
 
 typedef struct {
   int64_t start, stop;
   float score, bias;
   double evalue;
-  int32_t strand, pad;
+  int32_t strand, pass;
 } ixo_pipehit;
 
 typedef struct {
@@ -54,6 +62,10 @@ typedef struct {
   ixo_acct f;
   ixo_envacct e;
   uint64_t n_past_cyk, pos_past_cyk, n_hits;
+  uint64_t tr_nres[3];
+  ixo_acct tr_f[3];
+  ixo_envacct tr_e[3];
+  uint64_t tr_n_past_cyk[3], tr_pos_past_cyk[3], tr_hits[3];
 } ixo_pipeacct;
 
 /* seq: 1-indexed residues seq[1..L]; returns the number of reported hits

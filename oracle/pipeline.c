@@ -20,7 +20,7 @@ typedef struct {
   int64_t start, stop;
   float score, bias;
   double evalue;
-  int strand, removed;
+  int strand, removed, pass;
 } phit;
 
 /* cp9.rs:4053 cp9_seq2bands + :4075 the tau iteration; fills the bands of
@@ -95,9 +95,175 @@ static void push(phit **v, int64_t *n, int64_t *cap, phit h) {
   (*v)[(*n)++] = h;
 }
 
-/* the STD pass of one (chunk x strand) task on its chunk dsq[1..wl] */
+/* cp9.rs:4130 cp9_seq2bands_trunc with the pass's CP9 (q) for
+ * dsq[i0..j0]: Forward / Backward scanning unless the pass forces the first
+ * / final residue, posterior bands and occupancy, start / end prediction,
+ * candidacy, hmm2ij with do_trunc, ij2d with do_trunc. Arrays allocated here. */
+static void trunc_bands(const ixo_pipe *P, int q, int pass, const uint8_t *dsq, int i0, int j0, double tau,
+                        int32_t **jmin, int32_t **jmax, int32_t **hd_off, int32_t **hdmin, int32_t **hdmax,
+                        uint8_t *jv, uint8_t *lv, uint8_t *rv, uint8_t *tv) {
+  const ixo_cp9 *h = P->trcp9[q];
+  const int H = h->M, M = P->hbcm->M, L = j0 - i0 + 1;
+  const int fscan = !(pass == 2 || pass == 4), bscan = !(pass == 3 || pass == 4);
+  const size_t cells = (size_t)(L + 1) * (H + 1);
+  int32_t *mx[8];
+  for (int k = 0; k < 8; k++) mx[k] = (int32_t *)malloc(sizeof(int32_t) * cells);
+  int32_t *sca = (int32_t *)malloc(sizeof(int32_t) * (size_t)(L + 1));
+  int32_t *pn = (int32_t *)malloc(sizeof(int32_t) * 6 * (size_t)(H + 1));
+  int32_t *occ = (int32_t *)malloc(sizeof(int32_t) * (size_t)(H + 1));
+  int64_t bp;
+  ixo_cp9_forward(h, dsq, i0, j0, fscan, 0, mx[0], mx[1], mx[2], mx[3], sca, &bp);
+  ixo_cp9_backward(h, dsq, i0, j0, bscan, 0, mx[4], mx[5], mx[6], mx[7], sca, &bp);
+  ixo_cp9_fb2hmmbands(h, dsq, mx[0], mx[1], mx[2], mx[4], mx[5], mx[6], i0, j0, 1.0 - tau, fscan, pn);
+  ixo_cp9_occupancy(h, dsq, mx[0], mx[2], mx[4], mx[6], i0, j0, fscan, occ);
+  ixo_trband tb;
+  tb.jvalid = jv, tb.lvalid = lv, tb.rvalid = rv, tb.tvalid = tv;
+  ixo_tr_predict_start_end(pn, occ, H, i0, j0, 0.01f, 0.98f, &tb);
+  ixo_tr_candidates(P->hbcm, pass, &tb);
+  int32_t *imin = (int32_t *)malloc(sizeof(int32_t) * M), *imax = (int32_t *)malloc(sizeof(int32_t) * M);
+  *jmin = (int32_t *)malloc(sizeof(int32_t) * M);
+  *jmax = (int32_t *)malloc(sizeof(int32_t) * M);
+  *hd_off = (int32_t *)malloc(sizeof(int32_t) * (M + 1));
+  ixo_hmm2ij_bands_tr(P->hbcm, P->trhbcp9[q], pn, i0, j0, 1, &tb, imin, imax, *jmin, *jmax);
+  const int64_t rows = ixo_hb_ij2d(P->hbcm, imin, imax, *jmin, *jmax, 1, *hd_off, NULL, NULL, 0);
+  *hdmin = (int32_t *)malloc(sizeof(int32_t) * (size_t)(rows + 1));
+  *hdmax = (int32_t *)malloc(sizeof(int32_t) * (size_t)(rows + 1));
+  ixo_hb_ij2d(P->hbcm, imin, imax, *jmin, *jmax, 1, *hd_off, *hdmin, *hdmax, rows);
+  for (int k = 0; k < 8; k++) free(mx[k]);
+  free(sca);
+  free(pn);
+  free(occ);
+  free(imin);
+  free(imax);
+}
+
+/* cm_search.rs:2380-2420 + :2863 cm_trunc_stage: the truncated pass `pass`
+ * (q = pass - 2) of a chunk dsq[1..wl] whose STD pass had survivors: its
+ * subsequence (the chunk, or its first / last maxW residues), the filter
+ * (the STD windows when the subsequence is the chunk), trunc_env_def, the
+ * TrCYK filter and FTrInside; hits shifted back into chunk coordinates. */
+static void trunc_pass(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *dsq, int64_t wl, int64_t wgs,
+                       int64_t wge, int strand, int pass, const ixo_window *std_wins, int64_t std_nw,
+                       const ixo_acct *std_acct, ixo_pipeacct *A, phit **hits, int64_t *nh, int64_t *hcap) {
+  const int q = pass - 2;
+  const int64_t maxw = o->f3.maxw;
+  const uint8_t *sub = dsq;
+  int64_t sublen = wl, off = 0;
+  uint8_t *own = NULL;
+  if (!(wl <= maxw || pass == 4)) {
+    sublen = maxw;
+    off = pass == 2 ? 0 : wl - maxw;
+    own = (uint8_t *)malloc((size_t)maxw + 2);
+    own[0] = own[maxw + 1] = 255;
+    memcpy(own + 1, dsq + 1 + off, (size_t)maxw);
+    sub = own;
+  }
+  const int64_t wcap = 1 << 16;
+  ixo_window *wins = (ixo_window *)malloc(sizeof(ixo_window) * wcap);
+  ixo_acct fa;
+  int64_t nw;
+  if (sub == dsq) {
+    nw = std_nw < wcap ? std_nw : wcap;
+    memcpy(wins, std_wins, sizeof(ixo_window) * (size_t)nw);
+    fa = *std_acct;
+  } else {
+    memset(&fa, 0, sizeof(fa));
+    nw = ixo_f3_filter_sequence(P->msv, P->vit, P->fwd, P->bias, P->p7, sub, sublen, &o->f3, wins, wcap, &fa);
+    if (nw > wcap) nw = wcap;
+  }
+  {
+    const uint64_t *s = (const uint64_t *)&fa;
+    uint64_t *d = (uint64_t *)&A->tr_f[q];
+    for (size_t k = 0; k < sizeof(ixo_acct) / 8; k++) d[k] += s[k];
+  }
+  if (nw > 0) {
+    const int64_t ecap = 4096;
+    int32_t *es = (int32_t *)malloc(sizeof(int32_t) * ecap), *ee = (int32_t *)malloc(sizeof(int32_t) * ecap);
+    ixo_envacct ea;
+    int64_t ne = ixo_env_def_pass(P->gmt[q], pass, P->bias, sub, sublen, wins, nw, &o->env, es, ee, ecap, &ea);
+    if (ne > ecap) ne = ecap;
+    {
+      const uint64_t *s = (const uint64_t *)&ea;
+      uint64_t *d = (uint64_t *)&A->tr_e[q];
+      for (size_t k = 0; k < sizeof(ixo_envacct) / 8; k++) d[k] += s[k];
+    }
+    const int M = P->cm->M;
+    const float *pty = P->trp_l + (size_t)(pass == 4 ? 0 : (pass == 2 ? 1 : 2)) * M;
+    uint8_t *jv = (uint8_t *)malloc((size_t)M), *lv = (uint8_t *)malloc((size_t)M), *rv = (uint8_t *)malloc((size_t)M),
+            *tv = (uint8_t *)malloc((size_t)M);
+    ixo_trvalid val = {jv, lv, rv, tv};
+    const int ef = pass != 3, el = pass != 2;
+    for (int64_t k = 0; k < ne; k++) {
+      int e0 = es[k], e1 = ee[k];
+      int32_t *jmin, *jmax, *hd_off, *hdmin, *hdmax;
+      if (o->do_fcyk) {
+        trunc_bands(P, q, pass, sub, e0, e1, o->fcyk_tau, &jmin, &jmax, &hd_off, &hdmin, &hdmax, jv, lv, rv, tv);
+        ixo_bands b = {jmin, jmax, hd_off, hdmin, hdmax};
+        float vsc;
+        int vmode;
+        int64_t ei, ej;
+        ixo_trhb_scan(P->cm, P->sc, P->lmesc, P->rmesc, pty, &b, &val, sub, e0, e1, pass, 1, 0, 0.0f, o->do_null3,
+                      o->cyk_env_cutoff, &vsc, &vmode, &ei, &ej, NULL, 0, NULL);
+        free_bands(jmin, jmax, hd_off, hdmin, hdmax);
+        if (ixo_exp_surv((double)vsc, o->cyk_mu, o->cyk_lambda) > o->f6) continue;
+        if (o->do_fcykenv && ei != -1 && ej != -1) {
+          if (!ef) e0 = (int)ei;
+          if (!el) e1 = (int)ej;
+        }
+        A->tr_n_past_cyk[q]++;
+        A->tr_pos_past_cyk[q] += (uint64_t)(e1 - e0 + 1);
+      }
+      es[k] = e0, ee[k] = e1;
+      /* F7 for this envelope (the reference collects the survivors first;
+       * every survivor's final round is independent of the others) */
+      trunc_bands(P, q, pass, sub, e0, e1, o->final_tau, &jmin, &jmax, &hd_off, &hdmin, &hdmax, jv, lv, rv, tv);
+      ixo_bands b = {jmin, jmax, hd_off, hdmin, hdmax};
+      const int64_t hc = 4096;
+      ixo_trhit *th = (ixo_trhit *)malloc(sizeof(ixo_trhit) * hc);
+      float vsc;
+      int vmode;
+      int64_t ei, ej;
+      int64_t nt = ixo_trhb_scan(P->cm, P->sc, P->lmesc, P->rmesc, pty, &b, &val, sub, e0, e1, pass, 1,
+                                 o->do_cyk ? 0 : 1, o->pli_t, o->do_null3, 0.0f, &vsc, &vmode, &ei, &ej, th, hc, NULL);
+      if (nt > hc) nt = hc;
+      for (int64_t a = 0; a < nt; a++) {
+        phit h;
+        memset(&h, 0, sizeof(h));
+        const int64_t hi = th[a].i + off, hj = th[a].j + off;
+        if (strand) h.start = wge - hi + 1, h.stop = wge - hj + 1;
+        else h.start = wgs + hi - 1, h.stop = wgs + hj - 1;
+        h.score = th[a].score;
+        h.bias = th[a].bias;
+        h.evalue = ixo_exp_surv((double)h.score, o->fin_mu, o->fin_lambda) * o->ez;
+        h.strand = strand;
+        h.pass = pass;
+        push(hits, nh, hcap, h);
+        A->tr_hits[q]++;
+      }
+      free(th);
+      free_bands(jmin, jmax, hd_off, hdmin, hdmax);
+    }
+    free(jv);
+    free(lv);
+    free(rv);
+    free(tv);
+    free(es);
+    free(ee);
+  }
+  free(wins);
+  free(own);
+}
+
+/* the STD pass of one (chunk x strand) task on its chunk dsq[1..wl], then
+ * its truncated passes (h5 / h3: the chunk holds the sequence's 5' / 3' end
+ * on its strand) */
 static void one_task(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *dsq, int64_t wl, int64_t wgs,
-                     int64_t wge, int strand, ixo_pipeacct *A, phit **hits, int64_t *nh, int64_t *hcap) {
+                     int64_t wge, int strand, int h5, int h3, ixo_pipeacct *A, phit **hits, int64_t *nh,
+                     int64_t *hcap) {
+  const int64_t maxw = o->f3.maxw;
+  const int run[3] = {o->trunc_mode && h5, o->trunc_mode && h3, o->trunc_mode && h5 && h3 && wl <= maxw};
+  for (int q = 0; q < 3; q++)  /* charged before any early out (cm_search.rs:2283-2300) */
+    if (run[q]) A->tr_nres[q] += (uint64_t)(maxw < wl ? maxw : wl);
   const int64_t wcap = 1 << 16;
   ixo_window *wins = (ixo_window *)malloc(sizeof(ixo_window) * wcap);
   ixo_acct fa;
@@ -191,6 +357,7 @@ static void one_task(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *ds
       h.bias = raw[a].bias;
       h.evalue = ixo_exp_surv((double)h.score, o->fin_mu, o->fin_lambda) * o->ez;
       h.strand = strand;
+      h.pass = 1;
       push(hits, nh, hcap, h);
     }
     free(gone);
@@ -202,6 +369,8 @@ static void one_task(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *ds
   free(se);
   free(es);
   free(ee);
+  for (int q = 0; q < 3; q++)
+    if (run[q]) trunc_pass(P, o, dsq, wl, wgs, wge, strand, 2 + q, wins, nw, &fa, A, hits, nh, hcap);
   free(wins);
 }
 
@@ -218,11 +387,12 @@ static int fin_cmp(const void *x, const void *y) {
   if (a->evalue != b->evalue) return a->evalue < b->evalue ? -1 : 1;
   if (a->score != b->score) return a->score > b->score ? -1 : 1;
   if (a->start != b->start) return a->start < b->start ? -1 : 1;
+  if (a->pass != b->pass) return a->pass > b->pass ? -1 : 1;
   return (a->stop > b->stop) - (a->stop < b->stop);
 }
 
 typedef struct {
-  int64_t wgs, wge, wl, ctx;
+  int64_t wgs, wge, wl, ctx, L;
   int strand;
 } otask;
 
@@ -238,7 +408,7 @@ static int64_t make_tasks(const ixo_pipeopts *o, int64_t L, otask *out) {
     const int64_t wl = ctx + cnt, wge = wgs + wl - 1;
     for (int strand = 0; strand < 2; strand++) {
       if ((strand == 0 && !o->do_top) || (strand == 1 && !o->do_bot)) continue;
-      if (out) out[n] = (otask){wgs, wge, wl, ctx, strand};
+      if (out) out[n] = (otask){wgs, wge, wl, ctx, L, strand};
       n++;
     }
     new_start = wge + 1;
@@ -255,7 +425,8 @@ static void run_task(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *se
     dsq[k] = t->strand ? (x < 4 ? (uint8_t)(3 - x) : x) : x;
   }
   A->nres += (uint64_t)(t->wl - t->ctx);
-  one_task(P, o, dsq, t->wl, t->wgs, t->wge, t->strand, A, hits, nh, hcap);
+  const int h5 = t->strand ? t->wge == t->L : t->wgs == 1, h3 = t->strand ? t->wgs == 1 : t->wge == t->L;
+  one_task(P, o, dsq, t->wl, t->wgs, t->wge, t->strand, h5, h3, A, hits, nh, hcap);
 }
 
 /* finalize (cm_search.rs:1915): global overlap removal, E-value sort,
@@ -292,6 +463,7 @@ static int64_t finalize(const ixo_pipeopts *o, phit *hits, int64_t nh, ixo_pipeh
       out[n].bias = hits[q].bias;
       out[n].evalue = hits[q].evalue;
       out[n].strand = hits[q].strand;
+      out[n].pass = hits[q].pass;
     }
     n++;
   }

@@ -40,6 +40,7 @@ def _init():
         L.ixo_cp9_band_thresh.restype = C.c_int32
         L.ixo_cp9_fb2hmmbands.argtypes = [C.POINTER(Cp9), _u8p] + [_i32p] * 6 + [C.c_int, C.c_int, C.c_double,
                                                                                    C.c_int, _i32p]
+        L.ixo_cp9_occupancy.argtypes = [C.POINTER(Cp9), _u8p] + [_i32p] * 4 + [C.c_int, C.c_int, C.c_int, _i32p]
         _ready = True
     return L
 
@@ -87,6 +88,30 @@ class OracleCP9:
         lib().ixo_cp9_fb2hmmbands(C.byref(self.s), _p(d, _u8p), *(_p(m) for m in mats), int(i0), int(j0),
                                   1.0 - tau, int(scan), _p(pn))
         return pn.reshape(6, self.h["M"] + 1)
+
+
+    def trunc_pass_bands(self, dsq, i0, j0, tau, pass_idx):
+        """cp9_seq2bands_trunc's DPs for a truncated pass: (pn [6, M+1], occ
+        [M+1]); the Forward / Backward scan unless the pass forces the first
+        / final residue."""
+        fscan = pass_idx not in (2, 4)
+        bscan = pass_idx not in (3, 4)
+        f = self.forward(dsq, i0, j0, fscan, False)
+        b = self.backward(dsq, i0, j0, bscan, False)
+        pn = np.zeros(6 * (self.h["M"] + 1), np.int32)
+        d = np.ascontiguousarray(dsq, np.uint8)
+        mats = [np.ascontiguousarray(x) for x in (f["mmx"], f["imx"], f["dmx"], b["mmx"], b["imx"], b["dmx"])]
+        lib().ixo_cp9_fb2hmmbands(C.byref(self.s), _p(d, _u8p), *(_p(m) for m in mats), int(i0), int(j0),
+                                  1.0 - tau, int(fscan), _p(pn))
+        occ = np.zeros(self.h["M"] + 1, np.int32)
+        lib().ixo_cp9_occupancy(C.byref(self.s), _p(d, _u8p), _p(mats[0]), _p(mats[2]), _p(mats[3]), _p(mats[5]),
+                                int(i0), int(j0), int(fscan), _p(occ))
+        return pn.reshape(6, self.h["M"] + 1), occ
+
+
+class TrBand(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("sp1", "sp2", "ep1", "ep2", "rmarg_imin", "rmarg_imax", "lmarg_jmin",
+                                         "lmarg_jmax")] + [(n, _u8p) for n in ("jvalid", "lvalid", "rvalid", "tvalid")]
 
 
 # ---- HMM -> CM band conversion (oracle/hmmband.c) ----------------------------
@@ -145,6 +170,29 @@ class OracleHb:
         lib().ixo_hmm2ij_bands(C.byref(self.c), C.byref(self.p), _p(pn), int(i0), int(j0), int(search),
                                *[_p(a) for a in out])
         return (*out, pn)
+
+    def hmm2ij_trunc(self, pn, occ, i0, j0, pass_idx):
+        """cp9_seq2bands_trunc's steps 3-4: (imin, imax, jmin, jmax, jvalid,
+        lvalid, rvalid, tvalid, [sp1 sp2 ep1 ep2 rmarg_imin rmarg_imax
+        lmarg_jmin lmarg_jmax])."""
+        L = lib()
+        L.ixo_tr_predict_start_end.argtypes = [_i32p, _i32p, C.c_int, C.c_int, C.c_int, C.c_float, C.c_float,
+                                               C.POINTER(TrBand)]
+        L.ixo_tr_candidates.argtypes = [C.POINTER(HbCm), C.c_int, C.POINTER(TrBand)]
+        L.ixo_hmm2ij_bands_tr.argtypes = [C.POINTER(HbCm), C.POINTER(HbCp9), _i32p, C.c_int, C.c_int, C.c_int,
+                                          C.POINTER(TrBand)] + [_i32p] * 4
+        pn = np.ascontiguousarray(pn, np.int32).copy()
+        occ = np.ascontiguousarray(occ, np.int32)
+        val = [np.zeros(self.M, np.uint8) for _ in range(4)]
+        tb = TrBand()
+        tb.jvalid, tb.lvalid, tb.rvalid, tb.tvalid = [_p(v, _u8p) for v in val]
+        L.ixo_tr_predict_start_end(_p(pn), _p(occ), int(self.p.hmm_m), int(i0), int(j0), 0.01, 0.98, C.byref(tb))
+        L.ixo_tr_candidates(C.byref(self.c), int(pass_idx), C.byref(tb))
+        out = [np.zeros(self.M, np.int32) for _ in range(4)]
+        L.ixo_hmm2ij_bands_tr(C.byref(self.c), C.byref(self.p), _p(pn), int(i0), int(j0), 1, C.byref(tb),
+                              *[_p(a) for a in out])
+        spep = [tb.sp1, tb.sp2, tb.ep1, tb.ep2, tb.rmarg_imin, tb.rmarg_imax, tb.lmarg_jmin, tb.lmarg_jmax]
+        return (*out, *val, spep)
 
     def ij2d(self, imin, imax, jmin, jmax, trunc=False):
         a = [np.ascontiguousarray(x, np.int32) for x in (imin, imax, jmin, jmax)]

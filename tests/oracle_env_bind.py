@@ -22,7 +22,8 @@ class Domain(C.Structure):
 class EnvOpts(C.Structure):
     _fields_ = [("f4", C.c_double), ("f4b", C.c_double), ("f5", C.c_double), ("f5b", C.c_double),
                 ("gfmu", C.c_double), ("gflambda", C.c_double), ("rt1", C.c_float), ("rt2", C.c_float),
-                ("rt3", C.c_float), ("ns", C.c_int), ("do_gfwdbias", C.c_int), ("do_edefbias", C.c_int)]
+                ("rt3", C.c_float), ("ns", C.c_int), ("do_gfwdbias", C.c_int), ("do_edefbias", C.c_int),
+                ("lftau", C.c_double), ("lflambda", C.c_double), ("clen", C.c_int32), ("pad", C.c_int32)]
 
 
 class EnvAcct(C.Structure):
@@ -58,16 +59,22 @@ def lib():
                                           C.POINTER(EnvOpts), C.POINTER(C.c_int32), C.POINTER(C.c_int32), i64,
                                           C.POINTER(EnvAcct)]
             L.ixo_env_def_std.restype = i64
+            L.ixo_env_def_pass.argtypes = [C.POINTER(Gm), C.c_int, C.POINTER(OB.Bias), _u8p, i64,
+                                           C.POINTER(OB.Window), i64, C.POINTER(EnvOpts), C.POINTER(C.c_int32),
+                                           C.POINTER(C.c_int32), i64, C.POINTER(EnvAcct)]
+            L.ixo_env_def_pass.restype = i64
+            L.ixo_gm_build_trunc.argtypes = [C.POINTER(OB.P7), C.c_int, C.POINTER(Gm)]
             _ready = True
     return L
 
 
 def opts(f4=0.005, f4b=0.005, f5=0.005, f5b=0.005, gfmu=-5.0, gflambda=0.69, rt1=0.25, rt2=0.10, rt3=0.20, ns=200,
-         do_gfwdbias=True, do_edefbias=False):
+         do_gfwdbias=True, do_edefbias=False, lftau=-4.0, lflambda=0.69, clen=1):
     o = EnvOpts()
     o.f4, o.f4b, o.f5, o.f5b, o.gfmu, o.gflambda = f4, f4b, f5, f5b, gfmu, gflambda
     o.rt1, o.rt2, o.rt3, o.ns = rt1, rt2, rt3, ns
     o.do_gfwdbias, o.do_edefbias = int(bool(do_gfwdbias)), int(bool(do_edefbias))
+    o.lftau, o.lflambda, o.clen = lftau, lflambda, int(clen)
     return o
 
 
@@ -130,7 +137,9 @@ class OracleEnv:
                                        bx.ctypes.data_as(_f32p), rt1, rt2, rt3, ns, int(unihit), out, 256)
         return [(out[i].ienv, out[i].jenv, out[i].envsc) for i in range(min(n, 256))]
 
-    def env_def(self, dsq, L, windows, o):
+    def env_def(self, dsq, L, windows, o, pass_idx=1):
+        """trunc_env_def of one chunk for pass 1 (STD, gm) or a truncated
+        pass 2 / 3 / 4 (Rgm / Lgm / Tgm built here)."""
         n = len(windows)
         arr = (OB.Window * max(n, 1))()
         for k, (a, b) in enumerate(windows):
@@ -138,6 +147,12 @@ class OracleEnv:
         cap = 64 + 8 * n
         es, ee = (C.c_int32 * cap)(), (C.c_int32 * cap)()
         acct = EnvAcct()
-        ne = lib().ixo_env_def_std(C.byref(self.gm), C.byref(self.base.sbias), OB.u8(dsq), int(L), arr, n,
-                                   C.byref(o), es, ee, cap, C.byref(acct))
+        gm = self.gm
+        if pass_idx != 1:
+            gm = Gm()
+            lib().ixo_gm_build_trunc(C.byref(self.base.p7), int(pass_idx) - 2, C.byref(gm))
+        ne = lib().ixo_env_def_pass(C.byref(gm), int(pass_idx), C.byref(self.base.sbias), OB.u8(dsq), int(L), arr, n,
+                                    C.byref(o), es, ee, cap, C.byref(acct))
+        if pass_idx != 1:
+            lib().ixo_gm_free(C.byref(gm))
         return [(es[i], ee[i]) for i in range(min(ne, cap))], acct

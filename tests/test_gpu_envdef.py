@@ -108,3 +108,34 @@ def test_env_def_parity(ctx, M):
             assert getattr(ga, f) == getattr(wa, f), (kw, f)
         assert len(want) >= 3  # planted copies are found
     g.free()
+
+
+@pytest.mark.parametrize("M", [72, 120])
+def test_env_def_truncated_passes_parity(ctx, M):
+    # trunc_env_def with the 5' (Rgm), 3' (Lgm) and 5'&3' (Tgm) profiles: the
+    # local-tail statistics, the 3' pass's ln(1/clen) correction, the
+    # enforced first / final residue, and the profile a multidomain region
+    # leaves behind (unihit at that window's length) carried into the later
+    # windows of the chunk (rt3 = 0 makes every region multidomain)
+    p = _profile(M, 300 + M)
+    p.update(gfmu=-9.0, gflambda=0.69)
+    o = OracleEnv(p, 100)
+    g = ixg.P7(ctx, p, int(1.25 * M) + 20)
+    L = 4 * M + 80
+    d = _chunk(p, L, M + 7, [10, M + 30, 3 * M + 40])
+    W = int(1.3 * M)
+    cases = {2: [(1, L - 5), (1, 2 * M + 10), (1, M + 40), (3, M + 40)],
+             3: [(5, L), (M, L), (2 * M + 20, L), (M, L - 1)],
+             4: [(1, L), (2, L), (1, L)]}
+    for pass_idx, wins in cases.items():
+        for kw in (dict(), dict(rt3=0.0), dict(rt2=0.9, rt3=0.0), dict(do_edefbias=True, f5b=0.005)):
+            kw = dict(kw, lftau=-5.5, lflambda=0.69, clen=M, gfmu=-9.0, gflambda=0.69, f4=0.05, f5=0.05)
+            _say("env_def truncated", M, pass_idx, kw)
+            want, wa = o.env_def(d, L, wins, oopts(**kw), pass_idx=pass_idx)
+            got, ga = ixg.env_def(g, d, L, wins, ixg.env_opts(**kw), pass_idx=pass_idx)
+            assert got == want, (pass_idx, kw, got, want)
+            for f, _ in ixg.EnvAcct._fields_:
+                assert getattr(ga, f) == getattr(wa, f), (pass_idx, kw, f)
+            if "rt3" not in kw and pass_idx != 4:
+                assert len(want) >= 1, (pass_idx, kw)
+    g.free()

@@ -80,3 +80,77 @@ def test_search_std_parity(ctx, L0, nhom, seed):
     gcm.free()
     g7.free()
     hb.free()
+
+
+def _trunc_target(cm, body_len, seed, ends):
+    """A random target whose ends hold truncated homologs: ends is a list of
+    (end, strand, frac) — the homolog sampled from the CM, cut to frac of
+    its length at the target's 5' (end 0) or 3' (end 1) end, on the given
+    strand (1: its reverse complement)."""
+    from infernox_amd import synth_cm as S
+    r = np.random.default_rng(seed)
+    left, right = [], []
+    for k, (end, strand, frac) in enumerate(ends):
+        h = S.emit(cm, seed * 100 + k)
+        n = max(8, int(len(h) * frac))
+        if strand:
+            h = np.where(h < 4, 3 - h, h)[::-1]
+        # on its own strand the homolog is cut at the end that the target's
+        # end is for that strand
+        cut_front = (end == 0) != bool(strand)
+        part = h[len(h) - n:] if cut_front else h[:n]
+        (left if end == 0 else right).append(part)
+    body = r.integers(0, 4, body_len).astype(np.uint8)
+    seq = np.concatenate(left + [body] + right).astype(np.uint8)
+    return np.concatenate([[255], seq, [255]]).astype(np.uint8), len(seq)
+
+
+def _same_tr_acct(a, b):
+    _same_acct(a, b)
+    for q in range(3):
+        assert a.tr_nres[q] == b.tr_nres[q], q
+        for f, _ in ixg.Acct._fields_:
+            assert getattr(a.tr_f[q], f) == getattr(b.tr_f[q], f), (q, f)
+        for f, _ in ixg.EnvAcct._fields_:
+            assert getattr(a.tr_e[q], f) == getattr(b.tr_e[q], f), (q, f)
+        assert (a.tr_n_past_cyk[q], a.tr_pos_past_cyk[q], a.tr_hits[q]) == \
+            (b.tr_n_past_cyk[q], b.tr_pos_past_cyk[q], b.tr_hits[q]), q
+
+
+@pytest.mark.parametrize("body,seed,ends", [
+    (3000, 21, [(0, 0, 0.6), (1, 0, 0.6)]),
+    (3000, 22, [(0, 1, 0.5), (1, 1, 0.7)]),
+    (20, 23, [(0, 0, 0.7)]),
+    (230000, 24, [(0, 0, 0.6), (1, 1, 0.6), (1, 0, 0.5)]),
+])
+def test_search_truncated_passes_parity(ctx, body, seed, ends):
+    from infernox_amd import synth_cp9 as P
+    p7, cm, cp9, cmev = models.config1_models()
+    trdicts = P.build_trunc_cp9s(cm, 1)
+    dsq, L = _trunc_target(cm, body, seed, ends)
+    W = int(cm["W"])
+    g7 = ixg.P7(ctx, p7, W)
+    gcm = ixg.CM(ctx, cm)
+    g, l, ig, il = ixg.cm_trpenalties(cm)
+    gcm.set_trpenalties(g, l)
+    g9 = ixg.CP9(ctx, cp9)
+    hb = ixg.HbMap(cm, cp9)
+    tr9 = [ixg.CP9(ctx, h) for h in trdicts]
+    trhb = [ixg.HbMap(cm, h) for h in trdicts]
+    tgt = ixg.Target(ctx, dsq, L)
+    osr = OracleSearch(p7, cm, cp9, W, trcp9=trdicts)
+    for kw in (dict(trunc=True), dict(trunc=True, do_fcykenv=False), dict(trunc=True, do_cyk=True)):
+        _say("search truncated", body, seed, kw)
+        o = _opts(p7, cm, cmev, L, **kw)
+        o.env.lftau, o.env.lflambda, o.env.clen = p7["lftau"], p7["lflambda"], int(cm["clen"])
+        got, ga = ixg.search(ctx, g7, gcm, g9, hb, tr9, trhb, tgt, o)
+        want, wa = osr.search(dsq, L, o)
+        assert [h[:3] + h[6:] for h in got] == [h[:3] + h[6:] for h in want], (kw, got, want)
+        assert np.array_equal(np.float32([h[3] for h in got]), np.float32([h[3] for h in want]))
+        assert np.array_equal(np.float32([h[4] for h in got]), np.float32([h[4] for h in want]))
+        assert [h[5] for h in got] == [h[5] for h in want]
+        _same_tr_acct(ga, wa)
+        if kw == dict(trunc=True):
+            assert sum(wa.tr_hits) >= 1, want  # a truncated pass reported a planted end
+    for x in [tgt, g9, gcm, g7, hb] + tr9 + trhb:
+        x.free()

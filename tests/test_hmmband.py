@@ -77,3 +77,47 @@ def test_converted_bands_keep_the_homolog_parse(ss, seed):
     assert hb_cyk >= best - 1.0, (hb_cyk, best)
     mb, cells = ob.mx_mb(b, j0 - i0 + 1)
     assert cells > 0 and mb > 0
+
+
+def _trunc_case(ss, seed, cut, pad=30):
+    """A window whose planted homolog is cut at its 5' end (cut < 0: the
+    first -cut fraction removed) or its 3' end (cut > 0), touching the window
+    edge there."""
+    cm = S.build_cm(ss, seed)
+    hom = S.emit(cm, seed + 100)
+    r = np.random.default_rng(seed)
+    n = int(len(hom) * abs(cut))
+    part = hom[n:] if cut < 0 else hom[:len(hom) - n]
+    seq = (np.concatenate([part, r.integers(0, 4, pad)]) if cut < 0 else
+           np.concatenate([r.integers(0, 4, pad), part])).astype(np.uint8)
+    dsq = np.concatenate([[255], seq, [255]]).astype(np.uint8)
+    return cm, dsq, 1, len(seq)
+
+
+@pytest.mark.parametrize("ss,seed", STRUCTS[2:])
+def test_hmm2ij_trunc_matches_oracle(ss, seed):
+    # cp9_seq2bands_trunc's conversion for each truncated pass on windows
+    # whose homolog is cut at one end: the product's host code against the
+    # oracle's restatement (start / end prediction, marginal bands,
+    # candidacy, the widened and special-cased hmm2ij, ij2d with do_trunc)
+    for cut, passes in ((-0.4, (2, 4, 5)), (0.4, (3, 4, 5))):
+        cm, dsq, i0, j0 = _trunc_case(ss, seed, cut)
+        for q, cp9 in enumerate(P.build_trunc_cp9s(cm, seed)):
+            o9 = OracleCP9(cp9)
+            hb = ixg.HbMap(cm, cp9)
+            ob = OracleHb(cm, cp9)
+            for pass_idx in passes:
+                for tau in (1e-7, 1e-4):
+                    pn, occ = o9.trunc_pass_bands(dsq, i0, j0, tau, pass_idx)
+                    got = hb.hmm2ij_trunc(pn, occ, i0, j0, pass_idx)
+                    want = ob.hmm2ij_trunc(pn, occ, i0, j0, pass_idx)
+                    for k, (g, w) in enumerate(zip(got, want)):
+                        assert np.array_equal(np.asarray(g), np.asarray(w)), (ss, q, pass_idx, tau, k)
+                    imin, imax, jmin, jmax = got[:4]
+                    b = hb.ij2d(imin, imax, jmin, jmax, trunc=True)
+                    wb = ob.ij2d(imin, imax, jmin, jmax, trunc=True)
+                    for f in ("hd_off", "hdmin", "hdmax"):
+                        assert np.array_equal(b[f], wb[f]), (ss, f)
+                    sp1, sp2, ep1, ep2 = got[8][:4]
+                    assert 1 <= sp1 <= sp2 <= cp9["M"] + 1 and 0 <= ep2 <= ep1 <= cp9["M"]
+                    assert got[4][0] or got[5][0] or got[6][0] or got[7][0]  # some mode reaches the root

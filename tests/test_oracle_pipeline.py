@@ -45,3 +45,42 @@ def test_oracle_search_invariants():
     h4, a4 = osr.search(dsq, L, o, threads=4)
     assert h4 == hits
     assert bytes(a4) == bytes(a)
+
+
+def test_oracle_truncated_passes():
+    # homologs cut at the target's ends on both strands: the STD pass alone
+    # misses them, the terminal truncated passes (5' on the first maxW
+    # residues of the 5'-end chunk, 3' on the last maxW of the 3'-end one)
+    # report them with their pass; residues are charged per pass as
+    # min(maxW, chunk) (cm_search.rs:2283-2300); a target of at most maxW
+    # residues also runs the 5'&3' pass; the threaded run is identical
+    from infernox_amd import synth_cm as S, synth_cp9 as P
+    p7, cm, cp9, cmev = models.config1_models()
+    W = int(cm["W"])
+    tr = P.build_trunc_cp9s(cm, 1)
+    osr = OracleSearch(p7, cm, cp9, W, trcp9=tr)
+    r = np.random.default_rng(7)
+    h1, h2 = S.emit(cm, 11), S.emit(cm, 12)
+    seq = np.concatenate([h1[len(h1) // 3:], r.integers(0, 4, 3000), h2[:2 * len(h2) // 3]]).astype(np.uint8)
+    L = len(seq)
+    dsq = np.concatenate([[255], seq, [255]]).astype(np.uint8)
+    o = _opts(p7, cm, cmev, L)
+    o.env.lftau, o.env.lflambda, o.env.clen = p7["lftau"], p7["lflambda"], int(cm["clen"])
+    std, _ = osr.search(dsq, L, o)
+    o.trunc_mode = 1
+    hits, a = osr.search(dsq, L, o)
+    assert std == []
+    assert sorted(h[6] for h in hits) == [2, 3], hits
+    five = [h for h in hits if h[6] == 2][0]
+    three = [h for h in hits if h[6] == 3][0]
+    assert five[0] == 1 and five[2] == 0 and three[1] == L and three[2] == 0
+    assert list(a.tr_nres) == [2 * W, 2 * W, 0]  # both strands hold both ends of the one chunk
+    assert list(a.tr_hits[:2]) == [1, 1]
+    h4, a4 = osr.search(dsq, L, o, threads=4)
+    assert h4 == hits and bytes(a4) == bytes(a)
+    # a target shorter than maxW: 5', 3' and 5'&3' passes on the whole chunk
+    part = h1[: min(len(h1), W - 10)]
+    short = np.concatenate([[255], part, [255]]).astype(np.uint8)
+    Ls = len(part)
+    hs, as_ = osr.search(short, Ls, o)
+    assert list(as_.tr_nres) == [2 * Ls] * 3
