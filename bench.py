@@ -5,14 +5,16 @@ cloverleaf CM (stems 7/4/5/5 bp, loops 7-9 nt) estimated from a seeded
 40-sequence alignment (seed 1), its CP9 and p7 filter HMM, searched against
 a 1 Mb iid uniform ACGU target (seed 2) with 20 CM-sampled homologs planted
 on random strands, both strands, thresholds of the 2-20 Mb tier (F1 0.35,
-F3 = F3b = F4 = F4b = F5 = 0.005, F6 1e-4). One step = one ixg_search_std
-call over the whole target with its sequence resident in HBM: the STD pass
-F1 SSV -> F3 Forward -> F3b bias -> F4/F4b glocal Forward -> glocal Backward
-and domain definition -> F5 -> CP9 bands (tau iteration) -> F6 HB CYK with
-the CYK envelope redefinition -> CP9 bands at the final tau -> F7 HB Inside
-with null3 and greedy hit reporting -> genome coordinates, E-values,
-overlap removal and sort. Not in the step: the truncated-hit passes and
-hit alignment.
+F3 = F3b = F4 = F4b = F5 = 0.005, F6 1e-4). One step = one ixg_search call
+over the whole target with its sequence resident in HBM, cmsearch's default
+pipeline: the STD pass F1 SSV -> F3 Forward -> F3b bias -> F4/F4b glocal
+Forward -> glocal Backward and domain definition -> F5 -> CP9 bands (tau
+iteration) -> F6 HB CYK with the CYK envelope redefinition -> CP9 bands at
+the final tau -> F7 HB Inside with null3 and greedy hit reporting, and the
+terminal truncated passes (5' and 3' on the maxW residues at the target's
+ends, each with its filter cascade, envelope definition, truncated CP9
+bands, TrCYK filter and FTrInside) -> genome coordinates, E-values, overlap
+removal and sort. Not in the step: hit alignment.
 
 Secondary sections, each a rate beside the oracle's: the F1-F3b cascade on
 a 4.6 Mb genome with a 300-column profile, the HB / truncated HB / QDB
@@ -69,8 +71,9 @@ def search_opts(p7, cm, cmev, L):
     W = int(cm["W"])
     f3 = ixg.F3Opts(f1=0.35, f1b=1.0, f2=1.0, f2b=1.0, f3=0.005, f3b=0.005, cmw=W, maxw=W,
                     opts=ixg.DO_MSV | ixg.DO_FWD | ixg.DO_FWDBIAS)
-    env = ixg.env_opts(f4=0.005, f4b=0.005, f5=0.005, gfmu=p7["gfmu"], gflambda=p7["gflambda"])
-    return ixg.search_opts(f3, env, 1e-4, cmev["lc"], cmev["li"], 2.0 * L / (2 * W), W=W)
+    env = ixg.env_opts(f4=0.005, f4b=0.005, f5=0.005, gfmu=p7["gfmu"], gflambda=p7["gflambda"], lftau=p7["lftau"],
+                       lflambda=p7["lflambda"], clen=int(cm["clen"]))
+    return ixg.search_opts(f3, env, 1e-4, cmev["lc"], cmev["li"], 2.0 * L / (2 * W), W=W, trunc=True)
 
 
 def cpu_model():
@@ -108,13 +111,13 @@ def cpu_quota():
 
 
 def search_cpu_baseline(wl, opts, threads, budget_s):
-    """The oracle's STD-pass search (C restatement of the same pipeline, not
-    infernox) over the whole workload on `threads` threads, repeated while
-    under budget_s: Mres/s."""
+    """The oracle's search (C restatement of the same pipeline, the STD and
+    the truncated passes, not infernox) over the whole workload on `threads`
+    threads, repeated while under budget_s: Mres/s."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle_pipe_bind import OracleSearch
     p7, cm, cp9, _, dsq, L, _ = wl
-    o = OracleSearch(p7, cm, cp9, int(cm["W"]))
+    o = OracleSearch(p7, cm, cp9, int(cm["W"]), trcp9=synth_cp9.build_trunc_cp9s(cm, 1))
     runs, res, t0 = 0, 0, time.time()
     hits = None
     while runs == 0 or time.time() - t0 < budget_s / 2:
@@ -126,29 +129,34 @@ def search_cpu_baseline(wl, opts, threads, budget_s):
     return dict(value=res / dt / 1e6, unit="Mres/s", cores=threads, kind="port", nproc=nproc, affinity=aff,
                 cgroup_cpus=q, cpu_model=cpu_model(),
                 sample=f"the whole 1 Mb x 2 strands workload, {runs} run(s), oracle ixo_search_std_mt (C "
-                       f"restatement of the STD-pass pipeline, not infernox), {threads} threads, {dt:.1f} s; "
+                       f"restatement of the default pipeline, STD and truncated passes, not infernox), "
+                       f"{threads} threads, {dt:.1f} s; "
                        f"{len(hits)} hits"), hits
 
 
 def time_search(ctx, steps, warmup, wl):
-    """The headline step: ixg_search_std over the whole target. Returns the
-    line's fields and the last step's hits."""
+    """The headline step: ixg_search over the whole target (the STD and the
+    truncated passes). Returns the line's fields and the last step's hits."""
     p7, cm, cp9, cmev, dsq, L, planted = wl
     W = int(cm["W"])
     opts = search_opts(p7, cm, cmev, L)
     g7 = ixg.P7(ctx, p7, W)
     gcm = ixg.CM(ctx, cm)
+    gcm.set_trpenalties(*ixg.cm_trpenalties(cm)[:2])
     g9 = ixg.CP9(ctx, cp9)
     hb = ixg.HbMap(cm, cp9)
+    trd = synth_cp9.build_trunc_cp9s(cm, 1)
+    tr9 = [ixg.CP9(ctx, h) for h in trd]
+    trhb = [ixg.HbMap(cm, h) for h in trd]
     tgt = ixg.Target(ctx, dsq, L)
     for _ in range(warmup):
-        ixg.search_std(ctx, g7, gcm, g9, hb, tgt, opts)
+        ixg.search(ctx, g7, gcm, g9, hb, tr9, trhb, tgt, opts)
     ctx.kernel_timings(reset=True)
     st = dict.fromkeys(ixg.SEARCH_STAGES, 0.0)
     cyk_cells = ins_cells = 0
     t0 = time.perf_counter()
     for _ in range(steps):
-        hits, acct = ixg.search_std(ctx, g7, gcm, g9, hb, tgt, opts)
+        hits, acct = ixg.search(ctx, g7, gcm, g9, hb, tr9, trhb, tgt, opts)
         ms, cells = ixg.search_stats(ctx)
         for k in st:
             st[k] += ms[k] / steps
@@ -194,11 +202,12 @@ def time_search(ctx, steps, warmup, wl):
                          "f4b": int(acct.e.n_past_gfwdbias), "f5_envelopes": int(acct.e.n_past_edef),
                          "f6": int(acct.n_past_cyk), "hits": int(acct.n_hits)},
            "planted_found": f"{found} of {len(planted)}"}
-    tgt.free()
-    g9.free()
-    gcm.free()
-    g7.free()
-    hb.free()
+    out["survivors"]["truncated_passes"] = {
+        "residues": [int(x) for x in acct.tr_nres], "f3b": [int(acct.tr_f[q].n_past_fwdbias) for q in range(3)],
+        "f5_envelopes": [int(acct.tr_e[q].n_past_edef) for q in range(3)],
+        "f6": [int(x) for x in acct.tr_n_past_cyk], "hits": [int(x) for x in acct.tr_hits]}
+    for x in [tgt, g9, gcm, g7, hb] + tr9 + trhb:
+        x.free()
     return out, opts, hits
 
 
@@ -1007,11 +1016,13 @@ def main():
                                "7-9; alignment 40 seqs, seed 1), 1 Mb iid ACGU target (seed 2) + 20 planted "
                                "CM samples, both strands, tier thresholds F1 0.35, F3=F3b=F4=F4b=F5 0.005, F6 "
                                "1e-4",
-                   "timed_step": "one ixg_search_std call: STD pass F1 SSV, F3 Forward, F3b bias, F4/F4b glocal "
-                                 "Forward, glocal Backward + domain definition, F5, CP9 bands (tau iteration), "
-                                 "F6 HB CYK + envelope redefinition, CP9 bands at final tau, F7 HB Inside + "
-                                 "null3 + greedy hits, coordinates, E-values, overlap removal, sort; not the "
-                                 "truncated-hit passes, not alignment",
+                   "timed_step": "one ixg_search call, cmsearch's default pipeline: STD pass F1 SSV, F3 Forward, "
+                                 "F3b bias, F4/F4b glocal Forward, glocal Backward + domain definition, F5, CP9 "
+                                 "bands (tau iteration), F6 HB CYK + envelope redefinition, CP9 bands at final "
+                                 "tau, F7 HB Inside + null3 + greedy hits; the terminal truncated passes (5' and "
+                                 "3' on the maxW residues at each end, both strands: filter cascade, "
+                                 "Rgm/Lgm envelope definition, truncated CP9 bands, TrCYK filter, FTrInside); "
+                                 "coordinates, E-values, overlap removal, sort; not alignment",
                    "cm_states": int(cm["M"]), "cm_W": int(cm["W"]), "p7_M": int(p7["M"]),
                    "residues": sr["residues"]},
         "cyk_gcups": sr["cyk_gcups"],
