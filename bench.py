@@ -175,8 +175,10 @@ def time_search(ctx, steps, warmup, wl):
     # adds/max); HB CYK 8 per cell (the mean of the cloverleaf's 2-3
     # children x add+max, emission, clamp); HB Inside 24 per cell (the same
     # with FLogsum for max)
+    # the F4 launch holds every F3b window's glocal Forward and Backward
+    # (gfb); the F5 launch rescores the domains (gfwd)
     ops = {"ssv_sweep": 4 * f3cells[0], "fwd": 16 * f3cells[1],
-           "gfwd": 20 * acct.f.pos_past_fwdbias * M7, "gbwd": 20 * acct.e.pos_past_gfwdbias * M7,
+           "gfb": 2 * 20 * acct.f.pos_past_fwdbias * M7, "gfwd": 20 * acct.e.pos_past_edef * M7,
            "hb_scan": 8 * cyk_cells + 24 * ins_cells}
     kern = {k: {"ms_per_step": v[0] / steps, "launches_per_step": v[1] / steps,
                 "ms_per_launch": v[0] / max(1, v[1])} for k, v in kt.items() if v[1] > 0}

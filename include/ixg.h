@@ -596,7 +596,8 @@ int ixg_search_stats(ixg_ctx *ctx, double *ms, int64_t *cells);
  * read when the timings are queried (or a pool of 256 launches fills). */
 enum { IXG_K_SSV_SWEEP = 0, IXG_K_SSV_RESOLVE = 1, IXG_K_FWD = 2, IXG_K_BIAS = 3, IXG_K_VIT = 4, IXG_K_MSV = 5,
        IXG_K_HB_SCAN = 6, IXG_K_TRHB_SCAN = 7, IXG_K_COPY = 8, IXG_K_QDB_SCAN = 9, IXG_K_FWD_FULL = 10,
-       IXG_K_TRQDB_SCAN = 11, IXG_K_CP9 = 12, IXG_K_GFWD = 13, IXG_K_GBWD = 14, IXG_NKERNELS = 15 };
+       IXG_K_TRQDB_SCAN = 11, IXG_K_CP9 = 12, IXG_K_GFWD = 13, IXG_K_GBWD = 14, IXG_K_GFB = 15,
+       IXG_NKERNELS = 16 };
 int ixg_kernel_timings(ixg_ctx *ctx, double *ms, int64_t *launches, int reset);
 /* Measurement hook (no counterpart in the reference): copies `bytes`
  * device-to-device with a plain kernel reps times after one warm-up copy;

@@ -1157,3 +1157,78 @@ int ixg_last_timings(const ixg_ctx *c, double *ms5, int64_t *cells5) {
 }
 
 }  // extern "C"
+
+// ---- host worker pool of par_for ----
+#include <condition_variable>
+#include <deque>
+#include <mutex>
+namespace ixg {
+namespace {
+struct PoolJob {
+  int64_t n;
+  void (*body)(void *, int64_t);
+  void *arg;
+  std::atomic<int64_t> next{0}, done{0};
+  int users = 0;  // pool workers inside work() on this job (guarded by the pool mutex)
+};
+struct Pool {
+  std::mutex mu;
+  std::condition_variable cv, fin;
+  std::deque<PoolJob *> jobs;  // jobs that may have chunks left to claim
+  Pool() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    const int nt = (int)std::min<unsigned>(16u, hw ? hw : 1u) - 1;
+    for (int t = 0; t < nt; t++) std::thread([this] { loop(); }).detach();  // live as long as the process
+  }
+  static void work(PoolJob *j) {
+    for (;;) {
+      const int64_t k0 = j->next.fetch_add(4);
+      if (k0 >= j->n) break;
+      const int64_t k1 = std::min<int64_t>(j->n, k0 + 4);
+      for (int64_t k = k0; k < k1; k++) j->body(j->arg, k);
+      j->done.fetch_add(k1 - k0);
+    }
+  }
+  void loop() {
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      cv.wait(lk, [&] { return !jobs.empty(); });
+      PoolJob *j = jobs.front();
+      if (j->next.load() >= j->n) {  // fully claimed: retire it from the queue
+        jobs.pop_front();
+        continue;
+      }
+      j->users++;  // the owner does not return while a worker holds the job
+      lk.unlock();
+      work(j);
+      lk.lock();
+      j->users--;
+      fin.notify_all();
+    }
+  }
+};
+Pool &pool() {
+  static Pool *p = new Pool();  // never destroyed: workers may outlive static destruction
+  return *p;
+}
+}  // namespace
+
+void pool_run(int64_t n, void (*body)(void *, int64_t), void *arg) {
+  Pool &p = pool();
+  PoolJob j;
+  j.n = n, j.body = body, j.arg = arg;
+  {
+    std::lock_guard<std::mutex> lk(p.mu);
+    p.jobs.push_back(&j);
+  }
+  p.cv.notify_all();
+  Pool::work(&j);
+  std::unique_lock<std::mutex> lk(p.mu);
+  p.fin.wait(lk, [&] { return j.done.load() == j.n && j.users == 0; });
+  for (auto it = p.jobs.begin(); it != p.jobs.end(); ++it)  // still queued when no worker retired it
+    if (*it == &j) {
+      p.jobs.erase(it);
+      break;
+    }
+}
+}  // namespace ixg

@@ -84,30 +84,18 @@ bool p7_has_device(const ixg_p7 *p);
 int noalloc_setup(int device);
 int pinned(ixg_ctx *c, int slot, size_t n, void **out);
 
-// Runs f(k) for k in [0, n) on up to 16 host threads, in chunks of 4 items
-// (serially when n is small). f must not touch g_err; return codes are the
-// caller's business.
+// Runs f(k) for k in [0, n) on the calling thread and up to 15 pooled host
+// workers, in chunks of 4 items (serially when n is small). f must not touch
+// g_err; return codes are the caller's business. Calls from several threads
+// at once share the pool.
+void pool_run(int64_t n, void (*body)(void *, int64_t), void *arg);
 template <class F>
 void par_for(int64_t n, F &&f) {
-  const unsigned hw = std::thread::hardware_concurrency();
-  int nt = (int)std::min<unsigned>(16u, hw ? hw : 1u);
-  if (n < 16) nt = 1;
-  if (nt <= 1) {
+  if (n < 16 || std::thread::hardware_concurrency() <= 1) {
     for (int64_t k = 0; k < n; k++) f(k);
     return;
   }
-  std::atomic<int64_t> next(0);
-  auto work = [&]() {
-    for (;;) {
-      const int64_t k0 = next.fetch_add(4);
-      if (k0 >= n) break;
-      for (int64_t k = k0; k < std::min<int64_t>(n, k0 + 4); k++) f(k);
-    }
-  };
-  std::vector<std::thread> th;
-  for (int t = 1; t < nt; t++) th.emplace_back(work);
-  work();
-  for (auto &t : th) t.join();
+  pool_run(n, [](void *a, int64_t k) { (*static_cast<F *>(a))(k); }, &f);
 }
 
 }  // namespace ixg

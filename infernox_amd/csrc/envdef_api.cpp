@@ -23,11 +23,17 @@ namespace ixg {
 
 namespace {
 
-// Uploads the items, runs the glocal Forward (or Backward) and returns the
-// scores, the special rows of every item ([(len+1)][E N J B C], concatenated)
-// and, for Forward items with flag bit 1, their matrices.
+// Uploads the items, runs the glocal Forward (or, for every item with
+// backward set, else for the items flagged bit 4, the Backward) and returns
+// the scores, the special rows of every item ([(len+1)][E N J B C],
+// concatenated) and, for Forward items with flag bit 1, their matrices.
 int run_gfb(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, std::vector<GfbItem> &items, bool backward,
             std::vector<float> &sc, std::vector<float> &xall, std::vector<float> *mats) {
+  bool mixed = false;
+  for (auto &it : items) {
+    if (backward) it.flags |= 16;
+    mixed |= ((it.flags ^ items[0].flags) & 16) != 0;
+  }
   const int M = p7_gm_host(p).M;
   int64_t xtot = 0, mtot = 0;
   for (auto &it : items) {
@@ -49,8 +55,8 @@ int run_gfb(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, std::vector<GfbIt
   RCHK(scratch(c, S_GFB_SC, (size_t)n * 4, (void **)&d_sc));
   HIPCHK(hipMemcpyAsync(d_items, items.data(), items.size() * sizeof(GfbItem), hipMemcpyHostToDevice, c->stream));
   HIPCHK(ktime_begin(c));
-  HIPCHK(launch_gfb(p7_gm_dev(p), d_seq, d_items, n, d_x, d_m, d_sc, backward, c->stream));
-  HIPCHK(ktime_end(c, backward ? IXG_K_GBWD : IXG_K_GFWD));
+  HIPCHK(launch_gfb(p7_gm_dev(p), d_seq, d_items, n, d_x, d_m, d_sc, c->stream));
+  HIPCHK(ktime_end(c, mixed ? IXG_K_GFB : ((items[0].flags & 16) ? IXG_K_GBWD : IXG_K_GFWD)));
   HIPCHK(hipMemcpyAsync(sc.data(), d_sc, (size_t)n * 4, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipMemcpyAsync(xall.data(), d_x, (size_t)xtot * 4, hipMemcpyDeviceToHost, c->stream));
   if (mats && mtot) HIPCHK(hipMemcpyAsync(mats->data(), d_m, (size_t)mtot * 4, hipMemcpyDeviceToHost, c->stream));
@@ -404,9 +410,18 @@ int env_def_wave(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::v
     pp[t] = window_profile(p, W(t).pass, wlen(t), dirty[todo[t]]);
     items[t] = make_item(woff(t), wlen(t), pp[t].g, 0, pp[t].prof);
   }
-  // F4: the Forward of every window
-  std::vector<float> fsc, fx;
-  RCHK(run_gfb(c, p, d_seq, items, false, fsc, fx, nullptr));
+  // F4: the Forward of every window, and in the same launch its Backward
+  // (needed only past the F4 / F4b gates; computing it for every window
+  // takes it off the critical path of the stage)
+  std::vector<GfbItem> both(items);
+  for (size_t t = 0; t < nw; t++) {
+    both.push_back(items[t]);
+    both.back().flags |= 16;
+  }
+  std::vector<float> fbsc, fbx;
+  RCHK(run_gfb(c, p, d_seq, both, false, fbsc, fbx, nullptr));
+  std::vector<float> &fsc = fbsc, &fx = fbx;
+  for (size_t t = 0; t < nw; t++) items[t].xoff = both[t].xoff;
   std::vector<double> nullsc(nw);
   std::vector<size_t> s4;
   for (size_t t = 0; t < nw; t++) {
@@ -441,11 +456,10 @@ int env_def_wave(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::v
     s4b.push_back(t);
   }
   if (s4b.empty()) return IXG_OK;
-  // the Backward of the survivors (same configuration)
+  // the survivors' Backward rows (same configuration), from the launch above
   std::vector<GfbItem> bitems;
-  for (size_t t : s4b) bitems.push_back(items[t]);
-  std::vector<float> bsc, bx;
-  RCHK(run_gfb(c, p, d_seq, bitems, true, bsc, bx, nullptr));
+  for (size_t t : s4b) bitems.push_back(both[nw + t]);
+  const std::vector<float> &bx = fbx;
   // decoding and region finding (p7_generic.rs:1765-1836); each region is a
   // single domain or goes through the trace ensemble
   struct Region {

@@ -397,64 +397,56 @@ __device__ __forceinline__ void gbwd_item(const GmDev &g, const GfbShared &S, co
   }
 }
 
-// One item per wave; an item's local-end mode (flags bit 0) selects the
-// body, uniformly over the wave.
+// One item per wave; an item's direction (flags bit 4: Backward) and
+// local-end mode (flags bit 0) select the body, uniformly over the wave, so
+// Forward and Backward items share a launch.
 template <int KPL>
-__global__ __launch_bounds__(64 * GFB_WAVES) void gfwd_kernel(GmDev g, const uint8_t *seq, const GfbItem *items,
-                                                              int32_t n, float *xout, float *mout, float *sc) {
+__global__ __launch_bounds__(64 * GFB_WAVES) void gfb_kernel(GmDev g, const uint8_t *seq, const GfbItem *items,
+                                                             int32_t n, float *xout, float *mout, float *sc) {
   extern __shared__ float smem[];
   const GfbShared S = gfb_stage(g, smem);
   const int64_t it = (int64_t)blockIdx.x * GFB_WAVES + (threadIdx.x >> 6);
   if (it >= n) return;
   const GfbItem item = items[it];
-  if (item.flags & 1) gfwd_item<KPL, true>(g, S, seq, item, it, xout, mout, sc);
-  else gfwd_item<KPL, false>(g, S, seq, item, it, xout, mout, sc);
-}
-template <int KPL>
-__global__ __launch_bounds__(64 * GFB_WAVES) void gbwd_kernel(GmDev g, const uint8_t *seq, const GfbItem *items,
-                                                              int32_t n, float *xout, float *sc) {
-  extern __shared__ float smem[];
-  const GfbShared S = gfb_stage(g, smem);
-  const int64_t it = (int64_t)blockIdx.x * GFB_WAVES + (threadIdx.x >> 6);
-  if (it >= n) return;
-  const GfbItem item = items[it];
-  if (item.flags & 1) gbwd_item<KPL, true>(g, S, seq, item, it, xout, sc);
-  else gbwd_item<KPL, false>(g, S, seq, item, it, xout, sc);
+  const bool le = item.flags & 1;
+  if (item.flags & 16) {
+    if (le) gbwd_item<KPL, true>(g, S, seq, item, it, xout, sc);
+    else gbwd_item<KPL, false>(g, S, seq, item, it, xout, sc);
+  } else {
+    if (le) gfwd_item<KPL, true>(g, S, seq, item, it, xout, mout, sc);
+    else gfwd_item<KPL, false>(g, S, seq, item, it, xout, mout, sc);
+  }
 }
 
 template <int KPL>
 hipError_t gfb_launch(const GmDev &g, const uint8_t *seq, const GfbItem *items, int64_t n, float *xout,
-                      float *mout, float *sc, bool backward, hipStream_t s) {
+                      float *mout, float *sc, hipStream_t s) {
   const size_t lds = ((size_t)GFB_TBL + 3 * (size_t)(g.m + 1) + (size_t)GFB_WAVES * 3 * (g.m + 2)) * sizeof(float);
   if (lds > 160 * 1024) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)((n + GFB_WAVES - 1) / GFB_WAVES);
-  if (backward)
-    hipLaunchKernelGGL((gbwd_kernel<KPL>), dim3(grid), dim3(64 * GFB_WAVES), lds, s, g, seq, items, (int32_t)n, xout,
-                       sc);
-  else
-    hipLaunchKernelGGL((gfwd_kernel<KPL>), dim3(grid), dim3(64 * GFB_WAVES), lds, s, g, seq, items, (int32_t)n, xout,
-                       mout, sc);
+  hipLaunchKernelGGL((gfb_kernel<KPL>), dim3(grid), dim3(64 * GFB_WAVES), lds, s, g, seq, items, (int32_t)n, xout,
+                     mout, sc);
   return hipGetLastError();
 }
 
 hipError_t launch_gfb_kpl(const GmDev &g, const uint8_t *seq, const GfbItem *items, int64_t n, float *xout,
-                          float *mout, float *sc, bool backward, hipStream_t s) {
+                          float *mout, float *sc, hipStream_t s) {
   if (n == 0) return hipSuccess;
   if (g.m < 2 || n > INT32_MAX) return hipErrorInvalidValue;
   const int kpl = (g.m + 63) / 64;
-  if (kpl <= 1) return gfb_launch<1>(g, seq, items, n, xout, mout, sc, backward, s);
-  if (kpl <= 2) return gfb_launch<2>(g, seq, items, n, xout, mout, sc, backward, s);
-  if (kpl <= 4) return gfb_launch<4>(g, seq, items, n, xout, mout, sc, backward, s);
-  if (kpl <= 8) return gfb_launch<8>(g, seq, items, n, xout, mout, sc, backward, s);
-  if (kpl <= 16) return gfb_launch<16>(g, seq, items, n, xout, mout, sc, backward, s);
+  if (kpl <= 1) return gfb_launch<1>(g, seq, items, n, xout, mout, sc, s);
+  if (kpl <= 2) return gfb_launch<2>(g, seq, items, n, xout, mout, sc, s);
+  if (kpl <= 4) return gfb_launch<4>(g, seq, items, n, xout, mout, sc, s);
+  if (kpl <= 8) return gfb_launch<8>(g, seq, items, n, xout, mout, sc, s);
+  if (kpl <= 16) return gfb_launch<16>(g, seq, items, n, xout, mout, sc, s);
   return hipErrorInvalidValue;
 }
 
 }  // namespace
 
 hipError_t launch_gfb(const GmDev &g, const uint8_t *seq, const GfbItem *items, int64_t n, float *xout, float *mout,
-                      float *sc, bool backward, hipStream_t s) {
-  return launch_gfb_kpl(g, seq, items, n, xout, mout, sc, backward, s);
+                      float *sc, hipStream_t s) {
+  return launch_gfb_kpl(g, seq, items, n, xout, mout, sc, s);
 }
 
 }  // namespace ixg

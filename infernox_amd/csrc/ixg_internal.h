@@ -162,7 +162,7 @@ struct GmDev {
 // residues), its special-state scores [E N J C][LOOP MOVE], flags bit 0 =
 // local ends (esc = 0), bit 1 = keep the Forward matrix, bits 2-3 = the
 // profile (0 glocal, 1 5', 2 3', 3 5'&3'; GmDev.tsc holds the four transition
-// tables one after another); the specials go to
+// tables one after another), bit 4 = a Backward item; the specials go to
 // xout + xoff [(len+1)][E N J B C], the matrix to mout + moff
 // [(len+1)][M | I | D][M+1].
 struct GfbItem {
@@ -172,7 +172,7 @@ struct GfbItem {
   int64_t xoff, moff;
 };
 hipError_t launch_gfb(const GmDev &g, const uint8_t *seq, const GfbItem *items, int64_t n, float *xout, float *mout,
-                      float *sc, bool backward, hipStream_t s);
+                      float *sc, hipStream_t s);
 
 // Kernel launchers (p7_kernels.hip).
 constexpr int SWEEP_SEG = 4096;  // rows per F1 sweep work item (multiple of 64)

@@ -66,7 +66,7 @@ EXPORTS = [
     "ixg_env_def", "ixg_hbmap_create", "ixg_hbmap_free", "ixg_hmm2ij_bands", "ixg_ij2d_bands", "ixg_search_std", "ixg_search_stats",
 ]
 KERNELS = ["ssv_sweep", "ssv_resolve", "fwd", "bias", "vit", "msv", "hb_scan", "trhb_scan", "copy_probe",
-           "qdb_scan", "fwd_full", "trqdb_scan", "cp9", "gfwd", "gbwd"]
+           "qdb_scan", "fwd_full", "trqdb_scan", "cp9", "gfwd", "gbwd", "gfb"]
 
 
 class CmParams(C.Structure):
