@@ -168,6 +168,12 @@ typedef struct {
   int32_t ctx;     /* residues of maxW context shared with the previous chunk */
 } ixg_task;
 int ixg_target_create(ixg_ctx *ctx, const uint8_t *dsq, int64_t L, ixg_target **out);
+/* A target of nseq sequences (a database: cmsearch over several sequences,
+ * cm_search.rs:1774-1808 window_tasks over `fulls`), each held with its
+ * reverse complement; the batched scanners' envelopes and windows name the
+ * sequence in their `seq` field. dsq[s] is 1-indexed with L[s] residues. */
+int ixg_target_create_multi(ixg_ctx *ctx, const uint8_t *const *dsq, const int64_t *L, int64_t nseq,
+                            ixg_target **out);
 int ixg_target_free(ixg_ctx *ctx, ixg_target *t);
 /* Builds the reference's task list for one sequence of length L
  * (cm_search.rs:1774-1808, CM_MAX_RESIDUE_COUNT = 100000). Returns the count;
@@ -247,7 +253,8 @@ int ixg_inside_scan_hb(ixg_ctx *ctx, const ixg_cm *cm, const ixg_bands *b, const
  * by envelope in order). cells (may be NULL) receives the number of banded
  * cells filled. */
 typedef struct {
-  int32_t i0, j0, strand, pad;
+  int32_t i0, j0, strand;
+  int32_t seq;  /* sequence of a multi-sequence target (ixg_target_create_multi), else 0 */
   ixg_bands bands;
 } ixg_envelope;
 int ixg_hb_scan_batch(ixg_ctx *ctx, const ixg_cm *cm, const ixg_target *tgt, const ixg_envelope *env, int64_t n,
@@ -318,7 +325,7 @@ int ixg_trinside_scan_hb(ixg_ctx *ctx, const ixg_cm *cm, const ixg_bands *b, con
  * inside selects FTrInside; per-envelope vsc/vmode/envi/envj; hits of every
  * envelope appended in envelope order (env = k). */
 typedef struct {
-  int32_t i0, j0, strand, pad;
+  int32_t i0, j0, strand, seq;  /* seq as in ixg_envelope */
   ixg_bands bands;
   ixg_trvalid valid;
 } ixg_trenvelope;
@@ -340,7 +347,7 @@ int ixg_qdb_scan(ixg_ctx *ctx, const ixg_cm *cm, const uint8_t *dsq, int64_t L, 
 /* Batched form: n windows of a device-resident target (strand 1 = reverse
  * complement coordinates); hits tagged with env = window index, grouped by
  * window in order (the per-envelope loop of final_stage_inside_opt). */
-typedef struct { int32_t i0, j0, strand, pad; } ixg_qdbwin;
+typedef struct { int32_t i0, j0, strand, seq; } ixg_qdbwin;  /* seq as in ixg_envelope */
 int ixg_qdb_scan_batch(ixg_ctx *ctx, const ixg_cm *cm, const ixg_target *tgt, const ixg_qdbwin *win, int64_t n,
                        const int32_t *dmin, const int32_t *dmax, int inside, float cutoff, int do_null3,
                        ixg_hit *hits, int64_t cap, int64_t *nhits);
@@ -427,7 +434,7 @@ int ixg_cp9_bands(ixg_ctx *ctx, const ixg_cp9 *h, const uint8_t *dsq, int64_t L,
 /* Batched form over a device-resident target (the per-envelope loop that
  * calls cp9_seq2bands, cm_search.rs:2740 / :2816): pn [n][6][M+1];
  * fwd_sc / bwd_sc [n] may be NULL. */
-typedef struct { int32_t i0, j0, strand, pad; } ixg_cp9win;
+typedef struct { int32_t i0, j0, strand, seq; } ixg_cp9win;  /* seq as in ixg_envelope */
 int ixg_cp9_bands_batch(ixg_ctx *ctx, const ixg_cp9 *h, const ixg_target *tgt, const ixg_cp9win *win, int64_t n,
                         double tau, int mode, float *fwd_sc, float *bwd_sc, int32_t *pn);
 
@@ -554,6 +561,7 @@ typedef struct {
   float score, bias;
   double evalue;
   int32_t strand, pass; /* pass: IXG_PASS_* that found the hit */
+  int64_t seq;          /* the target's sequence (0 for a one-sequence target) */
 } ixg_searchhit;
 typedef struct {
   uint64_t nres;        /* residues searched: both strands, overlap context excluded */
@@ -576,7 +584,12 @@ int ixg_search_std(ixg_ctx *ctx, const ixg_p7 *p7, const ixg_cm *cm, const ixg_c
  * trmap[k] (k = 0 5', 1 3', 2 5'&3') are the pass's CP9 (Rcp9 / Lcp9 /
  * Tcp9 of cm_search.rs:953-960, local ends as the search's configuration)
  * and its band map; the CM's truncation penalties must be set
- * (ixg_cm_set_trpenalties). ixg_search_std is this with trunc_mode 0. */
+ * (ixg_cm_set_trpenalties). ixg_search_std is this with trunc_mode 0.
+ * Over a multi-sequence target (ixg_target_create_multi) it is the
+ * reference's search over several sequences (cm_search.rs:1300): one flat
+ * task list, overlap removal per sequence and strand, the E-value sort
+ * keyed (E, score desc, sequence, start, pass desc); hits name their
+ * sequence. o->ez is the caller's, from the whole database's size. */
 int ixg_search(ixg_ctx *ctx, const ixg_p7 *p7, const ixg_cm *cm, const ixg_cp9 *cp9, const ixg_hbmap *map,
                const ixg_cp9 *const *trcp9, const ixg_hbmap *const *trmap, const ixg_target *tgt,
                const ixg_searchopts *o, ixg_searchhit *out, int64_t cap, int64_t *n, ixg_searchacct *acct);

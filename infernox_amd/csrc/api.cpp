@@ -179,11 +179,14 @@ FwdDev p7_fwd_dev(const ixg_p7 *p) {
 }
 }  // namespace ixg
 
+// One sequence, or several back to back: per sequence s at soff[s] its
+// [fwd (L_s+2) | revcomp (L_s+2)], sentinel-padded; L is sequence 0's length.
 struct ixg_target {
-  uint8_t *d_both = nullptr;  // [fwd (L+2) | revcomp (L+2)], sentinel-padded
+  uint8_t *d_both = nullptr;
   int64_t L = 0;
   size_t bytes = 0;
-  std::vector<uint8_t> h_both;  // host copy of the same two strands
+  std::vector<uint8_t> h_both;  // host copy of the same buffer
+  std::vector<int64_t> soff, sL;
 };
 
 const uint8_t *ixg_target_device_base(const ixg_target *t, int64_t *L) {
@@ -193,6 +196,16 @@ const uint8_t *ixg_target_device_base(const ixg_target *t, int64_t *L) {
 const uint8_t *ixg_target_host_dsq(const ixg_target *t, int strand) {
   return t->h_both.data() + (strand ? t->L + 2 : 0);
 }
+// Strand `strand` of sequence `seq`: its offset in the device / host buffer
+// and its length; false when seq is not one of the target's sequences.
+bool ixg_target_view(const ixg_target *t, int64_t seq, int strand, int64_t *off, int64_t *L, const uint8_t **host) {
+  if (seq < 0 || seq >= (int64_t)t->soff.size()) return false;
+  *L = t->sL[seq];
+  *off = t->soff[seq] + (strand ? t->sL[seq] + 2 : 0);
+  if (host) *host = t->h_both.data() + *off;
+  return true;
+}
+int64_t ixg_target_nseq(const ixg_target *t) { return (int64_t)t->soff.size(); }
 
 
 extern "C" {
@@ -382,6 +395,8 @@ int ixg_target_create(ixg_ctx *c, const uint8_t *dsq, int64_t L, ixg_target **ou
   HIPCHK(hipSetDevice(c->device));
   auto t = std::make_unique<ixg_target>();
   t->L = L;
+  t->soff = {0};
+  t->sL = {L};
   t->bytes = (size_t)2 * (L + 2);
   RCHK(dmalloc(c, (void **)&t->d_both, t->bytes));
   t->h_both.assign(2 * (L + 2), 255);
@@ -393,6 +408,43 @@ int ixg_target_create(ixg_ctx *c, const uint8_t *dsq, int64_t L, ixg_target **ou
   }
   HIPCHK(hipMemcpyAsync(t->d_both, h, L + 2, hipMemcpyHostToDevice, c->stream));
   HIPCHK(launch_revcomp(t->d_both, t->d_both + L + 2, L, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  *out = t.release();
+  return IXG_OK;
+}
+
+int ixg_target_create_multi(ixg_ctx *c, const uint8_t *const *dsq, const int64_t *L, int64_t nseq,
+                            ixg_target **out) {
+  if (!c || !dsq || !L || nseq < 1 || !out) {
+    g_err = "ixg_target_create_multi: bad arguments";
+    return IXG_EINVAL;
+  }
+  auto t = std::make_unique<ixg_target>();
+  size_t tot = 0;
+  for (int64_t s = 0; s < nseq; s++) {
+    if (!dsq[s] || L[s] < 0 || !codes_ok_pub(dsq[s] + 1, L[s])) {
+      g_err = "ixg_target_create_multi: bad sequence";
+      return IXG_EINVAL;
+    }
+    t->soff.push_back((int64_t)tot);
+    t->sL.push_back(L[s]);
+    tot += (size_t)2 * (L[s] + 2);
+  }
+  t->L = L[0];
+  t->bytes = tot;
+  t->h_both.assign(tot, 255);
+  par_for(nseq, [&](int64_t s) {
+    uint8_t *h = t->h_both.data() + t->soff[s];
+    const int64_t n = L[s];
+    if (n) memcpy(h + 1, dsq[s] + 1, n);
+    for (int64_t j = 1; j <= n; j++) {  // reverse complement (cm_search.rs:1822-1827)
+      const uint8_t d = dsq[s][n + 1 - j];
+      h[n + 2 + j] = d < 4 ? (uint8_t)(3 - d) : d;
+    }
+  });
+  HIPCHK(hipSetDevice(c->device));
+  RCHK(dmalloc(c, (void **)&t->d_both, t->bytes));
+  HIPCHK(hipMemcpyAsync(t->d_both, t->h_both.data(), t->bytes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   *out = t.release();
   return IXG_OK;
@@ -1107,6 +1159,30 @@ int ixg_f3_filter_tasks(ixg_ctx *c, const ixg_p7 *p, const ixg_target *tgt, cons
   c->ms[4] = total.ms();
   return emit(res, out, cap, n);
 }
+
+}  // extern "C"
+
+namespace ixg {
+// F1-F3b of many sequence views of a target (HostTask bases in its device
+// buffer, any of its sequences and strands); windows carry their view's index
+int f3_filter_views(ixg_ctx *c, const ixg_p7 *p, const ixg_target *tgt, const std::vector<HostTask> &tasks,
+                    const ixg_f3opts &o, std::vector<ixg_window> &wins, std::vector<ixg_acct> &accts) {
+  HIPCHK(hipSetDevice(c->device));
+  for (const HostTask &t : tasks)
+    if (t.base < 0 || t.L < 0 || (size_t)(t.base + t.L + 2) > tgt->bytes) {
+      g_err = "f3_filter_views: a view outside the target";
+      return IXG_EINVAL;
+    }
+  std::vector<std::vector<Win>> res;
+  RCHK(cascade(c, p, tgt->d_both, tasks, o, res, accts));
+  int64_t n = 0;
+  for (auto &r : res) n += (int64_t)r.size();
+  wins.resize(n);
+  return emit(res, wins.data(), n, &n);
+}
+}  // namespace ixg
+
+extern "C" {
 
 int ixg_copy_probe(ixg_ctx *c, size_t bytes, int reps, double *ms_per_copy) {
   if (!c || bytes < 16 || reps < 1 || !ms_per_copy) return IXG_EINVAL;

@@ -822,7 +822,7 @@ static int single_env(ixg_ctx *c, const ixg_cm *cm, const ixg_bands *b, const ui
   e.i0 = i0;
   e.j0 = j0;
   e.strand = 0;
-  e.pad = 0;
+  e.seq = 0;
   e.bands = *b;
   std::vector<std::vector<float>> roots;
   RCHK(run_envelopes(c, cm, d, {0}, &e, 1, inside, nullptr, nullptr, roots, nullptr));
@@ -864,15 +864,18 @@ static int hb_batch(ixg_ctx *c, const ixg_cm *cm, const std::vector<const ixg_cm
                     int do_null3, float *vsc, int64_t *envi, int64_t *envj, ixg_hit *hits, int64_t cap,
                     int64_t *nhits, int64_t *cells) {
   HIPCHK(hipSetDevice(c->device));
-  int64_t L = 0;
-  const uint8_t *base = ixg_target_device_base(tgt, &L);
+  int64_t L0 = 0;
+  const uint8_t *base = ixg_target_device_base(tgt, &L0);
   std::vector<int64_t> off(n);
+  std::vector<const uint8_t *> host(n);
   for (int64_t k = 0; k < n; k++) {
-    if (env[k].i0 < 1 || env[k].j0 > L || env[k].j0 < env[k].i0 || (env[k].strand != 0 && env[k].strand != 1)) {
+    int64_t L = 0;
+    if ((env[k].strand != 0 && env[k].strand != 1) ||
+        !ixg_target_view(tgt, env[k].seq, env[k].strand, &off[k], &L, &host[k]) || env[k].i0 < 1 ||
+        env[k].j0 > L || env[k].j0 < env[k].i0) {
       g_err = "ixg_hb_scan_batch: envelope outside the target";
       return IXG_EINVAL;
     }
-    off[k] = env[k].strand ? L + 2 : 0;
   }
   c->ms[2] = 0;
   std::vector<std::vector<float>> roots;
@@ -881,8 +884,8 @@ static int hb_batch(ixg_ctx *c, const ixg_cm *cm, const std::vector<const ixg_cm
   par_for(n, [&](int64_t k) {
     root_summary(env[k].bands, roots[k].data(), env[k].i0, env[k].j0, env_cutoff, vsc[k], envi[k], envj[k]);
     if (report)
-      report_hits(percm ? *(*percm)[k] : *cm, env[k].bands, ixg_target_host_dsq(tgt, env[k].strand), env[k].i0, env[k].j0,
-                  roots[k].data(), cutoff, do_null3, (int32_t)k, per[k]);
+      report_hits(percm ? *(*percm)[k] : *cm, env[k].bands, host[k], env[k].i0, env[k].j0, roots[k].data(), cutoff,
+                  do_null3, (int32_t)k, per[k]);
   });
   std::vector<ixg_hit> out;
   for (auto &h : per) out.insert(out.end(), h.begin(), h.end());
@@ -1003,7 +1006,7 @@ static int tr_single(ixg_ctx *c, const ixg_cm *cm, const ixg_bands *b, const ixg
   e.i0 = i0;
   e.j0 = j0;
   e.strand = 0;
-  e.pad = 0;
+  e.seq = 0;
   e.bands = *b;
   std::vector<std::vector<float>> roots;
   RCHK(run_envelopes(c, cm, d, {0}, &e, 1, inside, &sel, valid, roots, nullptr));
@@ -1047,21 +1050,24 @@ int ixg_trhb_scan_batch(ixg_ctx *c, const ixg_cm *cm, const ixg_target *tgt, con
   TrSel sel;
   RCHK(tr_select(cm, pass_idx, local, sel));
   HIPCHK(hipSetDevice(c->device));
-  int64_t L = 0;
-  const uint8_t *base = ixg_target_device_base(tgt, &L);
+  int64_t L0 = 0;
+  const uint8_t *base = ixg_target_device_base(tgt, &L0);
   std::vector<int64_t> off(n);
+  std::vector<const uint8_t *> host(n);
   std::vector<ixg_envelope> ev(n);
   std::vector<ixg_trvalid> val(n);
   for (int64_t k = 0; k < n; k++) {
-    if (env[k].i0 < 1 || env[k].j0 > L || env[k].j0 < env[k].i0 || (env[k].strand != 0 && env[k].strand != 1)) {
+    int64_t L = 0;
+    if ((env[k].strand != 0 && env[k].strand != 1) ||
+        !ixg_target_view(tgt, env[k].seq, env[k].strand, &off[k], &L, &host[k]) || env[k].i0 < 1 ||
+        env[k].j0 > L || env[k].j0 < env[k].i0) {
       g_err = "ixg_trhb_scan_batch: envelope outside the target";
       return IXG_EINVAL;
     }
-    off[k] = env[k].strand ? L + 2 : 0;
     ev[k].i0 = env[k].i0;
     ev[k].j0 = env[k].j0;
     ev[k].strand = env[k].strand;
-    ev[k].pad = 0;
+    ev[k].seq = env[k].seq;
     ev[k].bands = env[k].bands;
     val[k] = env[k].valid;
   }
@@ -1073,7 +1079,7 @@ int ixg_trhb_scan_batch(ixg_ctx *c, const ixg_cm *cm, const ixg_target *tgt, con
     const int64_t nroot = (int64_t)roots[k].size() / 5;
     tr_root_summary(ev[k].bands, roots[k].data(), nroot, mode_mask(val[k], 0, sel.fill), ev[k].i0, ev[k].j0,
                     env_cutoff, vsc[k], vmode[k], envi[k], envj[k]);
-    tr_report(*cm, ev[k].bands, ixg_target_host_dsq(tgt, ev[k].strand), ev[k].i0, ev[k].j0, roots[k].data(), nroot,
+    tr_report(*cm, ev[k].bands, host[k], ev[k].i0, ev[k].j0, roots[k].data(), nroot,
               pass_idx, local, cutoff, do_null3, (int32_t)k, per[k]);
   });
   std::vector<ixg_trhit> out;

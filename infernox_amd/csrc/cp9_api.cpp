@@ -342,15 +342,17 @@ int ixg_cp9_bands_batch(ixg_ctx *c, const ixg_cp9 *h, const ixg_target *tgt, con
     g_err = "ixg_cp9_bands_batch: bad arguments";
     return IXG_EINVAL;
   }
-  int64_t L;
-  const uint8_t *base = ixg_target_device_base(tgt, &L);
+  int64_t L0;
+  const uint8_t *base = ixg_target_device_base(tgt, &L0);
   std::vector<CWin> w(n);
   for (int64_t k = 0; k < n; k++) {
-    if (win[k].i0 < 1 || win[k].j0 > L || win[k].j0 < win[k].i0 || (win[k].strand != 0 && win[k].strand != 1)) {
+    int64_t off = 0, L = 0;
+    if ((win[k].strand != 0 && win[k].strand != 1) || !ixg_target_view(tgt, win[k].seq, win[k].strand, &off, &L, nullptr) ||
+        win[k].i0 < 1 || win[k].j0 > L || win[k].j0 < win[k].i0) {
       g_err = "ixg_cp9_bands_batch: window outside the target";
       return IXG_EINVAL;
     }
-    w[k] = CWin{base + (win[k].strand ? L + 2 : 0), win[k].i0, win[k].j0};
+    w[k] = CWin{base + off, win[k].i0, win[k].j0};
   }
   return cp9_run(c, h, w, tau, mode, fwd_sc, bwd_sc, pn, nullptr);
 }

@@ -84,6 +84,10 @@ bool p7_has_device(const ixg_p7 *p);
 int noalloc_setup(int device);
 int pinned(ixg_ctx *c, int slot, size_t n, void **out);
 
+}  // namespace ixg
+bool ixg_target_view(const ixg_target *t, int64_t seq, int strand, int64_t *off, int64_t *L, const uint8_t **host);
+int64_t ixg_target_nseq(const ixg_target *t);
+namespace ixg {
 // Runs f(k) for k in [0, n) on the calling thread and up to 15 pooled host
 // workers, in chunks of 4 items (serially when n is small). f must not touch
 // g_err; return codes are the caller's business. Calls from several threads

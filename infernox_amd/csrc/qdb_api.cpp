@@ -837,15 +837,18 @@ int ixg_qdb_scan_batch(ixg_ctx *c, const ixg_cm *cm, const ixg_target *tgt, cons
     return IXG_EINVAL;
   }
   HIPCHK(hipSetDevice(c->device));
-  int64_t L = 0;
-  const uint8_t *base = ixg_target_device_base(tgt, &L);
+  int64_t L0 = 0;
+  const uint8_t *base = ixg_target_device_base(tgt, &L0);
   std::vector<QWin> w(n);
   for (int64_t k = 0; k < n; k++) {
-    if (win[k].i0 < 1 || win[k].j0 > L || win[k].j0 < win[k].i0 || (win[k].strand != 0 && win[k].strand != 1)) {
+    int64_t off = 0, L = 0;
+    const uint8_t *host = nullptr;
+    if ((win[k].strand != 0 && win[k].strand != 1) || !ixg_target_view(tgt, win[k].seq, win[k].strand, &off, &L, &host) ||
+        win[k].i0 < 1 || win[k].j0 > L || win[k].j0 < win[k].i0) {
       g_err = "ixg_qdb_scan_batch: window outside the target";
       return IXG_EINVAL;
     }
-    w[k] = QWin{win[k].i0, win[k].j0, win[k].strand ? L + 2 : 0, ixg_target_host_dsq(tgt, win[k].strand)};
+    w[k] = QWin{win[k].i0, win[k].j0, off, host};
   }
   std::vector<ixg_hit> out;
   RCHK(qdb_run(c, cm, base, w, dmin, dmax, inside ? 1 : 0, cutoff, do_null3, out));
@@ -907,15 +910,18 @@ int ixg_trqdb_scan_batch(ixg_ctx *c, const ixg_cm *cm, const ixg_target *tgt, co
     return IXG_EINVAL;
   }
   HIPCHK(hipSetDevice(c->device));
-  int64_t L = 0;
-  const uint8_t *base = ixg_target_device_base(tgt, &L);
+  int64_t L0 = 0;
+  const uint8_t *base = ixg_target_device_base(tgt, &L0);
   std::vector<QWin> w(n);
   for (int64_t k = 0; k < n; k++) {
-    if (win[k].i0 < 1 || win[k].j0 > L || win[k].j0 < win[k].i0 || (win[k].strand != 0 && win[k].strand != 1)) {
+    int64_t off = 0, L = 0;
+    const uint8_t *host = nullptr;
+    if ((win[k].strand != 0 && win[k].strand != 1) || !ixg_target_view(tgt, win[k].seq, win[k].strand, &off, &L, &host) ||
+        win[k].i0 < 1 || win[k].j0 > L || win[k].j0 < win[k].i0) {
       g_err = "ixg_trqdb_scan_batch: window outside the target";
       return IXG_EINVAL;
     }
-    w[k] = QWin{win[k].i0, win[k].j0, win[k].strand ? L + 2 : 0, ixg_target_host_dsq(tgt, win[k].strand)};
+    w[k] = QWin{win[k].i0, win[k].j0, off, host};
   }
   std::vector<ixg_trhit> out;
   std::vector<TrqSummary> sm;

@@ -1,4 +1,4 @@
-// A cmsearch over one resident sequence, F1 through F7, as
+// A cmsearch over the resident sequences of a target, F1 through F7, as
 // FaithfulSearcher::search runs it for every (chunk x strand) task
 // (cm_search.rs:1300, :1774 window_tasks, :1813 run_window_task, :2214
 // trunc_pipeline_one_strand, :2707 cm_std_stage, :2863 cm_trunc_stage, :1915
@@ -48,9 +48,21 @@ float hb_mx_mb(const ixg_hbmap &m, const int32_t *jmin, const int32_t *jmax, con
                const int32_t *hdmin, const int32_t *hdmax, int L, int64_t *ncells);
 int hbmap_M(const ixg_hbmap *m);
 int hbmap_hmm_m(const ixg_hbmap *m);
+int f3_filter_views(ixg_ctx *c, const ixg_p7 *p, const ixg_target *tgt, const std::vector<HostTask> &tasks,
+                    const ixg_f3opts &o, std::vector<ixg_window> &wins, std::vector<ixg_acct> &accts);
 }  // namespace ixg
 
 const uint8_t *ixg_target_device_base(const ixg_target *t, int64_t *L);
+
+namespace {
+// the device sequence an envelope's coordinates refer to: its sequence's
+// strand buffer
+const uint8_t *env_dsq(const ixg_target *tgt, const uint8_t *dbase, const ixg_envelope &e) {
+  int64_t off = 0, L = 0;
+  ixg_target_view(tgt, e.seq, e.strand, &off, &L, nullptr);
+  return dbase + off;
+}
+}  // namespace
 
 namespace {
 
@@ -66,8 +78,8 @@ struct EnvBands {
 // maxtau, after which the bands stand). skip[k] set when the final size is
 // still over the limit (the caller then drops the envelope, as
 // cm_std_stage does).
-int iterate_bands(ixg_ctx *c, const ixg_cp9 *cp9, const ixg_hbmap *map, const uint8_t *dbase, int64_t L,
-                  const std::vector<ixg_envelope> &env, double tau0, double maxtau, float limit,
+int iterate_bands(ixg_ctx *c, const ixg_cp9 *cp9, const ixg_hbmap *map, const ixg_target *tgt,
+                  const uint8_t *dbase, const std::vector<ixg_envelope> &env, double tau0, double maxtau, float limit,
                   std::vector<EnvBands> &out, std::vector<char> &skip) {
   const size_t n = env.size();
   const int M = hbmap_M(map), H = cp9_nodes(cp9);
@@ -91,7 +103,7 @@ int iterate_bands(ixg_ctx *c, const ixg_cp9 *cp9, const ixg_hbmap *map, const ui
       for (size_t k : todo)
         if (tau[k] == t) {
           grp.push_back(k);
-          dsq.push_back(dbase + (env[k].strand ? L + 2 : 0));
+          dsq.push_back(env_dsq(tgt, dbase, env[k]));
           i0.push_back(env[k].i0);
           j0.push_back(env[k].j0);
         }
@@ -138,8 +150,8 @@ struct TrEnvBands {
   std::vector<uint8_t> jv, lv, rv, tv;
   ixg_trvalid valid() const { return ixg_trvalid{jv.data(), lv.data(), rv.data(), tv.data()}; }
 };
-int trunc_bands(ixg_ctx *c, const ixg_cp9 *cp9, const ixg_hbmap *map, const uint8_t *dbase, int64_t L,
-                const std::vector<ixg_envelope> &env, int pass, double tau, std::vector<TrEnvBands> &out) {
+int trunc_bands(ixg_ctx *c, const ixg_cp9 *cp9, const ixg_hbmap *map, const ixg_target *tgt,
+                const uint8_t *dbase, const std::vector<ixg_envelope> &env, int pass, double tau, std::vector<TrEnvBands> &out) {
   const size_t n = env.size();
   const int M = hbmap_M(map), H = cp9_nodes(cp9);
   out.assign(n, {});
@@ -150,7 +162,7 @@ int trunc_bands(ixg_ctx *c, const ixg_cp9 *cp9, const ixg_hbmap *map, const uint
   std::vector<const uint8_t *> dsq(n);
   std::vector<int32_t> i0(n), j0(n), pn, occ;
   for (size_t k = 0; k < n; k++) {
-    dsq[k] = dbase + (env[k].strand ? L + 2 : 0);
+    dsq[k] = env_dsq(tgt, dbase, env[k]);
     i0[k] = env[k].i0;
     j0[k] = env[k].j0;
   }
@@ -175,6 +187,7 @@ int trunc_bands(ixg_ctx *c, const ixg_cp9 *cp9, const ixg_hbmap *map, const uint
 
 struct Hit {
   int32_t task;
+  int64_t seq;
   int64_t start, stop;
   float score, bias;
   double evalue;
@@ -222,14 +235,25 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
     c->search_ms[k] += std::chrono::duration<double, std::milli>(t - t_mark).count();
     t_mark = t;
   };
-  int64_t L = 0;
-  const uint8_t *dbase = ixg_target_device_base(tgt, &L);
+  int64_t L0 = 0;
+  const uint8_t *dbase = ixg_target_device_base(tgt, &L0);
+  const int64_t nseq = ixg_target_nseq(tgt);
   ixg_searchacct A{};
   const int64_t maxw = o->f3.maxw;
-  // (chunk x strand) tasks (cm_search.rs:1774)
-  const int64_t nt = ixg_window_tasks(L, maxw, o->do_top, o->do_bot, nullptr, 0);
-  std::vector<ixg_task> tasks(nt);
-  ixg_window_tasks(L, maxw, o->do_top, o->do_bot, tasks.data(), nt);
+  // the flat (sequence x chunk x strand) task list (cm_search.rs:1774-1808),
+  // chunk coordinates in their sequence; seq_len[s] its length
+  std::vector<ixg_task> tasks;
+  std::vector<int64_t> tseq, seq_len(nseq);
+  for (int64_t s = 0; s < nseq; s++) {
+    int64_t off = 0;
+    ixg_target_view(tgt, s, 0, &off, &seq_len[s], nullptr);
+    const int64_t k = ixg_window_tasks(seq_len[s], maxw, o->do_top, o->do_bot, nullptr, 0);
+    const size_t at = tasks.size();
+    tasks.resize(at + k);
+    ixg_window_tasks(seq_len[s], maxw, o->do_top, o->do_bot, tasks.data() + at, k);
+    tseq.resize(at + k, s);
+  }
+  const int64_t nt = (int64_t)tasks.size();
   // the truncated passes a task runs (cm_search.rs:2265-2280) and the
   // sequence each searches (:2380-2395): the chunk when it is at most maxW
   // long (or for 5'&3'), else its first (5') / last (3') maxW residues
@@ -238,10 +262,11 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
     int64_t start_offset;
   };
   std::vector<ixg_task> ftasks(tasks);  // chunk tasks, then the truncated passes' subsequences
+  std::vector<int64_t> fseq(tseq);
   std::vector<TrRun> truns;
   for (int64_t t = 0; t < nt && do_trunc; t++) {
     const ixg_task &k = tasks[t];
-    const int64_t wl = k.end - k.start + 1;
+    const int64_t wl = k.end - k.start + 1, L = seq_len[tseq[t]];
     const bool h5 = k.strand ? k.end == L : k.start == 1, h3 = k.strand ? k.start == 1 : k.end == L;
     const bool run[3] = {h5, h3, h5 && h3 && wl <= maxw};
     for (int q = 0; q < 3; q++) {
@@ -261,28 +286,30 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
       else sk.start = k.end - maxw + 1, sk.end = k.end;
       truns.push_back({(int32_t)t, pass, (int32_t)ftasks.size(), front ? 0 : wl - maxw});
       ftasks.push_back(sk);
+      fseq.push_back(tseq[t]);
     }
   }
+  // view_off[t]: the offset of task t's residues in its sequence's strand
+  // buffer (top strand: dsq[1] = seq[start]; bottom: dsq[k] =
+  // comp(seq[end-k+1]) = rc[L-end+k]); ht[t].base adds the strand buffer's
+  // offset in the target
   const int64_t nft = (int64_t)ftasks.size();
   std::vector<HostTask> ht(nft);
+  std::vector<int64_t> view_off(nft);
   for (int64_t t = 0; t < nft; t++) {
     const ixg_task &k = ftasks[t];
-    ht[t].base = k.strand == 0 ? (k.start - 1) : (L + 2) + (L - k.end);
+    int64_t soff = 0, L = 0;
+    ixg_target_view(tgt, fseq[t], k.strand, &soff, &L, nullptr);
+    view_off[t] = k.strand == 0 ? k.start - 1 : L - k.end;
+    ht[t].base = soff + view_off[t];
     ht[t].L = k.end - k.start + 1;
     if (t < nt) A.nres += (uint64_t)(ht[t].L - k.ctx);  // STD pass: new residues (cm_search.rs:2306-2311)
   }
   // F1-F3b over every chunk and truncated-pass subsequence in one call
-  std::vector<ixg_acct> fa(nft);
-  int64_t nw = 0;
-  std::vector<ixg_window> wins(4096);
-  int rc = ixg_f3_filter_tasks(c, p, tgt, ftasks.data(), nft, &o->f3, wins.data(), (int64_t)wins.size(), &nw,
-                               fa.data());
-  if (rc == IXG_ERANGE) {
-    wins.resize(nw);
-    rc = ixg_f3_filter_tasks(c, p, tgt, ftasks.data(), nft, &o->f3, wins.data(), nw, &nw, fa.data());
-  }
-  RCHK(rc);
-  wins.resize(nw);
+  std::vector<ixg_acct> fa;
+  std::vector<ixg_window> wins;
+  int rc = IXG_OK;
+  RCHK(f3_filter_views(c, p, tgt, ht, o->f3, wins, fa));
   stage(IXG_SS_F1_F3B);
   auto add_acct = [](const void *src, void *dst, size_t bytes) {
     const uint64_t *s = (const uint64_t *)src;
@@ -312,21 +339,18 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
   for (size_t w = 0; w < ew.size(); w++)
     add_acct(&ea[w], ew_run[w] < 0 ? &A.e : &A.tr_e[truns[ew_run[w]].pass - IXG_PASS_5P_ONLY_FORCE],
              sizeof(ixg_envacct));
-  // envelopes in the coordinates of their sequence view's strand buffer
-  auto view_off = [&](int32_t t) {
-    const ixg_task &k = ftasks[t];
-    return k.strand == 0 ? k.start - 1 : L - k.end;
-  };
+  // envelopes in the coordinates of their sequence's strand buffer
   std::vector<ixg_envelope> env;
   std::vector<int32_t> etask;
   std::vector<std::vector<ixg_envelope>> tenv(truns.size());
   for (size_t w = 0; w < ew.size(); w++) {
-    const int64_t off = view_off(ew[w].task);
+    const int64_t off = view_off[ew[w].task];
     for (auto &e : envs[w]) {
       ixg_envelope x{};
       x.i0 = (int32_t)(off + e.first);
       x.j0 = (int32_t)(off + e.second);
       x.strand = ftasks[ew[w].task].strand;
+      x.seq = (int32_t)fseq[ew[w].task];
       if (ew_run[w] < 0) {
         env.push_back(x);
         etask.push_back(ew[w].task);
@@ -345,7 +369,7 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
   } else if (!env.empty()) {
     std::vector<EnvBands> eb;
     std::vector<char> skip;
-    RCHK(iterate_bands(c, cp9, map, dbase, L, env, o->fcyk_tau, o->maxtau, o->size_limit, eb, skip));
+    RCHK(iterate_bands(c, cp9, map, tgt, dbase, env, o->fcyk_tau, o->maxtau, o->size_limit, eb, skip));
     stage(IXG_SS_F6_BANDS);
     std::vector<ixg_envelope> run;
     std::vector<size_t> ridx;
@@ -378,7 +402,7 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
   if (!surv.empty()) {
     std::vector<EnvBands> eb;
     std::vector<char> skip;
-    RCHK(iterate_bands(c, cp9, map, dbase, L, surv, o->final_tau, o->maxtau, o->size_limit, eb, skip));
+    RCHK(iterate_bands(c, cp9, map, tgt, dbase, surv, o->final_tau, o->maxtau, o->size_limit, eb, skip));
     stage(IXG_SS_F7_BANDS);
     std::vector<ixg_envelope> run;
     std::vector<size_t> ridx;
@@ -429,11 +453,12 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
           if (!gone[b] && !(v[b].j < v[a].i) && !(v[a].j < v[b].i)) gone[b] = 1;
       }
       const int32_t t = stask[ridx[r]];
-      const int64_t off = view_off(t);
+      const int64_t off = view_off[t];
       for (size_t a = 0; a < v.size(); a++) {
         if (gone[a]) continue;
         Hit h;
         h.task = t;
+        h.seq = tseq[t];
         h.pass = IXG_PASS_STD_ANY;
         genome_coords(tasks[t], v[a].i - off, v[a].j - off, h);
         h.score = v[a].score;
@@ -463,10 +488,10 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
       ps = pe, psrun = prun;
     } else {
       std::vector<TrEnvBands> tb;
-      RCHK(trunc_bands(c, trcp9[q], trmap[q], dbase, L, pe, pass, o->fcyk_tau, tb));
+      RCHK(trunc_bands(c, trcp9[q], trmap[q], tgt, dbase, pe, pass, o->fcyk_tau, tb));
       std::vector<ixg_trenvelope> te(pe.size());
       for (size_t k = 0; k < pe.size(); k++) {
-        te[k].i0 = pe[k].i0, te[k].j0 = pe[k].j0, te[k].strand = pe[k].strand, te[k].pad = 0;
+        te[k].i0 = pe[k].i0, te[k].j0 = pe[k].j0, te[k].strand = pe[k].strand, te[k].seq = pe[k].seq;
         te[k].bands = tb[k].b.view();
         te[k].valid = tb[k].valid();
       }
@@ -493,10 +518,10 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
     }
     if (ps.empty()) continue;
     std::vector<TrEnvBands> tb;
-    RCHK(trunc_bands(c, trcp9[q], trmap[q], dbase, L, ps, pass, o->final_tau, tb));
+    RCHK(trunc_bands(c, trcp9[q], trmap[q], tgt, dbase, ps, pass, o->final_tau, tb));
     std::vector<ixg_trenvelope> te(ps.size());
     for (size_t k = 0; k < ps.size(); k++) {
-      te[k].i0 = ps[k].i0, te[k].j0 = ps[k].j0, te[k].strand = ps[k].strand, te[k].pad = 0;
+      te[k].i0 = ps[k].i0, te[k].j0 = ps[k].j0, te[k].strand = ps[k].strand, te[k].seq = ps[k].seq;
       te[k].bands = tb[k].b.view();
       te[k].valid = tb[k].valid();
     }
@@ -518,9 +543,10 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
     th.resize(nh);
     for (const ixg_trhit &x : th) {
       const TrRun &R = truns[psrun[x.env]];
-      const int64_t off = view_off(R.task);  // the chunk's offset in its strand buffer
+      const int64_t off = view_off[R.task];  // the chunk's offset in its strand buffer
       Hit h;
       h.task = R.task;
+      h.seq = tseq[R.task];
       h.pass = pass;
       genome_coords(tasks[R.task], x.i - off, x.j - off, h);
       h.score = x.score;
@@ -530,13 +556,14 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
       A.tr_hits[q]++;
     }
   }
-  // finalize (cm_search.rs:1915): global overlap removal per strand, then the
-  // E-value sort and the reporting threshold
+  // finalize (cm_search.rs:1915): global overlap removal per sequence and
+  // strand (:3260), then the E-value sort and the reporting threshold
   {
     std::vector<size_t> ord(hits.size());
     for (size_t q = 0; q < ord.size(); q++) ord[q] = q;
     std::sort(ord.begin(), ord.end(), [&](size_t a, size_t b) {
       const Hit &x = hits[a], &y = hits[b];
+      if (x.seq != y.seq) return x.seq < y.seq;
       if (x.strand != y.strand) return x.strand < y.strand;
       if (x.score != y.score) return x.score > y.score;
       if (x.start != y.start) return x.start < y.start;
@@ -547,7 +574,7 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
       if (ha.removed) continue;
       for (size_t b = a + 1; b < ord.size(); b++) {
         Hit &hb = hits[ord[b]];
-        if (hb.strand != ha.strand) break;
+        if (hb.seq != ha.seq || hb.strand != ha.strand) break;
         if (hb.removed) continue;
         const bool ov = ha.strand == 0 ? (!(hb.stop < ha.start) && !(ha.stop < hb.start))
                                        : (!(hb.start < ha.stop) && !(ha.start < hb.stop));
@@ -561,6 +588,7 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
   std::sort(fin.begin(), fin.end(), [](const Hit &a, const Hit &b) {
     if (a.evalue != b.evalue) return a.evalue < b.evalue;
     if (a.score != b.score) return a.score > b.score;
+    if (a.seq != b.seq) return a.seq < b.seq;
     if (a.start != b.start) return a.start < b.start;
     if (a.pass != b.pass) return a.pass > b.pass;
     return a.stop < b.stop;
@@ -577,6 +605,7 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
       x.evalue = h.evalue;
       x.strand = h.strand;
       x.pass = h.pass;
+      x.seq = h.seq;
     }
     n++;
   }

@@ -86,11 +86,11 @@ class Hit(C.Structure):
 
 
 class Envelope(C.Structure):
-    _fields_ = [("i0", C.c_int32), ("j0", C.c_int32), ("strand", C.c_int32), ("pad", C.c_int32), ("bands", Bands)]
+    _fields_ = [("i0", C.c_int32), ("j0", C.c_int32), ("strand", C.c_int32), ("seq", C.c_int32), ("bands", Bands)]
 
 
 class QdbWin(C.Structure):
-    _fields_ = [("i0", C.c_int32), ("j0", C.c_int32), ("strand", C.c_int32), ("pad", C.c_int32)]
+    _fields_ = [("i0", C.c_int32), ("j0", C.c_int32), ("strand", C.c_int32), ("seq", C.c_int32)]
 
 
 class TrValid(C.Structure):
@@ -103,7 +103,7 @@ class TrHit(C.Structure):
 
 
 class TrEnvelope(C.Structure):
-    _fields_ = [("i0", C.c_int32), ("j0", C.c_int32), ("strand", C.c_int32), ("pad", C.c_int32), ("bands", Bands),
+    _fields_ = [("i0", C.c_int32), ("j0", C.c_int32), ("strand", C.c_int32), ("seq", C.c_int32), ("bands", Bands),
                 ("valid", TrValid)]
 
 _lib = None
@@ -426,6 +426,25 @@ class Target:
             self.h = C.c_void_p()
 
 
+class MultiTarget(Target):
+    """Several device-resident digitized sequences (a database), each with its
+    reverse complement; seqs is a list of 1-indexed dsq arrays."""
+
+    def __init__(self, ctx, seqs):
+        self.ctx = ctx
+        self.dsqs = [np.ascontiguousarray(d, np.uint8) for d in seqs]
+        self.lens = [len(d) - 2 for d in self.dsqs]
+        self.L = self.lens[0]
+        n = len(self.dsqs)
+        ptrs = (C.POINTER(C.c_uint8) * n)(*[u8ptr(d) for d in self.dsqs])
+        lens = (C.c_int64 * n)(*self.lens)
+        self.h = C.c_void_p()
+        L = lib()
+        L.ixg_target_create_multi.argtypes = [C.c_void_p, C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_int64),
+                                              C.c_int64, C.POINTER(C.c_void_p)]
+        check(L.ixg_target_create_multi(ctx.h, ptrs, lens, n, C.byref(self.h)), "ixg_target_create_multi")
+
+
 def window_tasks(L, maxw, do_top=True, do_bot=True):
     n = lib().ixg_window_tasks(int(L), int(maxw), int(do_top), int(do_bot), None, 0)
     arr = (Task * max(n, 1))()
@@ -706,7 +725,7 @@ class EnvBatch:
         for k, e in enumerate(envs):
             i0, j0, strand, b = e[:4]
             a = self.arr[k]
-            a.i0, a.j0, a.strand, a.pad = int(i0), int(j0), int(strand), 0
+            a.i0, a.j0, a.strand, a.seq = int(i0), int(j0), int(strand), 0
             bs = bands_struct(b)  # holds the arrays its pointers refer to
             a.bands = bs
             self.keep.append(bs)
@@ -785,7 +804,7 @@ def trqdb_scan_batch(ctx, cm, tgt, wins, pass_idx, local=True, inside=False, dma
     n = len(wins)
     arr = (QdbWin * max(n, 1))()
     for k, (i0, j0, strand) in enumerate(wins):
-        arr[k].i0, arr[k].j0, arr[k].strand, arr[k].pad = int(i0), int(j0), int(strand), 0
+        arr[k].i0, arr[k].j0, arr[k].strand, arr[k].seq = int(i0), int(j0), int(strand), 0
     i32p = C.POINTER(C.c_int32)
     dm = np.ascontiguousarray(dmax, np.int32) if dmax is not None else None
     vsc, vm, bf = np.zeros(max(n, 1), np.float32), np.zeros(max(n, 1), np.int32), np.zeros(max(n, 1), np.float32)
@@ -804,7 +823,7 @@ def qdb_scan_batch(ctx, cm, tgt, wins, inside=False, dmin=None, dmax=None, cutof
     n = len(wins)
     arr = (QdbWin * max(n, 1))()
     for k, (i0, j0, strand) in enumerate(wins):
-        arr[k].i0, arr[k].j0, arr[k].strand, arr[k].pad = int(i0), int(j0), int(strand), 0
+        arr[k].i0, arr[k].j0, arr[k].strand, arr[k].seq = int(i0), int(j0), int(strand), 0
     i32p = C.POINTER(C.c_int32)
     keep = [np.ascontiguousarray(x, np.int32) if x is not None else None for x in (dmin, dmax)]
     ptr = [k.ctypes.data_as(i32p) if k is not None else None for k in keep]
@@ -826,7 +845,7 @@ class Cp9Params(C.Structure):
 
 
 class Cp9Win(C.Structure):
-    _fields_ = [("i0", C.c_int32), ("j0", C.c_int32), ("strand", C.c_int32), ("pad", C.c_int32)]
+    _fields_ = [("i0", C.c_int32), ("j0", C.c_int32), ("strand", C.c_int32), ("seq", C.c_int32)]
 
 
 def _i32p(a):
@@ -906,7 +925,7 @@ def cp9_bands_batch(ctx, cp9, tgt, wins, tau, scan=True, mode=None):
     n, M1 = len(wins), cp9.M + 1
     arr = (Cp9Win * max(n, 1))()
     for k, (i0, j0, strand) in enumerate(wins):
-        arr[k].i0, arr[k].j0, arr[k].strand, arr[k].pad = int(i0), int(j0), int(strand), 0
+        arr[k].i0, arr[k].j0, arr[k].strand, arr[k].seq = int(i0), int(j0), int(strand), 0
     pn = np.zeros(max(n, 1) * 6 * M1, np.int32)
     fs, bs = np.zeros(max(n, 1), np.float32), np.zeros(max(n, 1), np.float32)
     check(_cp9_lib().ixg_cp9_bands_batch(ctx.h, cp9.h, tgt.h, arr, n, float(tau), cp9_mode(scan, mode), f32ptr(fs), f32ptr(bs),
@@ -1109,7 +1128,7 @@ class SearchOpts(C.Structure):
 
 class SearchHit(C.Structure):
     _fields_ = [("start", C.c_int64), ("stop", C.c_int64), ("score", C.c_float), ("bias", C.c_float),
-                ("evalue", C.c_double), ("strand", C.c_int32), ("pass_idx", C.c_int32)]
+                ("evalue", C.c_double), ("strand", C.c_int32), ("pass_idx", C.c_int32), ("seq", C.c_int64)]
 
 
 class SearchAcct(C.Structure):
@@ -1159,7 +1178,7 @@ def _search_lib():
     return L
 
 
-def _search_call(call, name, with_pass=False):
+def _search_call(call, name, with_pass=False, with_seq=False):
     cap = 256
     while True:
         out = (SearchHit * cap)()
@@ -1171,7 +1190,8 @@ def _search_call(call, name, with_pass=False):
             continue
         check(rc, name)
         return [(out[i].start, out[i].stop, out[i].strand, out[i].score, out[i].bias, out[i].evalue) +
-                ((out[i].pass_idx,) if with_pass else ()) for i in range(n.value)], acct
+                ((out[i].pass_idx,) if with_pass else ()) + ((out[i].seq,) if with_seq else ())
+                for i in range(n.value)], acct
 
 
 def search_std(ctx, p7, cm, cp9, hbmap, tgt, opts):
@@ -1182,16 +1202,17 @@ def search_std(ctx, p7, cm, cp9, hbmap, tgt, opts):
                         "ixg_search_std")
 
 
-def search(ctx, p7, cm, cp9, hbmap, trcp9, trmap, tgt, opts):
+def search(ctx, p7, cm, cp9, hbmap, trcp9, trmap, tgt, opts, with_seq=False):
     """ixg_search with the truncated passes' CP9s and band maps (5', 3',
-    5'&3'): ([(start, stop, strand, score, bias, evalue, pass)], SearchAcct)."""
+    5'&3'): ([(start, stop, strand, score, bias, evalue, pass)], SearchAcct);
+    with_seq appends each hit's sequence index (a MultiTarget's)."""
     L = _search_lib()
     vp = C.c_void_p
     tc = (vp * 3)(*[x.h.value if hasattr(x.h, 'value') else x.h for x in trcp9])
     tm = (vp * 3)(*[x.h.value if hasattr(x.h, 'value') else x.h for x in trmap])
     return _search_call(lambda out, cap, n, acct: L.ixg_search(ctx.h, p7.h, cm.h, cp9.h, hbmap.h, tc, tm, tgt.h,
                                                                C.byref(opts), out, cap, C.byref(n), C.byref(acct)),
-                        "ixg_search", with_pass=True)
+                        "ixg_search", with_pass=True, with_seq=with_seq)
 
 
 SEARCH_STAGES = ("f1_f3b", "f4_f5", "f6_bands", "f6_cyk", "f7_bands", "f7_inside", "finalize", "total")

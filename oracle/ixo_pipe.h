@@ -55,6 +55,7 @@ typedef struct {
   float score, bias;
   double evalue;
   int32_t strand, pass;
+  int64_t seq; /* index of the target sequence */
 } ixo_pipehit;
 
 typedef struct {
@@ -75,6 +76,12 @@ int64_t ixo_search_std(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *
 /* the same with the tasks on nthreads threads (identical result) */
 int64_t ixo_search_std_mt(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *seq, int64_t L,
                           ixo_pipehit *out, int64_t cap, ixo_pipeacct *A, int nthreads);
+/* cm_search.rs:1300 search over nseq sequences seqs[s][1..L[s]]: the flat
+ * (sequence x chunk x strand) task list, one finalize over all hits
+ * (overlap removal per sequence and strand; the E-value sort's third key is
+ * the sequence index, cm_search.rs:1922-1931) */
+int64_t ixo_search_multi_mt(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *const *seqs, const int64_t *L,
+                            int64_t nseq, ixo_pipehit *out, int64_t cap, ixo_pipeacct *A, int nthreads);
 
 #ifdef __cplusplus
 }

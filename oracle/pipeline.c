@@ -21,6 +21,7 @@ typedef struct {
   float score, bias;
   double evalue;
   int strand, removed, pass;
+  int64_t seq; /* index of the target sequence */
 } phit;
 
 /* cp9.rs:4053 cp9_seq2bands + :4075 the tau iteration; fills the bands of
@@ -377,6 +378,7 @@ static void one_task(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *ds
 static const phit *g_h;
 static int ov_cmp(const void *x, const void *y) {
   const phit *a = &g_h[*(const int64_t *)x], *b = &g_h[*(const int64_t *)y];
+  if (a->seq != b->seq) return a->seq < b->seq ? -1 : 1;
   if (a->strand != b->strand) return a->strand - b->strand;
   if (a->score != b->score) return a->score > b->score ? -1 : 1;
   if (a->start != b->start) return a->start < b->start ? -1 : 1;
@@ -386,18 +388,19 @@ static int fin_cmp(const void *x, const void *y) {
   const phit *a = (const phit *)x, *b = (const phit *)y;
   if (a->evalue != b->evalue) return a->evalue < b->evalue ? -1 : 1;
   if (a->score != b->score) return a->score > b->score ? -1 : 1;
+  if (a->seq != b->seq) return a->seq < b->seq ? -1 : 1;
   if (a->start != b->start) return a->start < b->start ? -1 : 1;
   if (a->pass != b->pass) return a->pass > b->pass ? -1 : 1;
   return (a->stop > b->stop) - (a->stop < b->stop);
 }
 
 typedef struct {
-  int64_t wgs, wge, wl, ctx, L;
+  int64_t wgs, wge, wl, ctx, L, seq;
   int strand;
 } otask;
 
-/* cm_search.rs:1774-1808 window_tasks for one sequence */
-static int64_t make_tasks(const ixo_pipeopts *o, int64_t L, otask *out) {
+/* cm_search.rs:1774-1808 window_tasks for sequence `seq` of length L */
+static int64_t make_tasks(const ixo_pipeopts *o, int64_t L, int64_t seq, otask *out) {
   const int64_t maxw = o->f3.maxw;
   int64_t n = 0, new_start = 1;
   int first = 1;
@@ -408,7 +411,7 @@ static int64_t make_tasks(const ixo_pipeopts *o, int64_t L, otask *out) {
     const int64_t wl = ctx + cnt, wge = wgs + wl - 1;
     for (int strand = 0; strand < 2; strand++) {
       if ((strand == 0 && !o->do_top) || (strand == 1 && !o->do_bot)) continue;
-      if (out) out[n] = (otask){wgs, wge, wl, ctx, L, strand};
+      if (out) out[n] = (otask){wgs, wge, wl, ctx, L, seq, strand};
       n++;
     }
     new_start = wge + 1;
@@ -417,8 +420,10 @@ static int64_t make_tasks(const ixo_pipeopts *o, int64_t L, otask *out) {
   return n;
 }
 
-static void run_task(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *seq, const otask *t, uint8_t *dsq,
-                     ixo_pipeacct *A, phit **hits, int64_t *nh, int64_t *hcap) {
+static void run_task(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *const *seqs, const otask *t,
+                     uint8_t *dsq, ixo_pipeacct *A, phit **hits, int64_t *nh, int64_t *hcap) {
+  const uint8_t *seq = seqs[t->seq];
+  const int64_t nh0 = *nh;
   dsq[0] = dsq[t->wl + 1] = 255;
   for (int64_t k = 1; k <= t->wl; k++) {
     const uint8_t x = t->strand ? seq[t->wge - (k - 1)] : seq[t->wgs + k - 1];
@@ -427,6 +432,7 @@ static void run_task(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *se
   A->nres += (uint64_t)(t->wl - t->ctx);
   const int h5 = t->strand ? t->wge == t->L : t->wgs == 1, h3 = t->strand ? t->wgs == 1 : t->wge == t->L;
   one_task(P, o, dsq, t->wl, t->wgs, t->wge, t->strand, h5, h3, A, hits, nh, hcap);
+  for (int64_t k = nh0; k < *nh; k++) (*hits)[k].seq = t->seq;
 }
 
 /* finalize (cm_search.rs:1915): global overlap removal, E-value sort,
@@ -441,7 +447,7 @@ static int64_t finalize(const ixo_pipeopts *o, phit *hits, int64_t nh, ixo_pipeh
     if (ha->removed) continue;
     for (int64_t b = a + 1; b < nh; b++) {
       phit *hb = &hits[ord[b]];
-      if (hb->strand != ha->strand) break;
+      if (hb->seq != ha->seq || hb->strand != ha->strand) break;
       if (hb->removed) continue;
       const int ov = ha->strand == 0 ? (!(hb->stop < ha->start) && !(ha->stop < hb->start))
                                      : (!(hb->start < ha->stop) && !(ha->start < hb->stop));
@@ -464,6 +470,7 @@ static int64_t finalize(const ixo_pipeopts *o, phit *hits, int64_t nh, ixo_pipeh
       out[n].evalue = hits[q].evalue;
       out[n].strand = hits[q].strand;
       out[n].pass = hits[q].pass;
+      out[n].seq = hits[q].seq;
     }
     n++;
   }
@@ -472,7 +479,12 @@ static int64_t finalize(const ixo_pipeopts *o, phit *hits, int64_t nh, ixo_pipeh
 
 int64_t ixo_search_std(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *seq, int64_t L, ixo_pipehit *out,
                        int64_t cap, ixo_pipeacct *A) {
-  return ixo_search_std_mt(P, o, seq, L, out, cap, A, 1);
+  return ixo_search_multi_mt(P, o, &seq, &L, 1, out, cap, A, 1);
+}
+
+int64_t ixo_search_std_mt(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *seq, int64_t L,
+                          ixo_pipehit *out, int64_t cap, ixo_pipeacct *A, int nthreads) {
+  return ixo_search_multi_mt(P, o, &seq, &L, 1, out, cap, A, nthreads);
 }
 
 /* the tasks run on nthreads threads (the reference's par_iter over tasks,
@@ -481,7 +493,7 @@ int64_t ixo_search_std(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *
 typedef struct {
   const ixo_pipe *P;
   const ixo_pipeopts *o;
-  const uint8_t *seq;
+  const uint8_t *const *seqs;
   const otask *tasks;
   int64_t nt;
   int64_t *next;
@@ -499,28 +511,30 @@ static void *worker(void *arg) {
     const int64_t t = (*w->next)++;
     pthread_mutex_unlock(w->mu);
     if (t >= w->nt) break;
-    run_task(w->P, w->o, w->seq, &w->tasks[t], dsq, &w->A, &w->hits, &w->nh, &w->hcap);
+    run_task(w->P, w->o, w->seqs, &w->tasks[t], dsq, &w->A, &w->hits, &w->nh, &w->hcap);
   }
   free(dsq);
   return NULL;
 }
 
-int64_t ixo_search_std_mt(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *seq, int64_t L,
-                          ixo_pipehit *out, int64_t cap, ixo_pipeacct *A, int nthreads) {
+int64_t ixo_search_multi_mt(const ixo_pipe *P, const ixo_pipeopts *o, const uint8_t *const *seqs, const int64_t *L,
+                            int64_t nseq, ixo_pipehit *out, int64_t cap, ixo_pipeacct *A, int nthreads) {
   memset(A, 0, sizeof(*A));
   (void)ixo_flogsum(0.f, 0.f); /* the lazily built tables, before any thread reads them */
   (void)ixo_ilogsum(0, 0);
   (void)ixo_p7_flogsum(0.f, 0.f);
-  const int64_t nt = make_tasks(o, L, NULL);
+  /* the flat task list of every sequence in order (cm_search.rs:1787) */
+  int64_t nt = 0;
+  for (int64_t s = 0; s < nseq; s++) nt += make_tasks(o, L[s], s, NULL);
   otask *tasks = (otask *)malloc(sizeof(otask) * (size_t)(nt + 1));
-  make_tasks(o, L, tasks);
+  for (int64_t s = 0, k = 0; s < nseq; s++) k += make_tasks(o, L[s], s, tasks + k);
   if (nthreads < 1) nthreads = 1;
   worker_t *w = (worker_t *)calloc((size_t)nthreads, sizeof(worker_t));
   pthread_t *th = (pthread_t *)malloc(sizeof(pthread_t) * (size_t)nthreads);
   pthread_mutex_t mu = PTHREAD_MUTEX_INITIALIZER;
   int64_t next = 0;
   for (int q = 0; q < nthreads; q++) {
-    w[q].P = P, w[q].o = o, w[q].seq = seq, w[q].tasks = tasks, w[q].nt = nt, w[q].next = &next, w[q].mu = &mu;
+    w[q].P = P, w[q].o = o, w[q].seqs = seqs, w[q].tasks = tasks, w[q].nt = nt, w[q].next = &next, w[q].mu = &mu;
     if (q) pthread_create(&th[q], NULL, worker, &w[q]);
   }
   worker(&w[0]);

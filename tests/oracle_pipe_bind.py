@@ -65,6 +65,10 @@ class OracleSearch:
         L.ixo_search_std.restype = C.c_int64
         L.ixo_search_std_mt.argtypes = L.ixo_search_std.argtypes + [C.c_int]
         L.ixo_search_std_mt.restype = C.c_int64
+        L.ixo_search_multi_mt.argtypes = [C.POINTER(Pipe), C.POINTER(ixg.SearchOpts),
+                                          C.POINTER(C.POINTER(C.c_uint8)), C.POINTER(C.c_int64), C.c_int64,
+                                          C.POINTER(ixg.SearchHit), C.c_int64, C.POINTER(ixg.SearchAcct), C.c_int]
+        L.ixo_search_multi_mt.restype = C.c_int64
 
     def search(self, dsq, L, opts, threads=1):
         """Hits (start, stop, strand, score, bias, evalue) and the counters;
@@ -77,3 +81,18 @@ class OracleSearch:
                                        int(L), out, cap, C.byref(acct), int(threads))
         return [(out[i].start, out[i].stop, out[i].strand, out[i].score, out[i].bias, out[i].evalue) +
                 ((out[i].pass_idx,) if opts.trunc_mode else ()) for i in range(min(n, cap))], acct
+
+    def search_multi(self, seqs, opts, threads=1):
+        """The search over several sequences (1-indexed dsq arrays): hits
+        (start, stop, strand, score, bias, evalue, pass, seq) and the counters."""
+        ds = [np.ascontiguousarray(d, np.uint8) for d in seqs]
+        n = len(ds)
+        ptrs = (C.POINTER(C.c_uint8) * n)(*[d.ctypes.data_as(C.POINTER(C.c_uint8)) for d in ds])
+        lens = (C.c_int64 * n)(*[len(d) - 2 for d in ds])
+        cap = 8192
+        out = (ixg.SearchHit * cap)()
+        acct = ixg.SearchAcct()
+        k = OB.lib().ixo_search_multi_mt(C.byref(self.P), C.byref(opts), ptrs, lens, n, out, cap, C.byref(acct),
+                                         int(threads))
+        return [(out[i].start, out[i].stop, out[i].strand, out[i].score, out[i].bias, out[i].evalue,
+                 out[i].pass_idx, out[i].seq) for i in range(min(k, cap))], acct

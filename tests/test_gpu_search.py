@@ -154,3 +154,64 @@ def test_search_truncated_passes_parity(ctx, body, seed, ends):
             assert sum(wa.tr_hits) >= 1, want  # a truncated pass reported a planted end
     for x in [tgt, g9, gcm, g7, hb] + tr9 + trhb:
         x.free()
+
+
+def _multi_set(cm, case):
+    from test_oracle_pipeline import _multi_seqs
+    if case == "small":
+        return _multi_seqs(cm, 40)
+    # a chromosome-sized sequence of three chunks among many short ones
+    big, L, _ = models.target_with_homologs(cm, 210000, 4, 9)
+    seqs = _multi_seqs(cm, 50)
+    r = np.random.default_rng(51)
+    for k in range(24):
+        n = int(np.clip(r.lognormal(np.log(600), 0.8), 20, 5000))
+        seqs.append(np.concatenate([[255], r.integers(0, 4, n).astype(np.uint8), [255]]).astype(np.uint8))
+    return seqs[:2] + [big] + seqs[2:]
+
+
+@pytest.mark.parametrize("case", ["small", "mixed"])
+def test_search_multi_sequence_parity(ctx, case):
+    # one ixg_search over a multi-sequence target against the oracle's
+    # search over the same sequences (hits with their sequence index, order,
+    # scores, E-values, every counter), and against one ixg_search per
+    # sequence at the same E-value scale (the union, in the same order)
+    from infernox_amd import synth_cp9 as P
+    from test_oracle_pipeline import _fin_key
+    p7, cm, cp9, cmev = models.config1_models()
+    trdicts = P.build_trunc_cp9s(cm, 1)
+    seqs = _multi_set(cm, case)
+    tot = sum(len(d) - 2 for d in seqs)
+    W = int(cm["W"])
+    g7 = ixg.P7(ctx, p7, W)
+    gcm = ixg.CM(ctx, cm)
+    g, l, ig, il = ixg.cm_trpenalties(cm)
+    gcm.set_trpenalties(g, l)
+    g9 = ixg.CP9(ctx, cp9)
+    hb = ixg.HbMap(cm, cp9)
+    tr9 = [ixg.CP9(ctx, h) for h in trdicts]
+    trhb = [ixg.HbMap(cm, h) for h in trdicts]
+    tgt = ixg.MultiTarget(ctx, seqs)
+    osr = OracleSearch(p7, cm, cp9, W, trcp9=trdicts)
+    o = _opts(p7, cm, cmev, tot, trunc=True)
+    o.env.lftau, o.env.lflambda, o.env.clen = p7["lftau"], p7["lflambda"], int(cm["clen"])
+    _say("search multi", case, len(seqs), tot)
+    got, ga = ixg.search(ctx, g7, gcm, g9, hb, tr9, trhb, tgt, o, with_seq=True)
+    want, wa = osr.search_multi(seqs, o, threads=8)
+    assert [h[:3] + h[6:] for h in got] == [h[:3] + h[6:] for h in want], (got, want)
+    assert np.array_equal(np.float32([h[3] for h in got]), np.float32([h[3] for h in want]))
+    assert np.array_equal(np.float32([h[4] for h in got]), np.float32([h[4] for h in want]))
+    assert [h[5] for h in got] == [h[5] for h in want]
+    _same_tr_acct(ga, wa)
+    assert len({h[7] for h in got}) >= 3, got
+    union = []
+    for s, d in enumerate(seqs):
+        if len(d) == 2:
+            continue  # an empty sequence has no tasks
+        one = ixg.Target(ctx, d, len(d) - 2)
+        hs, _ = ixg.search(ctx, g7, gcm, g9, hb, tr9, trhb, one, o)
+        union += [h + (s,) for h in hs]
+        one.free()
+    assert [h[:3] + h[6:] for h in got] == [h[:3] + h[6:] for h in sorted(union, key=_fin_key)]
+    for x in [tgt, g9, gcm, g7, hb] + tr9 + trhb:
+        x.free()

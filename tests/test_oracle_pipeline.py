@@ -84,3 +84,53 @@ def test_oracle_truncated_passes():
     Ls = len(part)
     hs, as_ = osr.search(short, Ls, o)
     assert list(as_.tr_nres) == [2 * Ls] * 3
+
+
+def _multi_seqs(cm, seed):
+    """Sequences of uneven length (one empty): homologs cut at ends, a full
+    homolog inside, random filler."""
+    from infernox_amd import synth_cm as S
+    r = np.random.default_rng(seed)
+    h1, h2, h3 = S.emit(cm, seed + 1), S.emit(cm, seed + 2), S.emit(cm, seed + 3)
+    parts = [
+        np.concatenate([h1[len(h1) // 3:], r.integers(0, 4, 900)]),
+        r.integers(0, 4, 0),
+        np.concatenate([r.integers(0, 4, 400), h2, r.integers(0, 4, 500)]),
+        np.concatenate([r.integers(0, 4, 1200), h3[:2 * len(h3) // 3]]),
+        r.integers(0, 4, 60),
+    ]
+    return [np.concatenate([[255], p.astype(np.uint8), [255]]).astype(np.uint8) for p in parts]
+
+
+def _fin_key(h):
+    # hit_sorter_by_evalue (cm_search.rs:1922-1931): E, score desc, sequence,
+    # start, pass desc, stop
+    return (h[5], -h[3], h[7], h[0], -h[6], h[1])
+
+
+def test_oracle_multi_sequence_search():
+    # a search over several sequences is the union of the per-sequence
+    # searches at the same E-value scale (the task list is flat and overlap
+    # removal runs per sequence and strand), in the five-key E-value order;
+    # counters add up; the threaded run is identical
+    from infernox_amd import synth_cp9 as P
+    p7, cm, cp9, cmev = models.config1_models()
+    W = int(cm["W"])
+    osr = OracleSearch(p7, cm, cp9, W, trcp9=P.build_trunc_cp9s(cm, 1))
+    seqs = _multi_seqs(cm, 40)
+    tot = sum(len(d) - 2 for d in seqs)
+    o = _opts(p7, cm, cmev, tot)
+    o.env.lftau, o.env.lflambda, o.env.clen = p7["lftau"], p7["lflambda"], int(cm["clen"])
+    o.trunc_mode = 1
+    hits, a = osr.search_multi(seqs, o)
+    union, nres, tr_nres = [], 0, [0, 0, 0]
+    for s, d in enumerate(seqs):
+        hs, as_ = osr.search(d, len(d) - 2, o)
+        union += [h + (s,) for h in hs]
+        nres += as_.nres
+        tr_nres = [x + y for x, y in zip(tr_nres, as_.tr_nres)]
+    assert hits == sorted(union, key=_fin_key)
+    assert a.nres == nres and list(a.tr_nres) == tr_nres
+    assert {h[7] for h in hits} >= {0, 2, 3}, hits  # the planted homologs are found in their sequences
+    h4, a4 = osr.search_multi(seqs, o, threads=4)
+    assert h4 == hits and bytes(a4) == bytes(a)
