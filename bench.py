@@ -922,6 +922,114 @@ def time_cascade(ctx, steps, warmup, cpu_s, cpu_threads):
     return out
 
 
+def batched_workload(n=10000):
+    """The batched uneven workload: n iid sequences at 45% GC with log-normal
+    lengths (median 2 kb, sigma 1.0, clipped to 200 nt - 200 kb, seed 4), 100
+    CM-sampled homologs inserted into random sequences on random strands, the
+    120-column three-hairpin CM."""
+    p7, cm, cp9, cmev = models.config3_models()
+    seqs, placed = models.lognormal_database(n, 2000.0, 1.0, 200, 200000, 0.45, 4, cm=cm, n_hom=100)
+    return p7, cm, cp9, cmev, seqs, placed
+
+
+def batched_opts(p7, cm, cmev, tot):
+    """The 20-200 Mb tier (cm_search.rs:1526-1528): F1 0.35, Viterbi F2 and
+    its bias filter F2b on at 0.15, F3=F3b=F4=F4b=F5 0.003, F6 1e-4."""
+    W = int(cm["W"])
+    f3 = ixg.F3Opts(f1=0.35, f1b=1.0, f2=0.15, f2b=0.15, f3=0.003, f3b=0.003, cmw=W, maxw=W,
+                    opts=ixg.DO_MSV | ixg.DO_VIT | ixg.DO_VITBIAS | ixg.DO_FWD | ixg.DO_FWDBIAS)
+    env = ixg.env_opts(f4=0.003, f4b=0.003, f5=0.003, gfmu=p7["gfmu"], gflambda=p7["gflambda"], lftau=p7["lftau"],
+                       lflambda=p7["lflambda"], clen=int(cm["clen"]))
+    return ixg.search_opts(f3, env, 1e-4, cmev["lc"], cmev["li"], 2.0 * tot / (2 * W), W=W, trunc=True)
+
+
+def time_batched(ctx, steps, cpu=None, per_seq_s=4.0):
+    """One ixg_search over the whole database (a multi-sequence target: one
+    flat task list, every stage one batch over all sequences) against the
+    plain form, one ixg_search per sequence (each its own target), timed on
+    the leading sequences for about per_seq_s seconds; both at the
+    database's E-value scale, so the per-sequence hits are the batched
+    call's hits on those sequences."""
+    p7, cm, cp9, cmev, seqs, placed = batched_workload()
+    tot = sum(len(x) - 2 for x in seqs)
+    opts = batched_opts(p7, cm, cmev, tot)
+    W = int(cm["W"])
+    g7 = ixg.P7(ctx, p7, W)
+    gcm = ixg.CM(ctx, cm)
+    gcm.set_trpenalties(*ixg.cm_trpenalties(cm)[:2])
+    g9 = ixg.CP9(ctx, cp9)
+    hb = ixg.HbMap(cm, cp9)
+    trd = synth_cp9.build_trunc_cp9s(cm, 1)
+    tr9 = [ixg.CP9(ctx, h) for h in trd]
+    trhb = [ixg.HbMap(cm, h) for h in trd]
+    tgt = ixg.MultiTarget(ctx, seqs)
+    hits, acct = ixg.search(ctx, g7, gcm, g9, hb, tr9, trhb, tgt, opts, with_seq=True)  # warm-up
+    print(json.dumps({"progress": "batched warm-up done"}), file=sys.stderr, flush=True)
+    st = dict.fromkeys(ixg.SEARCH_STAGES, 0.0)
+    ctx.kernel_timings(reset=True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        hits, acct = ixg.search(ctx, g7, gcm, g9, hb, tr9, trhb, tgt, opts, with_seq=True)
+        ms, cells = ixg.search_stats(ctx)
+        for k in st:
+            st[k] += ms[k] / steps
+    dt = (time.perf_counter() - t0) / steps
+    kt = ctx.kernel_timings(reset=True)
+    tgt.free()
+    found = sum(any(h[7] == s and min(h[0], h[1]) <= e and max(h[0], h[1]) >= b and h[2] == sd for h in hits)
+                for s, b, e, sd in placed)
+    # the plain form: one call per sequence, leading sequences, about per_seq_s
+    singles = []
+    for k in range(min(len(seqs), 4000)):
+        singles.append(ixg.Target(ctx, seqs[k], len(seqs[k]) - 2))
+    ixg.search(ctx, g7, gcm, g9, hb, tr9, trhb, singles[0], opts)
+    per, nres, t0 = [], 0, time.perf_counter()
+    k = 0
+    while k < len(singles) and (k < 50 or time.perf_counter() - t0 < per_seq_s):
+        hs, a = ixg.search(ctx, g7, gcm, g9, hb, tr9, trhb, singles[k], opts)
+        per += [h + (k,) for h in hs]
+        nres += a.nres
+        k += 1
+    dts = time.perf_counter() - t0
+    for x in singles:
+        x.free()
+    sub = [h for h in hits if h[7] < k]
+    key = lambda h: (h[5], -h[3], h[7], h[0], -h[6], h[1])  # noqa: E731  (the E-value sort's keys)
+    same = [h[:3] + h[6:] for h in sub] == [h[:3] + h[6:] for h in sorted(per, key=key)]
+    out = {"value": acct.nres / dt / 1e6, "unit": "Mres/s", "ms_per_step": dt * 1e3,
+           "workload": "10,000 iid sequences, 45% GC, log-normal lengths (median 2 kb, sigma 1.0, clipped to "
+                       "200 nt-200 kb, seed 4), 100 CM-sampled homologs inserted into random sequences; "
+                       "120-column three-hairpin CM (stems 8/10/9 bp, 40-seq alignment, seed 4); tier 20-200 Mb: "
+                       "F1 0.35, F2 = F2b 0.15 (Viterbi on), F3=F3b=F4=F4b=F5 0.003, F6 1e-4; default "
+                       "truncation; one ixg_search over a multi-sequence target per step",
+           "sequences": len(seqs), "residues": int(acct.nres), "cm_states": int(cm["M"]), "cm_W": W,
+           "stage_ms": st,
+           "kernels": {kk: {"ms_per_step": v[0] / steps, "launches_per_step": v[1] / steps}
+                       for kk, v in kt.items() if v[1] > 0},
+           "survivors": {"f1": int(acct.f.n_past_msv), "f2": int(acct.f.n_past_vit),
+                         "f3b": int(acct.f.n_past_fwdbias), "f5_envelopes": int(acct.e.n_past_edef),
+                         "f6": int(acct.n_past_cyk), "hits": int(acct.n_hits),
+                         "truncated_pass_hits": [int(x) for x in acct.tr_hits]},
+           "planted_found": f"{found} of {len(placed)}",
+           "per_sequence_calls": {"value": nres / dts / 1e6, "unit": "Mres/s", "sequences": k,
+                                  "ms_per_call": dts / k * 1e3, "same_hits_as_batched": bool(same)}}
+    out["batched_over_per_sequence"] = out["value"] / out["per_sequence_calls"]["value"]
+    if cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_pipe_bind import OracleSearch
+        o = OracleSearch(p7, cm, cp9, W, trcp9=trd)
+        budget, threads = cpu
+        n, t0 = 1000, time.perf_counter()
+        ohits, oa = o.search_multi(seqs[:n], opts, threads=threads)
+        dtc = time.perf_counter() - t0
+        out["cpu"] = {"value": oa.nres / dtc / 1e6, "unit": "Mres/s", "cores": threads, "kind": "port",
+                      "sample": f"the first {n} sequences ({oa.nres} residues searched), oracle "
+                                f"ixo_search_multi_mt (C restatement, not infernox), {dtc:.1f} s"}
+    for x in [g9, gcm, g7, hb] + tr9 + trhb:
+        x.free()
+    return out
+
+
 def default_threads():
     """Host threads for the CPU baselines: the CPUs this process may run on,
     capped by the cgroup CPU quota when one is set."""
@@ -948,6 +1056,8 @@ def main():
     ap.add_argument("--headline-only", action="store_true", help="only the headline search step")
     ap.add_argument("--serve-only", action="store_true", help="only run the serving-loop section")
     ap.add_argument("--multi-only", action="store_true", help="only run the several-CM HB launch section")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only run the batched uneven-database search section (10,000 sequences)")
     ap.add_argument("--hb-only", action="store_true",
                     help="only run the HB CYK / Inside section (959-state CM, 512 envelopes)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -975,6 +1085,11 @@ def main():
             cpu = None if a.no_cpu else (a.cpu_section_seconds, a.cpu_threads)
             print(json.dumps({"hb_scanners": time_cm(ctx, a.steps, a.warmup, cpu, cm_workload())}))
         return
+    if a.batched_only:
+        with ixg.Context(0) as ctx:
+            cpu = None if a.no_cpu else (a.cpu_section_seconds, a.cpu_threads)
+            print(json.dumps({"batched_uneven_search": time_batched(ctx, max(1, a.steps // 3), cpu)}))
+        return
     if a.serve_only:
         with ixg.Context(0) as ctx:
             print(json.dumps({"serving": time_serving(ctx)}))
@@ -998,6 +1113,7 @@ def main():
             p, d, _, _, _, _ = workload()
             extra["p7_forward_full"] = time_p7_forward(ctx, p, d, 2, cpu=cpu)
             extra["hb_multi"] = time_multi(ctx)
+            extra["batched_uneven_search"] = time_batched(ctx, max(1, a.steps // 3), cpu)
         if secondary and not a.no_serve:
             extra["serving"] = time_serving(ctx)
     p7, cm = wl[0], wl[1]

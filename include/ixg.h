@@ -593,13 +593,16 @@ int ixg_search_std(ixg_ctx *ctx, const ixg_p7 *p7, const ixg_cm *cm, const ixg_c
 int ixg_search(ixg_ctx *ctx, const ixg_p7 *p7, const ixg_cm *cm, const ixg_cp9 *cp9, const ixg_hbmap *map,
                const ixg_cp9 *const *trcp9, const ixg_hbmap *const *trmap, const ixg_target *tgt,
                const ixg_searchopts *o, ixg_searchhit *out, int64_t cap, int64_t *n, ixg_searchacct *acct);
-/* Host wall time (ms) of each stage of the last ixg_search_std call on the
+/* Host wall time (ms) of each stage of the last ixg_search call on the
  * context (IXG_SS_* index; a stage's time includes its launches, waits and
- * host work) and the HB cells its CYK (cells[0]) and Inside (cells[1])
- * launches computed. A measurement hook with no counterpart in the
- * reference. */
+ * host work) and the HB cells its STD CYK (cells[0]) and Inside (cells[1])
+ * launches computed. The truncated passes' CM stages run concurrently with
+ * the STD pass's F6 / F7 (each pass on an auxiliary context's stream):
+ * IXG_SS_TRUNC is their wall time from start to join, and IXG_SS_FINALIZE
+ * includes what of it was left after F7. A measurement hook with no
+ * counterpart in the reference. */
 enum { IXG_SS_F1_F3B = 0, IXG_SS_F4_F5 = 1, IXG_SS_F6_BANDS = 2, IXG_SS_F6_CYK = 3, IXG_SS_F7_BANDS = 4,
-       IXG_SS_F7_INSIDE = 5, IXG_SS_FINALIZE = 6, IXG_SS_TOTAL = 7, IXG_SS_N = 8 };
+       IXG_SS_F7_INSIDE = 5, IXG_SS_FINALIZE = 6, IXG_SS_TOTAL = 7, IXG_SS_TRUNC = 8, IXG_SS_N = 9 };
 int ixg_search_stats(ixg_ctx *ctx, double *ms, int64_t *cells);
 
 /* Per-kernel device time, cumulative since the last reset: HIP events around

@@ -240,6 +240,8 @@ int ixg_init(int device, size_t mem_budget, ixg_ctx **out) {
 
 int ixg_shutdown(ixg_ctx *c) {
   if (!c) return IXG_EINVAL;
+  for (auto *a : c->aux)
+    if (a) ixg_shutdown(a);
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->stream);
   for (auto &b : c->bufs) dfree(c, b.p, b.cap);
@@ -621,18 +623,27 @@ int run_f1(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<
   }
   // records by task, then by trigger row within a task (blocks append in
   // trigger order, but blocks of one task interleave)
-  raws.assign(tasks.size(), {});
-  std::vector<std::vector<std::pair<int64_t, int32_t>>> by(tasks.size());
-  for (int32_t i = 0; i < (int32_t)count; i++) by[h[i].task].push_back({h[i].row, i});
-  for (size_t t = 0; t < tasks.size(); t++) {
-    auto &v = by[t];
-    std::sort(v.begin(), v.end());
-    raws[t].reserve(v.size());
-    for (const auto &e : v) {
-      const RawWinDev &r = h[e.second];
+  // (a counting sort by task, then each task's records by (row, index) on
+  // the host workers)
+  const size_t ntk = tasks.size();
+  raws.assign(ntk, {});
+  std::vector<int64_t> tbeg(ntk + 1, 0);
+  for (int64_t i = 0; i < (int64_t)count; i++) tbeg[h[i].task + 1]++;
+  for (size_t t = 0; t < ntk; t++) tbeg[t + 1] += tbeg[t];
+  std::vector<int32_t> ord(count);
+  {
+    std::vector<int64_t> at(tbeg.begin(), tbeg.end() - 1);
+    for (int32_t i = 0; i < (int32_t)count; i++) ord[at[h[i].task]++] = i;
+  }
+  par_for((int64_t)ntk, [&](int64_t t) {
+    int32_t *b = ord.data() + tbeg[t], *e = ord.data() + tbeg[t + 1];
+    std::sort(b, e, [&](int32_t x, int32_t y) { return h[x].row != h[y].row ? h[x].row < h[y].row : x < y; });
+    raws[t].reserve(e - b);
+    for (const int32_t *q = b; q < e; q++) {
+      const RawWinDev &r = h[*q];
       raws[t].push_back({r.n, r.length, r.k, r.score});
     }
-  }
+  });
   float a = 0, b = 0, kr = 0;
   HIPCHK(hipEventElapsedTime(&a, c->ev[0], c->ev[1]));
   HIPCHK(hipEventElapsedTime(&b, c->ev[1], c->ev[2]));
@@ -656,7 +667,8 @@ int scores_impl(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::ve
   ovf.assign(n, 0);
   if (n == 0) return IXG_OK;
   std::vector<WinDev> wd(n);
-  for (int64_t i = 0; i < n; i++) {
+  par_range(n, 2048, [&](int64_t lo, int64_t hi) {
+   for (int64_t i = lo; i < hi; i++) {
     const WinRef &w = wl[i];
     const int64_t len = w.end - w.start + 1;
     wd[i].off = tasks[w.task].base + w.start - 1;
@@ -674,7 +686,8 @@ int scores_impl(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::ve
       wd[i].a = (float)unbiased_byteify(p->msv.scale_b, (float)std::log(3.0 / ((double)len + 3.0)));
       wd[i].b = (float)p->msv.tec_b;
     }
-  }
+   }
+  });
   // longest first (a stable counting sort by length), so windows that share
   // a wave or workgroup have similar lengths
   {
@@ -731,7 +744,8 @@ int cascade(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector
     std::vector<std::vector<RawW>> raws;
     RCHK(run_f1(c, p, d_seq, tasks, o.f1, raws));
     Timer tm;
-    for (size_t t = 0; t < nt; t++) wins[t] = split_windows(extend_and_merge(p->msv, raws[t], tasks[t].L), o.cmw);
+    par_for((int64_t)nt,
+            [&](int64_t t) { wins[t] = split_windows(extend_and_merge(p->msv, raws[t], tasks[t].L), o.cmw); });
     c->ms[3] += tm.ms();
   } else {  // deterministic 2*maxW tiling (cm_pipeline.rs:1893-1912)
     for (size_t t = 0; t < nt; t++) {
@@ -748,13 +762,13 @@ int cascade(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector
     }
   }
   // flat window list + per-window survival flags
-  std::vector<WinRef> all;
   std::vector<size_t> first(nt + 1, 0);
-  for (size_t t = 0; t < nt; t++) {
-    first[t] = all.size();
-    for (auto &w : wins[t]) all.push_back({(int32_t)t, w.start, w.end});
-  }
-  first[nt] = all.size();
+  for (size_t t = 0; t < nt; t++) first[t + 1] = first[t] + wins[t].size();
+  std::vector<WinRef> all(first[nt]);
+  par_range((int64_t)nt, 256, [&](int64_t lo, int64_t hi) {
+    for (int64_t t = lo; t < hi; t++)
+      for (size_t q = 0; q < wins[t].size(); q++) all[first[t] + q] = {(int32_t)t, wins[t][q].start, wins[t][q].end};
+  });
   const size_t N = all.size();
   std::vector<uint8_t> s_f1b(N, 1), s_f2(N, 1), s_f2b(N, 1), s_f3(N, 0), s_f3b(N, 0), alive(N, 1);
   std::vector<float> wb(N, -999.0f), biassc(N, 0.f), fwdsc(N, 0.f);
@@ -809,13 +823,20 @@ int cascade(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector
   if (do_vit) {  // F2/F2b (cm_pipeline.rs:1966-1997)
     alive_list(idx, sub);
     RCHK(scores_impl(c, p, d_seq, tasks, sub, 2, sc, ov));
+    std::vector<uint8_t> fail(idx.size(), 0);  // the P-values on the host workers
+    par_range((int64_t)idx.size(), 2048, [&](int64_t lo, int64_t hi) {
+      for (int64_t j = lo; j < hi; j++) {
+        if (ov[j]) continue;
+        const int64_t wlen = all[idx[j]].end - all[idx[j]].start + 1;
+        const double w = ((double)sc[j] - bg_null_one(wlen)) / LN2;
+        fail[j] = gumbel_surv(w, p->vit_lvmu, p->vit_lvlambda) > o.f2;
+      }
+    });
     std::vector<size_t> pass;
     for (size_t j = 0; j < idx.size(); j++) {
       const size_t i = idx[j];
       if (ov[j]) continue;
-      const int64_t wlen = all[i].end - all[i].start + 1;
-      double w = ((double)sc[j] - bg_null_one(wlen)) / LN2;
-      if (gumbel_surv(w, p->vit_lvmu, p->vit_lvlambda) > o.f2) {
+      if (fail[j]) {
         s_f2[i] = s_f2b[i] = 0;
         alive[i] = 0;
         continue;
@@ -826,14 +847,16 @@ int cascade(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector
       std::vector<size_t> bidx;
       for (size_t j : pass) bidx.push_back(idx[j]);
       RCHK(ensure_bias(bidx));
-      for (size_t j : pass) {
-        const size_t i = idx[j];
-        double w = ((double)sc[j] - (double)biassc[i]) / LN2;
-        if (gumbel_surv(w, p->vit_lvmu, p->vit_lvlambda) > o.f2b) {
-          s_f2b[i] = 0;
-          alive[i] = 0;
+      par_range((int64_t)pass.size(), 2048, [&](int64_t lo, int64_t hi) {
+        for (int64_t q = lo; q < hi; q++) {
+          const size_t j = pass[q], i = idx[j];
+          const double w = ((double)sc[j] - (double)biassc[i]) / LN2;
+          if (gumbel_surv(w, p->vit_lvmu, p->vit_lvlambda) > o.f2b) {
+            s_f2b[i] = 0;
+            alive[i] = 0;
+          }
         }
-      }
+      });
     }
   }
   if (do_fwd) {  // F3/F3b (cm_pipeline.rs:2013-2052)
@@ -882,7 +905,7 @@ int cascade(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector
   // survivors, merge of overlapping/abutting windows (cm_pipeline.rs:2056-2114)
   res.assign(nt, {});
   accts.assign(nt, ixg_acct{});
-  for (size_t t = 0; t < nt; t++) {
+  par_for((int64_t)nt, [&](int64_t t) {
     std::vector<Win> sv;
     for (size_t i = first[t]; i < first[t + 1]; i++)
       if (s_f3b[i]) sv.push_back({all[i].start, all[i].end, wb[i]});
@@ -920,7 +943,7 @@ int cascade(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector
         }
       }
     }
-  }
+  });
   c->ms[3] += tm.ms();
   return IXG_OK;
 }
@@ -1164,7 +1187,8 @@ int ixg_f3_filter_tasks(ixg_ctx *c, const ixg_p7 *p, const ixg_target *tgt, cons
 
 namespace ixg {
 // F1-F3b of many sequence views of a target (HostTask bases in its device
-// buffer, any of its sequences and strands); windows carry their view's index
+// buffer, any of its sequences and strands); windows carry their view's
+// index. The stage timings (ms, cells) add up over calls: the caller resets.
 int f3_filter_views(ixg_ctx *c, const ixg_p7 *p, const ixg_target *tgt, const std::vector<HostTask> &tasks,
                     const ixg_f3opts &o, std::vector<ixg_window> &wins, std::vector<ixg_acct> &accts) {
   HIPCHK(hipSetDevice(c->device));
@@ -1173,8 +1197,10 @@ int f3_filter_views(ixg_ctx *c, const ixg_p7 *p, const ixg_target *tgt, const st
       g_err = "f3_filter_views: a view outside the target";
       return IXG_EINVAL;
     }
+  Timer total;
   std::vector<std::vector<Win>> res;
   RCHK(cascade(c, p, tgt->d_both, tasks, o, res, accts));
+  c->ms[4] += total.ms();
   int64_t n = 0;
   for (auto &r : res) n += (int64_t)r.size();
   wins.resize(n);

@@ -102,6 +102,14 @@ void par_for(int64_t n, F &&f) {
   pool_run(n, [](void *a, int64_t k) { (*static_cast<F *>(a))(k); }, &f);
 }
 
+// f(lo, hi) over [0, n) in blocks of `grain` indices on the host workers
+// (for loops whose bodies are too small to hand out one index at a time)
+template <class F>
+void par_range(int64_t n, int64_t grain, F &&f) {
+  const int64_t nb = (n + grain - 1) / grain;
+  par_for(nb, [&](int64_t b) { f(b * grain, std::min(n, (b + 1) * grain)); });
+}
+
 }  // namespace ixg
 
 struct ixg_ctx {
@@ -131,8 +139,12 @@ struct ixg_ctx {
   double qdb_cand_rate = 0.125, trq_cand_rate = 0.125;
   // the last ixg_search_std call: host wall time per stage (IXG_SS_*) and
   // the HB cells its CYK and Inside launches computed
-  double search_ms[8] = {};
+  double search_ms[9] = {};
   int64_t search_cells[2] = {};
+  // auxiliary contexts (own stream and scratch) on which ixg_search runs the
+  // truncated passes' CM stages concurrently with the STD pass's; created on
+  // first use, freed with this context
+  ixg_ctx *aux[3] = {};
 };
 
 namespace ixg {

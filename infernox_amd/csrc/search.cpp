@@ -26,6 +26,8 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <string>
+#include <thread>
 #include "ctx.h"
 
 using namespace ixg;
@@ -229,6 +231,7 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
   const clk::time_point t_begin = clk::now();
   clk::time_point t_mark = t_begin;
   for (double &x : c->search_ms) x = 0;
+  for (int i = 0; i < 5; i++) c->ms[i] = 0, c->cells[i] = 0;
   c->search_cells[0] = c->search_cells[1] = 0;
   auto stage = [&](int k) {  // every stage ends with its results on the host
     const clk::time_point t = clk::now();
@@ -305,19 +308,51 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
     ht[t].L = k.end - k.start + 1;
     if (t < nt) A.nres += (uint64_t)(ht[t].L - k.ctx);  // STD pass: new residues (cm_search.rs:2306-2311)
   }
-  // F1-F3b over every chunk and truncated-pass subsequence in one call
+  // F1-F3b over every chunk and truncated-pass subsequence. With few
+  // subsequences, one call over all of them; with many (a database of short
+  // sequences, where every sequence has both ends), the chunks first and
+  // then only the subsequences of chunks with STD survivors, the ones whose
+  // results are used (cm_search.rs:2333-2335): the same windows and
+  // counters for less filter work.
   std::vector<ixg_acct> fa;
   std::vector<ixg_window> wins;
   int rc = IXG_OK;
-  RCHK(f3_filter_views(c, p, tgt, ht, o->f3, wins, fa));
+  const int64_t nsub = nft - nt;
+  if (nsub <= 256) {
+    RCHK(f3_filter_views(c, p, tgt, ht, o->f3, wins, fa));
+  } else {
+    std::vector<HostTask> h1(ht.begin(), ht.begin() + nt);
+    RCHK(f3_filter_views(c, p, tgt, h1, o->f3, wins, fa));
+    fa.resize(nft);
+    std::vector<char> has(nt, 0);
+    for (const ixg_window &w : wins) has[w.task] = 1;
+    std::vector<int32_t> pick;  // subsequence tasks to filter, ascending
+    for (const TrRun &R : truns)
+      if (R.sub >= nt && has[R.task]) pick.push_back(R.sub);
+    std::sort(pick.begin(), pick.end());
+    if (!pick.empty()) {
+      std::vector<HostTask> h2(pick.size());
+      for (size_t q = 0; q < pick.size(); q++) h2[q] = ht[pick[q]];
+      std::vector<ixg_window> w2;
+      std::vector<ixg_acct> a2;
+      RCHK(f3_filter_views(c, p, tgt, h2, o->f3, w2, a2));
+      for (size_t q = 0; q < pick.size(); q++) fa[pick[q]] = a2[q];
+      for (ixg_window w : w2) {
+        w.task = pick[w.task];
+        wins.push_back(w);
+      }
+    }
+  }
   stage(IXG_SS_F1_F3B);
   auto add_acct = [](const void *src, void *dst, size_t bytes) {
     const uint64_t *s = (const uint64_t *)src;
     uint64_t *d = (uint64_t *)dst;
     for (size_t q = 0; q < bytes / 8; q++) d[q] += s[q];
   };
-  std::vector<int64_t> nwin_of(nft, 0);
+  // windows come in task order: task t's are wins[wbeg[t] .. wbeg[t+1])
+  std::vector<int64_t> nwin_of(nft, 0), wbeg(nft + 1, 0);
   for (const ixg_window &w : wins) nwin_of[w.task]++;
+  for (int64_t t = 0; t < nft; t++) wbeg[t + 1] = wbeg[t] + nwin_of[t];
   for (int64_t t = 0; t < nt; t++) add_acct(&fa[t], &A.f, sizeof(ixg_acct));
   // envelope definition of the STD pass and of every truncated pass whose
   // chunk had STD survivors (cm_search.rs:2333-2335) and its own
@@ -329,8 +364,8 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
     const TrRun &R = truns[r];
     if (nwin_of[R.task] == 0) continue;  // no STD survivors: the truncated passes are skipped
     add_acct(&fa[R.sub], &A.tr_f[R.pass - IXG_PASS_5P_ONLY_FORCE], sizeof(ixg_acct));
-    for (const ixg_window &w : wins)
-      if (w.task == R.sub) ew.push_back({R.sub, w.start, w.end, R.pass}), ew_run.push_back((int32_t)r);
+    for (int64_t q = wbeg[R.sub]; q < wbeg[R.sub + 1]; q++)
+      ew.push_back({R.sub, wins[q].start, wins[q].end, R.pass}), ew_run.push_back((int32_t)r);
   }
   std::vector<std::vector<std::pair<int32_t, int32_t>>> envs;
   std::vector<ixg_envacct> ea;
@@ -359,6 +394,142 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
       }
     }
   }
+  // the truncated passes' CM stages (cm_search.rs:2863 cm_trunc_stage), one
+  // pass per call (the pass selects the CP9 and the scanners' planes) on
+  // context cc: TrCYK over bands at the filter tau with its gate and
+  // envelope redefinition (not at an enforced end), then FTrInside (or
+  // TrCYK) at the final tau; hits in the chunk's coordinates shifted by the
+  // pass's offset. Writes only its own TrOut.
+  struct TrOut {
+    std::vector<Hit> hits;
+    uint64_t n_past_cyk = 0, pos_past_cyk = 0, nhits = 0;
+    int rc = IXG_OK;
+    std::string err;
+  };
+  auto trunc_pass = [&](ixg_ctx *cc, int q, TrOut &out) -> int {
+    HIPCHK(hipSetDevice(cc->device));
+    const int pass = IXG_PASS_5P_ONLY_FORCE + q;
+    int rc = IXG_OK;
+    const bool ef = pass != IXG_PASS_3P_ONLY_FORCE, el = pass != IXG_PASS_5P_ONLY_FORCE;  // enforced first / final
+    std::vector<ixg_envelope> pe;
+    std::vector<int32_t> prun;
+    for (size_t r = 0; r < truns.size(); r++)
+      if (truns[r].pass == pass)
+        for (const ixg_envelope &x : tenv[r]) pe.push_back(x), prun.push_back((int32_t)r);
+    if (pe.empty()) return IXG_OK;
+    std::vector<ixg_envelope> ps;
+    std::vector<int32_t> psrun;
+    if (!o->do_fcyk) {
+      ps = pe, psrun = prun;
+    } else {
+      std::vector<TrEnvBands> tb;
+      RCHK(trunc_bands(cc, trcp9[q], trmap[q], tgt, dbase, pe, pass, o->fcyk_tau, tb));
+      std::vector<ixg_trenvelope> te(pe.size());
+      for (size_t k = 0; k < pe.size(); k++) {
+        te[k].i0 = pe[k].i0, te[k].j0 = pe[k].j0, te[k].strand = pe[k].strand, te[k].seq = pe[k].seq;
+        te[k].bands = tb[k].b.view();
+        te[k].valid = tb[k].valid();
+      }
+      std::vector<float> vsc(te.size());
+      std::vector<int32_t> vmode(te.size());
+      std::vector<int64_t> ei(te.size()), ej(te.size());
+      int64_t nh = 0;
+      // the filter's hits are not used: no room for them (IXG_ERANGE)
+      rc = ixg_trhb_scan_batch(cc, cm, tgt, te.data(), (int64_t)te.size(), 0, pass, 1, o->cyk_env_cutoff, 0.0f,
+                               o->do_null3, vsc.data(), vmode.data(), ei.data(), ej.data(), nullptr, 0, &nh, nullptr);
+      if (rc != IXG_ERANGE) RCHK(rc);
+      for (size_t k = 0; k < te.size(); k++) {
+        if (exp_surv((double)vsc[k], o->cyk_mu, o->cyk_lambda) > o->f6) continue;
+        ixg_envelope x = pe[k];
+        if (o->do_fcykenv && ei[k] != -1 && ej[k] != -1) {
+          if (!ef) x.i0 = (int32_t)ei[k];
+          if (!el) x.j0 = (int32_t)ej[k];
+        }
+        out.n_past_cyk++;
+        out.pos_past_cyk += (uint64_t)(x.j0 - x.i0 + 1);
+        ps.push_back(x);
+        psrun.push_back(prun[k]);
+      }
+    }
+    if (ps.empty()) return IXG_OK;
+    std::vector<TrEnvBands> tb;
+    RCHK(trunc_bands(cc, trcp9[q], trmap[q], tgt, dbase, ps, pass, o->final_tau, tb));
+    std::vector<ixg_trenvelope> te(ps.size());
+    for (size_t k = 0; k < ps.size(); k++) {
+      te[k].i0 = ps[k].i0, te[k].j0 = ps[k].j0, te[k].strand = ps[k].strand, te[k].seq = ps[k].seq;
+      te[k].bands = tb[k].b.view();
+      te[k].valid = tb[k].valid();
+    }
+    std::vector<float> vsc(te.size());
+    std::vector<int32_t> vmode(te.size());
+    std::vector<int64_t> ei(te.size()), ej(te.size());
+    int64_t nh = 0;
+    std::vector<ixg_trhit> th(1024);
+    rc = ixg_trhb_scan_batch(cc, cm, tgt, te.data(), (int64_t)te.size(), o->do_cyk ? 0 : 1, pass, 1, 0.0f, o->pli_t,
+                             o->do_null3, vsc.data(), vmode.data(), ei.data(), ej.data(), th.data(),
+                             (int64_t)th.size(), &nh, nullptr);
+    if (rc == IXG_ERANGE) {
+      th.resize(nh);
+      rc = ixg_trhb_scan_batch(cc, cm, tgt, te.data(), (int64_t)te.size(), o->do_cyk ? 0 : 1, pass, 1, 0.0f,
+                               o->pli_t, o->do_null3, vsc.data(), vmode.data(), ei.data(), ej.data(), th.data(), nh,
+                               &nh, nullptr);
+    }
+    RCHK(rc);
+    th.resize(nh);
+    for (const ixg_trhit &x : th) {
+      const TrRun &R = truns[psrun[x.env]];
+      const int64_t off = view_off[R.task];  // the chunk's offset in its strand buffer
+      Hit h;
+      h.task = R.task;
+      h.seq = tseq[R.task];
+      h.pass = pass;
+      genome_coords(tasks[R.task], x.i - off, x.j - off, h);
+      h.score = x.score;
+      h.bias = x.bias;
+      h.evalue = exp_surv((double)h.score, o->fin_mu, o->fin_lambda) * o->ez;
+      out.hits.push_back(h);
+      out.nhits++;
+    }
+    return IXG_OK;
+  };
+  // each truncated pass with envelopes runs on a thread of its own with an
+  // auxiliary context (its own stream and scratch), overlapping the STD
+  // pass's F6 / F7 launches on this context's stream; joined before finalize
+  TrOut tro[3];
+  std::vector<std::thread> trth;
+  std::vector<int> trq;
+  for (int q = 0; q < 3 && do_trunc; q++) {
+    bool any = false;
+    for (size_t r = 0; r < truns.size() && !any; r++) any = truns[r].pass == IXG_PASS_5P_ONLY_FORCE + q && !tenv[r].empty();
+    if (!any) continue;
+    if (!c->aux[q]) RCHK(ixg_init(c->device, c->budget, &c->aux[q]));
+    trq.push_back(q);
+  }
+  const clk::time_point t_tr = clk::now();
+  struct Joiner {  // every return path waits for the threads before the state they read goes away
+    std::vector<std::thread> &t;
+    ~Joiner() {
+      for (auto &x : t)
+        if (x.joinable()) x.join();
+    }
+  } joiner{trth};
+  for (int q : trq)
+    trth.emplace_back([&, q] {
+      tro[q].rc = trunc_pass(c->aux[q], q, tro[q]);
+      if (tro[q].rc != IXG_OK) tro[q].err = g_err;
+    });
+  auto join_trunc = [&]() -> int {
+    for (auto &t : trth) t.join();
+    trth.clear();
+    int first = IXG_OK;
+    for (int q : trq) {
+      ixg_ctx *x = c->aux[q];
+      HIPCHK(ktime_flush(x));
+      for (int k = 0; k < IXG_NKERNELS; k++) c->kms[k] += x->kms[k], c->kn[k] += x->kn[k], x->kms[k] = 0, x->kn[k] = 0;
+      if (tro[q].rc != IXG_OK && first == IXG_OK) first = tro[q].rc, g_err = tro[q].err;
+    }
+    return first;
+  };
   std::vector<Hit> hits;
   // F6: CYK filter over HMM bands at the filter tau
   std::vector<ixg_envelope> surv;
@@ -468,93 +639,17 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
       }
     }
   }
-  // the truncated passes' CM stages (cm_search.rs:2863 cm_trunc_stage), one
-  // pass at a time (the pass selects the CP9 and the scanners' planes):
-  // TrCYK over bands at the filter tau with its gate and envelope
-  // redefinition (not at an enforced end), then FTrInside (or TrCYK) at the
-  // final tau; hits in the chunk's coordinates shifted by the pass's offset
-  for (int q = 0; q < 3 && do_trunc; q++) {
-    const int pass = IXG_PASS_5P_ONLY_FORCE + q;
-    const bool ef = pass != IXG_PASS_3P_ONLY_FORCE, el = pass != IXG_PASS_5P_ONLY_FORCE;  // enforced first / final
-    std::vector<ixg_envelope> pe;
-    std::vector<int32_t> prun;
-    for (size_t r = 0; r < truns.size(); r++)
-      if (truns[r].pass == pass)
-        for (const ixg_envelope &x : tenv[r]) pe.push_back(x), prun.push_back((int32_t)r);
-    if (pe.empty()) continue;
-    std::vector<ixg_envelope> ps;
-    std::vector<int32_t> psrun;
-    if (!o->do_fcyk) {
-      ps = pe, psrun = prun;
-    } else {
-      std::vector<TrEnvBands> tb;
-      RCHK(trunc_bands(c, trcp9[q], trmap[q], tgt, dbase, pe, pass, o->fcyk_tau, tb));
-      std::vector<ixg_trenvelope> te(pe.size());
-      for (size_t k = 0; k < pe.size(); k++) {
-        te[k].i0 = pe[k].i0, te[k].j0 = pe[k].j0, te[k].strand = pe[k].strand, te[k].seq = pe[k].seq;
-        te[k].bands = tb[k].b.view();
-        te[k].valid = tb[k].valid();
-      }
-      std::vector<float> vsc(te.size());
-      std::vector<int32_t> vmode(te.size());
-      std::vector<int64_t> ei(te.size()), ej(te.size());
-      int64_t nh = 0;
-      // the filter's hits are not used: no room for them (IXG_ERANGE)
-      rc = ixg_trhb_scan_batch(c, cm, tgt, te.data(), (int64_t)te.size(), 0, pass, 1, o->cyk_env_cutoff, 0.0f,
-                               o->do_null3, vsc.data(), vmode.data(), ei.data(), ej.data(), nullptr, 0, &nh, nullptr);
-      if (rc != IXG_ERANGE) RCHK(rc);
-      for (size_t k = 0; k < te.size(); k++) {
-        if (exp_surv((double)vsc[k], o->cyk_mu, o->cyk_lambda) > o->f6) continue;
-        ixg_envelope x = pe[k];
-        if (o->do_fcykenv && ei[k] != -1 && ej[k] != -1) {
-          if (!ef) x.i0 = (int32_t)ei[k];
-          if (!el) x.j0 = (int32_t)ej[k];
-        }
-        A.tr_n_past_cyk[q]++;
-        A.tr_pos_past_cyk[q] += (uint64_t)(x.j0 - x.i0 + 1);
-        ps.push_back(x);
-        psrun.push_back(prun[k]);
-      }
-    }
-    if (ps.empty()) continue;
-    std::vector<TrEnvBands> tb;
-    RCHK(trunc_bands(c, trcp9[q], trmap[q], tgt, dbase, ps, pass, o->final_tau, tb));
-    std::vector<ixg_trenvelope> te(ps.size());
-    for (size_t k = 0; k < ps.size(); k++) {
-      te[k].i0 = ps[k].i0, te[k].j0 = ps[k].j0, te[k].strand = ps[k].strand, te[k].seq = ps[k].seq;
-      te[k].bands = tb[k].b.view();
-      te[k].valid = tb[k].valid();
-    }
-    std::vector<float> vsc(te.size());
-    std::vector<int32_t> vmode(te.size());
-    std::vector<int64_t> ei(te.size()), ej(te.size());
-    int64_t nh = 0;
-    std::vector<ixg_trhit> th(1024);
-    rc = ixg_trhb_scan_batch(c, cm, tgt, te.data(), (int64_t)te.size(), o->do_cyk ? 0 : 1, pass, 1, 0.0f, o->pli_t,
-                             o->do_null3, vsc.data(), vmode.data(), ei.data(), ej.data(), th.data(),
-                             (int64_t)th.size(), &nh, nullptr);
-    if (rc == IXG_ERANGE) {
-      th.resize(nh);
-      rc = ixg_trhb_scan_batch(c, cm, tgt, te.data(), (int64_t)te.size(), o->do_cyk ? 0 : 1, pass, 1, 0.0f,
-                               o->pli_t, o->do_null3, vsc.data(), vmode.data(), ei.data(), ej.data(), th.data(), nh,
-                               &nh, nullptr);
-    }
-    RCHK(rc);
-    th.resize(nh);
-    for (const ixg_trhit &x : th) {
-      const TrRun &R = truns[psrun[x.env]];
-      const int64_t off = view_off[R.task];  // the chunk's offset in its strand buffer
-      Hit h;
-      h.task = R.task;
-      h.seq = tseq[R.task];
-      h.pass = pass;
-      genome_coords(tasks[R.task], x.i - off, x.j - off, h);
-      h.score = x.score;
-      h.bias = x.bias;
-      h.evalue = exp_surv((double)h.score, o->fin_mu, o->fin_lambda) * o->ez;
-      hits.push_back(h);
-      A.tr_hits[q]++;
-    }
+  {
+    const int trc = join_trunc();
+    c->search_ms[IXG_SS_TRUNC] = std::chrono::duration<double, std::milli>(clk::now() - t_tr).count();
+    stage(IXG_SS_FINALIZE);
+    RCHK(trc);
+  }
+  for (int q : trq) {
+    hits.insert(hits.end(), tro[q].hits.begin(), tro[q].hits.end());
+    A.tr_n_past_cyk[q] += tro[q].n_past_cyk;
+    A.tr_pos_past_cyk[q] += tro[q].pos_past_cyk;
+    A.tr_hits[q] += tro[q].nhits;
   }
   // finalize (cm_search.rs:1915): global overlap removal per sequence and
   // strand (:3260), then the E-value sort and the reporting threshold

@@ -1215,7 +1215,8 @@ def search(ctx, p7, cm, cp9, hbmap, trcp9, trmap, tgt, opts, with_seq=False):
                         "ixg_search", with_pass=True, with_seq=with_seq)
 
 
-SEARCH_STAGES = ("f1_f3b", "f4_f5", "f6_bands", "f6_cyk", "f7_bands", "f7_inside", "finalize", "total")
+SEARCH_STAGES = ("f1_f3b", "f4_f5", "f6_bands", "f6_cyk", "f7_bands", "f7_inside", "finalize", "total",
+                 "truncated_passes")
 
 
 def search_stats(ctx):

@@ -12,7 +12,7 @@ this build:
 * its CP9 and p7 filter HMM from the CM's consensus columns
   (synth_cp9.build_cp9_like; the p7 takes the same match emissions with
   mostly match-to-match transitions), and E-value parameters calibrated on
-  random sequence (tests/golden/calibrate_config1.py, values held here);
+  random sequence (tests/golden/calibrate_models.py, values held here);
 * the target: iid uniform ACGU with CM-sampled homologs inserted on random
   strands.
 """
@@ -110,12 +110,12 @@ def p7_from_cm(cm, ev):
 
 
 # E-value parameters of the config-1 model (cloverleaf, alignment seed 1),
-# from tests/golden/calibrate_config1.py: p7 local MSV / Viterbi Gumbel mu,
+# from tests/golden/calibrate_models.py 1: p7 local MSV / Viterbi Gumbel mu,
 # local Forward and glocal Forward exponential tau (lambda ln 2); CM local
 # CYK and Inside exponential tails (mu, lambda) in bits.
-CONFIG1_EV = dict(lmmu=-7.7994, lmlambda=0.693147, lvmu=-8.7676, lvlambda=0.693147, lftau=-4.1874,
-                  lflambda=0.693147, gfmu=-18.0086, gflambda=0.693147)
-CONFIG1_CM_EV = dict(lc=(-1.4198, 0.693147), li=(-0.1401, 0.693147))
+CONFIG1_EV = dict(lmmu=-7.737, lmlambda=0.693147, lvmu=-8.7349, lvlambda=0.693147, lftau=-3.7808,
+                  lflambda=0.693147, gfmu=-17.6875, gflambda=0.693147)
+CONFIG1_CM_EV = dict(lc=(-1.0937, 0.693147), li=(0.1358, 0.693147))
 
 
 def config1_models(ev=None, cm_ev=None):
@@ -126,6 +126,59 @@ def config1_models(ev=None, cm_ev=None):
     cp9 = P.build_cp9_like(cm, 1, el=True)
     p7 = p7_from_cm(cm, ev or CONFIG1_EV)
     return p7, cm, cp9, (cm_ev or CONFIG1_CM_EV)
+
+
+def three_hairpin_ss():
+    """120-column structure of three hairpins in a row: stems of 8, 10 and
+    9 bp closing loops of 10, 12 and 8 nt, with 8 / 7 / 9 / 12-nt unpaired
+    stretches before, between and after them."""
+    return ("." * 8 + "<" * 8 + "." * 10 + ">" * 8 + "." * 7 + "<" * 10 + "." * 12 + ">" * 10 + "." * 9 + "<" * 9 +
+            "." * 8 + ">" * 9 + "." * 12)
+
+
+# the same for the three-hairpin model (calibrate_models.py 3)
+CONFIG3_EV = dict(lmmu=-8.4042, lmlambda=0.693147, lvmu=-9.323, lvlambda=0.693147, lftau=-3.805,
+                  lflambda=0.693147, gfmu=-36.98, gflambda=0.693147)
+CONFIG3_CM_EV = dict(lc=(-1.2905, 0.693147), li=(-0.1442, 0.693147))
+
+
+def config3_models(ev=None, cm_ev=None):
+    """(p7 dict, CM dict, CP9 dict, cm_ev) of the batched-workload model: the
+    three-hairpin structure, a 40-sequence alignment (seed 4) and the same
+    builders as config1_models."""
+    ss = three_hairpin_ss()
+    aln = alignment(ss, 40, seed=4)
+    cm = cm_from_alignment(ss, aln, seed=4)
+    cp9 = P.build_cp9_like(cm, 1, el=True)
+    p7 = p7_from_cm(cm, ev or CONFIG3_EV)
+    return p7, cm, cp9, (cm_ev or CONFIG3_CM_EV)
+
+
+def lognormal_database(n, median, sigma, lo, hi, gc, seed, cm=None, n_hom=0):
+    """n iid sequences (ACGU at the given GC) with log-normal lengths (the
+    given median and sigma, clipped to [lo, hi]); with a CM, n_hom
+    CM-sampled homologs inserted into randomly chosen sequences on random
+    strands. Returns (list of 1-indexed dsq arrays, [(seq, start, end,
+    strand)])."""
+    r = np.random.default_rng(seed)
+    lens = np.clip(np.rint(r.lognormal(np.log(median), sigma, n)), lo, hi).astype(np.int64)
+    p = [(1 - gc) / 2, gc / 2, gc / 2, (1 - gc) / 2]
+    seqs = [r.choice(4, size=int(k), p=p).astype(np.uint8) for k in lens]
+    placed = []
+    if cm is not None:
+        for k in range(n_hom):
+            s = int(r.integers(0, n))
+            h = S.emit(cm, seed * 100000 + k)
+            strand = int(r.integers(0, 2))
+            if strand:
+                h = np.where(h < 4, 3 - h, h)[::-1]
+            at = int(r.integers(0, len(seqs[s]) + 1))
+            seqs[s] = np.concatenate([seqs[s][:at], h.astype(np.uint8), seqs[s][at:]])
+            placed = [(q, a + len(h) if q == s and a > at else a, b + len(h) if q == s and a > at else b, sd)
+                      for q, a, b, sd in placed]
+            placed.append((s, at + 1, at + len(h), strand))
+    dsqs = [np.concatenate([[255], x, [255]]).astype(np.uint8) for x in seqs]
+    return dsqs, placed
 
 
 def target_with_homologs(cm, L, n, seed, gc=0.5):

@@ -1,16 +1,19 @@
-"""Calibrates the E-value parameters of the config-1 model
-(infernox_amd.models.config1_models) with the CPU oracle, in the manner of
+"""Calibrates the E-value parameters of a workload model
+(infernox_amd.models.config1_models, the 72-column cloverleaf; or
+config3_models, the 120-column three-hairpin CM: argument "1" or "3") with
+the CPU oracle, in the manner of
 HMMER's p7_Calibrate and of cmcalibrate's exponential tails, with lambda
 fixed at ln 2:
 
 * p7 local MSV and Viterbi Gumbel mu by maximum likelihood (known lambda)
   over 200 iid sequences of length 200;
-* p7 local Forward and glocal Forward exponential tau from the 4% tail of
-  200 iid sequences of length 100;
+* p7 local Forward exponential tau from the 5.5% tail of 200 iid sequences
+  of length 100, glocal Forward tau from the 6.5% tail of 200 of length
+  max(100, 2 clen) (the lengths and tail masses of cm_p7_calibrate.rs:273-283);
 * CM local CYK and Inside tails: the best root score of 200 iid windows of
   length 2W (non-banded scans), mu from their 4% tail.
 
-Prints the two dict literals that infernox_amd/models.py holds."""
+Prints the two dict literals that infernox_amd/models.py holds for it."""
 import math
 import os
 import sys
@@ -38,10 +41,13 @@ def tail_mu(x, frac=0.04):
 
 
 def main():
+    which = sys.argv[1] if len(sys.argv) > 1 else "1"
+    build = {"1": models.config1_models, "3": models.config3_models}[which]
     zero = dict(lmmu=0, lmlambda=LAM, lvmu=0, lvlambda=LAM, lftau=0, lflambda=LAM, gfmu=0, gflambda=LAM)
-    p7, cm, cp9, _ = models.config1_models(ev=zero)
+    p7, cm, cp9, _ = build(ev=zero)
     o = Oracle(p7, 200)
-    ge = OracleEnv(p7, 100)
+    gl = max(100, 2 * int(cm["clen"]))
+    ge = OracleEnv(p7, gl)
     r = np.random.default_rng(1001)
     ms, vs, fs, gs = [], [], [], []
     for _ in range(200):
@@ -50,11 +56,12 @@ def main():
         vs.append((o.vit_scalar(d, 200) - null1(200)) / LAM)
         d2 = np.concatenate([[255], r.integers(0, 4, 100), [255]]).astype(np.uint8)
         fs.append((o.fwd(d2, 100) - null1(100)) / LAM)
-        ge.config(100)
-        gs.append((ge.gforward(d2, 100)[0] - null1(100)) / LAM)
+        d3 = np.concatenate([[255], r.integers(0, 4, gl), [255]]).astype(np.uint8)
+        ge.config(gl)
+        gs.append((ge.gforward(d3, gl)[0] - null1(gl)) / LAM)
     ev = dict(lmmu=round(-math.log(np.mean(np.exp(-LAM * np.array(ms)))) / LAM, 4), lmlambda=round(LAM, 6),
               lvmu=round(-math.log(np.mean(np.exp(-LAM * np.array(vs)))) / LAM, 4), lvlambda=round(LAM, 6),
-              lftau=round(tail_mu(fs), 4), lflambda=round(LAM, 6), gfmu=round(tail_mu(gs), 4),
+              lftau=round(tail_mu(fs, 0.055), 4), lflambda=round(LAM, 6), gfmu=round(tail_mu(gs, 0.065), 4),
               gflambda=round(LAM, 6))
     q = OracleQdbCM(cm, local=True)
     W = int(cm["W"])
@@ -67,8 +74,8 @@ def main():
         _, root = q.scan(d, 1, n, 1, want_root=True)
         ins.append(float(root.max()))
     cm_ev = dict(lc=(round(tail_mu(cyk), 4), round(LAM, 6)), li=(round(tail_mu(ins), 4), round(LAM, 6)))
-    print("CONFIG1_EV =", ev)
-    print("CONFIG1_CM_EV =", cm_ev)
+    print(f"CONFIG{which}_EV =", ev)
+    print(f"CONFIG{which}_CM_EV =", cm_ev)
 
 
 if __name__ == "__main__":
