@@ -24,15 +24,22 @@ constexpr int GFB_WAVES = 4;
 enum { T_MM = 0, T_IM = 1, T_DM = 2, T_BM = 3, T_MD = 4, T_DD = 5, T_MI = 6, T_II = 7 };
 enum { XS_EL = 0, XS_EM = 1, XS_NL = 2, XS_NM = 3, XS_JL = 4, XS_JM = 5, XS_CL = 6, XS_CM = 7 };
 
-// FLogsum in nats (p7_generic.rs FLogsum), branch-free: the table is read at
-// index 0 when the smaller term is -inf or 15.7 or more below the larger, so
-// the lookup never waits on a branch.
+// FLogsum in nats (p7_generic.rs FLogsum): max + T[(int)(diff * 1000)] when
+// the smaller term is finite and diff < 15.7, else the max. Branch-free with
+// a short dependency chain (it is the serial step of the D recurrences): the
+// LDS copy of the table holds 0 from index LSUM_CUT on and the index is
+// clamped to LSUM_CUT, so every case that returns the max reads a 0:
+//  * diff >= 15.7f gives diff * 1000 >= 15700 after float rounding (and
+//    diff < 15.7f gives at most 15699.999, index 15699, as the reference);
+//  * a -inf smaller term gives diff = +inf, converted to INT_MAX, clamped;
+//  * both -inf gives NaN, converted to 0: -inf + T[0] = -inf = the max.
+// |a - b| is exactly mx - mn (IEEE subtraction is sign-symmetric), and the
+// sign of a zero max never matters (equal terms always take the table).
+constexpr int LSUM_CUT = 15700;
 __device__ __forceinline__ float lsum(const float *tbl, float a, float b) {
-  const float mx = a > b ? a : b, mn = a > b ? b : a;
-  const float diff = mx - mn;
-  const bool near = !(mn == -INFINITY) && diff < 15.7f;
-  const float tv = tbl[near ? (int)(diff * 1000.0f) : 0];
-  return near ? mx + tv : mx;
+  const float mx = fmaxf(a, b);
+  const int idx = min((int)(fabsf(a - b) * 1000.0f), LSUM_CUT);
+  return mx + tbl[idx];
 }
 
 // Orders this wave's LDS writes before its later LDS reads (cross-lane).
@@ -62,7 +69,7 @@ __device__ GfbShared gfb_stage(const GmDev &g, float *smem) {
   s.dd = s.md + (M + 1);
   s.bm = s.dd + (M + 1);
   s.wave = s.bm + (M + 1);
-  for (int t = threadIdx.x; t < GFB_TBL; t += blockDim.x) s.tbl[t] = g.tbl[t];
+  for (int t = threadIdx.x; t < GFB_TBL; t += blockDim.x) s.tbl[t] = t < LSUM_CUT ? g.tbl[t] : 0.0f;
   for (int k = threadIdx.x; k <= M; k += blockDim.x) {
     s.md[k] = g.tsc[k * 8 + T_MD];
     s.dd[k] = g.tsc[k * 8 + T_DD];
