@@ -276,13 +276,13 @@ struct Cp9WinDev {
 };
 // Dynamic LDS of the CP9 kernels: two local-end rows, the two serial chains'
 // terms, a transition column, the ILogsum table as 16-bit entries.
-inline size_t cp9_lds_bytes(int M, int nz) { return (size_t)5 * (M + 1) * 4 + (((size_t)nz * 2 + 15) & ~(size_t)15); }
+inline size_t cp9_lds_bytes(int M, int nz) { return (size_t)5 * (M + 1) * 4 + (((size_t)(nz + 1) * 2 + 15) & ~(size_t)15); }
 // Dynamic LDS of the wavefront kernels: nslots slots of 65 lanes, two
 // local-end rows and two match rows, the packed node words, the nlist entries of the local-end lists (node and slot,
-// 16 bits each), the ILogsum table.
+// 16 bits each), the ILogsum table (nz entries and a zero).
 inline size_t cp9_wave_lds_bytes(int M, int nz, int nslots, int nlist) {
   return (size_t)nslots * 65 * 4 + (size_t)4 * (M + 1) * 4 + (size_t)2 * (M + 2) * 4 + (size_t)nlist * 4 +
-         (((size_t)nz * 2 + 15) & ~(size_t)15);
+         (((size_t)(nz + 1) * 2 + 15) & ~(size_t)15);
 }
 constexpr size_t CP9_LDS_MAX = 65536 - 64;
 hipError_t launch_cp9_bands(const Cp9Dev &h, const Cp9WinDev *wins, int64_t n, int mode, int32_t thresh,

@@ -33,14 +33,17 @@ enum { O_MM = 0, O_IM, O_DM, O_BM, O_MI, O_II, O_DI, O_MD, O_ID, O_DD, O_ME, O_M
 __device__ __forceinline__ int32_t wa(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
 __device__ __forceinline__ int32_t ws(int32_t a, int32_t b) { return (int32_t)((uint32_t)a - (uint32_t)b); }
 
-// ILogsum (cp9.rs:65-74); entries past nz are zero, so max + tbl[diff] = max
-// there.
+// ILogsum (cp9.rs:65-74): max + T[max - min] when the smaller term is above
+// -INFTY, else the max. The reference's table is zero from nz on; the LDS
+// copy holds entries 0..nz (T[nz] = 0) and the index is clamped to nz, and
+// to nz as well when the smaller term is at or below -INFTY, so every case
+// that returns the max reads a 0 (a short dependency chain: this is the
+// serial step of the delete chains).
 __device__ __forceinline__ int32_t lsum(int32_t a, int32_t b, const uint16_t *tbl, int nz) {
   const int32_t mx = max(NINF, max(a, b)), mn = min(a, b);
   const uint32_t diff = (uint32_t)mx - (uint32_t)mn;
-  const bool use = mn > NINF && diff < (uint32_t)nz;
-  const int32_t t = tbl[use ? diff : 0];
-  return use ? wa(mx, t) : mx;
+  const uint32_t idx = mn > NINF ? min(diff, (uint32_t)nz) : (uint32_t)nz;
+  return wa(mx, (int32_t)tbl[idx]);
 }
 
 __device__ __forceinline__ float scorify(int32_t sc) { return (float)sc / 1000.0f; }  // cp9.rs:1257
@@ -55,7 +58,7 @@ __device__ __forceinline__ Lds lds_setup(const Cp9Dev &h, int zcol, int zshift) 
   const int M1 = h.M + 1;
   Lds s{lds, lds + 2 * M1, lds + 3 * M1, lds + 4 * M1, (uint16_t *)(lds + 5 * M1)};
   const IXG_GLOBAL int32_t *ilog = gmem(h.ilog), *otsc = gmem(h.otsc);
-  for (int x = threadIdx.x; x < h.nz; x += 64) s.tbl[x] = (uint16_t)ilog[x];
+  for (int x = threadIdx.x; x <= h.nz; x += 64) s.tbl[x] = x < h.nz ? (uint16_t)ilog[x] : (uint16_t)0;
   // z[k] = the transition zcol of node k - zshift
   for (int k = threadIdx.x; k <= h.M; k += 64) s.z[k] = k >= zshift ? otsc[(size_t)(k - zshift) * O_NT + zcol] : 0;
   return s;
@@ -455,7 +458,7 @@ __device__ __forceinline__ WaveLds wave_lds(const Cp9Dev &h, int nslots, const i
   const IXG_GLOBAL int32_t *ilog = gmem(h.ilog), *gn = gmem(node), *gs = gmem(slot);
   const IXG_GLOBAL uint32_t *gi = gmem(info);
   for (int x = threadIdx.x; x < 2 * (h.M + 2); x += 64) s.info[x] = gi[x];
-  for (int x = threadIdx.x; x < h.nz; x += 64) s.tbl[x] = (uint16_t)ilog[x];
+  for (int x = threadIdx.x; x <= h.nz; x += 64) s.tbl[x] = x < h.nz ? (uint16_t)ilog[x] : (uint16_t)0;
   for (int x = threadIdx.x; x < nlist; x += 64) {
     s.lnode[x] = (int16_t)gn[x];
     s.lslot[x] = (int16_t)gs[x];
