@@ -42,6 +42,13 @@ VALU_PEAK_TOPS = 256 * 4 * 16 * 2.4e9 / 1e12  # 256 CUs x 4 SIMD16 x 2.4 GHz, on
 # profiles/hbm_traffic_p7_cascade_genome4.6Mb_M300.md. They hold for the
 # default workload only and are reported as measured, not recomputed.
 TRAFFIC = {"ssv_sweep": 12360715, "ssv_resolve": 6054781, "fwd": 6628113, "bias": 189719}
+# The same for the headline search step's two largest kernels, per launch:
+# the HB scans (one CYK and one Inside launch per step, their mean: CYK 26940
+# KiB FETCH_SIZE / 40299 KiB WRITE_SIZE, Inside 2525 / 3658 KiB) and the glocal
+# Forward/Backward (mean of its two launches, 364 / 294 KiB), with the factors
+# above; profiles/search_config1_1Mb.md.
+SEARCH_TRAFFIC = {"hb_scan": int(((26940 + 2525) * 2.0 + (40299 + 3658) * 1.0) * 1024 / 2),
+                  "gfb": int((364 * 2.0 + 294 * 1.0) * 1024)}
 
 
 def workload():
@@ -194,7 +201,7 @@ def time_search(ctx, steps, warmup, wl):
                 "why_two_peaks": "the published 157.3 TFLOP/s counts packed FP32 FMA (2 flops x 2 lanes per "
                                  "lane-cycle); the max-plus / log-sum DP ops are single unpacked non-FMA lane "
                                  "ops, built without packed FP32, so 39.3 T lane-op/s is their ceiling",
-                "traffic": None}
+                "traffic": SEARCH_TRAFFIC.get(dom)}
     out = {"value": acct.nres / dt / 1e6, "ms_per_step": dt * 1e3, "residues": int(acct.nres),
            "stage_ms": st, "kernels": kern, "roofline": roof,
            "cyk_gcups": cyk_cells / (st["f6_cyk"] * 1e-3) / 1e9 if st["f6_cyk"] > 0 else None,
@@ -978,7 +985,17 @@ def time_batched(ctx, steps, cpu=None, per_seq_s=4.0):
     tgt.free()
     found = sum(any(h[7] == s and min(h[0], h[1]) <= e and max(h[0], h[1]) >= b and h[2] == sd for h in hits)
                 for s, b, e, sd in placed)
-    # the plain form: one call per sequence, leading sequences, about per_seq_s
+    out = {}
+    if per_seq_s > 0:
+        out["per_sequence_calls"] = per_sequence_calls(ctx, g7, gcm, g9, hb, tr9, trhb, seqs, opts, hits, per_seq_s)
+    return finish_batched(out, acct, dt, seqs, cm, W, st, kt, steps, placed, found, cpu, p7, cp9, trd, opts,
+                          [g9, gcm, g7, hb] + tr9 + trhb)
+
+
+def per_sequence_calls(ctx, g7, gcm, g9, hb, tr9, trhb, seqs, opts, hits, per_seq_s):
+    """The plain form: one ixg_search per sequence (each its own target) over
+    the leading sequences for about per_seq_s seconds; its rate and whether
+    its hits are the batched call's hits on those sequences."""
     singles = []
     for k in range(min(len(seqs), 4000)):
         singles.append(ixg.Target(ctx, seqs[k], len(seqs[k]) - 2))
@@ -996,6 +1013,11 @@ def time_batched(ctx, steps, cpu=None, per_seq_s=4.0):
     sub = [h for h in hits if h[7] < k]
     key = lambda h: (h[5], -h[3], h[7], h[0], -h[6], h[1])  # noqa: E731  (the E-value sort's keys)
     same = [h[:3] + h[6:] for h in sub] == [h[:3] + h[6:] for h in sorted(per, key=key)]
+    return {"value": nres / dts / 1e6, "unit": "Mres/s", "sequences": k, "ms_per_call": dts / k * 1e3,
+            "same_hits_as_batched": bool(same)}
+
+
+def finish_batched(extra, acct, dt, seqs, cm, W, st, kt, steps, placed, found, cpu, p7, cp9, trd, opts, handles):
     out = {"value": acct.nres / dt / 1e6, "unit": "Mres/s", "ms_per_step": dt * 1e3,
            "workload": "10,000 iid sequences, 45% GC, log-normal lengths (median 2 kb, sigma 1.0, clipped to "
                        "200 nt-200 kb, seed 4), 100 CM-sampled homologs inserted into random sequences; "
@@ -1010,10 +1032,10 @@ def time_batched(ctx, steps, cpu=None, per_seq_s=4.0):
                          "f3b": int(acct.f.n_past_fwdbias), "f5_envelopes": int(acct.e.n_past_edef),
                          "f6": int(acct.n_past_cyk), "hits": int(acct.n_hits),
                          "truncated_pass_hits": [int(x) for x in acct.tr_hits]},
-           "planted_found": f"{found} of {len(placed)}",
-           "per_sequence_calls": {"value": nres / dts / 1e6, "unit": "Mres/s", "sequences": k,
-                                  "ms_per_call": dts / k * 1e3, "same_hits_as_batched": bool(same)}}
-    out["batched_over_per_sequence"] = out["value"] / out["per_sequence_calls"]["value"]
+           "planted_found": f"{found} of {len(placed)}"}
+    out.update(extra)
+    if "per_sequence_calls" in out:
+        out["batched_over_per_sequence"] = out["value"] / out["per_sequence_calls"]["value"]
     if cpu:
         sys.path.insert(0, os.path.join(ROOT, "tests"))
         from oracle_pipe_bind import OracleSearch
@@ -1025,7 +1047,7 @@ def time_batched(ctx, steps, cpu=None, per_seq_s=4.0):
         out["cpu"] = {"value": oa.nres / dtc / 1e6, "unit": "Mres/s", "cores": threads, "kind": "port",
                       "sample": f"the first {n} sequences ({oa.nres} residues searched), oracle "
                                 f"ixo_search_multi_mt (C restatement, not infernox), {dtc:.1f} s"}
-    for x in [g9, gcm, g7, hb] + tr9 + trhb:
+    for x in handles:
         x.free()
     return out
 
@@ -1058,6 +1080,8 @@ def main():
     ap.add_argument("--multi-only", action="store_true", help="only run the several-CM HB launch section")
     ap.add_argument("--batched-only", action="store_true",
                     help="only run the batched uneven-database search section (10,000 sequences)")
+    ap.add_argument("--per-seq-seconds", type=float, default=4.0,
+                    help="time budget of the batched section's one-call-per-sequence comparison (0: skip it)")
     ap.add_argument("--hb-only", action="store_true",
                     help="only run the HB CYK / Inside section (959-state CM, 512 envelopes)")
     ap.add_argument("--copy-probe", action="store_true",
@@ -1088,7 +1112,8 @@ def main():
     if a.batched_only:
         with ixg.Context(0) as ctx:
             cpu = None if a.no_cpu else (a.cpu_section_seconds, a.cpu_threads)
-            print(json.dumps({"batched_uneven_search": time_batched(ctx, max(1, a.steps // 3), cpu)}))
+            print(json.dumps({"batched_uneven_search": time_batched(ctx, max(1, a.steps // 3), cpu,
+                                                                    a.per_seq_seconds)}))
         return
     if a.serve_only:
         with ixg.Context(0) as ctx:
@@ -1113,7 +1138,7 @@ def main():
             p, d, _, _, _, _ = workload()
             extra["p7_forward_full"] = time_p7_forward(ctx, p, d, 2, cpu=cpu)
             extra["hb_multi"] = time_multi(ctx)
-            extra["batched_uneven_search"] = time_batched(ctx, max(1, a.steps // 3), cpu)
+            extra["batched_uneven_search"] = time_batched(ctx, max(1, a.steps // 3), cpu, a.per_seq_seconds)
         if secondary and not a.no_serve:
             extra["serving"] = time_serving(ctx)
     p7, cm = wl[0], wl[1]

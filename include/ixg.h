@@ -598,8 +598,9 @@ int ixg_search(ixg_ctx *ctx, const ixg_p7 *p7, const ixg_cm *cm, const ixg_cp9 *
  * host work) and the HB cells its STD CYK (cells[0]) and Inside (cells[1])
  * launches computed. The truncated passes' CM stages run concurrently with
  * the STD pass's F6 / F7 (each pass on an auxiliary context's stream):
- * IXG_SS_TRUNC is their wall time from start to join, and IXG_SS_FINALIZE
- * includes what of it was left after F7. A measurement hook with no
+ * IXG_SS_TRUNC is the longest pass's wall time on its thread (0 when no
+ * truncated pass had envelopes), and IXG_SS_FINALIZE includes what of it
+ * was left after F7. A measurement hook with no
  * counterpart in the reference. */
 enum { IXG_SS_F1_F3B = 0, IXG_SS_F4_F5 = 1, IXG_SS_F6_BANDS = 2, IXG_SS_F6_CYK = 3, IXG_SS_F7_BANDS = 4,
        IXG_SS_F7_INSIDE = 5, IXG_SS_FINALIZE = 6, IXG_SS_TOTAL = 7, IXG_SS_TRUNC = 8, IXG_SS_N = 9 };

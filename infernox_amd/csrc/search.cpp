@@ -405,6 +405,7 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
     uint64_t n_past_cyk = 0, pos_past_cyk = 0, nhits = 0;
     int rc = IXG_OK;
     std::string err;
+    double ms = 0;  // the pass's wall time on its thread
   };
   auto trunc_pass = [&](ixg_ctx *cc, int q, TrOut &out) -> int {
     HIPCHK(hipSetDevice(cc->device));
@@ -505,7 +506,6 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
     if (!c->aux[q]) RCHK(ixg_init(c->device, c->budget, &c->aux[q]));
     trq.push_back(q);
   }
-  const clk::time_point t_tr = clk::now();
   struct Joiner {  // every return path waits for the threads before the state they read goes away
     std::vector<std::thread> &t;
     ~Joiner() {
@@ -515,8 +515,10 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
   } joiner{trth};
   for (int q : trq)
     trth.emplace_back([&, q] {
+      const clk::time_point t0 = clk::now();
       tro[q].rc = trunc_pass(c->aux[q], q, tro[q]);
       if (tro[q].rc != IXG_OK) tro[q].err = g_err;
+      tro[q].ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
     });
   auto join_trunc = [&]() -> int {
     for (auto &t : trth) t.join();
@@ -641,7 +643,7 @@ extern "C" int ixg_search(ixg_ctx *c, const ixg_p7 *p, const ixg_cm *cm, const i
   }
   {
     const int trc = join_trunc();
-    c->search_ms[IXG_SS_TRUNC] = std::chrono::duration<double, std::milli>(clk::now() - t_tr).count();
+    for (int q : trq) c->search_ms[IXG_SS_TRUNC] = std::max(c->search_ms[IXG_SS_TRUNC], tro[q].ms);
     stage(IXG_SS_FINALIZE);
     RCHK(trc);
   }
