@@ -656,6 +656,28 @@ int ixg_server_wait(ixg_server *s, uint64_t ticket, int64_t timeout_us, int *don
  * batch's status is returned for each of its requests. */
 int ixg_server_result(ixg_server *s, uint64_t ticket, ixg_task *tasks, int64_t tcap, int64_t *ntasks,
                       ixg_window *wins, int64_t wcap, int64_t *nwins, ixg_acct *acct, double *times_ms);
+/* A search request: the whole search (ixg_search) of dsq[1..L] against one
+ * resident model bundle with the caller's options; o->ez and the thresholds
+ * are the caller's (as cmsearch -Z fixes the E-value scale), so a request's
+ * result does not depend on what it is batched with. Queued requests for
+ * the same bundle and options (compared bytewise) run as one ixg_search over
+ * a multi-sequence target of their sequences. A served search of cmsearch
+ * (cm_search.rs:1300) with no counterpart in the reference's control flow. */
+typedef struct {
+  const ixg_p7 *p7;
+  const ixg_cm *cm;
+  const ixg_cp9 *cp9;
+  const ixg_hbmap *map;
+  const ixg_cp9 *trcp9[3];   /* the truncated passes' (used when o->trunc_mode is 1) */
+  const ixg_hbmap *trmap[3];
+} ixg_search_model;
+int ixg_server_submit_search(ixg_server *s, const ixg_search_model *m, const uint8_t *dsq, int64_t L,
+                             const ixg_searchopts *o, uint64_t *ticket);
+/* Waits for a search request, then writes its hits (ixg_search's for the
+ * sequence alone, seq = 0) and times_ms[4] as ixg_server_result, and retires
+ * the ticket; IXG_ERANGE (ticket kept) if cap is too small. */
+int ixg_server_search_result(ixg_server *s, uint64_t ticket, ixg_searchhit *hits, int64_t cap, int64_t *n,
+                             double *times_ms);
 /* stats4 = {batches, requests, largest batch in requests, residues searched}. */
 int ixg_server_stats(ixg_server *s, int64_t *stats4);
 /* Runs every queued request to completion, then stops the workers. */

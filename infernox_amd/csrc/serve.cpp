@@ -9,6 +9,14 @@
 // ixg_window_tasks + ixg_f3_filter_tasks give for the sequence alone
 // (windows per task, F1F3Acct per task), bit for bit.
 //
+// A search request (ixg_server_submit_search) is a whole search of one
+// sequence against one resident model bundle (p7, CM, CP9s, band maps) with
+// the caller's options (thresholds and E-value scale fixed by the caller, as
+// cmsearch -Z fixes them): a batch of such requests for the same bundle and
+// options is one ixg_search over a multi-sequence target of their
+// sequences, and each request's hits are the batch's hits on its sequence,
+// in the same order: what ixg_search gives for the sequence alone.
+//
 // Scheduling. Requests queue in arrival order. Each worker owns a context
 // (its own stream and grow-only scratch) and, when idle, takes the oldest
 // queued request together with every later queued request for the same model
@@ -46,8 +54,12 @@ using Clock = std::chrono::steady_clock;
 
 struct Req {
   uint64_t ticket = 0;
+  int kind = 0;  // 0: filter stage, 1: whole search
   const ixg_p7 *p7 = nullptr;
   ixg_f3opts o{};
+  ixg_search_model m{};
+  ixg_searchopts so{};
+  std::vector<ixg_searchhit> hits;
   int64_t L = 0;
   std::vector<uint8_t> seq;  // the L residue codes
   int state = 0;             // 0 queued, 1 running, 2 done
@@ -65,6 +77,10 @@ bool codes_valid(const uint8_t *p, int64_t n) {
   for (int64_t i = 0; i < n; i++) acc |= (uint8_t)(p[i] >= IXG_KP);
   if (acc) g_err = "digitized residue code >= 18 in the sequence";
   return !acc;
+}
+
+bool same_search(const Req &a, const Req &b) {  // same bundle and options (compared bytewise)
+  return !memcmp(&a.m, &b.m, sizeof(a.m)) && !memcmp(&a.so, &b.so, sizeof(a.so));
 }
 
 bool same_opts(const ixg_f3opts &a, const ixg_f3opts &b) {
@@ -102,7 +118,9 @@ struct ixg_server {
     res = head->L;
     for (auto it = queue.begin() + 1; it != queue.end(); ++it) {
       Req *q = *it;
-      if (q->p7 != head->p7 || !same_opts(q->o, head->o)) continue;
+      if (q->kind != head->kind) continue;
+      if (head->kind == 0 && (q->p7 != head->p7 || !same_opts(q->o, head->o))) continue;
+      if (head->kind == 1 && !same_search(*q, *head)) continue;
       if (res + q->L > max_batch || inflight + res + q->L > max_inflight) continue;
       b.push_back(q);
       res += q->L;
@@ -111,7 +129,46 @@ struct ixg_server {
     return b;
   }
 
+  // A batch of search requests: one ixg_search over a target holding their
+  // sequences; hits go back to their request (sequence index -> 0).
+  static int run_search_batch(ixg_ctx *c, std::vector<Req *> &b) {
+    std::vector<const uint8_t *> ptr(b.size());
+    std::vector<std::vector<uint8_t>> pad(b.size());
+    std::vector<int64_t> len(b.size());
+    for (size_t r = 0; r < b.size(); r++) {
+      pad[r].assign(b[r]->L + 2, 255);
+      if (b[r]->L) memcpy(pad[r].data() + 1, b[r]->seq.data(), b[r]->L);
+      ptr[r] = pad[r].data();
+      len[r] = b[r]->L;
+      b[r]->hits.clear();
+    }
+    ixg_target *t = nullptr;
+    RCHK(ixg_target_create_multi(c, ptr.data(), len.data(), (int64_t)b.size(), &t));
+    const Req &h = *b[0];
+    std::vector<ixg_searchhit> hits(256);
+    int64_t n = 0;
+    int rc = ixg_search(c, h.m.p7, h.m.cm, h.m.cp9, h.m.map, h.m.trcp9, h.m.trmap, t, &h.so, hits.data(),
+                        (int64_t)hits.size(), &n, nullptr);
+    if (rc == IXG_ERANGE) {
+      hits.resize(n);
+      rc = ixg_search(c, h.m.p7, h.m.cm, h.m.cp9, h.m.map, h.m.trcp9, h.m.trmap, t, &h.so, hits.data(), n, &n,
+                      nullptr);
+    }
+    const std::string err = g_err;
+    ixg_target_free(c, t);
+    g_err = err;
+    RCHK(rc);
+    hits.resize(n);
+    for (ixg_searchhit x : hits) {
+      Req &q = *b[x.seq];
+      x.seq = 0;
+      q.hits.push_back(x);
+    }
+    return IXG_OK;
+  }
+
   static int run_batch(ixg_ctx *c, std::vector<Req *> &b) {
+    if (b[0]->kind == 1) return run_search_batch(c, b);
     // layout per request: [255, seq, 255][255, revcomp(seq), 255]
     int64_t tot = 0;
     std::vector<int64_t> off(b.size());
@@ -246,6 +303,62 @@ int ixg_server_submit(ixg_server *s, const ixg_p7 *p7, const uint8_t *dsq, int64
   return IXG_OK;
 }
 
+int ixg_server_submit_search(ixg_server *s, const ixg_search_model *m, const uint8_t *dsq, int64_t L,
+                             const ixg_searchopts *o, uint64_t *ticket) {
+  if (!s || !m || !m->p7 || !m->cm || !m->cp9 || !m->map || !dsq || L < 0 || !o || !ticket || o->f3.maxw < 1 ||
+      o->f3.cmw < 1 || (o->trunc_mode == 1 && (!m->trcp9[0] || !m->trcp9[1] || !m->trcp9[2] || !m->trmap[0] ||
+                                               !m->trmap[1] || !m->trmap[2]))) {
+    g_err = "ixg_server_submit_search: bad arguments";
+    return IXG_EINVAL;
+  }
+  if (!codes_valid(dsq + 1, L)) return IXG_EINVAL;
+  auto q = std::make_unique<Req>();
+  q->kind = 1;
+  q->m = *m;
+  q->so = *o;
+  q->L = L;
+  q->seq.assign(dsq + 1, dsq + 1 + L);
+  q->t_sub = Clock::now();
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->stop) return IXG_EINVAL;
+  q->ticket = s->next++;
+  *ticket = q->ticket;
+  s->queue.push_back(q.get());
+  s->reqs.emplace(q->ticket, std::move(q));
+  s->cv_work.notify_one();
+  return IXG_OK;
+}
+
+int ixg_server_search_result(ixg_server *s, uint64_t ticket, ixg_searchhit *hits, int64_t cap, int64_t *n,
+                             double *times_ms) {
+  if (!s || !n || (cap > 0 && !hits)) return IXG_EINVAL;
+  std::unique_lock<std::mutex> lk(s->mu);
+  auto it = s->reqs.find(ticket);
+  if (it == s->reqs.end() || it->second->kind != 1) {
+    g_err = "ixg_server_search_result: no such search ticket";
+    return IXG_EINVAL;
+  }
+  Req *q = it->second.get();
+  s->cv_done.wait(lk, [&] { return q->state == 2; });
+  if (q->rc != IXG_OK) {
+    g_err = q->err;
+    const int rc = q->rc;
+    s->reqs.erase(it);
+    return rc;
+  }
+  *n = (int64_t)q->hits.size();
+  if (*n > cap) return IXG_ERANGE;  // ticket kept: ask again with room
+  if (*n) memcpy(hits, q->hits.data(), q->hits.size() * sizeof(ixg_searchhit));
+  if (times_ms) {
+    times_ms[0] = ms_between(q->t_sub, q->t_run);
+    times_ms[1] = ms_between(q->t_run, q->t_done);
+    times_ms[2] = ms_between(q->t_sub, q->t_done);
+    times_ms[3] = (double)q->batch_size;
+  }
+  s->reqs.erase(it);
+  return IXG_OK;
+}
+
 int ixg_server_wait(ixg_server *s, uint64_t ticket, int64_t timeout_us, int *done) {
   if (!s || !done) return IXG_EINVAL;
   std::unique_lock<std::mutex> lk(s->mu);
@@ -265,7 +378,7 @@ int ixg_server_result(ixg_server *s, uint64_t ticket, ixg_task *tasks, int64_t t
   if (!s || !ntasks || !nwins) return IXG_EINVAL;
   std::unique_lock<std::mutex> lk(s->mu);
   auto it = s->reqs.find(ticket);
-  if (it == s->reqs.end()) return IXG_EINVAL;
+  if (it == s->reqs.end() || it->second->kind != 0) return IXG_EINVAL;
   Req *q = it->second.get();
   s->cv_done.wait(lk, [&] { return q->state == 2; });
   if (q->rc != IXG_OK) {

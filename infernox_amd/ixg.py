@@ -511,6 +511,34 @@ class Server:
         tl = [(tasks[i].start, tasks[i].end, tasks[i].strand) for i in range(nt.value)]
         return tl, w, [accts[i] for i in range(nt.value)], tuple(times)
 
+    def submit_search(self, bundle, dsq, L, opts):
+        """A whole-search request against a SearchModel bundle (ixg_server_submit_search)."""
+        d = np.ascontiguousarray(dsq, np.uint8)
+        t = C.c_uint64()
+        L_ = _search_lib()
+        L_.ixg_server_submit_search.argtypes = [C.c_void_p, C.POINTER(SearchModel.S), C.POINTER(C.c_uint8), C.c_int64,
+                                                C.POINTER(SearchOpts), C.POINTER(C.c_uint64)]
+        check(L_.ixg_server_submit_search(self.h, C.byref(bundle.s), u8ptr(d), int(L), C.byref(opts), C.byref(t)),
+              "ixg_server_submit_search")
+        return t.value
+
+    def search_result(self, ticket, cap=64):
+        """([(start, stop, strand, score, bias, evalue, pass)], times_ms) of a search request."""
+        L_ = _search_lib()
+        L_.ixg_server_search_result.argtypes = [C.c_void_p, C.c_uint64, C.POINTER(SearchHit), C.c_int64,
+                                                C.POINTER(C.c_int64), C.POINTER(C.c_double)]
+        n = C.c_int64()
+        while True:
+            out, times = (SearchHit * max(cap, 1))(), (C.c_double * 4)()
+            rc = L_.ixg_server_search_result(self.h, ticket, out, cap, C.byref(n), times)
+            if rc == IXG_ERANGE:
+                cap = n.value
+                continue
+            check(rc, "ixg_server_search_result")
+            break
+        return [(out[i].start, out[i].stop, out[i].strand, out[i].score, out[i].bias, out[i].evalue,
+                 out[i].pass_idx) for i in range(n.value)], tuple(times)
+
     def stats(self):
         s = (C.c_int64 * 4)()
         check(lib().ixg_server_stats(self.h, s), "ixg_server_stats")
@@ -1213,6 +1241,25 @@ def search(ctx, p7, cm, cp9, hbmap, trcp9, trmap, tgt, opts, with_seq=False):
     return _search_call(lambda out, cap, n, acct: L.ixg_search(ctx.h, p7.h, cm.h, cp9.h, hbmap.h, tc, tm, tgt.h,
                                                                C.byref(opts), out, cap, C.byref(n), C.byref(acct)),
                         "ixg_search", with_pass=True, with_seq=with_seq)
+
+
+class SearchModel:
+    """A model bundle for served searches (ixg_search_model): the handles stay
+    owned by the caller and must outlive the server."""
+
+    class S(C.Structure):
+        _fields_ = [("p7", C.c_void_p), ("cm", C.c_void_p), ("cp9", C.c_void_p), ("map", C.c_void_p),
+                    ("trcp9", C.c_void_p * 3), ("trmap", C.c_void_p * 3)]
+
+    def __init__(self, p7, cm, cp9, hbmap, trcp9=None, trmap=None):
+        def hv(x):
+            return x.h.value if hasattr(x.h, "value") else x.h
+        self.s = SearchModel.S()
+        self.s.p7, self.s.cm, self.s.cp9, self.s.map = hv(p7), hv(cm), hv(cp9), hv(hbmap)
+        for q in range(3):
+            if trcp9:
+                self.s.trcp9[q] = hv(trcp9[q])
+                self.s.trmap[q] = hv(trmap[q])
 
 
 SEARCH_STAGES = ("f1_f3b", "f4_f5", "f6_bands", "f6_cyk", "f7_bands", "f7_inside", "finalize", "total",

@@ -128,3 +128,90 @@ def test_served_small_inflight_budget_serializes(ctx):
     finally:
         srv.free()
         models[0][1].free()
+
+
+def _bundle(ctx, which):
+    """A resident search bundle of a workload model: (SearchModel, opts,
+    handles to free, cm dict)."""
+    from infernox_amd import models, synth_cp9
+    p7, cm, cp9, cmev = models.config1_models() if which == 1 else models.config3_models()
+    W = int(cm["W"])
+    g7 = ixg.P7(ctx, p7, W)
+    gcm = ixg.CM(ctx, cm)
+    gcm.set_trpenalties(*ixg.cm_trpenalties(cm)[:2])
+    g9 = ixg.CP9(ctx, cp9)
+    hb = ixg.HbMap(cm, cp9)
+    trd = synth_cp9.build_trunc_cp9s(cm, 1)
+    tr9 = [ixg.CP9(ctx, h) for h in trd]
+    trhb = [ixg.HbMap(cm, h) for h in trd]
+    f3 = ixg.F3Opts(f1=0.35, f1b=1.0, f2=1.0, f2b=1.0, f3=0.02, f3b=0.02, cmw=W, maxw=W, opts=7)
+    env = ixg.env_opts(f4=0.02, f4b=0.02, f5=0.02, gfmu=p7["gfmu"], gflambda=p7["gflambda"], lftau=p7["lftau"],
+                       lflambda=p7["lflambda"], clen=int(cm["clen"]))
+    # a fixed E-value scale (a 2 Mb search space), as a server fixes it
+    o = ixg.search_opts(f3, env, 1e-4, cmev["lc"], cmev["li"], 2.0e6 / (2 * W), W=W, trunc=True)
+    return ixg.SearchModel(g7, gcm, g9, hb, tr9, trhb), o, [g7, gcm, g9, hb] + tr9 + trhb, cm
+
+
+def _search_requests(cm, n, seed):
+    """Sequences of 0-20 kb: random, with a CM-sampled homolog inside, or
+    with one cut at an end (the truncated passes)."""
+    from infernox_amd import synth_cm as S
+    r = np.random.default_rng(seed)
+    out = []
+    for k in range(n):
+        L0 = int(np.clip(r.lognormal(np.log(2000), 1.2), 0, 20000)) if k else 0
+        body = r.integers(0, 4, L0).astype(np.uint8)
+        kind = k % 3
+        if kind and L0 > 0:
+            h = S.emit(cm, seed * 1000 + k).astype(np.uint8)
+            if kind == 1:
+                at = int(r.integers(0, L0 + 1))
+                body = np.concatenate([body[:at], h, body[at:]])
+            else:
+                body = np.concatenate([body, h[: max(8, 2 * len(h) // 3)]])
+        out.append(np.concatenate([[255], body, [255]]).astype(np.uint8))
+    return out
+
+
+def test_served_search_equals_alone(ctx):
+    # whole searches (sequence, CM) served in a mix of two models of 230 and
+    # 374 states, in a burst and staggered with results collected while
+    # later requests arrive: every request's hits (coordinates, strand,
+    # float score and bias, E-value, pass, order) equal ixg_search on the
+    # sequence alone
+    bundles = [_bundle(ctx, 1), _bundle(ctx, 3)]
+    reqs = []
+    for b, (bm, o, _, cm) in enumerate(bundles):
+        reqs += [(b, d) for d in _search_requests(cm, 18, 61 + b)]
+    order = np.random.default_rng(7).permutation(len(reqs))
+    want = {}
+    for q in order:
+        b, d = reqs[q]
+        bm, o, hs, _ = bundles[b]
+        t = ixg.Target(ctx, d, len(d) - 2)
+        want[q], _ = ixg.search(ctx, *[x for x in hs[:3]], hs[3], hs[4:7], hs[7:10], t, o)
+        t.free()
+    srv = ixg.Server(ctx, nstreams=3, max_inflight=1 << 22, max_batch=1 << 20)
+    for phase in ("burst", "staggered"):
+        _say("served search", phase, len(reqs))
+        tickets = {}
+        for n_sub, q in enumerate(order):
+            b, d = reqs[q]
+            tickets[q] = srv.submit_search(bundles[b][0], d, len(d) - 2, bundles[b][1])
+            if phase == "staggered" and n_sub % 6 == 5:
+                time.sleep(0.002)
+        batched = 0
+        for q in order:
+            got, times = srv.search_result(tickets[q])
+            batched = max(batched, times[3])
+            assert [h[:3] + h[6:] for h in got] == [h[:3] + h[6:] for h in want[q]], (phase, q)
+            assert np.array_equal(np.float32([h[3] for h in got]), np.float32([h[3] for h in want[q]]))
+            assert np.array_equal(np.float32([h[4] for h in got]), np.float32([h[4] for h in want[q]]))
+            assert [h[5] for h in got] == [h[5] for h in want[q]]
+        if phase == "burst":
+            assert batched > 1  # requests of one model shared a search
+    assert sum(len(v) for v in want.values()) >= 8  # the planted homologs are reported
+    srv.free()
+    for _, _, hs, _ in bundles:
+        for x in hs:
+            x.free()
