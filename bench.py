@@ -873,6 +873,147 @@ def time_serving(ctx, nstreams=4, qps_fracs=(0.5, 1.0, 2.0, 4.0)):
             "sweep": sweep, "server_stats": st}
 
 
+def search_serving_workload(n_req=300, n_models=8, seed=5, median=5000.0, sigma=1.5):
+    """Whole-search requests (sequence, CM): lengths log-normal (median 5 kb,
+    sigma 1.5) clipped to [100, 1e6] nt, iid ACGU with a CM-sampled homolog
+    inserted into every third; models uniform from a pool of n_models CMs
+    with consensus lengths uniform in [50, 400] on random nested structures
+    (40-60% paired, 30-sequence alignments), each with its CP9s and p7 filter
+    (E-value parameters of the cloverleaf model: not calibrated per model)."""
+    from infernox_amd import synth_cm as S
+    rng = np.random.default_rng(seed)
+    pool = []
+    for k in range(n_models):
+        clen = int(rng.integers(50, 401))
+        ss = S.random_nested(clen, seed * 100 + k)
+        cm = models.cm_from_alignment(ss, models.alignment(ss, 30, seed * 100 + k), seed * 100 + k)
+        cp9 = synth_cp9.build_cp9_like(cm, 1, el=True)
+        p7 = models.p7_from_cm(cm, models.CONFIG1_EV)
+        pool.append((p7, cm, cp9, models.CONFIG1_CM_EV))
+    lens = np.clip(np.exp(np.log(median) + sigma * rng.standard_normal(n_req)), 100, 1_000_000).astype(np.int64)
+    mids = rng.integers(0, n_models, n_req)
+    seqs = []
+    for q in range(n_req):
+        body = rng.integers(0, 4, int(lens[q])).astype(np.uint8)
+        if q % 3 == 0:
+            h = S.emit(pool[mids[q]][1], seed * 10000 + q).astype(np.uint8)
+            at = int(rng.integers(0, len(body) + 1))
+            body = np.concatenate([body[:at], h, body[at:]])
+        seqs.append(np.concatenate([[255], body, [255]]).astype(np.uint8))
+    return pool, mids, seqs
+
+
+def time_search_serving(ctx, nstreams=4, qps_fracs=(0.5, 1.0, 2.0, 4.0)):
+    """Served whole searches (ixg_server_submit_search: requests of one model
+    batched into one ixg_search) against plain per-request calls (one
+    ixg_search per request on one context, each sequence already resident),
+    both measured under the same open-loop Poisson arrivals: latency from the
+    scheduled arrival to the result."""
+    import threading
+    pool, mids, seqs = search_serving_workload()
+    bundles, handles, opts = [], [], []
+    for p7, cm, cp9, cmev in pool:
+        W = int(cm["W"])
+        g7 = ixg.P7(ctx, p7, W)
+        gcm = ixg.CM(ctx, cm)
+        gcm.set_trpenalties(*ixg.cm_trpenalties(cm)[:2])
+        g9 = ixg.CP9(ctx, cp9)
+        hb = ixg.HbMap(cm, cp9)
+        trd = synth_cp9.build_trunc_cp9s(cm, 1)
+        tr9 = [ixg.CP9(ctx, h) for h in trd]
+        trhb = [ixg.HbMap(cm, h) for h in trd]
+        f3 = ixg.F3Opts(f1=0.35, f1b=1.0, f2=1.0, f2b=1.0, f3=0.02, f3b=0.02, cmw=W, maxw=W, opts=7)
+        env = ixg.env_opts(f4=0.02, f4b=0.02, f5=0.02, gfmu=p7["gfmu"], gflambda=p7["gflambda"],
+                           lftau=p7["lftau"], lflambda=p7["lflambda"], clen=int(cm["clen"]))
+        opts.append(ixg.search_opts(f3, env, 1e-4, cmev["lc"], cmev["li"], 2.0e6 / (2 * W), W=W, trunc=True))
+        hs = [g7, gcm, g9, hb] + tr9 + trhb
+        handles.append(hs)
+        bundles.append(ixg.SearchModel(g7, gcm, g9, hb, tr9, trhb))
+    n = len(seqs)
+    res = int(sum(2 * (len(d) - 2) for d in seqs))
+    tgts = [ixg.Target(ctx, d, len(d) - 2) for d in seqs]
+
+    def plain(q):
+        hs = handles[mids[q]]
+        return ixg.search(ctx, hs[0], hs[1], hs[2], hs[3], hs[4:7], hs[7:10], tgts[q], opts[mids[q]])[0]
+
+    for q in range(min(8, n)):  # warm-up
+        plain(q)
+    svc = []
+    for q in range(n):
+        t0 = time.perf_counter()
+        plain(q)
+        svc.append(time.perf_counter() - t0)
+    cap_qps = n / sum(svc)
+    srv = ixg.Server(ctx, nstreams=nstreams, max_inflight=1 << 24, max_batch=1 << 22)
+    rng = np.random.default_rng(11)
+    sweep = []
+    for frac in qps_fracs:
+        qps = frac * cap_qps
+        arr = np.cumsum(rng.exponential(1.0 / qps, n))
+        # plain: one request at a time, in arrival order, from its arrival on
+        lat_p, t0 = [], time.perf_counter()
+        for q in range(n):
+            now = time.perf_counter() - t0
+            if now < arr[q]:
+                time.sleep(arr[q] - now)
+            plain(q)
+            lat_p.append(time.perf_counter() - t0 - arr[q])
+        t_plain = time.perf_counter() - t0
+        # served: a submitter thread on the same schedule, results collected as they finish
+        tickets, done_t = [None] * n, [0.0] * n
+        t0 = time.perf_counter()
+
+        def submit():
+            for q in range(n):
+                now = time.perf_counter() - t0
+                if now < arr[q]:
+                    time.sleep(arr[q] - now)
+                tickets[q] = srv.submit_search(bundles[mids[q]], seqs[q], len(seqs[q]) - 2, opts[mids[q]])
+
+        th = threading.Thread(target=submit)
+        th.start()
+        pending = set(range(n))
+        while pending:
+            for q in sorted(pending):
+                if tickets[q] is not None and srv.done(tickets[q]):
+                    srv.search_result(tickets[q])
+                    done_t[q] = time.perf_counter() - t0
+                    pending.discard(q)
+            time.sleep(0.0002)
+        th.join()
+        t_srv = time.perf_counter() - t0
+        lat_s = [done_t[q] - arr[q] for q in range(n)]
+        sweep.append({"qps": qps,
+                      "plain": {"p50_ms": float(np.percentile(lat_p, 50) * 1e3),
+                                "p99_ms": float(np.percentile(lat_p, 99) * 1e3), "mres_s": res / t_plain / 1e6},
+                      "served": {"p50_ms": float(np.percentile(lat_s, 50) * 1e3),
+                                 "p99_ms": float(np.percentile(lat_s, 99) * 1e3), "mres_s": res / t_srv / 1e6}})
+        print(json.dumps({"progress": "search serving", "qps": qps}), file=sys.stderr, flush=True)
+    # peak: everything submitted at once
+    t0 = time.perf_counter()
+    tk = [srv.submit_search(bundles[mids[q]], seqs[q], len(seqs[q]) - 2, opts[mids[q]]) for q in range(n)]
+    for t in tk:
+        srv.search_result(t)
+    t_peak = time.perf_counter() - t0
+    st = srv.stats()
+    srv.free()
+    for t in tgts:
+        t.free()
+    for hs in handles:
+        for x in hs:
+            x.free()
+    return {"workload": f"{n} whole-search requests (sequence, CM), lengths log-normal median 5 kb sigma 1.5 "
+                        f"clipped to [100, 1e6] (seed 5), iid ACGU, a CM-sampled homolog in every third; "
+                        f"CMs uniform from {len(pool)} of consensus 50-400 on random nested structures; "
+                        f"default pipeline (STD + terminal truncated passes), F1 0.35 / F3-F5 0.02, "
+                        f"E-values over a fixed 2 Mb",
+            "cm_states": [int(p[1]["M"]) for p in pool], "residues": res, "streams": nstreams,
+            "plain_mres_s": res / sum(svc) / 1e6, "plain_capacity_qps": cap_qps,
+            "served_peak_mres_s": res / t_peak / 1e6, "served_vs_plain": (res / t_peak) / (res / sum(svc)),
+            "sweep": sweep, "server_stats": st}
+
+
 def time_cascade(ctx, steps, warmup, cpu_s, cpu_threads):
     """The F1-F3b cascade alone on the 4.6 Mb genome with the 300-column
     profile: one ixg_f3_filter_tasks call over every task per step."""
@@ -1117,7 +1258,7 @@ def main():
         return
     if a.serve_only:
         with ixg.Context(0) as ctx:
-            print(json.dumps({"serving": time_serving(ctx)}))
+            print(json.dumps({"serving": time_serving(ctx), "serving_search": time_search_serving(ctx)}))
         return
     wl = search_workload()
     secondary = not (a.headline_only or a.no_cm)
@@ -1141,6 +1282,7 @@ def main():
             extra["batched_uneven_search"] = time_batched(ctx, max(1, a.steps // 3), cpu, a.per_seq_seconds)
         if secondary and not a.no_serve:
             extra["serving"] = time_serving(ctx)
+            extra["serving_search"] = time_search_serving(ctx)
     p7, cm = wl[0], wl[1]
     line = {
         "metric": "Mres/s searched + CYK GCUPS at 1 GPU; p50/p99 latency at fixed QPS",
