@@ -215,3 +215,51 @@ def test_search_multi_sequence_parity(ctx, case):
     assert [h[:3] + h[6:] for h in got] == [h[:3] + h[6:] for h in sorted(union, key=_fin_key)]
     for x in [tgt, g9, gcm, g7, hb] + tr9 + trhb:
         x.free()
+
+
+def test_search_one_query_many_cms(ctx):
+    # cmscan-style: one query searched against 8 CMs of 50-400 consensus
+    # columns (random nested structures, 40-60% paired), each with the
+    # default pipeline, against the oracle's search (hits with their pass,
+    # scores, E-values and every counter)
+    from infernox_amd import synth_cm as S, synth_cp9 as P
+    r = np.random.default_rng(81)
+    cols = [50, 90, 140, 200, 250, 300, 350, 400]
+    pool = []
+    for k, clen in enumerate(cols):
+        ss = S.random_nested(clen, 800 + k)
+        cm = models.cm_from_alignment(ss, models.alignment(ss, 30, 800 + k), 800 + k)
+        pool.append((cm, P.build_cp9_like(cm, 1, el=True), models.p7_from_cm(cm, models.CONFIG1_EV)))
+    # the query: random sequence with a homolog of three of the models, one cut at the 3' end
+    parts = [r.integers(0, 4, 3000).astype(np.uint8)]
+    for k in (1, 4, 7):
+        parts += [S.emit(pool[k][0], 900 + k).astype(np.uint8), r.integers(0, 4, 2000).astype(np.uint8)]
+    h = S.emit(pool[2][0], 999).astype(np.uint8)
+    parts.append(h[: 2 * len(h) // 3])
+    seq = np.concatenate(parts)
+    dsq = np.concatenate([[255], seq, [255]]).astype(np.uint8)
+    L = len(seq)
+    tgt = ixg.Target(ctx, dsq, L)
+    nhits = 0
+    for k, (cm, cp9, p7) in enumerate(pool):
+        W = int(cm["W"])
+        trd = P.build_trunc_cp9s(cm, 1)
+        g7, gcm, g9, hb = ixg.P7(ctx, p7, W), ixg.CM(ctx, cm), ixg.CP9(ctx, cp9), ixg.HbMap(cm, cp9)
+        gcm.set_trpenalties(*ixg.cm_trpenalties(cm)[:2])
+        tr9 = [ixg.CP9(ctx, x) for x in trd]
+        trhb = [ixg.HbMap(cm, x) for x in trd]
+        o = _opts(p7, cm, models.CONFIG1_CM_EV, L, trunc=True)
+        o.env.lftau, o.env.lflambda, o.env.clen = p7["lftau"], p7["lflambda"], int(cm["clen"])
+        _say("one query, CM", k, int(cm["clen"]), int(cm["M"]))
+        got, ga = ixg.search(ctx, g7, gcm, g9, hb, tr9, trhb, tgt, o)
+        want, wa = OracleSearch(p7, cm, cp9, W, trcp9=trd).search(dsq, L, o, threads=8)
+        assert [h[:3] + h[6:] for h in got] == [h[:3] + h[6:] for h in want], (k, got, want)
+        assert np.array_equal(np.float32([h[3] for h in got]), np.float32([h[3] for h in want]))
+        assert np.array_equal(np.float32([h[4] for h in got]), np.float32([h[4] for h in want]))
+        assert [h[5] for h in got] == [h[5] for h in want]
+        _same_tr_acct(ga, wa)
+        nhits += len(got)
+        for x in [g9, gcm, g7, hb] + tr9 + trhb:
+            x.free()
+    tgt.free()
+    assert nhits >= 3
