@@ -1126,7 +1126,7 @@ def time_batched(ctx, steps, cpu=None, per_seq_s=4.0):
     tgt.free()
     found = sum(any(h[7] == s and min(h[0], h[1]) <= e and max(h[0], h[1]) >= b and h[2] == sd for h in hits)
                 for s, b, e, sd in placed)
-    out = {}
+    out = {"autotune": ctx.autotune_choices()}  # (log2 windows class, F3 Forward shape) chosen
     if per_seq_s > 0:
         out["per_sequence_calls"] = per_sequence_calls(ctx, g7, gcm, g9, hb, tr9, trhb, seqs, opts, hits, per_seq_s)
     return finish_batched(out, acct, dt, seqs, cm, W, st, kt, steps, placed, found, cpu, p7, cp9, trd, opts,
@@ -1173,7 +1173,8 @@ def finish_batched(extra, acct, dt, seqs, cm, W, st, kt, steps, placed, found, c
                          "f3b": int(acct.f.n_past_fwdbias), "f5_envelopes": int(acct.e.n_past_edef),
                          "f6": int(acct.n_past_cyk), "hits": int(acct.n_hits),
                          "truncated_pass_hits": [int(x) for x in acct.tr_hits]},
-           "planted_found": f"{found} of {len(placed)}"}
+           "planted_found": f"{found} of {len(placed)}",
+           "autotune_fwd_shapes": extra.pop("autotune", None)}
     out.update(extra)
     if "per_sequence_calls" in out:
         out["batched_over_per_sequence"] = out["value"] / out["per_sequence_calls"]["value"]

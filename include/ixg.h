@@ -616,6 +616,16 @@ enum { IXG_K_SSV_SWEEP = 0, IXG_K_SSV_RESOLVE = 1, IXG_K_FWD = 2, IXG_K_BIAS = 3
        IXG_K_TRQDB_SCAN = 11, IXG_K_CP9 = 12, IXG_K_GFWD = 13, IXG_K_GBWD = 14, IXG_K_GFB = 15,
        IXG_NKERNELS = 16 };
 int ixg_kernel_timings(ixg_ctx *ctx, double *ms, int64_t *launches, int reset);
+/* Launch-shape autotuning (no counterpart in the reference): the F3 Forward
+ * has three launch shapes with identical scores (stripe per wave with 64
+ * windows per wave, 4 lanes per window, the generic LDS form). mode 1 (the
+ * default) times every applicable shape on the first batch of each (model,
+ * log2 window count) class of at least 64 windows and keeps the fastest for
+ * the context; mode 0 takes the fixed order; mode 2 + v forces shape v. */
+int ixg_set_autotune(ixg_ctx *ctx, int mode);
+/* The context's choices so far: (log2 window count class, shape) pairs. */
+int ixg_autotune_choices(ixg_ctx *ctx, int32_t *cls, int32_t *shape, int64_t cap, int64_t *n);
+
 /* Measurement hook (no counterpart in the reference): copies `bytes`
  * device-to-device with a plain kernel reps times after one warm-up copy;
  * *ms_per_copy = mean kernel time. Its counters calibrate HBM traffic

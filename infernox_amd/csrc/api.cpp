@@ -659,6 +659,38 @@ int run_f1(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<
 }
 
 
+// The F3 Forward launch with the context's shape choice: forced, fixed, or
+// autotuned (the first batch of a (model, log2 window count) class of at
+// least 64 windows runs every applicable shape, timed with events on the
+// stream, and the fastest is kept; every shape writes the same scores).
+static hipError_t fwd_launch_tuned(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const WinDev *d_w, int64_t n,
+                                   float *d_sc) {
+  const FwdDev fd = p->fwd_dev();
+  if (c->autotune >= 2) return launch_fwd_variant(fd, d_seq, d_w, n, d_sc, c->autotune - 2, c->stream);
+  if (c->autotune == 0 || n < 64) return launch_fwd(fd, d_seq, d_w, n, d_sc, c->stream);
+  int cls = 0;
+  while (((int64_t)2 << cls) <= n) cls++;
+  const auto key = std::make_pair((const void *)p, cls);
+  auto it = c->tuned.find(key);
+  if (it != c->tuned.end()) return launch_fwd_variant(fd, d_seq, d_w, n, d_sc, it->second, c->stream);
+  int best = -1;
+  float best_ms = 0;
+  for (int v = 0; v < FWD_V_N; v++) {
+    if (!fwd_variant_ok(fd, v)) continue;
+    hipError_t e = hipEventRecord(c->ev[5], c->stream);
+    if (e == hipSuccess) e = launch_fwd_variant(fd, d_seq, d_w, n, d_sc, v, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev[6], c->stream);
+    if (e == hipSuccess) e = hipEventSynchronize(c->ev[6]);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, c->ev[5], c->ev[6]);
+    if (e != hipSuccess) return e;
+    if (best < 0 || ms < best_ms) best = v, best_ms = ms;
+  }
+  if (best < 0) return hipErrorInvalidValue;
+  c->tuned[key] = best;
+  return hipSuccess;  // the last shape's scores are in d_sc, the same as the best's
+}
+
 // Scores for a list of windows: kind 0 fwd, 1 bias, 2 vit, 3 msv (F1b).
 int scores_impl(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<HostTask> &tasks,
                 const std::vector<WinRef> &wl, int kind, std::vector<float> &sc, std::vector<uint8_t> &ovf) {
@@ -713,7 +745,7 @@ int scores_impl(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::ve
   int64_t cells = 0;
   for (auto &w : wd) cells += w.len;
   HIPCHK(ktime_begin(c));
-  if (kind == 0) e = launch_fwd(p->fwd_dev(), d_seq, d_w, n, d_sc, c->stream);
+  if (kind == 0) e = fwd_launch_tuned(c, p, d_seq, d_w, n, d_sc);
   else if (kind == 1) e = launch_bias(p->bias_dev(), d_seq, d_w, n, d_sc, c->stream);
   else if (kind == 2) e = launch_vit(p->vit_dev(), d_seq, d_w, n, d_sc, d_ovf, c->stream);
   else e = launch_msv(p->ssv_dev(), d_seq, d_w, n, d_sc, d_ovf, c->stream);
@@ -1209,6 +1241,30 @@ int f3_filter_views(ixg_ctx *c, const ixg_p7 *p, const ixg_target *tgt, const st
 }  // namespace ixg
 
 extern "C" {
+
+int ixg_set_autotune(ixg_ctx *c, int mode) {
+  if (!c || mode < 0 || mode > 1 + FWD_V_N) {
+    g_err = "ixg_set_autotune: bad arguments";
+    return IXG_EINVAL;
+  }
+  c->autotune = mode;
+  c->tuned.clear();
+  return IXG_OK;
+}
+
+int ixg_autotune_choices(ixg_ctx *c, int32_t *cls, int32_t *shape, int64_t cap, int64_t *n) {
+  if (!c || !n || (cap > 0 && (!cls || !shape))) {
+    g_err = "ixg_autotune_choices: bad arguments";
+    return IXG_EINVAL;
+  }
+  int64_t k = 0;
+  for (const auto &kv : c->tuned) {
+    if (k < cap) cls[k] = kv.first.second, shape[k] = kv.second;
+    k++;
+  }
+  *n = k;
+  return k > cap ? IXG_ERANGE : IXG_OK;
+}
 
 int ixg_copy_probe(ixg_ctx *c, size_t bytes, int reps, double *ms_per_copy) {
   if (!c || bytes < 16 || reps < 1 || !ms_per_copy) return IXG_EINVAL;

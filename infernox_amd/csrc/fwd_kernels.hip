@@ -434,15 +434,28 @@ struct MakeFwdTable<1, Q...> {
 };
 using FwdAll = MakeFwdTable<FWD_NQ_EXACT_MAX>::type;  // stripe counts 2..FWD_NQ_EXACT_MAX
 
-hipError_t launch_fwd(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin, float *sc,
-                      hipStream_t s) {
+bool fwd_variant_ok(const FwdDev &p, int v) {
+  if (v == FWD_V_ZW) return p.nq >= 2 && p.nq <= FWD_ZW_NQ_MAX && p.tz;
+  if (v == FWD_V_EXACT) return p.nq >= 2 && p.nq <= FWD_NQ_EXACT_MAX;
+  return v == FWD_V_GENERIC && (size_t)3 * p.nq * 64 * sizeof(float) <= 160 * 1024;
+}
+
+hipError_t launch_fwd_variant(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin, float *sc,
+                              int v, hipStream_t s) {
   if (nwin == 0) return hipSuccess;
-  if (p.nq >= 2 && p.nq <= FWD_ZW_NQ_MAX && p.tz) return launch_fwd_zw(p, seq, wins, nwin, sc, s);
-  if (p.nq >= 2 && p.nq <= FWD_NQ_EXACT_MAX) return FwdAll::fns[p.nq - 2](p, seq, wins, nwin, sc, s);
+  if (!fwd_variant_ok(p, v)) return hipErrorInvalidValue;
+  if (v == FWD_V_ZW) return launch_fwd_zw(p, seq, wins, nwin, sc, s);
+  if (v == FWD_V_EXACT) return FwdAll::fns[p.nq - 2](p, seq, wins, nwin, sc, s);
   const size_t lds = (size_t)3 * p.nq * 64 * sizeof(float);
-  if (lds > 160 * 1024) return hipErrorInvalidValue;
   hipLaunchKernelGGL(fwd_generic, dim3((unsigned)((nwin + 15) / 16)), dim3(64), lds, s, p, seq, wins, nwin, sc);
   return hipGetLastError();
+}
+
+hipError_t launch_fwd(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin, float *sc,
+                      hipStream_t s) {
+  for (int v : {FWD_V_ZW, FWD_V_EXACT, FWD_V_GENERIC})
+    if (fwd_variant_ok(p, v)) return launch_fwd_variant(p, seq, wins, nwin, sc, v, s);
+  return hipErrorInvalidValue;
 }
 
 }  // namespace ixg

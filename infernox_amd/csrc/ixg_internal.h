@@ -185,6 +185,14 @@ hipError_t launch_ssv_resolve(const SsvDev &p, const TaskDev *tasks, const Sweep
 hipError_t launch_fwd(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin,
                       float *sc, hipStream_t s);
 constexpr int FWD_ZW_NQ_MAX = 80;  // largest stripe count of the stripe-per-wave Forward kernel
+// The F3 Forward's launch shapes, all computing the same scores bit for bit:
+// stripe per wave with 64 windows per wave (nq <= 80), 4 lanes per window with
+// 16 windows per wave (nq <= 100), and the generic LDS form (any nq whose
+// three stripe rows fit in LDS). launch_fwd takes the first that applies.
+enum { FWD_V_ZW = 0, FWD_V_EXACT = 1, FWD_V_GENERIC = 2, FWD_V_N = 3 };
+bool fwd_variant_ok(const FwdDev &p, int v);
+hipError_t launch_fwd_variant(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin, float *sc,
+                              int v, hipStream_t s);
 hipError_t launch_fwd_zw(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin, float *sc,
                          hipStream_t s);
 hipError_t launch_fwd_full(const FwdDev &p, const uint8_t *seq, const WinDev *wins, int64_t nwin,

@@ -233,6 +233,20 @@ class Context:
     def mem_in_use(self):
         return lib().ixg_mem_in_use(self.h)
 
+    def set_autotune(self, mode):
+        """ixg_set_autotune: 1 autotune the F3 Forward's launch shape, 0 fixed
+        order, 2 + v force shape v (0 stripe per wave, 1 four lanes per
+        window, 2 generic)."""
+        check(lib().ixg_set_autotune(self.h, int(mode)), "ixg_set_autotune")
+
+    def autotune_choices(self):
+        """[(log2 window-count class, shape)] chosen so far."""
+        a, b, n = (C.c_int32 * 64)(), (C.c_int32 * 64)(), C.c_int64()
+        lib().ixg_autotune_choices.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int64,
+                                               C.POINTER(C.c_int64)]
+        check(lib().ixg_autotune_choices(self.h, a, b, 64, C.byref(n)), "ixg_autotune_choices")
+        return [(a[i], b[i]) for i in range(min(n.value, 64))]
+
     def timings(self):
         ms = (C.c_double * 5)()
         cells = (C.c_int64 * 5)()

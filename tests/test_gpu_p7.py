@@ -164,3 +164,32 @@ def test_f3_filter_tasks_both_strands(ctx):
             assert getattr(accts[t], f) == getattr(wa, f), (t, f)
     tgt.free()
     g.free()
+
+
+@pytest.mark.parametrize("M", [72, 300, 400])
+def test_fwd_launch_shapes_identical(ctx, M):
+    # the F3 Forward's launch shapes give the same scores bit for bit, and the
+    # autotuner (which runs each applicable shape once per window-count
+    # class and keeps the fastest) gives them too
+    rng = synth.Lcg(11 + M)
+    p = synth.lcg_profile(rng, M, EV)
+    g = ixg.P7(ctx, p, 200)
+    L = 40000
+    seq = synth.lcg_dsq(rng, L)
+    seq[0] = 255
+    offs, lens = _windows(np.random.default_rng(M + 5), L, 700)
+    nq = max(2, (M - 1) // 4 + 1)
+    shapes = [v for v, ok in ((0, nq <= 80), (1, nq <= 100), (2, True)) if ok]
+    _say("fwd shapes", M, shapes)
+    got = []
+    for v in shapes:
+        ctx.set_autotune(2 + v)
+        got.append(np.float32(g.fwd_scores(seq, offs, lens)))
+    ctx.set_autotune(1)
+    tuned = [np.float32(g.fwd_scores(seq, offs, lens)) for _ in range(2)]
+    choices = ctx.autotune_choices()
+    ctx.set_autotune(1)
+    for x in got[1:] + tuned:
+        assert np.array_equal(x.view(np.int32), got[0].view(np.int32))
+    assert len(choices) == 1 and choices[0][1] in shapes, choices
+    g.free()
