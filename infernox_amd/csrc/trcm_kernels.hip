@@ -226,7 +226,13 @@ __device__ void tr_cell(const CmDev &cm, const EnvDev &e, int v, int st, int j, 
   if (dR) A[2 * nc + c] = R;
 }
 
-constexpr int TCH = 1;  // cells of an insert chain fetched ahead of their serial combine
+// cells of an insert chain fetched ahead of their serial combine: two for
+// TrCYK, one for FTrInside (whose table lookups leave no registers for a
+// second cell's operands without spilling)
+template <bool INSIDE>
+struct Tch {
+  static constexpr int n = INSIDE ? 1 : 2;
+};
 
 // One j row of an IL state: J and L read their own (j, d-1), the previous
 // cell of the row, carried in registers; the other children's cells (rows
@@ -259,6 +265,7 @@ __device__ void tr_il_row(const CmDev &cm, const EnvDev &e, int v, int u, const 
   for (int yo = 0; yo < MAXC; yo++)
     if (yo < cn && cf + yo == v) selfo = yo;
   float pJ = 0.f, pL = 0.f;
+  constexpr int TCH = Tch<INSIDE>::n;
   for (int d0 = lo; d0 <= hi; d0 += TCH) {
     float a1[TCH][MAXC], l1[TCH][MAXC], a0[TCH][MAXC], r0[TCH][MAXC];
     int x[TCH];
@@ -343,6 +350,7 @@ __device__ void tr_ir_diag(const CmDev &cm, const EnvDev &e, int v, int u, const
   }
   float pJ = 0.f, pR = 0.f;
   bool have = false;
+  constexpr int TCH = Tch<INSIDE>::n;
   for (int jp0 = 0; jp0 < nj; jp0 += TCH) {
     float a1[TCH][MAXC], r1[TCH][MAXC], a0[TCH][MAXC], l0[TCH][MAXC];
     bool k1[TCH][MAXC], k0[TCH][MAXC], on[TCH];
