@@ -513,52 +513,80 @@ __global__ __launch_bounds__(TRHB_THREADS) void trhb_scan_kernel(CmDev cm, const
   const int ncv = ro[last] + (wl > 0 ? wl : 0) - ro[0];
   const int m0 = e.mask[0], nc = e.nc;
   const IXG_GLOBAL float *A = gmem(e.alpha);
-  for (int c = tid; c < ncv; c += nt) {
-    const int tgt = ro[0] + c;
-    const int r = hb_row_of(ro, last, tgt);
-    const int d = hmn[r] + (tgt - ro[r]), j = jmn + r;
-    float J = IMPOSSIBLE, L = IMPOSSIBLE, R = IMPOSSIBLE, T = IMPOSSIBLE, best = IMPOSSIBLE;
-    int bm = MODE_UNKNOWN, br = 0;
+  // root cells in runs of RCH consecutive d of one row: each begin state's row
+  // is looked up once per run; per cell the states go in list order with the
+  // reference's strict comparisons, as cell by cell
+  constexpr int RCH = 8;
+  const int cpr = (e.j0 - e.i0 + 2 + RCH - 1) / RCH;  // runs per row: d <= j0 - i0 + 1
+  (void)ncv;
+  for (int id = tid; id < nj * cpr; id += nt) {
+    const int r = id / cpr, k0 = (id - r * cpr) * RCH;
+    const int lo = hmn[r], hi = hmx[r], d0 = lo + k0;
+    if (d0 > hi) continue;
+    const int n = min(RCH, hi - d0 + 1);
+    const int j = jmn + r, c0 = ro[r] - ro[0] + k0;
+    float J[RCH], L[RCH], R[RCH], T[RCH], best[RCH];
+    int bm[RCH], br[RCH];
+#pragma unroll
+    for (int t = 0; t < RCH; t++) {
+      J[t] = L[t] = R[t] = T[t] = best[t] = IMPOSSIBLE;
+      bm[t] = MODE_UNKNOWN;
+      br[t] = 0;
+    }
     for (int q = 0; q < ny; q++) {
       const int y = ylist[q];
-      const int cy = cell_at(e, y, j, d);
-      if (cy < 0) continue;
+      const int jmy = e.jmin[y];
+      if (j < jmy || j > e.jmax[y]) continue;
+      const int h = e.hd_off[y] + (j - jmy);
+      const int ylo = e.hdmin[h], yhi = e.hdmax[h], yro = e.roff[h];
       const int my = e.mask[y];
       const float p = pty[y];
-      if ((m0 & BJ) && (my & BJ)) {
-        const float s = A[cy] + p;
-        if (s > J) {
-          J = s;
-          if (s > best) best = s, bm = MODE_J, br = y;
+      const bool bt = (m0 & BT) && (my & BT) && cm.sttype[y] == B_ST;
+#pragma unroll
+      for (int t = 0; t < RCH; t++) {
+        const int d = d0 + t;
+        if (t >= n || d < ylo || d > yhi) continue;
+        const int cy = yro + (d - ylo);
+        if ((m0 & BJ) && (my & BJ)) {
+          const float s = A[cy] + p;
+          if (s > J[t]) {
+            J[t] = s;
+            if (s > best[t]) best[t] = s, bm[t] = MODE_J, br[t] = y;
+          }
         }
-      }
-      if ((m0 & BL) && (my & BL)) {
-        const float s = A[nc + cy] + p;
-        if (s > L) {
-          L = s;
-          if (s > best) best = s, bm = MODE_L, br = y;
+        if ((m0 & BL) && (my & BL)) {
+          const float s = A[nc + cy] + p;
+          if (s > L[t]) {
+            L[t] = s;
+            if (s > best[t]) best[t] = s, bm[t] = MODE_L, br[t] = y;
+          }
         }
-      }
-      if ((m0 & BR) && (my & BR)) {
-        const float s = A[2 * nc + cy] + p;
-        if (s > R) {
-          R = s;
-          if (s > best) best = s, bm = MODE_R, br = y;
+        if ((m0 & BR) && (my & BR)) {
+          const float s = A[2 * nc + cy] + p;
+          if (s > R[t]) {
+            R[t] = s;
+            if (s > best[t]) best[t] = s, bm[t] = MODE_R, br[t] = y;
+          }
         }
-      }
-      if ((m0 & BT) && (my & BT) && cm.sttype[y] == B_ST) {
-        const float s = A[3 * nc + cy] + p;
-        if (s > T) {
-          T = s;
-          if (s > best) best = s, bm = MODE_T, br = y;
+        if (bt) {
+          const float s = A[3 * nc + cy] + p;
+          if (s > T[t]) {
+            T[t] = s;
+            if (s > best[t]) best[t] = s, bm[t] = MODE_T, br[t] = y;
+          }
         }
       }
     }
-    gmem(e.rout)[c] = J;
-    gmem(e.rout)[ncv + c] = L;
-    gmem(e.rout)[2 * ncv + c] = R;
-    gmem(e.rout)[3 * ncv + c] = T;
-    gmem(e.rout)[4 * ncv + c] = __int_as_float(br * 8 + bm);
+#pragma unroll
+    for (int t = 0; t < RCH; t++) {
+      if (t >= n) continue;
+      const int c = c0 + t;
+      gmem(e.rout)[c] = J[t];
+      gmem(e.rout)[ncv + c] = L[t];
+      gmem(e.rout)[2 * ncv + c] = R[t];
+      gmem(e.rout)[3 * ncv + c] = T[t];
+      gmem(e.rout)[4 * ncv + c] = __int_as_float(br[t] * 8 + bm[t]);
+    }
   }
 }
 
