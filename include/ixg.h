@@ -619,12 +619,13 @@ int ixg_kernel_timings(ixg_ctx *ctx, double *ms, int64_t *launches, int reset);
 /* Launch-shape autotuning (no counterpart in the reference): the F3 Forward
  * has three launch shapes with identical scores (stripe per wave with 64
  * windows per wave, 4 lanes per window, the generic LDS form). mode 1 (the
- * default) times every applicable shape on the first batch of each (model,
- * log2 window count) class of at least 64 windows and keeps the fastest for
- * the context; mode 0 takes the fixed order; mode 2 + v forces shape v. */
+ * default) times every applicable shape (a warm run, then a timed one) on
+ * the first batch of each (stripe count, log2 window count) class of at
+ * least 256 windows and keeps the fastest for the context; mode 0 takes the
+ * fixed order; mode 2 + v forces shape v. */
 int ixg_set_autotune(ixg_ctx *ctx, int mode);
-/* The context's choices so far: (log2 window count class, shape) pairs. */
-int ixg_autotune_choices(ixg_ctx *ctx, int32_t *cls, int32_t *shape, int64_t cap, int64_t *n);
+/* The context's choices so far: (stripe count, log2 window count class, shape). */
+int ixg_autotune_choices(ixg_ctx *ctx, int32_t *nq, int32_t *cls, int32_t *shape, int64_t cap, int64_t *n);
 
 /* Measurement hook (no counterpart in the reference): copies `bytes`
  * device-to-device with a plain kernel reps times after one warm-up copy;

@@ -660,24 +660,26 @@ int run_f1(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const std::vector<
 
 
 // The F3 Forward launch with the context's shape choice: forced, fixed, or
-// autotuned (the first batch of a (model, log2 window count) class of at
-// least 64 windows runs every applicable shape, timed with events on the
-// stream, and the fastest is kept; every shape writes the same scores).
+// autotuned: the first batch of a (stripe count, log2 window count) class of
+// at least 256 windows runs every applicable shape twice, the second run
+// timed with events on the stream (the first absorbs one-time launch costs),
+// and the fastest is kept for the class; every shape writes the same scores.
 static hipError_t fwd_launch_tuned(ixg_ctx *c, const ixg_p7 *p, const uint8_t *d_seq, const WinDev *d_w, int64_t n,
                                    float *d_sc) {
   const FwdDev fd = p->fwd_dev();
   if (c->autotune >= 2) return launch_fwd_variant(fd, d_seq, d_w, n, d_sc, c->autotune - 2, c->stream);
-  if (c->autotune == 0 || n < 64) return launch_fwd(fd, d_seq, d_w, n, d_sc, c->stream);
+  if (c->autotune == 0 || n < 256) return launch_fwd(fd, d_seq, d_w, n, d_sc, c->stream);
   int cls = 0;
   while (((int64_t)2 << cls) <= n) cls++;
-  const auto key = std::make_pair((const void *)p, cls);
+  const auto key = std::make_pair(fd.nq, cls);
   auto it = c->tuned.find(key);
   if (it != c->tuned.end()) return launch_fwd_variant(fd, d_seq, d_w, n, d_sc, it->second, c->stream);
   int best = -1;
   float best_ms = 0;
   for (int v = 0; v < FWD_V_N; v++) {
     if (!fwd_variant_ok(fd, v)) continue;
-    hipError_t e = hipEventRecord(c->ev[5], c->stream);
+    hipError_t e = launch_fwd_variant(fd, d_seq, d_w, n, d_sc, v, c->stream);
+    if (e == hipSuccess) e = hipEventRecord(c->ev[5], c->stream);
     if (e == hipSuccess) e = launch_fwd_variant(fd, d_seq, d_w, n, d_sc, v, c->stream);
     if (e == hipSuccess) e = hipEventRecord(c->ev[6], c->stream);
     if (e == hipSuccess) e = hipEventSynchronize(c->ev[6]);
@@ -1252,14 +1254,14 @@ int ixg_set_autotune(ixg_ctx *c, int mode) {
   return IXG_OK;
 }
 
-int ixg_autotune_choices(ixg_ctx *c, int32_t *cls, int32_t *shape, int64_t cap, int64_t *n) {
-  if (!c || !n || (cap > 0 && (!cls || !shape))) {
+int ixg_autotune_choices(ixg_ctx *c, int32_t *nq, int32_t *cls, int32_t *shape, int64_t cap, int64_t *n) {
+  if (!c || !n || (cap > 0 && (!nq || !cls || !shape))) {
     g_err = "ixg_autotune_choices: bad arguments";
     return IXG_EINVAL;
   }
   int64_t k = 0;
   for (const auto &kv : c->tuned) {
-    if (k < cap) cls[k] = kv.first.second, shape[k] = kv.second;
+    if (k < cap) nq[k] = kv.first.first, cls[k] = kv.first.second, shape[k] = kv.second;
     k++;
   }
   *n = k;

@@ -143,10 +143,10 @@ struct ixg_ctx {
   double search_ms[9] = {};
   int64_t search_cells[2] = {};
   // launch-shape autotuning of the F3 Forward (ixg_set_autotune): 1 times
-  // every applicable shape once per (model, window-count class) and keeps
-  // the fastest; 0 takes the fixed order; 2 + v forces shape v
+  // every applicable shape once per (stripe count, window-count class) and
+  // keeps the fastest; 0 takes the fixed order; 2 + v forces shape v
   int autotune = 1;
-  std::map<std::pair<const void *, int>, int> tuned;  // (model, log2 windows) -> shape
+  std::map<std::pair<int, int>, int> tuned;  // (stripe count, log2 windows) -> shape
   // auxiliary contexts (own stream and scratch) on which ixg_search runs the
   // truncated passes' CM stages concurrently with the STD pass's; created on
   // first use, freed with this context

@@ -264,6 +264,7 @@ int ixg_server_create(ixg_ctx *owner, int nstreams, size_t mem_budget, int64_t m
       for (auto *x : s->ctx) ixg_shutdown(x);
       return rc;
     }
+    c->autotune = 0;  // serving keeps the fixed launch shapes: no tuning runs inside a request's latency
     s->ctx.push_back(c);
     // staging sized up front for a full batch, so serving does not free and
     // reallocate pinned memory (a device-wide synchronising call) while other

@@ -240,12 +240,12 @@ class Context:
         check(lib().ixg_set_autotune(self.h, int(mode)), "ixg_set_autotune")
 
     def autotune_choices(self):
-        """[(log2 window-count class, shape)] chosen so far."""
-        a, b, n = (C.c_int32 * 64)(), (C.c_int32 * 64)(), C.c_int64()
-        lib().ixg_autotune_choices.argtypes = [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int64,
-                                               C.POINTER(C.c_int64)]
-        check(lib().ixg_autotune_choices(self.h, a, b, 64, C.byref(n)), "ixg_autotune_choices")
-        return [(a[i], b[i]) for i in range(min(n.value, 64))]
+        """[(stripe count, log2 window-count class, shape)] chosen so far."""
+        q, a, b, n = (C.c_int32 * 256)(), (C.c_int32 * 256)(), (C.c_int32 * 256)(), C.c_int64()
+        i32p = C.POINTER(C.c_int32)
+        lib().ixg_autotune_choices.argtypes = [C.c_void_p, i32p, i32p, i32p, C.c_int64, C.POINTER(C.c_int64)]
+        check(lib().ixg_autotune_choices(self.h, q, a, b, 256, C.byref(n)), "ixg_autotune_choices")
+        return [(q[i], a[i], b[i]) for i in range(min(n.value, 256))]
 
     def timings(self):
         ms = (C.c_double * 5)()

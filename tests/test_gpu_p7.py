@@ -191,5 +191,5 @@ def test_fwd_launch_shapes_identical(ctx, M):
     ctx.set_autotune(1)
     for x in got[1:] + tuned:
         assert np.array_equal(x.view(np.int32), got[0].view(np.int32))
-    assert len(choices) == 1 and choices[0][1] in shapes, choices
+    assert len(choices) == 1 and choices[0][0] == nq and choices[0][2] in shapes, choices
     g.free()
