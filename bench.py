@@ -141,7 +141,7 @@ def search_cpu_baseline(wl, opts, threads, budget_s):
                        f"{len(hits)} hits"), hits
 
 
-def time_search(ctx, steps, warmup, wl):
+def time_search(ctx, steps, warmup, wl, traffic=None):
     """The headline step: ixg_search over the whole target (the STD and the
     truncated passes). Returns the line's fields and the last step's hits."""
     p7, cm, cp9, cmev, dsq, L, planted = wl
@@ -201,7 +201,7 @@ def time_search(ctx, steps, warmup, wl):
                 "why_two_peaks": "the published 157.3 TFLOP/s counts packed FP32 FMA (2 flops x 2 lanes per "
                                  "lane-cycle); the max-plus / log-sum DP ops are single unpacked non-FMA lane "
                                  "ops, built without packed FP32, so 39.3 T lane-op/s is their ceiling",
-                "traffic": SEARCH_TRAFFIC.get(dom)}
+                "traffic": (SEARCH_TRAFFIC if traffic is None else traffic).get(dom)}
     out = {"value": acct.nres / dt / 1e6, "ms_per_step": dt * 1e3, "residues": int(acct.nres),
            "stage_ms": st, "kernels": kern, "roofline": roof,
            "cyk_gcups": cyk_cells / (st["f6_cyk"] * 1e-3) / 1e9 if st["f6_cyk"] > 0 else None,
@@ -1070,6 +1070,42 @@ def time_cascade(ctx, steps, warmup, cpu_s, cpu_threads):
     return out
 
 
+def genome_search_workload():
+    """The genome-scale workload: a 4.6 Mb order-2 Markov genome at 50.8% GC
+    (seed 3) with 100 CM-sampled homologs inserted on random strands, the
+    300-column 12-hairpin CM (959 states, W 398), its CP9s and p7 filter."""
+    p7, cm, cp9, cmev = models.config2_models()
+    dsq, L, planted = models.markov_genome_with_homologs(cm, 4_600_000, 100, 3)
+    return p7, cm, cp9, cmev, dsq, L, planted
+
+
+def time_genome_search(ctx, steps, warmup, cpu=None):
+    """One ixg_search over the genome (both strands, 92 chunk x strand tasks),
+    the headline's step on the genome-scale model; the oracle's same search
+    on the first 460 kb beside it."""
+    wl = genome_search_workload()
+    out, opts, hits = time_search(ctx, steps, warmup, wl, traffic={})  # traffic measured for the headline only
+    out["workload"] = ("4.6 Mb order-2 Markov genome at 50.8% GC (seed 3) + 100 CM-sampled homologs, both strands; "
+                       "300-column 12-hairpin CM (60-sequence alignment, seed 3; 959 states, W 398); tier 2-20 Mb "
+                       "(F1 0.35, F3=F3b=F4=F4b=F5 0.005, F6 1e-4); default truncation; one ixg_search per step")
+    out["unit"] = "Mres/s"
+    if cpu:
+        p7, cm, cp9, cmev, dsq, L, planted = wl
+        Ls = 460_000
+        sub = np.concatenate([dsq[:Ls + 1], [255]]).astype(np.uint8)
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_pipe_bind import OracleSearch
+        o = OracleSearch(p7, cm, cp9, int(cm["W"]), trcp9=synth_cp9.build_trunc_cp9s(cm, 1))
+        so = search_opts(p7, cm, cmev, L)  # the whole genome's E-value scale and tier
+        t0 = time.perf_counter()
+        _, a = o.search(sub, Ls, so, threads=cpu[1])
+        dt = time.perf_counter() - t0
+        out["cpu"] = {"value": a.nres / dt / 1e6, "unit": "Mres/s", "cores": cpu[1], "kind": "port",
+                      "sample": f"the first {Ls} residues of the genome, both strands, oracle ixo_search_std_mt "
+                                f"(C restatement, not infernox), {dt:.1f} s"}
+    return out
+
+
 def batched_workload(n=10000):
     """The batched uneven workload: n iid sequences at 45% GC with log-normal
     lengths (median 2 kb, sigma 1.0, clipped to 200 nt - 200 kb, seed 4), 100
@@ -1222,6 +1258,8 @@ def main():
     ap.add_argument("--multi-only", action="store_true", help="only run the several-CM HB launch section")
     ap.add_argument("--batched-only", action="store_true",
                     help="only run the batched uneven-database search section (10,000 sequences)")
+    ap.add_argument("--genome-only", action="store_true",
+                    help="only run the genome-scale search section (4.6 Mb, 959-state CM)")
     ap.add_argument("--per-seq-seconds", type=float, default=4.0,
                     help="time budget of the batched section's one-call-per-sequence comparison (0: skip it)")
     ap.add_argument("--hb-only", action="store_true",
@@ -1250,6 +1288,11 @@ def main():
         with ixg.Context(0) as ctx:
             cpu = None if a.no_cpu else (a.cpu_section_seconds, a.cpu_threads)
             print(json.dumps({"hb_scanners": time_cm(ctx, a.steps, a.warmup, cpu, cm_workload())}))
+        return
+    if a.genome_only:
+        with ixg.Context(0) as ctx:
+            cpu = None if a.no_cpu else (a.cpu_section_seconds, a.cpu_threads)
+            print(json.dumps({"genome_search": time_genome_search(ctx, max(1, a.steps // 3), 1, cpu)}))
         return
     if a.batched_only:
         with ixg.Context(0) as ctx:
@@ -1281,6 +1324,7 @@ def main():
             extra["p7_forward_full"] = time_p7_forward(ctx, p, d, 2, cpu=cpu)
             extra["hb_multi"] = time_multi(ctx)
             extra["batched_uneven_search"] = time_batched(ctx, max(1, a.steps // 3), cpu, a.per_seq_seconds)
+            extra["genome_search"] = time_genome_search(ctx, max(1, a.steps // 3), 1, cpu)
         if secondary and not a.no_serve:
             extra["serving"] = time_serving(ctx)
             extra["serving_search"] = time_search_serving(ctx)

@@ -154,6 +154,51 @@ def config3_models(ev=None, cm_ev=None):
     return p7, cm, cp9, (cm_ev or CONFIG3_CM_EV)
 
 
+# the same for the 300-column model (calibrate_models.py 2 50: 50 CM tail
+# samples, the non-banded scans of its 959 states being the slow part)
+CONFIG2_EV = dict(lmmu=-9.8474, lmlambda=0.693147, lvmu=-10.4678, lvlambda=0.693147, lftau=-6.026,
+                  lflambda=0.693147, gfmu=-119.6546, gflambda=0.693147)
+CONFIG2_CM_EV = dict(lc=(-0.8797, 0.693147), li=(1.1846, 0.693147))
+
+
+def config2_models(ev=None, cm_ev=None):
+    """(p7 dict, CM dict, CP9 dict, cm_ev) of the genome-scale model: a
+    300-column structure of 12 hairpins in multibranch loops (seed 3), a
+    60-sequence alignment (seed 3) and the same builders as config1_models."""
+    ss = S.multibranch(300, 12, 3)
+    aln = alignment(ss, 60, seed=3)
+    cm = cm_from_alignment(ss, aln, seed=3)
+    cp9 = P.build_cp9_like(cm, 1, el=True)
+    p7 = p7_from_cm(cm, ev or CONFIG2_EV)
+    return p7, cm, cp9, (cm_ev or CONFIG2_CM_EV)
+
+
+def markov_genome_with_homologs(cm, L, n, seed, gc=0.508):
+    """An order-2 Markov sequence of L residues at the given GC (seed) with n
+    CM-sampled homologs inserted at random positions on random strands.
+    Returns (dsq, total length, [(start, end, strand)])."""
+    from . import synth
+    base = synth.markov2_dsq(L, seed, gc=gc)[1:L + 1]
+    r = np.random.default_rng(seed + 1)
+    ins = sorted(int(x) for x in r.integers(0, L, n))
+    out, placed, last = [], [], 0
+    for k, pos in enumerate(ins):
+        h = S.emit(cm, seed * 1000 + k).astype(np.uint8)
+        if int(r.integers(0, 2)):
+            h = np.where(h < 4, 3 - h, h)[::-1].astype(np.uint8)
+            strand = 1
+        else:
+            strand = 0
+        out.append(base[last:pos])
+        start = sum(len(x) for x in out) + 1
+        out.append(h)
+        placed.append((start, start + len(h) - 1, strand))
+        last = pos
+    out.append(base[last:])
+    seq = np.concatenate(out).astype(np.uint8)
+    return np.concatenate([[255], seq, [255]]).astype(np.uint8), len(seq), placed
+
+
 def lognormal_database(n, median, sigma, lo, hi, gc, seed, cm=None, n_hom=0):
     """n iid sequences (ACGU at the given GC) with log-normal lengths (the
     given median and sigma, clipped to [lo, hi]); with a CM, n_hom

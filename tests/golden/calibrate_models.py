@@ -1,6 +1,7 @@
 """Calibrates the E-value parameters of a workload model
-(infernox_amd.models.config1_models, the 72-column cloverleaf; or
-config3_models, the 120-column three-hairpin CM: argument "1" or "3") with
+(infernox_amd.models.config1_models, the 72-column cloverleaf;
+config2_models, the 300-column 12-hairpin CM; or config3_models, the
+120-column three-hairpin CM: argument "1", "2" or "3") with
 the CPU oracle, in the manner of
 HMMER's p7_Calibrate and of cmcalibrate's exponential tails, with lambda
 fixed at ln 2:
@@ -11,7 +12,8 @@ fixed at ln 2:
   of length 100, glocal Forward tau from the 6.5% tail of 200 of length
   max(100, 2 clen) (the lengths and tail masses of cm_p7_calibrate.rs:273-283);
 * CM local CYK and Inside tails: the best root score of 200 iid windows of
-  length 2W (non-banded scans), mu from their 4% tail.
+  length 2W (non-banded scans; a second argument sets fewer for the large
+  model), mu from their 4% tail.
 
 Prints the two dict literals that infernox_amd/models.py holds for it."""
 import math
@@ -42,7 +44,8 @@ def tail_mu(x, frac=0.04):
 
 def main():
     which = sys.argv[1] if len(sys.argv) > 1 else "1"
-    build = {"1": models.config1_models, "3": models.config3_models}[which]
+    ncm = int(sys.argv[2]) if len(sys.argv) > 2 else 200  # CM tail samples (fewer for the large model)
+    build = {"1": models.config1_models, "2": models.config2_models, "3": models.config3_models}[which]
     zero = dict(lmmu=0, lmlambda=LAM, lvmu=0, lvlambda=LAM, lftau=0, lflambda=LAM, gfmu=0, gflambda=LAM)
     p7, cm, cp9, _ = build(ev=zero)
     o = Oracle(p7, 200)
@@ -66,7 +69,7 @@ def main():
     q = OracleQdbCM(cm, local=True)
     W = int(cm["W"])
     cyk, ins = [], []
-    for _ in range(200):
+    for _ in range(ncm):
         n = 2 * W
         d = np.concatenate([[255], r.integers(0, 4, n), [255]]).astype(np.uint8)
         _, root = q.scan(d, 1, n, 0, want_root=True)
