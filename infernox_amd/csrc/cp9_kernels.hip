@@ -492,11 +492,11 @@ __device__ __forceinline__ NodeW node_w(const G &g, int k, int x) {
 
 #define LW(a, b) lsum((a), (b), s.tbl, nz)
 
-__global__ __launch_bounds__(64) void cp9_fwd_wave(Cp9Dev h, const Cp9WinDev *wins, int do_scan, int align,
-                                                  float *best_out) {
+__device__ __forceinline__ void cp9_fwd_wave_body(const Cp9Dev &h, const Cp9WinDev *wins, int wi, int do_scan,
+                                          int align, float *best_out) {
   const WaveLds s = wave_lds(h, h.nsf, h.el_idx, h.el_slot, h.nel, h.finfo);
   const int t = threadIdx.x, M = h.M, M1 = M + 1, nz = h.nz;
-  const Cp9WinDev w = wins[blockIdx.x];
+  const Cp9WinDev w = wins[wi];
   const G g = gtabs(h, w);
   const int i0 = w.i0, j0 = w.j0, L = j0 - i0 + 1;
   const size_t cells = (size_t)(L + 1) * M1;
@@ -603,11 +603,16 @@ __global__ __launch_bounds__(64) void cp9_fwd_wave(Cp9Dev h, const Cp9WinDev *wi
   if (!align && t == 0) best_out[w.out] = best;
 }
 
-__global__ __launch_bounds__(64) void cp9_bwd_wave(Cp9Dev h, const Cp9WinDev *wins, int do_scan, int align,
-                                                  float *best_out) {
+__global__ __launch_bounds__(64) void cp9_fwd_wave(Cp9Dev h, const Cp9WinDev *wins, int do_scan, int align,
+                                              float *best_out) {
+  cp9_fwd_wave_body(h, wins, blockIdx.x, do_scan, align, best_out);
+}
+
+__device__ __forceinline__ void cp9_bwd_wave_body(const Cp9Dev &h, const Cp9WinDev *wins, int wi, int do_scan,
+                                          int align, float *best_out) {
   const WaveLds s = wave_lds(h, h.nsb, h.bel_k, h.bel_slot, h.nbel, h.binfo);
   const int t = threadIdx.x, M = h.M, M1 = M + 1, nz = h.nz;
-  const Cp9WinDev w = wins[blockIdx.x];
+  const Cp9WinDev w = wins[wi];
   const G g = gtabs(h, w);
   const int i0 = w.i0, j0 = w.j0, L = j0 - i0 + 1;
   const size_t cells = (size_t)(L + 1) * M1;
@@ -747,6 +752,11 @@ __global__ __launch_bounds__(64) void cp9_bwd_wave(Cp9Dev h, const Cp9WinDev *wi
   if (!align && t == 0) best_out[w.out] = best;
 }
 
+__global__ __launch_bounds__(64) void cp9_bwd_wave(Cp9Dev h, const Cp9WinDev *wins, int do_scan, int align,
+                                              float *best_out) {
+  cp9_bwd_wave_body(h, wins, blockIdx.x, do_scan, align, best_out);
+}
+
 #undef LW
 #undef N_MM
 #undef N_IM
@@ -768,6 +778,16 @@ __global__ __launch_bounds__(64) void cp9_bwd_wave(Cp9Dev h, const Cp9WinDev *wi
 
 }  // namespace
 
+// The Forward and Backward wavefront kernels in one launch: blocks [0, n)
+// run the Forward of window b, blocks [n, 2n) the Backward of window b - n.
+// The two write disjoint matrices (the Forward's three planes, the
+// Backward's three) and each its own score array, so they need no ordering.
+__global__ __launch_bounds__(64) void cp9_fb_wave(Cp9Dev h, const Cp9WinDev *wins, int n, int fscan, int bscan,
+                                                 int align, float *fsc, float *bsc) {
+  if ((int)blockIdx.x < n) cp9_fwd_wave_body(h, wins, (int)blockIdx.x, fscan, align, fsc);
+  else cp9_bwd_wave_body(h, wins, (int)blockIdx.x - n, bscan, align, bsc);
+}
+
 hipError_t launch_cp9_bands(const Cp9Dev &h, const Cp9WinDev *wins, int64_t n, int mode, int32_t thresh,
                             float *fsc, float *bsc, int32_t *pn, int32_t *occ, hipStream_t s) {
   if (n == 0) return hipSuccess;
@@ -777,14 +797,16 @@ hipError_t launch_cp9_bands(const Cp9Dev &h, const Cp9WinDev *wins, int64_t n, i
   const int align = mode & IXG_CP9_ALIGN ? 1 : 0;
   // the wavefront forms when their local-end slots fit in LDS
   const size_t lf = cp9_wave_lds_bytes(h.M, h.nz, h.nsf, h.nel), lb = cp9_wave_lds_bytes(h.M, h.nz, h.nsb, h.nbel);
-  if ((h.wave & 1) && lf <= CP9_LDS_MAX)
-    hipLaunchKernelGGL(cp9_fwd_wave, grid, block, lf, s, h, wins, fscan, align, fsc);
-  else
-    hipLaunchKernelGGL(cp9_fwd_kernel, grid, block, lds, s, h, wins, fscan, align, fsc);
-  if ((h.wave & 2) && lb <= CP9_LDS_MAX)
-    hipLaunchKernelGGL(cp9_bwd_wave, grid, block, lb, s, h, wins, bscan, align, bsc);
-  else
-    hipLaunchKernelGGL(cp9_bwd_kernel, grid, block, lds, s, h, wins, bscan, align, bsc);
+  const bool fw = (h.wave & 1) && lf <= CP9_LDS_MAX, bw = (h.wave & 2) && lb <= CP9_LDS_MAX;
+  if (fw && bw && n <= (int64_t)1 << 30) {  // both directions of every window in one launch
+    hipLaunchKernelGGL(cp9_fb_wave, dim3((unsigned)(2 * n)), block, std::max(lf, lb), s, h, wins, (int)n, fscan,
+                       bscan, align, fsc, bsc);
+  } else {
+    if (fw) hipLaunchKernelGGL(cp9_fwd_wave, grid, block, lf, s, h, wins, fscan, align, fsc);
+    else hipLaunchKernelGGL(cp9_fwd_kernel, grid, block, lds, s, h, wins, fscan, align, fsc);
+    if (bw) hipLaunchKernelGGL(cp9_bwd_wave, grid, block, lb, s, h, wins, bscan, align, bsc);
+    else hipLaunchKernelGGL(cp9_bwd_kernel, grid, block, lds, s, h, wins, bscan, align, bsc);
+  }
   hipLaunchKernelGGL(cp9_post_kernel, grid, block, lds, s, h, wins, fscan, thresh, pn, occ);
   return hipGetLastError();
 }
