@@ -1368,6 +1368,14 @@ def main():
         cb, cpu_hits = search_cpu_baseline(wl, opts, a.cpu_threads, a.cpu_seconds)
         cb["same_hits_as_gpu"] = [h[:3] for h in cpu_hits] == [h[:3] for h in gpu_hits]
         line["cpu_baseline"] = cb
+    # the ratios the targets are stated in, from the sections above (null when a section did not run)
+    bu, ss = line.get("batched_uneven_search", {}), line.get("serving_search", {})
+    line["ratios"] = {
+        "headline_gpu_over_cpu_baseline": line["value"] / line["cpu_baseline"]["value"] if "cpu_baseline" in line
+        else None,
+        "batched_over_per_sequence_calls": bu.get("batched_over_per_sequence"),
+        "served_search_over_plain_per_request": ss.get("served_vs_plain"),
+    }
     print(json.dumps(line))
 
 
