@@ -1219,7 +1219,11 @@ def finish_batched(extra, acct, dt, seqs, cm, W, st, kt, steps, placed, found, c
         from oracle_pipe_bind import OracleSearch
         o = OracleSearch(p7, cm, cp9, W, trcp9=trd)
         budget, threads = cpu
-        n, t0 = 1000, time.perf_counter()
+        # a sample sized to the section's CPU budget from a first 500-sequence run
+        t0 = time.perf_counter()
+        _, oa = o.search_multi(seqs[:500], opts, threads=threads)
+        n = int(min(len(seqs), max(500, 500 * budget / max(time.perf_counter() - t0, 1e-3))))
+        t0 = time.perf_counter()
         ohits, oa = o.search_multi(seqs[:n], opts, threads=threads)
         dtc = time.perf_counter() - t0
         out["cpu"] = {"value": oa.nres / dtc / 1e6, "unit": "Mres/s", "cores": threads, "kind": "port",
